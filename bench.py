@@ -1,0 +1,285 @@
+#!/usr/bin/env python3
+"""Benchmark of the DyCuckoo hot path on one MI355X.
+
+Workload (the large bulk configuration): 4 subtables sized for 2^28 unique uint32 keys at load
+factor 0.90 (about 2.4 GB of buckets), values splitmix32(key). One step =
+  clear the table; bulk insert 2^28 (key, value) pairs; 2^28 finds at 50% hit with uniformly
+  drawn lookups; 2^28 finds drawn Zipf(0.99) over the inserted keys.
+All inputs are generated on the device before timing (resident in HBM). Every phase is timed with
+HIP events on the stream the library launches on. `value` is the step's operations (3 * 2^28)
+per second of step time, in Mops/s.
+
+Roofline: the algorithmic bytes per operation follow the reference algorithm and layout
+(128-byte buckets, pos1 then pos2): find = 4 (key) + 4 (value) + 1 (hit flag) + 128 * p, with
+p the exact mean number of buckets the find order reads (dyc_probe_count over the same lookups,
+outside the timed region); insert = 8 (key, value) + i * (128 + 8), with i the mean number of
+bucket exchanges per key from the insert kernel's counter. Achieved GB/s = bytes / kernel time;
+peak = 8 TB/s.
+
+cpu_baseline: the sequential oracle (oracle/dyc_oracle.c, one core) on a bounded sample of the
+same workload (2^24 keys by default), timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Mops/s insert & find at 2^28 keys, % of HBM roofline; p99 batch latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--log2n", type=int, default=28, help="keys = 2^log2n (default 28)")
+    ap.add_argument("--theta", type=float, default=0.90, help="load factor the table is sized for")
+    ap.add_argument("--hit", type=float, default=0.5, help="hit rate of the uniform lookups")
+    ap.add_argument("--zipf", type=float, default=0.99)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--find-lanes", type=int, default=8)
+    ap.add_argument("--insert-lanes", type=int, default=16)
+    ap.add_argument("--cpu-log2n", type=int, default=24, help="CPU baseline sample: 2^k keys")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--json-out", default=None)
+    return ap.parse_args()
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
+
+
+def cpu_baseline(log2n: int, theta: float, hit: float, zipf_s: float, seed: int) -> dict:
+    """Sequential oracle on 2^log2n keys: insert, uniform finds, Zipf finds (one core)."""
+    import torch
+
+    import oracle as O
+    from databases_gpu_amd import workload as W
+
+    n = 1 << log2n
+    keys_t = W.keys_range(0, n, seed, "cpu")
+    keys = keys_t.numpy().view(np.uint32).copy()
+    vals = W.splitmix32_np(keys)
+    look = W.mixed_lookups(keys_t, n, hit, seed, n, seed + 7).numpy().view(np.uint32).copy()
+    zl = W.zipf_lookups(keys_t, n, zipf_s, seed + 11).numpy().view(np.uint32).copy()
+    tb = O.OracleTable(int(math.ceil(n / theta)), rescue_kicks=128)
+    t0 = time.perf_counter()
+    tb.insert(keys, vals)
+    t1 = time.perf_counter()
+    v, h = tb.find(look)
+    t2 = time.perf_counter()
+    tb.find(zl)
+    t3 = time.perf_counter()
+    ok = int(h.sum()) == int(round(hit * n))
+    # the read-only find also runs multithreaded over the same image, for reference
+    mt = O.lib().orc_max_threads()
+    out_v = np.empty(n, dtype=np.uint32)
+    out_h = np.empty(n, dtype=np.uint8)
+    s = tb._struct()
+    t4 = time.perf_counter()
+    O.lib().orc_find_mt(O.ctypes.byref(s), O._p(look), n, O._p(out_v), O._p(out_h), mt)
+    t5 = time.perf_counter()
+    del torch
+    total = t3 - t0
+    return {
+        "value": round(3 * n / total / 1e6, 3),
+        "unit": "Mops/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"2^{log2n} keys at load factor {theta}: insert + 2^{log2n} uniform finds ({hit:.0%} hit) "
+                  f"+ 2^{log2n} Zipf({zipf_s}) finds, sequential oracle",
+        "insert_mops": round(n / (t1 - t0) / 1e6, 3),
+        "find_uniform_mops": round(n / (t2 - t1) / 1e6, 3),
+        "find_zipf_mops": round(n / (t3 - t2) / 1e6, 3),
+        "find_uniform_mt_mops": round(n / (t5 - t4) / 1e6, 3),
+        "find_mt_threads": mt,
+        "seconds": round(total, 3),
+        "hits_ok": ok,
+    }
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the table does not shard "
+              "across GPUs in this build)", file=sys.stderr)
+        return 2
+    import torch
+
+    import databases_gpu_amd as D
+    from databases_gpu_amd import workload as W
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 2
+    D.load()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    n = 1 << args.log2n
+    hint = int(math.ceil(n / args.theta))
+    log(f"generating inputs: n=2^{args.log2n}, theta={args.theta}")
+    keys = W.keys_range(0, n, args.seed, dev)
+    vals = W.splitmix32(keys)
+    look_u = W.mixed_lookups(keys, n, args.hit, args.seed, n, args.seed + 7)
+    look_z = W.zipf_lookups(keys, n, args.zipf, args.seed + 11)
+    out_v = torch.empty(n, dtype=torch.int32, device=dev)
+    out_h = torch.empty(n, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    t = D.DyCuckoo(hint, find_lanes=args.find_lanes, insert_lanes=args.insert_lanes)
+    log(f"table: buckets {t.buckets}, capacity {t.capacity}, fill {n / t.capacity:.4f}")
+
+    phases = ("clear", "insert", "find_uniform", "find_zipf")
+    times = {p: [] for p in phases}
+    step_ms = []
+    kicks = []
+
+    def step(record: bool):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
+        ev[0].record(stream)
+        t.clear()
+        ev[1].record(stream)
+        t.insert(keys, vals)
+        ev[2].record(stream)
+        t.find(look_u, out_v, out_h)
+        ev[3].record(stream)
+        t.find(look_z, out_v, out_h)
+        ev[4].record(stream)
+        ev[4].synchronize()
+        st = t.stats()
+        if st["pending"] or st["lost"] or st["count"] != n:
+            raise RuntimeError(f"insert left the table inconsistent: {st}")
+        if record:
+            for i, p in enumerate(phases):
+                times[p].append(ev[i].elapsed_time(ev[i + 1]))
+            step_ms.append(ev[0].elapsed_time(ev[4]))
+            kicks.append(st["last_kicks"])
+
+    for w in range(args.warmup):
+        step(False)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for s in range(args.steps):
+        step(True)
+        log(f"step {s + 1}/{args.steps}: {step_ms[-1]:.2f} ms "
+            f"(insert {times['insert'][-1]:.2f}, find {times['find_uniform'][-1]:.2f}, "
+            f"zipf {times['find_zipf'][-1]:.2f})")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+
+    # ---- parity of the last step's outputs against the generator (not timed)
+    v, h = t.find(look_u, out_v, out_h)
+    present = torch.isin(look_u, keys)
+    parity_ok = bool((h.bool() == present).all()) and bool((v[present] == W.splitmix32(look_u[present])).all()) \
+        and bool((v[~present] == 0).all())
+    v, h = t.find(look_z, out_v, out_h)
+    parity_ok &= bool(h.bool().all()) and bool((v == W.splitmix32(look_z)).all())
+    log(f"parity vs generator: {'ok' if parity_ok else 'MISMATCH'}")
+
+    # ---- algorithmic bytes
+    probes_u, pos1_u = t.probe_count(look_u)
+    probes_z, pos1_z = t.probe_count(look_z)
+    p_u, p_z = probes_u / n, probes_z / n
+    i_bar = float(np.mean(kicks)) / n
+    bytes_find_u = (4 + 4 + 1 + 128 * p_u) * n
+    bytes_find_z = (4 + 4 + 1 + 128 * p_z) * n
+    bytes_insert = (8 + i_bar * 136) * n
+
+    def med(p):
+        return float(np.median(times[p]))
+
+    def roof(bytes_, ms, traffic=None):
+        gbs = bytes_ / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+    kernels = {
+        "insert": dict(roof(bytes_insert, med("insert")), ms=round(med("insert"), 3),
+                       mops=round(n / med("insert") / 1e3, 1), bytes_per_op=round(bytes_insert / n, 2),
+                       mean_exchanges=round(i_bar, 4)),
+        "find_uniform": dict(roof(bytes_find_u, med("find_uniform")), ms=round(med("find_uniform"), 3),
+                             mops=round(n / med("find_uniform") / 1e3, 1), bytes_per_op=round(bytes_find_u / n, 2),
+                             mean_probes=round(p_u, 5)),
+        "find_zipf": dict(roof(bytes_find_z, med("find_zipf")), ms=round(med("find_zipf"), 3),
+                          mops=round(n / med("find_zipf") / 1e3, 1), bytes_per_op=round(bytes_find_z / n, 2),
+                          mean_probes=round(p_z, 5)),
+    }
+    traffic_path = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_path):
+        try:
+            tr = json.load(open(traffic_path))
+            for k in kernels:
+                if k in tr and tr[k].get("log2n") == args.log2n:
+                    kernels[k]["traffic"] = tr[k]["bytes_per_launch"]
+        except (OSError, ValueError, KeyError):
+            pass
+    dominant = max(("insert", "find_uniform", "find_zipf"), key=med)
+    roofline = {k: kernels[dominant][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
+    roofline["kernel"] = dominant
+
+    ms_step = float(np.median(step_ms))
+    value = 3 * n / (ms_step * 1e-3) / 1e6
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"large bulk: 4 subtables, 2^{args.log2n} unique u32 keys, load factor {args.theta}, "
+                        f"bulk insert + 2^{args.log2n} finds at {args.hit:.0%} hit (uniform) + 2^{args.log2n} "
+                        f"finds Zipf({args.zipf})",
+            "keys": n, "theta": args.theta, "table_fill": round(n / t.capacity, 5), "buckets": t.buckets,
+            "table_bytes": t.capacity * 8, "find_lanes": args.find_lanes, "insert_lanes": args.insert_lanes,
+            "seed": args.seed,
+        },
+        "roofline": roofline,
+        "kernels": kernels,
+        "p50_ms": round(pct(step_ms, 50), 3),
+        "p99_ms": round(pct(step_ms, 99), 3),
+        "phase_p99_ms": {p: round(pct(times[p], 99), 3) for p in phases},
+        "wall_s": round(wall, 3),
+        "parity": "ok" if parity_ok else "mismatch",
+    }
+    t.close()
+    del keys, vals, look_u, look_z, out_v, out_h
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        log(f"cpu baseline: oracle on 2^{args.cpu_log2n} keys")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_log2n, args.theta, args.hit, args.zipf, args.seed)
+    else:
+        result["cpu_baseline"] = None
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0 if parity_ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,143 @@
+"""ctypes binding of libdyc.so (the C ABI declared in include/dyc.h).
+
+torch is imported before the library is loaded so that libdyc's dependency on the HIP runtime
+(soname libamdhip64.so.7) resolves to the runtime torch already loaded: one process, one HIP
+runtime.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdyc.so")
+
+SUBTABLES = 4
+SLOTS = 16
+
+DYC_OK = 0
+ERRORS = {
+    1: "DYC_ERR_INVALID",
+    2: "DYC_ERR_HIP",
+    3: "DYC_ERR_OOM",
+    4: "DYC_ERR_PENDING",
+    5: "DYC_ERR_ODD_BUCKETS",
+    6: "DYC_ERR_LOST",
+    7: "DYC_ERR_CORRUPT",
+}
+DYC_ERR_PENDING = 4
+DYC_ERR_ODD_BUCKETS = 5
+
+# every entry point include/dyc.h declares
+EXPORTS = (
+    "dyc_abi_version", "dyc_last_error", "dyc_config_init", "dyc_create", "dyc_destroy",
+    "dyc_insert", "dyc_find", "dyc_erase", "dyc_resize_up", "dyc_resize_down", "dyc_get_stats",
+    "dyc_count_occupied", "dyc_dump", "dyc_load", "dyc_pending", "dyc_clear", "dyc_probe_count",
+)
+
+
+class DycError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where}: {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("kick_bound", ctypes.c_uint32),
+        ("rescue_kicks", ctypes.c_uint32),
+        ("max_retries", ctypes.c_uint32),
+        ("max_fail_rounds", ctypes.c_uint32),
+        ("lower_bound", ctypes.c_double),
+        ("upper_bound", ctypes.c_double),
+        ("auto_resize", ctypes.c_uint32),
+        ("find_lanes", ctypes.c_uint32),
+        ("insert_lanes", ctypes.c_uint32),
+        ("block_threads", ctypes.c_uint32),
+        ("max_blocks", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("min_capacity", ctypes.c_uint64),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_uint64),
+        ("capacity", ctypes.c_uint64),
+        ("buckets", ctypes.c_uint32 * SUBTABLES),
+        ("pending", ctypes.c_uint64),
+        ("kicks", ctypes.c_uint64),
+        ("last_kicks", ctypes.c_uint64),
+        ("kick_failures", ctypes.c_uint64),
+        ("cas_retries", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64),
+        ("resizes_up", ctypes.c_uint64),
+        ("resizes_down", ctypes.c_uint64),
+        ("lost", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["buckets"] = list(self.buckets)
+        return d
+
+
+_lib = None
+
+
+def load(require: bool = True):
+    """Load libdyc.so; raise if it is missing (there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    import torch  # noqa: F401  (must precede the CDLL, see module docstring)
+
+    if not os.path.exists(LIB_PATH):
+        if not require:
+            return None
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m databases_gpu_amd._build` "
+            "(or __graft_entry__.build()); the library has no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u64, u32, ci = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+    sig = {
+        "dyc_abi_version": (ci, []),
+        "dyc_last_error": (ctypes.c_char_p, []),
+        "dyc_config_init": (ci, [ctypes.POINTER(Config)]),
+        "dyc_create": (ci, [ctypes.POINTER(Config), u64, ctypes.POINTER(vp)]),
+        "dyc_destroy": (ci, [vp]),
+        "dyc_insert": (ci, [vp, vp, vp, u64, vp]),
+        "dyc_find": (ci, [vp, vp, vp, vp, u64, vp]),
+        "dyc_erase": (ci, [vp, vp, vp, u64, vp]),
+        "dyc_resize_up": (ci, [vp, vp]),
+        "dyc_resize_down": (ci, [vp, vp]),
+        "dyc_get_stats": (ci, [vp, ctypes.POINTER(Stats)]),
+        "dyc_count_occupied": (ci, [vp, ctypes.POINTER(u64), vp]),
+        "dyc_dump": (ci, [vp, u32, vp, u64, vp]),
+        "dyc_load": (ci, [vp, vp, vp, vp]),
+        "dyc_pending": (ci, [vp, vp, u64, ctypes.POINTER(u64), vp]),
+        "dyc_clear": (ci, [vp, vp]),
+        "dyc_probe_count": (ci, [vp, vp, u64, ctypes.POINTER(u64), ctypes.POINTER(u64), vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, where: str):
+    if rc != DYC_OK:
+        msg = load().dyc_last_error().decode(errors="replace")
+        raise DycError(rc, where, msg)
+
+
+def default_config(**overrides) -> Config:
+    cfg = Config()
+    check(load().dyc_config_init(ctypes.byref(cfg)), "dyc_config_init")
+    for k, v in overrides.items():
+        if not hasattr(cfg, k):
+            raise KeyError(f"unknown config field {k}")
+        setattr(cfg, k, v)
+    return cfg

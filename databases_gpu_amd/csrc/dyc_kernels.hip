@@ -1,0 +1,607 @@
+// dyc_kernels.hip -- gfx950 kernels of the DyCuckoo hot path.
+//
+// Wave64 mapping shared by find and erase ("probe" kernels):
+//   * a wave owns a tile of 64 consecutive keys; lane i loads key i (one coalesced 256 B read)
+//     and hashes it (pair, both subtables, both bucket addresses) -- no lane hashes twice;
+//   * the wave splits into 64/G groups of G lanes; group g probes keys g*G .. g*G+G-1 of the
+//     tile, each lane reading 128/G bytes of every bucket (G = 8: one dwordx4 per lane, so a
+//     group reads a whole 128-byte bucket in one coalesced line). All G pos1 probes of a group
+//     are issued before the first compare, so a lane keeps G bucket loads in flight;
+//   * a 64-bit __ballot per probe gives every group its match mask; the group's answer is
+//     broadcast to its lanes, and lane g*G+s keeps the answer of key g*G+s -- the lane that
+//     loaded that key -- so results are stored with one coalesced write per tile.
+// pos2 is read only for keys pos1 did not resolve, in the reference order (cuckoo.cu:482-497).
+//
+// Insert: each group of G lanes carries one entry through the eviction chain with 64-bit CAS
+// on the victim slot. Groups are independent: a group that finishes takes its next key while
+// the rest of the wave keeps kicking, so no lane idles behind a long chain.
+#include "dyc_kernels.hpp"
+
+namespace dyc {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ inline uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
+
+__device__ inline uint64_t shfl64(uint64_t v, int src)
+{
+    uint32_t lo = __shfl((uint32_t)v, src);
+    uint32_t hi = __shfl((uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// A bucket location packed in 32 bits: subtable in the top 2 bits, bucket index below (bucket
+// counts are capped at 2^30). Shuffling this instead of a 64-bit address halves the cross-lane
+// traffic, and rebuilding the pointer from the descriptor keeps the loads in the global
+// address space (a pointer rebuilt from a shuffled integer would become a flat access, whose
+// completion the compiler can only track together with the shuffles).
+__device__ inline uint32_t pack_loc(uint32_t t, uint32_t bucket) { return (t << 30) | bucket; }
+__device__ inline uint64_t *loc_ptr(const Desc &D, uint32_t l)
+{
+    return sel4(l >> 30, D.sub[0], D.sub[1], D.sub[2], D.sub[3]) + (uint64_t)(l & 0x3fffffffu) * kSlots;
+}
+
+// Plain (non-atomic) read of SPL consecutive slots, 16-byte vector loads where possible.
+template <int SPL>
+__device__ inline void load_slots(const uint64_t *p, uint64_t (&e)[SPL])
+{
+    if constexpr (SPL == 1) {
+        uint2 v = *reinterpret_cast<const uint2 *>(p);
+        e[0] = ((uint64_t)v.y << 32) | v.x;
+    } else {
+#pragma unroll
+        for (int j = 0; j < SPL; j += 2) {
+            uint4 v = *reinterpret_cast<const uint4 *>(p + j);
+            e[j] = ((uint64_t)v.y << 32) | v.x;
+            e[j + 1] = ((uint64_t)v.w << 32) | v.z;
+        }
+    }
+}
+
+__device__ inline uint64_t atomic_load_slot(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline bool cas_slot(uint64_t *p, uint64_t &expected, uint64_t desired)
+{
+    return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum a per-lane value over the wave and add it to *dst once (all 64 lanes must be active).
+__device__ inline void wave_add(unsigned long long *dst, unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += (unsigned long long)shfl64(v, __lane_id() ^ off);
+    if (__lane_id() == 0 && v) atomicAdd(dst, v);
+}
+
+// ------------------------------------------------------------------------ find / erase
+template <int G, bool ERASE>
+__global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__restrict__ keys,
+                                                    uint32_t *__restrict__ vals, uint8_t *__restrict__ flags,
+                                                    uint64_t n, Counters *__restrict__ C)
+{
+    constexpr int SPL = kSlots / G;  // slots each lane reads per bucket
+    constexpr uint64_t GMASK = (G >= 64) ? ~0ull : ((1ull << G) - 1ull);
+    const int lane = __lane_id();
+    const int g = lane / G, s = lane % G;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    unsigned long long ndel = 0;
+
+    for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
+         base += waves * kWave) {
+        const uint64_t i = base + lane;
+        const bool inb = i < n;
+        const uint32_t k = inb ? keys[i] : 0u;  // past the end: key 0, which is reserved
+        const uint32_t p = pair_of(k);
+        const uint32_t loc1 = pack_loc(pos1(p), bucket_of(D, k, pos1(p)));
+        const uint32_t loc2 = pack_loc(pos2(p), bucket_of(D, k, pos2(p)));
+
+        uint64_t e[G][SPL];
+        uint32_t kk_key[G];
+        bool kk_valid[G];
+        uint32_t kk_loc[ERASE ? G : 1];  // erase writes back to the bucket it matched in
+        // phase 1: pos1 buckets of the group's G keys, all loads in flight together
+#pragma unroll
+        for (int kk = 0; kk < G; ++kk) {
+            const int src = g * G + kk;
+            kk_key[kk] = __shfl(k, src);
+            kk_valid[kk] = !reserved_key(kk_key[kk]);
+            const uint32_t l = __shfl(loc1, src);
+            if constexpr (ERASE) kk_loc[kk] = l;
+#pragma unroll
+            for (int j = 0; j < SPL; ++j) e[kk][j] = 0;
+            if (kk_valid[kk]) load_slots<SPL>(loc_ptr(D, l) + s * SPL, e[kk]);
+        }
+
+        uint32_t my_val = 0, my_flag = 0;
+        uint32_t done = 0;  // bit kk: key kk resolved in pos1
+#pragma unroll
+        for (int kk = 0; kk < G; ++kk) {
+            int first = -1;
+#pragma unroll
+            for (int j = SPL - 1; j >= 0; --j)
+                if (key_of(e[kk][j]) == kk_key[kk]) first = j;
+            const uint32_t gm = (uint32_t)((__ballot(first >= 0 && kk_valid[kk]) >> (g * G)) & GMASK);
+            if (gm) {
+                done |= 1u << kk;
+                if constexpr (ERASE) {
+                    int ok = 0;
+#pragma unroll
+                    for (int j = 0; j < SPL; ++j)
+                        if (kk_valid[kk] && key_of(e[kk][j]) == kk_key[kk]) {
+                            uint64_t exp = e[kk][j];
+                            ok |= cas_slot(loc_ptr(D, kk_loc[kk]) + s * SPL + j, exp, 0ull);
+                        }
+                    const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & GMASK);
+                    if (kk == s) my_flag = okm != 0;
+                } else {
+                    const int w = __builtin_ctz(gm);
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int j = 0; j < SPL; ++j)
+                        if (j == first) v = val_of(e[kk][j]);
+                    v = __shfl(v, g * G + w);
+                    if (kk == s) {
+                        my_val = v;
+                        my_flag = 1;
+                    }
+                }
+            }
+        }
+
+        // phase 2: pos2 buckets of the keys pos1 did not resolve
+#pragma unroll
+        for (int kk = 0; kk < G; ++kk) {
+            const uint32_t l = __shfl(loc2, g * G + kk);
+            if constexpr (ERASE) kk_loc[kk] = l;
+            if (kk_valid[kk] && !(done & (1u << kk))) load_slots<SPL>(loc_ptr(D, l) + s * SPL, e[kk]);
+        }
+#pragma unroll
+        for (int kk = 0; kk < G; ++kk) {
+            if (done & (1u << kk)) continue;  // group-uniform
+            int first = -1;
+#pragma unroll
+            for (int j = SPL - 1; j >= 0; --j)
+                if (key_of(e[kk][j]) == kk_key[kk]) first = j;
+            const uint32_t gm = (uint32_t)((__ballot(first >= 0 && kk_valid[kk]) >> (g * G)) & GMASK);
+            if (!gm) continue;
+            if constexpr (ERASE) {
+                int ok = 0;
+#pragma unroll
+                for (int j = 0; j < SPL; ++j)
+                    if (kk_valid[kk] && key_of(e[kk][j]) == kk_key[kk]) {
+                        uint64_t exp = e[kk][j];
+                        ok |= cas_slot(loc_ptr(D, kk_loc[kk]) + s * SPL + j, exp, 0ull);
+                    }
+                const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & GMASK);
+                if (kk == s) my_flag = okm != 0;
+            } else {
+                const int w = __builtin_ctz(gm);
+                uint32_t v = 0;
+#pragma unroll
+                for (int j = 0; j < SPL; ++j)
+                    if (j == first) v = val_of(e[kk][j]);
+                v = __shfl(v, g * G + w);
+                if (kk == s) {
+                    my_val = v;
+                    my_flag = 1;
+                }
+            }
+        }
+
+        if (inb) {
+            if (!ERASE && vals) vals[i] = my_val;
+            if (flags) flags[i] = (uint8_t)my_flag;
+        }
+        if (ERASE) ndel += my_flag;
+    }
+    if constexpr (ERASE) wave_add(&C->deleted, ndel);
+}
+
+// ------------------------------------------------------------------------------- insert
+// One group of G lanes per in-hand entry. Per iteration (cuckoo.cu:377-421):
+//   t = the other subtable of the entry's pair relative to the previous one; read the bucket;
+//   victim = first empty slot, else (first kick_bound iterations) the first slot whose resident
+//   pair index differs from t (cuckoo.cu:398-400), else slot (iteration & 15); after kick_bound
+//   iterations the victim is rescue_lane(). CAS the victim slot from the value read to the
+//   entry: on success carry the old value on (done if it was empty); on a lost race re-read.
+template <int G, bool FROM_ENTRIES>
+__global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__restrict__ keys,
+                                                     const uint32_t *__restrict__ vals,
+                                                     const uint64_t *__restrict__ entries, uint64_t n,
+                                                     uint32_t pre_fixed, InsertPolicy P, Counters *__restrict__ C,
+                                                     uint64_t *__restrict__ pend, uint64_t pend_cap)
+{
+    constexpr int SPL = kSlots / G;
+    constexpr int NG = kWave / G;
+    constexpr uint64_t GMASK = (G >= 64) ? ~0ull : ((1ull << G) - 1ull);
+    const int lane = __lane_id();
+    const int g = lane / G, s = lane % G;
+    const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / kWave) * NG;
+    uint64_t idx = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * NG + g;
+    const uint32_t total_kicks = P.kick_bound + P.rescue_kicks;
+
+    unsigned long long n_kicks = 0, n_rej = 0, n_retry = 0;
+    bool active = false;
+    uint64_t entry = 0;
+    uint32_t pre = kNoTable, kicks = 0, retries = 0;
+
+    auto fetch = [&]() {
+        active = false;
+        while (idx < n) {
+            uint64_t e;
+            uint32_t pr;
+            if constexpr (FROM_ENTRIES) {
+                e = entries[idx];
+                pr = pre_fixed;
+            } else {
+                e = pack(keys[idx], vals[idx]);
+                pr = kNoTable;
+            }
+            idx += groups;
+            const uint32_t k = key_of(e);
+            if (reserved_key(k)) {
+                if (s == 0) ++n_rej;
+                continue;
+            }
+            if (pr == kNoTable && (k & 1u)) pr = pos1(pair_of(k));  // odd keys start in pos2
+            entry = e;
+            pre = pr;
+            kicks = 0;
+            retries = 0;
+            active = true;
+            return;
+        }
+    };
+
+    auto give_up = [&]() {  // park the in-hand entry on the pending list
+        if (s == 0) {
+            unsigned long long at = atomicAdd(&C->pending, 1ull);
+            if (at < pend_cap)
+                pend[at] = entry;
+            else
+                atomicAdd(&C->lost, 1ull);
+        }
+    };
+
+    fetch();
+    while (__ballot(active)) {
+        if (active) {
+            const uint32_t k = key_of(entry);
+            const uint32_t t = other_pos(pair_of(k), pre);
+            uint64_t *b = bucket_ptr(D, k, t);
+            uint64_t cur[SPL];
+#pragma unroll
+            for (int j = 0; j < SPL; ++j) cur[j] = atomic_load_slot(b + s * SPL + j);
+
+            // victim slot (group-uniform)
+            int fe = -1;
+#pragma unroll
+            for (int j = SPL - 1; j >= 0; --j)
+                if (key_of(cur[j]) == 0u) fe = j;
+            uint32_t gm = (uint32_t)((__ballot(fe >= 0) >> (g * G)) & GMASK);
+            int slot;
+            if (gm) {
+                const int w = __builtin_ctz(gm);
+                slot = w * SPL + __shfl(fe, g * G + w);
+            } else if (kicks < P.kick_bound) {
+                int fo = -1;
+#pragma unroll
+                for (int j = SPL - 1; j >= 0; --j)
+                    if (pair_of(key_of(cur[j])) != t) fo = j;
+                gm = (uint32_t)((__ballot(fo >= 0) >> (g * G)) & GMASK);
+                if (gm) {
+                    const int w = __builtin_ctz(gm);
+                    slot = w * SPL + __shfl(fo, g * G + w);
+                } else {
+                    slot = (int)(kicks & 15u);
+                }
+            } else {
+                slot = (int)rescue_lane(k, kicks);
+            }
+            const int owner = slot / SPL, sj = slot % SPL;
+            uint64_t mine = 0;
+#pragma unroll
+            for (int j = 0; j < SPL; ++j)
+                if (j == sj) mine = cur[j];
+            uint64_t expected = shfl64(mine, g * G + owner);
+            int won = 0;
+            if (s == owner) {
+                uint64_t exp = expected;
+                won = cas_slot(b + slot, exp, entry);
+            }
+            won = __shfl(won, g * G + owner);
+            if (won) {
+                ++kicks;
+                if (expected == 0) {
+                    if (s == 0) n_kicks += kicks;
+                    fetch();
+                } else {
+                    entry = expected;
+                    pre = t;
+                    if (kicks >= total_kicks) {
+                        if (s == 0) n_kicks += kicks;
+                        give_up();
+                        fetch();
+                    }
+                }
+            } else {
+                if (s == 0) ++n_retry;
+                if (++retries > P.max_retries) {
+                    if (s == 0) n_kicks += kicks;
+                    give_up();
+                    fetch();
+                }
+            }
+        } else {
+            // idle groups wait for the rest of the wave; nothing to do
+        }
+    }
+    wave_add(&C->kicks, n_kicks);
+    wave_add(&C->rejected, n_rej);
+    wave_add(&C->retries, n_retry);
+}
+
+// ------------------------------------------------------------------------------- upsize
+// cuckoo.cu:600-627: slot i of old bucket j goes to slot i of new bucket h % 2n, which is j or
+// j + n. Each group of 8 lanes writes both destination buckets in full (entry or zero), so the
+// new subtable needs no separate clearing pass: 8 B read + 16 B written per old slot.
+__global__ __launch_bounds__(256) void upsize_kernel(const uint64_t *__restrict__ old, uint32_t old_nb,
+                                                     uint64_t *__restrict__ dst, uint32_t t, uint32_t m,
+                                                     uint32_t s1, uint32_t s2, Counters *__restrict__ C)
+{
+    constexpr int G = 8, NG = kWave / G;
+    const int lane = __lane_id();
+    const int g = lane / G, s = lane % G;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    const uint32_t nnb = old_nb * 2u;
+    unsigned long long bad = 0;
+    for (uint64_t tile = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * NG; tile < old_nb;
+         tile += waves * NG) {
+        const uint64_t j = tile + g;
+        if (j >= old_nb) continue;
+        uint64_t e[2];
+        load_slots<2>(old + j * kSlots + s * 2, e);
+        uint64_t lo[2], hi[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            lo[q] = hi[q] = 0;
+            if (e[q]) {
+                const uint32_t d = fast_mod(hash_t(key_of(e[q]), t), nnb, m, s1, s2);
+                if (d == j)
+                    lo[q] = e[q];
+                else if (d == j + old_nb)
+                    hi[q] = e[q];
+                else
+                    ++bad;
+            }
+        }
+        *reinterpret_cast<uint4 *>(dst + j * kSlots + s * 2) =
+            make_uint4((uint32_t)lo[0], (uint32_t)(lo[0] >> 32), (uint32_t)lo[1], (uint32_t)(lo[1] >> 32));
+        *reinterpret_cast<uint4 *>(dst + (j + old_nb) * kSlots + s * 2) =
+            make_uint4((uint32_t)hi[0], (uint32_t)(hi[0] >> 32), (uint32_t)hi[1], (uint32_t)(hi[1] >> 32));
+    }
+    wave_add(&C->misplaced, bad);
+}
+
+// ----------------------------------------------------------------------------- downsize
+// cuckoo.cu:664-699: new bucket g keeps old bucket g slot-for-slot and fills its empty slots
+// from old bucket g+n read back to front (slot i <- slot 15-i). Lane s of a group holds slots
+// 2s, 2s+1; their cross partners 15-2s, 14-2s are the two halves of lane 7-s's 16 bytes.
+// Entries of bucket g+n that found no hole are appended to the residual list; the caller
+// re-inserts them only after this kernel has finished (no race with the plain stores here).
+__global__ __launch_bounds__(256) void downsize_kernel(const uint64_t *__restrict__ old, uint32_t new_nb,
+                                                       uint64_t *__restrict__ dst, Counters *__restrict__ C,
+                                                       uint64_t *__restrict__ res, uint64_t res_cap)
+{
+    constexpr int G = 8, NG = kWave / G;
+    const int lane = __lane_id();
+    const int g = lane / G, s = lane % G;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    for (uint64_t tile = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * NG; tile < new_nb;
+         tile += waves * NG) {
+        const uint64_t j = tile + g;
+        const bool ok = j < new_nb;
+        uint64_t e[2] = {0, 0}, c[2] = {0, 0};
+        if (ok) {
+            load_slots<2>(old + j * kSlots + s * 2, e);
+            uint64_t x[2];
+            load_slots<2>(old + (j + new_nb) * kSlots + (7 - s) * 2, x);
+            c[0] = x[1];  // slot 15-2s
+            c[1] = x[0];  // slot 14-2s
+        }
+        uint64_t placed[2], r[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            placed[q] = e[q] ? e[q] : c[q];
+            r[q] = (placed[q] != c[q] && c[q] != 0) ? c[q] : 0;
+        }
+        if (ok)
+            *reinterpret_cast<uint4 *>(dst + j * kSlots + s * 2) =
+                make_uint4((uint32_t)placed[0], (uint32_t)(placed[0] >> 32), (uint32_t)placed[1],
+                           (uint32_t)(placed[1] >> 32));
+        const uint64_t b0 = __ballot(r[0] != 0), b1 = __ballot(r[1] != 0);
+        const uint32_t tot = __popcll(b0) + __popcll(b1);
+        if (tot == 0) continue;
+        unsigned long long at = 0;
+        if (lane == 0) at = atomicAdd(&C->pending, (unsigned long long)tot);
+        at = shfl64(at, 0);
+        const uint64_t o0 = at + __popcll(b0 & lanemask_lt());
+        const uint64_t o1 = at + __popcll(b0) + __popcll(b1 & lanemask_lt());
+        if (r[0]) {
+            if (o0 < res_cap)
+                res[o0] = r[0];
+            else
+                atomicAdd(&C->lost, 1ull);
+        }
+        if (r[1]) {
+            if (o1 < res_cap)
+                res[o1] = r[1];
+            else
+                atomicAdd(&C->lost, 1ull);
+        }
+    }
+}
+
+// Buckets the reference find order reads per key: 1 if pos1 holds the key, else 2 (0 for
+// reserved keys). One thread per key; a diagnostic pass, not on the hot path.
+__global__ __launch_bounds__(256) void probe_count_kernel(Desc D, const uint32_t *__restrict__ keys, uint64_t n,
+                                                          Counters *__restrict__ C)
+{
+    unsigned long long probes = 0, pos1_hits = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t k = keys[i];
+        if (reserved_key(k)) continue;
+        const uint64_t *b = bucket_ptr(D, k, pos1(pair_of(k)));
+        bool hit = false;
+        for (int j = 0; j < kSlots; ++j) hit |= key_of(b[j]) == k;
+        probes += hit ? 1 : 2;
+        pos1_hits += hit;
+    }
+    wave_add(&C->kicks, probes);
+    wave_add(&C->occupied, pos1_hits);
+}
+
+__global__ void zero_kernel(unsigned long long *p, uint64_t words)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void count_kernel(const uint64_t *__restrict__ sub, uint64_t slots,
+                                                    Counters *__restrict__ C)
+{
+    unsigned long long c = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += stride) c += sub[i] != 0;
+    wave_add(&C->occupied, c);
+}
+
+inline int grid_for(uint64_t work_waves, LaunchShape sh)
+{
+    const uint64_t wpb = (uint64_t)sh.block_threads / kWave;
+    uint64_t blocks = (work_waves + wpb - 1) / wpb;
+    if (blocks < 1) blocks = 1;
+    if (sh.max_blocks > 0 && blocks > (uint64_t)sh.max_blocks) blocks = (uint64_t)sh.max_blocks;
+    if (blocks > 0x7fffffffull) blocks = 0x7fffffffull;
+    return (int)blocks;
+}
+
+}  // namespace
+
+hipError_t launch_find(const Desc &D, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, int lanes,
+                       LaunchShape sh, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = grid_for((n + kWave - 1) / kWave, sh);
+    switch (lanes) {
+    case 4: probe_kernel<4, false><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, hit, n, nullptr); break;
+    case 16: probe_kernel<16, false><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, hit, n, nullptr); break;
+    default: probe_kernel<8, false><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, hit, n, nullptr); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, uint64_t n, int lanes, LaunchShape sh,
+                        Counters *C, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = grid_for((n + kWave - 1) / kWave, sh);
+    switch (lanes) {
+    case 4: probe_kernel<4, true><<<grid, sh.block_threads, 0, st>>>(D, keys, nullptr, deleted, n, C); break;
+    case 16: probe_kernel<16, true><<<grid, sh.block_threads, 0, st>>>(D, keys, nullptr, deleted, n, C); break;
+    default: probe_kernel<8, true><<<grid, sh.block_threads, 0, st>>>(D, keys, nullptr, deleted, n, C); break;
+    }
+    return hipGetLastError();
+}
+
+template <bool FE>
+static hipError_t insert_common(const Desc &D, const uint32_t *keys, const uint32_t *vals, const uint64_t *entries,
+                                uint64_t n, uint32_t pre, int lanes, InsertPolicy pol, LaunchShape sh, Counters *C,
+                                uint64_t *pend, uint64_t pend_cap, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int per_wave = kWave / lanes;  // entries in flight per wave
+    const int grid = grid_for((n + per_wave - 1) / per_wave, sh);
+    switch (lanes) {
+    case 8:
+        insert_kernel<8, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
+        break;
+    case 4:
+        insert_kernel<4, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
+        break;
+    default:
+        insert_kernel<16, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend,
+                                                                 pend_cap);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint64_t n, int lanes,
+                         InsertPolicy pol, LaunchShape sh, Counters *C, uint64_t *pend, uint64_t pend_cap,
+                         hipStream_t st)
+{
+    return insert_common<false>(D, keys, vals, nullptr, n, kNoTable, lanes, pol, sh, C, pend, pend_cap, st);
+}
+
+hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
+                                 InsertPolicy pol, LaunchShape sh, Counters *C, uint64_t *pend, uint64_t pend_cap,
+                                 hipStream_t st)
+{
+    return insert_common<true>(D, nullptr, nullptr, entries, n, pre, lanes, pol, sh, C, pend, pend_cap, st);
+}
+
+hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, uint32_t t, Counters *C,
+                         hipStream_t st)
+{
+    if (old_nb == 0) return hipSuccess;
+    const FastDiv f = make_fastdiv(old_nb * 2u);
+    const int grid = grid_for((old_nb + 7) / 8, LaunchShape{256, 0});
+    upsize_kernel<<<grid, 256, 0, st>>>(old, old_nb, dst, t, f.m, f.s1, f.s2, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_downsize(const uint64_t *old, uint32_t new_nb, uint64_t *dst, Counters *C, uint64_t *res,
+                           uint64_t res_cap, hipStream_t st)
+{
+    if (new_nb == 0) return hipSuccess;
+    const int grid = grid_for((new_nb + 7) / 8, LaunchShape{256, 0});
+    downsize_kernel<<<grid, 256, 0, st>>>(old, new_nb, dst, C, res, res_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(unsigned long long *p, uint64_t words, hipStream_t st)
+{
+    if (words == 0) return hipSuccess;
+    uint64_t blocks = (words + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    zero_kernel<<<(int)blocks, 256, 0, st>>>(p, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_count(const Desc &D, const uint32_t *keys, uint64_t n, Counters *C, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    probe_count_kernel<<<(int)blocks, 256, 0, st>>>(D, keys, n, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_count(const uint64_t *sub, uint64_t slots, Counters *C, hipStream_t st)
+{
+    uint64_t blocks = (slots + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    count_kernel<<<(int)blocks, 256, 0, st>>>(sub, slots, C);
+    return hipGetLastError();
+}
+
+}  // namespace dyc

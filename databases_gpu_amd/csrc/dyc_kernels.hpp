@@ -1,0 +1,59 @@
+// dyc_kernels.hpp -- launchers for the DyCuckoo HIP kernels (internal to libdyc).
+#pragma once
+
+#include "dyc_device.hpp"
+
+namespace dyc {
+
+// Per-call device counters, zeroed before each launch that uses them.
+struct Counters {
+    unsigned long long kicks;      // successful exchanges (insert)
+    unsigned long long rejected;   // reserved keys in an insert batch
+    unsigned long long pending;    // entries appended to the pending list (append index)
+    unsigned long long lost;       // appends that found the pending list full
+    unsigned long long deleted;    // erase hits
+    unsigned long long retries;    // CAS attempts that lost a race
+    unsigned long long misplaced;  // resize: entries outside their bucket pair (corrupt image)
+    unsigned long long occupied;   // count kernel
+};
+constexpr int kCounterWords = 8;
+
+struct LaunchShape {
+    int block_threads;  // multiple of 64
+    int max_blocks;     // 0 = as many as the work needs
+};
+
+struct InsertPolicy {
+    uint32_t kick_bound;    // iterations that follow the reference victim rule
+    uint32_t rescue_kicks;  // further iterations with the rescue victim rule
+    uint32_t max_retries;   // CAS races tolerated per entry before it is pended
+};
+
+// find: vals/hit may be null. lanes in {4, 8, 16}.
+hipError_t launch_find(const Desc &D, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, int lanes,
+                       LaunchShape sh, hipStream_t st);
+// erase: deleted may be null; counts into C->deleted.
+hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, uint64_t n, int lanes, LaunchShape sh,
+                        Counters *C, hipStream_t st);
+// insert keys/vals (start rule of cuckoo.cu:336-341) or packed entries with a fixed previous
+// table `pre` (kNoTable = use the key-parity start rule).
+hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint64_t n, int lanes,
+                         InsertPolicy pol, LaunchShape sh, Counters *C, uint64_t *pend, uint64_t pend_cap,
+                         hipStream_t st);
+hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
+                                 InsertPolicy pol, LaunchShape sh, Counters *C, uint64_t *pend, uint64_t pend_cap,
+                                 hipStream_t st);
+// upsize subtable t: old has old_nb buckets, dst has 2*old_nb and is written completely.
+hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, uint32_t t, Counters *C,
+                         hipStream_t st);
+// downsize compaction: old has 2*new_nb buckets, dst new_nb (written completely); residuals
+// are appended to res through C->pending.
+hipError_t launch_downsize(const uint64_t *old, uint32_t new_nb, uint64_t *dst, Counters *C, uint64_t *res,
+                           uint64_t res_cap, hipStream_t st);
+hipError_t launch_zero(unsigned long long *p, uint64_t words, hipStream_t st);
+// buckets the pos1-then-pos2 find order reads for keys -> C->kicks, pos1 hits -> C->occupied
+hipError_t launch_probe_count(const Desc &D, const uint32_t *keys, uint64_t n, Counters *C, hipStream_t st);
+// count non-empty slots of one subtable into C->occupied
+hipError_t launch_count(const uint64_t *sub, uint64_t slots, Counters *C, hipStream_t st);
+
+}  // namespace dyc

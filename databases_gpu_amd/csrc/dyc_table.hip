@@ -1,0 +1,561 @@
+// dyc_table.hip -- host side of libdyc: the C ABI of include/dyc.h.
+//
+// Owns the four subtables, the pending list (entries in hand after an eviction chain hit its
+// bound, or downsize residuals that could not be placed), per-call device counters, and the
+// resize policy of the reference host API (GPU/dyhash/api.cpp).
+#include "../../include/dyc.h"
+#include "dyc_kernels.hpp"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+using namespace dyc;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(DYC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr uint32_t kMaxBuckets = 1u << 30;  // packed bucket locations use 30 bits
+
+// api.cpp:4-11 (int argument; 1 passes the 6k+-1 test, so nextPrime(0) = 1)
+bool is_prime_ref(uint32_t n)
+{
+    int64_t num = (int32_t)n;
+    if (num == 2 || num == 3) return true;
+    if (num % 6 != 1 && num % 6 != 5) return false;
+    for (int64_t i = 5; i * i <= num; i += 6)
+        if (num % i == 0 || num % (i + 2) == 0) return false;
+    return true;
+}
+
+// api.cpp:47-51: s = nextPrime(size / 64), then rounded up to an even bucket count
+uint32_t initial_buckets(uint64_t hint)
+{
+    uint32_t s = (uint32_t)std::min<uint64_t>(hint / (kSubtables * kSlots), 0x7ffffff0u);
+    while (!is_prime_ref(s)) ++s;
+    return (s & 1u) ? s + 1u : s;
+}
+
+// api.cpp:116-122
+uint32_t choose_up(const uint32_t nb[4])
+{
+    for (uint32_t i = 0; i + 1 < kSubtables; ++i)
+        if (nb[i] == 2u * nb[i + 1]) return i + 1;
+    return 0;
+}
+
+// api.cpp:147-153
+uint32_t choose_down(const uint32_t nb[4])
+{
+    for (uint32_t i = 0; i + 1 < kSubtables; ++i)
+        if (nb[i] > nb[i + 1]) return i;
+    return kSubtables - 1;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+struct dyc_table {
+    dyc_config cfg;
+    int device = 0;
+    Desc D{};
+    uint64_t count = 0;
+    uint64_t *pend = nullptr;  // pending entries [0, pend_n)
+    uint64_t pend_cap = 0;
+    uint64_t pend_n = 0;
+    Counters *ctr = nullptr;    // device
+    Counters *hctr = nullptr;   // pinned host mirror
+    dyc_stats st{};
+
+    uint64_t capacity() const
+    {
+        uint64_t c = 0;
+        for (int i = 0; i < kSubtables; ++i) c += (uint64_t)D.nb[i] * kSlots;
+        return c;
+    }
+    LaunchShape shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.max_blocks}; }
+    InsertPolicy policy() const { return InsertPolicy{cfg.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
+};
+
+namespace {
+
+int zero_counters(dyc_table *t, hipStream_t s)
+{
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->ctr), kCounterWords, s));
+    return DYC_OK;
+}
+
+int read_counters(dyc_table *t, hipStream_t s)
+{
+    HIP_TRY(hipMemcpyAsync(t->hctr, t->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return DYC_OK;
+}
+
+int alloc_zeroed_sub(uint64_t **p, uint32_t nb, hipStream_t s)
+{
+    const size_t bytes = (size_t)nb * kBucketBytes;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(DYC_ERR_OOM, "hipMalloc of %zu bytes for a subtable failed", bytes);
+    }
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(*p), (uint64_t)nb * kSlots, s));
+    return DYC_OK;
+}
+
+// Make room for `extra` more pending entries after the current ones.
+int reserve_pending(dyc_table *t, uint64_t extra, hipStream_t s)
+{
+    const uint64_t need = t->pend_n + extra;
+    if (need <= t->pend_cap) return DYC_OK;
+    const uint64_t cap = std::max<uint64_t>(need, t->pend_cap * 2);
+    uint64_t *np = nullptr;
+    if (hipMalloc(&np, cap * sizeof(uint64_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DYC_ERR_OOM, "hipMalloc of the pending list (%llu entries) failed", (unsigned long long)cap);
+    }
+    if (t->pend_n) HIP_TRY(hipMemcpyAsync(np, t->pend, t->pend_n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (t->pend) HIP_TRY(hipFree(t->pend));
+    t->pend = np;
+    t->pend_cap = cap;
+    return DYC_OK;
+}
+
+// Absorb the counters of an insert-like launch that appended to pend + pend_n.
+int absorb_insert_counters(dyc_table *t, uint64_t room)
+{
+    const Counters &c = *t->hctr;
+    t->st.kicks += c.kicks;
+    t->st.last_kicks += c.kicks;
+    t->st.cas_retries += c.retries;
+    t->st.kick_failures += c.pending;
+    t->st.lost += c.lost;
+    t->pend_n += std::min<uint64_t>(c.pending, room);
+    if (c.lost) return fail(DYC_ERR_LOST, "pending list overflow: %llu entries lost", (unsigned long long)c.lost);
+    return DYC_OK;
+}
+
+int do_resize_up(dyc_table *t, hipStream_t s)
+{
+    const uint32_t sub = choose_up(t->D.nb);
+    const uint32_t old_nb = t->D.nb[sub];
+    if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^30 buckets", sub);
+    uint64_t *dst = nullptr;
+    const size_t bytes = (size_t)old_nb * 2 * kBucketBytes;
+    if (hipMalloc(&dst, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DYC_ERR_OOM, "hipMalloc of %zu bytes for upsize failed", bytes);
+    }
+    int rc = zero_counters(t, s);
+    if (rc) return rc;
+    HIP_TRY(launch_upsize(t->D.sub[sub], old_nb, dst, sub, t->ctr, s));
+    rc = read_counters(t, s);
+    if (rc) return rc;
+    HIP_TRY(hipFree(t->D.sub[sub]));
+    desc_set(t->D, (int)sub, dst, old_nb * 2);
+    t->st.resizes_up++;
+    if (t->hctr->misplaced)
+        return fail(DYC_ERR_CORRUPT, "upsize found %llu entries outside their bucket",
+                    (unsigned long long)t->hctr->misplaced);
+    return DYC_OK;
+}
+
+// Re-insert everything on the pending list, upsizing before each round (the reference would
+// have dropped these entries, cuckoo.cu:422-431).
+int place_pending(dyc_table *t, hipStream_t s)
+{
+    uint32_t rounds = 0;
+    while (t->pend_n > 0 && rounds < t->cfg.max_fail_rounds) {
+        int rc = do_resize_up(t, s);
+        if (rc) return rc;
+        // move the pending entries aside, then re-insert them appending failures to pend
+        const uint64_t m = t->pend_n;
+        uint64_t *src = nullptr;
+        if (hipMalloc(&src, m * sizeof(uint64_t)) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(DYC_ERR_OOM, "hipMalloc for pending re-insert failed");
+        }
+        HIP_TRY(hipMemcpyAsync(src, t->pend, m * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        t->pend_n = 0;
+        rc = zero_counters(t, s);
+        if (!rc) {
+            hipError_t e = launch_insert_entries(t->D, src, m, kNoTable, (int)t->cfg.insert_lanes, t->policy(),
+                                                 t->shape(), t->ctr, t->pend, t->pend_cap, s);
+            if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "pending re-insert launch: %s", hipGetErrorString(e));
+        }
+        if (!rc) rc = read_counters(t, s);
+        (void)hipFree(src);
+        if (rc) return rc;
+        rc = absorb_insert_counters(t, t->pend_cap);
+        if (rc) return rc;
+        ++rounds;
+    }
+    t->st.pending = t->pend_n;
+    if (t->pend_n) return fail(DYC_ERR_PENDING, "%llu entries still pending after %u upsize rounds",
+                               (unsigned long long)t->pend_n, rounds);
+    return DYC_OK;
+}
+
+int do_resize_down(dyc_table *t, hipStream_t s)
+{
+    const uint32_t sub = choose_down(t->D.nb);
+    const uint32_t old_nb = t->D.nb[sub];
+    if (old_nb & 1u)
+        return fail(DYC_ERR_ODD_BUCKETS, "downsize of subtable %u refused: %u buckets is odd", sub, old_nb);
+    const uint32_t new_nb = old_nb / 2;
+    uint64_t *dst = nullptr, *res = nullptr;
+    const uint64_t res_cap = (uint64_t)new_nb * kSlots;
+    if (hipMalloc(&dst, (size_t)new_nb * kBucketBytes) != hipSuccess ||
+        hipMalloc(&res, res_cap * sizeof(uint64_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (dst) (void)hipFree(dst);
+        return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
+    }
+    int rc = zero_counters(t, s);
+    if (!rc) {
+        hipError_t e = launch_downsize(t->D.sub[sub], new_nb, dst, t->ctr, res, res_cap, s);
+        if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "downsize launch: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = read_counters(t, s);
+    if (rc) {
+        (void)hipFree(dst);
+        (void)hipFree(res);
+        return rc;
+    }
+    const uint64_t nres = t->hctr->pending;
+    if (t->hctr->lost) {
+        (void)hipFree(dst);
+        (void)hipFree(res);
+        return fail(DYC_ERR_LOST, "downsize residual list overflow");
+    }
+    HIP_TRY(hipFree(t->D.sub[sub]));
+    desc_set(t->D, (int)sub, dst, new_nb);
+    t->st.resizes_down++;
+    // residuals continue from the shrunk subtable into the other one of their pair
+    // (warp_insert(..., num_table_to_resize), cuckoo.cu:719), after the compaction completed
+    if (nres) {
+        rc = reserve_pending(t, std::max<uint64_t>(nres, 1024), s);
+        if (!rc) rc = zero_counters(t, s);
+        if (!rc) {
+            hipError_t e = launch_insert_entries(t->D, res, nres, sub, (int)t->cfg.insert_lanes, t->policy(),
+                                                 t->shape(), t->ctr, t->pend + t->pend_n, t->pend_cap - t->pend_n, s);
+            if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "residual insert launch: %s", hipGetErrorString(e));
+        }
+        if (!rc) rc = read_counters(t, s);
+        if (!rc) rc = absorb_insert_counters(t, t->pend_cap - t->pend_n);
+    }
+    (void)hipFree(res);
+    if (rc) return rc;
+    if (t->pend_n) return place_pending(t, s);
+    return DYC_OK;
+}
+
+bool valid_lanes(uint32_t l) { return l == 4 || l == 8 || l == 16; }
+
+}  // namespace
+
+extern "C" {
+
+int dyc_abi_version(void) { return DYC_ABI_VERSION; }
+
+const char *dyc_last_error(void) { return g_err; }
+
+int dyc_config_init(dyc_config *cfg)
+{
+    if (!cfg) return fail(DYC_ERR_INVALID, "null config");
+    memset(cfg, 0, sizeof *cfg);
+    cfg->kick_bound = kDefaultKickBound;
+    cfg->rescue_kicks = 128;
+    cfg->max_retries = 64;
+    cfg->max_fail_rounds = 8;
+    cfg->lower_bound = 0.50;
+    cfg->upper_bound = 0.90;
+    cfg->auto_resize = 0;
+    cfg->find_lanes = 8;
+    cfg->insert_lanes = 16;
+    cfg->block_threads = 256;
+    cfg->max_blocks = 0;
+    cfg->min_capacity = 5033164ull;  // TABLE_NUM * 1024 * 1024 * 1.2 (api.cpp:496)
+    return DYC_OK;
+}
+
+int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out)
+{
+    if (!out) return fail(DYC_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    dyc_config cfg;
+    if (cfg_in)
+        cfg = *cfg_in;
+    else
+        dyc_config_init(&cfg);
+    if (!valid_lanes(cfg.find_lanes) || !valid_lanes(cfg.insert_lanes))
+        return fail(DYC_ERR_INVALID, "lanes per key must be 4, 8 or 16");
+    if (cfg.block_threads == 0 || cfg.block_threads % 64 || cfg.block_threads > 256)
+        return fail(DYC_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 256]");
+    if (!(cfg.upper_bound > 0.0 && cfg.upper_bound <= 1.0) || !(cfg.lower_bound >= 0.0) ||
+        cfg.lower_bound >= cfg.upper_bound)
+        return fail(DYC_ERR_INVALID, "need 0 <= lower_bound < upper_bound <= 1");
+    const uint32_t nb = initial_buckets(capacity_hint);
+    if (nb >= kMaxBuckets) return fail(DYC_ERR_INVALID, "capacity hint too large");
+
+    dyc_table *t = new dyc_table();
+    t->cfg = cfg;
+    HIP_TRY(hipGetDevice(&t->device));
+    int rc = DYC_OK;
+    if (hipMalloc(&t->ctr, sizeof(Counters)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(DYC_ERR_OOM, "counter allocation failed");
+    }
+    for (int i = 0; i < kSubtables && rc == DYC_OK; ++i) {
+        uint64_t *p = nullptr;
+        rc = alloc_zeroed_sub(&p, nb, nullptr);
+        if (rc == DYC_OK) desc_set(t->D, i, p, nb);
+    }
+    if (rc == DYC_OK) rc = reserve_pending(t, 1024, nullptr);
+    if (rc == DYC_OK && hipStreamSynchronize(nullptr) != hipSuccess) rc = fail(DYC_ERR_HIP, "create sync failed");
+    if (rc != DYC_OK) {
+        char keep[sizeof g_err];
+        memcpy(keep, g_err, sizeof keep);
+        dyc_destroy(t);
+        memcpy(g_err, keep, sizeof keep);
+        return rc;
+    }
+    *out = t;
+    return DYC_OK;
+}
+
+int dyc_destroy(dyc_table *t)
+{
+    if (!t) return DYC_OK;
+    DeviceGuard dg(t->device);
+    (void)hipDeviceSynchronize();
+    for (int i = 0; i < kSubtables; ++i)
+        if (t->D.sub[i]) (void)hipFree(t->D.sub[i]);
+    if (t->pend) (void)hipFree(t->pend);
+    if (t->ctr) (void)hipFree(t->ctr);
+    if (t->hctr) (void)hipHostFree(t->hctr);
+    delete t;
+    return DYC_OK;
+}
+
+int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    t->st.last_kicks = 0;
+    if (n == 0) return DYC_OK;
+    int rc;
+    if (t->cfg.auto_resize) {
+        while ((double)(t->count + n) > t->cfg.upper_bound * (double)t->capacity()) {
+            rc = do_resize_up(t, s);
+            if (rc) return rc;
+        }
+    }
+    rc = reserve_pending(t, std::max<uint64_t>(65536, n / 16), s);
+    if (rc) return rc;
+    rc = zero_counters(t, s);
+    if (rc) return rc;
+    const uint64_t room = t->pend_cap - t->pend_n;
+    HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->shape(), t->ctr,
+                          t->pend + t->pend_n, room, s));
+    rc = read_counters(t, s);
+    if (rc) return rc;
+    const uint64_t rejected = t->hctr->rejected;
+    t->st.rejected += rejected;
+    t->count += n - rejected;
+    rc = absorb_insert_counters(t, room);
+    t->st.pending = t->pend_n;
+    if (rc) return rc;
+    if (t->pend_n) return place_pending(t, s);
+    return DYC_OK;
+}
+
+int dyc_find(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    DeviceGuard dg(t->device);
+    HIP_TRY(launch_find(t->D, keys, vals, hit, n, (int)t->cfg.find_lanes, t->shape(), (hipStream_t)stream));
+    return DYC_OK;
+}
+
+int dyc_erase(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) return DYC_OK;
+    int rc = zero_counters(t, s);
+    if (rc) return rc;
+    HIP_TRY(launch_erase(t->D, keys, deleted, n, (int)t->cfg.find_lanes, t->shape(), t->ctr, s));
+    rc = read_counters(t, s);
+    if (rc) return rc;
+    t->count -= std::min<uint64_t>(t->count, t->hctr->deleted);
+    if (t->cfg.auto_resize && (double)t->count < t->cfg.lower_bound * (double)t->capacity() &&
+        t->capacity() > t->cfg.min_capacity) {
+        rc = do_resize_down(t, s);
+        if (rc == DYC_ERR_ODD_BUCKETS) rc = DYC_OK;  // policy: skip, the table stays larger
+        if (rc) return rc;
+    }
+    return DYC_OK;
+}
+
+int dyc_resize_up(dyc_table *t, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    DeviceGuard dg(t->device);
+    return do_resize_up(t, (hipStream_t)stream);
+}
+
+int dyc_resize_down(dyc_table *t, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    DeviceGuard dg(t->device);
+    return do_resize_down(t, (hipStream_t)stream);
+}
+
+int dyc_get_stats(const dyc_table *t, dyc_stats *out)
+{
+    if (!t || !out) return fail(DYC_ERR_INVALID, "null argument");
+    *out = t->st;
+    out->count = t->count;
+    out->capacity = t->capacity();
+    for (int i = 0; i < kSubtables; ++i) out->buckets[i] = t->D.nb[i];
+    out->pending = t->pend_n;
+    return DYC_OK;
+}
+
+int dyc_count_occupied(dyc_table *t, uint64_t *occupied, dyc_stream stream)
+{
+    if (!t || !occupied) return fail(DYC_ERR_INVALID, "null argument");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = zero_counters(t, s);
+    if (rc) return rc;
+    for (int i = 0; i < kSubtables; ++i) HIP_TRY(launch_count(t->D.sub[i], (uint64_t)t->D.nb[i] * kSlots, t->ctr, s));
+    rc = read_counters(t, s);
+    if (rc) return rc;
+    *occupied = t->hctr->occupied;
+    return DYC_OK;
+}
+
+int dyc_clear(dyc_table *t, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < kSubtables; ++i)
+        HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->D.sub[i]), (uint64_t)t->D.nb[i] * kSlots, s));
+    t->count = 0;
+    t->pend_n = 0;
+    t->st.pending = 0;
+    return DYC_OK;
+}
+
+int dyc_probe_count(dyc_table *t, const uint32_t *keys, uint64_t n, uint64_t *probes, uint64_t *pos1_hits,
+                    dyc_stream stream)
+{
+    if (!t || (n && !keys)) return fail(DYC_ERR_INVALID, "null argument");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = zero_counters(t, s);
+    if (rc) return rc;
+    HIP_TRY(launch_probe_count(t->D, keys, n, t->ctr, s));
+    rc = read_counters(t, s);
+    if (rc) return rc;
+    if (probes) *probes = t->hctr->kicks;
+    if (pos1_hits) *pos1_hits = t->hctr->occupied;
+    return DYC_OK;
+}
+
+int dyc_dump(dyc_table *t, uint32_t sub, uint64_t *host_slots, uint64_t n_slots, dyc_stream stream)
+{
+    if (!t || !host_slots || sub >= (uint32_t)kSubtables) return fail(DYC_ERR_INVALID, "bad argument");
+    const uint64_t slots = (uint64_t)t->D.nb[sub] * kSlots;
+    if (n_slots < slots) return fail(DYC_ERR_INVALID, "buffer holds %llu slots, subtable has %llu",
+                                     (unsigned long long)n_slots, (unsigned long long)slots);
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(host_slots, t->D.sub[sub], slots * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return DYC_OK;
+}
+
+int dyc_load(dyc_table *t, const uint32_t nb[4], const uint64_t *const images[4], dyc_stream stream)
+{
+    if (!t || !nb || !images) return fail(DYC_ERR_INVALID, "null argument");
+    for (int i = 0; i < kSubtables; ++i)
+        if (nb[i] == 0 || nb[i] >= kMaxBuckets || !images[i]) return fail(DYC_ERR_INVALID, "bad subtable %d", i);
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipStreamSynchronize(s));
+    uint64_t count = 0;
+    for (int i = 0; i < kSubtables; ++i) {
+        uint64_t *p = nullptr;
+        const size_t bytes = (size_t)nb[i] * kBucketBytes;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(DYC_ERR_OOM, "hipMalloc for load failed");
+        }
+        HIP_TRY(hipMemcpyAsync(p, images[i], bytes, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipFree(t->D.sub[i]));
+        desc_set(t->D, i, p, nb[i]);
+        for (uint64_t j = 0; j < (uint64_t)nb[i] * kSlots; ++j) count += images[i][j] != 0;
+    }
+    t->count = count;
+    t->pend_n = 0;
+    t->st.pending = 0;
+    return DYC_OK;
+}
+
+int dyc_pending(dyc_table *t, uint64_t *host_entries, uint64_t cap, uint64_t *n_out, dyc_stream stream)
+{
+    if (!t || !n_out) return fail(DYC_ERR_INVALID, "null argument");
+    *n_out = t->pend_n;
+    const uint64_t m = std::min(cap, t->pend_n);
+    if (m && host_entries) {
+        DeviceGuard dg(t->device);
+        HIP_TRY(hipMemcpyAsync(host_entries, t->pend, m * sizeof(uint64_t), hipMemcpyDeviceToHost, (hipStream_t)stream));
+        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    }
+    return DYC_OK;
+}
+
+}  // extern "C"

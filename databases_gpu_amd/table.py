@@ -1,0 +1,275 @@
+"""Python host interface of the MI355X DyCuckoo table.
+
+``DyCuckoo`` is a thin owner of a ``dyc_table*`` handle (include/dyc.h): device tensors in,
+device tensors out, on the current torch stream.
+
+``HashAPI`` mirrors the reference's host class ``hashAPI`` (GPU/dyhash/api.h:9-55) method for
+method -- host arrays in, host arrays out, synchronous -- so call sequences of the reference's
+drivers (GPU/dyhash/Main_backup_0910.cpp, GPU/warp/Main.cpp) carry over unchanged.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import SLOTS, SUBTABLES, DycError, check
+
+_U32 = torch.int32  # torch has no uint32 arithmetic on GPU; bits are reinterpreted as uint32
+
+
+def _as_u32_tensor(x, device) -> torch.Tensor:
+    """Return a contiguous 4-byte tensor on `device` whose bits are the uint32 values of x."""
+    if isinstance(x, torch.Tensor):
+        t = x
+        if t.dtype not in (torch.int32, torch.uint32):
+            if t.dtype == torch.int64:
+                t = (t & 0xFFFFFFFF).to(torch.int64)
+                t = torch.where(t >= 2**31, t - 2**32, t).to(torch.int32)
+            else:
+                raise TypeError(f"keys/values must be 32-bit, got {t.dtype}")
+        t = t.view(torch.int32) if t.dtype == torch.uint32 else t
+        return t.to(device).contiguous()
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.uint32)).view(np.int32)
+    return torch.from_numpy(a).to(device)
+
+
+def _stream_ptr(stream) -> ctypes.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class DyCuckoo:
+    """A dynamic 4-subtable cuckoo hash table of uint32 -> uint32 on one GPU."""
+
+    def __init__(self, capacity_hint: int, *, device=None, **config):
+        self.lib = _lib.load()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.cfg = _lib.default_config(**config)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_create(ctypes.byref(self.cfg), int(capacity_hint), ctypes.byref(h)), "dyc_create")
+        self.h = h
+
+    # -- lifecycle
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dyc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- operations on device tensors
+    def insert(self, keys: torch.Tensor, vals: torch.Tensor, stream=None) -> None:
+        """Insert keys[i] -> vals[i] (cuckoo_insert, cuckoo.cu:324-355)."""
+        k = _as_u32_tensor(keys, self.device)
+        v = _as_u32_tensor(vals, self.device)
+        if k.numel() != v.numel():
+            raise ValueError("keys and values differ in length")
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_insert(self.h, k.data_ptr(), v.data_ptr(), k.numel(), _stream_ptr(stream)),
+                  "dyc_insert")
+
+    def find(self, keys: torch.Tensor, vals: torch.Tensor | None = None, hit: torch.Tensor | None = None,
+             stream=None):
+        """Look keys up (cuckoo_search, cuckoo.cu:460-520). Returns (vals int32-as-uint32, hit uint8)."""
+        k = _as_u32_tensor(keys, self.device)
+        n = k.numel()
+        if vals is None:
+            vals = torch.empty(n, dtype=torch.int32, device=self.device)
+        if hit is None:
+            hit = torch.empty(n, dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_find(self.h, k.data_ptr(), vals.data_ptr(), hit.data_ptr(), n, _stream_ptr(stream)),
+                  "dyc_find")
+        return vals, hit
+
+    def erase(self, keys: torch.Tensor, deleted: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """Delete keys (cuckoo_delete, cuckoo.cu:524-587). Returns per-key deleted flags (uint8)."""
+        k = _as_u32_tensor(keys, self.device)
+        if deleted is None:
+            deleted = torch.empty(k.numel(), dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_erase(self.h, k.data_ptr(), deleted.data_ptr(), k.numel(), _stream_ptr(stream)),
+                  "dyc_erase")
+        return deleted
+
+    def resize_up(self, stream=None) -> None:
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_resize_up(self.h, _stream_ptr(stream)), "dyc_resize_up")
+
+    def resize_down(self, stream=None) -> bool:
+        """Halve one subtable; False when the chosen subtable has an odd bucket count."""
+        with torch.cuda.device(self.device):
+            rc = self.lib.dyc_resize_down(self.h, _stream_ptr(stream))
+        if rc == _lib.DYC_ERR_ODD_BUCKETS:
+            return False
+        check(rc, "dyc_resize_down")
+        return True
+
+    # -- inspection
+    def stats(self) -> dict:
+        st = _lib.Stats()
+        check(self.lib.dyc_get_stats(self.h, ctypes.byref(st)), "dyc_get_stats")
+        return st.as_dict()
+
+    @property
+    def buckets(self):
+        return self.stats()["buckets"]
+
+    @property
+    def capacity(self) -> int:
+        return self.stats()["capacity"]
+
+    def __len__(self) -> int:
+        return self.stats()["count"]
+
+    def count_occupied(self, stream=None) -> int:
+        out = ctypes.c_uint64()
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_count_occupied(self.h, ctypes.byref(out), _stream_ptr(stream)), "dyc_count_occupied")
+        return out.value
+
+    def clear(self, stream=None) -> None:
+        """Empty the table, keeping its subtable sizes."""
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_clear(self.h, _stream_ptr(stream)), "dyc_clear")
+
+    def probe_count(self, keys, stream=None):
+        """(buckets the pos1-then-pos2 find order reads for keys, keys resolved by pos1)."""
+        k = _as_u32_tensor(keys, self.device)
+        probes, hits = ctypes.c_uint64(), ctypes.c_uint64()
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_probe_count(self.h, k.data_ptr(), k.numel(), ctypes.byref(probes), ctypes.byref(hits),
+                                           _stream_ptr(stream)), "dyc_probe_count")
+        return probes.value, hits.value
+
+    def dump(self, stream=None):
+        """Host copies of the four subtables (uint64 slots, (key << 32) | value)."""
+        out = []
+        for t, nb in enumerate(self.buckets):
+            a = np.empty(nb * SLOTS, dtype=np.uint64)
+            with torch.cuda.device(self.device):
+                check(self.lib.dyc_dump(self.h, t, a.ctypes.data, a.size, _stream_ptr(stream)), "dyc_dump")
+            out.append(a)
+        return out
+
+    def load(self, images, stream=None) -> None:
+        imgs = [np.ascontiguousarray(x, dtype=np.uint64) for x in images]
+        if len(imgs) != SUBTABLES or any(x.size % SLOTS or x.size == 0 for x in imgs):
+            raise ValueError("need 4 non-empty images of whole buckets")
+        nb = (ctypes.c_uint32 * SUBTABLES)(*[x.size // SLOTS for x in imgs])
+        ptrs = (ctypes.c_void_p * SUBTABLES)(*[x.ctypes.data for x in imgs])
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_load(self.h, nb, ptrs, _stream_ptr(stream)), "dyc_load")
+
+    def pending(self, stream=None) -> np.ndarray:
+        n = ctypes.c_uint64()
+        check(self.lib.dyc_pending(self.h, None, 0, ctypes.byref(n), _stream_ptr(stream)), "dyc_pending")
+        a = np.empty(n.value, dtype=np.uint64)
+        if n.value:
+            with torch.cuda.device(self.device):
+                check(self.lib.dyc_pending(self.h, a.ctypes.data, a.size, ctypes.byref(n), _stream_ptr(stream)),
+                      "dyc_pending")
+        return a
+
+
+class HashAPI:
+    """Host-array facade with the reference's method names (GPU/dyhash/api.h:9-55).
+
+    ``hashAPI(size)`` sizes from a slot-count hint (api.cpp:47-56); ``hash_insert`` /
+    ``hash_search`` / ``hash_delete`` take host arrays and a size and run synchronously
+    (api.cpp:184-303); the ``*_batch`` calls walk a pool staged by ``set_data_to_gpu`` in
+    slices of ``batch_size`` (api.cpp:309-506), with the reference's resize policy
+    (fill bounds ``lower``/``upper``) applied around each batch.
+    """
+
+    BATCH_SIZE = 100000  # test_uilt.h:5
+    BATCH_NUM = 1000     # test_uilt.h:4
+
+    def __init__(self, size: int, *, batch_size: int | None = None, **config):
+        self.table = DyCuckoo(int(size), **config)
+        self.batch_size = batch_size or self.BATCH_SIZE
+        self._pool_k = self._pool_v = None
+        self.batch_reset()
+
+    # -- synchronous host-array operations (api.cpp:184-303)
+    def hash_insert(self, key, value, size: int) -> None:
+        k = np.asarray(key, dtype=np.uint32)[:size]
+        v = np.asarray(value, dtype=np.uint32)[:size]
+        self.table.insert(k, v)
+        torch.cuda.synchronize(self.table.device)
+
+    def hash_search(self, key, ans, size: int):
+        """Fills ans[:size] with the values (0 on a miss); returns the hit flags."""
+        k = np.asarray(key, dtype=np.uint32)[:size]
+        vals, hit = self.table.find(k)
+        ans[:size] = vals.cpu().numpy().view(np.uint32)
+        return hit.cpu().numpy()
+
+    def hash_delete(self, key, ans, size: int):
+        """Deletes key[:size]; ans[:size] receives the deleted flags (the reference leaves it unused)."""
+        k = np.asarray(key, dtype=np.uint32)[:size]
+        d = self.table.erase(k).cpu().numpy()
+        if ans is not None:
+            ans[:size] = d
+        return d
+
+    def resize_up(self) -> None:
+        self.table.resize_up()
+
+    def resize_low(self) -> bool:
+        return self.table.resize_down()
+
+    # -- batch API over a device pool (api.cpp:309-506)
+    def set_data_to_gpu(self, key, value, size: int) -> bool:
+        self._pool_k = _as_u32_tensor(np.asarray(key, dtype=np.uint32)[:size], self.table.device)
+        self._pool_v = _as_u32_tensor(np.asarray(value, dtype=np.uint32)[:size], self.table.device)
+        self._check = torch.empty(self.batch_size, dtype=torch.int32, device=self.table.device)
+        self._hit = torch.empty(self.batch_size, dtype=torch.uint8, device=self.table.device)
+        return True
+
+    def batch_reset(self) -> None:
+        self.batch_num_to_op = 0
+        self.batch_search_num_to_op = 0
+        self.batch_del_num_to_op = 0
+
+    def _slice(self, i: int):
+        lo = i * self.batch_size
+        hi = min(lo + self.batch_size, self._pool_k.numel())
+        if lo >= hi or i >= self.BATCH_NUM:
+            raise IndexError(f"batch {i} is beyond the staged pool")
+        return lo, hi
+
+    def hash_insert_batch(self) -> None:
+        lo, hi = self._slice(self.batch_num_to_op)
+        self.table.insert(self._pool_k[lo:hi], self._pool_v[lo:hi])
+        self.batch_num_to_op += 1
+
+    def hash_search_batch(self) -> None:
+        lo, hi = self._slice(self.batch_search_num_to_op)
+        self.table.find(self._pool_k[lo:hi], self._check[: hi - lo], self._hit[: hi - lo])
+        self.batch_search_num_to_op += 1
+
+    def hash_delete_batch(self) -> None:
+        lo, hi = self._slice(self.batch_del_num_to_op)
+        self.table.erase(self._pool_k[lo:hi])
+        self.batch_del_num_to_op += 1
+
+    def close(self):
+        self.table.close()
+
+
+__all__ = ["DyCuckoo", "HashAPI", "DycError"]

@@ -1,0 +1,155 @@
+"""Seeded synthetic inputs for the DyCuckoo workloads (keys, values, lookups), built on the device.
+
+* Keys: ``key(i) = mix(i)`` where ``mix`` is a seeded bijection of the 32-bit words (an
+  xor-multiply-xorshift chain, every step invertible), so ``key(0..n-1)`` are distinct and
+  ``key(n..2n-1)`` are distinct from them -- absent lookups are disjoint by construction. The two
+  reserved words (0 and 0xFFFFFFFF) are remapped to spare indices past the ones handed out.
+* Values: ``splitmix32(key)`` (a 32-bit splitmix finaliser), so about half the values are
+  >= 2^31 and 0 is possible.
+* Zipf(s) lookups: ranks drawn by inverting the CDF, exact over the first 2^20 ranks and from the
+  Euler-Maclaurin integral beyond, then mapped to inserted keys.
+Everything is plain torch int64 arithmetic on the device; results are int32 tensors whose bits
+are the uint32 values.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+M32 = 0xFFFFFFFF
+
+
+def _mulmod32(x: torch.Tensor, c: int) -> torch.Tensor:
+    """(x * c) mod 2^32 for x < 2^32 without leaving int64: split c's top bit off."""
+    lo = c & 0x7FFFFFFF
+    y = x * lo
+    if c & 0x80000000:
+        y = y + ((x << 31) & M32)
+    return y & M32
+
+
+def _mix32(x: torch.Tensor, seed: int) -> torch.Tensor:
+    """Seeded bijection of uint32 words held in int64 tensors."""
+    s1 = (seed * 0x9E3779B97F4A7C15 + 0x632BE59BD9B4E019) & M32
+    s2 = ((seed ^ 0x5851F42D) * 0x2545F491 + 0x1B873593) & M32
+    x = (x ^ s1) & M32
+    x = _mulmod32(x, 0x7FEB352D)
+    x = x ^ (x >> 15)
+    x = (x + s2) & M32
+    x = _mulmod32(x, 0x846CA68B)
+    x = x ^ (x >> 16)
+    x = _mulmod32(x, 0x94D049BB)
+    x = x ^ (x >> 13)
+    return x
+
+
+def _to_i32(x: torch.Tensor) -> torch.Tensor:
+    x = x & M32
+    return torch.where(x >= 2**31, x - 2**32, x).to(torch.int32)
+
+
+def u32_to_i64(x: torch.Tensor) -> torch.Tensor:
+    return x.to(torch.int64) & M32
+
+
+def keys_range(lo: int, hi: int, seed: int, device, n_total: int | None = None) -> torch.Tensor:
+    """key(i) for i in [lo, hi): distinct nonreserved uint32 words (as int32 bits).
+
+    n_total bounds the indices ever handed out (default hi); reserved words are replaced by
+    key(n_total + 2^31 + j), which no caller reaches."""
+    i = torch.arange(lo, hi, dtype=torch.int64, device=device)
+    k = _mix32(i, seed)
+    bad = (k == 0) | (k == M32)
+    if bool(bad.any()):
+        spare_base = (n_total or hi) + (1 << 31)
+        j = torch.cumsum(bad.to(torch.int64), 0) - 1
+        repl = _mix32(spare_base + j, seed)
+        # the spare chain itself could hit a reserved word; walk further until clean
+        step = 0
+        while bool(((repl == 0) | (repl == M32))[bad].any()):
+            step += 1
+            repl = torch.where((repl == 0) | (repl == M32), _mix32(spare_base + (1 << 20) * step + j, seed), repl)
+        k = torch.where(bad, repl, k)
+    return _to_i32(k)
+
+
+def splitmix32(keys: torch.Tensor) -> torch.Tensor:
+    """values of the workloads: splitmix32(key)"""
+    z = u32_to_i64(keys)
+    z = (z + 0x9E3779B9) & M32
+    z = _mulmod32(z ^ (z >> 16), 0x85EBCA6B)
+    z = _mulmod32(z ^ (z >> 13), 0xC2B2AE35)
+    z = z ^ (z >> 16)
+    return _to_i32(z)
+
+
+def splitmix32_np(keys: np.ndarray) -> np.ndarray:
+    z = (keys.astype(np.uint64) + np.uint64(0x9E3779B9)) & np.uint64(M32)
+    z = ((z ^ (z >> np.uint64(16))) * np.uint64(0x85EBCA6B)) & np.uint64(M32)
+    z = ((z ^ (z >> np.uint64(13))) * np.uint64(0xC2B2AE35)) & np.uint64(M32)
+    return (z ^ (z >> np.uint64(16))).astype(np.uint32)
+
+
+def shuffle(x: torch.Tensor, seed: int) -> torch.Tensor:
+    g = torch.Generator(device=x.device)
+    g.manual_seed(seed)
+    return x[torch.randperm(x.numel(), generator=g, device=x.device)]
+
+
+def mixed_lookups(present: torch.Tensor, n: int, hit_rate: float, key_seed: int, absent_lo: int,
+                  seed: int) -> torch.Tensor:
+    """n lookups: round(hit_rate*n) drawn without replacement from `present` (keys_range(0, m,
+    key_seed)), the rest the absent keys keys_range(absent_lo, ..., key_seed) -- the same
+    bijection at indexes past the inserted ones, hence disjoint -- shuffled together with `seed`."""
+    dev = present.device
+    nh = int(round(hit_rate * n))
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    if nh <= present.numel():
+        idx = torch.randperm(present.numel(), generator=g, device=dev)[:nh]
+    else:
+        idx = torch.randint(0, present.numel(), (nh,), generator=g, device=dev)
+    hits = present[idx]
+    miss = keys_range(absent_lo, absent_lo + (n - nh), key_seed, dev)
+    return shuffle(torch.cat([hits, miss]), seed + 1)
+
+
+def zipf_ranks(n_items: int, n_samples: int, s: float, seed: int, device) -> torch.Tensor:
+    """Zipf(s) ranks in [0, n_items) by inverse CDF (rank 0 most popular)."""
+    head = min(n_items, 1 << 20)
+    r = torch.arange(1, head + 1, dtype=torch.float64, device=device)
+    w = r.pow(-s)
+    cdf_head = torch.cumsum(w, 0)
+    h_head = float(cdf_head[-1])
+
+    def tail_mass(k: float) -> float:  # sum_{j=head+1}^{k} j^-s by Euler-Maclaurin
+        a = head + 0.5
+        b = k + 0.5
+        if abs(1 - s) < 1e-12:
+            return math.log(b / a)
+        return (b ** (1 - s) - a ** (1 - s)) / (1 - s)
+
+    total = h_head + (tail_mass(n_items) if n_items > head else 0.0)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    u = torch.rand(n_samples, generator=g, dtype=torch.float64, device=device) * total
+    in_head = u < h_head
+    ranks = torch.searchsorted(cdf_head, u.clamp(max=h_head * (1 - 1e-15)))
+    if n_items > head:
+        # invert the tail integral: b^(1-s) = a^(1-s) + (u - h_head)(1-s)
+        a = head + 0.5
+        t = (u - h_head).clamp(min=0)
+        if abs(1 - s) < 1e-12:
+            b = a * torch.exp(t)
+        else:
+            b = (a ** (1 - s) + t * (1 - s)).clamp(min=1e-300).pow(1.0 / (1 - s))
+        tail_rank = torch.floor(b - 0.5).clamp(head, n_items - 1).to(torch.int64)
+        ranks = torch.where(in_head, ranks, tail_rank)
+    return ranks.clamp(0, n_items - 1)
+
+
+def zipf_lookups(present: torch.Tensor, n: int, s: float, seed: int) -> torch.Tensor:
+    ranks = zipf_ranks(present.numel(), n, s, seed, present.device)
+    return present[ranks]

@@ -1,0 +1,159 @@
+/*
+ * dyc.h -- C ABI of the MI355X DyCuckoo library (libdyc.so).
+ *
+ * A dynamic cuckoo hash table of uint32 keys and uint32 values: 4 subtables of 128-byte
+ * buckets (16 slots of (uint64)key << 32 | value), two-layer pair hashing, bounded eviction,
+ * and single-subtable upsize/downsize -- the data structure of GPU/dyhash in the reference.
+ *
+ * The reference exposes a C++ class (GPU/dyhash/api.h:9-55, hashAPI) over a device-pointer
+ * layer (GPU/dyhash/cuckoo.h:100-114, gpu_lp_*). This header replaces both with an opaque
+ * handle and plain pointers; include/dyc_hashapi.hpp rebuilds the hashAPI class on top of it.
+ *
+ * Conventions
+ *   - Every function returns DYC_OK (0) or a DYC_ERR_* code; dyc_last_error() gives a message
+ *     for the calling thread's last failure.
+ *   - keys / vals / hit / deleted are DEVICE pointers (any memory the GPU can read or write:
+ *     hipMalloc, managed or pinned host memory). The caller owns them; the library owns the
+ *     table storage.
+ *   - `stream` is a hipStream_t (NULL = the default stream). Work is stream-ordered. One table
+ *     must be used from one stream (or externally serialised streams) at a time.
+ *   - Keys 0 and 0xFFFFFFFF are reserved (0 marks an empty slot): insert skips them and counts
+ *     them as rejected, find reports a miss, erase reports not-deleted.
+ *   - Insert does not look for an existing copy of the key (reference semantics,
+ *     dy_hash.h:42 update_on 0): the keys of one table should be unique.
+ *   - An entry is never dropped. An eviction chain that exceeds kick_bound + rescue_kicks
+ *     parks its entry on the table's pending list; dyc_insert then upsizes and re-inserts
+ *     pending entries (at most max_fail_rounds rounds per call).
+ */
+#ifndef DYC_H
+#define DYC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DYC_ABI_VERSION 1
+
+#define DYC_OK 0
+#define DYC_ERR_INVALID 1        /* bad argument (null handle, bad config, size limit) */
+#define DYC_ERR_HIP 2            /* a HIP runtime call failed */
+#define DYC_ERR_OOM 3            /* device allocation failed */
+#define DYC_ERR_PENDING 4        /* entries remain pending after max_fail_rounds upsizes */
+#define DYC_ERR_ODD_BUCKETS 5    /* downsize refused: the chosen subtable has an odd bucket count */
+#define DYC_ERR_LOST 6           /* the pending list overflowed: entries were lost (table corrupt) */
+#define DYC_ERR_CORRUPT 7        /* a resize found entries outside their bucket pair */
+
+typedef struct dyc_table dyc_table;   /* opaque */
+typedef void *dyc_stream;             /* hipStream_t */
+
+/* Runtime configuration; the reference fixes these at compile time (dy_hash.h:15-73,
+ * test_uilt.h:1-5). Fill with dyc_config_init() and override fields. */
+typedef struct dyc_config {
+    uint32_t kick_bound;       /* eviction iterations under the reference victim rule (MAX_ITER = 15) */
+    uint32_t rescue_kicks;     /* further iterations with a pseudo-random victim (default 128) */
+    uint32_t max_retries;      /* CAS races tolerated per entry before it is pended (default 64) */
+    uint32_t max_fail_rounds;  /* upsize + re-insert rounds per dyc_insert call (default 8) */
+    double lower_bound;        /* downsize when count < lower * capacity (default 0.50) */
+    double upper_bound;        /* upsize while count + n > upper * capacity (default 0.90) */
+    uint32_t auto_resize;      /* 1: apply the fill bounds in dyc_insert / dyc_erase (default 0) */
+    uint32_t find_lanes;       /* lanes per key for find/erase: 4, 8 or 16 (default 8) */
+    uint32_t insert_lanes;     /* lanes per key for insert: 4, 8 or 16 (default 16) */
+    uint32_t block_threads;    /* threads per block, multiple of 64, <= 256 (default 256) */
+    uint32_t max_blocks;       /* grid cap, 0 = one wave per 64 keys (default 0) */
+    uint32_t reserved0;
+    uint64_t min_capacity;     /* no automatic downsize at or below this many slots
+                                  (default 5033164 = TABLE_NUM*1024*1024*1.2, api.cpp:496) */
+} dyc_config;
+
+typedef struct dyc_stats {
+    uint64_t count;             /* live entries: placed + pending */
+    uint64_t capacity;          /* slots = 16 * sum(buckets) */
+    uint32_t buckets[4];        /* buckets per subtable */
+    uint64_t pending;           /* entries owned but not placed */
+    uint64_t kicks;             /* cumulative successful exchanges (insert + re-insert) */
+    uint64_t last_kicks;        /* exchanges of the most recent dyc_insert */
+    uint64_t kick_failures;     /* cumulative entries that exhausted the eviction bound */
+    uint64_t cas_retries;       /* cumulative CAS races */
+    uint64_t rejected;          /* cumulative reserved keys skipped by insert */
+    uint64_t resizes_up;
+    uint64_t resizes_down;
+    uint64_t lost;              /* must stay 0 */
+} dyc_stats;
+
+/* Defaults (see field comments). */
+int dyc_config_init(dyc_config *cfg);
+
+/* Replaces hashAPI::hashAPI(TYPE size) (api.cpp:26-91): 4 subtables of
+ * nextPrime(capacity_hint / 64) buckets each, rounded up to even (api.cpp:47-56), zeroed. */
+int dyc_create(const dyc_config *cfg, uint64_t capacity_hint, dyc_table **out);
+
+/* Replaces hashAPI::~hashAPI (api.cpp:93-109). Frees everything the table owns. */
+int dyc_destroy(dyc_table *t);
+
+/* Replaces gpu_lp_insert (cuckoo.h:100-104, cuckoo.cu:858-877) and the insert half of
+ * hashAPI::hash_insert / hash_insert_batch (api.cpp:184-265, 353-432).
+ * With auto_resize, upsizes first while count + n > upper * capacity (api.cpp:360-369).
+ * Synchronises `stream` once at the end (it reads the failure counters). */
+int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
+
+/* Replaces gpu_lp_search (cuckoo.h:106-108, cuckoo.cu:881-899) / hashAPI::hash_search
+ * (api.cpp:267-285). vals[i] = the value of keys[i] or 0 on a miss (what the reference's
+ * pre-zeroed output shows); hit[i] = 1/0. Either output may be NULL. Asynchronous. */
+int dyc_find(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, dyc_stream stream);
+
+/* Replaces gpu_lp_delete (cuckoo.h:110-112, cuckoo.cu:901-916) / hashAPI::hash_delete
+ * (api.cpp:289-303). Clears every copy of the key in the first bucket (pos1, then pos2) that
+ * holds it; deleted[i] = 1 iff this call removed it (may be NULL; the reference's `ans` is
+ * unused). With auto_resize, downsizes once when count < lower * capacity and capacity >
+ * min_capacity (api.cpp:490-505). Synchronises `stream` once (it reads the delete count). */
+int dyc_erase(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream);
+
+/* Replaces hashAPI::resize_up (api.cpp:112-141) + GPU_cuckoo_resize_up (cuckoo.cu:731-758):
+ * doubles the subtable chosen by api.cpp:116-122. Synchronous. */
+int dyc_resize_up(dyc_table *t, dyc_stream stream);
+
+/* Replaces hashAPI::resize_low (api.cpp:143-173) + GPU_cuckoo_resize_down (cuckoo.cu:762-788):
+ * halves the subtable chosen by api.cpp:147-153 and re-inserts the entries that no longer fit
+ * into the other subtable of their pair. DYC_ERR_ODD_BUCKETS if that subtable's bucket count
+ * is odd (the reference reads past the end there, cuckoo.cu:653,767-769). Synchronous. */
+int dyc_resize_down(dyc_table *t, dyc_stream stream);
+
+/* Counters and geometry (no reference counterpart beyond the occupancy print of
+ * computer_table, cuckoo.cu:818-848). Does not touch the device. */
+int dyc_get_stats(const dyc_table *t, dyc_stats *out);
+
+/* Occupied slots counted on the device (computer_table, cuckoo.cu:818-848). Synchronous. */
+int dyc_count_occupied(dyc_table *t, uint64_t *occupied, dyc_stream stream);
+
+/* Empty the table (every slot zeroed, count and pending reset); sizes are kept. Asynchronous.
+ * No reference counterpart (the reference allocates a new table, api.cpp:58-74). */
+int dyc_clear(dyc_table *t, dyc_stream stream);
+
+/* Diagnostic, not on the hot path: the number of buckets the find order (pos1, then pos2 on a
+ * miss, cuckoo.cu:482-497) reads for keys[0..n), and how many keys pos1 resolves. The
+ * algorithmic bytes of a find are 8 + 128 * probes / n (+1 with hit flags). Synchronous. */
+int dyc_probe_count(dyc_table *t, const uint32_t *keys, uint64_t n, uint64_t *probes, uint64_t *pos1_hits,
+                    dyc_stream stream);
+
+/* Copy subtable `sub` (buckets[sub] * 16 uint64 slots) into host memory. Synchronous. */
+int dyc_dump(dyc_table *t, uint32_t sub, uint64_t *host_slots, uint64_t n_slots, dyc_stream stream);
+
+/* Replace the table contents with host images: nb[s] buckets per subtable, images[s] holding
+ * nb[s] * 16 slots. Pending entries are discarded and count is recomputed. Synchronous. */
+int dyc_load(dyc_table *t, const uint32_t nb[4], const uint64_t *const images[4], dyc_stream stream);
+
+/* Copy up to `cap` pending entries to host memory; *n_out = number pending. */
+int dyc_pending(dyc_table *t, uint64_t *host_entries, uint64_t cap, uint64_t *n_out, dyc_stream stream);
+
+/* Message of the calling thread's last error ("" if none). */
+const char *dyc_last_error(void);
+
+/* DYC_ABI_VERSION of the loaded library. */
+int dyc_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DYC_H */

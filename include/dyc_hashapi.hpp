@@ -1,0 +1,185 @@
+// dyc_hashapi.hpp -- the reference's C++ host class, rebuilt over the C ABI of dyc.h.
+//
+// Same class name and method names as GPU/dyhash/api.h:9-55 so the reference drivers' call
+// sequences compile against it unchanged:
+//     hashAPI h(size);  h.hash_insert(k, v, n);  h.hash_search(k, ans, n);
+//     h.hash_delete(k, ans, n);  h.resize_up();  h.resize_low();
+//     h.set_data_to_gpu(k, v, n);  h.hash_insert_batch();  h.hash_search_batch();
+//     h.hash_delete_batch();  h.batch_reset();
+// Host arrays in and out, synchronous, like the reference (api.cpp:184-303). Differences:
+//   - hash_search writes 0 for a miss exactly as the reference's pre-zeroed output does;
+//     hash_search_hits() also returns the hit flags;
+//   - hash_delete writes per-key deleted flags into ans (the reference leaves ans unused);
+//   - errors throw dyc_error instead of aborting in checkCudaErrors / gtest EXPECT_*.
+// Header-only; link with libdyc.so and the HIP runtime (-lamdhip64).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dyc.h"
+
+#ifndef DYC_HASHAPI_TYPE
+#define DYC_HASHAPI_TYPE unsigned int  // dy_hash.h:16
+#endif
+
+class dyc_error : public std::runtime_error {
+  public:
+    dyc_error(int code, const std::string &what) : std::runtime_error(what), code_(code) {}
+    int code() const { return code_; }
+
+  private:
+    int code_;
+};
+
+class hashAPI {
+  public:
+    using TYPE = DYC_HASHAPI_TYPE;
+    static constexpr TYPE BATCH_SIZE = 100000;  // test_uilt.h:5
+    static constexpr int BATCH_NUM = 1000;      // test_uilt.h:4
+
+    // api.cpp:26-91: `size` is a slot-count hint. `cfg` may carry bounds, kick bound etc.
+    explicit hashAPI(TYPE size, const dyc_config *cfg = nullptr)
+    {
+        check(dyc_create(cfg, size, &t_), "dyc_create");
+    }
+    ~hashAPI()
+    {
+        release_pool();
+        dyc_destroy(t_);
+    }
+    hashAPI(const hashAPI &) = delete;
+    hashAPI &operator=(const hashAPI &) = delete;
+
+    void hash_insert(TYPE *k, TYPE *v, TYPE size)
+    {
+        Dev<uint32_t> dk(size), dv(size);
+        dk.put(k, size);
+        dv.put(v, size);
+        check(dyc_insert(t_, dk.p, dv.p, size, nullptr), "dyc_insert");
+    }
+
+    void hash_search(TYPE *k, TYPE *ans, TYPE size) { hash_search_hits(k, ans, nullptr, size); }
+
+    void hash_search_hits(TYPE *k, TYPE *ans, uint8_t *hit, TYPE size)
+    {
+        Dev<uint32_t> dk(size), dv(size);
+        Dev<uint8_t> dh(size);
+        dk.put(k, size);
+        check(dyc_find(t_, dk.p, dv.p, dh.p, size, nullptr), "dyc_find");
+        dv.get(ans, size);
+        if (hit) dh.get(hit, size);
+    }
+
+    void hash_delete(TYPE *k, TYPE *ans, TYPE size)
+    {
+        Dev<uint32_t> dk(size);
+        Dev<uint8_t> dd(size);
+        dk.put(k, size);
+        check(dyc_erase(t_, dk.p, dd.p, size, nullptr), "dyc_erase");
+        if (ans) {
+            std::vector<uint8_t> flags(size);
+            dd.get(flags.data(), size);
+            for (TYPE i = 0; i < size; ++i) ans[i] = flags[i];
+        }
+    }
+
+    // api.cpp:112-173 ("public for test")
+    void resize_up() { check(dyc_resize_up(t_, nullptr), "dyc_resize_up"); }
+    void resize_low()
+    {
+        int rc = dyc_resize_down(t_, nullptr);
+        if (rc != DYC_ERR_ODD_BUCKETS) check(rc, "dyc_resize_down");
+    }
+
+    // api.cpp:309-344: stage a device pool, then walk it in BATCH_SIZE slices
+    bool set_data_to_gpu(TYPE *key, TYPE *value, TYPE size)
+    {
+        release_pool();
+        pool_n_ = size;
+        if (hipMalloc(&pool_k_, sizeof(uint32_t) * (size_t)size) != hipSuccess ||
+            hipMalloc(&pool_v_, sizeof(uint32_t) * (size_t)size) != hipSuccess ||
+            hipMalloc(&check_, sizeof(uint32_t) * (size_t)BATCH_SIZE) != hipSuccess)
+            throw dyc_error(DYC_ERR_OOM, "set_data_to_gpu: hipMalloc failed");
+        hip(hipMemcpy(pool_k_, key, sizeof(uint32_t) * (size_t)size, hipMemcpyHostToDevice));
+        hip(hipMemcpy(pool_v_, value, sizeof(uint32_t) * (size_t)size, hipMemcpyHostToDevice));
+        return true;
+    }
+    void hash_insert_batch()
+    {
+        TYPE lo, n;
+        slice(batch_num_to_op_++, lo, n);
+        check(dyc_insert(t_, pool_k_ + lo, pool_v_ + lo, n, nullptr), "dyc_insert");
+    }
+    void hash_search_batch()
+    {
+        TYPE lo, n;
+        slice(batch_search_num_to_op_++, lo, n);
+        check(dyc_find(t_, pool_k_ + lo, check_, nullptr, n, nullptr), "dyc_find");
+        hip(hipDeviceSynchronize());
+    }
+    void hash_delete_batch()
+    {
+        TYPE lo, n;
+        slice(batch_del_num_to_op_++, lo, n);
+        check(dyc_erase(t_, pool_k_ + lo, nullptr, n, nullptr), "dyc_erase");
+    }
+    void batch_reset() { batch_num_to_op_ = batch_search_num_to_op_ = batch_del_num_to_op_ = 0; }
+
+    dyc_stats stats() const
+    {
+        dyc_stats s;
+        check(dyc_get_stats(t_, &s), "dyc_get_stats");
+        return s;
+    }
+    dyc_table *handle() { return t_; }
+
+  private:
+    template <typename T>
+    struct Dev {
+        T *p = nullptr;
+        explicit Dev(size_t n)
+        {
+            if (n && hipMalloc(&p, sizeof(T) * n) != hipSuccess) throw dyc_error(DYC_ERR_OOM, "hipMalloc failed");
+        }
+        ~Dev()
+        {
+            if (p) (void)hipFree(p);
+        }
+        void put(const void *h, size_t n) { hip(hipMemcpy(p, h, sizeof(T) * n, hipMemcpyHostToDevice)); }
+        void get(void *h, size_t n) { hip(hipMemcpy(h, p, sizeof(T) * n, hipMemcpyDeviceToHost)); }
+    };
+
+    static void hip(hipError_t e)
+    {
+        if (e != hipSuccess) throw dyc_error(DYC_ERR_HIP, hipGetErrorString(e));
+    }
+    static void check(int rc, const char *where)
+    {
+        if (rc != DYC_OK) throw dyc_error(rc, std::string(where) + ": " + dyc_last_error());
+    }
+    void slice(int i, TYPE &lo, TYPE &n) const
+    {
+        if (i >= BATCH_NUM || (uint64_t)i * BATCH_SIZE >= pool_n_)
+            throw dyc_error(DYC_ERR_INVALID, "batch index beyond the staged pool");
+        lo = (TYPE)(i * BATCH_SIZE);
+        n = (pool_n_ - lo < BATCH_SIZE) ? pool_n_ - lo : BATCH_SIZE;
+    }
+    void release_pool()
+    {
+        if (pool_k_) (void)hipFree(pool_k_);
+        if (pool_v_) (void)hipFree(pool_v_);
+        if (check_) (void)hipFree(check_);
+        pool_k_ = pool_v_ = check_ = nullptr;
+    }
+
+    dyc_table *t_ = nullptr;
+    uint32_t *pool_k_ = nullptr, *pool_v_ = nullptr, *check_ = nullptr;
+    uint64_t pool_n_ = 0;
+    int batch_num_to_op_ = 0, batch_search_num_to_op_ = 0, batch_del_num_to_op_ = 0;
+};

@@ -1,0 +1,299 @@
+"""CPU oracle for DyCuckoo -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import this module, and only as the checker (or as the timed CPU baseline). The product
+package ``databases_gpu_amd`` never imports it.
+
+``liboracle.so`` (built from ``oracle/dyc_oracle.c`` by ``oracle/Makefile``) restates the
+reference kernels one key at a time. ``OracleTable`` adds the host-side policy of
+``GPU/dyhash/api.cpp`` (sizing ``api.cpp:47-56``, resize choice ``api.cpp:112-173``, fill
+bounds ``api.cpp:175-181,353-369,490-505``) with the decisions documented in
+``dyc_oracle.h``: unsigned packing, no dropped entries, reserved keys 0/0xFFFFFFFF,
+true-occupancy bookkeeping.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+SUBTABLES = 4
+SLOTS = 16
+PRIME = 294967291
+KICK_BOUND = 15
+RESCUE_KICKS = 0
+NO_TABLE = 100
+RESERVED = (0, 0xFFFFFFFF)
+
+
+class _OrcTable(ctypes.Structure):
+    _fields_ = [("sub", ctypes.c_void_p * SUBTABLES), ("nb", ctypes.c_uint32 * SUBTABLES)]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        u32, u64, vp = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p
+        for name in ("orc_h1", "orc_h2", "orc_h3", "orc_h4", "orc_h5", "orc_pair", "orc_pos1",
+                     "orc_pos2", "orc_next_prime"):
+            getattr(L, name).restype = u32
+            getattr(L, name).argtypes = [u32]
+        L.orc_is_prime.restype = ctypes.c_int
+        L.orc_is_prime.argtypes = [u32]
+        L.orc_hash_t.restype = u32
+        L.orc_hash_t.argtypes = [u32, u32]
+        L.orc_get_pos.restype = u32
+        L.orc_get_pos.argtypes = [u32, u32]
+        L.orc_bucket.restype = u32
+        L.orc_bucket.argtypes = [u32, u32, u32]
+        L.orc_initial_buckets.restype = u32
+        L.orc_initial_buckets.argtypes = [u64]
+        L.orc_rescue_lane.restype = u32
+        L.orc_rescue_lane.argtypes = [u32, u32]
+        L.orc_choose_up.restype = u32
+        L.orc_choose_up.argtypes = [vp]
+        L.orc_choose_down.restype = u32
+        L.orc_choose_down.argtypes = [vp]
+        L.orc_hash_batch.restype = None
+        L.orc_hash_batch.argtypes = [vp, u64, u32, vp]
+        L.orc_insert_one.restype = ctypes.c_int
+        L.orc_insert_one.argtypes = [vp, u32, u32, u32, u32, vp, vp]
+        L.orc_insert_entry.restype = ctypes.c_int
+        L.orc_insert_entry.argtypes = [vp, u64, u32, u32, u32, vp, vp]
+        L.orc_insert.restype = u64
+        L.orc_insert.argtypes = [vp, vp, vp, u64, u32, u32, vp, vp, u64, vp]
+        L.orc_find.restype = None
+        L.orc_find.argtypes = [vp, vp, u64, vp, vp, vp]
+        L.orc_find_mt.restype = None
+        L.orc_find_mt.argtypes = [vp, vp, u64, vp, vp, ctypes.c_int]
+        L.orc_max_threads.restype = ctypes.c_int
+        L.orc_max_threads.argtypes = []
+        L.orc_erase.restype = u64
+        L.orc_erase.argtypes = [vp, vp, u64, vp]
+        L.orc_upsize.restype = None
+        L.orc_upsize.argtypes = [vp, u32, vp, u32]
+        L.orc_downsize_compact.restype = u64
+        L.orc_downsize_compact.argtypes = [vp, u32, vp, vp, u64]
+        L.orc_count.restype = u64
+        L.orc_count.argtypes = [vp]
+        L.orc_check_invariant.restype = u64
+        L.orc_check_invariant.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ---------------------------------------------------------------- scalar / batch hashes
+
+def hash_batch(keys, which: int) -> np.ndarray:
+    """which: 1..5 = h1..h5 (cuckoo.cu:27-64,110-120), 6 = pair index (cuckoo.cu:190-193)."""
+    k = _u32(keys)
+    out = np.empty_like(k)
+    lib().orc_hash_batch(_p(k), k.size, which, _p(out))
+    return out
+
+
+def initial_buckets(capacity_hint: int) -> int:
+    return int(lib().orc_initial_buckets(capacity_hint))
+
+
+def pack(keys, vals) -> np.ndarray:
+    return (_u32(keys).astype(np.uint64) << np.uint64(32)) | _u32(vals).astype(np.uint64)
+
+
+# ---------------------------------------------------------------- table with host policy
+
+class OracleTable:
+    """Sequential replay of a DyCuckoo table plus the reference host policy.
+
+    Images are numpy uint64 arrays, one per subtable, ``nb[t] * 16`` slots each: the same
+    layout ``dyc_dump`` writes.
+    """
+
+    def __init__(self, capacity_hint: int = 0, *, nb=None, kick_bound: int = KICK_BOUND,
+                 rescue_kicks: int = RESCUE_KICKS,
+                 lower: float = 0.5, upper: float = 0.9, auto_resize: bool = False,
+                 min_capacity: float = SUBTABLES * 1024 * 1024 * 1.2, max_fail_rounds: int = 8):
+        if nb is None:
+            b = initial_buckets(capacity_hint)
+            nb = [b] * SUBTABLES
+        self.nb = [int(x) for x in nb]
+        self.sub = [np.zeros(x * SLOTS, dtype=np.uint64) for x in self.nb]
+        self.kick_bound = kick_bound
+        self.rescue_kicks = rescue_kicks
+        self.lower, self.upper = lower, upper
+        self.auto_resize = auto_resize
+        self.min_capacity = min_capacity
+        self.max_fail_rounds = max_fail_rounds
+        self.count = 0
+        self.pending = np.zeros(0, dtype=np.uint64)  # entries owned but not placed
+        self.resizes = []  # ("up"|"down", subtable, new_nb)
+        self.kicks_total = 0
+        self.fail_total = 0
+
+    # -- image plumbing
+    def _struct(self):
+        s = _OrcTable()
+        for t in range(SUBTABLES):
+            s.sub[t] = self.sub[t].ctypes.data
+            s.nb[t] = self.nb[t]
+        return s
+
+    @property
+    def capacity(self) -> int:
+        return sum(self.nb) * SLOTS
+
+    def load_images(self, images):
+        self.sub = [np.ascontiguousarray(x, dtype=np.uint64).copy() for x in images]
+        self.nb = [x.size // SLOTS for x in self.sub]
+        self.count = int(sum(int(np.count_nonzero(x)) for x in self.sub))
+
+    def images(self):
+        return [x.copy() for x in self.sub]
+
+    def entries(self) -> np.ndarray:
+        """All non-empty entries, sorted (the key->value multiset)."""
+        allv = np.concatenate(self.sub + [self.pending])
+        return np.sort(allv[allv != 0])
+
+    def check_invariant(self) -> int:
+        s = self._struct()
+        return int(lib().orc_check_invariant(ctypes.byref(s)))
+
+    # -- operations
+    def _insert_entries(self, entries: np.ndarray, pre: int):
+        """Continue eviction chains for entries already in hand (residual / pending path)."""
+        L = lib()
+        s = self._struct()
+        fails = []
+        kicks = ctypes.c_uint32(0)
+        carry = ctypes.c_uint64(0)
+        for e in entries.tolist():
+            e = int(e)
+            if pre == NO_TABLE:
+                k = e >> 32
+                p = NO_TABLE if (k & 1) == 0 else int(L.orc_get_pos(L.orc_pair(k), NO_TABLE))
+            else:
+                p = pre
+            ok = L.orc_insert_entry(ctypes.byref(s), e, p, self.kick_bound, self.rescue_kicks,
+                                    ctypes.byref(kicks),
+                                    ctypes.byref(carry))
+            self.kicks_total += kicks.value
+            if not ok:
+                fails.append(carry.value)
+        return np.array(fails, dtype=np.uint64)
+
+    def _place_pending(self):
+        rounds = 0
+        while self.pending.size and rounds < self.max_fail_rounds:
+            self.resize_up()
+            pend, self.pending = self.pending, np.zeros(0, dtype=np.uint64)
+            self.pending = self._insert_entries(pend, NO_TABLE)
+            rounds += 1
+
+    def insert(self, keys, vals):
+        """Batch insert in input order (cuckoo_insert); returns (failures_before_retry, rejected)."""
+        keys, vals = _u32(keys), _u32(vals)
+        n = keys.size
+        if self.auto_resize:
+            while self.count + n > self.upper * self.capacity:
+                self.resize_up()
+        L = lib()
+        s = self._struct()
+        kicks = np.zeros(n, dtype=np.uint32)
+        fail = np.zeros(max(n, 1), dtype=np.uint64)
+        rej = ctypes.c_uint64(0)
+        nfail = int(L.orc_insert(ctypes.byref(s), _p(keys), _p(vals), n, self.kick_bound,
+                                 self.rescue_kicks, _p(kicks), _p(fail), fail.size, ctypes.byref(rej)))
+        self.kicks_total += int(kicks.sum(dtype=np.uint64))
+        self.fail_total += nfail
+        self.count += n - int(rej.value)
+        self.pending = np.concatenate([self.pending, fail[:nfail]])
+        self._place_pending()
+        self.last_kicks = kicks
+        return nfail, int(rej.value)
+
+    def find(self, keys):
+        keys = _u32(keys)
+        vals = np.zeros(keys.size, dtype=np.uint32)
+        hit = np.zeros(keys.size, dtype=np.uint8)
+        probes = ctypes.c_uint64(0)
+        s = self._struct()
+        lib().orc_find(ctypes.byref(s), _p(keys), keys.size, _p(vals), _p(hit), ctypes.byref(probes))
+        self.last_probes = int(probes.value)
+        return vals, hit
+
+    def erase(self, keys):
+        keys = _u32(keys)
+        deleted = np.zeros(keys.size, dtype=np.uint8)
+        s = self._struct()
+        ndel = int(lib().orc_erase(ctypes.byref(s), _p(keys), keys.size, _p(deleted)))
+        self.count -= ndel
+        if (self.auto_resize and self.count < self.lower * self.capacity
+                and self.capacity > self.min_capacity):
+            self.resize_down()
+        return deleted
+
+    def resize_up(self) -> int:
+        nb = (ctypes.c_uint32 * SUBTABLES)(*self.nb)
+        t = int(lib().orc_choose_up(ctypes.byref(nb)))
+        new = np.zeros(self.nb[t] * 2 * SLOTS, dtype=np.uint64)
+        lib().orc_upsize(_p(self.sub[t]), self.nb[t], _p(new), t)
+        self.sub[t] = new
+        self.nb[t] *= 2
+        self.resizes.append(("up", t, self.nb[t]))
+        return t
+
+    def resize_down(self, defer_residuals: bool = False):
+        """Halve the subtable chosen by api.cpp:147-153. Returns (t, residuals) or None when
+        that subtable has an odd bucket count (pairwise compaction is undefined there)."""
+        nb = (ctypes.c_uint32 * SUBTABLES)(*self.nb)
+        t = int(lib().orc_choose_down(ctypes.byref(nb)))
+        if self.nb[t] % 2:
+            return None
+        new = np.zeros(self.nb[t] // 2 * SLOTS, dtype=np.uint64)
+        res = np.zeros(self.nb[t] * SLOTS // 2 + 1, dtype=np.uint64)
+        nres = int(lib().orc_downsize_compact(_p(self.sub[t]), self.nb[t], _p(new), _p(res),
+                                               res.size))
+        self.sub[t] = new
+        self.nb[t] //= 2
+        self.resizes.append(("down", t, self.nb[t]))
+        res = res[:nres].copy()
+        if not defer_residuals:
+            fails = self._insert_entries(res, t)
+            if fails.size:
+                self.pending = np.concatenate([self.pending, fails])
+                self._place_pending()
+        return t, res
+
+
+def count_probes(images, keys) -> int:
+    """Buckets the pos1-then-pos2 find order reads for `keys` over a dumped image."""
+    t = OracleTable(nb=[x.size // SLOTS for x in images])
+    t.sub = [np.ascontiguousarray(x, dtype=np.uint64) for x in images]
+    t.find(keys)
+    return t.last_probes

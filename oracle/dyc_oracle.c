@@ -1,0 +1,427 @@
+/*
+ * dyc_oracle.c -- sequential CPU restatement of DyCuckoo (GPU/dyhash). TEST INFRASTRUCTURE:
+ * a checker for the HIP library, never part of it. See dyc_oracle.h for the contract.
+ *
+ * Every function cites the reference lines it restates (paths relative to the reference
+ * repository root). Parity status of each part is recorded in the tests that pin it:
+ *   hashes h1..h5     -- pinned by independent inverse functions (tests/test_oracle_hashes.py)
+ *   bucket / pair     -- pinned by big-integer arithmetic and the exhaustive pair table
+ *   sizing            -- pinned by a sieve (brute force)
+ *   insert replay     -- pinned by hand-derived placements, the structural invariant and
+ *                        set equality against a dict
+ *   find / erase      -- pinned by a dict of the inserted set
+ *   upsize / downsize -- pinned by the bucket-arithmetic identities h%2n in {h%n, h%n+n}
+ *                        and set preservation
+ */
+#include "dyc_oracle.h"
+
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define KEY_OF(e) ((uint32_t)((e) >> 32))
+#define VAL_OF(e) ((uint32_t)((e)&0xffffffffu))
+#define PACK(k, v) ((((uint64_t)(uint32_t)(k)) << 32) | (uint64_t)(uint32_t)(v))
+
+static int reserved_key(uint32_t k) { return k == 0u || k == 0xffffffffu; }
+
+/* ---------------------------------------------------------------- hashes */
+
+/* cuckoo.cu:27-36 -- Thomas Wang's 32-bit shift hash */
+uint32_t orc_h1(uint32_t k)
+{
+    k = ~k + (k << 15);
+    k = k ^ (k >> 12);
+    k = k + (k << 2);
+    k = k ^ (k >> 4);
+    k = k * 2057u;
+    k = k ^ (k >> 16);
+    return k;
+}
+
+/* cuckoo.cu:39-48 -- Robert Jenkins' 6-shift 32-bit hash */
+uint32_t orc_h2(uint32_t a)
+{
+    a = (a + 0x7ed55d16u) + (a << 12);
+    a = (a ^ 0xc761c23cu) ^ (a >> 19);
+    a = (a + 0x165667b1u) + (a << 5);
+    a = (a + 0xd3a2646cu) ^ (a << 9);
+    a = (a + 0xfd7046c5u) + (a << 3);
+    a = (a ^ 0xb55a4f09u) ^ (a >> 16);
+    return a;
+}
+
+/* cuckoo.cu:50-53 */
+uint32_t orc_h3(uint32_t k) { return ((k ^ 59064253u) + 72355969u) % ORC_PRIME; }
+
+/* cuckoo.cu:56-64 -- Thomas Wang's 32-bit shift-multiply hash */
+uint32_t orc_h4(uint32_t a)
+{
+    a = (a ^ 61u) ^ (a >> 16);
+    a = a + (a << 3);
+    a = a ^ (a >> 4);
+    a = a * 0x27d4eb2du;
+    a = a ^ (a >> 15);
+    return a;
+}
+
+/* cuckoo.cu:110-120 -- Jenkins' 7-shift hash, used only for the pair choice */
+uint32_t orc_h5(uint32_t a)
+{
+    a -= (a << 6);
+    a ^= (a >> 17);
+    a -= (a << 9);
+    a ^= (a << 4);
+    a -= (a << 3);
+    a ^= (a << 10);
+    a ^= (a >> 15);
+    return a;
+}
+
+/* cuckoo.cu:80-89: the hash of subtable t, reduced modulo PRIME_uint (dy_hash.h:24) */
+uint32_t orc_hash_t(uint32_t k, uint32_t t)
+{
+    switch (t % 4u) {
+    case 0: return orc_h1(k) % ORC_PRIME;
+    case 1: return orc_h2(k) % ORC_PRIME;
+    case 2: return orc_h3(k) % ORC_PRIME;
+    default: return orc_h4(k) % ORC_PRIME;
+    }
+}
+
+/* cuckoo.cu:76-90 with get_table_length = Lsize >> 4 (cuckoo.cu:17-18) */
+uint32_t orc_bucket(uint32_t k, uint32_t t, uint32_t nbuckets) { return orc_hash_t(k, t) % nbuckets; }
+
+/* cuckoo.cu:190-193 */
+uint32_t orc_pair(uint32_t k) { return orc_h5(k) % 6u; }
+
+/* cuckoo.cu:197-204 */
+uint32_t orc_pos1(uint32_t p)
+{
+    if (p & 1u) return 0u;
+    if (p & 2u) return 2u;
+    return 1u;
+}
+
+/* cuckoo.cu:206-213 */
+uint32_t orc_pos2(uint32_t p)
+{
+    if (p & 1u) return (p >> 1) + 1u;
+    if (p & 2u) return 3u;
+    return (p >> 2) + 2u;
+}
+
+/* cuckoo.cu:310-312 */
+uint32_t orc_get_pos(uint32_t p, uint32_t pre) { return pre == orc_pos1(p) ? orc_pos2(p) : orc_pos1(p); }
+
+void orc_hash_batch(const uint32_t *keys, uint64_t n, uint32_t which, uint32_t *out)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t k = keys[i];
+        switch (which) {
+        case 1: out[i] = orc_h1(k); break;
+        case 2: out[i] = orc_h2(k); break;
+        case 3: out[i] = orc_h3(k); break;
+        case 4: out[i] = orc_h4(k); break;
+        case 5: out[i] = orc_h5(k); break;
+        case 6: out[i] = orc_pair(k); break;
+        default: out[i] = 0; break;
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- sizing */
+
+/* api.cpp:4-11. Note the reference reports 1 as prime (the 6k+-1 test admits it). */
+int orc_is_prime(uint32_t n)
+{
+    int64_t num = (int32_t)n; /* the reference takes an int */
+    if (num == 2 || num == 3) return 1;
+    if (num % 6 != 1 && num % 6 != 5) return 0;
+    for (int64_t i = 5; i * i <= num; i += 6)
+        if (num % i == 0 || num % (i + 2) == 0) return 0;
+    return 1;
+}
+
+/* api.cpp:13-22 */
+uint32_t orc_next_prime(uint32_t n)
+{
+    while (!orc_is_prime(n)) ++n;
+    return n;
+}
+
+/* api.cpp:47-51: s = nextPrime(size / (TABLE_NUM*BUCKET_SIZE)); buckets = s rounded up to even */
+uint32_t orc_initial_buckets(uint64_t capacity_hint)
+{
+    uint32_t s = (uint32_t)(capacity_hint / (ORC_SUBTABLES * ORC_SLOTS));
+    s = orc_next_prime(s);
+    return (s & 1u) ? s + 1u : s;
+}
+
+/* ---------------------------------------------------------------- resize choice */
+
+/* api.cpp:116-122: the first subtable that is half the size of its left neighbour, else 0 */
+uint32_t orc_choose_up(const uint32_t nb[ORC_SUBTABLES])
+{
+    for (uint32_t i = 0; i + 1 < ORC_SUBTABLES; ++i)
+        if (nb[i] == 2u * nb[i + 1]) return i + 1;
+    return 0;
+}
+
+/* api.cpp:147-153: the first subtable larger than its right neighbour, else the last */
+uint32_t orc_choose_down(const uint32_t nb[ORC_SUBTABLES])
+{
+    for (uint32_t i = 0; i + 1 < ORC_SUBTABLES; ++i)
+        if (nb[i] > nb[i + 1]) return i;
+    return ORC_SUBTABLES - 1;
+}
+
+/* ---------------------------------------------------------------- insert */
+
+/* Victim lane for the rescue phase that follows the reference's kick bound: a multiplicative
+ * hash of the in-hand key and the iteration number (top 4 bits). Not in the reference, which
+ * drops the entry at the kick bound (cuckoo.cu:422-431); the rescue turns its nearly
+ * deterministic first-lane walk into a pseudo-random walk so that chains that cycle under
+ * the reference rule terminate. */
+uint32_t orc_rescue_lane(uint32_t key, uint32_t it)
+{
+    return (key * 0x9E3779B1u + it * 0x632BE5ABu) >> 28;
+}
+
+/* The eviction loop of warp_insert (cuckoo.cu:377-421), one key at a time:
+ *   t      = the other table of the in-hand key's pair, relative to `pre`
+ *   victim = first empty slot; else first slot whose resident pair index differs from t
+ *            (the reference compares a pair index with a table number, cuckoo.cu:398-400);
+ *            else slot (iteration & 15)
+ *   exchange, carry the old entry; done when the old entry is empty. */
+int orc_insert_entry(orc_table *tb, uint64_t entry, uint32_t pre, uint32_t kick_bound,
+                     uint32_t rescue_kicks, uint32_t *kicks, uint64_t *carry)
+{
+    uint32_t n_kicks = 0;
+    const uint32_t total = kick_bound + rescue_kicks;
+    for (uint32_t it = 0; it < total; ++it) {
+        uint32_t k = KEY_OF(entry);
+        uint32_t t = orc_get_pos(orc_pair(k), pre);
+        uint64_t *b = tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS;
+        int lane = -1;
+        for (int i = 0; i < ORC_SLOTS; ++i)
+            if (KEY_OF(b[i]) == 0u) { lane = i; break; }
+        if (lane < 0 && it < kick_bound) {
+            for (int i = 0; i < ORC_SLOTS; ++i)
+                if (orc_pair(KEY_OF(b[i])) != t) { lane = i; break; }
+            if (lane < 0) lane = (int)(it & 15u);
+        }
+        if (lane < 0) lane = (int)orc_rescue_lane(k, it);
+        uint64_t old = b[lane];
+        b[lane] = entry;
+        entry = old;
+        ++n_kicks;
+        if (entry == 0) {
+            if (kicks) *kicks = n_kicks;
+            return 1;
+        }
+        pre = t;
+    }
+    if (kicks) *kicks = n_kicks;
+    if (carry) *carry = entry;
+    return 0;
+}
+
+/* cuckoo_insert (cuckoo.cu:336-348): odd keys start in pos2, even keys in pos1 */
+int orc_insert_one(orc_table *tb, uint32_t key, uint32_t value, uint32_t kick_bound,
+                   uint32_t rescue_kicks, uint32_t *kicks, uint64_t *carry)
+{
+    uint32_t pre = ORC_NO_TABLE;
+    if (key & 1u) pre = orc_get_pos(orc_pair(key), ORC_NO_TABLE);
+    return orc_insert_entry(tb, PACK(key, value), pre, kick_bound, rescue_kicks, kicks, carry);
+}
+
+uint64_t orc_insert(orc_table *tb, const uint32_t *keys, const uint32_t *vals, uint64_t n,
+                    uint32_t kick_bound, uint32_t rescue_kicks, uint32_t *kicks_out,
+                    uint64_t *fail_out, uint64_t fail_cap, uint64_t *rejected)
+{
+    uint64_t nfail = 0, nrej = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t kicks = 0;
+        uint64_t carry = 0;
+        if (reserved_key(keys[i])) {
+            ++nrej;
+            if (kicks_out) kicks_out[i] = 0;
+            continue;
+        }
+        if (!orc_insert_one(tb, keys[i], vals[i], kick_bound, rescue_kicks, &kicks, &carry)) {
+            if (fail_out && nfail < fail_cap) fail_out[nfail] = carry;
+            ++nfail;
+        }
+        if (kicks_out) kicks_out[i] = kicks;
+    }
+    if (rejected) *rejected = nrej;
+    return nfail;
+}
+
+/* ---------------------------------------------------------------- find / erase */
+
+static int first_match(const uint64_t *b, uint32_t k)
+{
+    for (int i = 0; i < ORC_SLOTS; ++i)
+        if (KEY_OF(b[i]) == k) return i;
+    return -1;
+}
+
+/* cuckoo.cu:478-497: pos1 bucket, then pos2 bucket; a miss leaves 0 (api.cpp:275) */
+void orc_find(const orc_table *tb, const uint32_t *keys, uint64_t n, uint32_t *vals_out,
+              uint8_t *hit_out, uint64_t *probes_out)
+{
+    uint64_t probes = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t k = keys[i], v = 0;
+        uint8_t hit = 0;
+        if (!reserved_key(k)) {
+            uint32_t p = orc_pair(k);
+            uint32_t t = orc_pos1(p);
+            const uint64_t *b = tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS;
+            int lane = first_match(b, k);
+            ++probes;
+            if (lane < 0) {
+                t = orc_pos2(p);
+                b = tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS;
+                lane = first_match(b, k);
+                ++probes;
+            }
+            if (lane >= 0) {
+                v = VAL_OF(b[lane]);
+                hit = 1;
+            }
+        }
+        if (vals_out) vals_out[i] = v;
+        if (hit_out) hit_out[i] = hit;
+    }
+    if (probes_out) *probes_out = probes;
+}
+
+/* cuckoo.cu:541-563: clear every matching slot of the first bucket holding the key */
+static int erase_in(uint64_t *b, uint32_t k)
+{
+    int any = 0;
+    for (int i = 0; i < ORC_SLOTS; ++i)
+        if (KEY_OF(b[i]) == k) {
+            b[i] = 0;
+            any = 1;
+        }
+    return any;
+}
+
+uint64_t orc_erase(orc_table *tb, const uint32_t *keys, uint64_t n, uint8_t *deleted_out)
+{
+    uint64_t ndel = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t k = keys[i];
+        int del = 0;
+        if (!reserved_key(k)) {
+            uint32_t p = orc_pair(k);
+            uint32_t t = orc_pos1(p);
+            del = erase_in(tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS, k);
+            if (!del) {
+                t = orc_pos2(p);
+                del = erase_in(tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS, k);
+            }
+        }
+        ndel += (uint64_t)del;
+        if (deleted_out) deleted_out[i] = (uint8_t)del;
+    }
+    return ndel;
+}
+
+/* ---------------------------------------------------------------- resize */
+
+/* cuckoo.cu:612-624: slot i of old bucket j moves to slot i of new bucket h % 2n */
+void orc_upsize(const uint64_t *old_sub, uint32_t old_nb, uint64_t *new_sub, uint32_t t)
+{
+    uint32_t nnb = old_nb * 2u;
+    memset(new_sub, 0, (size_t)nnb * ORC_SLOTS * sizeof(uint64_t));
+    for (uint64_t j = 0; j < old_nb; ++j)
+        for (int i = 0; i < ORC_SLOTS; ++i) {
+            uint64_t e = old_sub[j * ORC_SLOTS + i];
+            if (e != 0) new_sub[(uint64_t)orc_bucket(KEY_OF(e), t, nnb) * ORC_SLOTS + i] = e;
+        }
+}
+
+/* cuckoo.cu:664-699: new bucket g takes old bucket g lane-for-lane and fills its holes from
+ * old bucket g+n read back to front (lane i <- slot 15-i); entries of old bucket g+n that
+ * found no hole are residuals. */
+uint64_t orc_downsize_compact(const uint64_t *old_sub, uint32_t old_nb, uint64_t *new_sub,
+                              uint64_t *res_out, uint64_t res_cap)
+{
+    if (old_nb & 1u) return UINT64_MAX;
+    uint32_t n = old_nb / 2u;
+    uint64_t nres = 0;
+    for (uint64_t g = 0; g < n; ++g) {
+        const uint64_t *lo = old_sub + g * ORC_SLOTS;
+        const uint64_t *hi = old_sub + (g + n) * ORC_SLOTS;
+        for (int i = 0; i < ORC_SLOTS; ++i) {
+            uint64_t e = lo[i];
+            uint64_t c = hi[ORC_SLOTS - 1 - i];
+            if (e == 0) e = c;
+            new_sub[g * ORC_SLOTS + i] = e;
+            if (e != c && c != 0) {
+                if (res_out && nres < res_cap) res_out[nres] = c;
+                ++nres;
+            }
+        }
+    }
+    return nres;
+}
+
+/* ---------------------------------------------------------------- checks */
+
+uint64_t orc_count(const orc_table *tb)
+{
+    uint64_t c = 0;
+    for (int t = 0; t < ORC_SUBTABLES; ++t)
+        for (uint64_t s = 0; s < (uint64_t)tb->nb[t] * ORC_SLOTS; ++s) c += tb->sub[t][s] != 0;
+    return c;
+}
+
+uint64_t orc_check_invariant(const orc_table *tb)
+{
+    uint64_t bad = 0;
+    for (uint32_t t = 0; t < ORC_SUBTABLES; ++t)
+        for (uint64_t b = 0; b < tb->nb[t]; ++b)
+            for (int i = 0; i < ORC_SLOTS; ++i) {
+                uint64_t e = tb->sub[t][b * ORC_SLOTS + i];
+                if (e == 0) continue;
+                uint32_t k = KEY_OF(e), p = orc_pair(k);
+                if (reserved_key(k) || (orc_pos1(p) != t && orc_pos2(p) != t) ||
+                    orc_bucket(k, t, tb->nb[t]) != b)
+                    ++bad;
+            }
+    return bad;
+}
+
+/* ---------------------------------------------------------------- CPU baseline */
+
+void orc_find_mt(const orc_table *tb, const uint32_t *keys, uint64_t n, uint32_t *vals_out,
+                 uint8_t *hit_out, int threads)
+{
+    const int64_t chunk = 1 << 16;
+    const int64_t nchunks = (int64_t)((n + chunk - 1) / chunk);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+#endif
+    for (int64_t c = 0; c < nchunks; ++c) {
+        uint64_t lo = (uint64_t)c * chunk;
+        uint64_t m = n - lo < (uint64_t)chunk ? n - lo : (uint64_t)chunk;
+        orc_find(tb, keys + lo, m, vals_out ? vals_out + lo : 0, hit_out ? hit_out + lo : 0, 0);
+    }
+    (void)threads;
+}
+
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,148 @@
+"""CPU checks of the drop-in boundary: libdyc.so loads, exports every entry point include/dyc.h
+declares, its structs have the C layout the Python binding assumes, and the C++ surfaces compile.
+No compute call is made here (there is no GPU in CI)."""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+CSRC = os.path.join(ROOT, "databases_gpu_amd", "csrc")
+
+
+def declared_entry_points():
+    text = open(os.path.join(INC, "dyc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dyc_[a-z_0-9]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import databases_gpu_amd._lib as L
+
+    if not os.path.exists(L.LIB_PATH):
+        import databases_gpu_amd._build as B
+
+        B.build()
+    return L.load()
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_entry_points()
+    assert len(names) >= 15
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "databases_gpu_amd", "libdyc.so")],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (dyc_\w+)$", out, flags=re.M))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert getattr(lib, n) is not None
+    import databases_gpu_amd._lib as L
+
+    assert sorted(L.EXPORTS) == names
+
+
+def test_abi_version_and_defaults(lib):
+    import databases_gpu_amd as D
+
+    assert lib.dyc_abi_version() == 1
+    cfg = D.default_config()
+    assert cfg.kick_bound == 15 and cfg.rescue_kicks == 128
+    assert cfg.lower_bound == 0.5 and cfg.upper_bound == 0.9
+    assert cfg.find_lanes == 8 and cfg.block_threads == 256
+    assert cfg.min_capacity == int(4 * 1024 * 1024 * 1.2)  # api.cpp:496
+
+
+def test_errors_without_a_device_are_reported_not_crashed(lib):
+    """Argument errors come back as codes with a message (no GPU call is reached)."""
+    import databases_gpu_amd._lib as L
+
+    cfg = L.default_config(find_lanes=3)
+    h = ctypes.c_void_p()
+    rc = lib.dyc_create(ctypes.byref(cfg), 1000, ctypes.byref(h))
+    assert rc == 1 and b"lanes" in lib.dyc_last_error()
+    assert lib.dyc_insert(None, None, None, 0, None) == 1
+    assert lib.dyc_find(None, None, None, None, 0, None) == 1
+    assert lib.dyc_get_stats(None, None) == 1
+
+
+def test_struct_layout_matches_c(lib, tmp_path):
+    import databases_gpu_amd._lib as L
+
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", INC, os.path.join(ROOT, "tests", "cpp", "layout.c"), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    c = [int(x) for x in out[0].split()[1:]]
+    s = [int(x) for x in out[1].split()[1:]]
+    assert c == [ctypes.sizeof(L.Config), L.Config.lower_bound.offset, L.Config.auto_resize.offset,
+                 L.Config.max_blocks.offset, L.Config.min_capacity.offset]
+    assert s == [ctypes.sizeof(L.Stats), L.Stats.buckets.offset, L.Stats.pending.offset, L.Stats.rejected.offset,
+                 L.Stats.lost.offset]
+
+
+def _hipcc():
+    return shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+
+def test_fast_division_is_exact(tmp_path):
+    """fast_mod(n, d) == n % d: the bucket index on the device uses it instead of a divide."""
+    exe = tmp_path / "fastdiv"
+    subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, os.path.join(ROOT, "tests", "cpp", "fastdiv_check.cpp"),
+                    "-o", str(exe)], check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert " bad 0" in r.stdout
+
+
+def test_device_header_hashes_match_the_oracle(oracle_lib, tmp_path):
+    """The kernels' per-key arithmetic (dyc_device.hpp, compiled for the host here) equals the
+    oracle's on edge and random keys."""
+    import numpy as np
+
+    import oracle as O
+
+    src = tmp_path / "dump.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "dyc_device.hpp"
+int main() {
+    unsigned k;
+    while (scanf("%u", &k) == 1) {
+        dyc::Desc D{};
+        unsigned nb[4] = {2u, 36424u, 72818u, 4660218u};
+        for (int t = 0; t < 4; ++t) dyc::desc_set(D, t, nullptr, nb[t]);
+        printf("%u %u %u %u %u %u %u %u %u %u %u %u\n", dyc::h1(k), dyc::h2(k), dyc::h3(k), dyc::h4(k), dyc::h5(k),
+               dyc::pair_of(k), dyc::bucket_of(D, k, 0), dyc::bucket_of(D, k, 1), dyc::bucket_of(D, k, 2),
+               dyc::bucket_of(D, k, 3), dyc::rescue_lane(k, 7), dyc::other_pos(dyc::pair_of(k), 100u));
+    }
+}
+''')
+    exe = tmp_path / "dump"
+    subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, str(src), "-o", str(exe)], check=True,
+                   capture_output=True)
+    rng = np.random.default_rng(3)
+    keys = np.concatenate([np.array([1, 2, 11, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFE], dtype=np.uint32),
+                           rng.integers(1, 2**32 - 1, size=3000, dtype=np.uint64).astype(np.uint32)])
+    out = subprocess.run([str(exe)], input="\n".join(map(str, keys.tolist())), capture_output=True, text=True,
+                         check=True).stdout
+    got = np.array([[int(x) for x in line.split()] for line in out.strip().split("\n")], dtype=np.uint64)
+    L = O.lib()
+    nbs = [2, 36424, 72818, 4660218]
+    for i, k in enumerate(keys.tolist()):
+        want = [L.orc_h1(k), L.orc_h2(k), L.orc_h3(k), L.orc_h4(k), L.orc_h5(k), L.orc_pair(k),
+                *[L.orc_bucket(k, t, nbs[t]) for t in range(4)], L.orc_rescue_lane(k, 7),
+                L.orc_get_pos(L.orc_pair(k), 100)]
+        assert list(got[i]) == want, k
+
+
+def test_cpp_surfaces_compile(tmp_path):
+    """include/dyc_hashapi.hpp and the reference-test driver build against the C ABI."""
+    obj = tmp_path / "driver.o"
+    subprocess.run([_hipcc(), "-O1", "-std=c++17", "-I", INC, "-c", os.path.join(ROOT, "tests", "cpp", "hashapi_driver.cpp"),
+                    "-o", str(obj)], check=True, capture_output=True)
+    assert obj.exists()

@@ -1,0 +1,374 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle, bit-exact.
+
+What is compared (parallel insert makes the slot layout nondeterministic, so layout is compared
+only where the reference fixes it):
+  find    -- values and hit flags, bit-exact, on images the oracle built and on images the GPU built
+  insert  -- the key->value multiset (dump), the placement invariant, and find results
+  erase   -- deleted flags and the whole post-erase image, byte-exact (erase is deterministic given
+             the pre-image)
+  upsize  -- the whole post-upsize image, byte-exact against the oracle's upsize of the same image
+  downsize-- byte-exact when no residual arises; otherwise multiset + invariant + find
+  policy  -- subtable sizes after every batch of an auto-resize stream, and every output
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def banner(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def D():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import databases_gpu_amd as D
+
+    D.load()
+    return D
+
+
+def u32(t: torch.Tensor) -> np.ndarray:
+    return t.cpu().numpy().view(np.uint32)
+
+
+def unique_keys(n, seed):
+    rng = np.random.default_rng(seed)
+    out = np.unique(rng.integers(1, 2**32 - 1, size=int(n * 1.3) + 64, dtype=np.uint64).astype(np.uint32))
+    rng.shuffle(out)
+    return out[:n]
+
+
+def absent_keys(n, exclude, seed):
+    a = unique_keys(n * 2 + 64, seed)
+    a = a[~np.isin(a, exclude)]
+    return a[:n]
+
+
+def vals_of(k):
+    from databases_gpu_amd.workload import splitmix32_np
+
+    return splitmix32_np(k)
+
+
+def oracle_from_images(images, **kw):
+    tb = O.OracleTable(nb=[x.size // 16 for x in images], **kw)
+    tb.sub = [np.ascontiguousarray(x).copy() for x in images]
+    tb.count = int(sum(np.count_nonzero(x) for x in images))
+    return tb
+
+
+def gpu_find(table, q):
+    v, h = table.find(torch.from_numpy(q.view(np.int32)).cuda())
+    torch.cuda.synchronize()
+    return u32(v), h.cpu().numpy()
+
+
+def oracle_build(n, theta, seed):
+    keys = unique_keys(n, seed)
+    vals = vals_of(keys)
+    tb = O.OracleTable(int(np.ceil(n / theta)), rescue_kicks=128)
+    tb.insert(keys, vals)
+    assert tb.pending.size == 0
+    return tb, keys, vals
+
+
+# --------------------------------------------------------------------------------- find
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_find_on_oracle_image(D, lanes):
+    banner(f"find on oracle image lanes={lanes}")
+    tb, keys, vals = oracle_build(200_000, 0.85, seed=21)
+    t = D.DyCuckoo(1000, find_lanes=lanes)
+    t.load(tb.images())
+    absent = absent_keys(100_000, keys, 22)
+    q = np.concatenate([keys[:100_000], absent, np.array([0, 0xFFFFFFFF, keys[5]], dtype=np.uint32)])
+    rng = np.random.default_rng(1)
+    rng.shuffle(q)
+    q = q[: q.size - 17]  # ragged: not a multiple of 64
+    want_v, want_h = tb.find(q)
+    got_v, got_h = gpu_find(t, q)
+    assert np.array_equal(got_h, want_h)
+    assert np.array_equal(got_v, want_v)
+    t.close()
+
+
+def test_find_on_empty_table_and_tiny_batches(D):
+    banner("find on empty table, n=0,1,63,65")
+    t = D.DyCuckoo(100)
+    for n in (0, 1, 63, 65):
+        q = unique_keys(n, 3) if n else np.zeros(0, dtype=np.uint32)
+        v, h = gpu_find(t, q)
+        assert v.size == n and not h.any() and not v.any()
+    t.close()
+
+
+# ------------------------------------------------------------------------------- insert
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_insert_c1_multiset_invariant_and_find(D, lanes):
+    """2^20 unique keys, values splitmix32(key), table sized for load factor 0.85, then 2^20
+    finds at 50% hit."""
+    n = 1 << 20
+    banner(f"insert 2^20 at 0.85, insert_lanes={lanes}")
+    keys = unique_keys(n, 100 + lanes)
+    vals = vals_of(keys)
+    hint = int(np.ceil(n / 0.85))
+    t = D.DyCuckoo(hint, insert_lanes=lanes)
+    t.insert(torch.from_numpy(keys.view(np.int32)).cuda(), torch.from_numpy(vals.view(np.int32)).cuda())
+    st = t.stats()
+    assert st["count"] == n and st["pending"] == 0 and st["lost"] == 0 and st["rejected"] == 0
+    assert st["buckets"] == [O.initial_buckets(hint)] * 4  # no resize happened
+    assert st["last_kicks"] >= n
+    assert t.count_occupied() == n
+    imgs = t.dump()
+    ref = oracle_from_images(imgs)
+    assert ref.check_invariant() == 0
+    assert np.array_equal(ref.entries(), np.sort(O.pack(keys, vals)))
+    q = np.concatenate([keys[: n // 2], absent_keys(n // 2, keys, 7)])
+    np.random.default_rng(2).shuffle(q)
+    want_v, want_h = ref.find(q)
+    got_v, got_h = gpu_find(t, q)
+    assert np.array_equal(got_h, want_h) and np.array_equal(got_v, want_v)
+    assert want_h.sum() == n // 2
+    t.close()
+
+
+def test_insert_matches_sequential_replay_set(D):
+    """The oracle's sequential replay and the GPU hold the same key->value set afterwards."""
+    n = 300_000
+    banner("insert vs sequential replay, 0.90")
+    tb, keys, vals = oracle_build(n, 0.90, seed=31)
+    t = D.DyCuckoo(int(np.ceil(n / 0.90)))
+    t.insert(torch.from_numpy(keys.view(np.int32)).cuda(), torch.from_numpy(vals.view(np.int32)).cuda())
+    gpu = oracle_from_images(t.dump())
+    assert gpu.nb == tb.nb
+    assert np.array_equal(gpu.entries(), tb.entries())
+    q = np.concatenate([keys, absent_keys(50_000, keys, 32)])
+    assert all(np.array_equal(a, b) for a, b in zip(gpu_find(t, q), tb.find(q)))
+    t.close()
+
+
+def test_insert_reserved_and_duplicate_keys(D):
+    banner("insert reserved + duplicate keys")
+    t = D.DyCuckoo(4096)
+    k = np.array([0, 5, 0xFFFFFFFF, 6, 6], dtype=np.uint32)
+    v = np.array([1, 2, 3, 4, 4], dtype=np.uint32)
+    t.insert(torch.from_numpy(k.view(np.int32)).cuda(), torch.from_numpy(v.view(np.int32)).cuda())
+    st = t.stats()
+    assert st["rejected"] == 2 and st["count"] == 3
+    gv, gh = gpu_find(t, np.array([0, 5, 0xFFFFFFFF, 6, 7], dtype=np.uint32))
+    assert list(gh) == [0, 1, 0, 1, 0] and list(gv) == [0, 2, 0, 4, 0]
+    imgs = t.dump()
+    assert sum(int(((x >> np.uint64(32)) == 6).sum()) for x in imgs) == 2  # stored twice (no upsert)
+    d = t.erase(torch.from_numpy(np.array([6, 6, 0], dtype=np.uint32).view(np.int32)).cuda()).cpu().numpy()
+    assert d.sum() == 1 and d[2] == 0  # one of the two identical requests wins each slot race
+    assert sum(int(((x >> np.uint64(32)) == 6).sum()) for x in t.dump()) == 0
+    t.close()
+
+
+def test_same_key_beyond_two_buckets_stays_pending(D):
+    """100 x (11, 9) into hashAPI(100) (Main_backup_0910.cpp:236-275): a key has 2 buckets = 32
+    slots; the rest cannot be placed by any resize. They stay pending, nothing is lost."""
+    banner("same key x100")
+    t = D.DyCuckoo(100, max_fail_rounds=2)
+    k = np.full(100, 11, dtype=np.uint32)
+    v = np.full(100, 9, dtype=np.uint32)
+    with pytest.raises(D.DycError) as ei:
+        t.insert(torch.from_numpy(k.view(np.int32)).cuda(), torch.from_numpy(v.view(np.int32)).cuda())
+    assert ei.value.code == 4  # DYC_ERR_PENDING
+    st = t.stats()
+    assert st["count"] == 100 and st["pending"] == 100 - 32 and st["lost"] == 0
+    assert np.all(t.pending() == O.pack([11], [9])[0])
+    gv, gh = gpu_find(t, np.arange(100, dtype=np.uint32))
+    assert gh.sum() == 1 and gh[11] == 1 and gv[11] == 9 and gv.sum() == 9
+    t.close()
+
+
+# -------------------------------------------------------------------------------- erase
+def test_erase_flags_and_image_byte_exact(D):
+    n = 400_000
+    banner("erase byte-exact")
+    tb, keys, vals = oracle_build(n, 0.88, seed=41)
+    t = D.DyCuckoo(1000)
+    t.load(tb.images())
+    q = np.concatenate([keys[: n // 3], absent_keys(50_000, keys, 42), np.array([0, 0xFFFFFFFF], np.uint32)])
+    np.random.default_rng(3).shuffle(q)
+    want = tb.erase(q)
+    got = t.erase(torch.from_numpy(q.view(np.int32)).cuda()).cpu().numpy()
+    assert np.array_equal(got, want)
+    assert t.stats()["count"] == n - n // 3
+    for a, b in zip(t.dump(), tb.images()):
+        assert np.array_equal(a, b)
+    t.close()
+
+
+# ------------------------------------------------------------------------------- resize
+def test_upsize_byte_exact(D):
+    banner("upsize byte-exact x5")
+    tb, keys, vals = oracle_build(150_000, 0.85, seed=51)
+    t = D.DyCuckoo(1000)
+    t.load(tb.images())
+    for _ in range(5):  # subtables 0,1,2,3,0
+        tb.resize_up()
+        t.resize_up()
+        assert t.buckets == tb.nb
+        for a, b in zip(t.dump(), tb.images()):
+            assert np.array_equal(a, b)
+    t.close()
+
+
+def _residual_free_image(n, hint, seed):
+    """An oracle image whose subtable chosen for downsize has no crosswise conflicts."""
+    tb, keys, vals = oracle_build(n, n / hint, seed)
+    sub = O.lib().orc_choose_down((O.ctypes.c_uint32 * 4)(*tb.nb))
+    half = tb.nb[sub] // 2
+    img = tb.sub[sub].reshape(-1, 16)
+    lo, hi = img[:half], img[half:, ::-1]
+    clash = (lo != 0) & (hi != 0)
+    drop = hi[clash]
+    hi[clash] = 0
+    tb.count -= int(drop.size)
+    gone = set((drop >> np.uint64(32)).tolist())
+    keep = np.array([k not in gone for k in keys.tolist()])
+    return tb, keys[keep], vals[keep], sub
+
+
+def test_downsize_without_residuals_byte_exact(D):
+    banner("downsize without residuals")
+    tb, keys, vals, sub = _residual_free_image(100_000, 400_000, 61)
+    t = D.DyCuckoo(1000)
+    t.load(tb.images())
+    got_sub = sub
+    t_before = t.stats()["resizes_down"]
+    assert t.resize_down()
+    t2, res = tb.resize_down(defer_residuals=True)
+    assert t2 == got_sub and res.size == 0
+    assert t.stats()["resizes_down"] == t_before + 1
+    for a, b in zip(t.dump(), tb.images()):
+        assert np.array_equal(a, b)
+    t.close()
+
+
+def test_downsize_with_residuals_preserves_everything(D):
+    banner("downsize with residuals x3")
+    n = 250_000
+    tb, keys, vals = oracle_build(n, 0.45, seed=71)
+    t = D.DyCuckoo(1000)
+    t.load(tb.images())
+    for _ in range(3):
+        assert t.resize_down()
+        tb.resize_down()
+        assert t.buckets == tb.nb
+        st = t.stats()
+        assert st["pending"] == 0 and st["lost"] == 0
+        g = oracle_from_images(t.dump())
+        assert g.check_invariant() == 0
+        assert np.array_equal(g.entries(), np.sort(O.pack(keys, vals)))
+    q = np.concatenate([keys, absent_keys(20_000, keys, 72)])
+    gv, gh = gpu_find(t, q)
+    assert gh[:n].all() and not gh[n:].any() and np.array_equal(gv[:n], vals)
+    t.close()
+
+
+def test_downsize_refuses_odd_bucket_count(D):
+    banner("downsize odd refusal")
+    t = D.DyCuckoo(1000)
+    imgs = [np.zeros(5 * 16, dtype=np.uint64) for _ in range(4)]
+    t.load(imgs)
+    assert t.resize_down() is False
+    assert t.buckets == [5, 5, 5, 5]
+    t.close()
+
+
+# ------------------------------------------------------------------------------- policy
+def test_auto_resize_stream_matches_oracle(D):
+    """A growth-then-shrink stream of mixed batches (insert -> find -> erase per batch) with
+    fill bounds [0.5, 0.9]: subtable sizes after every batch and every output equal the oracle's."""
+    banner("auto-resize stream")
+    cfg = dict(auto_resize=1, lower_bound=0.5, upper_bound=0.9, min_capacity=0)
+    t = D.DyCuckoo(40_000, **cfg)
+    tb = O.OracleTable(40_000, auto_resize=True, lower=0.5, upper=0.9, min_capacity=0, rescue_kicks=128)
+    pool = unique_keys(600_000, 81)
+    live = []
+    nxt = 0
+    rng = np.random.default_rng(82)
+    for b in range(24):
+        grow = b < 12
+        n_ins = 20_000 if grow else 4_000
+        n_del = 4_000 if grow else 20_000
+        ins = pool[nxt: nxt + n_ins]
+        nxt += n_ins
+        iv = vals_of(ins)
+        t.insert(torch.from_numpy(ins.view(np.int32)).cuda(), torch.from_numpy(iv.view(np.int32)).cuda())
+        tb.insert(ins, iv)
+        live.extend(ins.tolist())
+        fq = np.concatenate([rng.choice(np.array(live, dtype=np.uint32), 5000), pool[-5000:]])
+        assert all(np.array_equal(a, b) for a, b in zip(gpu_find(t, fq), tb.find(fq)))
+        di = rng.choice(len(live), size=min(n_del, len(live)), replace=False)
+        dk = np.array([live[i] for i in di], dtype=np.uint32)
+        dq = np.concatenate([dk, pool[-100:]])
+        got = t.erase(torch.from_numpy(dq.view(np.int32)).cuda()).cpu().numpy()
+        want = tb.erase(dq)
+        assert np.array_equal(got, want)
+        dset = set(dk.tolist())
+        live = [k for k in live if k not in dset]
+        st = t.stats()
+        assert st["buckets"] == tb.nb, f"batch {b}"
+        assert st["count"] == tb.count == len(live)
+    assert t.stats()["resizes_up"] > 0 and t.stats()["resizes_down"] > 0
+    g = oracle_from_images(t.dump())
+    assert np.array_equal(g.entries(), tb.entries())
+    t.close()
+
+
+# ------------------------------------------------------------------------ large sizes
+def test_large_insert_find_properties(D):
+    """2^25 keys at load factor 0.90: occupancy, every hit, no false hit (size-independent checks
+    on the device)."""
+    from databases_gpu_amd import workload as W
+
+    n = 1 << 25
+    banner("2^25 insert/find properties")
+    dev = torch.device("cuda")
+    keys = W.keys_range(0, n, 9, dev)
+    vals = W.splitmix32(keys)
+    t = D.DyCuckoo(int(np.ceil(n / 0.90)))
+    t.insert(keys, vals)
+    st = t.stats()
+    assert st["count"] == n and st["pending"] == 0 and st["lost"] == 0
+    assert t.count_occupied() == n
+    v, h = t.find(keys)
+    assert bool(h.bool().all()) and bool((v == vals).all())
+    absent = W.keys_range(n, n + (1 << 22), 9, dev)
+    v, h = t.find(absent)
+    assert not bool(h.bool().any()) and not bool(v.bool().any())
+    d = t.erase(keys[: 1 << 22])
+    assert int(d.sum()) == 1 << 22
+    v, h = t.find(keys[: 1 << 23])
+    assert not bool(h[: 1 << 22].bool().any()) and bool(h[1 << 22:].bool().all())
+    t.close()
+
+
+# ---------------------------------------------------------------------- C++ drop-in driver
+def test_reference_cases_through_cpp_hashapi(D, tmp_path):
+    banner("C++ hashAPI driver (reference gtest cases)")
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    exe = tmp_path / "driver"
+    libdir = os.path.join(ROOT, "databases_gpu_amd")
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "hashapi_driver.cpp"), "-L", libdir, "-ldyc",
+                    f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failed" in r.stdout
