@@ -53,10 +53,15 @@ def parse():
     ap.add_argument("--zipf", type=float, default=0.99)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--find-lanes", type=int, default=8)
-    ap.add_argument("--insert-lanes", type=int, default=16)
+    ap.add_argument("--insert-lanes", type=int, default=1)
+    ap.add_argument("--insert-blocks", type=int, default=0, help="insert grid (0 = library default)")
+    ap.add_argument("--max-blocks", type=int, default=0, help="find grid cap (0 = library default)")
     ap.add_argument("--cpu-log2n", type=int, default=24, help="CPU baseline sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one plain copy kernel of 2^log2n int32 (4 B read + 4 B written per element) "
+                         "before the steps, as the reference point for hardware byte counters")
     return ap.parse_args()
 
 
@@ -141,8 +146,13 @@ def main() -> int:
     out_v = torch.empty(n, dtype=torch.int32, device=dev)
     out_h = torch.empty(n, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        torch.neg(keys, out=out_v)  # one elementwise kernel: reads 4n bytes, writes 4n bytes
+        torch.cuda.synchronize()
+        log(f"calibration copy kernel: {4 * n} bytes read, {4 * n} bytes written")
 
-    t = D.DyCuckoo(hint, find_lanes=args.find_lanes, insert_lanes=args.insert_lanes)
+    t = D.DyCuckoo(hint, find_lanes=args.find_lanes, insert_lanes=args.insert_lanes,
+                   insert_blocks=args.insert_blocks, max_blocks=args.max_blocks)
     log(f"table: buckets {t.buckets}, capacity {t.capacity}, fill {n / t.capacity:.4f}")
 
     phases = ("clear", "insert", "find_uniform", "find_zipf")
@@ -255,6 +265,7 @@ def main() -> int:
                         f"finds Zipf({args.zipf})",
             "keys": n, "theta": args.theta, "table_fill": round(n / t.capacity, 5), "buckets": t.buckets,
             "table_bytes": t.capacity * 8, "find_lanes": args.find_lanes, "insert_lanes": args.insert_lanes,
+            "insert_blocks": args.insert_blocks, "max_blocks": args.max_blocks,
             "seed": args.seed,
         },
         "roofline": roofline,

@@ -55,7 +55,7 @@ class Config(ctypes.Structure):
         ("insert_lanes", ctypes.c_uint32),
         ("block_threads", ctypes.c_uint32),
         ("max_blocks", ctypes.c_uint32),
-        ("reserved0", ctypes.c_uint32),
+        ("insert_blocks", ctypes.c_uint32),
         ("min_capacity", ctypes.c_uint64),
     ]
 

@@ -60,11 +60,6 @@ __device__ inline void load_slots(const uint64_t *p, uint64_t (&e)[SPL])
     }
 }
 
-__device__ inline uint64_t atomic_load_slot(const uint64_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ inline bool cas_slot(uint64_t *p, uint64_t &expected, uint64_t desired)
 {
     return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -209,7 +204,12 @@ __global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__re
 //   victim = first empty slot, else (first kick_bound iterations) the first slot whose resident
 //   pair index differs from t (cuckoo.cu:398-400), else slot (iteration & 15); after kick_bound
 //   iterations the victim is rescue_lane(). CAS the victim slot from the value read to the
-//   entry: on success carry the old value on (done if it was empty); on a lost race re-read.
+//   entry: on success carry the old value on (done if it was empty).
+// The bucket is read with plain vector loads. The view may be stale (another XCD's L2 may hold
+// a newer line), but only the CAS decides: a stale view can make a CAS fail, never lose or
+// duplicate an entry. A failed CAS returns the slot's current value, which replaces that slot in
+// the view; the group then chooses again without re-reading, so every retry learns one slot's
+// true value and at most max_retries races are spent on one entry before it is pended.
 template <int G, bool FROM_ENTRIES>
 __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ vals,
@@ -269,15 +269,16 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
         }
     };
 
+    uint64_t cur[SPL];
+    bool fresh = true;  // group-uniform: the view must be (re)read for the current bucket
     fetch();
     while (__ballot(active)) {
         if (active) {
             const uint32_t k = key_of(entry);
             const uint32_t t = other_pos(pair_of(k), pre);
             uint64_t *b = bucket_ptr(D, k, t);
-            uint64_t cur[SPL];
-#pragma unroll
-            for (int j = 0; j < SPL; ++j) cur[j] = atomic_load_slot(b + s * SPL + j);
+            if (fresh) load_slots<SPL>(b + s * SPL, cur);
+            fresh = false;
 
             // victim slot (group-uniform)
             int fe = -1;
@@ -314,10 +315,16 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
             if (s == owner) {
                 uint64_t exp = expected;
                 won = cas_slot(b + slot, exp, entry);
+                if (!won) {  // the slot's current value replaces the stale one in the view
+#pragma unroll
+                    for (int j = 0; j < SPL; ++j)
+                        if (j == sj) cur[j] = exp;
+                }
             }
             won = __shfl(won, g * G + owner);
             if (won) {
                 ++kicks;
+                fresh = true;
                 if (expected == 0) {
                     if (s == 0) n_kicks += kicks;
                     fetch();
@@ -335,6 +342,7 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
                 if (++retries > P.max_retries) {
                     if (s == 0) n_kicks += kicks;
                     give_up();
+                    fresh = true;
                     fetch();
                 }
             }
@@ -483,15 +491,24 @@ __global__ __launch_bounds__(256) void count_kernel(const uint64_t *__restrict__
     wave_add(&C->occupied, c);
 }
 
-inline int grid_for(uint64_t work_waves, LaunchShape sh)
+// Blocks for `work_waves` waves of work. A launch may not exceed 2^32 work-items in total, and
+// every kernel here loops grid-stride, so the grid is capped (default_cap when the caller set
+// no max_blocks); the rest of the work is taken by the resident waves as they finish.
+inline int grid_for(uint64_t work_waves, LaunchShape sh, uint64_t default_cap = 1ull << 20)
 {
     const uint64_t wpb = (uint64_t)sh.block_threads / kWave;
     uint64_t blocks = (work_waves + wpb - 1) / wpb;
     if (blocks < 1) blocks = 1;
-    if (sh.max_blocks > 0 && blocks > (uint64_t)sh.max_blocks) blocks = (uint64_t)sh.max_blocks;
-    if (blocks > 0x7fffffffull) blocks = 0x7fffffffull;
+    const uint64_t cap = sh.max_blocks > 0 ? (uint64_t)sh.max_blocks : default_cap;
+    if (blocks > cap) blocks = cap;
+    const uint64_t hw_cap = ((1ull << 32) - 1) / (uint64_t)sh.block_threads;
+    if (blocks > hw_cap) blocks = hw_cap;
     return (int)blocks;
 }
+
+// Insert groups take their next key when they finish one, so a grid of a few times the resident
+// capacity (256 CUs x 2048 threads) keeps every CU busy without launching a block per key.
+constexpr uint64_t kInsertDefaultBlocks = 65536;
 
 }  // namespace
 
@@ -528,13 +545,19 @@ static hipError_t insert_common(const Desc &D, const uint32_t *keys, const uint3
 {
     if (n == 0) return hipSuccess;
     const int per_wave = kWave / lanes;  // entries in flight per wave
-    const int grid = grid_for((n + per_wave - 1) / per_wave, sh);
+    const int grid = grid_for((n + per_wave - 1) / per_wave, sh, kInsertDefaultBlocks);
     switch (lanes) {
     case 8:
         insert_kernel<8, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
         break;
     case 4:
         insert_kernel<4, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
+        break;
+    case 2:
+        insert_kernel<2, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
+        break;
+    case 1:
+        insert_kernel<1, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
         break;
     default:
         insert_kernel<16, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend,

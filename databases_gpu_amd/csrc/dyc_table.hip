@@ -104,6 +104,7 @@ struct dyc_table {
         return c;
     }
     LaunchShape shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.max_blocks}; }
+    LaunchShape insert_shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.insert_blocks}; }
     InsertPolicy policy() const { return InsertPolicy{cfg.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
 };
 
@@ -212,7 +213,7 @@ int place_pending(dyc_table *t, hipStream_t s)
         rc = zero_counters(t, s);
         if (!rc) {
             hipError_t e = launch_insert_entries(t->D, src, m, kNoTable, (int)t->cfg.insert_lanes, t->policy(),
-                                                 t->shape(), t->ctr, t->pend, t->pend_cap, s);
+                                                 t->insert_shape(), t->ctr, t->pend, t->pend_cap, s);
             if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "pending re-insert launch: %s", hipGetErrorString(e));
         }
         if (!rc) rc = read_counters(t, s);
@@ -270,7 +271,7 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         if (!rc) rc = zero_counters(t, s);
         if (!rc) {
             hipError_t e = launch_insert_entries(t->D, res, nres, sub, (int)t->cfg.insert_lanes, t->policy(),
-                                                 t->shape(), t->ctr, t->pend + t->pend_n, t->pend_cap - t->pend_n, s);
+                                                 t->insert_shape(), t->ctr, t->pend + t->pend_n, t->pend_cap - t->pend_n, s);
             if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "residual insert launch: %s", hipGetErrorString(e));
         }
         if (!rc) rc = read_counters(t, s);
@@ -283,6 +284,7 @@ int do_resize_down(dyc_table *t, hipStream_t s)
 }
 
 bool valid_lanes(uint32_t l) { return l == 4 || l == 8 || l == 16; }
+bool valid_insert_lanes(uint32_t l) { return l == 1 || l == 2 || valid_lanes(l); }
 
 }  // namespace
 
@@ -304,7 +306,7 @@ int dyc_config_init(dyc_config *cfg)
     cfg->upper_bound = 0.90;
     cfg->auto_resize = 0;
     cfg->find_lanes = 8;
-    cfg->insert_lanes = 16;
+    cfg->insert_lanes = 1;
     cfg->block_threads = 256;
     cfg->max_blocks = 0;
     cfg->min_capacity = 5033164ull;  // TABLE_NUM * 1024 * 1024 * 1.2 (api.cpp:496)
@@ -320,8 +322,8 @@ int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out
         cfg = *cfg_in;
     else
         dyc_config_init(&cfg);
-    if (!valid_lanes(cfg.find_lanes) || !valid_lanes(cfg.insert_lanes))
-        return fail(DYC_ERR_INVALID, "lanes per key must be 4, 8 or 16");
+    if (!valid_lanes(cfg.find_lanes) || !valid_insert_lanes(cfg.insert_lanes))
+        return fail(DYC_ERR_INVALID, "lanes per key must be 4, 8 or 16 (insert also 1 or 2)");
     if (cfg.block_threads == 0 || cfg.block_threads % 64 || cfg.block_threads > 256)
         return fail(DYC_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 256]");
     if (!(cfg.upper_bound > 0.0 && cfg.upper_bound <= 1.0) || !(cfg.lower_bound >= 0.0) ||
@@ -391,7 +393,7 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     rc = zero_counters(t, s);
     if (rc) return rc;
     const uint64_t room = t->pend_cap - t->pend_n;
-    HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->shape(), t->ctr,
+    HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(), t->ctr,
                           t->pend + t->pend_n, room, s));
     rc = read_counters(t, s);
     if (rc) return rc;

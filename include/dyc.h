@@ -59,10 +59,10 @@ typedef struct dyc_config {
     double upper_bound;        /* upsize while count + n > upper * capacity (default 0.90) */
     uint32_t auto_resize;      /* 1: apply the fill bounds in dyc_insert / dyc_erase (default 0) */
     uint32_t find_lanes;       /* lanes per key for find/erase: 4, 8 or 16 (default 8) */
-    uint32_t insert_lanes;     /* lanes per key for insert: 4, 8 or 16 (default 16) */
+    uint32_t insert_lanes;     /* lanes per key for insert: 1, 2, 4, 8 or 16 (default 1) */
     uint32_t block_threads;    /* threads per block, multiple of 64, <= 256 (default 256) */
-    uint32_t max_blocks;       /* grid cap, 0 = one wave per 64 keys (default 0) */
-    uint32_t reserved0;
+    uint32_t max_blocks;       /* grid cap for find/erase, 0 = one wave per 64 keys up to 2^20 blocks */
+    uint32_t insert_blocks;    /* grid of the insert kernel, 0 = default (65536 blocks) */
     uint64_t min_capacity;     /* no automatic downsize at or below this many slots
                                   (default 5033164 = TABLE_NUM*1024*1024*1.2, api.cpp:496) */
 } dyc_config;

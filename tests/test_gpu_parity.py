@@ -114,7 +114,7 @@ def test_find_on_empty_table_and_tiny_batches(D):
 
 
 # ------------------------------------------------------------------------------- insert
-@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
 def test_insert_c1_multiset_invariant_and_find(D, lanes):
     """2^20 unique keys, values splitmix32(key), table sized for load factor 0.85, then 2^20
     finds at 50% hit."""
