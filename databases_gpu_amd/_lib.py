@@ -57,6 +57,8 @@ class Config(ctypes.Structure):
         ("max_blocks", ctypes.c_uint32),
         ("insert_blocks", ctypes.c_uint32),
         ("min_capacity", ctypes.c_uint64),
+        ("insert_order", ctypes.c_uint32),
+        ("region_mib", ctypes.c_uint32),
     ]
 
 

@@ -134,11 +134,15 @@ __host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b)
 #endif
 }
 
-__host__ __device__ inline uint32_t fast_mod(uint32_t n, uint32_t d, uint32_t m, uint32_t s1, uint32_t s2)
+__host__ __device__ inline uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s1, uint32_t s2)
 {
     uint32_t t1 = mulhi32(m, n);
-    uint32_t q = (t1 + ((n - t1) >> s1)) >> s2;
-    return n - q * d;
+    return (t1 + ((n - t1) >> s1)) >> s2;
+}
+
+__host__ __device__ inline uint32_t fast_mod(uint32_t n, uint32_t d, uint32_t m, uint32_t s1, uint32_t s2)
+{
+    return n - fast_div(n, m, s1, s2) * d;
 }
 
 // ------------------------------------------------------------------------ table descriptor
@@ -171,6 +175,37 @@ __host__ __device__ inline uint64_t *bucket_ptr(const Desc &D, uint32_t k, uint3
 {
     uint64_t *base = sel4(t, D.sub[0], D.sub[1], D.sub[2], D.sub[3]);
     return base + (uint64_t)bucket_of(D, k, t) * kSlots;
+}
+
+// Bucket regions for ordering an insert batch: region r of the table is slice r of every
+// subtable, slice = ceil(nb[t] / R) buckets, so region(t, b) = b / slice[t] < R.
+struct Regions {
+    uint32_t R;
+    uint32_t m[kSubtables], s1[kSubtables], s2[kSubtables];  // FastDiv of slice[t]
+};
+
+// Region of the bucket a key is first tried in (odd keys start in pos2, cuckoo.cu:336-341).
+__host__ __device__ inline uint32_t first_region(const Desc &D, const Regions &M, uint32_t k)
+{
+    const uint32_t p = pair_of(k);
+    const uint32_t t = (k & 1u) ? pos2(p) : pos1(p);
+    const uint32_t b = bucket_of(D, k, t);
+    return fast_div(b, sel4(t, M.m[0], M.m[1], M.m[2], M.m[3]), sel4(t, M.s1[0], M.s1[1], M.s1[2], M.s1[3]),
+                    sel4(t, M.s2[0], M.s2[1], M.s2[2], M.s2[3]));
+}
+
+inline Regions make_regions(const Desc &D, uint32_t R)
+{
+    Regions M{};
+    M.R = R;
+    for (int t = 0; t < kSubtables; ++t) {
+        const uint32_t slice = (D.nb[t] + R - 1) / R;
+        FastDiv f = make_fastdiv(slice);
+        M.m[t] = f.m;
+        M.s1[t] = f.s1;
+        M.s2[t] = f.s2;
+    }
+    return M;
 }
 
 inline void desc_set(Desc &D, int t, uint64_t *sub, uint32_t nb)

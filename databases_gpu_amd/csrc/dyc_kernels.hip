@@ -210,6 +210,12 @@ __global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__re
 // duplicate an entry. A failed CAS returns the slot's current value, which replaces that slot in
 // the view; the group then chooses again without re-reading, so every retry learns one slot's
 // true value and at most max_retries races are spent on one entry before it is pended.
+//
+// Input is taken in chunks of kChunk indexes: wave w owns chunks w, w + W, w + 2W, ... and
+// hands each chunk's indexes to its idle groups by ballot rank as they free up, so no group
+// waits for the slowest chain of its wave. Resident waves sweep the input front to back
+// together; when the batch was ordered by bucket region (launch_partition), the window they
+// work on touches one region's buckets at a time.
 template <int G, bool FROM_ENTRIES>
 __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ vals,
@@ -220,44 +226,21 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
     constexpr int SPL = kSlots / G;
     constexpr int NG = kWave / G;
     constexpr uint64_t GMASK = (G >= 64) ? ~0ull : ((1ull << G) - 1ull);
+    constexpr uint64_t kChunk = 2 * NG;  // indexes a wave claims per queue access
     const int lane = __lane_id();
     const int g = lane / G, s = lane % G;
-    const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / kWave) * NG;
-    uint64_t idx = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * NG + g;
     const uint32_t total_kicks = P.kick_bound + P.rescue_kicks;
 
     unsigned long long n_kicks = 0, n_rej = 0, n_retry = 0;
     bool active = false;
     uint64_t entry = 0;
     uint32_t pre = kNoTable, kicks = 0, retries = 0;
-
-    auto fetch = [&]() {
-        active = false;
-        while (idx < n) {
-            uint64_t e;
-            uint32_t pr;
-            if constexpr (FROM_ENTRIES) {
-                e = entries[idx];
-                pr = pre_fixed;
-            } else {
-                e = pack(keys[idx], vals[idx]);
-                pr = kNoTable;
-            }
-            idx += groups;
-            const uint32_t k = key_of(e);
-            if (reserved_key(k)) {
-                if (s == 0) ++n_rej;
-                continue;
-            }
-            if (pr == kNoTable && (k & 1u)) pr = pos1(pair_of(k));  // odd keys start in pos2
-            entry = e;
-            pre = pr;
-            kicks = 0;
-            retries = 0;
-            active = true;
-            return;
-        }
-    };
+    uint64_t cur[SPL];
+    bool fresh = true;                        // group-uniform: (re)read the current bucket
+    uint64_t chunk_next = 0, chunk_end = 0;   // wave-uniform: the wave's current chunk
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    uint64_t next_chunk = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    bool drained = false;                     // wave-uniform: no chunk left
 
     auto give_up = [&]() {  // park the in-hand entry on the pending list
         if (s == 0) {
@@ -267,12 +250,57 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
             else
                 atomicAdd(&C->lost, 1ull);
         }
+        active = false;
     };
 
-    uint64_t cur[SPL];
-    bool fresh = true;  // group-uniform: the view must be (re)read for the current bucket
-    fetch();
-    while (__ballot(active)) {
+    while (true) {
+        // hand queue indexes to idle groups; each pass either fills every idle group, drains
+        // the queue, or consumes at least one index
+        while (!drained) {
+            const uint64_t idle = __ballot(!active && s == 0);  // bit g*G for every idle group
+            if (!idle) break;
+            if (chunk_next == chunk_end) {
+                // wave w takes chunks w, w + W, w + 2W, ... (W = waves in the grid)
+                const uint64_t base = next_chunk * kChunk;
+                if (base >= n) {
+                    drained = true;
+                    break;
+                }
+                next_chunk += waves;
+                chunk_next = base;
+                chunk_end = base + kChunk < n ? base + kChunk : n;
+            }
+            const uint64_t take_max = chunk_end - chunk_next;
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            const uint32_t take = (uint32_t)(take_max < nidle ? take_max : nidle);
+            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << (g * G)) - 1ull));
+            if (!active && rank < take) {
+                const uint64_t idx = chunk_next + rank;
+                uint64_t e;
+                uint32_t pr;
+                if constexpr (FROM_ENTRIES) {
+                    e = entries[idx];
+                    pr = pre_fixed;
+                } else {
+                    e = pack(keys[idx], vals[idx]);
+                    pr = kNoTable;
+                }
+                const uint32_t k = key_of(e);
+                if (reserved_key(k)) {
+                    if (s == 0) ++n_rej;
+                } else {
+                    if (pr == kNoTable && (k & 1u)) pr = pos1(pair_of(k));  // odd keys start in pos2
+                    entry = e;
+                    pre = pr;
+                    kicks = 0;
+                    retries = 0;
+                    fresh = true;
+                    active = true;
+                }
+            }
+            chunk_next += take;
+        }
+        if (!__ballot(active)) break;  // drained and every group idle
         if (active) {
             const uint32_t k = key_of(entry);
             const uint32_t t = other_pos(pair_of(k), pre);
@@ -327,14 +355,13 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
                 fresh = true;
                 if (expected == 0) {
                     if (s == 0) n_kicks += kicks;
-                    fetch();
+                    active = false;
                 } else {
                     entry = expected;
                     pre = t;
                     if (kicks >= total_kicks) {
                         if (s == 0) n_kicks += kicks;
                         give_up();
-                        fetch();
                     }
                 }
             } else {
@@ -342,17 +369,132 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
                 if (++retries > P.max_retries) {
                     if (s == 0) n_kicks += kicks;
                     give_up();
-                    fresh = true;
-                    fetch();
                 }
             }
-        } else {
-            // idle groups wait for the rest of the wave; nothing to do
         }
     }
     wave_add(&C->kicks, n_kicks);
     wave_add(&C->rejected, n_rej);
     wave_add(&C->retries, n_retry);
+}
+
+// -------------------------------------------------------------------------- partition
+// Orders an insert batch by the region of each key's first bucket (first_region), so that the
+// queue-driven insert kernel places one region's keys at a time. Three kernels: a histogram of
+// regions (LDS counters per block), an exclusive scan into per-region write cursors, and a
+// scatter. The scatter stages each tile of kTile keys in LDS sorted by region (a counting sort:
+// LDS histogram, block scan, rank by LDS atomic), claims one global range per region present in
+// the tile, and writes each region's run contiguously -- the only LDS use on the insert path is
+// this staging of the batch's hashed keys.
+constexpr int kPartThreads = 256;
+constexpr int kPartPer = 8;
+constexpr int kTile = kPartThreads * kPartPer;
+
+__global__ __launch_bounds__(kPartThreads) void partition_count_kernel(Desc D, Regions M,
+                                                                       const uint32_t *__restrict__ keys, uint64_t n,
+                                                                       unsigned long long *__restrict__ counts)
+{
+    __shared__ unsigned int hist[kMaxRegions];
+    for (uint32_t r = threadIdx.x; r < M.R; r += blockDim.x) hist[r] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        atomicAdd(&hist[first_region(D, M, keys[i])], 1u);
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < M.R; r += blockDim.x)
+        if (hist[r]) atomicAdd(&counts[r], (unsigned long long)hist[r]);
+}
+
+__global__ void partition_scan_kernel(const unsigned long long *__restrict__ counts,
+                                      unsigned long long *__restrict__ cursors, uint32_t R)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned long long acc = 0;
+        for (uint32_t r = 0; r < R; ++r) {
+            cursors[r] = acc;
+            acc += counts[r];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kPartThreads) void partition_scatter_kernel(Desc D, Regions M,
+                                                                         const uint32_t *__restrict__ keys,
+                                                                         const uint32_t *__restrict__ vals, uint64_t n,
+                                                                         unsigned long long *__restrict__ cursors,
+                                                                         uint64_t *__restrict__ out)
+{
+    __shared__ uint64_t stage[kTile];
+    __shared__ unsigned short stage_r[kTile];
+    __shared__ unsigned int hist[kMaxRegions];
+    __shared__ unsigned int lbase[kMaxRegions];
+    __shared__ unsigned long long gbase[kMaxRegions];
+    __shared__ unsigned int wsum[kPartThreads / kWave];
+    const uint32_t R = M.R;
+    constexpr uint32_t kPerThread = (kMaxRegions + kPartThreads - 1) / kPartThreads;  // regions per thread in the scan
+
+    for (uint64_t base = (uint64_t)blockIdx.x * kTile; base < n; base += (uint64_t)gridDim.x * kTile) {
+        for (uint32_t r = threadIdx.x; r < R; r += kPartThreads) hist[r] = 0;
+        __syncthreads();
+        uint64_t e[kPartPer];
+        uint32_t reg[kPartPer], rank[kPartPer];
+#pragma unroll
+        for (int j = 0; j < kPartPer; ++j) {
+            const uint64_t i = base + (uint64_t)j * kPartThreads + threadIdx.x;
+            reg[j] = 0xffffffffu;
+            if (i < n) {
+                const uint32_t k = keys[i];
+                e[j] = pack(k, vals[i]);
+                reg[j] = first_region(D, M, k);
+                rank[j] = atomicAdd(&hist[reg[j]], 1u);
+            }
+        }
+        __syncthreads();
+        // block-wide exclusive scan of hist[0, R): each thread sums kPerThread consecutive
+        // regions, the wave scans those sums with shuffles, the waves combine through LDS
+        uint32_t mine[kPerThread];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kPerThread; ++q) {
+            const uint32_t r = threadIdx.x * kPerThread + q;
+            mine[q] = r < R ? hist[r] : 0u;
+            sum += mine[q];
+        }
+        uint32_t incl = sum;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t up = __shfl_up(incl, off);
+            if (__lane_id() >= off) incl += up;
+        }
+        if (__lane_id() == kWave - 1) wsum[threadIdx.x / kWave] = incl;
+        __syncthreads();
+        uint32_t wbase = 0;
+        for (uint32_t w = 0; w < threadIdx.x / kWave; ++w) wbase += wsum[w];
+        uint32_t run = wbase + incl - sum;
+#pragma unroll
+        for (uint32_t q = 0; q < kPerThread; ++q) {
+            const uint32_t r = threadIdx.x * kPerThread + q;
+            if (r < R) {
+                lbase[r] = run;
+                if (mine[q]) gbase[r] = atomicAdd(&cursors[r], (unsigned long long)mine[q]);
+            }
+            run += mine[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kPartPer; ++j)
+            if (reg[j] != 0xffffffffu) {
+                const uint32_t at = lbase[reg[j]] + rank[j];
+                stage[at] = e[j];
+                stage_r[at] = (unsigned short)reg[j];
+            }
+        __syncthreads();
+        const uint64_t cnt = n - base < (uint64_t)kTile ? n - base : (uint64_t)kTile;
+        for (uint32_t at = threadIdx.x; at < cnt; at += kPartThreads) {
+            const uint32_t r = stage_r[at];
+            out[gbase[r] + (at - lbase[r])] = stage[at];
+        }
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------------------------- upsize
@@ -579,6 +721,19 @@ hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_
                                  hipStream_t st)
 {
     return insert_common<true>(D, nullptr, nullptr, entries, n, pre, lanes, pol, sh, C, pend, pend_cap, st);
+}
+
+hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,
+                            unsigned long long *counts, unsigned long long *cursors, uint64_t *out, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    if (M.R == 0 || M.R > kMaxRegions) return hipErrorInvalidValue;
+    uint64_t blocks = (n + kTile - 1) / kTile;
+    if (blocks > 8192) blocks = 8192;
+    partition_count_kernel<<<(int)blocks, kPartThreads, 0, st>>>(D, M, keys, n, counts);
+    partition_scan_kernel<<<1, 64, 0, st>>>(counts, cursors, M.R);
+    partition_scatter_kernel<<<(int)blocks, kPartThreads, 0, st>>>(D, M, keys, vals, n, cursors, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, uint32_t t, Counters *C,

@@ -15,8 +15,10 @@ struct Counters {
     unsigned long long retries;    // CAS attempts that lost a race
     unsigned long long misplaced;  // resize: entries outside their bucket pair (corrupt image)
     unsigned long long occupied;   // count kernel
+    unsigned long long pad[8];
 };
-constexpr int kCounterWords = 8;
+constexpr int kCounterWords = 16;
+constexpr uint32_t kMaxRegions = 1024;
 
 struct LaunchShape {
     int block_threads;  // multiple of 64
@@ -43,6 +45,10 @@ hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *va
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
                                  InsertPolicy pol, LaunchShape sh, Counters *C, uint64_t *pend, uint64_t pend_cap,
                                  hipStream_t st);
+// Order an insert batch by the region of its first bucket: out[0, n) receives the packed
+// entries, region by region. counts and cursors are R words each (counts zeroed by the caller).
+hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,
+                            unsigned long long *counts, unsigned long long *cursors, uint64_t *out, hipStream_t st);
 // upsize subtable t: old has old_nb buckets, dst has 2*old_nb and is written completely.
 hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, uint32_t t, Counters *C,
                          hipStream_t st);

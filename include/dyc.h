@@ -65,6 +65,9 @@ typedef struct dyc_config {
     uint32_t insert_blocks;    /* grid of the insert kernel, 0 = default (65536 blocks) */
     uint64_t min_capacity;     /* no automatic downsize at or below this many slots
                                   (default 5033164 = TABLE_NUM*1024*1024*1.2, api.cpp:496) */
+    uint32_t insert_order;     /* 0 or 1 = input order (default); 2 = order each batch by the bucket
+                                  region of its keys' first bucket before inserting */
+    uint32_t region_mib;       /* table bytes per bucket region when ordering (default 64) */
 } dyc_config;
 
 typedef struct dyc_stats {

@@ -80,7 +80,8 @@ def test_struct_layout_matches_c(lib, tmp_path):
     c = [int(x) for x in out[0].split()[1:]]
     s = [int(x) for x in out[1].split()[1:]]
     assert c == [ctypes.sizeof(L.Config), L.Config.lower_bound.offset, L.Config.auto_resize.offset,
-                 L.Config.max_blocks.offset, L.Config.min_capacity.offset]
+                 L.Config.max_blocks.offset, L.Config.min_capacity.offset, L.Config.insert_order.offset,
+                 L.Config.region_mib.offset]
     assert s == [ctypes.sizeof(L.Stats), L.Stats.buckets.offset, L.Stats.pending.offset, L.Stats.rejected.offset,
                  L.Stats.lost.offset]
 

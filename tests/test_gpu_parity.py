@@ -143,6 +143,26 @@ def test_insert_c1_multiset_invariant_and_find(D, lanes):
     t.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_insert_ordered_by_region(D, lanes):
+    """Batches ordered by bucket region (forced on, 1 MiB regions, so ~10 regions here) hold the
+    same key->value set as the input: the ordering moves keys, never adds or drops them."""
+    n = (1 << 20) + 333
+    banner(f"insert ordered by region, lanes={lanes}")
+    keys = unique_keys(n, 900 + lanes)
+    keys[17] = 0  # a reserved key rides along and is rejected
+    vals = vals_of(keys)
+    t = D.DyCuckoo(int(np.ceil(n / 0.9)), insert_lanes=lanes, insert_order=2, region_mib=1)
+    t.insert(torch.from_numpy(keys.view(np.int32)).cuda(), torch.from_numpy(vals.view(np.int32)).cuda())
+    st = t.stats()
+    assert st["count"] == n - 1 and st["rejected"] == 1 and st["pending"] == 0 and st["lost"] == 0
+    ref = oracle_from_images(t.dump())
+    assert ref.check_invariant() == 0
+    ok = keys != 0
+    assert np.array_equal(ref.entries(), np.sort(O.pack(keys[ok], vals[ok])))
+    t.close()
+
+
 def test_insert_matches_sequential_replay_set(D):
     """The oracle's sequential replay and the GPU hold the same key->value set afterwards."""
     n = 300_000
