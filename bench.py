@@ -59,6 +59,12 @@ def parse():
     ap.add_argument("--cpu-log2n", type=int, default=24, help="CPU baseline sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--workload", choices=("bulk", "dynamic"), default="bulk",
+                    help="bulk: the large bulk configuration (default, the headline line); dynamic: the "
+                         "resize stream (growth then shrink batches under fill bounds)")
+    ap.add_argument("--dyn-batches", type=int, default=200, help="dynamic: batches (half grow, half shrink)")
+    ap.add_argument("--dyn-log2ops", type=int, default=22, help="dynamic: 2^k operations per batch")
+    ap.add_argument("--dyn-log2init", type=int, default=22, help="dynamic: 2^k keys live before the stream")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one plain copy kernel of 2^log2n int32 (4 B read + 4 B written per element) "
                          "before the steps, as the reference point for hardware byte counters")
@@ -118,6 +124,89 @@ def cpu_baseline(log2n: int, theta: float, hit: float, zipf_s: float, seed: int)
     }
 
 
+def run_dynamic(args, dev) -> int:
+    """The dynamic resize stream: batches of 2^dyn_log2ops operations, insert -> find -> delete
+    in each batch; growth batches are 40/40/20 insert/find/delete, shrink batches 20/40/40; finds
+    hit 70%; fill bounds [0.50, 0.90] trigger single-subtable upsize/downsize in line. The mix is
+    honoured exactly, so the table peaks at initial + 0.2 * ops * batches/2 keys (2^26.4 with the
+    defaults) and returns to the initial count. Prints one JSON line."""
+    import torch
+
+    import databases_gpu_amd as D
+    from databases_gpu_amd import workload as W
+
+    stream = torch.cuda.current_stream()
+    ops = 1 << args.dyn_log2ops
+    init = 1 << args.dyn_log2init
+    half = args.dyn_batches // 2
+    log(f"dynamic stream: {args.dyn_batches} batches of 2^{args.dyn_log2ops} ops from 2^{args.dyn_log2init} keys")
+    ik, iv, batches = W.dynamic_stream(init, ops, half, args.dyn_batches - half, args.seed, dev)
+    torch.cuda.synchronize()
+    t = D.DyCuckoo(int(math.ceil(init / args.theta)), auto_resize=1, lower_bound=0.5, upper_bound=args.theta,
+                   min_capacity=0, insert_lanes=args.insert_lanes, find_lanes=args.find_lanes)
+    t.insert(ik, iv)
+    out_v = torch.empty(ops, dtype=torch.int32, device=dev)
+    out_h = torch.empty(ops, dtype=torch.uint8, device=dev)
+    out_d = torch.empty(ops, dtype=torch.uint8, device=dev)
+    lat, peak, hits_ok = [], 0, True
+    st0 = t.stats()
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for i, b in enumerate(batches):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        t.insert(b.ins_keys, b.ins_vals)
+        nf = b.find_keys.numel()
+        t.find(b.find_keys, out_v[:nf], out_h[:nf])
+        t.erase(b.del_keys, out_d[: b.del_keys.numel()])
+        e1.record(stream)
+        e1.synchronize()
+        lat.append(e0.elapsed_time(e1))
+        st = t.stats()
+        peak = max(peak, st["count"])
+        hits_ok &= int(out_d[: b.del_keys.numel()].sum()) == b.del_keys.numel()
+        hits_ok &= st["count"] == b.live_after and st["pending"] == 0 and st["lost"] == 0
+        if (i + 1) % 20 == 0:
+            log(f"batch {i + 1}/{len(batches)}: {lat[-1]:.2f} ms, {st['count']} keys, buckets {st['buckets']}")
+    wall = time.perf_counter() - wall0
+    st = t.stats()
+    total_ops = sum(b.ops for b in batches)
+    total_ms = float(np.sum(lat))
+    result = {
+        "metric": "Mops/s dynamic resize stream, p50/p99 batch latency",
+        "value": round(total_ops / (total_ms * 1e-3) / 1e6, 1),
+        "unit": "Mops/s",
+        "n_gpus": 1,
+        "steps": len(batches),
+        "warmup": 0,
+        "ms_per_step": round(total_ms / len(batches), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"dynamic resize stream: {len(batches)} batches of 2^{args.dyn_log2ops} ops "
+                               f"(grow 40/40/20, shrink 20/40/40 insert/find/delete, finds 70% hit), "
+                               f"fill bounds [0.50, {args.theta}]",
+                   "initial_keys": init, "peak_keys": peak, "final_keys": st["count"]},
+        "p50_ms": round(pct(lat, 50), 3),
+        "p99_ms": round(pct(lat, 99), 3),
+        "resizes_up": st["resizes_up"] - st0["resizes_up"],
+        "resizes_down": st["resizes_down"] - st0["resizes_down"],
+        "final_buckets": st["buckets"],
+        "wall_s": round(wall, 3),
+        "parity": "ok" if hits_ok else "mismatch",
+    }
+    t.close()
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0 if hits_ok else 1
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -136,6 +225,8 @@ def main() -> int:
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
+    if args.workload == "dynamic":
+        return run_dynamic(args, dev)
     n = 1 << args.log2n
     hint = int(math.ceil(n / args.theta))
     log(f"generating inputs: n=2^{args.log2n}, theta={args.theta}")

@@ -33,6 +33,10 @@ EXPORTS = (
     "dyc_abi_version", "dyc_last_error", "dyc_config_init", "dyc_create", "dyc_destroy",
     "dyc_insert", "dyc_find", "dyc_erase", "dyc_resize_up", "dyc_resize_down", "dyc_get_stats",
     "dyc_count_occupied", "dyc_dump", "dyc_load", "dyc_pending", "dyc_clear", "dyc_probe_count",
+    "dyc_group_create", "dyc_group_destroy", "dyc_group_get_launch", "dyc_group_set_launch",
+    "dyc_group_insert", "dyc_group_find", "dyc_group_erase", "dyc_group_insert_per_table",
+    "dyc_group_find_per_table", "dyc_group_table_stats", "dyc_group_dump", "dyc_group_pending",
+    "dyc_group_clear",
 )
 
 
@@ -84,6 +88,20 @@ class Stats(ctypes.Structure):
         return d
 
 
+class GroupLaunch(ctypes.Structure):
+    _fields_ = [
+        ("find_lanes", ctypes.c_uint32),
+        ("insert_lanes", ctypes.c_uint32),
+        ("find_blocks", ctypes.c_uint32),
+        ("insert_blocks", ctypes.c_uint32),
+        ("item_keys", ctypes.c_uint32),
+        ("kick_bound", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -120,6 +138,19 @@ def load(require: bool = True):
         "dyc_pending": (ci, [vp, vp, u64, ctypes.POINTER(u64), vp]),
         "dyc_clear": (ci, [vp, vp]),
         "dyc_probe_count": (ci, [vp, vp, u64, ctypes.POINTER(u64), ctypes.POINTER(u64), vp]),
+        "dyc_group_create": (ci, [ctypes.POINTER(Config), u32, vp, ctypes.POINTER(vp)]),
+        "dyc_group_destroy": (ci, [vp]),
+        "dyc_group_get_launch": (ci, [vp, ctypes.POINTER(GroupLaunch)]),
+        "dyc_group_set_launch": (ci, [vp, ctypes.POINTER(GroupLaunch)]),
+        "dyc_group_insert": (ci, [vp, vp, vp, vp, vp]),
+        "dyc_group_find": (ci, [vp, vp, vp, vp, vp, vp]),
+        "dyc_group_erase": (ci, [vp, vp, vp, vp, vp]),
+        "dyc_group_insert_per_table": (ci, [vp, vp, vp, vp, vp]),
+        "dyc_group_find_per_table": (ci, [vp, vp, vp, vp, vp, vp]),
+        "dyc_group_table_stats": (ci, [vp, u32, ctypes.POINTER(Stats)]),
+        "dyc_group_dump": (ci, [vp, u32, u32, vp, u64, vp]),
+        "dyc_group_pending": (ci, [vp, vp, vp, u64, ctypes.POINTER(u64), vp]),
+        "dyc_group_clear": (ci, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

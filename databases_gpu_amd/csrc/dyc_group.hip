@@ -1,0 +1,439 @@
+// dyc_group.hip -- many DyCuckoo tables served together: the dyc_group_* half of include/dyc.h.
+//
+// All subtables of all tables live in one device arena. A device directory holds one Desc per
+// table. A batch (keys segmented per table by host offsets) is cut into work items of at most
+// item_keys keys of one table, uploaded, and served by one persistent launch; the per-table
+// variants launch the single-table kernels once per table for comparison.
+#include "dyc_host.hpp"
+#include "dyc_kernels.hpp"
+
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+using namespace dyc;
+using dyc::host::DeviceGuard;
+using dyc::host::fail;
+
+struct dyc_group {
+    dyc_config cfg;
+    dyc_group_launch launch;
+    int device = 0;
+    uint32_t n = 0;
+    uint8_t *arena = nullptr;
+    size_t arena_bytes = 0;
+    std::vector<Desc> hdir;
+    Desc *ddir = nullptr;
+    std::vector<uint64_t> count;  // per table: placed + pending
+    Counters *ctr = nullptr;
+    Counters *hctr = nullptr;
+    unsigned long long *tcounts = nullptr;   // per-table counts of the last insert/erase
+    unsigned long long *htcounts = nullptr;  // pinned mirror
+    uint64_t *pend = nullptr;
+    uint32_t *pend_tag = nullptr;
+    uint64_t pend_cap = 0, pend_n = 0;
+    GroupItem *ditems = nullptr;
+    GroupItem *hitems = nullptr;  // pinned staging
+    size_t items_cap = 0;
+    hipEvent_t items_free = nullptr;  // the last upload from hitems has completed
+    dyc_stats st{};
+
+    InsertPolicy policy() const { return InsertPolicy{launch.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
+};
+
+namespace {
+
+constexpr uint32_t kDefaultItemKeys = 4096;
+
+int check_launch(const dyc_group_launch &l)
+{
+    const bool find_ok = l.find_lanes == 4 || l.find_lanes == 8 || l.find_lanes == 16;
+    const bool ins_ok = l.insert_lanes == 1 || l.insert_lanes == 2 || l.insert_lanes == 4 || l.insert_lanes == 8 ||
+                        l.insert_lanes == 16;
+    if (!find_ok || !ins_ok) return fail(DYC_ERR_INVALID, "group lanes: find 4/8/16, insert 1/2/4/8/16");
+    return DYC_OK;
+}
+
+int check_offsets(const dyc_group *g, const uint64_t *offsets)
+{
+    if (!offsets) return fail(DYC_ERR_INVALID, "null offsets");
+    if (offsets[0] != 0) return fail(DYC_ERR_INVALID, "offsets[0] must be 0");
+    for (uint32_t i = 0; i < g->n; ++i)
+        if (offsets[i + 1] < offsets[i]) return fail(DYC_ERR_INVALID, "offsets decrease at table %u", i);
+    return DYC_OK;
+}
+
+// Cut the batch into items and upload them; returns the item count in *n_items.
+int upload_items(dyc_group *g, const uint64_t *offsets, hipStream_t s, uint32_t *n_items)
+{
+    const uint64_t per = g->launch.item_keys ? g->launch.item_keys : kDefaultItemKeys;
+    uint64_t need = 0;
+    for (uint32_t i = 0; i < g->n; ++i) need += (offsets[i + 1] - offsets[i] + per - 1) / per;
+    if (need > 0xffffffffull) return fail(DYC_ERR_INVALID, "batch too large for 32-bit item indexes");
+    HIP_TRY(hipEventSynchronize(g->items_free));  // hitems is free to rewrite
+    if (need > g->items_cap) {
+        const size_t cap = std::max<size_t>(need, g->items_cap * 2);
+        if (g->ditems) HIP_TRY(hipFree(g->ditems));
+        if (g->hitems) HIP_TRY(hipHostFree(g->hitems));
+        g->ditems = nullptr;
+        g->hitems = nullptr;
+        g->items_cap = 0;
+        if (hipMalloc(&g->ditems, cap * sizeof(GroupItem)) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void **>(&g->hitems), cap * sizeof(GroupItem), hipHostMallocDefault) !=
+                hipSuccess) {
+            (void)hipGetLastError();
+            return fail(DYC_ERR_OOM, "work item buffers (%zu items) failed", cap);
+        }
+        g->items_cap = cap;
+    }
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < g->n; ++i)
+        for (uint64_t b = offsets[i]; b < offsets[i + 1]; b += per)
+            g->hitems[m++] = GroupItem{b, (uint32_t)std::min<uint64_t>(per, offsets[i + 1] - b), i};
+    if (m) HIP_TRY(hipMemcpyAsync(g->ditems, g->hitems, m * sizeof(GroupItem), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(g->items_free, s));
+    *n_items = m;
+    return DYC_OK;
+}
+
+int zero_and_run_prologue(dyc_group *g, hipStream_t s)
+{
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(g->ctr), kCounterWords, s));
+    HIP_TRY(launch_zero(g->tcounts, g->n, s));
+    return DYC_OK;
+}
+
+int read_back(dyc_group *g, hipStream_t s)
+{
+    HIP_TRY(hipMemcpyAsync(g->hctr, g->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(g->htcounts, g->tcounts, g->n * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return DYC_OK;
+}
+
+int reserve_pending(dyc_group *g, uint64_t extra, hipStream_t s)
+{
+    const uint64_t need = g->pend_n + extra;
+    if (need <= g->pend_cap) return DYC_OK;
+    const uint64_t cap = std::max<uint64_t>(need, g->pend_cap * 2);
+    uint64_t *np = nullptr;
+    uint32_t *nt = nullptr;
+    if (hipMalloc(&np, cap * sizeof(uint64_t)) != hipSuccess || hipMalloc(&nt, cap * sizeof(uint32_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (np) (void)hipFree(np);
+        return fail(DYC_ERR_OOM, "group pending list (%llu entries) failed", (unsigned long long)cap);
+    }
+    if (g->pend_n) {
+        HIP_TRY(hipMemcpyAsync(np, g->pend, g->pend_n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(nt, g->pend_tag, g->pend_n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (g->pend) HIP_TRY(hipFree(g->pend));
+    if (g->pend_tag) HIP_TRY(hipFree(g->pend_tag));
+    g->pend = np;
+    g->pend_tag = nt;
+    g->pend_cap = cap;
+    return DYC_OK;
+}
+
+// After an insert launch: absorb counters and per-table counts.
+int finish_insert(dyc_group *g, uint64_t room)
+{
+    const Counters &c = *g->hctr;
+    g->st.kicks += c.kicks;
+    g->st.last_kicks = c.kicks;
+    g->st.cas_retries += c.retries;
+    g->st.kick_failures += c.pending;
+    g->st.rejected += c.rejected;
+    g->st.lost += c.lost;
+    g->pend_n += std::min<uint64_t>(c.pending, room);
+    g->st.pending = g->pend_n;
+    for (uint32_t i = 0; i < g->n; ++i) g->count[i] += g->htcounts[i];
+    if (c.lost) return fail(DYC_ERR_LOST, "group pending list overflow: %llu entries lost", (unsigned long long)c.lost);
+    if (c.pending)
+        return fail(DYC_ERR_PENDING, "%llu entries exhausted the eviction bound (group tables do not resize)",
+                    (unsigned long long)c.pending);
+    return DYC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dyc_group_create(const dyc_config *cfg_in, uint32_t n_tables, const uint64_t *hints, dyc_group **out)
+{
+    if (!out || !hints || n_tables == 0) return fail(DYC_ERR_INVALID, "bad group arguments");
+    *out = nullptr;
+    dyc_config cfg;
+    if (cfg_in)
+        cfg = *cfg_in;
+    else
+        dyc_config_init(&cfg);
+    dyc_group *g = new dyc_group();
+    g->cfg = cfg;
+    g->launch = dyc_group_launch{cfg.find_lanes, cfg.insert_lanes, 0, 0, kDefaultItemKeys, cfg.kick_bound};
+    int rc = check_launch(g->launch);
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    g->n = n_tables;
+    g->hdir.resize(n_tables);
+    g->count.assign(n_tables, 0);
+    // arena layout: table i, subtable t at a 256-byte aligned offset
+    std::vector<uint32_t> nbs(n_tables);
+    size_t bytes = 0;
+    for (uint32_t i = 0; i < n_tables; ++i) {
+        nbs[i] = dyc::host::initial_buckets(hints[i]);
+        if (nbs[i] >= dyc::host::kMaxBuckets) {
+            delete g;
+            return fail(DYC_ERR_INVALID, "table %u: capacity hint too large", i);
+        }
+        bytes += kSubtables * (((size_t)nbs[i] * kBucketBytes + 255) & ~(size_t)255);
+    }
+    auto cleanup = [&](int code) {
+        char keep[512];
+        strncpy(keep, dyc_last_error(), sizeof keep - 1);
+        keep[sizeof keep - 1] = 0;
+        dyc_group_destroy(g);
+        fail(code, "%s", keep);
+        return code;
+    };
+    HIP_TRY(hipGetDevice(&g->device));
+    if (hipMalloc(&g->arena, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(DYC_ERR_OOM, "group arena of %zu bytes failed", bytes);
+        return cleanup(DYC_ERR_OOM);
+    }
+    g->arena_bytes = bytes;
+    size_t off = 0;
+    for (uint32_t i = 0; i < n_tables; ++i)
+        for (int t = 0; t < kSubtables; ++t) {
+            desc_set(g->hdir[i], t, reinterpret_cast<uint64_t *>(g->arena + off), nbs[i]);
+            off += ((size_t)nbs[i] * kBucketBytes + 255) & ~(size_t)255;
+        }
+    if (hipMalloc(&g->ddir, n_tables * sizeof(Desc)) != hipSuccess || hipMalloc(&g->ctr, sizeof(Counters)) != hipSuccess ||
+        hipMalloc(&g->tcounts, n_tables * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&g->hctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&g->htcounts), n_tables * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&g->items_free, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(DYC_ERR_OOM, "group bookkeeping allocation failed");
+        return cleanup(DYC_ERR_OOM);
+    }
+    if (hipMemcpy(g->ddir, g->hdir.data(), n_tables * sizeof(Desc), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_zero(reinterpret_cast<unsigned long long *>(g->arena), bytes / 8, nullptr) != hipSuccess ||
+        hipStreamSynchronize(nullptr) != hipSuccess) {
+        fail(DYC_ERR_HIP, "group initialisation failed");
+        return cleanup(DYC_ERR_HIP);
+    }
+    rc = reserve_pending(g, 65536, nullptr);
+    if (rc) return cleanup(rc);
+    *out = g;
+    return DYC_OK;
+}
+
+int dyc_group_destroy(dyc_group *g)
+{
+    if (!g) return DYC_OK;
+    DeviceGuard dg(g->device);
+    (void)hipDeviceSynchronize();
+    if (g->arena) (void)hipFree(g->arena);
+    if (g->ddir) (void)hipFree(g->ddir);
+    if (g->ctr) (void)hipFree(g->ctr);
+    if (g->tcounts) (void)hipFree(g->tcounts);
+    if (g->hctr) (void)hipHostFree(g->hctr);
+    if (g->htcounts) (void)hipHostFree(g->htcounts);
+    if (g->pend) (void)hipFree(g->pend);
+    if (g->pend_tag) (void)hipFree(g->pend_tag);
+    if (g->ditems) (void)hipFree(g->ditems);
+    if (g->hitems) (void)hipHostFree(g->hitems);
+    if (g->items_free) (void)hipEventDestroy(g->items_free);
+    delete g;
+    return DYC_OK;
+}
+
+int dyc_group_get_launch(const dyc_group *g, dyc_group_launch *out)
+{
+    if (!g || !out) return fail(DYC_ERR_INVALID, "null argument");
+    *out = g->launch;
+    return DYC_OK;
+}
+
+int dyc_group_set_launch(dyc_group *g, const dyc_group_launch *l)
+{
+    if (!g || !l) return fail(DYC_ERR_INVALID, "null argument");
+    int rc = check_launch(*l);
+    if (rc) return rc;
+    g->launch = *l;
+    return DYC_OK;
+}
+
+int dyc_group_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *offsets,
+                     dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = check_offsets(g, offsets);
+    if (rc) return rc;
+    const uint64_t total = offsets[g->n];
+    if (total && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    rc = reserve_pending(g, std::max<uint64_t>(65536, total / 16), s);
+    if (rc) return rc;
+    uint32_t n_items = 0;
+    rc = upload_items(g, offsets, s, &n_items);
+    if (rc) return rc;
+    rc = zero_and_run_prologue(g, s);
+    if (rc) return rc;
+    const uint64_t room = g->pend_cap - g->pend_n;
+    HIP_TRY(launch_group_insert(g->ddir, g->ditems, n_items, keys, vals, g->policy(), (int)g->launch.insert_lanes,
+                                (int)g->launch.insert_blocks, g->ctr,
+                                PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, 0, nullptr}, g->tcounts,
+                                s));
+    rc = read_back(g, s);
+    if (rc) return rc;
+    return finish_insert(g, room);
+}
+
+int dyc_group_find(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, uint32_t *vals, uint8_t *hit,
+                   dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = check_offsets(g, offsets);
+    if (rc) return rc;
+    if (offsets[g->n] && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t n_items = 0;
+    rc = upload_items(g, offsets, s, &n_items);
+    if (rc) return rc;
+    HIP_TRY(launch_group_find(g->ddir, g->ditems, n_items, keys, vals, hit, (int)g->launch.find_lanes,
+                              (int)g->launch.find_blocks, s));
+    return DYC_OK;
+}
+
+int dyc_group_erase(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, uint8_t *deleted, dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = check_offsets(g, offsets);
+    if (rc) return rc;
+    if (offsets[g->n] && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t n_items = 0;
+    rc = upload_items(g, offsets, s, &n_items);
+    if (rc) return rc;
+    rc = zero_and_run_prologue(g, s);
+    if (rc) return rc;
+    HIP_TRY(launch_group_erase(g->ddir, g->ditems, n_items, keys, deleted, g->tcounts, (int)g->launch.find_lanes,
+                               (int)g->launch.find_blocks, s));
+    rc = read_back(g, s);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < g->n; ++i) g->count[i] -= std::min<uint64_t>(g->count[i], g->htcounts[i]);
+    return DYC_OK;
+}
+
+int dyc_group_insert_per_table(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *offsets,
+                               dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = check_offsets(g, offsets);
+    if (rc) return rc;
+    if (offsets[g->n] && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    rc = reserve_pending(g, std::max<uint64_t>(65536, offsets[g->n] / 16), s);
+    if (rc) return rc;
+    rc = zero_and_run_prologue(g, s);
+    if (rc) return rc;
+    const uint64_t room = g->pend_cap - g->pend_n;
+    const LaunchShape sh{256, (int)g->launch.insert_blocks};
+    // one launch per table; all share the counters, so pended entries append one after another
+    for (uint32_t i = 0; i < g->n; ++i) {
+        const uint64_t m = offsets[i + 1] - offsets[i];
+        if (!m) continue;
+        HIP_TRY(launch_insert(g->hdir[i], keys + offsets[i], vals + offsets[i], m, (int)g->launch.insert_lanes,
+                              g->policy(), sh, g->ctr,
+                              PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, i, g->tcounts + i}, s));
+    }
+    rc = read_back(g, s);
+    if (rc) return rc;
+    return finish_insert(g, room);
+}
+
+int dyc_group_find_per_table(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, uint32_t *vals, uint8_t *hit,
+                             dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = check_offsets(g, offsets);
+    if (rc) return rc;
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    const LaunchShape sh{256, (int)g->launch.find_blocks};
+    for (uint32_t i = 0; i < g->n; ++i) {
+        const uint64_t m = offsets[i + 1] - offsets[i];
+        if (!m) continue;
+        HIP_TRY(launch_find(g->hdir[i], keys + offsets[i], vals ? vals + offsets[i] : nullptr,
+                            hit ? hit + offsets[i] : nullptr, m, (int)g->launch.find_lanes, sh, s));
+    }
+    return DYC_OK;
+}
+
+int dyc_group_table_stats(const dyc_group *g, uint32_t table, dyc_stats *out)
+{
+    if (!g || !out || table >= g->n) return fail(DYC_ERR_INVALID, "bad argument");
+    *out = g->st;
+    out->count = g->count[table];
+    uint64_t cap = 0;
+    for (int t = 0; t < kSubtables; ++t) {
+        out->buckets[t] = g->hdir[table].nb[t];
+        cap += (uint64_t)g->hdir[table].nb[t] * kSlots;
+    }
+    out->capacity = cap;
+    return DYC_OK;
+}
+
+int dyc_group_dump(dyc_group *g, uint32_t table, uint32_t sub, uint64_t *host_slots, uint64_t n_slots,
+                   dyc_stream stream)
+{
+    if (!g || table >= g->n || sub >= (uint32_t)kSubtables || !host_slots) return fail(DYC_ERR_INVALID, "bad argument");
+    const uint64_t slots = (uint64_t)g->hdir[table].nb[sub] * kSlots;
+    if (n_slots < slots) return fail(DYC_ERR_INVALID, "buffer too small");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(host_slots, g->hdir[table].sub[sub], slots * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return DYC_OK;
+}
+
+int dyc_group_pending(dyc_group *g, uint64_t *host_entries, uint32_t *host_tables, uint64_t cap, uint64_t *n_out,
+                      dyc_stream stream)
+{
+    if (!g || !n_out) return fail(DYC_ERR_INVALID, "null argument");
+    *n_out = g->pend_n;
+    const uint64_t m = std::min(cap, g->pend_n);
+    if (m) {
+        DeviceGuard dg(g->device);
+        hipStream_t s = (hipStream_t)stream;
+        if (host_entries) HIP_TRY(hipMemcpyAsync(host_entries, g->pend, m * 8, hipMemcpyDeviceToHost, s));
+        if (host_tables) HIP_TRY(hipMemcpyAsync(host_tables, g->pend_tag, m * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return DYC_OK;
+}
+
+int dyc_group_clear(dyc_group *g, dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    DeviceGuard dg(g->device);
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(g->arena), g->arena_bytes / 8, (hipStream_t)stream));
+    std::fill(g->count.begin(), g->count.end(), 0);
+    g->pend_n = 0;
+    g->st.pending = 0;
+    return DYC_OK;
+}
+
+}  // extern "C"

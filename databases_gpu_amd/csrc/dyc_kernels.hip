@@ -17,6 +17,8 @@
 // the rest of the wave keeps kicking, so no lane idles behind a long chain.
 #include "dyc_kernels.hpp"
 
+#include <algorithm>
+
 namespace dyc {
 
 namespace {
@@ -66,6 +68,13 @@ __device__ inline bool cas_slot(uint64_t *p, uint64_t &expected, uint64_t desire
                                                 __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ inline unsigned long long wave_sum(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += (unsigned long long)shfl64(v, __lane_id() ^ off);
+    return v;
+}
+
 // Sum a per-lane value over the wave and add it to *dst once (all 64 lanes must be active).
 __device__ inline void wave_add(unsigned long long *dst, unsigned long long v)
 {
@@ -75,20 +84,17 @@ __device__ inline void wave_add(unsigned long long *dst, unsigned long long v)
 }
 
 // ------------------------------------------------------------------------ find / erase
+// One wave probes the 64 keys keys[base, base + 64) clipped to n, in table D. Returns this
+// lane's flag (hit or deleted) for its own key; all 64 lanes must call it together.
 template <int G, bool ERASE>
-__global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__restrict__ keys,
-                                                    uint32_t *__restrict__ vals, uint8_t *__restrict__ flags,
-                                                    uint64_t n, Counters *__restrict__ C)
+__device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                      uint8_t *__restrict__ flags, uint64_t base, uint64_t n)
 {
     constexpr int SPL = kSlots / G;  // slots each lane reads per bucket
     constexpr uint64_t GMASK = (G >= 64) ? ~0ull : ((1ull << G) - 1ull);
     const int lane = __lane_id();
     const int g = lane / G, s = lane % G;
-    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
-    unsigned long long ndel = 0;
-
-    for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
-         base += waves * kWave) {
+    {
         const uint64_t i = base + lane;
         const bool inb = i < n;
         const uint32_t k = inb ? keys[i] : 0u;  // past the end: key 0, which is reserved
@@ -193,8 +199,20 @@ __global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__re
             if (!ERASE && vals) vals[i] = my_val;
             if (flags) flags[i] = (uint8_t)my_flag;
         }
-        if (ERASE) ndel += my_flag;
+        return my_flag;
     }
+}
+
+template <int G, bool ERASE>
+__global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__restrict__ keys,
+                                                    uint32_t *__restrict__ vals, uint8_t *__restrict__ flags,
+                                                    uint64_t n, Counters *__restrict__ C)
+{
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    unsigned long long ndel = 0;
+    for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
+         base += waves * kWave)
+        ndel += probe_tile<G, ERASE>(D, keys, vals, flags, base, n);
     if constexpr (ERASE) wave_add(&C->deleted, ndel);
 }
 
@@ -216,166 +234,199 @@ __global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__re
 // waits for the slowest chain of its wave. Resident waves sweep the input front to back
 // together; when the batch was ordered by bucket region (launch_partition), the window they
 // work on touches one region's buckets at a time.
+// State of one group's in-hand entry (group-uniform except the view, which is per lane).
+template <int SPL>
+struct Carry {
+    uint64_t entry;
+    uint32_t pre, kicks, retries;
+    uint64_t cur[SPL];  // this lane's SPL slots of the current bucket
+    bool fresh;         // the view must be (re)read
+    bool active;
+};
+
+struct InsertTally {
+    unsigned long long kicks = 0, rejected = 0, retries = 0;
+};
+
+// Take entry e (previous table pr, kNoTable for a fresh key) into the group's hand.
+template <int SPL>
+__device__ inline void carry_start(Carry<SPL> &c, uint64_t e, uint32_t pr, int s, InsertTally &tally)
+{
+    const uint32_t k = key_of(e);
+    if (reserved_key(k)) {
+        if (s == 0) ++tally.rejected;
+        return;
+    }
+    if (pr == kNoTable && (k & 1u)) pr = pos1(pair_of(k));  // odd keys start in pos2 (cuckoo.cu:336-341)
+    c.entry = e;
+    c.pre = pr;
+    c.kicks = 0;
+    c.retries = 0;
+    c.fresh = true;
+    c.active = true;
+}
+
+// One eviction-chain iteration for an active group (see the comment above insert_kernel).
+template <int G>
+__device__ inline void kick_step(const Desc &D, Carry<kSlots / G> &c, const InsertPolicy &P, Counters *C,
+                                 const PendSink &sink, uint32_t tag, InsertTally &tally)
+{
+    constexpr int SPL = kSlots / G;
+    constexpr uint64_t GMASK = (G >= 64) ? ~0ull : ((1ull << G) - 1ull);
+    const int lane = __lane_id();
+    const int g = lane / G, s = lane % G;
+    const uint32_t total_kicks = P.kick_bound + P.rescue_kicks;
+
+    auto give_up = [&]() {  // park the in-hand entry on the pending list
+        if (s == 0) {
+            unsigned long long at = atomicAdd(&C->pending, 1ull);
+            if (at < sink.cap) {
+                sink.pend[at] = c.entry;
+                if (sink.tags) sink.tags[at] = tag;
+            } else {
+                atomicAdd(&C->lost, 1ull);
+            }
+        }
+        c.active = false;
+    };
+
+    const uint32_t k = key_of(c.entry);
+    const uint32_t t = other_pos(pair_of(k), c.pre);
+    uint64_t *b = bucket_ptr(D, k, t);
+    if (c.fresh) load_slots<SPL>(b + s * SPL, c.cur);
+    c.fresh = false;
+
+    // victim slot (group-uniform)
+    int fe = -1;
+#pragma unroll
+    for (int j = SPL - 1; j >= 0; --j)
+        if (key_of(c.cur[j]) == 0u) fe = j;
+    uint32_t gm = (uint32_t)((__ballot(fe >= 0) >> (g * G)) & GMASK);
+    int slot;
+    if (gm) {
+        const int w = __builtin_ctz(gm);
+        slot = w * SPL + __shfl(fe, g * G + w);
+    } else if (c.kicks < P.kick_bound) {
+        int fo = -1;
+#pragma unroll
+        for (int j = SPL - 1; j >= 0; --j)
+            if (pair_of(key_of(c.cur[j])) != t) fo = j;
+        gm = (uint32_t)((__ballot(fo >= 0) >> (g * G)) & GMASK);
+        if (gm) {
+            const int w = __builtin_ctz(gm);
+            slot = w * SPL + __shfl(fo, g * G + w);
+        } else {
+            slot = (int)(c.kicks & 15u);
+        }
+    } else {
+        slot = (int)rescue_lane(k, c.kicks);
+    }
+    const int owner = slot / SPL, sj = slot % SPL;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j)
+        if (j == sj) mine = c.cur[j];
+    uint64_t expected = shfl64(mine, g * G + owner);
+    int won = 0;
+    if (s == owner) {
+        uint64_t exp = expected;
+        won = cas_slot(b + slot, exp, c.entry);
+        if (!won) {  // the slot's current value replaces the stale one in the view
+#pragma unroll
+            for (int j = 0; j < SPL; ++j)
+                if (j == sj) c.cur[j] = exp;
+        }
+    }
+    won = __shfl(won, g * G + owner);
+    if (won) {
+        ++c.kicks;
+        c.fresh = true;
+        if (expected == 0) {
+            if (s == 0) tally.kicks += c.kicks;
+            c.active = false;
+        } else {
+            c.entry = expected;
+            c.pre = t;
+            if (c.kicks >= total_kicks) {
+                if (s == 0) tally.kicks += c.kicks;
+                give_up();
+            }
+        }
+    } else {
+        if (s == 0) ++tally.retries;
+        if (++c.retries > P.max_retries) {
+            if (s == 0) tally.kicks += c.kicks;
+            give_up();
+        }
+    }
+}
+
+// Feed the wave's idle groups from input indexes [next, end), advancing next; returns when every
+// group is busy or the range is used up (some groups may then still be idle).
+template <int G, bool FROM_ENTRIES>
+__device__ inline void refill(Carry<kSlots / G> &c, uint64_t &next, uint64_t end, const uint32_t *__restrict__ keys,
+                              const uint32_t *__restrict__ vals, const uint64_t *__restrict__ entries,
+                              uint32_t pre_fixed, InsertTally &tally, uint64_t &taken)
+{
+    const int lane = __lane_id();
+    const int g = lane / G, s = lane % G;
+    while (next < end) {
+        const uint64_t idle = __ballot(!c.active && s == 0);  // bit g*G for every idle group
+        if (!idle) return;
+        const uint64_t avail = end - next;
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        const uint32_t take = (uint32_t)(avail < nidle ? avail : nidle);
+        const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << (g * G)) - 1ull));
+        if (!c.active && rank < take) {
+            const uint64_t idx = next + rank;
+            if constexpr (FROM_ENTRIES)
+                carry_start(c, entries[idx], pre_fixed, s, tally);
+            else
+                carry_start(c, pack(keys[idx], vals[idx]), kNoTable, s, tally);
+        }
+        next += take;
+        taken += take;
+    }
+}
+
 template <int G, bool FROM_ENTRIES>
 __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ vals,
                                                      const uint64_t *__restrict__ entries, uint64_t n,
                                                      uint32_t pre_fixed, InsertPolicy P, Counters *__restrict__ C,
-                                                     uint64_t *__restrict__ pend, uint64_t pend_cap)
+                                                     PendSink sink)
 {
-    constexpr int SPL = kSlots / G;
     constexpr int NG = kWave / G;
-    constexpr uint64_t GMASK = (G >= 64) ? ~0ull : ((1ull << G) - 1ull);
-    constexpr uint64_t kChunk = 2 * NG;  // indexes a wave claims per queue access
-    const int lane = __lane_id();
-    const int g = lane / G, s = lane % G;
-    const uint32_t total_kicks = P.kick_bound + P.rescue_kicks;
-
-    unsigned long long n_kicks = 0, n_rej = 0, n_retry = 0;
-    bool active = false;
-    uint64_t entry = 0;
-    uint32_t pre = kNoTable, kicks = 0, retries = 0;
-    uint64_t cur[SPL];
-    bool fresh = true;                        // group-uniform: (re)read the current bucket
-    uint64_t chunk_next = 0, chunk_end = 0;   // wave-uniform: the wave's current chunk
+    constexpr uint64_t kChunk = 2 * NG;  // indexes a wave takes at a time
+    Carry<kSlots / G> c;
+    c.active = false;
+    InsertTally tally;
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
     uint64_t next_chunk = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    bool drained = false;                     // wave-uniform: no chunk left
-
-    auto give_up = [&]() {  // park the in-hand entry on the pending list
-        if (s == 0) {
-            unsigned long long at = atomicAdd(&C->pending, 1ull);
-            if (at < pend_cap)
-                pend[at] = entry;
-            else
-                atomicAdd(&C->lost, 1ull);
-        }
-        active = false;
-    };
-
+    uint64_t next = 0, end = 0;  // wave-uniform: the wave's current chunk
+    uint64_t taken = 0;          // wave-uniform: input indexes this wave took
     while (true) {
-        // hand queue indexes to idle groups; each pass either fills every idle group, drains
-        // the queue, or consumes at least one index
-        while (!drained) {
-            const uint64_t idle = __ballot(!active && s == 0);  // bit g*G for every idle group
-            if (!idle) break;
-            if (chunk_next == chunk_end) {
-                // wave w takes chunks w, w + W, w + 2W, ... (W = waves in the grid)
-                const uint64_t base = next_chunk * kChunk;
-                if (base >= n) {
-                    drained = true;
-                    break;
-                }
+        refill<G, FROM_ENTRIES>(c, next, end, keys, vals, entries, pre_fixed, tally, taken);
+        if (next == end) {
+            // chunks w, w + W, w + 2W, ... (W = waves in the grid)
+            const uint64_t base = next_chunk * kChunk;
+            if (base < n) {
                 next_chunk += waves;
-                chunk_next = base;
-                chunk_end = base + kChunk < n ? base + kChunk : n;
-            }
-            const uint64_t take_max = chunk_end - chunk_next;
-            const uint32_t nidle = (uint32_t)__popcll(idle);
-            const uint32_t take = (uint32_t)(take_max < nidle ? take_max : nidle);
-            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << (g * G)) - 1ull));
-            if (!active && rank < take) {
-                const uint64_t idx = chunk_next + rank;
-                uint64_t e;
-                uint32_t pr;
-                if constexpr (FROM_ENTRIES) {
-                    e = entries[idx];
-                    pr = pre_fixed;
-                } else {
-                    e = pack(keys[idx], vals[idx]);
-                    pr = kNoTable;
-                }
-                const uint32_t k = key_of(e);
-                if (reserved_key(k)) {
-                    if (s == 0) ++n_rej;
-                } else {
-                    if (pr == kNoTable && (k & 1u)) pr = pos1(pair_of(k));  // odd keys start in pos2
-                    entry = e;
-                    pre = pr;
-                    kicks = 0;
-                    retries = 0;
-                    fresh = true;
-                    active = true;
-                }
-            }
-            chunk_next += take;
-        }
-        if (!__ballot(active)) break;  // drained and every group idle
-        if (active) {
-            const uint32_t k = key_of(entry);
-            const uint32_t t = other_pos(pair_of(k), pre);
-            uint64_t *b = bucket_ptr(D, k, t);
-            if (fresh) load_slots<SPL>(b + s * SPL, cur);
-            fresh = false;
-
-            // victim slot (group-uniform)
-            int fe = -1;
-#pragma unroll
-            for (int j = SPL - 1; j >= 0; --j)
-                if (key_of(cur[j]) == 0u) fe = j;
-            uint32_t gm = (uint32_t)((__ballot(fe >= 0) >> (g * G)) & GMASK);
-            int slot;
-            if (gm) {
-                const int w = __builtin_ctz(gm);
-                slot = w * SPL + __shfl(fe, g * G + w);
-            } else if (kicks < P.kick_bound) {
-                int fo = -1;
-#pragma unroll
-                for (int j = SPL - 1; j >= 0; --j)
-                    if (pair_of(key_of(cur[j])) != t) fo = j;
-                gm = (uint32_t)((__ballot(fo >= 0) >> (g * G)) & GMASK);
-                if (gm) {
-                    const int w = __builtin_ctz(gm);
-                    slot = w * SPL + __shfl(fo, g * G + w);
-                } else {
-                    slot = (int)(kicks & 15u);
-                }
-            } else {
-                slot = (int)rescue_lane(k, kicks);
-            }
-            const int owner = slot / SPL, sj = slot % SPL;
-            uint64_t mine = 0;
-#pragma unroll
-            for (int j = 0; j < SPL; ++j)
-                if (j == sj) mine = cur[j];
-            uint64_t expected = shfl64(mine, g * G + owner);
-            int won = 0;
-            if (s == owner) {
-                uint64_t exp = expected;
-                won = cas_slot(b + slot, exp, entry);
-                if (!won) {  // the slot's current value replaces the stale one in the view
-#pragma unroll
-                    for (int j = 0; j < SPL; ++j)
-                        if (j == sj) cur[j] = exp;
-                }
-            }
-            won = __shfl(won, g * G + owner);
-            if (won) {
-                ++kicks;
-                fresh = true;
-                if (expected == 0) {
-                    if (s == 0) n_kicks += kicks;
-                    active = false;
-                } else {
-                    entry = expected;
-                    pre = t;
-                    if (kicks >= total_kicks) {
-                        if (s == 0) n_kicks += kicks;
-                        give_up();
-                    }
-                }
-            } else {
-                if (s == 0) ++n_retry;
-                if (++retries > P.max_retries) {
-                    if (s == 0) n_kicks += kicks;
-                    give_up();
-                }
+                next = base;
+                end = base + kChunk < n ? base + kChunk : n;
+                continue;
             }
         }
+        if (!__ballot(c.active)) break;  // input exhausted and every group idle
+        if (c.active) kick_step<G>(D, c, P, C, sink, sink.tag, tally);
     }
-    wave_add(&C->kicks, n_kicks);
-    wave_add(&C->rejected, n_rej);
-    wave_add(&C->retries, n_retry);
+    if (sink.placed) {
+        const unsigned long long rej = wave_sum(tally.rejected);
+        if (__lane_id() == 0 && taken > rej) atomicAdd(sink.placed, taken - rej);
+    }
+    wave_add(&C->kicks, tally.kicks);
+    wave_add(&C->rejected, tally.rejected);
+    wave_add(&C->retries, tally.retries);
 }
 
 // -------------------------------------------------------------------------- partition
@@ -618,6 +669,70 @@ __global__ __launch_bounds__(256) void probe_count_kernel(Desc D, const uint32_t
     wave_add(&C->occupied, pos1_hits);
 }
 
+// ---------------------------------------------------------------------------- many tables
+// One persistent launch serves a batch spread over many tables. The batch is cut into items of
+// at most a few thousand keys of one table; wave w takes items w, w + W, w + 2W, ... and loads
+// the item's table descriptor from the directory with scalar loads (the item index is
+// wave-uniform), so the per-tile code is the single-table code with a different Desc.
+__device__ inline uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <int G, bool ERASE>
+__global__ __launch_bounds__(256) void group_probe_kernel(const Desc *__restrict__ dir,
+                                                          const GroupItem *__restrict__ items, uint32_t n_items,
+                                                          const uint32_t *__restrict__ keys,
+                                                          uint32_t *__restrict__ vals, uint8_t *__restrict__ flags,
+                                                          unsigned long long *__restrict__ table_counts)
+{
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
+         it += waves) {
+        const GroupItem item = items[it];
+        const Desc D = dir[item.table];
+        const uint64_t end = item.begin + item.len;
+        unsigned long long nd = 0;
+        for (uint64_t base = item.begin; base < end; base += kWave) nd += probe_tile<G, ERASE>(D, keys, vals, flags, base, end);
+        if constexpr (ERASE) {
+            nd = wave_sum(nd);
+            if (__lane_id() == 0 && nd) atomicAdd(&table_counts[item.table], nd);
+        }
+    }
+}
+
+// Groups of one wave share an item: they are fed from it until it is used up, and the wave
+// moves on only when all its groups have placed (or pended) their entries.
+template <int G>
+__global__ __launch_bounds__(256) void group_insert_kernel(const Desc *__restrict__ dir,
+                                                           const GroupItem *__restrict__ items, uint32_t n_items,
+                                                           const uint32_t *__restrict__ keys,
+                                                           const uint32_t *__restrict__ vals, InsertPolicy P,
+                                                           Counters *__restrict__ C, PendSink sink,
+                                                           unsigned long long *__restrict__ table_counts)
+{
+    Carry<kSlots / G> c;
+    c.active = false;
+    InsertTally tally;
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
+         it += waves) {
+        const GroupItem item = items[it];
+        const Desc D = dir[item.table];
+        uint64_t next = item.begin;
+        const uint64_t end = item.begin + item.len;
+        const unsigned long long rej0 = tally.rejected;
+        uint64_t taken = 0;
+        while (true) {
+            refill<G, false>(c, next, end, keys, vals, nullptr, kNoTable, tally, taken);
+            if (!__ballot(c.active)) break;  // item used up and every group idle
+            if (c.active) kick_step<G>(D, c, P, C, sink, item.table, tally);
+        }
+        const unsigned long long rej = wave_sum(tally.rejected - rej0);
+        if (__lane_id() == 0 && item.len > rej) atomicAdd(&table_counts[item.table], item.len - rej);
+    }
+    wave_add(&C->kicks, tally.kicks);
+    wave_add(&C->rejected, tally.rejected);
+    wave_add(&C->retries, tally.retries);
+}
+
 __global__ void zero_kernel(unsigned long long *p, uint64_t words)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
@@ -683,44 +798,41 @@ hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, u
 template <bool FE>
 static hipError_t insert_common(const Desc &D, const uint32_t *keys, const uint32_t *vals, const uint64_t *entries,
                                 uint64_t n, uint32_t pre, int lanes, InsertPolicy pol, LaunchShape sh, Counters *C,
-                                uint64_t *pend, uint64_t pend_cap, hipStream_t st)
+                                PendSink sink, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     const int per_wave = kWave / lanes;  // entries in flight per wave
     const int grid = grid_for((n + per_wave - 1) / per_wave, sh, kInsertDefaultBlocks);
     switch (lanes) {
     case 8:
-        insert_kernel<8, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
+        insert_kernel<8, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
         break;
     case 4:
-        insert_kernel<4, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
+        insert_kernel<4, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
         break;
     case 2:
-        insert_kernel<2, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
+        insert_kernel<2, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
         break;
     case 1:
-        insert_kernel<1, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend, pend_cap);
+        insert_kernel<1, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
         break;
     default:
-        insert_kernel<16, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, pend,
-                                                                 pend_cap);
+        insert_kernel<16, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
         break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint64_t n, int lanes,
-                         InsertPolicy pol, LaunchShape sh, Counters *C, uint64_t *pend, uint64_t pend_cap,
-                         hipStream_t st)
+                         InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st)
 {
-    return insert_common<false>(D, keys, vals, nullptr, n, kNoTable, lanes, pol, sh, C, pend, pend_cap, st);
+    return insert_common<false>(D, keys, vals, nullptr, n, kNoTable, lanes, pol, sh, C, sink, st);
 }
 
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
-                                 InsertPolicy pol, LaunchShape sh, Counters *C, uint64_t *pend, uint64_t pend_cap,
-                                 hipStream_t st)
+                                 InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st)
 {
-    return insert_common<true>(D, nullptr, nullptr, entries, n, pre, lanes, pol, sh, C, pend, pend_cap, st);
+    return insert_common<true>(D, nullptr, nullptr, entries, n, pre, lanes, pol, sh, C, sink, st);
 }
 
 hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,
@@ -752,6 +864,67 @@ hipError_t launch_downsize(const uint64_t *old, uint32_t new_nb, uint64_t *dst, 
     if (new_nb == 0) return hipSuccess;
     const int grid = grid_for((new_nb + 7) / 8, LaunchShape{256, 0});
     downsize_kernel<<<grid, 256, 0, st>>>(old, new_nb, dst, C, res, res_cap);
+    return hipGetLastError();
+}
+
+constexpr int kGroupDefaultBlocks = 2048;  // 8192 waves: 256 CUs x 32 resident waves
+
+hipError_t launch_group_find(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
+                             uint32_t *vals, uint8_t *hit, int lanes, int blocks, hipStream_t st)
+{
+    if (n_items == 0) return hipSuccess;
+    const int grid = blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
+    switch (lanes) {
+    case 4: group_probe_kernel<4, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr); break;
+    case 16: group_probe_kernel<16, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr); break;
+    default: group_probe_kernel<8, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_group_erase(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
+                              uint8_t *deleted, unsigned long long *table_counts, int lanes, int blocks,
+                              hipStream_t st)
+{
+    if (n_items == 0) return hipSuccess;
+    const int grid = blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
+    switch (lanes) {
+    case 4:
+        group_probe_kernel<4, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts);
+        break;
+    case 16:
+        group_probe_kernel<16, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts);
+        break;
+    default:
+        group_probe_kernel<8, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
+                               const uint32_t *vals, InsertPolicy pol, int lanes, int blocks, Counters *C,
+                               PendSink sink, unsigned long long *table_counts, hipStream_t st)
+{
+    if (n_items == 0) return hipSuccess;
+    const int grid = blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
+    switch (lanes) {
+    case 2:
+        group_insert_kernel<2><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
+        break;
+    case 4:
+        group_insert_kernel<4><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
+        break;
+    case 8:
+        group_insert_kernel<8><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
+        break;
+    case 16:
+        group_insert_kernel<16><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
+        break;
+    default:
+        group_insert_kernel<1><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
+        break;
+    }
     return hipGetLastError();
 }
 

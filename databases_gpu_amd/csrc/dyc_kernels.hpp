@@ -25,6 +25,15 @@ struct LaunchShape {
     int max_blocks;     // 0 = as many as the work needs
 };
 
+// Where an insert launch parks entries that exhaust the eviction bound, and what it reports.
+struct PendSink {
+    uint64_t *pend;              // pending list; the append index is Counters::pending
+    uint32_t *tags;              // optional: the table index of each pending entry
+    uint64_t cap;                // entries pend can hold
+    uint32_t tag;                // table index of this launch (ignored by group launches)
+    unsigned long long *placed;  // optional: += entries this launch took in (reserved keys excluded)
+};
+
 struct InsertPolicy {
     uint32_t kick_bound;    // iterations that follow the reference victim rule
     uint32_t rescue_kicks;  // further iterations with the rescue victim rule
@@ -40,15 +49,34 @@ hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, u
 // insert keys/vals (start rule of cuckoo.cu:336-341) or packed entries with a fixed previous
 // table `pre` (kNoTable = use the key-parity start rule).
 hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint64_t n, int lanes,
-                         InsertPolicy pol, LaunchShape sh, Counters *C, uint64_t *pend, uint64_t pend_cap,
-                         hipStream_t st);
+                         InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st);
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
-                                 InsertPolicy pol, LaunchShape sh, Counters *C, uint64_t *pend, uint64_t pend_cap,
-                                 hipStream_t st);
+                                 InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st);
 // Order an insert batch by the region of its first bucket: out[0, n) receives the packed
 // entries, region by region. counts and cursors are R words each (counts zeroed by the caller).
 hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,
                             unsigned long long *counts, unsigned long long *cursors, uint64_t *out, hipStream_t st);
+// ---- many tables in one launch
+// A work item: input indexes [begin, begin + len) that all belong to table `table`.
+struct GroupItem {
+    uint64_t begin;
+    uint32_t len;
+    uint32_t table;
+};
+
+// Persistent launches over a directory of table descriptors: each wave takes items w, w + W, ...
+// find: vals/hit may be null. erase: per-table deleted counts are added to table_counts.
+hipError_t launch_group_find(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
+                             uint32_t *vals, uint8_t *hit, int lanes, int blocks, hipStream_t st);
+hipError_t launch_group_erase(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
+                              uint8_t *deleted, unsigned long long *table_counts, int lanes, int blocks,
+                              hipStream_t st);
+// insert: failures go to sink.pend with their table index in sink.tags; per-table inserted
+// counts (placed or pended, reserved keys excluded) are added to table_counts.
+hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
+                               const uint32_t *vals, InsertPolicy pol, int lanes, int blocks, Counters *C,
+                               PendSink sink, unsigned long long *table_counts, hipStream_t st);
+
 // upsize subtable t: old has old_nb buckets, dst has 2*old_nb and is written completely.
 hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, uint32_t t, Counters *C,
                          hipStream_t st);

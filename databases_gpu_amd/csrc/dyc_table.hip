@@ -3,7 +3,7 @@
 // Owns the four subtables, the pending list (entries in hand after an eviction chain hit its
 // bound, or downsize residuals that could not be placed), per-call device counters, and the
 // resize policy of the reference host API (GPU/dyhash/api.cpp).
-#include "../../include/dyc.h"
+#include "dyc_host.hpp"
 #include "dyc_kernels.hpp"
 
 #include <stdarg.h>
@@ -13,27 +13,13 @@
 #include <algorithm>
 
 using namespace dyc;
+using dyc::host::DeviceGuard;
+using dyc::host::fail;
+using dyc::host::initial_buckets;
+using dyc::host::kMaxBuckets;
 
 namespace {
-
 thread_local char g_err[512] = "";
-
-int fail(int code, const char *fmt, ...)
-{
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof g_err, fmt, ap);
-    va_end(ap);
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess) return fail(DYC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-constexpr uint32_t kMaxBuckets = 1u << 30;  // packed bucket locations use 30 bits
 
 // api.cpp:4-11 (int argument; 1 passes the 6k+-1 test, so nextPrime(0) = 1)
 bool is_prime_ref(uint32_t n)
@@ -45,6 +31,21 @@ bool is_prime_ref(uint32_t n)
         if (num % i == 0 || num % (i + 2) == 0) return false;
     return true;
 }
+}  // namespace
+
+namespace dyc {
+namespace host {
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+const char *last_error() { return g_err; }
 
 // api.cpp:47-51: s = nextPrime(size / 64), then rounded up to an even bucket count
 uint32_t initial_buckets(uint64_t hint)
@@ -53,6 +54,11 @@ uint32_t initial_buckets(uint64_t hint)
     while (!is_prime_ref(s)) ++s;
     return (s & 1u) ? s + 1u : s;
 }
+
+}  // namespace host
+}  // namespace dyc
+
+namespace {
 
 // api.cpp:116-122
 uint32_t choose_up(const uint32_t nb[4])
@@ -69,19 +75,6 @@ uint32_t choose_down(const uint32_t nb[4])
         if (nb[i] > nb[i + 1]) return i;
     return kSubtables - 1;
 }
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard()
-    {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
 
 }  // namespace
 
@@ -112,6 +105,9 @@ struct dyc_table {
 };
 
 namespace {
+
+// Pending list of a table from index `at` on (an insert launch appends there).
+PendSink sink_at(const dyc_table *t, uint64_t at) { return PendSink{t->pend + at, nullptr, t->pend_cap - at, 0, nullptr}; }
 
 int zero_counters(dyc_table *t, hipStream_t s)
 {
@@ -216,7 +212,7 @@ int place_pending(dyc_table *t, hipStream_t s)
         rc = zero_counters(t, s);
         if (!rc) {
             hipError_t e = launch_insert_entries(t->D, src, m, kNoTable, (int)t->cfg.insert_lanes, t->policy(),
-                                                 t->insert_shape(), t->ctr, t->pend, t->pend_cap, s);
+                                                 t->insert_shape(), t->ctr, sink_at(t, 0), s);
             if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "pending re-insert launch: %s", hipGetErrorString(e));
         }
         if (!rc) rc = read_counters(t, s);
@@ -274,7 +270,7 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         if (!rc) rc = zero_counters(t, s);
         if (!rc) {
             hipError_t e = launch_insert_entries(t->D, res, nres, sub, (int)t->cfg.insert_lanes, t->policy(),
-                                                 t->insert_shape(), t->ctr, t->pend + t->pend_n, t->pend_cap - t->pend_n, s);
+                                                 t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s);
             if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "residual insert launch: %s", hipGetErrorString(e));
         }
         if (!rc) rc = read_counters(t, s);
@@ -332,7 +328,7 @@ extern "C" {
 
 int dyc_abi_version(void) { return DYC_ABI_VERSION; }
 
-const char *dyc_last_error(void) { return g_err; }
+const char *dyc_last_error(void) { return dyc::host::last_error(); }
 
 int dyc_config_init(dyc_config *cfg)
 {
@@ -447,12 +443,12 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
         rc = zero_counters(t, s);
         if (rc) return rc;
         HIP_TRY(launch_insert_entries(t->D, t->part, n, kNoTable, (int)t->cfg.insert_lanes, t->policy(),
-                                      t->insert_shape(), t->ctr, t->pend + t->pend_n, room, s));
+                                      t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s));
     } else {
         rc = zero_counters(t, s);
         if (rc) return rc;
         HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(), t->ctr,
-                              t->pend + t->pend_n, room, s));
+                              sink_at(t, t->pend_n), s));
     }
     rc = read_counters(t, s);
     if (rc) return rc;

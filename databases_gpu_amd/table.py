@@ -272,4 +272,122 @@ class HashAPI:
         self.table.close()
 
 
-__all__ = ["DyCuckoo", "HashAPI", "DycError"]
+__all__ = ["DyCuckoo", "DyCuckooGroup", "HashAPI", "DycError"]
+
+
+class DyCuckooGroup:
+    """Many independent tables served by one persistent launch per batch (dyc_group_*).
+
+    A batch is a flat device tensor of keys plus host offsets (n_tables + 1 entries): table i's
+    keys are keys[offsets[i]:offsets[i+1]].
+    """
+
+    def __init__(self, capacity_hints, *, device=None, **config):
+        self.lib = _lib.load()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.cfg = _lib.default_config(**config)
+        hints = np.ascontiguousarray(np.asarray(capacity_hints, dtype=np.uint64))
+        self.n = int(hints.size)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_group_create(ctypes.byref(self.cfg), self.n, hints.ctypes.data, ctypes.byref(h)),
+                  "dyc_group_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dyc_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _offsets(self, offsets) -> np.ndarray:
+        o = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+        if o.size != self.n + 1:
+            raise ValueError(f"need {self.n + 1} offsets, got {o.size}")
+        return o
+
+    @property
+    def launch(self) -> dict:
+        out = _lib.GroupLaunch()
+        check(self.lib.dyc_group_get_launch(self.h, ctypes.byref(out)), "dyc_group_get_launch")
+        return out.as_dict()
+
+    def set_launch(self, **fields) -> None:
+        cur = _lib.GroupLaunch()
+        check(self.lib.dyc_group_get_launch(self.h, ctypes.byref(cur)), "dyc_group_get_launch")
+        for k, v in fields.items():
+            setattr(cur, k, int(v))
+        check(self.lib.dyc_group_set_launch(self.h, ctypes.byref(cur)), "dyc_group_set_launch")
+
+    def insert(self, keys, vals, offsets, per_table: bool = False, stream=None) -> None:
+        k = _as_u32_tensor(keys, self.device)
+        v = _as_u32_tensor(vals, self.device)
+        o = self._offsets(offsets)
+        fn = self.lib.dyc_group_insert_per_table if per_table else self.lib.dyc_group_insert
+        with torch.cuda.device(self.device):
+            check(fn(self.h, k.data_ptr(), v.data_ptr(), o.ctypes.data, _stream_ptr(stream)), "dyc_group_insert")
+
+    def find(self, keys, offsets, vals=None, hit=None, per_table: bool = False, stream=None):
+        k = _as_u32_tensor(keys, self.device)
+        o = self._offsets(offsets)
+        n = k.numel()
+        if vals is None:
+            vals = torch.empty(n, dtype=torch.int32, device=self.device)
+        if hit is None:
+            hit = torch.empty(n, dtype=torch.uint8, device=self.device)
+        fn = self.lib.dyc_group_find_per_table if per_table else self.lib.dyc_group_find
+        with torch.cuda.device(self.device):
+            check(fn(self.h, k.data_ptr(), o.ctypes.data, vals.data_ptr(), hit.data_ptr(), _stream_ptr(stream)),
+                  "dyc_group_find")
+        return vals, hit
+
+    def erase(self, keys, offsets, deleted=None, stream=None):
+        k = _as_u32_tensor(keys, self.device)
+        o = self._offsets(offsets)
+        if deleted is None:
+            deleted = torch.empty(k.numel(), dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_group_erase(self.h, k.data_ptr(), o.ctypes.data, deleted.data_ptr(), _stream_ptr(stream)),
+                  "dyc_group_erase")
+        return deleted
+
+    def clear(self, stream=None) -> None:
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_group_clear(self.h, _stream_ptr(stream)), "dyc_group_clear")
+
+    def table_stats(self, table: int) -> dict:
+        st = _lib.Stats()
+        check(self.lib.dyc_group_table_stats(self.h, int(table), ctypes.byref(st)), "dyc_group_table_stats")
+        return st.as_dict()
+
+    def dump(self, table: int, stream=None):
+        out = []
+        for t, nb in enumerate(self.table_stats(table)["buckets"]):
+            a = np.empty(nb * SLOTS, dtype=np.uint64)
+            with torch.cuda.device(self.device):
+                check(self.lib.dyc_group_dump(self.h, int(table), t, a.ctypes.data, a.size, _stream_ptr(stream)),
+                      "dyc_group_dump")
+            out.append(a)
+        return out
+
+    def pending(self, stream=None):
+        n = ctypes.c_uint64()
+        check(self.lib.dyc_group_pending(self.h, None, None, 0, ctypes.byref(n), _stream_ptr(stream)),
+              "dyc_group_pending")
+        e = np.empty(n.value, dtype=np.uint64)
+        tb = np.empty(n.value, dtype=np.uint32)
+        if n.value:
+            check(self.lib.dyc_group_pending(self.h, e.ctypes.data, tb.ctypes.data, n.value, ctypes.byref(n),
+                                             _stream_ptr(stream)), "dyc_group_pending")
+        return e, tb

@@ -54,24 +54,35 @@ def u32_to_i64(x: torch.Tensor) -> torch.Tensor:
     return x.to(torch.int64) & M32
 
 
-def keys_range(lo: int, hi: int, seed: int, device, n_total: int | None = None) -> torch.Tensor:
-    """key(i) for i in [lo, hi): distinct nonreserved uint32 words (as int32 bits).
+SPARE_INDEX = 0xFFFFFFF0  # indexes at or above this are never handed out by keys_range
 
-    n_total bounds the indices ever handed out (default hi); reserved words are replaced by
-    key(n_total + 2^31 + j), which no caller reaches."""
+
+def _spare_keys(seed: int) -> tuple[int, int]:
+    """Replacements for the two indexes whose mix is a reserved word (mix is a bijection, so
+    exactly one index maps to 0 and one to 0xFFFFFFFF): mix of the top spare indexes."""
+    out = []
+    j = M32
+    while len(out) < 2:
+        v = int(_mix32(torch.tensor([j], dtype=torch.int64), seed)[0])
+        if v not in (0, M32):
+            out.append(v)
+        j -= 1
+        assert j >= SPARE_INDEX
+    return out[0], out[1]
+
+
+def keys_range(lo: int, hi: int, seed: int, device) -> torch.Tensor:
+    """key(i) for i in [lo, hi) (hi <= 0xFFFFFFF0): distinct nonreserved uint32 words (as int32
+    bits). Different ranges of one seed never share a key."""
+    if hi > SPARE_INDEX:
+        raise ValueError("key indexes must stay below 0xFFFFFFF0")
     i = torch.arange(lo, hi, dtype=torch.int64, device=device)
     k = _mix32(i, seed)
     bad = (k == 0) | (k == M32)
     if bool(bad.any()):
-        spare_base = (n_total or hi) + (1 << 31)
-        j = torch.cumsum(bad.to(torch.int64), 0) - 1
-        repl = _mix32(spare_base + j, seed)
-        # the spare chain itself could hit a reserved word; walk further until clean
-        step = 0
-        while bool(((repl == 0) | (repl == M32))[bad].any()):
-            step += 1
-            repl = torch.where((repl == 0) | (repl == M32), _mix32(spare_base + (1 << 20) * step + j, seed), repl)
-        k = torch.where(bad, repl, k)
+        r0, r1 = _spare_keys(seed)
+        k = torch.where(k == 0, torch.full_like(k, r0), k)
+        k = torch.where(k == M32, torch.full_like(k, r1), k)
     return _to_i32(k)
 
 
@@ -153,3 +164,60 @@ def zipf_ranks(n_items: int, n_samples: int, s: float, seed: int, device) -> tor
 def zipf_lookups(present: torch.Tensor, n: int, s: float, seed: int) -> torch.Tensor:
     ranks = zipf_ranks(present.numel(), n, s, seed, present.device)
     return present[ranks]
+
+
+class DynamicBatch:
+    """One batch of the dynamic stream: inserts, then finds, then deletes (in that order)."""
+
+    __slots__ = ("ins_keys", "ins_vals", "find_keys", "del_keys", "live_after")
+
+    def __init__(self, ins_keys, ins_vals, find_keys, del_keys, live_after):
+        self.ins_keys, self.ins_vals = ins_keys, ins_vals
+        self.find_keys, self.del_keys = find_keys, del_keys
+        self.live_after = live_after
+
+    @property
+    def ops(self) -> int:
+        return self.ins_keys.numel() + self.find_keys.numel() + self.del_keys.numel()
+
+
+def dynamic_stream(initial: int, batch_ops: int, grow_batches: int, shrink_batches: int, seed: int, device,
+                   grow_mix=(0.4, 0.4, 0.2), shrink_mix=(0.2, 0.4, 0.4), find_hit=0.7):
+    """The dynamic resize workload, pre-generated on the device.
+
+    `initial` keys are live before the first batch (returned separately). Each growth batch of
+    `batch_ops` operations is grow_mix = (insert, find, delete) shares: fresh keys inserted, finds
+    at `find_hit` over the keys live after the batch's inserts (the rest absent keys), deletes of
+    distinct keys live after the inserts. Shrink batches swap the insert and delete shares. Keys
+    are keys_range(i, ..., seed) with i handed out in order, so every insert is fresh; absent
+    lookups come from indexes far beyond any insert. Returns (initial_keys, initial_vals, batches).
+    """
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    next_key = 0
+    absent_base = 1 << 31
+    absent_next = 0
+    live = keys_range(0, initial, seed, device)
+    next_key = initial
+    init_keys, init_vals = live.clone(), splitmix32(live)
+    batches = []
+    for b in range(grow_batches + shrink_batches):
+        mix = grow_mix if b < grow_batches else shrink_mix
+        n_ins = int(round(mix[0] * batch_ops))
+        n_find = int(round(mix[1] * batch_ops))
+        n_del = batch_ops - n_ins - n_find
+        ins = keys_range(next_key, next_key + n_ins, seed, device)
+        next_key += n_ins
+        live = torch.cat([live, ins])
+        n_hit = int(round(find_hit * n_find))
+        hit_idx = torch.randint(0, live.numel(), (n_hit,), generator=g, device=device)
+        miss = keys_range(absent_base + absent_next, absent_base + absent_next + (n_find - n_hit), seed, device)
+        absent_next += n_find - n_hit
+        finds = torch.cat([live[hit_idx], miss])
+        finds = finds[torch.randperm(finds.numel(), generator=g, device=device)]
+        n_del = min(n_del, live.numel())
+        perm = torch.randperm(live.numel(), generator=g, device=device)
+        dels = live[perm[:n_del]]
+        live = live[perm[n_del:]]
+        batches.append(DynamicBatch(ins, splitmix32(ins), finds, dels, live.numel()))
+    return init_keys, init_vals, batches

@@ -156,6 +156,59 @@ const char *dyc_last_error(void);
 /* DYC_ABI_VERSION of the loaded library. */
 int dyc_abi_version(void);
 
+/* ---------------------------------------------------------------------------------------
+ * Many tables, one launch. No reference counterpart: the reference serves one table per
+ * process (cuckoo.cu:16). A group holds n_tables independent DyCuckoo tables (same layout,
+ * hashes and eviction rule as above) in one device arena. A batch is a CSR-style segmentation
+ * of device arrays: the keys of table i are keys[offsets[i] .. offsets[i+1]) (offsets is a HOST
+ * array of n_tables + 1 entries, offsets[0] = 0). One persistent kernel serves the whole batch
+ * (work items of up to item_keys keys of one table); the *_per_table variants launch the
+ * single-table kernels once per non-empty table instead -- the plain per-batch launch the
+ * persistent form is measured against. Group tables do not resize: size them for their keys.
+ * --------------------------------------------------------------------------------------- */
+typedef struct dyc_group dyc_group;   /* opaque */
+
+/* Launch shape of a group's kernels; dyc_group_tune() picks these by measurement. */
+typedef struct dyc_group_launch {
+    uint32_t find_lanes;     /* 4, 8 or 16 */
+    uint32_t insert_lanes;   /* 1, 2, 4, 8 or 16 */
+    uint32_t find_blocks;    /* persistent grid for find/erase (0 = default) */
+    uint32_t insert_blocks;  /* persistent grid for insert (0 = default) */
+    uint32_t item_keys;      /* keys per work item (0 = default 4096) */
+    uint32_t kick_bound;     /* eviction iterations under the reference victim rule */
+} dyc_group_launch;
+
+/* capacity_hints: host array of n_tables slot-count hints (sized per table as dyc_create). */
+int dyc_group_create(const dyc_config *cfg, uint32_t n_tables, const uint64_t *capacity_hints, dyc_group **out);
+int dyc_group_destroy(dyc_group *g);
+int dyc_group_get_launch(const dyc_group *g, dyc_group_launch *out);
+int dyc_group_set_launch(dyc_group *g, const dyc_group_launch *launch);
+
+/* Insert / find / erase over all tables of a batch. Insert and erase synchronise `stream`
+ * once (per-table counts); find is asynchronous. Insert entries that exhaust the eviction
+ * bound stay pending in the group (DYC_ERR_PENDING; dyc_group_pending lists them). */
+int dyc_group_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *offsets,
+                     dyc_stream stream);
+int dyc_group_find(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, uint32_t *vals, uint8_t *hit,
+                   dyc_stream stream);
+int dyc_group_erase(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, uint8_t *deleted,
+                    dyc_stream stream);
+int dyc_group_insert_per_table(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *offsets,
+                               dyc_stream stream);
+int dyc_group_find_per_table(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, uint32_t *vals,
+                             uint8_t *hit, dyc_stream stream);
+
+/* Per-table geometry and counts (count, capacity, buckets; kick/retry figures are group-wide). */
+int dyc_group_table_stats(const dyc_group *g, uint32_t table, dyc_stats *out);
+/* Copy subtable `sub` of `table` into host memory (buckets[sub] * 16 slots). Synchronous. */
+int dyc_group_dump(dyc_group *g, uint32_t table, uint32_t sub, uint64_t *host_slots, uint64_t n_slots,
+                   dyc_stream stream);
+/* Pending entries and the table each belongs to (host arrays of `cap`); *n_out = number pending. */
+int dyc_group_pending(dyc_group *g, uint64_t *host_entries, uint32_t *host_tables, uint64_t cap, uint64_t *n_out,
+                      dyc_stream stream);
+/* Empty every table of the group. Asynchronous. */
+int dyc_group_clear(dyc_group *g, dyc_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -392,3 +392,34 @@ def test_reference_cases_through_cpp_hashapi(D, tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failed" in r.stdout
+
+
+def test_dynamic_stream_small_matches_oracle(D):
+    """The dynamic resize workload (grow 40/40/20, shrink 20/40/40, finds 70% hit, bounds
+    [0.5, 0.9]) at 2^14 operations per batch: per-batch subtable sizes, find values and hits,
+    and delete flags equal the oracle's replay of the same batches."""
+    from databases_gpu_amd import workload as W
+
+    banner("dynamic stream small vs oracle")
+    dev = torch.device("cuda")
+    ik, iv, batches = W.dynamic_stream(1 << 14, 1 << 14, 16, 16, 5, dev)
+    t = D.DyCuckoo(int(np.ceil((1 << 14) / 0.9)), auto_resize=1, lower_bound=0.5, upper_bound=0.9, min_capacity=0)
+    tb = O.OracleTable(int(np.ceil((1 << 14) / 0.9)), auto_resize=True, lower=0.5, upper=0.9, min_capacity=0,
+                       rescue_kicks=128)
+    t.insert(ik, iv)
+    tb.insert(u32(ik), u32(iv))
+    ups = downs = 0
+    for i, b in enumerate(batches):
+        t.insert(b.ins_keys, b.ins_vals)
+        tb.insert(u32(b.ins_keys), u32(b.ins_vals))
+        v, h = t.find(b.find_keys)
+        rv, rh = tb.find(u32(b.find_keys))
+        assert np.array_equal(u32(v), rv) and np.array_equal(h.cpu().numpy(), rh), f"find, batch {i}"
+        d = t.erase(b.del_keys).cpu().numpy()
+        assert np.array_equal(d, tb.erase(u32(b.del_keys))), f"erase, batch {i}"
+        st = t.stats()
+        assert st["buckets"] == tb.nb and st["count"] == tb.count == b.live_after, f"sizes, batch {i}"
+    assert t.stats()["resizes_up"] > 0 and t.stats()["resizes_down"] > 0
+    g = oracle_from_images(t.dump())
+    assert g.check_invariant() == 0 and np.array_equal(g.entries(), tb.entries())
+    t.close()
