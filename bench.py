@@ -59,9 +59,19 @@ def parse():
     ap.add_argument("--cpu-log2n", type=int, default=24, help="CPU baseline sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
-    ap.add_argument("--workload", choices=("bulk", "dynamic"), default="bulk",
+    ap.add_argument("--workload", choices=("bulk", "dynamic", "batched"), default="bulk",
                     help="bulk: the large bulk configuration (default, the headline line); dynamic: the "
-                         "resize stream (growth then shrink batches under fill bounds)")
+                         "resize stream (growth then shrink batches under fill bounds); batched: many uneven "
+                         "tables in one persistent launch vs one launch per table")
+    ap.add_argument("--bt-tables", type=int, default=8192, help="batched: number of tables")
+    ap.add_argument("--bt-log2min", type=int, default=8, help="batched: log2 of the smallest table")
+    ap.add_argument("--bt-log2max", type=int, default=20, help="batched: log2 of the largest table")
+    ap.add_argument("--bt-theta", type=float, default=0.85, help="batched: per-table load factor")
+    ap.add_argument("--bt-hit", type=float, default=0.6, help="batched: find hit rate")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--tune-kick", action="store_true",
+                    help="bulk: let the autotuner also try eviction bound 0 (placement differs from the "
+                         "reference rule, results do not)")
     ap.add_argument("--dyn-batches", type=int, default=200, help="dynamic: batches (half grow, half shrink)")
     ap.add_argument("--dyn-log2ops", type=int, default=22, help="dynamic: 2^k operations per batch")
     ap.add_argument("--dyn-log2init", type=int, default=22, help="dynamic: 2^k keys live before the stream")
@@ -207,6 +217,108 @@ def run_dynamic(args, dev) -> int:
     return 0 if hits_ok else 1
 
 
+def run_batched(args, dev) -> int:
+    """Batched uneven tables: bt_tables tables with key counts log-uniform in [2^bt_log2min,
+    2^bt_log2max], each sized for load factor bt_theta; one step = clear, insert every table's
+    keys, then find bt_tables segments of lookups (bt_hit of each table's own keys, the rest
+    absent). Measured twice with the same kernels and launch shape: one persistent launch per
+    batch (the group path) and one launch per table. Prints one JSON line."""
+    import torch
+
+    import databases_gpu_amd as D
+    from databases_gpu_amd import autotune
+    from databases_gpu_amd import workload as W
+
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(args.seed)
+    sizes = np.exp(rng.uniform(np.log(2.0**args.bt_log2min), np.log(2.0**args.bt_log2max), size=args.bt_tables))
+    sizes = sizes.astype(np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    total = int(offs[-1])
+    log(f"batched: {args.bt_tables} tables, {total} keys (2^{math.log2(total):.2f}), load factor {args.bt_theta}")
+    keys = W.keys_range(0, total, args.seed, dev)
+    vals = W.splitmix32(keys)
+    hit_n = np.round(args.bt_hit * sizes).astype(np.int64)
+    mark = torch.zeros(total + 1, dtype=torch.int32, device=dev)
+    ones = torch.ones(args.bt_tables, dtype=torch.int32, device=dev)
+    mark.index_add_(0, torch.from_numpy((offs[:-1] + hit_n.astype(np.uint64)).astype(np.int64)).to(dev), ones)
+    mark.index_add_(0, torch.from_numpy(offs[1:].astype(np.int64)).to(dev), -ones)
+    absent = torch.cumsum(mark, 0, dtype=torch.int32)[:total].bool()
+    del mark
+    lookups = torch.where(absent, W.keys_range(total, 2 * total, args.seed, dev), keys)
+    out_v = torch.empty(total, dtype=torch.int32, device=dev)
+    out_h = torch.empty(total, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    hints = (np.ceil(sizes / args.bt_theta) + 64).astype(np.uint64)
+    g = D.DyCuckooGroup(hints)
+    tuned = None
+    if not args.no_tune:
+        log("autotuning launch shapes on the batch")
+        tuned = autotune.tune_group(g, keys, vals, offs, lookups, offs, log=None)
+        log(f"autotune: {tuned['installed']} ({tuned.get('trials', 0)} trials, {tuned['seconds']} s)")
+
+    def measure(per_table: bool):
+        ins, fnd, tot = [], [], []
+        for s in range(args.warmup + args.steps):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            e[0].record(stream)
+            g.clear()
+            e[1].record(stream)
+            g.insert(keys, vals, offs, per_table=per_table)
+            e[2].record(stream)
+            g.find(lookups, offs, out_v, out_h, per_table=per_table)
+            e[3].record(stream)
+            e[3].synchronize()
+            if s >= args.warmup:
+                ins.append(e[1].elapsed_time(e[2]))
+                fnd.append(e[2].elapsed_time(e[3]))
+                tot.append(e[1].elapsed_time(e[3]))
+        return ins, fnd, tot
+
+    ins_p, fnd_p, tot_p = measure(False)
+    ok = bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
+    log(f"persistent: insert {np.median(ins_p):.2f} ms, find {np.median(fnd_p):.2f} ms")
+    ins_t, fnd_t, tot_t = measure(True)
+    ok &= bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
+    log(f"per table:  insert {np.median(ins_t):.2f} ms, find {np.median(fnd_t):.2f} ms")
+    counts_ok = all(g.table_stats(i)["count"] == sizes[i] for i in range(0, args.bt_tables, max(1, args.bt_tables // 64)))
+    ok &= counts_ok
+    ops = 2 * total
+    result = {
+        "metric": "Mops/s batched uneven tables (insert all, then find all), persistent launch vs one launch per table",
+        "value": round(ops / (np.median(tot_p) * 1e-3) / 1e6, 1),
+        "unit": "Mops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(np.median(tot_p)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.bt_tables} tables, key counts log-uniform in [2^{args.bt_log2min}, "
+                               f"2^{args.bt_log2max}], load factor {args.bt_theta}, finds {args.bt_hit:.0%} hit",
+                   "total_keys": total, "tables": args.bt_tables, "launch": g.launch},
+        "persistent": {"insert_ms": round(float(np.median(ins_p)), 3), "find_ms": round(float(np.median(fnd_p)), 3),
+                       "p50_ms": round(pct(tot_p, 50), 3), "p99_ms": round(pct(tot_p, 99), 3),
+                       "mops": round(ops / (np.median(tot_p) * 1e-3) / 1e6, 1)},
+        "per_table": {"insert_ms": round(float(np.median(ins_t)), 3), "find_ms": round(float(np.median(fnd_t)), 3),
+                      "p50_ms": round(pct(tot_t, 50), 3), "p99_ms": round(pct(tot_t, 99), 3),
+                      "mops": round(ops / (np.median(tot_t) * 1e-3) / 1e6, 1)},
+        "speedup": round(float(np.median(tot_t) / np.median(tot_p)), 3),
+        "autotune": tuned,
+        "parity": "ok" if ok else "mismatch",
+    }
+    g.close()
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0 if ok else 1
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -227,6 +339,8 @@ def main() -> int:
     stream = torch.cuda.current_stream()
     if args.workload == "dynamic":
         return run_dynamic(args, dev)
+    if args.workload == "batched":
+        return run_batched(args, dev)
     n = 1 << args.log2n
     hint = int(math.ceil(n / args.theta))
     log(f"generating inputs: n=2^{args.log2n}, theta={args.theta}")
@@ -245,6 +359,13 @@ def main() -> int:
     t = D.DyCuckoo(hint, find_lanes=args.find_lanes, insert_lanes=args.insert_lanes,
                    insert_blocks=args.insert_blocks, max_blocks=args.max_blocks)
     log(f"table: buckets {t.buckets}, capacity {t.capacity}, fill {n / t.capacity:.4f}")
+    tuned = None
+    if not args.no_tune:
+        from databases_gpu_amd.autotune import tune_table
+
+        tuned = tune_table(t, keys, vals, look_u, log=log, kick_bounds=(15, 0) if args.tune_kick else (15,))
+        log(f"autotune installed {tuned['installed']} ({tuned['seconds']} s)")
+    cfg = t.config()
 
     phases = ("clear", "insert", "find_uniform", "find_zipf")
     times = {p: [] for p in phases}
@@ -355,8 +476,9 @@ def main() -> int:
                         f"bulk insert + 2^{args.log2n} finds at {args.hit:.0%} hit (uniform) + 2^{args.log2n} "
                         f"finds Zipf({args.zipf})",
             "keys": n, "theta": args.theta, "table_fill": round(n / t.capacity, 5), "buckets": t.buckets,
-            "table_bytes": t.capacity * 8, "find_lanes": args.find_lanes, "insert_lanes": args.insert_lanes,
-            "insert_blocks": args.insert_blocks, "max_blocks": args.max_blocks,
+            "table_bytes": t.capacity * 8, "find_lanes": cfg["find_lanes"], "insert_lanes": cfg["insert_lanes"],
+            "insert_blocks": cfg["insert_blocks"], "max_blocks": cfg["max_blocks"],
+            "kick_bound": cfg["kick_bound"], "autotuned": tuned is not None,
             "seed": args.seed,
         },
         "roofline": roofline,

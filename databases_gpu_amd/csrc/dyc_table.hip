@@ -322,6 +322,20 @@ int reserve_partition(dyc_table *t, uint64_t n)
 bool valid_lanes(uint32_t l) { return l == 4 || l == 8 || l == 16; }
 bool valid_insert_lanes(uint32_t l) { return l == 1 || l == 2 || valid_lanes(l); }
 
+int check_config(const dyc_config &cfg)
+{
+    if (!valid_lanes(cfg.find_lanes) || !valid_insert_lanes(cfg.insert_lanes))
+        return fail(DYC_ERR_INVALID, "lanes per key must be 4, 8 or 16 (insert also 1 or 2)");
+    if (cfg.block_threads == 0 || cfg.block_threads % 64 || cfg.block_threads > 256)
+        return fail(DYC_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 256]");
+    if (!(cfg.upper_bound > 0.0 && cfg.upper_bound <= 1.0) || !(cfg.lower_bound >= 0.0) ||
+        cfg.lower_bound >= cfg.upper_bound)
+        return fail(DYC_ERR_INVALID, "need 0 <= lower_bound < upper_bound <= 1");
+    if (cfg.insert_order > 2 || cfg.region_mib == 0)
+        return fail(DYC_ERR_INVALID, "insert_order must be 0, 1 or 2 and region_mib > 0");
+    return DYC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -360,15 +374,8 @@ int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out
         cfg = *cfg_in;
     else
         dyc_config_init(&cfg);
-    if (!valid_lanes(cfg.find_lanes) || !valid_insert_lanes(cfg.insert_lanes))
-        return fail(DYC_ERR_INVALID, "lanes per key must be 4, 8 or 16 (insert also 1 or 2)");
-    if (cfg.block_threads == 0 || cfg.block_threads % 64 || cfg.block_threads > 256)
-        return fail(DYC_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 256]");
-    if (!(cfg.upper_bound > 0.0 && cfg.upper_bound <= 1.0) || !(cfg.lower_bound >= 0.0) ||
-        cfg.lower_bound >= cfg.upper_bound)
-        return fail(DYC_ERR_INVALID, "need 0 <= lower_bound < upper_bound <= 1");
-    if (cfg.insert_order > 2 || cfg.region_mib == 0)
-        return fail(DYC_ERR_INVALID, "insert_order must be 0, 1 or 2 and region_mib > 0");
+    int rc0 = check_config(cfg);
+    if (rc0) return rc0;
     const uint32_t nb = initial_buckets(capacity_hint);
     if (nb >= kMaxBuckets) return fail(DYC_ERR_INVALID, "capacity hint too large");
 
@@ -505,6 +512,22 @@ int dyc_resize_down(dyc_table *t, dyc_stream stream)
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     DeviceGuard dg(t->device);
     return do_resize_down(t, (hipStream_t)stream);
+}
+
+int dyc_get_config(const dyc_table *t, dyc_config *out)
+{
+    if (!t || !out) return fail(DYC_ERR_INVALID, "null argument");
+    *out = t->cfg;
+    return DYC_OK;
+}
+
+int dyc_set_config(dyc_table *t, const dyc_config *cfg)
+{
+    if (!t || !cfg) return fail(DYC_ERR_INVALID, "null argument");
+    int rc = check_config(*cfg);
+    if (rc) return rc;
+    t->cfg = *cfg;
+    return DYC_OK;
 }
 
 int dyc_get_stats(const dyc_table *t, dyc_stats *out)

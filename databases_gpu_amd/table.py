@@ -119,6 +119,22 @@ class DyCuckoo:
         check(rc, "dyc_resize_down")
         return True
 
+    # -- configuration
+    def config(self) -> dict:
+        c = _lib.Config()
+        check(self.lib.dyc_get_config(self.h, ctypes.byref(c)), "dyc_get_config")
+        return {name: getattr(c, name) for name, _ in c._fields_}
+
+    def configure(self, **fields) -> None:
+        """Change run-time fields (lanes, grids, eviction bounds, fill bounds); sizes stay."""
+        c = _lib.Config()
+        check(self.lib.dyc_get_config(self.h, ctypes.byref(c)), "dyc_get_config")
+        for k, v in fields.items():
+            if not hasattr(c, k):
+                raise KeyError(f"unknown config field {k}")
+            setattr(c, k, v)
+        check(self.lib.dyc_set_config(self.h, ctypes.byref(c)), "dyc_set_config")
+
     # -- inspection
     def stats(self) -> dict:
         st = _lib.Stats()

@@ -123,6 +123,12 @@ int dyc_resize_up(dyc_table *t, dyc_stream stream);
  * is odd (the reference reads past the end there, cuckoo.cu:653,767-769). Synchronous. */
 int dyc_resize_down(dyc_table *t, dyc_stream stream);
 
+/* Read / replace the run-time configuration of a table (no reference counterpart: the
+ * reference's configuration is compile-time, dy_hash.h:15-73). Sizes are not touched; every
+ * field of dyc_config is otherwise taken, validated as in dyc_create. */
+int dyc_get_config(const dyc_table *t, dyc_config *out);
+int dyc_set_config(dyc_table *t, const dyc_config *cfg);
+
 /* Counters and geometry (no reference counterpart beyond the occupancy print of
  * computer_table, cuckoo.cu:818-848). Does not touch the device. */
 int dyc_get_stats(const dyc_table *t, dyc_stats *out);

@@ -163,6 +163,39 @@ def test_insert_ordered_by_region(D, lanes):
     t.close()
 
 
+def test_set_config_reshapes_one_table(D):
+    """Launch fields changed on a live table (dyc_set_config) leave every result unchanged; an
+    invalid configuration is refused and the old one stays."""
+    n = 200_000
+    banner("set_config across shapes on one table")
+    keys = unique_keys(n, 77)
+    vals = vals_of(keys)
+    t = D.DyCuckoo(int(np.ceil(n / 0.85)))
+    kd, vd = torch.from_numpy(keys.view(np.int32)).cuda(), torch.from_numpy(vals.view(np.int32)).cuda()
+    q = np.concatenate([keys[::2], absent_keys(n // 2, keys, 78)])
+    base = None
+    for il, fl, kb, ib in [(1, 8, 15, 0), (2, 4, 0, 37), (4, 16, 3, 1), (16, 8, 15, 4096)]:
+        t.configure(insert_lanes=il, find_lanes=fl, kick_bound=kb, insert_blocks=ib)
+        assert t.config()["insert_lanes"] == il and t.config()["kick_bound"] == kb
+        t.clear()
+        t.insert(kd, vd)
+        st = t.stats()
+        assert st["count"] == n and st["pending"] == 0 and st["lost"] == 0
+        ref = oracle_from_images(t.dump())
+        assert ref.check_invariant() == 0
+        assert np.array_equal(ref.entries(), np.sort(O.pack(keys, vals)))
+        got = gpu_find(t, q)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref.find(q)))
+        base = base or got
+        assert all(np.array_equal(a, b) for a, b in zip(got, base))
+    before = t.config()
+    for bad in ({"find_lanes": 3}, {"insert_lanes": 32}, {"block_threads": 96}, {"lower_bound": 0.95}):
+        with pytest.raises(D.DycError):
+            t.configure(**bad)
+        assert t.config() == before
+    t.close()
+
+
 def test_insert_matches_sequential_replay_set(D):
     """The oracle's sequential replay and the GPU hold the same key->value set afterwards."""
     n = 300_000
