@@ -16,8 +16,9 @@ outside the timed region); insert = 8 (key, value) + i * (128 + 8), with i the m
 bucket exchanges per key from the insert kernel's counter. Achieved GB/s = bytes / kernel time;
 peak = 8 TB/s.
 
-cpu_baseline: the sequential oracle (oracle/dyc_oracle.c, one core) on a bounded sample of the
-same workload (2^24 keys by default), timed on this host.
+cpu_baseline: the same DyCuckoo on the host (oracle/dyc_oracle.c: OpenMP over keys, 64-bit CAS
+exchanges, every thread OpenMP is given) on a bounded sample of the same workload (2^26 keys by
+default), with the sequential replay on 2^22 keys alongside, timed on this host.
 """
 from __future__ import annotations
 
@@ -56,7 +57,8 @@ def parse():
     ap.add_argument("--insert-lanes", type=int, default=1)
     ap.add_argument("--insert-blocks", type=int, default=0, help="insert grid (0 = library default)")
     ap.add_argument("--max-blocks", type=int, default=0, help="find grid cap (0 = library default)")
-    ap.add_argument("--cpu-log2n", type=int, default=24, help="CPU baseline sample: 2^k keys")
+    ap.add_argument("--cpu-log2n", type=int, default=26, help="CPU baseline sample (multithreaded): 2^k keys")
+    ap.add_argument("--cpu-seq-log2n", type=int, default=22, help="sequential CPU sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
     ap.add_argument("--workload", choices=("bulk", "dynamic", "batched"), default="bulk",
@@ -85,52 +87,67 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
 
 
-def cpu_baseline(log2n: int, theta: float, hit: float, zipf_s: float, seed: int) -> dict:
-    """Sequential oracle on 2^log2n keys: insert, uniform finds, Zipf finds (one core)."""
-    import torch
-
+def cpu_baseline(log2n: int, theta: float, hit: float, zipf_s: float, seed: int, seq_log2n: int) -> dict:
+    """The same workload on the host: multithreaded DyCuckoo (OpenMP over keys, 64-bit CAS
+    exchanges) on 2^log2n keys over every thread OpenMP is given, plus the sequential replay on
+    2^seq_log2n keys for information."""
     import oracle as O
     from databases_gpu_amd import workload as W
 
-    n = 1 << log2n
-    keys_t = W.keys_range(0, n, seed, "cpu")
-    keys = keys_t.numpy().view(np.uint32).copy()
-    vals = W.splitmix32_np(keys)
-    look = W.mixed_lookups(keys_t, n, hit, seed, n, seed + 7).numpy().view(np.uint32).copy()
-    zl = W.zipf_lookups(keys_t, n, zipf_s, seed + 11).numpy().view(np.uint32).copy()
+    def inputs(k):
+        n = 1 << k
+        keys_t = W.keys_range(0, n, seed, "cpu")
+        keys = keys_t.numpy().view(np.uint32).copy()
+        look = W.mixed_lookups(keys_t, n, hit, seed, n, seed + 7).numpy().view(np.uint32).copy()
+        zl = W.zipf_lookups(keys_t, n, zipf_s, seed + 11).numpy().view(np.uint32).copy()
+        return n, keys, W.splitmix32_np(keys), look, zl
+
+    threads = int(O.lib().orc_max_threads())
+    n, keys, vals, look, zl = inputs(log2n)
     tb = O.OracleTable(int(math.ceil(n / theta)), rescue_kicks=128)
-    t0 = time.perf_counter()
-    tb.insert(keys, vals)
-    t1 = time.perf_counter()
-    v, h = tb.find(look)
-    t2 = time.perf_counter()
-    tb.find(zl)
-    t3 = time.perf_counter()
-    ok = int(h.sum()) == int(round(hit * n))
-    # the read-only find also runs multithreaded over the same image, for reference
-    mt = O.lib().orc_max_threads()
     out_v = np.empty(n, dtype=np.uint32)
     out_h = np.empty(n, dtype=np.uint8)
+    t0 = time.perf_counter()
+    tb.insert_mt(keys, vals, threads)
+    t1 = time.perf_counter()
     s = tb._struct()
-    t4 = time.perf_counter()
-    O.lib().orc_find_mt(O.ctypes.byref(s), O._p(look), n, O._p(out_v), O._p(out_h), mt)
-    t5 = time.perf_counter()
-    del torch
-    total = t3 - t0
+    O.lib().orc_find_mt(O.ctypes.byref(s), O._p(look), n, O._p(out_v), O._p(out_h), threads)
+    t2 = time.perf_counter()
+    hits_ok = int(out_h.sum()) == int(round(hit * n))
+    O.lib().orc_find_mt(O.ctypes.byref(s), O._p(zl), n, O._p(out_v), O._p(out_h), threads)
+    t3 = time.perf_counter()
+    hits_ok &= bool(out_h.all()) and tb.pending.size == 0 and tb.count == n
+    del tb, keys, vals, look, zl, out_v, out_h
+
+    ns, keys, vals, look, zl = inputs(seq_log2n)
+    sq = O.OracleTable(int(math.ceil(ns / theta)), rescue_kicks=128)
+    u0 = time.perf_counter()
+    sq.insert(keys, vals)
+    u1 = time.perf_counter()
+    sq.find(look)
+    u2 = time.perf_counter()
+    sq.find(zl)
+    u3 = time.perf_counter()
     return {
-        "value": round(3 * n / total / 1e6, 3),
+        "value": round(3 * n / (t3 - t0) / 1e6, 3),
         "unit": "Mops/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"2^{log2n} keys at load factor {theta}: insert + 2^{log2n} uniform finds ({hit:.0%} hit) "
-                  f"+ 2^{log2n} Zipf({zipf_s}) finds, sequential oracle",
+        "sample": f"2^{log2n} keys at load factor {theta}: multithreaded insert + 2^{log2n} uniform finds "
+                  f"({hit:.0%} hit) + 2^{log2n} Zipf({zipf_s}) finds, {threads} OpenMP threads",
         "insert_mops": round(n / (t1 - t0) / 1e6, 3),
         "find_uniform_mops": round(n / (t2 - t1) / 1e6, 3),
         "find_zipf_mops": round(n / (t3 - t2) / 1e6, 3),
-        "find_uniform_mt_mops": round(n / (t5 - t4) / 1e6, 3),
-        "find_mt_threads": mt,
-        "seconds": round(total, 3),
-        "hits_ok": ok,
+        "seconds": round(t3 - t0, 3),
+        "hits_ok": hits_ok,
+        "sequential": {
+            "keys": ns, "cores": 1,
+            "value": round(3 * ns / (u3 - u0) / 1e6, 3),
+            "insert_mops": round(ns / (u1 - u0) / 1e6, 3),
+            "find_uniform_mops": round(ns / (u2 - u1) / 1e6, 3),
+            "find_zipf_mops": round(ns / (u3 - u2) / 1e6, 3),
+            "seconds": round(u3 - u0, 3),
+        },
     }
 
 
@@ -493,8 +510,9 @@ def main() -> int:
     del keys, vals, look_u, look_z, out_v, out_h
     torch.cuda.empty_cache()
     if not args.no_cpu:
-        log(f"cpu baseline: oracle on 2^{args.cpu_log2n} keys")
-        result["cpu_baseline"] = cpu_baseline(args.cpu_log2n, args.theta, args.hit, args.zipf, args.seed)
+        log(f"cpu baseline: multithreaded on 2^{args.cpu_log2n} keys, sequential on 2^{args.cpu_seq_log2n}")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_log2n, args.theta, args.hit, args.zipf, args.seed,
+                                              args.cpu_seq_log2n)
     else:
         result["cpu_baseline"] = None
     line = json.dumps(result)

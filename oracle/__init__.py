@@ -83,6 +83,8 @@ def lib():
         L.orc_find.argtypes = [vp, vp, u64, vp, vp, vp]
         L.orc_find_mt.restype = None
         L.orc_find_mt.argtypes = [vp, vp, u64, vp, vp, ctypes.c_int]
+        L.orc_insert_mt.restype = u64
+        L.orc_insert_mt.argtypes = [vp, vp, vp, u64, u32, u32, vp, u64, vp, vp, ctypes.c_int]
         L.orc_max_threads.restype = ctypes.c_int
         L.orc_max_threads.argtypes = []
         L.orc_erase.restype = u64
@@ -235,6 +237,25 @@ class OracleTable:
         self.pending = np.concatenate([self.pending, fail[:nfail]])
         self._place_pending()
         self.last_kicks = kicks
+        return nfail, int(rej.value)
+
+    def insert_mt(self, keys, vals, threads: int = 0):
+        """Multithreaded insert (CPU timing baseline): same final key->value set as insert(),
+        placement depends on thread timing. Returns (failures_before_retry, rejected)."""
+        keys, vals = _u32(keys), _u32(vals)
+        n = keys.size
+        L = lib()
+        s = self._struct()
+        fail = np.zeros(max(n, 1), dtype=np.uint64)
+        kicks, rej = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        nfail = int(L.orc_insert_mt(ctypes.byref(s), _p(keys), _p(vals), n, self.kick_bound, self.rescue_kicks,
+                                    _p(fail), fail.size, ctypes.byref(kicks), ctypes.byref(rej),
+                                    threads or int(L.orc_max_threads())))
+        self.kicks_total += int(kicks.value)
+        self.fail_total += nfail
+        self.count += n - int(rej.value)
+        self.pending = np.concatenate([self.pending, fail[:min(nfail, fail.size)]])
+        self._place_pending()
         return nfail, int(rej.value)
 
     def find(self, keys):

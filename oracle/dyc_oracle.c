@@ -417,6 +417,74 @@ void orc_find_mt(const orc_table *tb, const uint32_t *keys, uint64_t n, uint32_t
     (void)threads;
 }
 
+/* Multithreaded insert: the same start table, hashes and victim rule as orc_insert_entry, with
+ * the bucket read slot by slot (relaxed atomic loads) and each exchange a 64-bit CAS, OpenMP
+ * over keys. A CAS that loses a race re-reads the bucket and repeats the step (at most
+ * ORC_MT_RETRIES times per key). Entries still in hand after the bound, or after too many lost
+ * races, are written to fail_out (up to fail_cap) and counted in the return value. Placement
+ * depends on thread timing; the stored key->value set does not. Used only as the CPU timing
+ * baseline and checked against the sequential insert by tests. */
+#define ORC_MT_RETRIES 64u
+uint64_t orc_insert_mt(orc_table *tb, const uint32_t *keys, const uint32_t *vals, uint64_t n,
+                       uint32_t kick_bound, uint32_t rescue_kicks, uint64_t *fail_out, uint64_t fail_cap,
+                       uint64_t *kicks_out, uint64_t *rejected_out, int threads)
+{
+    uint64_t nfail = 0, kicks = 0, nrej = 0;
+    const uint32_t total = kick_bound + rescue_kicks;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(threads) reduction(+ : kicks, nrej)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const uint32_t key = keys[i];
+        if (reserved_key(key)) {
+            ++nrej;
+            continue;
+        }
+        uint32_t pre = ORC_NO_TABLE;
+        if (key & 1u) pre = orc_get_pos(orc_pair(key), ORC_NO_TABLE);
+        uint64_t entry = PACK(key, vals[i]);
+        uint32_t it = 0, races = 0;
+        int placed = 0;
+        while (it < total && races <= ORC_MT_RETRIES) {
+            const uint32_t k = KEY_OF(entry);
+            const uint32_t t = orc_get_pos(orc_pair(k), pre);
+            uint64_t *b = tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS;
+            uint64_t view[ORC_SLOTS];
+            for (int j = 0; j < ORC_SLOTS; ++j) view[j] = __atomic_load_n(&b[j], __ATOMIC_RELAXED);
+            int lane = -1;
+            for (int j = 0; j < ORC_SLOTS; ++j)
+                if (KEY_OF(view[j]) == 0u) { lane = j; break; }
+            if (lane < 0 && it < kick_bound) {
+                for (int j = 0; j < ORC_SLOTS; ++j)
+                    if (orc_pair(KEY_OF(view[j])) != t) { lane = j; break; }
+                if (lane < 0) lane = (int)(it & 15u);
+            }
+            if (lane < 0) lane = (int)orc_rescue_lane(k, it);
+            uint64_t old = view[lane];
+            if (!__atomic_compare_exchange_n(&b[lane], &old, entry, 0, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED)) {
+                ++races;
+                continue;
+            }
+            ++it;
+            entry = old;
+            if (entry == 0) {
+                placed = 1;
+                break;
+            }
+            pre = t;
+        }
+        kicks += it;
+        if (!placed) {
+            uint64_t at = __atomic_fetch_add(&nfail, 1, __ATOMIC_RELAXED);
+            if (fail_out && at < fail_cap) fail_out[at] = entry;
+        }
+    }
+    if (kicks_out) *kicks_out = kicks;
+    if (rejected_out) *rejected_out = nrej;
+    (void)threads;
+    return nfail;
+}
+
 int orc_max_threads(void)
 {
 #ifdef _OPENMP

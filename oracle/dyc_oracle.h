@@ -117,6 +117,11 @@ uint64_t orc_check_invariant(const orc_table *tb);
 /* Multithreaded find over the same image (OpenMP), used only as the CPU timing baseline. */
 void orc_find_mt(const orc_table *tb, const uint32_t *keys, uint64_t n, uint32_t *vals_out,
                  uint8_t *hit_out, int threads);
+/* Multithreaded insert (OpenMP over keys, 64-bit CAS per exchange), used only as the CPU timing
+ * baseline. Returns the number of entries left unplaced (written to fail_out up to fail_cap). */
+uint64_t orc_insert_mt(orc_table *tb, const uint32_t *keys, const uint32_t *vals, uint64_t n,
+                       uint32_t kick_bound, uint32_t rescue_kicks, uint64_t *fail_out, uint64_t fail_cap,
+                       uint64_t *kicks_out, uint64_t *rejected_out, int threads);
 int orc_max_threads(void);
 
 #ifdef __cplusplus

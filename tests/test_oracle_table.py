@@ -133,6 +133,28 @@ def test_bulk_insert_find_erase_against_dict(oracle_lib, n, theta, rescue):
     assert tb.count == n - n // 3
 
 
+@pytest.mark.parametrize("threads", [1, 4])
+def test_multithreaded_insert_holds_the_sequential_set(oracle_lib, threads):
+    """The CPU baseline's concurrent insert stores exactly the key->value set of the sequential
+    replay (placement may differ), keeps the placement invariant and rejects reserved keys."""
+    n = 1 << 17
+    keys = unique_keys(n, seed=5150)
+    keys[[7, 99]] = [0, 0xFFFFFFFF]
+    vals = splitmix32(keys)
+    seq = O.OracleTable(int(np.ceil(n / 0.9)), rescue_kicks=128)
+    seq.insert(keys, vals)
+    mt = O.OracleTable(int(np.ceil(n / 0.9)), rescue_kicks=128)
+    nfail, rej = mt.insert_mt(keys, vals, threads=threads)
+    assert rej == 2 and mt.pending.size == 0 and mt.count == n - 2
+    assert mt.check_invariant() == 0
+    assert np.array_equal(mt.entries(), seq.entries())
+    v, h = mt.find(keys)
+    ok = (keys != 0) & (keys != 0xFFFFFFFF)
+    assert h[ok].all() and not h[~ok].any() and np.array_equal(v[ok], vals[ok])
+    if threads == 1:  # one thread takes keys in input order: the sequential image exactly
+        assert all(np.array_equal(a, b) for a, b in zip(mt.sub, seq.sub))
+
+
 def test_reference_kick_bound_alone_fails_at_high_fill(oracle_lib):
     """With the reference rule only (rescue 0) a fill of 0.90 leaves entries in hand -- the
     reference drops them (cuckoo.cu:422-431); the oracle keeps them and upsizes."""
