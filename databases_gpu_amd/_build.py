@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdyc.so")
-SOURCES = ["dyc_kernels.hip", "dyc_table.hip", "dyc_group.hip"]
+SOURCES = ["dyc_kernels.hip", "dyc_table.hip", "dyc_group.hip", "dyc_server.hip"]
 ARCH = os.environ.get("DYC_OFFLOAD_ARCH", "gfx950")
 
 

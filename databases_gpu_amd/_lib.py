@@ -37,7 +37,11 @@ EXPORTS = (
     "dyc_group_insert", "dyc_group_find", "dyc_group_erase", "dyc_group_insert_per_table",
     "dyc_group_find_per_table", "dyc_group_table_stats", "dyc_group_dump", "dyc_group_pending",
     "dyc_group_clear",
+    "dyc_server_config_init", "dyc_server_create", "dyc_server_destroy", "dyc_server_submit", "dyc_server_poll",
+    "dyc_server_drain", "dyc_server_replay", "dyc_server_get_stats", "dyc_server_table_stats", "dyc_server_dump",
 )
+
+OP_INSERT, OP_FIND, OP_ERASE = 0, 1, 2
 
 
 class DycError(RuntimeError):
@@ -102,6 +106,43 @@ class GroupLaunch(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("table", ctypes.c_uint32),
+        ("op", ctypes.c_uint32),
+        ("keys", ctypes.c_void_p),
+        ("vals", ctypes.c_void_p),
+        ("out_vals", ctypes.c_void_p),
+        ("out_flags", ctypes.c_void_p),
+        ("n", ctypes.c_uint64),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_uint64),
+        ("tick_max_keys", ctypes.c_uint64),
+        ("tick_max_requests", ctypes.c_uint32),
+        ("pipeline_depth", ctypes.c_uint32),
+        ("item_keys", ctypes.c_uint32),
+        ("pool_cap", ctypes.c_uint32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "submitted", "completed", "failed", "ticks", "keys", "deferred_order", "deferred_memory", "deferred_slots",
+        "resizes_up", "resizes_down", "pool_entries", "pool_rounds", "arena_bytes", "arena_used", "arena_peak",
+        "queued", "in_flight")]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# numpy view of dyc_request, for building request arrays in bulk
+REQUEST_DTYPE = [("table", "<u4"), ("op", "<u4"), ("keys", "<u8"), ("vals", "<u8"), ("out_vals", "<u8"),
+                 ("out_flags", "<u8"), ("n", "<u8")]
+
 _lib = None
 
 
@@ -153,6 +194,16 @@ def load(require: bool = True):
         "dyc_group_dump": (ci, [vp, u32, u32, vp, u64, vp]),
         "dyc_group_pending": (ci, [vp, vp, vp, u64, ctypes.POINTER(u64), vp]),
         "dyc_group_clear": (ci, [vp, vp]),
+        "dyc_server_config_init": (ci, [ctypes.POINTER(ServerConfig)]),
+        "dyc_server_create": (ci, [ctypes.POINTER(Config), ctypes.POINTER(ServerConfig), u32, vp, ctypes.POINTER(vp)]),
+        "dyc_server_destroy": (ci, [vp]),
+        "dyc_server_submit": (ci, [vp, ctypes.POINTER(Request), ctypes.POINTER(u64)]),
+        "dyc_server_poll": (ci, [vp, vp, vp, u32, ctypes.POINTER(u32)]),
+        "dyc_server_drain": (ci, [vp]),
+        "dyc_server_replay": (ci, [vp, vp, u64, vp, u32, vp, vp]),
+        "dyc_server_get_stats": (ci, [vp, ctypes.POINTER(ServerStats)]),
+        "dyc_server_table_stats": (ci, [vp, u32, ctypes.POINTER(Stats)]),
+        "dyc_server_dump": (ci, [vp, u32, u32, vp, u64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -19,6 +19,23 @@ constexpr uint32_t kMaxBuckets = 1u << 30;  // packed bucket locations use 30 bi
 // Buckets per subtable for a slot-count hint (api.cpp:47-56).
 uint32_t initial_buckets(uint64_t hint);
 
+// The subtable an upsize doubles (api.cpp:116-122): the first i with nb[i] == 2 nb[i+1] gives
+// i + 1, else subtable 0.
+inline uint32_t choose_up(const uint32_t nb[4])
+{
+    for (uint32_t i = 0; i + 1 < kSubtables; ++i)
+        if (nb[i] == 2u * nb[i + 1]) return i + 1;
+    return 0;
+}
+
+// The subtable a downsize halves (api.cpp:147-153): the first i with nb[i] > nb[i+1], else 3.
+inline uint32_t choose_down(const uint32_t nb[4])
+{
+    for (uint32_t i = 0; i + 1 < kSubtables; ++i)
+        if (nb[i] > nb[i + 1]) return i;
+    return kSubtables - 1;
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev)

@@ -393,9 +393,11 @@ template <int G, bool FROM_ENTRIES>
 __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ vals,
                                                      const uint64_t *__restrict__ entries, uint64_t n,
+                                                     const unsigned long long *__restrict__ n_dev,
                                                      uint32_t pre_fixed, InsertPolicy P, Counters *__restrict__ C,
                                                      PendSink sink)
 {
+    if (n_dev && *n_dev < n) n = *n_dev;  // a count an earlier launch left on the device
     constexpr int NG = kWave / G;
     constexpr uint64_t kChunk = 2 * NG;  // indexes a wave takes at a time
     Carry<kSlots / G> c;
@@ -733,6 +735,90 @@ __global__ __launch_bounds__(256) void group_insert_kernel(const Desc *__restric
     wave_add(&C->retries, tally.retries);
 }
 
+// ------------------------------------------------------------------------------ serving
+// Items of a serving tick carry their request's own pointers; otherwise the tile code is the
+// single-table code. Finds and erases that miss both buckets also look in the server's pool of
+// entries that exhausted the eviction bound (normally empty: one scalar load per tile).
+template <bool ERASE>
+__device__ inline uint32_t pool_probe(const Pool &pool, uint64_t np, const ServeItem &item, uint64_t base, uint32_t f)
+{
+    const uint64_t i = base + __lane_id();
+    if (i >= item.len || f) return 0;
+    const uint32_t k = item.keys[i];
+    if (reserved_key(k)) return 0;
+    uint32_t got = 0;
+    for (uint64_t j = 0; j < np; ++j) {
+        const uint64_t e = __hip_atomic_load(pool.pend + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (key_of(e) != k || pool.tags[j] != item.table) continue;
+        if constexpr (ERASE) {  // the pool counts as one more bucket: clear every copy in it
+            uint64_t exp = e;
+            got |= (uint32_t)cas_slot(pool.pend + j, exp, 0ull);
+        } else {
+            if (item.out_vals) item.out_vals[i] = val_of(e);
+            got = 1;
+            break;
+        }
+    }
+    if (got && item.flags) item.flags[i] = 1;
+    return got;
+}
+
+template <int G, bool ERASE>
+__global__ __launch_bounds__(256) void serve_probe_kernel(const Desc *__restrict__ dir,
+                                                          const ServeItem *__restrict__ items, uint32_t n_items,
+                                                          Pool pool, unsigned long long *__restrict__ del_total)
+{
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    const unsigned long long pc = *pool.count;
+    const uint64_t np = pc < pool.cap ? pc : pool.cap;
+    for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
+         it += waves) {
+        const ServeItem item = items[it];
+        const Desc D = dir[item.table];
+        unsigned long long nd = 0;
+        for (uint64_t base = 0; base < item.len; base += kWave) {
+            uint32_t f = probe_tile<G, ERASE>(D, item.keys, ERASE ? nullptr : item.out_vals, item.flags, base, item.len);
+            if (np) f |= pool_probe<ERASE>(pool, np, item, base, f);
+            nd += f;
+        }
+        if constexpr (ERASE) {
+            nd = wave_sum(nd);
+            if (__lane_id() == 0 && nd) atomicAdd(&del_total[item.table], nd);
+        }
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void serve_insert_kernel(const Desc *__restrict__ dir,
+                                                           const ServeItem *__restrict__ items, uint32_t n_items,
+                                                           InsertPolicy P, Counters *__restrict__ C, PendSink sink,
+                                                           unsigned long long *__restrict__ ins_total)
+{
+    Carry<kSlots / G> c;
+    c.active = false;
+    InsertTally tally;
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
+         it += waves) {
+        const ServeItem item = items[it];
+        const Desc D = dir[item.table];
+        uint64_t next = 0;
+        const uint64_t end = item.len;
+        const unsigned long long rej0 = tally.rejected;
+        uint64_t taken = 0;
+        while (true) {
+            refill<G, false>(c, next, end, item.keys, item.vals, nullptr, kNoTable, tally, taken);
+            if (!__ballot(c.active)) break;
+            if (c.active) kick_step<G>(D, c, P, C, sink, item.table, tally);
+        }
+        const unsigned long long rej = wave_sum(tally.rejected - rej0);
+        if (__lane_id() == 0 && item.len > rej) atomicAdd(&ins_total[item.table], item.len - rej);
+    }
+    wave_add(&C->kicks, tally.kicks);
+    wave_add(&C->rejected, tally.rejected);
+    wave_add(&C->retries, tally.retries);
+}
+
 __global__ void zero_kernel(unsigned long long *p, uint64_t words)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
@@ -797,27 +883,32 @@ hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, u
 
 template <bool FE>
 static hipError_t insert_common(const Desc &D, const uint32_t *keys, const uint32_t *vals, const uint64_t *entries,
-                                uint64_t n, uint32_t pre, int lanes, InsertPolicy pol, LaunchShape sh, Counters *C,
-                                PendSink sink, hipStream_t st)
+                                uint64_t n, const unsigned long long *n_dev, uint32_t pre, int lanes,
+                                InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     const int per_wave = kWave / lanes;  // entries in flight per wave
     const int grid = grid_for((n + per_wave - 1) / per_wave, sh, kInsertDefaultBlocks);
     switch (lanes) {
     case 8:
-        insert_kernel<8, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
+        insert_kernel<8, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+                                                                      sink);
         break;
     case 4:
-        insert_kernel<4, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
+        insert_kernel<4, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+                                                                      sink);
         break;
     case 2:
-        insert_kernel<2, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
+        insert_kernel<2, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+                                                                      sink);
         break;
     case 1:
-        insert_kernel<1, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
+        insert_kernel<1, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+                                                                      sink);
         break;
     default:
-        insert_kernel<16, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C, sink);
+        insert_kernel<16, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+                                                                      sink);
         break;
     }
     return hipGetLastError();
@@ -826,13 +917,14 @@ static hipError_t insert_common(const Desc &D, const uint32_t *keys, const uint3
 hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint64_t n, int lanes,
                          InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st)
 {
-    return insert_common<false>(D, keys, vals, nullptr, n, kNoTable, lanes, pol, sh, C, sink, st);
+    return insert_common<false>(D, keys, vals, nullptr, n, nullptr, kNoTable, lanes, pol, sh, C, sink, st);
 }
 
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
-                                 InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st)
+                                 InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
+                                 const unsigned long long *n_dev)
 {
-    return insert_common<true>(D, nullptr, nullptr, entries, n, pre, lanes, pol, sh, C, sink, st);
+    return insert_common<true>(D, nullptr, nullptr, entries, n, n_dev, pre, lanes, pol, sh, C, sink, st);
 }
 
 hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,
@@ -924,6 +1016,52 @@ hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t
     default:
         group_insert_kernel<1><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
         break;
+    }
+    return hipGetLastError();
+}
+
+static int serve_grid(uint32_t n_items, int blocks)
+{
+    return blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
+}
+
+hipError_t launch_serve_find(const Desc *dir, const ServeItem *items, uint32_t n_items, Pool pool, int lanes,
+                             int blocks, hipStream_t st)
+{
+    if (n_items == 0) return hipSuccess;
+    const int grid = serve_grid(n_items, blocks);
+    switch (lanes) {
+    case 4: serve_probe_kernel<4, false><<<grid, 256, 0, st>>>(dir, items, n_items, pool, nullptr); break;
+    case 16: serve_probe_kernel<16, false><<<grid, 256, 0, st>>>(dir, items, n_items, pool, nullptr); break;
+    default: serve_probe_kernel<8, false><<<grid, 256, 0, st>>>(dir, items, n_items, pool, nullptr); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_erase(const Desc *dir, const ServeItem *items, uint32_t n_items, Pool pool,
+                              unsigned long long *del_total, int lanes, int blocks, hipStream_t st)
+{
+    if (n_items == 0) return hipSuccess;
+    const int grid = serve_grid(n_items, blocks);
+    switch (lanes) {
+    case 4: serve_probe_kernel<4, true><<<grid, 256, 0, st>>>(dir, items, n_items, pool, del_total); break;
+    case 16: serve_probe_kernel<16, true><<<grid, 256, 0, st>>>(dir, items, n_items, pool, del_total); break;
+    default: serve_probe_kernel<8, true><<<grid, 256, 0, st>>>(dir, items, n_items, pool, del_total); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_insert(const Desc *dir, const ServeItem *items, uint32_t n_items, InsertPolicy pol, int lanes,
+                               int blocks, Counters *C, PendSink sink, unsigned long long *ins_total, hipStream_t st)
+{
+    if (n_items == 0) return hipSuccess;
+    const int grid = serve_grid(n_items, blocks);
+    switch (lanes) {
+    case 2: serve_insert_kernel<2><<<grid, 256, 0, st>>>(dir, items, n_items, pol, C, sink, ins_total); break;
+    case 4: serve_insert_kernel<4><<<grid, 256, 0, st>>>(dir, items, n_items, pol, C, sink, ins_total); break;
+    case 8: serve_insert_kernel<8><<<grid, 256, 0, st>>>(dir, items, n_items, pol, C, sink, ins_total); break;
+    case 16: serve_insert_kernel<16><<<grid, 256, 0, st>>>(dir, items, n_items, pol, C, sink, ins_total); break;
+    default: serve_insert_kernel<1><<<grid, 256, 0, st>>>(dir, items, n_items, pol, C, sink, ins_total); break;
     }
     return hipGetLastError();
 }

@@ -50,8 +50,10 @@ hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, u
 // table `pre` (kNoTable = use the key-parity start rule).
 hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint64_t n, int lanes,
                          InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st);
+// n_dev (optional): a device count read at kernel start; entries [0, min(n, *n_dev)) are inserted.
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
-                                 InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st);
+                                 InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
+                                 const unsigned long long *n_dev = nullptr);
 // Order an insert batch by the region of its first bucket: out[0, n) receives the packed
 // entries, region by region. counts and cursors are R words each (counts zeroed by the caller).
 hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,
@@ -76,6 +78,35 @@ hipError_t launch_group_erase(const Desc *dir, const GroupItem *items, uint32_t 
 hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                                const uint32_t *vals, InsertPolicy pol, int lanes, int blocks, Counters *C,
                                PendSink sink, unsigned long long *table_counts, hipStream_t st);
+
+// ---- serving ticks
+// A work item of a serving tick: keys [0, len) of one request (pointers already offset to the
+// item's first key) against table `table` of the directory.
+struct ServeItem {
+    const uint32_t *keys;
+    const uint32_t *vals;  // insert: values
+    uint32_t *out_vals;    // find: values out (may be null)
+    uint8_t *flags;        // find: hit flags; erase: deleted flags (may be null)
+    uint32_t len;
+    uint32_t table;
+};
+
+// Entries that exhausted the eviction bound, shared by all tables of a server (tags = table)
+// and searched by its finds and erases until the host re-places them. count is the append index
+// (the `pending` word of the Counters its insert launches add to); entries beyond cap are lost.
+struct Pool {
+    uint64_t *pend;
+    const uint32_t *tags;
+    uint64_t cap;
+    const unsigned long long *count;
+};
+
+hipError_t launch_serve_insert(const Desc *dir, const ServeItem *items, uint32_t n_items, InsertPolicy pol, int lanes,
+                               int blocks, Counters *C, PendSink sink, unsigned long long *ins_total, hipStream_t st);
+hipError_t launch_serve_find(const Desc *dir, const ServeItem *items, uint32_t n_items, Pool pool, int lanes,
+                             int blocks, hipStream_t st);
+hipError_t launch_serve_erase(const Desc *dir, const ServeItem *items, uint32_t n_items, Pool pool,
+                              unsigned long long *del_total, int lanes, int blocks, hipStream_t st);
 
 // upsize subtable t: old has old_nb buckets, dst has 2*old_nb and is written completely.
 hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, uint32_t t, Counters *C,

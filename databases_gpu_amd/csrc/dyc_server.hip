@@ -1,0 +1,972 @@
+// dyc_server.hip -- the serving loop: the dyc_server_* part of include/dyc.h.
+//
+// Iteration-level continuous batching on one stream the server owns:
+//   * requests (one reference batch call each: insert, find or erase of one table) queue on the
+//     host in arrival order;
+//   * a "tick" admits every queued request that fits (slots, per-table order, arena space),
+//     enqueues the upsizes those inserts need, uploads the tick's work items (and the table
+//     directory if a resize changed it), and launches one fused insert, one fused find and one
+//     fused erase over all of them, then a copy of the device counters and an event;
+//   * up to pipeline_depth ticks are in flight: the next tick is built and enqueued while the
+//     previous one runs, so the device does not wait on the host between ticks;
+//   * a tick retires when its event has fired: its requests complete, table counts are
+//     refreshed from the counter snapshot, tables whose erases retired are checked for a
+//     downsize, and entries that exhausted the eviction bound are re-placed.
+// All table storage is carved from one arena (dyc_arena.hpp), so admission by free HBM is exact
+// and no hipMalloc/hipFree runs while ticks are in flight.
+#include "dyc_arena.hpp"
+#include "dyc_host.hpp"
+#include "dyc_kernels.hpp"
+
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <vector>
+
+using namespace dyc;
+using dyc::host::choose_down;
+using dyc::host::choose_up;
+using dyc::host::DeviceGuard;
+using dyc::host::fail;
+using dyc::host::initial_buckets;
+using dyc::host::kMaxBuckets;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr uint32_t kTargetItems = 8192;  // work items per tick: one per resident wave (256 CUs x 32)
+constexpr uint32_t kMinItemKeys = 64;
+constexpr uint32_t kMaxItemKeys = 4096;
+
+struct SrvTable {
+    Desc D{};
+    uint64_t off[kSubtables] = {};  // arena offsets of the subtables
+    uint64_t count = 0;             // live entries as of the last retired tick
+    uint64_t inflight_ins = 0;      // insert keys admitted in ticks not yet retired
+    uint32_t inflight = 0;          // ticks in flight that touch the table
+    bool check_down = false;        // an erase retired since the last downsize check
+    uint64_t sync_del = 0;          // entries deleted by synchronous erases (not in del_total)
+    uint64_t resizes_up = 0, resizes_down = 0;
+
+    uint64_t capacity() const
+    {
+        uint64_t c = 0;
+        for (int i = 0; i < kSubtables; ++i) c += (uint64_t)D.nb[i] * kSlots;
+        return c;
+    }
+};
+
+struct Req {
+    dyc_request r;
+    int status = DYC_OK;
+    int state = 0;  // 0 queued, 1 admitted, 2 done
+};
+
+struct TickTable {
+    uint32_t table;
+    uint64_t ins;  // insert keys admitted for the table in this tick
+    bool erase;
+};
+
+struct Tick {
+    ServeItem *h_items = nullptr;  // pinned staging: insert, find, erase items back to back
+    ServeItem *d_items = nullptr;
+    uint64_t items_cap = 0;
+    Desc *h_dir = nullptr;                 // pinned copy of the directory this tick uploads
+    unsigned long long *h_snap = nullptr;  // pinned: Counters, ins_total[n], del_total[n]
+    Counters *d_rctr = nullptr;            // downsize compaction counters
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    uint64_t seq = 0;
+    std::vector<uint64_t> reqs;
+    std::vector<TickTable> tables;
+};
+
+}  // namespace
+
+struct dyc_server {
+    dyc_config cfg;
+    dyc_server_config scfg;
+    int device = 0;
+    hipStream_t st = nullptr;
+    uint32_t n = 0;
+    uint8_t *base = nullptr;  // arena memory
+    Arena arena;
+    std::vector<SrvTable> t;
+    Desc *d_dir = nullptr;
+    bool dir_dirty = true;
+    unsigned long long *d_stat = nullptr;  // Counters words, then ins_total[n], then del_total[n]
+    uint64_t *d_pool = nullptr;
+    uint32_t *d_pool_tag = nullptr;
+    uint64_t *d_resolve = nullptr;  // staging for re-placing pool entries (pool_cap entries)
+    uint64_t pool_settled = 0;      // pool entries left by the last re-placement
+    uint64_t deleted_seen = 0;      // Counters::deleted as of the last synchronous erase
+    uint64_t misplaced_seen = 0;
+    Tick ticks[2];
+    uint64_t tick_seq = 0;
+    std::vector<Req> reqs;
+    std::deque<uint64_t> queue;                 // tickets waiting for admission, arrival order
+    std::deque<std::pair<uint64_t, int>> done;  // finished, not yet handed out
+    std::vector<uint32_t> down_list;            // tables with check_down set
+    int error = DYC_OK;                         // sticky: lost entries or a corrupt resize
+    dyc_server_stats ss{};
+    // admission scratch, indexed by table
+    std::vector<int8_t> phase;     // last op admitted for the table in the tick being built, -1 none
+    std::vector<uint8_t> blocked;  // 1 order, 2 memory
+    std::vector<uint8_t> in_tick;  // the tick being built works on the table (request or downsize)
+    std::vector<uint8_t> seen;     // listed in touched
+    std::vector<uint64_t> tick_ins;
+    std::vector<uint64_t> mem_head;  // the request a memory block held back (valid when blocked == 2)
+    std::vector<uint32_t> touched;
+    std::vector<ServeItem> items[3];
+
+    Counters *pc() const { return reinterpret_cast<Counters *>(d_stat); }
+    unsigned long long *ins_total() const { return d_stat + kCounterWords; }
+    unsigned long long *del_total() const { return d_stat + kCounterWords + n; }
+    size_t stat_bytes() const { return (kCounterWords + 2 * (size_t)n) * sizeof(unsigned long long); }
+    uint64_t *sub_ptr(uint64_t off) const { return reinterpret_cast<uint64_t *>(base + off); }
+    InsertPolicy policy() const { return InsertPolicy{cfg.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
+    LaunchShape shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.max_blocks}; }
+    LaunchShape insert_shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.insert_blocks}; }
+    Pool pool() const { return Pool{d_pool, d_pool_tag, scfg.pool_cap, &pc()->pending}; }
+    PendSink sink(uint32_t tag, unsigned long long *placed = nullptr) const
+    {
+        return PendSink{d_pool, d_pool_tag, scfg.pool_cap, tag, placed};
+    }
+    uint32_t depth() const { return scfg.pipeline_depth; }
+};
+
+namespace {
+
+double seconds_since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+int sticky(dyc_server *s, int rc)
+{
+    if (rc != DYC_OK && s->error == DYC_OK) s->error = rc;
+    return rc;
+}
+
+void refresh_counts(dyc_server *s, const unsigned long long *snap)
+{
+    const unsigned long long *ins = snap + kCounterWords, *del = snap + kCounterWords + s->n;
+    for (uint32_t i = 0; i < s->n; ++i) s->t[i].count = ins[i] - del[i] - s->t[i].sync_del;
+}
+
+// Double one subtable of table i (choose_up); *ok = false (nothing changed) if the arena has
+// no room for the new subtable.
+int upsize_now(dyc_server *s, uint32_t i, bool *ok)
+{
+    SrvTable &T = s->t[i];
+    *ok = false;
+    const uint32_t sub = choose_up(T.D.nb);
+    const uint32_t old_nb = T.D.nb[sub];
+    if ((uint64_t)old_nb * 2 >= kMaxBuckets) return DYC_OK;
+    const uint64_t off = s->arena.alloc((uint64_t)old_nb * 2 * kBucketBytes);
+    if (off == Arena::kNone) return DYC_OK;
+    HIP_TRY(launch_upsize(T.D.sub[sub], old_nb, s->sub_ptr(off), sub, s->pc(), s->st));
+    s->arena.release(T.off[sub]);
+    T.off[sub] = off;
+    desc_set(T.D, (int)sub, s->sub_ptr(off), old_nb * 2);
+    s->dir_dirty = true;
+    T.resizes_up++;
+    s->ss.resizes_up++;
+    *ok = true;
+    return DYC_OK;
+}
+
+// Halve one subtable of table i (choose_down) and re-insert the compaction's residuals into the
+// other subtable of their pair (the device holds their count). *done = false if refused (odd
+// bucket count) or the arena has no room for the new subtable and the residual list.
+int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
+{
+    SrvTable &T = s->t[i];
+    *done = false;
+    const uint32_t sub = choose_down(T.D.nb);
+    const uint32_t old_nb = T.D.nb[sub];
+    if ((old_nb & 1u) || old_nb < 2) return DYC_OK;
+    const uint32_t new_nb = old_nb / 2;
+    const uint64_t res_cap = (uint64_t)new_nb * kSlots;  // one pair of old buckets leaves at most 16
+    const uint64_t off = s->arena.alloc((uint64_t)new_nb * kBucketBytes);
+    if (off == Arena::kNone) return DYC_OK;
+    const uint64_t roff = s->arena.alloc(res_cap * sizeof(uint64_t));
+    if (roff == Arena::kNone) {
+        s->arena.release(off);
+        return DYC_OK;
+    }
+    uint64_t *res = s->sub_ptr(roff);
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(rctr), kCounterWords, s->st));
+    HIP_TRY(launch_downsize(T.D.sub[sub], new_nb, s->sub_ptr(off), rctr, res, res_cap, s->st));
+    s->arena.release(T.off[sub]);
+    T.off[sub] = off;
+    desc_set(T.D, (int)sub, s->sub_ptr(off), new_nb);
+    s->dir_dirty = true;
+    // residuals continue from the shrunk subtable into the other one of their pair
+    // (cuckoo.cu:719), after the compaction; failures go to the pool
+    HIP_TRY(launch_insert_entries(T.D, res, res_cap, sub, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
+                                  s->pc(), s->sink(i), s->st, &rctr->pending));
+    s->arena.release(roff);
+    T.resizes_down++;
+    s->ss.resizes_down++;
+    *done = true;
+    return DYC_OK;
+}
+
+bool wants_down(const dyc_server *s, const SrvTable &T)
+{
+    const uint64_t cap = T.capacity();
+    return (double)T.count < s->cfg.lower_bound * (double)cap && (double)cap > (double)s->cfg.min_capacity;
+}
+
+int read_stats_sync(dyc_server *s, unsigned long long *dst)
+{
+    HIP_TRY(hipMemcpyAsync(dst, s->d_stat, s->stat_bytes(), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return DYC_OK;
+}
+
+// Re-place the pool: up to max_fail_rounds rounds of (upsize each table that has pool entries,
+// re-insert them). Runs with the stream drained; entries that still fail stay in the pool,
+// where finds and erases keep seeing them.
+int resolve_pool(dyc_server *s, unsigned long long *scratch)
+{
+    HIP_TRY(hipStreamSynchronize(s->st));
+    std::vector<uint64_t> ent;
+    std::vector<uint32_t> tag;
+    for (uint32_t round = 0; round < s->cfg.max_fail_rounds; ++round) {
+        int rc = read_stats_sync(s, scratch);
+        if (rc) return rc;
+        const Counters &c = *reinterpret_cast<const Counters *>(scratch);
+        if (c.lost) return fail(DYC_ERR_LOST, "overflow pool full: %llu entries lost", (unsigned long long)c.lost);
+        const uint64_t m = std::min<uint64_t>(c.pending, s->scfg.pool_cap);
+        if (m == 0) {
+            s->pool_settled = 0;
+            return DYC_OK;
+        }
+        ent.resize(m);
+        tag.resize(m);
+        HIP_TRY(hipMemcpy(ent.data(), s->d_pool, m * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(tag.data(), s->d_pool_tag, m * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        // empty the pool (its append index), then re-insert table by table
+        HIP_TRY(launch_zero(&s->pc()->pending, 1, s->st));
+        std::vector<uint64_t> order(m);
+        for (uint64_t j = 0; j < m; ++j) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return tag[a] < tag[b]; });
+        std::vector<uint64_t> packed;
+        packed.reserve(m);
+        struct Span {
+            uint32_t table;
+            uint64_t begin, len;
+        };
+        std::vector<Span> spans;
+        for (uint64_t j : order) {
+            if (ent[j] == 0 || tag[j] >= s->n) continue;  // erased while pooled
+            if (spans.empty() || spans.back().table != tag[j]) spans.push_back(Span{tag[j], packed.size(), 0});
+            packed.push_back(ent[j]);
+            spans.back().len++;
+        }
+        if (!packed.empty())
+            HIP_TRY(hipMemcpyAsync(s->d_resolve, packed.data(), packed.size() * sizeof(uint64_t),
+                                   hipMemcpyHostToDevice, s->st));
+        for (const Span &sp : spans) {
+            bool ok = false;
+            rc = upsize_now(s, sp.table, &ok);
+            if (rc) return rc;
+            HIP_TRY(launch_insert_entries(s->t[sp.table].D, s->d_resolve + sp.begin, sp.len, kNoTable,
+                                          (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(), s->pc(),
+                                          s->sink(sp.table), s->st));
+        }
+        HIP_TRY(hipStreamSynchronize(s->st));
+        s->ss.pool_rounds++;
+    }
+    int rc = read_stats_sync(s, scratch);
+    if (rc) return rc;
+    const Counters &c = *reinterpret_cast<const Counters *>(scratch);
+    if (c.lost) return fail(DYC_ERR_LOST, "overflow pool full: %llu entries lost", (unsigned long long)c.lost);
+    s->pool_settled = std::min<uint64_t>(c.pending, s->scfg.pool_cap);
+    return DYC_OK;
+}
+
+void finish(dyc_server *s, uint64_t tk, int status)
+{
+    Req &R = s->reqs[tk];
+    R.state = 2;
+    R.status = status;
+    s->done.emplace_back(tk, status);
+    s->ss.completed++;
+    if (status != DYC_OK) s->ss.failed++;
+    s->ss.keys += R.r.n;
+}
+
+// Retire tick k (its event has fired).
+int retire(dyc_server *s, Tick &k)
+{
+    const Counters &c = *reinterpret_cast<const Counters *>(k.h_snap);
+    int status = s->error;
+    if (c.lost) status = sticky(s, fail(DYC_ERR_LOST, "overflow pool full: %llu entries lost", (unsigned long long)c.lost));
+    if (c.misplaced > s->misplaced_seen) {
+        s->misplaced_seen = c.misplaced;
+        status = sticky(s, fail(DYC_ERR_CORRUPT, "a resize found entries outside their bucket pair"));
+    }
+    refresh_counts(s, k.h_snap);
+    for (const TickTable &tt : k.tables) {
+        SrvTable &T = s->t[tt.table];
+        T.inflight--;
+        T.inflight_ins -= tt.ins;
+        if (tt.erase && !T.check_down) {
+            T.check_down = true;
+            s->down_list.push_back(tt.table);
+        }
+    }
+    k.busy = false;
+    if (status == DYC_OK && std::min<uint64_t>(c.pending, s->scfg.pool_cap) > s->pool_settled) {
+        int rc = resolve_pool(s, k.h_snap);
+        if (rc) status = sticky(s, rc);
+        else refresh_counts(s, k.h_snap);
+    }
+    for (uint64_t tk : k.reqs) finish(s, tk, status);
+    k.reqs.clear();
+    k.tables.clear();
+    return DYC_OK;
+}
+
+// Retire finished ticks, oldest first; with `wait`, block on the oldest one.
+int retire_ready(dyc_server *s, bool wait)
+{
+    while (true) {
+        Tick *oldest = nullptr;
+        for (uint32_t i = 0; i < s->depth(); ++i)
+            if (s->ticks[i].busy && (!oldest || s->ticks[i].seq < oldest->seq)) oldest = &s->ticks[i];
+        if (!oldest) return DYC_OK;
+        hipError_t e = wait ? hipEventSynchronize(oldest->done) : hipEventQuery(oldest->done);
+        if (e == hipErrorNotReady) return DYC_OK;
+        if (e != hipSuccess) return sticky(s, fail(DYC_ERR_HIP, "tick event: %s", hipGetErrorString(e)));
+        int rc = retire(s, *oldest);
+        if (rc) return rc;
+        wait = false;
+    }
+}
+
+void mark(dyc_server *s, uint32_t x)
+{
+    if (!s->seen[x]) {
+        s->seen[x] = 1;
+        s->touched.push_back(x);
+    }
+}
+
+// Admit queued requests into a free tick slot and enqueue it. *launched tells whether a tick
+// went out; *stuck is set when nothing could be admitted for lack of arena space while no tick
+// is in flight (nothing would ever free it).
+int build_tick(dyc_server *s, bool *launched, bool *stuck)
+{
+    *launched = false;
+    *stuck = false;
+    Tick *slot = nullptr;
+    bool any_busy = false;
+    for (uint32_t i = 0; i < s->depth(); ++i) {
+        if (!s->ticks[i].busy && !slot) slot = &s->ticks[i];
+        any_busy |= s->ticks[i].busy;
+    }
+    if (!slot) return DYC_OK;
+    Tick &k = *slot;
+    k.reqs.clear();
+    k.tables.clear();
+    for (uint32_t x : s->touched) {
+        s->phase[x] = -1;
+        s->blocked[x] = 0;
+        s->in_tick[x] = 0;
+        s->seen[x] = 0;
+        s->tick_ins[x] = 0;
+    }
+    s->touched.clear();
+    bool resized = false;
+
+    // downsizes of idle tables whose erases retired (freed space is admissible at once)
+    if (!s->down_list.empty()) {
+        std::vector<uint32_t> keep;
+        for (uint32_t x : s->down_list) {
+            SrvTable &T = s->t[x];
+            if (T.inflight) {
+                keep.push_back(x);
+                continue;
+            }
+            T.check_down = false;
+            if (!wants_down(s, T)) continue;
+            bool did = false;
+            int rc = downsize_now(s, x, k.d_rctr, &did);
+            if (rc) return sticky(s, rc);
+            if (did) {
+                resized = true;
+                mark(s, x);
+                s->in_tick[x] = 1;
+            }
+        }
+        s->down_list.swap(keep);
+    }
+
+    // admission, arrival order
+    uint64_t tick_keys = 0;
+    uint32_t nreq = 0;
+    bool memory_blocked = false;
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        const uint64_t tk = *it;
+        Req &R = s->reqs[tk];
+        const uint32_t x = R.r.table, op = R.r.op;
+        if (s->blocked[x]) {
+            ++it;
+            continue;
+        }
+        if (nreq > 0 && (nreq >= s->scfg.tick_max_requests || tick_keys + R.r.n > s->scfg.tick_max_keys)) {
+            s->ss.deferred_slots++;
+            break;
+        }
+        const int ph = s->phase[x];
+        if (ph > (int)op || ph == (int)DYC_OP_ERASE) {
+            mark(s, x);
+            s->blocked[x] = 1;
+            s->ss.deferred_order++;
+            ++it;
+            continue;
+        }
+        if (op == DYC_OP_INSERT) {
+            SrvTable &T = s->t[x];
+            bool room = true;
+            while ((double)(T.count + T.inflight_ins + s->tick_ins[x] + R.r.n) >
+                   s->cfg.upper_bound * (double)T.capacity()) {
+                bool ok = false;
+                int rc = upsize_now(s, x, &ok);
+                if (rc) return sticky(s, rc);
+                if (!ok) {
+                    room = false;
+                    break;
+                }
+                resized = true;
+            }
+            if (!room) {
+                mark(s, x);
+                s->blocked[x] = 2;
+                s->mem_head[x] = tk;
+                s->ss.deferred_memory++;
+                memory_blocked = true;
+                ++it;
+                continue;
+            }
+        }
+        mark(s, x);
+        s->in_tick[x] = 1;
+        if (op == DYC_OP_INSERT) s->tick_ins[x] += R.r.n;
+        s->phase[x] = (int8_t)op;
+        k.reqs.push_back(tk);
+        R.state = 1;
+        tick_keys += R.r.n;
+        ++nreq;
+        it = s->queue.erase(it);
+    }
+    if (k.reqs.empty() && !resized) {
+        *stuck = memory_blocked && !any_busy;
+        return DYC_OK;
+    }
+
+    // work items, sized so a tick has about one per resident wave
+    uint64_t per = s->scfg.item_keys;
+    if (per == 0) {
+        per = (tick_keys / kTargetItems + kMinItemKeys - 1) / kMinItemKeys * kMinItemKeys;
+        per = std::min<uint64_t>(std::max<uint64_t>(per, kMinItemKeys), kMaxItemKeys);
+    }
+    const uint64_t room_items = k.items_cap > nreq ? k.items_cap - nreq : 1;
+    per = std::max<uint64_t>(per, (tick_keys + room_items - 1) / room_items);
+    per = std::min<uint64_t>(per, 0xffffffffull);
+    for (auto &v : s->items) v.clear();
+    for (uint64_t tk : k.reqs) {
+        const dyc_request &r = s->reqs[tk].r;
+        for (uint64_t o = 0; o < r.n; o += per) {
+            ServeItem it{};
+            it.keys = r.keys + o;
+            it.vals = r.op == DYC_OP_INSERT ? r.vals + o : nullptr;
+            it.out_vals = (r.op == DYC_OP_FIND && r.out_vals) ? r.out_vals + o : nullptr;
+            it.flags = (r.op != DYC_OP_INSERT && r.out_flags) ? r.out_flags + o : nullptr;
+            it.len = (uint32_t)std::min<uint64_t>(per, r.n - o);
+            it.table = r.table;
+            s->items[r.op].push_back(it);
+        }
+    }
+    const uint64_t ni = s->items[0].size(), nf = s->items[1].size(), ne = s->items[2].size();
+    if (ni + nf + ne > k.items_cap) return sticky(s, fail(DYC_ERR_INVALID, "tick needs more work items than staged"));
+    memcpy(k.h_items, s->items[0].data(), ni * sizeof(ServeItem));
+    memcpy(k.h_items + ni, s->items[1].data(), nf * sizeof(ServeItem));
+    memcpy(k.h_items + ni + nf, s->items[2].data(), ne * sizeof(ServeItem));
+    for (uint32_t x : s->touched)
+        if (s->in_tick[x]) k.tables.push_back(TickTable{x, s->tick_ins[x], s->phase[x] == (int)DYC_OP_ERASE});
+    if (ni + nf + ne)
+        HIP_TRY(hipMemcpyAsync(k.d_items, k.h_items, (ni + nf + ne) * sizeof(ServeItem), hipMemcpyHostToDevice, s->st));
+    if (s->dir_dirty) {
+        for (uint32_t i = 0; i < s->n; ++i) k.h_dir[i] = s->t[i].D;
+        HIP_TRY(hipMemcpyAsync(s->d_dir, k.h_dir, s->n * sizeof(Desc), hipMemcpyHostToDevice, s->st));
+        s->dir_dirty = false;
+    }
+    HIP_TRY(launch_serve_insert(s->d_dir, k.d_items, (uint32_t)ni, s->policy(), (int)s->cfg.insert_lanes, 0, s->pc(),
+                                s->sink(0), s->ins_total(), s->st));
+    HIP_TRY(launch_serve_find(s->d_dir, k.d_items + ni, (uint32_t)nf, s->pool(), (int)s->cfg.find_lanes, 0, s->st));
+    HIP_TRY(launch_serve_erase(s->d_dir, k.d_items + ni + nf, (uint32_t)ne, s->pool(), s->del_total(),
+                               (int)s->cfg.find_lanes, 0, s->st));
+    HIP_TRY(hipMemcpyAsync(k.h_snap, s->d_stat, s->stat_bytes(), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipEventRecord(k.done, s->st));
+    for (const TickTable &tt : k.tables) {
+        SrvTable &T = s->t[tt.table];
+        T.inflight++;
+        T.inflight_ins += tt.ins;
+    }
+    k.busy = true;
+    k.seq = ++s->tick_seq;
+    s->ss.ticks++;
+    *launched = true;
+    return DYC_OK;
+}
+
+// One non-blocking step: retire what finished, then fill the free slots.
+int step(dyc_server *s)
+{
+    int rc = retire_ready(s, false);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < s->depth(); ++i) {
+        if (s->queue.empty() && s->down_list.empty()) break;
+        bool launched = false, stuck = false;
+        rc = build_tick(s, &launched, &stuck);
+        if (rc) return rc;
+        if (stuck) {
+            // nothing in flight can free arena space: fail each insert that waits for it (its
+            // table's later requests stay queued and run in order)
+            for (auto it = s->queue.begin(); it != s->queue.end();) {
+                const uint32_t x = s->reqs[*it].r.table;
+                if (s->blocked[x] == 2 && s->mem_head[x] == *it) {
+                    fail(DYC_ERR_OOM, "table %u cannot grow within the server's HBM budget", x);
+                    finish(s, *it, DYC_ERR_OOM);
+                    it = s->queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            continue;  // try again with the failed requests gone
+        }
+        if (!launched) break;
+    }
+    return DYC_OK;
+}
+
+bool idle(const dyc_server *s)
+{
+    for (uint32_t i = 0; i < s->depth(); ++i)
+        if (s->ticks[i].busy) return false;
+    return s->queue.empty();
+}
+
+int drain(dyc_server *s)
+{
+    while (true) {
+        int rc = step(s);
+        if (rc) return rc;
+        if (idle(s)) break;
+        rc = retire_ready(s, true);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return DYC_OK;
+}
+
+int check_request(const dyc_server *s, const dyc_request &r)
+{
+    if (r.table >= s->n) return fail(DYC_ERR_INVALID, "table %u out of range (%u tables)", r.table, s->n);
+    if (r.op > DYC_OP_ERASE) return fail(DYC_ERR_INVALID, "unknown op %u", r.op);
+    if (r.n && !r.keys) return fail(DYC_ERR_INVALID, "null keys");
+    if (r.op == DYC_OP_INSERT && r.n && !r.vals) return fail(DYC_ERR_INVALID, "insert needs values");
+    return DYC_OK;
+}
+
+// The reference's calling pattern: one request alone, plain single-table kernels, then a
+// stream synchronisation (api.cpp:185-300).
+int run_sync(dyc_server *s, uint64_t tk)
+{
+    Req &R = s->reqs[tk];
+    const dyc_request &r = R.r;
+    SrvTable &T = s->t[r.table];
+    unsigned long long *snap = s->ticks[0].h_snap;
+    int status = DYC_OK;
+    if (r.op == DYC_OP_INSERT) {
+        while ((double)(T.count + r.n) > s->cfg.upper_bound * (double)T.capacity()) {
+            bool ok = false;
+            int rc = upsize_now(s, r.table, &ok);
+            if (rc) return rc;
+            if (!ok) {
+                status = fail(DYC_ERR_OOM, "table %u cannot grow within the server's HBM budget", r.table);
+                break;
+            }
+        }
+        if (status == DYC_OK)
+            HIP_TRY(launch_insert(T.D, r.keys, r.vals, r.n, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
+                                  s->pc(), s->sink(r.table, s->ins_total() + r.table), s->st));
+    } else if (s->pool_settled > 0) {
+        // the pool holds entries: take the path that searches it
+        const uint64_t items = (r.n + kMaxItemKeys - 1) / kMaxItemKeys;
+        Tick &k = s->ticks[0];
+        if (items > k.items_cap) return fail(DYC_ERR_INVALID, "request too large for the staged items");
+        for (uint64_t i = 0; i < items; ++i) {
+            const uint64_t o = i * kMaxItemKeys;
+            ServeItem it{};
+            it.keys = r.keys + o;
+            it.out_vals = (r.op == DYC_OP_FIND && r.out_vals) ? r.out_vals + o : nullptr;
+            it.flags = r.out_flags ? r.out_flags + o : nullptr;
+            it.len = (uint32_t)std::min<uint64_t>(kMaxItemKeys, r.n - o);
+            it.table = r.table;
+            k.h_items[i] = it;
+        }
+        for (uint32_t i = 0; i < s->n; ++i) k.h_dir[i] = s->t[i].D;
+        HIP_TRY(hipMemcpyAsync(s->d_dir, k.h_dir, s->n * sizeof(Desc), hipMemcpyHostToDevice, s->st));
+        s->dir_dirty = false;
+        HIP_TRY(hipMemcpyAsync(k.d_items, k.h_items, items * sizeof(ServeItem), hipMemcpyHostToDevice, s->st));
+        if (r.op == DYC_OP_FIND)
+            HIP_TRY(launch_serve_find(s->d_dir, k.d_items, (uint32_t)items, s->pool(), (int)s->cfg.find_lanes, 0, s->st));
+        else
+            HIP_TRY(launch_serve_erase(s->d_dir, k.d_items, (uint32_t)items, s->pool(), s->del_total(),
+                                       (int)s->cfg.find_lanes, 0, s->st));
+    } else if (r.op == DYC_OP_FIND) {
+        HIP_TRY(launch_find(T.D, r.keys, r.out_vals, r.out_flags, r.n, (int)s->cfg.find_lanes, s->shape(), s->st));
+    } else {
+        HIP_TRY(launch_erase(T.D, r.keys, r.out_flags, r.n, (int)s->cfg.find_lanes, s->shape(), s->pc(), s->st));
+    }
+    int rc = read_stats_sync(s, snap);
+    if (rc) return rc;
+    const Counters &c = *reinterpret_cast<const Counters *>(snap);
+    if (c.deleted > s->deleted_seen) {
+        T.sync_del += c.deleted - s->deleted_seen;
+        s->deleted_seen = c.deleted;
+    }
+    refresh_counts(s, snap);
+    if (c.lost) status = sticky(s, fail(DYC_ERR_LOST, "overflow pool full"));
+    if (status == DYC_OK && std::min<uint64_t>(c.pending, s->scfg.pool_cap) > s->pool_settled) {
+        rc = resolve_pool(s, snap);
+        if (rc) status = sticky(s, rc);
+        else refresh_counts(s, snap);
+    }
+    if (status == DYC_OK && r.op == DYC_OP_ERASE && wants_down(s, T)) {
+        bool did = false;
+        rc = downsize_now(s, r.table, s->ticks[0].d_rctr, &did);
+        if (rc) return rc;
+        if (did) {
+            rc = read_stats_sync(s, snap);
+            if (rc) return rc;
+            if (std::min<uint64_t>(reinterpret_cast<const Counters *>(snap)->pending, s->scfg.pool_cap) >
+                s->pool_settled) {
+                rc = resolve_pool(s, snap);
+                if (rc) status = sticky(s, rc);
+            }
+        }
+    }
+    if (c.misplaced > s->misplaced_seen) {
+        s->misplaced_seen = c.misplaced;
+        status = sticky(s, fail(DYC_ERR_CORRUPT, "a resize found entries outside their bucket pair"));
+    }
+    finish(s, tk, status);
+    return DYC_OK;
+}
+
+void destroy(dyc_server *s)
+{
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    for (Tick &k : s->ticks) {
+        if (k.h_items) (void)hipHostFree(k.h_items);
+        if (k.d_items) (void)hipFree(k.d_items);
+        if (k.h_dir) (void)hipHostFree(k.h_dir);
+        if (k.h_snap) (void)hipHostFree(k.h_snap);
+        if (k.d_rctr) (void)hipFree(k.d_rctr);
+        if (k.done) (void)hipEventDestroy(k.done);
+    }
+    if (s->base) (void)hipFree(s->base);
+    if (s->d_dir) (void)hipFree(s->d_dir);
+    if (s->d_stat) (void)hipFree(s->d_stat);
+    if (s->d_pool) (void)hipFree(s->d_pool);
+    if (s->d_pool_tag) (void)hipFree(s->d_pool_tag);
+    if (s->d_resolve) (void)hipFree(s->d_resolve);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+}
+
+int create(dyc_server *s, uint32_t n_tables, const uint64_t *hints)
+{
+    s->n = n_tables;
+    s->t.resize(n_tables);
+    s->phase.assign(n_tables, -1);
+    s->blocked.assign(n_tables, 0);
+    s->in_tick.assign(n_tables, 0);
+    s->seen.assign(n_tables, 0);
+    s->tick_ins.assign(n_tables, 0);
+    s->mem_head.assign(n_tables, 0);
+    HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    uint64_t budget = s->scfg.hbm_budget;
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(hipMemGetInfo(&fr, &tot));
+        budget = fr / 2;
+    }
+    if (hipMalloc(&s->base, budget) != hipSuccess) {
+        s->base = nullptr;
+        (void)hipGetLastError();
+        return fail(DYC_ERR_OOM, "hipMalloc of the %llu-byte table arena failed", (unsigned long long)budget);
+    }
+    s->arena.reset(budget);
+    auto dev_alloc = [](void **p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            *p = nullptr;
+            (void)hipGetLastError();
+            return fail(DYC_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
+        }
+        return DYC_OK;
+    };
+    int rc = dev_alloc((void **)&s->d_dir, std::max<size_t>(1, n_tables) * sizeof(Desc));
+    if (!rc) rc = dev_alloc((void **)&s->d_stat, s->stat_bytes());
+    if (!rc) rc = dev_alloc((void **)&s->d_pool, s->scfg.pool_cap * sizeof(uint64_t));
+    if (!rc) rc = dev_alloc((void **)&s->d_pool_tag, s->scfg.pool_cap * sizeof(uint32_t));
+    if (!rc) rc = dev_alloc((void **)&s->d_resolve, s->scfg.pool_cap * sizeof(uint64_t));
+    if (rc) return rc;
+    HIP_TRY(launch_zero(s->d_stat, s->stat_bytes() / sizeof(unsigned long long), s->st));
+    const uint64_t items_cap = s->scfg.tick_max_keys / kMinItemKeys + s->scfg.tick_max_requests + 1;
+    for (uint32_t i = 0; i < 2; ++i) {
+        Tick &k = s->ticks[i];
+        k.items_cap = items_cap;
+        HIP_TRY(hipHostMalloc((void **)&k.h_items, items_cap * sizeof(ServeItem), hipHostMallocDefault));
+        rc = dev_alloc((void **)&k.d_items, items_cap * sizeof(ServeItem));
+        if (rc) return rc;
+        HIP_TRY(hipHostMalloc((void **)&k.h_dir, std::max<size_t>(1, n_tables) * sizeof(Desc), hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void **)&k.h_snap, s->stat_bytes(), hipHostMallocDefault));
+        rc = dev_alloc((void **)&k.d_rctr, sizeof(Counters));
+        if (rc) return rc;
+        HIP_TRY(hipEventCreateWithFlags(&k.done, hipEventDisableTiming));
+    }
+    for (uint32_t i = 0; i < n_tables; ++i) {
+        const uint32_t nb = initial_buckets(hints[i]);
+        if (nb == 0 || nb >= kMaxBuckets) return fail(DYC_ERR_INVALID, "table %u: capacity hint out of range", i);
+        SrvTable &T = s->t[i];
+        for (int sub = 0; sub < kSubtables; ++sub) {
+            const uint64_t off = s->arena.alloc((uint64_t)nb * kBucketBytes);
+            if (off == Arena::kNone)
+                return fail(DYC_ERR_OOM, "table %u does not fit in the %llu-byte HBM budget", i,
+                            (unsigned long long)budget);
+            T.off[sub] = off;
+            desc_set(T.D, sub, s->sub_ptr(off), nb);
+            HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(s->sub_ptr(off)), (uint64_t)nb * kSlots, s->st));
+        }
+    }
+    s->dir_dirty = true;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return DYC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dyc_server_config_init(dyc_server_config *scfg)
+{
+    if (!scfg) return fail(DYC_ERR_INVALID, "null config");
+    memset(scfg, 0, sizeof *scfg);
+    scfg->hbm_budget = 0;
+    scfg->tick_max_keys = 1ull << 23;
+    scfg->tick_max_requests = 4096;
+    scfg->pipeline_depth = 2;
+    scfg->item_keys = 0;
+    scfg->pool_cap = 65536;
+    return DYC_OK;
+}
+
+int dyc_server_create(const dyc_config *cfg, const dyc_server_config *scfg, uint32_t n_tables,
+                      const uint64_t *capacity_hints, dyc_server **out)
+{
+    if (!out || !capacity_hints || n_tables == 0) return fail(DYC_ERR_INVALID, "null argument or no tables");
+    *out = nullptr;
+    dyc_config c;
+    if (cfg) c = *cfg;
+    else dyc_config_init(&c);
+    dyc_server_config sc;
+    if (scfg) sc = *scfg;
+    else dyc_server_config_init(&sc);
+    if (sc.pipeline_depth < 1 || sc.pipeline_depth > 2) return fail(DYC_ERR_INVALID, "pipeline_depth must be 1 or 2");
+    if (sc.tick_max_keys == 0 || sc.tick_max_requests == 0 || sc.pool_cap == 0)
+        return fail(DYC_ERR_INVALID, "tick_max_keys, tick_max_requests and pool_cap must be positive");
+    if (sc.item_keys && (sc.item_keys % 64)) return fail(DYC_ERR_INVALID, "item_keys must be a multiple of 64");
+    const bool lanes_ok = (c.find_lanes == 4 || c.find_lanes == 8 || c.find_lanes == 16) &&
+                          (c.insert_lanes == 1 || c.insert_lanes == 2 || c.insert_lanes == 4 ||
+                           c.insert_lanes == 8 || c.insert_lanes == 16);
+    if (!lanes_ok) return fail(DYC_ERR_INVALID, "lanes per key: find 4/8/16, insert 1/2/4/8/16");
+    if (c.block_threads == 0 || c.block_threads % 64 || c.block_threads > 256)
+        return fail(DYC_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 256]");
+    if (!(c.upper_bound > 0.0 && c.upper_bound <= 1.0) || !(c.lower_bound >= 0.0) || c.lower_bound >= c.upper_bound)
+        return fail(DYC_ERR_INVALID, "need 0 <= lower_bound < upper_bound <= 1");
+    dyc_server *s = new dyc_server();
+    s->cfg = c;
+    s->scfg = sc;
+    HIP_TRY(hipGetDevice(&s->device));
+    int rc = create(s, n_tables, capacity_hints);
+    if (rc) {
+        destroy(s);
+        return rc;
+    }
+    *out = s;
+    return DYC_OK;
+}
+
+int dyc_server_destroy(dyc_server *s)
+{
+    if (!s) return DYC_OK;
+    DeviceGuard g(s->device);
+    destroy(s);
+    return DYC_OK;
+}
+
+int dyc_server_submit(dyc_server *s, const dyc_request *req, uint64_t *ticket)
+{
+    if (!s || !req) return fail(DYC_ERR_INVALID, "null argument");
+    int rc = check_request(s, *req);
+    if (rc) return rc;
+    Req R;
+    R.r = *req;
+    s->reqs.push_back(R);
+    const uint64_t tk = s->reqs.size() - 1;
+    s->queue.push_back(tk);
+    s->ss.submitted++;
+    if (ticket) *ticket = tk;
+    return DYC_OK;
+}
+
+int dyc_server_poll(dyc_server *s, uint64_t *done_tickets, int *done_status, uint32_t cap, uint32_t *n_done)
+{
+    if (!s) return fail(DYC_ERR_INVALID, "null server");
+    DeviceGuard g(s->device);
+    int rc = step(s);
+    uint32_t k = 0;
+    while (k < cap && !s->done.empty()) {
+        if (done_tickets) done_tickets[k] = s->done.front().first;
+        if (done_status) done_status[k] = s->done.front().second;
+        s->done.pop_front();
+        ++k;
+    }
+    if (n_done) *n_done = k;
+    return rc;
+}
+
+int dyc_server_drain(dyc_server *s)
+{
+    if (!s) return fail(DYC_ERR_INVALID, "null server");
+    DeviceGuard g(s->device);
+    int rc = drain(s);
+    if (rc) return rc;
+    return s->error;
+}
+
+int dyc_server_replay(dyc_server *s, const dyc_request *reqs, uint64_t n, const double *arrival_s, uint32_t mode,
+                      double *done_s, int *status)
+{
+    if (!s || (n && (!reqs || !arrival_s || !done_s))) return fail(DYC_ERR_INVALID, "null argument");
+    if (mode > 1) return fail(DYC_ERR_INVALID, "mode must be 0 (continuous) or 1 (synchronous)");
+    for (uint64_t i = 0; i < n; ++i) {
+        int rc = check_request(s, reqs[i]);
+        if (rc) return rc;
+        if (i && arrival_s[i] < arrival_s[i - 1]) return fail(DYC_ERR_INVALID, "arrival times must not decrease");
+    }
+    DeviceGuard g(s->device);
+    int rc = drain(s);  // start from a quiet server
+    if (rc) return rc;
+    s->done.clear();
+    const uint64_t first = s->reqs.size();
+    const Clock::time_point t0 = Clock::now();
+    if (mode == 1) {
+        for (uint64_t i = 0; i < n; ++i) {
+            while (seconds_since(t0) < arrival_s[i]) {
+            }
+            Req R;
+            R.r = reqs[i];
+            s->reqs.push_back(R);
+            s->ss.submitted++;
+            rc = run_sync(s, first + i);
+            if (rc) return sticky(s, rc);
+            done_s[i] = seconds_since(t0);
+            if (status) status[i] = s->reqs[first + i].status;
+        }
+        s->done.clear();
+        return DYC_OK;
+    }
+    uint64_t next = 0, finished = 0;
+    while (finished < n) {
+        double now = seconds_since(t0);
+        while (next < n && arrival_s[next] <= now) {
+            Req R;
+            R.r = reqs[next];
+            s->reqs.push_back(R);
+            s->queue.push_back(first + next);
+            s->ss.submitted++;
+            ++next;
+        }
+        rc = step(s);
+        if (rc) return sticky(s, rc);
+        if (!s->done.empty()) {
+            now = seconds_since(t0);
+            while (!s->done.empty()) {
+                const uint64_t tk = s->done.front().first;
+                if (tk >= first && tk < first + n) {
+                    done_s[tk - first] = now;
+                    if (status) status[tk - first] = s->done.front().second;
+                    ++finished;
+                }
+                s->done.pop_front();
+            }
+        }
+    }
+    return DYC_OK;
+}
+
+int dyc_server_get_stats(const dyc_server *s, dyc_server_stats *out)
+{
+    if (!s || !out) return fail(DYC_ERR_INVALID, "null argument");
+    *out = s->ss;
+    out->pool_entries = s->pool_settled;
+    out->arena_bytes = s->arena.size();
+    out->arena_used = s->arena.used();
+    out->arena_peak = s->arena.peak();
+    out->queued = s->queue.size();
+    uint64_t inflight = 0;
+    for (uint32_t i = 0; i < s->depth(); ++i)
+        if (s->ticks[i].busy) inflight += s->ticks[i].reqs.size();
+    out->in_flight = inflight;
+    return DYC_OK;
+}
+
+int dyc_server_table_stats(const dyc_server *s, uint32_t table, dyc_stats *out)
+{
+    if (!s || !out) return fail(DYC_ERR_INVALID, "null argument");
+    if (table >= s->n) return fail(DYC_ERR_INVALID, "table %u out of range", table);
+    const SrvTable &T = s->t[table];
+    memset(out, 0, sizeof *out);
+    out->count = T.count;
+    out->capacity = T.capacity();
+    for (int i = 0; i < kSubtables; ++i) out->buckets[i] = T.D.nb[i];
+    out->resizes_up = T.resizes_up;
+    out->resizes_down = T.resizes_down;
+    return DYC_OK;
+}
+
+int dyc_server_dump(dyc_server *s, uint32_t table, uint32_t sub, uint64_t *host_slots, uint64_t n_slots)
+{
+    if (!s || !host_slots) return fail(DYC_ERR_INVALID, "null argument");
+    if (table >= s->n || sub >= (uint32_t)kSubtables) return fail(DYC_ERR_INVALID, "table or subtable out of range");
+    DeviceGuard g(s->device);
+    int rc = drain(s);
+    if (rc) return rc;
+    const SrvTable &T = s->t[table];
+    const uint64_t slots = (uint64_t)T.D.nb[sub] * kSlots;
+    if (n_slots < slots) return fail(DYC_ERR_INVALID, "dump needs %llu slots", (unsigned long long)slots);
+    HIP_TRY(hipMemcpy(host_slots, T.D.sub[sub], slots * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return DYC_OK;
+}
+
+}  // extern "C"

@@ -13,6 +13,8 @@
 #include <algorithm>
 
 using namespace dyc;
+using dyc::host::choose_down;
+using dyc::host::choose_up;
 using dyc::host::DeviceGuard;
 using dyc::host::fail;
 using dyc::host::initial_buckets;
@@ -57,26 +59,6 @@ uint32_t initial_buckets(uint64_t hint)
 
 }  // namespace host
 }  // namespace dyc
-
-namespace {
-
-// api.cpp:116-122
-uint32_t choose_up(const uint32_t nb[4])
-{
-    for (uint32_t i = 0; i + 1 < kSubtables; ++i)
-        if (nb[i] == 2u * nb[i + 1]) return i + 1;
-    return 0;
-}
-
-// api.cpp:147-153
-uint32_t choose_down(const uint32_t nb[4])
-{
-    for (uint32_t i = 0; i + 1 < kSubtables; ++i)
-        if (nb[i] > nb[i + 1]) return i;
-    return kSubtables - 1;
-}
-
-}  // namespace
 
 struct dyc_table {
     dyc_config cfg;

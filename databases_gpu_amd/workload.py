@@ -221,3 +221,155 @@ def dynamic_stream(initial: int, batch_ops: int, grow_batches: int, shrink_batch
         live = live[perm[n_del:]]
         batches.append(DynamicBatch(ins, splitmix32(ins), finds, dels, live.numel()))
     return init_keys, init_vals, batches
+
+
+def keys_at(idx: torch.Tensor, seed: int) -> torch.Tensor:
+    """key(i) for an int64 tensor of indexes (each < 0xFFFFFFF0): keys_range without the range."""
+    k = _mix32(idx, seed)
+    bad = (k == 0) | (k == M32)
+    if bool(bad.any()):
+        r0, r1 = _spare_keys(seed)
+        k = torch.where(k == 0, torch.full_like(k, r0), k)
+        k = torch.where(k == M32, torch.full_like(k, r1), k)
+    return _to_i32(k)
+
+
+class ServingTrace:
+    """A request trace for the serving loop, with the exact result of every request.
+
+    Tables: table 0 is the resident table, tables 1.. the tenants. Table t owns key indexes
+    [base[t], base[t] + span[t]); its live keys are always the index window [lo, hi): inserts
+    append fresh indexes at hi, erases remove the oldest live indexes from lo, finds draw
+    round(find_hit * n) indexes uniformly from the window (with replacement) and the rest from
+    an absent range no table owns. Request i targets a table chosen with probability
+    proportional to the table's initial size, is an insert / find / erase with the `mix`
+    probabilities (one reference batch call), and has a log-uniform size in [batch_lo, batch_hi].
+    Arrivals are Poisson at `offered` keys per second (all at 0 when offered is None).
+
+    Device tensors (int32 bits of uint32 words, uint8 flags), one payload position per key:
+      keys, vals (splitmix32(key), what inserts store), exp_vals / exp_flags (a find's value
+      and hit; an erase's deleted flag, always 1 here; 0 elsewhere).
+    req: structured numpy array (table, op, n, off) in arrival order; arrivals: seconds.
+    init: per table the initial key indexes are [base, base + size) -- see initial_keys().
+    """
+
+    def __init__(self, n_requests: int, seed: int, device, offered: float | None = None, resident: int = 1 << 26,
+                 tenants: int = 256, tenant_lo: int = 1 << 12, tenant_hi: int = 1 << 20, batch_lo: int = 1 << 6,
+                 batch_hi: int = 1 << 18, mix=(0.3, 0.6, 0.1), find_hit: float = 0.5, theta: float = 0.85,
+                 resident_span: int = 1 << 29, tenant_span: int = 1 << 22):
+        rng = np.random.default_rng(seed)
+        self.seed = seed
+        self.device = device
+        sizes = [resident] + list(np.exp(rng.uniform(np.log(tenant_lo), np.log(tenant_hi), tenants)).astype(np.int64))
+        self.sizes = np.asarray(sizes, dtype=np.int64)
+        nt = self.sizes.size
+        self.span = np.asarray([resident_span] + [tenant_span] * tenants, dtype=np.int64)
+        self.base = np.concatenate([[0], np.cumsum(self.span)[:-1]]).astype(np.int64)
+        absent_base = int(self.base[-1] + self.span[-1])
+        if absent_base >= SPARE_INDEX:
+            raise ValueError("table index spans exceed the 32-bit key space")
+        self.hints = (np.ceil(self.sizes / theta) + 64).astype(np.uint64)
+        p = self.sizes / self.sizes.sum()
+        tables = rng.choice(nt, size=n_requests, p=p)
+        ops = rng.choice(3, size=n_requests, p=np.asarray(mix, dtype=np.float64) / sum(mix))
+        ns = np.exp(rng.uniform(np.log(batch_lo), np.log(batch_hi + 1), n_requests)).astype(np.int64)
+        ns = np.clip(ns, batch_lo, batch_hi)
+        lo = np.zeros(nt, dtype=np.int64)
+        hi = self.sizes.copy()
+        absent_next = 0
+        # segments of the payload: (length, kind, a, b); kind 0: index a + j, kind 1: a + floor(u * b)
+        seg_len, seg_kind, seg_a, seg_b, seg_hit = [], [], [], [], []
+        req = np.zeros(n_requests, dtype=[("table", "<u4"), ("op", "<u4"), ("n", "<u8"), ("off", "<u8")])
+        off = 0
+        for i in range(n_requests):
+            t, op, n = int(tables[i]), int(ops[i]), int(ns[i])
+            live = int(hi[t] - lo[t])
+            if op == 2 and live < n:  # erase at most what is live
+                n = live
+            if op == 0:
+                if hi[t] + n > self.span[t]:
+                    raise ValueError(f"table {t} outgrew its index span")
+                seg_len.append(n), seg_kind.append(0), seg_a.append(self.base[t] + hi[t]), seg_b.append(0)
+                seg_hit.append(0)
+                hi[t] += n
+            elif op == 2:
+                seg_len.append(n), seg_kind.append(0), seg_a.append(self.base[t] + lo[t]), seg_b.append(0)
+                seg_hit.append(1)
+                lo[t] += n
+            else:
+                nh = int(round(find_hit * n)) if live > 0 else 0
+                if nh:
+                    seg_len.append(nh), seg_kind.append(1), seg_a.append(self.base[t] + lo[t]), seg_b.append(live)
+                    seg_hit.append(1)
+                if n - nh:
+                    seg_len.append(n - nh), seg_kind.append(0), seg_a.append(absent_base + absent_next)
+                    seg_b.append(0), seg_hit.append(0)
+                    absent_next += n - nh
+            req[i] = (t, op, n, off)
+            off += n
+        if absent_base + absent_next >= SPARE_INDEX:
+            raise ValueError("absent key range exhausted")
+        self.req = req
+        self.total = off
+        self.final_window = (lo, hi)
+        if offered:
+            gaps = rng.exponential(float(ns.mean()) / offered, n_requests)
+            self.arrivals = np.cumsum(gaps) - gaps[0]
+        else:
+            self.arrivals = np.zeros(n_requests)
+        self.offered = offered
+
+        # payload on the device
+        L = torch.as_tensor(np.asarray(seg_len, dtype=np.int64), device=device)
+        total = int(L.sum()) if len(seg_len) else 0
+        assert total == off
+        if total:
+            starts = torch.cumsum(L, 0) - L
+            sid = torch.repeat_interleave(torch.arange(L.numel(), device=device), L)
+            j = torch.arange(total, dtype=torch.int64, device=device) - starts[sid]
+            kind = torch.as_tensor(np.asarray(seg_kind, dtype=np.int64), device=device)[sid]
+            a = torch.as_tensor(np.asarray(seg_a, dtype=np.int64), device=device)[sid]
+            b = torch.as_tensor(np.asarray(seg_b, dtype=np.int64), device=device)[sid]
+            g = torch.Generator(device=device)
+            g.manual_seed(seed + 1)
+            u = torch.rand(total, generator=g, dtype=torch.float64, device=device)
+            rnd = a + torch.minimum(torch.floor(u * b.to(torch.float64)).to(torch.int64), (b - 1).clamp(min=0))
+            idx = torch.where(kind == 0, a + j, rnd)
+            del u, rnd, j, a, b, kind
+            self.keys = keys_at(idx, seed)
+            del idx
+            hit = torch.as_tensor(np.asarray(seg_hit, dtype=np.uint8), device=device)[sid]
+            del sid
+        else:
+            self.keys = torch.zeros(0, dtype=torch.int32, device=device)
+            hit = torch.zeros(0, dtype=torch.uint8, device=device)
+        self.vals = splitmix32(self.keys)
+        # expected outputs: finds -> (value, hit); erases -> deleted = 1; inserts -> 0
+        op_of = torch.as_tensor(np.repeat(req["op"].astype(np.int64), req["n"].astype(np.int64)), device=device)
+        self.exp_flags = torch.where(op_of == 0, torch.zeros_like(hit), hit)
+        self.exp_vals = torch.where((op_of == 1) & (hit == 1), self.vals, torch.zeros_like(self.vals))
+
+    @property
+    def ops(self) -> int:
+        return int(self.total)
+
+    def initial_keys(self, t: int) -> torch.Tensor:
+        i = torch.arange(int(self.base[t]), int(self.base[t] + self.sizes[t]), dtype=torch.int64, device=self.device)
+        return keys_at(i, self.seed)
+
+    def requests(self, out_vals: torch.Tensor, out_flags: torch.Tensor) -> np.ndarray:
+        """The trace as dyc_request records pointing into this trace's payload and the given
+        output tensors (int32 / uint8, one position per payload key)."""
+        from ._lib import REQUEST_DTYPE
+
+        r = np.zeros(self.req.size, dtype=REQUEST_DTYPE)
+        off = self.req["off"].astype(np.uint64)
+        op = self.req["op"]
+        r["table"] = self.req["table"]
+        r["op"] = op
+        r["n"] = self.req["n"]
+        r["keys"] = np.uint64(self.keys.data_ptr()) + off * np.uint64(4)
+        r["vals"] = np.where(op == 0, np.uint64(self.vals.data_ptr()) + off * np.uint64(4), np.uint64(0))
+        r["out_vals"] = np.where(op == 1, np.uint64(out_vals.data_ptr()) + off * np.uint64(4), np.uint64(0))
+        r["out_flags"] = np.where(op != 0, np.uint64(out_flags.data_ptr()) + off, np.uint64(0))
+        return r

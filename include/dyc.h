@@ -215,6 +215,97 @@ int dyc_group_pending(dyc_group *g, uint64_t *host_entries, uint32_t *host_table
 /* Empty every table of the group. Asynchronous. */
 int dyc_group_clear(dyc_group *g, dyc_stream stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Serving loop: many resizable tables behind one request queue (no reference counterpart; the
+ * reference serves one synchronous batch per call, api.cpp:185-300). Requests are the
+ * reference's batch calls -- hash_insert / hash_search / hash_delete of one table -- submitted
+ * asynchronously. The server admits queued requests into "ticks" (continuous batching): each
+ * tick is one fused insert launch, one fused find launch and one fused erase launch over every
+ * admitted request, and the next tick is built and enqueued while the previous one runs.
+ *
+ * Admission, per tick, in arrival order:
+ *   - slots: at most tick_max_requests requests and tick_max_keys keys (a larger request is
+ *     admitted alone);
+ *   - order: a table's requests run in arrival order. Within a tick a table's requests must
+ *     follow insert -> find -> erase with at most one erase; a request that would break this,
+ *     and every later request of its table, waits for a later tick;
+ *   - memory: all table storage lives in one arena of hbm_budget bytes. An insert that needs
+ *     its table grown (count + n > upper_bound * capacity) is admitted only once the upsizes fit
+ *     in the arena's free space; otherwise it (and its table's later requests) waits.
+ * Resizes run in-line: upsizes are enqueued ahead of the tick that needs them; a table whose
+ * erases bring it below lower_bound * capacity (and above min_capacity) is downsized once it
+ * has no request in flight. Entries that exhaust the eviction bound go to a server-wide pool
+ * that finds and erases also search, and are re-placed (upsize + re-insert) when their tick
+ * completes, so every request's results equal those of running the requests one at a time in
+ * arrival order.
+ *
+ * keys / vals / out_vals / out_flags are device pointers that must stay valid (and inputs
+ * unchanged) until the request completes. One server is driven from one host thread; its work
+ * runs on a stream the server owns. */
+typedef struct dyc_server dyc_server;  /* opaque */
+
+#define DYC_OP_INSERT 0u  /* hash_insert: vals required; out_* unused */
+#define DYC_OP_FIND 1u    /* hash_search: out_vals (0 on a miss) and out_flags (hit) optional */
+#define DYC_OP_ERASE 2u   /* hash_delete: out_flags (deleted) optional */
+
+typedef struct dyc_request {
+    uint32_t table;
+    uint32_t op;
+    const uint32_t *keys;
+    const uint32_t *vals;
+    uint32_t *out_vals;
+    uint8_t *out_flags;
+    uint64_t n;
+} dyc_request;
+
+typedef struct dyc_server_config {
+    uint64_t hbm_budget;         /* arena bytes for all tables (0 = half of the free device memory) */
+    uint64_t tick_max_keys;      /* keys per tick over all requests (default 2^23) */
+    uint32_t tick_max_requests;  /* requests per tick (default 4096) */
+    uint32_t pipeline_depth;     /* ticks in flight at once, 1 or 2 (default 2) */
+    uint32_t item_keys;          /* keys per work item; 0 = chosen per tick from its size */
+    uint32_t pool_cap;           /* entries the overflow pool holds (default 65536) */
+} dyc_server_config;
+
+typedef struct dyc_server_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t ticks;
+    uint64_t keys;               /* keys of completed requests */
+    uint64_t deferred_order;     /* admission passes that held a request back for table order */
+    uint64_t deferred_memory;    /* admission passes that held a request back for arena space */
+    uint64_t deferred_slots;     /* admission passes that ended with requests left for lack of slots */
+    uint64_t resizes_up, resizes_down;
+    uint64_t pool_entries;       /* entries in the overflow pool now */
+    uint64_t pool_rounds;        /* re-placement rounds run */
+    uint64_t arena_bytes, arena_used, arena_peak;
+    uint64_t queued, in_flight;  /* requests waiting for admission / admitted, not completed */
+} dyc_server_stats;
+
+int dyc_server_config_init(dyc_server_config *scfg);
+/* n_tables tables, table i sized for capacity_hints[i] slots (reference sizing). cfg gives
+ * the eviction rule, fill bounds, min_capacity and lanes; auto_resize is implied. */
+int dyc_server_create(const dyc_config *cfg, const dyc_server_config *scfg, uint32_t n_tables,
+                      const uint64_t *capacity_hints, dyc_server **out);
+int dyc_server_destroy(dyc_server *s);
+/* Queue one request; *ticket numbers requests 0, 1, 2, ... in submission order. */
+int dyc_server_submit(dyc_server *s, const dyc_request *req, uint64_t *ticket);
+/* Make progress without blocking: retire finished ticks, admit and enqueue new ones. Writes up
+ * to cap finished tickets (and their status, DYC_OK or an error code) and sets *n_done. */
+int dyc_server_poll(dyc_server *s, uint64_t *done_tickets, int *done_status, uint32_t cap, uint32_t *n_done);
+/* Block until every submitted request has completed. */
+int dyc_server_drain(dyc_server *s);
+/* Replay a request trace in real time: request i is submitted at arrival_s[i] seconds after the
+ * call starts (non-decreasing). mode 0: continuous batching (submit + poll). mode 1: synchronous
+ * per-request launches -- each request alone, in arrival order, with the plain single-table
+ * kernels and a stream synchronisation, the reference's calling pattern. done_s[i] receives
+ * the completion time (seconds since the call started) and status[i] the request status. */
+int dyc_server_replay(dyc_server *s, const dyc_request *reqs, uint64_t n, const double *arrival_s, uint32_t mode,
+                      double *done_s, int *status);
+int dyc_server_get_stats(const dyc_server *s, dyc_server_stats *out);
+int dyc_server_table_stats(const dyc_server *s, uint32_t table, dyc_stats *out);
+/* Copy subtable `sub` of `table` into host memory (buckets[sub] * 16 slots); drains first. */
+int dyc_server_dump(dyc_server *s, uint32_t table, uint32_t sub, uint64_t *host_slots, uint64_t n_slots);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@ import re
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -84,10 +85,48 @@ def test_struct_layout_matches_c(lib, tmp_path):
                  L.Config.region_mib.offset]
     assert s == [ctypes.sizeof(L.Stats), L.Stats.buckets.offset, L.Stats.pending.offset, L.Stats.rejected.offset,
                  L.Stats.lost.offset]
+    r = [int(x) for x in out[2].split()[1:]]
+    assert r == [ctypes.sizeof(L.Request), L.Request.keys.offset, L.Request.vals.offset, L.Request.out_flags.offset,
+                 L.Request.n.offset]
+    assert np.dtype(L.REQUEST_DTYPE).itemsize == r[0]
+    sc = [int(x) for x in out[3].split()[1:]]
+    assert sc == [ctypes.sizeof(L.ServerConfig), L.ServerConfig.tick_max_requests.offset,
+                  L.ServerConfig.item_keys.offset, L.ServerConfig.pool_cap.offset]
+    ss = [int(x) for x in out[4].split()[1:]]
+    assert ss == [ctypes.sizeof(L.ServerStats), L.ServerStats.deferred_memory.offset, L.ServerStats.in_flight.offset]
 
 
 def _hipcc():
     return shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+
+def test_server_arena_allocator(tmp_path):
+    """The serving loop's HBM arena: random alloc/release against a byte map (no overlap,
+    alignment, exact accounting, full coalescing back to one block)."""
+    exe = tmp_path / "arena"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC, os.path.join(ROOT, "tests", "cpp", "arena_check.cpp"),
+                    "-o", str(exe)], check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "arena ok" in r.stdout
+
+
+def test_server_argument_errors(lib):
+    """Server entry points reject bad arguments with a code and a message before any GPU call."""
+    import databases_gpu_amd._lib as L
+
+    h = ctypes.c_void_p()
+    hints = (ctypes.c_uint64 * 1)(1000)
+    assert lib.dyc_server_create(None, None, 0, hints, ctypes.byref(h)) == 1
+    sc = L.ServerConfig()
+    assert lib.dyc_server_config_init(ctypes.byref(sc)) == 0
+    assert sc.pipeline_depth == 2 and sc.tick_max_keys == 1 << 23 and sc.pool_cap == 65536
+    sc.pipeline_depth = 3
+    assert lib.dyc_server_create(None, ctypes.byref(sc), 1, hints, ctypes.byref(h)) == 1
+    assert b"pipeline_depth" in lib.dyc_last_error()
+    assert lib.dyc_server_submit(None, None, None) == 1
+    assert lib.dyc_server_drain(None) == 1
+    assert lib.dyc_server_get_stats(None, None) == 1
 
 
 def test_fast_division_is_exact(tmp_path):
