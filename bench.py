@@ -61,10 +61,16 @@ def parse():
     ap.add_argument("--cpu-seq-log2n", type=int, default=22, help="sequential CPU sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
-    ap.add_argument("--workload", choices=("bulk", "dynamic", "batched"), default="bulk",
+    ap.add_argument("--workload", choices=("bulk", "dynamic", "batched", "serving"), default="bulk",
                     help="bulk: the large bulk configuration (default, the headline line); dynamic: the "
                          "resize stream (growth then shrink batches under fill bounds); batched: many uneven "
-                         "tables in one persistent launch vs one launch per table")
+                         "tables in one persistent launch vs one launch per table; serving: the request loop "
+                         "(continuous batching vs synchronous per-request launches) over a load sweep")
+    ap.add_argument("--sv-requests", type=int, default=4096, help="serving: requests per run")
+    ap.add_argument("--sv-loads", default="0.25,0.5,1,2,4,8,16,0",
+                    help="serving: offered loads in G keys/s (0 = every request arrives at once)")
+    ap.add_argument("--sv-depth", type=int, default=2, help="serving: ticks in flight")
+    ap.add_argument("--sv-tick-keys", type=int, default=1 << 23, help="serving: keys per tick")
     ap.add_argument("--bt-tables", type=int, default=8192, help="batched: number of tables")
     ap.add_argument("--bt-log2min", type=int, default=8, help="batched: log2 of the smallest table")
     ap.add_argument("--bt-log2max", type=int, default=20, help="batched: log2 of the largest table")
@@ -84,7 +90,8 @@ def parse():
 
 
 def pct(xs, q):
-    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
+    xs = np.asarray(xs, dtype=np.float64)
+    return float(np.percentile(xs, q)) if xs.size else None
 
 
 def cpu_baseline(log2n: int, theta: float, hit: float, zipf_s: float, seed: int, seq_log2n: int) -> dict:
@@ -234,6 +241,118 @@ def run_dynamic(args, dev) -> int:
     return 0 if hits_ok else 1
 
 
+def run_serving(args, dev) -> int:
+    """The serving loop: one resident table of 2^26 keys plus 256 tenant tables of 2^12-2^20 keys
+    (log-uniform), requests of log-uniform size in [2^6, 2^18] keys arriving Poisson at a fixed
+    offered load, 30% insert / 60% find (50% hit) / 10% delete batches, table chosen in
+    proportion to its size, fill bounds [0.50, 0.90] with in-line resizes. For every offered load
+    the same trace is replayed twice on fresh tables -- continuous batching (mode 0) and
+    synchronous per-request launches (mode 1) -- and every output of both is compared with the
+    trace's exact expected results. Prints one JSON line."""
+    import torch
+
+    import databases_gpu_amd as D
+    from databases_gpu_amd import workload as W
+
+    loads = [float(x) for x in args.sv_loads.split(",") if x.strip()]
+    log(f"serving: generating a {args.sv_requests}-request trace")
+    tr = W.ServingTrace(args.sv_requests, seed=args.seed, device=dev, offered=None)
+    ops = tr.ops
+    mean_n = float(tr.req["n"].mean())
+    log(f"serving: {ops} keys ({ops / args.sv_requests:.0f} per request), {tr.sizes.size} tables, "
+        f"{int(tr.sizes.sum())} initial keys; ops mix {np.bincount(tr.req['op'], minlength=3).tolist()}")
+    init = [tr.initial_keys(t) for t in range(tr.sizes.size)]
+    init_v = [W.splitmix32(k) for k in init]
+    ov = torch.zeros(ops, dtype=torch.int32, device=dev)
+    of = torch.zeros(ops, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    budget = 16 << 30
+
+    def fresh():
+        srv = D.DyCuckooServer(tr.hints, hbm_budget=budget, pipeline_depth=args.sv_depth,
+                               tick_max_keys=args.sv_tick_keys, insert_lanes=args.insert_lanes,
+                               find_lanes=args.find_lanes)
+        for t in range(tr.sizes.size):
+            srv.insert(t, init[t], init_v[t])
+        srv.drain()
+        return srv
+
+    # warm-up: first launches of every kernel, both modes, on a throwaway server
+    warm = W.ServingTrace(64, seed=args.seed + 99, device=dev, offered=None, resident=1 << 12, tenants=4,
+                          tenant_lo=256, tenant_hi=1024, batch_lo=16, batch_hi=1024)
+    wv = torch.zeros(warm.ops, dtype=torch.int32, device=dev)
+    wf = torch.zeros(warm.ops, dtype=torch.uint8, device=dev)
+    for mode in (0, 1):
+        ws = D.DyCuckooServer(warm.hints, hbm_budget=64 << 20)
+        ws.replay(warm.requests(wv, wf), warm.arrivals, mode=mode)
+        ws.close()
+
+    runs = []
+    all_ok = True
+    for load in loads:
+        rng = np.random.default_rng(args.seed + 1000 + int(load * 1000))
+        if load > 0:
+            gaps = rng.exponential(mean_n / (load * 1e9), args.sv_requests)
+            arrivals = np.cumsum(gaps) - gaps[0]
+        else:
+            arrivals = np.zeros(args.sv_requests)
+        for mode in (0, 1):
+            srv = fresh()
+            ov.zero_()
+            of.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            done, status = srv.replay(tr.requests(ov, of), arrivals, mode=mode)
+            wall = time.perf_counter() - t0
+            torch.cuda.synchronize()
+            ok = bool((status == 0).all()) and bool(torch.equal(of, tr.exp_flags)) and bool(torch.equal(ov, tr.exp_vals))
+            all_ok &= ok
+            lat_ms = (done - arrivals) * 1e3
+            span = float(done.max() - arrivals[0])
+            st = srv.stats()
+            run = {"mode": "continuous" if mode == 0 else "synchronous",
+                   "offered_gkeys_s": load if load > 0 else None,
+                   "mops": round(ops / span / 1e6, 1),
+                   "p50_ms": round(pct(lat_ms, 50), 3), "p99_ms": round(pct(lat_ms, 99), 3),
+                   "max_ms": round(float(lat_ms.max()), 3), "span_s": round(span, 4), "wall_s": round(wall, 4),
+                   "ticks": st["ticks"] if mode == 0 else None, "resizes_up": st["resizes_up"],
+                   "resizes_down": st["resizes_down"], "deferred_memory": st["deferred_memory"],
+                   "arena_peak_gb": round(st["arena_peak"] / 2**30, 3), "parity": "ok" if ok else "mismatch"}
+            runs.append(run)
+            log(f"load {load} G keys/s {run['mode']}: {run['mops']} Mops/s, p50 {run['p50_ms']} ms, "
+                f"p99 {run['p99_ms']} ms, ticks {run['ticks']}, ups {run['resizes_up']}, parity {run['parity']}")
+            srv.close()
+    sat = {r["mode"]: r for r in runs if r["offered_gkeys_s"] is None}
+    head = sat.get("continuous") or runs[0]
+    result = {
+        "metric": "serving: Mops/s and p50/p99 batch latency, continuous batching vs synchronous per-batch launches",
+        "value": head["mops"],
+        "unit": "Mops/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "serving: resident table 2^26 keys + 256 tenants of 2^12-2^20 keys (log-uniform), "
+                               "Poisson arrivals, batches log-uniform in [2^6, 2^18] keys, 30% insert / 60% find "
+                               "(50% hit) / 10% delete, table chosen in proportion to size, fill bounds "
+                               "[0.50, 0.90], in-line resizes",
+                   "requests": args.sv_requests, "keys": ops, "tables": int(tr.sizes.size),
+                   "initial_keys": int(tr.sizes.sum()), "pipeline_depth": args.sv_depth,
+                   "tick_max_keys": args.sv_tick_keys, "hbm_budget_gb": budget / 2**30, "seed": args.seed},
+        "runs": runs,
+        "saturation_speedup": (round(sat["continuous"]["mops"] / sat["synchronous"]["mops"], 2)
+                               if "continuous" in sat and "synchronous" in sat else None),
+        "parity": "ok" if all_ok else "mismatch",
+    }
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0 if all_ok else 1
+
+
 def run_batched(args, dev) -> int:
     """Batched uneven tables: bt_tables tables with key counts log-uniform in [2^bt_log2min,
     2^bt_log2max], each sized for load factor bt_theta; one step = clear, insert every table's
@@ -358,6 +477,8 @@ def main() -> int:
         return run_dynamic(args, dev)
     if args.workload == "batched":
         return run_batched(args, dev)
+    if args.workload == "serving":
+        return run_serving(args, dev)
     n = 1 << args.log2n
     hint = int(math.ceil(n / args.theta))
     log(f"generating inputs: n=2^{args.log2n}, theta={args.theta}")

@@ -126,12 +126,14 @@ class ServerConfig(ctypes.Structure):
         ("pipeline_depth", ctypes.c_uint32),
         ("item_keys", ctypes.c_uint32),
         ("pool_cap", ctypes.c_uint32),
+        ("one_launch", ctypes.c_uint32),
+        ("spin_cap", ctypes.c_uint32),
     ]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
-        "submitted", "completed", "failed", "ticks", "keys", "deferred_order", "deferred_memory", "deferred_slots",
+        "submitted", "completed", "failed", "ticks", "rounds", "keys", "deferred_order", "deferred_memory", "deferred_slots",
         "resizes_up", "resizes_down", "pool_entries", "pool_rounds", "arena_bytes", "arena_used", "arena_peak",
         "queued", "in_flight")]
 

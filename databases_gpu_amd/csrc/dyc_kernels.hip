@@ -62,6 +62,25 @@ __device__ inline void load_slots(const uint64_t *p, uint64_t (&e)[SPL])
     }
 }
 
+// Device-coherent read of SPL slots (agent-scope relaxed atomic loads, which skip lines another
+// XCD's writes may have left stale in this XCD's L2). For reads that must see exchanges made
+// earlier in the same launch by other waves.
+template <int SPL>
+__device__ inline void load_slots_coherent(const uint64_t *p, uint64_t (&e)[SPL])
+{
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) e[j] = __hip_atomic_load(p + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int SPL, bool COH>
+__device__ inline void read_slots(const uint64_t *p, uint64_t (&e)[SPL])
+{
+    if constexpr (COH)
+        load_slots_coherent<SPL>(p, e);
+    else
+        load_slots<SPL>(p, e);
+}
+
 __device__ inline bool cas_slot(uint64_t *p, uint64_t &expected, uint64_t desired)
 {
     return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -85,8 +104,9 @@ __device__ inline void wave_add(unsigned long long *dst, unsigned long long v)
 
 // ------------------------------------------------------------------------ find / erase
 // One wave probes the 64 keys keys[base, base + 64) clipped to n, in table D. Returns this
-// lane's flag (hit or deleted) for its own key; all 64 lanes must call it together.
-template <int G, bool ERASE>
+// lane's flag (hit or deleted) for its own key; all 64 lanes must call it together. COH: read the
+// buckets device-coherently (see load_slots_coherent).
+template <int G, bool ERASE, bool COH = false>
 __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                       uint8_t *__restrict__ flags, uint64_t base, uint64_t n)
 {
@@ -116,7 +136,7 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
             if constexpr (ERASE) kk_loc[kk] = l;
 #pragma unroll
             for (int j = 0; j < SPL; ++j) e[kk][j] = 0;
-            if (kk_valid[kk]) load_slots<SPL>(loc_ptr(D, l) + s * SPL, e[kk]);
+            if (kk_valid[kk]) read_slots<SPL, COH>(loc_ptr(D, l) + s * SPL, e[kk]);
         }
 
         uint32_t my_val = 0, my_flag = 0;
@@ -160,7 +180,7 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
         for (int kk = 0; kk < G; ++kk) {
             const uint32_t l = __shfl(loc2, g * G + kk);
             if constexpr (ERASE) kk_loc[kk] = l;
-            if (kk_valid[kk] && !(done & (1u << kk))) load_slots<SPL>(loc_ptr(D, l) + s * SPL, e[kk]);
+            if (kk_valid[kk] && !(done & (1u << kk))) read_slots<SPL, COH>(loc_ptr(D, l) + s * SPL, e[kk]);
         }
 #pragma unroll
         for (int kk = 0; kk < G; ++kk) {
@@ -283,6 +303,9 @@ __device__ inline void kick_step(const Desc &D, Carry<kSlots / G> &c, const Inse
             if (at < sink.cap) {
                 sink.pend[at] = c.entry;
                 if (sink.tags) sink.tags[at] = tag;
+                // rare: publish the entry to every XCD (a serving tick's later finds read the pool
+                // within the same launch)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             } else {
                 atomicAdd(&C->lost, 1ull);
             }
@@ -749,7 +772,9 @@ __device__ inline uint32_t pool_probe(const Pool &pool, uint64_t np, const Serve
     uint32_t got = 0;
     for (uint64_t j = 0; j < np; ++j) {
         const uint64_t e = __hip_atomic_load(pool.pend + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (key_of(e) != k || pool.tags[j] != item.table) continue;
+        if (key_of(e) != k ||
+            __hip_atomic_load(pool.tags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != item.table)
+            continue;
         if constexpr (ERASE) {  // the pool counts as one more bucket: clear every copy in it
             uint64_t exp = e;
             got |= (uint32_t)cas_slot(pool.pend + j, exp, 0ull);
@@ -763,24 +788,58 @@ __device__ inline uint32_t pool_probe(const Pool &pool, uint64_t np, const Serve
     return got;
 }
 
+// One item of finds or erases; returns this lane's count of hits (erase: deletions).
+template <int G, bool ERASE, bool COH = false>
+__device__ inline unsigned long long serve_probe_item(const Desc &D, const ServeItem &item, const Pool &pool,
+                                                      uint64_t np)
+{
+    unsigned long long nd = 0;
+    for (uint64_t base = 0; base < item.len; base += kWave) {
+        uint32_t f = probe_tile<G, ERASE, COH>(D, item.keys, ERASE ? nullptr : item.out_vals, item.flags, base,
+                                               item.len);
+        if (np) f |= pool_probe<ERASE>(pool, np, item, base, f);
+        nd += f;
+    }
+    return nd;
+}
+
+// One item of inserts; the wave's groups are all idle on return.
+template <int G>
+__device__ inline void serve_insert_item(const Desc &D, const ServeItem &item, Carry<kSlots / G> &c,
+                                         const InsertPolicy &P, Counters *C, const PendSink &sink, InsertTally &tally,
+                                         unsigned long long *ins_total)
+{
+    uint64_t next = 0;
+    const uint64_t end = item.len;
+    const unsigned long long rej0 = tally.rejected;
+    uint64_t taken = 0;
+    while (true) {
+        refill<G, false>(c, next, end, item.keys, item.vals, nullptr, kNoTable, tally, taken);
+        if (!__ballot(c.active)) break;
+        if (c.active) kick_step<G>(D, c, P, C, sink, item.table, tally);
+    }
+    const unsigned long long rej = wave_sum(tally.rejected - rej0);
+    if (__lane_id() == 0 && item.len > rej) atomicAdd(&ins_total[item.table], item.len - rej);
+}
+
+__device__ inline uint64_t pool_size(const Pool &pool)
+{
+    const unsigned long long pc = __hip_atomic_load(pool.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return pc < pool.cap ? pc : pool.cap;
+}
+
 template <int G, bool ERASE>
 __global__ __launch_bounds__(256) void serve_probe_kernel(const Desc *__restrict__ dir,
                                                           const ServeItem *__restrict__ items, uint32_t n_items,
                                                           Pool pool, unsigned long long *__restrict__ del_total)
 {
     const uint32_t waves = gridDim.x * (blockDim.x / kWave);
-    const unsigned long long pc = *pool.count;
-    const uint64_t np = pc < pool.cap ? pc : pool.cap;
+    const uint64_t np = pool_size(pool);
     for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
          it += waves) {
         const ServeItem item = items[it];
         const Desc D = dir[item.table];
-        unsigned long long nd = 0;
-        for (uint64_t base = 0; base < item.len; base += kWave) {
-            uint32_t f = probe_tile<G, ERASE>(D, item.keys, ERASE ? nullptr : item.out_vals, item.flags, base, item.len);
-            if (np) f |= pool_probe<ERASE>(pool, np, item, base, f);
-            nd += f;
-        }
+        unsigned long long nd = serve_probe_item<G, ERASE>(D, item, pool, np);
         if constexpr (ERASE) {
             nd = wave_sum(nd);
             if (__lane_id() == 0 && nd) atomicAdd(&del_total[item.table], nd);
@@ -801,22 +860,97 @@ __global__ __launch_bounds__(256) void serve_insert_kernel(const Desc *__restric
     for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
          it += waves) {
         const ServeItem item = items[it];
-        const Desc D = dir[item.table];
-        uint64_t next = 0;
-        const uint64_t end = item.len;
-        const unsigned long long rej0 = tally.rejected;
-        uint64_t taken = 0;
-        while (true) {
-            refill<G, false>(c, next, end, item.keys, item.vals, nullptr, kNoTable, tally, taken);
-            if (!__ballot(c.active)) break;
-            if (c.active) kick_step<G>(D, c, P, C, sink, item.table, tally);
-        }
-        const unsigned long long rej = wave_sum(tally.rejected - rej0);
-        if (__lane_id() == 0 && item.len > rej) atomicAdd(&ins_total[item.table], item.len - rej);
+        serve_insert_item<G>(dir[item.table], item, c, P, C, sink, tally, ins_total);
     }
     wave_add(&C->kicks, tally.kicks);
     wave_add(&C->rejected, tally.rejected);
     wave_add(&C->retries, tally.retries);
+}
+
+// A whole serving tick in one launch. Waves claim items in list order from a counter; an item
+// whose stage has a predecessor (the same table's previous round or phase) first waits for the
+// stage's ready flag; when a stage's last item finishes it raises the flag of the table's next
+// stage. No acquire/release fences: on this chip an agent-scope fence writes back or
+// invalidates the whole L2 of the XCD. Table changes are 64-bit exchanges (device-coherent, and
+// their results are consumed before the stage count drops), and finds and erases read buckets
+// with coherent loads. Items wait only on items earlier in the list, which were claimed by
+// running waves, so the earliest unfinished item never waits and the launch always drains. The
+// wait is bounded all the same: past spin_cap polls the wave records a stall and goes on (the
+// host reports the tick failed).
+template <int GI, int GF>
+__global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict__ dir, const TickItem *__restrict__ items,
+                                                         uint32_t n_items, unsigned long long *__restrict__ cursor,
+                                                         TickStages stages, InsertPolicy P, Counters *__restrict__ C,
+                                                         PendSink sink, Pool pool,
+                                                         unsigned long long *__restrict__ ins_total,
+                                                         unsigned long long *__restrict__ del_total, uint32_t spin_cap)
+{
+    Carry<kSlots / GI> c;
+    c.active = false;
+    InsertTally tally;
+    unsigned long long stalls = 0;
+    while (true) {
+        uint32_t it = 0;
+        if (__lane_id() == 0) it = (uint32_t)atomicAdd(cursor, 1ull);
+        it = wave_uniform(it);
+        if (it >= n_items) break;
+        const TickItem ti = items[it];
+        if (ti.wait) {
+            uint32_t ok = 1;
+            if (__lane_id() == 0) {
+                uint32_t spins = 0, back = 1;
+                const uint32_t *flag = stages.ready + (uint64_t)ti.stage * kReadyStride;
+                while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                    for (uint32_t b = 0; b < back; ++b) __builtin_amdgcn_s_sleep(4);
+                    back = back < 16 ? back * 2 : 16;
+                    if (++spins > spin_cap) {
+                        ok = 0;
+                        break;
+                    }
+                }
+            }
+            if (!wave_uniform(ok)) ++stalls;
+        }
+        const Desc D = dir[ti.item.table];
+        if (ti.op == 0) {
+            serve_insert_item<GI>(D, ti.item, c, P, C, sink, tally, ins_total);
+        } else if (ti.op == 1) {
+            serve_probe_item<GF, false, true>(D, ti.item, pool, pool_size(pool));
+        } else {
+            unsigned long long nd = wave_sum(serve_probe_item<GF, true, true>(D, ti.item, pool, pool_size(pool)));
+            if (__lane_id() == 0 && nd) atomicAdd(&del_total[ti.item.table], nd);
+        }
+        if (__lane_id() == 0) {
+            const uint32_t before =
+                __hip_atomic_fetch_sub(stages.left + ti.stage, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (before == 1) {
+                const uint32_t nx = stages.next[ti.stage];
+                if (nx != ~0u)
+                    __hip_atomic_store(stages.ready + (uint64_t)nx * kReadyStride, 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    wave_add(&C->kicks, tally.kicks);
+    wave_add(&C->rejected, tally.rejected);
+    wave_add(&C->retries, tally.retries);
+    if (__lane_id() == 0 && stalls) atomicAdd(&C->stalls, stalls);
+}
+
+// Copy between host-pinned and device memory with a kernel instead of a DMA transfer: a
+// serving tick moves a few KB each way, where queueing a copy engine costs more than the
+// transfer. Up to two (src, dst, words) segments per launch.
+struct CopySeg {
+    const unsigned long long *src;
+    unsigned long long *dst;
+    uint64_t words;
+};
+
+__global__ __launch_bounds__(256) void copy_kernel(CopySeg a, CopySeg b)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.words; i += stride) a.dst[i] = a.src[i];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < b.words; i += stride) b.dst[i] = b.src[i];
 }
 
 __global__ void zero_kernel(unsigned long long *p, uint64_t words)
@@ -1020,6 +1154,19 @@ hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t
     return hipGetLastError();
 }
 
+hipError_t launch_copy(const void *src0, void *dst0, uint64_t words0, const void *src1, void *dst1, uint64_t words1,
+                       hipStream_t st)
+{
+    const uint64_t words = std::max(words0, words1);
+    if (words == 0) return hipSuccess;
+    uint64_t blocks = (words + 255) / 256;
+    if (blocks > 256) blocks = 256;
+    copy_kernel<<<(int)blocks, 256, 0, st>>>(
+        CopySeg{static_cast<const unsigned long long *>(src0), static_cast<unsigned long long *>(dst0), words0},
+        CopySeg{static_cast<const unsigned long long *>(src1), static_cast<unsigned long long *>(dst1), words1});
+    return hipGetLastError();
+}
+
 static int serve_grid(uint32_t n_items, int blocks)
 {
     return blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
@@ -1064,6 +1211,57 @@ hipError_t launch_serve_insert(const Desc *dir, const ServeItem *items, uint32_t
     default: serve_insert_kernel<1><<<grid, 256, 0, st>>>(dir, items, n_items, pol, C, sink, ins_total); break;
     }
     return hipGetLastError();
+}
+
+template <int GI>
+static hipError_t tick_by_find_lanes(int grid, int find_lanes, const Desc *dir, const TickItem *items,
+                                     uint32_t n_items, unsigned long long *cursor, TickStages done,
+                                     InsertPolicy pol, Counters *C, PendSink sink, Pool pool,
+                                     unsigned long long *ins_total, unsigned long long *del_total, uint32_t spin_cap,
+                                     hipStream_t st)
+{
+    switch (find_lanes) {
+    case 4:
+        serve_tick_kernel<GI, 4><<<grid, 256, 0, st>>>(dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
+                                                       del_total, spin_cap);
+        break;
+    case 16:
+        serve_tick_kernel<GI, 16><<<grid, 256, 0, st>>>(dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
+                                                        del_total, spin_cap);
+        break;
+    default:
+        serve_tick_kernel<GI, 8><<<grid, 256, 0, st>>>(dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
+                                                       del_total, spin_cap);
+        break;
+    }
+    return hipGetLastError();
+}
+
+bool serve_tick_supported(int insert_lanes, int find_lanes)
+{
+    return (insert_lanes == 1 || insert_lanes == 2 || insert_lanes == 4) &&
+           (find_lanes == 4 || find_lanes == 8 || find_lanes == 16);
+}
+
+hipError_t launch_serve_tick(const Desc *dir, const TickItem *items, uint32_t n_items, unsigned long long *cursor,
+                             TickStages done, InsertPolicy pol, int insert_lanes, int find_lanes, int blocks,
+                             Counters *C, PendSink sink, Pool pool, unsigned long long *ins_total,
+                             unsigned long long *del_total, uint32_t spin_cap, hipStream_t st)
+{
+    if (n_items == 0) return hipSuccess;
+    if (!serve_tick_supported(insert_lanes, find_lanes)) return hipErrorInvalidValue;
+    const int grid = serve_grid(n_items, blocks);
+    switch (insert_lanes) {
+    case 2:
+        return tick_by_find_lanes<2>(grid, find_lanes, dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
+                                     del_total, spin_cap, st);
+    case 4:
+        return tick_by_find_lanes<4>(grid, find_lanes, dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
+                                     del_total, spin_cap, st);
+    default:
+        return tick_by_find_lanes<1>(grid, find_lanes, dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
+                                     del_total, spin_cap, st);
+    }
 }
 
 hipError_t launch_zero(unsigned long long *p, uint64_t words, hipStream_t st)

@@ -15,7 +15,8 @@ struct Counters {
     unsigned long long retries;    // CAS attempts that lost a race
     unsigned long long misplaced;  // resize: entries outside their bucket pair (corrupt image)
     unsigned long long occupied;   // count kernel
-    unsigned long long pad[8];
+    unsigned long long stalls;     // serving tick: dependency waits that hit their bound
+    unsigned long long pad[7];
 };
 constexpr int kCounterWords = 16;
 constexpr uint32_t kMaxRegions = 1024;
@@ -101,6 +102,35 @@ struct Pool {
     const unsigned long long *count;
 };
 
+// An item of a whole-tick launch: op 0 insert, 1 find, 2 erase. Items of one table and one
+// (round, op) form a stage; with `wait` set the item starts once its stage is ready (the
+// table's previous stage in the tick has finished).
+struct TickItem {
+    ServeItem item;
+    uint32_t op;
+    uint32_t stage;
+    uint32_t wait;
+    uint32_t pad;
+};
+
+// Per-stage control of a whole-tick launch, written by the host before each launch: left[s] =
+// items of stage s, next[s] = the table's next stage (~0u: none), ready[s * kReadyStride] =
+// 1 if nothing precedes the stage (flags padded to 64 bytes so waiters of different stages
+// poll different lines).
+constexpr uint32_t kReadyStride = 16;
+struct TickStages {
+    uint32_t *left;
+    const uint32_t *next;
+    uint32_t *ready;
+};
+
+bool serve_tick_supported(int insert_lanes, int find_lanes);
+// One launch for a whole tick (see serve_tick_kernel). cursor must be 0 at launch. spin_cap
+// bounds each wait for a stage.
+hipError_t launch_serve_tick(const Desc *dir, const TickItem *items, uint32_t n_items, unsigned long long *cursor,
+                             TickStages stages, InsertPolicy pol, int insert_lanes, int find_lanes, int blocks,
+                             Counters *C, PendSink sink, Pool pool, unsigned long long *ins_total,
+                             unsigned long long *del_total, uint32_t spin_cap, hipStream_t st);
 hipError_t launch_serve_insert(const Desc *dir, const ServeItem *items, uint32_t n_items, InsertPolicy pol, int lanes,
                                int blocks, Counters *C, PendSink sink, unsigned long long *ins_total, hipStream_t st);
 hipError_t launch_serve_find(const Desc *dir, const ServeItem *items, uint32_t n_items, Pool pool, int lanes,
@@ -116,6 +146,9 @@ hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, ui
 hipError_t launch_downsize(const uint64_t *old, uint32_t new_nb, uint64_t *dst, Counters *C, uint64_t *res,
                            uint64_t res_cap, hipStream_t st);
 hipError_t launch_zero(unsigned long long *p, uint64_t words, hipStream_t st);
+// Kernel copy of up to two segments of 8-byte words (either side may be host-pinned memory).
+hipError_t launch_copy(const void *src0, void *dst0, uint64_t words0, const void *src1, void *dst1, uint64_t words1,
+                       hipStream_t st);
 // buckets the pos1-then-pos2 find order reads for keys -> C->kicks, pos1 hits -> C->occupied
 hipError_t launch_probe_count(const Desc &D, const uint32_t *keys, uint64_t n, Counters *C, hipStream_t st);
 // count non-empty slots of one subtable into C->occupied

@@ -3,10 +3,13 @@
 // Iteration-level continuous batching on one stream the server owns:
 //   * requests (one reference batch call each: insert, find or erase of one table) queue on the
 //     host in arrival order;
-//   * a "tick" admits every queued request that fits (slots, per-table order, arena space),
-//     enqueues the upsizes those inserts need, uploads the tick's work items (and the table
-//     directory if a resize changed it), and launches one fused insert, one fused find and one
-//     fused erase over all of them, then a copy of the device counters and an event;
+//   * a "tick" admits every queued request that fits (slots, arena space), enqueues the upsizes
+//     those inserts need, stages the tick's work items (and the table directory if a resize
+//     changed it) with one copy kernel from pinned memory, and launches, round by round, one
+//     fused insert, one fused find and one fused erase over the round's requests; a copy kernel
+//     then writes the device counters to pinned memory and an event is recorded. A table's
+//     requests are spread over rounds so that within a round they follow insert -> find ->
+//     erase (at most one erase), which keeps every table's requests in arrival order;
 //   * up to pipeline_depth ticks are in flight: the next tick is built and enqueued while the
 //     previous one runs, so the device does not wait on the host between ticks;
 //   * a tick retires when its event has fired: its requests complete, table counts are
@@ -23,6 +26,7 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <unordered_map>
 #include <vector>
 
 using namespace dyc;
@@ -40,6 +44,10 @@ using Clock = std::chrono::steady_clock;
 constexpr uint32_t kTargetItems = 8192;  // work items per tick: one per resident wave (256 CUs x 32)
 constexpr uint32_t kMinItemKeys = 64;
 constexpr uint32_t kMaxItemKeys = 4096;
+constexpr uint32_t kMaxRounds = 64;    // phase-ordered rounds per tick
+constexpr uint32_t kChainItems = 1024;  // items per stage of a table with several stages
+static_assert(sizeof(ServeItem) % 8 == 0 && sizeof(TickItem) % 8 == 0 && sizeof(Desc) % 8 == 0,
+              "staged with 8-byte word copies");
 
 struct SrvTable {
     Desc D{};
@@ -75,6 +83,9 @@ struct Tick {
     ServeItem *h_items = nullptr;  // pinned staging: insert, find, erase items back to back
     ServeItem *d_items = nullptr;
     uint64_t items_cap = 0;
+    // one-launch ticks: word 0 = the launch's item cursor (staged as 0), then TickItems
+    unsigned long long *h_block = nullptr;  // pinned
+    unsigned long long *d_block = nullptr;
     Desc *h_dir = nullptr;                 // pinned copy of the directory this tick uploads
     unsigned long long *h_snap = nullptr;  // pinned: Counters, ins_total[n], del_total[n]
     Counters *d_rctr = nullptr;            // downsize compaction counters
@@ -105,6 +116,7 @@ struct dyc_server {
     uint64_t pool_settled = 0;      // pool entries left by the last re-placement
     uint64_t deleted_seen = 0;      // Counters::deleted as of the last synchronous erase
     uint64_t misplaced_seen = 0;
+    uint64_t stalls_seen = 0;
     Tick ticks[2];
     uint64_t tick_seq = 0;
     std::vector<Req> reqs;
@@ -120,8 +132,16 @@ struct dyc_server {
     std::vector<uint8_t> seen;     // listed in touched
     std::vector<uint64_t> tick_ins;
     std::vector<uint64_t> mem_head;  // the request a memory block held back (valid when blocked == 2)
+    std::vector<uint16_t> round;     // round of the table's last admitted request
+    std::vector<uint8_t> had_erase;  // the tick erases from the table
+    std::vector<int32_t> stage;      // one-launch ticks: the table's current stage key (round * 3 + op)
+    std::vector<uint32_t> last_sid;  // one-launch ticks: the table's current stage id, ~0u none
+    std::vector<uint32_t> st_left, st_next, st_ready;  // per stage of the tick being built
+    std::vector<uint32_t> nstages;   // stages (round, op) of the table in the tick being built
+    std::unordered_map<uint64_t, uint64_t> stage_keys;  // keys per (table, round, op)
     std::vector<uint32_t> touched;
-    std::vector<ServeItem> items[3];
+    std::vector<std::pair<uint64_t, uint32_t>> admitted;  // (ticket, round)
+    std::vector<ServeItem> items[kMaxRounds][3];
 
     Counters *pc() const { return reinterpret_cast<Counters *>(d_stat); }
     unsigned long long *ins_total() const { return d_stat + kCounterWords; }
@@ -220,9 +240,10 @@ bool wants_down(const dyc_server *s, const SrvTable &T)
     return (double)T.count < s->cfg.lower_bound * (double)cap && (double)cap > (double)s->cfg.min_capacity;
 }
 
+// dst is host-pinned (a tick's snapshot buffer)
 int read_stats_sync(dyc_server *s, unsigned long long *dst)
 {
-    HIP_TRY(hipMemcpyAsync(dst, s->d_stat, s->stat_bytes(), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(launch_copy(s->d_stat, dst, s->stat_bytes() / 8, nullptr, nullptr, 0, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     return DYC_OK;
 }
@@ -310,6 +331,10 @@ int retire(dyc_server *s, Tick &k)
         s->misplaced_seen = c.misplaced;
         status = sticky(s, fail(DYC_ERR_CORRUPT, "a resize found entries outside their bucket pair"));
     }
+    if (c.stalls > s->stalls_seen) {
+        s->stalls_seen = c.stalls;
+        status = sticky(s, fail(DYC_ERR_HIP, "a tick item waited past spin_cap for its table's earlier rounds"));
+    }
     refresh_counts(s, k.h_snap);
     for (const TickTable &tt : k.tables) {
         SrvTable &T = s->t[tt.table];
@@ -380,6 +405,11 @@ int build_tick(dyc_server *s, bool *launched, bool *stuck)
         s->in_tick[x] = 0;
         s->seen[x] = 0;
         s->tick_ins[x] = 0;
+        s->round[x] = 0;
+        s->had_erase[x] = 0;
+        s->stage[x] = -1;
+        s->last_sid[x] = ~0u;
+        s->nstages[x] = 0;
     }
     s->touched.clear();
     bool resized = false;
@@ -407,10 +437,14 @@ int build_tick(dyc_server *s, bool *launched, bool *stuck)
         s->down_list.swap(keep);
     }
 
-    // admission, arrival order
+    // admission, arrival order. A table's requests go into rounds: within a round they follow
+    // insert -> find -> erase with at most one erase; a request that would break that opens the
+    // table's next round. Round r runs after round r - 1 (three launches per round), so each
+    // table sees its requests in arrival order.
     uint64_t tick_keys = 0;
-    uint32_t nreq = 0;
+    uint32_t nreq = 0, rounds = 0;
     bool memory_blocked = false;
+    s->admitted.clear();
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         const uint64_t tk = *it;
         Req &R = s->reqs[tk];
@@ -424,12 +458,16 @@ int build_tick(dyc_server *s, bool *launched, bool *stuck)
             break;
         }
         const int ph = s->phase[x];
-        if (ph > (int)op || ph == (int)DYC_OP_ERASE) {
-            mark(s, x);
-            s->blocked[x] = 1;
-            s->ss.deferred_order++;
-            ++it;
-            continue;
+        uint32_t r = s->round[x];
+        if (ph >= 0 && (ph > (int)op || ph == (int)DYC_OP_ERASE)) {
+            if (r + 1 >= kMaxRounds) {
+                mark(s, x);
+                s->blocked[x] = 1;
+                s->ss.deferred_order++;
+                ++it;
+                continue;
+            }
+            ++r;
         }
         if (op == DYC_OP_INSERT) {
             SrvTable &T = s->t[x];
@@ -458,7 +496,12 @@ int build_tick(dyc_server *s, bool *launched, bool *stuck)
         mark(s, x);
         s->in_tick[x] = 1;
         if (op == DYC_OP_INSERT) s->tick_ins[x] += R.r.n;
+        if (op == DYC_OP_ERASE) s->had_erase[x] = 1;
+        if (s->phase[x] != (int8_t)op || s->round[x] != r || s->nstages[x] == 0) s->nstages[x]++;
         s->phase[x] = (int8_t)op;
+        s->round[x] = (uint16_t)r;
+        rounds = std::max(rounds, r + 1);
+        s->admitted.emplace_back(tk, r);
         k.reqs.push_back(tk);
         R.state = 1;
         tick_keys += R.r.n;
@@ -479,40 +522,125 @@ int build_tick(dyc_server *s, bool *launched, bool *stuck)
     const uint64_t room_items = k.items_cap > nreq ? k.items_cap - nreq : 1;
     per = std::max<uint64_t>(per, (tick_keys + room_items - 1) / room_items);
     per = std::min<uint64_t>(per, 0xffffffffull);
-    for (auto &v : s->items) v.clear();
-    for (uint64_t tk : k.reqs) {
-        const dyc_request &r = s->reqs[tk].r;
-        for (uint64_t o = 0; o < r.n; o += per) {
+    // A table with several stages (round, op) in this tick is a dependency chain: cut its
+    // stages into many small items (up to kChainItems per stage, at least one tile each) so
+    // that each stage finishes in about one tile's latency; other tables take the tick's size.
+    s->stage_keys.clear();
+    for (const auto &[tk, r] : s->admitted) {
+        const dyc_request &q = s->reqs[tk].r;
+        s->stage_keys[((uint64_t)q.table << 16 | r) * 3 + q.op] += q.n;
+    }
+    for (uint32_t r = 0; r < rounds; ++r)
+        for (auto &v : s->items[r]) v.clear();
+    for (const auto &[tk, r] : s->admitted) {
+        const dyc_request &q = s->reqs[tk].r;
+        uint64_t step_keys = per;
+        if (s->scfg.item_keys == 0 && s->nstages[q.table] > 1) {
+            const uint64_t sk = s->stage_keys[((uint64_t)q.table << 16 | r) * 3 + q.op];
+            step_keys = (sk / kChainItems + kMinItemKeys - 1) / kMinItemKeys * kMinItemKeys;
+            step_keys = std::max<uint64_t>(step_keys, std::max<uint64_t>(kMinItemKeys,
+                                                                          (tick_keys + room_items - 1) / room_items));
+        }
+        for (uint64_t o = 0; o < q.n; o += step_keys) {
             ServeItem it{};
-            it.keys = r.keys + o;
-            it.vals = r.op == DYC_OP_INSERT ? r.vals + o : nullptr;
-            it.out_vals = (r.op == DYC_OP_FIND && r.out_vals) ? r.out_vals + o : nullptr;
-            it.flags = (r.op != DYC_OP_INSERT && r.out_flags) ? r.out_flags + o : nullptr;
-            it.len = (uint32_t)std::min<uint64_t>(per, r.n - o);
-            it.table = r.table;
-            s->items[r.op].push_back(it);
+            it.keys = q.keys + o;
+            it.vals = q.op == DYC_OP_INSERT ? q.vals + o : nullptr;
+            it.out_vals = (q.op == DYC_OP_FIND && q.out_vals) ? q.out_vals + o : nullptr;
+            it.flags = (q.op != DYC_OP_INSERT && q.out_flags) ? q.out_flags + o : nullptr;
+            it.len = (uint32_t)std::min<uint64_t>(step_keys, q.n - o);
+            it.table = q.table;
+            s->items[r][q.op].push_back(it);
         }
     }
-    const uint64_t ni = s->items[0].size(), nf = s->items[1].size(), ne = s->items[2].size();
-    if (ni + nf + ne > k.items_cap) return sticky(s, fail(DYC_ERR_INVALID, "tick needs more work items than staged"));
-    memcpy(k.h_items, s->items[0].data(), ni * sizeof(ServeItem));
-    memcpy(k.h_items + ni, s->items[1].data(), nf * sizeof(ServeItem));
-    memcpy(k.h_items + ni + nf, s->items[2].data(), ne * sizeof(ServeItem));
     for (uint32_t x : s->touched)
-        if (s->in_tick[x]) k.tables.push_back(TickTable{x, s->tick_ins[x], s->phase[x] == (int)DYC_OP_ERASE});
-    if (ni + nf + ne)
-        HIP_TRY(hipMemcpyAsync(k.d_items, k.h_items, (ni + nf + ne) * sizeof(ServeItem), hipMemcpyHostToDevice, s->st));
-    if (s->dir_dirty) {
+        if (s->in_tick[x]) k.tables.push_back(TickTable{x, s->tick_ins[x], s->had_erase[x] != 0});
+    const bool dir_up = s->dir_dirty;
+    if (dir_up)
         for (uint32_t i = 0; i < s->n; ++i) k.h_dir[i] = s->t[i].D;
-        HIP_TRY(hipMemcpyAsync(s->d_dir, k.h_dir, s->n * sizeof(Desc), hipMemcpyHostToDevice, s->st));
-        s->dir_dirty = false;
+    s->dir_dirty = false;
+    uint64_t n_items = 0;
+    for (uint32_t r = 0; r < rounds; ++r)
+        for (uint32_t op = 0; op < 3; ++op) n_items += s->items[r][op].size();
+    if (n_items > k.items_cap) return sticky(s, fail(DYC_ERR_INVALID, "tick needs more work items than staged"));
+    const bool one = s->scfg.one_launch && serve_tick_supported((int)s->cfg.insert_lanes, (int)s->cfg.find_lanes);
+    if (one) {
+        // items in (round, op) order; a table's items of one (round, op) form a stage that waits
+        // for the table's previous stage in the tick
+        k.h_block[0] = 0;
+        TickItem *ti = reinterpret_cast<TickItem *>(k.h_block + 1);
+        std::vector<uint32_t> &left = s->st_left, &next = s->st_next;
+        left.clear();
+        next.clear();
+        s->st_ready.clear();
+        uint64_t at = 0;
+        for (uint32_t r = 0; r < rounds; ++r)
+            for (uint32_t op = 0; op < 3; ++op)
+                for (const ServeItem &it : s->items[r][op]) {
+                    const uint32_t x = it.table;
+                    const int32_t key = (int32_t)(r * 3 + op);
+                    if (s->stage[x] != key) {
+                        const uint32_t sid = (uint32_t)left.size();
+                        left.push_back(0);
+                        next.push_back(~0u);
+                        s->st_ready.push_back(s->last_sid[x] == ~0u ? 1u : 0u);
+                        if (s->last_sid[x] != ~0u) next[s->last_sid[x]] = sid;
+                        s->last_sid[x] = sid;
+                        s->stage[x] = key;
+                    }
+                    const uint32_t sid = s->last_sid[x];
+                    left[sid]++;
+                    TickItem &d = ti[at++];
+                    d.item = it;
+                    d.op = op;
+                    d.stage = sid;
+                    d.wait = s->st_ready[sid] ? 0u : 1u;
+                    d.pad = 0;
+                }
+        const uint64_t ns = left.size();
+        const uint64_t w_items = 1 + n_items * sizeof(TickItem) / 8;
+        const uint64_t w_half = (ns + 1) / 2;  // a u32 array of ns entries, in 8-byte words
+        uint32_t *h_left = reinterpret_cast<uint32_t *>(k.h_block + w_items);
+        uint32_t *h_next = reinterpret_cast<uint32_t *>(k.h_block + w_items + w_half);
+        uint32_t *h_ready = reinterpret_cast<uint32_t *>(k.h_block + w_items + 2 * w_half);
+        memcpy(h_left, left.data(), ns * 4);
+        memcpy(h_next, next.data(), ns * 4);
+        for (uint64_t i = 0; i < ns; ++i) h_ready[i * kReadyStride] = s->st_ready[i];
+        const uint64_t words = w_items + 2 * w_half + ns * kReadyStride / 2;
+        HIP_TRY(launch_copy(k.h_block, k.d_block, words, k.h_dir, s->d_dir, dir_up ? s->n * sizeof(Desc) / 8 : 0,
+                            s->st));
+        uint32_t *d_left = reinterpret_cast<uint32_t *>(k.d_block + w_items);
+        TickStages stg{d_left, reinterpret_cast<const uint32_t *>(k.d_block + w_items + w_half),
+                       reinterpret_cast<uint32_t *>(k.d_block + w_items + 2 * w_half)};
+        HIP_TRY(launch_serve_tick(s->d_dir, reinterpret_cast<const TickItem *>(k.d_block + 1), (uint32_t)n_items,
+                                  k.d_block, stg, s->policy(), (int)s->cfg.insert_lanes, (int)s->cfg.find_lanes, 0,
+                                  s->pc(), s->sink(0), s->pool(), s->ins_total(), s->del_total(), s->scfg.spin_cap,
+                                  s->st));
+    } else {
+        uint64_t at = 0;
+        for (uint32_t r = 0; r < rounds; ++r)
+            for (uint32_t op = 0; op < 3; ++op) {
+                const auto &v = s->items[r][op];
+                memcpy(k.h_items + at, v.data(), v.size() * sizeof(ServeItem));
+                at += v.size();
+            }
+        HIP_TRY(launch_copy(k.h_items, k.d_items, n_items * sizeof(ServeItem) / 8, k.h_dir, s->d_dir,
+                            dir_up ? s->n * sizeof(Desc) / 8 : 0, s->st));
+        at = 0;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            const uint64_t ni = s->items[r][0].size(), nf = s->items[r][1].size(), ne = s->items[r][2].size();
+            HIP_TRY(launch_serve_insert(s->d_dir, k.d_items + at, (uint32_t)ni, s->policy(), (int)s->cfg.insert_lanes,
+                                        0, s->pc(), s->sink(0), s->ins_total(), s->st));
+            at += ni;
+            HIP_TRY(launch_serve_find(s->d_dir, k.d_items + at, (uint32_t)nf, s->pool(), (int)s->cfg.find_lanes, 0,
+                                      s->st));
+            at += nf;
+            HIP_TRY(launch_serve_erase(s->d_dir, k.d_items + at, (uint32_t)ne, s->pool(), s->del_total(),
+                                       (int)s->cfg.find_lanes, 0, s->st));
+            at += ne;
+        }
     }
-    HIP_TRY(launch_serve_insert(s->d_dir, k.d_items, (uint32_t)ni, s->policy(), (int)s->cfg.insert_lanes, 0, s->pc(),
-                                s->sink(0), s->ins_total(), s->st));
-    HIP_TRY(launch_serve_find(s->d_dir, k.d_items + ni, (uint32_t)nf, s->pool(), (int)s->cfg.find_lanes, 0, s->st));
-    HIP_TRY(launch_serve_erase(s->d_dir, k.d_items + ni + nf, (uint32_t)ne, s->pool(), s->del_total(),
-                               (int)s->cfg.find_lanes, 0, s->st));
-    HIP_TRY(hipMemcpyAsync(k.h_snap, s->d_stat, s->stat_bytes(), hipMemcpyDeviceToHost, s->st));
+    s->ss.rounds += rounds;
+    HIP_TRY(launch_copy(s->d_stat, k.h_snap, s->stat_bytes() / 8, nullptr, nullptr, 0, s->st));
     HIP_TRY(hipEventRecord(k.done, s->st));
     for (const TickTable &tt : k.tables) {
         SrvTable &T = s->t[tt.table];
@@ -680,6 +808,8 @@ void destroy(dyc_server *s)
         if (k.d_items) (void)hipFree(k.d_items);
         if (k.h_dir) (void)hipHostFree(k.h_dir);
         if (k.h_snap) (void)hipHostFree(k.h_snap);
+        if (k.h_block) (void)hipHostFree(k.h_block);
+        if (k.d_block) (void)hipFree(k.d_block);
         if (k.d_rctr) (void)hipFree(k.d_rctr);
         if (k.done) (void)hipEventDestroy(k.done);
     }
@@ -689,6 +819,7 @@ void destroy(dyc_server *s)
     if (s->d_pool) (void)hipFree(s->d_pool);
     if (s->d_pool_tag) (void)hipFree(s->d_pool_tag);
     if (s->d_resolve) (void)hipFree(s->d_resolve);
+
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
@@ -703,6 +834,12 @@ int create(dyc_server *s, uint32_t n_tables, const uint64_t *hints)
     s->seen.assign(n_tables, 0);
     s->tick_ins.assign(n_tables, 0);
     s->mem_head.assign(n_tables, 0);
+    s->round.assign(n_tables, 0);
+    s->had_erase.assign(n_tables, 0);
+    s->stage.assign(n_tables, -1);
+    s->last_sid.assign(n_tables, ~0u);
+    s->nstages.assign(n_tables, 0);
+
     HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     uint64_t budget = s->scfg.hbm_budget;
     if (budget == 0) {
@@ -729,8 +866,10 @@ int create(dyc_server *s, uint32_t n_tables, const uint64_t *hints)
     if (!rc) rc = dev_alloc((void **)&s->d_pool, s->scfg.pool_cap * sizeof(uint64_t));
     if (!rc) rc = dev_alloc((void **)&s->d_pool_tag, s->scfg.pool_cap * sizeof(uint32_t));
     if (!rc) rc = dev_alloc((void **)&s->d_resolve, s->scfg.pool_cap * sizeof(uint64_t));
+
     if (rc) return rc;
     HIP_TRY(launch_zero(s->d_stat, s->stat_bytes() / sizeof(unsigned long long), s->st));
+
     const uint64_t items_cap = s->scfg.tick_max_keys / kMinItemKeys + s->scfg.tick_max_requests + 1;
     for (uint32_t i = 0; i < 2; ++i) {
         Tick &k = s->ticks[i];
@@ -741,6 +880,12 @@ int create(dyc_server *s, uint32_t n_tables, const uint64_t *hints)
         HIP_TRY(hipHostMalloc((void **)&k.h_dir, std::max<size_t>(1, n_tables) * sizeof(Desc), hipHostMallocDefault));
         HIP_TRY(hipHostMalloc((void **)&k.h_snap, s->stat_bytes(), hipHostMallocDefault));
         rc = dev_alloc((void **)&k.d_rctr, sizeof(Counters));
+        if (rc) return rc;
+        // cursor, items, then per stage (at most one per item): left, next, padded ready flag
+        const size_t block_bytes =
+            (1 + items_cap * sizeof(TickItem) / 8 + 2 * ((items_cap + 1) / 2) + items_cap * kReadyStride / 2) * 8;
+        HIP_TRY(hipHostMalloc((void **)&k.h_block, block_bytes, hipHostMallocDefault));
+        rc = dev_alloc((void **)&k.d_block, block_bytes);
         if (rc) return rc;
         HIP_TRY(hipEventCreateWithFlags(&k.done, hipEventDisableTiming));
     }
@@ -777,6 +922,8 @@ int dyc_server_config_init(dyc_server_config *scfg)
     scfg->pipeline_depth = 2;
     scfg->item_keys = 0;
     scfg->pool_cap = 65536;
+    scfg->one_launch = 1;
+    scfg->spin_cap = 1u << 24;
     return DYC_OK;
 }
 
@@ -792,8 +939,8 @@ int dyc_server_create(const dyc_config *cfg, const dyc_server_config *scfg, uint
     if (scfg) sc = *scfg;
     else dyc_server_config_init(&sc);
     if (sc.pipeline_depth < 1 || sc.pipeline_depth > 2) return fail(DYC_ERR_INVALID, "pipeline_depth must be 1 or 2");
-    if (sc.tick_max_keys == 0 || sc.tick_max_requests == 0 || sc.pool_cap == 0)
-        return fail(DYC_ERR_INVALID, "tick_max_keys, tick_max_requests and pool_cap must be positive");
+    if (sc.tick_max_keys == 0 || sc.tick_max_requests == 0 || sc.pool_cap == 0 || sc.spin_cap == 0)
+        return fail(DYC_ERR_INVALID, "tick_max_keys, tick_max_requests, pool_cap and spin_cap must be positive");
     if (sc.item_keys && (sc.item_keys % 64)) return fail(DYC_ERR_INVALID, "item_keys must be a multiple of 64");
     const bool lanes_ok = (c.find_lanes == 4 || c.find_lanes == 8 || c.find_lanes == 16) &&
                           (c.insert_lanes == 1 || c.insert_lanes == 2 || c.insert_lanes == 4 ||

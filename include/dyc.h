@@ -226,9 +226,10 @@ int dyc_group_clear(dyc_group *g, dyc_stream stream);
  * Admission, per tick, in arrival order:
  *   - slots: at most tick_max_requests requests and tick_max_keys keys (a larger request is
  *     admitted alone);
- *   - order: a table's requests run in arrival order. Within a tick a table's requests must
- *     follow insert -> find -> erase with at most one erase; a request that would break this,
- *     and every later request of its table, waits for a later tick;
+ *   - order: a table's requests run in arrival order. A tick runs in rounds (one fused insert,
+ *     find and erase launch each); within a round a table's requests follow insert -> find ->
+ *     erase with at most one erase, and a request that would break this goes to the table's next
+ *     round (at most 64 rounds per tick; past that it waits for a later tick);
  *   - memory: all table storage lives in one arena of hbm_budget bytes. An insert that needs
  *     its table grown (count + n > upper_bound * capacity) is admitted only once the upsizes fit
  *     in the arena's free space; otherwise it (and its table's later requests) waits.
@@ -265,11 +266,17 @@ typedef struct dyc_server_config {
     uint32_t pipeline_depth;     /* ticks in flight at once, 1 or 2 (default 2) */
     uint32_t item_keys;          /* keys per work item; 0 = chosen per tick from its size */
     uint32_t pool_cap;           /* entries the overflow pool holds (default 65536) */
+    uint32_t one_launch;         /* 1 (default): a tick is one launch whose items wait on their table's
+                                    earlier rounds; 0: three launches per round. One launch needs
+                                    insert_lanes 1, 2 or 4 (otherwise rounds are launched) */
+    uint32_t spin_cap;           /* polls an item waits for its table before the tick is failed
+                                    (default 2^24) */
 } dyc_server_config;
 
 typedef struct dyc_server_stats {
     uint64_t submitted, completed, failed;
     uint64_t ticks;
+    uint64_t rounds;             /* phase-ordered rounds over all ticks (three launches each at most) */
     uint64_t keys;               /* keys of completed requests */
     uint64_t deferred_order;     /* admission passes that held a request back for table order */
     uint64_t deferred_memory;    /* admission passes that held a request back for arena space */

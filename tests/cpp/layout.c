@@ -13,8 +13,9 @@ int main(void)
            offsetof(dyc_stats, pending), offsetof(dyc_stats, rejected), offsetof(dyc_stats, lost));
     printf("request %zu %zu %zu %zu %zu\n", sizeof(dyc_request), offsetof(dyc_request, keys),
            offsetof(dyc_request, vals), offsetof(dyc_request, out_flags), offsetof(dyc_request, n));
-    printf("server_config %zu %zu %zu %zu\n", sizeof(dyc_server_config), offsetof(dyc_server_config, tick_max_requests),
-           offsetof(dyc_server_config, item_keys), offsetof(dyc_server_config, pool_cap));
+    printf("server_config %zu %zu %zu %zu %zu\n", sizeof(dyc_server_config),
+           offsetof(dyc_server_config, tick_max_requests), offsetof(dyc_server_config, item_keys),
+           offsetof(dyc_server_config, pool_cap), offsetof(dyc_server_config, spin_cap));
     printf("server_stats %zu %zu %zu\n", sizeof(dyc_server_stats), offsetof(dyc_server_stats, deferred_memory),
            offsetof(dyc_server_stats, in_flight));
     return 0;

@@ -91,7 +91,7 @@ def test_struct_layout_matches_c(lib, tmp_path):
     assert np.dtype(L.REQUEST_DTYPE).itemsize == r[0]
     sc = [int(x) for x in out[3].split()[1:]]
     assert sc == [ctypes.sizeof(L.ServerConfig), L.ServerConfig.tick_max_requests.offset,
-                  L.ServerConfig.item_keys.offset, L.ServerConfig.pool_cap.offset]
+                  L.ServerConfig.item_keys.offset, L.ServerConfig.pool_cap.offset, L.ServerConfig.spin_cap.offset]
     ss = [int(x) for x in out[4].split()[1:]]
     assert ss == [ctypes.sizeof(L.ServerStats), L.ServerStats.deferred_memory.offset, L.ServerStats.in_flight.offset]
 
@@ -121,6 +121,7 @@ def test_server_argument_errors(lib):
     sc = L.ServerConfig()
     assert lib.dyc_server_config_init(ctypes.byref(sc)) == 0
     assert sc.pipeline_depth == 2 and sc.tick_max_keys == 1 << 23 and sc.pool_cap == 65536
+    assert sc.one_launch == 1 and sc.spin_cap == 1 << 24
     sc.pipeline_depth = 3
     assert lib.dyc_server_create(None, ctypes.byref(sc), 1, hints, ctypes.byref(h)) == 1
     assert b"pipeline_depth" in lib.dyc_last_error()

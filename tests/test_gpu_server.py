@@ -68,14 +68,15 @@ def check_contents(srv, tr):
         assert srv.table_stats(t)["count"] == live.size
 
 
-@pytest.mark.parametrize("mix,offered", [((0.3, 0.6, 0.1), None), ((0.2, 0.4, 0.4), None), ((0.3, 0.6, 0.1), 2e8)])
-def test_mix_equals_alone_equals_expected(D, mix, offered):
+@pytest.mark.parametrize("mix,offered,one_launch", [((0.3, 0.6, 0.1), None, 1), ((0.2, 0.4, 0.4), None, 1),
+                                                    ((0.3, 0.6, 0.1), 2e8, 1), ((0.3, 0.6, 0.1), None, 0)])
+def test_mix_equals_alone_equals_expected(D, mix, offered, one_launch):
     from databases_gpu_amd import workload as W
 
-    banner(f"server mix vs alone, mix={mix} offered={offered}")
+    banner(f"server mix vs alone, mix={mix} offered={offered} one_launch={one_launch}")
     tr = W.ServingTrace(1200, seed=21, device="cuda", offered=offered, resident=1 << 14, tenants=16, tenant_lo=256,
                         tenant_hi=4096, batch_lo=1, batch_hi=1 << 11, mix=mix)
-    kw = dict(min_capacity=0, tick_max_keys=1 << 14, hbm_budget=256 << 20)
+    kw = dict(min_capacity=0, tick_max_keys=1 << 14, hbm_budget=256 << 20, one_launch=one_launch)
     sa, ova, ofa, done_a, st_a = run_mode(D, tr, 0, **kw)
     assert (st_a == 0).all()
     assert torch.equal(ofa, tr.exp_flags) and torch.equal(ova, tr.exp_vals)
@@ -100,7 +101,7 @@ def test_depth_one_and_fixed_items(D):
     tr = W.ServingTrace(600, seed=5, device="cuda", offered=None, resident=1 << 13, tenants=8, tenant_lo=256,
                         tenant_hi=2048, batch_lo=1, batch_hi=1 << 10)
     srv, ov, of, _, st = run_mode(D, tr, 0, min_capacity=0, pipeline_depth=1, item_keys=64, tick_max_requests=7,
-                                  find_lanes=4, insert_lanes=2, hbm_budget=64 << 20)
+                                  find_lanes=4, insert_lanes=2, hbm_budget=64 << 20, one_launch=0)
     assert (st == 0).all()
     assert torch.equal(of, tr.exp_flags) and torch.equal(ov, tr.exp_vals)
     assert srv.stats()["deferred_slots"] > 0
@@ -201,3 +202,30 @@ def test_admission_by_free_hbm(D):
         results.append(st.copy())
         srv.close()
     assert np.array_equal(results[0], results[1])
+
+
+def test_one_launch_with_unsupported_lanes_falls_back_to_rounds(D):
+    from databases_gpu_amd import workload as W
+
+    banner("server insert lanes 8 (rounds launched), find lanes 16")
+    tr = W.ServingTrace(400, seed=8, device="cuda", offered=None, resident=1 << 12, tenants=4, tenant_lo=256,
+                        tenant_hi=2048, batch_lo=1, batch_hi=1 << 10)
+    srv, ov, of, _, st = run_mode(D, tr, 0, min_capacity=0, insert_lanes=8, find_lanes=16, hbm_budget=64 << 20)
+    assert (st == 0).all()
+    assert torch.equal(of, tr.exp_flags) and torch.equal(ov, tr.exp_vals)
+    check_contents(srv, tr)
+    srv.close()
+
+
+def test_dependency_wait_bound_fails_the_tick_not_the_gpu(D):
+    """A one-launch tick's dependency waits are bounded: with a bound of one poll, items that must
+    wait give up, the launch still drains, and the server reports the tick's requests failed."""
+    from databases_gpu_amd import workload as W
+
+    banner("server spin_cap 1: bounded waits end the launch")
+    tr = W.ServingTrace(800, seed=13, device="cuda", offered=None, resident=1 << 14, tenants=2, tenant_lo=256,
+                        tenant_hi=1024, batch_lo=1 << 8, batch_hi=1 << 12)
+    srv, ov, of, _, st = run_mode(D, tr, 0, min_capacity=0, spin_cap=1, hbm_budget=256 << 20)
+    assert (st == 2).any()  # DYC_ERR_HIP: a wait hit its bound
+    assert set(np.unique(st).tolist()) <= {0, 2}
+    srv.close()
