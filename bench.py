@@ -71,6 +71,11 @@ def parse():
                     help="serving: offered loads in G keys/s (0 = every request arrives at once)")
     ap.add_argument("--sv-depth", type=int, default=2, help="serving: ticks in flight")
     ap.add_argument("--sv-tick-keys", type=int, default=1 << 23, help="serving: keys per tick")
+    ap.add_argument("--sv-kick", type=int, default=0,
+                    help="serving: eviction bound under the reference victim rule before the rescue walk (both "
+                         "modes; results do not depend on it)")
+    ap.add_argument("--sv-one-launch", type=int, default=1, help="serving: 1 one launch per tick, 0 per round")
+    ap.add_argument("--sv-backoff", type=int, default=16, help="serving: poll backoff cap (x 256 cycles)")
     ap.add_argument("--bt-tables", type=int, default=8192, help="batched: number of tables")
     ap.add_argument("--bt-log2min", type=int, default=8, help="batched: log2 of the smallest table")
     ap.add_argument("--bt-log2max", type=int, default=20, help="batched: log2 of the largest table")
@@ -271,7 +276,8 @@ def run_serving(args, dev) -> int:
     def fresh():
         srv = D.DyCuckooServer(tr.hints, hbm_budget=budget, pipeline_depth=args.sv_depth,
                                tick_max_keys=args.sv_tick_keys, insert_lanes=args.insert_lanes,
-                               find_lanes=args.find_lanes)
+                               find_lanes=args.find_lanes, kick_bound=args.sv_kick, poll_backoff=args.sv_backoff,
+                               one_launch=args.sv_one_launch)
         for t in range(tr.sizes.size):
             srv.insert(t, init[t], init_v[t])
         srv.drain()
@@ -315,12 +321,14 @@ def run_serving(args, dev) -> int:
                    "mops": round(ops / span / 1e6, 1),
                    "p50_ms": round(pct(lat_ms, 50), 3), "p99_ms": round(pct(lat_ms, 99), 3),
                    "max_ms": round(float(lat_ms.max()), 3), "span_s": round(span, 4), "wall_s": round(wall, 4),
-                   "ticks": st["ticks"] if mode == 0 else None, "resizes_up": st["resizes_up"],
+                   "ticks": st["ticks"] if mode == 0 else None, "rounds": st["rounds"] if mode == 0 else None,
+                   "resizes_up": st["resizes_up"],
                    "resizes_down": st["resizes_down"], "deferred_memory": st["deferred_memory"],
                    "arena_peak_gb": round(st["arena_peak"] / 2**30, 3), "parity": "ok" if ok else "mismatch"}
             runs.append(run)
             log(f"load {load} G keys/s {run['mode']}: {run['mops']} Mops/s, p50 {run['p50_ms']} ms, "
-                f"p99 {run['p99_ms']} ms, ticks {run['ticks']}, ups {run['resizes_up']}, parity {run['parity']}")
+                f"p99 {run['p99_ms']} ms, ticks {run['ticks']}, rounds {run['rounds']}, ups {run['resizes_up']}, "
+                f"parity {run['parity']}")
             srv.close()
     sat = {r["mode"]: r for r in runs if r["offered_gkeys_s"] is None}
     head = sat.get("continuous") or runs[0]
@@ -339,7 +347,8 @@ def run_serving(args, dev) -> int:
                                "[0.50, 0.90], in-line resizes",
                    "requests": args.sv_requests, "keys": ops, "tables": int(tr.sizes.size),
                    "initial_keys": int(tr.sizes.sum()), "pipeline_depth": args.sv_depth,
-                   "tick_max_keys": args.sv_tick_keys, "hbm_budget_gb": budget / 2**30, "seed": args.seed},
+                   "tick_max_keys": args.sv_tick_keys, "hbm_budget_gb": budget / 2**30, "seed": args.seed,
+                   "kick_bound": args.sv_kick, "poll_backoff": args.sv_backoff, "one_launch": args.sv_one_launch},
         "runs": runs,
         "saturation_speedup": (round(sat["continuous"]["mops"] / sat["synchronous"]["mops"], 2)
                                if "continuous" in sat and "synchronous" in sat else None),

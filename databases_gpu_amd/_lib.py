@@ -128,6 +128,8 @@ class ServerConfig(ctypes.Structure):
         ("pool_cap", ctypes.c_uint32),
         ("one_launch", ctypes.c_uint32),
         ("spin_cap", ctypes.c_uint32),
+        ("poll_backoff", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
     ]
 
 

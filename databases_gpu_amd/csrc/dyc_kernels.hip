@@ -223,16 +223,32 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
     }
 }
 
+template <bool ERASE>
+__device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const uint32_t *keys, uint32_t *out_vals,
+                                         uint8_t *flags, uint64_t n, uint32_t table, uint64_t base, uint32_t f);
+
+__device__ inline uint64_t pool_size(const Pool &pool)
+{
+    const unsigned long long pc = __hip_atomic_load(pool.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return pc < pool.cap ? pc : pool.cap;
+}
+
+// pool.count == nullptr: the table has no overflow pool (a plain table); otherwise misses also
+// look in the pool entries tagged `table` (a serving loop's table).
 template <int G, bool ERASE>
 __global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                     uint32_t *__restrict__ vals, uint8_t *__restrict__ flags,
-                                                    uint64_t n, Counters *__restrict__ C)
+                                                    uint64_t n, Counters *__restrict__ C, Pool pool, uint32_t table)
 {
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    const uint64_t np = pool.count ? pool_size(pool) : 0;
     unsigned long long ndel = 0;
     for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
-         base += waves * kWave)
-        ndel += probe_tile<G, ERASE>(D, keys, vals, flags, base, n);
+         base += waves * kWave) {
+        uint32_t f = probe_tile<G, ERASE>(D, keys, vals, flags, base, n);
+        if (np) f |= pool_probe_at<ERASE>(pool, np, keys, vals, flags, n, table, base, f);
+        ndel += f;
+    }
     if constexpr (ERASE) wave_add(&C->deleted, ndel);
 }
 
@@ -763,29 +779,35 @@ __global__ __launch_bounds__(256) void group_insert_kernel(const Desc *__restric
 // single-table code. Finds and erases that miss both buckets also look in the server's pool of
 // entries that exhausted the eviction bound (normally empty: one scalar load per tile).
 template <bool ERASE>
-__device__ inline uint32_t pool_probe(const Pool &pool, uint64_t np, const ServeItem &item, uint64_t base, uint32_t f)
+__device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const uint32_t *keys, uint32_t *out_vals,
+                                         uint8_t *flags, uint64_t n, uint32_t table, uint64_t base, uint32_t f)
 {
     const uint64_t i = base + __lane_id();
-    if (i >= item.len || f) return 0;
-    const uint32_t k = item.keys[i];
+    if (i >= n || f) return 0;
+    const uint32_t k = keys[i];
     if (reserved_key(k)) return 0;
     uint32_t got = 0;
     for (uint64_t j = 0; j < np; ++j) {
         const uint64_t e = __hip_atomic_load(pool.pend + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (key_of(e) != k ||
-            __hip_atomic_load(pool.tags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != item.table)
+        if (key_of(e) != k || __hip_atomic_load(pool.tags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != table)
             continue;
         if constexpr (ERASE) {  // the pool counts as one more bucket: clear every copy in it
             uint64_t exp = e;
             got |= (uint32_t)cas_slot(pool.pend + j, exp, 0ull);
         } else {
-            if (item.out_vals) item.out_vals[i] = val_of(e);
+            if (out_vals) out_vals[i] = val_of(e);
             got = 1;
             break;
         }
     }
-    if (got && item.flags) item.flags[i] = 1;
+    if (got && flags) flags[i] = 1;
     return got;
+}
+
+template <bool ERASE>
+__device__ inline uint32_t pool_probe(const Pool &pool, uint64_t np, const ServeItem &item, uint64_t base, uint32_t f)
+{
+    return pool_probe_at<ERASE>(pool, np, item.keys, item.out_vals, item.flags, item.len, item.table, base, f);
 }
 
 // One item of finds or erases; returns this lane's count of hits (erase: deletions).
@@ -822,11 +844,6 @@ __device__ inline void serve_insert_item(const Desc &D, const ServeItem &item, C
     if (__lane_id() == 0 && item.len > rej) atomicAdd(&ins_total[item.table], item.len - rej);
 }
 
-__device__ inline uint64_t pool_size(const Pool &pool)
-{
-    const unsigned long long pc = __hip_atomic_load(pool.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return pc < pool.cap ? pc : pool.cap;
-}
 
 template <int G, bool ERASE>
 __global__ __launch_bounds__(256) void serve_probe_kernel(const Desc *__restrict__ dir,
@@ -883,7 +900,8 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
                                                          TickStages stages, InsertPolicy P, Counters *__restrict__ C,
                                                          PendSink sink, Pool pool,
                                                          unsigned long long *__restrict__ ins_total,
-                                                         unsigned long long *__restrict__ del_total, uint32_t spin_cap)
+                                                         unsigned long long *__restrict__ del_total, uint32_t spin_cap,
+                                                         uint32_t max_back)
 {
     Carry<kSlots / GI> c;
     c.active = false;
@@ -902,7 +920,7 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
                 const uint32_t *flag = stages.ready + (uint64_t)ti.stage * kReadyStride;
                 while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
                     for (uint32_t b = 0; b < back; ++b) __builtin_amdgcn_s_sleep(4);
-                    back = back < 16 ? back * 2 : 16;
+                    back = back < max_back ? back * 2 : max_back;
                     if (++spins > spin_cap) {
                         ok = 0;
                         break;
@@ -914,11 +932,21 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
         const Desc D = dir[ti.item.table];
         if (ti.op == 0) {
             serve_insert_item<GI>(D, ti.item, c, P, C, sink, tally, ins_total);
-        } else if (ti.op == 1) {
-            serve_probe_item<GF, false, true>(D, ti.item, pool, pool_size(pool));
         } else {
-            unsigned long long nd = wave_sum(serve_probe_item<GF, true, true>(D, ti.item, pool, pool_size(pool)));
-            if (__lane_id() == 0 && nd) atomicAdd(&del_total[ti.item.table], nd);
+            // an item that waited reads what earlier stages of this launch wrote: coherently;
+            // a table's first stage reads what earlier launches wrote: through the caches
+            unsigned long long nd;
+            const uint64_t np = pool_size(pool);
+            if (ti.op == 1)
+                nd = ti.wait ? serve_probe_item<GF, false, true>(D, ti.item, pool, np)
+                             : serve_probe_item<GF, false, false>(D, ti.item, pool, np);
+            else
+                nd = ti.wait ? serve_probe_item<GF, true, true>(D, ti.item, pool, np)
+                             : serve_probe_item<GF, true, false>(D, ti.item, pool, np);
+            if (ti.op == 2) {
+                nd = wave_sum(nd);
+                if (__lane_id() == 0 && nd) atomicAdd(&del_total[ti.item.table], nd);
+            }
         }
         if (__lane_id() == 0) {
             const uint32_t before =
@@ -990,27 +1018,29 @@ constexpr uint64_t kInsertDefaultBlocks = 65536;
 }  // namespace
 
 hipError_t launch_find(const Desc &D, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, int lanes,
-                       LaunchShape sh, hipStream_t st)
+                       LaunchShape sh, hipStream_t st, const Pool *pool, uint32_t table)
 {
     if (n == 0) return hipSuccess;
     const int grid = grid_for((n + kWave - 1) / kWave, sh);
+    const Pool p = pool ? *pool : Pool{};
     switch (lanes) {
-    case 4: probe_kernel<4, false><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, hit, n, nullptr); break;
-    case 16: probe_kernel<16, false><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, hit, n, nullptr); break;
-    default: probe_kernel<8, false><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, hit, n, nullptr); break;
+    case 4: probe_kernel<4, false><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, hit, n, nullptr, p, table); break;
+    case 16: probe_kernel<16, false><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, hit, n, nullptr, p, table); break;
+    default: probe_kernel<8, false><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, hit, n, nullptr, p, table); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, uint64_t n, int lanes, LaunchShape sh,
-                        Counters *C, hipStream_t st)
+                        Counters *C, hipStream_t st, const Pool *pool, uint32_t table)
 {
     if (n == 0) return hipSuccess;
     const int grid = grid_for((n + kWave - 1) / kWave, sh);
+    const Pool p = pool ? *pool : Pool{};
     switch (lanes) {
-    case 4: probe_kernel<4, true><<<grid, sh.block_threads, 0, st>>>(D, keys, nullptr, deleted, n, C); break;
-    case 16: probe_kernel<16, true><<<grid, sh.block_threads, 0, st>>>(D, keys, nullptr, deleted, n, C); break;
-    default: probe_kernel<8, true><<<grid, sh.block_threads, 0, st>>>(D, keys, nullptr, deleted, n, C); break;
+    case 4: probe_kernel<4, true><<<grid, sh.block_threads, 0, st>>>(D, keys, nullptr, deleted, n, C, p, table); break;
+    case 16: probe_kernel<16, true><<<grid, sh.block_threads, 0, st>>>(D, keys, nullptr, deleted, n, C, p, table); break;
+    default: probe_kernel<8, true><<<grid, sh.block_threads, 0, st>>>(D, keys, nullptr, deleted, n, C, p, table); break;
     }
     return hipGetLastError();
 }
@@ -1218,20 +1248,20 @@ static hipError_t tick_by_find_lanes(int grid, int find_lanes, const Desc *dir, 
                                      uint32_t n_items, unsigned long long *cursor, TickStages done,
                                      InsertPolicy pol, Counters *C, PendSink sink, Pool pool,
                                      unsigned long long *ins_total, unsigned long long *del_total, uint32_t spin_cap,
-                                     hipStream_t st)
+                                     uint32_t max_back, hipStream_t st)
 {
     switch (find_lanes) {
     case 4:
         serve_tick_kernel<GI, 4><<<grid, 256, 0, st>>>(dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
-                                                       del_total, spin_cap);
+                                                       del_total, spin_cap, max_back);
         break;
     case 16:
         serve_tick_kernel<GI, 16><<<grid, 256, 0, st>>>(dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
-                                                        del_total, spin_cap);
+                                                        del_total, spin_cap, max_back);
         break;
     default:
         serve_tick_kernel<GI, 8><<<grid, 256, 0, st>>>(dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
-                                                       del_total, spin_cap);
+                                                       del_total, spin_cap, max_back);
         break;
     }
     return hipGetLastError();
@@ -1246,21 +1276,22 @@ bool serve_tick_supported(int insert_lanes, int find_lanes)
 hipError_t launch_serve_tick(const Desc *dir, const TickItem *items, uint32_t n_items, unsigned long long *cursor,
                              TickStages done, InsertPolicy pol, int insert_lanes, int find_lanes, int blocks,
                              Counters *C, PendSink sink, Pool pool, unsigned long long *ins_total,
-                             unsigned long long *del_total, uint32_t spin_cap, hipStream_t st)
+                             unsigned long long *del_total, uint32_t spin_cap, uint32_t max_back, hipStream_t st)
 {
     if (n_items == 0) return hipSuccess;
     if (!serve_tick_supported(insert_lanes, find_lanes)) return hipErrorInvalidValue;
+    if (max_back == 0) max_back = 1;
     const int grid = serve_grid(n_items, blocks);
     switch (insert_lanes) {
     case 2:
         return tick_by_find_lanes<2>(grid, find_lanes, dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
-                                     del_total, spin_cap, st);
+                                     del_total, spin_cap, max_back, st);
     case 4:
         return tick_by_find_lanes<4>(grid, find_lanes, dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
-                                     del_total, spin_cap, st);
+                                     del_total, spin_cap, max_back, st);
     default:
         return tick_by_find_lanes<1>(grid, find_lanes, dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
-                                     del_total, spin_cap, st);
+                                     del_total, spin_cap, max_back, st);
     }
 }
 

@@ -41,12 +41,14 @@ struct InsertPolicy {
     uint32_t max_retries;   // CAS races tolerated per entry before it is pended
 };
 
-// find: vals/hit may be null. lanes in {4, 8, 16}.
+struct Pool;
+// find: vals/hit may be null. lanes in {4, 8, 16}. pool (optional): misses also look in the
+// entries of a serving loop's overflow pool tagged `table`.
 hipError_t launch_find(const Desc &D, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, int lanes,
-                       LaunchShape sh, hipStream_t st);
+                       LaunchShape sh, hipStream_t st, const Pool *pool = nullptr, uint32_t table = 0);
 // erase: deleted may be null; counts into C->deleted.
 hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, uint64_t n, int lanes, LaunchShape sh,
-                        Counters *C, hipStream_t st);
+                        Counters *C, hipStream_t st, const Pool *pool = nullptr, uint32_t table = 0);
 // insert keys/vals (start rule of cuckoo.cu:336-341) or packed entries with a fixed previous
 // table `pre` (kNoTable = use the key-parity start rule).
 hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint64_t n, int lanes,
@@ -126,11 +128,11 @@ struct TickStages {
 
 bool serve_tick_supported(int insert_lanes, int find_lanes);
 // One launch for a whole tick (see serve_tick_kernel). cursor must be 0 at launch. spin_cap
-// bounds each wait for a stage.
+// bounds each wait for a stage; a waiting wave backs off up to max_back x 256 cycles per poll.
 hipError_t launch_serve_tick(const Desc *dir, const TickItem *items, uint32_t n_items, unsigned long long *cursor,
                              TickStages stages, InsertPolicy pol, int insert_lanes, int find_lanes, int blocks,
                              Counters *C, PendSink sink, Pool pool, unsigned long long *ins_total,
-                             unsigned long long *del_total, uint32_t spin_cap, hipStream_t st);
+                             unsigned long long *del_total, uint32_t spin_cap, uint32_t max_back, hipStream_t st);
 hipError_t launch_serve_insert(const Desc *dir, const ServeItem *items, uint32_t n_items, InsertPolicy pol, int lanes,
                                int blocks, Counters *C, PendSink sink, unsigned long long *ins_total, hipStream_t st);
 hipError_t launch_serve_find(const Desc *dir, const ServeItem *items, uint32_t n_items, Pool pool, int lanes,

@@ -46,6 +46,7 @@ constexpr uint32_t kMinItemKeys = 64;
 constexpr uint32_t kMaxItemKeys = 4096;
 constexpr uint32_t kMaxRounds = 64;    // phase-ordered rounds per tick
 constexpr uint32_t kChainItems = 1024;  // items per stage of a table with several stages
+constexpr uint64_t kDirectItems = 256;  // ticks this small read their items from pinned memory
 static_assert(sizeof(ServeItem) % 8 == 0 && sizeof(TickItem) % 8 == 0 && sizeof(Desc) % 8 == 0,
               "staged with 8-byte word copies");
 
@@ -91,6 +92,9 @@ struct Tick {
     Counters *d_rctr = nullptr;            // downsize compaction counters
     hipEvent_t done = nullptr;
     bool busy = false;
+    bool has_snap = false;  // the tick copies the device counters out (it may change counts)
+    bool solo_erase = false;  // one erase through the plain kernel: its deletions are in Counters::deleted
+    uint32_t solo_table = 0;
     uint64_t seq = 0;
     std::vector<uint64_t> reqs;
     std::vector<TickTable> tables;
@@ -324,8 +328,20 @@ void finish(dyc_server *s, uint64_t tk, int status)
 // Retire tick k (its event has fired).
 int retire(dyc_server *s, Tick &k)
 {
-    const Counters &c = *reinterpret_cast<const Counters *>(k.h_snap);
     int status = s->error;
+    if (!k.has_snap) {  // finds only: no count, pool or resize changed
+        for (const TickTable &tt : k.tables) s->t[tt.table].inflight--;
+        k.busy = false;
+        for (uint64_t tk : k.reqs) finish(s, tk, status);
+        k.reqs.clear();
+        k.tables.clear();
+        return DYC_OK;
+    }
+    const Counters &c = *reinterpret_cast<const Counters *>(k.h_snap);
+    if (k.solo_erase && c.deleted > s->deleted_seen) {
+        s->t[k.solo_table].sync_del += c.deleted - s->deleted_seen;
+        s->deleted_seen = c.deleted;
+    }
     if (c.lost) status = sticky(s, fail(DYC_ERR_LOST, "overflow pool full: %llu entries lost", (unsigned long long)c.lost));
     if (c.misplaced > s->misplaced_seen) {
         s->misplaced_seen = c.misplaced;
@@ -380,6 +396,23 @@ void mark(dyc_server *s, uint32_t x)
         s->seen[x] = 1;
         s->touched.push_back(x);
     }
+}
+
+// Close a tick: counters out (if it may change them), completion event, bookkeeping.
+int finish_launch(dyc_server *s, Tick &k, bool *launched)
+{
+    if (k.has_snap) HIP_TRY(launch_copy(s->d_stat, k.h_snap, s->stat_bytes() / 8, nullptr, nullptr, 0, s->st));
+    HIP_TRY(hipEventRecord(k.done, s->st));
+    for (const TickTable &tt : k.tables) {
+        SrvTable &T = s->t[tt.table];
+        T.inflight++;
+        T.inflight_ins += tt.ins;
+    }
+    k.busy = true;
+    k.seq = ++s->tick_seq;
+    s->ss.ticks++;
+    *launched = true;
+    return DYC_OK;
 }
 
 // Admit queued requests into a free tick slot and enqueue it. *launched tells whether a tick
@@ -562,7 +595,36 @@ int build_tick(dyc_server *s, bool *launched, bool *stuck)
     for (uint32_t r = 0; r < rounds; ++r)
         for (uint32_t op = 0; op < 3; ++op) n_items += s->items[r][op].size();
     if (n_items > k.items_cap) return sticky(s, fail(DYC_ERR_INVALID, "tick needs more work items than staged"));
-    const bool one = s->scfg.one_launch && serve_tick_supported((int)s->cfg.insert_lanes, (int)s->cfg.find_lanes);
+    // Alone on an idle server (light load), a request goes out exactly as the synchronous path
+    // launches it -- the plain single-table kernel -- without waiting for it.
+    k.solo_erase = false;
+    if (k.reqs.size() == 1) {
+        const dyc_request &q = s->reqs[k.reqs.front()].r;
+        const SrvTable &T = s->t[q.table];
+        const Pool pool = s->pool();
+        k.has_snap = q.op != DYC_OP_FIND;
+        k.solo_erase = q.op == DYC_OP_ERASE;
+        k.solo_table = q.table;
+        if (dir_up) s->dir_dirty = true;  // the plain kernels take the descriptor by value
+        if (q.op == DYC_OP_INSERT)
+            HIP_TRY(launch_insert(T.D, q.keys, q.vals, q.n, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
+                                  s->pc(), s->sink(q.table, s->ins_total() + q.table), s->st));
+        else if (q.op == DYC_OP_FIND)
+            HIP_TRY(launch_find(T.D, q.keys, q.out_vals, q.out_flags, q.n, (int)s->cfg.find_lanes, s->shape(), s->st,
+                                &pool, q.table));
+        else
+            HIP_TRY(launch_erase(T.D, q.keys, q.out_flags, q.n, (int)s->cfg.find_lanes, s->shape(), s->pc(), s->st,
+                                 &pool, q.table));
+        s->ss.rounds += 1;
+        return finish_launch(s, k, launched);
+    }
+    // one launch pays off when a table needs several rounds; a single round goes out as up to
+    // three plain launches (lower latency when the load is light)
+    const bool one = s->scfg.one_launch && rounds > 1 &&
+                     serve_tick_supported((int)s->cfg.insert_lanes, (int)s->cfg.find_lanes);
+    uint64_t writes = 0;  // insert and erase items
+    for (uint32_t r = 0; r < rounds; ++r) writes += s->items[r][0].size() + s->items[r][2].size();
+    k.has_snap = writes > 0 || resized;
     if (one) {
         // items in (round, op) order; a table's items of one (round, op) form a stage that waits
         // for the table's previous stage in the tick
@@ -614,7 +676,7 @@ int build_tick(dyc_server *s, bool *launched, bool *stuck)
         HIP_TRY(launch_serve_tick(s->d_dir, reinterpret_cast<const TickItem *>(k.d_block + 1), (uint32_t)n_items,
                                   k.d_block, stg, s->policy(), (int)s->cfg.insert_lanes, (int)s->cfg.find_lanes, 0,
                                   s->pc(), s->sink(0), s->pool(), s->ins_total(), s->del_total(), s->scfg.spin_cap,
-                                  s->st));
+                                  s->scfg.poll_backoff, s->st));
     } else {
         uint64_t at = 0;
         for (uint32_t r = 0; r < rounds; ++r)
@@ -623,35 +685,30 @@ int build_tick(dyc_server *s, bool *launched, bool *stuck)
                 memcpy(k.h_items + at, v.data(), v.size() * sizeof(ServeItem));
                 at += v.size();
             }
-        HIP_TRY(launch_copy(k.h_items, k.d_items, n_items * sizeof(ServeItem) / 8, k.h_dir, s->d_dir,
-                            dir_up ? s->n * sizeof(Desc) / 8 : 0, s->st));
+        // a small tick's items are read where the host wrote them (pinned memory)
+        const bool direct = n_items <= kDirectItems;
+        const ServeItem *items = direct ? k.h_items : k.d_items;
+        if (direct) {
+            if (dir_up) HIP_TRY(launch_copy(k.h_dir, s->d_dir, s->n * sizeof(Desc) / 8, nullptr, nullptr, 0, s->st));
+        } else {
+            HIP_TRY(launch_copy(k.h_items, k.d_items, n_items * sizeof(ServeItem) / 8, k.h_dir, s->d_dir,
+                                dir_up ? s->n * sizeof(Desc) / 8 : 0, s->st));
+        }
         at = 0;
         for (uint32_t r = 0; r < rounds; ++r) {
             const uint64_t ni = s->items[r][0].size(), nf = s->items[r][1].size(), ne = s->items[r][2].size();
-            HIP_TRY(launch_serve_insert(s->d_dir, k.d_items + at, (uint32_t)ni, s->policy(), (int)s->cfg.insert_lanes,
-                                        0, s->pc(), s->sink(0), s->ins_total(), s->st));
+            HIP_TRY(launch_serve_insert(s->d_dir, items + at, (uint32_t)ni, s->policy(), (int)s->cfg.insert_lanes, 0,
+                                        s->pc(), s->sink(0), s->ins_total(), s->st));
             at += ni;
-            HIP_TRY(launch_serve_find(s->d_dir, k.d_items + at, (uint32_t)nf, s->pool(), (int)s->cfg.find_lanes, 0,
-                                      s->st));
+            HIP_TRY(launch_serve_find(s->d_dir, items + at, (uint32_t)nf, s->pool(), (int)s->cfg.find_lanes, 0, s->st));
             at += nf;
-            HIP_TRY(launch_serve_erase(s->d_dir, k.d_items + at, (uint32_t)ne, s->pool(), s->del_total(),
+            HIP_TRY(launch_serve_erase(s->d_dir, items + at, (uint32_t)ne, s->pool(), s->del_total(),
                                        (int)s->cfg.find_lanes, 0, s->st));
             at += ne;
         }
     }
     s->ss.rounds += rounds;
-    HIP_TRY(launch_copy(s->d_stat, k.h_snap, s->stat_bytes() / 8, nullptr, nullptr, 0, s->st));
-    HIP_TRY(hipEventRecord(k.done, s->st));
-    for (const TickTable &tt : k.tables) {
-        SrvTable &T = s->t[tt.table];
-        T.inflight++;
-        T.inflight_ins += tt.ins;
-    }
-    k.busy = true;
-    k.seq = ++s->tick_seq;
-    s->ss.ticks++;
-    *launched = true;
-    return DYC_OK;
+    return finish_launch(s, k, launched);
 }
 
 // One non-blocking step: retire what finished, then fill the free slots.
@@ -735,34 +792,18 @@ int run_sync(dyc_server *s, uint64_t tk)
         if (status == DYC_OK)
             HIP_TRY(launch_insert(T.D, r.keys, r.vals, r.n, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
                                   s->pc(), s->sink(r.table, s->ins_total() + r.table), s->st));
-    } else if (s->pool_settled > 0) {
-        // the pool holds entries: take the path that searches it
-        const uint64_t items = (r.n + kMaxItemKeys - 1) / kMaxItemKeys;
-        Tick &k = s->ticks[0];
-        if (items > k.items_cap) return fail(DYC_ERR_INVALID, "request too large for the staged items");
-        for (uint64_t i = 0; i < items; ++i) {
-            const uint64_t o = i * kMaxItemKeys;
-            ServeItem it{};
-            it.keys = r.keys + o;
-            it.out_vals = (r.op == DYC_OP_FIND && r.out_vals) ? r.out_vals + o : nullptr;
-            it.flags = r.out_flags ? r.out_flags + o : nullptr;
-            it.len = (uint32_t)std::min<uint64_t>(kMaxItemKeys, r.n - o);
-            it.table = r.table;
-            k.h_items[i] = it;
-        }
-        for (uint32_t i = 0; i < s->n; ++i) k.h_dir[i] = s->t[i].D;
-        HIP_TRY(hipMemcpyAsync(s->d_dir, k.h_dir, s->n * sizeof(Desc), hipMemcpyHostToDevice, s->st));
-        s->dir_dirty = false;
-        HIP_TRY(hipMemcpyAsync(k.d_items, k.h_items, items * sizeof(ServeItem), hipMemcpyHostToDevice, s->st));
-        if (r.op == DYC_OP_FIND)
-            HIP_TRY(launch_serve_find(s->d_dir, k.d_items, (uint32_t)items, s->pool(), (int)s->cfg.find_lanes, 0, s->st));
-        else
-            HIP_TRY(launch_serve_erase(s->d_dir, k.d_items, (uint32_t)items, s->pool(), s->del_total(),
-                                       (int)s->cfg.find_lanes, 0, s->st));
     } else if (r.op == DYC_OP_FIND) {
-        HIP_TRY(launch_find(T.D, r.keys, r.out_vals, r.out_flags, r.n, (int)s->cfg.find_lanes, s->shape(), s->st));
+        // a find changes nothing: launch, wait, done
+        const Pool pool = s->pool();
+        HIP_TRY(launch_find(T.D, r.keys, r.out_vals, r.out_flags, r.n, (int)s->cfg.find_lanes, s->shape(), s->st,
+                            &pool, r.table));
+        HIP_TRY(hipStreamSynchronize(s->st));
+        finish(s, tk, status);
+        return DYC_OK;
     } else {
-        HIP_TRY(launch_erase(T.D, r.keys, r.out_flags, r.n, (int)s->cfg.find_lanes, s->shape(), s->pc(), s->st));
+        const Pool pool = s->pool();
+        HIP_TRY(launch_erase(T.D, r.keys, r.out_flags, r.n, (int)s->cfg.find_lanes, s->shape(), s->pc(), s->st,
+                             &pool, r.table));
     }
     int rc = read_stats_sync(s, snap);
     if (rc) return rc;
@@ -924,6 +965,7 @@ int dyc_server_config_init(dyc_server_config *scfg)
     scfg->pool_cap = 65536;
     scfg->one_launch = 1;
     scfg->spin_cap = 1u << 24;
+    scfg->poll_backoff = 16;
     return DYC_OK;
 }
 

@@ -29,7 +29,7 @@ def _ptr(t):
 class DyCuckooServer:
     def __init__(self, capacity_hints, *, hbm_budget: int = 0, tick_max_keys: int = 1 << 23,
                  tick_max_requests: int = 4096, pipeline_depth: int = 2, item_keys: int = 0, pool_cap: int = 65536,
-                 one_launch: int = 1, spin_cap: int = 1 << 24, **table_config):
+                 one_launch: int = 1, spin_cap: int = 1 << 24, poll_backoff: int = 16, **table_config):
         self.lib = _lib.load()
         hints = np.ascontiguousarray(np.asarray(capacity_hints, dtype=np.uint64))
         cfg = _lib.default_config(**table_config)
@@ -37,7 +37,7 @@ class DyCuckooServer:
         check(self.lib.dyc_server_config_init(ctypes.byref(sc)), "dyc_server_config_init")
         sc.hbm_budget, sc.tick_max_keys, sc.tick_max_requests = hbm_budget, tick_max_keys, tick_max_requests
         sc.pipeline_depth, sc.item_keys, sc.pool_cap = pipeline_depth, item_keys, pool_cap
-        sc.one_launch, sc.spin_cap = one_launch, spin_cap
+        sc.one_launch, sc.spin_cap, sc.poll_backoff = one_launch, spin_cap, poll_backoff
         h = ctypes.c_void_p()
         check(self.lib.dyc_server_create(ctypes.byref(cfg), ctypes.byref(sc), hints.size, hints.ctypes.data,
                                          ctypes.byref(h)), "dyc_server_create")

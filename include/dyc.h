@@ -271,6 +271,9 @@ typedef struct dyc_server_config {
                                     insert_lanes 1, 2 or 4 (otherwise rounds are launched) */
     uint32_t spin_cap;           /* polls an item waits for its table before the tick is failed
                                     (default 2^24) */
+    uint32_t poll_backoff;       /* a waiting wave sleeps up to this many x 256 cycles between polls
+                                    (default 16) */
+    uint32_t pad;
 } dyc_server_config;
 
 typedef struct dyc_server_stats {

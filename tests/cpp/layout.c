@@ -15,7 +15,7 @@ int main(void)
            offsetof(dyc_request, vals), offsetof(dyc_request, out_flags), offsetof(dyc_request, n));
     printf("server_config %zu %zu %zu %zu %zu\n", sizeof(dyc_server_config),
            offsetof(dyc_server_config, tick_max_requests), offsetof(dyc_server_config, item_keys),
-           offsetof(dyc_server_config, pool_cap), offsetof(dyc_server_config, spin_cap));
+           offsetof(dyc_server_config, pool_cap), offsetof(dyc_server_config, poll_backoff));
     printf("server_stats %zu %zu %zu\n", sizeof(dyc_server_stats), offsetof(dyc_server_stats, deferred_memory),
            offsetof(dyc_server_stats, in_flight));
     return 0;

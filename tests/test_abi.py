@@ -91,7 +91,7 @@ def test_struct_layout_matches_c(lib, tmp_path):
     assert np.dtype(L.REQUEST_DTYPE).itemsize == r[0]
     sc = [int(x) for x in out[3].split()[1:]]
     assert sc == [ctypes.sizeof(L.ServerConfig), L.ServerConfig.tick_max_requests.offset,
-                  L.ServerConfig.item_keys.offset, L.ServerConfig.pool_cap.offset, L.ServerConfig.spin_cap.offset]
+                  L.ServerConfig.item_keys.offset, L.ServerConfig.pool_cap.offset, L.ServerConfig.poll_backoff.offset]
     ss = [int(x) for x in out[4].split()[1:]]
     assert ss == [ctypes.sizeof(L.ServerStats), L.ServerStats.deferred_memory.offset, L.ServerStats.in_flight.offset]
 
