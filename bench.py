@@ -61,11 +61,12 @@ def parse():
     ap.add_argument("--cpu-seq-log2n", type=int, default=22, help="sequential CPU sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
-    ap.add_argument("--workload", choices=("bulk", "dynamic", "batched", "serving"), default="bulk",
+    ap.add_argument("--workload", choices=("bulk", "dynamic", "batched", "serving", "resize"), default="bulk",
                     help="bulk: the large bulk configuration (default, the headline line); dynamic: the "
                          "resize stream (growth then shrink batches under fill bounds); batched: many uneven "
                          "tables in one persistent launch vs one launch per table; serving: the request loop "
-                         "(continuous batching vs synchronous per-request launches) over a load sweep")
+                         "(continuous batching vs synchronous per-request launches) over a load sweep; resize: "
+                         "single-subtable upsize and downsize of a 2^28-slot-class table")
     ap.add_argument("--sv-requests", type=int, default=4096, help="serving: requests per run")
     ap.add_argument("--sv-loads", default="0.25,0.5,1,2,4,8,16,0",
                     help="serving: offered loads in G keys/s (0 = every request arrives at once)")
@@ -244,6 +245,74 @@ def run_dynamic(args, dev) -> int:
         with open(args.json_out, "w") as f:
             f.write(line + "\n")
     return 0 if hits_ok else 1
+
+
+def run_resize(args, dev) -> int:
+    """Single-subtable resize at scale: a table sized for 2^log2n keys at load factor theta,
+    filled to 45 %, then alternately upsized (subtable chosen by the reference rule doubles) and
+    downsized back, steps times each. The upsize kernel reads the old subtable and writes both
+    halves of the new one (24 B per old slot); the downsize kernel reads both halves of the old
+    subtable and writes the new one (24 B per new slot), then residuals are re-inserted. Times
+    are the API calls (allocation, kernel, counter read) with HIP events; the kernels alone are
+    in the rocprofv3 trace of this command. Prints one JSON line."""
+    import torch
+
+    import databases_gpu_amd as D
+    from databases_gpu_amd import workload as W
+
+    stream = torch.cuda.current_stream()
+    n = 1 << args.log2n
+    hint = int(math.ceil(n / args.theta))
+    t = D.DyCuckoo(hint)
+    fill = int(0.45 * t.capacity)
+    keys = W.keys_range(0, fill, args.seed, dev)
+    t.insert(keys, W.splitmix32(keys))
+    torch.cuda.synchronize()
+    log(f"resize: buckets {t.buckets}, {fill} keys")
+    ups, downs, up_bytes, down_bytes = [], [], [], []
+    for s_ in range(args.warmup + args.steps):
+        nb0 = t.buckets
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        t.resize_up()
+        e[1].record(stream)
+        nb1 = t.buckets
+        t.resize_down()
+        e[2].record(stream)
+        e[2].synchronize()
+        nb2 = t.buckets
+        grown = [i for i in range(4) if nb1[i] != nb0[i]][0]
+        shrunk = [i for i in range(4) if nb2[i] != nb1[i]][0]
+        if s_ >= args.warmup:
+            ups.append(e[0].elapsed_time(e[1]))
+            downs.append(e[1].elapsed_time(e[2]))
+            up_bytes.append(24 * 16 * nb0[grown])
+            down_bytes.append(24 * 16 * nb2[shrunk])
+    st = t.stats()
+    v, h = t.find(keys)
+    ok = bool(h.bool().all()) and bool((v == W.splitmix32(keys)).all()) and st["count"] == fill and st["pending"] == 0
+    up_ms, down_ms = float(np.median(ups)), float(np.median(downs))
+    result = {
+        "metric": "single-subtable upsize / downsize, GB/s of algorithmic bytes per call",
+        "value": round(float(np.median(up_bytes)) / (up_ms * 1e-3) / 1e9, 1),
+        "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"table sized for 2^{args.log2n} keys at load factor {args.theta}, filled to 45 %",
+                   "buckets": st["buckets"], "keys": fill},
+        "upsize": {"ms": round(up_ms, 3), "bytes": int(np.median(up_bytes)),
+                   "gbs": round(float(np.median(up_bytes)) / (up_ms * 1e-3) / 1e9, 1)},
+        "downsize": {"ms": round(down_ms, 3), "bytes": int(np.median(down_bytes)),
+                     "gbs": round(float(np.median(down_bytes)) / (down_ms * 1e-3) / 1e9, 1)},
+        "resizes_up": st["resizes_up"], "resizes_down": st["resizes_down"],
+        "parity": "ok" if ok else "mismatch",
+    }
+    t.close()
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0 if ok else 1
 
 
 def run_serving(args, dev) -> int:
@@ -488,6 +557,8 @@ def main() -> int:
         return run_batched(args, dev)
     if args.workload == "serving":
         return run_serving(args, dev)
+    if args.workload == "resize":
+        return run_resize(args, dev)
     n = 1 << args.log2n
     hint = int(math.ceil(n / args.theta))
     log(f"generating inputs: n=2^{args.log2n}, theta={args.theta}")
