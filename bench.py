@@ -5,9 +5,11 @@ Workload (the large bulk configuration): 4 subtables sized for 2^28 unique uint3
 factor 0.90 (about 2.4 GB of buckets), values splitmix32(key). One step =
   clear the table; bulk insert 2^28 (key, value) pairs; 2^28 finds at 50% hit with uniformly
   drawn lookups; 2^28 finds drawn Zipf(0.99) over the inserted keys.
-All inputs are generated on the device before timing (resident in HBM). Every phase is timed with
-HIP events on the stream the library launches on. `value` is the step's operations (3 * 2^28)
-per second of step time, in Mops/s.
+All inputs are generated on the device before timing (resident in HBM). Before the steps the
+launch shapes and the eviction bound are tuned on the batch (untimed; --no-tune skips it,
+--reference-kick keeps the reference bound). Every phase is timed with HIP events on the stream
+the library launches on. `value` is the step's operations (3 * 2^28) per second of step time, in
+Mops/s.
 
 Roofline: the algorithmic bytes per operation follow the reference algorithm and layout
 (128-byte buckets, pos1 then pos2): find = 4 (key) + 4 (value) + 1 (hit flag) + 128 * p, with
@@ -61,12 +63,14 @@ def parse():
     ap.add_argument("--cpu-seq-log2n", type=int, default=22, help="sequential CPU sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
-    ap.add_argument("--workload", choices=("bulk", "dynamic", "batched", "serving", "resize"), default="bulk",
+    ap.add_argument("--workload", choices=("bulk", "dynamic", "batched", "serving", "resize", "memsys"),
+                    default="bulk",
                     help="bulk: the large bulk configuration (default, the headline line); dynamic: the "
                          "resize stream (growth then shrink batches under fill bounds); batched: many uneven "
                          "tables in one persistent launch vs one launch per table; serving: the request loop "
                          "(continuous batching vs synchronous per-request launches) over a load sweep; resize: "
-                         "single-subtable upsize and downsize of a 2^28-slot-class table")
+                         "single-subtable upsize and downsize of a 2^28-slot-class table; memsys: the device's "
+                         "random bucket-read, CAS and store rates at the table's size (the kernels' ceilings)")
     ap.add_argument("--sv-requests", type=int, default=4096, help="serving: requests per run")
     ap.add_argument("--sv-loads", default="0.25,0.5,1,2,4,8,16,0",
                     help="serving: offered loads in G keys/s (0 = every request arrives at once)")
@@ -83,9 +87,10 @@ def parse():
     ap.add_argument("--bt-theta", type=float, default=0.85, help="batched: per-table load factor")
     ap.add_argument("--bt-hit", type=float, default=0.6, help="batched: find hit rate")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
-    ap.add_argument("--tune-kick", action="store_true",
-                    help="bulk: let the autotuner also try eviction bound 0 (placement differs from the "
-                         "reference rule, results do not)")
+    ap.add_argument("--reference-kick", action="store_true",
+                    help="bulk: keep the reference eviction bound (15 iterations of its victim rule); by default "
+                         "the autotuner also tries bound 0 (the rescue walk from the first kick), which changes "
+                         "placement only -- stored sets and find results are the same")
     ap.add_argument("--dyn-batches", type=int, default=200, help="dynamic: batches (half grow, half shrink)")
     ap.add_argument("--dyn-log2ops", type=int, default=22, help="dynamic: 2^k operations per batch")
     ap.add_argument("--dyn-log2init", type=int, default=22, help="dynamic: 2^k keys live before the stream")
@@ -245,6 +250,39 @@ def run_dynamic(args, dev) -> int:
         with open(args.json_out, "w") as f:
             f.write(line + "\n")
     return 0 if hits_ok else 1
+
+
+def run_memsys(args, dev) -> int:
+    """The device's random-access ceilings at the size of the 2^28-key table (2.39 GB): random
+    128-byte bucket reads/s (one lane per bucket), random 64-bit CAS/s and random 8-byte
+    stores/s, 2^28 operations each (libdyc_memsys.so). Prints one JSON line with the ceilings
+    the bulk kernels are compared against."""
+    import ctypes
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "databases_gpu_amd", "libdyc_memsys.so"))
+    lib.dyc_memsys_probe.restype = ctypes.c_int
+    lib.dyc_memsys_probe.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    nb = 4 * 4660350  # buckets of the 2^28-key table at load factor 0.90
+    out = (ctypes.c_double * 3)()
+    rc = lib.dyc_memsys_probe(nb * 128, 1 << args.log2n, 3, out)
+    if rc:
+        print(f"dyc_memsys_probe failed: hipError {rc}", file=sys.stderr)
+        return 1
+    reads, cas, stores = out[0], out[1], out[2]
+    result = {
+        "metric": "device random-access ceilings at 2.39 GB: bucket reads/s, 64-bit CAS/s, 8-byte stores/s",
+        "value": round(reads / 1e9, 2), "unit": "G bucket reads/s", "n_gpus": 1, "higher_is_better": True,
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"random slots of a {nb * 128} B buffer, 2^{args.log2n} operations per pattern"},
+        "bucket_reads_per_s": reads, "bucket_read_gbs": round(reads * 128 / 1e9, 1),
+        "cas_per_s": cas, "stores_per_s": stores,
+    }
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0
 
 
 def run_resize(args, dev) -> int:
@@ -559,6 +597,8 @@ def main() -> int:
         return run_serving(args, dev)
     if args.workload == "resize":
         return run_resize(args, dev)
+    if args.workload == "memsys":
+        return run_memsys(args, dev)
     n = 1 << args.log2n
     hint = int(math.ceil(n / args.theta))
     log(f"generating inputs: n=2^{args.log2n}, theta={args.theta}")
@@ -581,7 +621,7 @@ def main() -> int:
     if not args.no_tune:
         from databases_gpu_amd.autotune import tune_table
 
-        tuned = tune_table(t, keys, vals, look_u, log=log, kick_bounds=(15, 0) if args.tune_kick else (15,))
+        tuned = tune_table(t, keys, vals, look_u, log=log, kick_bounds=(15,) if args.reference_kick else (15, 0))
         log(f"autotune installed {tuned['installed']} ({tuned['seconds']} s)")
     cfg = t.config()
 
@@ -668,6 +708,23 @@ def main() -> int:
             for k in kernels:
                 if k in tr and tr[k].get("log2n") == args.log2n:
                     kernels[k]["traffic"] = tr[k]["bytes_per_launch"]
+        except (OSError, ValueError, KeyError):
+            pass
+    # the device's random-access ceilings at this table size (bench.py --workload memsys)
+    memsys_path = os.path.join(ROOT, "profiles", "memsys.json")
+    if os.path.exists(memsys_path):
+        try:
+            ms_ = json.load(open(memsys_path))
+            ex = i_bar * n / (med("insert") * 1e-3)
+            kernels["insert"]["ceiling"] = {"kind": "random 64-bit CAS/s", "rate": round(ms_["cas_per_s"] / 1e9, 2),
+                                            "achieved": round(ex / 1e9, 2), "frac": round(ex / ms_["cas_per_s"], 3),
+                                            "unit": "G/s"}
+            for k, p_ in (("find_uniform", p_u), ("find_zipf", p_z)):
+                rd = p_ * n / (med(k) * 1e-3)
+                kernels[k]["ceiling"] = {"kind": "random 128-B bucket reads/s",
+                                         "rate": round(ms_["bucket_reads_per_s"] / 1e9, 2),
+                                         "achieved": round(rd / 1e9, 2), "frac": round(rd / ms_["bucket_reads_per_s"], 3),
+                                         "unit": "G/s"}
         except (OSError, ValueError, KeyError):
             pass
     dominant = max(("insert", "find_uniform", "find_zipf"), key=med)
