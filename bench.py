@@ -263,19 +263,19 @@ def run_memsys(args, dev) -> int:
     lib.dyc_memsys_probe.restype = ctypes.c_int
     lib.dyc_memsys_probe.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     nb = 4 * 4660350  # buckets of the 2^28-key table at load factor 0.90
-    out = (ctypes.c_double * 3)()
+    out = (ctypes.c_double * 5)()
     rc = lib.dyc_memsys_probe(nb * 128, 1 << args.log2n, 3, out)
     if rc:
         print(f"dyc_memsys_probe failed: hipError {rc}", file=sys.stderr)
         return 1
-    reads, cas, stores = out[0], out[1], out[2]
+    reads, cas, stores, half, read_cas = out[0], out[1], out[2], out[3], out[4]
     result = {
         "metric": "device random-access ceilings at 2.39 GB: bucket reads/s, 64-bit CAS/s, 8-byte stores/s",
         "value": round(reads / 1e9, 2), "unit": "G bucket reads/s", "n_gpus": 1, "higher_is_better": True,
         "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"random slots of a {nb * 128} B buffer, 2^{args.log2n} operations per pattern"},
         "bucket_reads_per_s": reads, "bucket_read_gbs": round(reads * 128 / 1e9, 1),
-        "cas_per_s": cas, "stores_per_s": stores,
+        "cas_per_s": cas, "stores_per_s": stores, "half_bucket_reads_per_s": half, "read_then_cas_per_s": read_cas,
     }
     line = json.dumps(result)
     print(line, flush=True)

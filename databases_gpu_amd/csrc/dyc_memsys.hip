@@ -47,6 +47,45 @@ __global__ void cas_kernel(unsigned long long *__restrict__ buf, uint64_t nslots
     if (acc == 0x9e3779b97f4a7c15ull) sink[0] = acc;
 }
 
+// one lane reads the first 64 bytes of a random bucket (4 x 16 B) per op
+__global__ void read_half_kernel(const uint4 *__restrict__ buf, uint64_t nb, uint64_t ops, uint64_t *sink)
+{
+    uint64_t acc = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ops; i += stride) {
+        const uint4 *p = buf + (uint64_t)(mix((uint32_t)i * 2654435761u + 17u) % nb) * 8;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint4 v = p[j];
+            acc += v.x ^ v.w;
+        }
+    }
+    if (acc == 0x9e3779b97f4a7c15ull) sink[0] = acc;
+}
+
+// read a random bucket, then CAS one of its slots from the value read (an insert exchange)
+__global__ void read_cas_kernel(unsigned long long *__restrict__ buf, uint64_t nb, uint64_t ops, uint64_t *sink)
+{
+    uint64_t acc = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ops; i += stride) {
+        const uint64_t b = mix((uint32_t)i * 2654435761u + 17u) % nb;
+        const uint4 *p = reinterpret_cast<const uint4 *>(buf + b * 16);
+        uint32_t h = 0;
+        unsigned long long first = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint4 v = p[j];
+            h += v.x ^ v.z;
+            if (j == 0) first = ((unsigned long long)v.y << 32) | v.x;
+        }
+        unsigned long long exp = first;
+        acc += __hip_atomic_compare_exchange_strong(buf + b * 16 + (h & 0u), &exp, exp + 1, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (acc == 0x9e3779b97f4a7c15ull) sink[0] = acc;
+}
+
 // one plain 8-byte store to a random slot per op
 __global__ void store_kernel(unsigned long long *__restrict__ buf, uint64_t nslots, uint64_t ops)
 {
@@ -60,7 +99,8 @@ __global__ void store_kernel(unsigned long long *__restrict__ buf, uint64_t nslo
 extern "C" {
 
 // Rates over a zeroed device buffer of `bytes` (rounded down to whole 128-byte buckets), `ops`
-// operations per pattern, best of `reps`: out[0] bucket reads/s, out[1] CAS/s, out[2] stores/s.
+// operations per pattern, best of `reps`: out[0] bucket reads/s, out[1] CAS/s, out[2] stores/s,
+// out[3] half-bucket (64 B) reads/s, out[4] read-a-bucket-then-CAS-a-slot/s.
 // Returns 0 or a hipError_t code.
 int dyc_memsys_probe(uint64_t bytes, uint64_t ops, int reps, double *out)
 {
@@ -78,13 +118,15 @@ int dyc_memsys_probe(uint64_t bytes, uint64_t ops, int reps, double *out)
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     const int grid = 16384;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 5; ++k) {
         float best = 1e30f;
         for (int r = 0; r < reps + 1; ++r) {  // the first run warms up
             (void)hipEventRecord(a, 0);
             if (k == 0) read_bucket_kernel<<<grid, 256>>>(reinterpret_cast<const uint4 *>(buf), nb, ops, sink);
             else if (k == 1) cas_kernel<<<grid, 256>>>(buf, nslots, ops, sink);
-            else store_kernel<<<grid, 256>>>(buf, nslots, ops);
+            else if (k == 2) store_kernel<<<grid, 256>>>(buf, nslots, ops);
+            else if (k == 3) read_half_kernel<<<grid, 256>>>(reinterpret_cast<const uint4 *>(buf), nb, ops, sink);
+            else read_cas_kernel<<<grid, 256>>>(buf, nb, ops, sink);
             (void)hipEventRecord(b, 0);
             if ((e = hipEventSynchronize(b)) != hipSuccess) break;
             float ms = 0;
