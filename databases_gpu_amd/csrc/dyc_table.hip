@@ -7,10 +7,13 @@
 #include "dyc_kernels.hpp"
 
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <thread>
+#include <vector>
 
 using namespace dyc;
 using dyc::host::choose_down;
@@ -60,6 +63,10 @@ uint32_t initial_buckets(uint64_t hint)
 }  // namespace host
 }  // namespace dyc
 
+// Pinned and device buffers of the host-pointer entry points (two of each, for double
+// buffering), and the copy streams that move them (allocated on first use).
+struct DycHostStage;
+
 struct dyc_table {
     dyc_config cfg;
     int device = 0;
@@ -73,6 +80,7 @@ struct dyc_table {
     uint64_t *part = nullptr;   // insert batch ordered by bucket region
     uint64_t part_cap = 0;
     unsigned long long *part_counts = nullptr;   // kMaxRegions counts, then kMaxRegions cursors
+    DycHostStage *stage = nullptr;
     dyc_stats st{};
 
     uint64_t capacity() const
@@ -320,6 +328,200 @@ int check_config(const dyc_config &cfg)
 
 }  // namespace
 
+struct DycHostStage {
+    uint32_t *hk[2] = {}, *hv[2] = {};  // pinned: keys; values in (insert) or out (find)
+    uint8_t *hf[2] = {};                // pinned: flags out
+    uint32_t *dk[2] = {}, *dv[2] = {};
+    uint8_t *df[2] = {};
+    hipStream_t h2d = nullptr, d2h = nullptr;
+    hipEvent_t in_ready[2] = {}, done[2] = {}, out_ready[2] = {};
+};
+
+namespace {
+
+constexpr uint64_t kStageKeys = 1ull << 21;  // keys per pipeline chunk
+
+void free_stage(dyc_table *t)
+{
+    DycHostStage *S = t->stage;
+    if (!S) return;
+    for (int b = 0; b < 2; ++b) {
+        if (S->hk[b]) (void)hipHostFree(S->hk[b]);
+        if (S->hv[b]) (void)hipHostFree(S->hv[b]);
+        if (S->hf[b]) (void)hipHostFree(S->hf[b]);
+        if (S->dk[b]) (void)hipFree(S->dk[b]);
+        if (S->dv[b]) (void)hipFree(S->dv[b]);
+        if (S->df[b]) (void)hipFree(S->df[b]);
+        if (S->in_ready[b]) (void)hipEventDestroy(S->in_ready[b]);
+        if (S->done[b]) (void)hipEventDestroy(S->done[b]);
+        if (S->out_ready[b]) (void)hipEventDestroy(S->out_ready[b]);
+    }
+    if (S->h2d) (void)hipStreamDestroy(S->h2d);
+    if (S->d2h) (void)hipStreamDestroy(S->d2h);
+    delete S;
+    t->stage = nullptr;
+}
+
+int get_stage(dyc_table *t, DycHostStage **out)
+{
+    if (!t->stage) {
+        DycHostStage *S = new DycHostStage();
+        t->stage = S;
+        bool ok = hipStreamCreateWithFlags(&S->h2d, hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&S->d2h, hipStreamNonBlocking) == hipSuccess;
+        for (int b = 0; b < 2 && ok; ++b) {
+            ok = hipHostMalloc((void **)&S->hk[b], kStageKeys * 4, hipHostMallocDefault) == hipSuccess &&
+                 hipHostMalloc((void **)&S->hv[b], kStageKeys * 4, hipHostMallocDefault) == hipSuccess &&
+                 hipHostMalloc((void **)&S->hf[b], kStageKeys, hipHostMallocDefault) == hipSuccess &&
+                 hipMalloc(&S->dk[b], kStageKeys * 4) == hipSuccess && hipMalloc(&S->dv[b], kStageKeys * 4) == hipSuccess &&
+                 hipMalloc(&S->df[b], kStageKeys) == hipSuccess &&
+                 hipEventCreateWithFlags(&S->in_ready[b], hipEventDisableTiming) == hipSuccess &&
+                 hipEventCreateWithFlags(&S->done[b], hipEventDisableTiming) == hipSuccess &&
+                 hipEventCreateWithFlags(&S->out_ready[b], hipEventDisableTiming) == hipSuccess;
+        }
+        if (!ok) {
+            (void)hipGetLastError();
+            free_stage(t);
+            return fail(DYC_ERR_OOM, "staging buffers for host-pointer calls could not be allocated");
+        }
+    }
+    *out = t->stage;
+    return DYC_OK;
+}
+
+// memcpy of large host ranges on up to 8 threads (pageable <-> pinned staging)
+void host_copy(void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t kSplit = 4u << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = bytes < kSplit ? 1u : std::min(8u, hw);
+    if (nt == 1) {
+        memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t per = ((bytes + nt - 1) / nt + 63) & ~(size_t)63;
+    std::vector<std::thread> th;
+    for (unsigned i = 1; i < nt && i * per < bytes; ++i) {
+        const size_t lo = i * per, len = std::min(per, bytes - lo);
+        th.emplace_back([=] { memcpy(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo, len); });
+    }
+    memcpy(dst, src, std::min(per, bytes));
+    for (auto &x : th) x.join();
+}
+
+// ---- checkpoint file: a 128-byte header, the four subtable images, then the pending entries,
+// all little-endian; FNV-1a (64-bit, over 8-byte words) guards each part and the header.
+struct CkptHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t subtables;
+    uint32_t buckets[4];
+    uint64_t count;
+    uint64_t pending;
+    uint64_t checksum[5];  // subtables 0..3, then the pending list
+    uint64_t header_check;  // over every byte before this field
+    uint8_t reserved[32];
+};
+static_assert(sizeof(CkptHeader) == 128, "checkpoint header is 128 bytes");
+constexpr char kCkptMagic[8] = {'D', 'Y', 'C', 'K', 'P', 'T', '0', '1'};
+constexpr uint64_t kFnvBasis = 0xcbf29ce484222325ull, kFnvPrime = 0x100000001b3ull;
+constexpr uint64_t kCkptChunk = 1ull << 23;  // 8-byte words per staged chunk (64 MiB)
+
+uint64_t fnv_words(uint64_t h, const uint64_t *w, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i) h = (h ^ w[i]) * kFnvPrime;
+    return h;
+}
+
+uint64_t fnv_bytes(const void *p, size_t n)
+{
+    uint64_t h = kFnvBasis;
+    for (size_t i = 0; i < n; ++i) h = (h ^ static_cast<const uint8_t *>(p)[i]) * kFnvPrime;
+    return h;
+}
+
+struct PinnedPair {
+    uint64_t *buf[2] = {nullptr, nullptr};
+    ~PinnedPair()
+    {
+        for (auto *b : buf)
+            if (b) (void)hipHostFree(b);
+    }
+    bool alloc(size_t bytes)
+    {
+        return hipHostMalloc((void **)&buf[0], bytes, hipHostMallocDefault) == hipSuccess &&
+               hipHostMalloc((void **)&buf[1], bytes, hipHostMallocDefault) == hipSuccess;
+    }
+};
+
+// Stream `words` device words to the file through two pinned buffers (copy of chunk i+1
+// overlaps the write of chunk i); returns the FNV-1a of the words.
+int dump_to_file(FILE *f, const uint64_t *dev, uint64_t words, PinnedPair &pp, uint64_t *hash, hipStream_t s)
+{
+    uint64_t h = kFnvBasis;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int rc = DYC_OK;
+    uint64_t off = 0, prev_n = 0;
+    int b = 0, prev_b = 0;
+    while (rc == DYC_OK) {
+        const uint64_t m = std::min(kCkptChunk, words - off);
+        if (m && (hipMemcpyAsync(pp.buf[b], dev + off, m * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                  hipEventRecord(ev[b], s) != hipSuccess))
+            rc = fail(DYC_ERR_HIP, "checkpoint copy failed");
+        if (prev_n && rc == DYC_OK) {  // write the previous chunk while this one copies
+            if (hipEventSynchronize(ev[prev_b]) != hipSuccess) {
+                rc = fail(DYC_ERR_HIP, "checkpoint copy failed");
+                break;
+            }
+            h = fnv_words(h, pp.buf[prev_b], prev_n);
+            if (fwrite(pp.buf[prev_b], 8, prev_n, f) != prev_n) rc = fail(DYC_ERR_INVALID, "checkpoint write failed");
+        }
+        if (!m) break;
+        prev_n = m;
+        prev_b = b;
+        b ^= 1;
+        off += m;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && rc == DYC_OK) rc = fail(DYC_ERR_HIP, "checkpoint copy failed");
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    *hash = h;
+    return rc;
+}
+
+// Read `words` words from the file into device memory through two pinned buffers; returns
+// their FNV-1a.
+int load_from_file(FILE *f, uint64_t *dev, uint64_t words, PinnedPair &pp, uint64_t *hash, hipStream_t s)
+{
+    uint64_t h = kFnvBasis;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    bool used[2] = {false, false};
+    int rc = DYC_OK;
+    int b = 0;
+    for (uint64_t off = 0; off < words && rc == DYC_OK; off += kCkptChunk, b ^= 1) {
+        const uint64_t m = std::min(kCkptChunk, words - off);
+        if (used[b] && hipEventSynchronize(ev[b]) != hipSuccess) rc = fail(DYC_ERR_HIP, "restore copy failed");
+        if (rc == DYC_OK && fread(pp.buf[b], 8, m, f) != m) rc = fail(DYC_ERR_CORRUPT, "checkpoint file is truncated");
+        if (rc != DYC_OK) break;
+        h = fnv_words(h, pp.buf[b], m);
+        if (hipMemcpyAsync(dev + off, pp.buf[b], m * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipEventRecord(ev[b], s) != hipSuccess)
+            rc = fail(DYC_ERR_HIP, "restore copy failed");
+        used[b] = true;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && rc == DYC_OK) rc = fail(DYC_ERR_HIP, "restore copy failed");
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    *hash = h;
+    return rc;
+}
+
+}  // namespace
+
 extern "C" {
 
 int dyc_abi_version(void) { return DYC_ABI_VERSION; }
@@ -400,6 +602,7 @@ int dyc_destroy(dyc_table *t)
     if (t->part) (void)hipFree(t->part);
     if (t->part_counts) (void)hipFree(t->part_counts);
     if (t->hctr) (void)hipHostFree(t->hctr);
+    free_stage(t);
     delete t;
     return DYC_OK;
 }
@@ -617,6 +820,243 @@ int dyc_pending(dyc_table *t, uint64_t *host_entries, uint64_t cap, uint64_t *n_
         HIP_TRY(hipMemcpyAsync(host_entries, t->pend, m * sizeof(uint64_t), hipMemcpyDeviceToHost, (hipStream_t)stream));
         HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
     }
+    return DYC_OK;
+}
+
+// ---- host-pointer entry points: chunks of kStageKeys keys move host -> pinned (threads) ->
+// device on a copy stream while the previous chunk's kernel runs on `stream`; results move back
+// on a second copy stream. Resizes and pending entries are handled once per call, as in the
+// device-pointer calls.
+int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    t->st.last_kicks = 0;
+    if (n == 0) return DYC_OK;
+    int rc;
+    if (t->cfg.auto_resize) {
+        while ((double)(t->count + n) > t->cfg.upper_bound * (double)t->capacity()) {
+            rc = do_resize_up(t, s);
+            if (rc) return rc;
+        }
+    }
+    rc = reserve_pending(t, std::max<uint64_t>(65536, n / 16), s);
+    if (rc) return rc;
+    const uint64_t room = t->pend_cap - t->pend_n;
+    DycHostStage *S = nullptr;
+    rc = get_stage(t, &S);
+    if (rc) return rc;
+    rc = zero_counters(t, s);
+    if (rc) return rc;
+    bool used[2] = {false, false};
+    uint64_t off = 0;
+    for (int c = 0; off < n; ++c) {
+        const int b = c & 1;
+        const uint64_t m = std::min(kStageKeys, n - off);
+        if (used[b]) HIP_TRY(hipEventSynchronize(S->done[b]));  // chunk c - 2 no longer reads buffer b
+        host_copy(S->hk[b], keys + off, m * 4);
+        host_copy(S->hv[b], vals + off, m * 4);
+        HIP_TRY(hipMemcpyAsync(S->dk[b], S->hk[b], m * 4, hipMemcpyHostToDevice, S->h2d));
+        HIP_TRY(hipMemcpyAsync(S->dv[b], S->hv[b], m * 4, hipMemcpyHostToDevice, S->h2d));
+        HIP_TRY(hipEventRecord(S->in_ready[b], S->h2d));
+        HIP_TRY(hipStreamWaitEvent(s, S->in_ready[b], 0));
+        HIP_TRY(launch_insert(t->D, S->dk[b], S->dv[b], m, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(),
+                              t->ctr, sink_at(t, t->pend_n), s));
+        HIP_TRY(hipEventRecord(S->done[b], s));
+        used[b] = true;
+        off += m;
+    }
+    rc = read_counters(t, s);
+    if (rc) return rc;
+    const uint64_t rejected = t->hctr->rejected;
+    t->st.rejected += rejected;
+    t->count += n - rejected;
+    rc = absorb_insert_counters(t, room);
+    t->st.pending = t->pend_n;
+    if (rc) return rc;
+    if (t->pend_n) return place_pending(t, s);
+    return DYC_OK;
+}
+
+namespace {
+// find (ERASE false) or erase over host arrays; see dyc_insert_host.
+int probe_host(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *flags, uint64_t n, hipStream_t s,
+               bool erase)
+{
+    DycHostStage *S = nullptr;
+    int rc = get_stage(t, &S);
+    if (rc) return rc;
+    bool used[2] = {false, false};
+    uint64_t prev_off[2] = {0, 0}, prev_m[2] = {0, 0};
+    auto drain = [&](int b) -> int {  // results of the chunk that used buffer b, to the caller
+        if (!used[b]) return DYC_OK;
+        HIP_TRY(hipEventSynchronize(S->out_ready[b]));
+        if (vals && !erase) host_copy(vals + prev_off[b], S->hv[b], prev_m[b] * 4);
+        if (flags) host_copy(flags + prev_off[b], S->hf[b], prev_m[b]);
+        used[b] = false;
+        return DYC_OK;
+    };
+    uint64_t off = 0;
+    for (int c = 0; off < n; ++c) {
+        const int b = c & 1;
+        const uint64_t m = std::min(kStageKeys, n - off);
+        rc = drain(b);
+        if (rc) return rc;
+        host_copy(S->hk[b], keys + off, m * 4);
+        HIP_TRY(hipMemcpyAsync(S->dk[b], S->hk[b], m * 4, hipMemcpyHostToDevice, S->h2d));
+        HIP_TRY(hipEventRecord(S->in_ready[b], S->h2d));
+        HIP_TRY(hipStreamWaitEvent(s, S->in_ready[b], 0));
+        if (erase)
+            HIP_TRY(launch_erase(t->D, S->dk[b], flags ? S->df[b] : nullptr, m, (int)t->cfg.find_lanes, t->shape(),
+                                 t->ctr, s));
+        else
+            HIP_TRY(launch_find(t->D, S->dk[b], vals ? S->dv[b] : nullptr, flags ? S->df[b] : nullptr, m,
+                                (int)t->cfg.find_lanes, t->shape(), s));
+        HIP_TRY(hipEventRecord(S->done[b], s));
+        HIP_TRY(hipStreamWaitEvent(S->d2h, S->done[b], 0));
+        if (vals && !erase) HIP_TRY(hipMemcpyAsync(S->hv[b], S->dv[b], m * 4, hipMemcpyDeviceToHost, S->d2h));
+        if (flags) HIP_TRY(hipMemcpyAsync(S->hf[b], S->df[b], m, hipMemcpyDeviceToHost, S->d2h));
+        HIP_TRY(hipEventRecord(S->out_ready[b], S->d2h));
+        used[b] = true;
+        prev_off[b] = off;
+        prev_m[b] = m;
+        off += m;
+    }
+    for (int b = 0; b < 2; ++b) {
+        rc = drain(b);
+        if (rc) return rc;
+    }
+    return DYC_OK;
+}
+}  // namespace
+
+int dyc_find_host(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    if (n == 0) return DYC_OK;
+    DeviceGuard dg(t->device);
+    return probe_host(t, keys, vals, hit, n, (hipStream_t)stream, false);
+}
+
+int dyc_erase_host(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    if (n == 0) return DYC_OK;
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = zero_counters(t, s);
+    if (rc) return rc;
+    rc = probe_host(t, keys, nullptr, deleted, n, s, true);
+    if (rc) return rc;
+    rc = read_counters(t, s);
+    if (rc) return rc;
+    t->count -= std::min<uint64_t>(t->count, t->hctr->deleted);
+    if (t->cfg.auto_resize && (double)t->count < t->cfg.lower_bound * (double)t->capacity() &&
+        t->capacity() > t->cfg.min_capacity) {
+        rc = do_resize_down(t, s);
+        if (rc == DYC_ERR_ODD_BUCKETS) rc = DYC_OK;
+        if (rc) return rc;
+    }
+    return DYC_OK;
+}
+
+int dyc_checkpoint_write(dyc_table *t, const char *path, dyc_stream stream)
+{
+    if (!t || !path) return fail(DYC_ERR_INVALID, "null argument");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipStreamSynchronize(s));
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(DYC_ERR_INVALID, "cannot open %s for writing", path);
+    CkptHeader h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.subtables = kSubtables;
+    for (int i = 0; i < kSubtables; ++i) h.buckets[i] = t->D.nb[i];
+    h.count = t->count;
+    h.pending = t->pend_n;
+    int rc = fwrite(&h, sizeof h, 1, f) == 1 ? DYC_OK : fail(DYC_ERR_INVALID, "checkpoint write failed");
+    PinnedPair pp;
+    if (rc == DYC_OK && !pp.alloc(kCkptChunk * 8)) rc = fail(DYC_ERR_OOM, "pinned staging for the checkpoint");
+    for (int i = 0; i < kSubtables && rc == DYC_OK; ++i)
+        rc = dump_to_file(f, t->D.sub[i], (uint64_t)t->D.nb[i] * kSlots, pp, &h.checksum[i], s);
+    if (rc == DYC_OK) rc = dump_to_file(f, t->pend, t->pend_n, pp, &h.checksum[4], s);
+    h.header_check = fnv_bytes(&h, offsetof(CkptHeader, header_check));
+    if (rc == DYC_OK && (fseek(f, 0, SEEK_SET) != 0 || fwrite(&h, sizeof h, 1, f) != 1))
+        rc = fail(DYC_ERR_INVALID, "checkpoint header write failed");
+    if (fclose(f) != 0 && rc == DYC_OK) rc = fail(DYC_ERR_INVALID, "checkpoint close failed");
+    return rc;
+}
+
+int dyc_checkpoint_read(const char *path, const dyc_config *cfg, dyc_table **out)
+{
+    if (!path || !out) return fail(DYC_ERR_INVALID, "null argument");
+    *out = nullptr;
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(DYC_ERR_INVALID, "cannot open %s", path);
+    CkptHeader h;
+    int rc = fread(&h, sizeof h, 1, f) == 1 ? DYC_OK : fail(DYC_ERR_CORRUPT, "checkpoint header is truncated");
+    if (rc == DYC_OK && (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1 || h.subtables != kSubtables))
+        rc = fail(DYC_ERR_CORRUPT, "not a DyCuckoo checkpoint (magic/version/subtables)");
+    if (rc == DYC_OK && h.header_check != fnv_bytes(&h, offsetof(CkptHeader, header_check)))
+        rc = fail(DYC_ERR_CORRUPT, "checkpoint header checksum mismatch");
+    uint64_t slots = 0;
+    for (int i = 0; i < kSubtables && rc == DYC_OK; ++i) {
+        if (h.buckets[i] == 0 || h.buckets[i] >= kMaxBuckets) rc = fail(DYC_ERR_CORRUPT, "bad bucket count");
+        slots += (uint64_t)h.buckets[i] * kSlots;
+    }
+    if (rc == DYC_OK && (fseek(f, 0, SEEK_END) != 0 ||
+                         (uint64_t)ftell(f) != sizeof h + (slots + h.pending) * 8 || fseek(f, sizeof h, SEEK_SET) != 0))
+        rc = fail(DYC_ERR_CORRUPT, "checkpoint size does not match its header");
+    dyc_table *t = nullptr;
+    if (rc == DYC_OK) rc = dyc_create(cfg, 64, &t);
+    PinnedPair pp;
+    if (rc == DYC_OK && !pp.alloc(kCkptChunk * 8)) rc = fail(DYC_ERR_OOM, "pinned staging for the restore");
+    for (int i = 0; i < kSubtables && rc == DYC_OK; ++i) {
+        uint64_t *p = nullptr;
+        if (hipMalloc(&p, (size_t)h.buckets[i] * kBucketBytes) != hipSuccess) {
+            (void)hipGetLastError();
+            rc = fail(DYC_ERR_OOM, "hipMalloc for the restore failed");
+            break;
+        }
+        (void)hipFree(t->D.sub[i]);
+        desc_set(t->D, i, p, h.buckets[i]);
+        uint64_t hash = 0;
+        rc = load_from_file(f, p, (uint64_t)h.buckets[i] * kSlots, pp, &hash, nullptr);
+        if (rc == DYC_OK && hash != h.checksum[i]) rc = fail(DYC_ERR_CORRUPT, "subtable %d checksum mismatch", i);
+    }
+    if (rc == DYC_OK) rc = reserve_pending(t, h.pending, nullptr);
+    if (rc == DYC_OK) {
+        uint64_t hash = 0;
+        rc = load_from_file(f, t->pend, h.pending, pp, &hash, nullptr);
+        if (rc == DYC_OK && hash != h.checksum[4]) rc = fail(DYC_ERR_CORRUPT, "pending list checksum mismatch");
+    }
+    fclose(f);
+    if (rc == DYC_OK) {
+        t->pend_n = h.pending;
+        t->st.pending = h.pending;
+        t->count = h.count;
+        uint64_t occ = 0;
+        rc = dyc_count_occupied(t, &occ, nullptr);
+        if (rc == DYC_OK && occ + h.pending != h.count)
+            rc = fail(DYC_ERR_CORRUPT, "checkpoint count %llu does not match its %llu stored entries",
+                      (unsigned long long)h.count, (unsigned long long)(occ + h.pending));
+    }
+    if (rc != DYC_OK) {
+        if (t) {
+            char keep[sizeof g_err];
+            memcpy(keep, g_err, sizeof keep);
+            dyc_destroy(t);
+            memcpy(g_err, keep, sizeof keep);
+        }
+        return rc;
+    }
+    *out = t;
     return DYC_OK;
 }
 

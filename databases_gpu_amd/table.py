@@ -36,6 +36,20 @@ def _as_u32_tensor(x, device) -> torch.Tensor:
     return torch.from_numpy(a).to(device)
 
 
+def _host_u32(x) -> np.ndarray:
+    """A C-contiguous numpy uint32 view of host data (no copy when it already is one)."""
+    if isinstance(x, torch.Tensor):
+        if x.is_cuda:
+            raise ValueError("host-array calls take host data; use the device-tensor calls for GPU tensors")
+        x = x.numpy()
+    a = np.asarray(x)
+    if a.dtype in (np.int32, np.uint32):
+        a = a.view(np.uint32)
+    else:
+        a = a.astype(np.uint32)
+    return np.ascontiguousarray(a)
+
+
 def _stream_ptr(stream) -> ctypes.c_void_p:
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)
@@ -105,6 +119,56 @@ class DyCuckoo:
             check(self.lib.dyc_erase(self.h, k.data_ptr(), deleted.data_ptr(), k.numel(), _stream_ptr(stream)),
                   "dyc_erase")
         return deleted
+
+    # -- operations on host arrays (pipelined host <-> device transfers)
+    def insert_host(self, keys, vals, stream=None) -> None:
+        """Insert from host arrays (numpy uint32, or anything convertible; memory maps work)."""
+        k = _host_u32(keys)
+        v = _host_u32(vals)
+        if k.size != v.size:
+            raise ValueError("keys and values differ in length")
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_insert_host(self.h, k.ctypes.data, v.ctypes.data, k.size, _stream_ptr(stream)),
+                  "dyc_insert_host")
+
+    def find_host(self, keys, stream=None):
+        """Look host keys up; returns (values uint32, hit uint8) as numpy arrays."""
+        k = _host_u32(keys)
+        vals = np.zeros(k.size, dtype=np.uint32)
+        hit = np.zeros(k.size, dtype=np.uint8)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_find_host(self.h, k.ctypes.data, vals.ctypes.data, hit.ctypes.data, k.size,
+                                         _stream_ptr(stream)), "dyc_find_host")
+        return vals, hit
+
+    def erase_host(self, keys, stream=None) -> np.ndarray:
+        k = _host_u32(keys)
+        deleted = np.zeros(k.size, dtype=np.uint8)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_erase_host(self.h, k.ctypes.data, deleted.ctypes.data, k.size, _stream_ptr(stream)),
+                  "dyc_erase_host")
+        return deleted
+
+    # -- checkpoint
+    def checkpoint(self, path: str, stream=None) -> None:
+        """Write the table (images, pending entries, counts, checksums) to `path`."""
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_checkpoint_write(self.h, str(path).encode(), _stream_ptr(stream)),
+                  "dyc_checkpoint_write")
+
+    @classmethod
+    def restore(cls, path: str, *, device=None, **config) -> "DyCuckoo":
+        """A table read back from a checkpoint file (its geometry; `config` for everything else)."""
+        self = cls.__new__(cls)
+        self.lib = _lib.load()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.cfg = _lib.default_config(**config)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_checkpoint_read(str(path).encode(), ctypes.byref(self.cfg), ctypes.byref(h)),
+                  "dyc_checkpoint_read")
+        self.h = h
+        return self
 
     def resize_up(self, stream=None) -> None:
         with torch.cuda.device(self.device):
@@ -222,23 +286,19 @@ class HashAPI:
         self.batch_reset()
 
     # -- synchronous host-array operations (api.cpp:184-303)
+    # (host arrays move through the library's pipelined host-pointer calls)
     def hash_insert(self, key, value, size: int) -> None:
-        k = np.asarray(key, dtype=np.uint32)[:size]
-        v = np.asarray(value, dtype=np.uint32)[:size]
-        self.table.insert(k, v)
-        torch.cuda.synchronize(self.table.device)
+        self.table.insert_host(np.asarray(key)[:size], np.asarray(value)[:size])
 
     def hash_search(self, key, ans, size: int):
         """Fills ans[:size] with the values (0 on a miss); returns the hit flags."""
-        k = np.asarray(key, dtype=np.uint32)[:size]
-        vals, hit = self.table.find(k)
-        ans[:size] = vals.cpu().numpy().view(np.uint32)
-        return hit.cpu().numpy()
+        vals, hit = self.table.find_host(np.asarray(key)[:size])
+        ans[:size] = vals
+        return hit
 
     def hash_delete(self, key, ans, size: int):
         """Deletes key[:size]; ans[:size] receives the deleted flags (the reference leaves it unused)."""
-        k = np.asarray(key, dtype=np.uint32)[:size]
-        d = self.table.erase(k).cpu().numpy()
+        d = self.table.erase_host(np.asarray(key)[:size])
         if ans is not None:
             ans[:size] = d
         return d

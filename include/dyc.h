@@ -123,6 +123,26 @@ int dyc_resize_up(dyc_table *t, dyc_stream stream);
  * is odd (the reference reads past the end there, cuckoo.cu:653,767-769). Synchronous. */
 int dyc_resize_down(dyc_table *t, dyc_stream stream);
 
+/* Host-pointer forms of insert / find / erase, for callers whose arrays live in host memory
+ * (the reference's hashAPI takes host arrays and copies them, api.cpp:204-209,272-283). The
+ * batch moves in chunks through library-owned pinned buffers: the copy of one chunk overlaps the
+ * kernel of the previous one, and results stream back on a second copy stream. Same semantics
+ * and resize policy as the device-pointer calls; synchronous (they return when the results are
+ * in the caller's arrays). */
+int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
+int dyc_find_host(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, dyc_stream stream);
+int dyc_erase_host(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream);
+
+/* Checkpoint a table to a file and restore it (no reference counterpart; SURVEY-listed
+ * table-image format). The file is a 128-byte header (magic "DYCKPT01", version 1, 4
+ * subtables, buckets per subtable, live count, pending count, FNV-1a checksums of every part and
+ * of the header), then the four subtable images and the pending entries as little-endian u64
+ * slots. Restore validates the header, the file size, every checksum and that the stored
+ * entries match the count (DYC_ERR_CORRUPT otherwise), and creates a table with `cfg` (NULL =
+ * defaults) and the file's geometry. */
+int dyc_checkpoint_write(dyc_table *t, const char *path, dyc_stream stream);
+int dyc_checkpoint_read(const char *path, const dyc_config *cfg, dyc_table **out);
+
 /* Read / replace the run-time configuration of a table (no reference counterpart: the
  * reference's configuration is compile-time, dy_hash.h:15-73). Sizes are not touched; every
  * field of dyc_config is otherwise taken, validated as in dyc_create. */

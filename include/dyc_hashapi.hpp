@@ -56,37 +56,26 @@ class hashAPI {
     hashAPI(const hashAPI &) = delete;
     hashAPI &operator=(const hashAPI &) = delete;
 
+    // Host arrays move through the library's pipelined staging (dyc_*_host): the copy of one
+    // chunk overlaps the kernel of the previous one.
     void hash_insert(TYPE *k, TYPE *v, TYPE size)
     {
-        Dev<uint32_t> dk(size), dv(size);
-        dk.put(k, size);
-        dv.put(v, size);
-        check(dyc_insert(t_, dk.p, dv.p, size, nullptr), "dyc_insert");
+        check(dyc_insert_host(t_, k, v, size, nullptr), "dyc_insert_host");
     }
 
     void hash_search(TYPE *k, TYPE *ans, TYPE size) { hash_search_hits(k, ans, nullptr, size); }
 
     void hash_search_hits(TYPE *k, TYPE *ans, uint8_t *hit, TYPE size)
     {
-        Dev<uint32_t> dk(size), dv(size);
-        Dev<uint8_t> dh(size);
-        dk.put(k, size);
-        check(dyc_find(t_, dk.p, dv.p, dh.p, size, nullptr), "dyc_find");
-        dv.get(ans, size);
-        if (hit) dh.get(hit, size);
+        check(dyc_find_host(t_, k, ans, hit, size, nullptr), "dyc_find_host");
     }
 
     void hash_delete(TYPE *k, TYPE *ans, TYPE size)
     {
-        Dev<uint32_t> dk(size);
-        Dev<uint8_t> dd(size);
-        dk.put(k, size);
-        check(dyc_erase(t_, dk.p, dd.p, size, nullptr), "dyc_erase");
-        if (ans) {
-            std::vector<uint8_t> flags(size);
-            dd.get(flags.data(), size);
+        std::vector<uint8_t> flags(ans ? size : 0);
+        check(dyc_erase_host(t_, k, ans ? flags.data() : nullptr, size, nullptr), "dyc_erase_host");
+        if (ans)
             for (TYPE i = 0; i < size; ++i) ans[i] = flags[i];
-        }
     }
 
     // api.cpp:112-173 ("public for test")
@@ -140,21 +129,6 @@ class hashAPI {
     dyc_table *handle() { return t_; }
 
   private:
-    template <typename T>
-    struct Dev {
-        T *p = nullptr;
-        explicit Dev(size_t n)
-        {
-            if (n && hipMalloc(&p, sizeof(T) * n) != hipSuccess) throw dyc_error(DYC_ERR_OOM, "hipMalloc failed");
-        }
-        ~Dev()
-        {
-            if (p) (void)hipFree(p);
-        }
-        void put(const void *h, size_t n) { hip(hipMemcpy(p, h, sizeof(T) * n, hipMemcpyHostToDevice)); }
-        void get(void *h, size_t n) { hip(hipMemcpy(h, p, sizeof(T) * n, hipMemcpyDeviceToHost)); }
-    };
-
     static void hip(hipError_t e)
     {
         if (e != hipSuccess) throw dyc_error(DYC_ERR_HIP, hipGetErrorString(e));
