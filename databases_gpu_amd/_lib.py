@@ -33,7 +33,7 @@ EXPORTS = (
     "dyc_abi_version", "dyc_last_error", "dyc_config_init", "dyc_create", "dyc_destroy",
     "dyc_insert", "dyc_find", "dyc_erase", "dyc_resize_up", "dyc_resize_down", "dyc_get_stats",
     "dyc_count_occupied", "dyc_dump", "dyc_load", "dyc_pending", "dyc_clear", "dyc_probe_count",
-    "dyc_get_config", "dyc_set_config", "dyc_insert_host", "dyc_find_host", "dyc_erase_host",
+    "dyc_get_config", "dyc_set_config", "dyc_upsert", "dyc_insert_host", "dyc_find_host", "dyc_erase_host",
     "dyc_checkpoint_write", "dyc_checkpoint_read", "dyc_group_create", "dyc_group_destroy", "dyc_group_get_launch", "dyc_group_set_launch",
     "dyc_group_insert", "dyc_group_find", "dyc_group_erase", "dyc_group_insert_per_table",
     "dyc_group_find_per_table", "dyc_group_table_stats", "dyc_group_dump", "dyc_group_pending",
@@ -184,6 +184,7 @@ def load(require: bool = True):
         "dyc_pending": (ci, [vp, vp, u64, ctypes.POINTER(u64), vp]),
         "dyc_clear": (ci, [vp, vp]),
         "dyc_probe_count": (ci, [vp, vp, u64, ctypes.POINTER(u64), ctypes.POINTER(u64), vp]),
+        "dyc_upsert": (ci, [vp, vp, vp, u64, vp, vp]),
         "dyc_insert_host": (ci, [vp, vp, vp, u64, vp]),
         "dyc_find_host": (ci, [vp, vp, vp, vp, u64, vp]),
         "dyc_erase_host": (ci, [vp, vp, vp, u64, vp]),

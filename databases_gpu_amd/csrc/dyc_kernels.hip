@@ -102,14 +102,19 @@ __device__ inline void wave_add(unsigned long long *dst, unsigned long long v)
     if (__lane_id() == 0 && v) atomicAdd(dst, v);
 }
 
-// ------------------------------------------------------------------------ find / erase
+// ---------------------------------------------------------------- find / erase / update
 // One wave probes the 64 keys keys[base, base + 64) clipped to n, in table D. Returns this
-// lane's flag (hit or deleted) for its own key; all 64 lanes must call it together. COH: read the
-// buckets device-coherently (see load_slots_coherent).
-template <int G, bool ERASE, bool COH = false>
-__device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                      uint8_t *__restrict__ flags, uint64_t base, uint64_t n)
+// lane's flag for its own key (hit, deleted or updated); all 64 lanes must call it together.
+// MODE 0 find: vals (optional) receives the value. MODE 1 erase: every copy of the key in the
+// first bucket (pos1, then pos2) that holds it is CAS'd to empty. MODE 2 update: those copies
+// are CAS'd to the new value vals[i] (an input). COH: read the buckets device-coherently (see
+// load_slots_coherent).
+template <int G, int MODE, bool COH = false>
+__device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                        uint8_t *__restrict__ flags, uint64_t base, uint64_t n)
 {
+    constexpr bool ERASE = MODE != 0;  // erase and update both rewrite the matching slots
+    constexpr bool UPDATE = MODE == 2;
     constexpr int SPL = kSlots / G;  // slots each lane reads per bucket
     constexpr uint64_t GMASK = (G >= 64) ? ~0ull : ((1ull << G) - 1ull);
     const int lane = __lane_id();
@@ -118,12 +123,14 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
         const uint64_t i = base + lane;
         const bool inb = i < n;
         const uint32_t k = inb ? keys[i] : 0u;  // past the end: key 0, which is reserved
+        const uint32_t nv = (UPDATE && inb) ? vals[i] : 0u;
         const uint32_t p = pair_of(k);
         const uint32_t loc1 = pack_loc(pos1(p), bucket_of(D, k, pos1(p)));
         const uint32_t loc2 = pack_loc(pos2(p), bucket_of(D, k, pos2(p)));
 
         uint64_t e[G][SPL];
         uint32_t kk_key[G];
+        uint32_t kk_new[UPDATE ? G : 1];  // update: the new value of key kk
         bool kk_valid[G];
         uint32_t kk_loc[ERASE ? G : 1];  // erase writes back to the bucket it matched in
         // phase 1: pos1 buckets of the group's G keys, all loads in flight together
@@ -131,6 +138,7 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
         for (int kk = 0; kk < G; ++kk) {
             const int src = g * G + kk;
             kk_key[kk] = __shfl(k, src);
+            if constexpr (UPDATE) kk_new[kk] = __shfl(nv, src);
             kk_valid[kk] = !reserved_key(kk_key[kk]);
             const uint32_t l = __shfl(loc1, src);
             if constexpr (ERASE) kk_loc[kk] = l;
@@ -156,7 +164,8 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
                     for (int j = 0; j < SPL; ++j)
                         if (kk_valid[kk] && key_of(e[kk][j]) == kk_key[kk]) {
                             uint64_t exp = e[kk][j];
-                            ok |= cas_slot(loc_ptr(D, kk_loc[kk]) + s * SPL + j, exp, 0ull);
+                            const uint64_t rep = UPDATE ? pack(kk_key[kk], kk_new[UPDATE ? kk : 0]) : 0ull;
+                            ok |= cas_slot(loc_ptr(D, kk_loc[kk]) + s * SPL + j, exp, rep);
                         }
                     const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & GMASK);
                     if (kk == s) my_flag = okm != 0;
@@ -197,7 +206,8 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
                 for (int j = 0; j < SPL; ++j)
                     if (kk_valid[kk] && key_of(e[kk][j]) == kk_key[kk]) {
                         uint64_t exp = e[kk][j];
-                        ok |= cas_slot(loc_ptr(D, kk_loc[kk]) + s * SPL + j, exp, 0ull);
+                        const uint64_t rep = UPDATE ? pack(kk_key[kk], kk_new[UPDATE ? kk : 0]) : 0ull;
+                        ok |= cas_slot(loc_ptr(D, kk_loc[kk]) + s * SPL + j, exp, rep);
                     }
                 const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & GMASK);
                 if (kk == s) my_flag = okm != 0;
@@ -221,6 +231,13 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
         }
         return my_flag;
     }
+}
+
+template <int G, bool ERASE, bool COH = false>
+__device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                      uint8_t *__restrict__ flags, uint64_t base, uint64_t n)
+{
+    return probe_tile_m<G, ERASE ? 1 : 0, COH>(D, keys, vals, flags, base, n);
 }
 
 template <bool ERASE>
@@ -250,6 +267,20 @@ __global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__re
         ndel += f;
     }
     if constexpr (ERASE) wave_add(&C->deleted, ndel);
+}
+
+// Upsert's first pass: rewrite the value of keys already stored (probe_tile_m mode 2).
+template <int G>
+__global__ __launch_bounds__(256) void update_kernel(Desc D, const uint32_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ vals, uint8_t *__restrict__ updated,
+                                                     uint64_t n, Counters *__restrict__ C)
+{
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    unsigned long long nu = 0;
+    for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
+         base += waves * kWave)
+        nu += probe_tile_m<G, 2>(D, keys, vals, updated, base, n);
+    wave_add(&C->updated, nu);
 }
 
 // ------------------------------------------------------------------------------- insert
@@ -405,7 +436,8 @@ __device__ inline void kick_step(const Desc &D, Carry<kSlots / G> &c, const Inse
 template <int G, bool FROM_ENTRIES>
 __device__ inline void refill(Carry<kSlots / G> &c, uint64_t &next, uint64_t end, const uint32_t *__restrict__ keys,
                               const uint32_t *__restrict__ vals, const uint64_t *__restrict__ entries,
-                              uint32_t pre_fixed, InsertTally &tally, uint64_t &taken)
+                              uint32_t pre_fixed, InsertTally &tally, uint64_t &taken,
+                              const uint8_t *__restrict__ skip = nullptr)
 {
     const int lane = __lane_id();
     const int g = lane / G, s = lane % G;
@@ -420,7 +452,7 @@ __device__ inline void refill(Carry<kSlots / G> &c, uint64_t &next, uint64_t end
             const uint64_t idx = next + rank;
             if constexpr (FROM_ENTRIES)
                 carry_start(c, entries[idx], pre_fixed, s, tally);
-            else
+            else if (!skip || !skip[idx])
                 carry_start(c, pack(keys[idx], vals[idx]), kNoTable, s, tally);
         }
         next += take;
@@ -434,7 +466,7 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
                                                      const uint64_t *__restrict__ entries, uint64_t n,
                                                      const unsigned long long *__restrict__ n_dev,
                                                      uint32_t pre_fixed, InsertPolicy P, Counters *__restrict__ C,
-                                                     PendSink sink)
+                                                     PendSink sink, const uint8_t *__restrict__ skip)
 {
     if (n_dev && *n_dev < n) n = *n_dev;  // a count an earlier launch left on the device
     constexpr int NG = kWave / G;
@@ -447,7 +479,7 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
     uint64_t next = 0, end = 0;  // wave-uniform: the wave's current chunk
     uint64_t taken = 0;          // wave-uniform: input indexes this wave took
     while (true) {
-        refill<G, FROM_ENTRIES>(c, next, end, keys, vals, entries, pre_fixed, tally, taken);
+        refill<G, FROM_ENTRIES>(c, next, end, keys, vals, entries, pre_fixed, tally, taken, skip);
         if (next == end) {
             // chunks w, w + W, w + 2W, ... (W = waves in the grid)
             const uint64_t base = next_chunk * kChunk;
@@ -1048,7 +1080,8 @@ hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, u
 template <bool FE>
 static hipError_t insert_common(const Desc &D, const uint32_t *keys, const uint32_t *vals, const uint64_t *entries,
                                 uint64_t n, const unsigned long long *n_dev, uint32_t pre, int lanes,
-                                InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st)
+                                InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
+                                const uint8_t *skip = nullptr)
 {
     if (n == 0) return hipSuccess;
     const int per_wave = kWave / lanes;  // entries in flight per wave
@@ -1056,32 +1089,47 @@ static hipError_t insert_common(const Desc &D, const uint32_t *keys, const uint3
     switch (lanes) {
     case 8:
         insert_kernel<8, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
-                                                                      sink);
+                                                                      sink, skip);
         break;
     case 4:
         insert_kernel<4, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
-                                                                      sink);
+                                                                      sink, skip);
         break;
     case 2:
         insert_kernel<2, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
-                                                                      sink);
+                                                                      sink, skip);
         break;
     case 1:
         insert_kernel<1, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
-                                                                      sink);
+                                                                      sink, skip);
         break;
     default:
         insert_kernel<16, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
-                                                                      sink);
+                                                                      sink, skip);
         break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint64_t n, int lanes,
-                         InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st)
+                         InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
+                         const uint8_t *skip)
 {
-    return insert_common<false>(D, keys, vals, nullptr, n, nullptr, kNoTable, lanes, pol, sh, C, sink, st);
+    return insert_common<false>(D, keys, vals, nullptr, n, nullptr, kNoTable, lanes, pol, sh, C, sink, st, skip);
+}
+
+hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint8_t *updated, uint64_t n,
+                         int lanes, LaunchShape sh, Counters *C, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = grid_for((n + kWave - 1) / kWave, sh);
+    uint32_t *v = const_cast<uint32_t *>(vals);  // read-only in update mode
+    switch (lanes) {
+    case 4: update_kernel<4><<<grid, sh.block_threads, 0, st>>>(D, keys, v, updated, n, C); break;
+    case 16: update_kernel<16><<<grid, sh.block_threads, 0, st>>>(D, keys, v, updated, n, C); break;
+    default: update_kernel<8><<<grid, sh.block_threads, 0, st>>>(D, keys, v, updated, n, C); break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,

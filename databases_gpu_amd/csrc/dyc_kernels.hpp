@@ -16,7 +16,8 @@ struct Counters {
     unsigned long long misplaced;  // resize: entries outside their bucket pair (corrupt image)
     unsigned long long occupied;   // count kernel
     unsigned long long stalls;     // serving tick: dependency waits that hit their bound
-    unsigned long long pad[7];
+    unsigned long long updated;    // upsert: keys whose stored value was replaced
+    unsigned long long pad[6];
 };
 constexpr int kCounterWords = 16;
 constexpr uint32_t kMaxRegions = 1024;
@@ -51,8 +52,14 @@ hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, u
                         Counters *C, hipStream_t st, const Pool *pool = nullptr, uint32_t table = 0);
 // insert keys/vals (start rule of cuckoo.cu:336-341) or packed entries with a fixed previous
 // table `pre` (kNoTable = use the key-parity start rule).
+// skip (optional): skip[i] != 0 leaves keys[i] out (an upsert already updated it in place).
 hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint64_t n, int lanes,
-                         InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st);
+                         InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
+                         const uint8_t *skip = nullptr);
+// update in place: every copy of keys[i] in the first bucket (pos1, then pos2) that holds it
+// takes the value vals[i]; updated[i] = 1 if there was one; the count goes to C->updated.
+hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint8_t *updated, uint64_t n,
+                         int lanes, LaunchShape sh, Counters *C, hipStream_t st);
 // n_dev (optional): a device count read at kernel start; entries [0, min(n, *n_dev)) are inserted.
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
                                  InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,

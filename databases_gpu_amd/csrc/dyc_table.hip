@@ -81,6 +81,8 @@ struct dyc_table {
     uint64_t part_cap = 0;
     unsigned long long *part_counts = nullptr;   // kMaxRegions counts, then kMaxRegions cursors
     DycHostStage *stage = nullptr;
+    uint8_t *upd = nullptr;  // upsert: per-key "updated in place" flags
+    uint64_t upd_cap = 0;
     dyc_stats st{};
 
     uint64_t capacity() const
@@ -602,6 +604,7 @@ int dyc_destroy(dyc_table *t)
     if (t->part) (void)hipFree(t->part);
     if (t->part_counts) (void)hipFree(t->part_counts);
     if (t->hctr) (void)hipHostFree(t->hctr);
+    if (t->upd) (void)hipFree(t->upd);
     free_stage(t);
     delete t;
     return DYC_OK;
@@ -647,6 +650,57 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     const uint64_t rejected = t->hctr->rejected;
     t->st.rejected += rejected;
     t->count += n - rejected;
+    rc = absorb_insert_counters(t, room);
+    t->st.pending = t->pend_n;
+    if (rc) return rc;
+    if (t->pend_n) return place_pending(t, s);
+    return DYC_OK;
+}
+
+int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, uint8_t *updated,
+               dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    t->st.last_kicks = 0;
+    if (n == 0) return DYC_OK;
+    int rc;
+    if (t->cfg.auto_resize) {  // sized as if every key were new (api.cpp:360-369)
+        while ((double)(t->count + n) > t->cfg.upper_bound * (double)t->capacity()) {
+            rc = do_resize_up(t, s);
+            if (rc) return rc;
+        }
+    }
+    uint8_t *flags = updated;
+    if (!flags) {
+        if (n > t->upd_cap) {
+            HIP_TRY(hipStreamSynchronize(s));
+            if (t->upd) HIP_TRY(hipFree(t->upd));
+            t->upd = nullptr;
+            t->upd_cap = 0;
+            if (hipMalloc(&t->upd, n) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(DYC_ERR_OOM, "hipMalloc of %llu upsert flags failed", (unsigned long long)n);
+            }
+            t->upd_cap = n;
+        }
+        flags = t->upd;
+    }
+    rc = reserve_pending(t, std::max<uint64_t>(65536, n / 16), s);
+    if (rc) return rc;
+    const uint64_t room = t->pend_cap - t->pend_n;
+    rc = zero_counters(t, s);
+    if (rc) return rc;
+    HIP_TRY(launch_update(t->D, keys, vals, flags, n, (int)t->cfg.find_lanes, t->shape(), t->ctr, s));
+    HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(), t->ctr,
+                          sink_at(t, t->pend_n), s, flags));
+    rc = read_counters(t, s);
+    if (rc) return rc;
+    const uint64_t rejected = t->hctr->rejected, upd = t->hctr->updated;
+    t->st.rejected += rejected;
+    t->count += n - rejected - upd;
     rc = absorb_insert_counters(t, room);
     t->st.pending = t->pend_n;
     if (rc) return rc;

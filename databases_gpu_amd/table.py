@@ -96,6 +96,19 @@ class DyCuckoo:
             check(self.lib.dyc_insert(self.h, k.data_ptr(), v.data_ptr(), k.numel(), _stream_ptr(stream)),
                   "dyc_insert")
 
+    def upsert(self, keys: torch.Tensor, vals: torch.Tensor, stream=None) -> torch.Tensor:
+        """Insert, or replace the value of keys already stored (keys distinct within the call).
+        Returns per-key flags: 1 where an existing entry was updated."""
+        k = _as_u32_tensor(keys, self.device)
+        v = _as_u32_tensor(vals, self.device)
+        if k.numel() != v.numel():
+            raise ValueError("keys and values differ in length")
+        updated = torch.empty(k.numel(), dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_upsert(self.h, k.data_ptr(), v.data_ptr(), k.numel(), updated.data_ptr(),
+                                      _stream_ptr(stream)), "dyc_upsert")
+        return updated
+
     def find(self, keys: torch.Tensor, vals: torch.Tensor | None = None, hit: torch.Tensor | None = None,
              stream=None):
         """Look keys up (cuckoo_search, cuckoo.cu:460-520). Returns (vals int32-as-uint32, hit uint8)."""

@@ -101,6 +101,14 @@ int dyc_destroy(dyc_table *t);
  * Synchronises `stream` once at the end (it reads the failure counters). */
 int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
 
+/* Insert with update (the paper's insert, tkde-revision/secs/cuckoo.tex:127-129; the reference
+ * code has it switched off, dy_hash.h:42 update_on 0). A key already stored gets vals[i] in every
+ * copy held by the first bucket (pos1, then pos2) that holds it -- what dyc_find returns -- and
+ * updated[i] = 1 (updated may be NULL); every other key is inserted as by dyc_insert. Keys within
+ * one call must be distinct; entries on the pending list are not searched. Synchronises once. */
+int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, uint8_t *updated,
+               dyc_stream stream);
+
 /* Replaces gpu_lp_search (cuckoo.h:106-108, cuckoo.cu:881-899) / hashAPI::hash_search
  * (api.cpp:267-285). vals[i] = the value of keys[i] or 0 on a miss (what the reference's
  * pre-zeroed output shows); hit[i] = 1/0. Either output may be NULL. Asynchronous. */

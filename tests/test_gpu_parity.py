@@ -196,6 +196,40 @@ def test_set_config_reshapes_one_table(D):
     t.close()
 
 
+def test_upsert_updates_stored_keys_and_inserts_the_rest(D):
+    """Upsert against a dict model: existing keys take the new value (in place, no second copy),
+    absent keys are inserted, reserved keys are rejected; counts and the placement invariant hold."""
+    n = 1 << 20
+    banner("upsert 2^20 over 2^20")
+    keys = unique_keys(2 * n, 501)
+    old, fresh = keys[:n], keys[n:]
+    t = D.DyCuckoo(int(np.ceil(1.6 * n / 0.85)))
+    t.insert(torch.from_numpy(old.view(np.int32)).cuda(), torch.from_numpy(vals_of(old).view(np.int32)).cuda())
+    # half of the stored keys with new values, half a batch of fresh keys, two reserved keys
+    up_keys = np.concatenate([old[::2], fresh[: n // 2], np.array([0, 0xFFFFFFFF], dtype=np.uint32)])
+    rng = np.random.default_rng(502)
+    rng.shuffle(up_keys)
+    up_vals = rng.integers(0, 2**32, size=up_keys.size, dtype=np.uint64).astype(np.uint32)
+    flags = t.upsert(torch.from_numpy(up_keys.view(np.int32)).cuda(), torch.from_numpy(up_vals.view(np.int32)).cuda())
+    model = dict(zip(old.tolist(), vals_of(old).tolist()))
+    for k, v in zip(up_keys.tolist(), up_vals.tolist()):
+        if k not in (0, 0xFFFFFFFF):
+            model[k] = v
+    f = flags.cpu().numpy()
+    stored = np.isin(up_keys, old)
+    assert np.array_equal(f.astype(bool), stored)
+    st = t.stats()
+    assert st["count"] == len(model) and st["pending"] == 0
+    ref = oracle_from_images(t.dump())
+    assert ref.check_invariant() == 0
+    mk = np.array(list(model.keys()), dtype=np.uint32)
+    mv = np.array(list(model.values()), dtype=np.uint32)
+    assert np.array_equal(ref.entries(), np.sort(O.pack(mk, mv)))  # one copy per key
+    got_v, got_h = gpu_find(t, mk)
+    assert got_h.all() and np.array_equal(got_v, mv)
+    t.close()
+
+
 def test_insert_matches_sequential_replay_set(D):
     """The oracle's sequential replay and the GPU hold the same key->value set afterwards."""
     n = 300_000
