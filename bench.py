@@ -63,7 +63,11 @@ def parse():
     ap.add_argument("--cpu-seq-log2n", type=int, default=22, help="sequential CPU sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
-    ap.add_argument("--workload", choices=("bulk", "dynamic", "batched", "serving", "resize", "memsys", "host"),
+    ap.add_argument("--subtables", type=int, default=4, help="subtable count d of the bulk table (2..6)")
+    ap.add_argument("--st-list", default="2,3,4,5,6", help="subtables: the subtable counts swept")
+    ap.add_argument("--st-kick", type=int, default=15, help="subtables: kick bound of every table")
+    ap.add_argument("--workload", choices=("bulk", "dynamic", "batched", "serving", "resize", "memsys", "host",
+                                           "subtables"),
                     default="bulk",
                     help="bulk: the large bulk configuration (default, the headline line); dynamic: the "
                          "resize stream (growth then shrink batches under fill bounds); batched: many uneven "
@@ -71,7 +75,8 @@ def parse():
                          "(continuous batching vs synchronous per-request launches) over a load sweep; resize: "
                          "single-subtable upsize and downsize of a 2^28-slot-class table; memsys: the device's "
                          "random bucket-read, CAS and store rates at the table's size (the kernels' ceilings); "
-                         "host: the bulk workload from host arrays (PCIe included), pipelined vs copy-then-run")
+                         "host: the bulk workload from host arrays (PCIe included), pipelined vs copy-then-run; "
+                         "subtables: insert and find throughput for each subtable count at a fixed table size")
     ap.add_argument("--sv-requests", type=int, default=4096, help="serving: requests per run")
     ap.add_argument("--sv-loads", default="0.25,0.5,1,2,4,8,16,0",
                     help="serving: offered loads in G keys/s (0 = every request arrives at once)")
@@ -390,8 +395,8 @@ def run_resize(args, dev) -> int:
         e[2].record(stream)
         e[2].synchronize()
         nb2 = t.buckets
-        grown = [i for i in range(4) if nb1[i] != nb0[i]][0]
-        shrunk = [i for i in range(4) if nb2[i] != nb1[i]][0]
+        grown = [i for i in range(len(nb0)) if nb1[i] != nb0[i]][0]
+        shrunk = [i for i in range(len(nb1)) if nb2[i] != nb1[i]][0]
         if s_ >= args.warmup:
             ups.append(e[0].elapsed_time(e[1]))
             downs.append(e[1].elapsed_time(e[2]))
@@ -416,6 +421,79 @@ def run_resize(args, dev) -> int:
         "parity": "ok" if ok else "mismatch",
     }
     t.close()
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0 if ok else 1
+
+
+def run_subtables(args, dev) -> int:
+    """Subtable count sweep (the paper's "varying the number of tables", experiment.tex:213-220):
+    for each d, a table of the same total size (hint = 2^log2n / theta slots, so each subtable
+    holds 1/d of it) is cleared, bulk-inserted with 2^log2n keys and probed with 2^log2n finds at
+    the --hit rate, steps times; the medians of each phase are reported per d. An insert that
+    overflows upsizes the table (counted in resizes_up). Prints one JSON line."""
+    import torch
+
+    import databases_gpu_amd as D
+    from databases_gpu_amd import workload as W
+
+    stream = torch.cuda.current_stream()
+    n = 1 << args.log2n
+    hint = int(math.ceil(n / args.theta))
+    keys = W.keys_range(0, n, args.seed, dev)
+    vals = W.splitmix32(keys)
+    look = W.mixed_lookups(keys, n, args.hit, args.seed, n, args.seed + 7)
+    present = torch.isin(look, keys)
+    out_v = torch.empty(n, dtype=torch.int32, device=dev)
+    out_h = torch.empty(n, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    per_d, ok = {}, True
+    for d in [int(x) for x in args.st_list.split(",")]:
+        t = D.DyCuckoo(hint, subtables=d, kick_bound=args.st_kick, find_lanes=args.find_lanes)
+        ins, fnd, kicks = [], [], []
+        for s_ in range(args.warmup + args.steps):
+            t.clear()
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record(stream)
+            t.insert(keys, vals)
+            e[1].record(stream)
+            t.find(look, out_v, out_h)
+            e[2].record(stream)
+            e[2].synchronize()
+            if s_ >= args.warmup:
+                ins.append(e[0].elapsed_time(e[1]))
+                fnd.append(e[1].elapsed_time(e[2]))
+                kicks.append(t.stats()["last_kicks"])
+        st = t.stats()
+        good = st["count"] == n and st["pending"] == 0 and st["lost"] == 0 and bool((out_h.bool() == present).all()) \
+            and bool((out_v[present] == W.splitmix32(look[present])).all())
+        probes, _ = t.probe_count(look)
+        ok &= good
+        i_ms, f_ms = float(np.median(ins)), float(np.median(fnd))
+        per_d[str(d)] = {"insert_ms": round(i_ms, 3), "insert_mops": round(n / i_ms / 1e3, 1),
+                         "find_ms": round(f_ms, 3), "find_mops": round(n / f_ms / 1e3, 1),
+                         "mean_exchanges": round(float(np.mean(kicks)) / n, 4), "mean_probes": round(probes / n, 4),
+                         "buckets": st["buckets"], "fill": round(n / st["capacity"], 4),
+                         "resizes_up": st["resizes_up"], "parity": "ok" if good else "mismatch"}
+        log(f"subtables {d}: insert {i_ms:.2f} ms ({n / i_ms / 1e3:.0f} Mops/s), find {f_ms:.2f} ms "
+            f"({n / f_ms / 1e3:.0f} Mops/s), buckets {st['buckets']}, {'ok' if good else 'MISMATCH'}")
+        t.close()
+        torch.cuda.empty_cache()
+    best = max(per_d.values(), key=lambda r: r["insert_mops"] + r["find_mops"])
+    result = {
+        "metric": "insert and find throughput per subtable count at a fixed table size, Mops/s",
+        "value": round(2 * n / (best["insert_ms"] + best["find_ms"]) / 1e3, 1), "unit": "Mops/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"2^{args.log2n} unique u32 keys into a table of 2^{args.log2n}/{args.theta} slots "
+                               f"split over d subtables, then 2^{args.log2n} finds at {args.hit:.0%} hit",
+                   "kick_bound": args.st_kick, "find_lanes": args.find_lanes},
+        "per_subtables": per_d,
+        "parity": "ok" if ok else "mismatch",
+    }
     line = json.dumps(result)
     print(line, flush=True)
     if args.json_out:
@@ -672,6 +750,8 @@ def main() -> int:
         return run_memsys(args, dev)
     if args.workload == "host":
         return run_host(args, dev)
+    if args.workload == "subtables":
+        return run_subtables(args, dev)
     n = 1 << args.log2n
     hint = int(math.ceil(n / args.theta))
     log(f"generating inputs: n=2^{args.log2n}, theta={args.theta}")
@@ -688,7 +768,7 @@ def main() -> int:
         log(f"calibration copy kernel: {4 * n} bytes read, {4 * n} bytes written")
 
     t = D.DyCuckoo(hint, find_lanes=args.find_lanes, insert_lanes=args.insert_lanes,
-                   insert_blocks=args.insert_blocks, max_blocks=args.max_blocks)
+                   insert_blocks=args.insert_blocks, max_blocks=args.max_blocks, subtables=args.subtables)
     log(f"table: buckets {t.buckets}, capacity {t.capacity}, fill {n / t.capacity:.4f}")
     tuned = None
     if not args.no_tune:
@@ -820,7 +900,7 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": f"large bulk: 4 subtables, 2^{args.log2n} unique u32 keys, load factor {args.theta}, "
+            "workload": f"large bulk: {args.subtables} subtables, 2^{args.log2n} unique u32 keys, load factor {args.theta}, "
                         f"bulk insert + 2^{args.log2n} finds at {args.hit:.0%} hit (uniform) + 2^{args.log2n} "
                         f"finds Zipf({args.zipf})",
             "keys": n, "theta": args.theta, "table_fill": round(n / t.capacity, 5), "buckets": t.buckets,

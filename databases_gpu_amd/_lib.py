@@ -12,7 +12,9 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdyc.so")
 
-SUBTABLES = 4
+SUBTABLES = 4          # default subtable count (TABLE_NUM)
+MAX_SUBTABLES = 6
+SUBTABLE_COUNTS = (2, 3, 4, 5, 6)
 SLOTS = 16
 
 DYC_OK = 0
@@ -68,6 +70,8 @@ class Config(ctypes.Structure):
         ("min_capacity", ctypes.c_uint64),
         ("insert_order", ctypes.c_uint32),
         ("region_mib", ctypes.c_uint32),
+        ("subtables", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
     ]
 
 
@@ -75,7 +79,7 @@ class Stats(ctypes.Structure):
     _fields_ = [
         ("count", ctypes.c_uint64),
         ("capacity", ctypes.c_uint64),
-        ("buckets", ctypes.c_uint32 * SUBTABLES),
+        ("buckets", ctypes.c_uint32 * MAX_SUBTABLES),
         ("pending", ctypes.c_uint64),
         ("kicks", ctypes.c_uint64),
         ("last_kicks", ctypes.c_uint64),
@@ -85,11 +89,13 @@ class Stats(ctypes.Structure):
         ("resizes_up", ctypes.c_uint64),
         ("resizes_down", ctypes.c_uint64),
         ("lost", ctypes.c_uint64),
+        ("subtables", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
-        d = {name: getattr(self, name) for name, _ in self._fields_}
-        d["buckets"] = list(self.buckets)
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved0"}
+        d["buckets"] = list(self.buckets)[: self.subtables]
         return d
 
 

@@ -1,8 +1,8 @@
 // dyc_device.hpp -- DyCuckoo table descriptor and per-key arithmetic shared by every kernel.
 //
-// Layout (unchanged from the reference, cuckoo.h:13-16,30-34): 4 subtables, each an array of
-// 128-byte buckets of 16 uint64 slots; slot = (uint64)key << 32 | value, empty = 0.
-// Keys 0 (empty marker) and 0xFFFFFFFF are reserved.
+// Layout (the reference's, cuckoo.h:13-16,30-34): d subtables (d = 4 by default, 2..6 allowed),
+// each an array of 128-byte buckets of 16 uint64 slots; slot = (uint64)key << 32 | value,
+// empty = 0. Keys 0 (empty marker) and 0xFFFFFFFF are reserved.
 //
 // All functions are __host__ __device__ so the host code computes the same numbers the kernels
 // do (dump checks, the fast-division parameters).
@@ -13,7 +13,9 @@
 
 namespace dyc {
 
-constexpr int kSubtables = 4;                  // dy_hash.h:32 TABLE_NUM
+constexpr int kDefaultSubtables = 4;           // dy_hash.h:32 TABLE_NUM
+constexpr int kMinSubtables = 2;               // the n-choose-2 variants, diff_hash_num_pos_fun.cpp:14-188
+constexpr int kMaxSubtables = 6;
 constexpr int kSlots = 16;                     // dy_hash.h:34 ELEM_NUM_P_gpu = 4
 constexpr int kBucketBytes = kSlots * 8;       // 128
 constexpr uint32_t kPrime = 294967291u;        // dy_hash.h:24 PRIME_uint
@@ -76,26 +78,17 @@ __host__ __device__ inline uint32_t h5(uint32_t a)
     return a;
 }
 
-// Subtable t's hash reduced mod PRIME_uint (cuckoo.cu:80-89). The four hashes are evaluated
-// branch-free and selected: t differs lane to lane, and a divergent switch would run all four
-// paths under masks anyway.
+// Subtable t's hash reduced mod PRIME_uint: h_{t % 4} (cuckoo.cu:76-90, `switch(num_table%4)`,
+// so subtables 4 and 5 reuse h1 and h2). The four hashes are evaluated branch-free and
+// selected: t differs lane to lane, and a divergent switch would run all four paths under
+// masks anyway.
 __host__ __device__ inline uint32_t hash_t(uint32_t k, uint32_t t)
 {
-    uint32_t a = h1(k), b = h2(k), c = h3(k), d = h4(k);
-    uint32_t x = t == 0 ? a : (t == 1 ? b : (t == 2 ? c : d));
+    const uint32_t a = h1(k), b = h2(k), c = h3(k), d = h4(k);
+    const uint32_t u = t & 3u;
+    const uint32_t x = u == 0 ? a : (u == 1 ? b : (u == 2 ? c : d));
     return x % kPrime;
 }
-
-// ------------------------------------------------------------------------- pair selection
-// Two-layer scheme (cuckoo.cu:190-213,310): a key may live only in the two subtables of pair
-// h5(k) % 6; pairs 0..5 = (1,2) (0,1) (2,3) (0,2) (1,3) (0,3).
-__host__ __device__ inline uint32_t pair_of(uint32_t k) { return h5(k) % 6u; }
-__host__ __device__ inline uint32_t pos1(uint32_t p) { return (p & 1u) ? 0u : ((p & 2u) ? 2u : 1u); }
-__host__ __device__ inline uint32_t pos2(uint32_t p)
-{
-    return (p & 1u) ? (p >> 1) + 1u : ((p & 2u) ? 3u : (p >> 2) + 2u);
-}
-__host__ __device__ inline uint32_t other_pos(uint32_t p, uint32_t pre) { return pre == pos1(p) ? pos2(p) : pos1(p); }
 
 // Victim lane of the rescue phase that follows the reference kick bound: the top 4 bits of a
 // multiplicative hash of the in-hand key and the iteration. (The reference drops the entry at
@@ -148,64 +141,159 @@ __host__ __device__ inline uint32_t fast_mod(uint32_t n, uint32_t d, uint32_t m,
 // ------------------------------------------------------------------------ table descriptor
 // Passed by value as a kernel argument (the reference keeps one process-global __constant__
 // descriptor, cuckoo.cu:16, which limits a process to one table). Fields are uniform, so they
-// live in SGPRs; per-lane selection by subtable is a v_cndmask chain.
+// live in SGPRs; per-lane selection by subtable is a v_cndmask chain over the kMaxSubtables
+// entries (entries at and past d are never selected).
 struct Desc {
-    uint64_t *sub[kSubtables];   // bucket arrays, nb[t] * 16 slots each
-    uint32_t nb[kSubtables];     // buckets per subtable
-    uint32_t m[kSubtables];      // FastDiv multipliers for nb[t]
-    uint32_t s1[kSubtables];
-    uint32_t s2[kSubtables];
+    uint64_t *sub[kMaxSubtables];  // bucket arrays, nb[t] * 16 slots each
+    uint32_t nb[kMaxSubtables];    // buckets per subtable
+    uint32_t m[kMaxSubtables];     // FastDiv multipliers for nb[t]
+    uint32_t sh[kMaxSubtables];    // FastDiv shifts for nb[t]: s1 | s2 << 8
+    uint32_t d;                    // subtables in use, kMinSubtables..kMaxSubtables
+    uint32_t np;                   // pairs: d (d - 1) / 2, and 2 for d = 2
+    uint32_t pm, psh;              // FastDiv of np (multiplier, s1 | s2 << 8)
+    uint64_t p1, p2;               // pos1 / pos2 of pair p in bits [4p, 4p + 4)
 };
 
+// Values are read at constant indices first and then selected: a conditional over the array
+// elements themselves would select an address, and a descriptor indexed that way is kept in
+// scratch memory.
 template <typename T>
-__host__ __device__ inline T sel4(uint32_t t, T a, T b, T c, T d)
+__host__ __device__ inline T selt(uint32_t t, const T (&a)[kMaxSubtables])
 {
-    return t == 0 ? a : (t == 1 ? b : (t == 2 ? c : d));
+    static_assert(kMaxSubtables == 6, "selection chain below lists six entries");
+    const T a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5];
+    T x = t == 1 ? a1 : a0;
+    x = t == 2 ? a2 : x;
+    x = t == 3 ? a3 : x;
+    x = t == 4 ? a4 : x;
+    return t == 5 ? a5 : x;
+}
+
+__host__ __device__ inline uint32_t fast_mod_sh(uint32_t n, uint32_t d, uint32_t m, uint32_t sh)
+{
+    return fast_mod(n, d, m, sh & 0xffu, sh >> 8);
+}
+
+// ------------------------------------------------------------------------- pair selection
+// Two-layer scheme (cuckoo.cu:190-213,310; the other subtable counts in
+// diff_hash_num_pos_fun.cpp:14-188): a key may live only in the two subtables pos1/pos2 of its
+// pair. The pair is h5(k) % C(d,2), except for d = 2 where it is k & 1
+// (diff_hash_num_pos_fun.cpp:16-19). For d = 4 the pairs 0..5 are (1,2) (0,1) (2,3) (0,2)
+// (1,3) (0,3).
+__host__ __device__ inline uint32_t pair_of(const Desc &D, uint32_t k)
+{
+    const uint32_t src = D.d == 2u ? k : h5(k);
+    return fast_mod_sh(src, D.np, D.pm, D.psh);
+}
+__host__ __device__ inline uint32_t pos1(const Desc &D, uint32_t p) { return (uint32_t)(D.p1 >> (4u * p)) & 15u; }
+__host__ __device__ inline uint32_t pos2(const Desc &D, uint32_t p) { return (uint32_t)(D.p2 >> (4u * p)) & 15u; }
+__host__ __device__ inline uint32_t other_pos(const Desc &D, uint32_t p, uint32_t pre)
+{
+    const uint32_t a = pos1(D, p);
+    return pre == a ? pos2(D, p) : a;
+}
+
+// The reference victim vote (cuckoo.cu:396-401) takes the first slot whose resident's pair
+// index differs from the table number being written (a pair compared with a table, kept for
+// layout parity); for d = 2 the vote is over every slot (diff_hash_num_pos_fun.cpp:3-6), so
+// a full bucket always gives up slot 0.
+__host__ __device__ inline bool vote_victim(const Desc &D, uint32_t resident_key, uint32_t t)
+{
+    return D.d == 2u || pair_of(D, resident_key) != t;
 }
 
 __host__ __device__ inline uint32_t bucket_of(const Desc &D, uint32_t k, uint32_t t)
 {
-    uint32_t h = hash_t(k, t);
-    return fast_mod(h, sel4(t, D.nb[0], D.nb[1], D.nb[2], D.nb[3]), sel4(t, D.m[0], D.m[1], D.m[2], D.m[3]),
-                    sel4(t, D.s1[0], D.s1[1], D.s1[2], D.s1[3]), sel4(t, D.s2[0], D.s2[1], D.s2[2], D.s2[3]));
+    return fast_mod_sh(hash_t(k, t), selt(t, D.nb), selt(t, D.m), selt(t, D.sh));
 }
 
 // Address of slot 0 of key k's bucket in subtable t.
 __host__ __device__ inline uint64_t *bucket_ptr(const Desc &D, uint32_t k, uint32_t t)
 {
-    uint64_t *base = sel4(t, D.sub[0], D.sub[1], D.sub[2], D.sub[3]);
-    return base + (uint64_t)bucket_of(D, k, t) * kSlots;
+    return selt(t, D.sub) + (uint64_t)bucket_of(D, k, t) * kSlots;
 }
 
 // Bucket regions for ordering an insert batch: region r of the table is slice r of every
 // subtable, slice = ceil(nb[t] / R) buckets, so region(t, b) = b / slice[t] < R.
 struct Regions {
     uint32_t R;
-    uint32_t m[kSubtables], s1[kSubtables], s2[kSubtables];  // FastDiv of slice[t]
+    uint32_t m[kMaxSubtables], sh[kMaxSubtables];  // FastDiv of slice[t]
 };
 
 // Region of the bucket a key is first tried in (odd keys start in pos2, cuckoo.cu:336-341).
 __host__ __device__ inline uint32_t first_region(const Desc &D, const Regions &M, uint32_t k)
 {
-    const uint32_t p = pair_of(k);
-    const uint32_t t = (k & 1u) ? pos2(p) : pos1(p);
+    const uint32_t p = pair_of(D, k);
+    const uint32_t t = (k & 1u) ? pos2(D, p) : pos1(D, p);
     const uint32_t b = bucket_of(D, k, t);
-    return fast_div(b, sel4(t, M.m[0], M.m[1], M.m[2], M.m[3]), sel4(t, M.s1[0], M.s1[1], M.s1[2], M.s1[3]),
-                    sel4(t, M.s2[0], M.s2[1], M.s2[2], M.s2[3]));
+    const uint32_t sh = selt(t, M.sh);
+    return fast_div(b, selt(t, M.m), sh & 0xffu, sh >> 8);
 }
+
+inline uint32_t pack_shift(const FastDiv &f) { return f.s1 | (f.s2 << 8); }
 
 inline Regions make_regions(const Desc &D, uint32_t R)
 {
     Regions M{};
     M.R = R;
-    for (int t = 0; t < kSubtables; ++t) {
+    for (uint32_t t = 0; t < D.d; ++t) {
         const uint32_t slice = (D.nb[t] + R - 1) / R;
         FastDiv f = make_fastdiv(slice);
         M.m[t] = f.m;
-        M.s1[t] = f.s1;
-        M.s2[t] = f.s2;
+        M.sh[t] = pack_shift(f);
     }
     return M;
+}
+
+// The (pos1, pos2) table of every supported subtable count, pair by pair.
+//   d = 2: (0,1) (1,0)                     diff_hash_num_pos_fun.cpp:14-29 (pos1 = p, pos2 = p ^ 1)
+//   d = 3: (0,1) (1,2) (0,2)               the shift form, diff_hash_num_pos_fun.cpp:37-55
+//   d = 4: (1,2) (0,1) (2,3) (0,2) (1,3) (0,3)                         cuckoo.cu:189-213
+//   d = 5: the d = 4 pairs, then (2,4) (0,4) (1,4) (3,4)       diff_hash_num_pos_fun.cpp:119-146
+//   d = 6: every pair (a,b), a < b, in lexicographic order     diff_hash_num_pos_fun.cpp:152-188
+inline bool pair_scheme(uint32_t d, uint32_t &np, uint64_t &p1, uint64_t &p2)
+{
+    static const uint8_t k2[][2] = {{0, 1}, {1, 0}};
+    static const uint8_t k3[][2] = {{0, 1}, {1, 2}, {0, 2}};
+    static const uint8_t k4[][2] = {{1, 2}, {0, 1}, {2, 3}, {0, 2}, {1, 3}, {0, 3}};
+    static const uint8_t k5[][2] = {{1, 2}, {0, 1}, {2, 3}, {0, 2}, {1, 3}, {0, 3}, {2, 4}, {0, 4}, {1, 4}, {3, 4}};
+    uint8_t k6[15][2];
+    uint32_t q = 0;
+    for (uint8_t a = 0; a < 6; ++a)
+        for (uint8_t b = a + 1; b < 6; ++b) {
+            k6[q][0] = a;
+            k6[q][1] = b;
+            ++q;
+        }
+    const uint8_t(*tab)[2] = nullptr;
+    switch (d) {
+    case 2: tab = k2; np = 2; break;
+    case 3: tab = k3; np = 3; break;
+    case 4: tab = k4; np = 6; break;
+    case 5: tab = k5; np = 10; break;
+    case 6: tab = k6; np = 15; break;
+    default: return false;
+    }
+    p1 = p2 = 0;
+    for (uint32_t i = 0; i < np; ++i) {
+        p1 |= (uint64_t)tab[i][0] << (4 * i);
+        p2 |= (uint64_t)tab[i][1] << (4 * i);
+    }
+    return true;
+}
+
+// An empty descriptor for d subtables (false if d is unsupported).
+inline bool desc_init(Desc &D, uint32_t d)
+{
+    D = Desc{};
+    uint32_t np;
+    if (!pair_scheme(d, np, D.p1, D.p2)) return false;
+    const FastDiv f = make_fastdiv(np);
+    D.d = d;
+    D.np = np;
+    D.pm = f.m;
+    D.psh = pack_shift(f);
+    return true;
 }
 
 inline void desc_set(Desc &D, int t, uint64_t *sub, uint32_t nb)
@@ -214,8 +302,7 @@ inline void desc_set(Desc &D, int t, uint64_t *sub, uint32_t nb)
     D.sub[t] = sub;
     D.nb[t] = nb;
     D.m[t] = f.m;
-    D.s1[t] = f.s1;
-    D.s2[t] = f.s2;
+    D.sh[t] = pack_shift(f);
 }
 
 }  // namespace dyc

@@ -170,6 +170,10 @@ int dyc_group_create(const dyc_config *cfg_in, uint32_t n_tables, const uint64_t
         cfg = *cfg_in;
     else
         dyc_config_init(&cfg);
+    const uint32_t d = dyc::host::config_subtables(cfg);
+    if (d < (uint32_t)kMinSubtables || d > (uint32_t)kMaxSubtables)
+        return fail(DYC_ERR_INVALID, "subtables must be in [%d, %d]", kMinSubtables, kMaxSubtables);
+    cfg.subtables = d;
     dyc_group *g = new dyc_group();
     g->cfg = cfg;
     g->launch = dyc_group_launch{cfg.find_lanes, cfg.insert_lanes, 0, 0, kDefaultItemKeys, cfg.kick_bound};
@@ -185,12 +189,12 @@ int dyc_group_create(const dyc_config *cfg_in, uint32_t n_tables, const uint64_t
     std::vector<uint32_t> nbs(n_tables);
     size_t bytes = 0;
     for (uint32_t i = 0; i < n_tables; ++i) {
-        nbs[i] = dyc::host::initial_buckets(hints[i]);
+        nbs[i] = dyc::host::initial_buckets(hints[i], d);
         if (nbs[i] >= dyc::host::kMaxBuckets) {
             delete g;
             return fail(DYC_ERR_INVALID, "table %u: capacity hint too large", i);
         }
-        bytes += kSubtables * (((size_t)nbs[i] * kBucketBytes + 255) & ~(size_t)255);
+        bytes += d * (((size_t)nbs[i] * kBucketBytes + 255) & ~(size_t)255);
     }
     auto cleanup = [&](int code) {
         char keep[512];
@@ -208,11 +212,13 @@ int dyc_group_create(const dyc_config *cfg_in, uint32_t n_tables, const uint64_t
     }
     g->arena_bytes = bytes;
     size_t off = 0;
-    for (uint32_t i = 0; i < n_tables; ++i)
-        for (int t = 0; t < kSubtables; ++t) {
-            desc_set(g->hdir[i], t, reinterpret_cast<uint64_t *>(g->arena + off), nbs[i]);
+    for (uint32_t i = 0; i < n_tables; ++i) {
+        desc_init(g->hdir[i], d);
+        for (uint32_t t = 0; t < d; ++t) {
+            desc_set(g->hdir[i], (int)t, reinterpret_cast<uint64_t *>(g->arena + off), nbs[i]);
             off += ((size_t)nbs[i] * kBucketBytes + 255) & ~(size_t)255;
         }
+    }
     if (hipMalloc(&g->ddir, n_tables * sizeof(Desc)) != hipSuccess || hipMalloc(&g->ctr, sizeof(Counters)) != hipSuccess ||
         hipMalloc(&g->tcounts, n_tables * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void **>(&g->hctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
@@ -388,18 +394,19 @@ int dyc_group_table_stats(const dyc_group *g, uint32_t table, dyc_stats *out)
     *out = g->st;
     out->count = g->count[table];
     uint64_t cap = 0;
-    for (int t = 0; t < kSubtables; ++t) {
+    for (uint32_t t = 0; t < g->hdir[table].d; ++t) {
         out->buckets[t] = g->hdir[table].nb[t];
         cap += (uint64_t)g->hdir[table].nb[t] * kSlots;
     }
     out->capacity = cap;
+    out->subtables = g->hdir[table].d;
     return DYC_OK;
 }
 
 int dyc_group_dump(dyc_group *g, uint32_t table, uint32_t sub, uint64_t *host_slots, uint64_t n_slots,
                    dyc_stream stream)
 {
-    if (!g || table >= g->n || sub >= (uint32_t)kSubtables || !host_slots) return fail(DYC_ERR_INVALID, "bad argument");
+    if (!g || table >= g->n || !host_slots || sub >= g->hdir[table].d) return fail(DYC_ERR_INVALID, "bad argument");
     const uint64_t slots = (uint64_t)g->hdir[table].nb[sub] * kSlots;
     if (n_slots < slots) return fail(DYC_ERR_INVALID, "buffer too small");
     DeviceGuard dg(g->device);

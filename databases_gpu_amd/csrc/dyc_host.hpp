@@ -14,26 +14,34 @@ namespace host {
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 const char *last_error();
 
-constexpr uint32_t kMaxBuckets = 1u << 30;  // packed bucket locations use 30 bits
+constexpr uint32_t kMaxBuckets = 1u << 29;  // packed bucket locations use 29 bits
 
-// Buckets per subtable for a slot-count hint (api.cpp:47-56).
-uint32_t initial_buckets(uint64_t hint);
+// Buckets per subtable of a d-subtable table for a slot-count hint (api.cpp:47-56, which
+// divides by TABLE_NUM * BUCKET_SIZE).
+uint32_t initial_buckets(uint64_t hint, uint32_t d);
 
 // The subtable an upsize doubles (api.cpp:116-122): the first i with nb[i] == 2 nb[i+1] gives
 // i + 1, else subtable 0.
-inline uint32_t choose_up(const uint32_t nb[4])
+inline uint32_t choose_up(const uint32_t *nb, uint32_t d)
 {
-    for (uint32_t i = 0; i + 1 < kSubtables; ++i)
+    for (uint32_t i = 0; i + 1 < d; ++i)
         if (nb[i] == 2u * nb[i + 1]) return i + 1;
     return 0;
 }
 
-// The subtable a downsize halves (api.cpp:147-153): the first i with nb[i] > nb[i+1], else 3.
-inline uint32_t choose_down(const uint32_t nb[4])
+// The subtable a downsize halves (api.cpp:147-153): the first i with nb[i] > nb[i+1], else the
+// last one.
+inline uint32_t choose_down(const uint32_t *nb, uint32_t d)
 {
-    for (uint32_t i = 0; i + 1 < kSubtables; ++i)
+    for (uint32_t i = 0; i + 1 < d; ++i)
         if (nb[i] > nb[i + 1]) return i;
-    return kSubtables - 1;
+    return d - 1;
+}
+
+// The subtable count a configuration asks for (0 means the default).
+inline uint32_t config_subtables(const dyc_config &cfg)
+{
+    return cfg.subtables ? cfg.subtables : (uint32_t)kDefaultSubtables;
 }
 
 struct DeviceGuard {

@@ -34,15 +34,15 @@ __device__ inline uint64_t shfl64(uint64_t v, int src)
     return ((uint64_t)hi << 32) | lo;
 }
 
-// A bucket location packed in 32 bits: subtable in the top 2 bits, bucket index below (bucket
-// counts are capped at 2^30). Shuffling this instead of a 64-bit address halves the cross-lane
+// A bucket location packed in 32 bits: subtable in the top 3 bits, bucket index below (bucket
+// counts are capped at 2^29). Shuffling this instead of a 64-bit address halves the cross-lane
 // traffic, and rebuilding the pointer from the descriptor keeps the loads in the global
 // address space (a pointer rebuilt from a shuffled integer would become a flat access, whose
 // completion the compiler can only track together with the shuffles).
-__device__ inline uint32_t pack_loc(uint32_t t, uint32_t bucket) { return (t << 30) | bucket; }
+__device__ inline uint32_t pack_loc(uint32_t t, uint32_t bucket) { return (t << 29) | bucket; }
 __device__ inline uint64_t *loc_ptr(const Desc &D, uint32_t l)
 {
-    return sel4(l >> 30, D.sub[0], D.sub[1], D.sub[2], D.sub[3]) + (uint64_t)(l & 0x3fffffffu) * kSlots;
+    return selt(l >> 29, D.sub) + (uint64_t)(l & 0x1fffffffu) * kSlots;
 }
 
 // Plain (non-atomic) read of SPL consecutive slots, 16-byte vector loads where possible.
@@ -124,9 +124,10 @@ __device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restric
         const bool inb = i < n;
         const uint32_t k = inb ? keys[i] : 0u;  // past the end: key 0, which is reserved
         const uint32_t nv = (UPDATE && inb) ? vals[i] : 0u;
-        const uint32_t p = pair_of(k);
-        const uint32_t loc1 = pack_loc(pos1(p), bucket_of(D, k, pos1(p)));
-        const uint32_t loc2 = pack_loc(pos2(p), bucket_of(D, k, pos2(p)));
+        const uint32_t p = pair_of(D, k);
+        const uint32_t t1 = pos1(D, p), t2 = pos2(D, p);
+        const uint32_t loc1 = pack_loc(t1, bucket_of(D, k, t1));
+        const uint32_t loc2 = pack_loc(t2, bucket_of(D, k, t2));
 
         uint64_t e[G][SPL];
         uint32_t kk_key[G];
@@ -252,8 +253,16 @@ __device__ inline uint64_t pool_size(const Pool &pool)
 
 // pool.count == nullptr: the table has no overflow pool (a plain table); otherwise misses also
 // look in the pool entries tagged `table` (a serving loop's table).
+// Waves per SIMD the find / erase kernels are held to (the occupancy their register use gives
+// with a four-subtable selection; the six-way one would otherwise cost a wave at G = 8).
 template <int G, bool ERASE>
-__global__ __launch_bounds__(256) void probe_kernel(Desc D, const uint32_t *__restrict__ keys,
+constexpr int probe_waves()
+{
+    return G == 4 ? (ERASE ? 6 : 7) : (G == 8 ? (ERASE ? 5 : 7) : (ERASE ? 4 : 5));
+}
+
+template <int G, bool ERASE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(probe_waves<G, ERASE>()))) void probe_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                     uint32_t *__restrict__ vals, uint8_t *__restrict__ flags,
                                                     uint64_t n, Counters *__restrict__ C, Pool pool, uint32_t table)
 {
@@ -317,14 +326,14 @@ struct InsertTally {
 
 // Take entry e (previous table pr, kNoTable for a fresh key) into the group's hand.
 template <int SPL>
-__device__ inline void carry_start(Carry<SPL> &c, uint64_t e, uint32_t pr, int s, InsertTally &tally)
+__device__ inline void carry_start(const Desc &D, Carry<SPL> &c, uint64_t e, uint32_t pr, int s, InsertTally &tally)
 {
     const uint32_t k = key_of(e);
     if (reserved_key(k)) {
         if (s == 0) ++tally.rejected;
         return;
     }
-    if (pr == kNoTable && (k & 1u)) pr = pos1(pair_of(k));  // odd keys start in pos2 (cuckoo.cu:336-341)
+    if (pr == kNoTable && (k & 1u)) pr = pos1(D, pair_of(D, k));  // odd keys start in pos2 (cuckoo.cu:336-341)
     c.entry = e;
     c.pre = pr;
     c.kicks = 0;
@@ -361,7 +370,7 @@ __device__ inline void kick_step(const Desc &D, Carry<kSlots / G> &c, const Inse
     };
 
     const uint32_t k = key_of(c.entry);
-    const uint32_t t = other_pos(pair_of(k), c.pre);
+    const uint32_t t = other_pos(D, pair_of(D, k), c.pre);
     uint64_t *b = bucket_ptr(D, k, t);
     if (c.fresh) load_slots<SPL>(b + s * SPL, c.cur);
     c.fresh = false;
@@ -380,7 +389,7 @@ __device__ inline void kick_step(const Desc &D, Carry<kSlots / G> &c, const Inse
         int fo = -1;
 #pragma unroll
         for (int j = SPL - 1; j >= 0; --j)
-            if (pair_of(key_of(c.cur[j])) != t) fo = j;
+            if (vote_victim(D, key_of(c.cur[j]), t)) fo = j;
         gm = (uint32_t)((__ballot(fo >= 0) >> (g * G)) & GMASK);
         if (gm) {
             const int w = __builtin_ctz(gm);
@@ -434,7 +443,7 @@ __device__ inline void kick_step(const Desc &D, Carry<kSlots / G> &c, const Inse
 // Feed the wave's idle groups from input indexes [next, end), advancing next; returns when every
 // group is busy or the range is used up (some groups may then still be idle).
 template <int G, bool FROM_ENTRIES>
-__device__ inline void refill(Carry<kSlots / G> &c, uint64_t &next, uint64_t end, const uint32_t *__restrict__ keys,
+__device__ inline void refill(const Desc &D, Carry<kSlots / G> &c, uint64_t &next, uint64_t end, const uint32_t *__restrict__ keys,
                               const uint32_t *__restrict__ vals, const uint64_t *__restrict__ entries,
                               uint32_t pre_fixed, InsertTally &tally, uint64_t &taken,
                               const uint8_t *__restrict__ skip = nullptr)
@@ -451,9 +460,9 @@ __device__ inline void refill(Carry<kSlots / G> &c, uint64_t &next, uint64_t end
         if (!c.active && rank < take) {
             const uint64_t idx = next + rank;
             if constexpr (FROM_ENTRIES)
-                carry_start(c, entries[idx], pre_fixed, s, tally);
+                carry_start(D, c, entries[idx], pre_fixed, s, tally);
             else if (!skip || !skip[idx])
-                carry_start(c, pack(keys[idx], vals[idx]), kNoTable, s, tally);
+                carry_start(D, c, pack(keys[idx], vals[idx]), kNoTable, s, tally);
         }
         next += take;
         taken += take;
@@ -479,7 +488,7 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
     uint64_t next = 0, end = 0;  // wave-uniform: the wave's current chunk
     uint64_t taken = 0;          // wave-uniform: input indexes this wave took
     while (true) {
-        refill<G, FROM_ENTRIES>(c, next, end, keys, vals, entries, pre_fixed, tally, taken, skip);
+        refill<G, FROM_ENTRIES>(D, c, next, end, keys, vals, entries, pre_fixed, tally, taken, skip);
         if (next == end) {
             // chunks w, w + W, w + 2W, ... (W = waves in the grid)
             const uint64_t base = next_chunk * kChunk;
@@ -732,7 +741,7 @@ __global__ __launch_bounds__(256) void probe_count_kernel(Desc D, const uint32_t
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t k = keys[i];
         if (reserved_key(k)) continue;
-        const uint64_t *b = bucket_ptr(D, k, pos1(pair_of(k)));
+        const uint64_t *b = bucket_ptr(D, k, pos1(D, pair_of(D, k)));
         bool hit = false;
         for (int j = 0; j < kSlots; ++j) hit |= key_of(b[j]) == k;
         probes += hit ? 1 : 2;
@@ -794,7 +803,7 @@ __global__ __launch_bounds__(256) void group_insert_kernel(const Desc *__restric
         const unsigned long long rej0 = tally.rejected;
         uint64_t taken = 0;
         while (true) {
-            refill<G, false>(c, next, end, keys, vals, nullptr, kNoTable, tally, taken);
+            refill<G, false>(D, c, next, end, keys, vals, nullptr, kNoTable, tally, taken);
             if (!__ballot(c.active)) break;  // item used up and every group idle
             if (c.active) kick_step<G>(D, c, P, C, sink, item.table, tally);
         }
@@ -868,7 +877,7 @@ __device__ inline void serve_insert_item(const Desc &D, const ServeItem &item, C
     const unsigned long long rej0 = tally.rejected;
     uint64_t taken = 0;
     while (true) {
-        refill<G, false>(c, next, end, item.keys, item.vals, nullptr, kNoTable, tally, taken);
+        refill<G, false>(D, c, next, end, item.keys, item.vals, nullptr, kNoTable, tally, taken);
         if (!__ballot(c.active)) break;
         if (c.active) kick_step<G>(D, c, P, C, sink, item.table, tally);
     }

@@ -52,7 +52,7 @@ static_assert(sizeof(ServeItem) % 8 == 0 && sizeof(TickItem) % 8 == 0 && sizeof(
 
 struct SrvTable {
     Desc D{};
-    uint64_t off[kSubtables] = {};  // arena offsets of the subtables
+    uint64_t off[kMaxSubtables] = {};  // arena offsets of the subtables
     uint64_t count = 0;             // live entries as of the last retired tick
     uint64_t inflight_ins = 0;      // insert keys admitted in ticks not yet retired
     uint32_t inflight = 0;          // ticks in flight that touch the table
@@ -63,7 +63,7 @@ struct SrvTable {
     uint64_t capacity() const
     {
         uint64_t c = 0;
-        for (int i = 0; i < kSubtables; ++i) c += (uint64_t)D.nb[i] * kSlots;
+        for (uint32_t i = 0; i < D.d; ++i) c += (uint64_t)D.nb[i] * kSlots;
         return c;
     }
 };
@@ -185,7 +185,7 @@ int upsize_now(dyc_server *s, uint32_t i, bool *ok)
 {
     SrvTable &T = s->t[i];
     *ok = false;
-    const uint32_t sub = choose_up(T.D.nb);
+    const uint32_t sub = choose_up(T.D.nb, T.D.d);
     const uint32_t old_nb = T.D.nb[sub];
     if ((uint64_t)old_nb * 2 >= kMaxBuckets) return DYC_OK;
     const uint64_t off = s->arena.alloc((uint64_t)old_nb * 2 * kBucketBytes);
@@ -208,7 +208,7 @@ int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
 {
     SrvTable &T = s->t[i];
     *done = false;
-    const uint32_t sub = choose_down(T.D.nb);
+    const uint32_t sub = choose_down(T.D.nb, T.D.d);
     const uint32_t old_nb = T.D.nb[sub];
     if ((old_nb & 1u) || old_nb < 2) return DYC_OK;
     const uint32_t new_nb = old_nb / 2;
@@ -931,10 +931,12 @@ int create(dyc_server *s, uint32_t n_tables, const uint64_t *hints)
         HIP_TRY(hipEventCreateWithFlags(&k.done, hipEventDisableTiming));
     }
     for (uint32_t i = 0; i < n_tables; ++i) {
-        const uint32_t nb = initial_buckets(hints[i]);
+        const uint32_t d = s->cfg.subtables;
+        const uint32_t nb = initial_buckets(hints[i], d);
         if (nb == 0 || nb >= kMaxBuckets) return fail(DYC_ERR_INVALID, "table %u: capacity hint out of range", i);
         SrvTable &T = s->t[i];
-        for (int sub = 0; sub < kSubtables; ++sub) {
+        desc_init(T.D, d);
+        for (int sub = 0; sub < (int)d; ++sub) {
             const uint64_t off = s->arena.alloc((uint64_t)nb * kBucketBytes);
             if (off == Arena::kNone)
                 return fail(DYC_ERR_OOM, "table %u does not fit in the %llu-byte HBM budget", i,
@@ -988,6 +990,9 @@ int dyc_server_create(const dyc_config *cfg, const dyc_server_config *scfg, uint
                           (c.insert_lanes == 1 || c.insert_lanes == 2 || c.insert_lanes == 4 ||
                            c.insert_lanes == 8 || c.insert_lanes == 16);
     if (!lanes_ok) return fail(DYC_ERR_INVALID, "lanes per key: find 4/8/16, insert 1/2/4/8/16");
+    c.subtables = dyc::host::config_subtables(c);
+    if (c.subtables < (uint32_t)kMinSubtables || c.subtables > (uint32_t)kMaxSubtables)
+        return fail(DYC_ERR_INVALID, "subtables must be in [%d, %d]", kMinSubtables, kMaxSubtables);
     if (c.block_threads == 0 || c.block_threads % 64 || c.block_threads > 256)
         return fail(DYC_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 256]");
     if (!(c.upper_bound > 0.0 && c.upper_bound <= 1.0) || !(c.lower_bound >= 0.0) || c.lower_bound >= c.upper_bound)
@@ -1138,7 +1143,8 @@ int dyc_server_table_stats(const dyc_server *s, uint32_t table, dyc_stats *out)
     memset(out, 0, sizeof *out);
     out->count = T.count;
     out->capacity = T.capacity();
-    for (int i = 0; i < kSubtables; ++i) out->buckets[i] = T.D.nb[i];
+    for (uint32_t i = 0; i < T.D.d; ++i) out->buckets[i] = T.D.nb[i];
+    out->subtables = T.D.d;
     out->resizes_up = T.resizes_up;
     out->resizes_down = T.resizes_down;
     return DYC_OK;
@@ -1147,7 +1153,7 @@ int dyc_server_table_stats(const dyc_server *s, uint32_t table, dyc_stats *out)
 int dyc_server_dump(dyc_server *s, uint32_t table, uint32_t sub, uint64_t *host_slots, uint64_t n_slots)
 {
     if (!s || !host_slots) return fail(DYC_ERR_INVALID, "null argument");
-    if (table >= s->n || sub >= (uint32_t)kSubtables) return fail(DYC_ERR_INVALID, "table or subtable out of range");
+    if (table >= s->n || sub >= s->t[table].D.d) return fail(DYC_ERR_INVALID, "table or subtable out of range");
     DeviceGuard g(s->device);
     int rc = drain(s);
     if (rc) return rc;

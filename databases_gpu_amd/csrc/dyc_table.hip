@@ -18,6 +18,7 @@
 using namespace dyc;
 using dyc::host::choose_down;
 using dyc::host::choose_up;
+using dyc::host::config_subtables;
 using dyc::host::DeviceGuard;
 using dyc::host::fail;
 using dyc::host::initial_buckets;
@@ -52,10 +53,10 @@ int fail(int code, const char *fmt, ...)
 
 const char *last_error() { return g_err; }
 
-// api.cpp:47-51: s = nextPrime(size / 64), then rounded up to an even bucket count
-uint32_t initial_buckets(uint64_t hint)
+// api.cpp:47-51: s = nextPrime(size / (TABLE_NUM * 16)), then rounded up to an even bucket count
+uint32_t initial_buckets(uint64_t hint, uint32_t d)
 {
-    uint32_t s = (uint32_t)std::min<uint64_t>(hint / (kSubtables * kSlots), 0x7ffffff0u);
+    uint32_t s = (uint32_t)std::min<uint64_t>(hint / ((uint64_t)d * kSlots), 0x7ffffff0u);
     while (!is_prime_ref(s)) ++s;
     return (s & 1u) ? s + 1u : s;
 }
@@ -88,7 +89,7 @@ struct dyc_table {
     uint64_t capacity() const
     {
         uint64_t c = 0;
-        for (int i = 0; i < kSubtables; ++i) c += (uint64_t)D.nb[i] * kSlots;
+        for (uint32_t i = 0; i < D.d; ++i) c += (uint64_t)D.nb[i] * kSlots;
         return c;
     }
     LaunchShape shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.max_blocks}; }
@@ -161,9 +162,9 @@ int absorb_insert_counters(dyc_table *t, uint64_t room)
 
 int do_resize_up(dyc_table *t, hipStream_t s)
 {
-    const uint32_t sub = choose_up(t->D.nb);
+    const uint32_t sub = choose_up(t->D.nb, t->D.d);
     const uint32_t old_nb = t->D.nb[sub];
-    if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^30 buckets", sub);
+    if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^29 buckets", sub);
     uint64_t *dst = nullptr;
     const size_t bytes = (size_t)old_nb * 2 * kBucketBytes;
     if (hipMalloc(&dst, bytes) != hipSuccess) {
@@ -222,7 +223,7 @@ int place_pending(dyc_table *t, hipStream_t s)
 
 int do_resize_down(dyc_table *t, hipStream_t s)
 {
-    const uint32_t sub = choose_down(t->D.nb);
+    const uint32_t sub = choose_down(t->D.nb, t->D.d);
     const uint32_t old_nb = t->D.nb[sub];
     if (old_nb & 1u)
         return fail(DYC_ERR_ODD_BUCKETS, "downsize of subtable %u refused: %u buckets is odd", sub, old_nb);
@@ -325,6 +326,9 @@ int check_config(const dyc_config &cfg)
         return fail(DYC_ERR_INVALID, "need 0 <= lower_bound < upper_bound <= 1");
     if (cfg.insert_order > 2 || cfg.region_mib == 0)
         return fail(DYC_ERR_INVALID, "insert_order must be 0, 1 or 2 and region_mib > 0");
+    const uint32_t d = config_subtables(cfg);
+    if (d < (uint32_t)kMinSubtables || d > (uint32_t)kMaxSubtables || cfg.reserved0 != 0)
+        return fail(DYC_ERR_INVALID, "subtables must be in [%d, %d] (reserved0 = 0)", kMinSubtables, kMaxSubtables);
     return DYC_OK;
 }
 
@@ -411,21 +415,23 @@ void host_copy(void *dst, const void *src, size_t bytes)
     for (auto &x : th) x.join();
 }
 
-// ---- checkpoint file: a 128-byte header, the four subtable images, then the pending entries,
+// ---- checkpoint file: a 128-byte header, the d subtable images, then the pending entries,
 // all little-endian; FNV-1a (64-bit, over 8-byte words) guards each part and the header.
 struct CkptHeader {
     char magic[8];
     uint32_t version;
     uint32_t subtables;
-    uint32_t buckets[4];
+    uint32_t buckets[kMaxSubtables];  // entries at and past `subtables` are 0
     uint64_t count;
     uint64_t pending;
-    uint64_t checksum[5];  // subtables 0..3, then the pending list
-    uint64_t header_check;  // over every byte before this field
-    uint8_t reserved[32];
+    uint64_t checksum[kMaxSubtables];  // subtable images (0 past `subtables`)
+    uint64_t pending_check;            // the pending list
+    uint64_t header_check;             // over every byte before this field
+    uint8_t reserved[8];
 };
 static_assert(sizeof(CkptHeader) == 128, "checkpoint header is 128 bytes");
 constexpr char kCkptMagic[8] = {'D', 'Y', 'C', 'K', 'P', 'T', '0', '1'};
+constexpr uint32_t kCkptVersion = 2;
 constexpr uint64_t kFnvBasis = 0xcbf29ce484222325ull, kFnvPrime = 0x100000001b3ull;
 constexpr uint64_t kCkptChunk = 1ull << 23;  // 8-byte words per staged chunk (64 MiB)
 
@@ -548,6 +554,7 @@ int dyc_config_init(dyc_config *cfg)
     cfg->min_capacity = 5033164ull;  // TABLE_NUM * 1024 * 1024 * 1.2 (api.cpp:496)
     cfg->insert_order = 0;
     cfg->region_mib = 64;
+    cfg->subtables = kDefaultSubtables;
     return DYC_OK;
 }
 
@@ -562,11 +569,13 @@ int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out
         dyc_config_init(&cfg);
     int rc0 = check_config(cfg);
     if (rc0) return rc0;
-    const uint32_t nb = initial_buckets(capacity_hint);
+    cfg.subtables = config_subtables(cfg);
+    const uint32_t nb = initial_buckets(capacity_hint, cfg.subtables);
     if (nb >= kMaxBuckets) return fail(DYC_ERR_INVALID, "capacity hint too large");
 
     dyc_table *t = new dyc_table();
     t->cfg = cfg;
+    desc_init(t->D, cfg.subtables);
     HIP_TRY(hipGetDevice(&t->device));
     int rc = DYC_OK;
     if (hipMalloc(&t->ctr, sizeof(Counters)) != hipSuccess ||
@@ -574,7 +583,7 @@ int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out
         (void)hipGetLastError();
         rc = fail(DYC_ERR_OOM, "counter allocation failed");
     }
-    for (int i = 0; i < kSubtables && rc == DYC_OK; ++i) {
+    for (uint32_t i = 0; i < t->D.d && rc == DYC_OK; ++i) {
         uint64_t *p = nullptr;
         rc = alloc_zeroed_sub(&p, nb, nullptr);
         if (rc == DYC_OK) desc_set(t->D, i, p, nb);
@@ -597,7 +606,7 @@ int dyc_destroy(dyc_table *t)
     if (!t) return DYC_OK;
     DeviceGuard dg(t->device);
     (void)hipDeviceSynchronize();
-    for (int i = 0; i < kSubtables; ++i)
+    for (uint32_t i = 0; i < t->D.d; ++i)
         if (t->D.sub[i]) (void)hipFree(t->D.sub[i]);
     if (t->pend) (void)hipFree(t->pend);
     if (t->ctr) (void)hipFree(t->ctr);
@@ -765,7 +774,10 @@ int dyc_set_config(dyc_table *t, const dyc_config *cfg)
     if (!t || !cfg) return fail(DYC_ERR_INVALID, "null argument");
     int rc = check_config(*cfg);
     if (rc) return rc;
+    if (config_subtables(*cfg) != t->D.d)
+        return fail(DYC_ERR_INVALID, "subtables is fixed at create (%u)", t->D.d);
     t->cfg = *cfg;
+    t->cfg.subtables = t->D.d;
     return DYC_OK;
 }
 
@@ -775,8 +787,9 @@ int dyc_get_stats(const dyc_table *t, dyc_stats *out)
     *out = t->st;
     out->count = t->count;
     out->capacity = t->capacity();
-    for (int i = 0; i < kSubtables; ++i) out->buckets[i] = t->D.nb[i];
+    for (uint32_t i = 0; i < t->D.d; ++i) out->buckets[i] = t->D.nb[i];
     out->pending = t->pend_n;
+    out->subtables = t->D.d;
     return DYC_OK;
 }
 
@@ -787,7 +800,7 @@ int dyc_count_occupied(dyc_table *t, uint64_t *occupied, dyc_stream stream)
     hipStream_t s = (hipStream_t)stream;
     int rc = zero_counters(t, s);
     if (rc) return rc;
-    for (int i = 0; i < kSubtables; ++i) HIP_TRY(launch_count(t->D.sub[i], (uint64_t)t->D.nb[i] * kSlots, t->ctr, s));
+    for (uint32_t i = 0; i < t->D.d; ++i) HIP_TRY(launch_count(t->D.sub[i], (uint64_t)t->D.nb[i] * kSlots, t->ctr, s));
     rc = read_counters(t, s);
     if (rc) return rc;
     *occupied = t->hctr->occupied;
@@ -799,7 +812,7 @@ int dyc_clear(dyc_table *t, dyc_stream stream)
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     DeviceGuard dg(t->device);
     hipStream_t s = (hipStream_t)stream;
-    for (int i = 0; i < kSubtables; ++i)
+    for (uint32_t i = 0; i < t->D.d; ++i)
         HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->D.sub[i]), (uint64_t)t->D.nb[i] * kSlots, s));
     t->count = 0;
     t->pend_n = 0;
@@ -825,7 +838,7 @@ int dyc_probe_count(dyc_table *t, const uint32_t *keys, uint64_t n, uint64_t *pr
 
 int dyc_dump(dyc_table *t, uint32_t sub, uint64_t *host_slots, uint64_t n_slots, dyc_stream stream)
 {
-    if (!t || !host_slots || sub >= (uint32_t)kSubtables) return fail(DYC_ERR_INVALID, "bad argument");
+    if (!t || !host_slots || sub >= t->D.d) return fail(DYC_ERR_INVALID, "bad argument");
     const uint64_t slots = (uint64_t)t->D.nb[sub] * kSlots;
     if (n_slots < slots) return fail(DYC_ERR_INVALID, "buffer holds %llu slots, subtable has %llu",
                                      (unsigned long long)n_slots, (unsigned long long)slots);
@@ -836,16 +849,16 @@ int dyc_dump(dyc_table *t, uint32_t sub, uint64_t *host_slots, uint64_t n_slots,
     return DYC_OK;
 }
 
-int dyc_load(dyc_table *t, const uint32_t nb[4], const uint64_t *const images[4], dyc_stream stream)
+int dyc_load(dyc_table *t, const uint32_t *nb, const uint64_t *const *images, dyc_stream stream)
 {
     if (!t || !nb || !images) return fail(DYC_ERR_INVALID, "null argument");
-    for (int i = 0; i < kSubtables; ++i)
+    for (uint32_t i = 0; i < t->D.d; ++i)
         if (nb[i] == 0 || nb[i] >= kMaxBuckets || !images[i]) return fail(DYC_ERR_INVALID, "bad subtable %d", i);
     DeviceGuard dg(t->device);
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipStreamSynchronize(s));
     uint64_t count = 0;
-    for (int i = 0; i < kSubtables; ++i) {
+    for (uint32_t i = 0; i < t->D.d; ++i) {
         uint64_t *p = nullptr;
         const size_t bytes = (size_t)nb[i] * kBucketBytes;
         if (hipMalloc(&p, bytes) != hipSuccess) {
@@ -1029,17 +1042,17 @@ int dyc_checkpoint_write(dyc_table *t, const char *path, dyc_stream stream)
     CkptHeader h;
     memset(&h, 0, sizeof h);
     memcpy(h.magic, kCkptMagic, 8);
-    h.version = 1;
-    h.subtables = kSubtables;
-    for (int i = 0; i < kSubtables; ++i) h.buckets[i] = t->D.nb[i];
+    h.version = kCkptVersion;
+    h.subtables = t->D.d;
+    for (uint32_t i = 0; i < t->D.d; ++i) h.buckets[i] = t->D.nb[i];
     h.count = t->count;
     h.pending = t->pend_n;
     int rc = fwrite(&h, sizeof h, 1, f) == 1 ? DYC_OK : fail(DYC_ERR_INVALID, "checkpoint write failed");
     PinnedPair pp;
     if (rc == DYC_OK && !pp.alloc(kCkptChunk * 8)) rc = fail(DYC_ERR_OOM, "pinned staging for the checkpoint");
-    for (int i = 0; i < kSubtables && rc == DYC_OK; ++i)
+    for (uint32_t i = 0; i < t->D.d && rc == DYC_OK; ++i)
         rc = dump_to_file(f, t->D.sub[i], (uint64_t)t->D.nb[i] * kSlots, pp, &h.checksum[i], s);
-    if (rc == DYC_OK) rc = dump_to_file(f, t->pend, t->pend_n, pp, &h.checksum[4], s);
+    if (rc == DYC_OK) rc = dump_to_file(f, t->pend, t->pend_n, pp, &h.pending_check, s);
     h.header_check = fnv_bytes(&h, offsetof(CkptHeader, header_check));
     if (rc == DYC_OK && (fseek(f, 0, SEEK_SET) != 0 || fwrite(&h, sizeof h, 1, f) != 1))
         rc = fail(DYC_ERR_INVALID, "checkpoint header write failed");
@@ -1055,12 +1068,13 @@ int dyc_checkpoint_read(const char *path, const dyc_config *cfg, dyc_table **out
     if (!f) return fail(DYC_ERR_INVALID, "cannot open %s", path);
     CkptHeader h;
     int rc = fread(&h, sizeof h, 1, f) == 1 ? DYC_OK : fail(DYC_ERR_CORRUPT, "checkpoint header is truncated");
-    if (rc == DYC_OK && (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1 || h.subtables != kSubtables))
+    if (rc == DYC_OK && (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != kCkptVersion ||
+                         h.subtables < (uint32_t)kMinSubtables || h.subtables > (uint32_t)kMaxSubtables))
         rc = fail(DYC_ERR_CORRUPT, "not a DyCuckoo checkpoint (magic/version/subtables)");
     if (rc == DYC_OK && h.header_check != fnv_bytes(&h, offsetof(CkptHeader, header_check)))
         rc = fail(DYC_ERR_CORRUPT, "checkpoint header checksum mismatch");
     uint64_t slots = 0;
-    for (int i = 0; i < kSubtables && rc == DYC_OK; ++i) {
+    for (uint32_t i = 0; i < h.subtables && rc == DYC_OK; ++i) {
         if (h.buckets[i] == 0 || h.buckets[i] >= kMaxBuckets) rc = fail(DYC_ERR_CORRUPT, "bad bucket count");
         slots += (uint64_t)h.buckets[i] * kSlots;
     }
@@ -1068,10 +1082,16 @@ int dyc_checkpoint_read(const char *path, const dyc_config *cfg, dyc_table **out
                          (uint64_t)ftell(f) != sizeof h + (slots + h.pending) * 8 || fseek(f, sizeof h, SEEK_SET) != 0))
         rc = fail(DYC_ERR_CORRUPT, "checkpoint size does not match its header");
     dyc_table *t = nullptr;
-    if (rc == DYC_OK) rc = dyc_create(cfg, 64, &t);
+    dyc_config c;
+    if (cfg)
+        c = *cfg;
+    else
+        dyc_config_init(&c);
+    c.subtables = h.subtables;
+    if (rc == DYC_OK) rc = dyc_create(&c, 64, &t);
     PinnedPair pp;
     if (rc == DYC_OK && !pp.alloc(kCkptChunk * 8)) rc = fail(DYC_ERR_OOM, "pinned staging for the restore");
-    for (int i = 0; i < kSubtables && rc == DYC_OK; ++i) {
+    for (uint32_t i = 0; i < h.subtables && rc == DYC_OK; ++i) {
         uint64_t *p = nullptr;
         if (hipMalloc(&p, (size_t)h.buckets[i] * kBucketBytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -1082,13 +1102,13 @@ int dyc_checkpoint_read(const char *path, const dyc_config *cfg, dyc_table **out
         desc_set(t->D, i, p, h.buckets[i]);
         uint64_t hash = 0;
         rc = load_from_file(f, p, (uint64_t)h.buckets[i] * kSlots, pp, &hash, nullptr);
-        if (rc == DYC_OK && hash != h.checksum[i]) rc = fail(DYC_ERR_CORRUPT, "subtable %d checksum mismatch", i);
+        if (rc == DYC_OK && hash != h.checksum[i]) rc = fail(DYC_ERR_CORRUPT, "subtable %u checksum mismatch", i);
     }
     if (rc == DYC_OK) rc = reserve_pending(t, h.pending, nullptr);
     if (rc == DYC_OK) {
         uint64_t hash = 0;
         rc = load_from_file(f, t->pend, h.pending, pp, &hash, nullptr);
-        if (rc == DYC_OK && hash != h.checksum[4]) rc = fail(DYC_ERR_CORRUPT, "pending list checksum mismatch");
+        if (rc == DYC_OK && hash != h.pending_check) rc = fail(DYC_ERR_CORRUPT, "pending list checksum mismatch");
     }
     fclose(f);
     if (rc == DYC_OK) {

@@ -15,8 +15,10 @@ import struct
 import numpy as np
 
 CKPT_MAGIC = b"DYCKPT01"
-# magic, version, subtables, buckets[4], count, pending, checksum[5], header_check, reserved[32]
-_CKPT_HEADER = struct.Struct("<8sII4IQQ5QQ32s")
+CKPT_VERSION = 2
+# magic, version, subtables d, buckets[6], count, pending, checksum[6] (subtable images),
+# pending-list checksum, header_check, reserved[8]; entries at and past d are 0
+_CKPT_HEADER = struct.Struct("<8sII6IQQ6QQQ8s")
 assert _CKPT_HEADER.size == 128
 
 
@@ -54,11 +56,15 @@ def checkpoint_header(path: str) -> dict:
         raw = f.read(_CKPT_HEADER.size)
     if len(raw) != _CKPT_HEADER.size:
         raise ValueError(f"{path}: truncated checkpoint header")
-    magic, version, subtables, b0, b1, b2, b3, count, pending, c0, c1, c2, c3, c4, hcheck, _ = _CKPT_HEADER.unpack(raw)
+    f = _CKPT_HEADER.unpack(raw)
+    magic, version, d = f[0], f[1], f[2]
     if magic != CKPT_MAGIC:
         raise ValueError(f"{path}: not a DyCuckoo checkpoint")
-    return {"version": version, "subtables": subtables, "buckets": [b0, b1, b2, b3], "count": count,
-            "pending": pending, "checksums": [c0, c1, c2, c3, c4], "header_check": hcheck}
+    if not 2 <= d <= 6:
+        raise ValueError(f"{path}: subtable count {d} out of range")
+    buckets, (count, pending), sums = f[3:9], f[9:11], f[11:17]
+    return {"version": version, "subtables": d, "buckets": list(buckets[:d]), "count": count,
+            "pending": pending, "checksums": list(sums[:d]), "pending_checksum": f[17], "header_check": f[18]}
 
 
 def fnv1a64_words(words: np.ndarray, h: int = 0xCBF29CE484222325) -> int:

@@ -119,10 +119,10 @@ class DyCuckooServer:
         return st.as_dict()
 
     def dump(self, table: int):
-        """The table's four subtable images (host uint64 arrays); drains the server first."""
+        """The table's subtable images (host uint64 arrays); drains the server first."""
         st = self.table_stats(table)
         out = []
-        for sub in range(_lib.SUBTABLES):
+        for sub in range(st["subtables"]):
             img = np.zeros(st["buckets"][sub] * _lib.SLOTS, dtype=np.uint64)
             check(self.lib.dyc_server_dump(self.h, table, sub, img.ctypes.data, img.size), "dyc_server_dump")
             out.append(img)

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import SLOTS, SUBTABLES, DycError, check
+from ._lib import SLOTS, DycError, check
 
 _U32 = torch.int32  # torch has no uint32 arithmetic on GPU; bits are reinterpreted as uint32
 
@@ -56,7 +56,8 @@ def _stream_ptr(stream) -> ctypes.c_void_p:
 
 
 class DyCuckoo:
-    """A dynamic 4-subtable cuckoo hash table of uint32 -> uint32 on one GPU."""
+    """A dynamic cuckoo hash table of uint32 -> uint32 on one GPU: 4 subtables by default,
+    2..6 with ``subtables=d`` (a configuration field)."""
 
     def __init__(self, capacity_hint: int, *, device=None, **config):
         self.lib = _lib.load()
@@ -223,6 +224,10 @@ class DyCuckoo:
         return self.stats()["buckets"]
 
     @property
+    def subtables(self) -> int:
+        return self.stats()["subtables"]
+
+    @property
     def capacity(self) -> int:
         return self.stats()["capacity"]
 
@@ -250,7 +255,7 @@ class DyCuckoo:
         return probes.value, hits.value
 
     def dump(self, stream=None):
-        """Host copies of the four subtables (uint64 slots, (key << 32) | value)."""
+        """Host copies of the subtables (uint64 slots, (key << 32) | value)."""
         out = []
         for t, nb in enumerate(self.buckets):
             a = np.empty(nb * SLOTS, dtype=np.uint64)
@@ -261,10 +266,11 @@ class DyCuckoo:
 
     def load(self, images, stream=None) -> None:
         imgs = [np.ascontiguousarray(x, dtype=np.uint64) for x in images]
-        if len(imgs) != SUBTABLES or any(x.size % SLOTS or x.size == 0 for x in imgs):
-            raise ValueError("need 4 non-empty images of whole buckets")
-        nb = (ctypes.c_uint32 * SUBTABLES)(*[x.size // SLOTS for x in imgs])
-        ptrs = (ctypes.c_void_p * SUBTABLES)(*[x.ctypes.data for x in imgs])
+        d = self.subtables
+        if len(imgs) != d or any(x.size % SLOTS or x.size == 0 for x in imgs):
+            raise ValueError(f"need {d} non-empty images of whole buckets")
+        nb = (ctypes.c_uint32 * d)(*[x.size // SLOTS for x in imgs])
+        ptrs = (ctypes.c_void_p * d)(*[x.ctypes.data for x in imgs])
         with torch.cuda.device(self.device):
             check(self.lib.dyc_load(self.h, nb, ptrs, _stream_ptr(stream)), "dyc_load")
 

@@ -1,9 +1,10 @@
 /*
  * dyc.h -- C ABI of the MI355X DyCuckoo library (libdyc.so).
  *
- * A dynamic cuckoo hash table of uint32 keys and uint32 values: 4 subtables of 128-byte
- * buckets (16 slots of (uint64)key << 32 | value), two-layer pair hashing, bounded eviction,
- * and single-subtable upsize/downsize -- the data structure of GPU/dyhash in the reference.
+ * A dynamic cuckoo hash table of uint32 keys and uint32 values: d subtables (4 by default,
+ * 2..6 through dyc_config.subtables) of 128-byte buckets (16 slots of (uint64)key << 32 |
+ * value), two-layer pair hashing, bounded eviction, and single-subtable upsize/downsize -- the
+ * data structure of GPU/dyhash in the reference.
  *
  * The reference exposes a C++ class (GPU/dyhash/api.h:9-55, hashAPI) over a device-pointer
  * layer (GPU/dyhash/cuckoo.h:100-114, gpu_lp_*). This header replaces both with an opaque
@@ -34,7 +35,7 @@
 extern "C" {
 #endif
 
-#define DYC_ABI_VERSION 1
+#define DYC_ABI_VERSION 2
 
 #define DYC_OK 0
 #define DYC_ERR_INVALID 1        /* bad argument (null handle, bad config, size limit) */
@@ -68,12 +69,16 @@ typedef struct dyc_config {
     uint32_t insert_order;     /* 0 or 1 = input order (default); 2 = order each batch by the bucket
                                   region of its keys' first bucket before inserting */
     uint32_t region_mib;       /* table bytes per bucket region when ordering (default 64) */
+    uint32_t subtables;        /* subtable count d, 2..6 (default 4 = TABLE_NUM, dy_hash.h:32); the
+                                  pair scheme of each d is in diff_hash_num_pos_fun.cpp:14-188.
+                                  Fixed at create: dyc_set_config must leave it unchanged */
+    uint32_t reserved0;        /* must be 0 */
 } dyc_config;
 
 typedef struct dyc_stats {
     uint64_t count;             /* live entries: placed + pending */
     uint64_t capacity;          /* slots = 16 * sum(buckets) */
-    uint32_t buckets[4];        /* buckets per subtable */
+    uint32_t buckets[6];        /* buckets per subtable; entries at and past `subtables` are 0 */
     uint64_t pending;           /* entries owned but not placed */
     uint64_t kicks;             /* cumulative successful exchanges (insert + re-insert) */
     uint64_t last_kicks;        /* exchanges of the most recent dyc_insert */
@@ -83,13 +88,16 @@ typedef struct dyc_stats {
     uint64_t resizes_up;
     uint64_t resizes_down;
     uint64_t lost;              /* must stay 0 */
+    uint32_t subtables;         /* subtable count d */
+    uint32_t reserved0;
 } dyc_stats;
 
 /* Defaults (see field comments). */
 int dyc_config_init(dyc_config *cfg);
 
-/* Replaces hashAPI::hashAPI(TYPE size) (api.cpp:26-91): 4 subtables of
- * nextPrime(capacity_hint / 64) buckets each, rounded up to even (api.cpp:47-56), zeroed. */
+/* Replaces hashAPI::hashAPI(TYPE size) (api.cpp:26-91): cfg->subtables subtables of
+ * nextPrime(capacity_hint / (16 * subtables)) buckets each, rounded up to even (api.cpp:47-56),
+ * zeroed. */
 int dyc_create(const dyc_config *cfg, uint64_t capacity_hint, dyc_table **out);
 
 /* Replaces hashAPI::~hashAPI (api.cpp:93-109). Frees everything the table owns. */
@@ -141,13 +149,13 @@ int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, ui
 int dyc_find_host(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, dyc_stream stream);
 int dyc_erase_host(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream);
 
-/* Checkpoint a table to a file and restore it (no reference counterpart; SURVEY-listed
- * table-image format). The file is a 128-byte header (magic "DYCKPT01", version 1, 4
- * subtables, buckets per subtable, live count, pending count, FNV-1a checksums of every part and
- * of the header), then the four subtable images and the pending entries as little-endian u64
- * slots. Restore validates the header, the file size, every checksum and that the stored
+/* Checkpoint a table to a file and restore it (no reference counterpart: the reference keeps
+ * tables in device memory only). The file is a 128-byte header (magic "DYCKPT01", version 2,
+ * subtable count d, buckets per subtable, live count, pending count, FNV-1a checksums of every
+ * part and of the header), then the d subtable images and the pending entries as little-endian
+ * u64 slots. Restore validates the header, the file size, every checksum and that the stored
  * entries match the count (DYC_ERR_CORRUPT otherwise), and creates a table with `cfg` (NULL =
- * defaults) and the file's geometry. */
+ * defaults; its subtables field is replaced by the file's d) and the file's geometry. */
 int dyc_checkpoint_write(dyc_table *t, const char *path, dyc_stream stream);
 int dyc_checkpoint_read(const char *path, const dyc_config *cfg, dyc_table **out);
 
@@ -178,8 +186,9 @@ int dyc_probe_count(dyc_table *t, const uint32_t *keys, uint64_t n, uint64_t *pr
 int dyc_dump(dyc_table *t, uint32_t sub, uint64_t *host_slots, uint64_t n_slots, dyc_stream stream);
 
 /* Replace the table contents with host images: nb[s] buckets per subtable, images[s] holding
- * nb[s] * 16 slots. Pending entries are discarded and count is recomputed. Synchronous. */
-int dyc_load(dyc_table *t, const uint32_t nb[4], const uint64_t *const images[4], dyc_stream stream);
+ * nb[s] * 16 slots, for s < the table's subtable count. Pending entries are discarded and count
+ * is recomputed. Synchronous. */
+int dyc_load(dyc_table *t, const uint32_t *nb, const uint64_t *const *images, dyc_stream stream);
 
 /* Copy up to `cap` pending entries to host memory; *n_out = number pending. */
 int dyc_pending(dyc_table *t, uint64_t *host_entries, uint64_t cap, uint64_t *n_out, dyc_stream stream);

@@ -23,7 +23,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
-SUBTABLES = 4
+SUBTABLES = 4          # default subtable count (dy_hash.h:32 TABLE_NUM)
+MAX_SUBTABLES = 6
+SUBTABLE_COUNTS = (2, 3, 4, 5, 6)
 SLOTS = 16
 PRIME = 294967291
 KICK_BOUND = 15
@@ -33,7 +35,8 @@ RESERVED = (0, 0xFFFFFFFF)
 
 
 class _OrcTable(ctypes.Structure):
-    _fields_ = [("sub", ctypes.c_void_p * SUBTABLES), ("nb", ctypes.c_uint32 * SUBTABLES)]
+    _fields_ = [("sub", ctypes.c_void_p * MAX_SUBTABLES), ("nb", ctypes.c_uint32 * MAX_SUBTABLES),
+                ("d", ctypes.c_uint32)]
 
 
 def build(force: bool = False) -> str:
@@ -61,6 +64,19 @@ def lib():
         L.orc_hash_t.argtypes = [u32, u32]
         L.orc_get_pos.restype = u32
         L.orc_get_pos.argtypes = [u32, u32]
+        L.orc_pairs.restype = u32
+        L.orc_pairs.argtypes = [u32]
+        for name in ("orc_pair_d", "orc_pos1_d", "orc_pos2_d"):
+            getattr(L, name).restype = u32
+            getattr(L, name).argtypes = [u32, u32]
+        L.orc_get_pos_d.restype = u32
+        L.orc_get_pos_d.argtypes = [u32, u32, u32]
+        L.orc_initial_buckets_d.restype = u32
+        L.orc_initial_buckets_d.argtypes = [u64, u32]
+        L.orc_choose_up_d.restype = u32
+        L.orc_choose_up_d.argtypes = [vp, u32]
+        L.orc_choose_down_d.restype = u32
+        L.orc_choose_down_d.argtypes = [vp, u32]
         L.orc_bucket.restype = u32
         L.orc_bucket.argtypes = [u32, u32, u32]
         L.orc_initial_buckets.restype = u32
@@ -119,8 +135,21 @@ def hash_batch(keys, which: int) -> np.ndarray:
     return out
 
 
-def initial_buckets(capacity_hint: int) -> int:
-    return int(lib().orc_initial_buckets(capacity_hint))
+def initial_buckets(capacity_hint: int, subtables: int = SUBTABLES) -> int:
+    return int(lib().orc_initial_buckets_d(capacity_hint, subtables))
+
+
+def pair_table(subtables: int):
+    """[(pos1, pos2)] of every pair index of a `subtables`-subtable table."""
+    L = lib()
+    return [(int(L.orc_pos1_d(p, subtables)), int(L.orc_pos2_d(p, subtables)))
+            for p in range(int(L.orc_pairs(subtables)))]
+
+
+def pair_of(keys, subtables: int = SUBTABLES) -> np.ndarray:
+    """Pair index of every key (choose_hash of the `subtables` variant)."""
+    L = lib()
+    return np.array([L.orc_pair_d(int(k), subtables) for k in _u32(keys).tolist()], dtype=np.uint32)
 
 
 def pack(keys, vals) -> np.ndarray:
@@ -136,13 +165,20 @@ class OracleTable:
     layout ``dyc_dump`` writes.
     """
 
-    def __init__(self, capacity_hint: int = 0, *, nb=None, kick_bound: int = KICK_BOUND,
+    def __init__(self, capacity_hint: int = 0, *, nb=None, subtables: int = 0, kick_bound: int = KICK_BOUND,
                  rescue_kicks: int = RESCUE_KICKS,
                  lower: float = 0.5, upper: float = 0.9, auto_resize: bool = False,
                  min_capacity: float = SUBTABLES * 1024 * 1024 * 1.2, max_fail_rounds: int = 8):
         if nb is None:
-            b = initial_buckets(capacity_hint)
-            nb = [b] * SUBTABLES
+            self.d = subtables or SUBTABLES
+            b = initial_buckets(capacity_hint, self.d)
+            nb = [b] * self.d
+        else:
+            self.d = len(nb)
+            if subtables and subtables != self.d:
+                raise ValueError("nb lists %d subtables, subtables=%d" % (self.d, subtables))
+        if self.d not in SUBTABLE_COUNTS:
+            raise ValueError("subtables must be in 2..6")
         self.nb = [int(x) for x in nb]
         self.sub = [np.zeros(x * SLOTS, dtype=np.uint64) for x in self.nb]
         self.kick_bound = kick_bound
@@ -160,9 +196,10 @@ class OracleTable:
     # -- image plumbing
     def _struct(self):
         s = _OrcTable()
-        for t in range(SUBTABLES):
+        for t in range(self.d):
             s.sub[t] = self.sub[t].ctypes.data
             s.nb[t] = self.nb[t]
+        s.d = self.d
         return s
 
     @property
@@ -172,6 +209,7 @@ class OracleTable:
     def load_images(self, images):
         self.sub = [np.ascontiguousarray(x, dtype=np.uint64).copy() for x in images]
         self.nb = [x.size // SLOTS for x in self.sub]
+        self.d = len(self.sub)
         self.count = int(sum(int(np.count_nonzero(x)) for x in self.sub))
 
     def images(self):
@@ -198,7 +236,7 @@ class OracleTable:
             e = int(e)
             if pre == NO_TABLE:
                 k = e >> 32
-                p = NO_TABLE if (k & 1) == 0 else int(L.orc_get_pos(L.orc_pair(k), NO_TABLE))
+                p = NO_TABLE if (k & 1) == 0 else int(L.orc_get_pos_d(L.orc_pair_d(k, self.d), NO_TABLE, self.d))
             else:
                 p = pre
             ok = L.orc_insert_entry(ctypes.byref(s), e, p, self.kick_bound, self.rescue_kicks,
@@ -280,8 +318,8 @@ class OracleTable:
         return deleted
 
     def resize_up(self) -> int:
-        nb = (ctypes.c_uint32 * SUBTABLES)(*self.nb)
-        t = int(lib().orc_choose_up(ctypes.byref(nb)))
+        nb = (ctypes.c_uint32 * self.d)(*self.nb)
+        t = int(lib().orc_choose_up_d(ctypes.byref(nb), self.d))
         new = np.zeros(self.nb[t] * 2 * SLOTS, dtype=np.uint64)
         lib().orc_upsize(_p(self.sub[t]), self.nb[t], _p(new), t)
         self.sub[t] = new
@@ -292,8 +330,8 @@ class OracleTable:
     def resize_down(self, defer_residuals: bool = False):
         """Halve the subtable chosen by api.cpp:147-153. Returns (t, residuals) or None when
         that subtable has an odd bucket count (pairwise compaction is undefined there)."""
-        nb = (ctypes.c_uint32 * SUBTABLES)(*self.nb)
-        t = int(lib().orc_choose_down(ctypes.byref(nb)))
+        nb = (ctypes.c_uint32 * self.d)(*self.nb)
+        t = int(lib().orc_choose_down_d(ctypes.byref(nb), self.d))
         if self.nb[t] % 2:
             return None
         new = np.zeros(self.nb[t] // 2 * SLOTS, dtype=np.uint64)

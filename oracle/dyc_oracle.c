@@ -5,7 +5,10 @@
  * Every function cites the reference lines it restates (paths relative to the reference
  * repository root). Parity status of each part is recorded in the tests that pin it:
  *   hashes h1..h5     -- pinned by independent inverse functions (tests/test_oracle_hashes.py)
- *   bucket / pair     -- pinned by big-integer arithmetic and the exhaustive pair table
+ *   bucket / pair     -- pinned by big-integer arithmetic and the exhaustive pair table;
+ *                        the pair tables of d = 2, 3, 5, 6 by the property that they list
+ *                        every pair of distinct subtables once, and by the reference's
+ *                        comments that spell the pairs out
  *   sizing            -- pinned by a sieve (brute force)
  *   insert replay     -- pinned by hand-derived placements, the structural invariant and
  *                        set equality against a dict
@@ -115,6 +118,85 @@ uint32_t orc_pos2(uint32_t p)
 /* cuckoo.cu:310-312 */
 uint32_t orc_get_pos(uint32_t p, uint32_t pre) { return pre == orc_pos1(p) ? orc_pos2(p) : orc_pos1(p); }
 
+/* ---- the other subtable counts (diff_hash_num_pos_fun.cpp:14-188, the n-choose-2 variants
+ * that cuckoo.cu:122-187 keeps commented out). Each d has its own choose_hash / pos1 / pos2;
+ * d = 4 is the code above. */
+
+/* number of pairs choose_hash ranges over: 2 for d = 2 (k & 1), else C(d, 2) */
+uint32_t orc_pairs(uint32_t d) { return d == 2u ? 2u : d * (d - 1u) / 2u; }
+
+/* choose_hash: d = 2 sig & 1 (:16-19); d = 3 h5 % 3 (:38-41); d = 4 h5 % 6 (cuckoo.cu:190-193);
+ * d = 5 h5 % 10 (:120-123); d = 6 h5 % 15 (:153-156) */
+uint32_t orc_pair_d(uint32_t k, uint32_t d)
+{
+    if (d == 2u) return k & 1u;
+    if (d == 4u) return orc_pair(k);
+    return orc_h5(k) % orc_pairs(d);
+}
+
+/* pos1 per d. d = 2: pos (:21-24). d = 3: the shift form, pos & 1 ? 1 : 0 (:43-48).
+ * d = 5: 3 for pos 9, else the d = 4 rule (:126-135). d = 6: the switch of :158-172, whose
+ * comma-separated case labels list the pairs (0,1)..(0,5), (1,2)..(1,5), (2,3)..(2,5), (3,4),
+ * (3,5), (4,5) in order: pos1 is the block the pair index falls in. */
+uint32_t orc_pos1_d(uint32_t p, uint32_t d)
+{
+    switch (d) {
+    case 2: return p;
+    case 3: return (p & 1u) ? 1u : 0u;
+    case 4: return orc_pos1(p);
+    case 5:
+        if (p == 9u) return 3u;
+        if (p & 1u) return 0u;
+        if (p & 2u) return 2u;
+        return 1u;
+    default:
+        if (p <= 4u) return 0u;
+        if (p <= 8u) return 1u;
+        if (p <= 11u) return 2u;
+        if (p <= 13u) return 3u;
+        return 4u;
+    }
+}
+
+/* pos2 per d. d = 2: pos ^ 1 (:26-29). d = 3: pos & 1 ? 2 : (pos >> 1) + 1 (:50-55).
+ * d = 5: 4 for pos 9; odd: (pos >> 1) + 1; pos & 2: (pos >> 2) + 3; else (pos >> 2) + 2
+ * (:137-146). d = 6: pos + 1, pos - 3, pos - 6, pos - 8, 5 by block (:174-188). */
+uint32_t orc_pos2_d(uint32_t p, uint32_t d)
+{
+    switch (d) {
+    case 2: return p ^ 1u;
+    case 3: return (p & 1u) ? 2u : (p >> 1) + 1u;
+    case 4: return orc_pos2(p);
+    case 5:
+        if (p == 9u) return 4u;
+        if (p & 1u) return (p >> 1) + 1u;
+        if (p & 2u) return (p >> 2) + 3u;
+        return (p >> 2) + 2u;
+    default:
+        if (p <= 4u) return p + 1u;
+        if (p <= 8u) return p - 3u;
+        if (p <= 11u) return p - 6u;
+        if (p <= 13u) return p - 8u;
+        return 5u;
+    }
+}
+
+uint32_t orc_get_pos_d(uint32_t p, uint32_t pre, uint32_t d)
+{
+    return pre == orc_pos1_d(p, d) ? orc_pos2_d(p, d) : orc_pos1_d(p, d);
+}
+
+/* Victim vote over a full bucket (cuckoo.cu:396-401): the first slot whose resident's pair
+ * index differs from the table number t. For d = 2 the vote is __ballot(true)
+ * (diff_hash_num_pos_fun.cpp:3-6): every slot qualifies, so slot 0 is taken. */
+static int vote_lane(const uint64_t *view, uint32_t t, uint32_t d)
+{
+    if (d == 2u) return 0;
+    for (int i = 0; i < ORC_SLOTS; ++i)
+        if (orc_pair_d(KEY_OF(view[i]), d) != t) return i;
+    return -1;
+}
+
 void orc_hash_batch(const uint32_t *keys, uint64_t n, uint32_t which, uint32_t *out)
 {
     for (uint64_t i = 0; i < n; ++i) {
@@ -152,30 +234,36 @@ uint32_t orc_next_prime(uint32_t n)
 }
 
 /* api.cpp:47-51: s = nextPrime(size / (TABLE_NUM*BUCKET_SIZE)); buckets = s rounded up to even */
-uint32_t orc_initial_buckets(uint64_t capacity_hint)
+uint32_t orc_initial_buckets_d(uint64_t capacity_hint, uint32_t d)
 {
-    uint32_t s = (uint32_t)(capacity_hint / (ORC_SUBTABLES * ORC_SLOTS));
+    uint32_t s = (uint32_t)(capacity_hint / ((uint64_t)d * ORC_SLOTS));
     s = orc_next_prime(s);
     return (s & 1u) ? s + 1u : s;
 }
 
+uint32_t orc_initial_buckets(uint64_t capacity_hint) { return orc_initial_buckets_d(capacity_hint, ORC_SUBTABLES); }
+
 /* ---------------------------------------------------------------- resize choice */
 
-/* api.cpp:116-122: the first subtable that is half the size of its left neighbour, else 0 */
-uint32_t orc_choose_up(const uint32_t nb[ORC_SUBTABLES])
+/* api.cpp:116-122 (loop bound TABLE_NUM): the first subtable that is half the size of its left
+ * neighbour, else 0 */
+uint32_t orc_choose_up_d(const uint32_t *nb, uint32_t d)
 {
-    for (uint32_t i = 0; i + 1 < ORC_SUBTABLES; ++i)
+    for (uint32_t i = 0; i + 1 < d; ++i)
         if (nb[i] == 2u * nb[i + 1]) return i + 1;
     return 0;
 }
 
 /* api.cpp:147-153: the first subtable larger than its right neighbour, else the last */
-uint32_t orc_choose_down(const uint32_t nb[ORC_SUBTABLES])
+uint32_t orc_choose_down_d(const uint32_t *nb, uint32_t d)
 {
-    for (uint32_t i = 0; i + 1 < ORC_SUBTABLES; ++i)
+    for (uint32_t i = 0; i + 1 < d; ++i)
         if (nb[i] > nb[i + 1]) return i;
-    return ORC_SUBTABLES - 1;
+    return d - 1;
 }
+
+uint32_t orc_choose_up(const uint32_t nb[ORC_SUBTABLES]) { return orc_choose_up_d(nb, ORC_SUBTABLES); }
+uint32_t orc_choose_down(const uint32_t nb[ORC_SUBTABLES]) { return orc_choose_down_d(nb, ORC_SUBTABLES); }
 
 /* ---------------------------------------------------------------- insert */
 
@@ -191,9 +279,8 @@ uint32_t orc_rescue_lane(uint32_t key, uint32_t it)
 
 /* The eviction loop of warp_insert (cuckoo.cu:377-421), one key at a time:
  *   t      = the other table of the in-hand key's pair, relative to `pre`
- *   victim = first empty slot; else first slot whose resident pair index differs from t
- *            (the reference compares a pair index with a table number, cuckoo.cu:398-400);
- *            else slot (iteration & 15)
+ *   victim = first empty slot; else the vote of vote_lane (the reference compares a pair index
+ *            with a table number, cuckoo.cu:398-400); else slot (iteration & 15)
  *   exchange, carry the old entry; done when the old entry is empty. */
 int orc_insert_entry(orc_table *tb, uint64_t entry, uint32_t pre, uint32_t kick_bound,
                      uint32_t rescue_kicks, uint32_t *kicks, uint64_t *carry)
@@ -202,14 +289,13 @@ int orc_insert_entry(orc_table *tb, uint64_t entry, uint32_t pre, uint32_t kick_
     const uint32_t total = kick_bound + rescue_kicks;
     for (uint32_t it = 0; it < total; ++it) {
         uint32_t k = KEY_OF(entry);
-        uint32_t t = orc_get_pos(orc_pair(k), pre);
+        uint32_t t = orc_get_pos_d(orc_pair_d(k, tb->d), pre, tb->d);
         uint64_t *b = tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS;
         int lane = -1;
         for (int i = 0; i < ORC_SLOTS; ++i)
             if (KEY_OF(b[i]) == 0u) { lane = i; break; }
         if (lane < 0 && it < kick_bound) {
-            for (int i = 0; i < ORC_SLOTS; ++i)
-                if (orc_pair(KEY_OF(b[i])) != t) { lane = i; break; }
+            lane = vote_lane(b, t, tb->d);
             if (lane < 0) lane = (int)(it & 15u);
         }
         if (lane < 0) lane = (int)orc_rescue_lane(k, it);
@@ -233,7 +319,7 @@ int orc_insert_one(orc_table *tb, uint32_t key, uint32_t value, uint32_t kick_bo
                    uint32_t rescue_kicks, uint32_t *kicks, uint64_t *carry)
 {
     uint32_t pre = ORC_NO_TABLE;
-    if (key & 1u) pre = orc_get_pos(orc_pair(key), ORC_NO_TABLE);
+    if (key & 1u) pre = orc_get_pos_d(orc_pair_d(key, tb->d), ORC_NO_TABLE, tb->d);
     return orc_insert_entry(tb, PACK(key, value), pre, kick_bound, rescue_kicks, kicks, carry);
 }
 
@@ -278,13 +364,13 @@ void orc_find(const orc_table *tb, const uint32_t *keys, uint64_t n, uint32_t *v
         uint32_t k = keys[i], v = 0;
         uint8_t hit = 0;
         if (!reserved_key(k)) {
-            uint32_t p = orc_pair(k);
-            uint32_t t = orc_pos1(p);
+            uint32_t p = orc_pair_d(k, tb->d);
+            uint32_t t = orc_pos1_d(p, tb->d);
             const uint64_t *b = tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS;
             int lane = first_match(b, k);
             ++probes;
             if (lane < 0) {
-                t = orc_pos2(p);
+                t = orc_pos2_d(p, tb->d);
                 b = tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS;
                 lane = first_match(b, k);
                 ++probes;
@@ -319,11 +405,11 @@ uint64_t orc_erase(orc_table *tb, const uint32_t *keys, uint64_t n, uint8_t *del
         uint32_t k = keys[i];
         int del = 0;
         if (!reserved_key(k)) {
-            uint32_t p = orc_pair(k);
-            uint32_t t = orc_pos1(p);
+            uint32_t p = orc_pair_d(k, tb->d);
+            uint32_t t = orc_pos1_d(p, tb->d);
             del = erase_in(tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS, k);
             if (!del) {
-                t = orc_pos2(p);
+                t = orc_pos2_d(p, tb->d);
                 del = erase_in(tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS, k);
             }
         }
@@ -378,7 +464,7 @@ uint64_t orc_downsize_compact(const uint64_t *old_sub, uint32_t old_nb, uint64_t
 uint64_t orc_count(const orc_table *tb)
 {
     uint64_t c = 0;
-    for (int t = 0; t < ORC_SUBTABLES; ++t)
+    for (uint32_t t = 0; t < tb->d; ++t)
         for (uint64_t s = 0; s < (uint64_t)tb->nb[t] * ORC_SLOTS; ++s) c += tb->sub[t][s] != 0;
     return c;
 }
@@ -386,13 +472,13 @@ uint64_t orc_count(const orc_table *tb)
 uint64_t orc_check_invariant(const orc_table *tb)
 {
     uint64_t bad = 0;
-    for (uint32_t t = 0; t < ORC_SUBTABLES; ++t)
+    for (uint32_t t = 0; t < tb->d; ++t)
         for (uint64_t b = 0; b < tb->nb[t]; ++b)
             for (int i = 0; i < ORC_SLOTS; ++i) {
                 uint64_t e = tb->sub[t][b * ORC_SLOTS + i];
                 if (e == 0) continue;
-                uint32_t k = KEY_OF(e), p = orc_pair(k);
-                if (reserved_key(k) || (orc_pos1(p) != t && orc_pos2(p) != t) ||
+                uint32_t k = KEY_OF(e), p = orc_pair_d(k, tb->d);
+                if (reserved_key(k) || (orc_pos1_d(p, tb->d) != t && orc_pos2_d(p, tb->d) != t) ||
                     orc_bucket(k, t, tb->nb[t]) != b)
                     ++bad;
             }
@@ -441,13 +527,13 @@ uint64_t orc_insert_mt(orc_table *tb, const uint32_t *keys, const uint32_t *vals
             continue;
         }
         uint32_t pre = ORC_NO_TABLE;
-        if (key & 1u) pre = orc_get_pos(orc_pair(key), ORC_NO_TABLE);
+        if (key & 1u) pre = orc_get_pos_d(orc_pair_d(key, tb->d), ORC_NO_TABLE, tb->d);
         uint64_t entry = PACK(key, vals[i]);
         uint32_t it = 0, races = 0;
         int placed = 0;
         while (it < total && races <= ORC_MT_RETRIES) {
             const uint32_t k = KEY_OF(entry);
-            const uint32_t t = orc_get_pos(orc_pair(k), pre);
+            const uint32_t t = orc_get_pos_d(orc_pair_d(k, tb->d), pre, tb->d);
             uint64_t *b = tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS;
             uint64_t view[ORC_SLOTS];
             for (int j = 0; j < ORC_SLOTS; ++j) view[j] = __atomic_load_n(&b[j], __ATOMIC_RELAXED);
@@ -455,8 +541,7 @@ uint64_t orc_insert_mt(orc_table *tb, const uint32_t *keys, const uint32_t *vals
             for (int j = 0; j < ORC_SLOTS; ++j)
                 if (KEY_OF(view[j]) == 0u) { lane = j; break; }
             if (lane < 0 && it < kick_bound) {
-                for (int j = 0; j < ORC_SLOTS; ++j)
-                    if (orc_pair(KEY_OF(view[j])) != t) { lane = j; break; }
+                lane = vote_lane(view, t, tb->d);
                 if (lane < 0) lane = (int)(it & 15u);
             }
             if (lane < 0) lane = (int)orc_rescue_lane(k, it);

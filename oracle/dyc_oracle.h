@@ -8,7 +8,7 @@
  * The oracle replays the reference semantics one key at a time (a sequential replay of the
  * 16-lane warp_insert / cuckoo_search / cuckoo_delete / resize kernels), on a caller-owned
  * table image whose byte layout is the layout the GPU library dumps:
- *   4 subtables; subtable t is nb[t] buckets x 16 slots of uint64 entries,
+ *   d subtables (4 by default); subtable t is nb[t] buckets x 16 slots of uint64 entries,
  *   entry = (uint64)key << 32 | value, empty slot = 0   (cuckoo.h:13-16, 30-34).
  *
  * Where the reference is ambiguous or buggy the oracle takes the documented decision:
@@ -27,16 +27,18 @@
 extern "C" {
 #endif
 
-#define ORC_SUBTABLES 4
+#define ORC_SUBTABLES 4            /* dy_hash.h:32 (TABLE_NUM), the default subtable count */
+#define ORC_MAX_SUBTABLES 6
 #define ORC_SLOTS 16
 #define ORC_PRIME 294967291u      /* dy_hash.h:24 (PRIME_uint) */
 #define ORC_KICK_BOUND 15         /* dy_hash.h:33 (MAX_ITER)   */
 #define ORC_NO_TABLE 100u         /* cuckoo.cu:336 initial pre_table_no */
 
-/* a caller-owned table image: sub[t] points at nb[t]*16 uint64 slots */
+/* a caller-owned table image of d subtables (2..6): sub[t] points at nb[t]*16 uint64 slots */
 typedef struct orc_table {
-    uint64_t *sub[ORC_SUBTABLES];
-    uint32_t nb[ORC_SUBTABLES];
+    uint64_t *sub[ORC_MAX_SUBTABLES];
+    uint32_t nb[ORC_MAX_SUBTABLES];
+    uint32_t d;
 } orc_table;
 
 /* ---- hashes (cuckoo.cu:27-64, 110-120) ---- */
@@ -51,6 +53,12 @@ uint32_t orc_pair(uint32_t k);                               /* cuckoo.cu:190-19
 uint32_t orc_pos1(uint32_t pair);                            /* cuckoo.cu:197-204 */
 uint32_t orc_pos2(uint32_t pair);                            /* cuckoo.cu:206-213 */
 uint32_t orc_get_pos(uint32_t pair, uint32_t pre);           /* cuckoo.cu:310-312 */
+/* the same for d subtables (diff_hash_num_pos_fun.cpp:14-188); d = 4 gives the above */
+uint32_t orc_pairs(uint32_t d);
+uint32_t orc_pair_d(uint32_t k, uint32_t d);
+uint32_t orc_pos1_d(uint32_t pair, uint32_t d);
+uint32_t orc_pos2_d(uint32_t pair, uint32_t d);
+uint32_t orc_get_pos_d(uint32_t pair, uint32_t pre, uint32_t d);
 
 /* vectorised helpers for tests: out[i] = f(keys[i]) */
 void orc_hash_batch(const uint32_t *keys, uint64_t n, uint32_t which, uint32_t *out);
@@ -59,10 +67,13 @@ void orc_hash_batch(const uint32_t *keys, uint64_t n, uint32_t which, uint32_t *
 int orc_is_prime(uint32_t n);
 uint32_t orc_next_prime(uint32_t n);
 uint32_t orc_initial_buckets(uint64_t capacity_hint);  /* buckets per subtable */
+uint32_t orc_initial_buckets_d(uint64_t capacity_hint, uint32_t d);
 
 /* ---- resize choice (api.cpp:116-122, 147-153) ---- */
 uint32_t orc_choose_up(const uint32_t nb[ORC_SUBTABLES]);
 uint32_t orc_choose_down(const uint32_t nb[ORC_SUBTABLES]);
+uint32_t orc_choose_up_d(const uint32_t *nb, uint32_t d);
+uint32_t orc_choose_down_d(const uint32_t *nb, uint32_t d);
 
 /* ---- operations ---- */
 /* Victim lane used after the reference kick bound (rescue phase); see dyc_oracle.c. */

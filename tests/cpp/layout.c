@@ -6,11 +6,12 @@
 
 int main(void)
 {
-    printf("config %zu %zu %zu %zu %zu %zu %zu\n", sizeof(dyc_config), offsetof(dyc_config, lower_bound),
+    printf("config %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(dyc_config), offsetof(dyc_config, lower_bound),
            offsetof(dyc_config, auto_resize), offsetof(dyc_config, max_blocks), offsetof(dyc_config, min_capacity),
-           offsetof(dyc_config, insert_order), offsetof(dyc_config, region_mib));
-    printf("stats %zu %zu %zu %zu %zu\n", sizeof(dyc_stats), offsetof(dyc_stats, buckets),
-           offsetof(dyc_stats, pending), offsetof(dyc_stats, rejected), offsetof(dyc_stats, lost));
+           offsetof(dyc_config, insert_order), offsetof(dyc_config, region_mib), offsetof(dyc_config, subtables));
+    printf("stats %zu %zu %zu %zu %zu %zu\n", sizeof(dyc_stats), offsetof(dyc_stats, buckets),
+           offsetof(dyc_stats, pending), offsetof(dyc_stats, rejected), offsetof(dyc_stats, lost),
+           offsetof(dyc_stats, subtables));
     printf("request %zu %zu %zu %zu %zu\n", sizeof(dyc_request), offsetof(dyc_request, keys),
            offsetof(dyc_request, vals), offsetof(dyc_request, out_flags), offsetof(dyc_request, n));
     printf("server_config %zu %zu %zu %zu %zu\n", sizeof(dyc_server_config),

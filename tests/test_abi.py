@@ -50,7 +50,7 @@ def test_library_exports_every_declared_symbol(lib):
 def test_abi_version_and_defaults(lib):
     import databases_gpu_amd as D
 
-    assert lib.dyc_abi_version() == 1
+    assert lib.dyc_abi_version() == 2
     cfg = D.default_config()
     assert cfg.kick_bound == 15 and cfg.rescue_kicks == 128
     assert cfg.lower_bound == 0.5 and cfg.upper_bound == 0.9
@@ -66,6 +66,11 @@ def test_errors_without_a_device_are_reported_not_crashed(lib):
     h = ctypes.c_void_p()
     rc = lib.dyc_create(ctypes.byref(cfg), 1000, ctypes.byref(h))
     assert rc == 1 and b"lanes" in lib.dyc_last_error()
+    assert L.default_config().subtables == 4
+    for d in (1, 7):  # subtable counts outside the 2..6 of the reference's pair schemes
+        cfg = L.default_config(subtables=d)
+        assert lib.dyc_create(ctypes.byref(cfg), 1000, ctypes.byref(h)) == 1 and b"subtables" in lib.dyc_last_error()
+        assert lib.dyc_group_create(ctypes.byref(cfg), 1, (ctypes.c_uint64 * 1)(1000), ctypes.byref(h)) == 1
     assert lib.dyc_insert(None, None, None, 0, None) == 1
     assert lib.dyc_find(None, None, None, None, 0, None) == 1
     assert lib.dyc_get_stats(None, None) == 1
@@ -82,9 +87,9 @@ def test_struct_layout_matches_c(lib, tmp_path):
     s = [int(x) for x in out[1].split()[1:]]
     assert c == [ctypes.sizeof(L.Config), L.Config.lower_bound.offset, L.Config.auto_resize.offset,
                  L.Config.max_blocks.offset, L.Config.min_capacity.offset, L.Config.insert_order.offset,
-                 L.Config.region_mib.offset]
+                 L.Config.region_mib.offset, L.Config.subtables.offset]
     assert s == [ctypes.sizeof(L.Stats), L.Stats.buckets.offset, L.Stats.pending.offset, L.Stats.rejected.offset,
-                 L.Stats.lost.offset]
+                 L.Stats.lost.offset, L.Stats.subtables.offset]
     r = [int(x) for x in out[2].split()[1:]]
     assert r == [ctypes.sizeof(L.Request), L.Request.keys.offset, L.Request.vals.offset, L.Request.out_flags.offset,
                  L.Request.n.offset]
@@ -142,7 +147,9 @@ def test_fast_division_is_exact(tmp_path):
 
 def test_device_header_hashes_match_the_oracle(oracle_lib, tmp_path):
     """The kernels' per-key arithmetic (dyc_device.hpp, compiled for the host here) equals the
-    oracle's on edge and random keys."""
+    oracle's on edge and random keys, for every subtable count. The device enumerates each
+    pair table; the oracle states the reference's per-d formulas, so this also checks one
+    representation against the other."""
     import numpy as np
 
     import oracle as O
@@ -153,13 +160,18 @@ def test_device_header_hashes_match_the_oracle(oracle_lib, tmp_path):
 #include "dyc_device.hpp"
 int main() {
     unsigned k;
+    const unsigned nb[6] = {2u, 36424u, 72818u, 4660218u, 1000003u, 98u};
     while (scanf("%u", &k) == 1) {
-        dyc::Desc D{};
-        unsigned nb[4] = {2u, 36424u, 72818u, 4660218u};
-        for (int t = 0; t < 4; ++t) dyc::desc_set(D, t, nullptr, nb[t]);
-        printf("%u %u %u %u %u %u %u %u %u %u %u %u\n", dyc::h1(k), dyc::h2(k), dyc::h3(k), dyc::h4(k), dyc::h5(k),
-               dyc::pair_of(k), dyc::bucket_of(D, k, 0), dyc::bucket_of(D, k, 1), dyc::bucket_of(D, k, 2),
-               dyc::bucket_of(D, k, 3), dyc::rescue_lane(k, 7), dyc::other_pos(dyc::pair_of(k), 100u));
+        for (unsigned d = 2; d <= 6; ++d) {
+            dyc::Desc D{};
+            if (!dyc::desc_init(D, d)) return 1;
+            for (unsigned t = 0; t < d; ++t) dyc::desc_set(D, (int)t, nullptr, nb[t]);
+            const unsigned p = dyc::pair_of(D, k);
+            printf("%u %u %u %u %u %u %u %u %u %u", dyc::h1(k), dyc::h2(k), dyc::h3(k), dyc::h4(k), dyc::h5(k), p,
+                   dyc::pos1(D, p), dyc::pos2(D, p), dyc::other_pos(D, p, 100u), dyc::rescue_lane(k, 7));
+            for (unsigned t = 0; t < d; ++t) printf(" %u", dyc::bucket_of(D, k, t));
+            printf("\n");
+        }
     }
 }
 ''')
@@ -168,17 +180,21 @@ int main() {
                    capture_output=True)
     rng = np.random.default_rng(3)
     keys = np.concatenate([np.array([1, 2, 11, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFE], dtype=np.uint32),
-                           rng.integers(1, 2**32 - 1, size=3000, dtype=np.uint64).astype(np.uint32)])
+                           rng.integers(1, 2**32 - 1, size=1500, dtype=np.uint64).astype(np.uint32)])
     out = subprocess.run([str(exe)], input="\n".join(map(str, keys.tolist())), capture_output=True, text=True,
                          check=True).stdout
-    got = np.array([[int(x) for x in line.split()] for line in out.strip().split("\n")], dtype=np.uint64)
+    lines = out.strip().split("\n")
+    assert len(lines) == 5 * keys.size
     L = O.lib()
-    nbs = [2, 36424, 72818, 4660218]
+    nbs = [2, 36424, 72818, 4660218, 1000003, 98]
     for i, k in enumerate(keys.tolist()):
-        want = [L.orc_h1(k), L.orc_h2(k), L.orc_h3(k), L.orc_h4(k), L.orc_h5(k), L.orc_pair(k),
-                *[L.orc_bucket(k, t, nbs[t]) for t in range(4)], L.orc_rescue_lane(k, 7),
-                L.orc_get_pos(L.orc_pair(k), 100)]
-        assert list(got[i]) == want, k
+        for j, d in enumerate(range(2, 7)):
+            got = [int(x) for x in lines[5 * i + j].split()]
+            p = L.orc_pair_d(k, d)
+            want = [L.orc_h1(k), L.orc_h2(k), L.orc_h3(k), L.orc_h4(k), L.orc_h5(k), p, L.orc_pos1_d(p, d),
+                    L.orc_pos2_d(p, d), L.orc_get_pos_d(p, 100, d), L.orc_rescue_lane(k, 7),
+                    *[L.orc_bucket(k, t, nbs[t]) for t in range(d)]]
+            assert got == want, (k, d)
 
 
 def test_cpp_surfaces_compile(tmp_path):

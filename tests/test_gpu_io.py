@@ -103,7 +103,8 @@ def test_checkpoint_round_trip_with_pending_entries(D, tmp_path):
     # the checksums the library wrote equal an independent computation over the images
     for i in range(4):
         assert hdr["checksums"][i] == io.fnv1a64_words(imgs[i])
-    assert hdr["checksums"][4] == io.fnv1a64_words(pend)
+    assert hdr["pending_checksum"] == io.fnv1a64_words(pend)
+    assert hdr["version"] == 2 and hdr["subtables"] == 4
     r = D.DyCuckoo.restore(path)
     assert r.stats()["buckets"] == st["buckets"] and r.stats()["count"] == st["count"]
     assert all(np.array_equal(x, y) for x, y in zip(r.dump(), imgs))

@@ -1,0 +1,225 @@
+"""GPU parity for subtable counts d = 2, 3, 5, 6 (and 4 alongside), through the C ABI, against
+the oracle: find bit-exact on oracle-built images, insert by multiset + placement invariant +
+find, erase and upsize byte-exact, downsize by multiset + invariant, and the group, serving and
+checkpoint paths with a non-default d."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DS = (2, 3, 4, 5, 6)
+
+
+def banner(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def D():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import databases_gpu_amd as D
+
+    D.load()
+    return D
+
+
+def unique_keys(n, seed):
+    rng = np.random.default_rng(seed)
+    out = np.unique(rng.integers(1, 2**32 - 1, size=int(n * 1.3) + 64, dtype=np.uint64).astype(np.uint32))
+    rng.shuffle(out)
+    return out[:n]
+
+
+def absent_keys(n, exclude, seed):
+    a = unique_keys(n * 2 + 64, seed)
+    return a[~np.isin(a, exclude)][:n]
+
+
+def vals_of(k):
+    from databases_gpu_amd.workload import splitmix32_np
+
+    return splitmix32_np(k)
+
+
+def cu(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).cuda()
+
+
+def gpu_find(t, q):
+    v, h = t.find(cu(q))
+    torch.cuda.synchronize()
+    return v.cpu().numpy().view(np.uint32), h.cpu().numpy()
+
+
+def oracle_from_images(images):
+    tb = O.OracleTable(nb=[x.size // 16 for x in images], rescue_kicks=128)
+    tb.sub = [np.ascontiguousarray(x).copy() for x in images]
+    tb.count = int(sum(np.count_nonzero(x) for x in images))
+    return tb
+
+
+def oracle_build(n, theta, d, seed):
+    keys = unique_keys(n, seed)
+    vals = vals_of(keys)
+    tb = O.OracleTable(int(np.ceil(n / theta)), subtables=d, rescue_kicks=128)
+    tb.insert(keys, vals)
+    assert tb.pending.size == 0
+    return tb, keys, vals
+
+
+@pytest.mark.parametrize("d", DS)
+def test_find_and_erase_on_oracle_image(D, d):
+    banner(f"find/erase on an oracle image, subtables={d}")
+    tb, keys, vals = oracle_build(120_000, 0.85, d, seed=300 + d)
+    t = D.DyCuckoo(1000, subtables=d)
+    assert t.subtables == d
+    t.load(tb.images())
+    q = np.concatenate([keys[:60_000], absent_keys(30_000, keys, 310 + d), np.array([0, 0xFFFFFFFF], np.uint32)])
+    np.random.default_rng(d).shuffle(q)
+    q = q[:-13]
+    want_v, want_h = tb.find(q)
+    got_v, got_h = gpu_find(t, q)
+    assert np.array_equal(got_h, want_h) and np.array_equal(got_v, want_v)
+    e = np.concatenate([keys[::3], absent_keys(5000, keys, 320 + d)])
+    want = tb.erase(e)
+    got = t.erase(cu(e)).cpu().numpy()
+    assert np.array_equal(got, want)
+    for a, b in zip(t.dump(), tb.images()):
+        assert np.array_equal(a, b)
+    t.close()
+
+
+@pytest.mark.parametrize("d", DS)
+def test_insert_multiset_invariant_and_find(D, d):
+    n = 1 << 19
+    banner(f"insert 2^19 at 0.88, subtables={d}")
+    keys = unique_keys(n, 400 + d)
+    vals = vals_of(keys)
+    hint = int(np.ceil(n / 0.88))
+    t = D.DyCuckoo(hint, subtables=d)
+    t.insert(cu(keys), cu(vals))
+    st = t.stats()
+    assert st["subtables"] == d and st["buckets"] == [O.initial_buckets(hint, d)] * d
+    assert st["count"] == n and st["pending"] == 0 and st["lost"] == 0
+    imgs = t.dump()
+    assert len(imgs) == d
+    ref = oracle_from_images(imgs)
+    assert ref.d == d and ref.check_invariant() == 0
+    assert np.array_equal(ref.entries(), np.sort(O.pack(keys, vals)))
+    q = np.concatenate([keys[: n // 2], absent_keys(n // 4, keys, 410 + d)])
+    want_v, want_h = ref.find(q)
+    got_v, got_h = gpu_find(t, q)
+    assert np.array_equal(got_h, want_h) and np.array_equal(got_v, want_v)
+    assert want_h.sum() == n // 2
+    t.close()
+
+
+@pytest.mark.parametrize("d", DS)
+def test_upsize_byte_exact_and_downsize_preserves(D, d):
+    banner(f"upsize x{d + 1} byte-exact, downsize x2, subtables={d}")
+    tb, keys, vals = oracle_build(60_000, 0.85, d, seed=500 + d)
+    t = D.DyCuckoo(1000, subtables=d)
+    t.load(tb.images())
+    for _ in range(d + 1):  # every subtable once, then subtable 0 again
+        tb.resize_up()
+        t.resize_up()
+        assert t.buckets == tb.nb
+        for a, b in zip(t.dump(), tb.images()):
+            assert np.array_equal(a, b)
+    for _ in range(2):
+        assert t.resize_down()
+        tb.resize_down()
+        assert t.buckets == tb.nb
+        st = t.stats()
+        assert st["pending"] == 0 and st["lost"] == 0
+        g = oracle_from_images(t.dump())
+        assert g.check_invariant() == 0
+        assert np.array_equal(g.entries(), np.sort(O.pack(keys, vals)))
+    t.close()
+
+
+def test_group_with_three_subtables_matches_the_oracle(D):
+    banner("group of 5 tables, subtables=3")
+    sizes = [3000, 70_000, 0, 512, 20_000]
+    keys = [unique_keys(s, 600 + i) if s else np.zeros(0, np.uint32) for i, s in enumerate(sizes)]
+    vals = [vals_of(k) for k in keys]
+    hints = [int(max(s, 1) / 0.85) + 64 for s in sizes]
+    g = D.DyCuckooGroup(hints, subtables=3)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    g.insert(cu(np.concatenate(keys)), cu(np.concatenate(vals)), offs)
+    q = [np.concatenate([k[: k.size // 2], absent_keys(100, k, 700 + i)]) for i, k in enumerate(keys)]
+    qo = np.concatenate([[0], np.cumsum([x.size for x in q])]).astype(np.uint64)
+    v, h = g.find(cu(np.concatenate(q)), qo)
+    v, h = v.cpu().numpy().view(np.uint32), h.cpu().numpy()
+    for i, n in enumerate(sizes):
+        st = g.table_stats(i)
+        assert st["subtables"] == 3 and st["buckets"] == [O.initial_buckets(hints[i], 3)] * 3 and st["count"] == n
+        ref = oracle_from_images(g.dump(i))
+        assert ref.d == 3 and ref.check_invariant() == 0
+        assert np.array_equal(ref.entries(), np.sort(O.pack(keys[i], vals[i])))
+        want_v, want_h = ref.find(q[i])
+        lo, hi = int(qo[i]), int(qo[i + 1])
+        assert np.array_equal(h[lo:hi], want_h) and np.array_equal(v[lo:hi], want_v)
+        assert want_h[: n // 2].all() and not want_h[n // 2:].any()
+    g.close()
+
+
+def test_serving_with_five_subtables(D):
+    """The continuous-batching loop on d = 5 tables: every result equals the trace's expected
+    results and the tables end holding exactly the live window."""
+    from databases_gpu_amd import workload as W
+
+    banner("serving replay mix, subtables=5")
+    tr = W.ServingTrace(600, seed=23, device="cuda", resident=1 << 14, tenants=8, tenant_lo=256,
+                        tenant_hi=4096, batch_lo=1, batch_hi=1 << 11, mix=(0.3, 0.5, 0.2))
+    srv = D.DyCuckooServer(tr.hints, subtables=5, min_capacity=0, tick_max_keys=1 << 14, hbm_budget=256 << 20)
+    init = [(tr.initial_keys(t),) for t in range(tr.sizes.size)]
+    init = [(k, W.splitmix32(k)) for (k,) in init]
+    torch.cuda.synchronize()  # the server's stream does not wait for torch's
+    for t, (k, v) in enumerate(init):
+        srv.insert(t, k, v)
+    srv.drain()
+    ov = torch.zeros(tr.ops, dtype=torch.int32, device="cuda")
+    of = torch.zeros(tr.ops, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    done, status = srv.replay(tr.requests(ov, of), tr.arrivals, mode=0)
+    torch.cuda.synchronize()
+    assert (status == 0).all()
+    assert torch.equal(of, tr.exp_flags) and torch.equal(ov, tr.exp_vals)
+    lo, hi = tr.final_window
+    for t in range(tr.sizes.size):
+        assert srv.table_stats(t)["subtables"] == 5
+        ref = oracle_from_images(srv.dump(t))
+        assert ref.d == 5 and ref.check_invariant() == 0
+        live = W.keys_at(torch.arange(int(tr.base[t] + lo[t]), int(tr.base[t] + hi[t]), dtype=torch.int64), tr.seed)
+        live = live.numpy().view(np.uint32)
+        assert np.array_equal(ref.entries(), np.sort(O.pack(live, W.splitmix32_np(live))))
+    srv.close()
+
+
+def test_checkpoint_round_trip_six_subtables(D, tmp_path):
+    banner("checkpoint round trip, subtables=6")
+    from databases_gpu_amd import io
+
+    keys = unique_keys(80_000, 801)
+    vals = vals_of(keys)
+    t = D.DyCuckoo(int(80_000 / 0.8), subtables=6)
+    t.insert(cu(keys), cu(vals))
+    t.resize_up()
+    path = tmp_path / "six.ckpt"
+    t.checkpoint(path)
+    hdr = io.checkpoint_header(path)
+    imgs = t.dump()
+    assert hdr["version"] == 2 and hdr["subtables"] == 6 and hdr["buckets"] == t.buckets
+    assert all(hdr["checksums"][i] == io.fnv1a64_words(imgs[i]) for i in range(6))
+    r = D.DyCuckoo.restore(path)
+    assert r.subtables == 6 and r.buckets == t.buckets
+    assert all(np.array_equal(x, y) for x, y in zip(r.dump(), imgs))
+    q = np.concatenate([keys[:3000], absent_keys(500, keys, 802)])
+    assert all(np.array_equal(a, b) for a, b in zip(gpu_find(r, q), gpu_find(t, q)))
+    r.close()
+    t.close()

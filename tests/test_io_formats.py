@@ -26,14 +26,25 @@ def test_dataset_round_trip_and_partial_read(tmp_path):
 
 
 def test_checkpoint_header_layout(tmp_path):
-    fields = dict(version=1, subtables=4, buckets=[6, 6, 12, 6], count=77, pending=3, checksums=[1, 2, 3, 4, 5],
-                  header_check=99)
-    raw = struct.pack("<8sII4IQQ5QQ32s", b"DYCKPT01", 1, 4, 6, 6, 12, 6, 77, 3, 1, 2, 3, 4, 5, 99, b"\0" * 32)
-    assert len(raw) == 128
+    # byte offsets of the 128-byte header: magic 0, version 8, subtables 12, buckets[6] 16,
+    # count 40, pending 48, checksum[6] 56, pending checksum 104, header check 112, reserved 120
+    raw = bytearray(128)
+    raw[0:8] = b"DYCKPT01"
+    struct.pack_into("<II", raw, 8, 2, 5)
+    struct.pack_into("<5I", raw, 16, 6, 6, 12, 6, 6)
+    struct.pack_into("<QQ", raw, 40, 77, 3)
+    struct.pack_into("<5Q", raw, 56, 1, 2, 3, 4, 5)
+    struct.pack_into("<QQ", raw, 104, 8, 99)
+    fields = dict(version=2, subtables=5, buckets=[6, 6, 12, 6, 6], count=77, pending=3, checksums=[1, 2, 3, 4, 5],
+                  pending_checksum=8, header_check=99)
     p = tmp_path / "t.ckpt"
-    p.write_bytes(raw + b"\0" * 64)
+    p.write_bytes(bytes(raw) + b"\0" * 64)
     assert io.checkpoint_header(p) == fields
-    p.write_bytes(b"NOTACKPT" + raw[8:])
+    p.write_bytes(b"NOTACKPT" + bytes(raw[8:]))
+    with pytest.raises(ValueError):
+        io.checkpoint_header(p)
+    struct.pack_into("<I", raw, 12, 7)  # subtable count out of range
+    p.write_bytes(bytes(raw))
     with pytest.raises(ValueError):
         io.checkpoint_header(p)
 
