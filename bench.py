@@ -453,9 +453,10 @@ def run_subtables(args, dev) -> int:
     per_d, ok = {}, True
     for d in [int(x) for x in args.st_list.split(",")]:
         t = D.DyCuckoo(hint, subtables=d, kick_bound=args.st_kick, find_lanes=args.find_lanes)
-        ins, fnd, kicks = [], [], []
+        ins, fnd, kicks, races = [], [], [], []
         for s_ in range(args.warmup + args.steps):
             t.clear()
+            races0 = t.stats()["cas_retries"]
             e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
             e[0].record(stream)
             t.insert(keys, vals)
@@ -463,10 +464,12 @@ def run_subtables(args, dev) -> int:
             t.find(look, out_v, out_h)
             e[2].record(stream)
             e[2].synchronize()
+            st0 = t.stats()
             if s_ >= args.warmup:
                 ins.append(e[0].elapsed_time(e[1]))
                 fnd.append(e[1].elapsed_time(e[2]))
-                kicks.append(t.stats()["last_kicks"])
+                kicks.append(st0["last_kicks"])
+                races.append(st0["cas_retries"] - races0)
         st = t.stats()
         good = st["count"] == n and st["pending"] == 0 and st["lost"] == 0 and bool((out_h.bool() == present).all()) \
             and bool((out_v[present] == W.splitmix32(look[present])).all())
@@ -476,6 +479,8 @@ def run_subtables(args, dev) -> int:
         per_d[str(d)] = {"insert_ms": round(i_ms, 3), "insert_mops": round(n / i_ms / 1e3, 1),
                          "find_ms": round(f_ms, 3), "find_mops": round(n / f_ms / 1e3, 1),
                          "mean_exchanges": round(float(np.mean(kicks)) / n, 4), "mean_probes": round(probes / n, 4),
+                         "cas_retries_per_op": round(float(np.mean(races)) / n, 5),
+                         "kick_failures": st["kick_failures"],
                          "buckets": st["buckets"], "fill": round(n / st["capacity"], 4),
                          "resizes_up": st["resizes_up"], "parity": "ok" if good else "mismatch"}
         log(f"subtables {d}: insert {i_ms:.2f} ms ({n / i_ms / 1e3:.0f} Mops/s), find {f_ms:.2f} ms "

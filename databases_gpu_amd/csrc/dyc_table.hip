@@ -84,6 +84,14 @@ struct dyc_table {
     DycHostStage *stage = nullptr;
     uint8_t *upd = nullptr;  // upsert: per-key "updated in place" flags
     uint64_t upd_cap = 0;
+    // Subtables created together (create, load, restore) share one allocation: the GPU's address
+    // translation covers one large allocation with far fewer TLB entries than one allocation per
+    // subtable, and random bucket access at 2^28 keys misses the per-CU TLB several times per
+    // key otherwise. A subtable a resize replaces gets its own allocation; the slab is freed when
+    // none of its subtables is still in use.
+    uint8_t *slab = nullptr;
+    uint32_t slab_live = 0;
+    bool in_slab[kMaxSubtables] = {};
     dyc_stats st{};
 
     uint64_t capacity() const
@@ -115,15 +123,54 @@ int read_counters(dyc_table *t, hipStream_t s)
     return DYC_OK;
 }
 
-int alloc_zeroed_sub(uint64_t **p, uint32_t nb, hipStream_t s)
+constexpr size_t kSlabAlign = 2u << 20;  // subtable regions of a slab start 2 MiB apart
+
+// Release subtable i's storage: its own allocation, or its region of the slab (the slab itself
+// once its last subtable is released). Callers synchronise first.
+int release_sub(dyc_table *t, uint32_t i)
 {
-    const size_t bytes = (size_t)nb * kBucketBytes;
-    if (hipMalloc(p, bytes) != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
-        return fail(DYC_ERR_OOM, "hipMalloc of %zu bytes for a subtable failed", bytes);
+    if (!t->D.sub[i]) return DYC_OK;
+    if (t->in_slab[i]) {
+        t->in_slab[i] = false;
+        if (--t->slab_live == 0) {
+            HIP_TRY(hipFree(t->slab));
+            t->slab = nullptr;
+        }
+    } else {
+        HIP_TRY(hipFree(t->D.sub[i]));
     }
-    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(*p), (uint64_t)nb * kSlots, s));
+    t->D.sub[i] = nullptr;
+    return DYC_OK;
+}
+
+// One allocation for d subtables of nb[i] buckets; *base receives it and sub[i] the regions.
+int alloc_slab(uint32_t d, const uint32_t *nb, uint8_t **base, uint64_t **sub)
+{
+    size_t off[kMaxSubtables + 1] = {0};
+    for (uint32_t i = 0; i < d; ++i)
+        off[i + 1] = off[i] + (((size_t)nb[i] * kBucketBytes + kSlabAlign - 1) & ~(kSlabAlign - 1));
+    if (hipMalloc(base, off[d]) != hipSuccess) {
+        *base = nullptr;
+        (void)hipGetLastError();
+        return fail(DYC_ERR_OOM, "hipMalloc of %zu bytes for the table failed", off[d]);
+    }
+    for (uint32_t i = 0; i < d; ++i) sub[i] = reinterpret_cast<uint64_t *>(*base + off[i]);
+    return DYC_OK;
+}
+
+// Replace every subtable of t by the regions of a new slab (the old storage is released).
+int install_slab(dyc_table *t, uint8_t *base, uint64_t *const *sub, const uint32_t *nb)
+{
+    for (uint32_t i = 0; i < t->D.d; ++i) {
+        const int rc = release_sub(t, i);
+        if (rc) return rc;
+    }
+    t->slab = base;
+    t->slab_live = t->D.d;
+    for (uint32_t i = 0; i < t->D.d; ++i) {
+        desc_set(t->D, (int)i, sub[i], nb[i]);
+        t->in_slab[i] = true;
+    }
     return DYC_OK;
 }
 
@@ -176,7 +223,8 @@ int do_resize_up(dyc_table *t, hipStream_t s)
     HIP_TRY(launch_upsize(t->D.sub[sub], old_nb, dst, sub, t->ctr, s));
     rc = read_counters(t, s);
     if (rc) return rc;
-    HIP_TRY(hipFree(t->D.sub[sub]));
+    rc = release_sub(t, sub);
+    if (rc) return rc;
     desc_set(t->D, (int)sub, dst, old_nb * 2);
     t->st.resizes_up++;
     if (t->hctr->misplaced)
@@ -253,7 +301,8 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         (void)hipFree(res);
         return fail(DYC_ERR_LOST, "downsize residual list overflow");
     }
-    HIP_TRY(hipFree(t->D.sub[sub]));
+    rc = release_sub(t, sub);
+    if (rc) return rc;
     desc_set(t->D, (int)sub, dst, new_nb);
     t->st.resizes_down++;
     // residuals continue from the shrunk subtable into the other one of their pair
@@ -583,10 +632,15 @@ int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out
         (void)hipGetLastError();
         rc = fail(DYC_ERR_OOM, "counter allocation failed");
     }
-    for (uint32_t i = 0; i < t->D.d && rc == DYC_OK; ++i) {
-        uint64_t *p = nullptr;
-        rc = alloc_zeroed_sub(&p, nb, nullptr);
-        if (rc == DYC_OK) desc_set(t->D, i, p, nb);
+    if (rc == DYC_OK) {
+        const uint32_t nbs[kMaxSubtables] = {nb, nb, nb, nb, nb, nb};
+        uint8_t *base = nullptr;
+        uint64_t *sub[kMaxSubtables] = {};
+        rc = alloc_slab(t->D.d, nbs, &base, sub);
+        if (rc == DYC_OK) rc = install_slab(t, base, sub, nbs);
+        for (uint32_t i = 0; i < t->D.d && rc == DYC_OK; ++i)
+            if (launch_zero(reinterpret_cast<unsigned long long *>(sub[i]), (uint64_t)nb * kSlots, nullptr) != hipSuccess)
+                rc = fail(DYC_ERR_HIP, "zeroing the table failed");
     }
     if (rc == DYC_OK) rc = reserve_pending(t, 1024, nullptr);
     if (rc == DYC_OK && hipStreamSynchronize(nullptr) != hipSuccess) rc = fail(DYC_ERR_HIP, "create sync failed");
@@ -606,8 +660,7 @@ int dyc_destroy(dyc_table *t)
     if (!t) return DYC_OK;
     DeviceGuard dg(t->device);
     (void)hipDeviceSynchronize();
-    for (uint32_t i = 0; i < t->D.d; ++i)
-        if (t->D.sub[i]) (void)hipFree(t->D.sub[i]);
+    for (uint32_t i = 0; i < t->D.d; ++i) (void)release_sub(t, i);
     if (t->pend) (void)hipFree(t->pend);
     if (t->ctr) (void)hipFree(t->ctr);
     if (t->part) (void)hipFree(t->part);
@@ -857,20 +910,22 @@ int dyc_load(dyc_table *t, const uint32_t *nb, const uint64_t *const *images, dy
     DeviceGuard dg(t->device);
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipStreamSynchronize(s));
+    uint8_t *base = nullptr;
+    uint64_t *sub[kMaxSubtables] = {};
+    int rc = alloc_slab(t->D.d, nb, &base, sub);
+    if (rc) return rc;
     uint64_t count = 0;
     for (uint32_t i = 0; i < t->D.d; ++i) {
-        uint64_t *p = nullptr;
         const size_t bytes = (size_t)nb[i] * kBucketBytes;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(DYC_ERR_OOM, "hipMalloc for load failed");
+        if (hipMemcpyAsync(sub[i], images[i], bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            (void)hipFree(base);
+            return fail(DYC_ERR_HIP, "copying subtable %u to the device failed", i);
         }
-        HIP_TRY(hipMemcpyAsync(p, images[i], bytes, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        HIP_TRY(hipFree(t->D.sub[i]));
-        desc_set(t->D, i, p, nb[i]);
         for (uint64_t j = 0; j < (uint64_t)nb[i] * kSlots; ++j) count += images[i][j] != 0;
     }
+    rc = install_slab(t, base, sub, nb);
+    if (rc) return rc;
     t->count = count;
     t->pend_n = 0;
     t->st.pending = 0;
@@ -1091,17 +1146,13 @@ int dyc_checkpoint_read(const char *path, const dyc_config *cfg, dyc_table **out
     if (rc == DYC_OK) rc = dyc_create(&c, 64, &t);
     PinnedPair pp;
     if (rc == DYC_OK && !pp.alloc(kCkptChunk * 8)) rc = fail(DYC_ERR_OOM, "pinned staging for the restore");
+    uint8_t *base = nullptr;
+    uint64_t *sub[kMaxSubtables] = {};
+    if (rc == DYC_OK) rc = alloc_slab(h.subtables, h.buckets, &base, sub);
+    if (rc == DYC_OK) rc = install_slab(t, base, sub, h.buckets);  // the table now owns the slab
     for (uint32_t i = 0; i < h.subtables && rc == DYC_OK; ++i) {
-        uint64_t *p = nullptr;
-        if (hipMalloc(&p, (size_t)h.buckets[i] * kBucketBytes) != hipSuccess) {
-            (void)hipGetLastError();
-            rc = fail(DYC_ERR_OOM, "hipMalloc for the restore failed");
-            break;
-        }
-        (void)hipFree(t->D.sub[i]);
-        desc_set(t->D, i, p, h.buckets[i]);
         uint64_t hash = 0;
-        rc = load_from_file(f, p, (uint64_t)h.buckets[i] * kSlots, pp, &hash, nullptr);
+        rc = load_from_file(f, sub[i], (uint64_t)h.buckets[i] * kSlots, pp, &hash, nullptr);
         if (rc == DYC_OK && hash != h.checksum[i]) rc = fail(DYC_ERR_CORRUPT, "subtable %u checksum mismatch", i);
     }
     if (rc == DYC_OK) rc = reserve_pending(t, h.pending, nullptr);

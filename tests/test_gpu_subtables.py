@@ -103,7 +103,11 @@ def test_insert_multiset_invariant_and_find(D, d):
     t = D.DyCuckoo(hint, subtables=d)
     t.insert(cu(keys), cu(vals))
     st = t.stats()
-    assert st["subtables"] == d and st["buckets"] == [O.initial_buckets(hint, d)] * d
+    assert st["subtables"] == d and len(st["buckets"]) == d
+    # a chain that exhausts its bound parks its entry and the call upsizes (the parallel insert
+    # makes that rare event timing-dependent); otherwise the geometry is the initial one
+    if st["resizes_up"] == 0:
+        assert st["buckets"] == [O.initial_buckets(hint, d)] * d
     assert st["count"] == n and st["pending"] == 0 and st["lost"] == 0
     imgs = t.dump()
     assert len(imgs) == d
