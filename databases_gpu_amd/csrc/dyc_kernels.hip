@@ -241,18 +241,18 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
     return probe_tile_m<G, ERASE ? 1 : 0, COH>(D, keys, vals, flags, base, n);
 }
 
-template <bool ERASE>
-__device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const uint32_t *keys, uint32_t *out_vals,
+template <int MODE>
+__device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const uint32_t *keys, uint32_t *vals,
                                          uint8_t *flags, uint64_t n, uint32_t table, uint64_t base, uint32_t f);
 
+// Live entries of a pool (0 for Pool{}: a table without one).
 __device__ inline uint64_t pool_size(const Pool &pool)
 {
+    if (!pool.count) return pool.n < pool.cap ? pool.n : pool.cap;
     const unsigned long long pc = __hip_atomic_load(pool.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return pc < pool.cap ? pc : pool.cap;
 }
 
-// pool.count == nullptr: the table has no overflow pool (a plain table); otherwise misses also
-// look in the pool entries tagged `table` (a serving loop's table).
 // Waves per SIMD the find / erase kernels are held to (the occupancy their register use gives
 // with a four-subtable selection; the six-way one would otherwise cost a wave at G = 8).
 template <int G, bool ERASE>
@@ -261,18 +261,20 @@ constexpr int probe_waves()
     return G == 4 ? (ERASE ? 6 : 7) : (G == 8 ? (ERASE ? 5 : 7) : (ERASE ? 4 : 5));
 }
 
+// Misses also look in the pool's entries of `table` (a serving table's overflow pool, a single
+// table's stash); an empty pool costs nothing.
 template <int G, bool ERASE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(probe_waves<G, ERASE>()))) void probe_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                     uint32_t *__restrict__ vals, uint8_t *__restrict__ flags,
                                                     uint64_t n, Counters *__restrict__ C, Pool pool, uint32_t table)
 {
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
-    const uint64_t np = pool.count ? pool_size(pool) : 0;
+    const uint64_t np = pool_size(pool);
     unsigned long long ndel = 0;
     for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
          base += waves * kWave) {
         uint32_t f = probe_tile<G, ERASE>(D, keys, vals, flags, base, n);
-        if (np) f |= pool_probe_at<ERASE>(pool, np, keys, vals, flags, n, table, base, f);
+        if (np) f |= pool_probe_at<ERASE ? 1 : 0>(pool, np, keys, vals, flags, n, table, base, f);
         ndel += f;
     }
     if constexpr (ERASE) wave_add(&C->deleted, ndel);
@@ -282,13 +284,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(probe_waves
 template <int G>
 __global__ __launch_bounds__(256) void update_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                      uint32_t *__restrict__ vals, uint8_t *__restrict__ updated,
-                                                     uint64_t n, Counters *__restrict__ C)
+                                                     uint64_t n, Counters *__restrict__ C, Pool pool)
 {
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    const uint64_t np = pool_size(pool);
     unsigned long long nu = 0;
     for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
-         base += waves * kWave)
-        nu += probe_tile_m<G, 2>(D, keys, vals, updated, base, n);
+         base += waves * kWave) {
+        uint32_t f = probe_tile_m<G, 2>(D, keys, vals, updated, base, n);
+        if (np) f |= pool_probe_at<2>(pool, np, keys, vals, updated, n, 0, base, f);
+        nu += f;
+    }
     wave_add(&C->updated, nu);
 }
 
@@ -819,24 +825,31 @@ __global__ __launch_bounds__(256) void group_insert_kernel(const Desc *__restric
 // Items of a serving tick carry their request's own pointers; otherwise the tile code is the
 // single-table code. Finds and erases that miss both buckets also look in the server's pool of
 // entries that exhausted the eviction bound (normally empty: one scalar load per tile).
-template <bool ERASE>
-__device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const uint32_t *keys, uint32_t *out_vals,
+
+// The pool search of one tile's lanes whose key missed both buckets (f = 0). The pool counts as
+// one more bucket after pos2. MODE 0 find: vals receives the first matching entry's value.
+// MODE 1 erase: every copy is CAS'd to 0. MODE 2 update: every copy is CAS'd to the new value
+// vals[i] (an input). Returns this lane's flag, also written to flags when set.
+template <int MODE>
+__device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const uint32_t *keys, uint32_t *vals,
                                          uint8_t *flags, uint64_t n, uint32_t table, uint64_t base, uint32_t f)
 {
     const uint64_t i = base + __lane_id();
     if (i >= n || f) return 0;
     const uint32_t k = keys[i];
     if (reserved_key(k)) return 0;
+    const uint32_t nv = MODE == 2 ? vals[i] : 0u;
     uint32_t got = 0;
     for (uint64_t j = 0; j < np; ++j) {
         const uint64_t e = __hip_atomic_load(pool.pend + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (key_of(e) != k || __hip_atomic_load(pool.tags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != table)
+        if (key_of(e) != k ||
+            (pool.tags && __hip_atomic_load(pool.tags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != table))
             continue;
-        if constexpr (ERASE) {  // the pool counts as one more bucket: clear every copy in it
+        if constexpr (MODE != 0) {
             uint64_t exp = e;
-            got |= (uint32_t)cas_slot(pool.pend + j, exp, 0ull);
+            got |= (uint32_t)cas_slot(pool.pend + j, exp, MODE == 2 ? pack(k, nv) : 0ull);
         } else {
-            if (out_vals) out_vals[i] = val_of(e);
+            if (vals) vals[i] = val_of(e);
             got = 1;
             break;
         }
@@ -848,7 +861,7 @@ __device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const ui
 template <bool ERASE>
 __device__ inline uint32_t pool_probe(const Pool &pool, uint64_t np, const ServeItem &item, uint64_t base, uint32_t f)
 {
-    return pool_probe_at<ERASE>(pool, np, item.keys, item.out_vals, item.flags, item.len, item.table, base, f);
+    return pool_probe_at<ERASE ? 1 : 0>(pool, np, item.keys, item.out_vals, item.flags, item.len, item.table, base, f);
 }
 
 // One item of finds or erases; returns this lane's count of hits (erase: deletions).
@@ -1128,15 +1141,16 @@ hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *va
 }
 
 hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint8_t *updated, uint64_t n,
-                         int lanes, LaunchShape sh, Counters *C, hipStream_t st)
+                         int lanes, LaunchShape sh, Counters *C, hipStream_t st, const Pool *pool)
 {
     if (n == 0) return hipSuccess;
     const int grid = grid_for((n + kWave - 1) / kWave, sh);
     uint32_t *v = const_cast<uint32_t *>(vals);  // read-only in update mode
+    const Pool p = pool ? *pool : Pool{};
     switch (lanes) {
-    case 4: update_kernel<4><<<grid, sh.block_threads, 0, st>>>(D, keys, v, updated, n, C); break;
-    case 16: update_kernel<16><<<grid, sh.block_threads, 0, st>>>(D, keys, v, updated, n, C); break;
-    default: update_kernel<8><<<grid, sh.block_threads, 0, st>>>(D, keys, v, updated, n, C); break;
+    case 4: update_kernel<4><<<grid, sh.block_threads, 0, st>>>(D, keys, v, updated, n, C, p); break;
+    case 16: update_kernel<16><<<grid, sh.block_threads, 0, st>>>(D, keys, v, updated, n, C, p); break;
+    default: update_kernel<8><<<grid, sh.block_threads, 0, st>>>(D, keys, v, updated, n, C, p); break;
     }
     return hipGetLastError();
 }

@@ -44,7 +44,7 @@ struct InsertPolicy {
 
 struct Pool;
 // find: vals/hit may be null. lanes in {4, 8, 16}. pool (optional): misses also look in the
-// entries of a serving loop's overflow pool tagged `table`.
+// pool's entries of `table`.
 hipError_t launch_find(const Desc &D, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, int lanes,
                        LaunchShape sh, hipStream_t st, const Pool *pool = nullptr, uint32_t table = 0);
 // erase: deleted may be null; counts into C->deleted.
@@ -57,9 +57,10 @@ hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *va
                          InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
                          const uint8_t *skip = nullptr);
 // update in place: every copy of keys[i] in the first bucket (pos1, then pos2) that holds it
-// takes the value vals[i]; updated[i] = 1 if there was one; the count goes to C->updated.
+// takes the value vals[i] (with a pool: failing that, every copy in the pool); updated[i] = 1 if
+// there was one; the count goes to C->updated.
 hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint8_t *updated, uint64_t n,
-                         int lanes, LaunchShape sh, Counters *C, hipStream_t st);
+                         int lanes, LaunchShape sh, Counters *C, hipStream_t st, const Pool *pool = nullptr);
 // n_dev (optional): a device count read at kernel start; entries [0, min(n, *n_dev)) are inserted.
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
                                  InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
@@ -104,11 +105,17 @@ struct ServeItem {
 // Entries that exhausted the eviction bound, shared by all tables of a server (tags = table)
 // and searched by its finds and erases until the host re-places them. count is the append index
 // (the `pending` word of the Counters its insert launches add to); entries beyond cap are lost.
+// Entries a table holds outside its buckets (a serving loop's overflow pool, or a single table's
+// pending list used as its stash). Finds, erases and updates that miss both buckets search it.
+// count != nullptr: the live size is min(*count, cap), read on the device (the pool grows while a
+// tick runs); otherwise it is n, known when the launch is made. tags == nullptr: every entry
+// belongs to the table; otherwise only those tagged with the launch's table index.
 struct Pool {
     uint64_t *pend;
     const uint32_t *tags;
     uint64_t cap;
     const unsigned long long *count;
+    uint64_t n;
 };
 
 // An item of a whole-tick launch: op 0 insert, 1 find, 2 erase. Items of one table and one

@@ -155,7 +155,7 @@ struct dyc_server {
     InsertPolicy policy() const { return InsertPolicy{cfg.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
     LaunchShape shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.max_blocks}; }
     LaunchShape insert_shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.insert_blocks}; }
-    Pool pool() const { return Pool{d_pool, d_pool_tag, scfg.pool_cap, &pc()->pending}; }
+    Pool pool() const { return Pool{d_pool, d_pool_tag, scfg.pool_cap, &pc()->pending, 0}; }
     PendSink sink(uint32_t tag, unsigned long long *placed = nullptr) const
     {
         return PendSink{d_pool, d_pool_tag, scfg.pool_cap, tag, placed};

@@ -110,6 +110,34 @@ namespace {
 // Pending list of a table from index `at` on (an insert launch appends there).
 PendSink sink_at(const dyc_table *t, uint64_t at) { return PendSink{t->pend + at, nullptr, t->pend_cap - at, 0, nullptr}; }
 
+// The pending list is the table's stash: finds, erases and updates that miss both buckets of a
+// key search it. Returns the pool to pass to a launch, or nullptr when nothing is pending.
+const Pool *stash_of(const dyc_table *t, Pool &p)
+{
+    p = Pool{t->pend, nullptr, t->pend_cap, nullptr, t->pend_n};
+    return t->pend_n ? &p : nullptr;
+}
+
+// An erase leaves the stash entries it deleted as zero words: drop them (synchronous; the list
+// is short whenever it is not empty).
+int compact_pending(dyc_table *t, hipStream_t s)
+{
+    if (t->pend_n == 0) return DYC_OK;
+    std::vector<uint64_t> h(t->pend_n);
+    HIP_TRY(hipMemcpyAsync(h.data(), t->pend, t->pend_n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < h.size(); ++i)
+        if (h[i]) h[m++] = h[i];
+    if (m != t->pend_n) {
+        if (m) HIP_TRY(hipMemcpyAsync(t->pend, h.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        t->pend_n = m;
+    }
+    t->st.pending = t->pend_n;
+    return DYC_OK;
+}
+
 int zero_counters(dyc_table *t, hipStream_t s)
 {
     HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->ctr), kCounterWords, s));
@@ -233,12 +261,15 @@ int do_resize_up(dyc_table *t, hipStream_t s)
     return DYC_OK;
 }
 
-// Re-insert everything on the pending list, upsizing before each round (the reference would
-// have dropped these entries, cuckoo.cu:422-431).
-int place_pending(dyc_table *t, hipStream_t s)
+// After an operation that found `before` entries pending and left more: re-insert everything on
+// the pending list, upsizing before each round (the reference would have dropped these entries,
+// cuckoo.cu:422-431). Entries that never fit (copies of one key beyond its two buckets' 32 slots)
+// stay in the stash without growing the table on every later call. DYC_ERR_PENDING when the
+// list is still longer than `before` after max_fail_rounds rounds.
+int place_pending(dyc_table *t, hipStream_t s, uint64_t before)
 {
     uint32_t rounds = 0;
-    while (t->pend_n > 0 && rounds < t->cfg.max_fail_rounds) {
+    while (t->pend_n > before && rounds < t->cfg.max_fail_rounds) {
         int rc = do_resize_up(t, s);
         if (rc) return rc;
         // move the pending entries aside, then re-insert them appending failures to pend
@@ -264,13 +295,15 @@ int place_pending(dyc_table *t, hipStream_t s)
         ++rounds;
     }
     t->st.pending = t->pend_n;
-    if (t->pend_n) return fail(DYC_ERR_PENDING, "%llu entries still pending after %u upsize rounds",
-                               (unsigned long long)t->pend_n, rounds);
+    if (t->pend_n > before)
+        return fail(DYC_ERR_PENDING, "%llu entries still pending after %u upsize rounds (%llu before the call)",
+                    (unsigned long long)t->pend_n, rounds, (unsigned long long)before);
     return DYC_OK;
 }
 
 int do_resize_down(dyc_table *t, hipStream_t s)
 {
+    const uint64_t pend_before = t->pend_n;
     const uint32_t sub = choose_down(t->D.nb, t->D.d);
     const uint32_t old_nb = t->D.nb[sub];
     if (old_nb & 1u)
@@ -320,7 +353,7 @@ int do_resize_down(dyc_table *t, hipStream_t s)
     }
     (void)hipFree(res);
     if (rc) return rc;
-    if (t->pend_n) return place_pending(t, s);
+    if (t->pend_n > pend_before) return place_pending(t, s, pend_before);
     return DYC_OK;
 }
 
@@ -674,6 +707,7 @@ int dyc_destroy(dyc_table *t)
 
 int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
 {
+    const uint64_t pend_before = t ? t->pend_n : 0;
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
@@ -715,13 +749,14 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     rc = absorb_insert_counters(t, room);
     t->st.pending = t->pend_n;
     if (rc) return rc;
-    if (t->pend_n) return place_pending(t, s);
+    if (t->pend_n > pend_before) return place_pending(t, s, pend_before);
     return DYC_OK;
 }
 
 int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, uint8_t *updated,
                dyc_stream stream)
 {
+    const uint64_t pend_before = t ? t->pend_n : 0;
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
@@ -755,7 +790,8 @@ int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     const uint64_t room = t->pend_cap - t->pend_n;
     rc = zero_counters(t, s);
     if (rc) return rc;
-    HIP_TRY(launch_update(t->D, keys, vals, flags, n, (int)t->cfg.find_lanes, t->shape(), t->ctr, s));
+    Pool sp;
+    HIP_TRY(launch_update(t->D, keys, vals, flags, n, (int)t->cfg.find_lanes, t->shape(), t->ctr, s, stash_of(t, sp)));
     HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(), t->ctr,
                           sink_at(t, t->pend_n), s, flags));
     rc = read_counters(t, s);
@@ -766,7 +802,7 @@ int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     rc = absorb_insert_counters(t, room);
     t->st.pending = t->pend_n;
     if (rc) return rc;
-    if (t->pend_n) return place_pending(t, s);
+    if (t->pend_n > pend_before) return place_pending(t, s, pend_before);
     return DYC_OK;
 }
 
@@ -775,7 +811,9 @@ int dyc_find(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *hit, u
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
     DeviceGuard dg(t->device);
-    HIP_TRY(launch_find(t->D, keys, vals, hit, n, (int)t->cfg.find_lanes, t->shape(), (hipStream_t)stream));
+    Pool sp;
+    HIP_TRY(launch_find(t->D, keys, vals, hit, n, (int)t->cfg.find_lanes, t->shape(), (hipStream_t)stream,
+                        stash_of(t, sp)));
     return DYC_OK;
 }
 
@@ -788,10 +826,13 @@ int dyc_erase(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_t n, 
     if (n == 0) return DYC_OK;
     int rc = zero_counters(t, s);
     if (rc) return rc;
-    HIP_TRY(launch_erase(t->D, keys, deleted, n, (int)t->cfg.find_lanes, t->shape(), t->ctr, s));
+    Pool sp;
+    HIP_TRY(launch_erase(t->D, keys, deleted, n, (int)t->cfg.find_lanes, t->shape(), t->ctr, s, stash_of(t, sp)));
     rc = read_counters(t, s);
     if (rc) return rc;
     t->count -= std::min<uint64_t>(t->count, t->hctr->deleted);
+    rc = compact_pending(t, s);
+    if (rc) return rc;
     if (t->cfg.auto_resize && (double)t->count < t->cfg.lower_bound * (double)t->capacity() &&
         t->capacity() > t->cfg.min_capacity) {
         rc = do_resize_down(t, s);
@@ -951,6 +992,7 @@ int dyc_pending(dyc_table *t, uint64_t *host_entries, uint64_t cap, uint64_t *n_
 // device-pointer calls.
 int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
 {
+    const uint64_t pend_before = t ? t->pend_n : 0;
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
@@ -998,7 +1040,7 @@ int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, ui
     rc = absorb_insert_counters(t, room);
     t->st.pending = t->pend_n;
     if (rc) return rc;
-    if (t->pend_n) return place_pending(t, s);
+    if (t->pend_n > pend_before) return place_pending(t, s, pend_before);
     return DYC_OK;
 }
 
@@ -1030,12 +1072,13 @@ int probe_host(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *flag
         HIP_TRY(hipMemcpyAsync(S->dk[b], S->hk[b], m * 4, hipMemcpyHostToDevice, S->h2d));
         HIP_TRY(hipEventRecord(S->in_ready[b], S->h2d));
         HIP_TRY(hipStreamWaitEvent(s, S->in_ready[b], 0));
+        Pool sp;
         if (erase)
             HIP_TRY(launch_erase(t->D, S->dk[b], flags ? S->df[b] : nullptr, m, (int)t->cfg.find_lanes, t->shape(),
-                                 t->ctr, s));
+                                 t->ctr, s, stash_of(t, sp)));
         else
             HIP_TRY(launch_find(t->D, S->dk[b], vals ? S->dv[b] : nullptr, flags ? S->df[b] : nullptr, m,
-                                (int)t->cfg.find_lanes, t->shape(), s));
+                                (int)t->cfg.find_lanes, t->shape(), s, stash_of(t, sp)));
         HIP_TRY(hipEventRecord(S->done[b], s));
         HIP_TRY(hipStreamWaitEvent(S->d2h, S->done[b], 0));
         if (vals && !erase) HIP_TRY(hipMemcpyAsync(S->hv[b], S->dv[b], m * 4, hipMemcpyDeviceToHost, S->d2h));
@@ -1077,6 +1120,8 @@ int dyc_erase_host(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_
     rc = read_counters(t, s);
     if (rc) return rc;
     t->count -= std::min<uint64_t>(t->count, t->hctr->deleted);
+    rc = compact_pending(t, s);
+    if (rc) return rc;
     if (t->cfg.auto_resize && (double)t->count < t->cfg.lower_bound * (double)t->capacity() &&
         t->capacity() > t->cfg.min_capacity) {
         rc = do_resize_down(t, s);

@@ -23,8 +23,10 @@
  *   - Insert does not look for an existing copy of the key (reference semantics,
  *     dy_hash.h:42 update_on 0): the keys of one table should be unique.
  *   - An entry is never dropped. An eviction chain that exceeds kick_bound + rescue_kicks
- *     parks its entry on the table's pending list; dyc_insert then upsizes and re-inserts
- *     pending entries (at most max_fail_rounds rounds per call).
+ *     parks its entry on the table's pending list; a call that leaves the list longer than it
+ *     found it upsizes and re-inserts pending entries (at most max_fail_rounds rounds). Entries still pending form the
+ *     table's stash: find, erase and upsert search it for keys neither bucket holds (so copies
+ *     of one key beyond its two buckets' 32 slots stay reachable).
  */
 #ifndef DYC_H
 #define DYC_H
@@ -41,7 +43,8 @@ extern "C" {
 #define DYC_ERR_INVALID 1        /* bad argument (null handle, bad config, size limit) */
 #define DYC_ERR_HIP 2            /* a HIP runtime call failed */
 #define DYC_ERR_OOM 3            /* device allocation failed */
-#define DYC_ERR_PENDING 4        /* entries remain pending after max_fail_rounds upsizes */
+#define DYC_ERR_PENDING 4        /* the call left more entries pending than it found, after
+                                    max_fail_rounds upsizes (they stay in the stash) */
 #define DYC_ERR_ODD_BUCKETS 5    /* downsize refused: the chosen subtable has an odd bucket count */
 #define DYC_ERR_LOST 6           /* the pending list overflowed: entries were lost (table corrupt) */
 #define DYC_ERR_CORRUPT 7        /* a resize found entries outside their bucket pair */
@@ -112,8 +115,9 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
 /* Insert with update (the paper's insert, tkde-revision/secs/cuckoo.tex:127-129; the reference
  * code has it switched off, dy_hash.h:42 update_on 0). A key already stored gets vals[i] in every
  * copy held by the first bucket (pos1, then pos2) that holds it -- what dyc_find returns -- and
- * updated[i] = 1 (updated may be NULL); every other key is inserted as by dyc_insert. Keys within
- * one call must be distinct; entries on the pending list are not searched. Synchronises once. */
+ * updated[i] = 1 (updated may be NULL); a key in neither bucket but in the stash gets it in every
+ * stash copy. Every other key is inserted as by dyc_insert. Keys within one call must be
+ * distinct. Synchronises once. */
 int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, uint8_t *updated,
                dyc_stream stream);
 

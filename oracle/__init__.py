@@ -247,9 +247,12 @@ class OracleTable:
                 fails.append(carry.value)
         return np.array(fails, dtype=np.uint64)
 
-    def _place_pending(self):
+    def _place_pending(self, before: int = 0):
+        """After an operation that found `before` entries pending and left more: upsize and
+        re-insert every pending entry, at most max_fail_rounds rounds, while more than `before`
+        remain. Entries that never fit stay as the stash without growing the table again."""
         rounds = 0
-        while self.pending.size and rounds < self.max_fail_rounds:
+        while self.pending.size > before and rounds < self.max_fail_rounds:
             self.resize_up()
             pend, self.pending = self.pending, np.zeros(0, dtype=np.uint64)
             self.pending = self._insert_entries(pend, NO_TABLE)
@@ -259,6 +262,7 @@ class OracleTable:
         """Batch insert in input order (cuckoo_insert); returns (failures_before_retry, rejected)."""
         keys, vals = _u32(keys), _u32(vals)
         n = keys.size
+        before = self.pending.size
         if self.auto_resize:
             while self.count + n > self.upper * self.capacity:
                 self.resize_up()
@@ -273,7 +277,7 @@ class OracleTable:
         self.fail_total += nfail
         self.count += n - int(rej.value)
         self.pending = np.concatenate([self.pending, fail[:nfail]])
-        self._place_pending()
+        self._place_pending(before)
         self.last_kicks = kicks
         return nfail, int(rej.value)
 
@@ -282,6 +286,7 @@ class OracleTable:
         placement depends on thread timing. Returns (failures_before_retry, rejected)."""
         keys, vals = _u32(keys), _u32(vals)
         n = keys.size
+        before = self.pending.size
         L = lib()
         s = self._struct()
         fail = np.zeros(max(n, 1), dtype=np.uint64)
@@ -293,7 +298,7 @@ class OracleTable:
         self.fail_total += nfail
         self.count += n - int(rej.value)
         self.pending = np.concatenate([self.pending, fail[:min(nfail, fail.size)]])
-        self._place_pending()
+        self._place_pending(before)
         return nfail, int(rej.value)
 
     def find(self, keys):
@@ -304,13 +309,42 @@ class OracleTable:
         s = self._struct()
         lib().orc_find(ctypes.byref(s), _p(keys), keys.size, _p(vals), _p(hit), ctypes.byref(probes))
         self.last_probes = int(probes.value)
+        # the pending entries are the table's stash, searched after pos2: the first entry (in
+        # list order) holding the key answers
+        miss, pk, first = self._stash_lookup(keys, hit == 0)
+        if miss.size:
+            vals[miss] = (self.pending[first] & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+            hit[miss] = 1
         return vals, hit
+
+    def _stash_lookup(self, keys, candidates):
+        """(indexes i among `candidates` whose key is in the stash, stash keys, index of the first
+        stash entry of each such key)."""
+        if not self.pending.size:
+            return np.zeros(0, dtype=np.int64), None, None
+        pkeys = (self.pending >> np.uint64(32)).astype(np.uint32)
+        pk, first = np.unique(pkeys, return_index=True)
+        idx = np.nonzero(candidates & (keys != 0) & (keys != 0xFFFFFFFF))[0]
+        pos = np.searchsorted(pk, keys[idx])
+        ok = pos < pk.size
+        ok[ok] = pk[pos[ok]] == keys[idx[ok]]
+        return idx[ok], pk, first[pos[ok]]
 
     def erase(self, keys):
         keys = _u32(keys)
         deleted = np.zeros(keys.size, dtype=np.uint8)
         s = self._struct()
         ndel = int(lib().orc_erase(ctypes.byref(s), _p(keys), keys.size, _p(deleted)))
+        # a key found in neither bucket has every copy in the stash cleared, by its first
+        # occurrence in the batch
+        miss, _, _ = self._stash_lookup(keys, deleted == 0)
+        if miss.size:
+            _, firsts = np.unique(keys[miss], return_index=True)
+            hits = miss[firsts]
+            deleted[hits] = 1
+            ndel += hits.size
+            gone = np.isin((self.pending >> np.uint64(32)).astype(np.uint32), keys[hits])
+            self.pending = self.pending[~gone]
         self.count -= ndel
         if (self.auto_resize and self.count < self.lower * self.capacity
                 and self.capacity > self.min_capacity):
@@ -343,10 +377,11 @@ class OracleTable:
         self.resizes.append(("down", t, self.nb[t]))
         res = res[:nres].copy()
         if not defer_residuals:
+            before = self.pending.size
             fails = self._insert_entries(res, t)
             if fails.size:
                 self.pending = np.concatenate([self.pending, fails])
-                self._place_pending()
+                self._place_pending(before)
         return t, res
 
 

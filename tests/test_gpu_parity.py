@@ -281,6 +281,47 @@ def test_same_key_beyond_two_buckets_stays_pending(D):
     t.close()
 
 
+def test_stash_serves_keys_missing_from_both_buckets(D):
+    """The pending list is the table's stash (as in the oracle): 40 copies of one key leave 8
+    pending; once two erases have cleared both of its buckets, find, find_host and upsert reach
+    the stash copies, and an erase clears them all."""
+    banner("stash: 40 copies of one key")
+    t = D.DyCuckoo(100, max_fail_rounds=0, rescue_kicks=16)
+    vals = np.arange(100, 140, dtype=np.uint32)
+    k = torch.full((40,), 11, dtype=torch.int32, device="cuda")
+    with pytest.raises(D.DycError) as ei:
+        t.insert(k, torch.from_numpy(vals.view(np.int32)).cuda())
+    assert ei.value.code == 4  # DYC_ERR_PENDING
+    assert t.buckets == [2, 2, 2, 2] and t.stats()["pending"] == 8 and t.stats()["count"] == 40
+    stash = t.pending()
+    assert np.all((stash >> np.uint64(32)) == 11)
+    stash_vals = set((stash & np.uint64(0xFFFFFFFF)).tolist())
+    assert len(stash_vals) == 8 and stash_vals <= set(vals.tolist())
+    one = torch.tensor([11], dtype=torch.int32, device="cuda")
+    for _ in range(2):
+        assert t.erase(one).cpu().tolist() == [1]
+    assert all(not x.any() for x in t.dump()) and t.stats()["count"] == 38
+    gv, gh = gpu_find(t, np.array([11, 12], np.uint32))
+    assert gh.tolist() == [1, 0] and int(gv[0]) in stash_vals
+    hv, hh = t.find_host(np.array([11, 12], np.uint32))
+    assert np.asarray(hh).tolist() == [1, 0] and int(np.asarray(hv)[0]) in stash_vals
+    # a later insert that places all its own keys neither grows the table nor retries the stash
+    t.configure(max_fail_rounds=3)
+    t.insert(torch.tensor([12, 13], dtype=torch.int32, device="cuda"), torch.tensor([5, 6], dtype=torch.int32,
+                                                                                    device="cuda"))
+    assert t.buckets == [2, 2, 2, 2] and t.stats()["pending"] == 8
+    assert t.erase(torch.tensor([12, 13], dtype=torch.int32, device="cuda")).cpu().tolist() == [1, 1]
+    upd = t.upsert(one, torch.tensor([777], dtype=torch.int32, device="cuda"))
+    assert upd.cpu().tolist() == [1] and t.stats()["count"] == 38
+    assert np.all((t.pending() & np.uint64(0xFFFFFFFF)) == 777)
+    assert gpu_find(t, np.array([11], np.uint32))[0].tolist() == [777]
+    assert t.erase(one).cpu().tolist() == [1]
+    st = t.stats()
+    assert st["pending"] == 0 and st["count"] == 37 and t.pending().size == 0
+    assert gpu_find(t, np.array([11], np.uint32))[1].tolist() == [0]
+    t.close()
+
+
 # -------------------------------------------------------------------------------- erase
 def test_erase_flags_and_image_byte_exact(D):
     n = 400_000

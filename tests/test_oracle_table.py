@@ -291,3 +291,30 @@ def test_auto_resize_policy(oracle_lib):
     assert [r[0] for r in tb.resizes] == ["up", "down"]
     v, h = tb.find(keys[:500])
     assert not h[:250].any() and h[250:].all() and np.array_equal(v[250:], vals[250:500])
+
+
+def test_stash_answers_keys_missing_from_both_buckets(oracle_lib):
+    """The pending list is the table's stash: 40 copies of one key fill its 32 bucket slots and
+    leave 8 pending; after two erases clear its pos1 and pos2 buckets (the first bucket holding
+    it each time, cuckoo.cu:541-563) the key is still found in the stash, and a third erase
+    clears every stash copy."""
+    tb = O.OracleTable(nb=[2, 2, 2, 2], max_fail_rounds=0, rescue_kicks=16)
+    vals = np.arange(100, 140, dtype=np.uint32)
+    tb.insert(np.full(40, 11, np.uint32), vals)
+    assert tb.pending.size == 8 and tb.count == 40
+    stash_vals = set((tb.pending & np.uint64(0xFFFFFFFF)).tolist())
+    assert len(stash_vals) == 8 and stash_vals <= set(vals.tolist())
+    for _ in range(2):
+        assert tb.erase([11]).tolist() == [1]
+    assert tb.count == 40 - 2 and sum(int(np.count_nonzero(x)) for x in tb.sub) == 0
+    v, h = tb.find([11, 12])
+    assert h.tolist() == [1, 0] and int(v[0]) == int(tb.pending[0] & np.uint64(0xFFFFFFFF))
+    # a later insert that places all its own keys neither grows the table nor retries the stash
+    tb.max_fail_rounds = 3
+    tb.insert([12, 13], [5, 6])
+    assert tb.nb == [2, 2, 2, 2] and tb.pending.size == 8 and tb.find([12, 13])[1].tolist() == [1, 1]
+    assert tb.erase([12, 13]).tolist() == [1, 1]
+    # two identical keys in one erase batch: the first clears the stash, the second finds nothing
+    assert tb.erase([11, 11]).tolist() == [1, 0]
+    assert tb.pending.size == 0 and tb.count == 40 - 3
+    assert tb.find([11])[1].tolist() == [0]
