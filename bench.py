@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--st-list", default="2,3,4,5,6", help="subtables: the subtable counts swept")
     ap.add_argument("--st-kick", type=int, default=15, help="subtables: kick bound of every table")
     ap.add_argument("--workload", choices=("bulk", "dynamic", "batched", "serving", "resize", "memsys", "host",
-                                           "subtables"),
+                                           "subtables", "wide"),
                     default="bulk",
                     help="bulk: the large bulk configuration (default, the headline line); dynamic: the "
                          "resize stream (growth then shrink batches under fill bounds); batched: many uneven "
@@ -76,7 +76,8 @@ def parse():
                          "single-subtable upsize and downsize of a 2^28-slot-class table; memsys: the device's "
                          "random bucket-read, CAS and store rates at the table's size (the kernels' ceilings); "
                          "host: the bulk workload from host arrays (PCIe included), pipelined vs copy-then-run; "
-                         "subtables: insert and find throughput for each subtable count at a fixed table size")
+                         "subtables: insert and find throughput for each subtable count at a fixed table size; "
+                         "wide: the bulk insert and find with 64-bit keys and values")
     ap.add_argument("--sv-requests", type=int, default=4096, help="serving: requests per run")
     ap.add_argument("--sv-loads", default="0.25,0.5,1,2,4,8,16,0",
                     help="serving: offered loads in G keys/s (0 = every request arrives at once)")
@@ -507,6 +508,98 @@ def run_subtables(args, dev) -> int:
     return 0 if ok else 1
 
 
+def run_wide(args, dev) -> int:
+    """Wide keys: 2^log2n unique u64 keys (low word a seeded bijection of the index, high word a
+    mix of it) and u64 values into a table sized for load factor theta, then 2^log2n finds at the
+    --hit rate; clear + insert + find per step, medians reported. Algorithmic bytes: insert
+    16 + x (128 key line + 3 x 8 key CAS / value exchange / key publish) per key (x = exchanges
+    per key), find 17 + 128 p + 8 h per lookup (p = key lines read, h = hit rate). Prints one
+    JSON line."""
+    import torch
+
+    import databases_gpu_amd as D
+    from databases_gpu_amd import workload as W
+
+    stream = torch.cuda.current_stream()
+    n = 1 << args.log2n
+    idx = torch.arange(0, n, dtype=torch.int64, device=dev)
+
+    def wide_keys(lo_idx):
+        lo = W.u32_to_i64(W.keys_range(int(lo_idx[0]), int(lo_idx[-1]) + 1, args.seed, dev))
+        hi = W.u32_to_i64(W.splitmix32(W.keys_range(int(lo_idx[0]), int(lo_idx[-1]) + 1, args.seed + 101, dev)))
+        return (hi << 32) | lo  # int64 bits of the u64 key; low word unique and nonzero
+
+    keys = wide_keys(idx)
+    golden = -7046029254386353131  # 0x9E3779B97F4A7C15 as int64 bits
+    vals = keys * golden + 12345
+    m_hit = int(n * args.hit)
+    perm = torch.randperm(n, device=dev, generator=torch.Generator(device=dev).manual_seed(args.seed))
+    absent = wide_keys(torch.arange(n, 2 * n - m_hit, dtype=torch.int64, device=dev)) if m_hit < n else keys[:0]
+    look = torch.cat([keys[perm[:m_hit]], absent])
+    look = look[torch.randperm(n, device=dev, generator=torch.Generator(device=dev).manual_seed(args.seed + 1))]
+    out_v = torch.empty(n, dtype=torch.int64, device=dev)
+    out_h = torch.empty(n, dtype=torch.uint8, device=dev)
+    del perm, idx
+    torch.cuda.synchronize()
+    t = D.DyCuckooWide(int(math.ceil(n / args.theta)), kick_bound=0 if not args.reference_kick else 15)
+    ins, fnd, kicks = [], [], []
+    for s_ in range(args.warmup + args.steps):
+        t.clear()
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        t.insert(keys, vals)
+        e[1].record(stream)
+        t.find(look, out_v, out_h)
+        e[2].record(stream)
+        e[2].synchronize()
+        st = t.stats()
+        if st["count"] != n or st["pending"] or st["lost"]:
+            raise RuntimeError(f"wide insert left the table inconsistent: {st}")
+        if s_ >= args.warmup:
+            ins.append(e[0].elapsed_time(e[1]))
+            fnd.append(e[1].elapsed_time(e[2]))
+            kicks.append(st["last_kicks"])
+        log(f"wide step {s_ + 1}: insert {ins[-1] if s_ >= args.warmup else 0:.2f} ms, "
+            f"find {fnd[-1] if s_ >= args.warmup else 0:.2f} ms")
+    hits = int(out_h.sum())
+    ok = hits == m_hit and bool((out_v[out_h.bool()] == look[out_h.bool()] * golden + 12345).all()) \
+        and bool((out_v[~out_h.bool()] == 0).all())
+    i_ms, f_ms = float(np.median(ins)), float(np.median(fnd))
+    x = float(np.mean(kicks)) / n
+    # key lines per lookup: 2 per miss, 1.5 per hit (pos1 holds half the keys: the start rule sends
+    # odd keys to pos2 first), as dyc_probe_count measures on the 32-bit table (1.75 at 50 % hit)
+    p_ = 2.0 * (1.0 - args.hit) + 1.5 * args.hit
+    b_ins = (16 + x * (128 + 24)) * n
+    b_find = (17 + 128 * p_ + 8 * args.hit) * n
+
+    def roof(b, ms):
+        gbs = b / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None}
+
+    result = {
+        "metric": "Mops/s insert & find with 64-bit keys and values",
+        "value": round(2 * n / (i_ms + f_ms) / 1e3, 1), "unit": "Mops/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"2^{args.log2n} unique u64 keys and u64 values at load factor {args.theta}, then "
+                               f"2^{args.log2n} finds at {args.hit:.0%} hit", "buckets": t.buckets,
+                   "kick_bound": 15 if args.reference_kick else 0},
+        "insert": dict(roof(b_ins, i_ms), ms=round(i_ms, 3), mops=round(n / i_ms / 1e3, 1),
+                       mean_exchanges=round(x, 4), bytes_per_op=round(b_ins / n, 1)),
+        "find": dict(roof(b_find, f_ms), ms=round(f_ms, 3), mops=round(n / f_ms / 1e3, 1),
+                     bytes_per_op=round(b_find / n, 1)),
+        "parity": "ok" if ok else "mismatch",
+    }
+    t.close()
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0 if ok else 1
+
+
 def run_serving(args, dev) -> int:
     """The serving loop: one resident table of 2^26 keys plus 256 tenant tables of 2^12-2^20 keys
     (log-uniform), requests of log-uniform size in [2^6, 2^18] keys arriving Poisson at a fixed
@@ -757,6 +850,8 @@ def main() -> int:
         return run_host(args, dev)
     if args.workload == "subtables":
         return run_subtables(args, dev)
+    if args.workload == "wide":
+        return run_wide(args, dev)
     n = 1 << args.log2n
     hint = int(math.ceil(n / args.theta))
     log(f"generating inputs: n=2^{args.log2n}, theta={args.theta}")

@@ -16,6 +16,7 @@
 // on the victim slot. Groups are independent: a group that finishes takes its next key while
 // the rest of the wave keeps kicking, so no lane idles behind a long chain.
 #include "dyc_kernels.hpp"
+#include "dyc_wave.hpp"
 
 #include <algorithm>
 
@@ -23,16 +24,6 @@ namespace dyc {
 
 namespace {
 
-constexpr int kWave = 64;
-
-__device__ inline uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
-
-__device__ inline uint64_t shfl64(uint64_t v, int src)
-{
-    uint32_t lo = __shfl((uint32_t)v, src);
-    uint32_t hi = __shfl((uint32_t)(v >> 32), src);
-    return ((uint64_t)hi << 32) | lo;
-}
 
 // A bucket location packed in 32 bits: subtable in the top 3 bits, bucket index below (bucket
 // counts are capped at 2^29). Shuffling this instead of a 64-bit address halves the cross-lane
@@ -81,26 +72,6 @@ __device__ inline void read_slots(const uint64_t *p, uint64_t (&e)[SPL])
         load_slots<SPL>(p, e);
 }
 
-__device__ inline bool cas_slot(uint64_t *p, uint64_t &expected, uint64_t desired)
-{
-    return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline unsigned long long wave_sum(unsigned long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += (unsigned long long)shfl64(v, __lane_id() ^ off);
-    return v;
-}
-
-// Sum a per-lane value over the wave and add it to *dst once (all 64 lanes must be active).
-__device__ inline void wave_add(unsigned long long *dst, unsigned long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += (unsigned long long)shfl64(v, __lane_id() ^ off);
-    if (__lane_id() == 0 && v) atomicAdd(dst, v);
-}
 
 // ---------------------------------------------------------------- find / erase / update
 // One wave probes the 64 keys keys[base, base + 64) clipped to n, in table D. Returns this
@@ -1053,17 +1024,6 @@ __global__ __launch_bounds__(256) void count_kernel(const uint64_t *__restrict__
 // Blocks for `work_waves` waves of work. A launch may not exceed 2^32 work-items in total, and
 // every kernel here loops grid-stride, so the grid is capped (default_cap when the caller set
 // no max_blocks); the rest of the work is taken by the resident waves as they finish.
-inline int grid_for(uint64_t work_waves, LaunchShape sh, uint64_t default_cap = 1ull << 20)
-{
-    const uint64_t wpb = (uint64_t)sh.block_threads / kWave;
-    uint64_t blocks = (work_waves + wpb - 1) / wpb;
-    if (blocks < 1) blocks = 1;
-    const uint64_t cap = sh.max_blocks > 0 ? (uint64_t)sh.max_blocks : default_cap;
-    if (blocks > cap) blocks = cap;
-    const uint64_t hw_cap = ((1ull << 32) - 1) / (uint64_t)sh.block_threads;
-    if (blocks > hw_cap) blocks = hw_cap;
-    return (int)blocks;
-}
 
 // Insert groups take their next key when they finish one, so a grid of a few times the resident
 // capacity (256 CUs x 2048 threads) keeps every CU busy without launching a block per key.

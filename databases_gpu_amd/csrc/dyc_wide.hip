@@ -1,0 +1,999 @@
+// dyc_wide.hip -- DyCuckoo with 64-bit keys and 64-bit values.
+//
+// The paper's wide-key layout (tkde-revision/secs/dynamic.tex:35-37): for keys larger than
+// 4 bytes a bucket holds fewer pairs, and keys and values are stored apart. A bucket is one
+// 128-byte line of 16 u64 keys, plus the parallel 128-byte line of their 16 u64 values in the
+// subtable's value array. A lookup reads the key line and, on a hit, 8 bytes of the value line.
+//
+// Placement is the narrow table's applied to fold(k) = lo32(k) ^ h5(hi32(k)): the same
+// geometry, pair scheme of every subtable count, start rule, victim vote and rescue walk.
+// fold(k) = k for keys below 2^32, so a wide table of such keys places them exactly where the
+// narrow table does. Keys 0 (empty) and 2^64 - 1 (a slot being rewritten) are reserved.
+//
+// A 16-byte entry cannot move with one 64-bit CAS, so an exchange claims the slot instead
+// (the paper locks the bucket, dynamic.tex:37-38):
+//   1. CAS the slot's key from the value the lane read to BUSY (a lost race patches the view);
+//   2. exchange the slot's value with the new value (the old one travels with the evicted key);
+//   3. wait for that exchange to be performed, then exchange the key word from BUSY to the new
+//      key, which publishes the slot.
+// All three are device-scope atomics performed at the memory side, so no cache maintenance is
+// needed between them; a slot holding BUSY is never chosen as a victim by another lane, and no
+// lane ever waits on one, so nothing can deadlock. Finds and erases run in separate launches.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/dyc.h"
+#include "dyc_host.hpp"
+#include "dyc_wave.hpp"
+
+namespace dyc {
+
+constexpr uint64_t kWideBusy = ~0ull;
+
+__host__ __device__ inline uint32_t fold64(uint64_t k) { return (uint32_t)k ^ h5((uint32_t)(k >> 32)); }
+__host__ __device__ inline bool wide_reserved(uint64_t k) { return k == 0ull || k == kWideBusy; }
+
+struct WideDesc {
+    Desc D;                        // key arrays (D.sub), geometry and pair scheme
+    uint64_t *val[kMaxSubtables];  // value arrays, slot for slot parallel to the key arrays
+};
+
+// Where an insert parks entries that exhaust the eviction bound (append index Counters::pending).
+struct WideSink {
+    uint64_t *keys, *vals;
+    uint64_t cap;
+};
+
+// The pending entries searched after pos2 by finds and erases (n = 0: none).
+struct WideStash {
+    uint64_t *keys;
+    const uint64_t *vals;
+    uint64_t n;
+};
+
+namespace {
+
+__device__ inline uint32_t wide_loc(uint32_t t, uint32_t bucket) { return (t << 29) | bucket; }
+__device__ inline uint64_t *wide_keys_at(const WideDesc &W, uint32_t l)
+{
+    return selt(l >> 29, W.D.sub) + (uint64_t)(l & 0x1fffffffu) * kSlots;
+}
+__device__ inline uint64_t *wide_vals_at(const WideDesc &W, uint32_t l)
+{
+    return selt(l >> 29, W.val) + (uint64_t)(l & 0x1fffffffu) * kSlots;
+}
+
+__device__ inline void load2(const uint64_t *p, uint64_t (&e)[2])
+{
+    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    e[0] = ((uint64_t)v.y << 32) | v.x;
+    e[1] = ((uint64_t)v.w << 32) | v.z;
+}
+
+__device__ inline void store2(uint64_t *p, uint64_t a, uint64_t b)
+{
+    *reinterpret_cast<uint4 *>(p) = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
+}
+
+// ------------------------------------------------------------------------------ find / erase
+// A wave owns 64 consecutive keys; groups of 8 lanes probe 8 keys each, lane s reading keys
+// 2s, 2s+1 of a bucket (one 128-byte line per group). The first bucket (pos1, then pos2) that
+// holds a key answers: find takes the first matching slot's value (read by the key's own lane
+// from the value line), erase CASes every matching key of that bucket to 0. Keys neither
+// bucket holds are then looked up in the stash.
+template <bool ERASE>
+__global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint64_t *__restrict__ keys,
+                                                         uint64_t *__restrict__ vals, uint8_t *__restrict__ flags,
+                                                         uint64_t n, Counters *__restrict__ C, WideStash stash)
+{
+    constexpr int G = 8;
+    const int lane = __lane_id();
+    const int g = lane / G, s = lane % G;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    unsigned long long ndel = 0;
+    for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
+         base += waves * kWave) {
+        const uint64_t i = base + lane;
+        const bool inb = i < n;
+        const uint64_t k = inb ? keys[i] : 0ull;
+        const uint32_t f = fold64(k);
+        const uint32_t p = pair_of(W.D, f);
+        const uint32_t t1 = pos1(W.D, p), t2 = pos2(W.D, p);
+        const uint32_t loc1 = wide_loc(t1, bucket_of(W.D, f, t1));
+        const uint32_t loc2 = wide_loc(t2, bucket_of(W.D, f, t2));
+
+        uint64_t e[G][2];
+        uint64_t kk_key[G];
+        bool kk_valid[G];
+        uint32_t kk_l[G];
+        uint32_t my_flag = 0, my_loc = 0, my_slot = 0, done = 0;
+#pragma unroll
+        for (int phase = 0; phase < 2; ++phase) {
+            const uint32_t lp = phase == 0 ? loc1 : loc2;
+#pragma unroll
+            for (int kk = 0; kk < G; ++kk) {
+                const int src = g * G + kk;
+                if (phase == 0) {
+                    kk_key[kk] = shfl64(k, src);
+                    kk_valid[kk] = !wide_reserved(kk_key[kk]);
+                }
+                kk_l[kk] = __shfl(lp, src);
+                e[kk][0] = e[kk][1] = 0;
+                if (kk_valid[kk] && !(done & (1u << kk))) load2(wide_keys_at(W, kk_l[kk]) + s * 2, e[kk]);
+            }
+#pragma unroll
+            for (int kk = 0; kk < G; ++kk) {
+                if (done & (1u << kk)) continue;  // group-uniform
+                const int first = e[kk][0] == kk_key[kk] ? 0 : (e[kk][1] == kk_key[kk] ? 1 : -1);
+                const uint32_t gm = (uint32_t)((__ballot(first >= 0 && kk_valid[kk]) >> (g * G)) & 0xffu);
+                if (!gm) continue;
+                done |= 1u << kk;
+                if constexpr (ERASE) {
+                    int ok = 0;
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        if (kk_valid[kk] && e[kk][j] == kk_key[kk]) {
+                            uint64_t exp = e[kk][j];
+                            ok |= cas_slot(wide_keys_at(W, kk_l[kk]) + s * 2 + j, exp, 0ull);
+                        }
+                    const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & 0xffu);
+                    if (kk == s) my_flag = okm != 0;
+                } else {
+                    const int w = __builtin_ctz(gm);
+                    const int fj = __shfl(first, g * G + w);
+                    if (kk == s) {
+                        my_flag = 1;
+                        my_loc = kk_l[kk];
+                        my_slot = (uint32_t)(w * 2 + fj);
+                    }
+                }
+            }
+        }
+        uint64_t my_val = 0;
+        if (!ERASE && my_flag) my_val = wide_vals_at(W, my_loc)[my_slot];
+        if (inb && !my_flag && stash.n && !wide_reserved(k)) {  // the stash counts as one more bucket
+            for (uint64_t j = 0; j < stash.n; ++j) {
+                const uint64_t sk = __hip_atomic_load(stash.keys + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (sk != k) continue;
+                if constexpr (ERASE) {
+                    uint64_t exp = sk;
+                    my_flag |= (uint32_t)cas_slot(stash.keys + j, exp, 0ull);
+                } else {
+                    my_val = stash.vals[j];
+                    my_flag = 1;
+                    break;
+                }
+            }
+        }
+        if (inb) {
+            if (!ERASE && vals) vals[i] = my_val;
+            if (flags) flags[i] = (uint8_t)my_flag;
+        }
+        ndel += my_flag;
+    }
+    if constexpr (ERASE) wave_add(&C->deleted, ndel);
+}
+
+// ---------------------------------------------------------------------------------- insert
+// One lane per in-hand entry; a lane that finishes its chain takes the next entry of its
+// grid-stride sequence, so it never waits behind a longer chain in its wave. The chain is the
+// narrow table's (victim = first empty slot; else, for kick_bound iterations, the first slot
+// whose resident's pair index differs from the table number; else slot iteration & 15; then
+// the rescue walk), over fold(k), with the three-step exchange described at the top.
+struct WideCarry {
+    uint64_t key, val;
+    uint32_t pre, kicks, retries;
+    bool active, fresh;
+    uint64_t cur[kSlots];  // the lane's view of the bucket it is about to write
+};
+
+__device__ inline void wide_give_up(WideCarry &c, const WideSink &sink, Counters *C)
+{
+    const unsigned long long at = atomicAdd(&C->pending, 1ull);
+    if (at < sink.cap) {
+        sink.keys[at] = c.key;
+        sink.vals[at] = c.val;
+    } else {
+        atomicAdd(&C->lost, 1ull);
+    }
+    c.active = false;
+}
+
+template <bool FROM_ENTRIES>
+__global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint64_t *__restrict__ keys,
+                                                          const uint64_t *__restrict__ vals, uint64_t n,
+                                                          uint32_t pre_fixed, InsertPolicy P, Counters *__restrict__ C,
+                                                          WideSink sink)
+{
+    const Desc &D = W.D;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t next = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total_kicks = P.kick_bound + P.rescue_kicks;
+    unsigned long long kicks = 0, rejected = 0, retries = 0;
+    WideCarry c;
+    c.active = false;
+    for (;;) {
+        while (!c.active && next < n) {  // take the next entry
+            const uint64_t k = keys[next];
+            const uint64_t v = vals[next];
+            next += stride;
+            if (wide_reserved(k)) {
+                ++rejected;
+                continue;
+            }
+            uint32_t pr = FROM_ENTRIES ? pre_fixed : kNoTable;
+            if (pr == kNoTable && (k & 1u)) pr = pos1(D, pair_of(D, fold64(k)));  // odd keys start in pos2
+            c.key = k;
+            c.val = v;
+            c.pre = pr;
+            c.kicks = 0;
+            c.retries = 0;
+            c.fresh = true;
+            c.active = true;
+        }
+        if (!__any(c.active)) break;
+        if (!c.active) continue;
+
+        const uint32_t f = fold64(c.key);
+        const uint32_t t = other_pos(D, pair_of(D, f), c.pre);
+        const uint32_t b = bucket_of(D, f, t);
+        uint64_t *kp = selt(t, D.sub) + (uint64_t)b * kSlots;
+        uint64_t *vp = selt(t, W.val) + (uint64_t)b * kSlots;
+        if (c.fresh) {
+#pragma unroll
+            for (int j = 0; j < kSlots; j += 2) {
+                uint64_t e[2];
+                load2(kp + j, e);
+                c.cur[j] = e[0];
+                c.cur[j + 1] = e[1];
+            }
+            c.fresh = false;
+        }
+        int slot = -1;
+#pragma unroll
+        for (int j = kSlots - 1; j >= 0; --j)
+            if (c.cur[j] == 0ull) slot = j;
+        if (slot < 0) {
+            if (c.kicks < P.kick_bound) {
+#pragma unroll
+                for (int j = kSlots - 1; j >= 0; --j)
+                    if (c.cur[j] != kWideBusy && vote_victim(D, fold64(c.cur[j]), t)) slot = j;
+                if (slot < 0) slot = (int)(c.kicks & 15u);
+            } else {
+                slot = (int)rescue_lane(f, c.kicks);
+            }
+        }
+        uint64_t expected = 0;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j)
+            if (j == slot) expected = c.cur[j];
+        // claim the slot; on a lost race (or a slot another lane is rewriting, refreshed with a
+        // BUSY -> BUSY CAS, which changes nothing) the slot's current key replaces the view's
+        uint64_t exp = expected;
+        const bool claimed = expected != kWideBusy ? cas_slot(kp + slot, exp, kWideBusy) : false;
+        if (expected == kWideBusy) (void)cas_slot(kp + slot, exp, kWideBusy);
+        if (!claimed) {
+#pragma unroll
+            for (int j = 0; j < kSlots; ++j)
+                if (j == slot) c.cur[j] = exp;
+        }
+        if (!claimed) {
+            ++retries;
+            if (++c.retries > P.max_retries) {
+                kicks += c.kicks;
+                wide_give_up(c, sink, C);
+            }
+            continue;
+        }
+        const uint64_t old_val =
+            __hip_atomic_exchange(vp + slot, c.val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the value exchange has returned, so it is performed before the key is published
+        __builtin_amdgcn_s_waitcnt(0);
+        (void)__hip_atomic_exchange(kp + slot, c.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ++c.kicks;
+        c.fresh = true;
+        if (expected == 0ull) {
+            kicks += c.kicks;
+            c.active = false;
+        } else {
+            c.key = expected;
+            c.val = old_val;
+            c.pre = t;
+            if (c.kicks >= total_kicks) {
+                kicks += c.kicks;
+                wide_give_up(c, sink, C);
+            }
+        }
+    }
+    wave_add(&C->kicks, kicks);
+    wave_add(&C->rejected, rejected);
+    wave_add(&C->retries, retries);
+}
+
+// ---------------------------------------------------------------------------------- resize
+// Upsize (cuckoo.cu:600-627): key and value of slot i of old bucket j move to slot i of new
+// bucket h % 2n, which is j or j + n; each 8-lane group writes both destination buckets of
+// both arrays in full (entry or zero).
+__global__ __launch_bounds__(256) void wide_upsize_kernel(const uint64_t *__restrict__ ok_, const uint64_t *__restrict__ ov,
+                                                          uint32_t old_nb, uint64_t *__restrict__ dk,
+                                                          uint64_t *__restrict__ dv, uint32_t t, uint32_t m,
+                                                          uint32_t s1, uint32_t s2, Counters *__restrict__ C)
+{
+    constexpr int G = 8, NG = kWave / G;
+    const int lane = __lane_id();
+    const int g = lane / G, s = lane % G;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    const uint32_t nnb = old_nb * 2u;
+    unsigned long long bad = 0;
+    for (uint64_t tile = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * NG; tile < old_nb;
+         tile += waves * NG) {
+        const uint64_t j = tile + g;
+        if (j >= old_nb) continue;
+        uint64_t k[2], v[2], lk[2] = {0, 0}, lv[2] = {0, 0}, hk[2] = {0, 0}, hv[2] = {0, 0};
+        load2(ok_ + j * kSlots + s * 2, k);
+        load2(ov + j * kSlots + s * 2, v);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (!k[q]) continue;
+            const uint32_t d = fast_mod(hash_t(fold64(k[q]), t), nnb, m, s1, s2);
+            if (d == j) {
+                lk[q] = k[q];
+                lv[q] = v[q];
+            } else if (d == j + old_nb) {
+                hk[q] = k[q];
+                hv[q] = v[q];
+            } else {
+                ++bad;
+            }
+        }
+        store2(dk + j * kSlots + s * 2, lk[0], lk[1]);
+        store2(dv + j * kSlots + s * 2, lv[0], lv[1]);
+        store2(dk + (j + old_nb) * kSlots + s * 2, hk[0], hk[1]);
+        store2(dv + (j + old_nb) * kSlots + s * 2, hv[0], hv[1]);
+    }
+    wave_add(&C->misplaced, bad);
+}
+
+// Downsize compaction (cuckoo.cu:664-699): new bucket g keeps old bucket g slot for slot and
+// fills its empty slots from old bucket g + n read back to front (slot i <- slot 15 - i); an
+// entry of bucket g + n is a residual unless it is the one placed there (the reference compares
+// the placed entry with it, cuckoo.cu:685-688), here key and value both.
+__global__ __launch_bounds__(256) void wide_downsize_kernel(const uint64_t *__restrict__ ok_,
+                                                            const uint64_t *__restrict__ ov, uint32_t new_nb,
+                                                            uint64_t *__restrict__ dk, uint64_t *__restrict__ dv,
+                                                            Counters *__restrict__ C, uint64_t *__restrict__ rk,
+                                                            uint64_t *__restrict__ rv, uint64_t res_cap)
+{
+    constexpr int G = 8, NG = kWave / G;
+    const int lane = __lane_id();
+    const int g = lane / G, s = lane % G;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    for (uint64_t tile = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * NG; tile < new_nb;
+         tile += waves * NG) {
+        const uint64_t j = tile + g;
+        const bool ok = j < new_nb;
+        uint64_t ek[2] = {0, 0}, ev[2] = {0, 0}, ck[2] = {0, 0}, cv[2] = {0, 0};
+        if (ok) {
+            load2(ok_ + j * kSlots + s * 2, ek);
+            load2(ov + j * kSlots + s * 2, ev);
+            uint64_t xk[2], xv[2];
+            load2(ok_ + (j + new_nb) * kSlots + (7 - s) * 2, xk);
+            load2(ov + (j + new_nb) * kSlots + (7 - s) * 2, xv);
+            ck[0] = xk[1];  // slot 15 - 2s
+            cv[0] = xv[1];
+            ck[1] = xk[0];  // slot 14 - 2s
+            cv[1] = xv[0];
+        }
+        uint64_t pk[2], pv[2], rk_[2], rv_[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            pk[q] = ek[q] ? ek[q] : ck[q];
+            pv[q] = ek[q] ? ev[q] : cv[q];
+            const bool res = ck[q] != 0 && !(pk[q] == ck[q] && pv[q] == cv[q]);
+            rk_[q] = res ? ck[q] : 0;
+            rv_[q] = res ? cv[q] : 0;
+        }
+        if (ok) {
+            store2(dk + j * kSlots + s * 2, pk[0], pk[1]);
+            store2(dv + j * kSlots + s * 2, pv[0], pv[1]);
+        }
+        const uint64_t b0 = __ballot(rk_[0] != 0), b1 = __ballot(rk_[1] != 0);
+        const uint32_t tot = __popcll(b0) + __popcll(b1);
+        if (tot == 0) continue;
+        unsigned long long at = 0;
+        if (lane == 0) at = atomicAdd(&C->pending, (unsigned long long)tot);
+        at = shfl64(at, 0);
+        const uint64_t o0 = at + __popcll(b0 & lanemask_lt());
+        const uint64_t o1 = at + __popcll(b0) + __popcll(b1 & lanemask_lt());
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (!rk_[q]) continue;
+            const uint64_t o = q == 0 ? o0 : o1;
+            if (o < res_cap) {
+                rk[o] = rk_[q];
+                rv[o] = rv_[q];
+            } else {
+                atomicAdd(&C->lost, 1ull);
+            }
+        }
+    }
+}
+
+constexpr uint64_t kWideInsertBlocks = 16384;
+
+}  // namespace
+
+// ------------------------------------------------------------------------------ launchers
+hipError_t launch_wide_probe(const WideDesc &W, const uint64_t *keys, uint64_t *vals, uint8_t *flags, uint64_t n,
+                             bool erase, LaunchShape sh, Counters *C, const WideStash &stash, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = grid_for((n + kWave - 1) / kWave, sh);
+    if (erase)
+        wide_probe_kernel<true><<<grid, sh.block_threads, 0, st>>>(W, keys, nullptr, flags, n, C, stash);
+    else
+        wide_probe_kernel<false><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, flags, n, C, stash);
+    return hipGetLastError();
+}
+
+hipError_t launch_wide_insert(const WideDesc &W, const uint64_t *keys, const uint64_t *vals, uint64_t n,
+                              uint32_t pre_fixed, bool from_entries, InsertPolicy pol, LaunchShape sh, Counters *C,
+                              const WideSink &sink, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = grid_for((n + kWave - 1) / kWave, sh, kWideInsertBlocks);
+    if (from_entries)
+        wide_insert_kernel<true><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, pre_fixed, pol, C, sink);
+    else
+        wide_insert_kernel<false><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, kNoTable, pol, C, sink);
+    return hipGetLastError();
+}
+
+hipError_t launch_wide_upsize(const uint64_t *ok_, const uint64_t *ov, uint32_t old_nb, uint64_t *dk, uint64_t *dv,
+                              uint32_t t, Counters *C, hipStream_t st)
+{
+    const FastDiv f = make_fastdiv(old_nb * 2u);
+    const int grid = grid_for(((uint64_t)old_nb + 7) / 8, LaunchShape{256, 0});
+    wide_upsize_kernel<<<grid, 256, 0, st>>>(ok_, ov, old_nb, dk, dv, t, f.m, f.s1, f.s2, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_wide_downsize(const uint64_t *ok_, const uint64_t *ov, uint32_t new_nb, uint64_t *dk, uint64_t *dv,
+                                Counters *C, uint64_t *rk, uint64_t *rv, uint64_t res_cap, hipStream_t st)
+{
+    const int grid = grid_for(((uint64_t)new_nb + 7) / 8, LaunchShape{256, 0});
+    wide_downsize_kernel<<<grid, 256, 0, st>>>(ok_, ov, new_nb, dk, dv, C, rk, rv, res_cap);
+    return hipGetLastError();
+}
+
+}  // namespace dyc
+
+// ------------------------------------------------------------------------------- host side
+using namespace dyc;
+using dyc::host::choose_down;
+using dyc::host::choose_up;
+using dyc::host::config_subtables;
+using dyc::host::DeviceGuard;
+using dyc::host::fail;
+using dyc::host::initial_buckets;
+using dyc::host::kMaxBuckets;
+
+struct dyc_wide {
+    dyc_config cfg;
+    int device = 0;
+    WideDesc W{};
+    // storage: subtable t's keys (nb * 128 B) then its values (nb * 128 B); the subtables a
+    // table is created or loaded with share one allocation (the slab), a resized one has its own
+    uint8_t *slab = nullptr;
+    uint32_t slab_live = 0;
+    uint8_t *own[kMaxSubtables] = {};
+    uint64_t count = 0;
+    uint64_t *pk = nullptr, *pv = nullptr;  // pending keys / values [0, pend_n): the stash
+    uint64_t pend_cap = 0, pend_n = 0;
+    Counters *ctr = nullptr, *hctr = nullptr;
+    dyc_stats st{};
+
+    uint64_t capacity() const
+    {
+        uint64_t c = 0;
+        for (uint32_t i = 0; i < W.D.d; ++i) c += (uint64_t)W.D.nb[i] * kSlots;
+        return c;
+    }
+    LaunchShape shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.max_blocks}; }
+    LaunchShape insert_shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.insert_blocks}; }
+    InsertPolicy policy() const { return InsertPolicy{cfg.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
+    WideSink sink_at(uint64_t at) const { return WideSink{pk + at, pv + at, pend_cap - at}; }
+    WideStash stash() const { return WideStash{pk, pv, pend_n}; }
+};
+
+namespace {
+
+constexpr size_t kWideAlign = 2u << 20;
+size_t wide_bytes(uint32_t nb) { return (size_t)nb * kBucketBytes; }  // one array of a subtable
+
+int wzero(dyc_wide *t, hipStream_t s)
+{
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->ctr), kCounterWords, s));
+    return DYC_OK;
+}
+
+int wread(dyc_wide *t, hipStream_t s)
+{
+    HIP_TRY(hipMemcpyAsync(t->hctr, t->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return DYC_OK;
+}
+
+// Release subtable i's storage (its own allocation, or its share of the slab). Synchronise first.
+int wrelease(dyc_wide *t, uint32_t i)
+{
+    if (!t->W.D.sub[i]) return DYC_OK;
+    if (t->own[i]) {
+        HIP_TRY(hipFree(t->own[i]));
+        t->own[i] = nullptr;
+    } else if (t->slab && --t->slab_live == 0) {
+        HIP_TRY(hipFree(t->slab));
+        t->slab = nullptr;
+    }
+    t->W.D.sub[i] = nullptr;
+    t->W.val[i] = nullptr;
+    return DYC_OK;
+}
+
+void wset(dyc_wide *t, uint32_t i, uint8_t *base, uint32_t nb)
+{
+    desc_set(t->W.D, (int)i, reinterpret_cast<uint64_t *>(base), nb);
+    t->W.val[i] = reinterpret_cast<uint64_t *>(base + wide_bytes(nb));
+}
+
+// One allocation for the key and value arrays of d subtables of nb[i] buckets; replaces every
+// subtable of t (zeroed when `zero`).
+int winstall_slab(dyc_wide *t, const uint32_t *nb, bool zero, hipStream_t s, uint8_t **region)
+{
+    const uint32_t d = t->W.D.d;
+    size_t off[kMaxSubtables + 1] = {0};
+    for (uint32_t i = 0; i < d; ++i) off[i + 1] = off[i] + ((2 * wide_bytes(nb[i]) + kWideAlign - 1) & ~(kWideAlign - 1));
+    uint8_t *base = nullptr;
+    if (hipMalloc(&base, off[d]) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DYC_ERR_OOM, "hipMalloc of %zu bytes for the table failed", off[d]);
+    }
+    for (uint32_t i = 0; i < d; ++i) {
+        const int rc = wrelease(t, i);
+        if (rc) return rc;
+    }
+    t->slab = base;
+    t->slab_live = d;
+    for (uint32_t i = 0; i < d; ++i) {
+        wset(t, i, base + off[i], nb[i]);
+        if (region) region[i] = base + off[i];
+        if (zero) HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(base + off[i]), 2 * (uint64_t)nb[i] * kSlots, s));
+    }
+    return DYC_OK;
+}
+
+int wreserve_pending(dyc_wide *t, uint64_t extra, hipStream_t s)
+{
+    if (t->pend_n + extra <= t->pend_cap) return DYC_OK;
+    const uint64_t cap = std::max<uint64_t>(2 * t->pend_cap, t->pend_n + extra);
+    uint64_t *nk = nullptr, *nv = nullptr;
+    if (hipMalloc(&nk, cap * 8) != hipSuccess || hipMalloc(&nv, cap * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        if (nk) (void)hipFree(nk);
+        return fail(DYC_ERR_OOM, "hipMalloc for the pending list failed");
+    }
+    if (t->pend_n) {
+        HIP_TRY(hipMemcpyAsync(nk, t->pk, t->pend_n * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(nv, t->pv, t->pend_n * 8, hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (t->pk) HIP_TRY(hipFree(t->pk));
+    if (t->pv) HIP_TRY(hipFree(t->pv));
+    t->pk = nk;
+    t->pv = nv;
+    t->pend_cap = cap;
+    return DYC_OK;
+}
+
+int wabsorb(dyc_wide *t, uint64_t room)
+{
+    const Counters &c = *t->hctr;
+    t->st.kicks += c.kicks;
+    t->st.last_kicks += c.kicks;
+    t->st.cas_retries += c.retries;
+    t->st.kick_failures += c.pending;
+    t->st.lost += c.lost;
+    t->pend_n += std::min<uint64_t>(c.pending, room);
+    if (c.lost) return fail(DYC_ERR_LOST, "pending list overflow: %llu entries lost", (unsigned long long)c.lost);
+    return DYC_OK;
+}
+
+int wresize_up(dyc_wide *t, hipStream_t s)
+{
+    const uint32_t sub = choose_up(t->W.D.nb, t->W.D.d);
+    const uint32_t old_nb = t->W.D.nb[sub];
+    if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^29 buckets", sub);
+    uint8_t *dst = nullptr;
+    if (hipMalloc(&dst, 2 * wide_bytes(old_nb * 2)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DYC_ERR_OOM, "hipMalloc for upsize failed");
+    }
+    int rc = wzero(t, s);
+    if (rc) return rc;
+    HIP_TRY(launch_wide_upsize(t->W.D.sub[sub], t->W.val[sub], old_nb, reinterpret_cast<uint64_t *>(dst),
+                               reinterpret_cast<uint64_t *>(dst + wide_bytes(old_nb * 2)), sub, t->ctr, s));
+    rc = wread(t, s);
+    if (rc) return rc;
+    rc = wrelease(t, sub);
+    if (rc) return rc;
+    wset(t, sub, dst, old_nb * 2);
+    t->own[sub] = dst;
+    t->st.resizes_up++;
+    if (t->hctr->misplaced)
+        return fail(DYC_ERR_CORRUPT, "upsize found %llu entries outside their bucket",
+                    (unsigned long long)t->hctr->misplaced);
+    return DYC_OK;
+}
+
+// Re-insert the pending entries after an operation that found `before` pending and left more
+// (upsizing before each round, at most max_fail_rounds rounds); see place_pending of the narrow
+// table.
+int wplace_pending(dyc_wide *t, hipStream_t s, uint64_t before)
+{
+    uint32_t rounds = 0;
+    while (t->pend_n > before && rounds < t->cfg.max_fail_rounds) {
+        int rc = wresize_up(t, s);
+        if (rc) return rc;
+        const uint64_t m = t->pend_n;
+        uint64_t *sk = nullptr, *sv = nullptr;
+        if (hipMalloc(&sk, m * 8) != hipSuccess || hipMalloc(&sv, m * 8) != hipSuccess) {
+            (void)hipGetLastError();
+            if (sk) (void)hipFree(sk);
+            return fail(DYC_ERR_OOM, "hipMalloc for pending re-insert failed");
+        }
+        HIP_TRY(hipMemcpyAsync(sk, t->pk, m * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(sv, t->pv, m * 8, hipMemcpyDeviceToDevice, s));
+        t->pend_n = 0;
+        rc = wzero(t, s);
+        if (!rc) {
+            const hipError_t e = launch_wide_insert(t->W, sk, sv, m, kNoTable, true, t->policy(), t->insert_shape(),
+                                                    t->ctr, t->sink_at(0), s);
+            if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "pending re-insert launch: %s", hipGetErrorString(e));
+        }
+        if (!rc) rc = wread(t, s);
+        (void)hipFree(sk);
+        (void)hipFree(sv);
+        if (rc) return rc;
+        rc = wabsorb(t, t->pend_cap);
+        if (rc) return rc;
+        ++rounds;
+    }
+    t->st.pending = t->pend_n;
+    if (t->pend_n > before)
+        return fail(DYC_ERR_PENDING, "%llu entries still pending after %u upsize rounds (%llu before the call)",
+                    (unsigned long long)t->pend_n, rounds, (unsigned long long)before);
+    return DYC_OK;
+}
+
+int wresize_down(dyc_wide *t, hipStream_t s)
+{
+    const uint64_t pend_before = t->pend_n;
+    const uint32_t sub = choose_down(t->W.D.nb, t->W.D.d);
+    const uint32_t old_nb = t->W.D.nb[sub];
+    if (old_nb & 1u)
+        return fail(DYC_ERR_ODD_BUCKETS, "downsize of subtable %u refused: %u buckets is odd", sub, old_nb);
+    const uint32_t new_nb = old_nb / 2;
+    const uint64_t res_cap = (uint64_t)new_nb * kSlots;
+    uint8_t *dst = nullptr;
+    uint64_t *rk = nullptr, *rv = nullptr;
+    if (hipMalloc(&dst, 2 * wide_bytes(new_nb)) != hipSuccess || hipMalloc(&rk, res_cap * 8) != hipSuccess ||
+        hipMalloc(&rv, res_cap * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        if (dst) (void)hipFree(dst);
+        if (rk) (void)hipFree(rk);
+        return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
+    }
+    int rc = wzero(t, s);
+    if (!rc) {
+        const hipError_t e =
+            launch_wide_downsize(t->W.D.sub[sub], t->W.val[sub], new_nb, reinterpret_cast<uint64_t *>(dst),
+                                 reinterpret_cast<uint64_t *>(dst + wide_bytes(new_nb)), t->ctr, rk, rv, res_cap, s);
+        if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "downsize launch: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = wread(t, s);
+    if (!rc && t->hctr->lost) rc = fail(DYC_ERR_LOST, "downsize residual list overflow");
+    if (rc) {
+        (void)hipFree(dst);
+        (void)hipFree(rk);
+        (void)hipFree(rv);
+        return rc;
+    }
+    const uint64_t nres = t->hctr->pending;
+    rc = wrelease(t, sub);
+    if (rc) return rc;
+    wset(t, sub, dst, new_nb);
+    t->own[sub] = dst;
+    t->st.resizes_down++;
+    // residuals continue from the shrunk subtable into the other one of their pair (cuckoo.cu:719)
+    if (nres) {
+        rc = wreserve_pending(t, std::max<uint64_t>(nres, 1024), s);
+        if (!rc) rc = wzero(t, s);
+        if (!rc) {
+            const hipError_t e = launch_wide_insert(t->W, rk, rv, nres, sub, true, t->policy(), t->insert_shape(),
+                                                    t->ctr, t->sink_at(t->pend_n), s);
+            if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "residual insert launch: %s", hipGetErrorString(e));
+        }
+        if (!rc) rc = wread(t, s);
+        if (!rc) rc = wabsorb(t, t->pend_cap - t->pend_n);
+    }
+    (void)hipFree(rk);
+    (void)hipFree(rv);
+    if (rc) return rc;
+    if (t->pend_n > pend_before) return wplace_pending(t, s, pend_before);
+    return DYC_OK;
+}
+
+// Erases leave the stash entries they deleted as zero keys: drop them (synchronous; the list
+// is short whenever it is not empty).
+int wcompact_pending(dyc_wide *t, hipStream_t s)
+{
+    if (t->pend_n == 0) return DYC_OK;
+    std::vector<uint64_t> k(t->pend_n), v(t->pend_n);
+    HIP_TRY(hipMemcpyAsync(k.data(), t->pk, t->pend_n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(v.data(), t->pv, t->pend_n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < k.size(); ++i)
+        if (k[i]) {
+            k[m] = k[i];
+            v[m] = v[i];
+            ++m;
+        }
+    if (m != t->pend_n) {
+        if (m) {
+            HIP_TRY(hipMemcpyAsync(t->pk, k.data(), m * 8, hipMemcpyHostToDevice, s));
+            HIP_TRY(hipMemcpyAsync(t->pv, v.data(), m * 8, hipMemcpyHostToDevice, s));
+        }
+        HIP_TRY(hipStreamSynchronize(s));
+        t->pend_n = m;
+    }
+    t->st.pending = t->pend_n;
+    return DYC_OK;
+}
+
+bool wide_check_config(const dyc_config &cfg)
+{
+    const uint32_t d = config_subtables(cfg);
+    return cfg.block_threads && cfg.block_threads % 64 == 0 && cfg.block_threads <= 256 &&
+           cfg.upper_bound > 0.0 && cfg.upper_bound <= 1.0 && cfg.lower_bound >= 0.0 &&
+           cfg.lower_bound < cfg.upper_bound && d >= (uint32_t)kMinSubtables && d <= (uint32_t)kMaxSubtables;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dyc_wide_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_wide **out)
+{
+    if (!out) return fail(DYC_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    dyc_config cfg;
+    if (cfg_in)
+        cfg = *cfg_in;
+    else
+        dyc_config_init(&cfg);
+    if (!wide_check_config(cfg))
+        return fail(DYC_ERR_INVALID, "bad configuration (block_threads, fill bounds or subtables)");
+    cfg.subtables = config_subtables(cfg);
+    const uint32_t nb = initial_buckets(capacity_hint, cfg.subtables);
+    if (nb >= kMaxBuckets) return fail(DYC_ERR_INVALID, "capacity hint too large");
+    dyc_wide *t = new dyc_wide();
+    t->cfg = cfg;
+    desc_init(t->W.D, cfg.subtables);
+    int rc = DYC_OK;
+    if (hipGetDevice(&t->device) != hipSuccess || hipMalloc(&t->ctr, sizeof(Counters)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(DYC_ERR_OOM, "counter allocation failed");
+    }
+    const uint32_t nbs[kMaxSubtables] = {nb, nb, nb, nb, nb, nb};
+    if (rc == DYC_OK) rc = winstall_slab(t, nbs, true, nullptr, nullptr);
+    if (rc == DYC_OK) rc = wreserve_pending(t, 1024, nullptr);
+    if (rc == DYC_OK && hipStreamSynchronize(nullptr) != hipSuccess) rc = fail(DYC_ERR_HIP, "create sync failed");
+    if (rc != DYC_OK) {
+        char keep[512];
+        strncpy(keep, dyc_last_error(), sizeof keep - 1);
+        keep[sizeof keep - 1] = 0;
+        dyc_wide_destroy(t);
+        fail(rc, "%s", keep);
+        return rc;
+    }
+    *out = t;
+    return DYC_OK;
+}
+
+int dyc_wide_destroy(dyc_wide *t)
+{
+    if (!t) return DYC_OK;
+    DeviceGuard dg(t->device);
+    (void)hipDeviceSynchronize();
+    for (uint32_t i = 0; i < t->W.D.d; ++i) (void)wrelease(t, i);
+    if (t->pk) (void)hipFree(t->pk);
+    if (t->pv) (void)hipFree(t->pv);
+    if (t->ctr) (void)hipFree(t->ctr);
+    if (t->hctr) (void)hipHostFree(t->hctr);
+    delete t;
+    return DYC_OK;
+}
+
+int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    t->st.last_kicks = 0;
+    if (n == 0) return DYC_OK;
+    const uint64_t pend_before = t->pend_n;
+    int rc;
+    if (t->cfg.auto_resize) {
+        while ((double)(t->count + n) > t->cfg.upper_bound * (double)t->capacity()) {
+            rc = wresize_up(t, s);
+            if (rc) return rc;
+        }
+    }
+    rc = wreserve_pending(t, std::max<uint64_t>(65536, n / 16), s);
+    if (rc) return rc;
+    const uint64_t room = t->pend_cap - t->pend_n;
+    rc = wzero(t, s);
+    if (rc) return rc;
+    HIP_TRY(launch_wide_insert(t->W, keys, vals, n, kNoTable, false, t->policy(), t->insert_shape(), t->ctr,
+                               t->sink_at(t->pend_n), s));
+    rc = wread(t, s);
+    if (rc) return rc;
+    const uint64_t rejected = t->hctr->rejected;
+    t->st.rejected += rejected;
+    t->count += n - rejected;
+    rc = wabsorb(t, room);
+    t->st.pending = t->pend_n;
+    if (rc) return rc;
+    if (t->pend_n > pend_before) return wplace_pending(t, s, pend_before);
+    return DYC_OK;
+}
+
+int dyc_wide_find(dyc_wide *t, const uint64_t *keys, uint64_t *vals, uint8_t *hit, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    DeviceGuard dg(t->device);
+    HIP_TRY(launch_wide_probe(t->W, keys, vals, hit, n, false, t->shape(), t->ctr, t->stash(), (hipStream_t)stream));
+    return DYC_OK;
+}
+
+int dyc_wide_erase(dyc_wide *t, const uint64_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) return DYC_OK;
+    int rc = wzero(t, s);
+    if (rc) return rc;
+    HIP_TRY(launch_wide_probe(t->W, keys, nullptr, deleted, n, true, t->shape(), t->ctr, t->stash(), s));
+    rc = wread(t, s);
+    if (rc) return rc;
+    t->count -= std::min<uint64_t>(t->count, t->hctr->deleted);
+    rc = wcompact_pending(t, s);
+    if (rc) return rc;
+    if (t->cfg.auto_resize && (double)t->count < t->cfg.lower_bound * (double)t->capacity() &&
+        t->capacity() > t->cfg.min_capacity) {
+        rc = wresize_down(t, s);
+        if (rc == DYC_ERR_ODD_BUCKETS) rc = DYC_OK;  // policy: skip, the table stays larger
+        if (rc) return rc;
+    }
+    return DYC_OK;
+}
+
+int dyc_wide_resize_up(dyc_wide *t, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    DeviceGuard dg(t->device);
+    return wresize_up(t, (hipStream_t)stream);
+}
+
+int dyc_wide_resize_down(dyc_wide *t, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    DeviceGuard dg(t->device);
+    return wresize_down(t, (hipStream_t)stream);
+}
+
+int dyc_wide_get_stats(const dyc_wide *t, dyc_stats *out)
+{
+    if (!t || !out) return fail(DYC_ERR_INVALID, "null argument");
+    *out = t->st;
+    out->count = t->count;
+    out->capacity = t->capacity();
+    for (uint32_t i = 0; i < kMaxSubtables; ++i) out->buckets[i] = i < t->W.D.d ? t->W.D.nb[i] : 0;
+    out->pending = t->pend_n;
+    out->subtables = t->W.D.d;
+    return DYC_OK;
+}
+
+int dyc_wide_clear(dyc_wide *t, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    for (uint32_t i = 0; i < t->W.D.d; ++i) {
+        HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->W.D.sub[i]), (uint64_t)t->W.D.nb[i] * kSlots, s));
+        HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->W.val[i]), (uint64_t)t->W.D.nb[i] * kSlots, s));
+    }
+    t->count = 0;
+    t->pend_n = 0;
+    t->st.pending = 0;
+    return DYC_OK;
+}
+
+int dyc_wide_dump(dyc_wide *t, uint32_t sub, uint64_t *host_keys, uint64_t *host_vals, uint64_t n_slots,
+                  dyc_stream stream)
+{
+    if (!t || sub >= t->W.D.d || (!host_keys && !host_vals)) return fail(DYC_ERR_INVALID, "bad argument");
+    const uint64_t slots = (uint64_t)t->W.D.nb[sub] * kSlots;
+    if (n_slots < slots) return fail(DYC_ERR_INVALID, "buffer holds %llu slots, subtable has %llu",
+                                     (unsigned long long)n_slots, (unsigned long long)slots);
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (host_keys) HIP_TRY(hipMemcpyAsync(host_keys, t->W.D.sub[sub], slots * 8, hipMemcpyDeviceToHost, s));
+    if (host_vals) HIP_TRY(hipMemcpyAsync(host_vals, t->W.val[sub], slots * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return DYC_OK;
+}
+
+int dyc_wide_load(dyc_wide *t, const uint32_t *nb, const uint64_t *const *key_images,
+                  const uint64_t *const *val_images, dyc_stream stream)
+{
+    if (!t || !nb || !key_images || !val_images) return fail(DYC_ERR_INVALID, "null argument");
+    for (uint32_t i = 0; i < t->W.D.d; ++i)
+        if (nb[i] == 0 || nb[i] >= kMaxBuckets || !key_images[i] || !val_images[i])
+            return fail(DYC_ERR_INVALID, "bad subtable %u", i);
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipStreamSynchronize(s));
+    uint8_t *region[kMaxSubtables] = {};
+    int rc = winstall_slab(t, nb, false, s, region);
+    if (rc) return rc;
+    uint64_t count = 0;
+    for (uint32_t i = 0; i < t->W.D.d; ++i) {
+        HIP_TRY(hipMemcpyAsync(region[i], key_images[i], wide_bytes(nb[i]), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(region[i] + wide_bytes(nb[i]), val_images[i], wide_bytes(nb[i]), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (uint64_t j = 0; j < (uint64_t)nb[i] * kSlots; ++j) count += key_images[i][j] != 0;
+    }
+    t->count = count;
+    t->pend_n = 0;
+    t->st.pending = 0;
+    return DYC_OK;
+}
+
+int dyc_wide_pending(dyc_wide *t, uint64_t *host_keys, uint64_t *host_vals, uint64_t cap, uint64_t *n_out,
+                     dyc_stream stream)
+{
+    if (!t || !n_out) return fail(DYC_ERR_INVALID, "null argument");
+    *n_out = t->pend_n;
+    const uint64_t m = std::min(cap, t->pend_n);
+    if (m) {
+        DeviceGuard dg(t->device);
+        hipStream_t s = (hipStream_t)stream;
+        if (host_keys) HIP_TRY(hipMemcpyAsync(host_keys, t->pk, m * 8, hipMemcpyDeviceToHost, s));
+        if (host_vals) HIP_TRY(hipMemcpyAsync(host_vals, t->pv, m * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return DYC_OK;
+}
+
+}  // extern "C"

@@ -285,6 +285,135 @@ class DyCuckoo:
         return a
 
 
+def _as_u64_tensor(x, device) -> torch.Tensor:
+    """A contiguous 8-byte tensor on `device` whose bits are the uint64 values of x."""
+    if isinstance(x, torch.Tensor):
+        if x.dtype not in (torch.int64, torch.uint64):
+            raise TypeError(f"wide keys/values must be 64-bit, got {x.dtype}")
+        t = x.view(torch.int64) if x.dtype == torch.uint64 else x
+        return t.to(device).contiguous()
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.uint64)).view(np.int64)
+    return torch.from_numpy(a).to(device)
+
+
+class DyCuckooWide:
+    """A dynamic cuckoo hash table of uint64 -> uint64 on one GPU (the paper's wide-key layout:
+    per bucket a line of 16 keys and a parallel line of 16 values). Tensors are int64 whose bits
+    are the uint64 keys and values; keys 0 and 2^64 - 1 are reserved."""
+
+    def __init__(self, capacity_hint: int, *, device=None, **config):
+        self.lib = _lib.load()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.cfg = _lib.default_config(**config)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_create(ctypes.byref(self.cfg), int(capacity_hint), ctypes.byref(h)),
+                  "dyc_wide_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dyc_wide_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def insert(self, keys, vals, stream=None) -> None:
+        k = _as_u64_tensor(keys, self.device)
+        v = _as_u64_tensor(vals, self.device)
+        if k.numel() != v.numel():
+            raise ValueError("keys and values differ in length")
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_insert(self.h, k.data_ptr(), v.data_ptr(), k.numel(), _stream_ptr(stream)),
+                  "dyc_wide_insert")
+
+    def find(self, keys, vals=None, hit=None, stream=None):
+        """Returns (values int64-as-uint64, hit uint8)."""
+        k = _as_u64_tensor(keys, self.device)
+        n = k.numel()
+        if vals is None:
+            vals = torch.empty(n, dtype=torch.int64, device=self.device)
+        if hit is None:
+            hit = torch.empty(n, dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_find(self.h, k.data_ptr(), vals.data_ptr(), hit.data_ptr(), n, _stream_ptr(stream)),
+                  "dyc_wide_find")
+        return vals, hit
+
+    def erase(self, keys, stream=None) -> torch.Tensor:
+        k = _as_u64_tensor(keys, self.device)
+        deleted = torch.empty(k.numel(), dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_erase(self.h, k.data_ptr(), deleted.data_ptr(), k.numel(), _stream_ptr(stream)),
+                  "dyc_wide_erase")
+        return deleted
+
+    def resize_up(self, stream=None) -> None:
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_resize_up(self.h, _stream_ptr(stream)), "dyc_wide_resize_up")
+
+    def resize_down(self, stream=None) -> bool:
+        """False when the chosen subtable has an odd bucket count (refused)."""
+        with torch.cuda.device(self.device):
+            rc = self.lib.dyc_wide_resize_down(self.h, _stream_ptr(stream))
+        if rc == _lib.DYC_ERR_ODD_BUCKETS:
+            return False
+        check(rc, "dyc_wide_resize_down")
+        return True
+
+    def clear(self, stream=None) -> None:
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_clear(self.h, _stream_ptr(stream)), "dyc_wide_clear")
+
+    def stats(self) -> dict:
+        st = _lib.Stats()
+        check(self.lib.dyc_wide_get_stats(self.h, ctypes.byref(st)), "dyc_wide_get_stats")
+        return st.as_dict()
+
+    @property
+    def buckets(self):
+        return self.stats()["buckets"]
+
+    def dump(self, stream=None):
+        """([key arrays], [value arrays]) of the subtables, host uint64."""
+        ks, vs = [], []
+        for t, nb in enumerate(self.buckets):
+            k = np.empty(nb * SLOTS, dtype=np.uint64)
+            v = np.empty(nb * SLOTS, dtype=np.uint64)
+            with torch.cuda.device(self.device):
+                check(self.lib.dyc_wide_dump(self.h, t, k.ctypes.data, v.ctypes.data, k.size, _stream_ptr(stream)),
+                      "dyc_wide_dump")
+            ks.append(k)
+            vs.append(v)
+        return ks, vs
+
+    def load(self, keys, vals, stream=None) -> None:
+        ks = [np.ascontiguousarray(x, dtype=np.uint64) for x in keys]
+        vs = [np.ascontiguousarray(x, dtype=np.uint64) for x in vals]
+        d = self.stats()["subtables"]
+        if len(ks) != d or len(vs) != d or any(a.size != b.size or a.size % SLOTS or not a.size for a, b in zip(ks, vs)):
+            raise ValueError(f"need {d} pairs of equal, non-empty images of whole buckets")
+        nb = (ctypes.c_uint32 * d)(*[x.size // SLOTS for x in ks])
+        kp = (ctypes.c_void_p * d)(*[x.ctypes.data for x in ks])
+        vp = (ctypes.c_void_p * d)(*[x.ctypes.data for x in vs])
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_load(self.h, nb, kp, vp, _stream_ptr(stream)), "dyc_wide_load")
+
+    def pending(self, stream=None):
+        n = ctypes.c_uint64()
+        check(self.lib.dyc_wide_pending(self.h, None, None, 0, ctypes.byref(n), _stream_ptr(stream)), "dyc_wide_pending")
+        k = np.empty(n.value, dtype=np.uint64)
+        v = np.empty(n.value, dtype=np.uint64)
+        if n.value:
+            check(self.lib.dyc_wide_pending(self.h, k.ctypes.data, v.ctypes.data, n.value, ctypes.byref(n),
+                                            _stream_ptr(stream)), "dyc_wide_pending")
+        return k, v
+
+
 class HashAPI:
     """Host-array facade with the reference's method names (GPU/dyhash/api.h:9-55).
 

@@ -357,6 +357,39 @@ int dyc_server_table_stats(const dyc_server *s, uint32_t table, dyc_stats *out);
 /* Copy subtable `sub` of `table` into host memory (buckets[sub] * 16 slots); drains first. */
 int dyc_server_dump(dyc_server *s, uint32_t table, uint32_t sub, uint64_t *host_slots, uint64_t n_slots);
 
+/* ---------------------------------------------------------------------------------------
+ * Wide keys: a DyCuckoo table of uint64 keys and uint64 values. No counterpart in the reference
+ * code (32-bit keys only); the paper's layout for keys larger than 4 bytes
+ * (tkde-revision/secs/dynamic.tex:35-37): a bucket is a 128-byte line of 16 keys plus the
+ * parallel line of their 16 values. Placement is the 32-bit table's (geometry, subtable count,
+ * pair scheme, victim rule, resize choice) applied to fold(k) = lo32(k) ^ h5(hi32(k)), so keys
+ * below 2^32 sit exactly where the 32-bit table puts them. Keys 0 and 2^64 - 1 are reserved.
+ * Same conventions as the 32-bit calls above (device pointers, status codes, stream order,
+ * insert and erase synchronise once, entries are never dropped and pending ones form a stash
+ * that find and erase search); dyc_config fields that apply: kick_bound, rescue_kicks,
+ * max_retries, max_fail_rounds, fill bounds, auto_resize, min_capacity, block_threads,
+ * max_blocks, insert_blocks, subtables. Insert takes one lane per key. */
+typedef struct dyc_wide dyc_wide;   /* opaque */
+
+int dyc_wide_create(const dyc_config *cfg, uint64_t capacity_hint, dyc_wide **out);
+int dyc_wide_destroy(dyc_wide *t);
+int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, dyc_stream stream);
+/* vals[i] = the value of keys[i] or 0, hit[i] = 1/0 (either may be NULL). Asynchronous. */
+int dyc_wide_find(dyc_wide *t, const uint64_t *keys, uint64_t *vals, uint8_t *hit, uint64_t n, dyc_stream stream);
+int dyc_wide_erase(dyc_wide *t, const uint64_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream);
+int dyc_wide_resize_up(dyc_wide *t, dyc_stream stream);
+int dyc_wide_resize_down(dyc_wide *t, dyc_stream stream);
+int dyc_wide_get_stats(const dyc_wide *t, dyc_stats *out);
+int dyc_wide_clear(dyc_wide *t, dyc_stream stream);
+/* Subtable `sub`'s key and value arrays (buckets[sub] * 16 words each; either may be NULL). */
+int dyc_wide_dump(dyc_wide *t, uint32_t sub, uint64_t *host_keys, uint64_t *host_vals, uint64_t n_slots,
+                  dyc_stream stream);
+/* Replace the contents with host images (per subtable: nb[s] buckets, keys and values). */
+int dyc_wide_load(dyc_wide *t, const uint32_t *nb, const uint64_t *const *key_images,
+                  const uint64_t *const *val_images, dyc_stream stream);
+int dyc_wide_pending(dyc_wide *t, uint64_t *host_keys, uint64_t *host_vals, uint64_t cap, uint64_t *n_out,
+                     dyc_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,11 @@ NO_TABLE = 100
 RESERVED = (0, 0xFFFFFFFF)
 
 
+class _OrcWide(ctypes.Structure):
+    _fields_ = [("keys", ctypes.c_void_p * MAX_SUBTABLES), ("vals", ctypes.c_void_p * MAX_SUBTABLES),
+                ("nb", ctypes.c_uint32 * MAX_SUBTABLES), ("d", ctypes.c_uint32)]
+
+
 class _OrcTable(ctypes.Structure):
     _fields_ = [("sub", ctypes.c_void_p * MAX_SUBTABLES), ("nb", ctypes.c_uint32 * MAX_SUBTABLES),
                 ("d", ctypes.c_uint32)]
@@ -113,6 +118,22 @@ def lib():
         L.orc_count.argtypes = [vp]
         L.orc_check_invariant.restype = u64
         L.orc_check_invariant.argtypes = [vp]
+        L.orc_fold64.restype = u32
+        L.orc_fold64.argtypes = [u64]
+        L.orc_wide_insert_entry.restype = ctypes.c_int
+        L.orc_wide_insert_entry.argtypes = [vp, u64, u64, u32, u32, u32, vp, vp, vp]
+        L.orc_wide_insert.restype = u64
+        L.orc_wide_insert.argtypes = [vp, vp, vp, u64, u32, u32, vp, vp, vp, u64, vp]
+        L.orc_wide_find.restype = None
+        L.orc_wide_find.argtypes = [vp, vp, u64, vp, vp]
+        L.orc_wide_erase.restype = u64
+        L.orc_wide_erase.argtypes = [vp, vp, u64, vp]
+        L.orc_wide_upsize.restype = None
+        L.orc_wide_upsize.argtypes = [vp, vp, u32, vp, vp, u32]
+        L.orc_wide_downsize_compact.restype = u64
+        L.orc_wide_downsize_compact.argtypes = [vp, vp, u32, vp, vp, vp, vp, u64]
+        L.orc_wide_check_invariant.restype = u64
+        L.orc_wide_check_invariant.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -391,3 +412,183 @@ def count_probes(images, keys) -> int:
     t.sub = [np.ascontiguousarray(x, dtype=np.uint64) for x in images]
     t.find(keys)
     return t.last_probes
+
+
+# ---------------------------------------------------------------- wide keys
+
+def _u64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def fold64(keys) -> np.ndarray:
+    """lo ^ h5(hi) of every 64-bit key (the 32-bit word the wide table hashes)."""
+    L = lib()
+    return np.array([L.orc_fold64(int(k)) for k in _u64(keys).tolist()], dtype=np.uint32)
+
+
+class OracleWideTable:
+    """Sequential replay of a wide-key DyCuckoo table (uint64 keys and values) plus the host
+    policy of OracleTable: sizing, auto-resize, pending entries re-placed by upsizing and kept
+    as a stash that find and erase search. Images are (keys, values) uint64 array pairs."""
+
+    def __init__(self, capacity_hint: int = 0, *, nb=None, subtables: int = 0, kick_bound: int = KICK_BOUND,
+                 rescue_kicks: int = RESCUE_KICKS, max_fail_rounds: int = 8, auto_resize: bool = False,
+                 lower: float = 0.5, upper: float = 0.9, min_capacity: float = SUBTABLES * 1024 * 1024 * 1.2):
+        if nb is None:
+            self.d = subtables or SUBTABLES
+            nb = [initial_buckets(capacity_hint, self.d)] * self.d
+        else:
+            self.d = len(nb)
+        self.nb = [int(x) for x in nb]
+        self.keys = [np.zeros(x * SLOTS, dtype=np.uint64) for x in self.nb]
+        self.vals = [np.zeros(x * SLOTS, dtype=np.uint64) for x in self.nb]
+        self.kick_bound, self.rescue_kicks, self.max_fail_rounds = kick_bound, rescue_kicks, max_fail_rounds
+        self.auto_resize, self.lower, self.upper, self.min_capacity = auto_resize, lower, upper, min_capacity
+        self.count = 0
+        self.pend_keys = np.zeros(0, dtype=np.uint64)
+        self.pend_vals = np.zeros(0, dtype=np.uint64)
+        self.resizes = []
+
+    def _struct(self):
+        s = _OrcWide()
+        for t in range(self.d):
+            s.keys[t] = self.keys[t].ctypes.data
+            s.vals[t] = self.vals[t].ctypes.data
+            s.nb[t] = self.nb[t]
+        s.d = self.d
+        return s
+
+    @property
+    def capacity(self) -> int:
+        return sum(self.nb) * SLOTS
+
+    def load_images(self, keys, vals):
+        self.keys = [_u64(x).copy() for x in keys]
+        self.vals = [_u64(x).copy() for x in vals]
+        self.nb = [x.size // SLOTS for x in self.keys]
+        self.d = len(self.keys)
+        self.count = int(sum(int(np.count_nonzero(x)) for x in self.keys))
+
+    def entries(self):
+        """All stored (key, value) pairs, table and stash, sorted by key then value."""
+        k = np.concatenate(self.keys + [self.pend_keys])
+        v = np.concatenate(self.vals + [self.pend_vals])
+        live = k != 0
+        k, v = k[live], v[live]
+        order = np.lexsort((v, k))
+        return k[order], v[order]
+
+    def check_invariant(self) -> int:
+        s = self._struct()
+        return int(lib().orc_wide_check_invariant(ctypes.byref(s)))
+
+    def _reinsert(self, keys, vals, pre):
+        L = lib()
+        s = self._struct()
+        fk, fv = [], []
+        kicks = ctypes.c_uint32(0)
+        ck, cv = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        for k, v in zip(keys.tolist(), vals.tolist()):
+            p = pre
+            if pre == NO_TABLE and (k & 1):
+                p = int(L.orc_get_pos_d(L.orc_pair_d(L.orc_fold64(k), self.d), NO_TABLE, self.d))
+            if not L.orc_wide_insert_entry(ctypes.byref(s), k, v, p, self.kick_bound, self.rescue_kicks,
+                                           ctypes.byref(kicks), ctypes.byref(ck), ctypes.byref(cv)):
+                fk.append(ck.value)
+                fv.append(cv.value)
+        return np.array(fk, dtype=np.uint64), np.array(fv, dtype=np.uint64)
+
+    def _place_pending(self, before: int = 0):
+        rounds = 0
+        while self.pend_keys.size > before and rounds < self.max_fail_rounds:
+            self.resize_up()
+            k, v = self.pend_keys, self.pend_vals
+            self.pend_keys, self.pend_vals = self._reinsert(k, v, NO_TABLE)
+            rounds += 1
+
+    def insert(self, keys, vals):
+        keys, vals = _u64(keys), _u64(vals)
+        n = keys.size
+        before = self.pend_keys.size
+        if self.auto_resize:
+            while self.count + n > self.upper * self.capacity:
+                self.resize_up()
+        s = self._struct()
+        kicks = np.zeros(n, dtype=np.uint32)
+        fk = np.zeros(max(n, 1), dtype=np.uint64)
+        fv = np.zeros(max(n, 1), dtype=np.uint64)
+        rej = ctypes.c_uint64(0)
+        nfail = int(lib().orc_wide_insert(ctypes.byref(s), _p(keys), _p(vals), n, self.kick_bound, self.rescue_kicks,
+                                          _p(kicks), _p(fk), _p(fv), fk.size, ctypes.byref(rej)))
+        self.count += n - int(rej.value)
+        self.pend_keys = np.concatenate([self.pend_keys, fk[:nfail]])
+        self.pend_vals = np.concatenate([self.pend_vals, fv[:nfail]])
+        self._place_pending(before)
+        self.last_kicks = kicks
+        return nfail, int(rej.value)
+
+    def find(self, keys):
+        keys = _u64(keys)
+        vals = np.zeros(keys.size, dtype=np.uint64)
+        hit = np.zeros(keys.size, dtype=np.uint8)
+        s = self._struct()
+        lib().orc_wide_find(ctypes.byref(s), _p(keys), keys.size, _p(vals), _p(hit))
+        for i in np.nonzero(hit == 0)[0]:  # the stash, after pos2: its first entry of the key
+            j = np.nonzero(self.pend_keys == keys[i])[0]
+            if j.size and keys[i] not in (0, 2**64 - 1):
+                vals[i] = self.pend_vals[j[0]]
+                hit[i] = 1
+        return vals, hit
+
+    def erase(self, keys):
+        keys = _u64(keys)
+        deleted = np.zeros(keys.size, dtype=np.uint8)
+        s = self._struct()
+        ndel = int(lib().orc_wide_erase(ctypes.byref(s), _p(keys), keys.size, _p(deleted)))
+        for i in np.nonzero(deleted == 0)[0]:  # the stash: every copy, by the key's first occurrence
+            gone = self.pend_keys == keys[i]
+            if gone.any() and keys[i] not in (0, 2**64 - 1):
+                self.pend_keys, self.pend_vals = self.pend_keys[~gone], self.pend_vals[~gone]
+                deleted[i] = 1
+                ndel += 1
+        self.count -= ndel
+        if self.auto_resize and self.count < self.lower * self.capacity and self.capacity > self.min_capacity:
+            self.resize_down()
+        return deleted
+
+    def resize_up(self) -> int:
+        nb = (ctypes.c_uint32 * self.d)(*self.nb)
+        t = int(lib().orc_choose_up_d(ctypes.byref(nb), self.d))
+        nk = np.zeros(self.nb[t] * 2 * SLOTS, dtype=np.uint64)
+        nv = np.zeros_like(nk)
+        lib().orc_wide_upsize(_p(self.keys[t]), _p(self.vals[t]), self.nb[t], _p(nk), _p(nv), t)
+        self.keys[t], self.vals[t] = nk, nv
+        self.nb[t] *= 2
+        self.resizes.append(("up", t, self.nb[t]))
+        return t
+
+    def resize_down(self, defer_residuals: bool = False):
+        nb = (ctypes.c_uint32 * self.d)(*self.nb)
+        t = int(lib().orc_choose_down_d(ctypes.byref(nb), self.d))
+        if self.nb[t] % 2:
+            return None
+        half = self.nb[t] // 2
+        nk = np.zeros(half * SLOTS, dtype=np.uint64)
+        nv = np.zeros_like(nk)
+        rk = np.zeros(half * SLOTS + 1, dtype=np.uint64)
+        rv = np.zeros_like(rk)
+        nres = int(lib().orc_wide_downsize_compact(_p(self.keys[t]), _p(self.vals[t]), self.nb[t], _p(nk), _p(nv),
+                                                    _p(rk), _p(rv), rk.size))
+        self.keys[t], self.vals[t] = nk, nv
+        self.nb[t] = half
+        self.resizes.append(("down", t, half))
+        rk, rv = rk[:nres].copy(), rv[:nres].copy()
+        if not defer_residuals and nres:
+            before = self.pend_keys.size
+            fk, fv = self._reinsert(rk, rv, t)
+            if fk.size:
+                self.pend_keys = np.concatenate([self.pend_keys, fk])
+                self.pend_vals = np.concatenate([self.pend_vals, fv])
+                self._place_pending(before)
+        return t, rk, rv
+

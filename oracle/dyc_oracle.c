@@ -485,6 +485,206 @@ uint64_t orc_check_invariant(const orc_table *tb)
     return bad;
 }
 
+/* ---------------------------------------------------------------- wide keys */
+
+#define WIDE_BUSY (~(uint64_t)0)
+
+static int wide_reserved(uint64_t k) { return k == 0 || k == WIDE_BUSY; }
+
+/* fold of a 64-bit key onto the 32-bit hashes: lo ^ h5(hi); h5(0) = 0, so keys below 2^32
+ * hash as the 32-bit table hashes them */
+uint32_t orc_fold64(uint64_t k) { return (uint32_t)k ^ orc_h5((uint32_t)(k >> 32)); }
+
+static uint64_t wide_at(const orc_wide *tb, uint32_t t, uint32_t fk)
+{
+    return (uint64_t)orc_bucket(fk, t, tb->nb[t]) * ORC_SLOTS;
+}
+
+/* warp_insert's loop (cuckoo.cu:377-421) over fold(k), key and value exchanged together */
+int orc_wide_insert_entry(orc_wide *tb, uint64_t key, uint64_t val, uint32_t pre, uint32_t kick_bound,
+                          uint32_t rescue_kicks, uint32_t *kicks, uint64_t *carry_key, uint64_t *carry_val)
+{
+    uint32_t n_kicks = 0;
+    const uint32_t total = kick_bound + rescue_kicks;
+    for (uint32_t it = 0; it < total; ++it) {
+        const uint32_t f = orc_fold64(key);
+        const uint32_t t = orc_get_pos_d(orc_pair_d(f, tb->d), pre, tb->d);
+        const uint64_t at = wide_at(tb, t, f);
+        uint64_t *bk = tb->keys[t] + at, *bv = tb->vals[t] + at;
+        int lane = -1;
+        for (int i = 0; i < ORC_SLOTS; ++i)
+            if (bk[i] == 0) { lane = i; break; }
+        if (lane < 0 && it < kick_bound) {
+            if (tb->d == 2u) {
+                lane = 0;
+            } else {
+                for (int i = 0; i < ORC_SLOTS; ++i)
+                    if (orc_pair_d(orc_fold64(bk[i]), tb->d) != t) { lane = i; break; }
+            }
+            if (lane < 0) lane = (int)(it & 15u);
+        }
+        if (lane < 0) lane = (int)orc_rescue_lane(f, it);
+        const uint64_t ok = bk[lane], ov = bv[lane];
+        bk[lane] = key;
+        bv[lane] = val;
+        ++n_kicks;
+        if (ok == 0) {
+            if (kicks) *kicks = n_kicks;
+            return 1;
+        }
+        key = ok;
+        val = ov;
+        pre = t;
+    }
+    if (kicks) *kicks = n_kicks;
+    if (carry_key) *carry_key = key;
+    if (carry_val) *carry_val = val;
+    return 0;
+}
+
+uint64_t orc_wide_insert(orc_wide *tb, const uint64_t *keys, const uint64_t *vals, uint64_t n, uint32_t kick_bound,
+                         uint32_t rescue_kicks, uint32_t *kicks_out, uint64_t *fail_keys, uint64_t *fail_vals,
+                         uint64_t fail_cap, uint64_t *rejected)
+{
+    uint64_t nfail = 0, nrej = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t kicks = 0;
+        uint64_t ck = 0, cv = 0;
+        if (wide_reserved(keys[i])) {
+            ++nrej;
+            if (kicks_out) kicks_out[i] = 0;
+            continue;
+        }
+        uint32_t pre = ORC_NO_TABLE;  /* odd keys start in pos2 (cuckoo.cu:336-341) */
+        if (keys[i] & 1u) pre = orc_get_pos_d(orc_pair_d(orc_fold64(keys[i]), tb->d), ORC_NO_TABLE, tb->d);
+        if (!orc_wide_insert_entry(tb, keys[i], vals[i], pre, kick_bound, rescue_kicks, &kicks, &ck, &cv)) {
+            if (nfail < fail_cap) {
+                if (fail_keys) fail_keys[nfail] = ck;
+                if (fail_vals) fail_vals[nfail] = cv;
+            }
+            ++nfail;
+        }
+        if (kicks_out) kicks_out[i] = kicks;
+    }
+    if (rejected) *rejected = nrej;
+    return nfail;
+}
+
+/* cuckoo.cu:478-497 over fold(k): pos1 bucket, then pos2; a miss gives 0 */
+void orc_wide_find(const orc_wide *tb, const uint64_t *keys, uint64_t n, uint64_t *vals_out, uint8_t *hit_out)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t k = keys[i];
+        uint64_t v = 0;
+        uint8_t hit = 0;
+        if (!wide_reserved(k)) {
+            const uint32_t f = orc_fold64(k), p = orc_pair_d(f, tb->d);
+            for (int step = 0; step < 2 && !hit; ++step) {
+                const uint32_t t = step == 0 ? orc_pos1_d(p, tb->d) : orc_pos2_d(p, tb->d);
+                const uint64_t at = wide_at(tb, t, f);
+                for (int j = 0; j < ORC_SLOTS; ++j)
+                    if (tb->keys[t][at + j] == k) {
+                        v = tb->vals[t][at + j];
+                        hit = 1;
+                        break;
+                    }
+            }
+        }
+        if (vals_out) vals_out[i] = v;
+        if (hit_out) hit_out[i] = hit;
+    }
+}
+
+uint64_t orc_wide_erase(orc_wide *tb, const uint64_t *keys, uint64_t n, uint8_t *deleted_out)
+{
+    uint64_t ndel = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t k = keys[i];
+        int del = 0;
+        if (!wide_reserved(k)) {
+            const uint32_t f = orc_fold64(k), p = orc_pair_d(f, tb->d);
+            for (int step = 0; step < 2 && !del; ++step) {
+                const uint32_t t = step == 0 ? orc_pos1_d(p, tb->d) : orc_pos2_d(p, tb->d);
+                const uint64_t at = wide_at(tb, t, f);
+                for (int j = 0; j < ORC_SLOTS; ++j)
+                    if (tb->keys[t][at + j] == k) {
+                        tb->keys[t][at + j] = 0;
+                        del = 1;
+                    }
+            }
+        }
+        ndel += (uint64_t)del;
+        if (deleted_out) deleted_out[i] = (uint8_t)del;
+    }
+    return ndel;
+}
+
+/* cuckoo.cu:612-624: slot i of old bucket j moves to slot i of new bucket h % 2n (keys and
+ * values); the new arrays are fully written */
+void orc_wide_upsize(const uint64_t *old_keys, const uint64_t *old_vals, uint32_t old_nb, uint64_t *new_keys,
+                     uint64_t *new_vals, uint32_t t)
+{
+    const uint32_t nnb = old_nb * 2u;
+    memset(new_keys, 0, (size_t)nnb * ORC_SLOTS * sizeof(uint64_t));
+    memset(new_vals, 0, (size_t)nnb * ORC_SLOTS * sizeof(uint64_t));
+    for (uint64_t j = 0; j < old_nb; ++j)
+        for (int i = 0; i < ORC_SLOTS; ++i) {
+            const uint64_t k = old_keys[j * ORC_SLOTS + i];
+            if (k == 0) continue;
+            const uint64_t at = (uint64_t)orc_bucket(orc_fold64(k), t, nnb) * ORC_SLOTS + i;
+            new_keys[at] = k;
+            new_vals[at] = old_vals[j * ORC_SLOTS + i];
+        }
+}
+
+/* cuckoo.cu:664-699: new bucket g keeps old bucket g slot for slot and fills its empty slots from
+ * old bucket g+n back to front; an entry of bucket g+n is a residual unless it equals (key and
+ * value) the one placed in its mirrored slot (the reference's comparison, cuckoo.cu:685-688) */
+uint64_t orc_wide_downsize_compact(const uint64_t *old_keys, const uint64_t *old_vals, uint32_t old_nb,
+                                   uint64_t *new_keys, uint64_t *new_vals, uint64_t *res_keys, uint64_t *res_vals,
+                                   uint64_t res_cap)
+{
+    if (old_nb & 1u) return UINT64_MAX;
+    const uint32_t n = old_nb / 2u;
+    uint64_t nres = 0;
+    for (uint64_t g = 0; g < n; ++g)
+        for (int i = 0; i < ORC_SLOTS; ++i) {
+            const uint64_t lo = g * ORC_SLOTS + i, hi = (g + n) * ORC_SLOTS + (ORC_SLOTS - 1 - i);
+            uint64_t k = old_keys[lo], v = old_vals[lo];
+            const uint64_t ck = old_keys[hi], cv = old_vals[hi];
+            if (k == 0) {
+                k = ck;
+                v = cv;
+            }
+            new_keys[lo] = k;
+            new_vals[lo] = v;
+            if (ck != 0 && !(k == ck && v == cv)) {
+                if (nres < res_cap) {
+                    res_keys[nres] = ck;
+                    res_vals[nres] = cv;
+                }
+                ++nres;
+            }
+        }
+    return nres;
+}
+
+uint64_t orc_wide_check_invariant(const orc_wide *tb)
+{
+    uint64_t bad = 0;
+    for (uint32_t t = 0; t < tb->d; ++t)
+        for (uint64_t b = 0; b < tb->nb[t]; ++b)
+            for (int i = 0; i < ORC_SLOTS; ++i) {
+                const uint64_t k = tb->keys[t][b * ORC_SLOTS + i];
+                if (k == 0) continue;
+                const uint32_t f = orc_fold64(k), p = orc_pair_d(f, tb->d);
+                if (wide_reserved(k) || (orc_pos1_d(p, tb->d) != t && orc_pos2_d(p, tb->d) != t) ||
+                    orc_bucket(f, t, tb->nb[t]) != b)
+                    ++bad;
+            }
+    return bad;
+}
+
 /* ---------------------------------------------------------------- CPU baseline */
 
 void orc_find_mt(const orc_table *tb, const uint32_t *keys, uint64_t n, uint32_t *vals_out,

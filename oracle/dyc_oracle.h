@@ -125,6 +125,33 @@ uint64_t orc_count(const orc_table *tb);
  * t in {pos1,pos2}(pair(key)) and b == bucket(key, t). Returns the number of violations. */
 uint64_t orc_check_invariant(const orc_table *tb);
 
+/* ---- wide keys: uint64 keys and values in parallel key / value arrays (tkde-revision/secs/
+ * dynamic.tex:35-37). Placement is the 32-bit table's applied to fold(k) = lo ^ h5(hi); an
+ * exchange swaps key and value together. Keys 0 and ~0 are reserved. ---- */
+typedef struct orc_wide {
+    uint64_t *keys[ORC_MAX_SUBTABLES];
+    uint64_t *vals[ORC_MAX_SUBTABLES];
+    uint32_t nb[ORC_MAX_SUBTABLES];
+    uint32_t d;
+} orc_wide;
+
+uint32_t orc_fold64(uint64_t k);
+int orc_wide_insert_entry(orc_wide *tb, uint64_t key, uint64_t val, uint32_t pre, uint32_t kick_bound,
+                          uint32_t rescue_kicks, uint32_t *kicks, uint64_t *carry_key, uint64_t *carry_val);
+/* Batch insert in input order; failed entries go to fail_keys/fail_vals. Returns #failures. */
+uint64_t orc_wide_insert(orc_wide *tb, const uint64_t *keys, const uint64_t *vals, uint64_t n, uint32_t kick_bound,
+                         uint32_t rescue_kicks, uint32_t *kicks_out, uint64_t *fail_keys, uint64_t *fail_vals,
+                         uint64_t fail_cap, uint64_t *rejected);
+void orc_wide_find(const orc_wide *tb, const uint64_t *keys, uint64_t n, uint64_t *vals_out, uint8_t *hit_out);
+/* Clears the keys of every matching slot of the first bucket holding the key (values stay). */
+uint64_t orc_wide_erase(orc_wide *tb, const uint64_t *keys, uint64_t n, uint8_t *deleted_out);
+void orc_wide_upsize(const uint64_t *old_keys, const uint64_t *old_vals, uint32_t old_nb, uint64_t *new_keys,
+                     uint64_t *new_vals, uint32_t t);
+uint64_t orc_wide_downsize_compact(const uint64_t *old_keys, const uint64_t *old_vals, uint32_t old_nb,
+                                   uint64_t *new_keys, uint64_t *new_vals, uint64_t *res_keys, uint64_t *res_vals,
+                                   uint64_t res_cap);
+uint64_t orc_wide_check_invariant(const orc_wide *tb);
+
 /* Multithreaded find over the same image (OpenMP), used only as the CPU timing baseline. */
 void orc_find_mt(const orc_table *tb, const uint32_t *keys, uint64_t n, uint32_t *vals_out,
                  uint8_t *hit_out, int threads);

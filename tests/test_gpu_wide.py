@@ -1,0 +1,195 @@
+"""GPU parity of the wide-key table (dyc_wide_*: uint64 keys and values) against the oracle:
+find bit-exact and erase byte-exact (key and value arrays) on oracle-built images, insert by
+multiset + placement invariant + find, upsize byte-exact, downsize by multiset + invariant, the
+stash, and the fill-bound policy over a grow / shrink stream."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def D():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import databases_gpu_amd as D
+
+    D.load()
+    return D
+
+
+def keys64(n, seed, lo=1, hi=2**64 - 2):
+    rng = np.random.default_rng(seed)
+    out = np.unique(rng.integers(lo, hi, size=int(n * 1.3) + 64, dtype=np.uint64))
+    rng.shuffle(out)
+    return out[:n]
+
+
+def absent(n, exclude, seed):
+    a = keys64(2 * n + 64, seed)
+    return a[~np.isin(a, exclude)][:n]
+
+
+def vals_of(k):
+    return (k * np.uint64(0x9E3779B97F4A7C15)) ^ np.uint64(0xD1B54A32D192ED03)
+
+
+def cu(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint64).view(np.int64)).cuda()
+
+
+def gfind(t, q):
+    v, h = t.find(cu(q))
+    torch.cuda.synchronize()
+    return v.cpu().numpy().view(np.uint64), h.cpu().numpy()
+
+
+def oracle_of(t):
+    ks, vs = t.dump()
+    ref = O.OracleWideTable(nb=[x.size // 16 for x in ks], rescue_kicks=128)
+    ref.load_images(ks, vs)
+    return ref
+
+
+def oracle_build(n, theta, d, seed):
+    keys = keys64(n, seed)
+    vals = vals_of(keys)
+    tb = O.OracleWideTable(int(np.ceil(n / theta)), subtables=d, rescue_kicks=128)
+    tb.insert(keys, vals)
+    assert tb.pend_keys.size == 0
+    return tb, keys, vals
+
+
+@pytest.mark.parametrize("d", (2, 4, 6))
+def test_wide_find_and_erase_on_oracle_image(D, d):
+    banner(f"wide find/erase on an oracle image, subtables={d}")
+    tb, keys, vals = oracle_build(100_000, 0.88, d, 700 + d)
+    t = D.DyCuckooWide(1000, subtables=d)
+    t.load(tb.keys, tb.vals)
+    q = np.concatenate([keys[:50_000], absent(20_000, keys, 701), np.array([0, 2**64 - 1], np.uint64)])
+    np.random.default_rng(d).shuffle(q)
+    q = q[:-11]
+    wv, wh = tb.find(q)
+    gv, gh = gfind(t, q)
+    assert np.array_equal(gh, wh) and np.array_equal(gv, wv)
+    e = np.concatenate([keys[::3], absent(4000, keys, 702)])
+    want = tb.erase(e)
+    got = t.erase(cu(e)).cpu().numpy()
+    assert np.array_equal(got, want)
+    ks, vs = t.dump()
+    assert all(np.array_equal(a, b) for a, b in zip(ks, tb.keys))
+    assert all(np.array_equal(a, b) for a, b in zip(vs, tb.vals))
+    assert t.stats()["count"] == tb.count
+    t.close()
+
+
+@pytest.mark.parametrize("d", (3, 4, 5))
+def test_wide_insert_multiset_invariant_and_find(D, d):
+    n = 1 << 20
+    banner(f"wide insert 2^20 at 0.90, subtables={d}")
+    keys = np.concatenate([keys64(n - 4096, 800 + d), keys64(4096, 810 + d, 1, 2**32 - 1)])  # some below 2^32
+    keys = np.unique(keys)
+    vals = vals_of(keys)
+    t = D.DyCuckooWide(int(np.ceil(keys.size / 0.9)), subtables=d)
+    t.insert(cu(keys), cu(vals))
+    st = t.stats()
+    assert st["count"] == keys.size and st["pending"] == 0 and st["lost"] == 0 and st["subtables"] == d
+    ref = oracle_of(t)
+    assert ref.check_invariant() == 0
+    k, v = ref.entries()
+    order = np.argsort(keys)
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+    q = np.concatenate([keys[: keys.size // 2], absent(keys.size // 4, keys, 820 + d)])
+    wv, wh = ref.find(q)
+    gv, gh = gfind(t, q)
+    assert np.array_equal(gh, wh) and np.array_equal(gv, wv)
+    assert wh.sum() == keys.size // 2
+    t.close()
+
+
+@pytest.mark.parametrize("d", (2, 4, 6))
+def test_wide_upsize_byte_exact_and_downsize_preserves(D, d):
+    banner(f"wide upsize x{d + 1}, downsize x2, subtables={d}")
+    tb, keys, vals = oracle_build(50_000, 0.85, d, 900 + d)
+    t = D.DyCuckooWide(1000, subtables=d)
+    t.load(tb.keys, tb.vals)
+    for _ in range(d + 1):
+        tb.resize_up()
+        t.resize_up()
+        assert t.buckets == tb.nb
+        ks, vs = t.dump()
+        assert all(np.array_equal(a, b) for a, b in zip(ks, tb.keys))
+        assert all(np.array_equal(a, b) for a, b in zip(vs, tb.vals))
+    order = np.argsort(keys)
+    for _ in range(2):
+        assert t.resize_down()
+        tb.resize_down()
+        assert t.buckets == tb.nb
+        st = t.stats()
+        assert st["pending"] == 0 and st["lost"] == 0
+        g = oracle_of(t)
+        assert g.check_invariant() == 0
+        k, v = g.entries()
+        assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+    t.close()
+
+
+def test_wide_stash_serves_keys_missing_from_both_buckets(D):
+    banner("wide stash: 40 copies of one key")
+    key = 0x0123456789ABCDEF
+    t = D.DyCuckooWide(100, max_fail_rounds=0, rescue_kicks=16)
+    vals = np.arange(1000, 1040, dtype=np.uint64)
+    with pytest.raises(D.DycError) as ei:
+        t.insert(cu(np.full(40, key, np.uint64)), cu(vals))
+    assert ei.value.code == 4  # DYC_ERR_PENDING
+    st = t.stats()
+    assert st["pending"] == 8 and st["count"] == 40
+    pk, pv = t.pending()
+    assert np.all(pk == key) and len(set(pv.tolist())) == 8 and set(pv.tolist()) <= set(vals.tolist())
+    one = cu(np.array([key], np.uint64))
+    for _ in range(2):
+        assert t.erase(one).cpu().tolist() == [1]
+    ks, _ = t.dump()
+    assert all(not x.any() for x in ks)
+    gv, gh = gfind(t, np.array([key, key + 1], np.uint64))
+    assert gh.tolist() == [1, 0] and int(gv[0]) in set(pv.tolist())
+    assert t.erase(one).cpu().tolist() == [1]
+    st = t.stats()
+    assert st["pending"] == 0 and st["count"] == 37
+    assert gfind(t, np.array([key], np.uint64))[1].tolist() == [0]
+    t.close()
+
+
+def test_wide_fill_bound_stream_matches_oracle_sizes(D):
+    """Grow then shrink under auto_resize with bounds [0.5, 0.9]: the subtable sizes after every
+    batch equal the oracle's (the policy depends only on counts), and every find is exact."""
+    banner("wide auto-resize stream")
+    kw = dict(auto_resize=True, lower=0.5, upper=0.9, min_capacity=0, rescue_kicks=128)
+    tb = O.OracleWideTable(20_000, **kw)
+    t = D.DyCuckooWide(20_000, auto_resize=1, lower_bound=0.5, upper_bound=0.9, min_capacity=0)
+    keys = keys64(120_000, 950)
+    vals = vals_of(keys)
+    live = np.zeros(0, np.uint64)
+    for b in range(6):
+        k, v = keys[b * 20_000:(b + 1) * 20_000], vals[b * 20_000:(b + 1) * 20_000]
+        tb.insert(k, v)
+        t.insert(cu(k), cu(v))
+        live = np.concatenate([live, k])
+        assert t.buckets == tb.nb, b
+    for b in range(5):
+        e = live[b * 20_000:(b + 1) * 20_000]
+        assert np.array_equal(t.erase(cu(e)).cpu().numpy(), tb.erase(e))
+        assert t.buckets == tb.nb, ("shrink", b)
+    rest = live[100_000:]
+    gv, gh = gfind(t, np.concatenate([rest, live[:1000]]))
+    assert gh[: rest.size].all() and not gh[rest.size:].any()
+    assert np.array_equal(gv[: rest.size], vals_of(rest))
+    assert t.stats()["count"] == tb.count == rest.size
+    t.close()
