@@ -94,6 +94,9 @@ def parse():
     ap.add_argument("--bt-theta", type=float, default=0.85, help="batched: per-table load factor")
     ap.add_argument("--bt-hit", type=float, default=0.6, help="batched: find hit rate")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--kick-bound", type=int, default=None,
+                    help="bulk: eviction bound of the table (with --no-tune it stays fixed; default: the library's 15, "
+                         "or what the autotuner installs)")
     ap.add_argument("--reference-kick", action="store_true",
                     help="bulk: keep the reference eviction bound (15 iterations of its victim rule); by default "
                          "the autotuner also tries bound 0 (the rescue walk from the first kick), which changes "
@@ -867,8 +870,9 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"calibration copy kernel: {4 * n} bytes read, {4 * n} bytes written")
 
+    extra = {} if args.kick_bound is None else {"kick_bound": args.kick_bound}
     t = D.DyCuckoo(hint, find_lanes=args.find_lanes, insert_lanes=args.insert_lanes,
-                   insert_blocks=args.insert_blocks, max_blocks=args.max_blocks, subtables=args.subtables)
+                   insert_blocks=args.insert_blocks, max_blocks=args.max_blocks, subtables=args.subtables, **extra)
     log(f"table: buckets {t.buckets}, capacity {t.capacity}, fill {n / t.capacity:.4f}")
     tuned = None
     if not args.no_tune:
@@ -958,8 +962,12 @@ def main() -> int:
     if os.path.exists(traffic_path):
         try:
             tr = json.load(open(traffic_path))
+            # a measurement applies to the configuration it was taken in (the grid does not change the
+            # bytes per key; the eviction bound does, through the exchanges per key)
+            same = {"insert": ("insert_lanes", "kick_bound"), "find_uniform": ("find_lanes",),
+                    "find_zipf": ("find_lanes",)}
             for k in kernels:
-                if k in tr and tr[k].get("log2n") == args.log2n:
+                if k in tr and tr[k].get("log2n") == args.log2n and all(tr[k].get(f) == cfg[f] for f in same[k]):
                     kernels[k]["traffic"] = tr[k]["bytes_per_launch"]
         except (OSError, ValueError, KeyError):
             pass
