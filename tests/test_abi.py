@@ -74,6 +74,17 @@ def test_errors_without_a_device_are_reported_not_crashed(lib):
     assert lib.dyc_insert(None, None, None, 0, None) == 1
     assert lib.dyc_find(None, None, None, None, 0, None) == 1
     assert lib.dyc_get_stats(None, None) == 1
+    # a subtable holds at most 2^29 - 1 buckets (bucket locations are packed in 29 bits): a hint
+    # past it is refused before any device call, for every subtable count and both key widths
+    for d in (2, 4, 6):
+        cfg = L.default_config(subtables=d)
+        too_big = ctypes.c_uint64(d * 16 * (1 << 29))
+        assert lib.dyc_create(ctypes.byref(cfg), too_big, ctypes.byref(h)) == 1
+        assert b"too large" in lib.dyc_last_error()
+        assert lib.dyc_wide_create(ctypes.byref(cfg), too_big, ctypes.byref(h)) == 1
+    assert lib.dyc_wide_insert(None, None, None, 0, None) == 1
+    assert lib.dyc_wide_find(None, None, None, None, 0, None) == 1
+    assert lib.dyc_wide_get_stats(None, None) == 1
 
 
 def test_struct_layout_matches_c(lib, tmp_path):
