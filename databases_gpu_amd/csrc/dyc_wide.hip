@@ -291,8 +291,9 @@ __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint
         }
         const uint64_t old_val =
             __hip_atomic_exchange(vp + slot, c.val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // the value exchange has returned, so it is performed before the key is published
-        __builtin_amdgcn_s_waitcnt(0);
+        // wait for the value exchange to return (so it is performed) before publishing the key;
+        // the memory clobber also keeps the compiler from moving the publish above the wait
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         (void)__hip_atomic_exchange(kp + slot, c.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ++c.kicks;
         c.fresh = true;
