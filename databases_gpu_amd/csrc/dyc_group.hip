@@ -40,11 +40,47 @@ struct dyc_group {
     dyc_stats st{};
 
     InsertPolicy policy() const { return InsertPolicy{launch.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
+    // The pending list is the group's stash: entries tagged with their table, searched by finds
+    // and erases of that table after both buckets. nullptr when nothing is pending.
+    const Pool *stash(Pool &p) const
+    {
+        p = Pool{pend, pend_tag, pend_cap, nullptr, pend_n};
+        return pend_n ? &p : nullptr;
+    }
 };
 
 namespace {
 
 constexpr uint32_t kDefaultItemKeys = 4096;
+
+// An erase leaves the stash entries it deleted as zero words: drop them with their tags
+// (synchronous; the list is short whenever it is not empty).
+int compact_stash(dyc_group *g, hipStream_t s)
+{
+    if (g->pend_n == 0) return DYC_OK;
+    std::vector<uint64_t> e(g->pend_n);
+    std::vector<uint32_t> tg(g->pend_n);
+    HIP_TRY(hipMemcpyAsync(e.data(), g->pend, g->pend_n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(tg.data(), g->pend_tag, g->pend_n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < e.size(); ++i)
+        if (e[i]) {
+            e[m] = e[i];
+            tg[m] = tg[i];
+            ++m;
+        }
+    if (m != g->pend_n) {
+        if (m) {
+            HIP_TRY(hipMemcpyAsync(g->pend, e.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+            HIP_TRY(hipMemcpyAsync(g->pend_tag, tg.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        }
+        HIP_TRY(hipStreamSynchronize(s));
+        g->pend_n = m;
+    }
+    g->st.pending = g->pend_n;
+    return DYC_OK;
+}
 
 int check_launch(const dyc_group_launch &l)
 {
@@ -316,8 +352,9 @@ int dyc_group_find(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, 
     uint32_t n_items = 0;
     rc = upload_items(g, offsets, s, &n_items);
     if (rc) return rc;
+    Pool sp;
     HIP_TRY(launch_group_find(g->ddir, g->ditems, n_items, keys, vals, hit, (int)g->launch.find_lanes,
-                              (int)g->launch.find_blocks, s));
+                              (int)g->launch.find_blocks, s, g->stash(sp)));
     return DYC_OK;
 }
 
@@ -334,12 +371,13 @@ int dyc_group_erase(dyc_group *g, const uint32_t *keys, const uint64_t *offsets,
     if (rc) return rc;
     rc = zero_and_run_prologue(g, s);
     if (rc) return rc;
+    Pool sp;
     HIP_TRY(launch_group_erase(g->ddir, g->ditems, n_items, keys, deleted, g->tcounts, (int)g->launch.find_lanes,
-                               (int)g->launch.find_blocks, s));
+                               (int)g->launch.find_blocks, s, g->stash(sp)));
     rc = read_back(g, s);
     if (rc) return rc;
     for (uint32_t i = 0; i < g->n; ++i) g->count[i] -= std::min<uint64_t>(g->count[i], g->htcounts[i]);
-    return DYC_OK;
+    return compact_stash(g, s);
 }
 
 int dyc_group_insert_per_table(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *offsets,
@@ -382,8 +420,9 @@ int dyc_group_find_per_table(dyc_group *g, const uint32_t *keys, const uint64_t 
     for (uint32_t i = 0; i < g->n; ++i) {
         const uint64_t m = offsets[i + 1] - offsets[i];
         if (!m) continue;
+        Pool sp;
         HIP_TRY(launch_find(g->hdir[i], keys + offsets[i], vals ? vals + offsets[i] : nullptr,
-                            hit ? hit + offsets[i] : nullptr, m, (int)g->launch.find_lanes, sh, s));
+                            hit ? hit + offsets[i] : nullptr, m, (int)g->launch.find_lanes, sh, s, g->stash(sp), i));
     }
     return DYC_OK;
 }

@@ -740,16 +740,21 @@ __global__ __launch_bounds__(256) void group_probe_kernel(const Desc *__restrict
                                                           const GroupItem *__restrict__ items, uint32_t n_items,
                                                           const uint32_t *__restrict__ keys,
                                                           uint32_t *__restrict__ vals, uint8_t *__restrict__ flags,
-                                                          unsigned long long *__restrict__ table_counts)
+                                                          unsigned long long *__restrict__ table_counts, Pool pool)
 {
     const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    const uint64_t np = pool_size(pool);
     for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
          it += waves) {
         const GroupItem item = items[it];
         const Desc D = dir[item.table];
         const uint64_t end = item.begin + item.len;
         unsigned long long nd = 0;
-        for (uint64_t base = item.begin; base < end; base += kWave) nd += probe_tile<G, ERASE>(D, keys, vals, flags, base, end);
+        for (uint64_t base = item.begin; base < end; base += kWave) {
+            uint32_t f = probe_tile<G, ERASE>(D, keys, vals, flags, base, end);
+            if (np) f |= pool_probe_at<ERASE ? 1 : 0>(pool, np, keys, vals, flags, end, item.table, base, f);
+            nd += f;
+        }
         if constexpr (ERASE) {
             nd = wave_sum(nd);
             if (__lane_id() == 0 && nd) atomicAdd(&table_counts[item.table], nd);
@@ -1157,33 +1162,35 @@ hipError_t launch_downsize(const uint64_t *old, uint32_t new_nb, uint64_t *dst, 
 constexpr int kGroupDefaultBlocks = 2048;  // 8192 waves: 256 CUs x 32 resident waves
 
 hipError_t launch_group_find(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
-                             uint32_t *vals, uint8_t *hit, int lanes, int blocks, hipStream_t st)
+                             uint32_t *vals, uint8_t *hit, int lanes, int blocks, hipStream_t st, const Pool *pool)
 {
+    const Pool p = pool ? *pool : Pool{};
     if (n_items == 0) return hipSuccess;
     const int grid = blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
     switch (lanes) {
-    case 4: group_probe_kernel<4, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr); break;
-    case 16: group_probe_kernel<16, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr); break;
-    default: group_probe_kernel<8, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr); break;
+    case 4: group_probe_kernel<4, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr, p); break;
+    case 16: group_probe_kernel<16, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr, p); break;
+    default: group_probe_kernel<8, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr, p); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_group_erase(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                               uint8_t *deleted, unsigned long long *table_counts, int lanes, int blocks,
-                              hipStream_t st)
+                              hipStream_t st, const Pool *pool)
 {
+    const Pool p = pool ? *pool : Pool{};
     if (n_items == 0) return hipSuccess;
     const int grid = blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
     switch (lanes) {
     case 4:
-        group_probe_kernel<4, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts);
+        group_probe_kernel<4, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts, p);
         break;
     case 16:
-        group_probe_kernel<16, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts);
+        group_probe_kernel<16, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts, p);
         break;
     default:
-        group_probe_kernel<8, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts);
+        group_probe_kernel<8, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts, p);
         break;
     }
     return hipGetLastError();

@@ -79,11 +79,13 @@ struct GroupItem {
 
 // Persistent launches over a directory of table descriptors: each wave takes items w, w + W, ...
 // find: vals/hit may be null. erase: per-table deleted counts are added to table_counts.
+// pool (optional): misses of an item of table i also look in the pool's entries tagged i.
 hipError_t launch_group_find(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
-                             uint32_t *vals, uint8_t *hit, int lanes, int blocks, hipStream_t st);
+                             uint32_t *vals, uint8_t *hit, int lanes, int blocks, hipStream_t st,
+                             const Pool *pool = nullptr);
 hipError_t launch_group_erase(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                               uint8_t *deleted, unsigned long long *table_counts, int lanes, int blocks,
-                              hipStream_t st);
+                              hipStream_t st, const Pool *pool = nullptr);
 // insert: failures go to sink.pend with their table index in sink.tags; per-table inserted
 // counts (placed or pended, reserved keys excluded) are added to table_counts.
 hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,

@@ -233,7 +233,9 @@ int dyc_group_set_launch(dyc_group *g, const dyc_group_launch *launch);
 
 /* Insert / find / erase over all tables of a batch. Insert and erase synchronise `stream`
  * once (per-table counts); find is asynchronous. Insert entries that exhaust the eviction
- * bound stay pending in the group (DYC_ERR_PENDING; dyc_group_pending lists them). */
+ * bound stay pending in the group, tagged with their table (DYC_ERR_PENDING; dyc_group_pending
+ * lists them); they are the group's stash, which finds and erases of that table search after
+ * both buckets. */
 int dyc_group_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *offsets,
                      dyc_stream stream);
 int dyc_group_find(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, uint32_t *vals, uint8_t *hit,

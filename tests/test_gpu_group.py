@@ -129,3 +129,37 @@ def test_group_launch_shapes_agree(D):
         if base is None:
             base = got
         g.close()
+
+
+def test_group_stash_serves_pending_entries_of_their_table(D):
+    """40 copies of one key into table 1 of a group (tables do not resize): 8 stay pending,
+    tagged with table 1. Once two erases have cleared the key's two buckets, a find of table 1
+    (both launch forms) still reaches a stash copy, table 0 does not see it, and one more erase
+    clears the stash."""
+    banner("group stash")
+    g = D.DyCuckooGroup([1000, 100, 1000])
+    key = 0x2468ACE1
+    sizes = np.array([0, 40, 0])
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    vals = np.arange(500, 540, dtype=np.uint32)
+    with pytest.raises(D.DycError) as ei:
+        g.insert(dev(np.full(40, key, np.uint32)), dev(vals), offs)
+    assert ei.value.code == 4  # DYC_ERR_PENDING
+    e, tb = g.pending()
+    assert e.size == 8 and np.all(tb == 1) and np.all((e >> np.uint64(32)) == key)
+    stash_vals = set((e & np.uint64(0xFFFFFFFF)).tolist())
+    one = np.array([0, 0, 1, 1], dtype=np.uint64)  # table 1 gets the single key
+    for _ in range(2):
+        assert g.erase(dev(np.array([key], np.uint32)), one).cpu().tolist() == [1]
+    q = np.array([key, key], np.uint32)
+    two = np.array([0, 1, 2, 2], dtype=np.uint64)  # one lookup in table 0, one in table 1
+    for per_table in (False, True):
+        v, h = g.find(dev(q), two, per_table=per_table)
+        v, h = v.cpu().numpy().view(np.uint32), h.cpu().numpy()
+        assert h.tolist() == [0, 1] and int(v[1]) in stash_vals
+    assert g.erase(dev(np.array([key], np.uint32)), one).cpu().tolist() == [1]
+    e, tb = g.pending()
+    assert e.size == 0 and g.table_stats(1)["pending"] == 0
+    v, h = g.find(dev(np.array([key], np.uint32)), one)
+    assert h.cpu().tolist() == [0]
+    g.close()
