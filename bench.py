@@ -197,8 +197,12 @@ def run_dynamic(args, dev) -> int:
     log(f"dynamic stream: {args.dyn_batches} batches of 2^{args.dyn_log2ops} ops from 2^{args.dyn_log2init} keys")
     ik, iv, batches = W.dynamic_stream(init, ops, half, args.dyn_batches - half, args.seed, dev)
     torch.cuda.synchronize()
+    # the random walk from the first collision (kick bound 0) is the measured faster policy; the
+    # reference's bound of 15 evictions is kept under --reference-kick
+    kick = args.kick_bound if args.kick_bound is not None else (15 if args.reference_kick else 0)
     t = D.DyCuckoo(int(math.ceil(init / args.theta)), auto_resize=1, lower_bound=0.5, upper_bound=args.theta,
-                   min_capacity=0, insert_lanes=args.insert_lanes, find_lanes=args.find_lanes)
+                   min_capacity=0, insert_lanes=args.insert_lanes, find_lanes=args.find_lanes,
+                   insert_blocks=args.insert_blocks, kick_bound=kick)
     t.insert(ik, iv)
     out_v = torch.empty(ops, dtype=torch.int32, device=dev)
     out_h = torch.empty(ops, dtype=torch.uint8, device=dev)
@@ -244,7 +248,8 @@ def run_dynamic(args, dev) -> int:
         "config": {"workload": f"dynamic resize stream: {len(batches)} batches of 2^{args.dyn_log2ops} ops "
                                f"(grow 40/40/20, shrink 20/40/40 insert/find/delete, finds 70% hit), "
                                f"fill bounds [0.50, {args.theta}]",
-                   "initial_keys": init, "peak_keys": peak, "final_keys": st["count"]},
+                   "initial_keys": init, "peak_keys": peak, "final_keys": st["count"], "kick_bound": kick,
+                   "insert_lanes": args.insert_lanes, "insert_blocks": args.insert_blocks},
         "p50_ms": round(pct(lat, 50), 3),
         "p99_ms": round(pct(lat, 99), 3),
         "resizes_up": st["resizes_up"] - st0["resizes_up"],
