@@ -142,9 +142,10 @@ int zero_and_run_prologue(dyc_group *g, hipStream_t s)
 
 int read_back(dyc_group *g, hipStream_t s)
 {
-    HIP_TRY(hipMemcpyAsync(g->hctr, g->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(g->hctr, g->ctr, kCounterBytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(g->htcounts, g->tcounts, g->n * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    fold_tallies(g->hctr);
     return DYC_OK;
 }
 
@@ -255,9 +256,9 @@ int dyc_group_create(const dyc_config *cfg_in, uint32_t n_tables, const uint64_t
             off += ((size_t)nbs[i] * kBucketBytes + 255) & ~(size_t)255;
         }
     }
-    if (hipMalloc(&g->ddir, n_tables * sizeof(Desc)) != hipSuccess || hipMalloc(&g->ctr, sizeof(Counters)) != hipSuccess ||
+    if (hipMalloc(&g->ddir, n_tables * sizeof(Desc)) != hipSuccess || hipMalloc(&g->ctr, kCounterBytes) != hipSuccess ||
         hipMalloc(&g->tcounts, n_tables * sizeof(unsigned long long)) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&g->hctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&g->hctr), kCounterBytes, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void **>(&g->htcounts), n_tables * sizeof(unsigned long long),
                       hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&g->items_free, hipEventDisableTiming) != hipSuccess) {

@@ -74,6 +74,86 @@ __device__ inline void read_slots(const uint64_t *p, uint64_t (&e)[SPL])
 
 
 // ---------------------------------------------------------------- find / erase / update
+
+// Erase and update write-back of one probe phase. Bit kk * SPL + j of mm says that this lane's slot
+// j of key kk's bucket slice holds key kk; each such slot is CAS'd from what was read to `rep` (0
+// for erase, the key with its new value for update). Returns bit kk set where one of this lane's
+// CASes on key kk succeeded. Up to kCasInFlight CASes are issued before any result is compared, so
+// a wave waits about once per phase instead of once per key of its groups; a lane with more matches
+// (a duplicated key, or several of its group's keys in its slice) takes the rest one at a time.
+constexpr int kCasInFlight = 3;
+
+template <int G, int SPL, bool UPDATE>
+__device__ inline void pick_match(const Desc &D, int b, const uint64_t (&e)[G][SPL], const uint32_t (&loc)[G],
+                                  const uint32_t (&key)[G], const uint32_t *nv, int s, uint64_t *&ptr,
+                                  uint64_t &exp, uint64_t &rep, uint32_t &kk_out)
+{
+    const int kk = b / SPL, j = b % SPL;
+    uint64_t x = 0;
+    uint32_t l = 0, k = 0, v = 0;
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        if (q == kk) {
+            l = loc[q];
+            k = key[q];
+            if constexpr (UPDATE) v = nv[q];
+        }
+#pragma unroll
+        for (int r = 0; r < SPL; ++r)
+            if (q * SPL + r == b) x = e[q][r];
+    }
+    ptr = loc_ptr(D, l) + s * SPL + j;
+    exp = x;
+    rep = UPDATE ? pack(k, v) : 0ull;
+    kk_out = (uint32_t)kk;
+}
+
+template <int G, int SPL, bool UPDATE>
+__device__ inline uint32_t write_matches(const Desc &D, uint32_t mm, const uint64_t (&e)[G][SPL],
+                                         const uint32_t (&loc)[G], const uint32_t (&key)[G], const uint32_t *nv,
+                                         int s)
+{
+    uint64_t found[kCasInFlight];
+    int bit[kCasInFlight];  // the match each in-flight CAS serves (-1: none)
+#pragma unroll
+    for (int q = 0; q < kCasInFlight; ++q) {
+        found[q] = 0;
+        bit[q] = -1;
+        if (mm) {
+            bit[q] = __builtin_ctz(mm);
+            mm &= mm - 1u;
+            uint64_t *ptr, x, rep;
+            uint32_t kk;
+            pick_match<G, SPL, UPDATE>(D, bit[q], e, loc, key, nv, s, ptr, x, rep, kk);
+            found[q] = cas_old(ptr, x, rep);
+        }
+    }
+    // The compiler would otherwise turn found == expected into the CAS's own success flag and
+    // test it (waiting for the CAS) before the next one issues; an empty asm hides the relation.
+#pragma unroll
+    for (int q = 0; q < kCasInFlight; ++q) asm volatile("" : "+v"(found[q]));
+    uint32_t ok = 0;
+#pragma unroll
+    for (int q = 0; q < kCasInFlight; ++q) {
+        if (bit[q] < 0) continue;
+        uint64_t x = 0;  // the expected value, selected again rather than held across the wait
+#pragma unroll
+        for (int kk = 0; kk < G; ++kk)
+#pragma unroll
+            for (int j = 0; j < SPL; ++j)
+                if (kk * SPL + j == bit[q]) x = e[kk][j];
+        if (found[q] == x) ok |= 1u << (bit[q] / SPL);
+    }
+    while (mm) {
+        const int b = __builtin_ctz(mm);
+        mm &= mm - 1u;
+        uint64_t *ptr, x, rep;
+        uint32_t kk;
+        pick_match<G, SPL, UPDATE>(D, b, e, loc, key, nv, s, ptr, x, rep, kk);
+        if (cas_old(ptr, x, rep) == x) ok |= 1u << kk;
+    }
+    return ok;
+}
 // One wave probes the 64 keys keys[base, base + 64) clipped to n, in table D. Returns this
 // lane's flag for its own key (hit, deleted or updated); all 64 lanes must call it together.
 // MODE 0 find: vals (optional) receives the value. MODE 1 erase: every copy of the key in the
@@ -121,6 +201,7 @@ __device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restric
 
         uint32_t my_val = 0, my_flag = 0;
         uint32_t done = 0;  // bit kk: key kk resolved in pos1
+        uint32_t mm = 0;    // erase/update: this lane's matching slots (write_matches)
 #pragma unroll
         for (int kk = 0; kk < G; ++kk) {
             int first = -1;
@@ -131,16 +212,9 @@ __device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restric
             if (gm) {
                 done |= 1u << kk;
                 if constexpr (ERASE) {
-                    int ok = 0;
 #pragma unroll
                     for (int j = 0; j < SPL; ++j)
-                        if (kk_valid[kk] && key_of(e[kk][j]) == kk_key[kk]) {
-                            uint64_t exp = e[kk][j];
-                            const uint64_t rep = UPDATE ? pack(kk_key[kk], kk_new[UPDATE ? kk : 0]) : 0ull;
-                            ok |= cas_slot(loc_ptr(D, kk_loc[kk]) + s * SPL + j, exp, rep);
-                        }
-                    const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & GMASK);
-                    if (kk == s) my_flag = okm != 0;
+                        if (kk_valid[kk] && key_of(e[kk][j]) == kk_key[kk]) mm |= 1u << (kk * SPL + j);
                 } else {
                     const int w = __builtin_ctz(gm);
                     uint32_t v = 0;
@@ -156,34 +230,42 @@ __device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restric
             }
         }
 
+        if constexpr (ERASE) {
+            const uint32_t ok = write_matches<G, SPL, UPDATE>(D, mm, e, kk_loc, kk_key, kk_new, s);
+#pragma unroll
+            for (int kk = 0; kk < G; ++kk) {
+                if (!(done & (1u << kk))) continue;  // group-uniform
+                const uint32_t okm = (uint32_t)((__ballot((ok >> kk) & 1u) >> (g * G)) & GMASK);
+                if (kk == s) my_flag = okm != 0;
+            }
+            mm = 0;
+        }
+
         // phase 2: pos2 buckets of the keys pos1 did not resolve
+        [[maybe_unused]] uint32_t done2 = 0;  // erase/update: bit kk: key kk found in pos2
 #pragma unroll
         for (int kk = 0; kk < G; ++kk) {
             const uint32_t l = __shfl(loc2, g * G + kk);
             if constexpr (ERASE) kk_loc[kk] = l;
             if (kk_valid[kk] && !(done & (1u << kk))) read_slots<SPL, COH>(loc_ptr(D, l) + s * SPL, e[kk]);
         }
+        // Straight-line over every key (a key pos1 resolved is masked out, not skipped), so the
+        // wait for the pos2 slots falls on every path before write_matches issues its CASes.
 #pragma unroll
         for (int kk = 0; kk < G; ++kk) {
-            if (done & (1u << kk)) continue;  // group-uniform
+            const bool todo = kk_valid[kk] && !(done & (1u << kk));
             int first = -1;
 #pragma unroll
             for (int j = SPL - 1; j >= 0; --j)
                 if (key_of(e[kk][j]) == kk_key[kk]) first = j;
-            const uint32_t gm = (uint32_t)((__ballot(first >= 0 && kk_valid[kk]) >> (g * G)) & GMASK);
-            if (!gm) continue;
+            const uint32_t gm = (uint32_t)((__ballot(first >= 0 && todo) >> (g * G)) & GMASK);
             if constexpr (ERASE) {
-                int ok = 0;
+                done2 |= (gm != 0 ? 1u : 0u) << kk;
 #pragma unroll
                 for (int j = 0; j < SPL; ++j)
-                    if (kk_valid[kk] && key_of(e[kk][j]) == kk_key[kk]) {
-                        uint64_t exp = e[kk][j];
-                        const uint64_t rep = UPDATE ? pack(kk_key[kk], kk_new[UPDATE ? kk : 0]) : 0ull;
-                        ok |= cas_slot(loc_ptr(D, kk_loc[kk]) + s * SPL + j, exp, rep);
-                    }
-                const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & GMASK);
-                if (kk == s) my_flag = okm != 0;
+                    mm |= (todo && key_of(e[kk][j]) == kk_key[kk] ? 1u : 0u) << (kk * SPL + j);
             } else {
+                if (!gm) continue;
                 const int w = __builtin_ctz(gm);
                 uint32_t v = 0;
 #pragma unroll
@@ -194,6 +276,16 @@ __device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restric
                     my_val = v;
                     my_flag = 1;
                 }
+            }
+        }
+
+        if constexpr (ERASE) {
+            const uint32_t ok = write_matches<G, SPL, UPDATE>(D, mm, e, kk_loc, kk_key, kk_new, s);
+#pragma unroll
+            for (int kk = 0; kk < G; ++kk) {
+                if (!(done2 & (1u << kk))) continue;  // group-uniform
+                const uint32_t okm = (uint32_t)((__ballot((ok >> kk) & 1u) >> (g * G)) & GMASK);
+                if (kk == s) my_flag = okm != 0;
             }
         }
 
@@ -225,11 +317,12 @@ __device__ inline uint64_t pool_size(const Pool &pool)
 }
 
 // Waves per SIMD the find / erase kernels are held to (the occupancy their register use gives
-// with a four-subtable selection; the six-way one would otherwise cost a wave at G = 8).
+// with a four-subtable selection; the six-way one would otherwise cost a wave at G = 8). Erase
+// keeps up to three CASes per lane in flight (write_matches), which costs G = 4 its sixth wave.
 template <int G, bool ERASE>
 constexpr int probe_waves()
 {
-    return G == 4 ? (ERASE ? 6 : 7) : (G == 8 ? (ERASE ? 5 : 7) : (ERASE ? 4 : 5));
+    return G == 4 ? (ERASE ? 5 : 7) : (G == 8 ? (ERASE ? 5 : 7) : (ERASE ? 4 : 5));
 }
 
 // Misses also look in the pool's entries of `table` (a serving table's overflow pool, a single
@@ -242,13 +335,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(probe_waves
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
     const uint64_t np = pool_size(pool);
     unsigned long long ndel = 0;
-    for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
-         base += waves * kWave) {
+    const uint32_t slot = wave_slot();
+    for (uint64_t base = (uint64_t)slot * kWave; base < n; base += waves * kWave) {
         uint32_t f = probe_tile<G, ERASE>(D, keys, vals, flags, base, n);
         if (np) f |= pool_probe_at<ERASE ? 1 : 0>(pool, np, keys, vals, flags, n, table, base, f);
         ndel += f;
     }
-    if constexpr (ERASE) wave_add(&C->deleted, ndel);
+    if constexpr (ERASE) wave_tally(C, &C->deleted, ndel, slot);
 }
 
 // Upsert's first pass: rewrite the value of keys already stored (probe_tile_m mode 2).
@@ -260,13 +353,13 @@ __global__ __launch_bounds__(256) void update_kernel(Desc D, const uint32_t *__r
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
     const uint64_t np = pool_size(pool);
     unsigned long long nu = 0;
-    for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
-         base += waves * kWave) {
+    const uint32_t slot = wave_slot();
+    for (uint64_t base = (uint64_t)slot * kWave; base < n; base += waves * kWave) {
         uint32_t f = probe_tile_m<G, 2>(D, keys, vals, updated, base, n);
         if (np) f |= pool_probe_at<2>(pool, np, keys, vals, updated, n, 0, base, f);
         nu += f;
     }
-    wave_add(&C->updated, nu);
+    wave_tally(C, &C->updated, nu, slot);
 }
 
 // ------------------------------------------------------------------------------- insert
@@ -299,6 +392,7 @@ struct Carry {
 
 struct InsertTally {
     unsigned long long kicks = 0, rejected = 0, retries = 0;
+    unsigned long long empty = 0;  // zero entries skipped by a from-entries insert
 };
 
 // Take entry e (previous table pr, kNoTable for a fresh key) into the group's hand.
@@ -436,9 +530,14 @@ __device__ inline void refill(const Desc &D, Carry<kSlots / G> &c, uint64_t &nex
         const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << (g * G)) - 1ull));
         if (!c.active && rank < take) {
             const uint64_t idx = next + rank;
-            if constexpr (FROM_ENTRIES)
-                carry_start(D, c, entries[idx], pre_fixed, s, tally);
-            else if (!skip || !skip[idx])
+            if constexpr (FROM_ENTRIES) {
+                // skip, for an entries insert, is the per-tile liveness of a slot-addressed list
+                const uint64_t e = (!skip || skip[idx / kResTileEntries]) ? entries[idx] : 0ull;
+                if (e)
+                    carry_start(D, c, e, pre_fixed, s, tally);
+                else if (s == 0)
+                    ++tally.empty;
+            } else if (!skip || !skip[idx])
                 carry_start(D, c, pack(keys[idx], vals[idx]), kNoTable, s, tally);
         }
         next += take;
@@ -446,6 +545,8 @@ __device__ inline void refill(const Desc &D, Carry<kSlots / G> &c, uint64_t &nex
     }
 }
 
+// skip (optional): a keys insert leaves out key i where skip[i] != 0; an entries insert reads it
+// as the tile flags of a slot-addressed residual list (entries of a tile flagged 0 are empty).
 template <int G, bool FROM_ENTRIES>
 __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ vals,
@@ -480,12 +581,12 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
         if (c.active) kick_step<G>(D, c, P, C, sink, sink.tag, tally);
     }
     if (sink.placed) {
-        const unsigned long long rej = wave_sum(tally.rejected);
+        const unsigned long long rej = wave_sum(tally.rejected) + wave_sum(tally.empty);
         if (__lane_id() == 0 && taken > rej) atomicAdd(sink.placed, taken - rej);
     }
-    wave_add(&C->kicks, tally.kicks);
-    wave_add(&C->rejected, tally.rejected);
-    wave_add(&C->retries, tally.retries);
+    wave_tally(C, &C->kicks, tally.kicks);
+    wave_tally(C, &C->rejected, tally.rejected);
+    wave_tally(C, &C->retries, tally.retries);
 }
 
 // -------------------------------------------------------------------------- partition
@@ -646,25 +747,33 @@ __global__ __launch_bounds__(256) void upsize_kernel(const uint64_t *__restrict_
         *reinterpret_cast<uint4 *>(dst + (j + old_nb) * kSlots + s * 2) =
             make_uint4((uint32_t)hi[0], (uint32_t)(hi[0] >> 32), (uint32_t)hi[1], (uint32_t)(hi[1] >> 32));
     }
-    wave_add(&C->misplaced, bad);
+    wave_tally(C, &C->misplaced, bad);
 }
 
 // ----------------------------------------------------------------------------- downsize
 // cuckoo.cu:664-699: new bucket g keeps old bucket g slot-for-slot and fills its empty slots
 // from old bucket g+n read back to front (slot i <- slot 15-i). Lane s of a group holds slots
 // 2s, 2s+1; their cross partners 15-2s, 14-2s are the two halves of lane 7-s's 16 bytes.
-// Entries of bucket g+n that found no hole are appended to the residual list; the caller
-// re-inserts them only after this kernel has finished (no race with the plain stores here).
+// An entry of bucket g+n that found no hole is a residual. res is slot-addressed: the residual
+// from slot i of new bucket g's partner goes to res[g * 16 + i], 0 where there is none, so no
+// wave reserves list space (an append index is one address, and atomics to one address are
+// serialized: one per wave cost 0.9 ms on a 582,560-bucket downsize). A wave compacts a tile of
+// kResTileBuckets buckets; live[tile] says whether it left any residual, and only such tiles
+// write their kResTileEntries list entries, so a nearly full table writes almost no list. The
+// caller re-inserts the list (launch_insert_entries with live) only after this kernel has
+// finished (no race with the plain stores here). Counters::residuals receives their number.
 __global__ __launch_bounds__(256) void downsize_kernel(const uint64_t *__restrict__ old, uint32_t new_nb,
                                                        uint64_t *__restrict__ dst, Counters *__restrict__ C,
-                                                       uint64_t *__restrict__ res, uint64_t res_cap)
+                                                       uint64_t *__restrict__ res, uint8_t *__restrict__ live)
 {
+    static_assert(kWave / 8 == (int)kResTileBuckets, "a wave compacts one liveness tile");
     constexpr int G = 8, NG = kWave / G;
     const int lane = __lane_id();
     const int g = lane / G, s = lane % G;
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
-    for (uint64_t tile = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * NG; tile < new_nb;
-         tile += waves * NG) {
+    const uint32_t slot = wave_slot();
+    unsigned long long nres = 0;
+    for (uint64_t tile = (uint64_t)slot * NG; tile < new_nb; tile += waves * NG) {
         const uint64_t j = tile + g;
         const bool ok = j < new_nb;
         uint64_t e[2] = {0, 0}, c[2] = {0, 0};
@@ -680,32 +789,19 @@ __global__ __launch_bounds__(256) void downsize_kernel(const uint64_t *__restric
         for (int q = 0; q < 2; ++q) {
             placed[q] = e[q] ? e[q] : c[q];
             r[q] = (placed[q] != c[q] && c[q] != 0) ? c[q] : 0;
+            nres += r[q] != 0;
         }
         if (ok)
             *reinterpret_cast<uint4 *>(dst + j * kSlots + s * 2) =
                 make_uint4((uint32_t)placed[0], (uint32_t)(placed[0] >> 32), (uint32_t)placed[1],
                            (uint32_t)(placed[1] >> 32));
-        const uint64_t b0 = __ballot(r[0] != 0), b1 = __ballot(r[1] != 0);
-        const uint32_t tot = __popcll(b0) + __popcll(b1);
-        if (tot == 0) continue;
-        unsigned long long at = 0;
-        if (lane == 0) at = atomicAdd(&C->pending, (unsigned long long)tot);
-        at = shfl64(at, 0);
-        const uint64_t o0 = at + __popcll(b0 & lanemask_lt());
-        const uint64_t o1 = at + __popcll(b0) + __popcll(b1 & lanemask_lt());
-        if (r[0]) {
-            if (o0 < res_cap)
-                res[o0] = r[0];
-            else
-                atomicAdd(&C->lost, 1ull);
-        }
-        if (r[1]) {
-            if (o1 < res_cap)
-                res[o1] = r[1];
-            else
-                atomicAdd(&C->lost, 1ull);
-        }
+        const bool any = __ballot(r[0] != 0 || r[1] != 0) != 0;  // wave-uniform
+        if (lane == 0) live[tile / NG] = any ? 1 : 0;
+        if (any && ok)
+            *reinterpret_cast<uint4 *>(res + j * kSlots + s * 2) =
+                make_uint4((uint32_t)r[0], (uint32_t)(r[0] >> 32), (uint32_t)r[1], (uint32_t)(r[1] >> 32));
     }
+    wave_tally(C, &C->residuals, nres, slot);
 }
 
 // Buckets the reference find order reads per key: 1 if pos1 holds the key, else 2 (0 for
@@ -724,8 +820,8 @@ __global__ __launch_bounds__(256) void probe_count_kernel(Desc D, const uint32_t
         probes += hit ? 1 : 2;
         pos1_hits += hit;
     }
-    wave_add(&C->kicks, probes);
-    wave_add(&C->occupied, pos1_hits);
+    wave_tally(C, &C->kicks, probes);
+    wave_tally(C, &C->occupied, pos1_hits);
 }
 
 // ---------------------------------------------------------------------------- many tables
@@ -792,9 +888,9 @@ __global__ __launch_bounds__(256) void group_insert_kernel(const Desc *__restric
         const unsigned long long rej = wave_sum(tally.rejected - rej0);
         if (__lane_id() == 0 && item.len > rej) atomicAdd(&table_counts[item.table], item.len - rej);
     }
-    wave_add(&C->kicks, tally.kicks);
-    wave_add(&C->rejected, tally.rejected);
-    wave_add(&C->retries, tally.retries);
+    wave_tally(C, &C->kicks, tally.kicks);
+    wave_tally(C, &C->rejected, tally.rejected);
+    wave_tally(C, &C->retries, tally.retries);
 }
 
 // ------------------------------------------------------------------------------ serving
@@ -909,9 +1005,9 @@ __global__ __launch_bounds__(256) void serve_insert_kernel(const Desc *__restric
         const ServeItem item = items[it];
         serve_insert_item<G>(dir[item.table], item, c, P, C, sink, tally, ins_total);
     }
-    wave_add(&C->kicks, tally.kicks);
-    wave_add(&C->rejected, tally.rejected);
-    wave_add(&C->retries, tally.retries);
+    wave_tally(C, &C->kicks, tally.kicks);
+    wave_tally(C, &C->rejected, tally.rejected);
+    wave_tally(C, &C->retries, tally.retries);
 }
 
 // A whole serving tick in one launch. Waves claim items in list order from a counter; an item
@@ -989,10 +1085,10 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
             }
         }
     }
-    wave_add(&C->kicks, tally.kicks);
-    wave_add(&C->rejected, tally.rejected);
-    wave_add(&C->retries, tally.retries);
-    if (__lane_id() == 0 && stalls) atomicAdd(&C->stalls, stalls);
+    wave_tally(C, &C->kicks, tally.kicks);
+    wave_tally(C, &C->rejected, tally.rejected);
+    wave_tally(C, &C->retries, tally.retries);
+    if (__lane_id() == 0 && stalls) tally_add(C, &C->stalls, stalls, wave_slot());
 }
 
 // Copy between host-pinned and device memory with a kernel instead of a DMA transfer: a
@@ -1023,7 +1119,7 @@ __global__ __launch_bounds__(256) void count_kernel(const uint64_t *__restrict__
     unsigned long long c = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += stride) c += sub[i] != 0;
-    wave_add(&C->occupied, c);
+    wave_tally(C, &C->occupied, c);
 }
 
 // Blocks for `work_waves` waves of work. A launch may not exceed 2^32 work-items in total, and
@@ -1141,9 +1237,9 @@ hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *va
 
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
                                  InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
-                                 const unsigned long long *n_dev)
+                                 const unsigned long long *n_dev, const uint8_t *live)
 {
-    return insert_common<true>(D, nullptr, nullptr, entries, n, n_dev, pre, lanes, pol, sh, C, sink, st);
+    return insert_common<true>(D, nullptr, nullptr, entries, n, n_dev, pre, lanes, pol, sh, C, sink, st, live);
 }
 
 hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,
@@ -1170,11 +1266,12 @@ hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, ui
 }
 
 hipError_t launch_downsize(const uint64_t *old, uint32_t new_nb, uint64_t *dst, Counters *C, uint64_t *res,
-                           uint64_t res_cap, hipStream_t st)
+                           uint64_t res_cap, uint8_t *live, hipStream_t st)
 {
     if (new_nb == 0) return hipSuccess;
-    const int grid = grid_for((new_nb + 7) / 8, LaunchShape{256, 0});
-    downsize_kernel<<<grid, 256, 0, st>>>(old, new_nb, dst, C, res, res_cap);
+    if (res_cap < (uint64_t)new_nb * kSlots || !live) return hipErrorInvalidValue;
+    const int grid = grid_for((new_nb + kResTileBuckets - 1) / kResTileBuckets, LaunchShape{256, 0});
+    downsize_kernel<<<grid, 256, 0, st>>>(old, new_nb, dst, C, res, live);
     return hipGetLastError();
 }
 

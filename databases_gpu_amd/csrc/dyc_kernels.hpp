@@ -5,7 +5,12 @@
 
 namespace dyc {
 
-// Per-call device counters, zeroed before each launch that uses them.
+// Per-call device counters, zeroed before each launch that uses them. This is the base copy; the
+// device block (kCounterBytes) holds it, padded to 256 B, followed by kTallyStripes stripes of the
+// same 16 words, each 256 B apart. Atomics to one address are serialized at the memory-side atomic
+// unit (about 12 ns each on MI355X), so per-wave tallies (kicks, deleted, ...) go to the stripe of
+// the wave's slot; append indexes (pending, lost) stay in the base copy, where the device reads
+// them back. The host copies the whole block and folds the stripes into the base (fold_tallies).
 struct Counters {
     unsigned long long kicks;      // successful exchanges (insert)
     unsigned long long rejected;   // reserved keys in an insert batch
@@ -17,9 +22,24 @@ struct Counters {
     unsigned long long occupied;   // count kernel
     unsigned long long stalls;     // serving tick: dependency waits that hit their bound
     unsigned long long updated;    // upsert: keys whose stored value was replaced
-    unsigned long long pad[6];
+    unsigned long long residuals;  // downsize: entries of the upper half that did not stay
+    unsigned long long pad[5];
 };
-constexpr int kCounterWords = 16;
+constexpr int kTallyStripes = 64;
+constexpr int kTallyStrideWords = 32;  // 256 B between stripes
+constexpr int kCounterWords = kTallyStrideWords * (1 + kTallyStripes);
+constexpr size_t kCounterBytes = (size_t)kCounterWords * sizeof(unsigned long long);
+static_assert(sizeof(Counters) <= kTallyStrideWords * sizeof(unsigned long long), "Counters exceeds a stripe");
+
+// Add every stripe of a counter block (kCounterWords words, as copied from the device) into its
+// base copy, in place. Call once per copy.
+inline void fold_tallies(void *block)
+{
+    constexpr int kWords = (int)(sizeof(Counters) / sizeof(unsigned long long));
+    unsigned long long *w = static_cast<unsigned long long *>(block);
+    for (int s = 1; s <= kTallyStripes; ++s)
+        for (int i = 0; i < kWords; ++i) w[i] += w[kTallyStrideWords * s + i];
+}
 constexpr uint32_t kMaxRegions = 1024;
 
 struct LaunchShape {
@@ -62,9 +82,12 @@ hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *va
 hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint8_t *updated, uint64_t n,
                          int lanes, LaunchShape sh, Counters *C, hipStream_t st, const Pool *pool = nullptr);
 // n_dev (optional): a device count read at kernel start; entries [0, min(n, *n_dev)) are inserted.
+// A zero entry is an empty place (an erased pending entry, a slot without a residual): skipped,
+// and counted neither as rejected nor as placed. live (optional, a slot-addressed residual list's
+// tile flags from launch_downsize): entries of a tile whose byte is 0 are empty and not read.
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
                                  InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
-                                 const unsigned long long *n_dev = nullptr);
+                                 const unsigned long long *n_dev = nullptr, const uint8_t *live = nullptr);
 // Order an insert batch by the region of its first bucket: out[0, n) receives the packed
 // entries, region by region. counts and cursors are R words each (counts zeroed by the caller).
 hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,
@@ -159,10 +182,16 @@ hipError_t launch_serve_erase(const Desc *dir, const ServeItem *items, uint32_t 
 // upsize subtable t: old has old_nb buckets, dst has 2*old_nb and is written completely.
 hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, uint32_t t, Counters *C,
                          hipStream_t st);
-// downsize compaction: old has 2*new_nb buckets, dst new_nb (written completely); residuals
-// are appended to res through C->pending.
+// downsize compaction: old has 2*new_nb buckets, dst new_nb (written completely). Residuals are
+// slot-addressed in res (res_cap >= new_nb * 16 entries): live (res_tiles(new_nb) bytes, written
+// completely) marks the tiles of kResTileEntries entries that hold any, and only those tiles of
+// res are written (0 where a slot has none). Their number is added to C->residuals. Re-insert
+// them with launch_insert_entries over new_nb * 16 entries with live.
+constexpr uint32_t kResTileBuckets = 8;
+constexpr uint64_t kResTileEntries = kResTileBuckets * kSlots;
+inline uint64_t res_tiles(uint32_t new_nb) { return (new_nb + kResTileBuckets - 1) / kResTileBuckets; }
 hipError_t launch_downsize(const uint64_t *old, uint32_t new_nb, uint64_t *dst, Counters *C, uint64_t *res,
-                           uint64_t res_cap, hipStream_t st);
+                           uint64_t res_cap, uint8_t *live, hipStream_t st);
 hipError_t launch_zero(unsigned long long *p, uint64_t words, hipStream_t st);
 // Kernel copy of up to two segments of 8-byte words (either side may be host-pinned memory).
 hipError_t launch_copy(const void *src0, void *dst0, uint64_t words0, const void *src1, void *dst1, uint64_t words1,

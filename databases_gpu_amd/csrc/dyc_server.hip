@@ -202,8 +202,9 @@ int upsize_now(dyc_server *s, uint32_t i, bool *ok)
 }
 
 // Halve one subtable of table i (choose_down) and re-insert the compaction's residuals into the
-// other subtable of their pair (the device holds their count). *done = false if refused (odd
-// bucket count) or the arena has no room for the new subtable and the residual list.
+// other subtable of their pair (the slot-addressed residual list is read whole, so the host
+// needs no count). *done = false if refused (odd bucket count) or the arena has no room for the
+// new subtable and the residual list.
 int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
 {
     SrvTable &T = s->t[i];
@@ -215,14 +216,15 @@ int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
     const uint64_t res_cap = (uint64_t)new_nb * kSlots;  // one pair of old buckets leaves at most 16
     const uint64_t off = s->arena.alloc((uint64_t)new_nb * kBucketBytes);
     if (off == Arena::kNone) return DYC_OK;
-    const uint64_t roff = s->arena.alloc(res_cap * sizeof(uint64_t));
+    const uint64_t roff = s->arena.alloc(res_cap * sizeof(uint64_t) + res_tiles(new_nb));  // list, tile flags
     if (roff == Arena::kNone) {
         s->arena.release(off);
         return DYC_OK;
     }
     uint64_t *res = s->sub_ptr(roff);
+    uint8_t *live = reinterpret_cast<uint8_t *>(res + res_cap);
     HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(rctr), kCounterWords, s->st));
-    HIP_TRY(launch_downsize(T.D.sub[sub], new_nb, s->sub_ptr(off), rctr, res, res_cap, s->st));
+    HIP_TRY(launch_downsize(T.D.sub[sub], new_nb, s->sub_ptr(off), rctr, res, res_cap, live, s->st));
     s->arena.release(T.off[sub]);
     T.off[sub] = off;
     desc_set(T.D, (int)sub, s->sub_ptr(off), new_nb);
@@ -230,7 +232,7 @@ int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
     // residuals continue from the shrunk subtable into the other one of their pair
     // (cuckoo.cu:719), after the compaction; failures go to the pool
     HIP_TRY(launch_insert_entries(T.D, res, res_cap, sub, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
-                                  s->pc(), s->sink(i), s->st, &rctr->pending));
+                                  s->pc(), s->sink(i), s->st, nullptr, live));
     s->arena.release(roff);
     T.resizes_down++;
     s->ss.resizes_down++;
@@ -249,6 +251,7 @@ int read_stats_sync(dyc_server *s, unsigned long long *dst)
 {
     HIP_TRY(launch_copy(s->d_stat, dst, s->stat_bytes() / 8, nullptr, nullptr, 0, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
+    fold_tallies(dst);
     return DYC_OK;
 }
 
@@ -337,6 +340,7 @@ int retire(dyc_server *s, Tick &k)
         k.tables.clear();
         return DYC_OK;
     }
+    fold_tallies(k.h_snap);  // the tick's copy has landed (its event fired); folded once, here
     const Counters &c = *reinterpret_cast<const Counters *>(k.h_snap);
     if (k.solo_erase && c.deleted > s->deleted_seen) {
         s->t[k.solo_table].sync_del += c.deleted - s->deleted_seen;
@@ -920,7 +924,7 @@ int create(dyc_server *s, uint32_t n_tables, const uint64_t *hints)
         if (rc) return rc;
         HIP_TRY(hipHostMalloc((void **)&k.h_dir, std::max<size_t>(1, n_tables) * sizeof(Desc), hipHostMallocDefault));
         HIP_TRY(hipHostMalloc((void **)&k.h_snap, s->stat_bytes(), hipHostMallocDefault));
-        rc = dev_alloc((void **)&k.d_rctr, sizeof(Counters));
+        rc = dev_alloc((void **)&k.d_rctr, kCounterBytes);
         if (rc) return rc;
         // cursor, items, then per stage (at most one per item): left, next, padded ready flag
         const size_t block_bytes =

@@ -146,8 +146,9 @@ int zero_counters(dyc_table *t, hipStream_t s)
 
 int read_counters(dyc_table *t, hipStream_t s)
 {
-    HIP_TRY(hipMemcpyAsync(t->hctr, t->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(t->hctr, t->ctr, kCounterBytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    fold_tallies(t->hctr);
     return DYC_OK;
 }
 
@@ -311,15 +312,17 @@ int do_resize_down(dyc_table *t, hipStream_t s)
     const uint32_t new_nb = old_nb / 2;
     uint64_t *dst = nullptr, *res = nullptr;
     const uint64_t res_cap = (uint64_t)new_nb * kSlots;
+    // the slot-addressed residual list, then its tile flags
     if (hipMalloc(&dst, (size_t)new_nb * kBucketBytes) != hipSuccess ||
-        hipMalloc(&res, res_cap * sizeof(uint64_t)) != hipSuccess) {
+        hipMalloc(&res, res_cap * sizeof(uint64_t) + res_tiles(new_nb)) != hipSuccess) {
         (void)hipGetLastError();
         if (dst) (void)hipFree(dst);
         return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
     }
+    uint8_t *live = reinterpret_cast<uint8_t *>(res + res_cap);
     int rc = zero_counters(t, s);
     if (!rc) {
-        hipError_t e = launch_downsize(t->D.sub[sub], new_nb, dst, t->ctr, res, res_cap, s);
+        hipError_t e = launch_downsize(t->D.sub[sub], new_nb, dst, t->ctr, res, res_cap, live, s);
         if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "downsize launch: %s", hipGetErrorString(e));
     }
     if (!rc) rc = read_counters(t, s);
@@ -328,12 +331,7 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         (void)hipFree(res);
         return rc;
     }
-    const uint64_t nres = t->hctr->pending;
-    if (t->hctr->lost) {
-        (void)hipFree(dst);
-        (void)hipFree(res);
-        return fail(DYC_ERR_LOST, "downsize residual list overflow");
-    }
+    const uint64_t nres = t->hctr->residuals;
     rc = release_sub(t, sub);
     if (rc) return rc;
     desc_set(t->D, (int)sub, dst, new_nb);
@@ -344,8 +342,10 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         rc = reserve_pending(t, std::max<uint64_t>(nres, 1024), s);
         if (!rc) rc = zero_counters(t, s);
         if (!rc) {
-            hipError_t e = launch_insert_entries(t->D, res, nres, sub, (int)t->cfg.insert_lanes, t->policy(),
-                                                 t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s);
+            // the slot-addressed list: its live tiles, zeros skipped
+            hipError_t e = launch_insert_entries(t->D, res, res_cap, sub, (int)t->cfg.insert_lanes, t->policy(),
+                                                 t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s, nullptr,
+                                                 live);
             if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "residual insert launch: %s", hipGetErrorString(e));
         }
         if (!rc) rc = read_counters(t, s);
@@ -660,8 +660,8 @@ int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out
     desc_init(t->D, cfg.subtables);
     HIP_TRY(hipGetDevice(&t->device));
     int rc = DYC_OK;
-    if (hipMalloc(&t->ctr, sizeof(Counters)) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+    if (hipMalloc(&t->ctr, kCounterBytes) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), kCounterBytes, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         rc = fail(DYC_ERR_OOM, "counter allocation failed");
     }

@@ -23,6 +23,15 @@ __device__ inline bool cas_slot(uint64_t *p, uint64_t &expected, uint64_t desire
                                                 __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// 64-bit compare-and-swap at agent scope that returns the value it found (== expected on
+// success). Nothing waits for the result until the caller compares it, so several can be in flight.
+__device__ inline uint64_t cas_old(uint64_t *p, uint64_t expected, uint64_t desired)
+{
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expected;
+}
+
 __device__ inline unsigned long long wave_sum(unsigned long long v)
 {
 #pragma unroll
@@ -30,12 +39,30 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-// Sum a per-lane value over the wave and add it to *dst once (all 64 lanes must be active).
-__device__ inline void wave_add(unsigned long long *dst, unsigned long long v)
+// This wave's index in the grid, as a wave-uniform (scalar) value.
+__device__ inline uint32_t wave_slot()
+{
+    return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+}
+
+// Add v (from one lane) to the tally `field` (a member of *C) in the stripe of wave slot `slot`.
+__device__ inline void tally_add(Counters *C, unsigned long long *field, unsigned long long v, uint32_t slot)
+{
+    unsigned long long *base = reinterpret_cast<unsigned long long *>(C);
+    atomicAdd(base + kTallyStrideWords * (1u + slot % kTallyStripes) + (field - base), v);
+}
+
+__device__ inline void wave_tally(Counters *C, unsigned long long *field, unsigned long long v, uint32_t slot)
 {
     v = wave_sum(v);
-    if (__lane_id() == 0 && v) atomicAdd(dst, v);
+    if (__lane_id() == 0 && v) tally_add(C, field, v, slot);
 }
+
+__device__ inline void wave_tally(Counters *C, unsigned long long *field, unsigned long long v)
+{
+    wave_tally(C, field, v, wave_slot());
+}
+
 
 // Blocks for work_waves waves of work under a launch shape (capped; 0 max_blocks = default_cap).
 inline int grid_for(uint64_t work_waves, LaunchShape sh, uint64_t default_cap = 1ull << 20)

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint6
         }
         ndel += my_flag;
     }
-    if constexpr (ERASE) wave_add(&C->deleted, ndel);
+    if constexpr (ERASE) wave_tally(C, &C->deleted, ndel);
 }
 
 // ---------------------------------------------------------------------------------- insert
@@ -207,7 +207,7 @@ template <bool FROM_ENTRIES>
 __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint64_t *__restrict__ keys,
                                                           const uint64_t *__restrict__ vals, uint64_t n,
                                                           uint32_t pre_fixed, InsertPolicy P, Counters *__restrict__ C,
-                                                          WideSink sink)
+                                                          WideSink sink, const uint8_t *__restrict__ live)
 {
     const Desc &D = W.D;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -218,9 +218,14 @@ __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint
     c.active = false;
     for (;;) {
         while (!c.active && next < n) {  // take the next entry
+            if (FROM_ENTRIES && live && !live[next / kResTileEntries]) {  // a residual tile with none
+                next += stride;
+                continue;
+            }
             const uint64_t k = keys[next];
             const uint64_t v = vals[next];
             next += stride;
+            if (FROM_ENTRIES && k == 0) continue;  // an empty place in a residual or pending list
             if (wide_reserved(k)) {
                 ++rejected;
                 continue;
@@ -310,9 +315,9 @@ __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint
             }
         }
     }
-    wave_add(&C->kicks, kicks);
-    wave_add(&C->rejected, rejected);
-    wave_add(&C->retries, retries);
+    wave_tally(C, &C->kicks, kicks);
+    wave_tally(C, &C->rejected, rejected);
+    wave_tally(C, &C->retries, retries);
 }
 
 // ---------------------------------------------------------------------------------- resize
@@ -356,25 +361,29 @@ __global__ __launch_bounds__(256) void wide_upsize_kernel(const uint64_t *__rest
         store2(dk + (j + old_nb) * kSlots + s * 2, hk[0], hk[1]);
         store2(dv + (j + old_nb) * kSlots + s * 2, hv[0], hv[1]);
     }
-    wave_add(&C->misplaced, bad);
+    wave_tally(C, &C->misplaced, bad);
 }
 
 // Downsize compaction (cuckoo.cu:664-699): new bucket g keeps old bucket g slot for slot and
 // fills its empty slots from old bucket g + n read back to front (slot i <- slot 15 - i); an
 // entry of bucket g + n is a residual unless it is the one placed there (the reference compares
-// the placed entry with it, cuckoo.cu:685-688), here key and value both.
+// the placed entry with it, cuckoo.cu:685-688), here key and value both. Residuals are
+// slot-addressed (slot i of new bucket g's partner -> rk/rv[g * 16 + i], key 0 where none), as
+// in the 32-bit table, so no wave reserves list space through a single append index; live[tile]
+// flags the tiles of kResTileBuckets buckets that left any, and only those are written.
 __global__ __launch_bounds__(256) void wide_downsize_kernel(const uint64_t *__restrict__ ok_,
                                                             const uint64_t *__restrict__ ov, uint32_t new_nb,
                                                             uint64_t *__restrict__ dk, uint64_t *__restrict__ dv,
                                                             Counters *__restrict__ C, uint64_t *__restrict__ rk,
-                                                            uint64_t *__restrict__ rv, uint64_t res_cap)
+                                                            uint64_t *__restrict__ rv, uint8_t *__restrict__ live)
 {
     constexpr int G = 8, NG = kWave / G;
     const int lane = __lane_id();
     const int g = lane / G, s = lane % G;
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
-    for (uint64_t tile = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * NG; tile < new_nb;
-         tile += waves * NG) {
+    const uint32_t slot = wave_slot();
+    unsigned long long nres = 0;
+    for (uint64_t tile = (uint64_t)slot * NG; tile < new_nb; tile += waves * NG) {
         const uint64_t j = tile + g;
         const bool ok = j < new_nb;
         uint64_t ek[2] = {0, 0}, ev[2] = {0, 0}, ck[2] = {0, 0}, cv[2] = {0, 0};
@@ -402,26 +411,15 @@ __global__ __launch_bounds__(256) void wide_downsize_kernel(const uint64_t *__re
             store2(dk + j * kSlots + s * 2, pk[0], pk[1]);
             store2(dv + j * kSlots + s * 2, pv[0], pv[1]);
         }
-        const uint64_t b0 = __ballot(rk_[0] != 0), b1 = __ballot(rk_[1] != 0);
-        const uint32_t tot = __popcll(b0) + __popcll(b1);
-        if (tot == 0) continue;
-        unsigned long long at = 0;
-        if (lane == 0) at = atomicAdd(&C->pending, (unsigned long long)tot);
-        at = shfl64(at, 0);
-        const uint64_t o0 = at + __popcll(b0 & lanemask_lt());
-        const uint64_t o1 = at + __popcll(b0) + __popcll(b1 & lanemask_lt());
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            if (!rk_[q]) continue;
-            const uint64_t o = q == 0 ? o0 : o1;
-            if (o < res_cap) {
-                rk[o] = rk_[q];
-                rv[o] = rv_[q];
-            } else {
-                atomicAdd(&C->lost, 1ull);
-            }
+        const bool any = __ballot(rk_[0] != 0 || rk_[1] != 0) != 0;  // wave-uniform
+        if (lane == 0) live[tile / NG] = any ? 1 : 0;
+        if (any && ok) {  // slot-addressed residuals, key 0 where none
+            store2(rk + j * kSlots + s * 2, rk_[0], rk_[1]);
+            store2(rv + j * kSlots + s * 2, rv_[0], rv_[1]);
         }
+        nres += (rk_[0] != 0) + (rk_[1] != 0);
     }
+    wave_tally(C, &C->residuals, nres, slot);
 }
 
 constexpr uint64_t kWideInsertBlocks = 16384;
@@ -443,14 +441,15 @@ hipError_t launch_wide_probe(const WideDesc &W, const uint64_t *keys, uint64_t *
 
 hipError_t launch_wide_insert(const WideDesc &W, const uint64_t *keys, const uint64_t *vals, uint64_t n,
                               uint32_t pre_fixed, bool from_entries, InsertPolicy pol, LaunchShape sh, Counters *C,
-                              const WideSink &sink, hipStream_t st)
+                              const WideSink &sink, hipStream_t st, const uint8_t *live = nullptr)
 {
     if (n == 0) return hipSuccess;
     const int grid = grid_for((n + kWave - 1) / kWave, sh, kWideInsertBlocks);
     if (from_entries)
-        wide_insert_kernel<true><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, pre_fixed, pol, C, sink);
+        wide_insert_kernel<true><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, pre_fixed, pol, C, sink, live);
     else
-        wide_insert_kernel<false><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, kNoTable, pol, C, sink);
+        wide_insert_kernel<false><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, kNoTable, pol, C, sink,
+                                                                     nullptr);
     return hipGetLastError();
 }
 
@@ -464,10 +463,13 @@ hipError_t launch_wide_upsize(const uint64_t *ok_, const uint64_t *ov, uint32_t 
 }
 
 hipError_t launch_wide_downsize(const uint64_t *ok_, const uint64_t *ov, uint32_t new_nb, uint64_t *dk, uint64_t *dv,
-                                Counters *C, uint64_t *rk, uint64_t *rv, uint64_t res_cap, hipStream_t st)
+                                Counters *C, uint64_t *rk, uint64_t *rv, uint64_t res_cap, uint8_t *live,
+                                hipStream_t st)
 {
-    const int grid = grid_for(((uint64_t)new_nb + 7) / 8, LaunchShape{256, 0});
-    wide_downsize_kernel<<<grid, 256, 0, st>>>(ok_, ov, new_nb, dk, dv, C, rk, rv, res_cap);
+    if (new_nb == 0) return hipSuccess;
+    if (res_cap < (uint64_t)new_nb * kSlots || !live) return hipErrorInvalidValue;
+    const int grid = grid_for(((uint64_t)new_nb + kResTileBuckets - 1) / kResTileBuckets, LaunchShape{256, 0});
+    wide_downsize_kernel<<<grid, 256, 0, st>>>(ok_, ov, new_nb, dk, dv, C, rk, rv, live);
     return hipGetLastError();
 }
 
@@ -524,8 +526,9 @@ int wzero(dyc_wide *t, hipStream_t s)
 
 int wread(dyc_wide *t, hipStream_t s)
 {
-    HIP_TRY(hipMemcpyAsync(t->hctr, t->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(t->hctr, t->ctr, kCounterBytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    fold_tallies(t->hctr);
     return DYC_OK;
 }
 
@@ -691,29 +694,31 @@ int wresize_down(dyc_wide *t, hipStream_t s)
     const uint64_t res_cap = (uint64_t)new_nb * kSlots;
     uint8_t *dst = nullptr;
     uint64_t *rk = nullptr, *rv = nullptr;
+    // rv is followed by the residual list's tile flags
     if (hipMalloc(&dst, 2 * wide_bytes(new_nb)) != hipSuccess || hipMalloc(&rk, res_cap * 8) != hipSuccess ||
-        hipMalloc(&rv, res_cap * 8) != hipSuccess) {
+        hipMalloc(&rv, res_cap * 8 + res_tiles(new_nb)) != hipSuccess) {
         (void)hipGetLastError();
         if (dst) (void)hipFree(dst);
         if (rk) (void)hipFree(rk);
         return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
     }
+    uint8_t *live = reinterpret_cast<uint8_t *>(rv + res_cap);
     int rc = wzero(t, s);
     if (!rc) {
         const hipError_t e =
             launch_wide_downsize(t->W.D.sub[sub], t->W.val[sub], new_nb, reinterpret_cast<uint64_t *>(dst),
-                                 reinterpret_cast<uint64_t *>(dst + wide_bytes(new_nb)), t->ctr, rk, rv, res_cap, s);
+                                 reinterpret_cast<uint64_t *>(dst + wide_bytes(new_nb)), t->ctr, rk, rv, res_cap,
+                                 live, s);
         if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "downsize launch: %s", hipGetErrorString(e));
     }
     if (!rc) rc = wread(t, s);
-    if (!rc && t->hctr->lost) rc = fail(DYC_ERR_LOST, "downsize residual list overflow");
     if (rc) {
         (void)hipFree(dst);
         (void)hipFree(rk);
         (void)hipFree(rv);
         return rc;
     }
-    const uint64_t nres = t->hctr->pending;
+    const uint64_t nres = t->hctr->residuals;
     rc = wrelease(t, sub);
     if (rc) return rc;
     wset(t, sub, dst, new_nb);
@@ -724,8 +729,9 @@ int wresize_down(dyc_wide *t, hipStream_t s)
         rc = wreserve_pending(t, std::max<uint64_t>(nres, 1024), s);
         if (!rc) rc = wzero(t, s);
         if (!rc) {
-            const hipError_t e = launch_wide_insert(t->W, rk, rv, nres, sub, true, t->policy(), t->insert_shape(),
-                                                    t->ctr, t->sink_at(t->pend_n), s);
+            // the slot-addressed list: its live tiles (key 0 marks an empty place)
+            const hipError_t e = launch_wide_insert(t->W, rk, rv, res_cap, sub, true, t->policy(), t->insert_shape(),
+                                                    t->ctr, t->sink_at(t->pend_n), s, live);
             if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "residual insert launch: %s", hipGetErrorString(e));
         }
         if (!rc) rc = wread(t, s);
@@ -796,8 +802,8 @@ int dyc_wide_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_wide *
     t->cfg = cfg;
     desc_init(t->W.D, cfg.subtables);
     int rc = DYC_OK;
-    if (hipGetDevice(&t->device) != hipSuccess || hipMalloc(&t->ctr, sizeof(Counters)) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+    if (hipGetDevice(&t->device) != hipSuccess || hipMalloc(&t->ctr, kCounterBytes) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), kCounterBytes, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         rc = fail(DYC_ERR_OOM, "counter allocation failed");
     }
