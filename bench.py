@@ -340,20 +340,21 @@ def run_host(args, dev) -> int:
 def run_memsys(args, dev) -> int:
     """The device's random-access ceilings at the size of the 2^28-key table (2.39 GB): random
     128-byte bucket reads/s (one lane per bucket), random 64-bit CAS/s and random 8-byte
-    stores/s, 2^28 operations each (libdyc_memsys.so). Prints one JSON line with the ceilings
-    the bulk kernels are compared against."""
+    stores/s, 2^28 operations each, and the rate of per-wave atomic adds onto one address and onto
+    64 (libdyc_memsys.so). Prints one JSON line with the ceilings the bulk kernels are compared
+    against."""
     import ctypes
 
     lib = ctypes.CDLL(os.path.join(ROOT, "databases_gpu_amd", "libdyc_memsys.so"))
     lib.dyc_memsys_probe.restype = ctypes.c_int
     lib.dyc_memsys_probe.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     nb = 4 * 4660350  # buckets of the 2^28-key table at load factor 0.90
-    out = (ctypes.c_double * 5)()
+    out = (ctypes.c_double * 7)()
     rc = lib.dyc_memsys_probe(nb * 128, 1 << args.log2n, 3, out)
     if rc:
         print(f"dyc_memsys_probe failed: hipError {rc}", file=sys.stderr)
         return 1
-    reads, cas, stores, half, read_cas = out[0], out[1], out[2], out[3], out[4]
+    reads, cas, stores, half, read_cas, one_addr, spread64 = out[0], out[1], out[2], out[3], out[4], out[5], out[6]
     result = {
         "metric": "device random-access ceilings at 2.39 GB: bucket reads/s, 64-bit CAS/s, 8-byte stores/s",
         "value": round(reads / 1e9, 2), "unit": "G bucket reads/s", "n_gpus": 1, "higher_is_better": True,
@@ -361,6 +362,7 @@ def run_memsys(args, dev) -> int:
         "config": {"workload": f"random slots of a {nb * 128} B buffer, 2^{args.log2n} operations per pattern"},
         "bucket_reads_per_s": reads, "bucket_read_gbs": round(reads * 128 / 1e9, 1),
         "cas_per_s": cas, "stores_per_s": stores, "half_bucket_reads_per_s": half, "read_then_cas_per_s": read_cas,
+        "wave_atomics_one_address_per_s": one_addr, "wave_atomics_64_addresses_per_s": spread64,
     }
     line = json.dumps(result)
     print(line, flush=True)

@@ -531,8 +531,7 @@ __device__ inline void refill(const Desc &D, Carry<kSlots / G> &c, uint64_t &nex
         if (!c.active && rank < take) {
             const uint64_t idx = next + rank;
             if constexpr (FROM_ENTRIES) {
-                // skip, for an entries insert, is the per-tile liveness of a slot-addressed list
-                const uint64_t e = (!skip || skip[idx / kResTileEntries]) ? entries[idx] : 0ull;
+                const uint64_t e = entries[idx];
                 if (e)
                     carry_start(D, c, e, pre_fixed, s, tally);
                 else if (s == 0)
@@ -558,6 +557,7 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
     if (n_dev && *n_dev < n) n = *n_dev;  // a count an earlier launch left on the device
     constexpr int NG = kWave / G;
     constexpr uint64_t kChunk = 2 * NG;  // indexes a wave takes at a time
+    static_assert(kResTileEntries % kChunk == 0, "a chunk lies inside one residual tile");
     Carry<kSlots / G> c;
     c.active = false;
     InsertTally tally;
@@ -572,6 +572,10 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
             const uint64_t base = next_chunk * kChunk;
             if (base < n) {
                 next_chunk += waves;
+                // an entries insert reads skip as the tile flags of a slot-addressed list; a chunk
+                // lies inside one tile, and a tile flagged 0 holds no entry
+                if constexpr (FROM_ENTRIES)
+                    if (skip && !skip[base / kResTileEntries]) continue;
                 next = base;
                 end = base + kChunk < n ? base + kChunk : n;
                 continue;

@@ -94,18 +94,28 @@ __global__ void store_kernel(unsigned long long *__restrict__ buf, uint64_t nslo
         buf[mix((uint32_t)i * 2246822519u + 5u) % nslots] = i;
 }
 
+// one 64-bit atomic add per wave onto one of `spread` counters 256 B apart (spread 1: a single
+// address), the pattern of per-wave tallies and append indexes
+__global__ void wave_atomic_kernel(unsigned long long *__restrict__ ctr, uint32_t spread)
+{
+    const uint32_t w = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if ((threadIdx.x & 63u) == 0) atomicAdd(ctr + (uint64_t)(w % spread) * 32, 1ull);
+}
+
 }  // namespace
 
 extern "C" {
 
 // Rates over a zeroed device buffer of `bytes` (rounded down to whole 128-byte buckets), `ops`
 // operations per pattern, best of `reps`: out[0] bucket reads/s, out[1] CAS/s, out[2] stores/s,
-// out[3] half-bucket (64 B) reads/s, out[4] read-a-bucket-then-CAS-a-slot/s.
+// out[3] half-bucket (64 B) reads/s, out[4] read-a-bucket-then-CAS-a-slot/s, out[5] per-wave
+// atomic adds/s onto a single address, out[6] the same spread over 64 addresses 256 B apart
+// (65,536 waves, one add each, for these two).
 // Returns 0 or a hipError_t code.
 int dyc_memsys_probe(uint64_t bytes, uint64_t ops, int reps, double *out)
 {
     const uint64_t nb = bytes / 128, nslots = nb * 16;
-    if (nb == 0 || ops == 0 || !out) return (int)hipErrorInvalidValue;
+    if (nb < 128 || ops == 0 || !out) return (int)hipErrorInvalidValue;  // the counters need 16 KB
     unsigned long long *buf = nullptr;
     uint64_t *sink = nullptr;
     hipError_t e = hipMalloc(&buf, nb * 128);
@@ -117,8 +127,8 @@ int dyc_memsys_probe(uint64_t bytes, uint64_t ops, int reps, double *out)
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
-    const int grid = 16384;
-    for (int k = 0; k < 5; ++k) {
+    const int grid = 16384;  // 65,536 waves
+    for (int k = 0; k < 7; ++k) {
         float best = 1e30f;
         for (int r = 0; r < reps + 1; ++r) {  // the first run warms up
             (void)hipEventRecord(a, 0);
@@ -126,14 +136,15 @@ int dyc_memsys_probe(uint64_t bytes, uint64_t ops, int reps, double *out)
             else if (k == 1) cas_kernel<<<grid, 256>>>(buf, nslots, ops, sink);
             else if (k == 2) store_kernel<<<grid, 256>>>(buf, nslots, ops);
             else if (k == 3) read_half_kernel<<<grid, 256>>>(reinterpret_cast<const uint4 *>(buf), nb, ops, sink);
-            else read_cas_kernel<<<grid, 256>>>(buf, nb, ops, sink);
+            else if (k == 4) read_cas_kernel<<<grid, 256>>>(buf, nb, ops, sink);
+            else wave_atomic_kernel<<<grid, 256>>>(buf, k == 5 ? 1u : 64u);
             (void)hipEventRecord(b, 0);
             if ((e = hipEventSynchronize(b)) != hipSuccess) break;
             float ms = 0;
             (void)hipEventElapsedTime(&ms, a, b);
             if (r > 0 && ms < best) best = ms;
         }
-        out[k] = (double)ops / (best * 1e-3);
+        out[k] = (k < 5 ? (double)ops : grid * 4.0) / (best * 1e-3);
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

@@ -202,9 +202,9 @@ int upsize_now(dyc_server *s, uint32_t i, bool *ok)
 }
 
 // Halve one subtable of table i (choose_down) and re-insert the compaction's residuals into the
-// other subtable of their pair (the slot-addressed residual list is read whole, so the host
-// needs no count). *done = false if refused (odd bucket count) or the arena has no room for the
-// new subtable and the residual list.
+// other subtable of their pair (the slot-addressed residual list is read by its live tiles, so
+// the host needs no count). *done = false if refused (odd bucket count) or the arena has no room
+// for the new subtable and the list's tile flags.
 int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
 {
     SrvTable &T = s->t[i];
@@ -213,27 +213,32 @@ int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
     const uint32_t old_nb = T.D.nb[sub];
     if ((old_nb & 1u) || old_nb < 2) return DYC_OK;
     const uint32_t new_nb = old_nb / 2;
-    const uint64_t res_cap = (uint64_t)new_nb * kSlots;  // one pair of old buckets leaves at most 16
     const uint64_t off = s->arena.alloc((uint64_t)new_nb * kBucketBytes);
     if (off == Arena::kNone) return DYC_OK;
-    const uint64_t roff = s->arena.alloc(res_cap * sizeof(uint64_t) + res_tiles(new_nb));  // list, tile flags
-    if (roff == Arena::kNone) {
+    const uint64_t loff = s->arena.alloc(res_tiles(new_nb));  // the residual list's tile flags
+    if (loff == Arena::kNone) {
         s->arena.release(off);
         return DYC_OK;
     }
-    uint64_t *res = s->sub_ptr(roff);
-    uint8_t *live = reinterpret_cast<uint8_t *>(res + res_cap);
+    uint8_t *live = reinterpret_cast<uint8_t *>(s->sub_ptr(loff));
+    // The residuals go to the old subtable's upper half, slot-addressed (a group writes pair g's
+    // residuals there only after reading both of its buckets); the old region is released once
+    // the re-insert that reads them is on the stream (the arena hands ranges out stream-ordered).
+    uint64_t *old = T.D.sub[sub];
+    uint64_t *res = old + (uint64_t)new_nb * kSlots;
+    const uint64_t res_n = (uint64_t)new_nb * kSlots;
     HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(rctr), kCounterWords, s->st));
-    HIP_TRY(launch_downsize(T.D.sub[sub], new_nb, s->sub_ptr(off), rctr, res, res_cap, live, s->st));
-    s->arena.release(T.off[sub]);
+    HIP_TRY(launch_downsize(old, new_nb, s->sub_ptr(off), rctr, res, res_n, live, s->st));
+    const uint64_t old_off = T.off[sub];
     T.off[sub] = off;
     desc_set(T.D, (int)sub, s->sub_ptr(off), new_nb);
     s->dir_dirty = true;
     // residuals continue from the shrunk subtable into the other one of their pair
     // (cuckoo.cu:719), after the compaction; failures go to the pool
-    HIP_TRY(launch_insert_entries(T.D, res, res_cap, sub, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
+    HIP_TRY(launch_insert_entries(T.D, res, res_n, sub, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
                                   s->pc(), s->sink(i), s->st, nullptr, live));
-    s->arena.release(roff);
+    s->arena.release(old_off);
+    s->arena.release(loff);
     T.resizes_down++;
     s->ss.resizes_down++;
     *done = true;

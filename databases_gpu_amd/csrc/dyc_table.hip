@@ -84,6 +84,8 @@ struct dyc_table {
     DycHostStage *stage = nullptr;
     uint8_t *upd = nullptr;  // upsert: per-key "updated in place" flags
     uint64_t upd_cap = 0;
+    uint8_t *live = nullptr;  // downsize: tile flags of the residual list
+    uint64_t live_cap = 0;
     // Subtables created together (create, load, restore) share one allocation: the GPU's address
     // translation covers one large allocation with far fewer TLB entries than one allocation per
     // subtable, and random bucket access at 2^28 keys misses the per-CU TLB several times per
@@ -154,22 +156,29 @@ int read_counters(dyc_table *t, hipStream_t s)
 
 constexpr size_t kSlabAlign = 2u << 20;  // subtable regions of a slab start 2 MiB apart
 
-// Release subtable i's storage: its own allocation, or its region of the slab (the slab itself
+// Release a subtable's storage: its own allocation, or its region of the slab (the slab itself
 // once its last subtable is released). Callers synchronise first.
-int release_sub(dyc_table *t, uint32_t i)
+int release_block(dyc_table *t, uint64_t *p, bool in_slab)
 {
-    if (!t->D.sub[i]) return DYC_OK;
-    if (t->in_slab[i]) {
-        t->in_slab[i] = false;
+    if (!p) return DYC_OK;
+    if (in_slab) {
         if (--t->slab_live == 0) {
             HIP_TRY(hipFree(t->slab));
             t->slab = nullptr;
         }
-    } else {
-        HIP_TRY(hipFree(t->D.sub[i]));
+        return DYC_OK;
     }
-    t->D.sub[i] = nullptr;
+    HIP_TRY(hipFree(p));
     return DYC_OK;
+}
+
+int release_sub(dyc_table *t, uint32_t i)
+{
+    uint64_t *p = t->D.sub[i];
+    const bool in_slab = t->in_slab[i];
+    t->D.sub[i] = nullptr;
+    t->in_slab[i] = false;
+    return release_block(t, p, in_slab);
 }
 
 // One allocation for d subtables of nb[i] buckets; *base receives it and sub[i] the regions.
@@ -310,30 +319,43 @@ int do_resize_down(dyc_table *t, hipStream_t s)
     if (old_nb & 1u)
         return fail(DYC_ERR_ODD_BUCKETS, "downsize of subtable %u refused: %u buckets is odd", sub, old_nb);
     const uint32_t new_nb = old_nb / 2;
-    uint64_t *dst = nullptr, *res = nullptr;
-    const uint64_t res_cap = (uint64_t)new_nb * kSlots;
-    // the slot-addressed residual list, then its tile flags
-    if (hipMalloc(&dst, (size_t)new_nb * kBucketBytes) != hipSuccess ||
-        hipMalloc(&res, res_cap * sizeof(uint64_t) + res_tiles(new_nb)) != hipSuccess) {
+    uint64_t *dst = nullptr;
+    if (hipMalloc(&dst, (size_t)new_nb * kBucketBytes) != hipSuccess) {
         (void)hipGetLastError();
-        if (dst) (void)hipFree(dst);
         return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
     }
-    uint8_t *live = reinterpret_cast<uint8_t *>(res + res_cap);
+    if (res_tiles(new_nb) > t->live_cap) {  // the residual list's tile flags, grown as needed
+        if (t->live) (void)hipFree(t->live);
+        t->live = nullptr;
+        t->live_cap = 0;
+        if (hipMalloc(&t->live, res_tiles(new_nb)) != hipSuccess) {
+            (void)hipGetLastError();
+            t->live = nullptr;
+            (void)hipFree(dst);
+            return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
+        }
+        t->live_cap = res_tiles(new_nb);
+    }
+    // The residuals go to the old subtable's upper half, slot-addressed: a group writes pair g's
+    // residuals there only after reading both of its buckets, and no other group reads them. The
+    // old subtable is released once the residuals are re-inserted.
+    uint64_t *old = t->D.sub[sub];
+    const bool old_in_slab = t->in_slab[sub];
+    uint64_t *res = old + (uint64_t)new_nb * kSlots;
+    const uint64_t res_n = (uint64_t)new_nb * kSlots;
     int rc = zero_counters(t, s);
     if (!rc) {
-        hipError_t e = launch_downsize(t->D.sub[sub], new_nb, dst, t->ctr, res, res_cap, live, s);
+        hipError_t e = launch_downsize(old, new_nb, dst, t->ctr, res, res_n, t->live, s);
         if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "downsize launch: %s", hipGetErrorString(e));
     }
     if (!rc) rc = read_counters(t, s);
     if (rc) {
         (void)hipFree(dst);
-        (void)hipFree(res);
         return rc;
     }
     const uint64_t nres = t->hctr->residuals;
-    rc = release_sub(t, sub);
-    if (rc) return rc;
+    t->D.sub[sub] = nullptr;  // the table now owns dst; old is released below
+    t->in_slab[sub] = false;
     desc_set(t->D, (int)sub, dst, new_nb);
     t->st.resizes_down++;
     // residuals continue from the shrunk subtable into the other one of their pair
@@ -343,15 +365,20 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         if (!rc) rc = zero_counters(t, s);
         if (!rc) {
             // the slot-addressed list: its live tiles, zeros skipped
-            hipError_t e = launch_insert_entries(t->D, res, res_cap, sub, (int)t->cfg.insert_lanes, t->policy(),
+            hipError_t e = launch_insert_entries(t->D, res, res_n, sub, (int)t->cfg.insert_lanes, t->policy(),
                                                  t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s, nullptr,
-                                                 live);
+                                                 t->live);
             if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "residual insert launch: %s", hipGetErrorString(e));
         }
         if (!rc) rc = read_counters(t, s);
         if (!rc) rc = absorb_insert_counters(t, t->pend_cap - t->pend_n);
     }
-    (void)hipFree(res);
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        (void)release_block(t, old, old_in_slab);
+        return rc;
+    }
+    rc = release_block(t, old, old_in_slab);  // read_counters synchronised the stream
     if (rc) return rc;
     if (t->pend_n > pend_before) return place_pending(t, s, pend_before);
     return DYC_OK;
@@ -700,6 +727,7 @@ int dyc_destroy(dyc_table *t)
     if (t->part_counts) (void)hipFree(t->part_counts);
     if (t->hctr) (void)hipHostFree(t->hctr);
     if (t->upd) (void)hipFree(t->upd);
+    if (t->live) (void)hipFree(t->live);
     free_stage(t);
     delete t;
     return DYC_OK;

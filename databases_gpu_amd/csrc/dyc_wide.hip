@@ -498,6 +498,8 @@ struct dyc_wide {
     uint64_t *pk = nullptr, *pv = nullptr;  // pending keys / values [0, pend_n): the stash
     uint64_t pend_cap = 0, pend_n = 0;
     Counters *ctr = nullptr, *hctr = nullptr;
+    uint8_t *live = nullptr;  // downsize: tile flags of the residual list
+    uint64_t live_cap = 0;
     dyc_stats st{};
 
     uint64_t capacity() const
@@ -532,20 +534,27 @@ int wread(dyc_wide *t, hipStream_t s)
     return DYC_OK;
 }
 
-// Release subtable i's storage (its own allocation, or its share of the slab). Synchronise first.
-int wrelease(dyc_wide *t, uint32_t i)
+// Release a subtable's storage: its own allocation `own`, or (own null) its share of the slab.
+// Synchronise first.
+int wrelease_block(dyc_wide *t, uint8_t *own)
 {
-    if (!t->W.D.sub[i]) return DYC_OK;
-    if (t->own[i]) {
-        HIP_TRY(hipFree(t->own[i]));
-        t->own[i] = nullptr;
+    if (own) {
+        HIP_TRY(hipFree(own));
     } else if (t->slab && --t->slab_live == 0) {
         HIP_TRY(hipFree(t->slab));
         t->slab = nullptr;
     }
+    return DYC_OK;
+}
+
+int wrelease(dyc_wide *t, uint32_t i)
+{
+    if (!t->W.D.sub[i]) return DYC_OK;
+    uint8_t *own = t->own[i];
+    t->own[i] = nullptr;
     t->W.D.sub[i] = nullptr;
     t->W.val[i] = nullptr;
-    return DYC_OK;
+    return wrelease_block(t, own);
 }
 
 void wset(dyc_wide *t, uint32_t i, uint8_t *base, uint32_t nb)
@@ -691,36 +700,46 @@ int wresize_down(dyc_wide *t, hipStream_t s)
     if (old_nb & 1u)
         return fail(DYC_ERR_ODD_BUCKETS, "downsize of subtable %u refused: %u buckets is odd", sub, old_nb);
     const uint32_t new_nb = old_nb / 2;
-    const uint64_t res_cap = (uint64_t)new_nb * kSlots;
     uint8_t *dst = nullptr;
-    uint64_t *rk = nullptr, *rv = nullptr;
-    // rv is followed by the residual list's tile flags
-    if (hipMalloc(&dst, 2 * wide_bytes(new_nb)) != hipSuccess || hipMalloc(&rk, res_cap * 8) != hipSuccess ||
-        hipMalloc(&rv, res_cap * 8 + res_tiles(new_nb)) != hipSuccess) {
+    if (hipMalloc(&dst, 2 * wide_bytes(new_nb)) != hipSuccess) {
         (void)hipGetLastError();
-        if (dst) (void)hipFree(dst);
-        if (rk) (void)hipFree(rk);
         return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
     }
-    uint8_t *live = reinterpret_cast<uint8_t *>(rv + res_cap);
+    if (res_tiles(new_nb) > t->live_cap) {  // the residual list's tile flags, grown as needed
+        if (t->live) (void)hipFree(t->live);
+        t->live = nullptr;
+        t->live_cap = 0;
+        if (hipMalloc(&t->live, res_tiles(new_nb)) != hipSuccess) {
+            (void)hipGetLastError();
+            t->live = nullptr;
+            (void)hipFree(dst);
+            return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
+        }
+        t->live_cap = res_tiles(new_nb);
+    }
+    // The residuals go to the upper halves of the old subtable's key and value arrays,
+    // slot-addressed: a group writes pair g's residuals there only after reading both of its
+    // buckets, and no other group reads them. The old subtable is released once they are
+    // re-inserted.
+    uint64_t *ok_old = t->W.D.sub[sub], *ov_old = t->W.val[sub];
+    uint8_t *old_own = t->own[sub];
+    const uint64_t res_n = (uint64_t)new_nb * kSlots;
+    uint64_t *rk = ok_old + res_n, *rv = ov_old + res_n;
     int rc = wzero(t, s);
     if (!rc) {
         const hipError_t e =
-            launch_wide_downsize(t->W.D.sub[sub], t->W.val[sub], new_nb, reinterpret_cast<uint64_t *>(dst),
-                                 reinterpret_cast<uint64_t *>(dst + wide_bytes(new_nb)), t->ctr, rk, rv, res_cap,
-                                 live, s);
+            launch_wide_downsize(ok_old, ov_old, new_nb, reinterpret_cast<uint64_t *>(dst),
+                                 reinterpret_cast<uint64_t *>(dst + wide_bytes(new_nb)), t->ctr, rk, rv, res_n,
+                                 t->live, s);
         if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "downsize launch: %s", hipGetErrorString(e));
     }
     if (!rc) rc = wread(t, s);
     if (rc) {
         (void)hipFree(dst);
-        (void)hipFree(rk);
-        (void)hipFree(rv);
         return rc;
     }
     const uint64_t nres = t->hctr->residuals;
-    rc = wrelease(t, sub);
-    if (rc) return rc;
+    t->own[sub] = nullptr;  // detach the old storage; the table now owns dst
     wset(t, sub, dst, new_nb);
     t->own[sub] = dst;
     t->st.resizes_down++;
@@ -730,15 +749,19 @@ int wresize_down(dyc_wide *t, hipStream_t s)
         if (!rc) rc = wzero(t, s);
         if (!rc) {
             // the slot-addressed list: its live tiles (key 0 marks an empty place)
-            const hipError_t e = launch_wide_insert(t->W, rk, rv, res_cap, sub, true, t->policy(), t->insert_shape(),
-                                                    t->ctr, t->sink_at(t->pend_n), s, live);
+            const hipError_t e = launch_wide_insert(t->W, rk, rv, res_n, sub, true, t->policy(), t->insert_shape(),
+                                                    t->ctr, t->sink_at(t->pend_n), s, t->live);
             if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "residual insert launch: %s", hipGetErrorString(e));
         }
         if (!rc) rc = wread(t, s);
         if (!rc) rc = wabsorb(t, t->pend_cap - t->pend_n);
     }
-    (void)hipFree(rk);
-    (void)hipFree(rv);
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        (void)wrelease_block(t, old_own);
+        return rc;
+    }
+    rc = wrelease_block(t, old_own);  // wread synchronised the stream
     if (rc) return rc;
     if (t->pend_n > pend_before) return wplace_pending(t, s, pend_before);
     return DYC_OK;
@@ -833,6 +856,7 @@ int dyc_wide_destroy(dyc_wide *t)
     if (t->pv) (void)hipFree(t->pv);
     if (t->ctr) (void)hipFree(t->ctr);
     if (t->hctr) (void)hipHostFree(t->hctr);
+    if (t->live) (void)hipFree(t->live);
     delete t;
     return DYC_OK;
 }
