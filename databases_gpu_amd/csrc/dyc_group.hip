@@ -334,7 +334,7 @@ int dyc_group_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, c
     const uint64_t room = g->pend_cap - g->pend_n;
     HIP_TRY(launch_group_insert(g->ddir, g->ditems, n_items, keys, vals, g->policy(), (int)g->launch.insert_lanes,
                                 (int)g->launch.insert_blocks, g->ctr,
-                                PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, 0, nullptr}, g->tcounts,
+                                PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, 0, 0, nullptr}, g->tcounts,
                                 s));
     rc = read_back(g, s);
     if (rc) return rc;
@@ -402,10 +402,12 @@ int dyc_group_insert_per_table(dyc_group *g, const uint32_t *keys, const uint32_
         if (!m) continue;
         HIP_TRY(launch_insert(g->hdir[i], keys + offsets[i], vals + offsets[i], m, (int)g->launch.insert_lanes,
                               g->policy(), sh, g->ctr,
-                              PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, i, g->tcounts + i}, s));
+                              PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, i, 0, g->tcounts + i}, s));
     }
     rc = read_back(g, s);
     if (rc) return rc;
+    // tcounts held each table's reserved keys; a table took in its batch less those
+    for (uint32_t i = 0; i < g->n; ++i) g->htcounts[i] = (offsets[i + 1] - offsets[i]) - g->htcounts[i];
     return finish_insert(g, room);
 }
 

@@ -584,9 +584,13 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
         if (!__ballot(c.active)) break;  // input exhausted and every group idle
         if (c.active) kick_step<G>(D, c, P, C, sink, sink.tag, tally);
     }
-    if (sink.placed) {
-        const unsigned long long rej = wave_sum(tally.rejected) + wave_sum(tally.empty);
-        if (__lane_id() == 0 && taken > rej) atomicAdd(sink.placed, taken - rej);
+    if (sink.count_placed || sink.rejected_at) {
+        const unsigned long long rej = wave_sum(tally.rejected);
+        const unsigned long long skipped = rej + wave_sum(tally.empty);
+        if (__lane_id() == 0) {
+            if (sink.count_placed && taken > skipped) tally_add(C, &C->placed, taken - skipped, wave_slot());
+            if (sink.rejected_at && rej) atomicAdd(sink.rejected_at, rej);
+        }
     }
     wave_tally(C, &C->kicks, tally.kicks);
     wave_tally(C, &C->rejected, tally.rejected);

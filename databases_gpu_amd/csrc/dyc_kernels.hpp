@@ -23,7 +23,8 @@ struct Counters {
     unsigned long long stalls;     // serving tick: dependency waits that hit their bound
     unsigned long long updated;    // upsert: keys whose stored value was replaced
     unsigned long long residuals;  // downsize: entries of the upper half that did not stay
-    unsigned long long pad[5];
+    unsigned long long placed;     // insert with PendSink::count_placed: entries taken in
+    unsigned long long pad[4];
 };
 constexpr int kTallyStripes = 64;
 constexpr int kTallyStrideWords = 32;  // 256 B between stripes
@@ -53,7 +54,11 @@ struct PendSink {
     uint32_t *tags;              // optional: the table index of each pending entry
     uint64_t cap;                // entries pend can hold
     uint32_t tag;                // table index of this launch (ignored by group launches)
-    unsigned long long *placed;  // optional: += entries this launch took in (reserved keys excluded)
+    uint32_t count_placed;       // nonzero: add the entries this launch took in (reserved keys and
+                                 // empty places excluded) to the tally Counters::placed
+    unsigned long long *rejected_at;  // optional: += reserved keys this launch met (an atomic only
+                                      // from a wave that met one; a keys insert of n took in n
+                                      // minus this)
 };
 
 struct InsertPolicy {

@@ -58,6 +58,7 @@ struct SrvTable {
     uint32_t inflight = 0;          // ticks in flight that touch the table
     bool check_down = false;        // an erase retired since the last downsize check
     uint64_t sync_del = 0;          // entries deleted by synchronous erases (not in del_total)
+    uint64_t sync_ins = 0;          // entries taken in by synchronous inserts (not in ins_total)
     uint64_t resizes_up = 0, resizes_down = 0;
 
     uint64_t capacity() const
@@ -94,6 +95,7 @@ struct Tick {
     bool busy = false;
     bool has_snap = false;  // the tick copies the device counters out (it may change counts)
     bool solo_erase = false;  // one erase through the plain kernel: its deletions are in Counters::deleted
+    bool solo_insert = false;  // one insert through the plain kernel: its entries are in Counters::placed
     uint32_t solo_table = 0;
     uint64_t seq = 0;
     std::vector<uint64_t> reqs;
@@ -119,6 +121,7 @@ struct dyc_server {
     uint64_t *d_resolve = nullptr;  // staging for re-placing pool entries (pool_cap entries)
     uint64_t pool_settled = 0;      // pool entries left by the last re-placement
     uint64_t deleted_seen = 0;      // Counters::deleted as of the last synchronous erase
+    uint64_t placed_seen = 0;       // Counters::placed as of the last synchronous insert
     uint64_t misplaced_seen = 0;
     uint64_t stalls_seen = 0;
     Tick ticks[2];
@@ -156,9 +159,11 @@ struct dyc_server {
     LaunchShape shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.max_blocks}; }
     LaunchShape insert_shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.insert_blocks}; }
     Pool pool() const { return Pool{d_pool, d_pool_tag, scfg.pool_cap, &pc()->pending, 0}; }
-    PendSink sink(uint32_t tag, unsigned long long *placed = nullptr) const
+    // count_placed: the launch adds the entries it takes in to Counters::placed (a synchronous
+    // insert, whose table the host credits from that tally)
+    PendSink sink(uint32_t tag, bool count_placed = false) const
     {
-        return PendSink{d_pool, d_pool_tag, scfg.pool_cap, tag, placed};
+        return PendSink{d_pool, d_pool_tag, scfg.pool_cap, tag, count_placed ? 1u : 0u, nullptr};
     }
     uint32_t depth() const { return scfg.pipeline_depth; }
 };
@@ -176,7 +181,7 @@ int sticky(dyc_server *s, int rc)
 void refresh_counts(dyc_server *s, const unsigned long long *snap)
 {
     const unsigned long long *ins = snap + kCounterWords, *del = snap + kCounterWords + s->n;
-    for (uint32_t i = 0; i < s->n; ++i) s->t[i].count = ins[i] - del[i] - s->t[i].sync_del;
+    for (uint32_t i = 0; i < s->n; ++i) s->t[i].count = ins[i] + s->t[i].sync_ins - del[i] - s->t[i].sync_del;
 }
 
 // Double one subtable of table i (choose_up); *ok = false (nothing changed) if the arena has
@@ -350,6 +355,10 @@ int retire(dyc_server *s, Tick &k)
     if (k.solo_erase && c.deleted > s->deleted_seen) {
         s->t[k.solo_table].sync_del += c.deleted - s->deleted_seen;
         s->deleted_seen = c.deleted;
+    }
+    if (k.solo_insert && c.placed > s->placed_seen) {
+        s->t[k.solo_table].sync_ins += c.placed - s->placed_seen;
+        s->placed_seen = c.placed;
     }
     if (c.lost) status = sticky(s, fail(DYC_ERR_LOST, "overflow pool full: %llu entries lost", (unsigned long long)c.lost));
     if (c.misplaced > s->misplaced_seen) {
@@ -607,17 +616,19 @@ int build_tick(dyc_server *s, bool *launched, bool *stuck)
     // Alone on an idle server (light load), a request goes out exactly as the synchronous path
     // launches it -- the plain single-table kernel -- without waiting for it.
     k.solo_erase = false;
+    k.solo_insert = false;
     if (k.reqs.size() == 1) {
         const dyc_request &q = s->reqs[k.reqs.front()].r;
         const SrvTable &T = s->t[q.table];
         const Pool pool = s->pool();
         k.has_snap = q.op != DYC_OP_FIND;
         k.solo_erase = q.op == DYC_OP_ERASE;
+        k.solo_insert = q.op == DYC_OP_INSERT;
         k.solo_table = q.table;
         if (dir_up) s->dir_dirty = true;  // the plain kernels take the descriptor by value
         if (q.op == DYC_OP_INSERT)
             HIP_TRY(launch_insert(T.D, q.keys, q.vals, q.n, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
-                                  s->pc(), s->sink(q.table, s->ins_total() + q.table), s->st));
+                                  s->pc(), s->sink(q.table, true), s->st));
         else if (q.op == DYC_OP_FIND)
             HIP_TRY(launch_find(T.D, q.keys, q.out_vals, q.out_flags, q.n, (int)s->cfg.find_lanes, s->shape(), s->st,
                                 &pool, q.table));
@@ -800,7 +811,7 @@ int run_sync(dyc_server *s, uint64_t tk)
         }
         if (status == DYC_OK)
             HIP_TRY(launch_insert(T.D, r.keys, r.vals, r.n, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
-                                  s->pc(), s->sink(r.table, s->ins_total() + r.table), s->st));
+                                  s->pc(), s->sink(r.table, true), s->st));
     } else if (r.op == DYC_OP_FIND) {
         // a find changes nothing: launch, wait, done
         const Pool pool = s->pool();
@@ -820,6 +831,10 @@ int run_sync(dyc_server *s, uint64_t tk)
     if (c.deleted > s->deleted_seen) {
         T.sync_del += c.deleted - s->deleted_seen;
         s->deleted_seen = c.deleted;
+    }
+    if (c.placed > s->placed_seen) {
+        T.sync_ins += c.placed - s->placed_seen;
+        s->placed_seen = c.placed;
     }
     refresh_counts(s, snap);
     if (c.lost) status = sticky(s, fail(DYC_ERR_LOST, "overflow pool full"));

@@ -110,7 +110,10 @@ struct dyc_table {
 namespace {
 
 // Pending list of a table from index `at` on (an insert launch appends there).
-PendSink sink_at(const dyc_table *t, uint64_t at) { return PendSink{t->pend + at, nullptr, t->pend_cap - at, 0, nullptr}; }
+PendSink sink_at(const dyc_table *t, uint64_t at)
+{
+    return PendSink{t->pend + at, nullptr, t->pend_cap - at, 0, 0, nullptr};
+}
 
 // The pending list is the table's stash: finds, erases and updates that miss both buckets of a
 // key search it. Returns the pool to pass to a launch, or nullptr when nothing is pending.
