@@ -81,14 +81,15 @@ def test_group_insert_find_match_oracle_per_table(D, per_table):
     g.close()
 
 
-def test_group_erase_and_reserved_keys(D):
-    banner("group erase + reserved keys")
+@pytest.mark.parametrize("per_table", [False, True])
+def test_group_erase_and_reserved_keys(D, per_table):
+    banner(f"group erase + reserved keys, per_table={per_table}")
     sizes = sizes_loguniform(11, 2**8, 2**12, 3)
     keys, vals, offs = batch(sizes, 4)
     keys[offs[2]] = 0xFFFFFFFF  # reserved: rejected by insert
     hints = (np.ceil(sizes / 0.85) + 64).astype(np.uint64)
     g = D.DyCuckooGroup(hints, insert_lanes=4)
-    g.insert(dev(keys), dev(vals), offs)
+    g.insert(dev(keys), dev(vals), offs, per_table=per_table)
     assert g.table_stats(2)["count"] == sizes[2] - 1 and g.table_stats(3)["count"] == sizes[3]
     # erase the first half of every table
     e_parts, e_offs = [], [0]

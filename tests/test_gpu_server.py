@@ -229,3 +229,39 @@ def test_dependency_wait_bound_fails_the_tick_not_the_gpu(D):
     assert (st == 2).any()  # DYC_ERR_HIP: a wait hit its bound
     assert set(np.unique(st).tolist()) <= {0, 2}
     srv.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_reserved_keys_in_single_request_and_synchronous_inserts(D, mode):
+    """A request alone on an idle server (mode 0) and the synchronous path (mode 1) both insert
+    through the plain kernel; the table is credited with the entries that launch took in, so
+    reserved keys (0 and 0xFFFFFFFF) in the batch are not counted."""
+    from databases_gpu_amd import _lib
+
+    banner(f"server: reserved keys in plain-kernel inserts, mode={mode}")
+    rng = np.random.default_rng(77)
+    keys = np.unique(rng.integers(1, 2**32 - 1, 50_000, dtype=np.uint64).astype(np.uint32))
+    rng.shuffle(keys)
+    k0, k1 = keys[:20_000].copy(), keys[20_000:32_000].copy()
+    k0[[5, 900]] = [0, 0xFFFFFFFF]
+    k1[77] = 0
+    t = [torch.from_numpy(a.view(np.int32)).cuda() for a in (k0, k0 ^ np.uint32(0x1234567), k1, k1 + np.uint32(9))]
+    flags = torch.zeros(k0.size, dtype=torch.uint8, device="cuda")
+    srv = D.DyCuckooServer([40_000, 40_000], min_capacity=0, hbm_budget=64 << 20)
+    req = np.zeros(3, dtype=np.dtype(_lib.REQUEST_DTYPE))
+    req[0] = (0, _lib.OP_INSERT, t[0].data_ptr(), t[1].data_ptr(), 0, 0, k0.size)
+    req[1] = (1, _lib.OP_INSERT, t[2].data_ptr(), t[3].data_ptr(), 0, 0, k1.size)
+    req[2] = (0, _lib.OP_ERASE, t[0].data_ptr(), 0, 0, flags.data_ptr(), k0.size)
+    torch.cuda.synchronize()
+    _, st = srv.replay(req, [0.0, 0.02, 0.04], mode=mode)  # spaced: each request alone
+    assert (st == 0).all()
+    assert srv.table_stats(1)["count"] == k1.size - 1
+    f = flags.cpu().numpy()
+    assert f.sum() == k0.size - 2 and f[5] == 0 and f[900] == 0
+    assert srv.table_stats(0)["count"] == 0
+    ref = O.OracleTable(nb=[x.size // 16 for x in srv.dump(1)])
+    ref.sub = srv.dump(1)
+    ok = k1 != 0
+    assert np.array_equal(ref.entries(), np.sort(O.pack(k1[ok], (k1 + np.uint32(9))[ok])))
+    srv.close()
+
