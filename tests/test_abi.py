@@ -208,6 +208,43 @@ int main() {
             assert got == want, (k, d)
 
 
+def test_counter_block_folds_every_stripe_into_the_base(tmp_path):
+    """Per-wave tallies land in one of 64 striped copies of the counter block; the host adds every
+    stripe into the base copy once (fold_tallies). A block whose stripe s holds 1000 * (s + 1) +
+    word, over base words 7 * word, must fold to the closed-form sums and leave the stripes."""
+    src = tmp_path / "fold.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <vector>
+#include "dyc_kernels.hpp"
+int main() {
+    using namespace dyc;
+    static_assert(sizeof(Counters) == 16 * sizeof(unsigned long long), "16 counter words");
+    std::vector<unsigned long long> w(kCounterWords, 0);
+    for (int i = 0; i < 16; ++i) w[i] = 7ull * i;
+    for (int s = 0; s < kTallyStripes; ++s)
+        for (int i = 0; i < 16; ++i) w[kTallyStrideWords * (s + 1) + i] = 1000ull * (s + 1) + i;
+    fold_tallies(w.data());
+    const Counters &c = *reinterpret_cast<const Counters *>(w.data());
+    int bad = 0;
+    const unsigned long long S = kTallyStripes;
+    for (int i = 0; i < 16; ++i)
+        bad += w[i] != 7ull * i + 1000ull * S * (S + 1) / 2 + S * i;
+    for (int s = 0; s < kTallyStripes; ++s)
+        for (int i = 0; i < 16; ++i) bad += w[kTallyStrideWords * (s + 1) + i] != 1000ull * (s + 1) + i;
+    bad += c.deleted != w[4] || c.placed != w[11];
+    printf("words %d stripes %d stride %d bad %d\n", kCounterWords, kTallyStripes, kTallyStrideWords, bad);
+    return bad != 0;
+}
+''')
+    exe = tmp_path / "fold"
+    subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, str(src), "-o", str(exe)], check=True,
+                   capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.split() == ["words", "2080", "stripes", "64", "stride", "32", "bad", "0"]
+
+
 def test_cpp_surfaces_compile(tmp_path):
     """include/dyc_hashapi.hpp and the reference-test driver build against the C ABI."""
     obj = tmp_path / "driver.o"
