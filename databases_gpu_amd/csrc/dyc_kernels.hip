@@ -550,11 +550,9 @@ template <int G, bool FROM_ENTRIES>
 __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ vals,
                                                      const uint64_t *__restrict__ entries, uint64_t n,
-                                                     const unsigned long long *__restrict__ n_dev,
                                                      uint32_t pre_fixed, InsertPolicy P, Counters *__restrict__ C,
                                                      PendSink sink, const uint8_t *__restrict__ skip)
 {
-    if (n_dev && *n_dev < n) n = *n_dev;  // a count an earlier launch left on the device
     constexpr int NG = kWave / G;
     constexpr uint64_t kChunk = 2 * NG;  // indexes a wave takes at a time
     static_assert(kResTileEntries % kChunk == 0, "a chunk lies inside one residual tile");
@@ -1188,7 +1186,7 @@ hipError_t launch_erase(const Desc &D, const uint32_t *keys, uint8_t *deleted, u
 
 template <bool FE>
 static hipError_t insert_common(const Desc &D, const uint32_t *keys, const uint32_t *vals, const uint64_t *entries,
-                                uint64_t n, const unsigned long long *n_dev, uint32_t pre, int lanes,
+                                uint64_t n, uint32_t pre, int lanes,
                                 InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
                                 const uint8_t *skip = nullptr)
 {
@@ -1198,23 +1196,23 @@ static hipError_t insert_common(const Desc &D, const uint32_t *keys, const uint3
     const int grid = grid_for(work_waves, sh, insert_default_cap(work_waves, sh));
     switch (lanes) {
     case 8:
-        insert_kernel<8, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+        insert_kernel<8, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C,
                                                                       sink, skip);
         break;
     case 4:
-        insert_kernel<4, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+        insert_kernel<4, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C,
                                                                       sink, skip);
         break;
     case 2:
-        insert_kernel<2, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+        insert_kernel<2, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C,
                                                                       sink, skip);
         break;
     case 1:
-        insert_kernel<1, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+        insert_kernel<1, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C,
                                                                       sink, skip);
         break;
     default:
-        insert_kernel<16, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, n_dev, pre, pol, C,
+        insert_kernel<16, FE><<<grid, sh.block_threads, 0, st>>>(D, keys, vals, entries, n, pre, pol, C,
                                                                       sink, skip);
         break;
     }
@@ -1225,7 +1223,7 @@ hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *va
                          InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
                          const uint8_t *skip)
 {
-    return insert_common<false>(D, keys, vals, nullptr, n, nullptr, kNoTable, lanes, pol, sh, C, sink, st, skip);
+    return insert_common<false>(D, keys, vals, nullptr, n, kNoTable, lanes, pol, sh, C, sink, st, skip);
 }
 
 hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint8_t *updated, uint64_t n,
@@ -1245,9 +1243,9 @@ hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *va
 
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
                                  InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
-                                 const unsigned long long *n_dev, const uint8_t *live)
+                                 const uint8_t *live)
 {
-    return insert_common<true>(D, nullptr, nullptr, entries, n, n_dev, pre, lanes, pol, sh, C, sink, st, live);
+    return insert_common<true>(D, nullptr, nullptr, entries, n, pre, lanes, pol, sh, C, sink, st, live);
 }
 
 hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,

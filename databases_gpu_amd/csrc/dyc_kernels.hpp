@@ -86,13 +86,12 @@ hipError_t launch_insert(const Desc &D, const uint32_t *keys, const uint32_t *va
 // there was one; the count goes to C->updated.
 hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *vals, uint8_t *updated, uint64_t n,
                          int lanes, LaunchShape sh, Counters *C, hipStream_t st, const Pool *pool = nullptr);
-// n_dev (optional): a device count read at kernel start; entries [0, min(n, *n_dev)) are inserted.
 // A zero entry is an empty place (an erased pending entry, a slot without a residual): skipped,
 // and counted neither as rejected nor as placed. live (optional, a slot-addressed residual list's
 // tile flags from launch_downsize): entries of a tile whose byte is 0 are empty and not read.
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
                                  InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
-                                 const unsigned long long *n_dev = nullptr, const uint8_t *live = nullptr);
+                                 const uint8_t *live = nullptr);
 // Order an insert batch by the region of its first bucket: out[0, n) receives the packed
 // entries, region by region. counts and cursors are R words each (counts zeroed by the caller).
 hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,

@@ -241,7 +241,7 @@ int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
     // residuals continue from the shrunk subtable into the other one of their pair
     // (cuckoo.cu:719), after the compaction; failures go to the pool
     HIP_TRY(launch_insert_entries(T.D, res, res_n, sub, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
-                                  s->pc(), s->sink(i), s->st, nullptr, live));
+                                  s->pc(), s->sink(i), s->st, live));
     s->arena.release(old_off);
     s->arena.release(loff);
     T.resizes_down++;

@@ -369,8 +369,7 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         if (!rc) {
             // the slot-addressed list: its live tiles, zeros skipped
             hipError_t e = launch_insert_entries(t->D, res, res_n, sub, (int)t->cfg.insert_lanes, t->policy(),
-                                                 t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s, nullptr,
-                                                 t->live);
+                                                 t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s, t->live);
             if (e != hipSuccess) rc = fail(DYC_ERR_HIP, "residual insert launch: %s", hipGetErrorString(e));
         }
         if (!rc) rc = read_counters(t, s);
