@@ -2,7 +2,11 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <immintrin.h>
 #include <stdint.h>
+
+#include <atomic>
+#include <chrono>
 
 #include "../../include/dyc.h"
 #include "dyc_device.hpp"
@@ -15,6 +19,26 @@ int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 const char *last_error();
 
 constexpr uint32_t kMaxBuckets = 1u << 29;  // packed bucket locations use 29 bits
+
+// Wait for a kernel on stream s to post seq to post[0] (host-pinned). Spins for up to spin_us,
+// which covers the small launch groups whose latency this saves; past that, or if the kernel
+// never posts (a fault), the stream is synchronised, which also returns the stream's error.
+inline hipError_t await_post(const volatile unsigned long long *post, unsigned long long seq, hipStream_t s,
+                             double spin_us = 2000.0)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    while (post[0] != seq) {
+        _mm_pause();
+        if (std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() > spin_us) {
+            const hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return e;
+            if (post[0] != seq) return hipErrorUnknown;
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return hipSuccess;
+}
 
 // Buckets per subtable of a d-subtable table for a slot-count hint (api.cpp:47-56, which
 // divides by TABLE_NUM * BUCKET_SIZE).

@@ -1464,6 +1464,29 @@ hipError_t launch_zero(unsigned long long *p, uint64_t words, hipStream_t st)
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(64) void fold_post_kernel(unsigned long long *block, unsigned long long *post,
+                                                       unsigned long long seq)
+{
+    constexpr int kWords = (int)(sizeof(Counters) / sizeof(unsigned long long));
+    const int i = threadIdx.x;
+    unsigned long long v = 0;
+    if (i < kWords) {
+        v = block[i];
+        for (int s = 1; s <= kTallyStripes; ++s) v += block[kTallyStrideWords * s + i];
+        post[1 + i] = v;  // stored once every load of the sum has returned
+    }
+    // the clears come after those stores in program order, so after every load of the block
+    for (int w = i; w < kCounterWords; w += 64) block[w] = 0;
+    __threadfence_system();
+    if (i == 0) post[0] = seq;
+}
+
+hipError_t launch_fold_post(Counters *C, unsigned long long *post, unsigned long long seq, hipStream_t st)
+{
+    fold_post_kernel<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long *>(C), post, seq);
+    return hipGetLastError();
+}
+
 hipError_t launch_probe_count(const Desc &D, const uint32_t *keys, uint64_t n, Counters *C, hipStream_t st)
 {
     if (n == 0) return hipSuccess;

@@ -197,6 +197,12 @@ inline uint64_t res_tiles(uint32_t new_nb) { return (new_nb + kResTileBuckets - 
 hipError_t launch_downsize(const uint64_t *old, uint32_t new_nb, uint64_t *dst, Counters *C, uint64_t *res,
                            uint64_t res_cap, uint8_t *live, hipStream_t st);
 hipError_t launch_zero(unsigned long long *p, uint64_t words, hipStream_t st);
+// Read a counter block back: one wave folds the stripes into the 16 counter words, writes them to
+// post[1..16] (host-pinned), then seq to post[0], and clears the whole block for the next launch
+// group. The host waits for post[0] == seq (await_post) instead of copying 16.6 KB and
+// synchronising the stream: 7 us against 20 us per read on MI355X.
+hipError_t launch_fold_post(Counters *C, unsigned long long *post, unsigned long long seq, hipStream_t st);
+constexpr int kPostWords = 1 + (int)(sizeof(Counters) / sizeof(unsigned long long));
 // Kernel copy of up to two segments of 8-byte words (either side may be host-pinned memory).
 hipError_t launch_copy(const void *src0, void *dst0, uint64_t words0, const void *src1, void *dst1, uint64_t words1,
                        hipStream_t st);

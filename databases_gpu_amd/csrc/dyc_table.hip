@@ -16,6 +16,7 @@
 #include <vector>
 
 using namespace dyc;
+using dyc::host::await_post;
 using dyc::host::choose_down;
 using dyc::host::choose_up;
 using dyc::host::config_subtables;
@@ -76,8 +77,11 @@ struct dyc_table {
     uint64_t *pend = nullptr;  // pending entries [0, pend_n)
     uint64_t pend_cap = 0;
     uint64_t pend_n = 0;
-    Counters *ctr = nullptr;    // device
-    Counters *hctr = nullptr;   // pinned host mirror
+    Counters *ctr = nullptr;    // device (kCounterBytes: base and stripes)
+    Counters *hctr = nullptr;   // host copy of the folded counters
+    unsigned long long *hpost = nullptr;  // pinned: the read-back words (launch_fold_post)
+    unsigned long long post_seq = 0;
+    bool ctr_clean = false;     // the device block is zero and no launch has used it since
     uint64_t *part = nullptr;   // insert batch ordered by bucket region
     uint64_t part_cap = 0;
     unsigned long long *part_counts = nullptr;   // kMaxRegions counts, then kMaxRegions cursors
@@ -143,17 +147,24 @@ int compact_pending(dyc_table *t, hipStream_t s)
     return DYC_OK;
 }
 
+// Callers zero the counters before a launch group and read them after it; a read clears the
+// device block, so the zero is launched only when something else left the block in use.
 int zero_counters(dyc_table *t, hipStream_t s)
 {
-    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->ctr), kCounterWords, s));
+    if (!t->ctr_clean) HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->ctr), kCounterWords, s));
+    t->ctr_clean = false;  // the launch group that follows uses it
     return DYC_OK;
 }
 
+// Fold, post and clear the counters (launch_fold_post) and wait for them. Every kernel queued on
+// s before has finished when this returns.
 int read_counters(dyc_table *t, hipStream_t s)
 {
-    HIP_TRY(hipMemcpyAsync(t->hctr, t->ctr, kCounterBytes, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    fold_tallies(t->hctr);
+    const unsigned long long seq = ++t->post_seq;
+    HIP_TRY(launch_fold_post(t->ctr, t->hpost, seq, s));
+    HIP_TRY(await_post(t->hpost, seq, s));
+    memcpy(t->hctr, const_cast<const unsigned long long *>(t->hpost) + 1, sizeof(Counters));
+    t->ctr_clean = true;
     return DYC_OK;
 }
 
@@ -690,9 +701,13 @@ int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out
     HIP_TRY(hipGetDevice(&t->device));
     int rc = DYC_OK;
     if (hipMalloc(&t->ctr, kCounterBytes) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), kCounterBytes, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&t->hpost), kPostWords * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         rc = fail(DYC_ERR_OOM, "counter allocation failed");
+    } else {
+        memset(t->hpost, 0, kPostWords * sizeof(unsigned long long));  // no sequence number yet
     }
     if (rc == DYC_OK) {
         const uint32_t nbs[kMaxSubtables] = {nb, nb, nb, nb, nb, nb};
@@ -728,6 +743,7 @@ int dyc_destroy(dyc_table *t)
     if (t->part) (void)hipFree(t->part);
     if (t->part_counts) (void)hipFree(t->part_counts);
     if (t->hctr) (void)hipHostFree(t->hctr);
+    if (t->hpost) (void)hipHostFree(t->hpost);
     if (t->upd) (void)hipFree(t->upd);
     if (t->live) (void)hipFree(t->live);
     free_stage(t);
