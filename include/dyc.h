@@ -109,7 +109,8 @@ int dyc_destroy(dyc_table *t);
 /* Replaces gpu_lp_insert (cuckoo.h:100-104, cuckoo.cu:858-877) and the insert half of
  * hashAPI::hash_insert / hash_insert_batch (api.cpp:184-265, 353-432).
  * With auto_resize, upsizes first while count + n > upper * capacity (api.cpp:360-369).
- * Synchronises `stream` once at the end (it reads the failure counters). */
+ * Waits once, at the end, for the work on `stream` (it reads the failure counters): when it
+ * returns, every kernel queued on `stream` before and by the call has finished. */
 int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
 
 /* Insert with update (the paper's insert, tkde-revision/secs/cuckoo.tex:127-129; the reference
@@ -117,7 +118,7 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
  * copy held by the first bucket (pos1, then pos2) that holds it -- what dyc_find returns -- and
  * updated[i] = 1 (updated may be NULL); a key in neither bucket but in the stash gets it in every
  * stash copy. Every other key is inserted as by dyc_insert. Keys within one call must be
- * distinct. Synchronises once. */
+ * distinct. Waits once, as dyc_insert. */
 int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, uint8_t *updated,
                dyc_stream stream);
 
@@ -130,7 +131,7 @@ int dyc_find(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *hit, u
  * (api.cpp:289-303). Clears every copy of the key in the first bucket (pos1, then pos2) that
  * holds it; deleted[i] = 1 iff this call removed it (may be NULL; the reference's `ans` is
  * unused). With auto_resize, downsizes once when count < lower * capacity and capacity >
- * min_capacity (api.cpp:490-505). Synchronises `stream` once (it reads the delete count). */
+ * min_capacity (api.cpp:490-505). Waits once, as dyc_insert (it reads the delete count). */
 int dyc_erase(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream);
 
 /* Replaces hashAPI::resize_up (api.cpp:112-141) + GPU_cuckoo_resize_up (cuckoo.cu:731-758):
