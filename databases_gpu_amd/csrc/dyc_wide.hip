@@ -477,6 +477,7 @@ hipError_t launch_wide_downsize(const uint64_t *ok_, const uint64_t *ov, uint32_
 
 // ------------------------------------------------------------------------------- host side
 using namespace dyc;
+using dyc::host::await_post;
 using dyc::host::choose_down;
 using dyc::host::choose_up;
 using dyc::host::config_subtables;
@@ -497,7 +498,10 @@ struct dyc_wide {
     uint64_t count = 0;
     uint64_t *pk = nullptr, *pv = nullptr;  // pending keys / values [0, pend_n): the stash
     uint64_t pend_cap = 0, pend_n = 0;
-    Counters *ctr = nullptr, *hctr = nullptr;
+    Counters *ctr = nullptr, *hctr = nullptr;  // device block (kCounterBytes); host folded copy
+    unsigned long long *hpost = nullptr;       // pinned: the read-back words (launch_fold_post)
+    unsigned long long post_seq = 0;
+    bool ctr_clean = false;  // the device block is zero and no launch has used it since
     uint8_t *live = nullptr;  // downsize: tile flags of the residual list
     uint64_t live_cap = 0;
     dyc_stats st{};
@@ -520,17 +524,24 @@ namespace {
 constexpr size_t kWideAlign = 2u << 20;
 size_t wide_bytes(uint32_t nb) { return (size_t)nb * kBucketBytes; }  // one array of a subtable
 
+// Callers zero the counters before a launch group and read them after it; a read clears the
+// device block, so the zero is launched only when something else left the block in use.
 int wzero(dyc_wide *t, hipStream_t s)
 {
-    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->ctr), kCounterWords, s));
+    if (!t->ctr_clean) HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->ctr), kCounterWords, s));
+    t->ctr_clean = false;
     return DYC_OK;
 }
 
+// Fold, post and clear the counters, and wait for them (every kernel queued on s before has
+// finished when this returns).
 int wread(dyc_wide *t, hipStream_t s)
 {
-    HIP_TRY(hipMemcpyAsync(t->hctr, t->ctr, kCounterBytes, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    fold_tallies(t->hctr);
+    const unsigned long long seq = ++t->post_seq;
+    HIP_TRY(launch_fold_post(t->ctr, t->hpost, seq, s));
+    HIP_TRY(await_post(t->hpost, seq, s));
+    memcpy(t->hctr, const_cast<const unsigned long long *>(t->hpost) + 1, sizeof(Counters));
+    t->ctr_clean = true;
     return DYC_OK;
 }
 
@@ -826,9 +837,13 @@ int dyc_wide_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_wide *
     desc_init(t->W.D, cfg.subtables);
     int rc = DYC_OK;
     if (hipGetDevice(&t->device) != hipSuccess || hipMalloc(&t->ctr, kCounterBytes) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), kCounterBytes, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(reinterpret_cast<void **>(&t->hctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&t->hpost), kPostWords * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         rc = fail(DYC_ERR_OOM, "counter allocation failed");
+    } else {
+        memset(t->hpost, 0, kPostWords * sizeof(unsigned long long));  // no sequence number yet
     }
     const uint32_t nbs[kMaxSubtables] = {nb, nb, nb, nb, nb, nb};
     if (rc == DYC_OK) rc = winstall_slab(t, nbs, true, nullptr, nullptr);
@@ -856,6 +871,7 @@ int dyc_wide_destroy(dyc_wide *t)
     if (t->pv) (void)hipFree(t->pv);
     if (t->ctr) (void)hipFree(t->ctr);
     if (t->hctr) (void)hipHostFree(t->hctr);
+    if (t->hpost) (void)hipHostFree(t->hpost);
     if (t->live) (void)hipFree(t->live);
     delete t;
     return DYC_OK;
