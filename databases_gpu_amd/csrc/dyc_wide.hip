@@ -294,12 +294,19 @@ __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint
             }
             continue;
         }
-        const uint64_t old_val =
-            __hip_atomic_exchange(vp + slot, c.val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // wait for the value exchange to return (so it is performed) before publishing the key;
-        // the memory clobber also keeps the compiler from moving the publish above the wait
+        // The claimed slot's value belongs to this lane until the key is published: an empty
+        // slot's value is written with an atomic store, a victim's is exchanged (its value goes
+        // on in hand). Agent-scope atomic stores cost less than exchanges (random 8-byte stores
+        // 24 G/s against 18 G/s CAS on MI355X) and stay coherent with the CAS on the same slot.
+        uint64_t old_val = 0;
+        if (expected == 0ull)
+            __hip_atomic_store(vp + slot, c.val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            old_val = __hip_atomic_exchange(vp + slot, c.val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // wait until the value write is performed before publishing the key; the memory clobber
+        // also keeps the compiler from moving the publish above the wait
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        (void)__hip_atomic_exchange(kp + slot, c.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(kp + slot, c.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ++c.kicks;
         c.fresh = true;
         if (expected == 0ull) {
