@@ -24,6 +24,7 @@ using dyc::host::DeviceGuard;
 using dyc::host::fail;
 using dyc::host::initial_buckets;
 using dyc::host::kMaxBuckets;
+using dyc::host::Reclaimer;
 
 namespace {
 thread_local char g_err[512] = "";
@@ -90,6 +91,7 @@ struct dyc_table {
     uint64_t upd_cap = 0;
     uint8_t *live = nullptr;  // downsize: tile flags of the residual list
     uint64_t live_cap = 0;
+    Reclaimer reclaim;  // frees retired subtables off the calling thread
     // Subtables created together (create, load, restore) share one allocation: the GPU's address
     // translation covers one large allocation with far fewer TLB entries than one allocation per
     // subtable, and random bucket access at 2^28 keys misses the per-CU TLB several times per
@@ -171,19 +173,31 @@ int read_counters(dyc_table *t, hipStream_t s)
 constexpr size_t kSlabAlign = 2u << 20;  // subtable regions of a slab start 2 MiB apart
 
 // Release a subtable's storage: its own allocation, or its region of the slab (the slab itself
-// once its last subtable is released). Callers synchronise first.
+// once its last subtable is released). The free itself runs on the reclaimer's thread, after
+// every kernel on the device has finished.
 int release_block(dyc_table *t, uint64_t *p, bool in_slab)
 {
     if (!p) return DYC_OK;
     if (in_slab) {
         if (--t->slab_live == 0) {
-            HIP_TRY(hipFree(t->slab));
+            t->reclaim.defer(t->device, t->slab);
             t->slab = nullptr;
         }
         return DYC_OK;
     }
-    HIP_TRY(hipFree(p));
+    t->reclaim.defer(t->device, p);
     return DYC_OK;
+}
+
+// hipMalloc for a resize; when it fails, wait for the frees still queued and try once more.
+hipError_t resize_malloc(dyc_table *t, void **p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) == hipSuccess) return hipSuccess;
+    (void)hipGetLastError();
+    t->reclaim.drain();
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
 }
 
 int release_sub(dyc_table *t, uint32_t i)
@@ -266,10 +280,8 @@ int do_resize_up(dyc_table *t, hipStream_t s)
     if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^29 buckets", sub);
     uint64_t *dst = nullptr;
     const size_t bytes = (size_t)old_nb * 2 * kBucketBytes;
-    if (hipMalloc(&dst, bytes) != hipSuccess) {
-        (void)hipGetLastError();
+    if (resize_malloc(t, reinterpret_cast<void **>(&dst), bytes) != hipSuccess)
         return fail(DYC_ERR_OOM, "hipMalloc of %zu bytes for upsize failed", bytes);
-    }
     int rc = zero_counters(t, s);
     if (rc) return rc;
     HIP_TRY(launch_upsize(t->D.sub[sub], old_nb, dst, sub, t->ctr, s));
@@ -334,10 +346,8 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         return fail(DYC_ERR_ODD_BUCKETS, "downsize of subtable %u refused: %u buckets is odd", sub, old_nb);
     const uint32_t new_nb = old_nb / 2;
     uint64_t *dst = nullptr;
-    if (hipMalloc(&dst, (size_t)new_nb * kBucketBytes) != hipSuccess) {
-        (void)hipGetLastError();
+    if (resize_malloc(t, reinterpret_cast<void **>(&dst), (size_t)new_nb * kBucketBytes) != hipSuccess)
         return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
-    }
     if (res_tiles(new_nb) > t->live_cap) {  // the residual list's tile flags, grown as needed
         if (t->live) (void)hipFree(t->live);
         t->live = nullptr;
@@ -391,7 +401,7 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         (void)release_block(t, old, old_in_slab);
         return rc;
     }
-    rc = release_block(t, old, old_in_slab);  // read_counters synchronised the stream
+    rc = release_block(t, old, old_in_slab);  // freed once the device is idle
     if (rc) return rc;
     if (t->pend_n > pend_before) return place_pending(t, s, pend_before);
     return DYC_OK;
@@ -738,6 +748,7 @@ int dyc_destroy(dyc_table *t)
     DeviceGuard dg(t->device);
     (void)hipDeviceSynchronize();
     for (uint32_t i = 0; i < t->D.d; ++i) (void)release_sub(t, i);
+    t->reclaim.drain();  // every subtable is freed when this returns
     if (t->pend) (void)hipFree(t->pend);
     if (t->ctr) (void)hipFree(t->ctr);
     if (t->part) (void)hipFree(t->part);

@@ -492,6 +492,7 @@ using dyc::host::DeviceGuard;
 using dyc::host::fail;
 using dyc::host::initial_buckets;
 using dyc::host::kMaxBuckets;
+using dyc::host::Reclaimer;
 
 struct dyc_wide {
     dyc_config cfg;
@@ -511,6 +512,7 @@ struct dyc_wide {
     bool ctr_clean = false;  // the device block is zero and no launch has used it since
     uint8_t *live = nullptr;  // downsize: tile flags of the residual list
     uint64_t live_cap = 0;
+    Reclaimer reclaim;  // frees retired subtables off the calling thread
     dyc_stats st{};
 
     uint64_t capacity() const
@@ -553,16 +555,27 @@ int wread(dyc_wide *t, hipStream_t s)
 }
 
 // Release a subtable's storage: its own allocation `own`, or (own null) its share of the slab.
-// Synchronise first.
+// The free runs on the reclaimer's thread, after every kernel on the device has finished.
 int wrelease_block(dyc_wide *t, uint8_t *own)
 {
     if (own) {
-        HIP_TRY(hipFree(own));
+        t->reclaim.defer(t->device, own);
     } else if (t->slab && --t->slab_live == 0) {
-        HIP_TRY(hipFree(t->slab));
+        t->reclaim.defer(t->device, t->slab);
         t->slab = nullptr;
     }
     return DYC_OK;
+}
+
+// hipMalloc for a resize; when it fails, wait for the frees still queued and try once more.
+hipError_t wresize_malloc(dyc_wide *t, uint8_t **p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) == hipSuccess) return hipSuccess;
+    (void)hipGetLastError();
+    t->reclaim.drain();
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
 }
 
 int wrelease(dyc_wide *t, uint32_t i)
@@ -649,10 +662,8 @@ int wresize_up(dyc_wide *t, hipStream_t s)
     const uint32_t old_nb = t->W.D.nb[sub];
     if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^29 buckets", sub);
     uint8_t *dst = nullptr;
-    if (hipMalloc(&dst, 2 * wide_bytes(old_nb * 2)) != hipSuccess) {
-        (void)hipGetLastError();
+    if (wresize_malloc(t, &dst, 2 * wide_bytes(old_nb * 2)) != hipSuccess)
         return fail(DYC_ERR_OOM, "hipMalloc for upsize failed");
-    }
     int rc = wzero(t, s);
     if (rc) return rc;
     HIP_TRY(launch_wide_upsize(t->W.D.sub[sub], t->W.val[sub], old_nb, reinterpret_cast<uint64_t *>(dst),
@@ -719,10 +730,8 @@ int wresize_down(dyc_wide *t, hipStream_t s)
         return fail(DYC_ERR_ODD_BUCKETS, "downsize of subtable %u refused: %u buckets is odd", sub, old_nb);
     const uint32_t new_nb = old_nb / 2;
     uint8_t *dst = nullptr;
-    if (hipMalloc(&dst, 2 * wide_bytes(new_nb)) != hipSuccess) {
-        (void)hipGetLastError();
+    if (wresize_malloc(t, &dst, 2 * wide_bytes(new_nb)) != hipSuccess)
         return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
-    }
     if (res_tiles(new_nb) > t->live_cap) {  // the residual list's tile flags, grown as needed
         if (t->live) (void)hipFree(t->live);
         t->live = nullptr;
@@ -779,7 +788,7 @@ int wresize_down(dyc_wide *t, hipStream_t s)
         (void)wrelease_block(t, old_own);
         return rc;
     }
-    rc = wrelease_block(t, old_own);  // wread synchronised the stream
+    rc = wrelease_block(t, old_own);  // freed once the device is idle
     if (rc) return rc;
     if (t->pend_n > pend_before) return wplace_pending(t, s, pend_before);
     return DYC_OK;
@@ -874,6 +883,7 @@ int dyc_wide_destroy(dyc_wide *t)
     DeviceGuard dg(t->device);
     (void)hipDeviceSynchronize();
     for (uint32_t i = 0; i < t->W.D.d; ++i) (void)wrelease(t, i);
+    t->reclaim.drain();  // every subtable is freed when this returns
     if (t->pk) (void)hipFree(t->pk);
     if (t->pv) (void)hipFree(t->pv);
     if (t->ctr) (void)hipFree(t->ctr);
