@@ -408,6 +408,30 @@ def test_downsize_with_residuals_preserves_everything(D):
     t.close()
 
 
+def test_destroy_returns_every_retired_subtable(D):
+    """Resizes hand the old subtables to the table's reclaimer thread; after close (dyc_destroy)
+    the device's free memory is back where it was, so every retired block was freed."""
+    banner("retired subtables freed by destroy")
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free0 = torch.cuda.mem_get_info()[0]
+    t = D.DyCuckoo(1 << 24)  # 4 subtables of 33.6 MB, one slab
+    keys = unique_keys(1 << 20, 91)
+    t.insert(torch.from_numpy(keys.view(np.int32)).cuda(), torch.from_numpy(vals_of(keys).view(np.int32)).cuda())
+    for _ in range(5):
+        t.resize_up()
+    for _ in range(3):
+        assert t.resize_down()
+    gv, gh = gpu_find(t, keys)
+    assert gh.all() and np.array_equal(gv, vals_of(keys))
+    t.close()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free1 = torch.cuda.mem_get_info()[0]
+    # the resizes retired the slab (134 MB) and about 270 MB of later subtables
+    assert free1 >= free0 - (16 << 20), (free0, free1)
+
+
 def test_downsize_refuses_odd_bucket_count(D):
     banner("downsize odd refusal")
     t = D.DyCuckoo(1000)
