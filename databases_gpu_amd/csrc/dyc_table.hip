@@ -466,7 +466,87 @@ int check_config(const dyc_config &cfg)
 
 }  // namespace
 
+namespace {
+
+// Splits a large host memcpy (pageable <-> pinned staging) over worker threads that live as long
+// as the staging buffers: starting threads for every copy cost a measurable share of each
+// pipelined chunk. copy() returns when every part is copied.
+class CopyPool {
+  public:
+    explicit CopyPool(unsigned workers)
+    {
+        for (unsigned i = 0; i < workers; ++i) th_.emplace_back([this] { loop(); });
+    }
+    CopyPool(const CopyPool &) = delete;
+    CopyPool &operator=(const CopyPool &) = delete;
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+
+    void copy(void *dst, const void *src, size_t bytes)
+    {
+        constexpr size_t kSplit = 4u << 20;
+        const size_t nparts = th_.size() + 1;
+        if (bytes < kSplit || nparts == 1) {
+            memcpy(dst, src, bytes);
+            return;
+        }
+        const size_t per = ((bytes + nparts - 1) / nparts + 63) & ~(size_t)63;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            parts_.clear();
+            for (size_t lo = per; lo < bytes; lo += per)
+                parts_.push_back(Part{static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo,
+                                      std::min(per, bytes - lo)});
+            next_ = 0;
+            left_ = parts_.size();
+        }
+        cv_.notify_all();
+        memcpy(dst, src, std::min(per, bytes));  // the first part on the calling thread
+        std::unique_lock<std::mutex> lk(m_);
+        done_cv_.wait(lk, [this] { return left_ == 0; });
+        parts_.clear();
+    }
+
+  private:
+    struct Part {
+        char *dst;
+        const char *src;
+        size_t len;
+    };
+
+    void loop()
+    {
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_.wait(lk, [this] { return stop_ || next_ < parts_.size(); });
+            if (stop_) return;
+            const Part p = parts_[next_++];
+            lk.unlock();
+            memcpy(p.dst, p.src, p.len);
+            lk.lock();
+            if (--left_ == 0) done_cv_.notify_one();
+        }
+    }
+
+    std::mutex m_;
+    std::condition_variable cv_, done_cv_;
+    std::vector<Part> parts_;
+    size_t next_ = 0, left_ = 0;
+    bool stop_ = false;
+    std::vector<std::thread> th_;
+};
+
+}  // namespace
+
 struct DycHostStage {
+    CopyPool *pool = nullptr;           // staging copies
     uint32_t *hk[2] = {}, *hv[2] = {};  // pinned: keys; values in (insert) or out (find)
     uint8_t *hf[2] = {};                // pinned: flags out
     uint32_t *dk[2] = {}, *dv[2] = {};
@@ -496,6 +576,7 @@ void free_stage(dyc_table *t)
     }
     if (S->h2d) (void)hipStreamDestroy(S->h2d);
     if (S->d2h) (void)hipStreamDestroy(S->d2h);
+    delete S->pool;
     delete S;
     t->stage = nullptr;
 }
@@ -505,6 +586,7 @@ int get_stage(dyc_table *t, DycHostStage **out)
     if (!t->stage) {
         DycHostStage *S = new DycHostStage();
         t->stage = S;
+        S->pool = new CopyPool(std::min(8u, std::max(1u, std::thread::hardware_concurrency())) - 1);
         bool ok = hipStreamCreateWithFlags(&S->h2d, hipStreamNonBlocking) == hipSuccess &&
                   hipStreamCreateWithFlags(&S->d2h, hipStreamNonBlocking) == hipSuccess;
         for (int b = 0; b < 2 && ok; ++b) {
@@ -525,26 +607,6 @@ int get_stage(dyc_table *t, DycHostStage **out)
     }
     *out = t->stage;
     return DYC_OK;
-}
-
-// memcpy of large host ranges on up to 8 threads (pageable <-> pinned staging)
-void host_copy(void *dst, const void *src, size_t bytes)
-{
-    constexpr size_t kSplit = 4u << 20;
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const unsigned nt = bytes < kSplit ? 1u : std::min(8u, hw);
-    if (nt == 1) {
-        memcpy(dst, src, bytes);
-        return;
-    }
-    const size_t per = ((bytes + nt - 1) / nt + 63) & ~(size_t)63;
-    std::vector<std::thread> th;
-    for (unsigned i = 1; i < nt && i * per < bytes; ++i) {
-        const size_t lo = i * per, len = std::min(per, bytes - lo);
-        th.emplace_back([=] { memcpy(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo, len); });
-    }
-    memcpy(dst, src, std::min(per, bytes));
-    for (auto &x : th) x.join();
 }
 
 // ---- checkpoint file: a 128-byte header, the d subtable images, then the pending entries,
@@ -1077,8 +1139,8 @@ int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, ui
         const int b = c & 1;
         const uint64_t m = std::min(kStageKeys, n - off);
         if (used[b]) HIP_TRY(hipEventSynchronize(S->done[b]));  // chunk c - 2 no longer reads buffer b
-        host_copy(S->hk[b], keys + off, m * 4);
-        host_copy(S->hv[b], vals + off, m * 4);
+        S->pool->copy(S->hk[b], keys + off, m * 4);
+        S->pool->copy(S->hv[b], vals + off, m * 4);
         HIP_TRY(hipMemcpyAsync(S->dk[b], S->hk[b], m * 4, hipMemcpyHostToDevice, S->h2d));
         HIP_TRY(hipMemcpyAsync(S->dv[b], S->hv[b], m * 4, hipMemcpyHostToDevice, S->h2d));
         HIP_TRY(hipEventRecord(S->in_ready[b], S->h2d));
@@ -1114,8 +1176,8 @@ int probe_host(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *flag
     auto drain = [&](int b) -> int {  // results of the chunk that used buffer b, to the caller
         if (!used[b]) return DYC_OK;
         HIP_TRY(hipEventSynchronize(S->out_ready[b]));
-        if (vals && !erase) host_copy(vals + prev_off[b], S->hv[b], prev_m[b] * 4);
-        if (flags) host_copy(flags + prev_off[b], S->hf[b], prev_m[b]);
+        if (vals && !erase) S->pool->copy(vals + prev_off[b], S->hv[b], prev_m[b] * 4);
+        if (flags) S->pool->copy(flags + prev_off[b], S->hf[b], prev_m[b]);
         used[b] = false;
         return DYC_OK;
     };
@@ -1125,7 +1187,7 @@ int probe_host(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *flag
         const uint64_t m = std::min(kStageKeys, n - off);
         rc = drain(b);
         if (rc) return rc;
-        host_copy(S->hk[b], keys + off, m * 4);
+        S->pool->copy(S->hk[b], keys + off, m * 4);
         HIP_TRY(hipMemcpyAsync(S->dk[b], S->hk[b], m * 4, hipMemcpyHostToDevice, S->h2d));
         HIP_TRY(hipEventRecord(S->in_ready[b], S->h2d));
         HIP_TRY(hipStreamWaitEvent(s, S->in_ready[b], 0));
