@@ -959,9 +959,9 @@ __device__ inline unsigned long long serve_probe_item(const Desc &D, const Serve
 
 // One item of inserts; the wave's groups are all idle on return.
 template <int G>
-__device__ inline void serve_insert_item(const Desc &D, const ServeItem &item, Carry<kSlots / G> &c,
-                                         const InsertPolicy &P, Counters *C, const PendSink &sink, InsertTally &tally,
-                                         unsigned long long *ins_total)
+__device__ inline unsigned long long serve_insert_item(const Desc &D, const ServeItem &item, Carry<kSlots / G> &c,
+                                                       const InsertPolicy &P, Counters *C, const PendSink &sink,
+                                                       InsertTally &tally)
 {
     uint64_t next = 0;
     const uint64_t end = item.len;
@@ -973,8 +973,29 @@ __device__ inline void serve_insert_item(const Desc &D, const ServeItem &item, C
         if (c.active) kick_step<G>(D, c, P, C, sink, item.table, tally);
     }
     const unsigned long long rej = wave_sum(tally.rejected - rej0);
-    if (__lane_id() == 0 && item.len > rej) atomicAdd(&ins_total[item.table], item.len - rej);
+    return item.len > rej ? item.len - rej : 0;  // entries the item took in (wave-uniform)
 }
+
+// A wave's running total for one table, added to that table's word when the wave moves on to
+// another table and at the end. Items of one table come in runs, and an atomic per item put
+// every item of a busy table on one address (atomics to one address are serialized).
+struct WaveTotal {
+    uint32_t table = ~0u;
+    unsigned long long n = 0;
+    __device__ void add(unsigned long long *totals, uint32_t t, unsigned long long v)  // v wave-uniform
+    {
+        if (t != table) {
+            flush(totals);
+            table = t;
+        }
+        n += v;
+    }
+    __device__ void flush(unsigned long long *totals)
+    {
+        if (__lane_id() == 0 && table != ~0u && n) atomicAdd(&totals[table], n);
+        n = 0;
+    }
+};
 
 
 template <int G, bool ERASE>
@@ -984,16 +1005,15 @@ __global__ __launch_bounds__(256) void serve_probe_kernel(const Desc *__restrict
 {
     const uint32_t waves = gridDim.x * (blockDim.x / kWave);
     const uint64_t np = pool_size(pool);
+    WaveTotal del;
     for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
          it += waves) {
         const ServeItem item = items[it];
         const Desc D = dir[item.table];
         unsigned long long nd = serve_probe_item<G, ERASE>(D, item, pool, np);
-        if constexpr (ERASE) {
-            nd = wave_sum(nd);
-            if (__lane_id() == 0 && nd) atomicAdd(&del_total[item.table], nd);
-        }
+        if constexpr (ERASE) del.add(del_total, item.table, wave_sum(nd));
     }
+    if constexpr (ERASE) del.flush(del_total);
 }
 
 template <int G>
@@ -1005,12 +1025,14 @@ __global__ __launch_bounds__(256) void serve_insert_kernel(const Desc *__restric
     Carry<kSlots / G> c;
     c.active = false;
     InsertTally tally;
+    WaveTotal ins;
     const uint32_t waves = gridDim.x * (blockDim.x / kWave);
     for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
          it += waves) {
         const ServeItem item = items[it];
-        serve_insert_item<G>(dir[item.table], item, c, P, C, sink, tally, ins_total);
+        ins.add(ins_total, item.table, serve_insert_item<G>(dir[item.table], item, c, P, C, sink, tally));
     }
+    ins.flush(ins_total);
     wave_tally(C, &C->kicks, tally.kicks);
     wave_tally(C, &C->rejected, tally.rejected);
     wave_tally(C, &C->retries, tally.retries);
@@ -1038,6 +1060,7 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
     Carry<kSlots / GI> c;
     c.active = false;
     InsertTally tally;
+    WaveTotal ins, del;
     unsigned long long stalls = 0;
     while (true) {
         uint32_t it = 0;
@@ -1063,7 +1086,7 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
         }
         const Desc D = dir[ti.item.table];
         if (ti.op == 0) {
-            serve_insert_item<GI>(D, ti.item, c, P, C, sink, tally, ins_total);
+            ins.add(ins_total, ti.item.table, serve_insert_item<GI>(D, ti.item, c, P, C, sink, tally));
         } else {
             // an item that waited reads what earlier stages of this launch wrote: coherently;
             // a table's first stage reads what earlier launches wrote: through the caches
@@ -1075,10 +1098,7 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
             else
                 nd = ti.wait ? serve_probe_item<GF, true, true>(D, ti.item, pool, np)
                              : serve_probe_item<GF, true, false>(D, ti.item, pool, np);
-            if (ti.op == 2) {
-                nd = wave_sum(nd);
-                if (__lane_id() == 0 && nd) atomicAdd(&del_total[ti.item.table], nd);
-            }
+            if (ti.op == 2) del.add(del_total, ti.item.table, wave_sum(nd));
         }
         if (__lane_id() == 0) {
             const uint32_t before =
@@ -1091,6 +1111,8 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
             }
         }
     }
+    ins.flush(ins_total);
+    del.flush(del_total);
     wave_tally(C, &C->kicks, tally.kicks);
     wave_tally(C, &C->rejected, tally.rejected);
     wave_tally(C, &C->retries, tally.retries);
