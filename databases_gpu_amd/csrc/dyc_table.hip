@@ -374,6 +374,9 @@ int do_resize_down(dyc_table *t, hipStream_t s)
     }
     if (!rc) rc = read_counters(t, s);
     if (rc) {
+        // a launch refused changes nothing; a read that fails means the stream faulted, and the
+        // compaction may already have rewritten the old upper half: the table is then lost
+        // with the device context
         (void)hipFree(dst);
         return rc;
     }
