@@ -42,7 +42,7 @@ EXPORTS = (
     "dyc_group_clear",
     "dyc_server_config_init", "dyc_server_create", "dyc_server_destroy", "dyc_server_submit", "dyc_server_poll",
     "dyc_server_drain", "dyc_server_replay", "dyc_server_get_stats", "dyc_server_table_stats", "dyc_server_dump",
-    "dyc_wide_create", "dyc_wide_destroy", "dyc_wide_insert", "dyc_wide_find", "dyc_wide_erase",
+    "dyc_wide_create", "dyc_wide_destroy", "dyc_wide_insert", "dyc_wide_upsert", "dyc_wide_find", "dyc_wide_erase",
     "dyc_wide_resize_up", "dyc_wide_resize_down", "dyc_wide_get_stats", "dyc_wide_clear", "dyc_wide_dump",
     "dyc_wide_load", "dyc_wide_pending",
 )
@@ -227,6 +227,7 @@ def load(require: bool = True):
         "dyc_wide_create": (ci, [ctypes.POINTER(Config), u64, ctypes.POINTER(vp)]),
         "dyc_wide_destroy": (ci, [vp]),
         "dyc_wide_insert": (ci, [vp, vp, vp, u64, vp]),
+        "dyc_wide_upsert": (ci, [vp, vp, vp, u64, vp, vp]),
         "dyc_wide_find": (ci, [vp, vp, vp, vp, u64, vp]),
         "dyc_wide_erase": (ci, [vp, vp, vp, u64, vp]),
         "dyc_wide_resize_up": (ci, [vp, vp]),

@@ -48,10 +48,10 @@ struct WideSink {
     uint64_t cap;
 };
 
-// The pending entries searched after pos2 by finds and erases (n = 0: none).
+// The pending entries searched after pos2 by finds, erases and updates (n = 0: none).
 struct WideStash {
     uint64_t *keys;
-    const uint64_t *vals;
+    uint64_t *vals;
     uint64_t n;
 };
 
@@ -79,17 +79,22 @@ __device__ inline void store2(uint64_t *p, uint64_t a, uint64_t b)
     *reinterpret_cast<uint4 *>(p) = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
 }
 
-// ------------------------------------------------------------------------------ find / erase
+// --------------------------------------------------------------------- find / erase / update
 // A wave owns 64 consecutive keys; groups of 8 lanes probe 8 keys each, lane s reading keys
 // 2s, 2s+1 of a bucket (one 128-byte line per group). The first bucket (pos1, then pos2) that
-// holds a key answers: find takes the first matching slot's value (read by the key's own lane
-// from the value line), erase CASes every matching key of that bucket to 0. Keys neither
-// bucket holds are then looked up in the stash.
-template <bool ERASE>
+// holds a key answers: find (MODE 0) takes the first matching slot's value (read by the key's
+// own lane from the value line), erase (MODE 1) CASes every matching key of that bucket to 0,
+// update (MODE 2) writes vals[i] to the value of every matching slot of that bucket. Keys
+// neither bucket holds are then looked up in the stash (erase and update act on every copy
+// there). Update: keys of one launch are distinct, so no two lanes write one value; the key word
+// is untouched, and the value is an agent-scope atomic store.
+template <int MODE>
 __global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint64_t *__restrict__ keys,
                                                          uint64_t *__restrict__ vals, uint8_t *__restrict__ flags,
                                                          uint64_t n, Counters *__restrict__ C, WideStash stash)
 {
+    constexpr bool ERASE = MODE == 1;
+    constexpr bool UPDATE = MODE == 2;
     constexpr int G = 8;
     const int lane = __lane_id();
     const int g = lane / G, s = lane % G;
@@ -100,6 +105,7 @@ __global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint6
         const uint64_t i = base + lane;
         const bool inb = i < n;
         const uint64_t k = inb ? keys[i] : 0ull;
+        const uint64_t nv = (UPDATE && inb) ? vals[i] : 0ull;  // update: the new value
         const uint32_t f = fold64(k);
         const uint32_t p = pair_of(W.D, f);
         const uint32_t t1 = pos1(W.D, p), t2 = pos2(W.D, p);
@@ -108,6 +114,7 @@ __global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint6
 
         uint64_t e[G][2];
         uint64_t kk_key[G];
+        uint64_t kk_new[UPDATE ? G : 1];
         bool kk_valid[G];
         uint32_t kk_l[G];
         uint32_t my_flag = 0, my_loc = 0, my_slot = 0, done = 0;
@@ -120,6 +127,7 @@ __global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint6
                 if (phase == 0) {
                     kk_key[kk] = shfl64(k, src);
                     kk_valid[kk] = !wide_reserved(kk_key[kk]);
+                    if constexpr (UPDATE) kk_new[kk] = shfl64(nv, src);
                 }
                 kk_l[kk] = __shfl(lp, src);
                 e[kk][0] = e[kk][1] = 0;
@@ -142,6 +150,13 @@ __global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint6
                         }
                     const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & 0xffu);
                     if (kk == s) my_flag = okm != 0;
+                } else if constexpr (UPDATE) {
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        if (kk_valid[kk] && e[kk][j] == kk_key[kk])
+                            __hip_atomic_store(wide_vals_at(W, kk_l[kk]) + s * 2 + j, kk_new[UPDATE ? kk : 0],
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (kk == s) my_flag = 1;
                 } else {
                     const int w = __builtin_ctz(gm);
                     const int fj = __shfl(first, g * G + w);
@@ -154,7 +169,7 @@ __global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint6
             }
         }
         uint64_t my_val = 0;
-        if (!ERASE && my_flag) my_val = wide_vals_at(W, my_loc)[my_slot];
+        if (MODE == 0 && my_flag) my_val = wide_vals_at(W, my_loc)[my_slot];
         if (inb && !my_flag && stash.n && !wide_reserved(k)) {  // the stash counts as one more bucket
             for (uint64_t j = 0; j < stash.n; ++j) {
                 const uint64_t sk = __hip_atomic_load(stash.keys + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -162,6 +177,9 @@ __global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint6
                 if constexpr (ERASE) {
                     uint64_t exp = sk;
                     my_flag |= (uint32_t)cas_slot(stash.keys + j, exp, 0ull);
+                } else if constexpr (UPDATE) {
+                    __hip_atomic_store(stash.vals + j, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    my_flag = 1;
                 } else {
                     my_val = stash.vals[j];
                     my_flag = 1;
@@ -170,12 +188,13 @@ __global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint6
             }
         }
         if (inb) {
-            if (!ERASE && vals) vals[i] = my_val;
+            if (MODE == 0 && vals) vals[i] = my_val;
             if (flags) flags[i] = (uint8_t)my_flag;
         }
         ndel += my_flag;
     }
     if constexpr (ERASE) wave_tally(C, &C->deleted, ndel);
+    if constexpr (UPDATE) wave_tally(C, &C->updated, ndel);
 }
 
 // ---------------------------------------------------------------------------------- insert
@@ -203,11 +222,13 @@ __device__ inline void wide_give_up(WideCarry &c, const WideSink &sink, Counters
     c.active = false;
 }
 
+// skip (optional): a keys insert leaves out key i where skip[i] != 0 (an upsert updated it in
+// place); an entries insert reads it as the tile flags of a slot-addressed residual list.
 template <bool FROM_ENTRIES>
 __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint64_t *__restrict__ keys,
                                                           const uint64_t *__restrict__ vals, uint64_t n,
                                                           uint32_t pre_fixed, InsertPolicy P, Counters *__restrict__ C,
-                                                          WideSink sink, const uint8_t *__restrict__ live)
+                                                          WideSink sink, const uint8_t *__restrict__ skip)
 {
     const Desc &D = W.D;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -218,7 +239,11 @@ __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint
     c.active = false;
     for (;;) {
         while (!c.active && next < n) {  // take the next entry
-            if (FROM_ENTRIES && live && !live[next / kResTileEntries]) {  // a residual tile with none
+            if (FROM_ENTRIES && skip && !skip[next / kResTileEntries]) {  // a residual tile with none
+                next += stride;
+                continue;
+            }
+            if (!FROM_ENTRIES && skip && skip[next]) {  // updated in place
                 next += stride;
                 continue;
             }
@@ -440,23 +465,35 @@ hipError_t launch_wide_probe(const WideDesc &W, const uint64_t *keys, uint64_t *
     if (n == 0) return hipSuccess;
     const int grid = grid_for((n + kWave - 1) / kWave, sh);
     if (erase)
-        wide_probe_kernel<true><<<grid, sh.block_threads, 0, st>>>(W, keys, nullptr, flags, n, C, stash);
+        wide_probe_kernel<1><<<grid, sh.block_threads, 0, st>>>(W, keys, nullptr, flags, n, C, stash);
     else
-        wide_probe_kernel<false><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, flags, n, C, stash);
+        wide_probe_kernel<0><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, flags, n, C, stash);
+    return hipGetLastError();
+}
+
+// Writes vals[i] to every copy of keys[i] that pos1, pos2 (the first that holds it) and the stash
+// hold; updated[i] = 1 where one was found. Keys must be distinct within the call.
+hipError_t launch_wide_update(const WideDesc &W, const uint64_t *keys, const uint64_t *vals, uint8_t *updated,
+                              uint64_t n, LaunchShape sh, Counters *C, const WideStash &stash, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = grid_for((n + kWave - 1) / kWave, sh);
+    wide_probe_kernel<2><<<grid, sh.block_threads, 0, st>>>(W, keys, const_cast<uint64_t *>(vals), updated, n, C,
+                                                            stash);
     return hipGetLastError();
 }
 
 hipError_t launch_wide_insert(const WideDesc &W, const uint64_t *keys, const uint64_t *vals, uint64_t n,
                               uint32_t pre_fixed, bool from_entries, InsertPolicy pol, LaunchShape sh, Counters *C,
-                              const WideSink &sink, hipStream_t st, const uint8_t *live = nullptr)
+                              const WideSink &sink, hipStream_t st, const uint8_t *skip = nullptr)
 {
     if (n == 0) return hipSuccess;
     const int grid = grid_for((n + kWave - 1) / kWave, sh, kWideInsertBlocks);
     if (from_entries)
-        wide_insert_kernel<true><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, pre_fixed, pol, C, sink, live);
+        wide_insert_kernel<true><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, pre_fixed, pol, C, sink, skip);
     else
         wide_insert_kernel<false><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, kNoTable, pol, C, sink,
-                                                                     nullptr);
+                                                                     skip);
     return hipGetLastError();
 }
 
@@ -512,6 +549,8 @@ struct dyc_wide {
     bool ctr_clean = false;  // the device block is zero and no launch has used it since
     uint8_t *live = nullptr;  // downsize: tile flags of the residual list
     uint64_t live_cap = 0;
+    uint8_t *upd = nullptr;  // upsert: per-key "updated in place" flags
+    uint64_t upd_cap = 0;
     Reclaimer reclaim;  // frees retired subtables off the calling thread
     dyc_stats st{};
 
@@ -890,6 +929,7 @@ int dyc_wide_destroy(dyc_wide *t)
     if (t->hctr) (void)hipHostFree(t->hctr);
     if (t->hpost) (void)hipHostFree(t->hpost);
     if (t->live) (void)hipFree(t->live);
+    if (t->upd) (void)hipFree(t->upd);
     delete t;
     return DYC_OK;
 }
@@ -922,6 +962,58 @@ int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     const uint64_t rejected = t->hctr->rejected;
     t->st.rejected += rejected;
     t->count += n - rejected;
+    rc = wabsorb(t, room);
+    t->st.pending = t->pend_n;
+    if (rc) return rc;
+    if (t->pend_n > pend_before) return wplace_pending(t, s, pend_before);
+    return DYC_OK;
+}
+
+int dyc_wide_upsert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, uint8_t *updated,
+                    dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    t->st.last_kicks = 0;
+    if (n == 0) return DYC_OK;
+    const uint64_t pend_before = t->pend_n;
+    int rc;
+    if (t->cfg.auto_resize) {  // sized as if every key were new
+        while ((double)(t->count + n) > t->cfg.upper_bound * (double)t->capacity()) {
+            rc = wresize_up(t, s);
+            if (rc) return rc;
+        }
+    }
+    uint8_t *flags = updated;
+    if (!flags) {
+        if (n > t->upd_cap) {
+            HIP_TRY(hipStreamSynchronize(s));
+            if (t->upd) HIP_TRY(hipFree(t->upd));
+            t->upd = nullptr;
+            t->upd_cap = 0;
+            if (hipMalloc(&t->upd, n) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(DYC_ERR_OOM, "hipMalloc of %llu upsert flags failed", (unsigned long long)n);
+            }
+            t->upd_cap = n;
+        }
+        flags = t->upd;
+    }
+    rc = wreserve_pending(t, std::max<uint64_t>(65536, n / 16), s);
+    if (rc) return rc;
+    const uint64_t room = t->pend_cap - t->pend_n;
+    rc = wzero(t, s);
+    if (rc) return rc;
+    HIP_TRY(launch_wide_update(t->W, keys, vals, flags, n, t->shape(), t->ctr, t->stash(), s));
+    HIP_TRY(launch_wide_insert(t->W, keys, vals, n, kNoTable, false, t->policy(), t->insert_shape(), t->ctr,
+                               t->sink_at(t->pend_n), s, flags));
+    rc = wread(t, s);
+    if (rc) return rc;
+    const uint64_t rejected = t->hctr->rejected, upd = t->hctr->updated;
+    t->st.rejected += rejected;
+    t->count += n - rejected - upd;
     rc = wabsorb(t, room);
     t->st.pending = t->pend_n;
     if (rc) return rc;

@@ -331,6 +331,19 @@ class DyCuckooWide:
             check(self.lib.dyc_wide_insert(self.h, k.data_ptr(), v.data_ptr(), k.numel(), _stream_ptr(stream)),
                   "dyc_wide_insert")
 
+    def upsert(self, keys, vals, stream=None) -> torch.Tensor:
+        """Insert, or replace the value of keys already stored (keys distinct within the call).
+        Returns per-key flags: 1 where an existing entry was updated."""
+        k = _as_u64_tensor(keys, self.device)
+        v = _as_u64_tensor(vals, self.device)
+        if k.numel() != v.numel():
+            raise ValueError("keys and values differ in length")
+        updated = torch.empty(k.numel(), dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_upsert(self.h, k.data_ptr(), v.data_ptr(), k.numel(), updated.data_ptr(),
+                                           _stream_ptr(stream)), "dyc_wide_upsert")
+        return updated
+
     def find(self, keys, vals=None, hit=None, stream=None):
         """Returns (values int64-as-uint64, hit uint8)."""
         k = _as_u64_tensor(keys, self.device)

@@ -377,6 +377,13 @@ typedef struct dyc_wide dyc_wide;   /* opaque */
 int dyc_wide_create(const dyc_config *cfg, uint64_t capacity_hint, dyc_wide **out);
 int dyc_wide_destroy(dyc_wide *t);
 int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, dyc_stream stream);
+/* Insert with update for wide keys, with dyc_upsert's meaning (the paper's insert,
+ * tkde-revision/secs/cuckoo.tex:127-129): a stored key gets vals[i] in every copy held by the
+ * first bucket (pos1, then pos2) that holds it, or else in every stash copy, and updated[i] = 1
+ * (updated may be NULL); every other key is inserted. Keys within one call must be distinct.
+ * Waits once, as dyc_wide_insert. */
+int dyc_wide_upsert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, uint8_t *updated,
+                    dyc_stream stream);
 /* vals[i] = the value of keys[i] or 0, hit[i] = 1/0 (either may be NULL). Asynchronous. */
 int dyc_wide_find(dyc_wide *t, const uint64_t *keys, uint64_t *vals, uint8_t *hit, uint64_t n, dyc_stream stream);
 int dyc_wide_erase(dyc_wide *t, const uint64_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream);

@@ -193,3 +193,63 @@ def test_wide_fill_bound_stream_matches_oracle_sizes(D):
     assert np.array_equal(gv[: rest.size], vals_of(rest))
     assert t.stats()["count"] == tb.count == rest.size
     t.close()
+
+
+@pytest.mark.parametrize("d", (2, 4))
+def test_wide_upsert_updates_stored_keys_and_inserts_the_rest(D, d):
+    """Upsert against a dict model: stored keys take the new value in place (no second copy),
+    absent keys are inserted, the two reserved keys are rejected; counts, the placement
+    invariant and finds agree."""
+    n = 1 << 19
+    banner(f"wide upsert 2^19 over 2^19, subtables={d}")
+    keys = keys64(2 * n, 900 + d)
+    old, fresh = keys[:n], keys[n:]
+    t = D.DyCuckooWide(int(np.ceil(1.6 * n / 0.85)), subtables=d)
+    t.insert(cu(old), cu(vals_of(old)))
+    up_keys = np.concatenate([old[::2], fresh[: n // 2], np.array([0, 2**64 - 1], dtype=np.uint64)])
+    rng = np.random.default_rng(910 + d)
+    rng.shuffle(up_keys)
+    up_vals = rng.integers(0, 2**64 - 1, size=up_keys.size, dtype=np.uint64)
+    flags = t.upsert(cu(up_keys), cu(up_vals)).cpu().numpy()
+    model = dict(zip(old.tolist(), vals_of(old).tolist()))
+    for k, v in zip(up_keys.tolist(), up_vals.tolist()):
+        if k not in (0, 2**64 - 1):
+            model[k] = v
+    assert np.array_equal(flags.astype(bool), np.isin(up_keys, old))
+    st = t.stats()
+    assert st["count"] == len(model) and st["pending"] == 0 and st["rejected"] == 2
+    ref = oracle_of(t)
+    assert ref.check_invariant() == 0
+    mk = np.array(list(model.keys()), dtype=np.uint64)
+    mv = np.array(list(model.values()), dtype=np.uint64)
+    order = np.argsort(mk)
+    k, v = ref.entries()  # one copy per key
+    assert np.array_equal(k, mk[order]) and np.array_equal(v, mv[order])
+    gv, gh = gfind(t, mk)
+    assert gh.all() and np.array_equal(gv, mv)
+    t.close()
+
+
+def test_wide_upsert_reaches_keys_held_only_in_the_stash(D):
+    """A key in neither of its buckets but in the stash gets the new value in every stash copy;
+    a fresh key of the same call is inserted."""
+    banner("wide upsert: stash-only key")
+    key = 0x0123456789ABCDEF
+    t = D.DyCuckooWide(100, max_fail_rounds=0, rescue_kicks=16)
+    with pytest.raises(D.DycError) as ei:
+        t.insert(cu(np.full(40, key, np.uint64)), cu(np.arange(1000, 1040, dtype=np.uint64)))
+    assert ei.value.code == 4  # DYC_ERR_PENDING
+    one = cu(np.array([key], np.uint64))
+    for _ in range(2):  # clears the copies in pos1's bucket, then pos2's
+        assert t.erase(one).cpu().tolist() == [1]
+    assert t.stats()["pending"] == 8
+    other = key + 2
+    flags = t.upsert(cu(np.array([key, other], np.uint64)), cu(np.array([777, 778], np.uint64)))
+    assert flags.cpu().tolist() == [1, 0]
+    pk, pv = t.pending()
+    assert np.all(pk == key) and np.all(pv == 777) and pk.size == 8
+    gv, gh = gfind(t, np.array([key, other], np.uint64))
+    assert gh.tolist() == [1, 1] and gv.tolist() == [777, 778]
+    st = t.stats()
+    assert st["count"] == 39 and st["pending"] == 8
+    t.close()
