@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--cpu-seq-log2n", type=int, default=22, help="sequential CPU sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--memsys-regions-mb", default="",
+                    help="memsys: also measure over these buffer sizes in MB, comma-separated (e.g. 32,192,2386)")
     ap.add_argument("--subtables", type=int, default=4, help="subtable count d of the bulk table (2..6)")
     ap.add_argument("--st-list", default="2,3,4,5,6", help="subtables: the subtable counts swept")
     ap.add_argument("--st-kick", type=int, default=15, help="subtables: kick bound of every table")
@@ -364,6 +366,19 @@ def run_memsys(args, dev) -> int:
         "cas_per_s": cas, "stores_per_s": stores, "half_bucket_reads_per_s": half, "read_then_cas_per_s": read_cas,
         "wave_atomics_one_address_per_s": one_addr, "wave_atomics_64_addresses_per_s": spread64,
     }
+    regions = []
+    for mb in [int(x) for x in args.memsys_regions_mb.split(",") if x.strip()]:
+        # the same patterns confined to a smaller buffer: below the 256 MB last-level cache (MALL)
+        # the lines stay on chip, which shows what locality alone would buy each pattern
+        print(f"[memsys] region {mb} MB", file=sys.stderr, flush=True)
+        rc = lib.dyc_memsys_probe(mb << 20, 1 << args.log2n, 3, out)
+        if rc:
+            print(f"dyc_memsys_probe failed: hipError {rc}", file=sys.stderr)
+            return 1
+        regions.append({"MB": mb, "bucket_reads_per_s": out[0], "cas_per_s": out[1], "stores_per_s": out[2],
+                        "half_bucket_reads_per_s": out[3], "read_then_cas_per_s": out[4]})
+    if regions:
+        result["regions"] = regions
     line = json.dumps(result)
     print(line, flush=True)
     if args.json_out:
