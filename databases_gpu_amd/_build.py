@@ -10,7 +10,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdyc.so")
 PROBE_LIB = os.path.join(HERE, "libdyc_memsys.so")
-SOURCES = ["dyc_kernels.hip", "dyc_table.hip", "dyc_group.hip", "dyc_server.hip", "dyc_wide.hip"]
+SOURCES = ["dyc_kernels.hip", "dyc_bulk.hip", "dyc_table.hip", "dyc_group.hip", "dyc_server.hip", "dyc_wide.hip"]
 ARCH = os.environ.get("DYC_OFFLOAD_ARCH", "gfx950")
 
 

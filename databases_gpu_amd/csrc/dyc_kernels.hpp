@@ -92,6 +92,30 @@ hipError_t launch_update(const Desc &D, const uint32_t *keys, const uint32_t *va
 hipError_t launch_insert_entries(const Desc &D, const uint64_t *entries, uint64_t n, uint32_t pre, int lanes,
                                  InsertPolicy pol, LaunchShape sh, Counters *C, PendSink sink, hipStream_t st,
                                  const uint8_t *live = nullptr);
+// ---- bulk insert by bucket fill (insert_order 3)
+// A chunk is kChunkBuckets consecutive buckets of one subtable; chunk ids of subtable t start
+// at base[t]; total = base[d] chunks, bits = bits of the largest chunk id.
+constexpr uint32_t kChunkShift = 9;
+constexpr uint32_t kChunkBuckets = 1u << kChunkShift;
+struct Chunks {
+    uint32_t base[kMaxSubtables + 1];
+    uint32_t total, bits;
+};
+Chunks make_chunks(const Desc &D);
+// (chunk id, entry) per key: round 1 from keys/vals (entries == nullptr) for the start table,
+// round 2 from entries for the other table of the key's pair.
+hipError_t launch_bulk_pack(const Desc &D, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals,
+                            const uint64_t *entries, uint64_t n, uint32_t *cid, uint64_t *out, hipStream_t st);
+// Radix sort of the pairs by chunk id; *sorted_in = 0 if the result is in (cid0, ent0), else 1.
+hipError_t bulk_sort_bytes(uint64_t n, uint32_t bits, size_t *bytes);
+hipError_t bulk_sort(void *tmp, size_t tmp_bytes, uint32_t *cid0, uint32_t *cid1, uint64_t *ent0, uint64_t *ent1,
+                     uint64_t n, uint32_t bits, hipStream_t st, int *sorted_in);
+// Fill free slots chunk by chunk from sorted pairs (n < 2^32); start holds total + 1 words.
+// Entries that do not fit are appended to ovf (*ovf_n, zeroed by the caller, counts them;
+// ovf_cap >= n). Placements go to the kick tally, reserved keys to the rejected tally.
+hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *cid, const uint64_t *ent, uint64_t n,
+                            uint32_t *start, uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C,
+                            hipStream_t st);
 // Order an insert batch by the region of its first bucket: out[0, n) receives the packed
 // entries, region by region. counts and cursors are R words each (counts zeroed by the caller).
 hipError_t launch_partition(const Desc &D, const Regions &M, const uint32_t *keys, const uint32_t *vals, uint64_t n,

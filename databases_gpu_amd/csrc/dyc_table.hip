@@ -86,6 +86,17 @@ struct dyc_table {
     uint64_t *part = nullptr;   // insert batch ordered by bucket region
     uint64_t part_cap = 0;
     unsigned long long *part_counts = nullptr;   // kMaxRegions counts, then kMaxRegions cursors
+    // bulk fill (insert_order 3): (chunk id, entry) pairs and their sort buffers, the chunk
+    // starts, the radix sort's scratch and the overflow count (device, and a pinned read-back)
+    uint32_t *bk_cid[2] = {};
+    uint64_t *bk_ent[2] = {};
+    uint64_t bk_cap = 0;
+    uint32_t *bk_start = nullptr;
+    uint64_t bk_start_cap = 0;
+    void *bk_tmp = nullptr;
+    size_t bk_tmp_bytes = 0;
+    unsigned long long *bk_ovf = nullptr;
+    unsigned long long *bk_hovf = nullptr;
     DycHostStage *stage = nullptr;
     uint8_t *upd = nullptr;  // upsert: per-key "updated in place" flags
     uint64_t upd_cap = 0;
@@ -447,6 +458,123 @@ int reserve_partition(dyc_table *t, uint64_t n)
     return DYC_OK;
 }
 
+// Bulk fill (insert_order 3) takes batches of at least two keys per bucket: below that the
+// fill's pass over every chunk's free-slot masks costs more than the atomics it saves.
+bool use_bulk_fill(const dyc_table *t, uint64_t n)
+{
+    uint64_t buckets = 0;
+    for (uint32_t i = 0; i < t->D.d; ++i) buckets += t->D.nb[i];
+    return t->cfg.insert_order == 3 && n >= 2 * buckets && n < (1ull << 31);
+}
+
+int bulk_alloc(void **p, size_t bytes, const char *what)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(DYC_ERR_OOM, "hipMalloc of the bulk-fill %s (%llu bytes) failed", what,
+                    (unsigned long long)bytes);
+    }
+    return DYC_OK;
+}
+
+// Grow-only workspace of the bulk fill for n keys over the chunks Ch.
+int reserve_bulk(dyc_table *t, uint64_t n, const Chunks &Ch)
+{
+    size_t tmp = 0;
+    HIP_TRY(bulk_sort_bytes(n, Ch.bits, &tmp));
+    const bool grow = n > t->bk_cap || Ch.total + 1ull > t->bk_start_cap || tmp > t->bk_tmp_bytes;
+    if (grow) HIP_TRY(hipDeviceSynchronize());  // a buffer being replaced may still be in use
+    int rc = DYC_OK;
+    if (n > t->bk_cap) {
+        for (int b = 0; b < 2; ++b) {
+            if (t->bk_cid[b]) HIP_TRY(hipFree(t->bk_cid[b]));
+            if (t->bk_ent[b]) HIP_TRY(hipFree(t->bk_ent[b]));
+            t->bk_cid[b] = nullptr;
+            t->bk_ent[b] = nullptr;
+        }
+        t->bk_cap = 0;
+        for (int b = 0; b < 2 && !rc; ++b) {
+            rc = bulk_alloc((void **)&t->bk_cid[b], n * sizeof(uint32_t), "chunk ids");
+            if (!rc) rc = bulk_alloc((void **)&t->bk_ent[b], n * sizeof(uint64_t), "entries");
+        }
+        if (rc) return rc;
+        t->bk_cap = n;
+    }
+    if (Ch.total + 1ull > t->bk_start_cap) {
+        if (t->bk_start) HIP_TRY(hipFree(t->bk_start));
+        t->bk_start = nullptr;
+        t->bk_start_cap = 0;
+        rc = bulk_alloc((void **)&t->bk_start, (Ch.total + 1ull) * sizeof(uint32_t), "chunk starts");
+        if (rc) return rc;
+        t->bk_start_cap = Ch.total + 1ull;
+    }
+    if (tmp > t->bk_tmp_bytes) {
+        if (t->bk_tmp) HIP_TRY(hipFree(t->bk_tmp));
+        t->bk_tmp = nullptr;
+        t->bk_tmp_bytes = 0;
+        rc = bulk_alloc(&t->bk_tmp, tmp, "sort scratch");
+        if (rc) return rc;
+        t->bk_tmp_bytes = tmp;
+    }
+    if (!t->bk_ovf) {
+        rc = bulk_alloc((void **)&t->bk_ovf, 256, "overflow count");
+        if (rc) return rc;
+    }
+    if (!t->bk_hovf && hipHostMalloc((void **)&t->bk_hovf, 64, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        t->bk_hovf = nullptr;
+        return fail(DYC_ERR_OOM, "hipHostMalloc of the bulk-fill read-back word failed");
+    }
+    return DYC_OK;
+}
+
+// One fill round over n sorted-or-not pairs in (bk_cid[in], bk_ent[in]): sort, fill, and read
+// back how many entries overflowed into bk_ent[*out] (the other buffer of the sorted pair).
+int bulk_round(dyc_table *t, const Chunks &Ch, int in, uint64_t n, hipStream_t s, int *out, uint64_t *overflow)
+{
+    int sorted_in = 0;
+    HIP_TRY(bulk_sort(t->bk_tmp, t->bk_tmp_bytes, t->bk_cid[in], t->bk_cid[1 - in], t->bk_ent[in], t->bk_ent[1 - in],
+                      n, Ch.bits, s, &sorted_in));
+    const int r = sorted_in == 0 ? in : 1 - in;
+    HIP_TRY(launch_zero(t->bk_ovf, 1, s));
+    HIP_TRY(launch_bulk_fill(t->D, Ch, t->bk_cid[r], t->bk_ent[r], n, t->bk_start, t->bk_ent[1 - r], t->bk_cap,
+                             t->bk_ovf, t->ctr, s));
+    HIP_TRY(hipMemcpyAsync(t->bk_hovf, t->bk_ovf, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *overflow = *t->bk_hovf;
+    *out = 1 - r;
+    if (*overflow > n) return fail(DYC_ERR_CORRUPT, "bulk fill overflow count %llu exceeds the batch (%llu)",
+                                   (unsigned long long)*overflow, (unsigned long long)n);
+    return DYC_OK;
+}
+
+// insert_order 3: place keys by bucket fill in their start bucket, then the overflow in the
+// other bucket of their pair, then the rest by the eviction insert. Counters are zeroed here
+// and read by the caller.
+int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, hipStream_t s)
+{
+    const Chunks Ch = make_chunks(t->D);
+    int rc = reserve_bulk(t, n, Ch);
+    if (rc) return rc;
+    rc = zero_counters(t, s);
+    if (rc) return rc;
+    HIP_TRY(launch_bulk_pack(t->D, Ch, keys, vals, nullptr, n, t->bk_cid[0], t->bk_ent[0], s));
+    int ov = 0;
+    uint64_t m1 = 0, m2 = 0;
+    rc = bulk_round(t, Ch, 0, n, s, &ov, &m1);
+    if (rc || m1 == 0) return rc;
+    // round 2 reads the overflow list bk_ent[ov] and writes the pairs into the other buffers
+    const int in2 = 1 - ov;
+    HIP_TRY(launch_bulk_pack(t->D, Ch, nullptr, nullptr, t->bk_ent[ov], m1, t->bk_cid[in2], t->bk_ent[in2], s));
+    int ov2 = 0;
+    rc = bulk_round(t, Ch, in2, m1, s, &ov2, &m2);
+    if (rc || m2 == 0) return rc;
+    HIP_TRY(launch_insert_entries(t->D, t->bk_ent[ov2], m2, kNoTable, (int)t->cfg.insert_lanes, t->policy(),
+                                  t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s));
+    return DYC_OK;
+}
+
 bool valid_lanes(uint32_t l) { return l == 4 || l == 8 || l == 16; }
 bool valid_insert_lanes(uint32_t l) { return l == 1 || l == 2 || valid_lanes(l); }
 
@@ -459,8 +587,8 @@ int check_config(const dyc_config &cfg)
     if (!(cfg.upper_bound > 0.0 && cfg.upper_bound <= 1.0) || !(cfg.lower_bound >= 0.0) ||
         cfg.lower_bound >= cfg.upper_bound)
         return fail(DYC_ERR_INVALID, "need 0 <= lower_bound < upper_bound <= 1");
-    if (cfg.insert_order > 2 || cfg.region_mib == 0)
-        return fail(DYC_ERR_INVALID, "insert_order must be 0, 1 or 2 and region_mib > 0");
+    if (cfg.insert_order > 3 || cfg.region_mib == 0)
+        return fail(DYC_ERR_INVALID, "insert_order must be 0..3 and region_mib > 0");
     const uint32_t d = config_subtables(cfg);
     if (d < (uint32_t)kMinSubtables || d > (uint32_t)kMaxSubtables || cfg.reserved0 != 0)
         return fail(DYC_ERR_INVALID, "subtables must be in [%d, %d] (reserved0 = 0)", kMinSubtables, kMaxSubtables);
@@ -818,6 +946,14 @@ int dyc_destroy(dyc_table *t)
     if (t->ctr) (void)hipFree(t->ctr);
     if (t->part) (void)hipFree(t->part);
     if (t->part_counts) (void)hipFree(t->part_counts);
+    for (int b = 0; b < 2; ++b) {
+        if (t->bk_cid[b]) (void)hipFree(t->bk_cid[b]);
+        if (t->bk_ent[b]) (void)hipFree(t->bk_ent[b]);
+    }
+    if (t->bk_start) (void)hipFree(t->bk_start);
+    if (t->bk_tmp) (void)hipFree(t->bk_tmp);
+    if (t->bk_ovf) (void)hipFree(t->bk_ovf);
+    if (t->bk_hovf) (void)hipHostFree(t->bk_hovf);
     if (t->hctr) (void)hipHostFree(t->hctr);
     if (t->hpost) (void)hipHostFree(t->hpost);
     if (t->upd) (void)hipFree(t->upd);
@@ -847,7 +983,10 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     if (rc) return rc;
     const uint64_t room = t->pend_cap - t->pend_n;
     const uint32_t R = insert_regions(t, n);
-    if (R) {
+    if (use_bulk_fill(t, n)) {
+        rc = bulk_insert(t, keys, vals, n, s);
+        if (rc) return rc;
+    } else if (R) {
         rc = reserve_partition(t, n);
         if (rc) return rc;
         HIP_TRY(launch_zero(t->part_counts, kMaxRegions, s));

@@ -13,12 +13,14 @@
 // A 16-byte entry cannot move with one 64-bit CAS, so an exchange claims the slot instead
 // (the paper locks the bucket, dynamic.tex:37-38):
 //   1. CAS the slot's key from the value the lane read to BUSY (a lost race patches the view);
-//   2. exchange the slot's value with the new value (the old one travels with the evicted key);
-//   3. wait for that exchange to be performed, then exchange the key word from BUSY to the new
-//      key, which publishes the slot.
-// All three are device-scope atomics performed at the memory side, so no cache maintenance is
-// needed between them; a slot holding BUSY is never chosen as a victim by another lane, and no
-// lane ever waits on one, so nothing can deadlock. Finds and erases run in separate launches.
+//   2. write the slot's value: an agent-scope atomic store for an empty slot, an exchange for a
+//      victim's (its old value travels with the evicted key);
+//   3. wait for that write to be performed, then publish the key with an agent-scope atomic
+//      store (BUSY -> the new key).
+// Every step is coherent at agent scope (memory-side atomics, write-through stores), so no cache
+// maintenance is needed between them; a slot holding BUSY is never chosen as a victim by another
+// lane, and no lane ever waits on one, so nothing can deadlock. Finds, erases and updates run in
+// separate launches.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>

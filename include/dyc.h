@@ -70,7 +70,11 @@ typedef struct dyc_config {
     uint64_t min_capacity;     /* no automatic downsize at or below this many slots
                                   (default 5033164 = TABLE_NUM*1024*1024*1.2, api.cpp:496) */
     uint32_t insert_order;     /* 0 or 1 = input order (default); 2 = order each batch by the bucket
-                                  region of its keys' first bucket before inserting */
+                                  region of its keys' first bucket before inserting; 3 = bulk
+                                  fill for batches of at least two keys per bucket: sort the batch
+                                  by 512-bucket chunk of its start bucket, fill free slots chunk by
+                                  chunk without atomics, the overflow likewise in its other bucket,
+                                  then the eviction insert for the rest (workspace 24 B per key) */
     uint32_t region_mib;       /* table bytes per bucket region when ordering (default 64) */
     uint32_t subtables;        /* subtable count d, 2..6 (default 4 = TABLE_NUM, dy_hash.h:32); the
                                   pair scheme of each d is in diff_hash_num_pos_fun.cpp:14-188.

@@ -1,0 +1,155 @@
+"""GPU parity of the bulk-fill insert (insert_order 3): sort by bucket chunk, fill free slots
+chunk by chunk, the overflow in its other bucket, the rest by the eviction insert. Slot layout
+after a parallel insert is not fixed, so each case checks the key->value multiset of the
+table (plus pending entries) against what was inserted, the placement invariant of every
+stored entry on the oracle, counts, and find results."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def D():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import databases_gpu_amd as D
+
+    D.load()
+    return D
+
+
+def unique_keys(n, seed):
+    rng = np.random.default_rng(seed)
+    out = np.unique(rng.integers(1, 2**32 - 1, size=int(n * 1.3) + 64, dtype=np.uint64).astype(np.uint32))
+    rng.shuffle(out)
+    return out[:n]
+
+
+def vals_of(k):
+    from databases_gpu_amd.workload import splitmix32_np
+
+    return splitmix32_np(k)
+
+
+def cu(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).cuda()
+
+
+def stored(t):
+    """Sorted packed entries of the subtables and the pending list, and the oracle of the image."""
+    images = t.dump()
+    ref = O.OracleTable(nb=[x.size // 16 for x in images])
+    ref.sub = [np.ascontiguousarray(x).copy() for x in images]
+    ref.count = int(sum(np.count_nonzero(x) for x in images))
+    ents = [x[x != 0] for x in images]
+    ents.append(t.pending())  # packed entries
+    return np.sort(np.concatenate(ents)), ref
+
+
+def gfind(t, q):
+    v, h = t.find(cu(q))
+    torch.cuda.synchronize()
+    return v.cpu().numpy().view(np.uint32), h.cpu().numpy()
+
+
+@pytest.mark.parametrize("d", (2, 4, 6))
+def test_bulk_fill_insert_multiset_invariant_and_find(D, d):
+    n = 1 << 20
+    banner(f"bulk fill insert 2^20 at 0.90, subtables={d}")
+    keys = unique_keys(n, 1200 + d)
+    vals = vals_of(keys)
+    t = D.DyCuckoo(int(np.ceil(n / 0.90)), subtables=d, insert_order=3)
+    t.insert(cu(keys), cu(vals))
+    st = t.stats()
+    assert st["count"] == n and st["lost"] == 0 and st["rejected"] == 0
+    ents, ref = stored(t)
+    assert ref.check_invariant() == 0
+    assert np.array_equal(ents, np.sort(O.pack(keys, vals)))
+    assert st["last_kicks"] >= n  # every placement counts at least once
+    gv, gh = gfind(t, keys)
+    assert gh.all() and np.array_equal(gv, vals)
+    t.close()
+
+
+def test_bulk_fill_into_a_partly_filled_resized_table(D):
+    """Unequal subtables (one upsized), existing entries, reserved keys and duplicate copies in
+    the batch: the fill takes only free slots, rejects the reserved keys, and keeps every copy."""
+    banner("bulk fill into a partly filled table after an upsize")
+    first = unique_keys(300_000, 1300)
+    t = D.DyCuckoo(600_000, insert_order=0, auto_resize=False)
+    t.insert(cu(first), cu(vals_of(first)))
+    t.resize_up()
+    nb = t.buckets
+    assert len(set(nb)) == 2  # one subtable doubled
+    t.configure(insert_order=3)
+    fresh = unique_keys(700_000, 1301)
+    fresh = fresh[~np.isin(fresh, first)][:250_000]
+    dup = first[:5000]  # second copies of stored keys
+    batch = np.concatenate([fresh, dup, np.array([0, 0xFFFFFFFF, 0], dtype=np.uint32)])
+    rng = np.random.default_rng(1302)
+    rng.shuffle(batch)
+    assert batch.size >= 2 * sum(nb)  # large enough for the bulk fill
+    bv = vals_of(batch)
+    t.insert(cu(batch), cu(bv))
+    st = t.stats()
+    keep = (batch != 0) & (batch != 0xFFFFFFFF)
+    assert st["count"] == first.size + int(keep.sum()) and st["lost"] == 0
+    assert st["rejected"] == 3
+    ents, ref = stored(t)
+    assert ref.check_invariant() == 0
+    want = np.sort(np.concatenate([O.pack(first, vals_of(first)), O.pack(batch[keep], bv[keep])]))
+    assert np.array_equal(ents, want)
+    gv, gh = gfind(t, fresh)
+    assert gh.all() and np.array_equal(gv, vals_of(fresh))
+    t.close()
+
+
+def test_bulk_fill_overflow_reaches_the_eviction_insert(D):
+    """Many copies of a few keys overflow their start bucket and their other bucket, so all three
+    stages run; every copy is stored or pending, and the placement invariant holds."""
+    banner("bulk fill: overflow through both rounds and the eviction insert")
+    base = unique_keys(3000, 1400)
+    hot = base[:6]
+    batch = np.concatenate([base, np.repeat(hot, 11)])  # 12 copies of each hot key
+    rng = np.random.default_rng(1401)
+    rng.shuffle(batch)
+    t = D.DyCuckoo(8192, insert_order=3, auto_resize=False)
+    assert batch.size >= 2 * sum(t.buckets)
+    bv = vals_of(batch)
+    t.insert(cu(batch), cu(bv))
+    st = t.stats()
+    assert st["count"] == batch.size and st["lost"] == 0
+    ents, ref = stored(t)
+    assert ref.check_invariant() == 0
+    assert np.array_equal(ents, np.sort(O.pack(batch, bv)))
+    gv, gh = gfind(t, base)
+    assert gh.all() and np.array_equal(gv, vals_of(base))
+    t.close()
+
+
+def test_bulk_fill_matches_the_eviction_insert_on_the_same_batch(D):
+    """The two insert paths store the same multiset and answer finds alike."""
+    n = 1 << 19
+    banner("bulk fill vs eviction insert, 2^19")
+    keys = unique_keys(n, 1500)
+    vals = vals_of(keys)
+    out = []
+    for order in (0, 3):
+        t = D.DyCuckoo(int(np.ceil(n / 0.85)), insert_order=order)
+        t.insert(cu(keys), cu(vals))
+        ents, ref = stored(t)
+        assert ref.check_invariant() == 0
+        q = np.concatenate([keys[::3], unique_keys(50_000, 1501)])
+        out.append((ents, *gfind(t, q), t.stats()["count"]))
+        t.close()
+    a, b = out
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert a[3] == b[3] == n
