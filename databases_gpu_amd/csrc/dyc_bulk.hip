@@ -80,14 +80,30 @@ __global__ __launch_bounds__(256) void bulk_pack_kernel(Desc D, Chunks Ch, const
 
 // start[c] = the first index of chunk c in the sorted chunk ids, for c in [0, total]; index i
 // writes the starts of the chunks in (cid[i - 1], cid[i]] (with cid[-1] = -1, cid[n] = total).
+// A thread takes four consecutive indexes with one 16-byte load (cid is 16-byte aligned).
 __global__ __launch_bounds__(256) void bulk_bounds_kernel(const uint32_t *__restrict__ cid, uint64_t n,
                                                           uint32_t total, uint32_t *__restrict__ start)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += stride) {
-        const int64_t prev = i == 0 ? -1 : (int64_t)min(cid[i - 1], total);
-        const int64_t cur = i == n ? (int64_t)total : (int64_t)min(cid[i], total);
-        for (int64_t c = prev + 1; c <= cur; ++c) start[c] = (uint32_t)i;
+    const uint64_t quads = n / 4 + 1;  // the last quad holds index n (and the tail)
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < quads; g += stride) {
+        const uint64_t i0 = g * 4;
+        uint32_t v[4];
+        if (i0 + 4 <= n) {
+            const uint4 q = *reinterpret_cast<const uint4 *>(cid + i0);
+            v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = i0 + u < n ? cid[i0 + u] : total;
+        }
+        int64_t prev = i0 == 0 ? -1 : (int64_t)min(cid[i0 - 1], total);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (i0 + u > n) break;
+            const int64_t cur = i0 + u == n ? (int64_t)total : (int64_t)min(v[u], total);
+            for (int64_t c = prev + 1; c <= cur; ++c) start[c] = (uint32_t)(i0 + u);
+            prev = cur;
+        }
     }
 }
 
@@ -315,7 +331,7 @@ hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *cid
                             hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    const int grid = grid_for((n + 1 + kWave - 1) / kWave, LaunchShape{256, 0}, 16384);
+    const int grid = grid_for((n / 4 + 1 + kWave - 1) / kWave, LaunchShape{256, 0}, 16384);
     bulk_bounds_kernel<<<grid, 256, 0, st>>>(cid, n, Ch.total, start);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
