@@ -568,7 +568,8 @@ def run_wide(args, dev) -> int:
     out_h = torch.empty(n, dtype=torch.uint8, device=dev)
     del perm, idx
     torch.cuda.synchronize()
-    t = D.DyCuckooWide(int(math.ceil(n / args.theta)), kick_bound=0 if not args.reference_kick else 15)
+    t = D.DyCuckooWide(int(math.ceil(n / args.theta)), kick_bound=0 if not args.reference_kick else 15,
+                       insert_order=args.insert_order)
     ins, fnd, kicks = [], [], []
     for s_ in range(args.warmup + args.steps):
         t.clear()
@@ -611,7 +612,7 @@ def run_wide(args, dev) -> int:
         "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"2^{args.log2n} unique u64 keys and u64 values at load factor {args.theta}, then "
                                f"2^{args.log2n} finds at {args.hit:.0%} hit", "buckets": t.buckets,
-                   "kick_bound": 15 if args.reference_kick else 0},
+                   "kick_bound": 15 if args.reference_kick else 0, "insert_order": args.insert_order},
         "insert": dict(roof(b_ins, i_ms), ms=round(i_ms, 3), mops=round(n / i_ms / 1e3, 1),
                        mean_exchanges=round(x, 4), bytes_per_op=round(b_ins / n, 1)),
         "find": dict(roof(b_find, f_ms), ms=round(f_ms, 3), mops=round(n / f_ms / 1e3, 1),

@@ -35,17 +35,6 @@ __device__ inline uint32_t start_table(const Desc &D, uint32_t k)
     return (k & 1u) ? pos2(D, p) : pos1(D, p);
 }
 
-__device__ inline uint32_t chunk_base(const Chunks &Ch, uint32_t t)
-{
-    const uint32_t b0 = Ch.base[0], b1 = Ch.base[1], b2 = Ch.base[2], b3 = Ch.base[3], b4 = Ch.base[4],
-                   b5 = Ch.base[5];
-    uint32_t x = t == 1 ? b1 : b0;
-    x = t == 2 ? b2 : x;
-    x = t == 3 ? b3 : x;
-    x = t == 4 ? b4 : x;
-    return t == 5 ? b5 : x;
-}
-
 // Round 1 (IN_ENTRIES false): keys/vals, target = the start table; reserved keys go to chunk 0
 // and are rejected by its fill. Round 2 (IN_ENTRIES true): overflow entries, target = the other
 // table of the key's pair.
@@ -326,14 +315,42 @@ hipError_t bulk_sort(void *tmp, size_t tmp_bytes, uint32_t *cid0, uint32_t *cid1
     return hipSuccess;
 }
 
+hipError_t bulk_sort_wide_bytes(uint64_t n, uint32_t bits, size_t *bytes)
+{
+    rocprim::double_buffer<uint32_t> kb(nullptr, nullptr);
+    rocprim::double_buffer<WideEntry> vb(nullptr, nullptr);
+    *bytes = 0;
+    return rocprim::radix_sort_pairs(nullptr, *bytes, kb, vb, (unsigned int)n, 0u, bits);
+}
+
+hipError_t bulk_sort_wide(void *tmp, size_t tmp_bytes, uint32_t *cid0, uint32_t *cid1, WideEntry *ent0,
+                          WideEntry *ent1, uint64_t n, uint32_t bits, hipStream_t st, int *sorted_in)
+{
+    *sorted_in = 0;
+    if (n == 0) return hipSuccess;
+    rocprim::double_buffer<uint32_t> kb(cid0, cid1);
+    rocprim::double_buffer<WideEntry> vb(ent0, ent1);
+    const hipError_t e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (unsigned int)n, 0u, bits, st);
+    if (e != hipSuccess) return e;
+    *sorted_in = kb.current() == cid0 ? 0 : 1;
+    if ((vb.current() == ent0 ? 0 : 1) != *sorted_in) return hipErrorUnknown;
+    return hipSuccess;
+}
+
+hipError_t launch_bulk_bounds(const uint32_t *cid, uint64_t n, uint32_t total, uint32_t *start, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = grid_for((n / 4 + 1 + kWave - 1) / kWave, LaunchShape{256, 0}, 16384);
+    bulk_bounds_kernel<<<grid, 256, 0, st>>>(cid, n, total, start);
+    return hipGetLastError();
+}
+
 hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *cid, const uint64_t *ent, uint64_t n,
                             uint32_t *start, uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C,
                             hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    const int grid = grid_for((n / 4 + 1 + kWave - 1) / kWave, LaunchShape{256, 0}, 16384);
-    bulk_bounds_kernel<<<grid, 256, 0, st>>>(cid, n, Ch.total, start);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_bulk_bounds(cid, n, Ch.total, start, st);
     if (e != hipSuccess) return e;
     if (n >= (1ull << 32) || Ch.total == 0) return hipErrorInvalidValue;
     uint64_t buckets = 0;

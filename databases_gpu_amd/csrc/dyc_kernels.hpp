@@ -102,6 +102,25 @@ struct Chunks {
     uint32_t total, bits;
 };
 Chunks make_chunks(const Desc &D);
+__host__ __device__ inline uint32_t chunk_base(const Chunks &Ch, uint32_t t)
+{
+    const uint32_t b0 = Ch.base[0], b1 = Ch.base[1], b2 = Ch.base[2], b3 = Ch.base[3], b4 = Ch.base[4],
+                   b5 = Ch.base[5];
+    uint32_t x = t == 1 ? b1 : b0;
+    x = t == 2 ? b2 : x;
+    x = t == 3 ? b3 : x;
+    x = t == 4 ? b4 : x;
+    return t == 5 ? b5 : x;
+}
+// A wide-key entry as the bulk fill of the wide table moves it.
+struct WideEntry {
+    uint64_t k, v;
+};
+hipError_t bulk_sort_wide_bytes(uint64_t n, uint32_t bits, size_t *bytes);
+hipError_t bulk_sort_wide(void *tmp, size_t tmp_bytes, uint32_t *cid0, uint32_t *cid1, WideEntry *ent0,
+                          WideEntry *ent1, uint64_t n, uint32_t bits, hipStream_t st, int *sorted_in);
+// start[c] = first index of chunk c in n sorted chunk ids, c in [0, total] (start holds total + 1).
+hipError_t launch_bulk_bounds(const uint32_t *cid, uint64_t n, uint32_t total, uint32_t *start, hipStream_t st);
 // (chunk id, entry) per key: round 1 from keys/vals (entries == nullptr) for the start table,
 // round 2 from entries for the other table of the key's pair.
 hipError_t launch_bulk_pack(const Desc &D, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals,

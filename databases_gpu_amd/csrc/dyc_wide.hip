@@ -458,6 +458,222 @@ __global__ __launch_bounds__(256) void wide_downsize_kernel(const uint64_t *__re
 
 constexpr uint64_t kWideInsertBlocks = 16384;
 
+// ------------------------------------------------------------ bulk fill (insert_order 3)
+// The narrow table's bulk fill (dyc_bulk.hip) over fold(k): (chunk id, entry) pairs sorted by
+// chunk, one workgroup per chunk filling free slots, a second round in the other bucket, the
+// eviction insert for the rest. The dense round keeps the chunk's key lines and value lines in
+// LDS (2 x 64 KB), so one 1024-thread workgroup runs per CU.
+template <bool IN_ENTRIES>
+__global__ __launch_bounds__(256) void wide_bulk_pack_kernel(WideDesc W, Chunks Ch, const uint64_t *__restrict__ keys,
+                                                             const uint64_t *__restrict__ vals,
+                                                             const WideEntry *__restrict__ in, uint64_t n,
+                                                             uint32_t *__restrict__ cid, WideEntry *__restrict__ out)
+{
+    const Desc &D = W.D;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        WideEntry e;
+        uint32_t c = 0;
+        if constexpr (IN_ENTRIES) {
+            e = in[i];
+            const uint32_t f = fold64(e.k), p = pair_of(D, f);
+            const uint32_t t = other_pos(D, p, (e.k & 1u) ? pos2(D, p) : pos1(D, p));
+            c = chunk_base(Ch, t) + (bucket_of(D, f, t) >> kChunkShift);
+        } else {
+            e.k = keys[i];
+            e.v = vals[i];
+            if (!wide_reserved(e.k)) {  // reserved keys go to chunk 0, whose fill rejects them
+                const uint32_t f = fold64(e.k), p = pair_of(D, f);
+                const uint32_t t = (e.k & 1u) ? pos2(D, p) : pos1(D, p);
+                c = chunk_base(Ch, t) + (bucket_of(D, f, t) >> kChunkShift);
+            }
+        }
+        cid[i] = c;
+        out[i] = e;
+    }
+}
+
+__device__ inline uint32_t wide_nth_set_bit(uint32_t m, uint32_t r)
+{
+    for (uint32_t q = 0; q < r; ++q) m &= m - 1u;
+    return __builtin_ctz(m);
+}
+
+template <bool DENSE, int THREADS>
+__global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chunks Ch, const uint32_t *__restrict__ start,
+                                                                 const WideEntry *__restrict__ ent, uint64_t n,
+                                                                 WideEntry *__restrict__ ovf, uint64_t ovf_cap,
+                                                                 unsigned long long *__restrict__ ovf_n,
+                                                                 Counters *__restrict__ C)
+{
+    const Desc &D = W.D;
+    constexpr uint32_t kImg = DENSE ? kChunkBuckets * kSlots : 2;
+    __shared__ uint64_t kimg[kImg], vimg[kImg];  // DENSE: the chunk's key and value lines
+    __shared__ uint32_t fm[kChunkBuckets];       // free-slot mask of each bucket
+    __shared__ uint32_t cnt[kChunkBuckets];      // entries per bucket, then slots taken
+    __shared__ uint32_t wred[THREADS / kWave];
+    __shared__ unsigned long long obase;
+    __shared__ uint32_t onext;
+    const uint32_t c = blockIdx.x;
+    // clamped to the entries, so a range is never read past them whatever the bounds hold
+    const uint32_t hi = (uint32_t)min((uint64_t)start[c + 1], n);
+    const uint32_t lo = min(start[c], hi);
+    unsigned long long placed = 0, rejected = 0;
+    if (lo < hi) {  // block-uniform
+        uint32_t t = 0;
+        for (uint32_t q = 1; q < D.d; ++q)
+            if (Ch.base[q] <= c) t = q;
+        const uint32_t nb = selt(t, D.nb);
+        const uint32_t b0 = (c - chunk_base(Ch, t)) << kChunkShift;
+        const uint32_t len = min(kChunkBuckets, nb - b0);
+        uint64_t *const ksub = selt(t, D.sub);
+        uint64_t *const vsub = selt(t, W.val);
+        const uint4 *const ksrc = reinterpret_cast<const uint4 *>(ksub + (uint64_t)b0 * kSlots);
+        constexpr uint32_t kParts = kChunkBuckets * kSlots / 2;  // 16-byte parts of a chunk's lines
+        if constexpr (DENSE) {
+            const uint4 *const vsrc = reinterpret_cast<const uint4 *>(vsub + (uint64_t)b0 * kSlots);
+            uint4 *const ki = reinterpret_cast<uint4 *>(kimg);
+            uint4 *const vi = reinterpret_cast<uint4 *>(vimg);
+            for (uint32_t q = threadIdx.x; q < kParts; q += THREADS) {
+                const bool in = q < len * (kSlots / 2);
+                ki[q] = in ? ksrc[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
+                vi[q] = in ? vsrc[q] : make_uint4(0u, 0u, 0u, 0u);
+            }
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) {
+                uint32_t m = 0;
+#pragma unroll
+                for (int q = 0; q < kSlots; ++q) m |= (uint32_t)(kimg[j * kSlots + q] == 0ull) << q;
+                fm[j] = m;
+                cnt[j] = 0;
+            }
+        } else {
+            constexpr uint32_t kPer = kParts / THREADS;
+            uint4 v[kPer];
+#pragma unroll
+            for (uint32_t u = 0; u < kPer; ++u) {
+                const uint32_t q = threadIdx.x + u * THREADS;
+                v[u] = q < len * (kSlots / 2) ? ksrc[q] : make_uint4(1u, 1u, 1u, 1u);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kPer; ++u) {
+                const uint32_t q = threadIdx.x + u * THREADS;
+                const bool e0 = (v[u].x | v[u].y) == 0u, e1 = (v[u].z | v[u].w) == 0u;
+                const uint64_t m0 = __ballot(e0), m1 = __ballot(e1);
+                const uint32_t lane = __lane_id();
+                if ((lane & 7u) == 0) {
+                    const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
+                    uint32_t m = 0;
+#pragma unroll
+                    for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
+                    fm[q / 8] = m;
+                    cnt[q / 8] = 0;
+                }
+            }
+        }
+        if (threadIdx.x == 0) onext = 0;
+        __syncthreads();
+        constexpr uint32_t kBatch = 8;
+        uint32_t stray = 0;
+        for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * THREADS) {
+            uint64_t kb[kBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                const uint32_t i = i0 + u * THREADS;
+                kb[u] = i < hi ? ent[i].k : 0ull;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                if (i0 + u * THREADS >= hi || wide_reserved(kb[u])) continue;
+                const uint32_t b = bucket_of(D, fold64(kb[u]), t);
+                if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
+                else ++stray;
+            }
+        }
+        __syncthreads();
+        uint32_t excess = stray;
+        for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) {
+            const uint32_t f = (uint32_t)__builtin_popcount(fm[j]);
+            excess += cnt[j] > f ? cnt[j] - f : 0u;
+        }
+        excess = (uint32_t)wave_sum((unsigned long long)excess);
+        if (__lane_id() == 0) wred[threadIdx.x / kWave] = excess;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (uint32_t w = 0; w < THREADS / kWave; ++w) tot += wred[w];
+            obase = tot ? atomicAdd(ovf_n, (unsigned long long)tot) : 0ull;
+        }
+        for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) cnt[j] = 0;
+        __syncthreads();
+        for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * THREADS) {
+            WideEntry eb[kBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                const uint32_t i = i0 + u * THREADS;
+                eb[u] = i < hi ? ent[i] : WideEntry{0ull, 0ull};
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                if (i0 + u * THREADS >= hi) continue;
+                const WideEntry e = eb[u];
+                if (wide_reserved(e.k)) {
+                    ++rejected;
+                    continue;
+                }
+                const uint32_t b = bucket_of(D, fold64(e.k), t);
+                bool fits = false;
+                if (b >= b0 && b - b0 < len) {
+                    const uint32_t j = b - b0;
+                    const uint32_t r = atomicAdd(&cnt[j], 1u);
+                    const uint32_t m = fm[j];
+                    if (r < (uint32_t)__builtin_popcount(m)) {
+                        const uint32_t slot = wide_nth_set_bit(m, r);
+                        if constexpr (DENSE) {
+                            kimg[j * kSlots + slot] = e.k;
+                            vimg[j * kSlots + slot] = e.v;
+                        } else {
+                            ksub[(uint64_t)b * kSlots + slot] = e.k;
+                            vsub[(uint64_t)b * kSlots + slot] = e.v;
+                        }
+                        fits = true;
+                        ++placed;
+                    }
+                }
+                if (!fits) {  // the counted excess reserved exactly these places
+                    const unsigned long long at = obase + atomicAdd(&onext, 1u);
+                    if (at < ovf_cap) ovf[at] = e;
+                }
+            }
+        }
+        if constexpr (DENSE) {
+            __syncthreads();
+            uint4 *const kd = reinterpret_cast<uint4 *>(ksub + (uint64_t)b0 * kSlots);
+            uint4 *const vd = reinterpret_cast<uint4 *>(vsub + (uint64_t)b0 * kSlots);
+            const uint4 *const ki = reinterpret_cast<const uint4 *>(kimg);
+            const uint4 *const vi = reinterpret_cast<const uint4 *>(vimg);
+            for (uint32_t q = threadIdx.x; q < len * (kSlots / 2); q += THREADS) {
+                kd[q] = ki[q];
+                vd[q] = vi[q];
+            }
+        }
+    }
+    wave_tally(C, &C->kicks, placed);
+    wave_tally(C, &C->rejected, rejected);
+}
+
+// The last overflow list (entries) as the key and value arrays the eviction insert reads.
+__global__ __launch_bounds__(256) void wide_split_kernel(const WideEntry *__restrict__ in, uint64_t n,
+                                                         uint64_t *__restrict__ keys, uint64_t *__restrict__ vals)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const WideEntry e = in[i];
+        keys[i] = e.k;
+        vals[i] = e.v;
+    }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------ launchers
@@ -496,6 +712,43 @@ hipError_t launch_wide_insert(const WideDesc &W, const uint64_t *keys, const uin
     else
         wide_insert_kernel<false><<<grid, sh.block_threads, 0, st>>>(W, keys, vals, n, kNoTable, pol, C, sink,
                                                                      skip);
+    return hipGetLastError();
+}
+
+hipError_t launch_wide_bulk_pack(const WideDesc &W, const Chunks &Ch, const uint64_t *keys, const uint64_t *vals,
+                                 const WideEntry *in, uint64_t n, uint32_t *cid, WideEntry *out, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = grid_for((n + kWave - 1) / kWave, LaunchShape{256, 0}, 16384);
+    if (in)
+        wide_bulk_pack_kernel<true><<<grid, 256, 0, st>>>(W, Ch, nullptr, nullptr, in, n, cid, out);
+    else
+        wide_bulk_pack_kernel<false><<<grid, 256, 0, st>>>(W, Ch, keys, vals, nullptr, n, cid, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_wide_bulk_fill(const WideDesc &W, const Chunks &Ch, const uint32_t *cid, const WideEntry *ent,
+                                 uint64_t n, uint32_t *start, WideEntry *ovf, uint64_t ovf_cap,
+                                 unsigned long long *ovf_n, Counters *C, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    if (n >= (1ull << 32) || Ch.total == 0) return hipErrorInvalidValue;
+    hipError_t e = launch_bulk_bounds(cid, n, Ch.total, start, st);
+    if (e != hipSuccess) return e;
+    uint64_t buckets = 0;
+    for (uint32_t t = 0; t < W.D.d; ++t) buckets += W.D.nb[t];
+    if (n >= 2 * buckets)
+        wide_bulk_fill_kernel<true, 1024><<<Ch.total, 1024, 0, st>>>(W, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C);
+    else
+        wide_bulk_fill_kernel<false, 256><<<Ch.total, 256, 0, st>>>(W, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_wide_split(const WideEntry *in, uint64_t n, uint64_t *keys, uint64_t *vals, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = grid_for((n + kWave - 1) / kWave, LaunchShape{256, 0}, 16384);
+    wide_split_kernel<<<grid, 256, 0, st>>>(in, n, keys, vals);
     return hipGetLastError();
 }
 
@@ -553,6 +806,17 @@ struct dyc_wide {
     uint64_t live_cap = 0;
     uint8_t *upd = nullptr;  // upsert: per-key "updated in place" flags
     uint64_t upd_cap = 0;
+    // bulk fill (insert_order 3): (chunk id, entry) pairs and their sort buffers, the chunk
+    // starts, the radix sort's scratch and the overflow count (device, and a pinned read-back)
+    uint32_t *bk_cid[2] = {};
+    WideEntry *bk_ent[2] = {};
+    uint64_t bk_cap = 0;
+    uint32_t *bk_start = nullptr;
+    uint64_t bk_start_cap = 0;
+    void *bk_tmp = nullptr;
+    size_t bk_tmp_bytes = 0;
+    unsigned long long *bk_ovf = nullptr;
+    unsigned long long *bk_hovf = nullptr;
     Reclaimer reclaim;  // frees retired subtables off the calling thread
     dyc_stats st{};
 
@@ -863,6 +1127,117 @@ int wcompact_pending(dyc_wide *t, hipStream_t s)
     return DYC_OK;
 }
 
+// Bulk fill for batches of at least two keys per bucket (insert_order 3), as for 32-bit keys.
+bool wuse_bulk(const dyc_wide *t, uint64_t n)
+{
+    uint64_t buckets = 0;
+    for (uint32_t i = 0; i < t->W.D.d; ++i) buckets += t->W.D.nb[i];
+    return t->cfg.insert_order == 3 && n >= 2 * buckets && n < (1ull << 31);
+}
+
+int walloc(void **p, size_t bytes, const char *what)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(DYC_ERR_OOM, "hipMalloc of the bulk-fill %s (%llu bytes) failed", what,
+                    (unsigned long long)bytes);
+    }
+    return DYC_OK;
+}
+
+int wreserve_bulk(dyc_wide *t, uint64_t n, const Chunks &Ch)
+{
+    size_t tmp = 0;
+    HIP_TRY(bulk_sort_wide_bytes(n, Ch.bits, &tmp));
+    if (n > t->bk_cap || Ch.total + 1ull > t->bk_start_cap || tmp > t->bk_tmp_bytes)
+        HIP_TRY(hipDeviceSynchronize());  // a buffer being replaced may still be in use
+    int rc = DYC_OK;
+    if (n > t->bk_cap) {
+        for (int b = 0; b < 2; ++b) {
+            if (t->bk_cid[b]) HIP_TRY(hipFree(t->bk_cid[b]));
+            if (t->bk_ent[b]) HIP_TRY(hipFree(t->bk_ent[b]));
+            t->bk_cid[b] = nullptr;
+            t->bk_ent[b] = nullptr;
+        }
+        t->bk_cap = 0;
+        for (int b = 0; b < 2 && !rc; ++b) {
+            rc = walloc((void **)&t->bk_cid[b], n * sizeof(uint32_t), "chunk ids");
+            if (!rc) rc = walloc((void **)&t->bk_ent[b], n * sizeof(WideEntry), "entries");
+        }
+        if (rc) return rc;
+        t->bk_cap = n;
+    }
+    if (Ch.total + 1ull > t->bk_start_cap) {
+        if (t->bk_start) HIP_TRY(hipFree(t->bk_start));
+        t->bk_start = nullptr;
+        t->bk_start_cap = 0;
+        rc = walloc((void **)&t->bk_start, (Ch.total + 1ull) * sizeof(uint32_t), "chunk starts");
+        if (rc) return rc;
+        t->bk_start_cap = Ch.total + 1ull;
+    }
+    if (tmp > t->bk_tmp_bytes) {
+        if (t->bk_tmp) HIP_TRY(hipFree(t->bk_tmp));
+        t->bk_tmp = nullptr;
+        t->bk_tmp_bytes = 0;
+        rc = walloc(&t->bk_tmp, tmp, "sort scratch");
+        if (rc) return rc;
+        t->bk_tmp_bytes = tmp;
+    }
+    if (!t->bk_ovf && (rc = walloc((void **)&t->bk_ovf, 256, "overflow count"))) return rc;
+    if (!t->bk_hovf && hipHostMalloc((void **)&t->bk_hovf, 64, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        t->bk_hovf = nullptr;
+        return fail(DYC_ERR_OOM, "hipHostMalloc of the bulk-fill read-back word failed");
+    }
+    return DYC_OK;
+}
+
+// Sort the n pairs in buffer `in`, fill, and read back how many entries overflowed into
+// bk_ent[*out].
+int wbulk_round(dyc_wide *t, const Chunks &Ch, int in, uint64_t n, hipStream_t s, int *out, uint64_t *overflow)
+{
+    int sorted_in = 0;
+    HIP_TRY(bulk_sort_wide(t->bk_tmp, t->bk_tmp_bytes, t->bk_cid[in], t->bk_cid[1 - in], t->bk_ent[in],
+                           t->bk_ent[1 - in], n, Ch.bits, s, &sorted_in));
+    const int r = sorted_in == 0 ? in : 1 - in;
+    HIP_TRY(launch_zero(t->bk_ovf, 1, s));
+    HIP_TRY(launch_wide_bulk_fill(t->W, Ch, t->bk_cid[r], t->bk_ent[r], n, t->bk_start, t->bk_ent[1 - r], t->bk_cap,
+                                  t->bk_ovf, t->ctr, s));
+    HIP_TRY(hipMemcpyAsync(t->bk_hovf, t->bk_ovf, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *overflow = *t->bk_hovf;
+    *out = 1 - r;
+    if (*overflow > n)
+        return fail(DYC_ERR_CORRUPT, "bulk fill overflow count %llu exceeds the batch (%llu)",
+                    (unsigned long long)*overflow, (unsigned long long)n);
+    return DYC_OK;
+}
+
+// insert_order 3 after the counters were zeroed: fill in the start bucket, then the other
+// bucket, then the eviction insert (from key and value arrays split out of the last overflow).
+int wbulk_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, hipStream_t s)
+{
+    const Chunks Ch = make_chunks(t->W.D);
+    int rc = wreserve_bulk(t, n, Ch);
+    if (rc) return rc;
+    HIP_TRY(launch_wide_bulk_pack(t->W, Ch, keys, vals, nullptr, n, t->bk_cid[0], t->bk_ent[0], s));
+    int ov = 0, ov2 = 0;
+    uint64_t m1 = 0, m2 = 0;
+    rc = wbulk_round(t, Ch, 0, n, s, &ov, &m1);
+    if (rc || m1 == 0) return rc;
+    const int in2 = 1 - ov;
+    HIP_TRY(launch_wide_bulk_pack(t->W, Ch, nullptr, nullptr, t->bk_ent[ov], m1, t->bk_cid[in2], t->bk_ent[in2], s));
+    rc = wbulk_round(t, Ch, in2, m1, s, &ov2, &m2);
+    if (rc || m2 == 0) return rc;
+    // the other entry buffer (16 B per key of capacity) holds m2 keys and then m2 values
+    uint64_t *const sk = reinterpret_cast<uint64_t *>(t->bk_ent[1 - ov2]);
+    HIP_TRY(launch_wide_split(t->bk_ent[ov2], m2, sk, sk + m2, s));
+    HIP_TRY(launch_wide_insert(t->W, sk, sk + m2, m2, kNoTable, true, t->policy(), t->insert_shape(), t->ctr,
+                               t->sink_at(t->pend_n), s));
+    return DYC_OK;
+}
+
 bool wide_check_config(const dyc_config &cfg)
 {
     const uint32_t d = config_subtables(cfg);
@@ -932,6 +1307,14 @@ int dyc_wide_destroy(dyc_wide *t)
     if (t->hpost) (void)hipHostFree(t->hpost);
     if (t->live) (void)hipFree(t->live);
     if (t->upd) (void)hipFree(t->upd);
+    for (int b = 0; b < 2; ++b) {
+        if (t->bk_cid[b]) (void)hipFree(t->bk_cid[b]);
+        if (t->bk_ent[b]) (void)hipFree(t->bk_ent[b]);
+    }
+    if (t->bk_start) (void)hipFree(t->bk_start);
+    if (t->bk_tmp) (void)hipFree(t->bk_tmp);
+    if (t->bk_ovf) (void)hipFree(t->bk_ovf);
+    if (t->bk_hovf) (void)hipHostFree(t->bk_hovf);
     delete t;
     return DYC_OK;
 }
@@ -957,8 +1340,13 @@ int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     const uint64_t room = t->pend_cap - t->pend_n;
     rc = wzero(t, s);
     if (rc) return rc;
-    HIP_TRY(launch_wide_insert(t->W, keys, vals, n, kNoTable, false, t->policy(), t->insert_shape(), t->ctr,
-                               t->sink_at(t->pend_n), s));
+    if (wuse_bulk(t, n)) {
+        rc = wbulk_insert(t, keys, vals, n, s);
+        if (rc) return rc;
+    } else {
+        HIP_TRY(launch_wide_insert(t->W, keys, vals, n, kNoTable, false, t->policy(), t->insert_shape(), t->ctr,
+                                   t->sink_at(t->pend_n), s));
+    }
     rc = wread(t, s);
     if (rc) return rc;
     const uint64_t rejected = t->hctr->rejected;

@@ -253,3 +253,64 @@ def test_wide_upsert_reaches_keys_held_only_in_the_stash(D):
     st = t.stats()
     assert st["count"] == 39 and st["pending"] == 8
     t.close()
+
+
+def wide_stored(t):
+    """(keys, values) of the subtables and the pending list, sorted by (key, value)."""
+    ks, vs = t.dump()
+    k = np.concatenate([x[x != 0] for x in ks])
+    v = np.concatenate([y[x != 0] for x, y in zip(ks, vs)])
+    pk, pv = t.pending()
+    k = np.concatenate([k, np.asarray(pk, np.uint64)])
+    v = np.concatenate([v, np.asarray(pv, np.uint64)])
+    o = np.lexsort((v, k))
+    return k[o], v[o]
+
+
+@pytest.mark.parametrize("d", (2, 4))
+def test_wide_bulk_fill_insert_multiset_invariant_and_find(D, d):
+    """insert_order 3: chunk sort, fill of free slots, second round in the other bucket, eviction
+    insert for the rest; reserved keys rejected."""
+    n = 1 << 20
+    banner(f"wide bulk fill insert 2^20 at 0.90, subtables={d}")
+    keys = np.concatenate([keys64(n - 4096, 1600 + d), keys64(4096, 1610 + d, 1, 2**32 - 1)])
+    keys = np.unique(keys)
+    vals = vals_of(keys)
+    batch = np.concatenate([keys, np.array([0, 2**64 - 1], dtype=np.uint64)])
+    bv = np.concatenate([vals, np.array([5, 6], dtype=np.uint64)])
+    t = D.DyCuckooWide(int(np.ceil(keys.size / 0.9)), subtables=d, insert_order=3)
+    assert batch.size >= 2 * sum(t.buckets)
+    t.insert(cu(batch), cu(bv))
+    st = t.stats()
+    assert st["count"] == keys.size and st["lost"] == 0 and st["rejected"] == 2
+    ref = oracle_of(t)
+    assert ref.check_invariant() == 0
+    k, v = wide_stored(t)
+    order = np.lexsort((vals, keys))
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+    gv, gh = gfind(t, keys)
+    assert gh.all() and np.array_equal(gv, vals)
+    t.close()
+
+
+def test_wide_bulk_fill_overflow_reaches_the_eviction_insert(D):
+    banner("wide bulk fill: overflow through both rounds and the eviction insert")
+    base = keys64(3000, 1700)
+    batch = np.concatenate([base, np.repeat(base[:6], 11)])  # 12 copies of six keys
+    rng = np.random.default_rng(1701)
+    rng.shuffle(batch)
+    bv = vals_of(batch) ^ np.arange(batch.size, dtype=np.uint64)  # copies carry distinct values
+    t = D.DyCuckooWide(8192, insert_order=3, auto_resize=False)
+    assert batch.size >= 2 * sum(t.buckets)
+    t.insert(cu(batch), cu(bv))
+    st = t.stats()
+    assert st["count"] == batch.size and st["lost"] == 0
+    assert oracle_of(t).check_invariant() == 0
+    k, v = wide_stored(t)
+    o = np.lexsort((bv, batch))
+    assert np.array_equal(k, batch[o]) and np.array_equal(v, bv[o])
+    single = base[6:]  # one copy each: find returns its value
+    want = dict(zip(batch.tolist(), bv.tolist()))
+    gv, gh = gfind(t, single)
+    assert gh.all() and np.array_equal(gv, np.array([want[x] for x in single.tolist()], dtype=np.uint64))
+    t.close()
