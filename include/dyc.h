@@ -74,7 +74,9 @@ typedef struct dyc_config {
                                   fill for batches of at least two keys per bucket: sort the batch
                                   by 512-bucket chunk of its start bucket, fill free slots chunk by
                                   chunk without atomics, the overflow likewise in its other bucket,
-                                  then the eviction insert for the rest (workspace 24 B per key) */
+                                  then the eviction insert for the rest (workspace 24 B per key;
+                                  wide tables 40 B). It reads two overflow counts back, so the
+                                  call waits twice more on `stream` before its final wait */
     uint32_t region_mib;       /* table bytes per bucket region when ordering (default 64) */
     uint32_t subtables;        /* subtable count d, 2..6 (default 4 = TABLE_NUM, dy_hash.h:32); the
                                   pair scheme of each d is in diff_hash_num_pos_fun.cpp:14-188.
