@@ -554,10 +554,8 @@ int bulk_round(dyc_table *t, const Chunks &Ch, int in, uint64_t n, hipStream_t s
 // and read by the caller.
 int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, hipStream_t s)
 {
-    const Chunks Ch = make_chunks(t->D);
-    int rc = reserve_bulk(t, n, Ch);
-    if (rc) return rc;
-    rc = zero_counters(t, s);
+    const Chunks Ch = make_chunks(t->D);  // the workspace was reserved for it
+    int rc = zero_counters(t, s);
     if (rc) return rc;
     HIP_TRY(launch_bulk_pack(t->D, Ch, keys, vals, nullptr, n, t->bk_cid[0], t->bk_ent[0], s));
     int ov = 0;
@@ -983,7 +981,13 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     if (rc) return rc;
     const uint64_t room = t->pend_cap - t->pend_n;
     const uint32_t R = insert_regions(t, n);
-    if (use_bulk_fill(t, n)) {
+    bool bulk = use_bulk_fill(t, n);
+    if (bulk) {
+        rc = reserve_bulk(t, n, make_chunks(t->D));
+        if (rc == DYC_ERR_OOM) bulk = false;  // no room for the workspace: the eviction insert needs none
+        else if (rc) return rc;
+    }
+    if (bulk) {
         rc = bulk_insert(t, keys, vals, n, s);
         if (rc) return rc;
     } else if (R) {

@@ -1218,9 +1218,8 @@ int wbulk_round(dyc_wide *t, const Chunks &Ch, int in, uint64_t n, hipStream_t s
 // bucket, then the eviction insert (from key and value arrays split out of the last overflow).
 int wbulk_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, hipStream_t s)
 {
-    const Chunks Ch = make_chunks(t->W.D);
-    int rc = wreserve_bulk(t, n, Ch);
-    if (rc) return rc;
+    const Chunks Ch = make_chunks(t->W.D);  // the workspace was reserved for it
+    int rc = DYC_OK;
     HIP_TRY(launch_wide_bulk_pack(t->W, Ch, keys, vals, nullptr, n, t->bk_cid[0], t->bk_ent[0], s));
     int ov = 0, ov2 = 0;
     uint64_t m1 = 0, m2 = 0;
@@ -1338,9 +1337,15 @@ int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     rc = wreserve_pending(t, std::max<uint64_t>(65536, n / 16), s);
     if (rc) return rc;
     const uint64_t room = t->pend_cap - t->pend_n;
+    bool bulk = wuse_bulk(t, n);
+    if (bulk) {
+        rc = wreserve_bulk(t, n, make_chunks(t->W.D));
+        if (rc == DYC_ERR_OOM) bulk = false;  // no room for the workspace: the eviction insert needs none
+        else if (rc) return rc;
+    }
     rc = wzero(t, s);
     if (rc) return rc;
-    if (wuse_bulk(t, n)) {
+    if (bulk) {
         rc = wbulk_insert(t, keys, vals, n, s);
         if (rc) return rc;
     } else {
