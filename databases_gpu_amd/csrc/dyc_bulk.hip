@@ -150,30 +150,7 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
                 cnt[j] = 0;
             }
         } else {
-            // lane l of a wave reads part l % 8 of bucket (q / 8); a bucket's 8 parts sit in 8
-            // consecutive lanes, whose two ballot bits per part make its 16-bit mask
-            constexpr uint32_t kPer = kParts / THREADS;  // parts per thread, all loaded before use
-            uint4 v[kPer];
-#pragma unroll
-            for (uint32_t u = 0; u < kPer; ++u) {
-                const uint32_t q = threadIdx.x + u * THREADS;
-                v[u] = q < len * (kSlots / 2) ? src[q] : make_uint4(1u, 1u, 1u, 1u);  // past the end: full
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kPer; ++u) {
-                const uint32_t q = threadIdx.x + u * THREADS;
-                const bool e0 = (v[u].x | v[u].y) == 0u, e1 = (v[u].z | v[u].w) == 0u;
-                const uint64_t m0 = __ballot(e0), m1 = __ballot(e1);
-                const uint32_t lane = __lane_id();
-                if ((lane & 7u) == 0) {
-                    const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
-                    uint32_t m = 0;
-#pragma unroll
-                    for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
-                    fm[q / 8] = m;
-                    cnt[q / 8] = 0;
-                }
-            }
+            for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += blockDim.x) cnt[j] = 0;
         }
         if (threadIdx.x == 0) onext = 0;
         __syncthreads();
@@ -199,6 +176,34 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
             }
         }
         __syncthreads();
+        if constexpr (!DENSE) {
+            // free masks of the buckets that received entries only (a bucket's 8 parts are read
+            // by 8 consecutive lanes and folded by ballot; the others read as full)
+            // lane l of a wave reads part l % 8 of bucket (q / 8); a bucket's 8 parts sit in 8
+            // consecutive lanes, whose two ballot bits per part make its 16-bit mask
+            constexpr uint32_t kPer = kParts / THREADS;  // parts per thread, all loaded before use
+            uint4 v[kPer];
+#pragma unroll
+            for (uint32_t u = 0; u < kPer; ++u) {
+                const uint32_t q = threadIdx.x + u * THREADS;
+                v[u] = q < len * (kSlots / 2) && cnt[q / 8] ? src[q] : make_uint4(1u, 1u, 1u, 1u);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kPer; ++u) {
+                const uint32_t q = threadIdx.x + u * THREADS;
+                const bool e0 = (v[u].x | v[u].y) == 0u, e1 = (v[u].z | v[u].w) == 0u;
+                const uint64_t m0 = __ballot(e0), m1 = __ballot(e1);
+                const uint32_t lane = __lane_id();
+                if ((lane & 7u) == 0) {
+                    const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
+                    uint32_t m = 0;
+#pragma unroll
+                    for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
+                    fm[q / 8] = m;
+                }
+            }
+            __syncthreads();
+        }
         uint32_t excess = stray;
         for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += blockDim.x) {
             const uint32_t f = (uint32_t)__builtin_popcount(fm[j]);
