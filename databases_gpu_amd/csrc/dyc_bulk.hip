@@ -7,10 +7,11 @@
 //      chunk of kChunkBuckets consecutive buckets of one subtable; write (chunk, entry) pairs;
 //   2. sort the pairs by chunk (rocPRIM radix sort over the chunk id's bits);
 //   3. bounds: the first sorted index of every chunk;
-//   4. fill: one workgroup per chunk owns its buckets for the launch. It reads the buckets'
+//   4. fill: one workgroup per chunk owns its buckets for the launch. It takes the buckets'
 //      free-slot masks into LDS, counts the entries per bucket, reserves one range of the
 //      overflow list for the chunk's excess, then gives each entry the next free slot of its
-//      bucket (an LDS counter) with a plain store, or appends it to the overflow list;
+//      bucket (an LDS counter) or appends it to the overflow list. A dense round fills a copy
+//      of the chunk in LDS and writes it back whole; a sparse one stores each entry directly;
 //   5. the overflow is packed again for the key's other table (round 2) and filled the same way;
 //   6. what still overflows goes through the eviction insert (launch_insert_entries).
 // The placement rules that define a key's buckets are unchanged; only the order in which free
@@ -106,8 +107,9 @@ __device__ inline uint32_t nth_set_bit(uint32_t m, uint32_t r)
 // launch (every entry of the chunk targets one of them), so slots are taken with LDS counters.
 // DENSE (a round with at least two entries per bucket): the chunk's 512 buckets are copied into
 // LDS, filled there and written back whole, so every global access is a coalesced 16-byte
-// load or store. Otherwise the free-slot masks are read coalesced (8 lanes per bucket, folded
-// by ballot) and each placed entry is one store into its slot.
+// load or store. Otherwise the entries are counted first, the free-slot masks of the buckets
+// that received any are read coalesced (8 lanes per bucket, folded by ballot), and each placed
+// entry is one store into its slot.
 template <bool DENSE, int THREADS>
 __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, const uint32_t *__restrict__ start,
                                                         const uint64_t *__restrict__ ent, uint64_t n,
