@@ -345,11 +345,11 @@ def run_memsys(args, dev) -> int:
     """The device's random-access ceilings at the size of the 2^28-key table (2.39 GB): random
     128-byte bucket reads/s (one lane per bucket), random 64-bit CAS/s and random 8-byte
     stores/s, 2^28 operations each, and the rate of per-wave atomic adds onto one address and onto
-    64 (libdyc_memsys.so). Prints one JSON line with the ceilings the bulk kernels are compared
+    64 (bench_memsys.so, built from bench_memsys.hip). Prints one JSON line with the ceilings the bulk kernels are compared
     against."""
     import ctypes
 
-    lib = ctypes.CDLL(os.path.join(ROOT, "databases_gpu_amd", "libdyc_memsys.so"))
+    lib = ctypes.CDLL(os.path.join(ROOT, "bench_memsys.so"))
     lib.dyc_memsys_probe.restype = ctypes.c_int
     lib.dyc_memsys_probe.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     nb = 4 * 4660350  # buckets of the 2^28-key table at load factor 0.90

@@ -1,7 +1,7 @@
-// dyc_memsys.hip -- device memory-rate probe (libdyc_memsys.so), the ceilings the hash kernels
-// are judged against: at the size of a table, how many random 128-byte bucket reads, random
-// 64-bit CAS and random 8-byte stores per second the device sustains. A measurement tool, not
-// part of the table library.
+// bench_memsys.hip -- device memory-rate probe of bench.py --workload memsys (bench_memsys.so),
+// the ceilings the hash kernels are judged against: at the size of a table, how many random
+// 128-byte bucket reads, random 64-bit CAS and random 8-byte stores per second the device
+// sustains. A measurement tool for the benchmark, not part of the table library.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 

@@ -9,7 +9,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdyc.so")
-PROBE_LIB = os.path.join(HERE, "libdyc_memsys.so")
 SOURCES = ["dyc_kernels.hip", "dyc_bulk.hip", "dyc_table.hip", "dyc_group.hip", "dyc_server.hip", "dyc_wide.hip"]
 ARCH = os.environ.get("DYC_OFFLOAD_ARCH", "gfx950")
 
@@ -48,12 +47,16 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
-    # the device memory-rate probe: a separate library, not part of the table ABI
-    tmp = PROBE_LIB + ".tmp"
-    cmd = [cc, *flags(), "-shared", os.path.join(CSRC, "dyc_memsys.hip"), "-o", tmp]
-    subprocess.run(cmd, check=True, stderr=None if verbose else subprocess.PIPE)
-    os.replace(tmp, PROBE_LIB)
     return LIB
+
+
+def build_probe(src: str, out: str, verbose: bool = False) -> str:
+    """Compile a stand-alone HIP probe library (not part of libdyc) with the library's flags."""
+    tmp = out + ".tmp"
+    cmd = [hipcc(), *flags(), "-shared", src, "-o", tmp]
+    subprocess.run(cmd, check=True, stderr=None if verbose else subprocess.PIPE)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

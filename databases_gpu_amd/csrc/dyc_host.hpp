@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <immintrin.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #include <atomic>
 #include <chrono>
@@ -22,6 +23,17 @@ namespace host {
 // Record a message for dyc_last_error() and return `code`.
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 const char *last_error();
+
+// Keeps the calling thread's last error message across a step whose failure the caller
+// recovers from (a fallback), so that a call that succeeds leaves no message of its own.
+class ErrorKeep {
+  public:
+    ErrorKeep() { snprintf(msg_, sizeof msg_, "%s", last_error()); }
+    void restore() const { (void)fail(0, "%s", msg_); }
+
+  private:
+    char msg_[512];
+};
 
 constexpr uint32_t kMaxBuckets = 1u << 29;  // packed bucket locations use 29 bits
 

@@ -200,12 +200,37 @@ int release_block(dyc_table *t, uint64_t *p, bool in_slab)
     return DYC_OK;
 }
 
-// hipMalloc for a resize; when it fails, wait for the frees still queued and try once more.
-hipError_t resize_malloc(dyc_table *t, void **p, size_t bytes)
+// Free the bulk-fill and region-ordering workspaces (grown on demand, kept between calls); the
+// next call that needs them allocates them again. The caller has synchronised the table's stream.
+void release_workspace(dyc_table *t)
+{
+    for (int b = 0; b < 2; ++b) {
+        if (t->bk_cid[b]) (void)hipFree(t->bk_cid[b]);
+        if (t->bk_ent[b]) (void)hipFree(t->bk_ent[b]);
+        t->bk_cid[b] = nullptr;
+        t->bk_ent[b] = nullptr;
+    }
+    t->bk_cap = 0;
+    if (t->bk_start) (void)hipFree(t->bk_start);
+    t->bk_start = nullptr;
+    t->bk_start_cap = 0;
+    if (t->bk_tmp) (void)hipFree(t->bk_tmp);
+    t->bk_tmp = nullptr;
+    t->bk_tmp_bytes = 0;
+    if (t->part) (void)hipFree(t->part);
+    t->part = nullptr;
+    t->part_cap = 0;
+}
+
+// hipMalloc for a resize; when it fails, wait for the frees still queued, release the idle
+// insert workspaces (up to 24 B per key of the largest bulk-filled batch) and try once more.
+hipError_t resize_malloc(dyc_table *t, void **p, size_t bytes, hipStream_t s)
 {
     if (hipMalloc(p, bytes) == hipSuccess) return hipSuccess;
     (void)hipGetLastError();
     t->reclaim.drain();
+    if (hipStreamSynchronize(s) == hipSuccess) release_workspace(t);
+    (void)hipGetLastError();
     const hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) (void)hipGetLastError();
     return e;
@@ -291,7 +316,7 @@ int do_resize_up(dyc_table *t, hipStream_t s)
     if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^29 buckets", sub);
     uint64_t *dst = nullptr;
     const size_t bytes = (size_t)old_nb * 2 * kBucketBytes;
-    if (resize_malloc(t, reinterpret_cast<void **>(&dst), bytes) != hipSuccess)
+    if (resize_malloc(t, reinterpret_cast<void **>(&dst), bytes, s) != hipSuccess)
         return fail(DYC_ERR_OOM, "hipMalloc of %zu bytes for upsize failed", bytes);
     int rc = zero_counters(t, s);
     if (rc) return rc;
@@ -357,7 +382,7 @@ int do_resize_down(dyc_table *t, hipStream_t s)
         return fail(DYC_ERR_ODD_BUCKETS, "downsize of subtable %u refused: %u buckets is odd", sub, old_nb);
     const uint32_t new_nb = old_nb / 2;
     uint64_t *dst = nullptr;
-    if (resize_malloc(t, reinterpret_cast<void **>(&dst), (size_t)new_nb * kBucketBytes) != hipSuccess)
+    if (resize_malloc(t, reinterpret_cast<void **>(&dst), (size_t)new_nb * kBucketBytes, s) != hipSuccess)
         return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
     if (res_tiles(new_nb) > t->live_cap) {  // the residual list's tile flags, grown as needed
         if (t->live) (void)hipFree(t->live);
@@ -435,7 +460,7 @@ uint32_t insert_regions(const dyc_table *t, uint64_t n)
     return (uint32_t)std::max<uint64_t>(R, 1);
 }
 
-int reserve_partition(dyc_table *t, uint64_t n)
+int reserve_partition(dyc_table *t, uint64_t n, hipStream_t s)
 {
     if (!t->part_counts) {
         if (hipMalloc(&t->part_counts, 2 * kMaxRegions * sizeof(unsigned long long)) != hipSuccess) {
@@ -445,7 +470,7 @@ int reserve_partition(dyc_table *t, uint64_t n)
     }
     if (n <= t->part_cap) return DYC_OK;
     if (t->part) {
-        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipStreamSynchronize(s));  // the table's work is ordered on s
         HIP_TRY(hipFree(t->part));
         t->part = nullptr;
         t->part_cap = 0;
@@ -479,12 +504,13 @@ int bulk_alloc(void **p, size_t bytes, const char *what)
 }
 
 // Grow-only workspace of the bulk fill for n keys over the chunks Ch.
-int reserve_bulk(dyc_table *t, uint64_t n, const Chunks &Ch)
+int reserve_bulk(dyc_table *t, uint64_t n, const Chunks &Ch, hipStream_t s)
 {
     size_t tmp = 0;
     HIP_TRY(bulk_sort_bytes(n, Ch.bits, &tmp));
     const bool grow = n > t->bk_cap || Ch.total + 1ull > t->bk_start_cap || tmp > t->bk_tmp_bytes;
-    if (grow) HIP_TRY(hipDeviceSynchronize());  // a buffer being replaced may still be in use
+    // a buffer being replaced may still be in use by the table's earlier work, ordered on s
+    if (grow) HIP_TRY(hipStreamSynchronize(s));
     int rc = DYC_OK;
     if (n > t->bk_cap) {
         for (int b = 0; b < 2; ++b) {
@@ -983,15 +1009,20 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     const uint32_t R = insert_regions(t, n);
     bool bulk = use_bulk_fill(t, n);
     if (bulk) {
-        rc = reserve_bulk(t, n, make_chunks(t->D));
-        if (rc == DYC_ERR_OOM) bulk = false;  // no room for the workspace: the eviction insert needs none
-        else if (rc) return rc;
+        const dyc::host::ErrorKeep keep;
+        rc = reserve_bulk(t, n, make_chunks(t->D), s);
+        if (rc == DYC_ERR_OOM) {  // no room for the workspace: the eviction insert needs none
+            bulk = false;
+            keep.restore();  // the call goes on, so it leaves no failure message
+        } else if (rc) {
+            return rc;
+        }
     }
     if (bulk) {
         rc = bulk_insert(t, keys, vals, n, s);
         if (rc) return rc;
     } else if (R) {
-        rc = reserve_partition(t, n);
+        rc = reserve_partition(t, n, s);
         if (rc) return rc;
         HIP_TRY(launch_zero(t->part_counts, kMaxRegions, s));
         HIP_TRY(launch_partition(t->D, make_regions(t->D, R), keys, vals, n, t->part_counts,

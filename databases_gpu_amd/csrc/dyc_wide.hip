@@ -877,12 +877,34 @@ int wrelease_block(dyc_wide *t, uint8_t *own)
     return DYC_OK;
 }
 
-// hipMalloc for a resize; when it fails, wait for the frees still queued and try once more.
-hipError_t wresize_malloc(dyc_wide *t, uint8_t **p, size_t bytes)
+// Free the bulk-fill workspace (grown on demand, kept between calls); the next bulk fill
+// allocates it again. The caller has synchronised the table's stream.
+void wrelease_workspace(dyc_wide *t)
+{
+    for (int b = 0; b < 2; ++b) {
+        if (t->bk_cid[b]) (void)hipFree(t->bk_cid[b]);
+        if (t->bk_ent[b]) (void)hipFree(t->bk_ent[b]);
+        t->bk_cid[b] = nullptr;
+        t->bk_ent[b] = nullptr;
+    }
+    t->bk_cap = 0;
+    if (t->bk_start) (void)hipFree(t->bk_start);
+    t->bk_start = nullptr;
+    t->bk_start_cap = 0;
+    if (t->bk_tmp) (void)hipFree(t->bk_tmp);
+    t->bk_tmp = nullptr;
+    t->bk_tmp_bytes = 0;
+}
+
+// hipMalloc for a resize; when it fails, wait for the frees still queued, release the idle
+// bulk-fill workspace (up to 40 B per key of the largest bulk-filled batch) and try once more.
+hipError_t wresize_malloc(dyc_wide *t, uint8_t **p, size_t bytes, hipStream_t s)
 {
     if (hipMalloc(p, bytes) == hipSuccess) return hipSuccess;
     (void)hipGetLastError();
     t->reclaim.drain();
+    if (hipStreamSynchronize(s) == hipSuccess) wrelease_workspace(t);
+    (void)hipGetLastError();
     const hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) (void)hipGetLastError();
     return e;
@@ -972,7 +994,7 @@ int wresize_up(dyc_wide *t, hipStream_t s)
     const uint32_t old_nb = t->W.D.nb[sub];
     if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^29 buckets", sub);
     uint8_t *dst = nullptr;
-    if (wresize_malloc(t, &dst, 2 * wide_bytes(old_nb * 2)) != hipSuccess)
+    if (wresize_malloc(t, &dst, 2 * wide_bytes(old_nb * 2), s) != hipSuccess)
         return fail(DYC_ERR_OOM, "hipMalloc for upsize failed");
     int rc = wzero(t, s);
     if (rc) return rc;
@@ -1040,7 +1062,7 @@ int wresize_down(dyc_wide *t, hipStream_t s)
         return fail(DYC_ERR_ODD_BUCKETS, "downsize of subtable %u refused: %u buckets is odd", sub, old_nb);
     const uint32_t new_nb = old_nb / 2;
     uint8_t *dst = nullptr;
-    if (wresize_malloc(t, &dst, 2 * wide_bytes(new_nb)) != hipSuccess)
+    if (wresize_malloc(t, &dst, 2 * wide_bytes(new_nb), s) != hipSuccess)
         return fail(DYC_ERR_OOM, "hipMalloc for downsize failed");
     if (res_tiles(new_nb) > t->live_cap) {  // the residual list's tile flags, grown as needed
         if (t->live) (void)hipFree(t->live);
@@ -1151,12 +1173,13 @@ int walloc(void **p, size_t bytes, const char *what)
     return DYC_OK;
 }
 
-int wreserve_bulk(dyc_wide *t, uint64_t n, const Chunks &Ch)
+int wreserve_bulk(dyc_wide *t, uint64_t n, const Chunks &Ch, hipStream_t s)
 {
     size_t tmp = 0;
     HIP_TRY(bulk_sort_wide_bytes(n, Ch.bits, &tmp));
+    // a buffer being replaced may still be in use by the table's earlier work, ordered on s
     if (n > t->bk_cap || Ch.total + 1ull > t->bk_start_cap || tmp > t->bk_tmp_bytes)
-        HIP_TRY(hipDeviceSynchronize());  // a buffer being replaced may still be in use
+        HIP_TRY(hipStreamSynchronize(s));
     int rc = DYC_OK;
     if (n > t->bk_cap) {
         for (int b = 0; b < 2; ++b) {
@@ -1344,9 +1367,14 @@ int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     const uint64_t room = t->pend_cap - t->pend_n;
     bool bulk = wuse_bulk(t, n);
     if (bulk) {
-        rc = wreserve_bulk(t, n, make_chunks(t->W.D));
-        if (rc == DYC_ERR_OOM) bulk = false;  // no room for the workspace: the eviction insert needs none
-        else if (rc) return rc;
+        const dyc::host::ErrorKeep keep;
+        rc = wreserve_bulk(t, n, make_chunks(t->W.D), s);
+        if (rc == DYC_ERR_OOM) {  // no room for the workspace: the eviction insert needs none
+            bulk = false;
+            keep.restore();  // the call goes on, so it leaves no failure message
+        } else if (rc) {
+            return rc;
+        }
     }
     rc = wzero(t, s);
     if (rc) return rc;

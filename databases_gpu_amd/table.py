@@ -491,7 +491,16 @@ class HashAPI:
         return lo, hi
 
     def hash_insert_batch(self) -> None:
+        """api.cpp:353-369: upsize while count + batch_size exceeds upper * capacity (the bound
+        counts the full batch size whatever the slice holds, and applies whatever auto_resize
+        says), then insert the slice."""
         lo, hi = self._slice(self.batch_num_to_op)
+        upper = self.table.config()["upper_bound"]
+        while True:
+            st = self.table.stats()
+            if st["count"] + self.batch_size <= upper * st["capacity"]:
+                break
+            self.table.resize_up()
         self.table.insert(self._pool_k[lo:hi], self._pool_v[lo:hi])
         self.batch_num_to_op += 1
 
@@ -501,9 +510,15 @@ class HashAPI:
         self.batch_search_num_to_op += 1
 
     def hash_delete_batch(self) -> None:
+        """api.cpp:466-506: erase the slice, then downsize once when count < lower * capacity
+        and capacity > min_capacity (with auto_resize the erase has applied that rule itself)."""
         lo, hi = self._slice(self.batch_del_num_to_op)
         self.table.erase(self._pool_k[lo:hi])
         self.batch_del_num_to_op += 1
+        cfg, st = self.table.config(), self.table.stats()
+        if (not cfg["auto_resize"] and st["count"] < cfg["lower_bound"] * st["capacity"]
+                and st["capacity"] > cfg["min_capacity"]):
+            self.table.resize_down()
 
     def close(self):
         self.table.close()

@@ -155,7 +155,9 @@ int dyc_resize_down(dyc_table *t, dyc_stream stream);
  * batch moves in chunks through library-owned pinned buffers: the copy of one chunk overlaps the
  * kernel of the previous one, and results stream back on a second copy stream. Same semantics
  * and resize policy as the device-pointer calls; synchronous (they return when the results are
- * in the caller's arrays). */
+ * in the caller's arrays). dyc_insert_host places each staged chunk with the eviction insert
+ * whatever insert_order says (a chunk is too small for the bulk fill); the stored set is the
+ * same. */
 int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
 int dyc_find_host(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *hit, uint64_t n, dyc_stream stream);
 int dyc_erase_host(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_t n, dyc_stream stream);

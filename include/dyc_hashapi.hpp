@@ -99,11 +99,20 @@ class hashAPI {
         hip(hipMemcpy(pool_v_, value, sizeof(uint32_t) * (size_t)size, hipMemcpyHostToDevice));
         return true;
     }
+    // api.cpp:353-369: while the table could not take a whole BATCH_SIZE batch under the upper
+    // fill bound, upsize (the reference's api_resize, then the call re-enters itself); then
+    // insert the slice. The bound counts BATCH_SIZE whatever the slice holds, and applies
+    // whatever the table's auto_resize says.
     void hash_insert_batch()
     {
         TYPE lo, n;
-        slice(batch_num_to_op_++, lo, n);
+        slice(batch_num_to_op_, lo, n);
+        const dyc_config c = config();
+        for (dyc_stats s = stats(); (double)(s.count + BATCH_SIZE) > c.upper_bound * (double)s.capacity;
+             s = stats())
+            check(dyc_resize_up(t_, nullptr), "dyc_resize_up");
         check(dyc_insert(t_, pool_k_ + lo, pool_v_ + lo, n, nullptr), "dyc_insert");
+        ++batch_num_to_op_;
     }
     void hash_search_batch()
     {
@@ -112,14 +121,30 @@ class hashAPI {
         check(dyc_find(t_, pool_k_ + lo, check_, nullptr, n, nullptr), "dyc_find");
         hip(hipDeviceSynchronize());
     }
+    // api.cpp:466-506: erase the slice, then downsize once when the count is below the lower
+    // fill bound and the capacity above min_capacity (with auto_resize, dyc_erase has applied
+    // this same rule already). A subtable with an odd bucket count is left as it is.
     void hash_delete_batch()
     {
         TYPE lo, n;
-        slice(batch_del_num_to_op_++, lo, n);
+        slice(batch_del_num_to_op_, lo, n);
         check(dyc_erase(t_, pool_k_ + lo, nullptr, n, nullptr), "dyc_erase");
+        ++batch_del_num_to_op_;
+        const dyc_config c = config();
+        const dyc_stats s = stats();
+        if (!c.auto_resize && (double)s.count < c.lower_bound * (double)s.capacity && s.capacity > c.min_capacity) {
+            const int rc = dyc_resize_down(t_, nullptr);
+            if (rc != DYC_ERR_ODD_BUCKETS) check(rc, "dyc_resize_down");
+        }
     }
     void batch_reset() { batch_num_to_op_ = batch_search_num_to_op_ = batch_del_num_to_op_ = 0; }
 
+    dyc_config config() const
+    {
+        dyc_config c;
+        check(dyc_get_config(t_, &c), "dyc_get_config");
+        return c;
+    }
     dyc_stats stats() const
     {
         dyc_stats s;

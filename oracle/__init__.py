@@ -302,6 +302,25 @@ class OracleTable:
         self.last_kicks = kicks
         return nfail, int(rej.value)
 
+    def insert_batch(self, keys, vals, batch_size: int):
+        """hashAPI::hash_insert_batch (api.cpp:353-369): while count + batch_size exceeds
+        upper * capacity, upsize (api_resize, api.cpp:175-181, then the call re-enters itself),
+        then insert the slice. The bound always uses the full batch size, whatever the slice
+        holds, and applies whatever auto_resize says."""
+        while self.count + batch_size > self.upper * self.capacity:
+            self.resize_up()
+        return self.insert(keys, vals)
+
+    def erase_batch(self, keys):
+        """hashAPI::hash_delete_batch (api.cpp:466-506): erase the slice, then downsize once when
+        count < lower * capacity and capacity > min_capacity (api.cpp:494-505). With auto_resize
+        set, erase() has applied that same rule already, so it is not applied twice."""
+        deleted = self.erase(keys)
+        if (not self.auto_resize and self.count < self.lower * self.capacity
+                and self.capacity > self.min_capacity):
+            self.resize_down()
+        return deleted
+
     def insert_mt(self, keys, vals, threads: int = 0):
         """Multithreaded insert (CPU timing baseline): same final key->value set as insert(),
         placement depends on thread timing. Returns (failures_before_retry, rejected)."""

@@ -3,6 +3,7 @@
 // "<= 2% mismatches" tolerance. Prints one line per case; exit status = number of failed cases.
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <random>
 #include <set>
 #include <vector>
@@ -142,8 +143,55 @@ static bool batch_test()
     return ok;
 }
 
-int main()
+// The batch API's resize policy (api.cpp:353-369, 466-506) on a table sized far below its data:
+// 24 batches in, then 20 batches out, with the reference's fill bounds 0.8 / 0.5 (test_uilt.h:1-2)
+// and auto_resize off. Prints the subtable sizes and the count after every batch, which the
+// caller compares with the oracle's replay of the same batch sizes, and checks every key.
+static bool batch_resize_trace()
 {
+    const TYPE batches = 24, out = 20, n = batches * hashAPI::BATCH_SIZE;
+    auto key = unique_keys(n, 777);
+    std::vector<TYPE> value(n), chck(n, 0);
+    for (TYPE i = 0; i < n; ++i) value[i] = i + 1;
+    dyc_config cfg;
+    dyc_config_init(&cfg);
+    cfg.upper_bound = 0.8;
+    cfg.lower_bound = 0.5;
+    cfg.min_capacity = 0;
+    hashAPI h(1u << 18, &cfg);
+    h.set_data_to_gpu(key.data(), value.data(), n);
+    auto show = [&](const char *op, TYPE b) {
+        const dyc_stats s = h.stats();
+        printf("SIZES %s %u", op, b);
+        for (uint32_t i = 0; i < s.subtables; ++i) printf(" %u", s.buckets[i]);
+        printf(" count %llu\n", (unsigned long long)s.count);
+    };
+    show("start", 0);
+    for (TYPE b = 0; b < batches; ++b) {
+        h.hash_insert_batch();
+        show("insert", b);
+    }
+    h.hash_search(key.data(), chck.data(), n);
+    bool ok = chck == value;
+    for (TYPE b = 0; b < out; ++b) {
+        h.hash_delete_batch();
+        show("delete", b);
+    }
+    std::fill(chck.begin(), chck.end(), 0);
+    h.hash_search(key.data(), chck.data(), n);
+    const TYPE gone = out * hashAPI::BATCH_SIZE;
+    for (TYPE i = 0; i < n; ++i) ok &= (i < gone) ? chck[i] == 0 : chck[i] == value[i];
+    const dyc_stats s = h.stats();
+    return ok && s.count == n - gone && s.pending == 0 && s.lost == 0 && s.resizes_up > 0 && s.resizes_down > 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc > 1 && strcmp(argv[1], "batch-resize") == 0) {
+        report("BatchApiResize", batch_resize_trace());
+        printf("%d failed\n", failures);
+        return failures;
+    }
     const TYPE sizes[] = {100, 200, 1024, 2048, 4096, 1u << 16, 1u << 20};  // Main_backup_0910.cpp:210
     for (TYPE n : sizes) {
         char name[64];

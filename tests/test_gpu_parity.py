@@ -512,18 +512,53 @@ def test_large_insert_find_properties(D):
 
 
 # ---------------------------------------------------------------------- C++ drop-in driver
-def test_reference_cases_through_cpp_hashapi(D, tmp_path):
-    banner("C++ hashAPI driver (reference gtest cases)")
+@pytest.fixture(scope="module")
+def driver(D, tmp_path_factory):
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-    exe = tmp_path / "driver"
+    exe = tmp_path_factory.mktemp("driver") / "driver"
     libdir = os.path.join(ROOT, "databases_gpu_amd")
     subprocess.run([hipcc, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cpp", "hashapi_driver.cpp"), "-L", libdir, "-ldyc",
                     f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True, capture_output=True)
-    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    return str(exe)
+
+
+def test_reference_cases_through_cpp_hashapi(driver):
+    banner("C++ hashAPI driver (reference gtest cases)")
+    r = subprocess.run([driver], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failed" in r.stdout
+
+
+def test_hashapi_batch_resize_policy_matches_oracle(driver):
+    """The C++ hashAPI's batch calls on a table sized far below its data (2^18-slot hint, 24
+    batches of BATCH_SIZE in, 20 out, bounds 0.8 / 0.5, auto_resize off): the subtable sizes and
+    the count after every batch equal the oracle's replay of hash_insert_batch /
+    hash_delete_batch (api.cpp:353-369, 466-506). Sizes depend only on the counts, so the oracle
+    replays the same batch sizes with its own keys."""
+    banner("C++ hashAPI batch resize policy vs oracle")
+    r = subprocess.run([driver, "batch-resize"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr
+    got = []
+    for line in r.stdout.splitlines():
+        if line.startswith("SIZES "):
+            f = line.split()
+            got.append((f[1], int(f[2]), [int(x) for x in f[3:-2]], int(f[-1])))
+    B = 100000
+    tb = O.OracleTable(1 << 18, upper=0.8, lower=0.5, min_capacity=0, rescue_kicks=128)
+    keys = unique_keys(24 * B, 777)
+    vals = vals_of(keys)
+    want = [("start", 0, list(tb.nb), 0)]
+    for b in range(24):
+        tb.insert_batch(keys[b * B:(b + 1) * B], vals[b * B:(b + 1) * B], B)
+        want.append(("insert", b, list(tb.nb), tb.count))
+    for b in range(20):
+        tb.erase_batch(keys[b * B:(b + 1) * B])
+        want.append(("delete", b, list(tb.nb), tb.count))
+    assert [w[0] for w in tb.resizes].count("up") > 0 and [w[0] for w in tb.resizes].count("down") > 0
+    assert got == want
 
 
 def test_dynamic_stream_small_matches_oracle(D):

@@ -293,6 +293,35 @@ def test_auto_resize_policy(oracle_lib):
     assert not h[:250].any() and h[250:].all() and np.array_equal(v[250:], vals[250:500])
 
 
+@pytest.mark.parametrize("auto", [False, True])
+def test_batch_api_resize_policy(oracle_lib, auto):
+    """hash_insert_batch bounds the fill with the full BATCH_SIZE whatever the slice holds
+    (api.cpp:359-369); hash_delete_batch downsizes once under the lower bound (api.cpp:494-505),
+    also when auto_resize applies the same rule inside the erase. Worked by hand: 4 x 8 buckets =
+    512 slots, upper 0.8 -> 409.6; batch size 100."""
+    tb = O.OracleTable(nb=[8, 8, 8, 8], auto_resize=auto, upper=0.8, lower=0.5, min_capacity=0,
+                       rescue_kicks=128)
+    keys = unique_keys(400, seed=12)
+    vals = splitmix32(keys)
+    sizes = []
+    lo = 0
+    for m in (100, 100, 100, 5, 10, 10):  # slices: the last three are short
+        tb.insert_batch(keys[lo:lo + m], vals[lo:lo + m], 100)
+        lo += m
+        sizes.append(list(tb.nb))
+    # 300 + 100 = 400 and 305 + 100 = 405 fit under 409.6; 315 + 100 = 415 does not (a slice-sized
+    # bound, 315 + 10, would not have grown the table)
+    assert sizes == [[8, 8, 8, 8]] * 5 + [[16, 8, 8, 8]]
+    assert tb.count == 325
+    tb.erase_batch(keys[:100])  # 225 < 0.5 * 640 -> down once (choose_down picks subtable 0)
+    assert tb.nb == [8, 8, 8, 8] and tb.count == 225  # 225 < 256 still, but only one downsize
+    tb.erase_batch(keys[100:200])  # 125 < 256 -> down: all equal, so the last subtable
+    assert tb.nb == [8, 8, 8, 4]
+    assert [r[0] for r in tb.resizes] == ["up", "down", "down"]
+    v, h = tb.find(keys[:325])
+    assert not h[:200].any() and h[200:].all() and np.array_equal(v[200:], vals[200:325])
+
+
 def test_stash_answers_keys_missing_from_both_buckets(oracle_lib):
     """The pending list is the table's stash: 40 copies of one key fill its 32 bucket slots and
     leave 8 pending; after two erases clear its pos1 and pos2 buckets (the first bucket holding
