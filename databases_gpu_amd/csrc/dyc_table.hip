@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -599,6 +600,38 @@ int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64
     return DYC_OK;
 }
 
+// The counter block of dyc_insert_noalloc: a library-static device array instead of a table's
+// own block, so that the call allocates nothing. Each device holds its own copy until the
+// process ends. In host code `scratch` names a host-side stand-in, not a device address: kernels
+// and copies take the address hipGetSymbolAddress gives for the device (noalloc_counters).
+static __device__ __align__(256) unsigned long long scratch[kCounterWords];
+static_assert(kCounterBytes == DYC_NOALLOC_SCRATCH_BYTES, "DYC_NOALLOC_SCRATCH_BYTES is the counter block's size");
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+Counters *g_scratch[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on `device` (the current device), looked up once per device.
+int noalloc_counters(int device, Counters **out)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        return fail(DYC_ERR_INVALID, "device %d is beyond the %d devices dyc_insert_noalloc serves", device,
+                    kMaxScratchDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(DYC_ERR_HIP, "hipGetSymbolAddress of the static counter block failed: %s",
+                        hipGetErrorString(e));
+        }
+        g_scratch[device] = static_cast<Counters *>(p);
+    }
+    *out = g_scratch[device];
+    return DYC_OK;
+}
+
 bool valid_lanes(uint32_t l) { return l == 4 || l == 8 || l == 16; }
 bool valid_insert_lanes(uint32_t l) { return l == 1 || l == 2 || valid_lanes(l); }
 
@@ -946,7 +979,7 @@ int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out
             if (launch_zero(reinterpret_cast<unsigned long long *>(sub[i]), (uint64_t)nb * kSlots, nullptr) != hipSuccess)
                 rc = fail(DYC_ERR_HIP, "zeroing the table failed");
     }
-    if (rc == DYC_OK) rc = reserve_pending(t, 1024, nullptr);
+    if (rc == DYC_OK) rc = reserve_pending(t, DYC_NOALLOC_MAX_KEYS, nullptr);
     if (rc == DYC_OK && hipStreamSynchronize(nullptr) != hipSuccess) rc = fail(DYC_ERR_HIP, "create sync failed");
     if (rc != DYC_OK) {
         char keep[sizeof g_err];
@@ -1047,6 +1080,43 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     if (rc) return rc;
     if (t->pend_n > pend_before) return place_pending(t, s, pend_before);
     return DYC_OK;
+}
+
+int dyc_insert_noalloc(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    if (n > DYC_NOALLOC_MAX_KEYS)
+        return fail(DYC_ERR_INVALID, "dyc_insert_noalloc takes at most %u keys per call (got %llu)", DYC_NOALLOC_MAX_KEYS,
+                    (unsigned long long)n);
+    const uint64_t room = t->pend_cap - t->pend_n;
+    if (n > room)
+        return fail(DYC_ERR_INVALID,
+                    "dyc_insert_noalloc: %llu keys, but the pending list has room for %llu entries and cannot grow "
+                    "without allocating (dyc_insert re-places pending entries and grows it)",
+                    (unsigned long long)n, (unsigned long long)room);
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    t->st.last_kicks = 0;
+    if (n == 0) return DYC_OK;
+    Counters *C = nullptr;
+    int rc = noalloc_counters(t->device, &C);
+    if (rc) return rc;
+    // the block keeps the last call's contents (a call that failed may have left it dirty)
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(C), kCounterWords, s));
+    HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(), C,
+                          sink_at(t, t->pend_n), s));
+    // read back as read_counters does, from the static block (the fold also clears it)
+    const unsigned long long seq = ++t->post_seq;
+    HIP_TRY(launch_fold_post(C, t->hpost, seq, s));
+    HIP_TRY(await_post(t->hpost, seq, s));
+    memcpy(t->hctr, const_cast<const unsigned long long *>(t->hpost) + 1, sizeof(Counters));
+    const uint64_t rejected = t->hctr->rejected;
+    t->st.rejected += rejected;
+    t->count += n - rejected;
+    rc = absorb_insert_counters(t, room);
+    t->st.pending = t->pend_n;
+    return rc;
 }
 
 int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, uint8_t *updated,

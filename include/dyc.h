@@ -119,6 +119,27 @@ int dyc_destroy(dyc_table *t);
  * returns, every kernel queued on `stream` before and by the call has finished. */
 int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
 
+/* Insert without allocating. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Keys are placed as dyc_insert places them with the eviction insert
+ * (insert_order 0; the bulk fill's workspace is never used), and the call makes no hipMalloc,
+ * hipFree or other allocating call:
+ *   - its counter block is a library-static device array of DYC_NOALLOC_SCRATCH_BYTES bytes (one
+ *     copy per device, held until the process ends), not the table's own block;
+ *   - nothing is resized (auto_resize is not applied here). Entries that exhaust the eviction
+ *     bound join the table's stash (dyc_stats.pending), where find, erase and upsert see them;
+ *     a later dyc_insert or dyc_resize_up + dyc_insert re-places them.
+ * DYC_ERR_INVALID, with the table unchanged, for n > DYC_NOALLOC_MAX_KEYS or n larger than the
+ * free room of the table's pending list (dyc_create reserves DYC_NOALLOC_MAX_KEYS entries): the
+ * list cannot grow here, and every entry must keep a place.
+ * Every call on one device shares that device's counter block, which keeps the last call's
+ * contents (each call clears it first): calls that overlap on one GPU -- from other host threads
+ * or on other streams, for any table -- race, and the caller must serialise them. Waits once, as
+ * dyc_insert. */
+#define DYC_NOALLOC_MAX_KEYS 65536u
+#define DYC_NOALLOC_SCRATCH_BYTES 16640u
+int dyc_insert_noalloc(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
+
 /* Insert with update (the paper's insert, tkde-revision/secs/cuckoo.tex:127-129; the reference
  * code has it switched off, dy_hash.h:42 update_on 0). A key already stored gets vals[i] in every
  * copy held by the first bucket (pos1, then pos2) that holds it -- what dyc_find returns -- and
