@@ -111,7 +111,72 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one plain copy kernel of 2^log2n int32 (4 B read + 4 B written per element) "
                          "before the steps, as the reference point for hardware byte counters")
+    ap.add_argument("--traffic-csv", nargs="+", default=None, metavar="CSV",
+                    help="no GPU: read rocprofv3 --pmc counter_collection CSVs of one bulk run made with "
+                         "--calibrate-copy --steps 1 --warmup 0 (FETCH_SIZE, WRITE_SIZE, and optionally "
+                         "TCC_HIT_sum/TCC_MISS_sum) and write HBM bytes per launch to --json-out")
     return ap.parse_args()
+
+
+# ---------------------------------------------------------------- hardware byte counters
+def traffic_from_csv(paths, log2n: int, cfg: dict) -> dict:
+    """HBM bytes of the bulk step's launches from rocprofv3 --pmc counter CSVs (one counter set per
+    file) of `bench.py --calibrate-copy --steps 1 --warmup 0`: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled by the factor the same counter shows for the calibration copy kernel
+    (2^log2n int32 read and written), summed per stage of the insert; L2 hit rate of each find from
+    TCC_HIT_sum / TCC_MISS_sum when present."""
+    import csv
+    import collections
+
+    disp = collections.OrderedDict()  # dispatch id -> {"name", counters...}
+    for path in paths:
+        for r in csv.DictReader(open(path)):
+            d = disp.setdefault(int(r["Dispatch_Id"]), {"name": r["Kernel_Name"]})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    seq = [disp[k] for k in sorted(disp)]
+    n = 1 << log2n
+    i0 = next(i for i, d in enumerate(seq) if "pack_kernel<false" in d["name"])
+    copy = [d for d in seq[:i0] if "neg" in d["name"].lower()][-1]  # the calibration copy, torch.neg
+    fetch_f = 4 * n / (copy["FETCH_SIZE"] * 1024)
+    write_f = 4 * n / (copy["WRITE_SIZE"] * 1024)
+
+    def short(name):
+        name = name.replace("(anonymous namespace)::", "").replace("void ", "").replace("dyc::", "")
+        return name.split("(")[0]
+
+    def nbytes(d):
+        return d.get("FETCH_SIZE", 0.0) * 1024 * fetch_f + d.get("WRITE_SIZE", 0.0) * 1024 * write_f
+
+    # the step: clear, insert (first pack of a batch .. the counter read-back), uniform finds, Zipf finds
+    i1 = next(i for i in range(i0, len(seq)) if "fold_post_kernel" in seq[i]["name"])
+    stages = collections.OrderedDict()
+    for d in seq[i0:i1 + 1]:
+        st = stages.setdefault(short(d["name"]), {"launches": 0, "fetch_bytes": 0.0, "write_bytes": 0.0})
+        st["launches"] += 1
+        st["fetch_bytes"] += d.get("FETCH_SIZE", 0.0) * 1024 * fetch_f
+        st["write_bytes"] += d.get("WRITE_SIZE", 0.0) * 1024 * write_f
+    for st in stages.values():
+        st["bytes_per_key"] = round((st["fetch_bytes"] + st["write_bytes"]) / n, 2)
+        st["fetch_bytes"], st["write_bytes"] = int(st["fetch_bytes"]), int(st["write_bytes"])
+    finds = [d for d in seq[i1 + 1:] if "probe_kernel" in d["name"]][:2]
+    insert_bytes = sum(nbytes(d) for d in seq[i0:i1 + 1])
+    out = {
+        "_note": "HBM bytes per launch of the large bulk step (2^%d keys, load factor 0.90) from rocprofv3 --pmc "
+                 "FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the factor the same counter shows "
+                 "for a plain copy kernel of 2^%d int32 (bench.py --calibrate-copy). FETCH_SIZE counts what L2 "
+                 "fetches from the fabric, so a line the MALL serves counts as fetched. Written by bench.py "
+                 "--traffic-csv." % (log2n, log2n),
+        "calibration": {"fetch_factor": round(fetch_f, 5), "write_factor": round(write_f, 5)},
+        "config": cfg,
+        "insert": {"log2n": log2n, "bytes_per_launch": int(insert_bytes),
+                   "bytes_per_op": round(insert_bytes / n, 2), "stages": stages},
+    }
+    for name, d in zip(("find_uniform", "find_zipf"), finds):
+        e = {"log2n": log2n, "bytes_per_launch": int(nbytes(d)), "bytes_per_op": round(nbytes(d) / n, 2)}
+        if "TCC_HIT_sum" in d and "TCC_MISS_sum" in d:
+            e["l2_hit_rate"] = round(d["TCC_HIT_sum"] / max(1.0, d["TCC_HIT_sum"] + d["TCC_MISS_sum"]), 4)
+        out[name] = e
+    return out
 
 
 def pct(xs, q):
@@ -848,6 +913,16 @@ def run_batched(args, dev) -> int:
 
 def main() -> int:
     args = parse()
+    if args.traffic_csv:
+        cfg = {"insert_order": args.insert_order, "insert_lanes": args.insert_lanes,
+               "insert_blocks": args.insert_blocks, "kick_bound": args.kick_bound, "find_lanes": args.find_lanes}
+        out = traffic_from_csv(args.traffic_csv, args.log2n, cfg)
+        text = json.dumps(out, indent=1)
+        if args.json_out:
+            with open(args.json_out, "w") as f:
+                f.write(text + "\n")
+        print(text)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the table does not shard "
               "across GPUs in this build)", file=sys.stderr)
@@ -984,20 +1059,51 @@ def main() -> int:
                           mops=round(n / med("find_zipf") / 1e3, 1), bytes_per_op=round(bytes_find_z / n, 2),
                           mean_probes=round(p_z, 5)),
     }
+    # Fields below that this run cannot measure come from committed files, and say so: HBM bytes
+    # per launch need a rocprofv3 --pmc run of their own (profiles/traffic.json, bench.py
+    # --traffic-csv), and the device's random-access ceilings a memsys run (profiles/memsys.json).
     traffic_path = os.path.join(ROOT, "profiles", "traffic.json")
+    tr = {}
     if os.path.exists(traffic_path):
         try:
             tr = json.load(open(traffic_path))
+            tcfg = tr.get("config", {})
             # a measurement applies to the configuration it was taken in (the grid does not change the
-            # bytes per key; the eviction bound does, through the exchanges per key)
+            # bytes per key; the eviction bound and the insert order do)
             same = {"insert": ("insert_lanes", "kick_bound", "insert_order"), "find_uniform": ("find_lanes",),
                     "find_zipf": ("find_lanes",)}
             for k in kernels:
-                if k in tr and tr[k].get("log2n") == args.log2n and \
-                        all(tr[k].get(f, 0 if f == "insert_order" else None) == cfg[f] for f in same[k]):
+                if k in tr and tr[k].get("log2n") == args.log2n and all(tcfg.get(f) == cfg[f] for f in same[k]):
                     kernels[k]["traffic"] = tr[k]["bytes_per_launch"]
+                    kernels[k]["traffic_source"] = "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; " \
+                                                   "same configuration, stored, not measured in this run)"
+                    if "l2_hit_rate" in tr[k]:
+                        kernels[k]["l2_hit_rate"] = tr[k]["l2_hit_rate"]
         except (OSError, ValueError, KeyError):
-            pass
+            tr = {}
+    if cfg["insert_order"] == 3:
+        # the bulk fill is a sequence of launches (pack, partition passes, fills, eviction insert of
+        # the rest): `achieved`/`frac` keep the reference algorithm's bytes per key (8 + i * 136),
+        # and the measured HBM rate of all its launches sits beside them
+        ins = kernels["insert"]
+        ins["launches"] = "bulk fill: pack, partition by chunk (2 digit passes), chunk fill, second round, " \
+                          "eviction insert of the rest"
+        if ins["traffic"]:
+            hbm = ins["traffic"] / (med("insert") * 1e-3) / 1e9
+            ins["hbm_achieved"] = round(hbm, 1)
+            ins["hbm_frac"] = round(hbm / HBM_PEAK_GBS, 4)
+            ins["stages_bytes_per_key"] = {k: v["bytes_per_key"] for k, v in tr["insert"].get("stages", {}).items()}
+    # Zipf lookups hit the same hot buckets over and over; L2 and the MALL serve those, so the
+    # algorithmic bytes exceed what HBM moves and would read above the HBM peak. The kernel is
+    # reported against the bytes HBM measured (bound "cache"), the algorithmic rate beside.
+    zf = kernels["find_zipf"]
+    zf["algorithmic_gbs"] = zf["achieved"]
+    zf["bound"] = "cache"
+    if zf["traffic"]:
+        hbm = zf["traffic"] / (med("find_zipf") * 1e-3) / 1e9
+        zf["achieved"], zf["frac"] = round(hbm, 1), round(hbm / HBM_PEAK_GBS, 4)
+    else:
+        zf["achieved"], zf["frac"] = None, None
     # the device's random-access ceilings at this table size (bench.py --workload memsys)
     memsys_path = os.path.join(ROOT, "profiles", "memsys.json")
     if os.path.exists(memsys_path):
@@ -1005,27 +1111,28 @@ def main() -> int:
             ms_ = json.load(open(memsys_path))
             ex = i_bar * n / (med("insert") * 1e-3)
             rc_ = ms_.get("read_then_cas_per_s", ms_["cas_per_s"])
+            src = "profiles/memsys.json (bench.py --workload memsys; stored, not measured in this run)"
             if cfg["insert_order"] != 3:  # the eviction insert: one bucket read and one CAS per exchange
                 kernels["insert"]["ceiling"] = {"kind": "random bucket read then 64-bit CAS per s", "rate": round(rc_ / 1e9, 2),
                                                 "achieved": round(ex / 1e9, 2), "frac": round(ex / rc_, 3),
-                                                "unit": "G/s"}
-            for k, p_ in (("find_uniform", p_u), ("find_zipf", p_z)):
-                rd = p_ * n / (med(k) * 1e-3)
-                kernels[k]["ceiling"] = {"kind": "random 128-B bucket reads/s",
-                                         "rate": round(ms_["bucket_reads_per_s"] / 1e9, 2),
-                                         "achieved": round(rd / 1e9, 2), "frac": round(rd / ms_["bucket_reads_per_s"], 3),
-                                         "unit": "G/s"}
+                                                "unit": "G/s", "source": src}
+            rd = p_u * n / (med("find_uniform") * 1e-3)
+            kernels["find_uniform"]["ceiling"] = {"kind": "random 128-B bucket reads/s",
+                                                  "rate": round(ms_["bucket_reads_per_s"] / 1e9, 2),
+                                                  "achieved": round(rd / 1e9, 2),
+                                                  "frac": round(rd / ms_["bucket_reads_per_s"], 3), "unit": "G/s",
+                                                  "source": src}
         except (OSError, ValueError, KeyError):
             pass
     # the roofline names one kernel, whose average launch time a kernel trace can confirm; the
-    # bulk-fill insert (insert_order 3) is a sequence of launches (pack, radix sort, fill, second
-    # round, eviction insert), so it is reported in `kernels` only
-    single = ("find_uniform", "find_zipf") if cfg["insert_order"] == 3 else ("insert", "find_uniform", "find_zipf")
-    if cfg["insert_order"] == 3:
-        kernels["insert"]["launches"] = "bulk fill: pack, radix sort, chunk fill, second round, eviction insert"
+    # bulk-fill insert (insert_order 3) is a sequence of launches and the Zipf finds are bound by
+    # the caches, so with the bulk fill the roofline is the uniform finds'
+    single = ("find_uniform",) if cfg["insert_order"] == 3 else ("insert", "find_uniform")
     dominant = max(single, key=med)
     roofline = {k: kernels[dominant][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
     roofline["kernel"] = dominant
+    if kernels[dominant].get("traffic_source"):
+        roofline["traffic_source"] = kernels[dominant]["traffic_source"]
 
     ms_step = float(np.median(step_ms))
     value = 3 * n / (ms_step * 1e-3) / 1e6

@@ -4,9 +4,12 @@
 // the insert's ceiling however local the accesses are. A large batch can place most of its keys
 // without any global atomic instead:
 //   1. pack: each key's target bucket (the start rule's table, cuckoo.cu:336-341) names a
-//      chunk of kChunkBuckets consecutive buckets of one subtable; write (chunk, entry) pairs;
-//   2. sort the pairs by chunk (rocPRIM radix sort over the chunk id's bits);
-//   3. bounds: the first sorted index of every chunk;
+//      chunk of kChunkBuckets consecutive buckets of one subtable; write the chunk ids and count
+//      their lowest digit per block;
+//   2. partition the entries by chunk (dyc_part.hpp: a least-significant-digit radix partition,
+//      8 bits of chunk id per pass, with no wait between blocks), which also gives the first
+//      index of every chunk;
+//   3. (the first index of every chunk comes with the partition's last pass);
 //   4. fill: one workgroup per chunk owns its buckets for the launch. It takes the buckets'
 //      free-slot masks into LDS, counts the entries per bucket, reserves one range of the
 //      overflow list for the chunk's excess, then gives each entry the next free slot of its
@@ -21,80 +24,57 @@
 #include <stdint.h>
 #include <string.h>
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include "dyc_kernels.hpp"
+#include "dyc_part.hpp"
 #include "dyc_wave.hpp"
 
 namespace dyc {
 
 namespace {
 
-__device__ inline uint32_t start_table(const Desc &D, uint32_t k)
+// Chunk ids of the batch in input order, and each block's counts of digit P (the partition's
+// first pass). Round 1 (IN_ENTRIES false): keys, target = the start table; reserved keys go to
+// chunk 0 and are rejected by its fill. Round 2 (IN_ENTRIES true): overflow entries, target = the
+// other table of the key's pair.
+template <bool IN_ENTRIES, typename CidT>
+__global__ __launch_bounds__(kCountThreads) void bulk_pack_kernel(Desc D, Chunks Ch, const uint32_t *__restrict__ keys,
+                                                                  const uint64_t *__restrict__ in, PartGrid G,
+                                                                  RadixPass P, CidT *__restrict__ cid,
+                                                                  uint32_t *__restrict__ partial)
 {
-    const uint32_t p = pair_of(D, k);
-    return (k & 1u) ? pos2(D, p) : pos1(D, p);
-}
-
-// Round 1 (IN_ENTRIES false): keys/vals, target = the start table; reserved keys go to chunk 0
-// and are rejected by its fill. Round 2 (IN_ENTRIES true): overflow entries, target = the other
-// table of the key's pair.
-template <bool IN_ENTRIES>
-__global__ __launch_bounds__(256) void bulk_pack_kernel(Desc D, Chunks Ch, const uint32_t *__restrict__ keys,
-                                                        const uint32_t *__restrict__ vals,
-                                                        const uint64_t *__restrict__ in, uint64_t n,
-                                                        uint32_t *__restrict__ cid, uint64_t *__restrict__ out)
-{
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint64_t e;
-        uint32_t k, c = 0;
-        if constexpr (IN_ENTRIES) {
-            e = in[i];
-            k = key_of(e);
-            const uint32_t p = pair_of(D, k);
-            const uint32_t t = other_pos(D, p, (k & 1u) ? pos2(D, p) : pos1(D, p));
-            c = chunk_base(Ch, t) + (bucket_of(D, k, t) >> kChunkShift);
-        } else {
-            k = keys[i];
-            e = pack(k, vals[i]);
-            if (!reserved_key(k)) {
-                const uint32_t t = start_table(D, k);
-                c = chunk_base(Ch, t) + (bucket_of(D, k, t) >> kChunkShift);
+    __shared__ uint32_t h[kMaxRadix];
+    const uint32_t R = 1u << P.bits;
+    for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) h[d] = 0;
+    __syncthreads();
+    const uint64_t lo = count_begin(G, blockIdx.x);
+    const uint64_t hi = count_end(G, lo);
+    for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)kCountThreads * kCountItems) {
+        uint32_t k[kCountItems];
+#pragma unroll
+        for (int u = 0; u < kCountItems; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kCountThreads;
+            if (i < hi) {
+                if constexpr (IN_ENTRIES) k[u] = key_of(in[i]);
+                else k[u] = keys[i];
             }
         }
-        cid[i] = c;
-        out[i] = e;
-    }
-}
-
-// start[c] = the first index of chunk c in the sorted chunk ids, for c in [0, total]; index i
-// writes the starts of the chunks in (cid[i - 1], cid[i]] (with cid[-1] = -1, cid[n] = total).
-// A thread takes four consecutive indexes with one 16-byte load (cid is 16-byte aligned).
-__global__ __launch_bounds__(256) void bulk_bounds_kernel(const uint32_t *__restrict__ cid, uint64_t n,
-                                                          uint32_t total, uint32_t *__restrict__ start)
-{
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t quads = n / 4 + 1;  // the last quad holds index n (and the tail)
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < quads; g += stride) {
-        const uint64_t i0 = g * 4;
-        uint32_t v[4];
-        if (i0 + 4 <= n) {
-            const uint4 q = *reinterpret_cast<const uint4 *>(cid + i0);
-            v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
-        } else {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = i0 + u < n ? cid[i0 + u] : total;
-        }
-        int64_t prev = i0 == 0 ? -1 : (int64_t)min(cid[i0 - 1], total);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (i0 + u > n) break;
-            const int64_t cur = i0 + u == n ? (int64_t)total : (int64_t)min(v[u], total);
-            for (int64_t c = prev + 1; c <= cur; ++c) start[c] = (uint32_t)(i0 + u);
-            prev = cur;
+        for (int u = 0; u < kCountItems; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kCountThreads;
+            if (i >= hi) continue;
+            uint32_t c = 0;
+            if (IN_ENTRIES || !reserved_key(k[u])) {
+                const uint32_t p = pair_of(D, k[u]);
+                uint32_t t = (k[u] & 1u) ? pos2(D, p) : pos1(D, p);
+                if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
+                c = chunk_base(Ch, t) + (bucket_of(D, k[u], t) >> kChunkShift);
+            }
+            cid[i] = (CidT)c;
+            atomicAdd(&h[digit_of(c, P)], 1u);
         }
     }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) partial[(uint64_t)blockIdx.x * R + d] = h[d];
 }
 
 __device__ inline uint32_t nth_set_bit(uint32_t m, uint32_t r)
@@ -140,16 +120,30 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
         const uint4 *const src = reinterpret_cast<const uint4 *>(sub + (uint64_t)b0 * kSlots);
         constexpr uint32_t kParts = kChunkBuckets * kSlots / 2;  // 16-byte parts of the chunk
         if constexpr (DENSE) {
+            // the chunk into LDS; a bucket's 8 parts sit in 8 consecutive lanes, whose two ballot
+            // bits per part make its free-slot mask (no strided LDS reads)
             uint4 *const im = reinterpret_cast<uint4 *>(img);
-            for (uint32_t q = threadIdx.x; q < kParts; q += blockDim.x)
-                im[q] = q < len * (kSlots / 2) ? src[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
-            __syncthreads();
-            for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += blockDim.x) {
-                uint32_t m = 0;
+            constexpr uint32_t kPer = kParts / THREADS;
+            uint4 v[kPer];
 #pragma unroll
-                for (int q = 0; q < kSlots; ++q) m |= (uint32_t)(img[j * kSlots + q] == 0ull) << q;
-                fm[j] = m;
-                cnt[j] = 0;
+            for (uint32_t u = 0; u < kPer; ++u) {
+                const uint32_t q = threadIdx.x + u * THREADS;
+                v[u] = q < len * (kSlots / 2) ? src[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
+            }
+            const uint32_t lane = __lane_id();
+#pragma unroll
+            for (uint32_t u = 0; u < kPer; ++u) {
+                const uint32_t q = threadIdx.x + u * THREADS;
+                im[q] = v[u];
+                const uint64_t m0 = __ballot((v[u].x | v[u].y) == 0u), m1 = __ballot((v[u].z | v[u].w) == 0u);
+                if ((lane & 7u) == 0) {
+                    const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
+                    uint32_t m = 0;
+#pragma unroll
+                    for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
+                    fm[q / 8] = m;
+                    cnt[q / 8] = 0;
+                }
             }
         } else {
             for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += blockDim.x) cnt[j] = 0;
@@ -172,7 +166,7 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
             for (uint32_t u = 0; u < kBatch; ++u) {
                 const uint32_t k = key_of(eb[u]);
                 if (i0 + u * blockDim.x >= hi || reserved_key(k)) continue;
-                const uint32_t b = bucket_of(D, k, t);
+                const uint32_t b = bucket_of_uniform(D, k, t);
                 if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
                 else ++stray;
             }
@@ -238,7 +232,7 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
                     ++rejected;
                     continue;
                 }
-                const uint32_t b = bucket_of(D, k, t);
+                const uint32_t b = bucket_of_uniform(D, k, t);
                 bool fits = false;
                 if (b >= b0 && b - b0 < len) {
                     const uint32_t j = b - b0;
@@ -288,77 +282,52 @@ Chunks make_chunks(const Desc &D)
     return Ch;
 }
 
-hipError_t launch_bulk_pack(const Desc &D, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals,
-                            const uint64_t *entries, uint64_t n, uint32_t *cid, uint64_t *out, hipStream_t st)
+namespace {
+
+// The 8-byte entries' partition: 4096-entry tiles (256 threads), two blocks per CU.
+constexpr int kBulkSortThreads = 256;
+constexpr uint32_t kBulkSortTile = kBulkSortThreads * kSortItems;
+
+template <typename CidT>
+hipError_t bulk_partition(const Desc &D, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals, int in,
+                          uint64_t n, const BulkSpace &ws, uint32_t max_blocks, hipStream_t st, int *out)
+{
+    RadixPass P[kMaxRadixPasses];
+    const uint32_t np = radix_passes(Ch.bits, P);
+    const PartGrid G = part_grid(n, max_blocks, kBulkSortTile);
+    CidT *const cid[2] = {static_cast<CidT *>(ws.cid[0]), static_cast<CidT *>(ws.cid[1])};
+    uint64_t *const ent[2] = {static_cast<uint64_t *>(ws.ent[0]), static_cast<uint64_t *>(ws.ent[1])};
+    if (keys) {
+        bulk_pack_kernel<false, CidT><<<G.blocks * G.sub, kCountThreads, 0, st>>>(D, Ch, keys, nullptr, G, P[0], cid[0],
+                                                                          ws.partial);
+        return partition_passes<uint64_t, KeyValSource, CidT, kBulkSortThreads>(
+            KeyValSource{keys, vals}, 0, ent, cid, G, P, np, ws.partial, ws.total, ws.start, Ch.total, st, out);
+    }
+    bulk_pack_kernel<true, CidT><<<G.blocks * G.sub, kCountThreads, 0, st>>>(D, Ch, nullptr, ent[in], G, P[0], cid[0],
+                                                                     ws.partial);
+    return partition_passes<uint64_t, EntrySource<uint64_t>, CidT, kBulkSortThreads>(
+        EntrySource<uint64_t>{ent[in]}, 1 - in, ent, cid, G, P, np, ws.partial, ws.total, ws.start, Ch.total, st, out);
+}
+
+}  // namespace
+
+size_t bulk_cid_bytes(const Chunks &Ch) { return Ch.total <= 65536u ? 2 : 4; }
+
+hipError_t launch_bulk_partition(const Desc &D, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals, int in,
+                                 uint64_t n, const BulkSpace &ws, uint32_t max_blocks, hipStream_t st, int *out)
+{
+    *out = keys ? 0 : 1 - in;
+    if (n == 0) return hipSuccess;
+    if (n >= (1ull << 32) || Ch.total == 0 || Ch.bits > kMaxRadixPasses * kMaxRadixBits) return hipErrorInvalidValue;
+    if (bulk_cid_bytes(Ch) == 2)
+        return bulk_partition<uint16_t>(D, Ch, keys, vals, in, n, ws, max_blocks, st, out);
+    return bulk_partition<uint32_t>(D, Ch, keys, vals, in, n, ws, max_blocks, st, out);
+}
+
+hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *start, const uint64_t *ent, uint64_t n,
+                            uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    const int grid = grid_for((n + kWave - 1) / kWave, LaunchShape{256, 0}, 16384);
-    if (entries)
-        bulk_pack_kernel<true><<<grid, 256, 0, st>>>(D, Ch, nullptr, nullptr, entries, n, cid, out);
-    else
-        bulk_pack_kernel<false><<<grid, 256, 0, st>>>(D, Ch, keys, vals, nullptr, n, cid, out);
-    return hipGetLastError();
-}
-
-hipError_t bulk_sort_bytes(uint64_t n, uint32_t bits, size_t *bytes)
-{
-    rocprim::double_buffer<uint32_t> kb(nullptr, nullptr);
-    rocprim::double_buffer<uint64_t> vb(nullptr, nullptr);
-    *bytes = 0;
-    return rocprim::radix_sort_pairs(nullptr, *bytes, kb, vb, (unsigned int)n, 0u, bits);
-}
-
-hipError_t bulk_sort(void *tmp, size_t tmp_bytes, uint32_t *cid0, uint32_t *cid1, uint64_t *ent0, uint64_t *ent1,
-                     uint64_t n, uint32_t bits, hipStream_t st, int *sorted_in)
-{
-    *sorted_in = 0;
-    if (n == 0) return hipSuccess;
-    rocprim::double_buffer<uint32_t> kb(cid0, cid1);
-    rocprim::double_buffer<uint64_t> vb(ent0, ent1);
-    const hipError_t e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (unsigned int)n, 0u, bits, st);
-    if (e != hipSuccess) return e;
-    *sorted_in = kb.current() == cid0 ? 0 : 1;
-    if ((vb.current() == ent0 ? 0 : 1) != *sorted_in) return hipErrorUnknown;
-    return hipSuccess;
-}
-
-hipError_t bulk_sort_wide_bytes(uint64_t n, uint32_t bits, size_t *bytes)
-{
-    rocprim::double_buffer<uint32_t> kb(nullptr, nullptr);
-    rocprim::double_buffer<WideEntry> vb(nullptr, nullptr);
-    *bytes = 0;
-    return rocprim::radix_sort_pairs(nullptr, *bytes, kb, vb, (unsigned int)n, 0u, bits);
-}
-
-hipError_t bulk_sort_wide(void *tmp, size_t tmp_bytes, uint32_t *cid0, uint32_t *cid1, WideEntry *ent0,
-                          WideEntry *ent1, uint64_t n, uint32_t bits, hipStream_t st, int *sorted_in)
-{
-    *sorted_in = 0;
-    if (n == 0) return hipSuccess;
-    rocprim::double_buffer<uint32_t> kb(cid0, cid1);
-    rocprim::double_buffer<WideEntry> vb(ent0, ent1);
-    const hipError_t e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (unsigned int)n, 0u, bits, st);
-    if (e != hipSuccess) return e;
-    *sorted_in = kb.current() == cid0 ? 0 : 1;
-    if ((vb.current() == ent0 ? 0 : 1) != *sorted_in) return hipErrorUnknown;
-    return hipSuccess;
-}
-
-hipError_t launch_bulk_bounds(const uint32_t *cid, uint64_t n, uint32_t total, uint32_t *start, hipStream_t st)
-{
-    if (n == 0) return hipSuccess;
-    const int grid = grid_for((n / 4 + 1 + kWave - 1) / kWave, LaunchShape{256, 0}, 16384);
-    bulk_bounds_kernel<<<grid, 256, 0, st>>>(cid, n, total, start);
-    return hipGetLastError();
-}
-
-hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *cid, const uint64_t *ent, uint64_t n,
-                            uint32_t *start, uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C,
-                            hipStream_t st)
-{
-    if (n == 0) return hipSuccess;
-    hipError_t e = launch_bulk_bounds(cid, n, Ch.total, start, st);
-    if (e != hipSuccess) return e;
     if (n >= (1ull << 32) || Ch.total == 0) return hipErrorInvalidValue;
     uint64_t buckets = 0;
     for (uint32_t t = 0; t < D.d; ++t) buckets += D.nb[t];

@@ -207,6 +207,20 @@ __host__ __device__ inline uint32_t bucket_of(const Desc &D, uint32_t k, uint32_
     return fast_mod_sh(hash_t(k, t), selt(t, D.nb), selt(t, D.m), selt(t, D.sh));
 }
 
+// bucket_of for a subtable t that every lane of the wave shares: a scalar branch evaluates the
+// one hash that t uses instead of all four.
+__device__ inline uint32_t bucket_of_uniform(const Desc &D, uint32_t k, uint32_t t)
+{
+    uint32_t x;
+    switch (t & 3u) {
+    case 0: x = h1(k) % kPrime; break;
+    case 1: x = h2(k) % kPrime; break;
+    case 2: x = h3(k); break;
+    default: x = h4(k) % kPrime; break;
+    }
+    return fast_mod_sh(x, selt(t, D.nb), selt(t, D.m), selt(t, D.sh));
+}
+
 // Address of slot 0 of key k's bucket in subtable t.
 __host__ __device__ inline uint64_t *bucket_ptr(const Desc &D, uint32_t k, uint32_t t)
 {

@@ -14,7 +14,7 @@
 #include <thread>
 #include <utility>
 
-#include "../../include/dyc.h"
+#include "dyc.h"
 #include "dyc_device.hpp"
 
 namespace dyc {

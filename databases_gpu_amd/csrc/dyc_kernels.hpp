@@ -116,24 +116,35 @@ __host__ __device__ inline uint32_t chunk_base(const Chunks &Ch, uint32_t t)
 struct WideEntry {
     uint64_t k, v;
 };
-hipError_t bulk_sort_wide_bytes(uint64_t n, uint32_t bits, size_t *bytes);
-hipError_t bulk_sort_wide(void *tmp, size_t tmp_bytes, uint32_t *cid0, uint32_t *cid1, WideEntry *ent0,
-                          WideEntry *ent1, uint64_t n, uint32_t bits, hipStream_t st, int *sorted_in);
-// start[c] = first index of chunk c in n sorted chunk ids, c in [0, total] (start holds total + 1).
-hipError_t launch_bulk_bounds(const uint32_t *cid, uint64_t n, uint32_t total, uint32_t *start, hipStream_t st);
-// (chunk id, entry) per key: round 1 from keys/vals (entries == nullptr) for the start table,
-// round 2 from entries for the other table of the key's pair.
-hipError_t launch_bulk_pack(const Desc &D, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals,
-                            const uint64_t *entries, uint64_t n, uint32_t *cid, uint64_t *out, hipStream_t st);
-// Radix sort of the pairs by chunk id; *sorted_in = 0 if the result is in (cid0, ent0), else 1.
-hipError_t bulk_sort_bytes(uint64_t n, uint32_t bits, size_t *bytes);
-hipError_t bulk_sort(void *tmp, size_t tmp_bytes, uint32_t *cid0, uint32_t *cid1, uint64_t *ent0, uint64_t *ent1,
-                     uint64_t n, uint32_t bits, hipStream_t st, int *sorted_in);
-// Fill free slots chunk by chunk from sorted pairs (n < 2^32); start holds total + 1 words.
-// Entries that do not fit are appended to ovf (*ovf_n, zeroed by the caller, counts them;
+// Workspace of a bulk fill's partition by chunk (dyc_part.hpp) of up to `cap` entries: the
+// entries (8 bytes each, or a WideEntry) and their chunk ids (bulk_cid_bytes each) in two
+// buffers each, the per-block digit counts (kPartialWords), the digit totals (256 words) and
+// the chunk starts (chunks + 1 words).
+constexpr uint64_t kPartialWords = 2048ull * 256;
+constexpr uint64_t kMaxRadixWords = 256;
+struct BulkSpace {
+    void *ent[2];
+    void *cid[2];
+    uint32_t *partial;
+    uint32_t *total;
+    uint32_t *start;
+};
+size_t bulk_cid_bytes(const Chunks &Ch);
+// Scatter blocks of a partition launch: two per CU on 256 CUs, of 4096-entry tiles. One per CU
+// (8192-entry tiles, or 4096) and three per CU measured slower.
+constexpr uint32_t kBulkSortBlocks = 512;
+constexpr uint32_t kWideSortBlocks = 512;
+// Group n entries by chunk (n < 2^32): round 1 from keys/vals for the start table (keys !=
+// nullptr; reserved keys go to chunk 0), round 2 from the entries in ws.ent[in] for the other
+// table of each key's pair. The sorted entries end in ws.ent[*out] (the round-2 input buffer is
+// reused), and ws.start receives the first index of every chunk.
+hipError_t launch_bulk_partition(const Desc &D, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals, int in,
+                                 uint64_t n, const BulkSpace &ws, uint32_t max_blocks, hipStream_t st, int *out);
+// Fill free slots chunk by chunk from the partitioned entries (n < 2^32); start holds total + 1
+// words. Entries that do not fit are appended to ovf (*ovf_n, zeroed by the caller, counts them;
 // ovf_cap >= n). Placements go to the kick tally, reserved keys to the rejected tally.
-hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *cid, const uint64_t *ent, uint64_t n,
-                            uint32_t *start, uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C,
+hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *start, const uint64_t *ent, uint64_t n,
+                            uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C,
                             hipStream_t st);
 // Order an insert batch by the region of its first bucket: out[0, n) receives the packed
 // entries, region by region. counts and cursors are R words each (counts zeroed by the caller).

@@ -87,15 +87,16 @@ struct dyc_table {
     uint64_t *part = nullptr;   // insert batch ordered by bucket region
     uint64_t part_cap = 0;
     unsigned long long *part_counts = nullptr;   // kMaxRegions counts, then kMaxRegions cursors
-    // bulk fill (insert_order 3): (chunk id, entry) pairs and their sort buffers, the chunk
-    // starts, the radix sort's scratch and the overflow count (device, and a pinned read-back)
-    uint32_t *bk_cid[2] = {};
+    // bulk fill (insert_order 3): the partition's entry and chunk-id buffers (bk_cid_bytes per
+    // chunk id), its per-block counts and digit totals, the chunk starts and the overflow count
+    // (device, and a pinned read-back)
+    void *bk_cid[2] = {};
     uint64_t *bk_ent[2] = {};
     uint64_t bk_cap = 0;
+    size_t bk_cid_bytes = 0;
     uint32_t *bk_start = nullptr;
     uint64_t bk_start_cap = 0;
-    void *bk_tmp = nullptr;
-    size_t bk_tmp_bytes = 0;
+    uint32_t *bk_partial = nullptr;  // kPartialWords, then 256 digit totals
     unsigned long long *bk_ovf = nullptr;
     unsigned long long *bk_hovf = nullptr;
     DycHostStage *stage = nullptr;
@@ -212,19 +213,17 @@ void release_workspace(dyc_table *t)
         t->bk_ent[b] = nullptr;
     }
     t->bk_cap = 0;
+    t->bk_cid_bytes = 0;
     if (t->bk_start) (void)hipFree(t->bk_start);
     t->bk_start = nullptr;
     t->bk_start_cap = 0;
-    if (t->bk_tmp) (void)hipFree(t->bk_tmp);
-    t->bk_tmp = nullptr;
-    t->bk_tmp_bytes = 0;
     if (t->part) (void)hipFree(t->part);
     t->part = nullptr;
     t->part_cap = 0;
 }
 
 // hipMalloc for a resize; when it fails, wait for the frees still queued, release the idle
-// insert workspaces (up to 24 B per key of the largest bulk-filled batch) and try once more.
+// insert workspaces (up to 20-24 B per key of the largest bulk-filled batch) and try once more.
 hipError_t resize_malloc(dyc_table *t, void **p, size_t bytes, hipStream_t s)
 {
     if (hipMalloc(p, bytes) == hipSuccess) return hipSuccess;
@@ -504,16 +503,16 @@ int bulk_alloc(void **p, size_t bytes, const char *what)
     return DYC_OK;
 }
 
-// Grow-only workspace of the bulk fill for n keys over the chunks Ch.
+// Grow-only workspace of the bulk fill for n keys over the chunks Ch: two buffers of n entries
+// and n chunk ids (2 or 4 bytes), the partition's counts and the chunk starts.
 int reserve_bulk(dyc_table *t, uint64_t n, const Chunks &Ch, hipStream_t s)
 {
-    size_t tmp = 0;
-    HIP_TRY(bulk_sort_bytes(n, Ch.bits, &tmp));
-    const bool grow = n > t->bk_cap || Ch.total + 1ull > t->bk_start_cap || tmp > t->bk_tmp_bytes;
+    const size_t cb = bulk_cid_bytes(Ch);
+    const bool grow = n > t->bk_cap || cb > t->bk_cid_bytes || Ch.total + 1ull > t->bk_start_cap;
     // a buffer being replaced may still be in use by the table's earlier work, ordered on s
     if (grow) HIP_TRY(hipStreamSynchronize(s));
     int rc = DYC_OK;
-    if (n > t->bk_cap) {
+    if (n > t->bk_cap || cb > t->bk_cid_bytes) {
         for (int b = 0; b < 2; ++b) {
             if (t->bk_cid[b]) HIP_TRY(hipFree(t->bk_cid[b]));
             if (t->bk_ent[b]) HIP_TRY(hipFree(t->bk_ent[b]));
@@ -521,12 +520,14 @@ int reserve_bulk(dyc_table *t, uint64_t n, const Chunks &Ch, hipStream_t s)
             t->bk_ent[b] = nullptr;
         }
         t->bk_cap = 0;
+        t->bk_cid_bytes = 0;
         for (int b = 0; b < 2 && !rc; ++b) {
-            rc = bulk_alloc((void **)&t->bk_cid[b], n * sizeof(uint32_t), "chunk ids");
+            rc = bulk_alloc(&t->bk_cid[b], n * cb, "chunk ids");
             if (!rc) rc = bulk_alloc((void **)&t->bk_ent[b], n * sizeof(uint64_t), "entries");
         }
         if (rc) return rc;
         t->bk_cap = n;
+        t->bk_cid_bytes = cb;
     }
     if (Ch.total + 1ull > t->bk_start_cap) {
         if (t->bk_start) HIP_TRY(hipFree(t->bk_start));
@@ -536,13 +537,9 @@ int reserve_bulk(dyc_table *t, uint64_t n, const Chunks &Ch, hipStream_t s)
         if (rc) return rc;
         t->bk_start_cap = Ch.total + 1ull;
     }
-    if (tmp > t->bk_tmp_bytes) {
-        if (t->bk_tmp) HIP_TRY(hipFree(t->bk_tmp));
-        t->bk_tmp = nullptr;
-        t->bk_tmp_bytes = 0;
-        rc = bulk_alloc(&t->bk_tmp, tmp, "sort scratch");
+    if (!t->bk_partial) {
+        rc = bulk_alloc((void **)&t->bk_partial, (kPartialWords + kMaxRadixWords) * sizeof(uint32_t), "partition counts");
         if (rc) return rc;
-        t->bk_tmp_bytes = tmp;
     }
     if (!t->bk_ovf) {
         rc = bulk_alloc((void **)&t->bk_ovf, 256, "overflow count");
@@ -556,17 +553,22 @@ int reserve_bulk(dyc_table *t, uint64_t n, const Chunks &Ch, hipStream_t s)
     return DYC_OK;
 }
 
-// One fill round over n sorted-or-not pairs in (bk_cid[in], bk_ent[in]): sort, fill, and read
-// back how many entries overflowed into bk_ent[*out] (the other buffer of the sorted pair).
-int bulk_round(dyc_table *t, const Chunks &Ch, int in, uint64_t n, hipStream_t s, int *out, uint64_t *overflow)
+BulkSpace bulk_space(const dyc_table *t)
 {
-    int sorted_in = 0;
-    HIP_TRY(bulk_sort(t->bk_tmp, t->bk_tmp_bytes, t->bk_cid[in], t->bk_cid[1 - in], t->bk_ent[in], t->bk_ent[1 - in],
-                      n, Ch.bits, s, &sorted_in));
-    const int r = sorted_in == 0 ? in : 1 - in;
+    return BulkSpace{{t->bk_ent[0], t->bk_ent[1]}, {t->bk_cid[0], t->bk_cid[1]}, t->bk_partial,
+                     t->bk_partial + kPartialWords, t->bk_start};
+}
+
+// One fill round over n entries: round 1 from keys/vals (keys != nullptr), round 2 from the
+// entries in bk_ent[in]. Partition them by chunk, fill, and read back how many entries overflowed
+// into bk_ent[*out] (the buffer the partition did not end in).
+int bulk_round(dyc_table *t, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals, int in, uint64_t n,
+               hipStream_t s, int *out, uint64_t *overflow)
+{
+    int r = 0;
+    HIP_TRY(launch_bulk_partition(t->D, Ch, keys, vals, in, n, bulk_space(t), kBulkSortBlocks, s, &r));
     HIP_TRY(launch_zero(t->bk_ovf, 1, s));
-    HIP_TRY(launch_bulk_fill(t->D, Ch, t->bk_cid[r], t->bk_ent[r], n, t->bk_start, t->bk_ent[1 - r], t->bk_cap,
-                             t->bk_ovf, t->ctr, s));
+    HIP_TRY(launch_bulk_fill(t->D, Ch, t->bk_start, t->bk_ent[r], n, t->bk_ent[1 - r], t->bk_cap, t->bk_ovf, t->ctr, s));
     HIP_TRY(hipMemcpyAsync(t->bk_hovf, t->bk_ovf, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     *overflow = *t->bk_hovf;
@@ -584,16 +586,11 @@ int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64
     const Chunks Ch = make_chunks(t->D);  // the workspace was reserved for it
     int rc = zero_counters(t, s);
     if (rc) return rc;
-    HIP_TRY(launch_bulk_pack(t->D, Ch, keys, vals, nullptr, n, t->bk_cid[0], t->bk_ent[0], s));
-    int ov = 0;
+    int ov = 0, ov2 = 0;
     uint64_t m1 = 0, m2 = 0;
-    rc = bulk_round(t, Ch, 0, n, s, &ov, &m1);
+    rc = bulk_round(t, Ch, keys, vals, 0, n, s, &ov, &m1);
     if (rc || m1 == 0) return rc;
-    // round 2 reads the overflow list bk_ent[ov] and writes the pairs into the other buffers
-    const int in2 = 1 - ov;
-    HIP_TRY(launch_bulk_pack(t->D, Ch, nullptr, nullptr, t->bk_ent[ov], m1, t->bk_cid[in2], t->bk_ent[in2], s));
-    int ov2 = 0;
-    rc = bulk_round(t, Ch, in2, m1, s, &ov2, &m2);
+    rc = bulk_round(t, Ch, nullptr, nullptr, ov, m1, s, &ov2, &m2);  // the overflow, in its other table
     if (rc || m2 == 0) return rc;
     HIP_TRY(launch_insert_entries(t->D, t->bk_ent[ov2], m2, kNoTable, (int)t->cfg.insert_lanes, t->policy(),
                                   t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s));
@@ -1008,7 +1005,7 @@ int dyc_destroy(dyc_table *t)
         if (t->bk_ent[b]) (void)hipFree(t->bk_ent[b]);
     }
     if (t->bk_start) (void)hipFree(t->bk_start);
-    if (t->bk_tmp) (void)hipFree(t->bk_tmp);
+    if (t->bk_partial) (void)hipFree(t->bk_partial);
     if (t->bk_ovf) (void)hipFree(t->bk_ovf);
     if (t->bk_hovf) (void)hipHostFree(t->bk_hovf);
     if (t->hctr) (void)hipHostFree(t->hctr);

@@ -28,8 +28,9 @@
 #include <algorithm>
 #include <vector>
 
-#include "../../include/dyc.h"
+#include "dyc.h"
 #include "dyc_host.hpp"
+#include "dyc_part.hpp"
 #include "dyc_wave.hpp"
 
 namespace dyc {
@@ -463,34 +464,50 @@ constexpr uint64_t kWideInsertBlocks = 16384;
 // chunk, one workgroup per chunk filling free slots, a second round in the other bucket, the
 // eviction insert for the rest. The dense round keeps the chunk's key lines and value lines in
 // LDS (2 x 64 KB), so one 1024-thread workgroup runs per CU.
-template <bool IN_ENTRIES>
-__global__ __launch_bounds__(256) void wide_bulk_pack_kernel(WideDesc W, Chunks Ch, const uint64_t *__restrict__ keys,
-                                                             const uint64_t *__restrict__ vals,
-                                                             const WideEntry *__restrict__ in, uint64_t n,
-                                                             uint32_t *__restrict__ cid, WideEntry *__restrict__ out)
+// Chunk ids of the batch in input order and each block's counts of the partition's first digit
+// (dyc_part.hpp), over fold(k): round 1 from the keys (start table; reserved keys go to chunk 0,
+// whose fill rejects them), round 2 from overflow entries (the other table of the pair).
+template <bool IN_ENTRIES, typename CidT>
+__global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc W, Chunks Ch,
+                                                                       const uint64_t *__restrict__ keys,
+                                                                       const WideEntry *__restrict__ in, PartGrid G,
+                                                                       RadixPass P, CidT *__restrict__ cid,
+                                                                       uint32_t *__restrict__ partial)
 {
     const Desc &D = W.D;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        WideEntry e;
-        uint32_t c = 0;
-        if constexpr (IN_ENTRIES) {
-            e = in[i];
-            const uint32_t f = fold64(e.k), p = pair_of(D, f);
-            const uint32_t t = other_pos(D, p, (e.k & 1u) ? pos2(D, p) : pos1(D, p));
-            c = chunk_base(Ch, t) + (bucket_of(D, f, t) >> kChunkShift);
-        } else {
-            e.k = keys[i];
-            e.v = vals[i];
-            if (!wide_reserved(e.k)) {  // reserved keys go to chunk 0, whose fill rejects them
-                const uint32_t f = fold64(e.k), p = pair_of(D, f);
-                const uint32_t t = (e.k & 1u) ? pos2(D, p) : pos1(D, p);
-                c = chunk_base(Ch, t) + (bucket_of(D, f, t) >> kChunkShift);
+    __shared__ uint32_t h[kMaxRadix];
+    const uint32_t R = 1u << P.bits;
+    for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) h[d] = 0;
+    __syncthreads();
+    const uint64_t lo = count_begin(G, blockIdx.x);
+    const uint64_t hi = count_end(G, lo);
+    for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)kCountThreads * kCountItems) {
+        uint64_t k[kCountItems];
+#pragma unroll
+        for (int u = 0; u < kCountItems; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kCountThreads;
+            if (i < hi) {
+                if constexpr (IN_ENTRIES) k[u] = in[i].k;
+                else k[u] = keys[i];
             }
         }
-        cid[i] = c;
-        out[i] = e;
+#pragma unroll
+        for (int u = 0; u < kCountItems; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kCountThreads;
+            if (i >= hi) continue;
+            uint32_t c = 0;
+            if (IN_ENTRIES || !wide_reserved(k[u])) {
+                const uint32_t f = fold64(k[u]), p = pair_of(D, f);
+                uint32_t t = (k[u] & 1u) ? pos2(D, p) : pos1(D, p);
+                if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
+                c = chunk_base(Ch, t) + (bucket_of(D, f, t) >> kChunkShift);
+            }
+            cid[i] = (CidT)c;
+            atomicAdd(&h[digit_of(c, P)], 1u);
+        }
     }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) partial[(uint64_t)blockIdx.x * R + d] = h[d];
 }
 
 __device__ inline uint32_t wide_nth_set_bit(uint32_t m, uint32_t r)
@@ -534,18 +551,24 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
             const uint4 *const vsrc = reinterpret_cast<const uint4 *>(vsub + (uint64_t)b0 * kSlots);
             uint4 *const ki = reinterpret_cast<uint4 *>(kimg);
             uint4 *const vi = reinterpret_cast<uint4 *>(vimg);
+            // a bucket's 8 key parts sit in 8 consecutive lanes, whose two ballot bits per part
+            // make its free-slot mask (no strided LDS reads)
+            static_assert(kParts % THREADS == 0, "every lane takes the same number of parts");
+            const uint32_t lane = __lane_id();
             for (uint32_t q = threadIdx.x; q < kParts; q += THREADS) {
                 const bool in = q < len * (kSlots / 2);
-                ki[q] = in ? ksrc[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
+                const uint4 kv = in ? ksrc[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
+                ki[q] = kv;
                 vi[q] = in ? vsrc[q] : make_uint4(0u, 0u, 0u, 0u);
-            }
-            __syncthreads();
-            for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) {
-                uint32_t m = 0;
+                const uint64_t m0 = __ballot((kv.x | kv.y) == 0u), m1 = __ballot((kv.z | kv.w) == 0u);
+                if ((lane & 7u) == 0) {
+                    const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
+                    uint32_t m = 0;
 #pragma unroll
-                for (int q = 0; q < kSlots; ++q) m |= (uint32_t)(kimg[j * kSlots + q] == 0ull) << q;
-                fm[j] = m;
-                cnt[j] = 0;
+                    for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
+                    fm[q / 8] = m;
+                    cnt[q / 8] = 0;
+                }
             }
         } else {
             for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += blockDim.x) cnt[j] = 0;
@@ -564,7 +587,7 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
 #pragma unroll
             for (uint32_t u = 0; u < kBatch; ++u) {
                 if (i0 + u * THREADS >= hi || wide_reserved(kb[u])) continue;
-                const uint32_t b = bucket_of(D, fold64(kb[u]), t);
+                const uint32_t b = bucket_of_uniform(D, fold64(kb[u]), t);
                 if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
                 else ++stray;
             }
@@ -626,7 +649,7 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
                     ++rejected;
                     continue;
                 }
-                const uint32_t b = bucket_of(D, fold64(e.k), t);
+                const uint32_t b = bucket_of_uniform(D, fold64(e.k), t);
                 bool fits = false;
                 if (b >= b0 && b - b0 < len) {
                     const uint32_t j = b - b0;
@@ -720,26 +743,51 @@ hipError_t launch_wide_insert(const WideDesc &W, const uint64_t *keys, const uin
     return hipGetLastError();
 }
 
-hipError_t launch_wide_bulk_pack(const WideDesc &W, const Chunks &Ch, const uint64_t *keys, const uint64_t *vals,
-                                 const WideEntry *in, uint64_t n, uint32_t *cid, WideEntry *out, hipStream_t st)
+// Wide entries (16 bytes): 4096-entry tiles (256 threads), two blocks per CU.
+constexpr int kWideSortThreads = 256;
+constexpr uint32_t kWideSortTile = kWideSortThreads * kSortItems;
+
+template <typename CidT>
+hipError_t wide_partition(const WideDesc &W, const Chunks &Ch, const uint64_t *keys, const uint64_t *vals, int in,
+                          uint64_t n, const BulkSpace &ws, uint32_t max_blocks, hipStream_t st, int *out)
 {
-    if (n == 0) return hipSuccess;
-    const int grid = grid_for((n + kWave - 1) / kWave, LaunchShape{256, 0}, 16384);
-    if (in)
-        wide_bulk_pack_kernel<true><<<grid, 256, 0, st>>>(W, Ch, nullptr, nullptr, in, n, cid, out);
-    else
-        wide_bulk_pack_kernel<false><<<grid, 256, 0, st>>>(W, Ch, keys, vals, nullptr, n, cid, out);
-    return hipGetLastError();
+    RadixPass P[kMaxRadixPasses];
+    const uint32_t np = radix_passes(Ch.bits, P);
+    const PartGrid G = part_grid(n, max_blocks, kWideSortTile);
+    CidT *const cid[2] = {static_cast<CidT *>(ws.cid[0]), static_cast<CidT *>(ws.cid[1])};
+    WideEntry *const ent[2] = {static_cast<WideEntry *>(ws.ent[0]), static_cast<WideEntry *>(ws.ent[1])};
+    if (keys) {
+        wide_bulk_pack_kernel<false, CidT><<<G.blocks * G.sub, kCountThreads, 0, st>>>(W, Ch, keys, nullptr, G, P[0], cid[0],
+                                                                               ws.partial);
+        return partition_passes<WideEntry, WideKeyValSource, CidT, kWideSortThreads>(
+            WideKeyValSource{keys, vals}, 0, ent, cid, G, P, np, ws.partial, ws.total, ws.start, Ch.total, st, out);
+    }
+    wide_bulk_pack_kernel<true, CidT><<<G.blocks * G.sub, kCountThreads, 0, st>>>(W, Ch, nullptr, ent[in], G, P[0], cid[0],
+                                                                          ws.partial);
+    return partition_passes<WideEntry, EntrySource<WideEntry>, CidT, kWideSortThreads>(
+        EntrySource<WideEntry>{ent[in]}, 1 - in, ent, cid, G, P, np, ws.partial, ws.total, ws.start, Ch.total, st,
+        out);
 }
 
-hipError_t launch_wide_bulk_fill(const WideDesc &W, const Chunks &Ch, const uint32_t *cid, const WideEntry *ent,
-                                 uint64_t n, uint32_t *start, WideEntry *ovf, uint64_t ovf_cap,
-                                 unsigned long long *ovf_n, Counters *C, hipStream_t st)
+// Group n wide entries by chunk, as launch_bulk_partition does for the 32-bit table (round 1
+// from keys/vals, round 2 from ws.ent[in]); the sorted entries end in ws.ent[*out].
+hipError_t launch_wide_partition(const WideDesc &W, const Chunks &Ch, const uint64_t *keys, const uint64_t *vals,
+                                 int in, uint64_t n, const BulkSpace &ws, uint32_t max_blocks, hipStream_t st,
+                                 int *out)
+{
+    *out = keys ? 0 : 1 - in;
+    if (n == 0) return hipSuccess;
+    if (n >= (1ull << 32) || Ch.total == 0 || Ch.bits > kMaxRadixPasses * kMaxRadixBits) return hipErrorInvalidValue;
+    if (bulk_cid_bytes(Ch) == 2) return wide_partition<uint16_t>(W, Ch, keys, vals, in, n, ws, max_blocks, st, out);
+    return wide_partition<uint32_t>(W, Ch, keys, vals, in, n, ws, max_blocks, st, out);
+}
+
+hipError_t launch_wide_bulk_fill(const WideDesc &W, const Chunks &Ch, const uint32_t *start, const WideEntry *ent,
+                                 uint64_t n, WideEntry *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C,
+                                 hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     if (n >= (1ull << 32) || Ch.total == 0) return hipErrorInvalidValue;
-    hipError_t e = launch_bulk_bounds(cid, n, Ch.total, start, st);
-    if (e != hipSuccess) return e;
     uint64_t buckets = 0;
     for (uint32_t t = 0; t < W.D.d; ++t) buckets += W.D.nb[t];
     if (n >= 2 * buckets)
@@ -813,13 +861,13 @@ struct dyc_wide {
     uint64_t upd_cap = 0;
     // bulk fill (insert_order 3): (chunk id, entry) pairs and their sort buffers, the chunk
     // starts, the radix sort's scratch and the overflow count (device, and a pinned read-back)
-    uint32_t *bk_cid[2] = {};
+    void *bk_cid[2] = {};
     WideEntry *bk_ent[2] = {};
     uint64_t bk_cap = 0;
+    size_t bk_cid_bytes = 0;
     uint32_t *bk_start = nullptr;
     uint64_t bk_start_cap = 0;
-    void *bk_tmp = nullptr;
-    size_t bk_tmp_bytes = 0;
+    uint32_t *bk_partial = nullptr;  // kPartialWords, then 256 digit totals
     unsigned long long *bk_ovf = nullptr;
     unsigned long long *bk_hovf = nullptr;
     Reclaimer reclaim;  // frees retired subtables off the calling thread
@@ -888,16 +936,14 @@ void wrelease_workspace(dyc_wide *t)
         t->bk_ent[b] = nullptr;
     }
     t->bk_cap = 0;
+    t->bk_cid_bytes = 0;
     if (t->bk_start) (void)hipFree(t->bk_start);
     t->bk_start = nullptr;
     t->bk_start_cap = 0;
-    if (t->bk_tmp) (void)hipFree(t->bk_tmp);
-    t->bk_tmp = nullptr;
-    t->bk_tmp_bytes = 0;
 }
 
 // hipMalloc for a resize; when it fails, wait for the frees still queued, release the idle
-// bulk-fill workspace (up to 40 B per key of the largest bulk-filled batch) and try once more.
+// bulk-fill workspace (up to 36-40 B per key of the largest bulk-filled batch) and try once more.
 hipError_t wresize_malloc(dyc_wide *t, uint8_t **p, size_t bytes, hipStream_t s)
 {
     if (hipMalloc(p, bytes) == hipSuccess) return hipSuccess;
@@ -1175,13 +1221,11 @@ int walloc(void **p, size_t bytes, const char *what)
 
 int wreserve_bulk(dyc_wide *t, uint64_t n, const Chunks &Ch, hipStream_t s)
 {
-    size_t tmp = 0;
-    HIP_TRY(bulk_sort_wide_bytes(n, Ch.bits, &tmp));
+    const size_t cb = bulk_cid_bytes(Ch);
     // a buffer being replaced may still be in use by the table's earlier work, ordered on s
-    if (n > t->bk_cap || Ch.total + 1ull > t->bk_start_cap || tmp > t->bk_tmp_bytes)
-        HIP_TRY(hipStreamSynchronize(s));
+    if (n > t->bk_cap || cb > t->bk_cid_bytes || Ch.total + 1ull > t->bk_start_cap) HIP_TRY(hipStreamSynchronize(s));
     int rc = DYC_OK;
-    if (n > t->bk_cap) {
+    if (n > t->bk_cap || cb > t->bk_cid_bytes) {
         for (int b = 0; b < 2; ++b) {
             if (t->bk_cid[b]) HIP_TRY(hipFree(t->bk_cid[b]));
             if (t->bk_ent[b]) HIP_TRY(hipFree(t->bk_ent[b]));
@@ -1189,12 +1233,14 @@ int wreserve_bulk(dyc_wide *t, uint64_t n, const Chunks &Ch, hipStream_t s)
             t->bk_ent[b] = nullptr;
         }
         t->bk_cap = 0;
+        t->bk_cid_bytes = 0;
         for (int b = 0; b < 2 && !rc; ++b) {
-            rc = walloc((void **)&t->bk_cid[b], n * sizeof(uint32_t), "chunk ids");
+            rc = walloc(&t->bk_cid[b], n * cb, "chunk ids");
             if (!rc) rc = walloc((void **)&t->bk_ent[b], n * sizeof(WideEntry), "entries");
         }
         if (rc) return rc;
         t->bk_cap = n;
+        t->bk_cid_bytes = cb;
     }
     if (Ch.total + 1ull > t->bk_start_cap) {
         if (t->bk_start) HIP_TRY(hipFree(t->bk_start));
@@ -1204,14 +1250,9 @@ int wreserve_bulk(dyc_wide *t, uint64_t n, const Chunks &Ch, hipStream_t s)
         if (rc) return rc;
         t->bk_start_cap = Ch.total + 1ull;
     }
-    if (tmp > t->bk_tmp_bytes) {
-        if (t->bk_tmp) HIP_TRY(hipFree(t->bk_tmp));
-        t->bk_tmp = nullptr;
-        t->bk_tmp_bytes = 0;
-        rc = walloc(&t->bk_tmp, tmp, "sort scratch");
-        if (rc) return rc;
-        t->bk_tmp_bytes = tmp;
-    }
+    if (!t->bk_partial &&
+        (rc = walloc((void **)&t->bk_partial, (kPartialWords + kMaxRadixWords) * sizeof(uint32_t), "partition counts")))
+        return rc;
     if (!t->bk_ovf && (rc = walloc((void **)&t->bk_ovf, 256, "overflow count"))) return rc;
     if (!t->bk_hovf && hipHostMalloc((void **)&t->bk_hovf, 64, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
@@ -1221,17 +1262,19 @@ int wreserve_bulk(dyc_wide *t, uint64_t n, const Chunks &Ch, hipStream_t s)
     return DYC_OK;
 }
 
-// Sort the n pairs in buffer `in`, fill, and read back how many entries overflowed into
-// bk_ent[*out].
-int wbulk_round(dyc_wide *t, const Chunks &Ch, int in, uint64_t n, hipStream_t s, int *out, uint64_t *overflow)
+// One fill round over n entries: round 1 from keys/vals (keys != nullptr), round 2 from the
+// entries in bk_ent[in]. Partition them by chunk, fill, and read back how many entries overflowed
+// into bk_ent[*out] (the buffer the partition did not end in).
+int wbulk_round(dyc_wide *t, const Chunks &Ch, const uint64_t *keys, const uint64_t *vals, int in, uint64_t n,
+                hipStream_t s, int *out, uint64_t *overflow)
 {
-    int sorted_in = 0;
-    HIP_TRY(bulk_sort_wide(t->bk_tmp, t->bk_tmp_bytes, t->bk_cid[in], t->bk_cid[1 - in], t->bk_ent[in],
-                           t->bk_ent[1 - in], n, Ch.bits, s, &sorted_in));
-    const int r = sorted_in == 0 ? in : 1 - in;
+    const BulkSpace ws{{t->bk_ent[0], t->bk_ent[1]}, {t->bk_cid[0], t->bk_cid[1]}, t->bk_partial,
+                       t->bk_partial + kPartialWords, t->bk_start};
+    int r = 0;
+    HIP_TRY(launch_wide_partition(t->W, Ch, keys, vals, in, n, ws, kWideSortBlocks, s, &r));
     HIP_TRY(launch_zero(t->bk_ovf, 1, s));
-    HIP_TRY(launch_wide_bulk_fill(t->W, Ch, t->bk_cid[r], t->bk_ent[r], n, t->bk_start, t->bk_ent[1 - r], t->bk_cap,
-                                  t->bk_ovf, t->ctr, s));
+    HIP_TRY(launch_wide_bulk_fill(t->W, Ch, t->bk_start, t->bk_ent[r], n, t->bk_ent[1 - r], t->bk_cap, t->bk_ovf,
+                                  t->ctr, s));
     HIP_TRY(hipMemcpyAsync(t->bk_hovf, t->bk_ovf, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     *overflow = *t->bk_hovf;
@@ -1242,20 +1285,16 @@ int wbulk_round(dyc_wide *t, const Chunks &Ch, int in, uint64_t n, hipStream_t s
     return DYC_OK;
 }
 
-// insert_order 3 after the counters were zeroed: fill in the start bucket, then the other
-// bucket, then the eviction insert (from key and value arrays split out of the last overflow).
+// insert_order 3 for wide keys: fill the start bucket, then the other bucket, then the eviction
+// insert for the rest (the narrow table's bulk_insert over fold(k)).
 int wbulk_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, hipStream_t s)
 {
     const Chunks Ch = make_chunks(t->W.D);  // the workspace was reserved for it
-    int rc = DYC_OK;
-    HIP_TRY(launch_wide_bulk_pack(t->W, Ch, keys, vals, nullptr, n, t->bk_cid[0], t->bk_ent[0], s));
     int ov = 0, ov2 = 0;
     uint64_t m1 = 0, m2 = 0;
-    rc = wbulk_round(t, Ch, 0, n, s, &ov, &m1);
+    int rc = wbulk_round(t, Ch, keys, vals, 0, n, s, &ov, &m1);
     if (rc || m1 == 0) return rc;
-    const int in2 = 1 - ov;
-    HIP_TRY(launch_wide_bulk_pack(t->W, Ch, nullptr, nullptr, t->bk_ent[ov], m1, t->bk_cid[in2], t->bk_ent[in2], s));
-    rc = wbulk_round(t, Ch, in2, m1, s, &ov2, &m2);
+    rc = wbulk_round(t, Ch, nullptr, nullptr, ov, m1, s, &ov2, &m2);  // the overflow, in its other table
     if (rc || m2 == 0) return rc;
     // the other entry buffer (16 B per key of capacity) holds m2 keys and then m2 values
     uint64_t *const sk = reinterpret_cast<uint64_t *>(t->bk_ent[1 - ov2]);
@@ -1339,7 +1378,7 @@ int dyc_wide_destroy(dyc_wide *t)
         if (t->bk_ent[b]) (void)hipFree(t->bk_ent[b]);
     }
     if (t->bk_start) (void)hipFree(t->bk_start);
-    if (t->bk_tmp) (void)hipFree(t->bk_tmp);
+    if (t->bk_partial) (void)hipFree(t->bk_partial);
     if (t->bk_ovf) (void)hipFree(t->bk_ovf);
     if (t->bk_hovf) (void)hipHostFree(t->bk_hovf);
     delete t;

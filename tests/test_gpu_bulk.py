@@ -153,3 +153,31 @@ def test_bulk_fill_matches_the_eviction_insert_on_the_same_batch(D):
     a, b = out
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
     assert a[3] == b[3] == n
+
+
+def test_bulk_fill_partition_with_wide_chunk_ids(D):
+    """A table of more than 2^16 chunks (4 x 8,388,618 buckets, 4.3 GB): the partition carries
+    32-bit chunk ids over three digit passes. 80 M keys at 15 % fill: every key is found with its
+    value, absent keys miss, and the device count of occupied slots equals the batch (n distinct
+    keys found in a table holding n entries: the stored multiset is the batch)."""
+    from databases_gpu_amd import workload as W
+
+    banner("bulk fill, 65,540 chunks (32-bit chunk ids, 3 passes), 80 M keys")
+    dev = torch.device("cuda")
+    n = 80_000_000
+    keys = W.keys_range(0, n, 21, dev)
+    vals = W.splitmix32(keys)
+    t = D.DyCuckoo(1 << 29, insert_order=3)
+    try:
+        assert sum(-(-b // 512) for b in t.buckets) > 65536
+        t.insert(keys, vals)
+        st = t.stats()
+        assert st["count"] == n and st["pending"] == 0 and st["lost"] == 0, st
+        assert t.count_occupied() == n
+        v, h = t.find(keys)
+        assert bool(h.bool().all()) and bool((v == vals).all())
+        absent = W.keys_range(n, n + (1 << 22), 21, dev)
+        v, h = t.find(absent)
+        assert not bool(h.bool().any())
+    finally:
+        t.close()
