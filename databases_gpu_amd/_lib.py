@@ -44,7 +44,7 @@ EXPORTS = (
     "dyc_server_drain", "dyc_server_replay", "dyc_server_get_stats", "dyc_server_table_stats", "dyc_server_dump",
     "dyc_wide_create", "dyc_wide_destroy", "dyc_wide_insert", "dyc_wide_upsert", "dyc_wide_find", "dyc_wide_erase",
     "dyc_wide_resize_up", "dyc_wide_resize_down", "dyc_wide_get_stats", "dyc_wide_clear", "dyc_wide_dump",
-    "dyc_wide_load", "dyc_wide_pending", "dyc_insert_noalloc",
+    "dyc_wide_load", "dyc_wide_pending", "dyc_insert_noalloc", "dyc_insert_async", "dyc_sync",
 )
 
 OP_INSERT, OP_FIND, OP_ERASE = 0, 1, 2
@@ -186,6 +186,8 @@ def load(require: bool = True):
         "dyc_destroy": (ci, [vp]),
         "dyc_insert": (ci, [vp, vp, vp, u64, vp]),
         "dyc_insert_noalloc": (ci, [vp, vp, vp, u64, vp]),
+        "dyc_insert_async": (ci, [vp, vp, vp, u64, vp]),
+        "dyc_sync": (ci, [vp]),
         "dyc_find": (ci, [vp, vp, vp, vp, u64, vp]),
         "dyc_erase": (ci, [vp, vp, vp, u64, vp]),
         "dyc_resize_up": (ci, [vp, vp]),

@@ -71,6 +71,21 @@ uint32_t initial_buckets(uint64_t hint, uint32_t d)
 // buffering), and the copy streams that move them (allocated on first use).
 struct DycHostStage;
 
+// One insert dyc_insert_async left in flight: its own counter block, pinned read-back words and
+// staging list for entries that exhaust the eviction bound, so that inserts on several streams
+// run at the same time against one table (allocated on first use, kept by the table).
+struct AsyncSlot {
+    Counters *ctr = nullptr;
+    unsigned long long *post = nullptr;
+    unsigned long long seq = 0;
+    uint64_t *pend = nullptr;
+    uint64_t cap = 0;
+    uint64_t n = 0;
+    hipStream_t stream = nullptr;
+    bool busy = false;
+};
+constexpr int kAsyncSlots = 8;
+
 struct dyc_table {
     dyc_config cfg;
     int device = 0;
@@ -105,6 +120,8 @@ struct dyc_table {
     uint8_t *live = nullptr;  // downsize: tile flags of the residual list
     uint64_t live_cap = 0;
     Reclaimer reclaim;  // frees retired subtables off the calling thread
+    AsyncSlot async[kAsyncSlots];
+    uint64_t async_keys = 0;  // keys of the inserts in flight
     // Subtables created together (create, load, restore) share one allocation: the GPU's address
     // translation covers one large allocation with far fewer TLB entries than one allocation per
     // subtable, and random bucket access at 2^28 keys misses the per-CU TLB several times per
@@ -597,6 +614,104 @@ int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64
     return DYC_OK;
 }
 
+// Complete the inserts dyc_insert_async left in flight: wait for each one's counters, absorb them
+// as dyc_insert does, move its failed entries to the pending list, and re-place those (upsize and
+// re-insert) as dyc_insert would. Every other call on the table does this first.
+int settle(dyc_table *t)
+{
+    bool any = false;
+    for (const AsyncSlot &a : t->async) any |= a.busy;
+    if (!any) return DYC_OK;
+    const uint64_t before = t->pend_n;
+    int rc = DYC_OK;
+    hipStream_t last = nullptr;
+    for (AsyncSlot &a : t->async) {
+        if (!a.busy) continue;
+        a.busy = false;
+        const hipError_t e = await_post(a.post, a.seq, a.stream);
+        if (e != hipSuccess) {
+            if (!rc) rc = fail(DYC_ERR_HIP, "an insert in flight on a stream failed: %s", hipGetErrorString(e));
+            continue;
+        }
+        Counters c;
+        memcpy(&c, const_cast<const unsigned long long *>(a.post) + 1, sizeof c);
+        t->st.rejected += c.rejected;
+        t->count += a.n - c.rejected;
+        t->st.kicks += c.kicks;
+        t->st.last_kicks += c.kicks;
+        t->st.cas_retries += c.retries;
+        t->st.kick_failures += c.pending;
+        t->st.lost += c.lost;
+        if (c.lost && !rc) rc = fail(DYC_ERR_LOST, "pending staging overflow: %llu entries lost", (unsigned long long)c.lost);
+        const uint64_t m = std::min<uint64_t>(c.pending, a.cap);
+        if (m) {
+            int r = reserve_pending(t, m, a.stream);
+            if (!r && (hipMemcpyAsync(t->pend + t->pend_n, a.pend, m * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                                      a.stream) != hipSuccess ||
+                       hipStreamSynchronize(a.stream) != hipSuccess))
+                r = fail(DYC_ERR_HIP, "moving failed entries of an insert in flight to the pending list failed");
+            if (r) {
+                if (!rc) rc = r;
+                continue;
+            }
+            t->pend_n += m;
+        }
+        last = a.stream;
+    }
+    t->async_keys = 0;
+    t->st.pending = t->pend_n;
+    if (!rc && t->pend_n > before) rc = place_pending(t, last, before);
+    return rc;
+}
+
+#define SETTLE(t)                  \
+    do {                           \
+        const int rc_ = settle(t); \
+        if (rc_) return rc_;       \
+    } while (0)
+
+// Counter block, read-back words and a staging list for n entries in slot a (grown on demand;
+// the slot is idle).
+int async_reserve(AsyncSlot &a, uint64_t n)
+{
+    if (!a.ctr && hipMalloc(&a.ctr, kCounterBytes) != hipSuccess) {
+        (void)hipGetLastError();
+        a.ctr = nullptr;
+        return fail(DYC_ERR_OOM, "hipMalloc of an async counter block failed");
+    }
+    if (!a.post) {
+        if (hipHostMalloc(reinterpret_cast<void **>(&a.post), kPostWords * sizeof(unsigned long long),
+                          hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            a.post = nullptr;
+            return fail(DYC_ERR_OOM, "hipHostMalloc of async read-back words failed");
+        }
+        memset(a.post, 0, kPostWords * sizeof(unsigned long long));
+    }
+    if (n > a.cap) {
+        if (a.pend) (void)hipFree(a.pend);
+        a.pend = nullptr;
+        a.cap = 0;
+        if (hipMalloc(&a.pend, n * sizeof(uint64_t)) != hipSuccess) {
+            (void)hipGetLastError();
+            a.pend = nullptr;
+            return fail(DYC_ERR_OOM, "hipMalloc of an async staging list (%llu entries) failed", (unsigned long long)n);
+        }
+        a.cap = n;
+    }
+    return DYC_OK;
+}
+
+void async_free(dyc_table *t)
+{
+    for (AsyncSlot &a : t->async) {
+        if (a.ctr) (void)hipFree(a.ctr);
+        if (a.post) (void)hipHostFree(a.post);
+        if (a.pend) (void)hipFree(a.pend);
+        a = AsyncSlot{};
+    }
+}
+
 // The counter block of dyc_insert_noalloc: a library-static device array instead of a table's
 // own block, so that the call allocates nothing. Each device holds its own copy until the
 // process ends. In host code `scratch` names a host-side stand-in, not a device address: kernels
@@ -993,7 +1108,9 @@ int dyc_destroy(dyc_table *t)
 {
     if (!t) return DYC_OK;
     DeviceGuard dg(t->device);
+    (void)settle(t);
     (void)hipDeviceSynchronize();
+    async_free(t);
     for (uint32_t i = 0; i < t->D.d; ++i) (void)release_sub(t, i);
     t->reclaim.drain();  // every subtable is freed when this returns
     if (t->pend) (void)hipFree(t->pend);
@@ -1019,10 +1136,11 @@ int dyc_destroy(dyc_table *t)
 
 int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
 {
-    const uint64_t pend_before = t ? t->pend_n : 0;
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
+    SETTLE(t);
+    const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = 0;
     if (n == 0) return DYC_OK;
@@ -1079,6 +1197,56 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     return DYC_OK;
 }
 
+int dyc_insert_async(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(t->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) return DYC_OK;
+    int rc;
+    if (t->cfg.auto_resize &&
+        (double)(t->count + t->async_keys + n) > t->cfg.upper_bound * (double)t->capacity()) {
+        // a resize replaces subtables the inserts in flight write: complete them first
+        rc = settle(t);
+        if (rc) return rc;
+        while ((double)(t->count + n) > t->cfg.upper_bound * (double)t->capacity()) {
+            rc = do_resize_up(t, s);
+            if (rc) return rc;
+        }
+    }
+    AsyncSlot *a = nullptr;
+    for (AsyncSlot &x : t->async)
+        if (!x.busy) {
+            a = &x;
+            break;
+        }
+    if (!a) {  // every slot in flight: complete them
+        rc = settle(t);
+        if (rc) return rc;
+        a = &t->async[0];
+    }
+    rc = async_reserve(*a, n);
+    if (rc) return rc;
+    if (t->async_keys == 0) t->st.last_kicks = 0;
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(a->ctr), kCounterWords, s));
+    HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(), a->ctr,
+                          PendSink{a->pend, nullptr, a->cap, 0, 0, nullptr}, s));
+    HIP_TRY(launch_fold_post(a->ctr, a->post, ++a->seq, s));
+    a->busy = true;
+    a->n = n;
+    a->stream = s;
+    t->async_keys += n;
+    return DYC_OK;
+}
+
+int dyc_sync(dyc_table *t)
+{
+    if (!t) return fail(DYC_ERR_INVALID, "null table");
+    DeviceGuard dg(t->device);
+    return settle(t);
+}
+
 int dyc_insert_noalloc(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
 {
     if (!t) return fail(DYC_ERR_INVALID, "null table");
@@ -1086,6 +1254,10 @@ int dyc_insert_noalloc(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
     if (n > DYC_NOALLOC_MAX_KEYS)
         return fail(DYC_ERR_INVALID, "dyc_insert_noalloc takes at most %u keys per call (got %llu)", DYC_NOALLOC_MAX_KEYS,
                     (unsigned long long)n);
+    for (const AsyncSlot &a : t->async)
+        if (a.busy)
+            return fail(DYC_ERR_INVALID, "dyc_insert_noalloc: inserts of dyc_insert_async are in flight (dyc_sync "
+                                         "completes them, and may allocate)");
     const uint64_t room = t->pend_cap - t->pend_n;
     if (n > room)
         return fail(DYC_ERR_INVALID,
@@ -1119,10 +1291,11 @@ int dyc_insert_noalloc(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
 int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, uint8_t *updated,
                dyc_stream stream)
 {
-    const uint64_t pend_before = t ? t->pend_n : 0;
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
+    SETTLE(t);
+    const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = 0;
     if (n == 0) return DYC_OK;
@@ -1174,6 +1347,7 @@ int dyc_find(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *hit, u
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
     DeviceGuard dg(t->device);
+    SETTLE(t);
     Pool sp;
     HIP_TRY(launch_find(t->D, keys, vals, hit, n, (int)t->cfg.find_lanes, t->shape(), (hipStream_t)stream,
                         stash_of(t, sp)));
@@ -1185,6 +1359,7 @@ int dyc_erase(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_t n, 
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
     DeviceGuard dg(t->device);
+    SETTLE(t);
     hipStream_t s = (hipStream_t)stream;
     if (n == 0) return DYC_OK;
     int rc = zero_counters(t, s);
@@ -1209,6 +1384,7 @@ int dyc_resize_up(dyc_table *t, dyc_stream stream)
 {
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     DeviceGuard dg(t->device);
+    SETTLE(t);
     return do_resize_up(t, (hipStream_t)stream);
 }
 
@@ -1216,6 +1392,7 @@ int dyc_resize_down(dyc_table *t, dyc_stream stream)
 {
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     DeviceGuard dg(t->device);
+    SETTLE(t);
     return do_resize_down(t, (hipStream_t)stream);
 }
 
@@ -1254,6 +1431,7 @@ int dyc_count_occupied(dyc_table *t, uint64_t *occupied, dyc_stream stream)
 {
     if (!t || !occupied) return fail(DYC_ERR_INVALID, "null argument");
     DeviceGuard dg(t->device);
+    SETTLE(t);
     hipStream_t s = (hipStream_t)stream;
     int rc = zero_counters(t, s);
     if (rc) return rc;
@@ -1268,6 +1446,7 @@ int dyc_clear(dyc_table *t, dyc_stream stream)
 {
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     DeviceGuard dg(t->device);
+    SETTLE(t);
     hipStream_t s = (hipStream_t)stream;
     for (uint32_t i = 0; i < t->D.d; ++i)
         HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(t->D.sub[i]), (uint64_t)t->D.nb[i] * kSlots, s));
@@ -1282,6 +1461,7 @@ int dyc_probe_count(dyc_table *t, const uint32_t *keys, uint64_t n, uint64_t *pr
 {
     if (!t || (n && !keys)) return fail(DYC_ERR_INVALID, "null argument");
     DeviceGuard dg(t->device);
+    SETTLE(t);
     hipStream_t s = (hipStream_t)stream;
     int rc = zero_counters(t, s);
     if (rc) return rc;
@@ -1300,6 +1480,7 @@ int dyc_dump(dyc_table *t, uint32_t sub, uint64_t *host_slots, uint64_t n_slots,
     if (n_slots < slots) return fail(DYC_ERR_INVALID, "buffer holds %llu slots, subtable has %llu",
                                      (unsigned long long)n_slots, (unsigned long long)slots);
     DeviceGuard dg(t->device);
+    SETTLE(t);
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipMemcpyAsync(host_slots, t->D.sub[sub], slots * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1312,6 +1493,7 @@ int dyc_load(dyc_table *t, const uint32_t *nb, const uint64_t *const *images, dy
     for (uint32_t i = 0; i < t->D.d; ++i)
         if (nb[i] == 0 || nb[i] >= kMaxBuckets || !images[i]) return fail(DYC_ERR_INVALID, "bad subtable %d", i);
     DeviceGuard dg(t->device);
+    SETTLE(t);
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipStreamSynchronize(s));
     uint8_t *base = nullptr;
@@ -1339,6 +1521,10 @@ int dyc_load(dyc_table *t, const uint32_t *nb, const uint64_t *const *images, dy
 int dyc_pending(dyc_table *t, uint64_t *host_entries, uint64_t cap, uint64_t *n_out, dyc_stream stream)
 {
     if (!t || !n_out) return fail(DYC_ERR_INVALID, "null argument");
+    {
+        DeviceGuard dg(t->device);
+        SETTLE(t);
+    }
     *n_out = t->pend_n;
     const uint64_t m = std::min(cap, t->pend_n);
     if (m && host_entries) {
@@ -1355,10 +1541,11 @@ int dyc_pending(dyc_table *t, uint64_t *host_entries, uint64_t cap, uint64_t *n_
 // device-pointer calls.
 int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
 {
-    const uint64_t pend_before = t ? t->pend_n : 0;
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
+    SETTLE(t);
+    const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = 0;
     if (n == 0) return DYC_OK;
@@ -1466,6 +1653,7 @@ int dyc_find_host(dyc_table *t, const uint32_t *keys, uint32_t *vals, uint8_t *h
     if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
     if (n == 0) return DYC_OK;
     DeviceGuard dg(t->device);
+    SETTLE(t);
     return probe_host(t, keys, vals, hit, n, (hipStream_t)stream, false);
 }
 
@@ -1475,6 +1663,7 @@ int dyc_erase_host(dyc_table *t, const uint32_t *keys, uint8_t *deleted, uint64_
     if (n && !keys) return fail(DYC_ERR_INVALID, "null keys");
     if (n == 0) return DYC_OK;
     DeviceGuard dg(t->device);
+    SETTLE(t);
     hipStream_t s = (hipStream_t)stream;
     int rc = zero_counters(t, s);
     if (rc) return rc;
@@ -1498,6 +1687,7 @@ int dyc_checkpoint_write(dyc_table *t, const char *path, dyc_stream stream)
 {
     if (!t || !path) return fail(DYC_ERR_INVALID, "null argument");
     DeviceGuard dg(t->device);
+    SETTLE(t);
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipStreamSynchronize(s));
     FILE *f = fopen(path, "wb");

@@ -97,6 +97,25 @@ class DyCuckoo:
             check(self.lib.dyc_insert(self.h, k.data_ptr(), v.data_ptr(), k.numel(), _stream_ptr(stream)),
                   "dyc_insert")
 
+    def insert_async(self, keys: torch.Tensor, vals: torch.Tensor, stream=None) -> None:
+        """Enqueue an insert on `stream` without waiting (dyc_insert_async): inserts on several
+        streams run at once; sync() (or any other call) completes them. The tensors must stay
+        alive until then."""
+        k = _as_u32_tensor(keys, self.device)
+        v = _as_u32_tensor(vals, self.device)
+        if k.numel() != v.numel():
+            raise ValueError("keys and values differ in length")
+        self._async_hold = getattr(self, "_async_hold", []) + [k, v]
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_insert_async(self.h, k.data_ptr(), v.data_ptr(), k.numel(), _stream_ptr(stream)),
+                  "dyc_insert_async")
+
+    def sync(self) -> None:
+        """Complete every insert_async in flight (dyc_sync)."""
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_sync(self.h), "dyc_sync")
+        self._async_hold = []
+
     def upsert(self, keys: torch.Tensor, vals: torch.Tensor, stream=None) -> torch.Tensor:
         """Insert, or replace the value of keys already stored (keys distinct within the call).
         Returns per-key flags: 1 where an existing entry was updated."""

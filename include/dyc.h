@@ -17,7 +17,8 @@
  *     hipMalloc, managed or pinned host memory). The caller owns them; the library owns the
  *     table storage.
  *   - `stream` is a hipStream_t (NULL = the default stream). Work is stream-ordered. One table
- *     must be used from one stream (or externally serialised streams) at a time.
+ *     is driven from one host thread, and its calls run on one stream at a time (or on externally
+ *     serialised streams), except dyc_insert_async, whose inserts run on several streams at once.
  *   - Keys 0 and 0xFFFFFFFF are reserved (0 marks an empty slot): insert skips them and counts
  *     them as rejected, find reports a miss, erase reports not-deleted.
  *   - Insert does not look for an existing copy of the key (reference semantics,
@@ -119,6 +120,23 @@ int dyc_destroy(dyc_table *t);
  * returns, every kernel queued on `stream` before and by the call has finished. */
 int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
 
+/* Insert on a stream without waiting, so that inserts on several streams run at the same time
+ * against one table: the reference's stream constructor and per-stream insert
+ * (GPU/old_version/cuckoo_gtest/api.h:27-31, gpu_lp_insert(..., stream), cuckoo.cu:865-888) as
+ * its HashStreamTEST.Four uses them (Main.cpp:72-113: one host thread, four streams, one insert
+ * enqueued on each). Each call in flight has its own counter block and its own list for entries
+ * that exhaust the eviction bound (up to 8 calls in flight; a ninth completes the others first).
+ * keys/vals must stay valid until the call completes. dyc_sync(t) completes every insert in
+ * flight: it waits for them, updates the count and counters, and re-places failed entries as
+ * dyc_insert does (upsize + re-insert); every other call on the table does this first, so the
+ * table is never observed half-way. Keys are placed with the eviction insert (insert_order is not
+ * applied: the bulk fill's workspace belongs to the table). With auto_resize, a call that would
+ * take the table past the upper bound completes the calls in flight and upsizes first. The keys of
+ * calls in flight together should be unique, as for one dyc_insert. dyc_get_stats reports
+ * completed calls only. */
+int dyc_insert_async(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
+int dyc_sync(dyc_table *t);
+
 /* Insert without allocating. No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). Keys are placed as dyc_insert places them with the eviction insert
@@ -129,7 +147,8 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
  *   - nothing is resized (auto_resize is not applied here). Entries that exhaust the eviction
  *     bound join the table's stash (dyc_stats.pending), where find, erase and upsert see them;
  *     a later dyc_insert or dyc_resize_up + dyc_insert re-places them.
- * DYC_ERR_INVALID, with the table unchanged, for n > DYC_NOALLOC_MAX_KEYS or n larger than the
+ * DYC_ERR_INVALID, with the table unchanged, while inserts of dyc_insert_async are in flight
+ * (dyc_sync completes them, and may allocate), for n > DYC_NOALLOC_MAX_KEYS, or for n larger than the
  * free room of the table's pending list (dyc_create reserves DYC_NOALLOC_MAX_KEYS entries): the
  * list cannot grow here, and every entry must keep a place.
  * Every call on one device shares that device's counter block, which keeps the last call's

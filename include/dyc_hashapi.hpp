@@ -78,6 +78,16 @@ class hashAPI {
             for (TYPE i = 0; i < size; ++i) ans[i] = flags[i];
     }
 
+    // The reference's multi-stream form (GPU/old_version/cuckoo_gtest/api.h:27-31 and
+    // gpu_lp_insert(..., stream), cuckoo.cu:865-888): inserts of device arrays enqueued on
+    // different streams run at the same time (dyc_insert_async); sync() completes them, and so does
+    // any other call.
+    void hash_insert_async(TYPE *d_k, TYPE *d_v, TYPE size, hipStream_t stream)
+    {
+        check(dyc_insert_async(t_, d_k, d_v, size, stream), "dyc_insert_async");
+    }
+    void sync() { check(dyc_sync(t_), "dyc_sync"); }
+
     // api.cpp:112-173 ("public for test")
     void resize_up() { check(dyc_resize_up(t_, nullptr), "dyc_resize_up"); }
     void resize_low()

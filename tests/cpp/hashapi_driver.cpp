@@ -143,6 +143,44 @@ static bool batch_test()
     return ok;
 }
 
+// HashStreamTEST.Four (GPU/old_version/cuckoo_gtest/Main.cpp:37-138): one host thread, four
+// streams, 1M keys per stream copied and inserted on its own stream, all in flight at once, into a
+// table made for 5M slots; then every key is searched.
+static bool stream_test(TYPE size, TYPE init_size, int nstreams)
+{
+    const TYPE n = size * (TYPE)nstreams;
+    auto key = unique_keys(n, 4444);
+    std::vector<TYPE> value(n), chck(n, 0);
+    for (TYPE i = 0; i < n; ++i) value[i] = i + 1;
+    hashAPI h(init_size);
+    std::vector<hipStream_t> st(nstreams);
+    std::vector<TYPE *> dk(nstreams), dv(nstreams);
+    bool ok = true;
+    for (int i = 0; i < nstreams; ++i) {
+        ok &= hipStreamCreate(&st[i]) == hipSuccess;
+        ok &= hipMalloc(&dk[i], sizeof(TYPE) * size) == hipSuccess && hipMalloc(&dv[i], sizeof(TYPE) * size) == hipSuccess;
+    }
+    if (!ok) return false;
+    for (int i = 0; i < nstreams; ++i) {
+        ok &= hipMemcpyAsync(dk[i], key.data() + (size_t)i * size, sizeof(TYPE) * size, hipMemcpyHostToDevice, st[i]) ==
+              hipSuccess;
+        ok &= hipMemcpyAsync(dv[i], value.data() + (size_t)i * size, sizeof(TYPE) * size, hipMemcpyHostToDevice,
+                             st[i]) == hipSuccess;
+        h.hash_insert_async(dk[i], dv[i], size, st[i]);
+    }
+    h.sync();
+    h.hash_search(key.data(), chck.data(), n);
+    ok &= chck == value;
+    const dyc_stats s = h.stats();
+    ok &= s.count == n && s.pending == 0 && s.lost == 0;
+    for (int i = 0; i < nstreams; ++i) {
+        (void)hipFree(dk[i]);
+        (void)hipFree(dv[i]);
+        (void)hipStreamDestroy(st[i]);
+    }
+    return ok;
+}
+
 // The batch API's resize policy (api.cpp:353-369, 466-506) on a table sized far below its data:
 // 24 batches in, then 20 batches out, with the reference's fill bounds 0.8 / 0.5 (test_uilt.h:1-2)
 // and auto_resize off. Prints the subtable sizes and the count after every batch, which the
@@ -208,6 +246,7 @@ int main(int argc, char **argv)
     }
     report("RandTestWithUpDown", resize_test(1000000));
     report("BatchApi", batch_test());
+    report("HashStreamTEST.Four", stream_test(1000000, 5000000, 4));
     printf("%d failed\n", failures);
     return failures;
 }
