@@ -45,6 +45,7 @@ EXPORTS = (
     "dyc_wide_create", "dyc_wide_destroy", "dyc_wide_insert", "dyc_wide_upsert", "dyc_wide_find", "dyc_wide_erase",
     "dyc_wide_resize_up", "dyc_wide_resize_down", "dyc_wide_get_stats", "dyc_wide_clear", "dyc_wide_dump",
     "dyc_wide_load", "dyc_wide_pending", "dyc_insert_noalloc", "dyc_insert_async", "dyc_sync",
+    "dyc_insert_counted",
 )
 
 OP_INSERT, OP_FIND, OP_ERASE = 0, 1, 2
@@ -96,11 +97,11 @@ class Stats(ctypes.Structure):
         ("resizes_down", ctypes.c_uint64),
         ("lost", ctypes.c_uint64),
         ("subtables", ctypes.c_uint32),
-        ("reserved0", ctypes.c_uint32),
+        ("last_max_chain", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
-        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved0"}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
         d["buckets"] = list(self.buckets)[: self.subtables]
         return d
 
@@ -187,6 +188,7 @@ def load(require: bool = True):
         "dyc_insert": (ci, [vp, vp, vp, u64, vp]),
         "dyc_insert_noalloc": (ci, [vp, vp, vp, u64, vp]),
         "dyc_insert_async": (ci, [vp, vp, vp, u64, vp]),
+        "dyc_insert_counted": (ci, [vp, vp, vp, u64, vp, vp]),
         "dyc_sync": (ci, [vp]),
         "dyc_find": (ci, [vp, vp, vp, vp, u64, vp]),
         "dyc_erase": (ci, [vp, vp, vp, u64, vp]),

@@ -384,6 +384,7 @@ __global__ __launch_bounds__(256) void update_kernel(Desc D, const uint32_t *__r
 template <int SPL>
 struct Carry {
     uint64_t entry;
+    uint64_t idx;  // input index of the key that started the chain
     uint32_t pre, kicks, retries;
     uint64_t cur[SPL];  // this lane's SPL slots of the current bucket
     bool fresh;         // the view must be (re)read
@@ -393,19 +394,37 @@ struct Carry {
 struct InsertTally {
     unsigned long long kicks = 0, rejected = 0, retries = 0;
     unsigned long long empty = 0;  // zero entries skipped by a from-entries insert
+    unsigned long long max_chain = 0;
 };
+
+// A chain ended after c.kicks exchanges (its entry placed or parked): tally it, and record it
+// for the input index that started it.
+template <int SPL>
+__device__ inline void chain_end(const Carry<SPL> &c, const PendSink &sink, int s, InsertTally &tally)
+{
+    if (s == 0) {
+        tally.kicks += c.kicks;
+        tally.max_chain = tally.max_chain > c.kicks ? tally.max_chain : c.kicks;
+        if (sink.chain) sink.chain[c.idx] = c.kicks;
+    }
+}
 
 // Take entry e (previous table pr, kNoTable for a fresh key) into the group's hand.
 template <int SPL>
-__device__ inline void carry_start(const Desc &D, Carry<SPL> &c, uint64_t e, uint32_t pr, int s, InsertTally &tally)
+__device__ inline void carry_start(const Desc &D, Carry<SPL> &c, uint64_t e, uint32_t pr, int s, InsertTally &tally,
+                                   uint64_t idx = 0, uint32_t *chain = nullptr)
 {
     const uint32_t k = key_of(e);
     if (reserved_key(k)) {
-        if (s == 0) ++tally.rejected;
+        if (s == 0) {
+            ++tally.rejected;
+            if (chain) chain[idx] = 0;
+        }
         return;
     }
     if (pr == kNoTable && (k & 1u)) pr = pos1(D, pair_of(D, k));  // odd keys start in pos2 (cuckoo.cu:336-341)
     c.entry = e;
+    c.idx = idx;
     c.pre = pr;
     c.kicks = 0;
     c.retries = 0;
@@ -492,20 +511,20 @@ __device__ inline void kick_step(const Desc &D, Carry<kSlots / G> &c, const Inse
         ++c.kicks;
         c.fresh = true;
         if (expected == 0) {
-            if (s == 0) tally.kicks += c.kicks;
+            chain_end(c, sink, s, tally);
             c.active = false;
         } else {
             c.entry = expected;
             c.pre = t;
             if (c.kicks >= total_kicks) {
-                if (s == 0) tally.kicks += c.kicks;
+                chain_end(c, sink, s, tally);
                 give_up();
             }
         }
     } else {
         if (s == 0) ++tally.retries;
         if (++c.retries > P.max_retries) {
-            if (s == 0) tally.kicks += c.kicks;
+            chain_end(c, sink, s, tally);
             give_up();
         }
     }
@@ -517,7 +536,7 @@ template <int G, bool FROM_ENTRIES>
 __device__ inline void refill(const Desc &D, Carry<kSlots / G> &c, uint64_t &next, uint64_t end, const uint32_t *__restrict__ keys,
                               const uint32_t *__restrict__ vals, const uint64_t *__restrict__ entries,
                               uint32_t pre_fixed, InsertTally &tally, uint64_t &taken,
-                              const uint8_t *__restrict__ skip = nullptr)
+                              const uint8_t *__restrict__ skip = nullptr, uint32_t *chain = nullptr)
 {
     const int lane = __lane_id();
     const int g = lane / G, s = lane % G;
@@ -533,11 +552,13 @@ __device__ inline void refill(const Desc &D, Carry<kSlots / G> &c, uint64_t &nex
             if constexpr (FROM_ENTRIES) {
                 const uint64_t e = entries[idx];
                 if (e)
-                    carry_start(D, c, e, pre_fixed, s, tally);
+                    carry_start(D, c, e, pre_fixed, s, tally, idx, chain);
                 else if (s == 0)
                     ++tally.empty;
             } else if (!skip || !skip[idx])
-                carry_start(D, c, pack(keys[idx], vals[idx]), kNoTable, s, tally);
+                carry_start(D, c, pack(keys[idx], vals[idx]), kNoTable, s, tally, idx, chain);
+            else if (chain && s == 0)
+                chain[idx] = 0;
         }
         next += take;
         taken += take;
@@ -564,7 +585,7 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
     uint64_t next = 0, end = 0;  // wave-uniform: the wave's current chunk
     uint64_t taken = 0;          // wave-uniform: input indexes this wave took
     while (true) {
-        refill<G, FROM_ENTRIES>(D, c, next, end, keys, vals, entries, pre_fixed, tally, taken, skip);
+        refill<G, FROM_ENTRIES>(D, c, next, end, keys, vals, entries, pre_fixed, tally, taken, skip, sink.chain);
         if (next == end) {
             // chunks w, w + W, w + 2W, ... (W = waves in the grid)
             const uint64_t base = next_chunk * kChunk;
@@ -593,6 +614,7 @@ __global__ __launch_bounds__(256) void insert_kernel(Desc D, const uint32_t *__r
     wave_tally(C, &C->kicks, tally.kicks);
     wave_tally(C, &C->rejected, tally.rejected);
     wave_tally(C, &C->retries, tally.retries);
+    wave_tally_max(C, &C->max_chain, tally.max_chain);
 }
 
 // -------------------------------------------------------------------------- partition
@@ -1494,7 +1516,10 @@ __global__ __launch_bounds__(64) void fold_post_kernel(unsigned long long *block
     unsigned long long v = 0;
     if (i < kWords) {
         v = block[i];
-        for (int s = 1; s <= kTallyStripes; ++s) v += block[kTallyStrideWords * s + i];
+        for (int s = 1; s <= kTallyStripes; ++s) {
+            const unsigned long long x = block[kTallyStrideWords * s + i];
+            v = i == kMaxChainWord ? (v > x ? v : x) : v + x;
+        }
         post[1 + i] = v;  // stored once every load of the sum has returned
     }
     // the clears come after those stores in program order, so after every load of the block

@@ -1,6 +1,8 @@
 // dyc_kernels.hpp -- launchers for the DyCuckoo HIP kernels (internal to libdyc).
 #pragma once
 
+#include <stddef.h>
+
 #include "dyc_device.hpp"
 
 namespace dyc {
@@ -24,8 +26,11 @@ struct Counters {
     unsigned long long updated;    // upsert: keys whose stored value was replaced
     unsigned long long residuals;  // downsize: entries of the upper half that did not stay
     unsigned long long placed;     // insert with PendSink::count_placed: entries taken in
-    unsigned long long pad[4];
+    unsigned long long max_chain;  // insert: the longest eviction chain (exchanges); folded by max
+    unsigned long long pad[3];
 };
+constexpr int kMaxChainWord = 12;  // index of max_chain: stripes fold into it by max, not by sum
+static_assert(offsetof(Counters, max_chain) == kMaxChainWord * sizeof(unsigned long long), "max_chain's word");
 constexpr int kTallyStripes = 64;
 constexpr int kTallyStrideWords = 32;  // 256 B between stripes
 constexpr int kCounterWords = kTallyStrideWords * (1 + kTallyStripes);
@@ -39,7 +44,10 @@ inline void fold_tallies(void *block)
     constexpr int kWords = (int)(sizeof(Counters) / sizeof(unsigned long long));
     unsigned long long *w = static_cast<unsigned long long *>(block);
     for (int s = 1; s <= kTallyStripes; ++s)
-        for (int i = 0; i < kWords; ++i) w[i] += w[kTallyStrideWords * s + i];
+        for (int i = 0; i < kWords; ++i) {
+            const unsigned long long x = w[kTallyStrideWords * s + i];
+            w[i] = i == kMaxChainWord ? (w[i] > x ? w[i] : x) : w[i] + x;
+        }
 }
 constexpr uint32_t kMaxRegions = 1024;
 
@@ -59,6 +67,8 @@ struct PendSink {
     unsigned long long *rejected_at;  // optional: += reserved keys this launch met (an atomic only
                                       // from a wave that met one; a keys insert of n took in n
                                       // minus this)
+    uint32_t *chain;             // optional: chain[i] = exchanges of the chain input i started
+                                 // (the reference's iterator_count, cuckoo.cu:349-351)
 };
 
 struct InsertPolicy {

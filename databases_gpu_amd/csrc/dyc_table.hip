@@ -321,6 +321,7 @@ int absorb_insert_counters(dyc_table *t, uint64_t room)
     t->st.cas_retries += c.retries;
     t->st.kick_failures += c.pending;
     t->st.lost += c.lost;
+    t->st.last_max_chain = std::max<uint32_t>(t->st.last_max_chain, (uint32_t)std::min<uint64_t>(c.max_chain, ~0u));
     t->pend_n += std::min<uint64_t>(c.pending, room);
     if (c.lost) return fail(DYC_ERR_LOST, "pending list overflow: %llu entries lost", (unsigned long long)c.lost);
     return DYC_OK;
@@ -642,6 +643,7 @@ int settle(dyc_table *t)
         t->st.cas_retries += c.retries;
         t->st.kick_failures += c.pending;
         t->st.lost += c.lost;
+        t->st.last_max_chain = std::max<uint32_t>(t->st.last_max_chain, (uint32_t)std::min<uint64_t>(c.max_chain, ~0u));
         if (c.lost && !rc) rc = fail(DYC_ERR_LOST, "pending staging overflow: %llu entries lost", (unsigned long long)c.lost);
         const uint64_t m = std::min<uint64_t>(c.pending, a.cap);
         if (m) {
@@ -1136,13 +1138,19 @@ int dyc_destroy(dyc_table *t)
 
 int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream)
 {
+    return dyc_insert_counted(t, keys, vals, n, nullptr, stream);
+}
+
+int dyc_insert_counted(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, uint32_t *iter_count,
+                       dyc_stream stream)
+{
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
     SETTLE(t);
     const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
-    t->st.last_kicks = 0;
+    t->st.last_kicks = t->st.last_max_chain = 0;
     if (n == 0) return DYC_OK;
     int rc;
     if (t->cfg.auto_resize) {
@@ -1154,8 +1162,9 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     rc = reserve_pending(t, std::max<uint64_t>(65536, n / 16), s);
     if (rc) return rc;
     const uint64_t room = t->pend_cap - t->pend_n;
-    const uint32_t R = insert_regions(t, n);
-    bool bulk = use_bulk_fill(t, n);
+    // per-key counts follow input order, which the bulk fill and the region partition do not keep
+    const uint32_t R = iter_count ? 0 : insert_regions(t, n);
+    bool bulk = !iter_count && use_bulk_fill(t, n);
     if (bulk) {
         const dyc::host::ErrorKeep keep;
         rc = reserve_bulk(t, n, make_chunks(t->D), s);
@@ -1182,8 +1191,10 @@ int dyc_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     } else {
         rc = zero_counters(t, s);
         if (rc) return rc;
+        PendSink sink = sink_at(t, t->pend_n);
+        sink.chain = iter_count;
         HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(), t->ctr,
-                              sink_at(t, t->pend_n), s));
+                              sink, s));
     }
     rc = read_counters(t, s);
     if (rc) return rc;
@@ -1228,7 +1239,7 @@ int dyc_insert_async(dyc_table *t, const uint32_t *keys, const uint32_t *vals, u
     }
     rc = async_reserve(*a, n);
     if (rc) return rc;
-    if (t->async_keys == 0) t->st.last_kicks = 0;
+    if (t->async_keys == 0) t->st.last_kicks = t->st.last_max_chain = 0;
     HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(a->ctr), kCounterWords, s));
     HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(), a->ctr,
                           PendSink{a->pend, nullptr, a->cap, 0, 0, nullptr}, s));
@@ -1266,7 +1277,7 @@ int dyc_insert_noalloc(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
                     (unsigned long long)n, (unsigned long long)room);
     DeviceGuard dg(t->device);
     hipStream_t s = (hipStream_t)stream;
-    t->st.last_kicks = 0;
+    t->st.last_kicks = t->st.last_max_chain = 0;
     if (n == 0) return DYC_OK;
     Counters *C = nullptr;
     int rc = noalloc_counters(t->device, &C);
@@ -1297,7 +1308,7 @@ int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     SETTLE(t);
     const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
-    t->st.last_kicks = 0;
+    t->st.last_kicks = t->st.last_max_chain = 0;
     if (n == 0) return DYC_OK;
     int rc;
     if (t->cfg.auto_resize) {  // sized as if every key were new (api.cpp:360-369)
@@ -1547,7 +1558,7 @@ int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, ui
     SETTLE(t);
     const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
-    t->st.last_kicks = 0;
+    t->st.last_kicks = t->st.last_max_chain = 0;
     if (n == 0) return DYC_OK;
     int rc;
     if (t->cfg.auto_resize) {

@@ -63,6 +63,21 @@ __device__ inline void wave_tally(Counters *C, unsigned long long *field, unsign
     wave_tally(C, field, v, wave_slot());
 }
 
+// The wave's largest v into the tally `field` of the wave's stripe (a field the fold takes the
+// maximum of, not the sum).
+__device__ inline void wave_tally_max(Counters *C, unsigned long long *field, unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long x = shfl64(v, __lane_id() ^ off);
+        v = v > x ? v : x;
+    }
+    if (__lane_id() == 0 && v) {
+        unsigned long long *base = reinterpret_cast<unsigned long long *>(C);
+        atomicMax(base + kTallyStrideWords * (1u + wave_slot() % kTallyStripes) + (field - base), v);
+    }
+}
+
 
 // Blocks for work_waves waves of work under a launch shape (capped; 0 max_blocks = default_cap).
 inline int grid_for(uint64_t work_waves, LaunchShape sh, uint64_t default_cap = 1ull << 20)

@@ -97,6 +97,20 @@ class DyCuckoo:
             check(self.lib.dyc_insert(self.h, k.data_ptr(), v.data_ptr(), k.numel(), _stream_ptr(stream)),
                   "dyc_insert")
 
+    def insert_counted(self, keys: torch.Tensor, vals: torch.Tensor, stream=None) -> torch.Tensor:
+        """Insert (eviction insert, input order) and return each key's eviction-chain length as
+        int32: the reference's iterator_count (cuckoo.cu:349-351; api.cpp:219-243 reports its
+        maximum and mean). Stats.last_max_chain holds the maximum."""
+        k = _as_u32_tensor(keys, self.device)
+        v = _as_u32_tensor(vals, self.device)
+        if k.numel() != v.numel():
+            raise ValueError("keys and values differ in length")
+        counts = torch.zeros(k.numel(), dtype=torch.int32, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_insert_counted(self.h, k.data_ptr(), v.data_ptr(), k.numel(), counts.data_ptr(),
+                                              _stream_ptr(stream)), "dyc_insert_counted")
+        return counts
+
     def insert_async(self, keys: torch.Tensor, vals: torch.Tensor, stream=None) -> None:
         """Enqueue an insert on `stream` without waiting (dyc_insert_async): inserts on several
         streams run at once; sync() (or any other call) completes them. The tensors must stay

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define DYC_ABI_VERSION 2
+#define DYC_ABI_VERSION 3
 
 #define DYC_OK 0
 #define DYC_ERR_INVALID 1        /* bad argument (null handle, bad config, size limit) */
@@ -99,7 +99,9 @@ typedef struct dyc_stats {
     uint64_t resizes_down;
     uint64_t lost;              /* must stay 0 */
     uint32_t subtables;         /* subtable count d */
-    uint32_t reserved0;
+    uint32_t last_max_chain;    /* the longest eviction chain (exchanges) of the most recent insert call
+                                   (32-bit tables; the maximum of the reference's iterator_count,
+                                   api.cpp:219-243) */
 } dyc_stats;
 
 /* Defaults (see field comments). */
@@ -158,6 +160,16 @@ int dyc_sync(dyc_table *t);
 #define DYC_NOALLOC_MAX_KEYS 65536u
 #define DYC_NOALLOC_SCRATCH_BYTES 16640u
 int dyc_insert_noalloc(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, dyc_stream stream);
+
+/* dyc_insert that also reports every key's eviction chain: iter_count[i] = the exchanges of the
+ * chain keys[i] started (its placement counts as one; 0 for a reserved key, and for a chain that
+ * parked on the pending list before its first exchange because its CAS retries ran out), the reference's
+ * iterator_count (gpu_lp_insert(..., d_iter_count), cuckoo.h:100-104, written at cuckoo.cu:349-351
+ * under ITERAOTOR_COUNT_FLAG; api.cpp:219-243 prints their maximum and mean). iter_count is a
+ * device array of n words. Keys are placed with the eviction insert whatever insert_order says.
+ * Re-placement of failed entries after an upsize is not counted again. Waits once, as dyc_insert. */
+int dyc_insert_counted(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, uint32_t *iter_count,
+                       dyc_stream stream);
 
 /* Insert with update (the paper's insert, tkde-revision/secs/cuckoo.tex:127-129; the reference
  * code has it switched off, dy_hash.h:42 update_on 0). A key already stored gets vals[i] in every

@@ -50,7 +50,7 @@ def test_library_exports_every_declared_symbol(lib):
 def test_abi_version_and_defaults(lib):
     import databases_gpu_amd as D
 
-    assert lib.dyc_abi_version() == 2
+    assert lib.dyc_abi_version() == 3
     cfg = D.default_config()
     assert cfg.kick_bound == 15 and cfg.rescue_kicks == 128
     assert cfg.lower_bound == 0.5 and cfg.upper_bound == 0.9
@@ -72,6 +72,7 @@ def test_errors_without_a_device_are_reported_not_crashed(lib):
         assert lib.dyc_create(ctypes.byref(cfg), 1000, ctypes.byref(h)) == 1 and b"subtables" in lib.dyc_last_error()
         assert lib.dyc_group_create(ctypes.byref(cfg), 1, (ctypes.c_uint64 * 1)(1000), ctypes.byref(h)) == 1
     assert lib.dyc_insert(None, None, None, 0, None) == 1
+    assert lib.dyc_insert_counted(None, None, None, 0, None, None) == 1
     assert lib.dyc_find(None, None, None, None, 0, None) == 1
     assert lib.dyc_get_stats(None, None) == 1
     # a subtable holds at most 2^29 - 1 buckets (bucket locations are packed in 29 bits): a hint
@@ -210,8 +211,9 @@ int main() {
 
 def test_counter_block_folds_every_stripe_into_the_base(tmp_path):
     """Per-wave tallies land in one of 64 striped copies of the counter block; the host adds every
-    stripe into the base copy once (fold_tallies). A block whose stripe s holds 1000 * (s + 1) +
-    word, over base words 7 * word, must fold to the closed-form sums and leave the stripes."""
+    stripe into the base copy once (fold_tallies), except max_chain (word 12), where it keeps the
+    largest. A block whose stripe s holds 1000 * (s + 1) + word, over base words 7 * word, must fold
+    to the closed-form sums (word 12: the last stripe's 1000 * 64 + 12) and leave the stripes."""
     src = tmp_path / "fold.cpp"
     src.write_text(r'''
 #include <cstdio>
@@ -229,10 +231,10 @@ int main() {
     int bad = 0;
     const unsigned long long S = kTallyStripes;
     for (int i = 0; i < 16; ++i)
-        bad += w[i] != 7ull * i + 1000ull * S * (S + 1) / 2 + S * i;
+        bad += w[i] != (i == kMaxChainWord ? 1000ull * S + i : 7ull * i + 1000ull * S * (S + 1) / 2 + S * i);
     for (int s = 0; s < kTallyStripes; ++s)
         for (int i = 0; i < 16; ++i) bad += w[kTallyStrideWords * (s + 1) + i] != 1000ull * (s + 1) + i;
-    bad += c.deleted != w[4] || c.placed != w[11];
+    bad += c.deleted != w[4] || c.placed != w[11] || c.max_chain != w[12] || kMaxChainWord != 12;
     printf("words %d stripes %d stride %d bad %d\n", kCounterWords, kTallyStripes, kTallyStrideWords, bad);
     return bad != 0;
 }
