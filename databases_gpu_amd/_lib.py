@@ -143,6 +143,9 @@ class ServerConfig(ctypes.Structure):
         ("one_launch", ctypes.c_uint32),
         ("spin_cap", ctypes.c_uint32),
         ("poll_backoff", ctypes.c_uint32),
+        ("resident", ctypes.c_uint32),
+        ("resident_blocks", ctypes.c_uint32),
+        ("resident_idle_ms", ctypes.c_uint32),
         ("pad", ctypes.c_uint32),
     ]
 
@@ -151,7 +154,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "submitted", "completed", "failed", "ticks", "rounds", "keys", "deferred_order", "deferred_memory", "deferred_slots",
         "resizes_up", "resizes_down", "pool_entries", "pool_rounds", "arena_bytes", "arena_used", "arena_peak",
-        "queued", "in_flight")]
+        "queued", "in_flight", "loop_launches", "loop_items", "loop_pauses")]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -72,6 +72,18 @@ __device__ inline void read_slots(const uint64_t *p, uint64_t (&e)[SPL])
         load_slots<SPL>(p, e);
 }
 
+// An input word (key or value). COH: read device-coherently, for a launch that outlives the
+// caller's writes to its inputs (a buffer read earlier in the launch may be rewritten and
+// handed in again, and this XCD's L2 may still hold the old line).
+template <bool COH>
+__device__ inline uint32_t load_in(const uint32_t *p)
+{
+    if constexpr (COH)
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return *p;
+}
+
 
 // ---------------------------------------------------------------- find / erase / update
 
@@ -158,8 +170,8 @@ __device__ inline uint32_t write_matches(const Desc &D, uint32_t mm, const uint6
 // lane's flag for its own key (hit, deleted or updated); all 64 lanes must call it together.
 // MODE 0 find: vals (optional) receives the value. MODE 1 erase: every copy of the key in the
 // first bucket (pos1, then pos2) that holds it is CAS'd to empty. MODE 2 update: those copies
-// are CAS'd to the new value vals[i] (an input). COH: read the buckets device-coherently (see
-// load_slots_coherent).
+// are CAS'd to the new value vals[i] (an input). COH: read the buckets and the inputs
+// device-coherently (see load_slots_coherent, load_in).
 template <int G, int MODE, bool COH = false>
 __device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                         uint8_t *__restrict__ flags, uint64_t base, uint64_t n)
@@ -173,8 +185,8 @@ __device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restric
     {
         const uint64_t i = base + lane;
         const bool inb = i < n;
-        const uint32_t k = inb ? keys[i] : 0u;  // past the end: key 0, which is reserved
-        const uint32_t nv = (UPDATE && inb) ? vals[i] : 0u;
+        const uint32_t k = inb ? load_in<COH>(keys + i) : 0u;  // past the end: key 0, which is reserved
+        const uint32_t nv = (UPDATE && inb) ? load_in<COH>(vals + i) : 0u;
         const uint32_t p = pair_of(D, k);
         const uint32_t t1 = pos1(D, p), t2 = pos2(D, p);
         const uint32_t loc1 = pack_loc(t1, bucket_of(D, k, t1));
@@ -532,7 +544,7 @@ __device__ inline void kick_step(const Desc &D, Carry<kSlots / G> &c, const Inse
 
 // Feed the wave's idle groups from input indexes [next, end), advancing next; returns when every
 // group is busy or the range is used up (some groups may then still be idle).
-template <int G, bool FROM_ENTRIES>
+template <int G, bool FROM_ENTRIES, bool COH = false>
 __device__ inline void refill(const Desc &D, Carry<kSlots / G> &c, uint64_t &next, uint64_t end, const uint32_t *__restrict__ keys,
                               const uint32_t *__restrict__ vals, const uint64_t *__restrict__ entries,
                               uint32_t pre_fixed, InsertTally &tally, uint64_t &taken,
@@ -556,7 +568,8 @@ __device__ inline void refill(const Desc &D, Carry<kSlots / G> &c, uint64_t &nex
                 else if (s == 0)
                     ++tally.empty;
             } else if (!skip || !skip[idx])
-                carry_start(D, c, pack(keys[idx], vals[idx]), kNoTable, s, tally, idx, chain);
+                carry_start(D, c, pack(load_in<COH>(keys + idx), load_in<COH>(vals + idx)), kNoTable, s, tally, idx,
+                            chain);
             else if (chain && s == 0)
                 chain[idx] = 0;
         }
@@ -979,8 +992,9 @@ __device__ inline unsigned long long serve_probe_item(const Desc &D, const Serve
     return nd;
 }
 
-// One item of inserts; the wave's groups are all idle on return.
-template <int G>
+// One item of inserts; the wave's groups are all idle on return. COH: read the inputs
+// device-coherently (load_in).
+template <int G, bool COH = false>
 __device__ inline unsigned long long serve_insert_item(const Desc &D, const ServeItem &item, Carry<kSlots / G> &c,
                                                        const InsertPolicy &P, Counters *C, const PendSink &sink,
                                                        InsertTally &tally)
@@ -990,7 +1004,7 @@ __device__ inline unsigned long long serve_insert_item(const Desc &D, const Serv
     const unsigned long long rej0 = tally.rejected;
     uint64_t taken = 0;
     while (true) {
-        refill<G, false>(D, c, next, end, item.keys, item.vals, nullptr, kNoTable, tally, taken);
+        refill<G, false, COH>(D, c, next, end, item.keys, item.vals, nullptr, kNoTable, tally, taken);
         if (!__ballot(c.active)) break;
         if (c.active) kick_step<G>(D, c, P, C, sink, item.table, tally);
     }
@@ -1139,6 +1153,219 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
     wave_tally(C, &C->rejected, tally.rejected);
     wave_tally(C, &C->retries, tally.retries);
     if (__lane_id() == 0 && stalls) tally_add(C, &C->stalls, stalls, wave_slot());
+}
+
+// ----------------------------------------------------------- device-resident serving loop
+// One launch serves requests as the host publishes them (LoopItem). Wave 0 of block 0 is the
+// front wave: it polls the control block in host memory and copies the published item count
+// into the device word avail. Every other wave claims item indexes from the device cursor in
+// order and waits, polling device memory, until its index is published. An item of a table
+// whose earlier requests have not finished waits until tbl_done[table] reaches its need. Items
+// wait only on items with smaller indexes, which running waves have claimed (the grid is
+// resident as a whole), so the earliest unfinished item never waits and every published item
+// finishes. A finished item writes its results back to memory (a system-scope release), then
+// counts itself into its request's progress word; the wave that counts a request's last item
+// advances tbl_done, writes the request's record and appends the request to the completion
+// queue in host memory.
+// Every wait is bounded. The front wave ends the launch on the host's stop flag or after
+// idle_ticks without new items: it raises quit, and waves waiting for an index past the
+// published ones leave. An item gives up waiting for its table after spin_cap polls, or at once
+// once the host has given up on the launch (stop = 2 raises abort), and counts as a stall (the
+// host fails its request). A ring entry whose stamp does not match after
+// spin_cap polls raises the fault flag and quit.
+// Tables, inputs and the directory are read device-coherently: the launch outlives the writes
+// of resizes and of the caller, and this XCD's L2 may hold lines they made stale.
+
+__device__ inline uint64_t ld_system(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ inline uint64_t ld_agent(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint64_t uniform64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (uint64_t)hi << 32 | lo;
+}
+
+__device__ inline void sleep_units(uint32_t n)  // n x 256 cycles
+{
+    for (uint32_t b = 0; b < n; ++b) __builtin_amdgcn_s_sleep(4);
+}
+
+// The W words lanes 0..W-1 hold, as one wave-uniform T.
+template <int W, typename T>
+__device__ inline T wave_gather(uint64_t v)
+{
+    static_assert(sizeof(T) == W * 8, "T is W words");
+    uint64_t w[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), j);
+        w[j] = (uint64_t)hi << 32 | lo;
+    }
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
+__device__ inline Desc load_desc_coherent(const Desc *p)
+{
+    constexpr int W = sizeof(Desc) / 8;
+    const unsigned long long *w = reinterpret_cast<const unsigned long long *>(p);
+    const uint64_t v = __lane_id() < W ? ld_agent(w + __lane_id()) : 0;
+    return wave_gather<W, Desc>(v);
+}
+
+__device__ inline void loop_front(const LoopArgs &A)
+{
+    unsigned long long *avail = A.words + kLoopAvail;
+    uint64_t seen = uniform64(ld_agent(avail));
+    uint64_t last = __builtin_amdgcn_s_memrealtime();
+    while (true) {
+        const uint64_t t = uniform64(ld_system(&A.ctl->tail));
+        const uint64_t stop = uniform64(ld_system(&A.ctl->stop));
+        if (stop) {
+            if (stop == 2 && __lane_id() == 0)
+                __hip_atomic_store(A.words + kLoopAbort, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        const uint64_t now = __builtin_amdgcn_s_memrealtime();
+        if (t != seen) {
+            if (__lane_id() == 0) __hip_atomic_store(avail, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            seen = t;
+            last = now;
+            continue;
+        }
+        if (now - last > A.idle_ticks) break;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    if (__lane_id() == 0) __hip_atomic_store(A.words + kLoopQuit, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int GI, int GF>
+__global__ __launch_bounds__(256) void serve_loop_kernel(LoopArgs A, InsertPolicy P, Counters *__restrict__ C,
+                                                         PendSink sink, Pool pool,
+                                                         unsigned long long *__restrict__ ins_total,
+                                                         unsigned long long *__restrict__ del_total)
+{
+    if (blockIdx.x == 0 && threadIdx.x < kWave) {
+        loop_front(A);
+        return;
+    }
+    Carry<kSlots / GI> c;
+    c.active = false;
+    InsertTally tally;
+    WaveTotal ins, del;
+    const int lane = __lane_id();
+    unsigned long long *cursor = A.words + kLoopCursor, *avail = A.words + kLoopAvail, *quit = A.words + kLoopQuit;
+    while (true) {
+        uint64_t idx = 0;
+        if (lane == 0) idx = atomicAdd(cursor, 1ull);
+        idx = uniform64(idx);
+        // wait until the index is published, or the launch ends
+        bool go = true;
+        for (uint32_t back = 1;;) {
+            if (idx < uniform64(ld_agent(avail))) break;
+            // the front wave stores avail before quit (a release), and this reload is issued only
+            // after quit was read, so it sees the last published count
+            if (uniform64(ld_agent(quit))) {
+                go = idx < uniform64(ld_agent(avail));
+                break;
+            }
+            sleep_units(back);
+            back = back < A.max_back ? back * 2 : A.max_back;
+        }
+        if (!go) break;
+        // the item, 8 words from host memory; its stamp tells that the host finished writing it
+        const unsigned long long *src = reinterpret_cast<const unsigned long long *>(A.ring + (idx & A.ring_mask));
+        LoopItem it;
+        bool ok = false;
+        for (uint32_t spins = 0; spins <= A.spin_cap; ++spins) {
+            it = wave_gather<8, LoopItem>(lane < 8 ? ld_system(src + lane) : 0ull);
+            if (it.stamp == (uint32_t)(idx + 1)) {
+                ok = true;
+                break;
+            }
+            sleep_units(1);
+        }
+        if (!ok) {
+            if (lane == 0) {
+                __hip_atomic_store(A.words + kLoopFault, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(quit, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            break;
+        }
+        const uint32_t table = it.item.table;
+        uint32_t stalled = 0;
+        if (it.need) {
+            uint32_t back = 1, spins = 0;
+            while (uniform64(ld_agent(A.tbl_done + table)) < it.need) {
+                if (++spins > A.spin_cap || uniform64(ld_agent(A.words + kLoopAbort))) {
+                    stalled = 1;
+                    break;
+                }
+                sleep_units(back);
+                back = back < A.max_back ? back * 2 : A.max_back;
+            }
+        }
+        const Desc D = load_desc_coherent(A.dir + table);
+        unsigned long long acc;
+        if (it.op == 0) {
+            acc = serve_insert_item<GI, true>(D, it.item, c, P, C, sink, tally);
+            ins.add(ins_total, table, acc);
+        } else {
+            const uint64_t np = pool_size(pool);
+            if (it.op == 1) {
+                acc = wave_sum(serve_probe_item<GF, false, true>(D, it.item, pool, np));
+            } else {
+                acc = wave_sum(serve_probe_item<GF, true, true>(D, it.item, pool, np));
+                del.add(del_total, table, acc);
+            }
+        }
+        if (lane == 0 && stalled) atomicAdd(A.req_stall + it.req, 1u);
+        // results and table writes to memory before the item counts as done
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        if (lane == 0) {
+            const unsigned long long old = atomicAdd(A.req_word + it.req, (acc << kLoopItemBits) | 1ull);
+            if ((old & ((1ull << kLoopItemBits) - 1)) + 1 == it.n_items) {
+                const unsigned long long total = (old >> kLoopItemBits) + acc;
+                const uint32_t stalls = __hip_atomic_load(A.req_stall + it.req, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(A.req_word + it.req, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(A.req_stall + it.req, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicAdd(A.tbl_done + table, 1ull);  // the table's next requests may start
+                LoopRec *r = A.rec + it.req;
+                __hip_atomic_store(&r->acc, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(&r->pool, pool.count ? ld_agent(pool.count) : 0ull, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(&r->lost, ld_agent(&C->lost), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(&r->stalls, stalls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+                const unsigned long long pos = atomicAdd(A.words + kLoopCqTail, 1ull);
+                __hip_atomic_store(A.cq + (pos & A.req_mask), ((pos + 1) << 32) | it.req, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+    }
+    ins.flush(ins_total);
+    del.flush(del_total);
+    wave_tally(C, &C->kicks, tally.kicks);
+    wave_tally(C, &C->rejected, tally.rejected);
+    wave_tally(C, &C->retries, tally.retries);
+}
+
+__global__ void loop_reset_kernel(unsigned long long *words)
+{
+    words[kLoopCursor] = words[kLoopAvail];
+    words[kLoopQuit] = 0;
+    words[kLoopFault] = 0;
+    words[kLoopAbort] = 0;
 }
 
 // Copy between host-pinned and device memory with a kernel instead of a DMA transfer: a
@@ -1497,6 +1724,75 @@ hipError_t launch_serve_tick(const Desc *dir, const TickItem *items, uint32_t n_
         return tick_by_find_lanes<1>(grid, find_lanes, dir, items, n_items, cursor, done, pol, C, sink, pool, ins_total,
                                      del_total, spin_cap, max_back, st);
     }
+}
+
+template <int GI>
+static hipError_t loop_by_find_lanes(int grid, int find_lanes, const LoopArgs &A, InsertPolicy pol, Counters *C,
+                                     PendSink sink, Pool pool, unsigned long long *ins_total,
+                                     unsigned long long *del_total, hipStream_t st)
+{
+    switch (find_lanes) {
+    case 4: serve_loop_kernel<GI, 4><<<grid, 256, 0, st>>>(A, pol, C, sink, pool, ins_total, del_total); break;
+    case 16: serve_loop_kernel<GI, 16><<<grid, 256, 0, st>>>(A, pol, C, sink, pool, ins_total, del_total); break;
+    default: serve_loop_kernel<GI, 8><<<grid, 256, 0, st>>>(A, pol, C, sink, pool, ins_total, del_total); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_loop(const LoopArgs &A, InsertPolicy pol, int insert_lanes, int find_lanes, int blocks,
+                             Counters *C, PendSink sink, Pool pool, unsigned long long *ins_total,
+                             unsigned long long *del_total, hipStream_t st)
+{
+    if (!serve_tick_supported(insert_lanes, find_lanes) || blocks < 2) return hipErrorInvalidValue;
+    switch (insert_lanes) {
+    case 2: return loop_by_find_lanes<2>(blocks, find_lanes, A, pol, C, sink, pool, ins_total, del_total, st);
+    case 4: return loop_by_find_lanes<4>(blocks, find_lanes, A, pol, C, sink, pool, ins_total, del_total, st);
+    default: return loop_by_find_lanes<1>(blocks, find_lanes, A, pol, C, sink, pool, ins_total, del_total, st);
+    }
+}
+
+hipError_t launch_loop_reset(unsigned long long *words, hipStream_t st)
+{
+    loop_reset_kernel<<<1, 1, 0, st>>>(words);
+    return hipGetLastError();
+}
+
+template <int GI, int GF>
+static int loop_blocks_per_cu()
+{
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, serve_loop_kernel<GI, GF>, 256, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return per;
+}
+
+template <int GI>
+static int loop_blocks_by_find_lanes(int find_lanes)
+{
+    switch (find_lanes) {
+    case 4: return loop_blocks_per_cu<GI, 4>();
+    case 16: return loop_blocks_per_cu<GI, 16>();
+    default: return loop_blocks_per_cu<GI, 8>();
+    }
+}
+
+int serve_loop_capacity(int insert_lanes, int find_lanes)
+{
+    if (!serve_tick_supported(insert_lanes, find_lanes)) return 0;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    int per = 0;
+    switch (insert_lanes) {
+    case 2: per = loop_blocks_by_find_lanes<2>(find_lanes); break;
+    case 4: per = loop_blocks_by_find_lanes<4>(find_lanes); break;
+    default: per = loop_blocks_by_find_lanes<1>(find_lanes); break;
+    }
+    return per * cus;
 }
 
 hipError_t launch_zero(unsigned long long *p, uint64_t words, hipStream_t st)

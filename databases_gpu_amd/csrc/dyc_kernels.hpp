@@ -247,6 +247,82 @@ hipError_t launch_serve_find(const Desc *dir, const ServeItem *items, uint32_t n
 hipError_t launch_serve_erase(const Desc *dir, const ServeItem *items, uint32_t n_items, Pool pool,
                               unsigned long long *del_total, int lanes, int blocks, hipStream_t st);
 
+// ---- device-resident serving loop
+// One launch that keeps running and takes work items from a ring in host memory as the host
+// publishes them. Host memory (coherent pinned, hipHostMallocCoherent): the ring, the control
+// block, the per-request records and the completion queue. Device memory: the loop words, the
+// per-table completion counts and the per-request progress words.
+//
+// A work item, 64 bytes, written by the host; stamp (the low 32 bits of the item's index + 1)
+// is written last, and the control block's tail after the items it publishes.
+struct LoopItem {
+    ServeItem item;
+    uint32_t op;       // 0 insert, 1 find, 2 erase
+    uint32_t req;      // request slot (index into the records and progress words)
+    uint64_t need;     // the item starts once tbl_done[item.table] >= need
+    uint32_t n_items;  // items of the request
+    uint32_t stamp;
+};
+static_assert(sizeof(LoopItem) == 64, "one 64-byte line per item");
+
+// Host-written control: tail = items published; stop = 1 asks the launch to end, stop = 2 also
+// cuts every wait for a table short (the host gave up on the launch).
+struct LoopCtl {
+    unsigned long long tail;
+    unsigned long long pad0[7];
+    unsigned long long stop;
+    unsigned long long pad1[7];
+};
+
+// Written by the device for a finished request, before its completion queue entry: acc = keys
+// taken in (insert), hits (find) or deletions (erase); pool = the overflow pool's append index
+// when the request finished; stalls = items that gave up waiting for the table's earlier requests.
+// The record of a slot is read only after the slot's completion queue entry.
+struct LoopRec {
+    unsigned long long acc;
+    unsigned long long pool;
+    uint32_t stalls;
+    uint32_t pad;
+    unsigned long long lost;  // entries lost so far (the pool was full)
+};
+
+// Device words of the loop, each on its own 128-byte line: item cursor, items published as the
+// front wave last saw them, quit flag, completion queue tail, fault flag, abort flag (the host
+// gave up: waits for a table's earlier requests end at once, as stalls).
+constexpr int kLoopCursor = 0, kLoopAvail = 16, kLoopQuit = 32, kLoopCqTail = 48, kLoopFault = 64, kLoopAbort = 80;
+constexpr int kLoopWords = 96;
+// Per-request progress word: items done in the low kLoopItemBits bits, acc above them.
+constexpr int kLoopItemBits = 24;
+
+struct LoopArgs {
+    const Desc *dir;
+    const LoopItem *ring;  // host memory
+    uint64_t ring_mask;    // ring entries - 1 (a power of two)
+    const LoopCtl *ctl;    // host memory
+    LoopRec *rec;          // host memory, one per request slot
+    unsigned long long *cq;  // host memory: entry p = (p + 1) << 32 | request slot
+    uint32_t req_mask;     // request slots - 1 (a power of two; also the completion queue's)
+    uint32_t spin_cap;     // polls an item waits for its table's earlier requests
+    unsigned long long *words;     // kLoopWords, device
+    unsigned long long *tbl_done;  // requests finished per table, device
+    unsigned long long *req_word;  // per request slot, device, 0 when free
+    uint32_t *req_stall;           // per request slot, device, 0 when free
+    uint64_t idle_ticks;   // the launch ends after this long without new items (100 MHz clock)
+    uint32_t max_back;     // a waiting wave sleeps up to max_back x 256 cycles between polls
+    uint32_t pad;
+};
+
+// Launch the loop on st with `blocks` blocks of 256 threads (all of them resident at once).
+// words[kLoopCursor] must equal words[kLoopAvail] and the quit flag must be 0 (loop_reset).
+hipError_t launch_serve_loop(const LoopArgs &A, InsertPolicy pol, int insert_lanes, int find_lanes, int blocks,
+                             Counters *C, PendSink sink, Pool pool, unsigned long long *ins_total,
+                             unsigned long long *del_total, hipStream_t st);
+// Before a launch: cursor = avail (items claimed past the published ones by the previous
+// launch's waves were not worked on), quit, fault and abort = 0.
+hipError_t launch_loop_reset(unsigned long long *words, hipStream_t st);
+// Blocks of 256 threads the loop kernel can keep resident on this device (its occupancy).
+int serve_loop_capacity(int insert_lanes, int find_lanes);
+
 // upsize subtable t: old has old_nb buckets, dst has 2*old_nb and is written completely.
 hipError_t launch_upsize(const uint64_t *old, uint32_t old_nb, uint64_t *dst, uint32_t t, Counters *C,
                          hipStream_t st);

@@ -17,6 +17,18 @@
 //     downsize, and entries that exhausted the eviction bound are re-placed.
 // All table storage is carved from one arena (dyc_arena.hpp), so admission by free HBM is exact
 // and no hipMalloc/hipFree runs while ticks are in flight.
+//
+// Device-resident mode (resident = 1, or replay mode 2): instead of ticks, one launch of the loop
+// kernel (serve_loop_kernel) stays on the stream and takes work items from a ring in host memory.
+// Submission admits a request at once: it is cut into items, each stamped with the count of the
+// table's earlier requests it must wait for (a table's requests run as stages: a run of inserts,
+// a run of finds, or one erase, each after the previous stage), and the ring's tail is advanced.
+// The device appends finished requests to a completion queue in host memory, which poll reads.
+// Resizes run on a second stream while the loop keeps serving other tables: a table is resized
+// only when it has no request in flight, and its next requests are published once the resize
+// and the directory update have finished. Re-placing the overflow pool needs every table quiet:
+// the host stops publishing, waits for the loop to finish what it has, stops the launch, re-places
+// the pool and starts the loop again.
 #include "dyc_arena.hpp"
 #include "dyc_host.hpp"
 #include "dyc_kernels.hpp"
@@ -102,6 +114,46 @@ struct Tick {
     std::vector<TickTable> tables;
 };
 
+// A request slot of the resident loop (index into the device's per-request words and the
+// host-memory records).
+struct LoopSlot {
+    uint64_t ticket = 0;
+    uint64_t pseq = 0;  // publish order
+    uint64_t n = 0;
+    uint32_t table = 0, op = 0;
+};
+
+constexpr uint64_t kLoopRingItems = 1ull << 18;  // 16 MiB of items in host memory
+constexpr uint32_t kLoopSlots = 1u << 16;        // requests in flight at once
+
+struct Loop {
+    bool on = false;       // requests go through the loop
+    bool running = false;  // a launch is on the stream and has not been seen to end
+    hipStream_t st2 = nullptr;  // resizes and counter reads while the loop is on
+    hipEvent_t ended = nullptr;  // recorded after each launch
+    LoopCtl *ctl = nullptr;      // host memory (coherent)
+    LoopItem *ring = nullptr;
+    LoopRec *rec = nullptr;
+    unsigned long long *cq = nullptr;
+    unsigned long long *d_words = nullptr, *d_tbl_done = nullptr, *d_req_word = nullptr;
+    uint32_t *d_req_stall = nullptr;
+    int blocks = 0;
+    uint64_t tail = 0;     // items published
+    uint64_t cq_head = 0;  // completion entries read
+    uint64_t pseq = 0;     // requests published
+    std::vector<LoopSlot> slot;
+    std::vector<uint32_t> free_slots;
+    // requests in publish order from the oldest unfinished one: (first item, finished)
+    uint64_t order_base = 0;
+    std::deque<std::pair<uint64_t, bool>> order;
+    std::vector<uint64_t> pub;    // per table: requests published through the loop
+    std::vector<uint64_t> need;   // per table: the current stage's wait count
+    std::vector<int8_t> last_op;  // per table: the current stage's op (-1: a new stage starts)
+    bool want_resolve = false;
+    uint64_t in_flight() const { return slot.size() - free_slots.size(); }
+    uint64_t oldest_item() const { return order.empty() ? tail : order.front().first; }
+};
+
 }  // namespace
 
 struct dyc_server {
@@ -149,8 +201,12 @@ struct dyc_server {
     std::vector<uint32_t> touched;
     std::vector<std::pair<uint64_t, uint32_t>> admitted;  // (ticket, round)
     std::vector<ServeItem> items[kMaxRounds][3];
+    Loop loop;
 
     Counters *pc() const { return reinterpret_cast<Counters *>(d_stat); }
+    // stream of resizes, pool re-placement and counter reads: the serving stream, or the second
+    // stream while the loop holds the serving stream
+    hipStream_t ms() const { return loop.on ? loop.st2 : st; }
     unsigned long long *ins_total() const { return d_stat + kCounterWords; }
     unsigned long long *del_total() const { return d_stat + kCounterWords + n; }
     size_t stat_bytes() const { return (kCounterWords + 2 * (size_t)n) * sizeof(unsigned long long); }
@@ -195,7 +251,7 @@ int upsize_now(dyc_server *s, uint32_t i, bool *ok)
     if ((uint64_t)old_nb * 2 >= kMaxBuckets) return DYC_OK;
     const uint64_t off = s->arena.alloc((uint64_t)old_nb * 2 * kBucketBytes);
     if (off == Arena::kNone) return DYC_OK;
-    HIP_TRY(launch_upsize(T.D.sub[sub], old_nb, s->sub_ptr(off), sub, s->pc(), s->st));
+    HIP_TRY(launch_upsize(T.D.sub[sub], old_nb, s->sub_ptr(off), sub, s->pc(), s->ms()));
     s->arena.release(T.off[sub]);
     T.off[sub] = off;
     desc_set(T.D, (int)sub, s->sub_ptr(off), old_nb * 2);
@@ -232,8 +288,8 @@ int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
     uint64_t *old = T.D.sub[sub];
     uint64_t *res = old + (uint64_t)new_nb * kSlots;
     const uint64_t res_n = (uint64_t)new_nb * kSlots;
-    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(rctr), kCounterWords, s->st));
-    HIP_TRY(launch_downsize(old, new_nb, s->sub_ptr(off), rctr, res, res_n, live, s->st));
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(rctr), kCounterWords, s->ms()));
+    HIP_TRY(launch_downsize(old, new_nb, s->sub_ptr(off), rctr, res, res_n, live, s->ms()));
     const uint64_t old_off = T.off[sub];
     T.off[sub] = off;
     desc_set(T.D, (int)sub, s->sub_ptr(off), new_nb);
@@ -241,7 +297,7 @@ int downsize_now(dyc_server *s, uint32_t i, Counters *rctr, bool *done)
     // residuals continue from the shrunk subtable into the other one of their pair
     // (cuckoo.cu:719), after the compaction; failures go to the pool
     HIP_TRY(launch_insert_entries(T.D, res, res_n, sub, (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(),
-                                  s->pc(), s->sink(i), s->st, live));
+                                  s->pc(), s->sink(i), s->ms(), live));
     s->arena.release(old_off);
     s->arena.release(loff);
     T.resizes_down++;
@@ -259,8 +315,8 @@ bool wants_down(const dyc_server *s, const SrvTable &T)
 // dst is host-pinned (a tick's snapshot buffer)
 int read_stats_sync(dyc_server *s, unsigned long long *dst)
 {
-    HIP_TRY(launch_copy(s->d_stat, dst, s->stat_bytes() / 8, nullptr, nullptr, 0, s->st));
-    HIP_TRY(hipStreamSynchronize(s->st));
+    HIP_TRY(launch_copy(s->d_stat, dst, s->stat_bytes() / 8, nullptr, nullptr, 0, s->ms()));
+    HIP_TRY(hipStreamSynchronize(s->ms()));
     fold_tallies(dst);
     return DYC_OK;
 }
@@ -270,7 +326,7 @@ int read_stats_sync(dyc_server *s, unsigned long long *dst)
 // where finds and erases keep seeing them.
 int resolve_pool(dyc_server *s, unsigned long long *scratch)
 {
-    HIP_TRY(hipStreamSynchronize(s->st));
+    HIP_TRY(hipStreamSynchronize(s->ms()));
     std::vector<uint64_t> ent;
     std::vector<uint32_t> tag;
     for (uint32_t round = 0; round < s->cfg.max_fail_rounds; ++round) {
@@ -288,7 +344,7 @@ int resolve_pool(dyc_server *s, unsigned long long *scratch)
         HIP_TRY(hipMemcpy(ent.data(), s->d_pool, m * sizeof(uint64_t), hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(tag.data(), s->d_pool_tag, m * sizeof(uint32_t), hipMemcpyDeviceToHost));
         // empty the pool (its append index), then re-insert table by table
-        HIP_TRY(launch_zero(&s->pc()->pending, 1, s->st));
+        HIP_TRY(launch_zero(&s->pc()->pending, 1, s->ms()));
         std::vector<uint64_t> order(m);
         for (uint64_t j = 0; j < m; ++j) order[j] = j;
         std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return tag[a] < tag[b]; });
@@ -307,16 +363,16 @@ int resolve_pool(dyc_server *s, unsigned long long *scratch)
         }
         if (!packed.empty())
             HIP_TRY(hipMemcpyAsync(s->d_resolve, packed.data(), packed.size() * sizeof(uint64_t),
-                                   hipMemcpyHostToDevice, s->st));
+                                   hipMemcpyHostToDevice, s->ms()));
         for (const Span &sp : spans) {
             bool ok = false;
             rc = upsize_now(s, sp.table, &ok);
             if (rc) return rc;
             HIP_TRY(launch_insert_entries(s->t[sp.table].D, s->d_resolve + sp.begin, sp.len, kNoTable,
                                           (int)s->cfg.insert_lanes, s->policy(), s->insert_shape(), s->pc(),
-                                          s->sink(sp.table), s->st));
+                                          s->sink(sp.table), s->ms()));
         }
-        HIP_TRY(hipStreamSynchronize(s->st));
+        HIP_TRY(hipStreamSynchronize(s->ms()));
         s->ss.pool_rounds++;
     }
     int rc = read_stats_sync(s, scratch);
@@ -865,9 +921,482 @@ int run_sync(dyc_server *s, uint64_t tk)
     return DYC_OK;
 }
 
+// ------------------------------------------------------------------ device-resident loop
+
+constexpr double kLoopGiveUpS = 60.0;  // host-side bound on waiting for the loop to finish or stop
+
+// Copy the directory to the device on the maintenance stream and wait for it.
+int loop_upload_dir(dyc_server *s)
+{
+    Desc *h = s->ticks[0].h_dir;  // pinned; the ticks are idle while the loop is on
+    for (uint32_t i = 0; i < s->n; ++i) h[i] = s->t[i].D;
+    HIP_TRY(launch_copy(h, s->d_dir, s->n * sizeof(Desc) / 8, nullptr, nullptr, 0, s->ms()));
+    HIP_TRY(hipStreamSynchronize(s->ms()));
+    s->dir_dirty = false;
+    return DYC_OK;
+}
+
+int loop_alloc(dyc_server *s)
+{
+    Loop &L = s->loop;
+    if (L.ctl) return DYC_OK;
+    const int cap = serve_loop_capacity((int)s->cfg.insert_lanes, (int)s->cfg.find_lanes);
+    if (cap < 2) return fail(DYC_ERR_INVALID, "the resident loop needs insert_lanes 1, 2 or 4 and find_lanes 4, 8 or 16");
+    // default: half the resident capacity, so resizes on the second stream (and the caller's own
+    // kernels) find room beside the loop instead of waiting for it to end
+    L.blocks = s->scfg.resident_blocks ? std::min<int>((int)s->scfg.resident_blocks, cap) : cap / 2;
+    if (L.blocks < 2) L.blocks = 2;
+    HIP_TRY(hipStreamCreateWithFlags(&L.st2, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&L.ended, hipEventDisableTiming));
+    const unsigned fl = hipHostMallocCoherent;
+    HIP_TRY(hipHostMalloc((void **)&L.ctl, sizeof(LoopCtl), fl));
+    HIP_TRY(hipHostMalloc((void **)&L.ring, kLoopRingItems * sizeof(LoopItem), fl));
+    HIP_TRY(hipHostMalloc((void **)&L.rec, kLoopSlots * sizeof(LoopRec), fl));
+    HIP_TRY(hipHostMalloc((void **)&L.cq, kLoopSlots * sizeof(unsigned long long), fl));
+    memset(L.ctl, 0, sizeof(LoopCtl));
+    memset(L.ring, 0, kLoopRingItems * sizeof(LoopItem));
+    memset(L.rec, 0, kLoopSlots * sizeof(LoopRec));
+    memset(L.cq, 0, kLoopSlots * sizeof(unsigned long long));
+    const size_t dev_bytes = (kLoopWords + (size_t)s->n + kLoopSlots) * 8 + (size_t)kLoopSlots * 4;
+    unsigned long long *d = nullptr;
+    if (hipMalloc((void **)&d, dev_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DYC_ERR_OOM, "hipMalloc of the loop's %zu bytes failed", dev_bytes);
+    }
+    L.d_words = d;
+    L.d_tbl_done = d + kLoopWords;
+    L.d_req_word = L.d_tbl_done + s->n;
+    L.d_req_stall = reinterpret_cast<uint32_t *>(L.d_req_word + kLoopSlots);
+    HIP_TRY(launch_zero(d, dev_bytes / 8, L.st2));
+    HIP_TRY(hipStreamSynchronize(L.st2));
+    L.slot.assign(kLoopSlots, LoopSlot{});
+    L.free_slots.resize(kLoopSlots);
+    for (uint32_t i = 0; i < kLoopSlots; ++i) L.free_slots[i] = kLoopSlots - 1 - i;
+    L.pub.assign(s->n, 0);
+    L.need.assign(s->n, 0);
+    L.last_op.assign(s->n, -1);
+    return DYC_OK;
+}
+
+void loop_free(dyc_server *s)
+{
+    Loop &L = s->loop;
+    if (L.st2) (void)hipStreamSynchronize(L.st2);
+    if (L.ctl) (void)hipHostFree(L.ctl);
+    if (L.ring) (void)hipHostFree(L.ring);
+    if (L.rec) (void)hipHostFree(L.rec);
+    if (L.cq) (void)hipHostFree(L.cq);
+    if (L.d_words) (void)hipFree(L.d_words);
+    if (L.ended) (void)hipEventDestroy(L.ended);
+    if (L.st2) (void)hipStreamDestroy(L.st2);
+    L = Loop{};
+}
+
+int loop_launch(dyc_server *s)
+{
+    Loop &L = s->loop;
+    __atomic_store_n(&L.ctl->stop, 0ull, __ATOMIC_RELEASE);
+    HIP_TRY(launch_loop_reset(L.d_words, s->st));
+    LoopArgs A{};
+    A.dir = s->d_dir;
+    A.ring = L.ring;
+    A.ring_mask = kLoopRingItems - 1;
+    A.ctl = L.ctl;
+    A.rec = L.rec;
+    A.cq = L.cq;
+    A.req_mask = kLoopSlots - 1;
+    A.spin_cap = s->scfg.spin_cap;
+    A.words = L.d_words;
+    A.tbl_done = L.d_tbl_done;
+    A.req_word = L.d_req_word;
+    A.req_stall = L.d_req_stall;
+    A.idle_ticks = (uint64_t)std::max<uint32_t>(1, s->scfg.resident_idle_ms) * 100000ull;  // 100 MHz clock
+    A.max_back = std::max<uint32_t>(1, s->scfg.poll_backoff);
+    HIP_TRY(launch_serve_loop(A, s->policy(), (int)s->cfg.insert_lanes, (int)s->cfg.find_lanes, L.blocks, s->pc(),
+                              s->sink(0), s->pool(), s->ins_total(), s->del_total(), s->st));
+    HIP_TRY(hipEventRecord(L.ended, s->st));
+    L.running = true;
+    s->ss.loop_launches++;
+    return DYC_OK;
+}
+
+// Notice a launch that has ended (idle time-out, stop or fault).
+int loop_check_ended(dyc_server *s)
+{
+    Loop &L = s->loop;
+    if (!L.running) return DYC_OK;
+    const hipError_t e = hipEventQuery(L.ended);
+    if (e == hipErrorNotReady) return DYC_OK;
+    if (e != hipSuccess) return sticky(s, fail(DYC_ERR_HIP, "loop launch: %s", hipGetErrorString(e)));
+    L.running = false;
+    unsigned long long fault = 0;
+    HIP_TRY(hipMemcpy(&fault, L.d_words + kLoopFault, sizeof fault, hipMemcpyDeviceToHost));
+    if (fault) return sticky(s, fail(DYC_ERR_HIP, "the loop found a ring entry the host had not finished writing"));
+    return DYC_OK;
+}
+
+// Read the completion queue: finish each request it names.
+int loop_collect(dyc_server *s, uint32_t *got)
+{
+    Loop &L = s->loop;
+    *got = 0;
+    while (true) {
+        const unsigned long long e = __atomic_load_n(L.cq + (L.cq_head & (kLoopSlots - 1)), __ATOMIC_ACQUIRE);
+        if ((uint32_t)(e >> 32) != (uint32_t)(L.cq_head + 1)) return DYC_OK;
+        L.cq_head++;
+        const uint32_t si = (uint32_t)e;
+        if (si >= kLoopSlots || L.slot[si].pseq < L.order_base)
+            return sticky(s, fail(DYC_ERR_CORRUPT, "the loop's completion queue names slot %u", si));
+        const LoopRec *r = L.rec + si;
+        const unsigned long long acc = __atomic_load_n(&r->acc, __ATOMIC_ACQUIRE);
+        const unsigned long long pool = __atomic_load_n(&r->pool, __ATOMIC_ACQUIRE);
+        const unsigned long long lost = __atomic_load_n(&r->lost, __ATOMIC_ACQUIRE);
+        const uint32_t stalls = __atomic_load_n(&r->stalls, __ATOMIC_ACQUIRE);
+        const LoopSlot q = L.slot[si];
+        SrvTable &T = s->t[q.table];
+        if (q.op == DYC_OP_INSERT) {
+            T.count += acc;
+            T.inflight_ins -= q.n;
+        } else if (q.op == DYC_OP_ERASE) {
+            T.count -= std::min<uint64_t>(T.count, acc);
+            if (!T.check_down) {
+                T.check_down = true;
+                s->down_list.push_back(q.table);
+            }
+        }
+        T.inflight--;
+        int status = s->error;
+        if (lost) status = sticky(s, fail(DYC_ERR_LOST, "overflow pool full: %llu entries lost", lost));
+        if (stalls)
+            status = sticky(s, fail(DYC_ERR_HIP, "a loop item waited past spin_cap for its table's earlier requests"));
+        if (std::min<uint64_t>(pool, s->scfg.pool_cap) > s->pool_settled) L.want_resolve = true;
+        L.order[q.pseq - L.order_base].second = true;
+        while (!L.order.empty() && L.order.front().second) {
+            L.order.pop_front();
+            L.order_base++;
+        }
+        L.free_slots.push_back(si);
+        finish(s, q.ticket, status);
+        ++*got;
+    }
+}
+
+// Keys per work item of a request: about one item per resident wave for a lone request (the
+// tick rule), and at most 65,536 items per request (the ring holds 2^18).
+uint64_t loop_item_keys(const dyc_server *s, uint64_t n)
+{
+    uint64_t per = s->scfg.item_keys;
+    if (per == 0) {
+        per = (n / kTargetItems + kMinItemKeys - 1) / kMinItemKeys * kMinItemKeys;
+        per = std::min<uint64_t>(std::max<uint64_t>(per, kMinItemKeys), kMaxItemKeys);
+    }
+    const uint64_t floor_per = ((n + 65535) / 65536 + kMinItemKeys - 1) / kMinItemKeys * kMinItemKeys;
+    return std::min<uint64_t>(std::max(per, floor_per), 0xffffffc0ull);
+}
+
+void loop_clear_scratch(dyc_server *s)
+{
+    for (uint32_t x : s->touched) {
+        s->blocked[x] = 0;
+        s->seen[x] = 0;
+    }
+    s->touched.clear();
+}
+
+// Publish queued requests in arrival order. *stuck: nothing could be published for lack of
+// arena space while nothing is in flight.
+int loop_publish(dyc_server *s, bool *stuck)
+{
+    Loop &L = s->loop;
+    *stuck = false;
+    loop_clear_scratch(s);
+    bool memory_blocked = false;
+    uint64_t published = 0;
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        const uint64_t tk = *it;
+        Req &R = s->reqs[tk];
+        const uint32_t x = R.r.table, op = R.r.op;
+        if (s->blocked[x]) {
+            ++it;
+            continue;
+        }
+        if (L.free_slots.empty()) {
+            s->ss.deferred_slots++;
+            break;
+        }
+        const uint64_t per = loop_item_keys(s, R.r.n);
+        const uint64_t n_items = std::max<uint64_t>(1, (R.r.n + per - 1) / per);
+        if (L.tail + n_items - L.oldest_item() > kLoopRingItems) {
+            s->ss.deferred_slots++;
+            break;
+        }
+        SrvTable &T = s->t[x];
+        if (op == DYC_OP_INSERT) {
+            bool room = true, busy = false;
+            while ((double)(T.count + T.inflight_ins + R.r.n) > s->cfg.upper_bound * (double)T.capacity()) {
+                if (T.inflight) {  // a table resizes only with no request in flight
+                    busy = true;
+                    break;
+                }
+                bool ok = false;
+                int rc = upsize_now(s, x, &ok);
+                if (rc) return sticky(s, rc);
+                if (!ok) {
+                    room = false;
+                    break;
+                }
+            }
+            if (s->dir_dirty) {
+                int rc = loop_upload_dir(s);
+                if (rc) return sticky(s, rc);
+            }
+            if (busy || !room) {
+                mark(s, x);
+                s->blocked[x] = busy ? 1 : 2;
+                if (busy) {
+                    s->ss.deferred_order++;
+                } else {
+                    s->mem_head[x] = tk;
+                    s->ss.deferred_memory++;
+                    memory_blocked = true;
+                }
+                ++it;
+                continue;
+            }
+        }
+        // a table's requests run in stages: a run of inserts, a run of finds, or one erase
+        if (L.last_op[x] != (int8_t)op || op == DYC_OP_ERASE) {
+            L.need[x] = L.pub[x];
+            L.last_op[x] = (int8_t)op;
+        }
+        L.pub[x]++;
+        const uint32_t si = L.free_slots.back();
+        L.free_slots.pop_back();
+        LoopSlot &q = L.slot[si];
+        q.ticket = tk;
+        q.pseq = L.pseq++;
+        q.n = R.r.n;
+        q.table = x;
+        q.op = op;
+        L.order.emplace_back(L.tail, false);
+        for (uint64_t j = 0; j < n_items; ++j) {
+            const uint64_t o = j * per;
+            LoopItem &d = L.ring[(L.tail + j) & (kLoopRingItems - 1)];
+            d.item.keys = R.r.keys + o;
+            d.item.vals = op == DYC_OP_INSERT ? R.r.vals + o : nullptr;
+            d.item.out_vals = (op == DYC_OP_FIND && R.r.out_vals) ? R.r.out_vals + o : nullptr;
+            d.item.flags = (op != DYC_OP_INSERT && R.r.out_flags) ? R.r.out_flags + o : nullptr;
+            d.item.len = (uint32_t)std::min<uint64_t>(per, R.r.n - std::min(o, R.r.n));
+            d.item.table = x;
+            d.op = op;
+            d.req = si;
+            d.need = L.need[x];
+            d.n_items = (uint32_t)n_items;
+            __atomic_store_n(&d.stamp, (uint32_t)(L.tail + j + 1), __ATOMIC_RELEASE);  // last
+        }
+        L.tail += n_items;
+        T.inflight++;
+        if (op == DYC_OP_INSERT) T.inflight_ins += R.r.n;
+        R.state = 1;
+        s->ss.loop_items += n_items;
+        s->ss.rounds++;
+        it = s->queue.erase(it);
+        ++published;
+    }
+    if (published) __atomic_store_n(&L.ctl->tail, L.tail, __ATOMIC_RELEASE);
+    *stuck = !published && memory_blocked && L.in_flight() == 0;
+    return DYC_OK;
+}
+
+// Wait until every published request has finished, then stop the launch. On time-out the host
+// gives up: the launch is told to cut its waits short (its requests fail) and to end.
+int loop_quiesce(dyc_server *s)
+{
+    Loop &L = s->loop;
+    const Clock::time_point t0 = Clock::now();
+    while (L.in_flight()) {
+        uint32_t got = 0;
+        int rc = loop_collect(s, &got);
+        if (rc) return rc;
+        if (got) continue;
+        rc = loop_check_ended(s);
+        if (rc) return rc;
+        if (!L.running && L.in_flight()) {  // it ended on idle before it saw the last items
+            rc = loop_launch(s);
+            if (rc) return sticky(s, rc);
+        }
+        if (seconds_since(t0) > kLoopGiveUpS) {
+            __atomic_store_n(&L.ctl->stop, 2ull, __ATOMIC_RELEASE);
+            return sticky(s, fail(DYC_ERR_HIP, "the loop did not finish its requests in %.0f s", kLoopGiveUpS));
+        }
+    }
+    if (L.running) {
+        __atomic_store_n(&L.ctl->stop, 1ull, __ATOMIC_RELEASE);
+        while (true) {
+            const hipError_t e = hipEventQuery(L.ended);
+            if (e == hipSuccess) break;
+            if (e != hipErrorNotReady) return sticky(s, fail(DYC_ERR_HIP, "loop launch: %s", hipGetErrorString(e)));
+            if (seconds_since(t0) > kLoopGiveUpS) {
+                __atomic_store_n(&L.ctl->stop, 2ull, __ATOMIC_RELEASE);
+                return sticky(s, fail(DYC_ERR_HIP, "the loop did not stop in %.0f s", kLoopGiveUpS));
+            }
+        }
+        int rc = loop_check_ended(s);
+        if (rc) return rc;
+    }
+    return DYC_OK;
+}
+
+// Re-place the overflow pool with the loop stopped (every table quiet), as a retiring tick does.
+int loop_resolve(dyc_server *s)
+{
+    int rc = loop_quiesce(s);
+    if (rc) return rc;
+    s->ss.loop_pauses++;
+    unsigned long long *snap = s->ticks[0].h_snap;
+    rc = read_stats_sync(s, snap);
+    if (rc) return sticky(s, rc);
+    refresh_counts(s, snap);
+    const Counters &c = *reinterpret_cast<const Counters *>(snap);
+    if (c.lost) return sticky(s, fail(DYC_ERR_LOST, "overflow pool full: %llu entries lost", (unsigned long long)c.lost));
+    if (std::min<uint64_t>(c.pending, s->scfg.pool_cap) > s->pool_settled) {
+        rc = resolve_pool(s, snap);
+        if (rc) return sticky(s, rc);
+        refresh_counts(s, snap);
+    }
+    s->loop.want_resolve = false;
+    if (s->dir_dirty) return loop_upload_dir(s);
+    return DYC_OK;
+}
+
+// Downsize the tables whose erases finished and that have no request in flight.
+int loop_downsizes(dyc_server *s)
+{
+    if (s->down_list.empty()) return DYC_OK;
+    std::vector<uint32_t> keep;
+    bool resized = false;
+    for (uint32_t x : s->down_list) {
+        SrvTable &T = s->t[x];
+        if (T.inflight) {
+            keep.push_back(x);
+            continue;
+        }
+        T.check_down = false;
+        if (!wants_down(s, T)) continue;
+        bool did = false;
+        int rc = downsize_now(s, x, s->ticks[0].d_rctr, &did);
+        if (rc) return sticky(s, rc);
+        resized |= did;
+    }
+    s->down_list.swap(keep);
+    if (!resized) return DYC_OK;
+    int rc = loop_upload_dir(s);
+    if (rc) return sticky(s, rc);
+    // residuals that found no place went to the pool
+    unsigned long long *snap = s->ticks[0].h_snap;
+    rc = read_stats_sync(s, snap);
+    if (rc) return sticky(s, rc);
+    const Counters &c = *reinterpret_cast<const Counters *>(snap);
+    if (c.misplaced > s->misplaced_seen) {
+        s->misplaced_seen = c.misplaced;
+        return sticky(s, fail(DYC_ERR_CORRUPT, "a resize found entries outside their bucket pair"));
+    }
+    if (std::min<uint64_t>(c.pending, s->scfg.pool_cap) > s->pool_settled) s->loop.want_resolve = true;
+    return DYC_OK;
+}
+
+// One non-blocking step of the loop mode: finish what completed, resize and re-place what is
+// due, publish what is queued, and make sure a launch is there to take it.
+int loop_step(dyc_server *s)
+{
+    Loop &L = s->loop;
+    uint32_t got = 0;
+    int rc = loop_collect(s, &got);
+    if (rc) return rc;
+    rc = loop_check_ended(s);
+    if (rc) return rc;
+    if (L.want_resolve && s->error == DYC_OK) {
+        rc = loop_resolve(s);
+        if (rc) return rc;
+    }
+    rc = loop_downsizes(s);
+    if (rc) return rc;
+    while (!s->queue.empty()) {
+        bool stuck = false;
+        rc = loop_publish(s, &stuck);
+        if (rc) return rc;
+        if (!stuck) break;
+        // nothing in flight can free arena space: fail each insert that waits for it
+        for (auto it = s->queue.begin(); it != s->queue.end();) {
+            const uint32_t x = s->reqs[*it].r.table;
+            if (s->blocked[x] == 2 && s->mem_head[x] == *it) {
+                fail(DYC_ERR_OOM, "table %u cannot grow within the server's HBM budget", x);
+                finish(s, *it, DYC_ERR_OOM);
+                it = s->queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    if (!L.running && L.in_flight()) {
+        rc = loop_launch(s);
+        if (rc) return sticky(s, rc);
+    }
+    return DYC_OK;
+}
+
+// Switch between host-built ticks and the loop; both directions start from an idle server.
+int set_loop(dyc_server *s, bool on)
+{
+    if (s->loop.on == on) return DYC_OK;
+    if (on) {
+        int rc = drain(s);  // ticks
+        if (rc) return rc;
+        rc = loop_alloc(s);
+        if (rc) return rc;
+        s->loop.on = true;
+        std::fill(s->loop.last_op.begin(), s->loop.last_op.end(), (int8_t)-1);
+        return loop_upload_dir(s);
+    }
+    int rc = loop_quiesce(s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->loop.st2));
+    s->loop.on = false;
+    s->dir_dirty = true;  // ticks upload the directory before their first launch
+    return DYC_OK;
+}
+
+// Block until every submitted request has finished (the launch keeps running).
+int loop_drain(dyc_server *s)
+{
+    Clock::time_point last = Clock::now();
+    uint64_t seen = s->ss.completed;
+    while (!s->queue.empty() || s->loop.in_flight()) {
+        int rc = loop_step(s);
+        if (rc) return rc;
+        if (s->ss.completed != seen) {
+            seen = s->ss.completed;
+            last = Clock::now();
+        } else if (seconds_since(last) > kLoopGiveUpS) {
+            __atomic_store_n(&s->loop.ctl->stop, 2ull, __ATOMIC_RELEASE);
+            return sticky(s, fail(DYC_ERR_HIP, "no request finished in %.0f s", kLoopGiveUpS));
+        }
+    }
+    return DYC_OK;
+}
+
+int progress(dyc_server *s) { return s->loop.on ? loop_step(s) : step(s); }
+
+int drain_any(dyc_server *s) { return s->loop.on ? loop_drain(s) : drain(s); }
+
 void destroy(dyc_server *s)
 {
+    if (s->loop.on) {
+        (void)loop_drain(s);
+        (void)loop_quiesce(s);
+    }
     if (s->st) (void)hipStreamSynchronize(s->st);
+    loop_free(s);
     for (Tick &k : s->ticks) {
         if (k.h_items) (void)hipHostFree(k.h_items);
         if (k.d_items) (void)hipFree(k.d_items);
@@ -992,6 +1521,9 @@ int dyc_server_config_init(dyc_server_config *scfg)
     scfg->one_launch = 1;
     scfg->spin_cap = 1u << 24;
     scfg->poll_backoff = 16;
+    scfg->resident = 0;
+    scfg->resident_blocks = 0;
+    scfg->resident_idle_ms = 100;
     return DYC_OK;
 }
 
@@ -1021,11 +1553,15 @@ int dyc_server_create(const dyc_config *cfg, const dyc_server_config *scfg, uint
         return fail(DYC_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 256]");
     if (!(c.upper_bound > 0.0 && c.upper_bound <= 1.0) || !(c.lower_bound >= 0.0) || c.lower_bound >= c.upper_bound)
         return fail(DYC_ERR_INVALID, "need 0 <= lower_bound < upper_bound <= 1");
+    if (sc.resident > 1) return fail(DYC_ERR_INVALID, "resident must be 0 or 1");
+    if (sc.resident && !serve_tick_supported((int)c.insert_lanes, (int)c.find_lanes))
+        return fail(DYC_ERR_INVALID, "the resident loop needs insert_lanes 1, 2 or 4");
     dyc_server *s = new dyc_server();
     s->cfg = c;
     s->scfg = sc;
     HIP_TRY(hipGetDevice(&s->device));
     int rc = create(s, n_tables, capacity_hints);
+    if (!rc && sc.resident) rc = set_loop(s, true);
     if (rc) {
         destroy(s);
         return rc;
@@ -1061,7 +1597,7 @@ int dyc_server_poll(dyc_server *s, uint64_t *done_tickets, int *done_status, uin
 {
     if (!s) return fail(DYC_ERR_INVALID, "null server");
     DeviceGuard g(s->device);
-    int rc = step(s);
+    int rc = progress(s);
     uint32_t k = 0;
     while (k < cap && !s->done.empty()) {
         if (done_tickets) done_tickets[k] = s->done.front().first;
@@ -1077,7 +1613,7 @@ int dyc_server_drain(dyc_server *s)
 {
     if (!s) return fail(DYC_ERR_INVALID, "null server");
     DeviceGuard g(s->device);
-    int rc = drain(s);
+    int rc = drain_any(s);
     if (rc) return rc;
     return s->error;
 }
@@ -1086,14 +1622,18 @@ int dyc_server_replay(dyc_server *s, const dyc_request *reqs, uint64_t n, const 
                       double *done_s, int *status)
 {
     if (!s || (n && (!reqs || !arrival_s || !done_s))) return fail(DYC_ERR_INVALID, "null argument");
-    if (mode > 1) return fail(DYC_ERR_INVALID, "mode must be 0 (continuous) or 1 (synchronous)");
+    if (mode > 2) return fail(DYC_ERR_INVALID, "mode must be 0 (ticks), 1 (synchronous) or 2 (resident loop)");
+    if (mode == 2 && !serve_tick_supported((int)s->cfg.insert_lanes, (int)s->cfg.find_lanes))
+        return fail(DYC_ERR_INVALID, "the resident loop needs insert_lanes 1, 2 or 4");
     for (uint64_t i = 0; i < n; ++i) {
         int rc = check_request(s, reqs[i]);
         if (rc) return rc;
         if (i && arrival_s[i] < arrival_s[i - 1]) return fail(DYC_ERR_INVALID, "arrival times must not decrease");
     }
     DeviceGuard g(s->device);
-    int rc = drain(s);  // start from a quiet server
+    int rc = drain_any(s);  // start from a quiet server
+    if (rc) return rc;
+    rc = set_loop(s, mode == 2);
     if (rc) return rc;
     s->done.clear();
     const uint64_t first = s->reqs.size();
@@ -1112,11 +1652,16 @@ int dyc_server_replay(dyc_server *s, const dyc_request *reqs, uint64_t n, const 
             if (status) status[i] = s->reqs[first + i].status;
         }
         s->done.clear();
-        return DYC_OK;
+        return set_loop(s, s->scfg.resident != 0);
     }
     uint64_t next = 0, finished = 0;
+    double last_done = 0.0;  // nothing finished or arrived for kLoopGiveUpS with work outstanding: give up
     while (finished < n) {
         double now = seconds_since(t0);
+        if (next > finished && now - std::max(last_done, arrival_s[next - 1]) > kLoopGiveUpS) {
+            if (s->loop.on) __atomic_store_n(&s->loop.ctl->stop, 2ull, __ATOMIC_RELEASE);
+            return sticky(s, fail(DYC_ERR_HIP, "replay: no request finished in %.0f s", kLoopGiveUpS));
+        }
         while (next < n && arrival_s[next] <= now) {
             Req R;
             R.r = reqs[next];
@@ -1125,7 +1670,7 @@ int dyc_server_replay(dyc_server *s, const dyc_request *reqs, uint64_t n, const 
             s->ss.submitted++;
             ++next;
         }
-        rc = step(s);
+        rc = progress(s);
         if (rc) return sticky(s, rc);
         if (!s->done.empty()) {
             now = seconds_since(t0);
@@ -1135,12 +1680,13 @@ int dyc_server_replay(dyc_server *s, const dyc_request *reqs, uint64_t n, const 
                     done_s[tk - first] = now;
                     if (status) status[tk - first] = s->done.front().second;
                     ++finished;
+                    last_done = now;
                 }
                 s->done.pop_front();
             }
         }
     }
-    return DYC_OK;
+    return set_loop(s, s->scfg.resident != 0);
 }
 
 int dyc_server_get_stats(const dyc_server *s, dyc_server_stats *out)
@@ -1152,7 +1698,7 @@ int dyc_server_get_stats(const dyc_server *s, dyc_server_stats *out)
     out->arena_used = s->arena.used();
     out->arena_peak = s->arena.peak();
     out->queued = s->queue.size();
-    uint64_t inflight = 0;
+    uint64_t inflight = s->loop.on ? s->loop.in_flight() : 0;
     for (uint32_t i = 0; i < s->depth(); ++i)
         if (s->ticks[i].busy) inflight += s->ticks[i].reqs.size();
     out->in_flight = inflight;
@@ -1179,7 +1725,7 @@ int dyc_server_dump(dyc_server *s, uint32_t table, uint32_t sub, uint64_t *host_
     if (!s || !host_slots) return fail(DYC_ERR_INVALID, "null argument");
     if (table >= s->n || sub >= s->t[table].D.d) return fail(DYC_ERR_INVALID, "table or subtable out of range");
     DeviceGuard g(s->device);
-    int rc = drain(s);
+    int rc = drain_any(s);
     if (rc) return rc;
     const SrvTable &T = s->t[table];
     const uint64_t slots = (uint64_t)T.D.nb[sub] * kSlots;

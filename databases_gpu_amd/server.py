@@ -4,6 +4,8 @@ A server holds many resizable DyCuckoo tables in one HBM arena and serves queued
 the reference's batch calls (hash_insert / hash_search / hash_delete, GPU/dyhash/api.cpp:185-300)
 of one table each -- by continuous batching: every tick fuses the admitted requests into one
 insert, one find and one erase launch, and the next tick is enqueued while the current one runs.
+With resident=1 one launch stays on the GPU instead and takes requests from a ring in host
+memory as they are submitted (the device-resident loop).
 Inputs and outputs are device tensors (int32 bits of uint32 keys/values, uint8 flags) that must
 stay alive until their request completes.
 """
@@ -29,7 +31,8 @@ def _ptr(t):
 class DyCuckooServer:
     def __init__(self, capacity_hints, *, hbm_budget: int = 0, tick_max_keys: int = 1 << 23,
                  tick_max_requests: int = 4096, pipeline_depth: int = 2, item_keys: int = 0, pool_cap: int = 65536,
-                 one_launch: int = 1, spin_cap: int = 1 << 24, poll_backoff: int = 16, **table_config):
+                 one_launch: int = 1, spin_cap: int = 1 << 24, poll_backoff: int = 16, resident: int = 0,
+                 resident_blocks: int = 0, resident_idle_ms: int = 100, **table_config):
         self.lib = _lib.load()
         hints = np.ascontiguousarray(np.asarray(capacity_hints, dtype=np.uint64))
         cfg = _lib.default_config(**table_config)
@@ -38,6 +41,7 @@ class DyCuckooServer:
         sc.hbm_budget, sc.tick_max_keys, sc.tick_max_requests = hbm_budget, tick_max_keys, tick_max_requests
         sc.pipeline_depth, sc.item_keys, sc.pool_cap = pipeline_depth, item_keys, pool_cap
         sc.one_launch, sc.spin_cap, sc.poll_backoff = one_launch, spin_cap, poll_backoff
+        sc.resident, sc.resident_blocks, sc.resident_idle_ms = resident, resident_blocks, resident_idle_ms
         h = ctypes.c_void_p()
         check(self.lib.dyc_server_create(ctypes.byref(cfg), ctypes.byref(sc), hints.size, hints.ctypes.data,
                                          ctypes.byref(h)), "dyc_server_create")
@@ -94,8 +98,9 @@ class DyCuckooServer:
 
     def replay(self, requests: np.ndarray, arrivals, mode: int = 0):
         """Replay a request trace in real time (requests: structured array of REQUEST_DTYPE whose
-        pointers the caller keeps alive). mode 0 continuous batching, 1 synchronous per-request
-        launches. Returns (done_s, status) arrays."""
+        pointers the caller keeps alive). mode 0 continuous batching with host-built ticks, 1
+        synchronous per-request launches, 2 continuous batching through the device-resident loop.
+        Returns (done_s, status) arrays."""
         requests = np.ascontiguousarray(requests)
         assert requests.dtype == np.dtype(_lib.REQUEST_DTYPE)
         arr = np.ascontiguousarray(np.asarray(arrivals, dtype=np.float64))

@@ -377,6 +377,15 @@ typedef struct dyc_server_config {
                                     (default 2^24) */
     uint32_t poll_backoff;       /* a waiting wave sleeps up to this many x 256 cycles between polls
                                     (default 16) */
+    uint32_t resident;           /* 1: submit / poll / drain go through the device-resident loop: one
+                                    launch stays on the GPU and takes requests from a ring in host
+                                    memory as they are submitted (no launch per tick). Needs
+                                    insert_lanes 1, 2 or 4. Default 0: host-built ticks */
+    uint32_t resident_blocks;    /* blocks of 256 threads of the loop launch (0: half of what the GPU
+                                    keeps resident at once, leaving room for resizes beside it; never
+                                    more than that capacity) */
+    uint32_t resident_idle_ms;   /* the loop launch ends after this long without new requests; the
+                                    next submission starts it again (default 100) */
     uint32_t pad;
 } dyc_server_config;
 
@@ -393,6 +402,9 @@ typedef struct dyc_server_stats {
     uint64_t pool_rounds;        /* re-placement rounds run */
     uint64_t arena_bytes, arena_used, arena_peak;
     uint64_t queued, in_flight;  /* requests waiting for admission / admitted, not completed */
+    uint64_t loop_launches;      /* launches of the device-resident loop */
+    uint64_t loop_items;         /* work items published to it */
+    uint64_t loop_pauses;        /* times it was stopped to re-place the overflow pool */
 } dyc_server_stats;
 
 int dyc_server_config_init(dyc_server_config *scfg);
@@ -409,7 +421,8 @@ int dyc_server_poll(dyc_server *s, uint64_t *done_tickets, int *done_status, uin
 /* Block until every submitted request has completed. */
 int dyc_server_drain(dyc_server *s);
 /* Replay a request trace in real time: request i is submitted at arrival_s[i] seconds after the
- * call starts (non-decreasing). mode 0: continuous batching (submit + poll). mode 1: synchronous
+ * call starts (non-decreasing). mode 0: continuous batching with host-built ticks (submit + poll).
+ * mode 2: continuous batching through the device-resident loop (see resident). mode 1: synchronous
  * per-request launches -- each request alone, in arrival order, with the plain single-table
  * kernels and a stream synchronisation, the reference's calling pattern. done_s[i] receives
  * the completion time (seconds since the call started) and status[i] the request status. */

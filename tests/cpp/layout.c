@@ -14,10 +14,12 @@ int main(void)
            offsetof(dyc_stats, subtables));
     printf("request %zu %zu %zu %zu %zu\n", sizeof(dyc_request), offsetof(dyc_request, keys),
            offsetof(dyc_request, vals), offsetof(dyc_request, out_flags), offsetof(dyc_request, n));
-    printf("server_config %zu %zu %zu %zu %zu\n", sizeof(dyc_server_config),
+    printf("server_config %zu %zu %zu %zu %zu %zu %zu\n", sizeof(dyc_server_config),
            offsetof(dyc_server_config, tick_max_requests), offsetof(dyc_server_config, item_keys),
-           offsetof(dyc_server_config, pool_cap), offsetof(dyc_server_config, poll_backoff));
-    printf("server_stats %zu %zu %zu\n", sizeof(dyc_server_stats), offsetof(dyc_server_stats, deferred_memory),
-           offsetof(dyc_server_stats, in_flight));
+           offsetof(dyc_server_config, pool_cap), offsetof(dyc_server_config, poll_backoff),
+           offsetof(dyc_server_config, resident), offsetof(dyc_server_config, resident_idle_ms));
+    printf("server_stats %zu %zu %zu %zu\n", sizeof(dyc_server_stats), offsetof(dyc_server_stats, deferred_memory),
+           offsetof(dyc_server_stats, in_flight), offsetof(dyc_server_stats, loop_pauses));
+    printf("stats_tail %zu\n", offsetof(dyc_stats, last_max_chain));
     return 0;
 }

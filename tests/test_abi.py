@@ -108,9 +108,12 @@ def test_struct_layout_matches_c(lib, tmp_path):
     assert np.dtype(L.REQUEST_DTYPE).itemsize == r[0]
     sc = [int(x) for x in out[3].split()[1:]]
     assert sc == [ctypes.sizeof(L.ServerConfig), L.ServerConfig.tick_max_requests.offset,
-                  L.ServerConfig.item_keys.offset, L.ServerConfig.pool_cap.offset, L.ServerConfig.poll_backoff.offset]
+                  L.ServerConfig.item_keys.offset, L.ServerConfig.pool_cap.offset, L.ServerConfig.poll_backoff.offset,
+                  L.ServerConfig.resident.offset, L.ServerConfig.resident_idle_ms.offset]
     ss = [int(x) for x in out[4].split()[1:]]
-    assert ss == [ctypes.sizeof(L.ServerStats), L.ServerStats.deferred_memory.offset, L.ServerStats.in_flight.offset]
+    assert ss == [ctypes.sizeof(L.ServerStats), L.ServerStats.deferred_memory.offset, L.ServerStats.in_flight.offset,
+                  L.ServerStats.loop_pauses.offset]
+    assert [int(x) for x in out[5].split()[1:]] == [L.Stats.last_max_chain.offset]
 
 
 def _hipcc():
@@ -139,6 +142,15 @@ def test_server_argument_errors(lib):
     assert lib.dyc_server_config_init(ctypes.byref(sc)) == 0
     assert sc.pipeline_depth == 2 and sc.tick_max_keys == 1 << 23 and sc.pool_cap == 65536
     assert sc.one_launch == 1 and sc.spin_cap == 1 << 24
+    assert sc.resident == 0 and sc.resident_blocks == 0 and sc.resident_idle_ms == 100
+    sc.resident = 2
+    assert lib.dyc_server_create(None, ctypes.byref(sc), 1, hints, ctypes.byref(h)) == 1
+    assert b"resident" in lib.dyc_last_error()
+    sc.resident = 1
+    cfg = L.default_config(insert_lanes=8)
+    assert lib.dyc_server_create(ctypes.byref(cfg), ctypes.byref(sc), 1, hints, ctypes.byref(h)) == 1
+    assert b"insert_lanes" in lib.dyc_last_error()
+    sc.resident = 0
     sc.pipeline_depth = 3
     assert lib.dyc_server_create(None, ctypes.byref(sc), 1, hints, ctypes.byref(h)) == 1
     assert b"pipeline_depth" in lib.dyc_last_error()
