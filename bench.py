@@ -85,6 +85,12 @@ def parse():
     ap.add_argument("--sv-requests", type=int, default=4096, help="serving: requests per run")
     ap.add_argument("--sv-loads", default="0.25,0.5,1,2,4,8,16,0",
                     help="serving: offered loads in G keys/s (0 = every request arrives at once)")
+    ap.add_argument("--sv-modes", default="0,2,1",
+                    help="serving: replay modes per load (0 host-built ticks, 2 device-resident loop, 1 synchronous)")
+    ap.add_argument("--sv-resident-blocks", type=int, default=0,
+                    help="serving: blocks of the resident loop launch (0 = half its resident capacity)")
+    ap.add_argument("--sv-item-keys", type=int, default=0,
+                    help="serving: keys per work item (0 = chosen per tick, or per request in the resident loop)")
     ap.add_argument("--sv-depth", type=int, default=2, help="serving: ticks in flight")
     ap.add_argument("--sv-tick-keys", type=int, default=1 << 23, help="serving: keys per tick")
     ap.add_argument("--sv-kick", type=int, default=0,
@@ -698,8 +704,9 @@ def run_serving(args, dev) -> int:
     (log-uniform), requests of log-uniform size in [2^6, 2^18] keys arriving Poisson at a fixed
     offered load, 30% insert / 60% find (50% hit) / 10% delete batches, table chosen in
     proportion to its size, fill bounds [0.50, 0.90] with in-line resizes. For every offered load
-    the same trace is replayed twice on fresh tables -- continuous batching (mode 0) and
-    synchronous per-request launches (mode 1) -- and every output of both is compared with the
+    the same trace is replayed on fresh tables once per mode -- continuous batching with
+    host-built ticks (mode 0), continuous batching through the device-resident loop (mode 2) and
+    synchronous per-request launches (mode 1) -- and every output of each is compared with the
     trace's exact expected results. Prints one JSON line."""
     import torch
 
@@ -707,6 +714,8 @@ def run_serving(args, dev) -> int:
     from databases_gpu_amd import workload as W
 
     loads = [float(x) for x in args.sv_loads.split(",") if x.strip()]
+    modes = [int(x) for x in args.sv_modes.split(",") if x.strip()]
+    mode_name = {0: "continuous", 1: "synchronous", 2: "resident"}
     log(f"serving: generating a {args.sv_requests}-request trace")
     tr = W.ServingTrace(args.sv_requests, seed=args.seed, device=dev, offered=None)
     ops = tr.ops
@@ -724,7 +733,8 @@ def run_serving(args, dev) -> int:
         srv = D.DyCuckooServer(tr.hints, hbm_budget=budget, pipeline_depth=args.sv_depth,
                                tick_max_keys=args.sv_tick_keys, insert_lanes=args.insert_lanes,
                                find_lanes=args.find_lanes, kick_bound=args.sv_kick, poll_backoff=args.sv_backoff,
-                               one_launch=args.sv_one_launch)
+                               one_launch=args.sv_one_launch, resident_blocks=args.sv_resident_blocks,
+                               item_keys=args.sv_item_keys)
         for t in range(tr.sizes.size):
             srv.insert(t, init[t], init_v[t])
         srv.drain()
@@ -735,8 +745,8 @@ def run_serving(args, dev) -> int:
                           tenant_lo=256, tenant_hi=1024, batch_lo=16, batch_hi=1024)
     wv = torch.zeros(warm.ops, dtype=torch.int32, device=dev)
     wf = torch.zeros(warm.ops, dtype=torch.uint8, device=dev)
-    for mode in (0, 1):
-        ws = D.DyCuckooServer(warm.hints, hbm_budget=64 << 20)
+    for mode in modes:
+        ws = D.DyCuckooServer(warm.hints, hbm_budget=64 << 20, resident_blocks=args.sv_resident_blocks)
         ws.replay(warm.requests(wv, wf), warm.arrivals, mode=mode)
         ws.close()
 
@@ -749,7 +759,7 @@ def run_serving(args, dev) -> int:
             arrivals = np.cumsum(gaps) - gaps[0]
         else:
             arrivals = np.zeros(args.sv_requests)
-        for mode in (0, 1):
+        for mode in modes:
             srv = fresh()
             ov.zero_()
             of.zero_()
@@ -763,12 +773,14 @@ def run_serving(args, dev) -> int:
             lat_ms = (done - arrivals) * 1e3
             span = float(done.max() - arrivals[0])
             st = srv.stats()
-            run = {"mode": "continuous" if mode == 0 else "synchronous",
+            run = {"mode": mode_name[mode],
                    "offered_gkeys_s": load if load > 0 else None,
                    "mops": round(ops / span / 1e6, 1),
                    "p50_ms": round(pct(lat_ms, 50), 3), "p99_ms": round(pct(lat_ms, 99), 3),
                    "max_ms": round(float(lat_ms.max()), 3), "span_s": round(span, 4), "wall_s": round(wall, 4),
                    "ticks": st["ticks"] if mode == 0 else None, "rounds": st["rounds"] if mode == 0 else None,
+                   "loop_launches": st["loop_launches"] if mode == 2 else None,
+                   "loop_pauses": st["loop_pauses"] if mode == 2 else None,
                    "resizes_up": st["resizes_up"],
                    "resizes_down": st["resizes_down"], "deferred_memory": st["deferred_memory"],
                    "arena_peak_gb": round(st["arena_peak"] / 2**30, 3), "parity": "ok" if ok else "mismatch"}
@@ -799,6 +811,8 @@ def run_serving(args, dev) -> int:
         "runs": runs,
         "saturation_speedup": (round(sat["continuous"]["mops"] / sat["synchronous"]["mops"], 2)
                                if "continuous" in sat and "synchronous" in sat else None),
+        "resident_saturation_speedup": (round(sat["resident"]["mops"] / sat["synchronous"]["mops"], 2)
+                                        if "resident" in sat and "synchronous" in sat else None),
         "parity": "ok" if all_ok else "mismatch",
     }
     line = json.dumps(result)

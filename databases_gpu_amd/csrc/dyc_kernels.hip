@@ -84,6 +84,23 @@ __device__ inline uint32_t load_in(const uint32_t *p)
         return *p;
 }
 
+// A result word or flag. COH: a system-scope store, which leaves no dirty line in this XCD's
+// L2, so a launch that keeps running can publish results by waiting for its stores to complete
+// (publish_wait) instead of writing the whole L2 back.
+template <bool COH, typename T>
+__device__ inline void store_out(T *p, T v)
+{
+    if constexpr (COH)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else
+        *p = v;
+}
+
+// Wait until this wave's earlier memory operations have completed; the compiler keeps memory
+// accesses on their side of it. Used after system-scope stores and atomics only, which need no
+// cache write-back to be seen by other agents.
+__device__ inline void publish_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 
 // ---------------------------------------------------------------- find / erase / update
 
@@ -171,7 +188,8 @@ __device__ inline uint32_t write_matches(const Desc &D, uint32_t mm, const uint6
 // MODE 0 find: vals (optional) receives the value. MODE 1 erase: every copy of the key in the
 // first bucket (pos1, then pos2) that holds it is CAS'd to empty. MODE 2 update: those copies
 // are CAS'd to the new value vals[i] (an input). COH: read the buckets and the inputs
-// device-coherently (see load_slots_coherent, load_in).
+// device-coherently and write the results with system-scope stores (load_slots_coherent,
+// load_in, store_out).
 template <int G, int MODE, bool COH = false>
 __device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                         uint8_t *__restrict__ flags, uint64_t base, uint64_t n)
@@ -302,8 +320,8 @@ __device__ inline uint32_t probe_tile_m(const Desc &D, const uint32_t *__restric
         }
 
         if (inb) {
-            if (!ERASE && vals) vals[i] = my_val;
-            if (flags) flags[i] = (uint8_t)my_flag;
+            if (!ERASE && vals) store_out<COH>(vals + i, my_val);
+            if (flags) store_out<COH>(flags + i, (uint8_t)my_flag);
         }
         return my_flag;
     }
@@ -316,7 +334,7 @@ __device__ inline uint32_t probe_tile(const Desc &D, const uint32_t *__restrict_
     return probe_tile_m<G, ERASE ? 1 : 0, COH>(D, keys, vals, flags, base, n);
 }
 
-template <int MODE>
+template <int MODE, bool COH = false>
 __device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const uint32_t *keys, uint32_t *vals,
                                          uint8_t *flags, uint64_t n, uint32_t table, uint64_t base, uint32_t f);
 
@@ -943,7 +961,7 @@ __global__ __launch_bounds__(256) void group_insert_kernel(const Desc *__restric
 // one more bucket after pos2. MODE 0 find: vals receives the first matching entry's value.
 // MODE 1 erase: every copy is CAS'd to 0. MODE 2 update: every copy is CAS'd to the new value
 // vals[i] (an input). Returns this lane's flag, also written to flags when set.
-template <int MODE>
+template <int MODE, bool COH>
 __device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const uint32_t *keys, uint32_t *vals,
                                          uint8_t *flags, uint64_t n, uint32_t table, uint64_t base, uint32_t f)
 {
@@ -962,19 +980,20 @@ __device__ inline uint32_t pool_probe_at(const Pool &pool, uint64_t np, const ui
             uint64_t exp = e;
             got |= (uint32_t)cas_slot(pool.pend + j, exp, MODE == 2 ? pack(k, nv) : 0ull);
         } else {
-            if (vals) vals[i] = val_of(e);
+            if (vals) store_out<COH>(vals + i, val_of(e));
             got = 1;
             break;
         }
     }
-    if (got && flags) flags[i] = 1;
+    if (got && flags) store_out<COH>(flags + i, (uint8_t)1);
     return got;
 }
 
-template <bool ERASE>
+template <bool ERASE, bool COH = false>
 __device__ inline uint32_t pool_probe(const Pool &pool, uint64_t np, const ServeItem &item, uint64_t base, uint32_t f)
 {
-    return pool_probe_at<ERASE ? 1 : 0>(pool, np, item.keys, item.out_vals, item.flags, item.len, item.table, base, f);
+    return pool_probe_at<ERASE ? 1 : 0, COH>(pool, np, item.keys, item.out_vals, item.flags, item.len, item.table, base,
+                                             f);
 }
 
 // One item of finds or erases; returns this lane's count of hits (erase: deletions).
@@ -986,7 +1005,7 @@ __device__ inline unsigned long long serve_probe_item(const Desc &D, const Serve
     for (uint64_t base = 0; base < item.len; base += kWave) {
         uint32_t f = probe_tile<G, ERASE, COH>(D, item.keys, ERASE ? nullptr : item.out_vals, item.flags, base,
                                                item.len);
-        if (np) f |= pool_probe<ERASE>(pool, np, item, base, f);
+        if (np) f |= pool_probe<ERASE, COH>(pool, np, item, base, f);
         nd += f;
     }
     return nd;
@@ -1163,10 +1182,10 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
 // whose earlier requests have not finished waits until tbl_done[table] reaches its need. Items
 // wait only on items with smaller indexes, which running waves have claimed (the grid is
 // resident as a whole), so the earliest unfinished item never waits and every published item
-// finishes. A finished item writes its results back to memory (a system-scope release), then
-// counts itself into its request's progress word; the wave that counts a request's last item
-// advances tbl_done, writes the request's record and appends the request to the completion
-// queue in host memory.
+// finishes. A finished item's results are system-scope stores and its table writes atomics: it
+// waits for them to complete, then counts itself into its request's progress word; the wave
+// that counts a request's last item advances tbl_done, writes the request's record and appends
+// the request to the completion queue in host memory.
 // Every wait is bounded. The front wave ends the launch on the host's stop flag or after
 // idle_ticks without new items: it raises quit, and waves waiting for an index past the
 // published ones leave. An item gives up waiting for its table after spin_cap polls, or at once
@@ -1330,8 +1349,9 @@ __global__ __launch_bounds__(256) void serve_loop_kernel(LoopArgs A, InsertPolic
             }
         }
         if (lane == 0 && stalled) atomicAdd(A.req_stall + it.req, 1u);
-        // results and table writes to memory before the item counts as done
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        // the item's results (system-scope stores) and table writes (atomics) are complete
+        // before it counts as done
+        publish_wait();
         if (lane == 0) {
             const unsigned long long old = atomicAdd(A.req_word + it.req, (acc << kLoopItemBits) | 1ull);
             if ((old & ((1ull << kLoopItemBits) - 1)) + 1 == it.n_items) {
@@ -1346,7 +1366,7 @@ __global__ __launch_bounds__(256) void serve_loop_kernel(LoopArgs A, InsertPolic
                                    __HIP_MEMORY_SCOPE_SYSTEM);
                 __hip_atomic_store(&r->lost, ld_agent(&C->lost), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 __hip_atomic_store(&r->stalls, stalls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+                publish_wait();  // the record is in host memory before the queue entry
                 const unsigned long long pos = atomicAdd(A.words + kLoopCqTail, 1ull);
                 __hip_atomic_store(A.cq + (pos & A.req_mask), ((pos + 1) << 32) | it.req, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);

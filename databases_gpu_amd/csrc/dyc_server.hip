@@ -1081,13 +1081,18 @@ int loop_collect(dyc_server *s, uint32_t *got)
     }
 }
 
-// Keys per work item of a request: about one item per resident wave for a lone request (the
-// tick rule), and at most 65,536 items per request (the ring holds 2^18).
+// Keys per work item of a request: one item per worker wave of the launch (each item costs a
+// read of its descriptor from host memory, so a wave takes one item of the request rather than
+// several small ones), and at most 65,536 items per request (the ring holds 2^18). Sizing items
+// by the keys already in flight instead (larger items under load) cut the saturated rate of the
+// 257-table serving trace from 3.0 to 0.4-0.7 G keys/s: a table's later stages wait for its
+// earlier ones, and a large item keeps its stage open longer.
 uint64_t loop_item_keys(const dyc_server *s, uint64_t n)
 {
     uint64_t per = s->scfg.item_keys;
     if (per == 0) {
-        per = (n / kTargetItems + kMinItemKeys - 1) / kMinItemKeys * kMinItemKeys;
+        const uint64_t waves = (uint64_t)s->loop.blocks * 4 - 1;
+        per = ((n + waves - 1) / waves + kMinItemKeys - 1) / kMinItemKeys * kMinItemKeys;
         per = std::min<uint64_t>(std::max<uint64_t>(per, kMinItemKeys), kMaxItemKeys);
     }
     const uint64_t floor_per = ((n + 65535) / 65536 + kMinItemKeys - 1) / kMinItemKeys * kMinItemKeys;
