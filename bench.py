@@ -864,9 +864,10 @@ def run_batched(args, dev) -> int:
         log(f"autotune: {tuned['installed']} ({tuned.get('trials', 0)} trials, {tuned['seconds']} s)")
 
     def measure(per_table: bool):
-        ins, fnd, tot = [], [], []
+        ins, fnd, tot, wall = [], [], [], []
         for s in range(args.warmup + args.steps):
             e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            t0 = time.perf_counter()
             e[0].record(stream)
             g.clear()
             e[1].record(stream)
@@ -879,16 +880,28 @@ def run_batched(args, dev) -> int:
                 ins.append(e[1].elapsed_time(e[2]))
                 fnd.append(e[2].elapsed_time(e[3]))
                 tot.append(e[1].elapsed_time(e[3]))
-        return ins, fnd, tot
+                wall.append((time.perf_counter() - t0) * 1e3)
+        return ins, fnd, tot, wall
 
-    ins_p, fnd_p, tot_p = measure(False)
+    ins_p, fnd_p, tot_p, wall_p = measure(False)
+    # the same step (clear, insert, find) captured once into a HIP graph and replayed
+    plan = g.plan(keys, vals, offs, lookups, offs, out_v, out_h)
+    wall_g = []
+    for s in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        plan.run()
+        if s >= args.warmup:
+            wall_g.append((time.perf_counter() - t0) * 1e3)
+    ok_graph = bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
+    plan.close()
+    log(f"step wall time: eager calls {np.median(wall_p):.3f} ms, graph replay {np.median(wall_g):.3f} ms")
     ok = bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
     log(f"persistent: insert {np.median(ins_p):.2f} ms, find {np.median(fnd_p):.2f} ms")
-    ins_t, fnd_t, tot_t = measure(True)
+    ins_t, fnd_t, tot_t, _ = measure(True)
     ok &= bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
     log(f"per table:  insert {np.median(ins_t):.2f} ms, find {np.median(fnd_t):.2f} ms")
     counts_ok = all(g.table_stats(i)["count"] == sizes[i] for i in range(0, args.bt_tables, max(1, args.bt_tables // 64)))
-    ok &= counts_ok
+    ok &= counts_ok and ok_graph
     ops = 2 * total
     result = {
         "metric": "Mops/s batched uneven tables (insert all, then find all), persistent launch vs one launch per table",
@@ -909,6 +922,9 @@ def run_batched(args, dev) -> int:
         "persistent": {"insert_ms": round(float(np.median(ins_p)), 3), "find_ms": round(float(np.median(fnd_p)), 3),
                        "p50_ms": round(pct(tot_p, 50), 3), "p99_ms": round(pct(tot_p, 99), 3),
                        "mops": round(ops / (np.median(tot_p) * 1e-3) / 1e6, 1)},
+        "step_wall_ms": {"eager": round(float(np.median(wall_p)), 3), "graph": round(float(np.median(wall_g)), 3),
+                         "note": "clear + insert + find incl. host work and the final wait; graph = one replay of "
+                                 "the captured step (dyc_group_plan_run)"},
         "per_table": {"insert_ms": round(float(np.median(ins_t)), 3), "find_ms": round(float(np.median(fnd_t)), 3),
                       "p50_ms": round(pct(tot_t, 50), 3), "p99_ms": round(pct(tot_t, 99), 3),
                       "mops": round(ops / (np.median(tot_t) * 1e-3) / 1e6, 1)},

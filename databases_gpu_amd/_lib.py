@@ -45,7 +45,7 @@ EXPORTS = (
     "dyc_wide_create", "dyc_wide_destroy", "dyc_wide_insert", "dyc_wide_upsert", "dyc_wide_find", "dyc_wide_erase",
     "dyc_wide_resize_up", "dyc_wide_resize_down", "dyc_wide_get_stats", "dyc_wide_clear", "dyc_wide_dump",
     "dyc_wide_load", "dyc_wide_pending", "dyc_insert_noalloc", "dyc_insert_async", "dyc_sync",
-    "dyc_insert_counted",
+    "dyc_insert_counted", "dyc_group_plan_create", "dyc_group_plan_run", "dyc_group_plan_destroy",
 )
 
 OP_INSERT, OP_FIND, OP_ERASE = 0, 1, 2
@@ -114,6 +114,8 @@ class GroupLaunch(ctypes.Structure):
         ("insert_blocks", ctypes.c_uint32),
         ("item_keys", ctypes.c_uint32),
         ("kick_bound", ctypes.c_uint32),
+        ("insert_waves_per_eu", ctypes.c_uint32),
+        ("find_waves_per_eu", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -192,6 +194,9 @@ def load(require: bool = True):
         "dyc_insert_noalloc": (ci, [vp, vp, vp, u64, vp]),
         "dyc_insert_async": (ci, [vp, vp, vp, u64, vp]),
         "dyc_insert_counted": (ci, [vp, vp, vp, u64, vp, vp]),
+        "dyc_group_plan_create": (ci, [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]),
+        "dyc_group_plan_run": (ci, [vp, vp]),
+        "dyc_group_plan_destroy": (ci, [vp]),
         "dyc_sync": (ci, [vp]),
         "dyc_find": (ci, [vp, vp, vp, vp, u64, vp]),
         "dyc_erase": (ci, [vp, vp, vp, u64, vp]),

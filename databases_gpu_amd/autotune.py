@@ -2,11 +2,14 @@
 
 The kernels take their shape at run time: lanes per key ("bucket width": how many lanes share
 one 128-byte bucket read -- the bucket itself stays 16 slots), grid size (resident waves per
-CU), keys per work item, and the eviction bound (iterations under the reference victim rule
-before the pseudo-random rescue takes over; results are the same, placement and speed differ).
+CU), keys per work item, the eviction bound (iterations under the reference victim rule
+before the pseudo-random rescue takes over; results are the same, placement and speed differ),
+and, for groups, occupancy: the persistent kernels exist in builds for at least 6 or 8 waves
+per SIMD (fewer registers each) besides the compiler's own choice.
 `tune_group` times every candidate on the caller's own batch (or a sample of its tables) with
-HIP events and installs the fastest insert shape and the fastest find shape. Results are cached
-in a JSON file keyed by device name and a workload signature.
+HIP events and installs the fastest insert shape and the fastest find shape (the occupancy
+builds are tried on the best shape of the grid). Results are cached in a JSON file keyed by
+device name and a workload signature.
 """
 from __future__ import annotations
 
@@ -31,6 +34,7 @@ FIND_SPACE = {
     "find_blocks": (1024, 2048, 4096, 8192),
     "item_keys": (2048, 8192),
 }
+WAVES_PER_EU = (6, 8)  # occupancy builds tried on the best grid shape (0 = the compiler's own)
 
 
 def _time(fn, reps: int) -> float:
@@ -88,7 +92,22 @@ def tune_group(group, keys: torch.Tensor, vals: torch.Tensor, offsets, lookups: 
         trials = 0
         ins_best = (float("inf"), None)
         for combo in itertools.product(*INSERT_SPACE.values()):
-            shape = dict(zip(INSERT_SPACE.keys(), combo))
+            shape = dict(zip(INSERT_SPACE.keys(), combo), insert_waves_per_eu=0)
+            group.set_launch(**shape)
+
+            def run():
+                group.clear()
+                group.insert(keys, vals, offsets)
+
+            run()
+            ms = _time(run, reps)
+            trials += 1
+            if log:
+                log(f"autotune insert {shape}: {ms:.3f} ms")
+            if ms < ins_best[0]:
+                ins_best = (ms, shape)
+        for w in WAVES_PER_EU:
+            shape = dict(ins_best[1], insert_waves_per_eu=w)
             group.set_launch(**shape)
 
             def run():
@@ -109,7 +128,16 @@ def tune_group(group, keys: torch.Tensor, vals: torch.Tensor, offsets, lookups: 
         out_h = torch.empty(lookups.numel(), dtype=torch.uint8, device=lookups.device)
         find_best = (float("inf"), None)
         for combo in itertools.product(*FIND_SPACE.values()):
-            shape = dict(zip(FIND_SPACE.keys(), combo))
+            shape = dict(zip(FIND_SPACE.keys(), combo), find_waves_per_eu=0)
+            group.set_launch(**shape)
+            ms = _time(lambda: group.find(lookups, lookup_offsets, out_v, out_h), reps)
+            trials += 1
+            if log:
+                log(f"autotune find {shape}: {ms:.3f} ms")
+            if ms < find_best[0]:
+                find_best = (ms, shape)
+        for w in WAVES_PER_EU:
+            shape = dict(find_best[1], find_waves_per_eu=w)
             group.set_launch(**shape)
             ms = _time(lambda: group.find(lookups, lookup_offsets, out_v, out_h), reps)
             trials += 1

@@ -88,6 +88,9 @@ int check_launch(const dyc_group_launch &l)
     const bool ins_ok = l.insert_lanes == 1 || l.insert_lanes == 2 || l.insert_lanes == 4 || l.insert_lanes == 8 ||
                         l.insert_lanes == 16;
     if (!find_ok || !ins_ok) return fail(DYC_ERR_INVALID, "group lanes: find 4/8/16, insert 1/2/4/8/16");
+    auto wpe_ok = [](uint32_t w) { return w == 0 || w == 1 || w == 6 || w == 8; };
+    if (!wpe_ok(l.insert_waves_per_eu) || !wpe_ok(l.find_waves_per_eu))
+        return fail(DYC_ERR_INVALID, "group waves per EU: 0 (compiler's choice), 6 or 8");
     return DYC_OK;
 }
 
@@ -213,7 +216,7 @@ int dyc_group_create(const dyc_config *cfg_in, uint32_t n_tables, const uint64_t
     cfg.subtables = d;
     dyc_group *g = new dyc_group();
     g->cfg = cfg;
-    g->launch = dyc_group_launch{cfg.find_lanes, cfg.insert_lanes, 0, 0, kDefaultItemKeys, cfg.kick_bound};
+    g->launch = dyc_group_launch{cfg.find_lanes, cfg.insert_lanes, 0, 0, kDefaultItemKeys, cfg.kick_bound, 0, 0};
     int rc = check_launch(g->launch);
     if (rc) {
         delete g;
@@ -335,7 +338,7 @@ int dyc_group_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, c
     HIP_TRY(launch_group_insert(g->ddir, g->ditems, n_items, keys, vals, g->policy(), (int)g->launch.insert_lanes,
                                 (int)g->launch.insert_blocks, g->ctr,
                                 PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, 0, 0, nullptr}, g->tcounts,
-                                s));
+                                s, (int)g->launch.insert_waves_per_eu));
     rc = read_back(g, s);
     if (rc) return rc;
     return finish_insert(g, room);
@@ -355,7 +358,7 @@ int dyc_group_find(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, 
     if (rc) return rc;
     Pool sp;
     HIP_TRY(launch_group_find(g->ddir, g->ditems, n_items, keys, vals, hit, (int)g->launch.find_lanes,
-                              (int)g->launch.find_blocks, s, g->stash(sp)));
+                              (int)g->launch.find_blocks, s, g->stash(sp), (int)g->launch.find_waves_per_eu));
     return DYC_OK;
 }
 
@@ -374,7 +377,7 @@ int dyc_group_erase(dyc_group *g, const uint32_t *keys, const uint64_t *offsets,
     if (rc) return rc;
     Pool sp;
     HIP_TRY(launch_group_erase(g->ddir, g->ditems, n_items, keys, deleted, g->tcounts, (int)g->launch.find_lanes,
-                               (int)g->launch.find_blocks, s, g->stash(sp)));
+                               (int)g->launch.find_blocks, s, g->stash(sp), (int)g->launch.find_waves_per_eu));
     rc = read_back(g, s);
     if (rc) return rc;
     for (uint32_t i = 0; i < g->n; ++i) g->count[i] -= std::min<uint64_t>(g->count[i], g->htcounts[i]);
@@ -472,6 +475,132 @@ int dyc_group_pending(dyc_group *g, uint64_t *host_entries, uint32_t *host_table
         HIP_TRY(hipStreamSynchronize(s));
     }
     return DYC_OK;
+}
+
+// ------------------------------------------------------------------- captured steps
+struct dyc_group_plan {
+    dyc_group *g = nullptr;
+    GroupItem *d_items = nullptr;  // insert items, then find items
+    uint32_t n_ins = 0, n_find = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t room = 0;  // pending entries the insert may append
+    uint64_t *pend = nullptr;  // the pending list the graph writes (a larger eager insert may move it)
+};
+
+int dyc_group_plan_destroy(dyc_group_plan *p)
+{
+    if (!p) return DYC_OK;
+    DeviceGuard dg(p->g->device);
+    (void)hipDeviceSynchronize();
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->d_items) (void)hipFree(p->d_items);
+    delete p;
+    return DYC_OK;
+}
+
+int dyc_group_plan_create(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *ins_offsets,
+                          const uint32_t *lookups, const uint64_t *find_offsets, uint32_t *vals_out, uint8_t *hit,
+                          dyc_group_plan **out)
+{
+    if (!g || !out) return fail(DYC_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int rc = check_offsets(g, ins_offsets);
+    if (!rc) rc = check_offsets(g, find_offsets);
+    if (rc) return rc;
+    if (ins_offsets[g->n] && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    if (find_offsets[g->n] && !lookups) return fail(DYC_ERR_INVALID, "null lookups");
+    DeviceGuard dg(g->device);
+    rc = reserve_pending(g, std::max<uint64_t>(65536, ins_offsets[g->n] / 16), nullptr);
+    if (rc) return rc;
+    // the work items of both launches, cut as the eager calls cut them, uploaded once
+    const uint64_t per = g->launch.item_keys ? g->launch.item_keys : kDefaultItemKeys;
+    std::vector<GroupItem> items;
+    for (const uint64_t *o : {ins_offsets, find_offsets})
+        for (uint32_t i = 0; i < g->n; ++i)
+            for (uint64_t b = o[i]; b < o[i + 1]; b += per)
+                items.push_back(GroupItem{b, (uint32_t)std::min<uint64_t>(per, o[i + 1] - b), i});
+    if (items.size() > 0xffffffffull) return fail(DYC_ERR_INVALID, "batch too large for 32-bit item indexes");
+    dyc_group_plan *p = new dyc_group_plan();
+    p->g = g;
+    for (uint32_t i = 0; i < g->n; ++i) p->n_ins += (uint32_t)((ins_offsets[i + 1] - ins_offsets[i] + per - 1) / per);
+    p->n_find = (uint32_t)(items.size() - p->n_ins);
+    p->room = g->pend_cap;
+    p->pend = g->pend;
+    auto undo = [&](int code) {
+        char keep[512];
+        strncpy(keep, dyc_last_error(), sizeof keep - 1);
+        keep[sizeof keep - 1] = 0;
+        dyc_group_plan_destroy(p);
+        fail(code, "%s", keep);
+        return code;
+    };
+    if (!items.empty()) {
+        if (hipMalloc(&p->d_items, items.size() * sizeof(GroupItem)) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(DYC_ERR_OOM, "plan work items (%zu) failed", items.size());
+            return undo(DYC_ERR_OOM);
+        }
+        if (hipMemcpy(p->d_items, items.data(), items.size() * sizeof(GroupItem), hipMemcpyHostToDevice) != hipSuccess) {
+            fail(DYC_ERR_HIP, "plan work item upload failed");
+            return undo(DYC_ERR_HIP);
+        }
+    }
+    // capture: clear, counters to zero, insert, find (its stash is the pending list, whose size
+    // the launch reads on the device), counters out to pinned memory. Zeroing is done by a fill
+    // kernel, not a memset node.
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        fail(DYC_ERR_HIP, "capture stream");
+        return undo(DYC_ERR_HIP);
+    }
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        const Pool stash{g->pend, g->pend_tag, g->pend_cap, &g->ctr->pending, 0};
+        hipError_t k = launch_zero(reinterpret_cast<unsigned long long *>(g->arena), g->arena_bytes / 8, cs);
+        if (k == hipSuccess) k = launch_zero(reinterpret_cast<unsigned long long *>(g->ctr), kCounterWords, cs);
+        if (k == hipSuccess) k = launch_zero(g->tcounts, g->n, cs);
+        if (k == hipSuccess)
+            k = launch_group_insert(g->ddir, p->d_items, p->n_ins, keys, vals, g->policy(), (int)g->launch.insert_lanes,
+                                    (int)g->launch.insert_blocks, g->ctr,
+                                    PendSink{g->pend, g->pend_tag, g->pend_cap, 0, 0, nullptr}, g->tcounts, cs,
+                                    (int)g->launch.insert_waves_per_eu);
+        if (k == hipSuccess)
+            k = launch_group_find(g->ddir, p->d_items + p->n_ins, p->n_find, lookups, vals_out, hit,
+                                  (int)g->launch.find_lanes, (int)g->launch.find_blocks, cs, &stash,
+                                  (int)g->launch.find_waves_per_eu);
+        if (k == hipSuccess)
+            k = launch_copy(g->ctr, g->hctr, kCounterWords, g->tcounts, g->htcounts, g->n, cs);
+        e = hipStreamEndCapture(cs, &p->graph);
+        if (k != hipSuccess) e = k;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+        fail(DYC_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        return undo(DYC_ERR_HIP);
+    }
+    *out = p;
+    return DYC_OK;
+}
+
+int dyc_group_plan_run(dyc_group_plan *p, dyc_stream stream)
+{
+    if (!p) return fail(DYC_ERR_INVALID, "null plan");
+    dyc_group *g = p->g;
+    if (g->pend != p->pend || g->pend_cap != p->room)
+        return fail(DYC_ERR_INVALID, "the group's pending list moved since the plan was made (a larger insert grew "
+                                     "it): make the plan again");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipGraphLaunch(p->exec, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    fold_tallies(g->hctr);
+    // the step cleared every table: counts and the pending list start again from zero
+    std::fill(g->count.begin(), g->count.end(), 0);
+    g->pend_n = 0;
+    return finish_insert(g, p->room);
 }
 
 int dyc_group_clear(dyc_group *g, dyc_stream stream)

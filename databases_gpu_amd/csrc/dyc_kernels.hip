@@ -890,8 +890,10 @@ __global__ __launch_bounds__(256) void probe_count_kernel(Desc D, const uint32_t
 // wave-uniform), so the per-tile code is the single-table code with a different Desc.
 __device__ inline uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-template <int G, bool ERASE>
-__global__ __launch_bounds__(256) void group_probe_kernel(const Desc *__restrict__ dir,
+// W: the least waves per SIMD the compiler must leave room for (1: its own choice); the group
+// launches take it as an occupancy knob, chosen by measurement.
+template <int G, bool ERASE, int W>
+__global__ __launch_bounds__(256, W) void group_probe_kernel(const Desc *__restrict__ dir,
                                                           const GroupItem *__restrict__ items, uint32_t n_items,
                                                           const uint32_t *__restrict__ keys,
                                                           uint32_t *__restrict__ vals, uint8_t *__restrict__ flags,
@@ -919,8 +921,8 @@ __global__ __launch_bounds__(256) void group_probe_kernel(const Desc *__restrict
 
 // Groups of one wave share an item: they are fed from it until it is used up, and the wave
 // moves on only when all its groups have placed (or pended) their entries.
-template <int G>
-__global__ __launch_bounds__(256) void group_insert_kernel(const Desc *__restrict__ dir,
+template <int G, int W>
+__global__ __launch_bounds__(256, W) void group_insert_kernel(const Desc *__restrict__ dir,
                                                            const GroupItem *__restrict__ items, uint32_t n_items,
                                                            const uint32_t *__restrict__ keys,
                                                            const uint32_t *__restrict__ vals, InsertPolicy P,
@@ -1574,63 +1576,91 @@ hipError_t launch_downsize(const uint64_t *old, uint32_t new_nb, uint64_t *dst, 
 
 constexpr int kGroupDefaultBlocks = 2048;  // 8192 waves: 256 CUs x 32 resident waves
 
+template <int G, bool ERASE>
+static void group_probe_w(int wpe, int grid, hipStream_t st, const Desc *dir, const GroupItem *items, uint32_t n_items,
+                          const uint32_t *keys, uint32_t *vals, uint8_t *flags, unsigned long long *table_counts,
+                          const Pool &p)
+{
+    switch (wpe) {
+    case 6:
+        group_probe_kernel<G, ERASE, 6><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, flags, table_counts, p);
+        break;
+    case 8:
+        group_probe_kernel<G, ERASE, 8><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, flags, table_counts, p);
+        break;
+    default:
+        group_probe_kernel<G, ERASE, 1><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, flags, table_counts, p);
+        break;
+    }
+}
+
 hipError_t launch_group_find(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
-                             uint32_t *vals, uint8_t *hit, int lanes, int blocks, hipStream_t st, const Pool *pool)
+                             uint32_t *vals, uint8_t *hit, int lanes, int blocks, hipStream_t st, const Pool *pool,
+                             int waves_per_eu)
 {
     const Pool p = pool ? *pool : Pool{};
     if (n_items == 0) return hipSuccess;
     const int grid = blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
     switch (lanes) {
-    case 4: group_probe_kernel<4, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr, p); break;
-    case 16: group_probe_kernel<16, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr, p); break;
-    default: group_probe_kernel<8, false><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, hit, nullptr, p); break;
+    case 4: group_probe_w<4, false>(waves_per_eu, grid, st, dir, items, n_items, keys, vals, hit, nullptr, p); break;
+    case 16: group_probe_w<16, false>(waves_per_eu, grid, st, dir, items, n_items, keys, vals, hit, nullptr, p); break;
+    default: group_probe_w<8, false>(waves_per_eu, grid, st, dir, items, n_items, keys, vals, hit, nullptr, p); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_group_erase(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                               uint8_t *deleted, unsigned long long *table_counts, int lanes, int blocks,
-                              hipStream_t st, const Pool *pool)
+                              hipStream_t st, const Pool *pool, int waves_per_eu)
 {
     const Pool p = pool ? *pool : Pool{};
     if (n_items == 0) return hipSuccess;
     const int grid = blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
     switch (lanes) {
     case 4:
-        group_probe_kernel<4, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts, p);
+        group_probe_w<4, true>(waves_per_eu, grid, st, dir, items, n_items, keys, nullptr, deleted, table_counts, p);
         break;
     case 16:
-        group_probe_kernel<16, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts, p);
+        group_probe_w<16, true>(waves_per_eu, grid, st, dir, items, n_items, keys, nullptr, deleted, table_counts, p);
         break;
     default:
-        group_probe_kernel<8, true><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, deleted, table_counts, p);
+        group_probe_w<8, true>(waves_per_eu, grid, st, dir, items, n_items, keys, nullptr, deleted, table_counts, p);
         break;
     }
     return hipGetLastError();
 }
 
+template <int G>
+static void group_insert_w(int wpe, int grid, hipStream_t st, const Desc *dir, const GroupItem *items,
+                           uint32_t n_items, const uint32_t *keys, const uint32_t *vals, InsertPolicy pol, Counters *C,
+                           PendSink sink, unsigned long long *table_counts)
+{
+    switch (wpe) {
+    case 6:
+        group_insert_kernel<G, 6><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
+        break;
+    case 8:
+        group_insert_kernel<G, 8><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
+        break;
+    default:
+        group_insert_kernel<G, 1><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
+        break;
+    }
+}
+
 hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                                const uint32_t *vals, InsertPolicy pol, int lanes, int blocks, Counters *C,
-                               PendSink sink, unsigned long long *table_counts, hipStream_t st)
+                               PendSink sink, unsigned long long *table_counts, hipStream_t st, int waves_per_eu)
 {
     if (n_items == 0) return hipSuccess;
     const int grid = blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
+    const int w = waves_per_eu;
     switch (lanes) {
-    case 2:
-        group_insert_kernel<2><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
-        break;
-    case 4:
-        group_insert_kernel<4><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
-        break;
-    case 8:
-        group_insert_kernel<8><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
-        break;
-    case 16:
-        group_insert_kernel<16><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
-        break;
-    default:
-        group_insert_kernel<1><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
-        break;
+    case 2: group_insert_w<2>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
+    case 4: group_insert_w<4>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
+    case 8: group_insert_w<8>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
+    case 16: group_insert_w<16>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
+    default: group_insert_w<1>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
     }
     return hipGetLastError();
 }

@@ -171,17 +171,20 @@ struct GroupItem {
 // Persistent launches over a directory of table descriptors: each wave takes items w, w + W, ...
 // find: vals/hit may be null. erase: per-table deleted counts are added to table_counts.
 // pool (optional): misses of an item of table i also look in the pool's entries tagged i.
+// waves_per_eu: 6 or 8 builds the kernel for at least that many waves per SIMD (fewer
+// registers); anything else leaves the register budget to the compiler.
 hipError_t launch_group_find(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                              uint32_t *vals, uint8_t *hit, int lanes, int blocks, hipStream_t st,
-                             const Pool *pool = nullptr);
+                             const Pool *pool = nullptr, int waves_per_eu = 0);
 hipError_t launch_group_erase(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                               uint8_t *deleted, unsigned long long *table_counts, int lanes, int blocks,
-                              hipStream_t st, const Pool *pool = nullptr);
+                              hipStream_t st, const Pool *pool = nullptr, int waves_per_eu = 0);
 // insert: failures go to sink.pend with their table index in sink.tags; per-table inserted
 // counts (placed or pended, reserved keys excluded) are added to table_counts.
 hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                                const uint32_t *vals, InsertPolicy pol, int lanes, int blocks, Counters *C,
-                               PendSink sink, unsigned long long *table_counts, hipStream_t st);
+                               PendSink sink, unsigned long long *table_counts, hipStream_t st,
+                               int waves_per_eu = 0);
 
 // ---- serving ticks
 // A work item of a serving tick: keys [0, len) of one request (pointers already offset to the

@@ -10,6 +10,7 @@ drivers (GPU/dyhash/Main_backup_0910.cpp, GPU/warp/Main.cpp) carry over unchange
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 import torch
@@ -580,6 +581,8 @@ class DyCuckooGroup:
         self.h = h
 
     def close(self):
+        for p in list(getattr(self, "_plans", ())):  # a plan must go before its group
+            p.close()
         if getattr(self, "h", None):
             self.lib.dyc_group_destroy(self.h)
             self.h = None
@@ -637,6 +640,31 @@ class DyCuckooGroup:
                   "dyc_group_find")
         return vals, hit
 
+    def plan(self, keys, vals, offsets, lookups, lookup_offsets, out_vals=None, out_hit=None) -> "GroupPlan":
+        """Capture the step clear -> insert(keys, vals) -> find(lookups) into a HIP graph
+        (dyc_group_plan_create); GroupPlan.run() replays it with one launch. The tensors are
+        fixed at capture (the plan keeps them alive); their contents may change between runs."""
+        k = _as_u32_tensor(keys, self.device)
+        v = _as_u32_tensor(vals, self.device)
+        q = _as_u32_tensor(lookups, self.device)
+        o = self._offsets(offsets)
+        fo = self._offsets(lookup_offsets)
+        if out_vals is None:
+            out_vals = torch.empty(q.numel(), dtype=torch.int32, device=self.device)
+        if out_hit is None:
+            out_hit = torch.empty(q.numel(), dtype=torch.uint8, device=self.device)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            torch.cuda.synchronize(self.device)
+            check(self.lib.dyc_group_plan_create(self.h, k.data_ptr(), v.data_ptr(), o.ctypes.data, q.data_ptr(),
+                                                 fo.ctypes.data, out_vals.data_ptr(), out_hit.data_ptr(),
+                                                 ctypes.byref(h)), "dyc_group_plan_create")
+        plan = GroupPlan(self, h, (k, v, q, out_vals, out_hit))
+        if not hasattr(self, "_plans"):
+            self._plans = weakref.WeakSet()
+        self._plans.add(plan)
+        return plan
+
     def erase(self, keys, offsets, deleted=None, stream=None):
         k = _as_u32_tensor(keys, self.device)
         o = self._offsets(offsets)
@@ -676,3 +704,27 @@ class DyCuckooGroup:
             check(self.lib.dyc_group_pending(self.h, e.ctypes.data, tb.ctypes.data, n.value, ctypes.byref(n),
                                              _stream_ptr(stream)), "dyc_group_pending")
         return e, tb
+
+
+class GroupPlan:
+    """A captured group step (see DyCuckooGroup.plan)."""
+
+    def __init__(self, group: DyCuckooGroup, h, tensors):
+        self.group, self.h, self.tensors = group, h, tensors
+        self.out_vals, self.out_hit = tensors[3], tensors[4]
+
+    def run(self, stream=None) -> None:
+        """One graph launch; waits for it and takes in the counters (as the eager insert does)."""
+        with torch.cuda.device(self.group.device):
+            check(self.group.lib.dyc_group_plan_run(self.h, _stream_ptr(stream)), "dyc_group_plan_run")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.group.lib.dyc_group_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

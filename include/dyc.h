@@ -284,6 +284,9 @@ typedef struct dyc_group_launch {
     uint32_t insert_blocks;  /* persistent grid for insert (0 = default) */
     uint32_t item_keys;      /* keys per work item (0 = default 4096) */
     uint32_t kick_bound;     /* eviction iterations under the reference victim rule */
+    uint32_t insert_waves_per_eu;  /* occupancy: 6 or 8 = the insert kernel built for at least that
+                                      many waves per SIMD (fewer registers); 0 = the compiler's own */
+    uint32_t find_waves_per_eu;    /* the same for find and erase */
 } dyc_group_launch;
 
 /* capacity_hints: host array of n_tables slot-count hints (sized per table as dyc_create). */
@@ -318,6 +321,24 @@ int dyc_group_pending(dyc_group *g, uint64_t *host_entries, uint32_t *host_table
                       dyc_stream stream);
 /* Empty every table of the group. Asynchronous. */
 int dyc_group_clear(dyc_group *g, dyc_stream stream);
+
+/* A group step captured once into a HIP graph and replayed with one launch: clear every table,
+ * insert keys/vals (segmented by ins_offsets, as dyc_group_insert), then find lookups (segmented
+ * by find_offsets, as dyc_group_find) into vals_out / hit (optional). The pointers and offsets
+ * are fixed when the plan is made; the buffers' contents may change between runs. Each run
+ * equals dyc_group_clear + dyc_group_insert + dyc_group_find with the launch shape the group had
+ * when the plan was made, and returns what that insert would (DYC_ERR_PENDING when entries
+ * exhausted the eviction bound; the find of the same run searched them). It waits once, at the
+ * end. No counterpart in the reference, whose calls launch and synchronise one by one
+ * (cuckoo.cu:858-916); for callers that replay the same step shape many times (a per-batch
+ * build-and-probe join over fixed partition buffers). A plan keeps work-item buffers of its own;
+ * destroy plans before their group. */
+typedef struct dyc_group_plan dyc_group_plan;  /* opaque */
+int dyc_group_plan_create(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *ins_offsets,
+                          const uint32_t *lookups, const uint64_t *find_offsets, uint32_t *vals_out, uint8_t *hit,
+                          dyc_group_plan **out);
+int dyc_group_plan_run(dyc_group_plan *p, dyc_stream stream);
+int dyc_group_plan_destroy(dyc_group_plan *p);
 
 /* ---------------------------------------------------------------------------------------
  * Serving loop: many resizable tables behind one request queue (no reference counterpart; the

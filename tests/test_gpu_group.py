@@ -164,3 +164,109 @@ def test_group_stash_serves_pending_entries_of_their_table(D):
     v, h = g.find(dev(np.array([key], np.uint32)), one)
     assert h.cpu().tolist() == [0]
     g.close()
+
+
+def test_group_occupancy_builds_agree(D):
+    """The occupancy builds (at least 6 or 8 waves per SIMD) of the group kernels give the same
+    results as the compiler's own, for every lane shape, including the ones that spill."""
+    banner("group waves-per-EU builds")
+    sizes = sizes_loguniform(40, 2**8, 2**12, 15)
+    keys, vals, offs = batch(sizes, 16)
+    hints = (np.ceil(sizes / 0.85) + 64).astype(np.uint64)
+    for il, fl in [(1, 16), (2, 4), (4, 8)]:
+        for w in (0, 6, 8):
+            g = D.DyCuckooGroup(hints)
+            g.set_launch(insert_lanes=il, find_lanes=fl, insert_waves_per_eu=w, find_waves_per_eu=w)
+            assert g.launch["insert_waves_per_eu"] == w
+            g.insert(dev(keys), dev(vals), offs)
+            v, h = g.find(dev(keys), offs)
+            assert h.cpu().numpy().all() and np.array_equal(v.cpu().numpy().view(np.uint32), vals)
+            d = g.erase(dev(keys[::2]), np.array([0, keys[::2].size] + [keys[::2].size] * (len(sizes) - 1),
+                                                  dtype=np.uint64))
+            assert d.cpu().numpy()[: int(sizes[0] + 1) // 2].all()
+            g.close()
+    g = D.DyCuckooGroup(hints)
+    with pytest.raises(D.DycError):
+        g.set_launch(insert_waves_per_eu=5)
+    g.close()
+
+
+def test_group_graph_plan_replays_equal_eager_steps(D):
+    """A captured step (clear -> insert -> find, one graph launch) replayed several times, with
+    the inputs rewritten and eager work between the replays, equals the eager calls on a twin
+    group every time (values, hits, counts, images), and the oracle on every table."""
+    from databases_gpu_amd.workload import splitmix32_np
+
+    banner("group graph plan: 4 replays vs eager")
+    sizes = sizes_loguniform(50, 2**6, 2**12, 21)
+    sizes[7] = 0
+    hints = (np.ceil(np.maximum(sizes, 1) / 0.85) + 64).astype(np.uint64)
+    keys0, vals0, offs = batch(sizes, 22)
+    qk = dev(keys0)  # lookups: each table's own keys (rewritten below with absent ones mixed in)
+    kk, vv = dev(keys0), dev(vals0)
+    gp = D.DyCuckooGroup(hints, insert_lanes=2, find_lanes=4)
+    ge = D.DyCuckooGroup(hints, insert_lanes=2, find_lanes=4)
+    plan = gp.plan(kk, vv, offs, qk, offs)
+    for r in range(4):
+        keys, _, _ = batch(sizes, 100 + r)
+        vals = splitmix32_np(keys) ^ np.uint32(r)
+        look = keys.copy()
+        look[1::3] ^= np.uint32(0x5A5A5A5A)  # mostly absent keys
+        kk.copy_(dev(keys))
+        vv.copy_(dev(vals))
+        qk.copy_(dev(look))
+        torch.cuda.synchronize()
+        _ = (kk.float() * 2).sum().item()  # eager work between replays
+        plan.run()
+        pv = plan.out_vals.cpu().numpy().view(np.uint32).copy()
+        ph = plan.out_hit.cpu().numpy().copy()
+        ge.clear()
+        ge.insert(kk, vv, offs)
+        ev, eh = ge.find(qk, offs)
+        assert np.array_equal(ph, eh.cpu().numpy()) and np.array_equal(pv, ev.cpu().numpy().view(np.uint32))
+        for i in range(len(sizes)):
+            assert gp.table_stats(i)["count"] == sizes[i] == ge.table_stats(i)["count"]
+        for i in (0, 7, 13, 49):
+            ref = O.OracleTable(nb=gp.table_stats(i)["buckets"])
+            ref.sub = gp.dump(i)
+            assert ref.check_invariant() == 0
+            lo, hi = int(offs[i]), int(offs[i + 1])
+            assert np.array_equal(ref.entries(), np.sort(O.pack(keys[lo:hi], vals[lo:hi])))
+            rv, rh = ref.find(look[lo:hi])
+            assert np.array_equal(ph[lo:hi], rh) and np.array_equal(pv[lo:hi][rh == 1], rv[rh == 1])
+    plan.close()
+    gp.close()
+    ge.close()
+
+
+def test_group_graph_plan_stash_and_moved_pending_list(D):
+    """Entries that exhaust the bound in a replay are found by the same replay's find (the stash
+    size is read on the device) and reported as DYC_ERR_PENDING; a plan whose group's pending
+    list has since grown refuses to run instead of writing the old one."""
+    banner("group graph plan: stash + moved pending list")
+    g = D.DyCuckooGroup([1000, 100, 1000])
+    key = 0x2468ACE1
+    offs = np.array([0, 0, 40, 40], dtype=np.uint64)
+    kk = dev(np.full(40, key, np.uint32))
+    vv = dev(np.arange(500, 540, dtype=np.uint32))
+    q = dev(np.array([key, key], np.uint32))
+    qo = np.array([0, 1, 2, 2], dtype=np.uint64)
+    plan = g.plan(kk, vv, offs, q, qo)
+    for _ in range(2):
+        with pytest.raises(D.DycError) as ei:
+            plan.run()
+        assert ei.value.code == 4
+        assert plan.out_hit.cpu().tolist() == [0, 1]
+        e, tb = g.pending()
+        assert e.size == 8 and np.all(tb == 1)
+    big = 70_000 * 20  # an eager insert this large grows the pending list
+    sizes = np.array([big, 0, 0])
+    g.clear()
+    keys, vals, o2 = batch(sizes, 31)
+    with pytest.raises(D.DycError):  # table 0 holds ~1000 slots: most of the batch cannot be placed
+        g.insert(dev(keys), dev(vals), o2)
+    with pytest.raises(D.DycError) as ei:
+        plan.run()
+    assert ei.value.code == 1 and "plan again" in str(ei.value)
+    plan.close()
+    g.close()
