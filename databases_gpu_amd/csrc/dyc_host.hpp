@@ -16,6 +16,7 @@
 
 #include "dyc.h"
 #include "dyc_device.hpp"
+#include "dyc_threads.hpp"
 
 namespace dyc {
 namespace host {
@@ -85,62 +86,19 @@ inline uint32_t config_subtables(const dyc_config &cfg)
     return cfg.subtables ? cfg.subtables : (uint32_t)kDefaultSubtables;
 }
 
-// Frees device allocations on a helper thread. A hipFree costs about 300 us on MI355X whatever
-// the size (it also waits for the device), and a resize retires one subtable; handed to the
-// helper, the free no longer sits in the calling thread's path (a helper-thread free slowed a
-// loop of launch + synchronise on the calling thread by about 3 us per free). The helper starts
-// on first use; drain() waits until every handed-over block is freed and stops it.
-class Reclaimer {
-  public:
-    Reclaimer() = default;
-    Reclaimer(const Reclaimer &) = delete;
-    Reclaimer &operator=(const Reclaimer &) = delete;
-    ~Reclaimer() { drain(); }
-
-    void defer(int device, void *p)
+// Frees device allocations on a helper thread (ReclaimerT, dyc_threads.hpp). A hipFree costs
+// about 300 us on MI355X whatever the size (it also waits for the device), and a resize retires
+// one subtable; handed to the helper, the free no longer sits in the calling thread's path (a
+// helper-thread free slowed a loop of launch + synchronise on the calling thread by about 3 us
+// per free).
+struct HipFree {
+    void operator()(int device, void *p) const
     {
-        if (!p) return;
-        std::lock_guard<std::mutex> lk(m_);
-        q_.emplace_back(device, p);
-        if (!th_.joinable()) th_ = std::thread([this] { loop(); });
-        cv_.notify_one();
+        (void)hipSetDevice(device);
+        (void)hipFree(p);
     }
-
-    void drain()
-    {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            if (!th_.joinable()) return;
-            stop_ = true;
-        }
-        cv_.notify_one();
-        th_.join();
-        std::lock_guard<std::mutex> lk(m_);
-        stop_ = false;
-    }
-
-  private:
-    void loop()
-    {
-        std::unique_lock<std::mutex> lk(m_);
-        for (;;) {
-            cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
-            if (q_.empty()) return;  // stop_ with nothing left
-            const std::pair<int, void *> job = q_.front();
-            q_.pop_front();
-            lk.unlock();
-            (void)hipSetDevice(job.first);
-            (void)hipFree(job.second);
-            lk.lock();
-        }
-    }
-
-    std::mutex m_;
-    std::condition_variable cv_;
-    std::deque<std::pair<int, void *>> q_;
-    std::thread th_;
-    bool stop_ = false;
 };
+using Reclaimer = ReclaimerT<HipFree>;
 
 struct DeviceGuard {
     int prev = -1;

@@ -25,6 +25,7 @@ using dyc::host::DeviceGuard;
 using dyc::host::fail;
 using dyc::host::initial_buckets;
 using dyc::host::kMaxBuckets;
+using dyc::host::CopyPool;
 using dyc::host::Reclaimer;
 
 namespace {
@@ -765,85 +766,6 @@ int check_config(const dyc_config &cfg)
         return fail(DYC_ERR_INVALID, "subtables must be in [%d, %d] (reserved0 = 0)", kMinSubtables, kMaxSubtables);
     return DYC_OK;
 }
-
-}  // namespace
-
-namespace {
-
-// Splits a large host memcpy (pageable <-> pinned staging) over worker threads that live as long
-// as the staging buffers: starting threads for every copy cost a measurable share of each
-// pipelined chunk. copy() returns when every part is copied.
-class CopyPool {
-  public:
-    explicit CopyPool(unsigned workers)
-    {
-        for (unsigned i = 0; i < workers; ++i) th_.emplace_back([this] { loop(); });
-    }
-    CopyPool(const CopyPool &) = delete;
-    CopyPool &operator=(const CopyPool &) = delete;
-    ~CopyPool()
-    {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto &t : th_) t.join();
-    }
-
-    void copy(void *dst, const void *src, size_t bytes)
-    {
-        constexpr size_t kSplit = 4u << 20;
-        const size_t nparts = th_.size() + 1;
-        if (bytes < kSplit || nparts == 1) {
-            memcpy(dst, src, bytes);
-            return;
-        }
-        const size_t per = ((bytes + nparts - 1) / nparts + 63) & ~(size_t)63;
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            parts_.clear();
-            for (size_t lo = per; lo < bytes; lo += per)
-                parts_.push_back(Part{static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo,
-                                      std::min(per, bytes - lo)});
-            next_ = 0;
-            left_ = parts_.size();
-        }
-        cv_.notify_all();
-        memcpy(dst, src, std::min(per, bytes));  // the first part on the calling thread
-        std::unique_lock<std::mutex> lk(m_);
-        done_cv_.wait(lk, [this] { return left_ == 0; });
-        parts_.clear();
-    }
-
-  private:
-    struct Part {
-        char *dst;
-        const char *src;
-        size_t len;
-    };
-
-    void loop()
-    {
-        std::unique_lock<std::mutex> lk(m_);
-        for (;;) {
-            cv_.wait(lk, [this] { return stop_ || next_ < parts_.size(); });
-            if (stop_) return;
-            const Part p = parts_[next_++];
-            lk.unlock();
-            memcpy(p.dst, p.src, p.len);
-            lk.lock();
-            if (--left_ == 0) done_cv_.notify_one();
-        }
-    }
-
-    std::mutex m_;
-    std::condition_variable cv_, done_cv_;
-    std::vector<Part> parts_;
-    size_t next_ = 0, left_ = 0;
-    bool stop_ = false;
-    std::vector<std::thread> th_;
-};
 
 }  // namespace
 

@@ -18,10 +18,10 @@
  */
 #include "dyc_oracle.h"
 
+#include <pthread.h>
+#include <stdlib.h>
 #include <string.h>
-#ifdef _OPENMP
-#include <omp.h>
-#endif
+#include <unistd.h>
 
 #define KEY_OF(e) ((uint32_t)((e) >> 32))
 #define VAL_OF(e) ((uint32_t)((e)&0xffffffffu))
@@ -687,51 +687,92 @@ uint64_t orc_wide_check_invariant(const orc_wide *tb)
 
 /* ---------------------------------------------------------------- CPU baseline */
 
+/* A parallel loop over [0, n) in chunks taken from a shared counter, on `threads` POSIX threads
+ * (the calling thread is one of them). Plain threads rather than OpenMP, so that the sanitizer
+ * builds see every synchronisation (the joins and the atomics). */
+typedef struct mt_loop {
+    uint64_t n, chunk, next;
+    void (*body)(struct mt_loop *, uint64_t lo, uint64_t hi);
+    void *ctx;
+    uint64_t sum0, sum1, sum2; /* per-loop totals, added atomically by each thread */
+} mt_loop;
+
+static void *mt_worker(void *arg)
+{
+    mt_loop *L = (mt_loop *)arg;
+    for (;;) {
+        const uint64_t lo = __atomic_fetch_add(&L->next, L->chunk, __ATOMIC_RELAXED);
+        if (lo >= L->n) return 0;
+        L->body(L, lo, lo + L->chunk < L->n ? lo + L->chunk : L->n);
+    }
+}
+
+static void mt_run(mt_loop *L, int threads)
+{
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    int started = 0;
+    for (int i = 1; i < threads; ++i)
+        if (pthread_create(&th[started], 0, mt_worker, L) == 0) ++started;
+    mt_worker(L);
+    for (int i = 0; i < started; ++i) pthread_join(th[i], 0);
+}
+
+typedef struct {
+    const orc_table *tb;
+    const uint32_t *keys;
+    uint32_t *vals_out;
+    uint8_t *hit_out;
+} find_ctx;
+
+static void find_body(mt_loop *L, uint64_t lo, uint64_t hi)
+{
+    const find_ctx *c = (const find_ctx *)L->ctx;
+    orc_find(c->tb, c->keys + lo, hi - lo, c->vals_out ? c->vals_out + lo : 0, c->hit_out ? c->hit_out + lo : 0, 0);
+}
+
 void orc_find_mt(const orc_table *tb, const uint32_t *keys, uint64_t n, uint32_t *vals_out,
                  uint8_t *hit_out, int threads)
 {
-    const int64_t chunk = 1 << 16;
-    const int64_t nchunks = (int64_t)((n + chunk - 1) / chunk);
-#ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
-#endif
-    for (int64_t c = 0; c < nchunks; ++c) {
-        uint64_t lo = (uint64_t)c * chunk;
-        uint64_t m = n - lo < (uint64_t)chunk ? n - lo : (uint64_t)chunk;
-        orc_find(tb, keys + lo, m, vals_out ? vals_out + lo : 0, hit_out ? hit_out + lo : 0, 0);
-    }
-    (void)threads;
+    find_ctx c = {tb, keys, vals_out, hit_out};
+    mt_loop L = {n, 1u << 16, 0, find_body, &c, 0, 0, 0};
+    mt_run(&L, threads);
 }
 
 /* Multithreaded insert: the same start table, hashes and victim rule as orc_insert_entry, with
- * the bucket read slot by slot (relaxed atomic loads) and each exchange a 64-bit CAS, OpenMP
- * over keys. A CAS that loses a race re-reads the bucket and repeats the step (at most
+ * the bucket read slot by slot (relaxed atomic loads) and each exchange a 64-bit CAS, keys
+ * shared out in chunks of 4096 over POSIX threads (mt_run). A CAS that loses a race re-reads the bucket and repeats the step (at most
  * ORC_MT_RETRIES times per key). Entries still in hand after the bound, or after too many lost
  * races, are written to fail_out (up to fail_cap) and counted in the return value. Placement
  * depends on thread timing; the stored key->value set does not. Used only as the CPU timing
  * baseline and checked against the sequential insert by tests. */
 #define ORC_MT_RETRIES 64u
-uint64_t orc_insert_mt(orc_table *tb, const uint32_t *keys, const uint32_t *vals, uint64_t n,
-                       uint32_t kick_bound, uint32_t rescue_kicks, uint64_t *fail_out, uint64_t fail_cap,
-                       uint64_t *kicks_out, uint64_t *rejected_out, int threads)
+typedef struct {
+    orc_table *tb;
+    const uint32_t *keys, *vals;
+    uint32_t kick_bound, total;
+    uint64_t *fail_out, fail_cap;
+    uint64_t nfail;
+} insert_ctx;
+
+static void insert_body(mt_loop *L, uint64_t lo, uint64_t hi)
 {
-    uint64_t nfail = 0, kicks = 0, nrej = 0;
-    const uint32_t total = kick_bound + rescue_kicks;
-#ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 4096) num_threads(threads) reduction(+ : kicks, nrej)
-#endif
-    for (int64_t i = 0; i < (int64_t)n; ++i) {
-        const uint32_t key = keys[i];
+    insert_ctx *c = (insert_ctx *)L->ctx;
+    orc_table *tb = c->tb;
+    uint64_t kicks = 0, nrej = 0;
+    for (uint64_t i = lo; i < hi; ++i) {
+        const uint32_t key = c->keys[i];
         if (reserved_key(key)) {
             ++nrej;
             continue;
         }
         uint32_t pre = ORC_NO_TABLE;
         if (key & 1u) pre = orc_get_pos_d(orc_pair_d(key, tb->d), ORC_NO_TABLE, tb->d);
-        uint64_t entry = PACK(key, vals[i]);
+        uint64_t entry = PACK(key, c->vals[i]);
         uint32_t it = 0, races = 0;
         int placed = 0;
-        while (it < total && races <= ORC_MT_RETRIES) {
+        while (it < c->total && races <= ORC_MT_RETRIES) {
             const uint32_t k = KEY_OF(entry);
             const uint32_t t = orc_get_pos_d(orc_pair_d(k, tb->d), pre, tb->d);
             uint64_t *b = tb->sub[t] + (uint64_t)orc_bucket(k, t, tb->nb[t]) * ORC_SLOTS;
@@ -740,7 +781,7 @@ uint64_t orc_insert_mt(orc_table *tb, const uint32_t *keys, const uint32_t *vals
             int lane = -1;
             for (int j = 0; j < ORC_SLOTS; ++j)
                 if (KEY_OF(view[j]) == 0u) { lane = j; break; }
-            if (lane < 0 && it < kick_bound) {
+            if (lane < 0 && it < c->kick_bound) {
                 lane = vote_lane(view, t, tb->d);
                 if (lane < 0) lane = (int)(it & 15u);
             }
@@ -760,21 +801,30 @@ uint64_t orc_insert_mt(orc_table *tb, const uint32_t *keys, const uint32_t *vals
         }
         kicks += it;
         if (!placed) {
-            uint64_t at = __atomic_fetch_add(&nfail, 1, __ATOMIC_RELAXED);
-            if (fail_out && at < fail_cap) fail_out[at] = entry;
+            uint64_t at = __atomic_fetch_add(&c->nfail, 1, __ATOMIC_RELAXED);
+            if (c->fail_out && at < c->fail_cap) c->fail_out[at] = entry;
         }
     }
-    if (kicks_out) *kicks_out = kicks;
-    if (rejected_out) *rejected_out = nrej;
-    (void)threads;
-    return nfail;
+    __atomic_fetch_add(&L->sum0, kicks, __ATOMIC_RELAXED);
+    __atomic_fetch_add(&L->sum1, nrej, __ATOMIC_RELAXED);
+}
+
+uint64_t orc_insert_mt(orc_table *tb, const uint32_t *keys, const uint32_t *vals, uint64_t n,
+                       uint32_t kick_bound, uint32_t rescue_kicks, uint64_t *fail_out, uint64_t fail_cap,
+                       uint64_t *kicks_out, uint64_t *rejected_out, int threads)
+{
+    insert_ctx c = {tb, keys, vals, kick_bound, kick_bound + rescue_kicks, fail_out, fail_cap, 0};
+    mt_loop L = {n, 4096, 0, insert_body, &c, 0, 0, 0};
+    mt_run(&L, threads);
+    if (kicks_out) *kicks_out = L.sum0;
+    if (rejected_out) *rejected_out = L.sum1;
+    return c.nfail;
 }
 
 int orc_max_threads(void)
 {
-#ifdef _OPENMP
-    return omp_get_max_threads();
-#else
-    return 1;
-#endif
+    const char *e = getenv("OMP_NUM_THREADS");  /* the thread-count convention of this image */
+    if (e && atoi(e) > 0) return atoi(e);
+    const long n = sysconf(_SC_NPROCESSORS_ONLN);
+    return n > 0 ? (int)n : 1;
 }

@@ -152,10 +152,10 @@ uint64_t orc_wide_downsize_compact(const uint64_t *old_keys, const uint64_t *old
                                    uint64_t res_cap);
 uint64_t orc_wide_check_invariant(const orc_wide *tb);
 
-/* Multithreaded find over the same image (OpenMP), used only as the CPU timing baseline. */
+/* Multithreaded find over the same image (POSIX threads), used only as the CPU timing baseline. */
 void orc_find_mt(const orc_table *tb, const uint32_t *keys, uint64_t n, uint32_t *vals_out,
                  uint8_t *hit_out, int threads);
-/* Multithreaded insert (OpenMP over keys, 64-bit CAS per exchange), used only as the CPU timing
+/* Multithreaded insert (POSIX threads over key chunks, 64-bit CAS per exchange), used only as the CPU timing
  * baseline. Returns the number of entries left unplaced (written to fail_out up to fail_cap). */
 uint64_t orc_insert_mt(orc_table *tb, const uint32_t *keys, const uint32_t *vals, uint64_t n,
                        uint32_t kick_bound, uint32_t rescue_kicks, uint64_t *fail_out, uint64_t fail_cap,
