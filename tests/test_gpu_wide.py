@@ -314,3 +314,48 @@ def test_wide_bulk_fill_overflow_reaches_the_eviction_insert(D):
     gv, gh = gfind(t, single)
     assert gh.all() and np.array_equal(gv, np.array([want[x] for x in single.tolist()], dtype=np.uint64))
     t.close()
+
+
+def test_wide_bulk_fill_into_a_partly_filled_erased_resized_table(D):
+    """The wide dense fill on a table that already holds entries, has erased slots and unequal
+    subtables after an upsize: the first keys go in by the eviction insert (batches below the
+    bulk threshold), a fifth of them are erased, one subtable is doubled, then one large batch of
+    fresh keys, second copies of stored keys, re-inserts of erased keys and reserved keys takes
+    the bulk fill. Multiset, invariant, counts and finds are checked against the expected set."""
+    banner("wide bulk fill into a partly filled, erased, upsized table")
+    first = keys64(300_000, 1800)
+    t = D.DyCuckooWide(600_000, insert_order=3, auto_resize=False)
+    nb0 = sum(t.buckets)
+    for lo in range(0, first.size, 50_000):  # below 2 x buckets: the eviction insert
+        part = first[lo:lo + 50_000]
+        assert part.size < 2 * nb0
+        t.insert(cu(part), cu(vals_of(part)))
+    gone = first[::5]
+    assert t.erase(cu(gone)).cpu().numpy().all()
+    t.resize_up()
+    nb = t.buckets
+    assert len(set(nb)) == 2  # one subtable doubled
+    kept = np.setdiff1d(first, gone)
+    fresh = absent(250_000, first, 1801)
+    dup = kept[:5000]  # second copies of stored keys
+    back = gone[:3000]  # erased keys inserted again
+    batch = np.concatenate([fresh, dup, back, np.array([0, 2**64 - 1, 0], dtype=np.uint64)])
+    rng = np.random.default_rng(1802)
+    rng.shuffle(batch)
+    assert batch.size >= 2 * sum(nb)  # large enough for the bulk fill
+    bv = vals_of(batch) ^ np.uint64(0x5555)
+    t.insert(cu(batch), cu(bv))
+    st = t.stats()
+    keep = (batch != 0) & (batch != 2**64 - 1)
+    assert st["count"] == kept.size + int(keep.sum()) and st["lost"] == 0
+    assert oracle_of(t).check_invariant() == 0
+    k, v = wide_stored(t)
+    wk = np.concatenate([kept, batch[keep]])
+    wv = np.concatenate([vals_of(kept), bv[keep]])
+    o = np.lexsort((wv, wk))
+    assert np.array_equal(k, wk[o]) and np.array_equal(v, wv[o])
+    gv, gh = gfind(t, np.concatenate([fresh, back]))
+    assert gh.all() and np.array_equal(gv, vals_of(np.concatenate([fresh, back])) ^ np.uint64(0x5555))
+    gv, gh = gfind(t, gone[3000:3100])
+    assert not gh.any()
+    t.close()
