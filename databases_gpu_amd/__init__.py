@@ -7,7 +7,7 @@ raises.
 """
 from ._lib import SLOTS, SUBTABLES, Config, DycError, Stats, default_config, load  # noqa: F401
 from .server import DyCuckooServer  # noqa: F401
-from .table import DyCuckoo, DyCuckooGroup, DyCuckooWide, HashAPI  # noqa: F401
+from .table import DyCuckoo, DyCuckooGroup, DyCuckooWide, DyCuckooWideGroup, HashAPI  # noqa: F401
 
-__all__ = ["DyCuckoo", "DyCuckooGroup", "DyCuckooServer", "DyCuckooWide", "HashAPI", "DycError", "Config", "Stats", "default_config", "load", "SLOTS",
+__all__ = ["DyCuckoo", "DyCuckooGroup", "DyCuckooServer", "DyCuckooWide", "DyCuckooWideGroup", "HashAPI", "DycError", "Config", "Stats", "default_config", "load", "SLOTS",
            "SUBTABLES"]

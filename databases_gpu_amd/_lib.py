@@ -46,6 +46,9 @@ EXPORTS = (
     "dyc_wide_resize_up", "dyc_wide_resize_down", "dyc_wide_get_stats", "dyc_wide_clear", "dyc_wide_dump",
     "dyc_wide_load", "dyc_wide_pending", "dyc_insert_noalloc", "dyc_insert_async", "dyc_sync",
     "dyc_insert_counted", "dyc_group_plan_create", "dyc_group_plan_run", "dyc_group_plan_destroy",
+    "dyc_wide_group_create", "dyc_wide_group_destroy", "dyc_wide_group_insert", "dyc_wide_group_find",
+    "dyc_wide_group_erase", "dyc_wide_group_table_stats", "dyc_wide_group_dump", "dyc_wide_group_pending",
+    "dyc_wide_group_clear",
 )
 
 OP_INSERT, OP_FIND, OP_ERASE = 0, 1, 2
@@ -197,6 +200,15 @@ def load(require: bool = True):
         "dyc_group_plan_create": (ci, [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]),
         "dyc_group_plan_run": (ci, [vp, vp]),
         "dyc_group_plan_destroy": (ci, [vp]),
+        "dyc_wide_group_create": (ci, [vp, u32, vp, ctypes.POINTER(vp)]),
+        "dyc_wide_group_destroy": (ci, [vp]),
+        "dyc_wide_group_insert": (ci, [vp, vp, vp, vp, vp]),
+        "dyc_wide_group_find": (ci, [vp, vp, vp, vp, vp, vp]),
+        "dyc_wide_group_erase": (ci, [vp, vp, vp, vp, vp]),
+        "dyc_wide_group_table_stats": (ci, [vp, u32, vp]),
+        "dyc_wide_group_dump": (ci, [vp, u32, u32, vp, vp, u64, vp]),
+        "dyc_wide_group_pending": (ci, [vp, vp, vp, vp, u64, ctypes.POINTER(u64), vp]),
+        "dyc_wide_group_clear": (ci, [vp, vp]),
         "dyc_sync": (ci, [vp]),
         "dyc_find": (ci, [vp, vp, vp, vp, u64, vp]),
         "dyc_erase": (ci, [vp, vp, vp, u64, vp]),

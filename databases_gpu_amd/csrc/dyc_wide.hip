@@ -46,16 +46,21 @@ struct WideDesc {
 };
 
 // Where an insert parks entries that exhaust the eviction bound (append index Counters::pending).
+// tags (optional, groups): the table index of each parked entry.
 struct WideSink {
     uint64_t *keys, *vals;
     uint64_t cap;
+    uint32_t *tags = nullptr;
 };
 
-// The pending entries searched after pos2 by finds, erases and updates (n = 0: none).
+// The pending entries searched after pos2 by finds, erases and updates (n = 0: none). tags
+// (optional, groups): only entries tagged `table` belong to the table being probed.
 struct WideStash {
     uint64_t *keys;
     uint64_t *vals;
     uint64_t n;
+    const uint32_t *tags = nullptr;
+    uint32_t table = 0;
 };
 
 namespace {
@@ -91,113 +96,122 @@ __device__ inline void store2(uint64_t *p, uint64_t a, uint64_t b)
 // neither bucket holds are then looked up in the stash (erase and update act on every copy
 // there). Update: keys of one launch are distinct, so no two lanes write one value; the key word
 // is untouched, and the value is an agent-scope atomic store.
+// One wave probes keys [base, base + 64) clipped to n; returns this lane's flag (found, deleted
+// or updated), also written to flags. All 64 lanes call it together.
 template <int MODE>
-__global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint64_t *__restrict__ keys,
-                                                         uint64_t *__restrict__ vals, uint8_t *__restrict__ flags,
-                                                         uint64_t n, Counters *__restrict__ C, WideStash stash)
+__device__ inline uint32_t wide_probe_tile(const WideDesc &W, const uint64_t *__restrict__ keys,
+                                           uint64_t *__restrict__ vals, uint8_t *__restrict__ flags, uint64_t base,
+                                           uint64_t n, const WideStash &stash)
 {
     constexpr bool ERASE = MODE == 1;
     constexpr bool UPDATE = MODE == 2;
     constexpr int G = 8;
     const int lane = __lane_id();
     const int g = lane / G, s = lane % G;
+    const uint64_t i = base + lane;
+    const bool inb = i < n;
+    const uint64_t k = inb ? keys[i] : 0ull;
+    const uint64_t nv = (UPDATE && inb) ? vals[i] : 0ull;  // update: the new value
+    const uint32_t f = fold64(k);
+    const uint32_t p = pair_of(W.D, f);
+    const uint32_t t1 = pos1(W.D, p), t2 = pos2(W.D, p);
+    const uint32_t loc1 = wide_loc(t1, bucket_of(W.D, f, t1));
+    const uint32_t loc2 = wide_loc(t2, bucket_of(W.D, f, t2));
+
+    uint64_t e[G][2];
+    uint64_t kk_key[G];
+    uint64_t kk_new[UPDATE ? G : 1];
+    bool kk_valid[G];
+    uint32_t kk_l[G];
+    uint32_t my_flag = 0, my_loc = 0, my_slot = 0, done = 0;
+#pragma unroll
+    for (int phase = 0; phase < 2; ++phase) {
+        const uint32_t lp = phase == 0 ? loc1 : loc2;
+#pragma unroll
+        for (int kk = 0; kk < G; ++kk) {
+            const int src = g * G + kk;
+            if (phase == 0) {
+                kk_key[kk] = shfl64(k, src);
+                kk_valid[kk] = !wide_reserved(kk_key[kk]);
+                if constexpr (UPDATE) kk_new[kk] = shfl64(nv, src);
+            }
+            kk_l[kk] = __shfl(lp, src);
+            e[kk][0] = e[kk][1] = 0;
+            if (kk_valid[kk] && !(done & (1u << kk))) load2(wide_keys_at(W, kk_l[kk]) + s * 2, e[kk]);
+        }
+#pragma unroll
+        for (int kk = 0; kk < G; ++kk) {
+            if (done & (1u << kk)) continue;  // group-uniform
+            const int first = e[kk][0] == kk_key[kk] ? 0 : (e[kk][1] == kk_key[kk] ? 1 : -1);
+            const uint32_t gm = (uint32_t)((__ballot(first >= 0 && kk_valid[kk]) >> (g * G)) & 0xffu);
+            if (!gm) continue;
+            done |= 1u << kk;
+            if constexpr (ERASE) {
+                int ok = 0;
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (kk_valid[kk] && e[kk][j] == kk_key[kk]) {
+                        uint64_t exp = e[kk][j];
+                        ok |= cas_slot(wide_keys_at(W, kk_l[kk]) + s * 2 + j, exp, 0ull);
+                    }
+                const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & 0xffu);
+                if (kk == s) my_flag = okm != 0;
+            } else if constexpr (UPDATE) {
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (kk_valid[kk] && e[kk][j] == kk_key[kk])
+                        __hip_atomic_store(wide_vals_at(W, kk_l[kk]) + s * 2 + j, kk_new[UPDATE ? kk : 0],
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (kk == s) my_flag = 1;
+            } else {
+                const int w = __builtin_ctz(gm);
+                const int fj = __shfl(first, g * G + w);
+                if (kk == s) {
+                    my_flag = 1;
+                    my_loc = kk_l[kk];
+                    my_slot = (uint32_t)(w * 2 + fj);
+                }
+            }
+        }
+    }
+    uint64_t my_val = 0;
+    if (MODE == 0 && my_flag) my_val = wide_vals_at(W, my_loc)[my_slot];
+    if (inb && !my_flag && stash.n && !wide_reserved(k)) {  // the stash counts as one more bucket
+        for (uint64_t j = 0; j < stash.n; ++j) {
+            const uint64_t sk = __hip_atomic_load(stash.keys + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (sk != k || (stash.tags && stash.tags[j] != stash.table)) continue;
+            if constexpr (ERASE) {
+                uint64_t exp = sk;
+                my_flag |= (uint32_t)cas_slot(stash.keys + j, exp, 0ull);
+            } else if constexpr (UPDATE) {
+                __hip_atomic_store(stash.vals + j, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                my_flag = 1;
+            } else {
+                my_val = stash.vals[j];
+                my_flag = 1;
+                break;
+            }
+        }
+    }
+    if (inb) {
+        if (MODE == 0 && vals) vals[i] = my_val;
+        if (flags) flags[i] = (uint8_t)my_flag;
+    }
+    return my_flag;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void wide_probe_kernel(WideDesc W, const uint64_t *__restrict__ keys,
+                                                         uint64_t *__restrict__ vals, uint8_t *__restrict__ flags,
+                                                         uint64_t n, Counters *__restrict__ C, WideStash stash)
+{
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
     unsigned long long ndel = 0;
     for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
-         base += waves * kWave) {
-        const uint64_t i = base + lane;
-        const bool inb = i < n;
-        const uint64_t k = inb ? keys[i] : 0ull;
-        const uint64_t nv = (UPDATE && inb) ? vals[i] : 0ull;  // update: the new value
-        const uint32_t f = fold64(k);
-        const uint32_t p = pair_of(W.D, f);
-        const uint32_t t1 = pos1(W.D, p), t2 = pos2(W.D, p);
-        const uint32_t loc1 = wide_loc(t1, bucket_of(W.D, f, t1));
-        const uint32_t loc2 = wide_loc(t2, bucket_of(W.D, f, t2));
-
-        uint64_t e[G][2];
-        uint64_t kk_key[G];
-        uint64_t kk_new[UPDATE ? G : 1];
-        bool kk_valid[G];
-        uint32_t kk_l[G];
-        uint32_t my_flag = 0, my_loc = 0, my_slot = 0, done = 0;
-#pragma unroll
-        for (int phase = 0; phase < 2; ++phase) {
-            const uint32_t lp = phase == 0 ? loc1 : loc2;
-#pragma unroll
-            for (int kk = 0; kk < G; ++kk) {
-                const int src = g * G + kk;
-                if (phase == 0) {
-                    kk_key[kk] = shfl64(k, src);
-                    kk_valid[kk] = !wide_reserved(kk_key[kk]);
-                    if constexpr (UPDATE) kk_new[kk] = shfl64(nv, src);
-                }
-                kk_l[kk] = __shfl(lp, src);
-                e[kk][0] = e[kk][1] = 0;
-                if (kk_valid[kk] && !(done & (1u << kk))) load2(wide_keys_at(W, kk_l[kk]) + s * 2, e[kk]);
-            }
-#pragma unroll
-            for (int kk = 0; kk < G; ++kk) {
-                if (done & (1u << kk)) continue;  // group-uniform
-                const int first = e[kk][0] == kk_key[kk] ? 0 : (e[kk][1] == kk_key[kk] ? 1 : -1);
-                const uint32_t gm = (uint32_t)((__ballot(first >= 0 && kk_valid[kk]) >> (g * G)) & 0xffu);
-                if (!gm) continue;
-                done |= 1u << kk;
-                if constexpr (ERASE) {
-                    int ok = 0;
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        if (kk_valid[kk] && e[kk][j] == kk_key[kk]) {
-                            uint64_t exp = e[kk][j];
-                            ok |= cas_slot(wide_keys_at(W, kk_l[kk]) + s * 2 + j, exp, 0ull);
-                        }
-                    const uint32_t okm = (uint32_t)((__ballot(ok) >> (g * G)) & 0xffu);
-                    if (kk == s) my_flag = okm != 0;
-                } else if constexpr (UPDATE) {
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        if (kk_valid[kk] && e[kk][j] == kk_key[kk])
-                            __hip_atomic_store(wide_vals_at(W, kk_l[kk]) + s * 2 + j, kk_new[UPDATE ? kk : 0],
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (kk == s) my_flag = 1;
-                } else {
-                    const int w = __builtin_ctz(gm);
-                    const int fj = __shfl(first, g * G + w);
-                    if (kk == s) {
-                        my_flag = 1;
-                        my_loc = kk_l[kk];
-                        my_slot = (uint32_t)(w * 2 + fj);
-                    }
-                }
-            }
-        }
-        uint64_t my_val = 0;
-        if (MODE == 0 && my_flag) my_val = wide_vals_at(W, my_loc)[my_slot];
-        if (inb && !my_flag && stash.n && !wide_reserved(k)) {  // the stash counts as one more bucket
-            for (uint64_t j = 0; j < stash.n; ++j) {
-                const uint64_t sk = __hip_atomic_load(stash.keys + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (sk != k) continue;
-                if constexpr (ERASE) {
-                    uint64_t exp = sk;
-                    my_flag |= (uint32_t)cas_slot(stash.keys + j, exp, 0ull);
-                } else if constexpr (UPDATE) {
-                    __hip_atomic_store(stash.vals + j, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    my_flag = 1;
-                } else {
-                    my_val = stash.vals[j];
-                    my_flag = 1;
-                    break;
-                }
-            }
-        }
-        if (inb) {
-            if (MODE == 0 && vals) vals[i] = my_val;
-            if (flags) flags[i] = (uint8_t)my_flag;
-        }
-        ndel += my_flag;
-    }
-    if constexpr (ERASE) wave_tally(C, &C->deleted, ndel);
-    if constexpr (UPDATE) wave_tally(C, &C->updated, ndel);
+         base += waves * kWave)
+        ndel += wide_probe_tile<MODE>(W, keys, vals, flags, base, n, stash);
+    if constexpr (MODE == 1) wave_tally(C, &C->deleted, ndel);
+    if constexpr (MODE == 2) wave_tally(C, &C->updated, ndel);
 }
 
 // ---------------------------------------------------------------------------------- insert
@@ -213,16 +227,116 @@ struct WideCarry {
     uint64_t cur[kSlots];  // the lane's view of the bucket it is about to write
 };
 
-__device__ inline void wide_give_up(WideCarry &c, const WideSink &sink, Counters *C)
+__device__ inline void wide_give_up(WideCarry &c, const WideSink &sink, Counters *C, uint32_t tag = 0)
 {
     const unsigned long long at = atomicAdd(&C->pending, 1ull);
     if (at < sink.cap) {
         sink.keys[at] = c.key;
         sink.vals[at] = c.val;
+        if (sink.tags) sink.tags[at] = tag;
     } else {
         atomicAdd(&C->lost, 1ull);
     }
     c.active = false;
+}
+
+// Take entry (k, v) into the lane's hand; pr is its previous table (kNoTable for a fresh key).
+__device__ inline void wide_take(WideCarry &c, const Desc &D, uint64_t k, uint64_t v, uint32_t pr)
+{
+    if (pr == kNoTable && (k & 1u)) pr = pos1(D, pair_of(D, fold64(k)));  // odd keys start in pos2
+    c.key = k;
+    c.val = v;
+    c.pre = pr;
+    c.kicks = 0;
+    c.retries = 0;
+    c.fresh = true;
+    c.active = true;
+}
+
+// One iteration of an active lane's eviction chain (see the top of this file); finished chains
+// add their exchanges to kicks.
+__device__ inline void wide_step(WideCarry &c, const WideDesc &W, const InsertPolicy &P, Counters *C,
+                                 const WideSink &sink, uint32_t tag, unsigned long long &kicks,
+                                 unsigned long long &retries)
+{
+    const Desc &D = W.D;
+    const uint32_t total_kicks = P.kick_bound + P.rescue_kicks;
+    const uint32_t f = fold64(c.key);
+    const uint32_t t = other_pos(D, pair_of(D, f), c.pre);
+    const uint32_t b = bucket_of(D, f, t);
+    uint64_t *kp = selt(t, D.sub) + (uint64_t)b * kSlots;
+    uint64_t *vp = selt(t, W.val) + (uint64_t)b * kSlots;
+    if (c.fresh) {
+#pragma unroll
+        for (int j = 0; j < kSlots; j += 2) {
+            uint64_t e[2];
+            load2(kp + j, e);
+            c.cur[j] = e[0];
+            c.cur[j + 1] = e[1];
+        }
+        c.fresh = false;
+    }
+    int slot = -1;
+#pragma unroll
+    for (int j = kSlots - 1; j >= 0; --j)
+        if (c.cur[j] == 0ull) slot = j;
+    if (slot < 0) {
+        if (c.kicks < P.kick_bound) {
+#pragma unroll
+            for (int j = kSlots - 1; j >= 0; --j)
+                if (c.cur[j] != kWideBusy && vote_victim(D, fold64(c.cur[j]), t)) slot = j;
+            if (slot < 0) slot = (int)(c.kicks & 15u);
+        } else {
+            slot = (int)rescue_lane(f, c.kicks);
+        }
+    }
+    uint64_t expected = 0;
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j)
+        if (j == slot) expected = c.cur[j];
+    // claim the slot; on a lost race (or a slot another lane is rewriting, refreshed with a
+    // BUSY -> BUSY CAS, which changes nothing) the slot's current key replaces the view's
+    uint64_t exp = expected;
+    const bool claimed = expected != kWideBusy ? cas_slot(kp + slot, exp, kWideBusy) : false;
+    if (expected == kWideBusy) (void)cas_slot(kp + slot, exp, kWideBusy);
+    if (!claimed) {
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j)
+            if (j == slot) c.cur[j] = exp;
+        ++retries;
+        if (++c.retries > P.max_retries) {
+            kicks += c.kicks;
+            wide_give_up(c, sink, C, tag);
+        }
+        return;
+    }
+    // The claimed slot's value belongs to this lane until the key is published: an empty
+    // slot's value is written with an atomic store, a victim's is exchanged (its value goes
+    // on in hand). Agent-scope atomic stores cost less than exchanges (random 8-byte stores
+    // 24 G/s against 18 G/s CAS on MI355X) and stay coherent with the CAS on the same slot.
+    uint64_t old_val = 0;
+    if (expected == 0ull)
+        __hip_atomic_store(vp + slot, c.val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        old_val = __hip_atomic_exchange(vp + slot, c.val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // wait until the value write is performed before publishing the key; the memory clobber
+    // also keeps the compiler from moving the publish above the wait
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __hip_atomic_store(kp + slot, c.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ++c.kicks;
+    c.fresh = true;
+    if (expected == 0ull) {
+        kicks += c.kicks;
+        c.active = false;
+    } else {
+        c.key = expected;
+        c.val = old_val;
+        c.pre = t;
+        if (c.kicks >= total_kicks) {
+            kicks += c.kicks;
+            wide_give_up(c, sink, C, tag);
+        }
+    }
 }
 
 // skip (optional): a keys insert leaves out key i where skip[i] != 0 (an upsert updated it in
@@ -233,10 +347,8 @@ __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint
                                                           uint32_t pre_fixed, InsertPolicy P, Counters *__restrict__ C,
                                                           WideSink sink, const uint8_t *__restrict__ skip)
 {
-    const Desc &D = W.D;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint64_t next = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t total_kicks = P.kick_bound + P.rescue_kicks;
     unsigned long long kicks = 0, rejected = 0, retries = 0;
     WideCarry c;
     c.active = false;
@@ -258,97 +370,84 @@ __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint
                 ++rejected;
                 continue;
             }
-            uint32_t pr = FROM_ENTRIES ? pre_fixed : kNoTable;
-            if (pr == kNoTable && (k & 1u)) pr = pos1(D, pair_of(D, fold64(k)));  // odd keys start in pos2
-            c.key = k;
-            c.val = v;
-            c.pre = pr;
-            c.kicks = 0;
-            c.retries = 0;
-            c.fresh = true;
-            c.active = true;
+            wide_take(c, W.D, k, v, FROM_ENTRIES ? pre_fixed : kNoTable);
         }
         if (!__any(c.active)) break;
-        if (!c.active) continue;
+        if (c.active) wide_step(c, W, P, C, sink, 0, kicks, retries);
+    }
+    wave_tally(C, &C->kicks, kicks);
+    wave_tally(C, &C->rejected, rejected);
+    wave_tally(C, &C->retries, retries);
+}
 
-        const uint32_t f = fold64(c.key);
-        const uint32_t t = other_pos(D, pair_of(D, f), c.pre);
-        const uint32_t b = bucket_of(D, f, t);
-        uint64_t *kp = selt(t, D.sub) + (uint64_t)b * kSlots;
-        uint64_t *vp = selt(t, W.val) + (uint64_t)b * kSlots;
-        if (c.fresh) {
-#pragma unroll
-            for (int j = 0; j < kSlots; j += 2) {
-                uint64_t e[2];
-                load2(kp + j, e);
-                c.cur[j] = e[0];
-                c.cur[j + 1] = e[1];
+// ------------------------------------------------------------------------------ many tables
+// The group forms: one persistent launch over work items of one table each (GroupItem), the
+// table's descriptor read from a directory, the per-tile and per-chain code above unchanged.
+// Pending entries carry their table in sink.tags / stash.tags.
+template <int MODE>
+__global__ __launch_bounds__(256) void wide_group_probe_kernel(const WideDesc *__restrict__ dir,
+                                                               const GroupItem *__restrict__ items, uint32_t n_items,
+                                                               const uint64_t *__restrict__ keys,
+                                                               uint64_t *__restrict__ vals,
+                                                               uint8_t *__restrict__ flags,
+                                                               unsigned long long *__restrict__ table_counts,
+                                                               WideStash stash)
+{
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+         it < n_items; it += waves) {
+        const GroupItem item = items[it];
+        const WideDesc W = dir[item.table];
+        WideStash st = stash;
+        st.table = item.table;
+        const uint64_t end = item.begin + item.len;
+        unsigned long long nd = 0;
+        for (uint64_t base = item.begin; base < end; base += kWave)
+            nd += wide_probe_tile<MODE>(W, keys, vals, flags, base, end, st);
+        if constexpr (MODE == 1) {
+            nd = wave_sum(nd);
+            if (__lane_id() == 0 && nd) atomicAdd(&table_counts[item.table], nd);
+        }
+    }
+}
+
+// Lanes of a wave take the item's entries one each (lane-per-chain, as wide_insert_kernel); the
+// wave moves to its next item when the item is used up and every lane's chain has ended.
+__global__ __launch_bounds__(256) void wide_group_insert_kernel(const WideDesc *__restrict__ dir,
+                                                                const GroupItem *__restrict__ items, uint32_t n_items,
+                                                                const uint64_t *__restrict__ keys,
+                                                                const uint64_t *__restrict__ vals, InsertPolicy P,
+                                                                Counters *__restrict__ C, WideSink sink,
+                                                                unsigned long long *__restrict__ table_counts)
+{
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    unsigned long long kicks = 0, rejected = 0, retries = 0;
+    WideCarry c;
+    c.active = false;
+    for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+         it < n_items; it += waves) {
+        const GroupItem item = items[it];
+        const WideDesc W = dir[item.table];
+        const uint64_t end = item.begin + item.len;
+        uint64_t next = item.begin + __lane_id();
+        unsigned long long rej = 0;
+        for (;;) {
+            while (!c.active && next < end) {
+                const uint64_t k = keys[next];
+                const uint64_t v = vals[next];
+                next += kWave;
+                if (wide_reserved(k)) {
+                    ++rej;
+                    continue;
+                }
+                wide_take(c, W.D, k, v, kNoTable);
             }
-            c.fresh = false;
+            if (!__any(c.active)) break;
+            if (c.active) wide_step(c, W, P, C, sink, item.table, kicks, retries);
         }
-        int slot = -1;
-#pragma unroll
-        for (int j = kSlots - 1; j >= 0; --j)
-            if (c.cur[j] == 0ull) slot = j;
-        if (slot < 0) {
-            if (c.kicks < P.kick_bound) {
-#pragma unroll
-                for (int j = kSlots - 1; j >= 0; --j)
-                    if (c.cur[j] != kWideBusy && vote_victim(D, fold64(c.cur[j]), t)) slot = j;
-                if (slot < 0) slot = (int)(c.kicks & 15u);
-            } else {
-                slot = (int)rescue_lane(f, c.kicks);
-            }
-        }
-        uint64_t expected = 0;
-#pragma unroll
-        for (int j = 0; j < kSlots; ++j)
-            if (j == slot) expected = c.cur[j];
-        // claim the slot; on a lost race (or a slot another lane is rewriting, refreshed with a
-        // BUSY -> BUSY CAS, which changes nothing) the slot's current key replaces the view's
-        uint64_t exp = expected;
-        const bool claimed = expected != kWideBusy ? cas_slot(kp + slot, exp, kWideBusy) : false;
-        if (expected == kWideBusy) (void)cas_slot(kp + slot, exp, kWideBusy);
-        if (!claimed) {
-#pragma unroll
-            for (int j = 0; j < kSlots; ++j)
-                if (j == slot) c.cur[j] = exp;
-        }
-        if (!claimed) {
-            ++retries;
-            if (++c.retries > P.max_retries) {
-                kicks += c.kicks;
-                wide_give_up(c, sink, C);
-            }
-            continue;
-        }
-        // The claimed slot's value belongs to this lane until the key is published: an empty
-        // slot's value is written with an atomic store, a victim's is exchanged (its value goes
-        // on in hand). Agent-scope atomic stores cost less than exchanges (random 8-byte stores
-        // 24 G/s against 18 G/s CAS on MI355X) and stay coherent with the CAS on the same slot.
-        uint64_t old_val = 0;
-        if (expected == 0ull)
-            __hip_atomic_store(vp + slot, c.val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else
-            old_val = __hip_atomic_exchange(vp + slot, c.val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // wait until the value write is performed before publishing the key; the memory clobber
-        // also keeps the compiler from moving the publish above the wait
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __hip_atomic_store(kp + slot, c.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ++c.kicks;
-        c.fresh = true;
-        if (expected == 0ull) {
-            kicks += c.kicks;
-            c.active = false;
-        } else {
-            c.key = expected;
-            c.val = old_val;
-            c.pre = t;
-            if (c.kicks >= total_kicks) {
-                kicks += c.kicks;
-                wide_give_up(c, sink, C);
-            }
-        }
+        rejected += rej;
+        rej = wave_sum(rej);
+        if (__lane_id() == 0 && item.len > rej) atomicAdd(&table_counts[item.table], item.len - rej);
     }
     wave_tally(C, &C->kicks, kicks);
     wave_tally(C, &C->rejected, rejected);
@@ -743,6 +842,32 @@ hipError_t launch_wide_insert(const WideDesc &W, const uint64_t *keys, const uin
     return hipGetLastError();
 }
 
+constexpr int kWideGroupBlocks = 2048;
+
+hipError_t launch_wide_group_probe(const WideDesc *dir, const GroupItem *items, uint32_t n_items, const uint64_t *keys,
+                                   uint64_t *vals, uint8_t *flags, bool erase, unsigned long long *table_counts,
+                                   int blocks, const WideStash &stash, hipStream_t st)
+{
+    if (n_items == 0) return hipSuccess;
+    const int grid = blocks > 0 ? blocks : std::min<int>(kWideGroupBlocks, (int)((n_items + 3) / 4));
+    if (erase)
+        wide_group_probe_kernel<1><<<grid, 256, 0, st>>>(dir, items, n_items, keys, nullptr, flags, table_counts, stash);
+    else
+        wide_group_probe_kernel<0><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, flags, nullptr, stash);
+    return hipGetLastError();
+}
+
+hipError_t launch_wide_group_insert(const WideDesc *dir, const GroupItem *items, uint32_t n_items,
+                                    const uint64_t *keys, const uint64_t *vals, InsertPolicy pol, int blocks,
+                                    Counters *C, const WideSink &sink, unsigned long long *table_counts,
+                                    hipStream_t st)
+{
+    if (n_items == 0) return hipSuccess;
+    const int grid = blocks > 0 ? blocks : std::min<int>(kWideGroupBlocks, (int)((n_items + 3) / 4));
+    wide_group_insert_kernel<<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
+    return hipGetLastError();
+}
+
 // Wide entries (16 bytes): 4096-entry tiles (256 threads), two blocks per CU.
 constexpr int kWideSortThreads = 256;
 constexpr uint32_t kWideSortTile = kWideSortThreads * kSortItems;
@@ -882,8 +1007,8 @@ struct dyc_wide {
     LaunchShape shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.max_blocks}; }
     LaunchShape insert_shape() const { return LaunchShape{(int)cfg.block_threads, (int)cfg.insert_blocks}; }
     InsertPolicy policy() const { return InsertPolicy{cfg.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
-    WideSink sink_at(uint64_t at) const { return WideSink{pk + at, pv + at, pend_cap - at}; }
-    WideStash stash() const { return WideStash{pk, pv, pend_n}; }
+    WideSink sink_at(uint64_t at) const { return WideSink{pk + at, pv + at, pend_cap - at, nullptr}; }
+    WideStash stash() const { return WideStash{pk, pv, pend_n, nullptr, 0}; }
 };
 
 namespace {
@@ -1616,6 +1741,395 @@ int dyc_wide_pending(dyc_wide *t, uint64_t *host_keys, uint64_t *host_vals, uint
         if (host_vals) HIP_TRY(hipMemcpyAsync(host_vals, t->pv, m * 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
     }
+    return DYC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------ wide-key groups
+// Many wide tables in one arena behind a directory of WideDesc, served by one persistent launch
+// per batch (the dyc_group_* design with 64-bit keys and values). Tables do not resize; entries
+// that exhaust the eviction bound wait in a pending list tagged with their table, which that
+// table's finds and erases also search.
+struct dyc_wide_group {
+    dyc_config cfg;
+    int device = 0;
+    uint32_t n = 0;
+    uint8_t *arena = nullptr;
+    size_t arena_bytes = 0;
+    std::vector<WideDesc> hdir;
+    WideDesc *ddir = nullptr;
+    std::vector<uint64_t> count;  // per table: placed + pending
+    Counters *ctr = nullptr;
+    Counters *hctr = nullptr;                // pinned, kCounterBytes
+    unsigned long long *tcounts = nullptr;   // per-table counts of the last insert / erase
+    unsigned long long *htcounts = nullptr;  // pinned mirror
+    uint64_t *pk = nullptr, *pv = nullptr;
+    uint32_t *ptag = nullptr;
+    uint64_t pend_cap = 0, pend_n = 0;
+    GroupItem *ditems = nullptr;
+    GroupItem *hitems = nullptr;  // pinned staging
+    size_t items_cap = 0;
+    hipEvent_t items_free = nullptr;
+    dyc_stats st{};
+
+    InsertPolicy policy() const { return InsertPolicy{cfg.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
+    WideStash stash() const { return WideStash{pk, pv, pend_n, ptag, 0}; }
+};
+
+namespace {
+
+constexpr uint32_t kWideGroupItemKeys = 4096;
+
+int wg_check_offsets(const dyc_wide_group *g, const uint64_t *offsets)
+{
+    if (!offsets) return fail(DYC_ERR_INVALID, "null offsets");
+    if (offsets[0] != 0) return fail(DYC_ERR_INVALID, "offsets[0] must be 0");
+    for (uint32_t i = 0; i < g->n; ++i)
+        if (offsets[i + 1] < offsets[i]) return fail(DYC_ERR_INVALID, "offsets decrease at table %u", i);
+    return DYC_OK;
+}
+
+int wg_upload_items(dyc_wide_group *g, const uint64_t *offsets, hipStream_t s, uint32_t *n_items)
+{
+    const uint64_t per = kWideGroupItemKeys;
+    uint64_t need = 0;
+    for (uint32_t i = 0; i < g->n; ++i) need += (offsets[i + 1] - offsets[i] + per - 1) / per;
+    if (need > 0xffffffffull) return fail(DYC_ERR_INVALID, "batch too large for 32-bit item indexes");
+    HIP_TRY(hipEventSynchronize(g->items_free));  // hitems is free to rewrite
+    if (need > g->items_cap) {
+        const size_t cap = std::max<size_t>(need, g->items_cap * 2);
+        if (g->ditems) HIP_TRY(hipFree(g->ditems));
+        if (g->hitems) HIP_TRY(hipHostFree(g->hitems));
+        g->ditems = nullptr;
+        g->hitems = nullptr;
+        g->items_cap = 0;
+        if (hipMalloc(&g->ditems, cap * sizeof(GroupItem)) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void **>(&g->hitems), cap * sizeof(GroupItem), hipHostMallocDefault) !=
+                hipSuccess) {
+            (void)hipGetLastError();
+            return fail(DYC_ERR_OOM, "work item buffers (%zu items) failed", cap);
+        }
+        g->items_cap = cap;
+    }
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < g->n; ++i)
+        for (uint64_t b = offsets[i]; b < offsets[i + 1]; b += per)
+            g->hitems[m++] = GroupItem{b, (uint32_t)std::min<uint64_t>(per, offsets[i + 1] - b), i};
+    if (m) HIP_TRY(hipMemcpyAsync(g->ditems, g->hitems, m * sizeof(GroupItem), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(g->items_free, s));
+    *n_items = m;
+    return DYC_OK;
+}
+
+int wg_reserve_pending(dyc_wide_group *g, uint64_t extra, hipStream_t s)
+{
+    const uint64_t need = g->pend_n + extra;
+    if (need <= g->pend_cap) return DYC_OK;
+    const uint64_t cap = std::max<uint64_t>(need, g->pend_cap * 2);
+    uint64_t *nk = nullptr, *nv = nullptr;
+    uint32_t *nt = nullptr;
+    if (hipMalloc(&nk, cap * 8) != hipSuccess || hipMalloc(&nv, cap * 8) != hipSuccess ||
+        hipMalloc(&nt, cap * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        if (nk) (void)hipFree(nk);
+        if (nv) (void)hipFree(nv);
+        return fail(DYC_ERR_OOM, "wide group pending list (%llu entries) failed", (unsigned long long)cap);
+    }
+    if (g->pend_n) {
+        HIP_TRY(hipMemcpyAsync(nk, g->pk, g->pend_n * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(nv, g->pv, g->pend_n * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(nt, g->ptag, g->pend_n * 4, hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (g->pk) HIP_TRY(hipFree(g->pk));
+    if (g->pv) HIP_TRY(hipFree(g->pv));
+    if (g->ptag) HIP_TRY(hipFree(g->ptag));
+    g->pk = nk;
+    g->pv = nv;
+    g->ptag = nt;
+    g->pend_cap = cap;
+    return DYC_OK;
+}
+
+int wg_prologue(dyc_wide_group *g, hipStream_t s)
+{
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(g->ctr), kCounterWords, s));
+    HIP_TRY(launch_zero(g->tcounts, g->n, s));
+    return DYC_OK;
+}
+
+int wg_read_back(dyc_wide_group *g, hipStream_t s)
+{
+    HIP_TRY(hipMemcpyAsync(g->hctr, g->ctr, kCounterBytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(g->htcounts, g->tcounts, g->n * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    fold_tallies(g->hctr);
+    return DYC_OK;
+}
+
+// An erase leaves the pending entries it deleted as zero keys: drop them with their values and
+// tags (synchronous; the list is short whenever it is not empty).
+int wg_compact_stash(dyc_wide_group *g, hipStream_t s)
+{
+    if (g->pend_n == 0) return DYC_OK;
+    std::vector<uint64_t> k(g->pend_n), v(g->pend_n);
+    std::vector<uint32_t> tg(g->pend_n);
+    HIP_TRY(hipMemcpyAsync(k.data(), g->pk, g->pend_n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(v.data(), g->pv, g->pend_n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(tg.data(), g->ptag, g->pend_n * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < k.size(); ++i)
+        if (k[i]) {
+            k[m] = k[i];
+            v[m] = v[i];
+            tg[m] = tg[i];
+            ++m;
+        }
+    if (m != g->pend_n) {
+        if (m) {
+            HIP_TRY(hipMemcpyAsync(g->pk, k.data(), m * 8, hipMemcpyHostToDevice, s));
+            HIP_TRY(hipMemcpyAsync(g->pv, v.data(), m * 8, hipMemcpyHostToDevice, s));
+            HIP_TRY(hipMemcpyAsync(g->ptag, tg.data(), m * 4, hipMemcpyHostToDevice, s));
+        }
+        HIP_TRY(hipStreamSynchronize(s));
+        g->pend_n = m;
+    }
+    g->st.pending = g->pend_n;
+    return DYC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dyc_wide_group_destroy(dyc_wide_group *g)
+{
+    if (!g) return DYC_OK;
+    DeviceGuard dg(g->device);
+    (void)hipDeviceSynchronize();
+    if (g->arena) (void)hipFree(g->arena);
+    if (g->ddir) (void)hipFree(g->ddir);
+    if (g->ctr) (void)hipFree(g->ctr);
+    if (g->tcounts) (void)hipFree(g->tcounts);
+    if (g->hctr) (void)hipHostFree(g->hctr);
+    if (g->htcounts) (void)hipHostFree(g->htcounts);
+    if (g->pk) (void)hipFree(g->pk);
+    if (g->pv) (void)hipFree(g->pv);
+    if (g->ptag) (void)hipFree(g->ptag);
+    if (g->ditems) (void)hipFree(g->ditems);
+    if (g->hitems) (void)hipHostFree(g->hitems);
+    if (g->items_free) (void)hipEventDestroy(g->items_free);
+    delete g;
+    return DYC_OK;
+}
+
+int dyc_wide_group_create(const dyc_config *cfg_in, uint32_t n_tables, const uint64_t *hints, dyc_wide_group **out)
+{
+    if (!out || !hints || n_tables == 0) return fail(DYC_ERR_INVALID, "bad group arguments");
+    *out = nullptr;
+    dyc_config cfg;
+    if (cfg_in)
+        cfg = *cfg_in;
+    else
+        dyc_config_init(&cfg);
+    if (!wide_check_config(cfg)) return fail(DYC_ERR_INVALID, "bad configuration (block_threads, fill bounds or subtables)");
+    cfg.subtables = config_subtables(cfg);
+    const uint32_t d = cfg.subtables;
+    std::vector<uint32_t> nbs(n_tables);
+    size_t bytes = 0;
+    for (uint32_t i = 0; i < n_tables; ++i) {
+        nbs[i] = initial_buckets(hints[i], d);
+        if (nbs[i] >= kMaxBuckets) return fail(DYC_ERR_INVALID, "table %u: capacity hint too large", i);
+        bytes += (size_t)d * 2 * ((wide_bytes(nbs[i]) + 255) & ~(size_t)255);
+    }
+    dyc_wide_group *g = new dyc_wide_group();
+    g->cfg = cfg;
+    g->n = n_tables;
+    g->hdir.resize(n_tables);
+    g->count.assign(n_tables, 0);
+    auto cleanup = [&](int code) {
+        char keep[512];
+        strncpy(keep, dyc_last_error(), sizeof keep - 1);
+        keep[sizeof keep - 1] = 0;
+        dyc_wide_group_destroy(g);
+        fail(code, "%s", keep);
+        return code;
+    };
+    HIP_TRY(hipGetDevice(&g->device));
+    if (hipMalloc(&g->arena, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(DYC_ERR_OOM, "wide group arena of %zu bytes failed", bytes);
+        return cleanup(DYC_ERR_OOM);
+    }
+    g->arena_bytes = bytes;
+    size_t off = 0;
+    for (uint32_t i = 0; i < n_tables; ++i) {
+        desc_init(g->hdir[i].D, d);
+        for (uint32_t t = 0; t < d; ++t) {
+            const size_t a = (wide_bytes(nbs[i]) + 255) & ~(size_t)255;
+            desc_set(g->hdir[i].D, (int)t, reinterpret_cast<uint64_t *>(g->arena + off), nbs[i]);
+            g->hdir[i].val[t] = reinterpret_cast<uint64_t *>(g->arena + off + a);
+            off += 2 * a;
+        }
+    }
+    if (hipMalloc(&g->ddir, n_tables * sizeof(WideDesc)) != hipSuccess || hipMalloc(&g->ctr, kCounterBytes) != hipSuccess ||
+        hipMalloc(&g->tcounts, n_tables * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&g->hctr), kCounterBytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&g->htcounts), n_tables * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&g->items_free, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(DYC_ERR_OOM, "wide group bookkeeping allocation failed");
+        return cleanup(DYC_ERR_OOM);
+    }
+    if (hipMemcpy(g->ddir, g->hdir.data(), n_tables * sizeof(WideDesc), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_zero(reinterpret_cast<unsigned long long *>(g->arena), bytes / 8, nullptr) != hipSuccess ||
+        hipStreamSynchronize(nullptr) != hipSuccess) {
+        fail(DYC_ERR_HIP, "wide group initialisation failed");
+        return cleanup(DYC_ERR_HIP);
+    }
+    int rc = wg_reserve_pending(g, 65536, nullptr);
+    if (rc) return cleanup(rc);
+    *out = g;
+    return DYC_OK;
+}
+
+int dyc_wide_group_insert(dyc_wide_group *g, const uint64_t *keys, const uint64_t *vals, const uint64_t *offsets,
+                          dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = wg_check_offsets(g, offsets);
+    if (rc) return rc;
+    const uint64_t total = offsets[g->n];
+    if (total && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    rc = wg_reserve_pending(g, std::max<uint64_t>(65536, total / 16), s);
+    if (rc) return rc;
+    uint32_t n_items = 0;
+    rc = wg_upload_items(g, offsets, s, &n_items);
+    if (rc) return rc;
+    rc = wg_prologue(g, s);
+    if (rc) return rc;
+    const uint64_t room = g->pend_cap - g->pend_n;
+    HIP_TRY(launch_wide_group_insert(g->ddir, g->ditems, n_items, keys, vals, g->policy(), (int)g->cfg.insert_blocks,
+                                     g->ctr, WideSink{g->pk + g->pend_n, g->pv + g->pend_n, room, g->ptag + g->pend_n},
+                                     g->tcounts, s));
+    rc = wg_read_back(g, s);
+    if (rc) return rc;
+    const Counters &c = *g->hctr;
+    g->st.kicks += c.kicks;
+    g->st.last_kicks = c.kicks;
+    g->st.cas_retries += c.retries;
+    g->st.kick_failures += c.pending;
+    g->st.rejected += c.rejected;
+    g->st.lost += c.lost;
+    g->pend_n += std::min<uint64_t>(c.pending, room);
+    g->st.pending = g->pend_n;
+    for (uint32_t i = 0; i < g->n; ++i) g->count[i] += g->htcounts[i];
+    if (c.lost) return fail(DYC_ERR_LOST, "wide group pending list overflow: %llu entries lost", (unsigned long long)c.lost);
+    if (c.pending)
+        return fail(DYC_ERR_PENDING, "%llu entries exhausted the eviction bound (group tables do not resize)",
+                    (unsigned long long)c.pending);
+    return DYC_OK;
+}
+
+int dyc_wide_group_find(dyc_wide_group *g, const uint64_t *keys, const uint64_t *offsets, uint64_t *vals, uint8_t *hit,
+                        dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = wg_check_offsets(g, offsets);
+    if (rc) return rc;
+    if (offsets[g->n] && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t n_items = 0;
+    rc = wg_upload_items(g, offsets, s, &n_items);
+    if (rc) return rc;
+    HIP_TRY(launch_wide_group_probe(g->ddir, g->ditems, n_items, keys, vals, hit, false, nullptr, (int)g->cfg.max_blocks,
+                                    g->stash(), s));
+    return DYC_OK;
+}
+
+int dyc_wide_group_erase(dyc_wide_group *g, const uint64_t *keys, const uint64_t *offsets, uint8_t *deleted,
+                         dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = wg_check_offsets(g, offsets);
+    if (rc) return rc;
+    if (offsets[g->n] && !keys) return fail(DYC_ERR_INVALID, "null keys");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t n_items = 0;
+    rc = wg_upload_items(g, offsets, s, &n_items);
+    if (rc) return rc;
+    rc = wg_prologue(g, s);
+    if (rc) return rc;
+    HIP_TRY(launch_wide_group_probe(g->ddir, g->ditems, n_items, keys, nullptr, deleted, true, g->tcounts,
+                                    (int)g->cfg.max_blocks, g->stash(), s));
+    rc = wg_read_back(g, s);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < g->n; ++i) g->count[i] -= std::min<uint64_t>(g->count[i], g->htcounts[i]);
+    return wg_compact_stash(g, s);
+}
+
+int dyc_wide_group_table_stats(const dyc_wide_group *g, uint32_t table, dyc_stats *out)
+{
+    if (!g || !out || table >= g->n) return fail(DYC_ERR_INVALID, "bad argument");
+    *out = g->st;
+    out->count = g->count[table];
+    uint64_t cap = 0;
+    for (uint32_t t = 0; t < g->hdir[table].D.d; ++t) {
+        out->buckets[t] = g->hdir[table].D.nb[t];
+        cap += (uint64_t)g->hdir[table].D.nb[t] * kSlots;
+    }
+    out->capacity = cap;
+    out->subtables = g->hdir[table].D.d;
+    return DYC_OK;
+}
+
+int dyc_wide_group_dump(dyc_wide_group *g, uint32_t table, uint32_t sub, uint64_t *host_keys, uint64_t *host_vals,
+                        uint64_t n_slots, dyc_stream stream)
+{
+    if (!g || table >= g->n || sub >= g->hdir[table].D.d || (!host_keys && !host_vals))
+        return fail(DYC_ERR_INVALID, "bad argument");
+    const uint64_t slots = (uint64_t)g->hdir[table].D.nb[sub] * kSlots;
+    if (n_slots < slots) return fail(DYC_ERR_INVALID, "buffer too small");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (host_keys) HIP_TRY(hipMemcpyAsync(host_keys, g->hdir[table].D.sub[sub], slots * 8, hipMemcpyDeviceToHost, s));
+    if (host_vals) HIP_TRY(hipMemcpyAsync(host_vals, g->hdir[table].val[sub], slots * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return DYC_OK;
+}
+
+int dyc_wide_group_pending(dyc_wide_group *g, uint64_t *host_keys, uint64_t *host_vals, uint32_t *host_tables,
+                           uint64_t cap, uint64_t *n_out, dyc_stream stream)
+{
+    if (!g || !n_out) return fail(DYC_ERR_INVALID, "null argument");
+    *n_out = g->pend_n;
+    const uint64_t m = std::min(cap, g->pend_n);
+    if (m) {
+        DeviceGuard dg(g->device);
+        hipStream_t s = (hipStream_t)stream;
+        if (host_keys) HIP_TRY(hipMemcpyAsync(host_keys, g->pk, m * 8, hipMemcpyDeviceToHost, s));
+        if (host_vals) HIP_TRY(hipMemcpyAsync(host_vals, g->pv, m * 8, hipMemcpyDeviceToHost, s));
+        if (host_tables) HIP_TRY(hipMemcpyAsync(host_tables, g->ptag, m * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return DYC_OK;
+}
+
+int dyc_wide_group_clear(dyc_wide_group *g, dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    DeviceGuard dg(g->device);
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(g->arena), g->arena_bytes / 8, (hipStream_t)stream));
+    std::fill(g->count.begin(), g->count.end(), 0);
+    g->pend_n = 0;
+    g->st.pending = 0;
     return DYC_OK;
 }
 

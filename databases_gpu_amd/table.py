@@ -728,3 +728,103 @@ class GroupPlan:
             self.close()
         except Exception:
             pass
+
+
+class DyCuckooWideGroup:
+    """Many wide-key tables (uint64 -> uint64) served by one persistent launch per batch
+    (dyc_wide_group_*). A batch is a flat device tensor of int64 keys plus host offsets
+    (n_tables + 1 entries): table i's keys are keys[offsets[i]:offsets[i+1]]."""
+
+    def __init__(self, capacity_hints, *, device=None, **config):
+        self.lib = _lib.load()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.cfg = _lib.default_config(**config)
+        hints = np.ascontiguousarray(np.asarray(capacity_hints, dtype=np.uint64))
+        self.n = int(hints.size)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_group_create(ctypes.byref(self.cfg), self.n, hints.ctypes.data, ctypes.byref(h)),
+                  "dyc_wide_group_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dyc_wide_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _offsets(self, offsets) -> np.ndarray:
+        o = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+        if o.size != self.n + 1:
+            raise ValueError(f"need {self.n + 1} offsets, got {o.size}")
+        return o
+
+    def insert(self, keys, vals, offsets, stream=None) -> None:
+        k = _as_u64_tensor(keys, self.device)
+        v = _as_u64_tensor(vals, self.device)
+        o = self._offsets(offsets)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_group_insert(self.h, k.data_ptr(), v.data_ptr(), o.ctypes.data, _stream_ptr(stream)),
+                  "dyc_wide_group_insert")
+
+    def find(self, keys, offsets, vals=None, hit=None, stream=None):
+        k = _as_u64_tensor(keys, self.device)
+        o = self._offsets(offsets)
+        if vals is None:
+            vals = torch.empty(k.numel(), dtype=torch.int64, device=self.device)
+        if hit is None:
+            hit = torch.empty(k.numel(), dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_group_find(self.h, k.data_ptr(), o.ctypes.data, vals.data_ptr(), hit.data_ptr(),
+                                               _stream_ptr(stream)), "dyc_wide_group_find")
+        return vals, hit
+
+    def erase(self, keys, offsets, deleted=None, stream=None):
+        k = _as_u64_tensor(keys, self.device)
+        o = self._offsets(offsets)
+        if deleted is None:
+            deleted = torch.empty(k.numel(), dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_group_erase(self.h, k.data_ptr(), o.ctypes.data, deleted.data_ptr(),
+                                                _stream_ptr(stream)), "dyc_wide_group_erase")
+        return deleted
+
+    def clear(self, stream=None) -> None:
+        with torch.cuda.device(self.device):
+            check(self.lib.dyc_wide_group_clear(self.h, _stream_ptr(stream)), "dyc_wide_group_clear")
+
+    def table_stats(self, table: int) -> dict:
+        st = _lib.Stats()
+        check(self.lib.dyc_wide_group_table_stats(self.h, int(table), ctypes.byref(st)), "dyc_wide_group_table_stats")
+        return st.as_dict()
+
+    def dump(self, table: int):
+        """(key images, value images) of the table's subtables, host uint64 arrays."""
+        st = self.table_stats(table)
+        ks, vs = [], []
+        for sub, nb in enumerate(st["buckets"]):
+            k = np.zeros(nb * SLOTS, dtype=np.uint64)
+            v = np.zeros(nb * SLOTS, dtype=np.uint64)
+            check(self.lib.dyc_wide_group_dump(self.h, int(table), sub, k.ctypes.data, v.ctypes.data, k.size, None),
+                  "dyc_wide_group_dump")
+            ks.append(k)
+            vs.append(v)
+        return ks, vs
+
+    def pending(self):
+        """(keys, values, tables) of the entries waiting outside their tables' buckets."""
+        n = ctypes.c_uint64()
+        check(self.lib.dyc_wide_group_pending(self.h, None, None, None, 0, ctypes.byref(n), None),
+              "dyc_wide_group_pending")
+        k = np.zeros(n.value, dtype=np.uint64)
+        v = np.zeros(n.value, dtype=np.uint64)
+        t = np.zeros(n.value, dtype=np.uint32)
+        if n.value:
+            check(self.lib.dyc_wide_group_pending(self.h, k.ctypes.data, v.ctypes.data, t.ctypes.data, n.value,
+                                                  ctypes.byref(n), None), "dyc_wide_group_pending")
+        return k, v, t

@@ -494,6 +494,31 @@ int dyc_wide_load(dyc_wide *t, const uint32_t *nb, const uint64_t *const *key_im
 int dyc_wide_pending(dyc_wide *t, uint64_t *host_keys, uint64_t *host_vals, uint64_t cap, uint64_t *n_out,
                      dyc_stream stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Many wide-key tables in one launch: the dyc_group_* calls with uint64 keys and values
+ * (tables of the dyc_wide_* layout). No counterpart in the reference (one table per process,
+ * 32-bit keys). Keys of table i are keys[offsets[i] .. offsets[i+1]) (host offsets, n_tables + 1
+ * entries). Tables have fixed sizes; entries that exhaust the eviction bound stay pending,
+ * tagged with their table, and that table's finds and erases search them (the insert returns
+ * DYC_ERR_PENDING). cfg gives the eviction rule, subtable count, insert_blocks (insert grid) and
+ * max_blocks (find / erase grid; 0 = default). Keys 0 and 2^64 - 1 are reserved. */
+typedef struct dyc_wide_group dyc_wide_group; /* opaque */
+int dyc_wide_group_create(const dyc_config *cfg, uint32_t n_tables, const uint64_t *capacity_hints,
+                          dyc_wide_group **out);
+int dyc_wide_group_destroy(dyc_wide_group *g);
+int dyc_wide_group_insert(dyc_wide_group *g, const uint64_t *keys, const uint64_t *vals, const uint64_t *offsets,
+                          dyc_stream stream);
+int dyc_wide_group_find(dyc_wide_group *g, const uint64_t *keys, const uint64_t *offsets, uint64_t *vals,
+                        uint8_t *hit, dyc_stream stream);
+int dyc_wide_group_erase(dyc_wide_group *g, const uint64_t *keys, const uint64_t *offsets, uint8_t *deleted,
+                         dyc_stream stream);
+int dyc_wide_group_table_stats(const dyc_wide_group *g, uint32_t table, dyc_stats *out);
+int dyc_wide_group_dump(dyc_wide_group *g, uint32_t table, uint32_t sub, uint64_t *host_keys, uint64_t *host_vals,
+                        uint64_t n_slots, dyc_stream stream);
+int dyc_wide_group_pending(dyc_wide_group *g, uint64_t *host_keys, uint64_t *host_vals, uint32_t *host_tables,
+                           uint64_t cap, uint64_t *n_out, dyc_stream stream);
+int dyc_wide_group_clear(dyc_wide_group *g, dyc_stream stream);
+
 #ifdef __cplusplus
 }
 #endif
