@@ -103,6 +103,8 @@ def parse():
     ap.add_argument("--bt-log2max", type=int, default=20, help="batched: log2 of the largest table")
     ap.add_argument("--bt-theta", type=float, default=0.85, help="batched: per-table load factor")
     ap.add_argument("--bt-hit", type=float, default=0.6, help="batched: find hit rate")
+    ap.add_argument("--bt-wide", action="store_true",
+                    help="batched: uint64 keys and values (wide-key groups; no autotuning or graph form)")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--kick-bound", type=int, default=None,
                     help="bulk: eviction bound of the table (with --no-tune it stays fixed; default: the library's 15, "
@@ -844,6 +846,10 @@ def run_batched(args, dev) -> int:
     log(f"batched: {args.bt_tables} tables, {total} keys (2^{math.log2(total):.2f}), load factor {args.bt_theta}")
     keys = W.keys_range(0, total, args.seed, dev)
     vals = W.splitmix32(keys)
+    if args.bt_wide:  # uint64 keys: the distinct low words above, high words of a second sequence
+        hi = W.keys_range(total, 2 * total, args.seed + 1, dev)
+        keys = (hi.to(torch.int64) << 32) | (keys.to(torch.int64) & 0xFFFFFFFF)
+        vals = keys * 0x9E3779B1 + 12345
     hit_n = np.round(args.bt_hit * sizes).astype(np.int64)
     mark = torch.zeros(total + 1, dtype=torch.int32, device=dev)
     ones = torch.ones(args.bt_tables, dtype=torch.int32, device=dev)
@@ -851,14 +857,17 @@ def run_batched(args, dev) -> int:
     mark.index_add_(0, torch.from_numpy(offs[1:].astype(np.int64)).to(dev), -ones)
     absent = torch.cumsum(mark, 0, dtype=torch.int32)[:total].bool()
     del mark
-    lookups = torch.where(absent, W.keys_range(total, 2 * total, args.seed, dev), keys)
-    out_v = torch.empty(total, dtype=torch.int32, device=dev)
+    miss = W.keys_range(total, 2 * total, args.seed, dev)
+    if args.bt_wide:
+        miss = (miss.to(torch.int64) << 32) | 0x5A5A5A5A  # high words no stored key has with this low word
+    lookups = torch.where(absent, miss, keys)
+    out_v = torch.empty(total, dtype=keys.dtype, device=dev)
     out_h = torch.empty(total, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
     hints = (np.ceil(sizes / args.bt_theta) + 64).astype(np.uint64)
-    g = D.DyCuckooGroup(hints)
+    g = D.DyCuckooWideGroup(hints) if args.bt_wide else D.DyCuckooGroup(hints)
     tuned = None
-    if not args.no_tune:
+    if not args.no_tune and not args.bt_wide:
         log("autotuning launch shapes on the batch")
         tuned = autotune.tune_group(g, keys, vals, offs, lookups, offs, log=None)
         log(f"autotune: {tuned['installed']} ({tuned.get('trials', 0)} trials, {tuned['seconds']} s)")
@@ -884,17 +893,19 @@ def run_batched(args, dev) -> int:
         return ins, fnd, tot, wall
 
     ins_p, fnd_p, tot_p, wall_p = measure(False)
-    # the same step (clear, insert, find) captured once into a HIP graph and replayed
-    plan = g.plan(keys, vals, offs, lookups, offs, out_v, out_h)
-    wall_g = []
-    for s in range(args.warmup + args.steps):
-        t0 = time.perf_counter()
-        plan.run()
-        if s >= args.warmup:
-            wall_g.append((time.perf_counter() - t0) * 1e3)
-    ok_graph = bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
-    plan.close()
-    log(f"step wall time: eager calls {np.median(wall_p):.3f} ms, graph replay {np.median(wall_g):.3f} ms")
+    ok_graph, wall_g = True, [float("nan")]
+    if not args.bt_wide:
+        # the same step (clear, insert, find) captured once into a HIP graph and replayed
+        plan = g.plan(keys, vals, offs, lookups, offs, out_v, out_h)
+        wall_g = []
+        for s in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            plan.run()
+            if s >= args.warmup:
+                wall_g.append((time.perf_counter() - t0) * 1e3)
+        ok_graph = bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
+        plan.close()
+        log(f"step wall time: eager calls {np.median(wall_p):.3f} ms, graph replay {np.median(wall_g):.3f} ms")
     ok = bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
     log(f"persistent: insert {np.median(ins_p):.2f} ms, find {np.median(fnd_p):.2f} ms")
     ins_t, fnd_t, tot_t, _ = measure(True)
@@ -914,15 +925,18 @@ def run_batched(args, dev) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
+        "dtype": "u64" if args.bt_wide else "u32",
         "data": "synthetic",
         "config": {"workload": f"{args.bt_tables} tables, key counts log-uniform in [2^{args.bt_log2min}, "
-                               f"2^{args.bt_log2max}], load factor {args.bt_theta}, finds {args.bt_hit:.0%} hit",
-                   "total_keys": total, "tables": args.bt_tables, "launch": g.launch},
+                               f"2^{args.bt_log2max}], load factor {args.bt_theta}, finds {args.bt_hit:.0%} hit"
+                               + (", uint64 keys and values" if args.bt_wide else ""),
+                   "total_keys": total, "tables": args.bt_tables,
+                   "launch": None if args.bt_wide else g.launch},
         "persistent": {"insert_ms": round(float(np.median(ins_p)), 3), "find_ms": round(float(np.median(fnd_p)), 3),
                        "p50_ms": round(pct(tot_p, 50), 3), "p99_ms": round(pct(tot_p, 99), 3),
                        "mops": round(ops / (np.median(tot_p) * 1e-3) / 1e6, 1)},
-        "step_wall_ms": {"eager": round(float(np.median(wall_p)), 3), "graph": round(float(np.median(wall_g)), 3),
+        "step_wall_ms": {"eager": round(float(np.median(wall_p)), 3),
+                         "graph": None if args.bt_wide else round(float(np.median(wall_g)), 3),
                          "note": "clear + insert + find incl. host work and the final wait; graph = one replay of "
                                  "the captured step (dyc_group_plan_run)"},
         "per_table": {"insert_ms": round(float(np.median(ins_t)), 3), "find_ms": round(float(np.median(fnd_t)), 3),

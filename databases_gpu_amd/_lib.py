@@ -48,7 +48,7 @@ EXPORTS = (
     "dyc_insert_counted", "dyc_group_plan_create", "dyc_group_plan_run", "dyc_group_plan_destroy",
     "dyc_wide_group_create", "dyc_wide_group_destroy", "dyc_wide_group_insert", "dyc_wide_group_find",
     "dyc_wide_group_erase", "dyc_wide_group_table_stats", "dyc_wide_group_dump", "dyc_wide_group_pending",
-    "dyc_wide_group_clear",
+    "dyc_wide_group_clear", "dyc_wide_group_insert_per_table", "dyc_wide_group_find_per_table",
 )
 
 OP_INSERT, OP_FIND, OP_ERASE = 0, 1, 2
@@ -209,6 +209,8 @@ def load(require: bool = True):
         "dyc_wide_group_dump": (ci, [vp, u32, u32, vp, vp, u64, vp]),
         "dyc_wide_group_pending": (ci, [vp, vp, vp, vp, u64, ctypes.POINTER(u64), vp]),
         "dyc_wide_group_clear": (ci, [vp, vp]),
+        "dyc_wide_group_insert_per_table": (ci, [vp, vp, vp, vp, vp]),
+        "dyc_wide_group_find_per_table": (ci, [vp, vp, vp, vp, vp, vp]),
         "dyc_sync": (ci, [vp]),
         "dyc_find": (ci, [vp, vp, vp, vp, u64, vp]),
         "dyc_erase": (ci, [vp, vp, vp, u64, vp]),

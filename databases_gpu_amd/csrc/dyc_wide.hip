@@ -46,11 +46,15 @@ struct WideDesc {
 };
 
 // Where an insert parks entries that exhaust the eviction bound (append index Counters::pending).
-// tags (optional, groups): the table index of each parked entry.
+// Groups: tags (optional) receives the table index of each parked entry (`tag` for a
+// single-table launch, the item's table for a group launch), and rejected_at (optional) counts
+// the launch's reserved keys.
 struct WideSink {
     uint64_t *keys, *vals;
     uint64_t cap;
     uint32_t *tags = nullptr;
+    uint32_t tag = 0;
+    unsigned long long *rejected_at = nullptr;
 };
 
 // The pending entries searched after pos2 by finds, erases and updates (n = 0: none). tags
@@ -373,7 +377,11 @@ __global__ __launch_bounds__(256) void wide_insert_kernel(WideDesc W, const uint
             wide_take(c, W.D, k, v, FROM_ENTRIES ? pre_fixed : kNoTable);
         }
         if (!__any(c.active)) break;
-        if (c.active) wide_step(c, W, P, C, sink, 0, kicks, retries);
+        if (c.active) wide_step(c, W, P, C, sink, sink.tag, kicks, retries);
+    }
+    if (sink.rejected_at) {
+        const unsigned long long rej = wave_sum(rejected);
+        if (__lane_id() == 0 && rej) atomicAdd(sink.rejected_at, rej);
     }
     wave_tally(C, &C->kicks, kicks);
     wave_tally(C, &C->rejected, rejected);
@@ -2073,6 +2081,73 @@ int dyc_wide_group_erase(dyc_wide_group *g, const uint64_t *keys, const uint64_t
     if (rc) return rc;
     for (uint32_t i = 0; i < g->n; ++i) g->count[i] -= std::min<uint64_t>(g->count[i], g->htcounts[i]);
     return wg_compact_stash(g, s);
+}
+
+// The one-launch-per-table forms (the baseline the group launch is measured against): the
+// single-table kernels once per table, on the group's own tables and pending list.
+int dyc_wide_group_insert_per_table(dyc_wide_group *g, const uint64_t *keys, const uint64_t *vals,
+                                    const uint64_t *offsets, dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = wg_check_offsets(g, offsets);
+    if (rc) return rc;
+    if (offsets[g->n] && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    rc = wg_reserve_pending(g, std::max<uint64_t>(65536, offsets[g->n] / 16), s);
+    if (rc) return rc;
+    rc = wg_prologue(g, s);
+    if (rc) return rc;
+    const uint64_t room = g->pend_cap - g->pend_n;
+    const LaunchShape sh{256, (int)g->cfg.insert_blocks};
+    // one launch per table; all share the counters, so pended entries append one after another,
+    // each tagged with its launch's table; tcounts[i] receives table i's reserved keys
+    for (uint32_t i = 0; i < g->n; ++i) {
+        const uint64_t m = offsets[i + 1] - offsets[i];
+        if (!m) continue;
+        HIP_TRY(launch_wide_insert(g->hdir[i], keys + offsets[i], vals + offsets[i], m, kNoTable, false, g->policy(),
+                                   sh, g->ctr,
+                                   WideSink{g->pk + g->pend_n, g->pv + g->pend_n, room, g->ptag + g->pend_n, i,
+                                            g->tcounts + i},
+                                   s));
+    }
+    rc = wg_read_back(g, s);
+    if (rc) return rc;
+    const Counters &c = *g->hctr;
+    g->st.kicks += c.kicks;
+    g->st.last_kicks = c.kicks;
+    g->st.cas_retries += c.retries;
+    g->st.kick_failures += c.pending;
+    g->st.rejected += c.rejected;
+    g->st.lost += c.lost;
+    g->pend_n += std::min<uint64_t>(c.pending, room);
+    g->st.pending = g->pend_n;
+    for (uint32_t i = 0; i < g->n; ++i) g->count[i] += (offsets[i + 1] - offsets[i]) - g->htcounts[i];
+    if (c.lost) return fail(DYC_ERR_LOST, "wide group pending list overflow: %llu entries lost", (unsigned long long)c.lost);
+    if (c.pending)
+        return fail(DYC_ERR_PENDING, "%llu entries exhausted the eviction bound (group tables do not resize)",
+                    (unsigned long long)c.pending);
+    return DYC_OK;
+}
+
+int dyc_wide_group_find_per_table(dyc_wide_group *g, const uint64_t *keys, const uint64_t *offsets, uint64_t *vals,
+                                  uint8_t *hit, dyc_stream stream)
+{
+    if (!g) return fail(DYC_ERR_INVALID, "null group");
+    int rc = wg_check_offsets(g, offsets);
+    if (rc) return rc;
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    const LaunchShape sh{256, (int)g->cfg.max_blocks};
+    for (uint32_t i = 0; i < g->n; ++i) {
+        const uint64_t m = offsets[i + 1] - offsets[i];
+        if (!m) continue;
+        WideStash st = g->stash();
+        st.table = i;
+        HIP_TRY(launch_wide_probe(g->hdir[i], keys + offsets[i], vals ? vals + offsets[i] : nullptr,
+                                  hit ? hit + offsets[i] : nullptr, m, false, sh, g->ctr, st, s));
+    }
+    return DYC_OK;
 }
 
 int dyc_wide_group_table_stats(const dyc_wide_group *g, uint32_t table, dyc_stats *out)

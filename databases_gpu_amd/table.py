@@ -764,24 +764,25 @@ class DyCuckooWideGroup:
             raise ValueError(f"need {self.n + 1} offsets, got {o.size}")
         return o
 
-    def insert(self, keys, vals, offsets, stream=None) -> None:
+    def insert(self, keys, vals, offsets, per_table: bool = False, stream=None) -> None:
         k = _as_u64_tensor(keys, self.device)
         v = _as_u64_tensor(vals, self.device)
         o = self._offsets(offsets)
+        fn = self.lib.dyc_wide_group_insert_per_table if per_table else self.lib.dyc_wide_group_insert
         with torch.cuda.device(self.device):
-            check(self.lib.dyc_wide_group_insert(self.h, k.data_ptr(), v.data_ptr(), o.ctypes.data, _stream_ptr(stream)),
-                  "dyc_wide_group_insert")
+            check(fn(self.h, k.data_ptr(), v.data_ptr(), o.ctypes.data, _stream_ptr(stream)), "dyc_wide_group_insert")
 
-    def find(self, keys, offsets, vals=None, hit=None, stream=None):
+    def find(self, keys, offsets, vals=None, hit=None, per_table: bool = False, stream=None):
         k = _as_u64_tensor(keys, self.device)
         o = self._offsets(offsets)
         if vals is None:
             vals = torch.empty(k.numel(), dtype=torch.int64, device=self.device)
         if hit is None:
             hit = torch.empty(k.numel(), dtype=torch.uint8, device=self.device)
+        fn = self.lib.dyc_wide_group_find_per_table if per_table else self.lib.dyc_wide_group_find
         with torch.cuda.device(self.device):
-            check(self.lib.dyc_wide_group_find(self.h, k.data_ptr(), o.ctypes.data, vals.data_ptr(), hit.data_ptr(),
-                                               _stream_ptr(stream)), "dyc_wide_group_find")
+            check(fn(self.h, k.data_ptr(), o.ctypes.data, vals.data_ptr(), hit.data_ptr(), _stream_ptr(stream)),
+                  "dyc_wide_group_find")
         return vals, hit
 
     def erase(self, keys, offsets, deleted=None, stream=None):

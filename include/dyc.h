@@ -512,6 +512,11 @@ int dyc_wide_group_find(dyc_wide_group *g, const uint64_t *keys, const uint64_t 
                         uint8_t *hit, dyc_stream stream);
 int dyc_wide_group_erase(dyc_wide_group *g, const uint64_t *keys, const uint64_t *offsets, uint8_t *deleted,
                          dyc_stream stream);
+/* The same batches with one launch of the single-table kernels per table (the baseline). */
+int dyc_wide_group_insert_per_table(dyc_wide_group *g, const uint64_t *keys, const uint64_t *vals,
+                                    const uint64_t *offsets, dyc_stream stream);
+int dyc_wide_group_find_per_table(dyc_wide_group *g, const uint64_t *keys, const uint64_t *offsets, uint64_t *vals,
+                                  uint8_t *hit, dyc_stream stream);
 int dyc_wide_group_table_stats(const dyc_wide_group *g, uint32_t table, dyc_stats *out);
 int dyc_wide_group_dump(dyc_wide_group *g, uint32_t table, uint32_t sub, uint64_t *host_keys, uint64_t *host_vals,
                         uint64_t n_slots, dyc_stream stream);

@@ -52,9 +52,9 @@ def oracle_of(g, i):
     return ref
 
 
-@pytest.mark.parametrize("d", (2, 4))
-def test_wide_group_insert_find_match_oracle_per_table(D, d):
-    banner(f"wide group insert/find, subtables={d}")
+@pytest.mark.parametrize("d,per_table", [(2, False), (4, False), (4, True)])
+def test_wide_group_insert_find_match_oracle_per_table(D, d, per_table):
+    banner(f"wide group insert/find, subtables={d}, per_table={per_table}")
     sizes = sizes_loguniform(29, 2**6, 2**13, 40 + d)
     sizes[3] = 0
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
@@ -66,7 +66,7 @@ def test_wide_group_insert_find_match_oracle_per_table(D, d):
     vals = vals_of(keys)
     hints = (np.ceil(np.maximum(sizes, 1) / 0.85) + 64).astype(np.uint64)
     g = D.DyCuckooWideGroup(hints, subtables=d)
-    g.insert(cu(keys), cu(vals), offs)
+    g.insert(cu(keys), cu(vals), offs, per_table=per_table)
     # lookups: per table its own keys, then half as many of the next table's keys (absent here)
     q_parts, q_offs = [], [0]
     for i in range(len(sizes)):
@@ -77,7 +77,7 @@ def test_wide_group_insert_find_match_oracle_per_table(D, d):
         q_parts.append(q)
         q_offs.append(q_offs[-1] + q.size)
     q = np.concatenate(q_parts)
-    v, h = g.find(cu(q), np.array(q_offs, dtype=np.uint64))
+    v, h = g.find(cu(q), np.array(q_offs, dtype=np.uint64), per_table=per_table)
     v, h = v.cpu().numpy().view(np.uint64), h.cpu().numpy()
     for i in range(len(sizes)):
         st = g.table_stats(i)
@@ -95,8 +95,9 @@ def test_wide_group_insert_find_match_oracle_per_table(D, d):
     g.close()
 
 
-def test_wide_group_erase_and_reserved_keys(D):
-    banner("wide group erase + reserved keys")
+@pytest.mark.parametrize("per_table", [False, True])
+def test_wide_group_erase_and_reserved_keys(D, per_table):
+    banner(f"wide group erase + reserved keys, per_table={per_table}")
     sizes = sizes_loguniform(11, 2**8, 2**12, 50)
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
     keys = keys64(int(offs[-1]), 51)
@@ -104,7 +105,7 @@ def test_wide_group_erase_and_reserved_keys(D):
     vals = vals_of(keys)
     hints = (np.ceil(sizes / 0.85) + 64).astype(np.uint64)
     g = D.DyCuckooWideGroup(hints)
-    g.insert(cu(keys), cu(vals), offs)
+    g.insert(cu(keys), cu(vals), offs, per_table=per_table)
     assert g.table_stats(2)["count"] == sizes[2] - 1 and g.table_stats(3)["count"] == sizes[3]
     e_parts, e_offs = [], [0]
     for i in range(len(sizes)):
@@ -126,25 +127,27 @@ def test_wide_group_erase_and_reserved_keys(D):
     g.close()
 
 
-def test_wide_group_stash_serves_pending_entries_of_their_table(D):
-    """40 copies of one key into table 1 (tables do not resize): 8 stay pending, tagged 1. After
-    two erases clear the key's buckets, table 1's find still reaches a stash copy, table 0 does
-    not see it, and one more erase clears the stash."""
-    banner("wide group stash")
+@pytest.mark.parametrize("per_table", [False, True])
+def test_wide_group_stash_serves_pending_entries_of_their_table(D, per_table):
+    """40 copies of one key into table 1 (tables do not resize): 8 stay pending, tagged 1 (also
+    when the single-table kernel parked them). After two erases clear the key's buckets, table 1's
+    find still reaches a stash copy, table 0 does not see it, and one more erase clears the stash."""
+    banner(f"wide group stash, per_table={per_table}")
     g = D.DyCuckooWideGroup([1000, 100, 1000])
     key = np.uint64(0x0123456789ABCDEF)
     offs = np.array([0, 0, 40, 40], dtype=np.uint64)
     vals = np.arange(500, 540, dtype=np.uint64)
     with pytest.raises(D.DycError) as ei:
-        g.insert(cu(np.full(40, key, np.uint64)), cu(vals), offs)
+        g.insert(cu(np.full(40, key, np.uint64)), cu(vals), offs, per_table=per_table)
     assert ei.value.code == 4  # DYC_ERR_PENDING
+    assert g.table_stats(1)["count"] == 40
     pk, pv, pt = g.pending()
     assert pk.size == 8 and np.all(pt == 1) and np.all(pk == key)
     one = np.array([0, 0, 1, 1], dtype=np.uint64)
     for _ in range(2):
         assert g.erase(cu(np.array([key])), one).cpu().tolist() == [1]
     two = np.array([0, 1, 2, 2], dtype=np.uint64)
-    v, h = g.find(cu(np.array([key, key])), two)
+    v, h = g.find(cu(np.array([key, key])), two, per_table=per_table)
     assert h.cpu().tolist() == [0, 1] and int(v.cpu().numpy().view(np.uint64)[1]) in set(pv.tolist())
     assert g.erase(cu(np.array([key])), one).cpu().tolist() == [1]
     pk, _, _ = g.pending()
