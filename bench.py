@@ -61,7 +61,8 @@ def parse():
                     help="bulk: dyc_config.insert_order (3 = bulk fill, the default; 0 = eviction insert in input order)")
     ap.add_argument("--insert-blocks", type=int, default=0, help="insert grid (0 = library default)")
     ap.add_argument("--max-blocks", type=int, default=0, help="find grid cap (0 = library default)")
-    ap.add_argument("--cpu-log2n", type=int, default=26, help="CPU baseline sample (multithreaded): 2^k keys")
+    ap.add_argument("--cpu-log2n", type=int, default=28,
+                    help="CPU baseline (multithreaded): 2^k keys (default: the GPU workload's full size)")
     ap.add_argument("--cpu-seq-log2n", type=int, default=22, help="sequential CPU sample: 2^k keys")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--json-out", default=None)
@@ -193,9 +194,9 @@ def pct(xs, q):
 
 
 def cpu_baseline(log2n: int, theta: float, hit: float, zipf_s: float, seed: int, seq_log2n: int) -> dict:
-    """The same workload on the host: multithreaded DyCuckoo (OpenMP over keys, 64-bit CAS
-    exchanges) on 2^log2n keys over every thread OpenMP is given, plus the sequential replay on
-    2^seq_log2n keys for information."""
+    """The same workload on the host: multithreaded DyCuckoo (POSIX threads over key chunks,
+    64-bit CAS exchanges) on 2^log2n keys over every host thread the box gives (OMP_NUM_THREADS),
+    plus the sequential replay on 2^seq_log2n keys for information."""
     import oracle as O
     from databases_gpu_amd import workload as W
 
@@ -239,7 +240,7 @@ def cpu_baseline(log2n: int, theta: float, hit: float, zipf_s: float, seed: int,
         "cores": threads,
         "kind": "port",
         "sample": f"2^{log2n} keys at load factor {theta}: multithreaded insert + 2^{log2n} uniform finds "
-                  f"({hit:.0%} hit) + 2^{log2n} Zipf({zipf_s}) finds, {threads} OpenMP threads",
+                  f"({hit:.0%} hit) + 2^{log2n} Zipf({zipf_s}) finds, {threads} POSIX threads",
         "insert_mops": round(n / (t1 - t0) / 1e6, 3),
         "find_uniform_mops": round(n / (t2 - t1) / 1e6, 3),
         "find_zipf_mops": round(n / (t3 - t2) / 1e6, 3),
