@@ -1184,10 +1184,11 @@ __global__ __launch_bounds__(256) void serve_tick_kernel(const Desc *__restrict_
 // whose earlier requests have not finished waits until tbl_done[table] reaches its need. Items
 // wait only on items with smaller indexes, which running waves have claimed (the grid is
 // resident as a whole), so the earliest unfinished item never waits and every published item
-// finishes. A finished item's results are system-scope stores and its table writes atomics: it
-// waits for them to complete, then counts itself into its request's progress word; the wave
-// that counts a request's last item advances tbl_done, writes the request's record and appends
-// the request to the completion queue in host memory.
+// finishes. A finished item's table writes are atomics whose results it has used: it counts
+// itself into its request's progress word at once, and the wave that counts the request's last
+// item advances tbl_done. Its results are system-scope stores: it waits for them, then counts
+// itself into the request's output count, and the wave that counts the last item writes the
+// request's record and appends the request to the completion queue in host memory.
 // Every wait is bounded. The front wave ends the launch on the host's stop flag or after
 // idle_ticks without new items: it raises quit, and waves waiting for an index past the
 // published ones leave. An item gives up waiting for its table after spin_cap polls, or at once
@@ -1324,6 +1325,9 @@ __global__ __launch_bounds__(256) void serve_loop_kernel(LoopArgs A, InsertPolic
             break;
         }
         const uint32_t table = it.item.table;
+        // the descriptor can be read before the wait: a table is resized only while none of its
+        // requests is in flight, and the host writes the directory before it publishes items
+        const Desc D = load_desc_coherent(A.dir + table);
         uint32_t stalled = 0;
         if (it.need) {
             uint32_t back = 1, spins = 0;
@@ -1336,7 +1340,6 @@ __global__ __launch_bounds__(256) void serve_loop_kernel(LoopArgs A, InsertPolic
                 back = back < A.max_back ? back * 2 : A.max_back;
             }
         }
-        const Desc D = load_desc_coherent(A.dir + table);
         unsigned long long acc;
         if (it.op == 0) {
             acc = serve_insert_item<GI, true>(D, it.item, c, P, C, sink, tally);
@@ -1350,18 +1353,26 @@ __global__ __launch_bounds__(256) void serve_loop_kernel(LoopArgs A, InsertPolic
                 del.add(del_total, table, acc);
             }
         }
-        if (lane == 0 && stalled) atomicAdd(A.req_stall + it.req, 1u);
-        // the item's results (system-scope stores) and table writes (atomics) are complete
-        // before it counts as done
-        publish_wait();
+        // Two counts per request. The table's work of the item is done here (its exchanges and
+        // erasures are atomics whose results the wave has used; a parked entry was fenced), so
+        // the item counts at once towards releasing the table's next requests. Its results are
+        // system-scope stores, which it waits for before counting towards the host's completion.
         if (lane == 0) {
             const unsigned long long old = atomicAdd(A.req_word + it.req, (acc << kLoopItemBits) | 1ull);
-            if ((old & ((1ull << kLoopItemBits) - 1)) + 1 == it.n_items) {
-                const unsigned long long total = (old >> kLoopItemBits) + acc;
+            if ((old & ((1ull << kLoopItemBits) - 1)) + 1 == it.n_items)
+                atomicAdd(A.tbl_done + table, 1ull);  // the table's next requests may start
+            if (stalled) atomicAdd(A.req_stall + it.req, 1u);
+        }
+        publish_wait();
+        if (lane == 0) {
+            const uint32_t done = atomicAdd(A.req_out + it.req, 1u) + 1;
+            if (done == it.n_items) {
+                // every item counted into req_word before it counted here
+                const unsigned long long total = ld_agent(A.req_word + it.req) >> kLoopItemBits;
                 const uint32_t stalls = __hip_atomic_load(A.req_stall + it.req, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(A.req_word + it.req, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(A.req_out + it.req, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(A.req_stall + it.req, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                atomicAdd(A.tbl_done + table, 1ull);  // the table's next requests may start
                 LoopRec *r = A.rec + it.req;
                 __hip_atomic_store(&r->acc, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 __hip_atomic_store(&r->pool, pool.count ? ld_agent(pool.count) : 0ull, __ATOMIC_RELAXED,

@@ -308,7 +308,8 @@ struct LoopArgs {
     uint32_t spin_cap;     // polls an item waits for its table's earlier requests
     unsigned long long *words;     // kLoopWords, device
     unsigned long long *tbl_done;  // requests finished per table, device
-    unsigned long long *req_word;  // per request slot, device, 0 when free
+    unsigned long long *req_word;  // per request slot, device, 0 when free: items whose table work is done
+    uint32_t *req_out;             // per request slot, device, 0 when free: items whose results are written
     uint32_t *req_stall;           // per request slot, device, 0 when free
     uint64_t idle_ticks;   // the launch ends after this long without new items (100 MHz clock)
     uint32_t max_back;     // a waiting wave sleeps up to max_back x 256 cycles between polls

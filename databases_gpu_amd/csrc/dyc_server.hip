@@ -136,7 +136,7 @@ struct Loop {
     LoopRec *rec = nullptr;
     unsigned long long *cq = nullptr;
     unsigned long long *d_words = nullptr, *d_tbl_done = nullptr, *d_req_word = nullptr;
-    uint32_t *d_req_stall = nullptr;
+    uint32_t *d_req_out = nullptr, *d_req_stall = nullptr;
     int blocks = 0;
     uint64_t tail = 0;     // items published
     uint64_t cq_head = 0;  // completion entries read
@@ -957,7 +957,7 @@ int loop_alloc(dyc_server *s)
     memset(L.ring, 0, kLoopRingItems * sizeof(LoopItem));
     memset(L.rec, 0, kLoopSlots * sizeof(LoopRec));
     memset(L.cq, 0, kLoopSlots * sizeof(unsigned long long));
-    const size_t dev_bytes = (kLoopWords + (size_t)s->n + kLoopSlots) * 8 + (size_t)kLoopSlots * 4;
+    const size_t dev_bytes = (kLoopWords + (size_t)s->n + kLoopSlots) * 8 + (size_t)kLoopSlots * 8;
     unsigned long long *d = nullptr;
     if (hipMalloc((void **)&d, dev_bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -966,7 +966,8 @@ int loop_alloc(dyc_server *s)
     L.d_words = d;
     L.d_tbl_done = d + kLoopWords;
     L.d_req_word = L.d_tbl_done + s->n;
-    L.d_req_stall = reinterpret_cast<uint32_t *>(L.d_req_word + kLoopSlots);
+    L.d_req_out = reinterpret_cast<uint32_t *>(L.d_req_word + kLoopSlots);
+    L.d_req_stall = L.d_req_out + kLoopSlots;
     HIP_TRY(launch_zero(d, dev_bytes / 8, L.st2));
     HIP_TRY(hipStreamSynchronize(L.st2));
     L.slot.assign(kLoopSlots, LoopSlot{});
@@ -1009,6 +1010,7 @@ int loop_launch(dyc_server *s)
     A.words = L.d_words;
     A.tbl_done = L.d_tbl_done;
     A.req_word = L.d_req_word;
+    A.req_out = L.d_req_out;
     A.req_stall = L.d_req_stall;
     A.idle_ticks = (uint64_t)std::max<uint32_t>(1, s->scfg.resident_idle_ms) * 100000ull;  // 100 MHz clock
     A.max_back = std::max<uint32_t>(1, s->scfg.poll_backoff);
