@@ -936,40 +936,58 @@ int loop_upload_dir(dyc_server *s)
     return DYC_OK;
 }
 
+void loop_free(dyc_server *s);
+
+// Everything the loop needs, allocated on first use; on any failure nothing is kept.
 int loop_alloc(dyc_server *s)
 {
     Loop &L = s->loop;
     if (L.ctl) return DYC_OK;
     const int cap = serve_loop_capacity((int)s->cfg.insert_lanes, (int)s->cfg.find_lanes);
     if (cap < 2) return fail(DYC_ERR_INVALID, "the resident loop needs insert_lanes 1, 2 or 4 and find_lanes 4, 8 or 16");
+    auto undo = [&](int code) {
+        loop_free(s);
+        return code;
+    };
     // default: half the resident capacity, so resizes on the second stream (and the caller's own
     // kernels) find room beside the loop instead of waiting for it to end
-    L.blocks = s->scfg.resident_blocks ? std::min<int>((int)s->scfg.resident_blocks, cap) : cap / 2;
-    if (L.blocks < 2) L.blocks = 2;
-    HIP_TRY(hipStreamCreateWithFlags(&L.st2, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&L.ended, hipEventDisableTiming));
+    const int blocks = s->scfg.resident_blocks ? std::min<int>((int)s->scfg.resident_blocks, cap) : cap / 2;
     const unsigned fl = hipHostMallocCoherent;
-    HIP_TRY(hipHostMalloc((void **)&L.ctl, sizeof(LoopCtl), fl));
-    HIP_TRY(hipHostMalloc((void **)&L.ring, kLoopRingItems * sizeof(LoopItem), fl));
-    HIP_TRY(hipHostMalloc((void **)&L.rec, kLoopSlots * sizeof(LoopRec), fl));
-    HIP_TRY(hipHostMalloc((void **)&L.cq, kLoopSlots * sizeof(unsigned long long), fl));
-    memset(L.ctl, 0, sizeof(LoopCtl));
-    memset(L.ring, 0, kLoopRingItems * sizeof(LoopItem));
-    memset(L.rec, 0, kLoopSlots * sizeof(LoopRec));
-    memset(L.cq, 0, kLoopSlots * sizeof(unsigned long long));
     const size_t dev_bytes = (kLoopWords + (size_t)s->n + kLoopSlots) * 8 + (size_t)kLoopSlots * 8;
     unsigned long long *d = nullptr;
-    if (hipMalloc((void **)&d, dev_bytes) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&L.st2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&L.ended, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void **)&L.ring, kLoopRingItems * sizeof(LoopItem), fl) != hipSuccess ||
+        hipHostMalloc((void **)&L.rec, kLoopSlots * sizeof(LoopRec), fl) != hipSuccess ||
+        hipHostMalloc((void **)&L.cq, kLoopSlots * sizeof(unsigned long long), fl) != hipSuccess ||
+        hipMalloc((void **)&d, dev_bytes) != hipSuccess) {
         (void)hipGetLastError();
-        return fail(DYC_ERR_OOM, "hipMalloc of the loop's %zu bytes failed", dev_bytes);
+        L.d_words = d;
+        fail(DYC_ERR_OOM, "the resident loop's buffers (%zu bytes of device memory, about 19 MiB pinned) failed",
+             dev_bytes);
+        return undo(DYC_ERR_OOM);
     }
     L.d_words = d;
     L.d_tbl_done = d + kLoopWords;
     L.d_req_word = L.d_tbl_done + s->n;
     L.d_req_out = reinterpret_cast<uint32_t *>(L.d_req_word + kLoopSlots);
     L.d_req_stall = L.d_req_out + kLoopSlots;
-    HIP_TRY(launch_zero(d, dev_bytes / 8, L.st2));
-    HIP_TRY(hipStreamSynchronize(L.st2));
+    memset(L.ring, 0, kLoopRingItems * sizeof(LoopItem));
+    memset(L.rec, 0, kLoopSlots * sizeof(LoopRec));
+    memset(L.cq, 0, kLoopSlots * sizeof(unsigned long long));
+    if (launch_zero(d, dev_bytes / 8, L.st2) != hipSuccess || hipStreamSynchronize(L.st2) != hipSuccess) {
+        fail(DYC_ERR_HIP, "zeroing the resident loop's words failed");
+        return undo(DYC_ERR_HIP);
+    }
+    // the control block last: its presence marks the loop as allocated
+    if (hipHostMalloc((void **)&L.ctl, sizeof(LoopCtl), fl) != hipSuccess) {
+        (void)hipGetLastError();
+        L.ctl = nullptr;
+        fail(DYC_ERR_OOM, "the resident loop's control block failed");
+        return undo(DYC_ERR_OOM);
+    }
+    memset(L.ctl, 0, sizeof(LoopCtl));
+    L.blocks = std::max(blocks, 2);
     L.slot.assign(kLoopSlots, LoopSlot{});
     L.free_slots.resize(kLoopSlots);
     for (uint32_t i = 0; i < kLoopSlots; ++i) L.free_slots[i] = kLoopSlots - 1 - i;
