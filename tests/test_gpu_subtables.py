@@ -172,15 +172,17 @@ def test_group_with_three_subtables_matches_the_oracle(D):
     g.close()
 
 
-def test_serving_with_five_subtables(D):
-    """The continuous-batching loop on d = 5 tables: every result equals the trace's expected
-    results and the tables end holding exactly the live window."""
+@pytest.mark.parametrize("d,mode", [(5, 0), (2, 2), (6, 2)])
+def test_serving_with_other_subtable_counts(D, d, mode):
+    """The continuous-batching loop (host-built ticks, mode 0, or the device-resident loop, mode
+    2) on d = 5, 2 and 6 tables: every result equals the trace's expected results and the
+    tables end holding exactly the live window."""
     from databases_gpu_amd import workload as W
 
-    banner("serving replay mix, subtables=5")
+    banner(f"serving replay mix, subtables={d}, mode={mode}")
     tr = W.ServingTrace(600, seed=23, device="cuda", resident=1 << 14, tenants=8, tenant_lo=256,
                         tenant_hi=4096, batch_lo=1, batch_hi=1 << 11, mix=(0.3, 0.5, 0.2))
-    srv = D.DyCuckooServer(tr.hints, subtables=5, min_capacity=0, tick_max_keys=1 << 14, hbm_budget=256 << 20)
+    srv = D.DyCuckooServer(tr.hints, subtables=d, min_capacity=0, tick_max_keys=1 << 14, hbm_budget=256 << 20)
     init = [(tr.initial_keys(t),) for t in range(tr.sizes.size)]
     init = [(k, W.splitmix32(k)) for (k,) in init]
     torch.cuda.synchronize()  # the server's stream does not wait for torch's
@@ -190,15 +192,15 @@ def test_serving_with_five_subtables(D):
     ov = torch.zeros(tr.ops, dtype=torch.int32, device="cuda")
     of = torch.zeros(tr.ops, dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
-    done, status = srv.replay(tr.requests(ov, of), tr.arrivals, mode=0)
+    done, status = srv.replay(tr.requests(ov, of), tr.arrivals, mode=mode)
     torch.cuda.synchronize()
     assert (status == 0).all()
     assert torch.equal(of, tr.exp_flags) and torch.equal(ov, tr.exp_vals)
     lo, hi = tr.final_window
     for t in range(tr.sizes.size):
-        assert srv.table_stats(t)["subtables"] == 5
+        assert srv.table_stats(t)["subtables"] == d
         ref = oracle_from_images(srv.dump(t))
-        assert ref.d == 5 and ref.check_invariant() == 0
+        assert ref.d == d and ref.check_invariant() == 0
         live = W.keys_at(torch.arange(int(tr.base[t] + lo[t]), int(tr.base[t] + hi[t]), dtype=torch.int64), tr.seed)
         live = live.numpy().view(np.uint32)
         assert np.array_equal(ref.entries(), np.sort(O.pack(live, W.splitmix32_np(live))))
