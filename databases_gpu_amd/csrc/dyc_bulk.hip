@@ -77,12 +77,6 @@ __global__ __launch_bounds__(kCountThreads) void bulk_pack_kernel(Desc D, Chunks
     for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) partial[(uint64_t)blockIdx.x * R + d] = h[d];
 }
 
-__device__ inline uint32_t nth_set_bit(uint32_t m, uint32_t r)
-{
-    for (uint32_t q = 0; q < r; ++q) m &= m - 1u;
-    return __builtin_ctz(m);
-}
-
 // One workgroup per chunk. Buckets of the chunk are written only by this workgroup during the
 // launch (every entry of the chunk targets one of them), so slots are taken with LDS counters.
 // DENSE (a round with at least two entries per bucket): the chunk's 512 buckets are copied into
@@ -95,7 +89,7 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
                                                         const uint64_t *__restrict__ ent, uint64_t n,
                                                         uint64_t *__restrict__ ovf, uint64_t ovf_cap,
                                                         unsigned long long *__restrict__ ovf_n,
-                                                        Counters *__restrict__ C)
+                                                        Counters *__restrict__ C, bool zeroed)
 {
     constexpr uint32_t kImg = DENSE ? kChunkBuckets * kSlots : 2;
     __shared__ uint64_t img[kImg];           // DENSE: the chunk's buckets
@@ -120,29 +114,39 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
         const uint4 *const src = reinterpret_cast<const uint4 *>(sub + (uint64_t)b0 * kSlots);
         constexpr uint32_t kParts = kChunkBuckets * kSlots / 2;  // 16-byte parts of the chunk
         if constexpr (DENSE) {
-            // the chunk into LDS; a bucket's 8 parts sit in 8 consecutive lanes, whose two ballot
-            // bits per part make its free-slot mask (no strided LDS reads)
             uint4 *const im = reinterpret_cast<uint4 *>(img);
-            constexpr uint32_t kPer = kParts / THREADS;
-            uint4 v[kPer];
+            if (zeroed) {
+                // every slot of the table is known to be zero (the caller's host state says so):
+                // the chunk's image starts empty in LDS without reading it
+                for (uint32_t q = threadIdx.x; q < kParts; q += THREADS) im[q] = make_uint4(0u, 0u, 0u, 0u);
+                for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) {
+                    fm[j] = j < len ? 0xffffu : 0u;  // past the end: full
+                    cnt[j] = 0;
+                }
+            } else {
+                // the chunk into LDS; a bucket's 8 parts sit in 8 consecutive lanes, whose two ballot
+                // bits per part make its free-slot mask (no strided LDS reads)
+                constexpr uint32_t kPer = kParts / THREADS;
+                uint4 v[kPer];
 #pragma unroll
-            for (uint32_t u = 0; u < kPer; ++u) {
-                const uint32_t q = threadIdx.x + u * THREADS;
-                v[u] = q < len * (kSlots / 2) ? src[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
-            }
-            const uint32_t lane = __lane_id();
+                for (uint32_t u = 0; u < kPer; ++u) {
+                    const uint32_t q = threadIdx.x + u * THREADS;
+                    v[u] = q < len * (kSlots / 2) ? src[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
+                }
+                const uint32_t lane = __lane_id();
 #pragma unroll
-            for (uint32_t u = 0; u < kPer; ++u) {
-                const uint32_t q = threadIdx.x + u * THREADS;
-                im[q] = v[u];
-                const uint64_t m0 = __ballot((v[u].x | v[u].y) == 0u), m1 = __ballot((v[u].z | v[u].w) == 0u);
-                if ((lane & 7u) == 0) {
-                    const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
-                    uint32_t m = 0;
+                for (uint32_t u = 0; u < kPer; ++u) {
+                    const uint32_t q = threadIdx.x + u * THREADS;
+                    im[q] = v[u];
+                    const uint64_t m0 = __ballot((v[u].x | v[u].y) == 0u), m1 = __ballot((v[u].z | v[u].w) == 0u);
+                    if ((lane & 7u) == 0) {
+                        const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
+                        uint32_t m = 0;
 #pragma unroll
-                    for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
-                    fm[q / 8] = m;
-                    cnt[q / 8] = 0;
+                        for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
+                        fm[q / 8] = m;
+                        cnt[q / 8] = 0;
+                    }
                 }
             }
         } else {
@@ -152,23 +156,63 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
         __syncthreads();
         // count: entries per bucket (an entry whose bucket falls outside the chunk, which the
         // pack rule excludes, would go to the overflow list). Entries are loaded kBatch at a
-        // time before any is used, so a thread waits for memory once per batch.
+        // time before any is used, so a thread waits for memory once per batch. A chunk with at
+        // most kHold batches per thread keeps them in registers for the placement, so each entry
+        // is read from memory once.
         constexpr uint32_t kBatch = 8;
+        constexpr uint32_t kHold = DENSE ? 2 : 1;
+        const bool held = hi - lo <= kHold * kBatch * THREADS;  // block-uniform
+        uint64_t hb[kHold][kBatch];
+        // held entries also keep their place from the count: the bucket (9 bits) and the rank
+        // among the chunk's entries of that bucket (the count's atomic returns it), or kReject /
+        // kStray. A bucket's first popcount(free mask) ranks take its free slots.
+        constexpr uint32_t kReject = ~0u, kStray = ~0u - 1u;
+        static_assert(kChunkBuckets <= 512 && kHold * kBatch * THREADS < (1u << 22), "rank and bucket fit 32 bits");
+        uint32_t hp[kHold][kBatch];
         uint32_t stray = 0;
-        for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * blockDim.x) {
-            uint64_t eb[kBatch];
+        auto load_batch = [&](uint32_t i0, uint64_t (&eb)[kBatch]) {
 #pragma unroll
             for (uint32_t u = 0; u < kBatch; ++u) {
-                const uint32_t i = i0 + u * blockDim.x;
+                const uint32_t i = i0 + u * THREADS;
                 eb[u] = i < hi ? ent[i] : 0ull;
             }
+        };
+        auto count_batch = [&](uint32_t i0, const uint64_t (&eb)[kBatch]) {
 #pragma unroll
             for (uint32_t u = 0; u < kBatch; ++u) {
                 const uint32_t k = key_of(eb[u]);
-                if (i0 + u * blockDim.x >= hi || reserved_key(k)) continue;
+                if (i0 + u * THREADS >= hi || reserved_key(k)) continue;
                 const uint32_t b = bucket_of_uniform(D, k, t);
                 if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
                 else ++stray;
+            }
+        };
+        if (held) {
+#pragma unroll
+            for (uint32_t h = 0; h < kHold; ++h) load_batch(lo + threadIdx.x + h * kBatch * THREADS, hb[h]);
+#pragma unroll
+            for (uint32_t h = 0; h < kHold; ++h) {
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) {
+                    const uint32_t k = key_of(hb[h][u]);
+                    uint32_t at = kReject;  // also past the range (never used)
+                    if (lo + threadIdx.x + (h * kBatch + u) * THREADS < hi && !reserved_key(k)) {
+                        const uint32_t b = bucket_of_uniform(D, k, t);
+                        if (b >= b0 && b - b0 < len) {
+                            at = (atomicAdd(&cnt[b - b0], 1u) << 9) | (b - b0);
+                        } else {
+                            at = kStray;
+                            ++stray;
+                        }
+                    }
+                    hp[h][u] = at;
+                }
+            }
+        } else {
+            for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * THREADS) {
+                uint64_t eb[kBatch];
+                load_batch(i0, eb);
+                count_batch(i0, eb);
             }
         }
         __syncthreads();
@@ -213,19 +257,20 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
             for (uint32_t w = 0; w < blockDim.x / kWave; ++w) tot += wred[w];
             obase = tot ? atomicAdd(ovf_n, (unsigned long long)tot) : 0ull;
         }
-        for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += blockDim.x) cnt[j] = 0;
+        if (!held)  // the placement counts again
+            for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += blockDim.x) cnt[j] = 0;
         __syncthreads();
+        auto overflow = [&](uint64_t e) {
+            // the counted excess reserved exactly these places; the list holds every entry of the
+            // batch, so the check fails only if the chunk ranges overlapped
+            const unsigned long long at = obase + atomicAdd(&onext, 1u);
+            if (at < ovf_cap) ovf[at] = e;
+        };
         // place
-        for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * blockDim.x) {
-            uint64_t eb[kBatch];
+        auto place_batch = [&](uint32_t i0, const uint64_t (&eb)[kBatch]) {
 #pragma unroll
             for (uint32_t u = 0; u < kBatch; ++u) {
-                const uint32_t i = i0 + u * blockDim.x;
-                eb[u] = i < hi ? ent[i] : 0ull;
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                if (i0 + u * blockDim.x >= hi) continue;
+                if (i0 + u * THREADS >= hi) continue;
                 const uint64_t e = eb[u];
                 const uint32_t k = key_of(e);
                 if (reserved_key(k)) {
@@ -239,19 +284,46 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
                     const uint32_t r = atomicAdd(&cnt[j], 1u);
                     const uint32_t m = fm[j];
                     if (r < (uint32_t)__builtin_popcount(m)) {
-                        const uint32_t slot = nth_set_bit(m, r);
+                        const uint32_t slot = nth_set_bit16(m, r);
                         if constexpr (DENSE) img[j * kSlots + slot] = e;
                         else sub[(uint64_t)b * kSlots + slot] = e;
                         fits = true;
                         ++placed;
                     }
                 }
-                if (!fits) {
-                    // the counted excess reserved exactly these places; the list holds every entry
-                    // of the batch, so the check fails only if the chunk ranges overlapped
-                    const unsigned long long at = obase + atomicAdd(&onext, 1u);
-                    if (at < ovf_cap) ovf[at] = e;
+                if (!fits) overflow(e);
+            }
+        };
+        if (held) {
+#pragma unroll
+            for (uint32_t h = 0; h < kHold; ++h) {
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) {
+                    if (lo + threadIdx.x + (h * kBatch + u) * THREADS >= hi) continue;
+                    const uint32_t at = hp[h][u];
+                    if (at == kReject) {
+                        ++rejected;
+                        continue;
+                    }
+                    if (at != kStray) {
+                        const uint32_t j = at & 511u, r = at >> 9;
+                        const uint32_t m = fm[j];
+                        if (r < (uint32_t)__builtin_popcount(m)) {
+                            const uint32_t slot = nth_set_bit16(m, r);
+                            if constexpr (DENSE) img[j * kSlots + slot] = hb[h][u];
+                            else sub[(uint64_t)(b0 + j) * kSlots + slot] = hb[h][u];
+                            ++placed;
+                            continue;
+                        }
+                    }
+                    overflow(hb[h][u]);
                 }
+            }
+        } else {
+            for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * THREADS) {
+                uint64_t eb[kBatch];
+                load_batch(i0, eb);
+                place_batch(i0, eb);
             }
         }
         if constexpr (DENSE) {
@@ -325,7 +397,8 @@ hipError_t launch_bulk_partition(const Desc &D, const Chunks &Ch, const uint32_t
 }
 
 hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *start, const uint64_t *ent, uint64_t n,
-                            uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C, hipStream_t st)
+                            uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C, bool zeroed,
+                            hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     if (n >= (1ull << 32) || Ch.total == 0) return hipErrorInvalidValue;
@@ -334,9 +407,9 @@ hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *sta
     // dense: copying every chunk through LDS costs less than scattered stores; its 68 KB of LDS
     // allow two workgroups per CU, so they are 512 threads wide to keep more loads in flight
     if (n >= 2 * buckets)
-        bulk_fill_kernel<true, 512><<<Ch.total, 512, 0, st>>>(D, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C);
+        bulk_fill_kernel<true, 512><<<Ch.total, 512, 0, st>>>(D, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C, zeroed);
     else
-        bulk_fill_kernel<false, 256><<<Ch.total, 256, 0, st>>>(D, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C);
+        bulk_fill_kernel<false, 256><<<Ch.total, 256, 0, st>>>(D, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C, false);
     return hipGetLastError();
 }
 

@@ -98,6 +98,19 @@ __host__ __device__ inline uint32_t rescue_lane(uint32_t k, uint32_t it)
     return (k * 0x9E3779B1u + it * 0x632BE5ABu) >> 28;
 }
 
+// Position of the set bit of rank r (0 = lowest) in a 16-bit slot mask m, for r < popcount(m):
+// four halving steps, each keeping the half that holds it (no loop, no divergence).
+__host__ __device__ inline uint32_t nth_set_bit16(uint32_t m, uint32_t r)
+{
+    uint32_t pos = 0, c = (uint32_t)__builtin_popcount(m & 0xffu);
+    if (r >= c) r -= c, m >>= 8, pos += 8;
+    c = (uint32_t)__builtin_popcount(m & 0xfu);
+    if (r >= c) r -= c, m >>= 4, pos += 4;
+    c = (uint32_t)__builtin_popcount(m & 0x3u);
+    if (r >= c) r -= c, m >>= 2, pos += 2;
+    return pos + (r >= (m & 1u) ? 1u : 0u);
+}
+
 // ------------------------------------------------------------------ division by a runtime d
 // n % d for the per-subtable bucket count d without a hardware divide: Granlund & Montgomery,
 // "Division by Invariant Integers using Multiplication" (PLDI 1994), Fig. 4.1 (round-up

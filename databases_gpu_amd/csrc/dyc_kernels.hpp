@@ -152,9 +152,10 @@ hipError_t launch_bulk_partition(const Desc &D, const Chunks &Ch, const uint32_t
                                  uint64_t n, const BulkSpace &ws, uint32_t max_blocks, hipStream_t st, int *out);
 // Fill free slots chunk by chunk from the partitioned entries (n < 2^32); start holds total + 1
 // words. Entries that do not fit are appended to ovf (*ovf_n, zeroed by the caller, counts them;
-// ovf_cap >= n). Placements go to the kick tally, reserved keys to the rejected tally.
+// ovf_cap >= n). Placements go to the kick tally, reserved keys to the rejected tally. zeroed:
+// every slot of the table is zero (a fresh or cleared table), so a dense round need not read it.
 hipError_t launch_bulk_fill(const Desc &D, const Chunks &Ch, const uint32_t *start, const uint64_t *ent, uint64_t n,
-                            uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C,
+                            uint64_t *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C, bool zeroed,
                             hipStream_t st);
 // Order an insert batch by the region of its first bucket: out[0, n) receives the packed
 // entries, region by region. counts and cursors are R words each (counts zeroed by the caller).

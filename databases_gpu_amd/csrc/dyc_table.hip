@@ -100,6 +100,9 @@ struct dyc_table {
     unsigned long long *hpost = nullptr;  // pinned: the read-back words (launch_fold_post)
     unsigned long long post_seq = 0;
     bool ctr_clean = false;     // the device block is zero and no launch has used it since
+    // every slot is zero: set when the table is created or cleared, dropped by every call that
+    // may store an entry (a bulk fill into such a table need not read it)
+    bool zeroed = false;
     uint64_t *part = nullptr;   // insert batch ordered by bucket region
     uint64_t part_cap = 0;
     unsigned long long *part_counts = nullptr;   // kMaxRegions counts, then kMaxRegions cursors
@@ -281,6 +284,7 @@ int alloc_slab(uint32_t d, const uint32_t *nb, uint8_t **base, uint64_t **sub)
 // Replace every subtable of t by the regions of a new slab (the old storage is released).
 int install_slab(dyc_table *t, uint8_t *base, uint64_t *const *sub, const uint32_t *nb)
 {
+    t->zeroed = false;  // new memory: only the caller knows what it holds
     for (uint32_t i = 0; i < t->D.d; ++i) {
         const int rc = release_sub(t, i);
         if (rc) return rc;
@@ -330,6 +334,7 @@ int absorb_insert_counters(dyc_table *t, uint64_t room)
 
 int do_resize_up(dyc_table *t, hipStream_t s)
 {
+    t->zeroed = false;
     const uint32_t sub = choose_up(t->D.nb, t->D.d);
     const uint32_t old_nb = t->D.nb[sub];
     if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^29 buckets", sub);
@@ -359,6 +364,7 @@ int do_resize_up(dyc_table *t, hipStream_t s)
 // list is still longer than `before` after max_fail_rounds rounds.
 int place_pending(dyc_table *t, hipStream_t s, uint64_t before)
 {
+    t->zeroed = false;
     uint32_t rounds = 0;
     while (t->pend_n > before && rounds < t->cfg.max_fail_rounds) {
         int rc = do_resize_up(t, s);
@@ -394,6 +400,7 @@ int place_pending(dyc_table *t, hipStream_t s, uint64_t before)
 
 int do_resize_down(dyc_table *t, hipStream_t s)
 {
+    t->zeroed = false;
     const uint64_t pend_before = t->pend_n;
     const uint32_t sub = choose_down(t->D.nb, t->D.d);
     const uint32_t old_nb = t->D.nb[sub];
@@ -582,12 +589,13 @@ BulkSpace bulk_space(const dyc_table *t)
 // entries in bk_ent[in]. Partition them by chunk, fill, and read back how many entries overflowed
 // into bk_ent[*out] (the buffer the partition did not end in).
 int bulk_round(dyc_table *t, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals, int in, uint64_t n,
-               hipStream_t s, int *out, uint64_t *overflow)
+               bool zeroed, hipStream_t s, int *out, uint64_t *overflow)
 {
     int r = 0;
     HIP_TRY(launch_bulk_partition(t->D, Ch, keys, vals, in, n, bulk_space(t), kBulkSortBlocks, s, &r));
     HIP_TRY(launch_zero(t->bk_ovf, 1, s));
-    HIP_TRY(launch_bulk_fill(t->D, Ch, t->bk_start, t->bk_ent[r], n, t->bk_ent[1 - r], t->bk_cap, t->bk_ovf, t->ctr, s));
+    HIP_TRY(launch_bulk_fill(t->D, Ch, t->bk_start, t->bk_ent[r], n, t->bk_ent[1 - r], t->bk_cap, t->bk_ovf, t->ctr,
+                             zeroed, s));
     HIP_TRY(hipMemcpyAsync(t->bk_hovf, t->bk_ovf, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     *overflow = *t->bk_hovf;
@@ -600,16 +608,19 @@ int bulk_round(dyc_table *t, const Chunks &Ch, const uint32_t *keys, const uint3
 // insert_order 3: place keys by bucket fill in their start bucket, then the overflow in the
 // other bucket of their pair, then the rest by the eviction insert. Counters are zeroed here
 // and read by the caller.
-int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, hipStream_t s)
+int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, bool was_zeroed, hipStream_t s)
 {
     const Chunks Ch = make_chunks(t->D);  // the workspace was reserved for it
     int rc = zero_counters(t, s);
     if (rc) return rc;
     int ov = 0, ov2 = 0;
     uint64_t m1 = 0, m2 = 0;
-    rc = bulk_round(t, Ch, keys, vals, 0, n, s, &ov, &m1);
+    // an empty table (count 0, nothing pending) that no call has stored into since it was
+    // created or cleared holds zeros only
+    const bool zeroed = was_zeroed && t->count == 0 && t->pend_n == 0;
+    rc = bulk_round(t, Ch, keys, vals, 0, n, zeroed, s, &ov, &m1);
     if (rc || m1 == 0) return rc;
-    rc = bulk_round(t, Ch, nullptr, nullptr, ov, m1, s, &ov2, &m2);  // the overflow, in its other table
+    rc = bulk_round(t, Ch, nullptr, nullptr, ov, m1, false, s, &ov2, &m2);  // the overflow, in its other table
     if (rc || m2 == 0) return rc;
     HIP_TRY(launch_insert_entries(t->D, t->bk_ent[ov2], m2, kNoTable, (int)t->cfg.insert_lanes, t->policy(),
                                   t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s));
@@ -1017,6 +1028,7 @@ int dyc_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_table **out
     }
     if (rc == DYC_OK) rc = reserve_pending(t, DYC_NOALLOC_MAX_KEYS, nullptr);
     if (rc == DYC_OK && hipStreamSynchronize(nullptr) != hipSuccess) rc = fail(DYC_ERR_HIP, "create sync failed");
+    t->zeroed = rc == DYC_OK;
     if (rc != DYC_OK) {
         char keep[sizeof g_err];
         memcpy(keep, g_err, sizeof keep);
@@ -1070,6 +1082,8 @@ int dyc_insert_counted(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
     SETTLE(t);
+    bool zeroed = t->zeroed;  // every slot zero before this call (dropped by any resize below)
+    t->zeroed = false;
     const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = t->st.last_max_chain = 0;
@@ -1079,6 +1093,7 @@ int dyc_insert_counted(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
         while ((double)(t->count + n) > t->cfg.upper_bound * (double)t->capacity()) {
             rc = do_resize_up(t, s);
             if (rc) return rc;
+            zeroed = false;
         }
     }
     rc = reserve_pending(t, std::max<uint64_t>(65536, n / 16), s);
@@ -1098,7 +1113,7 @@ int dyc_insert_counted(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
         }
     }
     if (bulk) {
-        rc = bulk_insert(t, keys, vals, n, s);
+        rc = bulk_insert(t, keys, vals, n, zeroed, s);
         if (rc) return rc;
     } else if (R) {
         rc = reserve_partition(t, n, s);
@@ -1135,6 +1150,7 @@ int dyc_insert_async(dyc_table *t, const uint32_t *keys, const uint32_t *vals, u
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
+    t->zeroed = false;
     hipStream_t s = (hipStream_t)stream;
     if (n == 0) return DYC_OK;
     int rc;
@@ -1198,6 +1214,7 @@ int dyc_insert_noalloc(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
                     "without allocating (dyc_insert re-places pending entries and grows it)",
                     (unsigned long long)n, (unsigned long long)room);
     DeviceGuard dg(t->device);
+    t->zeroed = false;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = t->st.last_max_chain = 0;
     if (n == 0) return DYC_OK;
@@ -1228,6 +1245,7 @@ int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
     SETTLE(t);
+    t->zeroed = false;
     const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = t->st.last_max_chain = 0;
@@ -1386,6 +1404,7 @@ int dyc_clear(dyc_table *t, dyc_stream stream)
     t->count = 0;
     t->pend_n = 0;
     t->st.pending = 0;
+    t->zeroed = true;
     return DYC_OK;
 }
 
@@ -1427,6 +1446,7 @@ int dyc_load(dyc_table *t, const uint32_t *nb, const uint64_t *const *images, dy
         if (nb[i] == 0 || nb[i] >= kMaxBuckets || !images[i]) return fail(DYC_ERR_INVALID, "bad subtable %d", i);
     DeviceGuard dg(t->device);
     SETTLE(t);
+    t->zeroed = false;
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipStreamSynchronize(s));
     uint8_t *base = nullptr;
@@ -1478,6 +1498,7 @@ int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, ui
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
     SETTLE(t);
+    t->zeroed = false;
     const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = t->st.last_max_chain = 0;

@@ -169,6 +169,17 @@ def test_fast_division_is_exact(tmp_path):
     assert " bad 0" in r.stdout
 
 
+def test_nth_set_bit16_is_exact(tmp_path):
+    """nth_set_bit16(m, r): the slot of the bulk fill's r-th placement into a bucket with free-slot
+    mask m, checked for all 2^16 masks and every rank against clearing the lowest bit r times."""
+    exe = tmp_path / "bits"
+    subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, os.path.join(ROOT, "tests", "cpp", "bits_check.cpp"),
+                    "-o", str(exe)], check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "checked 524288 bad 0" in r.stdout
+
+
 def test_device_header_hashes_match_the_oracle(oracle_lib, tmp_path):
     """The kernels' per-key arithmetic (dyc_device.hpp, compiled for the host here) equals the
     oracle's on edge and random keys, for every subtable count. The device enumerates each

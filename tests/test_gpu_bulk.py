@@ -112,6 +112,35 @@ def test_bulk_fill_into_a_partly_filled_resized_table(D):
     t.close()
 
 
+def test_bulk_fill_into_fresh_filled_and_cleared_tables(D):
+    """A fresh or cleared table is known to hold zeros only, and its first dense round does not
+    read the table; any later batch reads it. Batch A into a fresh table, batch B on top (read
+    path: A and B both stored), clear, batch B again (zeroed path: nothing of A survives)."""
+    banner("bulk fill: fresh table, second batch on top, then clear and refill")
+    n = 1 << 19
+    keys = unique_keys(2 * n, 1500)
+    a, b = keys[:n], keys[n:]
+    t = D.DyCuckoo(int(np.ceil(2 * n / 0.9)), insert_order=3, auto_resize=False)
+    assert n >= 2 * sum(t.buckets)  # both batches take the bulk fill
+    t.insert(cu(a), cu(vals_of(a)))
+    t.insert(cu(b), cu(vals_of(b)))
+    st = t.stats()
+    assert st["count"] == 2 * n and st["lost"] == 0
+    ents, ref = stored(t)
+    assert ref.check_invariant() == 0
+    assert np.array_equal(ents, np.sort(O.pack(keys, vals_of(keys))))
+    t.clear()
+    t.insert(cu(b), cu(vals_of(b)))
+    st = t.stats()
+    assert st["count"] == n and st["lost"] == 0
+    ents, ref = stored(t)
+    assert ref.check_invariant() == 0
+    assert np.array_equal(ents, np.sort(O.pack(b, vals_of(b))))
+    gv, gh = gfind(t, keys)
+    assert not gh[:n].any() and gh[n:].all() and np.array_equal(gv[n:], vals_of(b))
+    t.close()
+
+
 def test_bulk_fill_overflow_reaches_the_eviction_insert(D):
     """Many copies of a few keys overflow their start bucket and their other bucket, so all three
     stages run; every copy is stored or pending, and the placement invariant holds."""
