@@ -274,14 +274,41 @@ __global__ __launch_bounds__(256) void part_seg_write_kernel(uint32_t *__restric
     }
 }
 
+// The lanes among vm (a ballot) whose digit equals this lane's d, a digit of `bits` bits (BITS if
+// it is positive). Per bit: sb = 0 or ~0 (the lane's bit), and the mask keeps the lanes whose bit
+// equals it, m &= ~(ballot ^ sb), on the two 32-bit halves.
+template <int BITS>
+__device__ inline uint64_t match_mask(uint64_t vm, uint32_t d, uint32_t bits)
+{
+    uint32_t mlo = (uint32_t)vm, mhi = (uint32_t)(vm >> 32);
+    auto step = [&](uint32_t b) {
+        const uint32_t sb = (uint32_t)((int32_t)(d << (31u - b)) >> 31);
+        const uint64_t bb = __ballot(sb != 0u);  // lanes outside vm drop out through the AND
+        mlo &= ~((uint32_t)bb ^ sb);
+        mhi &= ~((uint32_t)(bb >> 32) ^ sb);
+    };
+    if constexpr (BITS > 0) {
+#pragma unroll
+        for (uint32_t b = 0; b < (uint32_t)BITS; ++b) step(b);
+    } else {
+        for (uint32_t b = 0; b < bits; ++b) step(b);
+    }
+    return ((uint64_t)mhi << 32) | mlo;
+}
+
 // One pass: the entries of each block's range (from src, chunk ids from cid) go to their places
-// in ent_out (chunk ids to cid_out unless it is null), ordered by digit P, stably: by digit, then
-// index, as every pass after the first of a least-significant-digit partition needs. partial
-// holds the blocks' exclusive prefixes per digit and total the digits' totals
-// (part_colscan_kernel). The next tile's loads are issued before the current tile is ranked: a
-// barrier here waits for LDS operations only, so they stay in flight while the tile is placed
-// and written out.
-template <typename Entry, typename Src, typename CidT, int THREADS>
+// in ent_out (chunk ids to cid_out unless it is null), ordered by digit P. partial holds the
+// blocks' exclusive prefixes per digit and total the digits' totals (part_colscan_kernel). The
+// next tile's loads are issued before the current tile is ranked: a barrier here waits for LDS
+// operations only, so they stay in flight while the tile is placed and written out.
+// Ranking within a wave (BITS):
+//   > 0  stably (by digit, then index), with one ballot per digit bit; BITS = P.bits;
+//   0    the same with P.bits read at run time;
+//   < 0  not stably: each entry's rank is what its wave's LDS counter of the digit returned
+//        when the entry added itself, in whatever order the lanes' atomics were applied. The
+//        first pass of a least-significant-digit partition may do this (its input order means
+//        nothing); every later pass must keep the order of the previous digits.
+template <typename Entry, typename Src, typename CidT, int THREADS, int BITS>
 __global__ __launch_bounds__(THREADS) void part_scatter_kernel(Src src, const CidT *__restrict__ cid, PartGrid G,
                                                                RadixPass P, const uint32_t *__restrict__ partial,
                                                                const uint32_t *__restrict__ total,
@@ -331,23 +358,25 @@ __global__ __launch_bounds__(THREADS) void part_scatter_kernel(Src src, const Ci
         if (t0 + kTile < hi) load_tile(t0 + kTile, en, cn);
         for (uint32_t x = tid; x < kWaves * kMaxRadix; x += THREADS) (&wcnt[0][0])[x] = 0;
         __syncthreads();  // wcnt is zero; the previous tile's output is done with buf and gdel
-        // rank within the wave: the lanes of one digit in an item (their match mask) take
-        // consecutive ranks in lane order after the digit's count so far
         uint32_t r[kSortItems];
+        if constexpr (BITS < 0) {
 #pragma unroll
-        for (int u = 0; u < kSortItems; ++u) {
-            const bool valid = wbase + u * kWave + lane < tn;
-            const uint32_t d = digit_of(c[u], P);
-            uint64_t m = __ballot(valid);
-            for (uint32_t b = 0; b < P.bits; ++b) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t bb = __ballot(valid && bit);
-                m &= bit ? bb : ~bb;
+            for (int u = 0; u < kSortItems; ++u)
+                r[u] = wbase + u * kWave + lane < tn ? atomicAdd(&wcnt[w][digit_of(c[u], P)], 1u) : 0u;
+        } else {
+            // the lanes of one digit in an item (their match mask) take consecutive ranks in lane
+            // order after the digit's count so far
+            const uint32_t bits = P.bits;
+#pragma unroll
+            for (int u = 0; u < kSortItems; ++u) {
+                const bool valid = wbase + u * kWave + lane < tn;
+                const uint32_t d = digit_of(c[u], P);
+                const uint64_t m = match_mask<BITS>(__ballot(valid), d, bits);
+                const uint32_t below = (uint32_t)__popcll(m & lanemask);
+                const uint32_t base = valid ? wcnt[w][d] : 0u;
+                r[u] = base + below;
+                if (valid && below == 0) wcnt[w][d] = base + (uint32_t)__popcll(m);
             }
-            const uint32_t below = (uint32_t)__popcll(m & lanemask);
-            const uint32_t base = valid ? wcnt[w][d] : 0u;
-            r[u] = base + below;
-            if (valid && below == 0) wcnt[w][d] = base + (uint32_t)__popcll(m);
         }
         __syncthreads();
         // per digit: wave offsets (exclusive over the waves), the tile's first place, and the
@@ -427,11 +456,15 @@ hipError_t partition_passes(Src src0, int first, Entry *const ent[2], CidT *cons
                                                                                  last ? start : nullptr);
         part_colscan_kernel<<<R, 256, 0, st>>>(partial, G.blocks * G.sub, R, total);
         CidT *const cout = last ? nullptr : cid[(p + 1) & 1];
+        // the first pass ranks by atomics; the later ones stably, 8-bit digits unrolled
         if (p == 0)
-            part_scatter_kernel<Entry, Src, CidT, THREADS>
+            part_scatter_kernel<Entry, Src, CidT, THREADS, -1>
                 <<<G.blocks, THREADS, 0, st>>>(src0, cid[0], G, P[p], partial, total, ent[e], cout);
+        else if (P[p].bits == kMaxRadixBits)
+            part_scatter_kernel<Entry, EntrySource<Entry>, CidT, THREADS, kMaxRadixBits><<<G.blocks, THREADS, 0, st>>>(
+                EntrySource<Entry>{ent[1 - e]}, cid[p & 1], G, P[p], partial, total, ent[e], cout);
         else
-            part_scatter_kernel<Entry, EntrySource<Entry>, CidT, THREADS><<<G.blocks, THREADS, 0, st>>>(
+            part_scatter_kernel<Entry, EntrySource<Entry>, CidT, THREADS, 0><<<G.blocks, THREADS, 0, st>>>(
                 EntrySource<Entry>{ent[1 - e]}, cid[p & 1], G, P[p], partial, total, ent[e], cout);
         if (!last) e = 1 - e;
     }
