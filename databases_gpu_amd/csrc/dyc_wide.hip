@@ -617,18 +617,12 @@ __global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc 
     for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) partial[(uint64_t)blockIdx.x * R + d] = h[d];
 }
 
-__device__ inline uint32_t wide_nth_set_bit(uint32_t m, uint32_t r)
-{
-    for (uint32_t q = 0; q < r; ++q) m &= m - 1u;
-    return __builtin_ctz(m);
-}
-
 template <bool DENSE, int THREADS>
 __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chunks Ch, const uint32_t *__restrict__ start,
                                                                  const WideEntry *__restrict__ ent, uint64_t n,
                                                                  WideEntry *__restrict__ ovf, uint64_t ovf_cap,
                                                                  unsigned long long *__restrict__ ovf_n,
-                                                                 Counters *__restrict__ C)
+                                                                 Counters *__restrict__ C, bool zeroed)
 {
     const Desc &D = W.D;
     constexpr uint32_t kImg = DENSE ? kChunkBuckets * kSlots : 2;
@@ -662,7 +656,19 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
             // make its free-slot mask (no strided LDS reads)
             static_assert(kParts % THREADS == 0, "every lane takes the same number of parts");
             const uint32_t lane = __lane_id();
-            for (uint32_t q = threadIdx.x; q < kParts; q += THREADS) {
+            if (zeroed) {
+                // every slot of the table is known to be zero (the caller's host state says so):
+                // the chunk's lines start empty in LDS without reading them
+                for (uint32_t q = threadIdx.x; q < kParts; q += THREADS) {
+                    ki[q] = make_uint4(0u, 0u, 0u, 0u);
+                    vi[q] = make_uint4(0u, 0u, 0u, 0u);
+                }
+                for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) {
+                    fm[j] = j < len ? 0xffffu : 0u;  // past the end: full
+                    cnt[j] = 0;
+                }
+            }
+            for (uint32_t q = threadIdx.x; q < kParts && !zeroed; q += THREADS) {
                 const bool in = q < len * (kSlots / 2);
                 const uint4 kv = in ? ksrc[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
                 ki[q] = kv;
@@ -682,21 +688,53 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
         }
         if (threadIdx.x == 0) onext = 0;
         __syncthreads();
+        // count: entries per bucket. A chunk with at most kBatch entries per thread keeps them in
+        // registers for the placement, each with its bucket (9 bits) and its rank among the
+        // chunk's entries of that bucket (what the count's atomic returned), or kReject / kStray;
+        // a bucket's first popcount(free mask) ranks take its free slots. Larger chunks count
+        // here and count again as they place.
         constexpr uint32_t kBatch = 8;
+        constexpr uint32_t kReject = ~0u, kStray = ~0u - 1u;
+        static_assert(kChunkBuckets <= 512 && kBatch * THREADS < (1u << 22), "rank and bucket fit 32 bits");
+        const bool held = hi - lo <= kBatch * THREADS;  // block-uniform
+        WideEntry hb[kBatch];
+        uint32_t hp[kBatch];
         uint32_t stray = 0;
-        for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * THREADS) {
-            uint64_t kb[kBatch];
+        if (held) {
 #pragma unroll
             for (uint32_t u = 0; u < kBatch; ++u) {
-                const uint32_t i = i0 + u * THREADS;
-                kb[u] = i < hi ? ent[i].k : 0ull;
+                const uint32_t i = lo + threadIdx.x + u * THREADS;
+                hb[u] = i < hi ? ent[i] : WideEntry{0ull, 0ull};
             }
 #pragma unroll
             for (uint32_t u = 0; u < kBatch; ++u) {
-                if (i0 + u * THREADS >= hi || wide_reserved(kb[u])) continue;
-                const uint32_t b = bucket_of_uniform(D, fold64(kb[u]), t);
-                if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
-                else ++stray;
+                uint32_t at = kReject;  // also past the range (never used)
+                if (lo + threadIdx.x + u * THREADS < hi && !wide_reserved(hb[u].k)) {
+                    const uint32_t b = bucket_of_uniform(D, fold64(hb[u].k), t);
+                    if (b >= b0 && b - b0 < len) {
+                        at = (atomicAdd(&cnt[b - b0], 1u) << 9) | (b - b0);
+                    } else {
+                        at = kStray;
+                        ++stray;
+                    }
+                }
+                hp[u] = at;
+            }
+        } else {
+            for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * THREADS) {
+                uint64_t kb[kBatch];
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) {
+                    const uint32_t i = i0 + u * THREADS;
+                    kb[u] = i < hi ? ent[i].k : 0ull;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) {
+                    if (i0 + u * THREADS >= hi || wide_reserved(kb[u])) continue;
+                    const uint32_t b = bucket_of_uniform(D, fold64(kb[u]), t);
+                    if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
+                    else ++stray;
+                }
             }
         }
         __syncthreads();
@@ -739,9 +777,44 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
             for (uint32_t w = 0; w < THREADS / kWave; ++w) tot += wred[w];
             obase = tot ? atomicAdd(ovf_n, (unsigned long long)tot) : 0ull;
         }
-        for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) cnt[j] = 0;
+        if (!held)  // the placement counts again
+            for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) cnt[j] = 0;
         __syncthreads();
-        for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * THREADS) {
+        auto put = [&](uint32_t j, uint32_t slot, const WideEntry &e) {
+            if constexpr (DENSE) {
+                kimg[j * kSlots + slot] = e.k;
+                vimg[j * kSlots + slot] = e.v;
+            } else {
+                ksub[(uint64_t)(b0 + j) * kSlots + slot] = e.k;
+                vsub[(uint64_t)(b0 + j) * kSlots + slot] = e.v;
+            }
+        };
+        auto overflow = [&](const WideEntry &e) {  // the counted excess reserved exactly these places
+            const unsigned long long at = obase + atomicAdd(&onext, 1u);
+            if (at < ovf_cap) ovf[at] = e;
+        };
+        if (held) {
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                if (lo + threadIdx.x + u * THREADS >= hi) continue;
+                const uint32_t at = hp[u];
+                if (at == kReject) {
+                    ++rejected;
+                    continue;
+                }
+                if (at != kStray) {
+                    const uint32_t j = at & 511u, r = at >> 9;
+                    const uint32_t m = fm[j];
+                    if (r < (uint32_t)__builtin_popcount(m)) {
+                        put(j, nth_set_bit16(m, r), hb[u]);
+                        ++placed;
+                        continue;
+                    }
+                }
+                overflow(hb[u]);
+            }
+        }
+        for (uint32_t i0 = lo + threadIdx.x; i0 < hi && !held; i0 += kBatch * THREADS) {
             WideEntry eb[kBatch];
 #pragma unroll
             for (uint32_t u = 0; u < kBatch; ++u) {
@@ -763,22 +836,12 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
                     const uint32_t r = atomicAdd(&cnt[j], 1u);
                     const uint32_t m = fm[j];
                     if (r < (uint32_t)__builtin_popcount(m)) {
-                        const uint32_t slot = wide_nth_set_bit(m, r);
-                        if constexpr (DENSE) {
-                            kimg[j * kSlots + slot] = e.k;
-                            vimg[j * kSlots + slot] = e.v;
-                        } else {
-                            ksub[(uint64_t)b * kSlots + slot] = e.k;
-                            vsub[(uint64_t)b * kSlots + slot] = e.v;
-                        }
+                        put(j, nth_set_bit16(m, r), e);
                         fits = true;
                         ++placed;
                     }
                 }
-                if (!fits) {  // the counted excess reserved exactly these places
-                    const unsigned long long at = obase + atomicAdd(&onext, 1u);
-                    if (at < ovf_cap) ovf[at] = e;
-                }
+                if (!fits) overflow(e);
             }
         }
         if constexpr (DENSE) {
@@ -917,16 +980,16 @@ hipError_t launch_wide_partition(const WideDesc &W, const Chunks &Ch, const uint
 
 hipError_t launch_wide_bulk_fill(const WideDesc &W, const Chunks &Ch, const uint32_t *start, const WideEntry *ent,
                                  uint64_t n, WideEntry *ovf, uint64_t ovf_cap, unsigned long long *ovf_n, Counters *C,
-                                 hipStream_t st)
+                                 bool zeroed, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     if (n >= (1ull << 32) || Ch.total == 0) return hipErrorInvalidValue;
     uint64_t buckets = 0;
     for (uint32_t t = 0; t < W.D.d; ++t) buckets += W.D.nb[t];
     if (n >= 2 * buckets)
-        wide_bulk_fill_kernel<true, 1024><<<Ch.total, 1024, 0, st>>>(W, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C);
+        wide_bulk_fill_kernel<true, 1024><<<Ch.total, 1024, 0, st>>>(W, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C, zeroed);
     else
-        wide_bulk_fill_kernel<false, 256><<<Ch.total, 256, 0, st>>>(W, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C);
+        wide_bulk_fill_kernel<false, 256><<<Ch.total, 256, 0, st>>>(W, Ch, start, ent, n, ovf, ovf_cap, ovf_n, C, false);
     return hipGetLastError();
 }
 
@@ -988,6 +1051,9 @@ struct dyc_wide {
     unsigned long long *hpost = nullptr;       // pinned: the read-back words (launch_fold_post)
     unsigned long long post_seq = 0;
     bool ctr_clean = false;  // the device block is zero and no launch has used it since
+    // every key and value word is zero: set when the table is created or cleared, dropped by
+    // every call that may store an entry (a bulk fill into such a table need not read it)
+    bool zeroed = false;
     uint8_t *live = nullptr;  // downsize: tile flags of the residual list
     uint64_t live_cap = 0;
     uint8_t *upd = nullptr;  // upsert: per-key "updated in place" flags
@@ -1109,6 +1175,7 @@ void wset(dyc_wide *t, uint32_t i, uint8_t *base, uint32_t nb)
 // subtable of t (zeroed when `zero`).
 int winstall_slab(dyc_wide *t, const uint32_t *nb, bool zero, hipStream_t s, uint8_t **region)
 {
+    t->zeroed = false;  // new memory: only the caller knows what it holds
     const uint32_t d = t->W.D.d;
     size_t off[kMaxSubtables + 1] = {0};
     for (uint32_t i = 0; i < d; ++i) off[i + 1] = off[i] + ((2 * wide_bytes(nb[i]) + kWideAlign - 1) & ~(kWideAlign - 1));
@@ -1169,6 +1236,7 @@ int wabsorb(dyc_wide *t, uint64_t room)
 
 int wresize_up(dyc_wide *t, hipStream_t s)
 {
+    t->zeroed = false;
     const uint32_t sub = choose_up(t->W.D.nb, t->W.D.d);
     const uint32_t old_nb = t->W.D.nb[sub];
     if ((uint64_t)old_nb * 2 >= kMaxBuckets) return fail(DYC_ERR_INVALID, "subtable %u cannot grow past 2^29 buckets", sub);
@@ -1197,6 +1265,7 @@ int wresize_up(dyc_wide *t, hipStream_t s)
 // table.
 int wplace_pending(dyc_wide *t, hipStream_t s, uint64_t before)
 {
+    t->zeroed = false;
     uint32_t rounds = 0;
     while (t->pend_n > before && rounds < t->cfg.max_fail_rounds) {
         int rc = wresize_up(t, s);
@@ -1234,6 +1303,7 @@ int wplace_pending(dyc_wide *t, hipStream_t s, uint64_t before)
 
 int wresize_down(dyc_wide *t, hipStream_t s)
 {
+    t->zeroed = false;
     const uint64_t pend_before = t->pend_n;
     const uint32_t sub = choose_down(t->W.D.nb, t->W.D.d);
     const uint32_t old_nb = t->W.D.nb[sub];
@@ -1399,7 +1469,7 @@ int wreserve_bulk(dyc_wide *t, uint64_t n, const Chunks &Ch, hipStream_t s)
 // entries in bk_ent[in]. Partition them by chunk, fill, and read back how many entries overflowed
 // into bk_ent[*out] (the buffer the partition did not end in).
 int wbulk_round(dyc_wide *t, const Chunks &Ch, const uint64_t *keys, const uint64_t *vals, int in, uint64_t n,
-                hipStream_t s, int *out, uint64_t *overflow)
+                bool zeroed, hipStream_t s, int *out, uint64_t *overflow)
 {
     const BulkSpace ws{{t->bk_ent[0], t->bk_ent[1]}, {t->bk_cid[0], t->bk_cid[1]}, t->bk_partial,
                        t->bk_partial + kPartialWords, t->bk_start};
@@ -1407,7 +1477,7 @@ int wbulk_round(dyc_wide *t, const Chunks &Ch, const uint64_t *keys, const uint6
     HIP_TRY(launch_wide_partition(t->W, Ch, keys, vals, in, n, ws, kWideSortBlocks, s, &r));
     HIP_TRY(launch_zero(t->bk_ovf, 1, s));
     HIP_TRY(launch_wide_bulk_fill(t->W, Ch, t->bk_start, t->bk_ent[r], n, t->bk_ent[1 - r], t->bk_cap, t->bk_ovf,
-                                  t->ctr, s));
+                                  t->ctr, zeroed, s));
     HIP_TRY(hipMemcpyAsync(t->bk_hovf, t->bk_ovf, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     *overflow = *t->bk_hovf;
@@ -1420,14 +1490,17 @@ int wbulk_round(dyc_wide *t, const Chunks &Ch, const uint64_t *keys, const uint6
 
 // insert_order 3 for wide keys: fill the start bucket, then the other bucket, then the eviction
 // insert for the rest (the narrow table's bulk_insert over fold(k)).
-int wbulk_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, hipStream_t s)
+int wbulk_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, bool was_zeroed, hipStream_t s)
 {
     const Chunks Ch = make_chunks(t->W.D);  // the workspace was reserved for it
     int ov = 0, ov2 = 0;
     uint64_t m1 = 0, m2 = 0;
-    int rc = wbulk_round(t, Ch, keys, vals, 0, n, s, &ov, &m1);
+    // an empty table (count 0, nothing pending) that no call has stored into since it was
+    // created or cleared holds zeros only
+    const bool zeroed = was_zeroed && t->count == 0 && t->pend_n == 0;
+    int rc = wbulk_round(t, Ch, keys, vals, 0, n, zeroed, s, &ov, &m1);
     if (rc || m1 == 0) return rc;
-    rc = wbulk_round(t, Ch, nullptr, nullptr, ov, m1, s, &ov2, &m2);  // the overflow, in its other table
+    rc = wbulk_round(t, Ch, nullptr, nullptr, ov, m1, false, s, &ov2, &m2);  // the overflow, in its other table
     if (rc || m2 == 0) return rc;
     // the other entry buffer (16 B per key of capacity) holds m2 keys and then m2 values
     uint64_t *const sk = reinterpret_cast<uint64_t *>(t->bk_ent[1 - ov2]);
@@ -1480,6 +1553,7 @@ int dyc_wide_create(const dyc_config *cfg_in, uint64_t capacity_hint, dyc_wide *
     if (rc == DYC_OK) rc = winstall_slab(t, nbs, true, nullptr, nullptr);
     if (rc == DYC_OK) rc = wreserve_pending(t, 1024, nullptr);
     if (rc == DYC_OK && hipStreamSynchronize(nullptr) != hipSuccess) rc = fail(DYC_ERR_HIP, "create sync failed");
+    t->zeroed = rc == DYC_OK;
     if (rc != DYC_OK) {
         char keep[512];
         strncpy(keep, dyc_last_error(), sizeof keep - 1);
@@ -1526,12 +1600,15 @@ int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = 0;
     if (n == 0) return DYC_OK;
+    bool zeroed = t->zeroed;  // every word zero before this call (dropped by any resize below)
+    t->zeroed = false;
     const uint64_t pend_before = t->pend_n;
     int rc;
     if (t->cfg.auto_resize) {
         while ((double)(t->count + n) > t->cfg.upper_bound * (double)t->capacity()) {
             rc = wresize_up(t, s);
             if (rc) return rc;
+            zeroed = false;
         }
     }
     rc = wreserve_pending(t, std::max<uint64_t>(65536, n / 16), s);
@@ -1551,7 +1628,7 @@ int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     rc = wzero(t, s);
     if (rc) return rc;
     if (bulk) {
-        rc = wbulk_insert(t, keys, vals, n, s);
+        rc = wbulk_insert(t, keys, vals, n, zeroed, s);
         if (rc) return rc;
     } else {
         HIP_TRY(launch_wide_insert(t->W, keys, vals, n, kNoTable, false, t->policy(), t->insert_shape(), t->ctr,
@@ -1575,6 +1652,7 @@ int dyc_wide_upsert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     if (!t) return fail(DYC_ERR_INVALID, "null table");
     if (n && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(t->device);
+    t->zeroed = false;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = 0;
     if (n == 0) return DYC_OK;
@@ -1692,6 +1770,7 @@ int dyc_wide_clear(dyc_wide *t, dyc_stream stream)
     t->count = 0;
     t->pend_n = 0;
     t->st.pending = 0;
+    t->zeroed = true;
     return DYC_OK;
 }
 

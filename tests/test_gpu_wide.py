@@ -293,6 +293,37 @@ def test_wide_bulk_fill_insert_multiset_invariant_and_find(D, d):
     t.close()
 
 
+def test_wide_bulk_fill_into_fresh_filled_and_cleared_tables(D):
+    """A fresh or cleared wide table holds zeros only, and its first dense round reads neither
+    its key nor its value lines; any later batch reads them. Batch A into a fresh table, batch B
+    on top (both stored, A's values intact), clear, batch B again (nothing of A survives)."""
+    banner("wide bulk fill: fresh table, second batch on top, then clear and refill")
+    n = 1 << 19
+    keys = keys64(2 * n, 1650)
+    a, b = keys[:n], keys[n:]
+    vals = vals_of(keys)
+    t = D.DyCuckooWide(int(np.ceil(2 * n / 0.9)), insert_order=3, auto_resize=False)
+    assert n >= 2 * sum(t.buckets)  # both batches take the bulk fill
+    t.insert(cu(a), cu(vals[:n]))
+    t.insert(cu(b), cu(vals[n:]))
+    st = t.stats()
+    assert st["count"] == 2 * n and st["lost"] == 0
+    assert oracle_of(t).check_invariant() == 0
+    k, v = wide_stored(t)
+    o = np.lexsort((vals, keys))
+    assert np.array_equal(k, keys[o]) and np.array_equal(v, vals[o])
+    t.clear()
+    t.insert(cu(b), cu(vals[n:]))
+    assert t.stats()["count"] == n
+    assert oracle_of(t).check_invariant() == 0
+    k, v = wide_stored(t)
+    o = np.lexsort((vals[n:], b))
+    assert np.array_equal(k, b[o]) and np.array_equal(v, vals[n:][o])
+    gv, gh = gfind(t, keys)
+    assert not gh[:n].any() and gh[n:].all() and np.array_equal(gv[n:], vals[n:])
+    t.close()
+
+
 def test_wide_bulk_fill_overflow_reaches_the_eviction_insert(D):
     banner("wide bulk fill: overflow through both rounds and the eviction insert")
     base = keys64(3000, 1700)
