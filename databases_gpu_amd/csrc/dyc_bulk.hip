@@ -43,8 +43,15 @@ __global__ __launch_bounds__(kCountThreads) void bulk_pack_kernel(Desc D, Chunks
                                                                   uint32_t *__restrict__ partial)
 {
     __shared__ uint32_t h[kMaxRadix];
+    // per subtable: buckets, FastDiv multiplier and shifts, first chunk -- one LDS read per key
+    // in place of a selection chain over the descriptor's arrays for each of them
+    __shared__ uint4 sub_tab[kMaxSubtables];
     const uint32_t R = 1u << P.bits;
     for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) h[d] = 0;
+    if (threadIdx.x < kMaxSubtables) {
+        const uint32_t t = threadIdx.x;
+        sub_tab[t] = make_uint4(selt(t, D.nb), selt(t, D.m), selt(t, D.sh), chunk_base(Ch, t));
+    }
     __syncthreads();
     const uint64_t lo = count_begin(G, blockIdx.x);
     const uint64_t hi = count_end(G, lo);
@@ -67,7 +74,8 @@ __global__ __launch_bounds__(kCountThreads) void bulk_pack_kernel(Desc D, Chunks
                 const uint32_t p = pair_of(D, k[u]);
                 uint32_t t = (k[u] & 1u) ? pos2(D, p) : pos1(D, p);
                 if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
-                c = chunk_base(Ch, t) + (bucket_of(D, k[u], t) >> kChunkShift);
+                const uint4 q = sub_tab[t];
+                c = q.w + (fast_mod_sh(hash_t(k[u], t), q.x, q.y, q.z) >> kChunkShift);
             }
             cid[i] = (CidT)c;
             atomicAdd(&h[digit_of(c, P)], 1u);

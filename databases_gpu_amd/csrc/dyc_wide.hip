@@ -583,8 +583,15 @@ __global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc 
 {
     const Desc &D = W.D;
     __shared__ uint32_t h[kMaxRadix];
+    // per subtable: buckets, FastDiv multiplier and shifts, first chunk -- one LDS read per key
+    // in place of a selection chain over the descriptor's arrays for each of them
+    __shared__ uint4 sub_tab[kMaxSubtables];
     const uint32_t R = 1u << P.bits;
     for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) h[d] = 0;
+    if (threadIdx.x < kMaxSubtables) {
+        const uint32_t t = threadIdx.x;
+        sub_tab[t] = make_uint4(selt(t, D.nb), selt(t, D.m), selt(t, D.sh), chunk_base(Ch, t));
+    }
     __syncthreads();
     const uint64_t lo = count_begin(G, blockIdx.x);
     const uint64_t hi = count_end(G, lo);
@@ -607,7 +614,8 @@ __global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc 
                 const uint32_t f = fold64(k[u]), p = pair_of(D, f);
                 uint32_t t = (k[u] & 1u) ? pos2(D, p) : pos1(D, p);
                 if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
-                c = chunk_base(Ch, t) + (bucket_of(D, f, t) >> kChunkShift);
+                const uint4 q = sub_tab[t];
+                c = q.w + (fast_mod_sh(hash_t(f, t), q.x, q.y, q.z) >> kChunkShift);
             }
             cid[i] = (CidT)c;
             atomicAdd(&h[digit_of(c, P)], 1u);
