@@ -1114,14 +1114,20 @@ def main() -> int:
             tr = json.load(open(traffic_path))
             tcfg = tr.get("config", {})
             # a measurement applies to the configuration it was taken in (the grid does not change the
-            # bytes per key; the eviction bound and the insert order do)
-            same = {"insert": ("insert_lanes", "kick_bound", "insert_order"), "find_uniform": ("find_lanes",),
-                    "find_zipf": ("find_lanes",)}
+            # bytes per key; the eviction bound and the insert order do, and so do the eviction
+            # insert's lanes when that insert places every key; under the bulk fill it places only
+            # the few keys both fill rounds leave, so its shape is named, not required to match)
+            same = {"insert": ("kick_bound", "insert_order") + (() if cfg["insert_order"] == 3 else ("insert_lanes",)),
+                    "find_uniform": ("find_lanes",), "find_zipf": ("find_lanes",)}
             for k in kernels:
                 if k in tr and tr[k].get("log2n") == args.log2n and all(tcfg.get(f) == cfg[f] for f in same[k]):
                     kernels[k]["traffic"] = tr[k]["bytes_per_launch"]
-                    kernels[k]["traffic_source"] = "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; " \
-                                                   "same configuration, stored, not measured in this run)"
+                    src = "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; same configuration, " \
+                          "stored, not measured in this run)"
+                    if k == "insert" and tcfg.get("insert_lanes") != cfg["insert_lanes"]:
+                        src = src[:-1] + "; its eviction stage ran %s lanes per key, this run's %s)" % (
+                            tcfg.get("insert_lanes"), cfg["insert_lanes"])
+                    kernels[k]["traffic_source"] = src
                     if "l2_hit_rate" in tr[k]:
                         kernels[k]["l2_hit_rate"] = tr[k]["l2_hit_rate"]
         except (OSError, ValueError, KeyError):
