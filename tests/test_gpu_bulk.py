@@ -141,6 +141,37 @@ def test_bulk_fill_into_fresh_filled_and_cleared_tables(D):
     t.close()
 
 
+def test_bulk_fill_chunk_beyond_the_register_hold(D):
+    """A dense-round chunk with more entries than its threads hold in registers (8192 for the
+    1024-thread fill) counts them, then reads them again to place them. 9000 keys whose start
+    bucket lies in subtable 0 (one chunk: the subtables have fewer than 512 buckets) plus 2000
+    others: subtable 0 fills, the rest overflows into the other bucket of each pair."""
+    banner("bulk fill: one chunk of 9000 entries (beyond the register hold)")
+    pool = unique_keys(60_000, 1550)
+    pairs = O.pair_table(4)
+    p = O.hash_batch(pool, 6)
+    start = np.array([pairs[x][1] if k & 1 else pairs[x][0] for x, k in zip(p.tolist(), pool.tolist())])
+    keys = np.concatenate([pool[start == 0][:9000], pool[start != 0][:2000]])
+    assert keys.size == 11_000
+    rng = np.random.default_rng(1551)
+    rng.shuffle(keys)
+    vals = vals_of(keys)
+    t = D.DyCuckoo(20_000, insert_order=3, auto_resize=False)
+    nb = t.buckets
+    assert max(nb) < 512 and keys.size >= 2 * sum(nb) and 9000 > 16 * nb[0]
+    t.insert(cu(keys), cu(vals))
+    st = t.stats()
+    assert st["count"] == keys.size and st["lost"] == 0
+    ents, ref = stored(t)
+    assert ref.check_invariant() == 0
+    assert np.array_equal(ents, np.sort(O.pack(keys, vals)))
+    # subtable 0 is (nearly) full: a bucket may draw fewer than 16 of the 9000 start keys
+    assert np.count_nonzero(t.dump()[0]) >= 0.99 * 16 * nb[0]
+    gv, gh = gfind(t, keys)
+    assert gh.all() and np.array_equal(gv, vals)
+    t.close()
+
+
 def test_bulk_fill_overflow_reaches_the_eviction_insert(D):
     """Many copies of a few keys overflow their start bucket and their other bucket, so all three
     stages run; every copy is stored or pending, and the placement invariant holds."""
