@@ -101,6 +101,7 @@ class Stats(ctypes.Structure):
         ("lost", ctypes.c_uint64),
         ("subtables", ctypes.c_uint32),
         ("last_max_chain", ctypes.c_uint32),
+        ("last_filled", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -38,6 +38,22 @@ struct dyc_group {
     size_t items_cap = 0;
     hipEvent_t items_free = nullptr;  // the last upload from hitems has completed
     dyc_stats st{};
+    // every slot of every table is zero: set at create and clear, dropped by every insert
+    bool zeroed = false;
+    // bulk fill (cfg.insert_order 3): partitioned entries and the overflow (8 B per key each),
+    // local chunk ids (1 B per key), per-block chunk counts, per-chunk start / count / table,
+    // the participating tables (device and pinned) and their overflow counts (device and pinned);
+    // grown on demand, kept by the group
+    uint64_t *bk_ent = nullptr, *bk_ovf = nullptr;
+    uint8_t *bk_lcid = nullptr;
+    uint64_t bk_cap = 0;
+    uint32_t *bk_partial = nullptr;
+    uint64_t bk_partial_cap = 0;
+    uint32_t *bk_cstart = nullptr, *bk_ccount = nullptr, *bk_ctable = nullptr;
+    uint64_t bk_chunk_cap = 0;
+    BulkTable *bk_tabs = nullptr, *bk_htabs = nullptr;
+    unsigned long long *bk_ovfn = nullptr, *bk_hovfn = nullptr;
+    uint32_t bk_tabs_cap = 0;
 
     InsertPolicy policy() const { return InsertPolicy{launch.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
     // The pending list is the group's stash: entries tagged with their table, searched by finds
@@ -103,12 +119,12 @@ int check_offsets(const dyc_group *g, const uint64_t *offsets)
     return DYC_OK;
 }
 
-// Cut the batch into items and upload them; returns the item count in *n_items.
-int upload_items(dyc_group *g, const uint64_t *offsets, hipStream_t s, uint32_t *n_items)
+// Cut segments (begin, len, table) into items of at most `per` indexes and upload them; returns
+// the item count in *n_items.
+int upload_segments(dyc_group *g, const std::vector<GroupItem> &segs, uint64_t per, hipStream_t s, uint32_t *n_items)
 {
-    const uint64_t per = g->launch.item_keys ? g->launch.item_keys : kDefaultItemKeys;
     uint64_t need = 0;
-    for (uint32_t i = 0; i < g->n; ++i) need += (offsets[i + 1] - offsets[i] + per - 1) / per;
+    for (const GroupItem &x : segs) need += (x.len + per - 1) / per;
     if (need > 0xffffffffull) return fail(DYC_ERR_INVALID, "batch too large for 32-bit item indexes");
     HIP_TRY(hipEventSynchronize(g->items_free));  // hitems is free to rewrite
     if (need > g->items_cap) {
@@ -127,13 +143,27 @@ int upload_items(dyc_group *g, const uint64_t *offsets, hipStream_t s, uint32_t 
         g->items_cap = cap;
     }
     uint32_t m = 0;
-    for (uint32_t i = 0; i < g->n; ++i)
-        for (uint64_t b = offsets[i]; b < offsets[i + 1]; b += per)
-            g->hitems[m++] = GroupItem{b, (uint32_t)std::min<uint64_t>(per, offsets[i + 1] - b), i};
+    for (const GroupItem &x : segs)
+        for (uint64_t b = x.begin; b < x.begin + x.len; b += per)
+            g->hitems[m++] = GroupItem{b, (uint32_t)std::min<uint64_t>(per, x.begin + x.len - b), x.table};
     if (m) HIP_TRY(hipMemcpyAsync(g->ditems, g->hitems, m * sizeof(GroupItem), hipMemcpyHostToDevice, s));
     HIP_TRY(hipEventRecord(g->items_free, s));
     *n_items = m;
     return DYC_OK;
+}
+
+// Cut the batch into items (at most item_keys keys of one table) and upload them.
+int upload_items(dyc_group *g, const uint64_t *offsets, hipStream_t s, uint32_t *n_items)
+{
+    const uint64_t per = g->launch.item_keys ? g->launch.item_keys : kDefaultItemKeys;
+    std::vector<GroupItem> segs;
+    segs.reserve(g->n);
+    for (uint32_t i = 0; i < g->n; ++i) {
+        const uint64_t len = offsets[i + 1] - offsets[i];
+        if (len > 0xffffffffull) return fail(DYC_ERR_INVALID, "table %u's batch is too large for 32-bit lengths", i);
+        if (len) segs.push_back(GroupItem{offsets[i], (uint32_t)len, i});
+    }
+    return upload_segments(g, segs, per, s, n_items);
 }
 
 int zero_and_run_prologue(dyc_group *g, hipStream_t s)
@@ -194,6 +224,195 @@ int finish_insert(dyc_group *g, uint64_t room)
     if (c.pending)
         return fail(DYC_ERR_PENDING, "%llu entries exhausted the eviction bound (group tables do not resize)",
                     (unsigned long long)c.pending);
+    return DYC_OK;
+}
+
+// ------------------------------------------------------------------------------ bulk fill
+int bulk_alloc(void **p, size_t bytes, const char *what)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(DYC_ERR_OOM, "hipMalloc of the group bulk-fill %s (%zu bytes) failed", what, bytes);
+    }
+    return DYC_OK;
+}
+
+// Grow-only workspace of a bulk-filled batch: n keys, `blocks` partition blocks, `chunks` chunks
+// and `tabs` participating tables. The caller has synchronized the stream (nothing in flight uses
+// a buffer being replaced).
+int reserve_group_bulk(dyc_group *g, uint64_t n, uint64_t blocks, uint64_t chunks, uint32_t tabs)
+{
+    int rc = DYC_OK;
+    auto regrow = [&](void **p, size_t bytes, const char *what) {
+        if (rc) return;
+        if (*p) (void)hipFree(*p);
+        rc = bulk_alloc(p, bytes, what);
+    };
+    if (n > g->bk_cap) {
+        g->bk_cap = 0;
+        regrow((void **)&g->bk_ent, n * 8, "entries");
+        regrow((void **)&g->bk_ovf, n * 8, "overflow list");
+        regrow((void **)&g->bk_lcid, n, "chunk ids");
+        if (rc) return rc;
+        g->bk_cap = n;
+    }
+    if (blocks * kGroupBulkChunks > g->bk_partial_cap) {
+        g->bk_partial_cap = 0;
+        regrow((void **)&g->bk_partial, blocks * kGroupBulkChunks * sizeof(uint32_t), "block counts");
+        if (rc) return rc;
+        g->bk_partial_cap = blocks * kGroupBulkChunks;
+    }
+    if (chunks > g->bk_chunk_cap) {
+        g->bk_chunk_cap = 0;
+        regrow((void **)&g->bk_cstart, chunks * sizeof(uint32_t), "chunk starts");
+        regrow((void **)&g->bk_ccount, chunks * sizeof(uint32_t), "chunk counts");
+        regrow((void **)&g->bk_ctable, chunks * sizeof(uint32_t), "chunk tables");
+        if (rc) return rc;
+        g->bk_chunk_cap = chunks;
+    }
+    if (tabs > g->bk_tabs_cap) {
+        g->bk_tabs_cap = 0;
+        regrow((void **)&g->bk_tabs, tabs * sizeof(BulkTable), "table list");
+        regrow((void **)&g->bk_ovfn, tabs * sizeof(unsigned long long), "overflow counts");
+        if (rc) return rc;
+        if (g->bk_htabs) (void)hipHostFree(g->bk_htabs);
+        if (g->bk_hovfn) (void)hipHostFree(g->bk_hovfn);
+        g->bk_htabs = nullptr;
+        g->bk_hovfn = nullptr;
+        if (hipHostMalloc((void **)&g->bk_htabs, tabs * sizeof(BulkTable), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&g->bk_hovfn, tabs * sizeof(unsigned long long), hipHostMallocDefault) !=
+                hipSuccess) {
+            (void)hipGetLastError();
+            return fail(DYC_ERR_OOM, "pinned group bulk-fill table list failed");
+        }
+        g->bk_tabs_cap = tabs;
+    }
+    return DYC_OK;
+}
+
+// One fill round of a group bulk insert over the tables `tabs` (their off and n set; the rest is
+// filled in here): round 1 reads keys/vals, round 2 the packed entries in bk_ovf. The batch is
+// partitioned per table into bk_ent, every chunk is filled, and what overflows lands in the
+// table's region of bk_ovf; *ovf receives the overflow count of each table (synchronous).
+int group_bulk_round(dyc_group *g, std::vector<BulkTable> &tabs, const uint32_t *keys, const uint32_t *vals,
+                     const uint64_t *entries, uint64_t total, bool zeroed, hipStream_t s, std::vector<uint64_t> *ovf)
+{
+    std::vector<GroupItem> blocks;
+    uint64_t buckets = 0, entries_n = 0;
+    uint32_t chunks = 0;
+    for (BulkTable &T : tabs) {
+        const Desc &D = g->hdir[T.table];
+        T.first_block = (uint32_t)blocks.size();
+        for (uint64_t b = T.off; b < T.off + T.n; b += kGroupBulkBlockKeys)
+            blocks.push_back(GroupItem{b, (uint32_t)std::min<uint64_t>(kGroupBulkBlockKeys, T.off + T.n - b), T.table});
+        T.blocks = (uint32_t)blocks.size() - T.first_block;
+        T.chunk0 = chunks;
+        chunks += T.chunks;
+        entries_n += T.n;
+        for (uint32_t t = 0; t < D.d; ++t) buckets += D.nb[t];
+    }
+    const uint32_t J = (uint32_t)tabs.size(), NB = (uint32_t)blocks.size();
+    // dense (chunk images through LDS) from two entries per bucket, as for one table
+    const bool dense = entries_n >= 2 * buckets;
+    HIP_TRY(hipStreamSynchronize(s));  // buffers being replaced are no longer in use
+    int rc = reserve_group_bulk(g, total, NB, chunks, J);
+    if (rc) return rc;
+    uint32_t n_up = 0;
+    rc = upload_segments(g, blocks, kGroupBulkBlockKeys, s, &n_up);
+    if (rc) return rc;
+    if (n_up != NB) return fail(DYC_ERR_CORRUPT, "group bulk fill: %u blocks uploaded for %u", n_up, NB);
+    memcpy(g->bk_htabs, tabs.data(), J * sizeof(BulkTable));
+    HIP_TRY(hipMemcpyAsync(g->bk_tabs, g->bk_htabs, J * sizeof(BulkTable), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_group_bulk_pack(g->ddir, g->ditems, NB, keys, entries, g->bk_lcid, g->bk_partial, s));
+    HIP_TRY(launch_group_bulk_scan(g->bk_tabs, J, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s));
+    HIP_TRY(launch_group_bulk_scatter(g->ditems, NB, keys, vals, entries, g->bk_lcid, g->bk_partial, total, g->bk_ent,
+                                      s));
+    HIP_TRY(launch_zero(g->bk_ovfn, J, s));
+    HIP_TRY(launch_group_bulk_fill(g->ddir, g->bk_tabs, chunks, g->bk_cstart, g->bk_ccount, g->bk_ctable, g->bk_ent,
+                                   g->bk_ovf, g->bk_ovfn, g->tcounts, g->ctr, dense, dense && zeroed, s));
+    HIP_TRY(hipMemcpyAsync(g->bk_hovfn, g->bk_ovfn, J * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    ovf->resize(J);
+    for (uint32_t j = 0; j < J; ++j) {
+        (*ovf)[j] = g->bk_hovfn[j];
+        if ((*ovf)[j] > tabs[j].n)
+            return fail(DYC_ERR_CORRUPT, "group bulk fill: table %u overflowed %llu of %u entries", tabs[j].table,
+                        (unsigned long long)(*ovf)[j], tabs[j].n);
+    }
+    return DYC_OK;
+}
+
+// insert_order 3 for a group: each table's batch is partitioned by its own chunks in one pass
+// (the batch is grouped by table already) and every chunk is filled by one workgroup; what
+// overflows its start bucket gets a second round for the other bucket of its pair, and what
+// overflows that goes through the group's eviction insert. Declines (*taken false, nothing
+// launched) when a participating table has more than kGroupBulkChunks chunks, the batch has
+// fewer than two keys per bucket of those tables or 2^32 keys or more, or the workspace cannot
+// be allocated. Counters and per-table counts are zeroed here and read by the caller; *left
+// receives the entries handed to the eviction insert.
+int group_bulk_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *offsets, bool zeroed,
+                      hipStream_t s, uint64_t room, bool *taken, uint64_t *left)
+{
+    *taken = false;
+    *left = 0;
+    const uint64_t total = offsets[g->n];
+    if (total == 0 || total >= (1ull << 32)) return DYC_OK;
+    std::vector<BulkTable> tabs;
+    uint64_t buckets = 0, chunks = 0, blocks = 0;
+    for (uint32_t i = 0; i < g->n; ++i) {
+        const uint64_t n = offsets[i + 1] - offsets[i];
+        if (!n) continue;
+        const Desc &D = g->hdir[i];
+        uint32_t ch = 0;
+        for (uint32_t t = 0; t < D.d; ++t) {
+            ch += (D.nb[t] + kChunkBuckets - 1) / kChunkBuckets;
+            buckets += D.nb[t];
+        }
+        if (ch > kGroupBulkChunks) return DYC_OK;
+        tabs.push_back(BulkTable{offsets[i], (uint32_t)n, i, 0, 0, 0, ch});
+        chunks += ch;
+        blocks += (n + kGroupBulkBlockKeys - 1) / kGroupBulkBlockKeys;
+    }
+    if (total < 2 * buckets) return DYC_OK;
+    {
+        HIP_TRY(hipStreamSynchronize(s));
+        const dyc::host::ErrorKeep keep;
+        const int rc = reserve_group_bulk(g, total, blocks, chunks, (uint32_t)tabs.size());
+        if (rc == DYC_ERR_OOM) {  // no room for the workspace: the eviction insert needs none
+            keep.restore();
+            return DYC_OK;
+        }
+        if (rc) return rc;
+    }
+    *taken = true;
+    int rc = zero_and_run_prologue(g, s);
+    if (rc) return rc;
+    std::vector<uint64_t> ovf;
+    rc = group_bulk_round(g, tabs, keys, vals, nullptr, total, zeroed, s, &ovf);
+    if (rc) return rc;
+    std::vector<BulkTable> tabs2;  // round 2: each table's overflow, for the other bucket
+    for (uint32_t j = 0; j < tabs.size(); ++j)
+        if (ovf[j]) tabs2.push_back(BulkTable{tabs[j].off, (uint32_t)ovf[j], tabs[j].table, 0, 0, 0, tabs[j].chunks});
+    if (!tabs2.empty()) {
+        rc = group_bulk_round(g, tabs2, nullptr, nullptr, g->bk_ovf, total, false, s, &ovf);
+        if (rc) return rc;
+    }
+    // the overflow of table j sits in bk_ovf[off, off + count): the eviction insert's items
+    std::vector<GroupItem> segs;
+    for (uint32_t j = 0; j < tabs2.size(); ++j)
+        if (ovf[j]) {
+            segs.push_back(GroupItem{tabs2[j].off, (uint32_t)ovf[j], tabs2[j].table});
+            *left += ovf[j];
+        }
+    uint32_t n_items = 0;
+    const uint64_t per = g->launch.item_keys ? g->launch.item_keys : kDefaultItemKeys;
+    rc = upload_segments(g, segs, per, s, &n_items);
+    if (rc) return rc;
+    HIP_TRY(launch_group_insert(g->ddir, g->ditems, n_items, nullptr, nullptr, g->policy(), (int)g->launch.insert_lanes,
+                                (int)g->launch.insert_blocks, g->ctr,
+                                PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, 0, 0, nullptr}, g->tcounts,
+                                s, 0, g->bk_ovf));
     return DYC_OK;
 }
 
@@ -277,6 +496,7 @@ int dyc_group_create(const dyc_config *cfg_in, uint32_t n_tables, const uint64_t
     }
     rc = reserve_pending(g, 65536, nullptr);
     if (rc) return cleanup(rc);
+    g->zeroed = true;
     *out = g;
     return DYC_OK;
 }
@@ -297,6 +517,12 @@ int dyc_group_destroy(dyc_group *g)
     if (g->ditems) (void)hipFree(g->ditems);
     if (g->hitems) (void)hipHostFree(g->hitems);
     if (g->items_free) (void)hipEventDestroy(g->items_free);
+    for (void *p : {(void *)g->bk_ent, (void *)g->bk_ovf, (void *)g->bk_lcid, (void *)g->bk_partial,
+                    (void *)g->bk_cstart, (void *)g->bk_ccount, (void *)g->bk_ctable, (void *)g->bk_tabs,
+                    (void *)g->bk_ovfn})
+        if (p) (void)hipFree(p);
+    if (g->bk_htabs) (void)hipHostFree(g->bk_htabs);
+    if (g->bk_hovfn) (void)hipHostFree(g->bk_hovfn);
     delete g;
     return DYC_OK;
 }
@@ -327,14 +553,29 @@ int dyc_group_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, c
     if (total && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
+    const bool zeroed = g->zeroed;
+    g->zeroed = false;
+    g->st.last_filled = 0;
     rc = reserve_pending(g, std::max<uint64_t>(65536, total / 16), s);
     if (rc) return rc;
+    const uint64_t room = g->pend_cap - g->pend_n;
+    if (g->cfg.insert_order == 3) {
+        bool taken = false;
+        uint64_t left = 0;
+        rc = group_bulk_insert(g, keys, vals, offsets, zeroed, s, room, &taken, &left);
+        if (rc) return rc;
+        if (taken) {
+            rc = read_back(g, s);
+            if (rc) return rc;
+            g->st.last_filled = total - g->hctr->rejected - left;
+            return finish_insert(g, room);
+        }
+    }
     uint32_t n_items = 0;
     rc = upload_items(g, offsets, s, &n_items);
     if (rc) return rc;
     rc = zero_and_run_prologue(g, s);
     if (rc) return rc;
-    const uint64_t room = g->pend_cap - g->pend_n;
     HIP_TRY(launch_group_insert(g->ddir, g->ditems, n_items, keys, vals, g->policy(), (int)g->launch.insert_lanes,
                                 (int)g->launch.insert_blocks, g->ctr,
                                 PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, 0, 0, nullptr}, g->tcounts,
@@ -393,6 +634,8 @@ int dyc_group_insert_per_table(dyc_group *g, const uint32_t *keys, const uint32_
     if (offsets[g->n] && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
+    g->zeroed = false;
+    g->st.last_filled = 0;
     rc = reserve_pending(g, std::max<uint64_t>(65536, offsets[g->n] / 16), s);
     if (rc) return rc;
     rc = zero_and_run_prologue(g, s);
@@ -594,6 +837,8 @@ int dyc_group_plan_run(dyc_group_plan *p, dyc_stream stream)
                                      "it): make the plan again");
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
+    g->zeroed = false;
+    g->st.last_filled = 0;
     HIP_TRY(hipGraphLaunch(p->exec, s));
     HIP_TRY(hipStreamSynchronize(s));
     fold_tallies(g->hctr);
@@ -611,6 +856,7 @@ int dyc_group_clear(dyc_group *g, dyc_stream stream)
     std::fill(g->count.begin(), g->count.end(), 0);
     g->pend_n = 0;
     g->st.pending = 0;
+    g->zeroed = true;
     return DYC_OK;
 }
 

@@ -920,14 +920,16 @@ __global__ __launch_bounds__(256, W) void group_probe_kernel(const Desc *__restr
 }
 
 // Groups of one wave share an item: they are fed from it until it is used up, and the wave
-// moves on only when all its groups have placed (or pended) their entries.
-template <int G, int W>
+// moves on only when all its groups have placed (or pended) their entries. FROM_ENTRIES: the
+// items index packed entries (a bulk fill's overflow) instead of keys/vals.
+template <int G, int W, bool FROM_ENTRIES = false>
 __global__ __launch_bounds__(256, W) void group_insert_kernel(const Desc *__restrict__ dir,
                                                            const GroupItem *__restrict__ items, uint32_t n_items,
                                                            const uint32_t *__restrict__ keys,
                                                            const uint32_t *__restrict__ vals, InsertPolicy P,
                                                            Counters *__restrict__ C, PendSink sink,
-                                                           unsigned long long *__restrict__ table_counts)
+                                                           unsigned long long *__restrict__ table_counts,
+                                                           const uint64_t *__restrict__ entries = nullptr)
 {
     Carry<kSlots / G> c;
     c.active = false;
@@ -942,7 +944,7 @@ __global__ __launch_bounds__(256, W) void group_insert_kernel(const Desc *__rest
         const unsigned long long rej0 = tally.rejected;
         uint64_t taken = 0;
         while (true) {
-            refill<G, false>(D, c, next, end, keys, vals, nullptr, kNoTable, tally, taken);
+            refill<G, FROM_ENTRIES>(D, c, next, end, keys, vals, entries, kNoTable, tally, taken);
             if (!__ballot(c.active)) break;  // item used up and every group idle
             if (c.active) kick_step<G>(D, c, P, C, sink, item.table, tally);
         }
@@ -1644,8 +1646,13 @@ hipError_t launch_group_erase(const Desc *dir, const GroupItem *items, uint32_t 
 template <int G>
 static void group_insert_w(int wpe, int grid, hipStream_t st, const Desc *dir, const GroupItem *items,
                            uint32_t n_items, const uint32_t *keys, const uint32_t *vals, InsertPolicy pol, Counters *C,
-                           PendSink sink, unsigned long long *table_counts)
+                           PendSink sink, unsigned long long *table_counts, const uint64_t *entries)
 {
+    if (entries) {  // a bulk fill's overflow: the compiler's occupancy only
+        group_insert_kernel<G, 1, true>
+            <<<grid, 256, 0, st>>>(dir, items, n_items, nullptr, nullptr, pol, C, sink, table_counts, entries);
+        return;
+    }
     switch (wpe) {
     case 6:
         group_insert_kernel<G, 6><<<grid, 256, 0, st>>>(dir, items, n_items, keys, vals, pol, C, sink, table_counts);
@@ -1661,17 +1668,19 @@ static void group_insert_w(int wpe, int grid, hipStream_t st, const Desc *dir, c
 
 hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                                const uint32_t *vals, InsertPolicy pol, int lanes, int blocks, Counters *C,
-                               PendSink sink, unsigned long long *table_counts, hipStream_t st, int waves_per_eu)
+                               PendSink sink, unsigned long long *table_counts, hipStream_t st, int waves_per_eu,
+                               const uint64_t *entries)
 {
     if (n_items == 0) return hipSuccess;
     const int grid = blocks > 0 ? blocks : std::min<int>(kGroupDefaultBlocks, (int)((n_items + 3) / 4));
     const int w = waves_per_eu;
+    const uint64_t *const e = entries;
     switch (lanes) {
-    case 2: group_insert_w<2>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
-    case 4: group_insert_w<4>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
-    case 8: group_insert_w<8>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
-    case 16: group_insert_w<16>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
-    default: group_insert_w<1>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts); break;
+    case 2: group_insert_w<2>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts, e); break;
+    case 4: group_insert_w<4>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts, e); break;
+    case 8: group_insert_w<8>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts, e); break;
+    case 16: group_insert_w<16>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts, e); break;
+    default: group_insert_w<1>(w, grid, st, dir, items, n_items, keys, vals, pol, C, sink, table_counts, e); break;
     }
     return hipGetLastError();
 }

@@ -182,10 +182,52 @@ hipError_t launch_group_erase(const Desc *dir, const GroupItem *items, uint32_t 
                               hipStream_t st, const Pool *pool = nullptr, int waves_per_eu = 0);
 // insert: failures go to sink.pend with their table index in sink.tags; per-table inserted
 // counts (placed or pended, reserved keys excluded) are added to table_counts.
+// insert: an item's entries are keys/vals[i], or entries[i] (packed) when entries is not null.
 hipError_t launch_group_insert(const Desc *dir, const GroupItem *items, uint32_t n_items, const uint32_t *keys,
                                const uint32_t *vals, InsertPolicy pol, int lanes, int blocks, Counters *C,
                                PendSink sink, unsigned long long *table_counts, hipStream_t st,
-                               int waves_per_eu = 0);
+                               int waves_per_eu = 0, const uint64_t *entries = nullptr);
+
+// ---- bulk fill of many tables (dyc_group.hip, insert_order 3)
+// The batch is grouped by table already (host offsets), so one pass partitions each table's
+// entries by its own chunks (at most kGroupBulkChunks per table, one 8-bit digit): a table's
+// local chunk lc numbers its chunks subtable by subtable (chunks of 512 buckets, the last of a
+// subtable shorter), and the call's chunks are numbered table by table (chunk0).
+constexpr uint32_t kGroupBulkChunks = 256;
+constexpr uint32_t kGroupBulkBlockKeys = 16384;  // entries per partition block (within one table)
+struct BulkTable {
+    uint64_t off;                  // its entries' first index in the batch
+    uint32_t n;                    // its entries
+    uint32_t table;                // its index in the group
+    uint32_t first_block, blocks;  // its partition blocks (GroupItems, contiguous)
+    uint32_t chunk0, chunks;       // its chunks in the call's numbering
+};
+static_assert(sizeof(BulkTable) == 32, "BulkTable is uploaded as a host array");
+// Per partition block (blocks[b]: one table's range): every key's local chunk into lcid, and the
+// block's count of each local chunk into partial[b * 256 + lc]. Round 1 (entries null): keys,
+// chunk of the start bucket (reserved keys count as chunk 0, whose fill rejects them). Round 2:
+// packed entries (a fill's overflow), chunk of the other bucket of the key's pair.
+hipError_t launch_group_bulk_pack(const Desc *dir, const GroupItem *blocks, uint32_t n_blocks, const uint32_t *keys,
+                                  const uint64_t *entries, uint8_t *lcid, uint32_t *partial, hipStream_t st);
+// Per table: partial[b * 256 + lc] becomes the place of block b's first entry of chunk lc in the
+// partitioned batch, and chunk c = chunk0 + lc of the call gets cstart[c], ccount[c] and
+// ctable[c] (the table's index in tabs).
+hipError_t launch_group_bulk_scan(const BulkTable *tabs, uint32_t n_tabs, uint32_t *partial, uint32_t *cstart,
+                                  uint32_t *ccount, uint32_t *ctable, hipStream_t st);
+// The pass itself: keys/vals (or packed entries, when not null) of every block to ent[0, n) by
+// local chunk (no stable order).
+hipError_t launch_group_bulk_scatter(const GroupItem *blocks, uint32_t n_blocks, const uint32_t *keys,
+                                     const uint32_t *vals, const uint64_t *entries, const uint8_t *lcid,
+                                     const uint32_t *partial, uint64_t n, uint64_t *ent, hipStream_t st);
+// One workgroup per chunk of the call (dense: the chunk's image in LDS; sparse: stores into the
+// free slots of the buckets that received entries). Entries that do not fit go to the table's
+// region of ovf, ovf[off, off + n) of its BulkTable, counted in ovf_n[j] (j its index in tabs;
+// zeroed by the caller). Placements are added to table_counts and to the kick tally, reserved
+// keys to the rejected tally. zeroed (dense only): every slot of every table is zero.
+hipError_t launch_group_bulk_fill(const Desc *dir, const BulkTable *tabs, uint32_t n_chunks, const uint32_t *cstart,
+                                  const uint32_t *ccount, const uint32_t *ctable, const uint64_t *ent, uint64_t *ovf,
+                                  unsigned long long *ovf_n, unsigned long long *table_counts, Counters *C, bool dense,
+                                  bool zeroed, hipStream_t st);
 
 // ---- serving ticks
 // A work item of a serving tick: keys [0, len) of one request (pointers already offset to the

@@ -308,11 +308,31 @@ __device__ inline uint64_t match_mask(uint64_t vm, uint32_t d, uint32_t bits)
 //        when the entry added itself, in whatever order the lanes' atomics were applied. The
 //        first pass of a least-significant-digit partition may do this (its input order means
 //        nothing); every later pass must keep the order of the previous digits.
-template <typename Entry, typename Src, typename CidT, int THREADS, int BITS>
-__global__ __launch_bounds__(THREADS) void part_scatter_kernel(Src src, const CidT *__restrict__ cid, PartGrid G,
-                                                               RadixPass P, const uint32_t *__restrict__ partial,
-                                                               const uint32_t *__restrict__ total,
-                                                               Entry *__restrict__ ent_out, CidT *__restrict__ cid_out)
+// Where block b's entries lie and where its entries of each digit go (Layout):
+//   begin(b), end(b, begin): its input range;
+//   base(b, d, R, tmp): the global place of its first entry of digit d (every thread of the block
+//   calls it, d = threadIdx.x, and uses the result for d < R; tmp: kScanWaves words of LDS);
+//   n: the entry count (no place reaches it).
+// PassLayout is a digit pass over the whole batch: fixed block ranges (PartGrid), places from
+// the per-block digit counts after part_colscan_kernel and the digit totals.
+struct PassLayout {
+    PartGrid G;
+    const uint32_t *partial, *total;
+    uint64_t n;
+    __device__ uint64_t begin(uint32_t b) const { return part_begin(G, b); }
+    __device__ uint64_t end(uint32_t, uint64_t lo) const { return part_end(G, lo); }
+    __device__ uint32_t base(uint32_t b, uint32_t d, uint32_t R, uint32_t *tmp) const
+    {
+        uint32_t all;
+        const uint32_t dbase = block_exclusive_scan(d < R ? total[d] : 0u, tmp, &all);
+        return d < R ? dbase + partial[(uint64_t)b * G.sub * R + d] : 0u;
+    }
+};
+
+template <typename Entry, typename Src, typename CidT, int THREADS, int BITS, typename Layout = PassLayout>
+__global__ __launch_bounds__(THREADS) void part_scatter_kernel(Src src, const CidT *__restrict__ cid, Layout Lay,
+                                                               RadixPass P, Entry *__restrict__ ent_out,
+                                                               CidT *__restrict__ cid_out)
 {
     constexpr int kWaves = THREADS / kWave;
     constexpr uint32_t kTile = THREADS * kSortItems;
@@ -327,13 +347,11 @@ __global__ __launch_bounds__(THREADS) void part_scatter_kernel(Src src, const Ci
     const uint32_t R = 1u << P.bits;
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid / kWave;
     {
-        const uint32_t t = tid < R ? total[tid] : 0u;
-        uint32_t all;
-        const uint32_t dbase = block_exclusive_scan(t, tmp, &all);
-        if (tid < R) gbase[tid] = dbase + partial[(uint64_t)blockIdx.x * G.sub * R + tid];
+        const uint32_t b = Lay.base(blockIdx.x, tid, R, tmp);
+        if (tid < R) gbase[tid] = b;
     }
-    const uint64_t lo = part_begin(G, blockIdx.x);
-    const uint64_t hi = part_end(G, lo);
+    const uint64_t lo = Lay.begin(blockIdx.x);
+    const uint64_t hi = Lay.end(blockIdx.x, lo);
     const uint64_t lanemask = lanemask_lt();
     // wave w holds tile entries w * kSortItems * 64 + u * 64 + lane: stable ranking follows
     // (u, lane) inside the wave, and the waves in order
@@ -416,7 +434,7 @@ __global__ __launch_bounds__(THREADS) void part_scatter_kernel(Src src, const Ci
         for (uint32_t q = tid; q < tn; q += THREADS) {
             const uint32_t cq = (uint32_t)cbuf[q];
             const uint64_t g = (uint64_t)(uint32_t)(gdel[digit_of(cq, P)] + q);  // < n by construction
-            if (g < G.n) {
+            if (g < Lay.n) {
                 ent_out[g] = buf[q];
                 if (cid_out) cid_out[g] = (CidT)cq;
             }
@@ -457,15 +475,16 @@ hipError_t partition_passes(Src src0, int first, Entry *const ent[2], CidT *cons
         part_colscan_kernel<<<R, 256, 0, st>>>(partial, G.blocks * G.sub, R, total);
         CidT *const cout = last ? nullptr : cid[(p + 1) & 1];
         // the first pass ranks by atomics; the later ones stably, 8-bit digits unrolled
+        const PassLayout lay{G, partial, total, G.n};
         if (p == 0)
             part_scatter_kernel<Entry, Src, CidT, THREADS, -1>
-                <<<G.blocks, THREADS, 0, st>>>(src0, cid[0], G, P[p], partial, total, ent[e], cout);
+                <<<G.blocks, THREADS, 0, st>>>(src0, cid[0], lay, P[p], ent[e], cout);
         else if (P[p].bits == kMaxRadixBits)
             part_scatter_kernel<Entry, EntrySource<Entry>, CidT, THREADS, kMaxRadixBits><<<G.blocks, THREADS, 0, st>>>(
-                EntrySource<Entry>{ent[1 - e]}, cid[p & 1], G, P[p], partial, total, ent[e], cout);
+                EntrySource<Entry>{ent[1 - e]}, cid[p & 1], lay, P[p], ent[e], cout);
         else
             part_scatter_kernel<Entry, EntrySource<Entry>, CidT, THREADS, 0><<<G.blocks, THREADS, 0, st>>>(
-                EntrySource<Entry>{ent[1 - e]}, cid[p & 1], G, P[p], partial, total, ent[e], cout);
+                EntrySource<Entry>{ent[1 - e]}, cid[p & 1], lay, P[p], ent[e], cout);
         if (!last) e = 1 - e;
     }
     const uint32_t nseg = (chunks + kScanSeg - 1) / kScanSeg;  // <= kPartialWords: partial is free now

@@ -608,8 +608,10 @@ int bulk_round(dyc_table *t, const Chunks &Ch, const uint32_t *keys, const uint3
 // insert_order 3: place keys by bucket fill in their start bucket, then the overflow in the
 // other bucket of their pair, then the rest by the eviction insert. Counters are zeroed here
 // and read by the caller.
-int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, bool was_zeroed, hipStream_t s)
+int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, bool was_zeroed, hipStream_t s,
+                uint64_t *left)
 {
+    *left = 0;
     const Chunks Ch = make_chunks(t->D);  // the workspace was reserved for it
     int rc = zero_counters(t, s);
     if (rc) return rc;
@@ -622,6 +624,7 @@ int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64
     if (rc || m1 == 0) return rc;
     rc = bulk_round(t, Ch, nullptr, nullptr, ov, m1, false, s, &ov2, &m2);  // the overflow, in its other table
     if (rc || m2 == 0) return rc;
+    *left = m2;
     HIP_TRY(launch_insert_entries(t->D, t->bk_ent[ov2], m2, kNoTable, (int)t->cfg.insert_lanes, t->policy(),
                                   t->insert_shape(), t->ctr, sink_at(t, t->pend_n), s));
     return DYC_OK;
@@ -1087,6 +1090,7 @@ int dyc_insert_counted(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
     const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = t->st.last_max_chain = 0;
+    t->st.last_filled = 0;
     if (n == 0) return DYC_OK;
     int rc;
     if (t->cfg.auto_resize) {
@@ -1112,8 +1116,9 @@ int dyc_insert_counted(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
             return rc;
         }
     }
+    uint64_t left = 0;  // entries the bulk fill left to the eviction insert
     if (bulk) {
-        rc = bulk_insert(t, keys, vals, n, zeroed, s);
+        rc = bulk_insert(t, keys, vals, n, zeroed, s, &left);
         if (rc) return rc;
     } else if (R) {
         rc = reserve_partition(t, n, s);
@@ -1138,6 +1143,7 @@ int dyc_insert_counted(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
     const uint64_t rejected = t->hctr->rejected;
     t->st.rejected += rejected;
     t->count += n - rejected;
+    if (bulk) t->st.last_filled = n - rejected - left;
     rc = absorb_insert_counters(t, room);
     t->st.pending = t->pend_n;
     if (rc) return rc;
@@ -1177,7 +1183,7 @@ int dyc_insert_async(dyc_table *t, const uint32_t *keys, const uint32_t *vals, u
     }
     rc = async_reserve(*a, n);
     if (rc) return rc;
-    if (t->async_keys == 0) t->st.last_kicks = t->st.last_max_chain = 0;
+    if (t->async_keys == 0) t->st.last_kicks = t->st.last_max_chain = t->st.last_filled = 0;
     HIP_TRY(launch_zero(reinterpret_cast<unsigned long long *>(a->ctr), kCounterWords, s));
     HIP_TRY(launch_insert(t->D, keys, vals, n, (int)t->cfg.insert_lanes, t->policy(), t->insert_shape(), a->ctr,
                           PendSink{a->pend, nullptr, a->cap, 0, 0, nullptr}, s));
@@ -1217,6 +1223,7 @@ int dyc_insert_noalloc(dyc_table *t, const uint32_t *keys, const uint32_t *vals,
     t->zeroed = false;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = t->st.last_max_chain = 0;
+    t->st.last_filled = 0;
     if (n == 0) return DYC_OK;
     Counters *C = nullptr;
     int rc = noalloc_counters(t->device, &C);
@@ -1249,6 +1256,7 @@ int dyc_upsert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_
     const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = t->st.last_max_chain = 0;
+    t->st.last_filled = 0;
     if (n == 0) return DYC_OK;
     int rc;
     if (t->cfg.auto_resize) {  // sized as if every key were new (api.cpp:360-369)
@@ -1502,6 +1510,7 @@ int dyc_insert_host(dyc_table *t, const uint32_t *keys, const uint32_t *vals, ui
     const uint64_t pend_before = t->pend_n;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = t->st.last_max_chain = 0;
+    t->st.last_filled = 0;
     if (n == 0) return DYC_OK;
     int rc;
     if (t->cfg.auto_resize) {

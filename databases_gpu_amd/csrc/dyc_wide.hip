@@ -1498,8 +1498,10 @@ int wbulk_round(dyc_wide *t, const Chunks &Ch, const uint64_t *keys, const uint6
 
 // insert_order 3 for wide keys: fill the start bucket, then the other bucket, then the eviction
 // insert for the rest (the narrow table's bulk_insert over fold(k)).
-int wbulk_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, bool was_zeroed, hipStream_t s)
+int wbulk_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, bool was_zeroed, hipStream_t s,
+                 uint64_t *left)
 {
+    *left = 0;
     const Chunks Ch = make_chunks(t->W.D);  // the workspace was reserved for it
     int ov = 0, ov2 = 0;
     uint64_t m1 = 0, m2 = 0;
@@ -1510,6 +1512,7 @@ int wbulk_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64
     if (rc || m1 == 0) return rc;
     rc = wbulk_round(t, Ch, nullptr, nullptr, ov, m1, false, s, &ov2, &m2);  // the overflow, in its other table
     if (rc || m2 == 0) return rc;
+    *left = m2;
     // the other entry buffer (16 B per key of capacity) holds m2 keys and then m2 values
     uint64_t *const sk = reinterpret_cast<uint64_t *>(t->bk_ent[1 - ov2]);
     HIP_TRY(launch_wide_split(t->bk_ent[ov2], m2, sk, sk + m2, s));
@@ -1607,6 +1610,7 @@ int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     DeviceGuard dg(t->device);
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = 0;
+    t->st.last_filled = 0;
     if (n == 0) return DYC_OK;
     bool zeroed = t->zeroed;  // every word zero before this call (dropped by any resize below)
     t->zeroed = false;
@@ -1635,8 +1639,9 @@ int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     }
     rc = wzero(t, s);
     if (rc) return rc;
+    uint64_t left = 0;  // entries the bulk fill left to the eviction insert
     if (bulk) {
-        rc = wbulk_insert(t, keys, vals, n, zeroed, s);
+        rc = wbulk_insert(t, keys, vals, n, zeroed, s, &left);
         if (rc) return rc;
     } else {
         HIP_TRY(launch_wide_insert(t->W, keys, vals, n, kNoTable, false, t->policy(), t->insert_shape(), t->ctr,
@@ -1647,6 +1652,7 @@ int dyc_wide_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     const uint64_t rejected = t->hctr->rejected;
     t->st.rejected += rejected;
     t->count += n - rejected;
+    if (bulk) t->st.last_filled = n - rejected - left;
     rc = wabsorb(t, room);
     t->st.pending = t->pend_n;
     if (rc) return rc;
@@ -1663,6 +1669,7 @@ int dyc_wide_upsert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uin
     t->zeroed = false;
     hipStream_t s = (hipStream_t)stream;
     t->st.last_kicks = 0;
+    t->st.last_filled = 0;
     if (n == 0) return DYC_OK;
     const uint64_t pend_before = t->pend_n;
     int rc;

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define DYC_ABI_VERSION 3
+#define DYC_ABI_VERSION 4
 
 #define DYC_OK 0
 #define DYC_ERR_INVALID 1        /* bad argument (null handle, bad config, size limit) */
@@ -102,6 +102,8 @@ typedef struct dyc_stats {
     uint32_t last_max_chain;    /* the longest eviction chain (exchanges) of the most recent insert call
                                    (32-bit tables; the maximum of the reference's iterator_count,
                                    api.cpp:219-243) */
+    uint64_t last_filled;       /* entries the most recent insert call placed by bucket fill
+                                   (insert_order 3); 0 when the eviction insert took the batch */
 } dyc_stats;
 
 /* Defaults (see field comments). */

@@ -20,6 +20,6 @@ int main(void)
            offsetof(dyc_server_config, resident), offsetof(dyc_server_config, resident_idle_ms));
     printf("server_stats %zu %zu %zu %zu\n", sizeof(dyc_server_stats), offsetof(dyc_server_stats, deferred_memory),
            offsetof(dyc_server_stats, in_flight), offsetof(dyc_server_stats, loop_pauses));
-    printf("stats_tail %zu\n", offsetof(dyc_stats, last_max_chain));
+    printf("stats_tail %zu %zu\n", offsetof(dyc_stats, last_max_chain), offsetof(dyc_stats, last_filled));
     return 0;
 }

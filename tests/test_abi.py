@@ -50,7 +50,7 @@ def test_library_exports_every_declared_symbol(lib):
 def test_abi_version_and_defaults(lib):
     import databases_gpu_amd as D
 
-    assert lib.dyc_abi_version() == 3
+    assert lib.dyc_abi_version() == 4
     cfg = D.default_config()
     assert cfg.kick_bound == 15 and cfg.rescue_kicks == 128
     assert cfg.lower_bound == 0.5 and cfg.upper_bound == 0.9
@@ -113,7 +113,7 @@ def test_struct_layout_matches_c(lib, tmp_path):
     ss = [int(x) for x in out[4].split()[1:]]
     assert ss == [ctypes.sizeof(L.ServerStats), L.ServerStats.deferred_memory.offset, L.ServerStats.in_flight.offset,
                   L.ServerStats.loop_pauses.offset]
-    assert [int(x) for x in out[5].split()[1:]] == [L.Stats.last_max_chain.offset]
+    assert [int(x) for x in out[5].split()[1:]] == [L.Stats.last_max_chain.offset, L.Stats.last_filled.offset]
 
 
 def _hipcc():

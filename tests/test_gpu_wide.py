@@ -283,6 +283,7 @@ def test_wide_bulk_fill_insert_multiset_invariant_and_find(D, d):
     t.insert(cu(batch), cu(bv))
     st = t.stats()
     assert st["count"] == keys.size and st["lost"] == 0 and st["rejected"] == 2
+    assert 0.95 * keys.size < st["last_filled"] <= keys.size
     ref = oracle_of(t)
     assert ref.check_invariant() == 0
     k, v = wide_stored(t)
