@@ -561,7 +561,38 @@ __global__ __launch_bounds__(THREADS) void group_bulk_fill_kernel(
     wave_tally(C, &C->rejected, rejected);
 }
 
+__global__ __launch_bounds__(256) void group_bulk_clip_kernel(const GroupItem *__restrict__ blocks,
+                                                              const uint32_t *__restrict__ btab, uint32_t n_blocks,
+                                                              const BulkTable *__restrict__ tabs, uint32_t n_tabs,
+                                                              const unsigned long long *__restrict__ ovf_n,
+                                                              GroupItem *__restrict__ out, BulkTable *__restrict__ tabs_out)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n_blocks) {
+        const GroupItem b = blocks[x];
+        const uint32_t j = btab[x];
+        const uint64_t lim = tabs[j].off + min(ovf_n[j], (unsigned long long)tabs[j].n);
+        const uint64_t len = b.begin < lim ? min((uint64_t)b.len, lim - b.begin) : 0ull;
+        out[x] = GroupItem{b.begin, (uint32_t)len, b.table};
+    }
+    if (tabs_out && x < n_tabs) {
+        BulkTable T = tabs[x];
+        T.n = (uint32_t)min(ovf_n[x], (unsigned long long)T.n);
+        tabs_out[x] = T;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_group_bulk_clip(const GroupItem *blocks, const uint32_t *btab, uint32_t n_blocks,
+                                  const BulkTable *tabs, uint32_t n_tabs, const unsigned long long *ovf_n,
+                                  GroupItem *out, BulkTable *tabs_out, hipStream_t st)
+{
+    const uint32_t n = std::max(n_blocks, tabs_out ? n_tabs : 0u);
+    if (n == 0) return hipSuccess;
+    group_bulk_clip_kernel<<<(n + 255) / 256, 256, 0, st>>>(blocks, btab, n_blocks, tabs, n_tabs, ovf_n, out, tabs_out);
+    return hipGetLastError();
+}
 
 hipError_t launch_group_bulk_pack(const Desc *dir, const GroupItem *blocks, uint32_t n_blocks, const uint32_t *keys,
                                   const uint64_t *entries, uint8_t *lcid, uint32_t *partial, hipStream_t st)

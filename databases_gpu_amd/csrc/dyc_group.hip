@@ -40,10 +40,12 @@ struct dyc_group {
     dyc_stats st{};
     // every slot of every table is zero: set at create and clear, dropped by every insert
     bool zeroed = false;
-    // bulk fill (cfg.insert_order 3): partitioned entries and the overflow (8 B per key each),
-    // local chunk ids (1 B per key), per-block chunk counts, per-chunk start / count / table,
-    // the participating tables (device and pinned) and their overflow counts (device and pinned);
-    // grown on demand, kept by the group
+    // bulk fill (cfg.insert_order 3), grown on demand and kept by the group. Shared by the eager
+    // calls and the plans (a plan refuses to run once one of these moved): partitioned entries and
+    // the overflow (8 B per key each), local chunk ids (1 B per key), per-block chunk counts,
+    // per-chunk start / count / table, and the overflow counts of both rounds (2 per table). The
+    // eager calls' own block and table arrays (device, with pinned staging) and the read-back of
+    // the last round's overflow counts.
     uint64_t *bk_ent = nullptr, *bk_ovf = nullptr;
     uint8_t *bk_lcid = nullptr;
     uint64_t bk_cap = 0;
@@ -51,9 +53,12 @@ struct dyc_group {
     uint64_t bk_partial_cap = 0;
     uint32_t *bk_cstart = nullptr, *bk_ccount = nullptr, *bk_ctable = nullptr;
     uint64_t bk_chunk_cap = 0;
-    BulkTable *bk_tabs = nullptr, *bk_htabs = nullptr;
     unsigned long long *bk_ovfn = nullptr, *bk_hovfn = nullptr;
     uint32_t bk_tabs_cap = 0;
+    GroupItem *bk_blk = nullptr, *bk_hblk = nullptr;  // device: 3 per block (rounds 1, 2, eviction)
+    uint32_t *bk_btab = nullptr, *bk_hbtab = nullptr;
+    uint64_t bk_blk_cap = 0;
+    BulkTable *bk_tabs = nullptr, *bk_htabs = nullptr;  // device: 2 per table (rounds 1, 2)
 
     InsertPolicy policy() const { return InsertPolicy{launch.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
     // The pending list is the group's stash: entries tagged with their table, searched by finds
@@ -249,6 +254,16 @@ int reserve_group_bulk(dyc_group *g, uint64_t n, uint64_t blocks, uint64_t chunk
         if (*p) (void)hipFree(*p);
         rc = bulk_alloc(p, bytes, what);
     };
+    auto regrow_pinned = [&](void **p, size_t bytes) {
+        if (rc) return;
+        if (*p) (void)hipHostFree(*p);
+        *p = nullptr;
+        if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            rc = fail(DYC_ERR_OOM, "pinned group bulk-fill staging (%zu bytes) failed", bytes);
+        }
+    };
     if (n > g->bk_cap) {
         g->bk_cap = 0;
         regrow((void **)&g->bk_ent, n * 8, "entries");
@@ -263,6 +278,15 @@ int reserve_group_bulk(dyc_group *g, uint64_t n, uint64_t blocks, uint64_t chunk
         if (rc) return rc;
         g->bk_partial_cap = blocks * kGroupBulkChunks;
     }
+    if (blocks > g->bk_blk_cap) {
+        g->bk_blk_cap = 0;
+        regrow((void **)&g->bk_blk, 3 * blocks * sizeof(GroupItem), "blocks");
+        regrow((void **)&g->bk_btab, blocks * sizeof(uint32_t), "block tables");
+        regrow_pinned((void **)&g->bk_hblk, blocks * sizeof(GroupItem));
+        regrow_pinned((void **)&g->bk_hbtab, blocks * sizeof(uint32_t));
+        if (rc) return rc;
+        g->bk_blk_cap = blocks;
+    }
     if (chunks > g->bk_chunk_cap) {
         g->bk_chunk_cap = 0;
         regrow((void **)&g->bk_cstart, chunks * sizeof(uint32_t), "chunk starts");
@@ -273,93 +297,34 @@ int reserve_group_bulk(dyc_group *g, uint64_t n, uint64_t blocks, uint64_t chunk
     }
     if (tabs > g->bk_tabs_cap) {
         g->bk_tabs_cap = 0;
-        regrow((void **)&g->bk_tabs, tabs * sizeof(BulkTable), "table list");
-        regrow((void **)&g->bk_ovfn, tabs * sizeof(unsigned long long), "overflow counts");
+        regrow((void **)&g->bk_tabs, 2 * tabs * sizeof(BulkTable), "table list");
+        regrow((void **)&g->bk_ovfn, 2 * tabs * sizeof(unsigned long long), "overflow counts");
+        regrow_pinned((void **)&g->bk_htabs, tabs * sizeof(BulkTable));
+        regrow_pinned((void **)&g->bk_hovfn, tabs * sizeof(unsigned long long));
         if (rc) return rc;
-        if (g->bk_htabs) (void)hipHostFree(g->bk_htabs);
-        if (g->bk_hovfn) (void)hipHostFree(g->bk_hovfn);
-        g->bk_htabs = nullptr;
-        g->bk_hovfn = nullptr;
-        if (hipHostMalloc((void **)&g->bk_htabs, tabs * sizeof(BulkTable), hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&g->bk_hovfn, tabs * sizeof(unsigned long long), hipHostMallocDefault) !=
-                hipSuccess) {
-            (void)hipGetLastError();
-            return fail(DYC_ERR_OOM, "pinned group bulk-fill table list failed");
-        }
         g->bk_tabs_cap = tabs;
     }
     return DYC_OK;
 }
 
-// One fill round of a group bulk insert over the tables `tabs` (their off and n set; the rest is
-// filled in here): round 1 reads keys/vals, round 2 the packed entries in bk_ovf. The batch is
-// partitioned per table into bk_ent, every chunk is filled, and what overflows lands in the
-// table's region of bk_ovf; *ovf receives the overflow count of each table (synchronous).
-int group_bulk_round(dyc_group *g, std::vector<BulkTable> &tabs, const uint32_t *keys, const uint32_t *vals,
-                     const uint64_t *entries, uint64_t total, bool zeroed, hipStream_t s, std::vector<uint64_t> *ovf)
-{
-    std::vector<GroupItem> blocks;
-    uint64_t buckets = 0, entries_n = 0;
-    uint32_t chunks = 0;
-    for (BulkTable &T : tabs) {
-        const Desc &D = g->hdir[T.table];
-        T.first_block = (uint32_t)blocks.size();
-        for (uint64_t b = T.off; b < T.off + T.n; b += kGroupBulkBlockKeys)
-            blocks.push_back(GroupItem{b, (uint32_t)std::min<uint64_t>(kGroupBulkBlockKeys, T.off + T.n - b), T.table});
-        T.blocks = (uint32_t)blocks.size() - T.first_block;
-        T.chunk0 = chunks;
-        chunks += T.chunks;
-        entries_n += T.n;
-        for (uint32_t t = 0; t < D.d; ++t) buckets += D.nb[t];
-    }
-    const uint32_t J = (uint32_t)tabs.size(), NB = (uint32_t)blocks.size();
-    // dense (chunk images through LDS) from two entries per bucket, as for one table
-    const bool dense = entries_n >= 2 * buckets;
-    HIP_TRY(hipStreamSynchronize(s));  // buffers being replaced are no longer in use
-    int rc = reserve_group_bulk(g, total, NB, chunks, J);
-    if (rc) return rc;
-    uint32_t n_up = 0;
-    rc = upload_segments(g, blocks, kGroupBulkBlockKeys, s, &n_up);
-    if (rc) return rc;
-    if (n_up != NB) return fail(DYC_ERR_CORRUPT, "group bulk fill: %u blocks uploaded for %u", n_up, NB);
-    memcpy(g->bk_htabs, tabs.data(), J * sizeof(BulkTable));
-    HIP_TRY(hipMemcpyAsync(g->bk_tabs, g->bk_htabs, J * sizeof(BulkTable), hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_group_bulk_pack(g->ddir, g->ditems, NB, keys, entries, g->bk_lcid, g->bk_partial, s));
-    HIP_TRY(launch_group_bulk_scan(g->bk_tabs, J, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s));
-    HIP_TRY(launch_group_bulk_scatter(g->ditems, NB, keys, vals, entries, g->bk_lcid, g->bk_partial, total, g->bk_ent,
-                                      s));
-    HIP_TRY(launch_zero(g->bk_ovfn, J, s));
-    HIP_TRY(launch_group_bulk_fill(g->ddir, g->bk_tabs, chunks, g->bk_cstart, g->bk_ccount, g->bk_ctable, g->bk_ent,
-                                   g->bk_ovf, g->bk_ovfn, g->tcounts, g->ctr, dense, dense && zeroed, s));
-    HIP_TRY(hipMemcpyAsync(g->bk_hovfn, g->bk_ovfn, J * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    ovf->resize(J);
-    for (uint32_t j = 0; j < J; ++j) {
-        (*ovf)[j] = g->bk_hovfn[j];
-        if ((*ovf)[j] > tabs[j].n)
-            return fail(DYC_ERR_CORRUPT, "group bulk fill: table %u overflowed %llu of %u entries", tabs[j].table,
-                        (unsigned long long)(*ovf)[j], tabs[j].n);
-    }
-    return DYC_OK;
-}
-
-// insert_order 3 for a group: each table's batch is partitioned by its own chunks in one pass
-// (the batch is grouped by table already) and every chunk is filled by one workgroup; what
-// overflows its start bucket gets a second round for the other bucket of its pair, and what
-// overflows that goes through the group's eviction insert. Declines (*taken false, nothing
-// launched) when a participating table has more than kGroupBulkChunks chunks, the batch has
-// fewer than two keys per bucket of those tables or 2^32 keys or more, or the workspace cannot
-// be allocated. Counters and per-table counts are zeroed here and read by the caller; *left
-// receives the entries handed to the eviction insert.
-int group_bulk_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *offsets, bool zeroed,
-                      hipStream_t s, uint64_t room, bool *taken, uint64_t *left)
-{
-    *taken = false;
-    *left = 0;
-    const uint64_t total = offsets[g->n];
-    if (total == 0 || total >= (1ull << 32)) return DYC_OK;
+// One batch's bulk fill as the host lays it out: the participating tables, their partition
+// blocks (at most kGroupBulkBlockKeys keys of one table each) and each block's table index.
+struct BulkLayout {
     std::vector<BulkTable> tabs;
-    uint64_t buckets = 0, chunks = 0, blocks = 0;
+    std::vector<GroupItem> blocks;
+    std::vector<uint32_t> btab;
+    uint64_t total = 0;
+    uint32_t chunks = 0;
+};
+
+// Lays out a batch for the bulk fill; false when it does not qualify: a participating table has
+// more than kGroupBulkChunks chunks, the batch has fewer than two keys per bucket of those tables
+// or 2^32 keys or more.
+bool bulk_layout(const dyc_group *g, const uint64_t *offsets, BulkLayout *L)
+{
+    L->total = offsets[g->n];
+    if (L->total == 0 || L->total >= (1ull << 32)) return false;
+    uint64_t buckets = 0;
     for (uint32_t i = 0; i < g->n; ++i) {
         const uint64_t n = offsets[i + 1] - offsets[i];
         if (!n) continue;
@@ -369,51 +334,121 @@ int group_bulk_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, 
             ch += (D.nb[t] + kChunkBuckets - 1) / kChunkBuckets;
             buckets += D.nb[t];
         }
-        if (ch > kGroupBulkChunks) return DYC_OK;
-        tabs.push_back(BulkTable{offsets[i], (uint32_t)n, i, 0, 0, 0, ch});
-        chunks += ch;
-        blocks += (n + kGroupBulkBlockKeys - 1) / kGroupBulkBlockKeys;
+        if (ch > kGroupBulkChunks) return false;
+        const uint32_t j = (uint32_t)L->tabs.size();
+        BulkTable T{offsets[i], (uint32_t)n, i, (uint32_t)L->blocks.size(), 0, L->chunks, ch};
+        for (uint64_t b = offsets[i]; b < offsets[i + 1]; b += kGroupBulkBlockKeys) {
+            L->blocks.push_back(GroupItem{b, (uint32_t)std::min<uint64_t>(kGroupBulkBlockKeys, offsets[i + 1] - b), i});
+            L->btab.push_back(j);
+        }
+        T.blocks = (uint32_t)L->blocks.size() - T.first_block;
+        L->chunks += ch;
+        L->tabs.push_back(T);
     }
-    if (total < 2 * buckets) return DYC_OK;
+    return L->total >= 2 * buckets;
+}
+
+// Device arrays of one laid-out batch: blocks of round 1, 2 and the eviction insert (nb each),
+// each block's table index, and the tables of rounds 1 and 2 (j each).
+struct BulkArrays {
+    GroupItem *blk;
+    uint32_t *btab;
+    BulkTable *tabs;
+    uint32_t nb, j, chunks;
+};
+
+// The whole bulk fill of a laid-out batch, launched on s with no read-back in between, so a
+// graph can hold it: round 1 (keys/vals -> entries by each table's chunks -> chunk fill), round 2
+// (each table's overflow, for the other bucket of its pair), then the eviction insert of what
+// overflows that. Each round's next work comes from the overflow counts on the device (clip).
+// Counters and per-table counts accumulate (the caller zeroes them); the last round's overflow
+// counts are left in bk_ovfn + j. zeroed: every slot of every table is zero (round 1 then does
+// not read the tables).
+hipError_t enqueue_group_bulk(dyc_group *g, const BulkArrays &A, const uint32_t *keys, const uint32_t *vals,
+                              uint64_t total, bool zeroed, PendSink sink, hipStream_t s)
+{
+    GroupItem *const blk1 = A.blk, *const blk2 = A.blk + A.nb, *const items = A.blk + 2 * A.nb;
+    BulkTable *const tabs1 = A.tabs, *const tabs2 = A.tabs + A.j;
+    unsigned long long *const ovf1 = g->bk_ovfn, *const ovf2 = g->bk_ovfn + A.j;
+    // round 2 is dense where half the keys overflow round 1: with two subtables every key starts
+    // in subtable 0 (pair k & 1, diff_hash_num_pos_fun.cpp:14-31)
+    const bool dense2 = g->cfg.subtables == 2;
+    hipError_t e;
+    if ((e = launch_group_bulk_pack(g->ddir, blk1, A.nb, keys, nullptr, g->bk_lcid, g->bk_partial, s)) != hipSuccess ||
+        (e = launch_group_bulk_scan(tabs1, A.j, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s)) !=
+            hipSuccess ||
+        (e = launch_group_bulk_scatter(blk1, A.nb, keys, vals, nullptr, g->bk_lcid, g->bk_partial, total, g->bk_ent,
+                                       s)) != hipSuccess ||
+        (e = launch_zero(ovf1, 2 * (uint64_t)A.j, s)) != hipSuccess ||
+        (e = launch_group_bulk_fill(g->ddir, tabs1, A.chunks, g->bk_cstart, g->bk_ccount, g->bk_ctable, g->bk_ent,
+                                    g->bk_ovf, ovf1, g->tcounts, g->ctr, true, zeroed, s)) != hipSuccess ||
+        (e = launch_group_bulk_clip(blk1, A.btab, A.nb, tabs1, A.j, ovf1, blk2, tabs2, s)) != hipSuccess ||
+        (e = launch_group_bulk_pack(g->ddir, blk2, A.nb, nullptr, g->bk_ovf, g->bk_lcid, g->bk_partial, s)) !=
+            hipSuccess ||
+        (e = launch_group_bulk_scan(tabs2, A.j, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s)) !=
+            hipSuccess ||
+        (e = launch_group_bulk_scatter(blk2, A.nb, nullptr, nullptr, g->bk_ovf, g->bk_lcid, g->bk_partial, total,
+                                       g->bk_ent, s)) != hipSuccess ||
+        (e = launch_group_bulk_fill(g->ddir, tabs2, A.chunks, g->bk_cstart, g->bk_ccount, g->bk_ctable, g->bk_ent,
+                                    g->bk_ovf, ovf2, g->tcounts, g->ctr, dense2, false, s)) != hipSuccess ||
+        (e = launch_group_bulk_clip(blk1, A.btab, A.nb, tabs1, A.j, ovf2, items, nullptr, s)) != hipSuccess)
+        return e;
+    return launch_group_insert(g->ddir, items, A.nb, nullptr, nullptr, g->policy(), (int)g->launch.insert_lanes,
+                               (int)g->launch.insert_blocks, g->ctr, sink, g->tcounts, s, 0, g->bk_ovf);
+}
+
+// Sizes the shared workspace for a layout (synchronizes s first); DYC_ERR_OOM leaves no message
+// behind for a caller that falls back to the eviction insert.
+int reserve_for(dyc_group *g, const BulkLayout &L, hipStream_t s)
+{
+    HIP_TRY(hipStreamSynchronize(s));  // buffers being replaced are no longer in use
+    return reserve_group_bulk(g, L.total, L.blocks.size(), L.chunks, (uint32_t)L.tabs.size());
+}
+
+// insert_order 3 for a group: each table's batch is partitioned by its own chunks in one pass
+// (the batch is grouped by table already) and every chunk is filled by one workgroup; what
+// overflows its start bucket gets a second round for the other bucket of its pair, and what
+// overflows that goes through the group's eviction insert. Declines (*n_tabs 0, nothing
+// launched) when bulk_layout does, or when the workspace cannot be allocated. Counters and
+// per-table counts are zeroed here and read by the caller, who also finds the overflow counts of
+// the last round (*n_tabs of them) in bk_hovfn after its wait.
+int group_bulk_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, const uint64_t *offsets, bool zeroed,
+                      hipStream_t s, uint64_t room, uint32_t *n_tabs)
+{
+    *n_tabs = 0;
+    BulkLayout L;
+    if (!bulk_layout(g, offsets, &L)) return DYC_OK;
     {
-        HIP_TRY(hipStreamSynchronize(s));
         const dyc::host::ErrorKeep keep;
-        const int rc = reserve_group_bulk(g, total, blocks, chunks, (uint32_t)tabs.size());
+        const int rc = reserve_for(g, L, s);
         if (rc == DYC_ERR_OOM) {  // no room for the workspace: the eviction insert needs none
             keep.restore();
             return DYC_OK;
         }
         if (rc) return rc;
     }
-    *taken = true;
+    const uint32_t NB = (uint32_t)L.blocks.size(), J = (uint32_t)L.tabs.size();
+    // the pinned staging is free: the previous call waited for its work before returning
+    memcpy(g->bk_hblk, L.blocks.data(), NB * sizeof(GroupItem));
+    memcpy(g->bk_hbtab, L.btab.data(), NB * sizeof(uint32_t));
+    memcpy(g->bk_htabs, L.tabs.data(), J * sizeof(BulkTable));
+    HIP_TRY(hipMemcpyAsync(g->bk_blk, g->bk_hblk, NB * sizeof(GroupItem), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(g->bk_btab, g->bk_hbtab, NB * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(g->bk_tabs, g->bk_htabs, J * sizeof(BulkTable), hipMemcpyHostToDevice, s));
     int rc = zero_and_run_prologue(g, s);
     if (rc) return rc;
-    std::vector<uint64_t> ovf;
-    rc = group_bulk_round(g, tabs, keys, vals, nullptr, total, zeroed, s, &ovf);
-    if (rc) return rc;
-    std::vector<BulkTable> tabs2;  // round 2: each table's overflow, for the other bucket
-    for (uint32_t j = 0; j < tabs.size(); ++j)
-        if (ovf[j]) tabs2.push_back(BulkTable{tabs[j].off, (uint32_t)ovf[j], tabs[j].table, 0, 0, 0, tabs[j].chunks});
-    if (!tabs2.empty()) {
-        rc = group_bulk_round(g, tabs2, nullptr, nullptr, g->bk_ovf, total, false, s, &ovf);
-        if (rc) return rc;
-    }
-    // the overflow of table j sits in bk_ovf[off, off + count): the eviction insert's items
-    std::vector<GroupItem> segs;
-    for (uint32_t j = 0; j < tabs2.size(); ++j)
-        if (ovf[j]) {
-            segs.push_back(GroupItem{tabs2[j].off, (uint32_t)ovf[j], tabs2[j].table});
-            *left += ovf[j];
-        }
-    uint32_t n_items = 0;
-    const uint64_t per = g->launch.item_keys ? g->launch.item_keys : kDefaultItemKeys;
-    rc = upload_segments(g, segs, per, s, &n_items);
-    if (rc) return rc;
-    HIP_TRY(launch_group_insert(g->ddir, g->ditems, n_items, nullptr, nullptr, g->policy(), (int)g->launch.insert_lanes,
-                                (int)g->launch.insert_blocks, g->ctr,
-                                PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, 0, 0, nullptr}, g->tcounts,
-                                s, 0, g->bk_ovf));
+    const BulkArrays A{g->bk_blk, g->bk_btab, g->bk_tabs, NB, J, L.chunks};
+    HIP_TRY(enqueue_group_bulk(g, A, keys, vals, L.total, zeroed,
+                               PendSink{g->pend + g->pend_n, g->pend_tag + g->pend_n, room, 0, 0, nullptr}, s));
+    HIP_TRY(hipMemcpyAsync(g->bk_hovfn, g->bk_ovfn + J, J * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    *n_tabs = J;
     return DYC_OK;
+}
+
+// The group's shared bulk-fill buffers a plan's graph uses (any of them moving voids the plan).
+std::vector<const void *> bulk_workspace(const dyc_group *g)
+{
+    return {g->bk_ent, g->bk_ovf, g->bk_lcid, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, g->bk_ovfn};
 }
 
 }  // namespace
@@ -519,10 +554,10 @@ int dyc_group_destroy(dyc_group *g)
     if (g->items_free) (void)hipEventDestroy(g->items_free);
     for (void *p : {(void *)g->bk_ent, (void *)g->bk_ovf, (void *)g->bk_lcid, (void *)g->bk_partial,
                     (void *)g->bk_cstart, (void *)g->bk_ccount, (void *)g->bk_ctable, (void *)g->bk_tabs,
-                    (void *)g->bk_ovfn})
+                    (void *)g->bk_ovfn, (void *)g->bk_blk, (void *)g->bk_btab})
         if (p) (void)hipFree(p);
-    if (g->bk_htabs) (void)hipHostFree(g->bk_htabs);
-    if (g->bk_hovfn) (void)hipHostFree(g->bk_hovfn);
+    for (void *p : {(void *)g->bk_htabs, (void *)g->bk_hovfn, (void *)g->bk_hblk, (void *)g->bk_hbtab})
+        if (p) (void)hipHostFree(p);
     delete g;
     return DYC_OK;
 }
@@ -560,13 +595,14 @@ int dyc_group_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, c
     if (rc) return rc;
     const uint64_t room = g->pend_cap - g->pend_n;
     if (g->cfg.insert_order == 3) {
-        bool taken = false;
-        uint64_t left = 0;
-        rc = group_bulk_insert(g, keys, vals, offsets, zeroed, s, room, &taken, &left);
+        uint32_t n_tabs = 0;
+        rc = group_bulk_insert(g, keys, vals, offsets, zeroed, s, room, &n_tabs);
         if (rc) return rc;
-        if (taken) {
-            rc = read_back(g, s);
+        if (n_tabs) {
+            rc = read_back(g, s);  // waits: the overflow counts are in bk_hovfn too
             if (rc) return rc;
+            uint64_t left = 0;
+            for (uint32_t j = 0; j < n_tabs; ++j) left += g->bk_hovfn[j];
             g->st.last_filled = total - g->hctr->rejected - left;
             return finish_insert(g, room);
         }
@@ -729,7 +765,19 @@ struct dyc_group_plan {
     hipGraphExec_t exec = nullptr;
     uint64_t room = 0;  // pending entries the insert may append
     uint64_t *pend = nullptr;  // the pending list the graph writes (a larger eager insert may move it)
+    // insert_order 3: the batch's own bulk-fill arrays, the group's shared workspace the graph
+    // uses (an eager insert that grows it moves these), and the last round's overflow counts
+    // (pinned, written by the graph) for last_filled
+    bool bulk = false;
+    uint64_t total = 0;
+    GroupItem *bk_blk = nullptr;
+    uint32_t *bk_btab = nullptr;
+    BulkTable *bk_tabs = nullptr;
+    uint32_t bk_j = 0;
+    unsigned long long *hovfn = nullptr;
+    std::vector<const void *> shared;
 };
+
 
 int dyc_group_plan_destroy(dyc_group_plan *p)
 {
@@ -739,6 +787,9 @@ int dyc_group_plan_destroy(dyc_group_plan *p)
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->d_items) (void)hipFree(p->d_items);
+    for (void *q : {(void *)p->bk_blk, (void *)p->bk_btab, (void *)p->bk_tabs})
+        if (q) (void)hipFree(q);
+    if (p->hovfn) (void)hipHostFree(p->hovfn);
     delete p;
     return DYC_OK;
 }
@@ -790,6 +841,37 @@ int dyc_group_plan_create(dyc_group *g, const uint32_t *keys, const uint32_t *va
             return undo(DYC_ERR_HIP);
         }
     }
+    // insert_order 3: the bulk fill of this batch, with its own block and table arrays
+    BulkLayout L;
+    if (g->cfg.insert_order == 3 && bulk_layout(g, ins_offsets, &L)) {
+        const dyc::host::ErrorKeep keep;
+        const int r = reserve_for(g, L, nullptr);
+        if (r == DYC_ERR_OOM) {
+            keep.restore();  // the plan holds the eviction insert instead
+        } else if (r) {
+            return undo(r);
+        } else {
+            const uint32_t NB = (uint32_t)L.blocks.size(), J = (uint32_t)L.tabs.size();
+            p->bulk = true;
+            p->total = L.total;
+            p->bk_j = J;
+            if (hipMalloc(&p->bk_blk, 3ull * NB * sizeof(GroupItem)) != hipSuccess ||
+                hipMalloc(&p->bk_btab, NB * sizeof(uint32_t)) != hipSuccess ||
+                hipMalloc(&p->bk_tabs, 2ull * J * sizeof(BulkTable)) != hipSuccess ||
+                hipHostMalloc((void **)&p->hovfn, J * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                fail(DYC_ERR_OOM, "plan bulk-fill arrays failed");
+                return undo(DYC_ERR_OOM);
+            }
+            if (hipMemcpy(p->bk_blk, L.blocks.data(), NB * sizeof(GroupItem), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(p->bk_btab, L.btab.data(), NB * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(p->bk_tabs, L.tabs.data(), J * sizeof(BulkTable), hipMemcpyHostToDevice) != hipSuccess) {
+                fail(DYC_ERR_HIP, "plan bulk-fill upload failed");
+                return undo(DYC_ERR_HIP);
+            }
+            p->shared = bulk_workspace(g);
+        }
+    }
     // capture: clear, counters to zero, insert, find (its stash is the pending list, whose size
     // the launch reads on the device), counters out to pinned memory. Zeroing is done by a fill
     // kernel, not a memset node.
@@ -804,11 +886,17 @@ int dyc_group_plan_create(dyc_group *g, const uint32_t *keys, const uint32_t *va
         hipError_t k = launch_zero(reinterpret_cast<unsigned long long *>(g->arena), g->arena_bytes / 8, cs);
         if (k == hipSuccess) k = launch_zero(reinterpret_cast<unsigned long long *>(g->ctr), kCounterWords, cs);
         if (k == hipSuccess) k = launch_zero(g->tcounts, g->n, cs);
-        if (k == hipSuccess)
+        const PendSink sink{g->pend, g->pend_tag, g->pend_cap, 0, 0, nullptr};
+        if (k == hipSuccess && p->bulk) {
+            // the graph clears every table first, so round 1 need not read them
+            const BulkArrays A{p->bk_blk, p->bk_btab, p->bk_tabs, (uint32_t)L.blocks.size(), p->bk_j, L.chunks};
+            k = enqueue_group_bulk(g, A, keys, vals, p->total, true, sink, cs);
+            if (k == hipSuccess) k = launch_copy(g->bk_ovfn + p->bk_j, p->hovfn, p->bk_j, nullptr, nullptr, 0, cs);
+        } else if (k == hipSuccess) {
             k = launch_group_insert(g->ddir, p->d_items, p->n_ins, keys, vals, g->policy(), (int)g->launch.insert_lanes,
-                                    (int)g->launch.insert_blocks, g->ctr,
-                                    PendSink{g->pend, g->pend_tag, g->pend_cap, 0, 0, nullptr}, g->tcounts, cs,
+                                    (int)g->launch.insert_blocks, g->ctr, sink, g->tcounts, cs,
                                     (int)g->launch.insert_waves_per_eu);
+        }
         if (k == hipSuccess)
             k = launch_group_find(g->ddir, p->d_items + p->n_ins, p->n_find, lookups, vals_out, hit,
                                   (int)g->launch.find_lanes, (int)g->launch.find_blocks, cs, &stash,
@@ -835,6 +923,9 @@ int dyc_group_plan_run(dyc_group_plan *p, dyc_stream stream)
     if (g->pend != p->pend || g->pend_cap != p->room)
         return fail(DYC_ERR_INVALID, "the group's pending list moved since the plan was made (a larger insert grew "
                                      "it): make the plan again");
+    if (p->bulk && bulk_workspace(g) != p->shared)
+        return fail(DYC_ERR_INVALID, "the group's bulk-fill workspace moved since the plan was made (a larger bulk "
+                                     "insert grew it): make the plan again");
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
     g->zeroed = false;
@@ -845,6 +936,11 @@ int dyc_group_plan_run(dyc_group_plan *p, dyc_stream stream)
     // the step cleared every table: counts and the pending list start again from zero
     std::fill(g->count.begin(), g->count.end(), 0);
     g->pend_n = 0;
+    if (p->bulk) {
+        uint64_t left = 0;
+        for (uint32_t j = 0; j < p->bk_j; ++j) left += p->hovfn[j];
+        g->st.last_filled = p->total - g->hctr->rejected - left;
+    }
     return finish_insert(g, p->room);
 }
 

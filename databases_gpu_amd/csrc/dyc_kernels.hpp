@@ -224,6 +224,13 @@ hipError_t launch_group_bulk_scatter(const GroupItem *blocks, uint32_t n_blocks,
 // region of ovf, ovf[off, off + n) of its BulkTable, counted in ovf_n[j] (j its index in tabs;
 // zeroed by the caller). Placements are added to table_counts and to the kick tally, reserved
 // keys to the rejected tally. zeroed (dense only): every slot of every table is zero.
+// The next round's work from this round's overflow counts, on the device (no read-back): block b
+// of blocks (table tabs[btab[b]]) clipped to its table's overflow, out[b] = its part of
+// [off, off + ovf_n[j]); tabs_out (optional) = tabs with n = ovf_n[j]. Blocks past the overflow
+// become empty.
+hipError_t launch_group_bulk_clip(const GroupItem *blocks, const uint32_t *btab, uint32_t n_blocks,
+                                  const BulkTable *tabs, uint32_t n_tabs, const unsigned long long *ovf_n,
+                                  GroupItem *out, BulkTable *tabs_out, hipStream_t st);
 hipError_t launch_group_bulk_fill(const Desc *dir, const BulkTable *tabs, uint32_t n_chunks, const uint32_t *cstart,
                                   const uint32_t *ccount, const uint32_t *ctable, const uint64_t *ent, uint64_t *ovf,
                                   unsigned long long *ovf_n, unsigned long long *table_counts, Counters *C, bool dense,

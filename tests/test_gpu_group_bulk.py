@@ -199,3 +199,64 @@ def test_group_bulk_fill_overfull_table_keeps_entries_in_the_stash(D):
     v, h = g.find(dev(keys), offs)
     assert h.cpu().numpy().all() and np.array_equal(v.cpu().numpy().view(np.uint32), vals)
     g.close()
+
+
+def test_group_bulk_fill_graph_plan_replays_equal_eager_steps(D):
+    """A plan of a group with insert_order 3 captures the whole bulk fill (both rounds and the
+    eviction insert, chained on the device by the overflow counts). Four replays with the inputs
+    rewritten and eager work in between each equal the eager bulk calls on a twin group (finds,
+    counts, last_filled) and the oracle on sampled tables; an eager bulk insert that grows the
+    shared workspace makes the plan refuse to run."""
+    banner("group bulk fill: graph plan replays vs eager")
+    sizes = sizes_loguniform(40, 2**8, 2**14, 110)
+    sizes[9] = 0
+    hints = (np.ceil(np.maximum(sizes, 1) / 0.85) + 64).astype(np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    total = int(sizes.sum())
+    k0 = unique_keys(total, 111)
+    kk, vv, qk = dev(k0), dev(vals_of(k0)), dev(k0)
+    gp = D.DyCuckooGroup(hints, insert_order=3)
+    ge = D.DyCuckooGroup(hints, insert_order=3)
+    plan = gp.plan(kk, vv, offs, qk, offs)
+    for r in range(4):
+        keys = unique_keys(total, 120 + r)
+        vals = vals_of(keys) ^ np.uint32(r)
+        look = keys.copy()
+        look[1::3] ^= np.uint32(0x5A5A5A5A)  # mostly absent keys
+        kk.copy_(dev(keys))
+        vv.copy_(dev(vals))
+        qk.copy_(dev(look))
+        torch.cuda.synchronize()
+        _ = (kk.float() * 2).sum().item()  # eager work between replays
+        plan.run()
+        pv = plan.out_vals.cpu().numpy().view(np.uint32).copy()
+        ph = plan.out_hit.cpu().numpy().copy()
+        ge.clear()
+        ge.insert(kk, vv, offs)
+        ev, eh = ge.find(qk, offs)
+        assert np.array_equal(ph, eh.cpu().numpy()) and np.array_equal(pv, ev.cpu().numpy().view(np.uint32))
+        assert gp.table_stats(0)["last_filled"] > 0.9 * total and ge.table_stats(0)["last_filled"] > 0.9 * total
+        for i in range(len(sizes)):
+            assert gp.table_stats(i)["count"] == sizes[i] == ge.table_stats(i)["count"]
+        for i in (0, 9, 21, 39):
+            ref = O.OracleTable(nb=gp.table_stats(i)["buckets"])
+            ref.sub = gp.dump(i)
+            assert ref.check_invariant() == 0
+            lo, hi = int(offs[i]), int(offs[i + 1])
+            assert np.array_equal(ref.entries(), np.sort(O.pack(keys[lo:hi], vals[lo:hi])))
+            rv, rh = ref.find(look[lo:hi])
+            assert np.array_equal(ph[lo:hi], rh) and np.array_equal(pv[lo:hi][rh == 1], rv[rh == 1])
+    # an eager bulk insert of a larger batch (5 % more keys per table, still fitting) grows the
+    # shared workspace: the plan refuses to run
+    big = np.ceil(sizes * 1.05).astype(np.int64)
+    bk = unique_keys(int(big.sum()), 130)
+    boffs = np.concatenate([[0], np.cumsum(big)]).astype(np.uint64)
+    gp.clear()
+    gp.insert(dev(bk), dev(vals_of(bk)), boffs)
+    assert gp.table_stats(0)["last_filled"] > 0.9 * bk.size
+    with pytest.raises(D.DycError) as ei:
+        plan.run()
+    assert ei.value.code == 1  # DYC_ERR_INVALID: make the plan again
+    plan.close()
+    gp.close()
+    ge.close()
