@@ -872,8 +872,8 @@ def run_batched(args, dev) -> int:
     torch.cuda.synchronize()
     hints = (np.ceil(sizes / args.bt_theta) + 64).astype(np.uint64)
     order = args.bt_insert_order if args.bt_insert_order >= 0 else (3 if total >= (1 << 22) else 0)
-    bulk = order == 3 and not args.bt_wide
-    g = D.DyCuckooWideGroup(hints) if args.bt_wide else D.DyCuckooGroup(hints, insert_order=order)
+    bulk = order == 3
+    g = D.DyCuckooWideGroup(hints, insert_order=order) if args.bt_wide else D.DyCuckooGroup(hints, insert_order=order)
     tuned = None
     if not args.no_tune and not args.bt_wide:
         log("autotuning launch shapes on the batch")
@@ -901,12 +901,15 @@ def run_batched(args, dev) -> int:
         return ins, fnd, tot, wall
 
     ins_p, fnd_p, tot_p, wall_p = measure(False)
-    filled = g.table_stats(0)["last_filled"] if not args.bt_wide else 0
+    filled = g.table_stats(0)["last_filled"]
     ok = bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
     evict = None
     if bulk:  # the same launch shapes with the eviction insert for every key
-        g0 = D.DyCuckooGroup(hints, insert_order=0)
-        g0.set_launch(**g.launch)
+        if args.bt_wide:
+            g0 = D.DyCuckooWideGroup(hints, insert_order=0)
+        else:
+            g0 = D.DyCuckooGroup(hints, insert_order=0)
+            g0.set_launch(**g.launch)
         ins_e, fnd_e, tot_e, _ = measure(False, g0)
         ok &= bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
         g0.close()

@@ -506,20 +506,6 @@ __global__ __launch_bounds__(kGroupBulkChunks) void group_bulk_scan_kernel(const
     }
 }
 
-// The group pass's layout for part_scatter_kernel: block b covers blocks[b], and partial holds
-// the place of its first entry of each local chunk.
-struct SegLayout {
-    const GroupItem *blocks;
-    const uint32_t *partial;
-    uint64_t n;
-    __device__ uint64_t begin(uint32_t b) const { return blocks[b].begin; }
-    __device__ uint64_t end(uint32_t b, uint64_t lo) const { return lo + blocks[b].len; }
-    __device__ uint32_t base(uint32_t b, uint32_t d, uint32_t R, uint32_t *) const
-    {
-        return d < R ? partial[(uint64_t)b * kGroupBulkChunks + d] : 0u;
-    }
-};
-
 template <bool DENSE, int THREADS>
 __global__ __launch_bounds__(THREADS) void group_bulk_fill_kernel(
     const Desc *__restrict__ dir, const BulkTable *__restrict__ tabs, const uint32_t *__restrict__ cstart,

@@ -4,6 +4,7 @@
 // table. A batch (keys segmented per table by host offsets) is cut into work items of at most
 // item_keys keys of one table, uploaded, and served by one persistent launch; the per-table
 // variants launch the single-table kernels once per table for comparison.
+#include "dyc_bulk_layout.hpp"
 #include "dyc_host.hpp"
 #include "dyc_kernels.hpp"
 
@@ -13,6 +14,7 @@
 #include <vector>
 
 using namespace dyc;
+using dyc::host::BulkLayout;
 using dyc::host::DeviceGuard;
 using dyc::host::fail;
 
@@ -307,47 +309,6 @@ int reserve_group_bulk(dyc_group *g, uint64_t n, uint64_t blocks, uint64_t chunk
     return DYC_OK;
 }
 
-// One batch's bulk fill as the host lays it out: the participating tables, their partition
-// blocks (at most kGroupBulkBlockKeys keys of one table each) and each block's table index.
-struct BulkLayout {
-    std::vector<BulkTable> tabs;
-    std::vector<GroupItem> blocks;
-    std::vector<uint32_t> btab;
-    uint64_t total = 0;
-    uint32_t chunks = 0;
-};
-
-// Lays out a batch for the bulk fill; false when it does not qualify: a participating table has
-// more than kGroupBulkChunks chunks, the batch has fewer than two keys per bucket of those tables
-// or 2^32 keys or more.
-bool bulk_layout(const dyc_group *g, const uint64_t *offsets, BulkLayout *L)
-{
-    L->total = offsets[g->n];
-    if (L->total == 0 || L->total >= (1ull << 32)) return false;
-    uint64_t buckets = 0;
-    for (uint32_t i = 0; i < g->n; ++i) {
-        const uint64_t n = offsets[i + 1] - offsets[i];
-        if (!n) continue;
-        const Desc &D = g->hdir[i];
-        uint32_t ch = 0;
-        for (uint32_t t = 0; t < D.d; ++t) {
-            ch += (D.nb[t] + kChunkBuckets - 1) / kChunkBuckets;
-            buckets += D.nb[t];
-        }
-        if (ch > kGroupBulkChunks) return false;
-        const uint32_t j = (uint32_t)L->tabs.size();
-        BulkTable T{offsets[i], (uint32_t)n, i, (uint32_t)L->blocks.size(), 0, L->chunks, ch};
-        for (uint64_t b = offsets[i]; b < offsets[i + 1]; b += kGroupBulkBlockKeys) {
-            L->blocks.push_back(GroupItem{b, (uint32_t)std::min<uint64_t>(kGroupBulkBlockKeys, offsets[i + 1] - b), i});
-            L->btab.push_back(j);
-        }
-        T.blocks = (uint32_t)L->blocks.size() - T.first_block;
-        L->chunks += ch;
-        L->tabs.push_back(T);
-    }
-    return L->total >= 2 * buckets;
-}
-
 // Device arrays of one laid-out batch: blocks of round 1, 2 and the eviction insert (nb each),
 // each block's table index, and the tables of rounds 1 and 2 (j each).
 struct BulkArrays {
@@ -417,7 +378,8 @@ int group_bulk_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, 
 {
     *n_tabs = 0;
     BulkLayout L;
-    if (!bulk_layout(g, offsets, &L)) return DYC_OK;
+    if (!dyc::host::bulk_layout(g->n, offsets, [g](uint32_t i) -> const Desc & { return g->hdir[i]; }, &L))
+        return DYC_OK;
     {
         const dyc::host::ErrorKeep keep;
         const int rc = reserve_for(g, L, s);
@@ -843,7 +805,8 @@ int dyc_group_plan_create(dyc_group *g, const uint32_t *keys, const uint32_t *va
     }
     // insert_order 3: the bulk fill of this batch, with its own block and table arrays
     BulkLayout L;
-    if (g->cfg.insert_order == 3 && bulk_layout(g, ins_offsets, &L)) {
+    if (g->cfg.insert_order == 3 &&
+        dyc::host::bulk_layout(g->n, ins_offsets, [g](uint32_t i) -> const Desc & { return g->hdir[i]; }, &L)) {
         const dyc::host::ErrorKeep keep;
         const int r = reserve_for(g, L, nullptr);
         if (r == DYC_ERR_OOM) {

@@ -329,6 +329,20 @@ struct PassLayout {
     }
 };
 
+// A pass over a batch grouped by table (the group bulk fill): block b covers blocks[b] (one
+// table's range) and partial holds the place of its first entry of each local chunk.
+struct SegLayout {
+    const GroupItem *blocks;
+    const uint32_t *partial;
+    uint64_t n;
+    __device__ uint64_t begin(uint32_t b) const { return blocks[b].begin; }
+    __device__ uint64_t end(uint32_t b, uint64_t lo) const { return lo + blocks[b].len; }
+    __device__ uint32_t base(uint32_t b, uint32_t d, uint32_t R, uint32_t *) const
+    {
+        return d < R ? partial[(uint64_t)b * kGroupBulkChunks + d] : 0u;
+    }
+};
+
 template <typename Entry, typename Src, typename CidT, int THREADS, int BITS, typename Layout = PassLayout>
 __global__ __launch_bounds__(THREADS) void part_scatter_kernel(Src src, const CidT *__restrict__ cid, Layout Lay,
                                                                RadixPass P, Entry *__restrict__ ent_out,

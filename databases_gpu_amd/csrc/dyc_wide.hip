@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "dyc.h"
+#include "dyc_bulk_layout.hpp"
 #include "dyc_host.hpp"
 #include "dyc_part.hpp"
 #include "dyc_wave.hpp"
@@ -625,12 +626,16 @@ __global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc 
     for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) partial[(uint64_t)blockIdx.x * R + d] = h[d];
 }
 
+// The fill of one chunk of a wide table by the whole workgroup: buckets [b0, b0 + len) of
+// subtable t, entries ent[lo, hi) (all targeting subtable t); entries that do not fit go to
+// ovf[0, ovf_cap) at the index *ovf_n hands out. Placements and rejected reserved keys are added
+// to this lane's `placed` and `rejected`. Every thread of the workgroup calls it alike.
 template <bool DENSE, int THREADS>
-__global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chunks Ch, const uint32_t *__restrict__ start,
-                                                                 const WideEntry *__restrict__ ent, uint64_t n,
-                                                                 WideEntry *__restrict__ ovf, uint64_t ovf_cap,
-                                                                 unsigned long long *__restrict__ ovf_n,
-                                                                 Counters *__restrict__ C, bool zeroed)
+__device__ inline void wide_fill_chunk(const WideDesc &W, uint32_t t, uint32_t b0, uint32_t len,
+                                       const WideEntry *__restrict__ ent, uint32_t lo, uint32_t hi,
+                                       WideEntry *__restrict__ ovf, uint64_t ovf_cap,
+                                       unsigned long long *__restrict__ ovf_n, bool zeroed,
+                                       unsigned long long &placed, unsigned long long &rejected)
 {
     const Desc &D = W.D;
     constexpr uint32_t kImg = DENSE ? kChunkBuckets * kSlots : 2;
@@ -640,6 +645,228 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
     __shared__ uint32_t wred[THREADS / kWave];
     __shared__ unsigned long long obase;
     __shared__ uint32_t onext;
+    uint64_t *const ksub = selt(t, D.sub);
+    uint64_t *const vsub = selt(t, W.val);
+    const uint4 *const ksrc = reinterpret_cast<const uint4 *>(ksub + (uint64_t)b0 * kSlots);
+    constexpr uint32_t kParts = kChunkBuckets * kSlots / 2;  // 16-byte parts of a chunk's lines
+    if constexpr (DENSE) {
+        const uint4 *const vsrc = reinterpret_cast<const uint4 *>(vsub + (uint64_t)b0 * kSlots);
+        uint4 *const ki = reinterpret_cast<uint4 *>(kimg);
+        uint4 *const vi = reinterpret_cast<uint4 *>(vimg);
+        // a bucket's 8 key parts sit in 8 consecutive lanes, whose two ballot bits per part
+        // make its free-slot mask (no strided LDS reads)
+        static_assert(kParts % THREADS == 0, "every lane takes the same number of parts");
+        const uint32_t lane = __lane_id();
+        if (zeroed) {
+            // every slot of the table is known to be zero (the caller's host state says so):
+            // the chunk's lines start empty in LDS without reading them
+            for (uint32_t q = threadIdx.x; q < kParts; q += THREADS) {
+                ki[q] = make_uint4(0u, 0u, 0u, 0u);
+                vi[q] = make_uint4(0u, 0u, 0u, 0u);
+            }
+            for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) {
+                fm[j] = j < len ? 0xffffu : 0u;  // past the end: full
+                cnt[j] = 0;
+            }
+        }
+        for (uint32_t q = threadIdx.x; q < kParts && !zeroed; q += THREADS) {
+            const bool in = q < len * (kSlots / 2);
+            const uint4 kv = in ? ksrc[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
+            ki[q] = kv;
+            vi[q] = in ? vsrc[q] : make_uint4(0u, 0u, 0u, 0u);
+            const uint64_t m0 = __ballot((kv.x | kv.y) == 0u), m1 = __ballot((kv.z | kv.w) == 0u);
+            if ((lane & 7u) == 0) {
+                const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
+                uint32_t m = 0;
+#pragma unroll
+                for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
+                fm[q / 8] = m;
+                cnt[q / 8] = 0;
+            }
+        }
+    } else {
+        for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += blockDim.x) cnt[j] = 0;
+    }
+    if (threadIdx.x == 0) onext = 0;
+    __syncthreads();
+    // count: entries per bucket. A chunk with at most kBatch entries per thread keeps them in
+    // registers for the placement, each with its bucket (9 bits) and its rank among the
+    // chunk's entries of that bucket (what the count's atomic returned), or kReject / kStray;
+    // a bucket's first popcount(free mask) ranks take its free slots. Larger chunks count
+    // here and count again as they place.
+    constexpr uint32_t kBatch = 8;
+    constexpr uint32_t kReject = ~0u, kStray = ~0u - 1u;
+    static_assert(kChunkBuckets <= 512 && kBatch * THREADS < (1u << 22), "rank and bucket fit 32 bits");
+    const bool held = hi - lo <= kBatch * THREADS;  // block-uniform
+    WideEntry hb[kBatch];
+    uint32_t hp[kBatch];
+    uint32_t stray = 0;
+    if (held) {
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            const uint32_t i = lo + threadIdx.x + u * THREADS;
+            hb[u] = i < hi ? ent[i] : WideEntry{0ull, 0ull};
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            uint32_t at = kReject;  // also past the range (never used)
+            if (lo + threadIdx.x + u * THREADS < hi && !wide_reserved(hb[u].k)) {
+                const uint32_t b = bucket_of_uniform(D, fold64(hb[u].k), t);
+                if (b >= b0 && b - b0 < len) {
+                    at = (atomicAdd(&cnt[b - b0], 1u) << 9) | (b - b0);
+                } else {
+                    at = kStray;
+                    ++stray;
+                }
+            }
+            hp[u] = at;
+        }
+    } else {
+        for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * THREADS) {
+            uint64_t kb[kBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                const uint32_t i = i0 + u * THREADS;
+                kb[u] = i < hi ? ent[i].k : 0ull;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                if (i0 + u * THREADS >= hi || wide_reserved(kb[u])) continue;
+                const uint32_t b = bucket_of_uniform(D, fold64(kb[u]), t);
+                if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
+                else ++stray;
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (!DENSE) {
+        // free masks of the buckets that received entries only (a bucket's 8 parts are read
+        // by 8 consecutive lanes and folded by ballot; the others read as full)
+        constexpr uint32_t kPer = kParts / THREADS;
+        uint4 v[kPer];
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; ++u) {
+            const uint32_t q = threadIdx.x + u * THREADS;
+            v[u] = q < len * (kSlots / 2) && cnt[q / 8] ? ksrc[q] : make_uint4(1u, 1u, 1u, 1u);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; ++u) {
+            const uint32_t q = threadIdx.x + u * THREADS;
+            const bool e0 = (v[u].x | v[u].y) == 0u, e1 = (v[u].z | v[u].w) == 0u;
+            const uint64_t m0 = __ballot(e0), m1 = __ballot(e1);
+            const uint32_t lane = __lane_id();
+            if ((lane & 7u) == 0) {
+                const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
+                uint32_t m = 0;
+#pragma unroll
+                for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
+                fm[q / 8] = m;
+            }
+        }
+        __syncthreads();
+    }
+    uint32_t excess = stray;
+    for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) {
+        const uint32_t f = (uint32_t)__builtin_popcount(fm[j]);
+        excess += cnt[j] > f ? cnt[j] - f : 0u;
+    }
+    excess = (uint32_t)wave_sum((unsigned long long)excess);
+    if (__lane_id() == 0) wred[threadIdx.x / kWave] = excess;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < THREADS / kWave; ++w) tot += wred[w];
+        obase = tot ? atomicAdd(ovf_n, (unsigned long long)tot) : 0ull;
+    }
+    if (!held)  // the placement counts again
+        for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) cnt[j] = 0;
+    __syncthreads();
+    auto put = [&](uint32_t j, uint32_t slot, const WideEntry &e) {
+        if constexpr (DENSE) {
+            kimg[j * kSlots + slot] = e.k;
+            vimg[j * kSlots + slot] = e.v;
+        } else {
+            ksub[(uint64_t)(b0 + j) * kSlots + slot] = e.k;
+            vsub[(uint64_t)(b0 + j) * kSlots + slot] = e.v;
+        }
+    };
+    auto overflow = [&](const WideEntry &e) {  // the counted excess reserved exactly these places
+        const unsigned long long at = obase + atomicAdd(&onext, 1u);
+        if (at < ovf_cap) ovf[at] = e;
+    };
+    if (held) {
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            if (lo + threadIdx.x + u * THREADS >= hi) continue;
+            const uint32_t at = hp[u];
+            if (at == kReject) {
+                ++rejected;
+                continue;
+            }
+            if (at != kStray) {
+                const uint32_t j = at & 511u, r = at >> 9;
+                const uint32_t m = fm[j];
+                if (r < (uint32_t)__builtin_popcount(m)) {
+                    put(j, nth_set_bit16(m, r), hb[u]);
+                    ++placed;
+                    continue;
+                }
+            }
+            overflow(hb[u]);
+        }
+    }
+    for (uint32_t i0 = lo + threadIdx.x; i0 < hi && !held; i0 += kBatch * THREADS) {
+        WideEntry eb[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            const uint32_t i = i0 + u * THREADS;
+            eb[u] = i < hi ? ent[i] : WideEntry{0ull, 0ull};
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            if (i0 + u * THREADS >= hi) continue;
+            const WideEntry e = eb[u];
+            if (wide_reserved(e.k)) {
+                ++rejected;
+                continue;
+            }
+            const uint32_t b = bucket_of_uniform(D, fold64(e.k), t);
+            bool fits = false;
+            if (b >= b0 && b - b0 < len) {
+                const uint32_t j = b - b0;
+                const uint32_t r = atomicAdd(&cnt[j], 1u);
+                const uint32_t m = fm[j];
+                if (r < (uint32_t)__builtin_popcount(m)) {
+                    put(j, nth_set_bit16(m, r), e);
+                    fits = true;
+                    ++placed;
+                }
+            }
+            if (!fits) overflow(e);
+        }
+    }
+    if constexpr (DENSE) {
+        __syncthreads();
+        uint4 *const kd = reinterpret_cast<uint4 *>(ksub + (uint64_t)b0 * kSlots);
+        uint4 *const vd = reinterpret_cast<uint4 *>(vsub + (uint64_t)b0 * kSlots);
+        const uint4 *const ki = reinterpret_cast<const uint4 *>(kimg);
+        const uint4 *const vi = reinterpret_cast<const uint4 *>(vimg);
+        for (uint32_t q = threadIdx.x; q < len * (kSlots / 2); q += THREADS) {
+            kd[q] = ki[q];
+            vd[q] = vi[q];
+        }
+    }
+}
+
+// One workgroup per chunk of one wide table (chunks numbered subtable by subtable, Ch); the
+// entries of chunk c are ent[start[c], start[c + 1]).
+template <bool DENSE, int THREADS>
+__global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chunks Ch, const uint32_t *__restrict__ start,
+                                                                 const WideEntry *__restrict__ ent, uint64_t n,
+                                                                 WideEntry *__restrict__ ovf, uint64_t ovf_cap,
+                                                                 unsigned long long *__restrict__ ovf_n,
+                                                                 Counters *__restrict__ C, bool zeroed)
+{
     const uint32_t c = blockIdx.x;
     // clamped to the entries, so a range is never read past them whatever the bounds hold
     const uint32_t hi = (uint32_t)min((uint64_t)start[c + 1], n);
@@ -647,222 +874,11 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
     unsigned long long placed = 0, rejected = 0;
     if (lo < hi) {  // block-uniform
         uint32_t t = 0;
-        for (uint32_t q = 1; q < D.d; ++q)
+        for (uint32_t q = 1; q < W.D.d; ++q)
             if (Ch.base[q] <= c) t = q;
-        const uint32_t nb = selt(t, D.nb);
         const uint32_t b0 = (c - chunk_base(Ch, t)) << kChunkShift;
-        const uint32_t len = min(kChunkBuckets, nb - b0);
-        uint64_t *const ksub = selt(t, D.sub);
-        uint64_t *const vsub = selt(t, W.val);
-        const uint4 *const ksrc = reinterpret_cast<const uint4 *>(ksub + (uint64_t)b0 * kSlots);
-        constexpr uint32_t kParts = kChunkBuckets * kSlots / 2;  // 16-byte parts of a chunk's lines
-        if constexpr (DENSE) {
-            const uint4 *const vsrc = reinterpret_cast<const uint4 *>(vsub + (uint64_t)b0 * kSlots);
-            uint4 *const ki = reinterpret_cast<uint4 *>(kimg);
-            uint4 *const vi = reinterpret_cast<uint4 *>(vimg);
-            // a bucket's 8 key parts sit in 8 consecutive lanes, whose two ballot bits per part
-            // make its free-slot mask (no strided LDS reads)
-            static_assert(kParts % THREADS == 0, "every lane takes the same number of parts");
-            const uint32_t lane = __lane_id();
-            if (zeroed) {
-                // every slot of the table is known to be zero (the caller's host state says so):
-                // the chunk's lines start empty in LDS without reading them
-                for (uint32_t q = threadIdx.x; q < kParts; q += THREADS) {
-                    ki[q] = make_uint4(0u, 0u, 0u, 0u);
-                    vi[q] = make_uint4(0u, 0u, 0u, 0u);
-                }
-                for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) {
-                    fm[j] = j < len ? 0xffffu : 0u;  // past the end: full
-                    cnt[j] = 0;
-                }
-            }
-            for (uint32_t q = threadIdx.x; q < kParts && !zeroed; q += THREADS) {
-                const bool in = q < len * (kSlots / 2);
-                const uint4 kv = in ? ksrc[q] : make_uint4(~0u, ~0u, ~0u, ~0u);  // past the end: full
-                ki[q] = kv;
-                vi[q] = in ? vsrc[q] : make_uint4(0u, 0u, 0u, 0u);
-                const uint64_t m0 = __ballot((kv.x | kv.y) == 0u), m1 = __ballot((kv.z | kv.w) == 0u);
-                if ((lane & 7u) == 0) {
-                    const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
-                    uint32_t m = 0;
-#pragma unroll
-                    for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
-                    fm[q / 8] = m;
-                    cnt[q / 8] = 0;
-                }
-            }
-        } else {
-            for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += blockDim.x) cnt[j] = 0;
-        }
-        if (threadIdx.x == 0) onext = 0;
-        __syncthreads();
-        // count: entries per bucket. A chunk with at most kBatch entries per thread keeps them in
-        // registers for the placement, each with its bucket (9 bits) and its rank among the
-        // chunk's entries of that bucket (what the count's atomic returned), or kReject / kStray;
-        // a bucket's first popcount(free mask) ranks take its free slots. Larger chunks count
-        // here and count again as they place.
-        constexpr uint32_t kBatch = 8;
-        constexpr uint32_t kReject = ~0u, kStray = ~0u - 1u;
-        static_assert(kChunkBuckets <= 512 && kBatch * THREADS < (1u << 22), "rank and bucket fit 32 bits");
-        const bool held = hi - lo <= kBatch * THREADS;  // block-uniform
-        WideEntry hb[kBatch];
-        uint32_t hp[kBatch];
-        uint32_t stray = 0;
-        if (held) {
-#pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                const uint32_t i = lo + threadIdx.x + u * THREADS;
-                hb[u] = i < hi ? ent[i] : WideEntry{0ull, 0ull};
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                uint32_t at = kReject;  // also past the range (never used)
-                if (lo + threadIdx.x + u * THREADS < hi && !wide_reserved(hb[u].k)) {
-                    const uint32_t b = bucket_of_uniform(D, fold64(hb[u].k), t);
-                    if (b >= b0 && b - b0 < len) {
-                        at = (atomicAdd(&cnt[b - b0], 1u) << 9) | (b - b0);
-                    } else {
-                        at = kStray;
-                        ++stray;
-                    }
-                }
-                hp[u] = at;
-            }
-        } else {
-            for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBatch * THREADS) {
-                uint64_t kb[kBatch];
-#pragma unroll
-                for (uint32_t u = 0; u < kBatch; ++u) {
-                    const uint32_t i = i0 + u * THREADS;
-                    kb[u] = i < hi ? ent[i].k : 0ull;
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < kBatch; ++u) {
-                    if (i0 + u * THREADS >= hi || wide_reserved(kb[u])) continue;
-                    const uint32_t b = bucket_of_uniform(D, fold64(kb[u]), t);
-                    if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
-                    else ++stray;
-                }
-            }
-        }
-        __syncthreads();
-        if constexpr (!DENSE) {
-            // free masks of the buckets that received entries only (a bucket's 8 parts are read
-            // by 8 consecutive lanes and folded by ballot; the others read as full)
-            constexpr uint32_t kPer = kParts / THREADS;
-            uint4 v[kPer];
-#pragma unroll
-            for (uint32_t u = 0; u < kPer; ++u) {
-                const uint32_t q = threadIdx.x + u * THREADS;
-                v[u] = q < len * (kSlots / 2) && cnt[q / 8] ? ksrc[q] : make_uint4(1u, 1u, 1u, 1u);
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kPer; ++u) {
-                const uint32_t q = threadIdx.x + u * THREADS;
-                const bool e0 = (v[u].x | v[u].y) == 0u, e1 = (v[u].z | v[u].w) == 0u;
-                const uint64_t m0 = __ballot(e0), m1 = __ballot(e1);
-                const uint32_t lane = __lane_id();
-                if ((lane & 7u) == 0) {
-                    const uint32_t a = (uint32_t)(m0 >> lane) & 0xffu, b = (uint32_t)(m1 >> lane) & 0xffu;
-                    uint32_t m = 0;
-#pragma unroll
-                    for (int p = 0; p < 8; ++p) m |= (((a >> p) & 1u) << (2 * p)) | (((b >> p) & 1u) << (2 * p + 1));
-                    fm[q / 8] = m;
-                }
-            }
-            __syncthreads();
-        }
-        uint32_t excess = stray;
-        for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) {
-            const uint32_t f = (uint32_t)__builtin_popcount(fm[j]);
-            excess += cnt[j] > f ? cnt[j] - f : 0u;
-        }
-        excess = (uint32_t)wave_sum((unsigned long long)excess);
-        if (__lane_id() == 0) wred[threadIdx.x / kWave] = excess;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t tot = 0;
-            for (uint32_t w = 0; w < THREADS / kWave; ++w) tot += wred[w];
-            obase = tot ? atomicAdd(ovf_n, (unsigned long long)tot) : 0ull;
-        }
-        if (!held)  // the placement counts again
-            for (uint32_t j = threadIdx.x; j < kChunkBuckets; j += THREADS) cnt[j] = 0;
-        __syncthreads();
-        auto put = [&](uint32_t j, uint32_t slot, const WideEntry &e) {
-            if constexpr (DENSE) {
-                kimg[j * kSlots + slot] = e.k;
-                vimg[j * kSlots + slot] = e.v;
-            } else {
-                ksub[(uint64_t)(b0 + j) * kSlots + slot] = e.k;
-                vsub[(uint64_t)(b0 + j) * kSlots + slot] = e.v;
-            }
-        };
-        auto overflow = [&](const WideEntry &e) {  // the counted excess reserved exactly these places
-            const unsigned long long at = obase + atomicAdd(&onext, 1u);
-            if (at < ovf_cap) ovf[at] = e;
-        };
-        if (held) {
-#pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                if (lo + threadIdx.x + u * THREADS >= hi) continue;
-                const uint32_t at = hp[u];
-                if (at == kReject) {
-                    ++rejected;
-                    continue;
-                }
-                if (at != kStray) {
-                    const uint32_t j = at & 511u, r = at >> 9;
-                    const uint32_t m = fm[j];
-                    if (r < (uint32_t)__builtin_popcount(m)) {
-                        put(j, nth_set_bit16(m, r), hb[u]);
-                        ++placed;
-                        continue;
-                    }
-                }
-                overflow(hb[u]);
-            }
-        }
-        for (uint32_t i0 = lo + threadIdx.x; i0 < hi && !held; i0 += kBatch * THREADS) {
-            WideEntry eb[kBatch];
-#pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                const uint32_t i = i0 + u * THREADS;
-                eb[u] = i < hi ? ent[i] : WideEntry{0ull, 0ull};
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                if (i0 + u * THREADS >= hi) continue;
-                const WideEntry e = eb[u];
-                if (wide_reserved(e.k)) {
-                    ++rejected;
-                    continue;
-                }
-                const uint32_t b = bucket_of_uniform(D, fold64(e.k), t);
-                bool fits = false;
-                if (b >= b0 && b - b0 < len) {
-                    const uint32_t j = b - b0;
-                    const uint32_t r = atomicAdd(&cnt[j], 1u);
-                    const uint32_t m = fm[j];
-                    if (r < (uint32_t)__builtin_popcount(m)) {
-                        put(j, nth_set_bit16(m, r), e);
-                        fits = true;
-                        ++placed;
-                    }
-                }
-                if (!fits) overflow(e);
-            }
-        }
-        if constexpr (DENSE) {
-            __syncthreads();
-            uint4 *const kd = reinterpret_cast<uint4 *>(ksub + (uint64_t)b0 * kSlots);
-            uint4 *const vd = reinterpret_cast<uint4 *>(vsub + (uint64_t)b0 * kSlots);
-            const uint4 *const ki = reinterpret_cast<const uint4 *>(kimg);
-            const uint4 *const vi = reinterpret_cast<const uint4 *>(vimg);
-            for (uint32_t q = threadIdx.x; q < len * (kSlots / 2); q += THREADS) {
-                kd[q] = ki[q];
-                vd[q] = vi[q];
-            }
-        }
+        const uint32_t len = min(kChunkBuckets, selt(t, W.D.nb) - b0);
+        wide_fill_chunk<DENSE, THREADS>(W, t, b0, len, ent, lo, hi, ovf, ovf_cap, ovf_n, zeroed, placed, rejected);
     }
     wave_tally(C, &C->kicks, placed);
     wave_tally(C, &C->rejected, rejected);
@@ -881,6 +897,114 @@ __global__ __launch_bounds__(256) void wide_split_kernel(const WideEntry *__rest
 }
 
 }  // namespace
+
+// ------------------------------------------------------------------ bulk fill of many tables
+// The wide form of the group bulk fill (dyc_bulk.hip): keys fold to 32 bits for the pair and the
+// bucket (fold64), and entries are 16-byte key/value pairs.
+
+template <bool IN_ENTRIES>
+__global__ __launch_bounds__(kCountThreads) void wide_group_bulk_pack_kernel(const WideDesc *__restrict__ dir,
+                                                                             const GroupItem *__restrict__ blocks,
+                                                                             const uint64_t *__restrict__ keys,
+                                                                             const WideEntry *__restrict__ in,
+                                                                             uint8_t *__restrict__ lcid,
+                                                                             uint32_t *__restrict__ partial)
+{
+    __shared__ uint32_t h[kGroupBulkChunks];
+    __shared__ uint4 sub_tab[kMaxSubtables];  // per subtable: buckets, divider, the table's first local chunk of it
+    const GroupItem item = blocks[blockIdx.x];
+    const Desc &D = dir[item.table].D;
+    for (uint32_t d = threadIdx.x; d < kGroupBulkChunks; d += kCountThreads) h[d] = 0;
+    if (threadIdx.x < kMaxSubtables) {
+        const uint32_t t = threadIdx.x;
+        uint32_t first = 0;
+        for (uint32_t q = 0; q < t && q < D.d; ++q) first += (selt(q, D.nb) + kChunkBuckets - 1) >> kChunkShift;
+        sub_tab[t] = make_uint4(selt(t, D.nb), selt(t, D.m), selt(t, D.sh), first);
+    }
+    __syncthreads();
+    const uint64_t lo = item.begin, hi = item.begin + item.len;
+    for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)kCountThreads * kCountItems) {
+        uint64_t k[kCountItems];
+#pragma unroll
+        for (int u = 0; u < kCountItems; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kCountThreads;
+            if constexpr (IN_ENTRIES) k[u] = i < hi ? in[i].k : 0ull;
+            else k[u] = i < hi ? keys[i] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kCountItems; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kCountThreads;
+            if (i >= hi) continue;
+            uint32_t lc = 0;
+            if (IN_ENTRIES || !wide_reserved(k[u])) {
+                const uint32_t f = fold64(k[u]), p = pair_of(D, f);
+                uint32_t t = (k[u] & 1u) ? pos2(D, p) : pos1(D, p);
+                if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
+                const uint4 q = sub_tab[t];
+                lc = q.w + (fast_mod_sh(hash_t(f, t), q.x, q.y, q.z) >> kChunkShift);
+            }
+            lcid[i] = (uint8_t)lc;  // < kGroupBulkChunks: the host admits tables of at most that many
+            atomicAdd(&h[lc & (kGroupBulkChunks - 1)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < kGroupBulkChunks; d += kCountThreads)
+        partial[(uint64_t)blockIdx.x * kGroupBulkChunks + d] = h[d];
+}
+
+template <bool DENSE, int THREADS>
+__global__ __launch_bounds__(THREADS) void wide_group_bulk_fill_kernel(
+    const WideDesc *__restrict__ dir, const BulkTable *__restrict__ tabs, const uint32_t *__restrict__ cstart,
+    const uint32_t *__restrict__ ccount, const uint32_t *__restrict__ ctable, const WideEntry *__restrict__ ent,
+    WideEntry *__restrict__ ovf, unsigned long long *__restrict__ ovf_n, unsigned long long *__restrict__ table_counts,
+    Counters *__restrict__ C, bool zeroed)
+{
+    const uint32_t c = blockIdx.x;
+    const uint32_t lo = cstart[c], cnt = ccount[c];
+    unsigned long long placed = 0, rejected = 0;
+    uint32_t table = 0;
+    if (cnt) {  // block-uniform
+        const uint32_t j = ctable[c];
+        const BulkTable T = tabs[j];
+        table = T.table;
+        const WideDesc &W = dir[T.table];
+        uint32_t lc = c - T.chunk0, t = 0;
+        bool found = false;
+        for (uint32_t q = 0; q < W.D.d && !found; ++q) {
+            const uint32_t k = (selt(q, W.D.nb) + kChunkBuckets - 1) >> kChunkShift;
+            if (lc < k) {
+                t = q;
+                found = true;
+            } else {
+                lc -= k;
+            }
+        }
+        if (found) {
+            const uint32_t b0 = lc << kChunkShift;
+            const uint32_t len = min(kChunkBuckets, selt(t, W.D.nb) - b0);
+            wide_fill_chunk<DENSE, THREADS>(W, t, b0, len, ent, lo, lo + cnt, ovf + T.off, T.n, ovf_n + j, zeroed,
+                                            placed, rejected);
+        }
+    }
+    const unsigned long long wp = wave_sum(placed);
+    if (__lane_id() == 0 && wp) atomicAdd(&table_counts[table], wp);
+    wave_tally(C, &C->kicks, placed);
+    wave_tally(C, &C->rejected, rejected);
+}
+
+// The overflow entries of every item as the key and value arrays the group insert reads (same
+// indexes); one workgroup per item.
+__global__ __launch_bounds__(256) void wide_items_split_kernel(const GroupItem *__restrict__ items,
+                                                               const WideEntry *__restrict__ in,
+                                                               uint64_t *__restrict__ keys, uint64_t *__restrict__ vals)
+{
+    const GroupItem it = items[blockIdx.x];
+    for (uint64_t i = it.begin + threadIdx.x; i < it.begin + it.len; i += blockDim.x) {
+        const WideEntry e = in[i];
+        keys[i] = e.k;
+        vals[i] = e.v;
+    }
+}
 
 // ------------------------------------------------------------------------------ launchers
 hipError_t launch_wide_probe(const WideDesc &W, const uint64_t *keys, uint64_t *vals, uint8_t *flags, uint64_t n,
@@ -1874,6 +1998,26 @@ struct dyc_wide_group {
     size_t items_cap = 0;
     hipEvent_t items_free = nullptr;
     dyc_stats st{};
+    // every key and value word of every table is zero: set at create and clear, dropped by every
+    // insert
+    bool zeroed = false;
+    // bulk fill (cfg.insert_order 3), grown on demand: partitioned entries and the overflow (16 B
+    // per key each; the entry buffer also holds the last overflow split into keys and values),
+    // local chunk ids, per-block chunk counts, per-chunk start / count / table, overflow counts
+    // of both rounds, and the block and table arrays (device, with pinned staging)
+    WideEntry *bk_ent = nullptr, *bk_ovf = nullptr;
+    uint8_t *bk_lcid = nullptr;
+    uint64_t bk_cap = 0;
+    uint32_t *bk_partial = nullptr;
+    uint64_t bk_partial_cap = 0;
+    uint32_t *bk_cstart = nullptr, *bk_ccount = nullptr, *bk_ctable = nullptr;
+    uint64_t bk_chunk_cap = 0;
+    unsigned long long *bk_ovfn = nullptr, *bk_hovfn = nullptr;
+    GroupItem *bk_blk = nullptr, *bk_hblk = nullptr;
+    uint32_t *bk_btab = nullptr, *bk_hbtab = nullptr;
+    uint64_t bk_blk_cap = 0;
+    BulkTable *bk_tabs = nullptr, *bk_htabs = nullptr;
+    uint32_t bk_tabs_cap = 0;
 
     InsertPolicy policy() const { return InsertPolicy{cfg.kick_bound, cfg.rescue_kicks, cfg.max_retries}; }
     WideStash stash() const { return WideStash{pk, pv, pend_n, ptag, 0}; }
@@ -2002,6 +2146,177 @@ int wg_compact_stash(dyc_wide_group *g, hipStream_t s)
     return DYC_OK;
 }
 
+// ------------------------------------------------------------------ bulk fill
+int wg_bulk_alloc(void **p, size_t bytes, const char *what)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(DYC_ERR_OOM, "hipMalloc of the wide group bulk-fill %s (%zu bytes) failed", what, bytes);
+    }
+    return DYC_OK;
+}
+
+// Grow-only workspace of a laid-out batch (the caller has synchronized the stream).
+int wg_reserve_bulk(dyc_wide_group *g, const dyc::host::BulkLayout &L)
+{
+    const uint64_t n = L.total, blocks = L.blocks.size(), chunks = L.chunks;
+    const uint32_t tabs = (uint32_t)L.tabs.size();
+    int rc = DYC_OK;
+    auto regrow = [&](void **p, size_t bytes, const char *what) {
+        if (rc) return;
+        if (*p) (void)hipFree(*p);
+        rc = wg_bulk_alloc(p, bytes, what);
+    };
+    auto regrow_pinned = [&](void **p, size_t bytes) {
+        if (rc) return;
+        if (*p) (void)hipHostFree(*p);
+        *p = nullptr;
+        if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            rc = fail(DYC_ERR_OOM, "pinned wide group bulk-fill staging (%zu bytes) failed", bytes);
+        }
+    };
+    if (n > g->bk_cap) {
+        g->bk_cap = 0;
+        regrow((void **)&g->bk_ent, n * sizeof(WideEntry), "entries");
+        regrow((void **)&g->bk_ovf, n * sizeof(WideEntry), "overflow list");
+        regrow((void **)&g->bk_lcid, n, "chunk ids");
+        if (rc) return rc;
+        g->bk_cap = n;
+    }
+    if (blocks * kGroupBulkChunks > g->bk_partial_cap) {
+        g->bk_partial_cap = 0;
+        regrow((void **)&g->bk_partial, blocks * kGroupBulkChunks * sizeof(uint32_t), "block counts");
+        if (rc) return rc;
+        g->bk_partial_cap = blocks * kGroupBulkChunks;
+    }
+    if (blocks > g->bk_blk_cap) {
+        g->bk_blk_cap = 0;
+        regrow((void **)&g->bk_blk, 3 * blocks * sizeof(GroupItem), "blocks");
+        regrow((void **)&g->bk_btab, blocks * sizeof(uint32_t), "block tables");
+        regrow_pinned((void **)&g->bk_hblk, blocks * sizeof(GroupItem));
+        regrow_pinned((void **)&g->bk_hbtab, blocks * sizeof(uint32_t));
+        if (rc) return rc;
+        g->bk_blk_cap = blocks;
+    }
+    if (chunks > g->bk_chunk_cap) {
+        g->bk_chunk_cap = 0;
+        regrow((void **)&g->bk_cstart, chunks * sizeof(uint32_t), "chunk starts");
+        regrow((void **)&g->bk_ccount, chunks * sizeof(uint32_t), "chunk counts");
+        regrow((void **)&g->bk_ctable, chunks * sizeof(uint32_t), "chunk tables");
+        if (rc) return rc;
+        g->bk_chunk_cap = chunks;
+    }
+    if (tabs > g->bk_tabs_cap) {
+        g->bk_tabs_cap = 0;
+        regrow((void **)&g->bk_tabs, 2 * tabs * sizeof(BulkTable), "table list");
+        regrow((void **)&g->bk_ovfn, 2 * tabs * sizeof(unsigned long long), "overflow counts");
+        regrow_pinned((void **)&g->bk_htabs, tabs * sizeof(BulkTable));
+        regrow_pinned((void **)&g->bk_hovfn, tabs * sizeof(unsigned long long));
+        if (rc) return rc;
+        g->bk_tabs_cap = tabs;
+    }
+    return DYC_OK;
+}
+
+// The whole wide bulk fill of a laid-out batch on s, rounds chained on the device as in the
+// 32-bit group fill (dyc_group.hip): round 1, round 2 for the other bucket, then the eviction
+// insert of what overflows that (its entries split into key and value arrays first).
+hipError_t wg_enqueue_bulk(dyc_wide_group *g, uint32_t NB, uint32_t J, uint32_t chunks, const uint64_t *keys,
+                           const uint64_t *vals, uint64_t total, bool zeroed, const WideSink &sink, hipStream_t s)
+{
+    GroupItem *const blk1 = g->bk_blk, *const blk2 = g->bk_blk + NB, *const items = g->bk_blk + 2 * NB;
+    BulkTable *const tabs1 = g->bk_tabs, *const tabs2 = g->bk_tabs + J;
+    unsigned long long *const ovf1 = g->bk_ovfn, *const ovf2 = g->bk_ovfn + J;
+    const bool dense2 = g->cfg.subtables == 2;  // every key starts in subtable 0: half overflow round 1
+    uint8_t *const no_cid = nullptr;
+    const SegLayout lay1{blk1, g->bk_partial, total}, lay2{blk2, g->bk_partial, total};
+    hipError_t e;
+    wide_group_bulk_pack_kernel<false><<<NB, kCountThreads, 0, s>>>(g->ddir, blk1, keys, nullptr, g->bk_lcid,
+                                                                    g->bk_partial);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_group_bulk_scan(tabs1, J, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s)) !=
+        hipSuccess)
+        return e;
+    part_scatter_kernel<WideEntry, WideKeyValSource, uint8_t, kWideSortThreads, -1, SegLayout>
+        <<<NB, kWideSortThreads, 0, s>>>(WideKeyValSource{keys, vals}, g->bk_lcid, lay1, RadixPass{0, 8}, g->bk_ent,
+                                         no_cid);
+    if ((e = hipGetLastError()) != hipSuccess || (e = launch_zero(ovf1, 2 * (uint64_t)J, s)) != hipSuccess) return e;
+    wide_group_bulk_fill_kernel<true, 1024><<<chunks, 1024, 0, s>>>(g->ddir, tabs1, g->bk_cstart, g->bk_ccount,
+                                                                   g->bk_ctable, g->bk_ent, g->bk_ovf, ovf1,
+                                                                   g->tcounts, g->ctr, zeroed);
+    if ((e = hipGetLastError()) != hipSuccess ||
+        (e = launch_group_bulk_clip(blk1, g->bk_btab, NB, tabs1, J, ovf1, blk2, tabs2, s)) != hipSuccess)
+        return e;
+    wide_group_bulk_pack_kernel<true><<<NB, kCountThreads, 0, s>>>(g->ddir, blk2, nullptr, g->bk_ovf, g->bk_lcid,
+                                                                   g->bk_partial);
+    if ((e = hipGetLastError()) != hipSuccess ||
+        (e = launch_group_bulk_scan(tabs2, J, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s)) !=
+            hipSuccess)
+        return e;
+    part_scatter_kernel<WideEntry, EntrySource<WideEntry>, uint8_t, kWideSortThreads, -1, SegLayout>
+        <<<NB, kWideSortThreads, 0, s>>>(EntrySource<WideEntry>{g->bk_ovf}, g->bk_lcid, lay2, RadixPass{0, 8},
+                                         g->bk_ent, no_cid);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (dense2)
+        wide_group_bulk_fill_kernel<true, 1024><<<chunks, 1024, 0, s>>>(g->ddir, tabs2, g->bk_cstart, g->bk_ccount,
+                                                                       g->bk_ctable, g->bk_ent, g->bk_ovf, ovf2,
+                                                                       g->tcounts, g->ctr, false);
+    else
+        wide_group_bulk_fill_kernel<false, 256><<<chunks, 256, 0, s>>>(g->ddir, tabs2, g->bk_cstart, g->bk_ccount,
+                                                                      g->bk_ctable, g->bk_ent, g->bk_ovf, ovf2,
+                                                                      g->tcounts, g->ctr, false);
+    if ((e = hipGetLastError()) != hipSuccess ||
+        (e = launch_group_bulk_clip(blk1, g->bk_btab, NB, tabs1, J, ovf2, items, nullptr, s)) != hipSuccess)
+        return e;
+    // the entry buffer (16 B per key) is free now: keys, then values, at the overflow's indexes
+    uint64_t *const sk = reinterpret_cast<uint64_t *>(g->bk_ent);
+    uint64_t *const sv = sk + total;
+    wide_items_split_kernel<<<NB, 256, 0, s>>>(items, g->bk_ovf, sk, sv);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return launch_wide_group_insert(g->ddir, items, NB, sk, sv, g->policy(), (int)g->cfg.insert_blocks, g->ctr, sink,
+                                    g->tcounts, s);
+}
+
+// insert_order 3 for a wide group (see group_bulk_insert in dyc_group.hip). Declines (*n_tabs 0)
+// when the batch does not qualify or the workspace cannot be allocated; counters and per-table
+// counts are zeroed here; the last round's overflow counts (*n_tabs) are in bk_hovfn after the
+// caller's wait.
+int wg_bulk_insert(dyc_wide_group *g, const uint64_t *keys, const uint64_t *vals, const uint64_t *offsets,
+                   bool zeroed, hipStream_t s, uint64_t room, uint32_t *n_tabs)
+{
+    *n_tabs = 0;
+    dyc::host::BulkLayout L;
+    if (!dyc::host::bulk_layout(g->n, offsets, [g](uint32_t i) -> const Desc & { return g->hdir[i].D; }, &L))
+        return DYC_OK;
+    {
+        HIP_TRY(hipStreamSynchronize(s));  // buffers being replaced are no longer in use
+        const dyc::host::ErrorKeep keep;
+        const int rc = wg_reserve_bulk(g, L);
+        if (rc == DYC_ERR_OOM) {  // no room for the workspace: the eviction insert needs none
+            keep.restore();
+            return DYC_OK;
+        }
+        if (rc) return rc;
+    }
+    const uint32_t NB = (uint32_t)L.blocks.size(), J = (uint32_t)L.tabs.size();
+    memcpy(g->bk_hblk, L.blocks.data(), NB * sizeof(GroupItem));
+    memcpy(g->bk_hbtab, L.btab.data(), NB * sizeof(uint32_t));
+    memcpy(g->bk_htabs, L.tabs.data(), J * sizeof(BulkTable));
+    HIP_TRY(hipMemcpyAsync(g->bk_blk, g->bk_hblk, NB * sizeof(GroupItem), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(g->bk_btab, g->bk_hbtab, NB * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(g->bk_tabs, g->bk_htabs, J * sizeof(BulkTable), hipMemcpyHostToDevice, s));
+    const int rc = wg_prologue(g, s);
+    if (rc) return rc;
+    HIP_TRY(wg_enqueue_bulk(g, NB, J, L.chunks, keys, vals, L.total, zeroed,
+                            WideSink{g->pk + g->pend_n, g->pv + g->pend_n, room, g->ptag + g->pend_n}, s));
+    HIP_TRY(hipMemcpyAsync(g->bk_hovfn, g->bk_ovfn + J, J * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    *n_tabs = J;
+    return DYC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2023,6 +2338,12 @@ int dyc_wide_group_destroy(dyc_wide_group *g)
     if (g->ditems) (void)hipFree(g->ditems);
     if (g->hitems) (void)hipHostFree(g->hitems);
     if (g->items_free) (void)hipEventDestroy(g->items_free);
+    for (void *p : {(void *)g->bk_ent, (void *)g->bk_ovf, (void *)g->bk_lcid, (void *)g->bk_partial,
+                    (void *)g->bk_cstart, (void *)g->bk_ccount, (void *)g->bk_ctable, (void *)g->bk_tabs,
+                    (void *)g->bk_ovfn, (void *)g->bk_blk, (void *)g->bk_btab})
+        if (p) (void)hipFree(p);
+    for (void *p : {(void *)g->bk_htabs, (void *)g->bk_hovfn, (void *)g->bk_hblk, (void *)g->bk_hbtab})
+        if (p) (void)hipHostFree(p);
     delete g;
     return DYC_OK;
 }
@@ -2094,6 +2415,7 @@ int dyc_wide_group_create(const dyc_config *cfg_in, uint32_t n_tables, const uin
     }
     int rc = wg_reserve_pending(g, 65536, nullptr);
     if (rc) return cleanup(rc);
+    g->zeroed = true;
     *out = g;
     return DYC_OK;
 }
@@ -2108,20 +2430,36 @@ int dyc_wide_group_insert(dyc_wide_group *g, const uint64_t *keys, const uint64_
     if (total && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
+    const bool zeroed = g->zeroed;
+    g->zeroed = false;
+    g->st.last_filled = 0;
     rc = wg_reserve_pending(g, std::max<uint64_t>(65536, total / 16), s);
     if (rc) return rc;
-    uint32_t n_items = 0;
-    rc = wg_upload_items(g, offsets, s, &n_items);
-    if (rc) return rc;
-    rc = wg_prologue(g, s);
-    if (rc) return rc;
     const uint64_t room = g->pend_cap - g->pend_n;
-    HIP_TRY(launch_wide_group_insert(g->ddir, g->ditems, n_items, keys, vals, g->policy(), (int)g->cfg.insert_blocks,
-                                     g->ctr, WideSink{g->pk + g->pend_n, g->pv + g->pend_n, room, g->ptag + g->pend_n},
-                                     g->tcounts, s));
+    uint32_t n_tabs = 0;
+    if (g->cfg.insert_order == 3) {
+        rc = wg_bulk_insert(g, keys, vals, offsets, zeroed, s, room, &n_tabs);
+        if (rc) return rc;
+    }
+    if (!n_tabs) {
+        uint32_t n_items = 0;
+        rc = wg_upload_items(g, offsets, s, &n_items);
+        if (rc) return rc;
+        rc = wg_prologue(g, s);
+        if (rc) return rc;
+        HIP_TRY(launch_wide_group_insert(g->ddir, g->ditems, n_items, keys, vals, g->policy(),
+                                         (int)g->cfg.insert_blocks, g->ctr,
+                                         WideSink{g->pk + g->pend_n, g->pv + g->pend_n, room, g->ptag + g->pend_n},
+                                         g->tcounts, s));
+    }
     rc = wg_read_back(g, s);
     if (rc) return rc;
     const Counters &c = *g->hctr;
+    if (n_tabs) {
+        uint64_t left = 0;
+        for (uint32_t j = 0; j < n_tabs; ++j) left += g->bk_hovfn[j];
+        g->st.last_filled = total - c.rejected - left;
+    }
     g->st.kicks += c.kicks;
     g->st.last_kicks = c.kicks;
     g->st.cas_retries += c.retries;
@@ -2188,6 +2526,8 @@ int dyc_wide_group_insert_per_table(dyc_wide_group *g, const uint64_t *keys, con
     if (offsets[g->n] && (!keys || !vals)) return fail(DYC_ERR_INVALID, "null keys/vals");
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
+    g->zeroed = false;
+    g->st.last_filled = 0;
     rc = wg_reserve_pending(g, std::max<uint64_t>(65536, offsets[g->n] / 16), s);
     if (rc) return rc;
     rc = wg_prologue(g, s);
@@ -2299,6 +2639,7 @@ int dyc_wide_group_clear(dyc_wide_group *g, dyc_stream stream)
     std::fill(g->count.begin(), g->count.end(), 0);
     g->pend_n = 0;
     g->st.pending = 0;
+    g->zeroed = true;
     return DYC_OK;
 }
 

@@ -154,3 +154,54 @@ def test_wide_group_stash_serves_pending_entries_of_their_table(D, per_table):
     assert pk.size == 0 and g.table_stats(1)["pending"] == 0
     assert g.find(cu(np.array([key])), one)[1].cpu().tolist() == [0]
     g.close()
+
+
+@pytest.mark.parametrize("d", (2, 4))
+def test_wide_group_bulk_fill_matches_oracle_per_table(D, d):
+    """insert_order 3: each table's batch partitioned by its own chunks, chunk fill, a second
+    round for the other bucket, the eviction insert of the rest (rounds chained on the device).
+    A batch on top of it reads the tables; a refill after clear does not; every table equals the
+    wide oracle each time, and last_filled says the fill placed nearly every key."""
+    banner(f"wide group bulk fill, subtables={d}")
+    sizes = sizes_loguniform(23, 2**8, 2**15, 140 + d)
+    sizes[4] = 0
+    total = int(sizes.sum())
+    keys = keys64(2 * total, 141 + d)
+    keys[: min(300, keys.size)] &= np.uint64(0xFFFFFFFF)  # some keys below 2^32 (fold(k) = k)
+    keys[keys == 0] = 9
+    keys = np.unique(keys)[: 2 * total]
+    np.random.default_rng(142 + d).shuffle(keys)
+    vals = vals_of(keys)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    hints = (np.ceil(2 * np.maximum(sizes, 1) / 0.85) + 64).astype(np.uint64)
+    g = D.DyCuckooWideGroup(hints, subtables=d, insert_order=3)
+
+    def batch(lo):
+        return keys[lo:lo + total], vals[lo:lo + total]
+
+    def check(parts):
+        for i in range(len(sizes)):
+            st = g.table_stats(i)
+            ks = np.concatenate([p[0][offs[i]:offs[i + 1]] for p in parts])
+            vs = np.concatenate([p[1][offs[i]:offs[i + 1]] for p in parts])
+            assert st["count"] == ks.size and st["subtables"] == d
+            ref = oracle_of(g, i)
+            assert ref.check_invariant() == 0
+            ek, ev = ref.entries()
+            o = np.lexsort((vs, ks))
+            assert np.array_equal(ek, ks[o]) and np.array_equal(ev, vs[o])
+
+    a, b = batch(0), batch(total)
+    for part in (a, b):
+        g.insert(cu(part[0]), cu(part[1]), offs)
+        assert g.table_stats(0)["last_filled"] > 0.9 * total
+    check([a, b])
+    g.clear()
+    g.insert(cu(b[0]), cu(b[1]), offs)
+    assert g.table_stats(0)["last_filled"] > 0.9 * total
+    check([b])
+    v, h = g.find(cu(b[0]), offs)
+    assert h.cpu().numpy().all() and np.array_equal(v.cpu().numpy().view(np.uint64), b[1])
+    _, h = g.find(cu(a[0]), offs)
+    assert not h.cpu().numpy().any()
+    g.close()
