@@ -92,13 +92,13 @@ __global__ __launch_bounds__(kCountThreads) void bulk_pack_kernel(Desc D, Chunks
 // load or store. Otherwise the entries are counted first, the free-slot masks of the buckets
 // that received any are read coalesced (8 lanes per bucket, folded by ballot), and each placed
 // entry is one store into its slot.
-// The fill of one chunk by the whole workgroup: buckets [b0, b0 + len) of subtable t of table D,
-// entries ent[lo, hi) (every one targets subtable t). Entries that do not fit are appended to
+// The fill of one chunk by the whole workgroup: buckets [b0, b0 + len) of subtable S,
+// entries ent[lo, hi) (every one targets S). Entries that do not fit are appended to
 // ovf[0, ovf_cap) at the index *ovf_n hands out (one atomic for the chunk's whole excess).
 // Placements and rejected reserved keys are added to this lane's `placed` and `rejected`.
 // Every thread of the workgroup calls it with the same arguments.
 template <bool DENSE, int THREADS>
-__device__ inline void fill_chunk(const Desc &D, uint32_t t, uint32_t b0, uint32_t len,
+__device__ inline void fill_chunk(const SubRef &S, uint32_t b0, uint32_t len,
                                   const uint64_t *__restrict__ ent, uint32_t lo, uint32_t hi,
                                   uint64_t *__restrict__ ovf, uint64_t ovf_cap, unsigned long long *__restrict__ ovf_n,
                                   bool zeroed, unsigned long long &placed, unsigned long long &rejected)
@@ -110,7 +110,7 @@ __device__ inline void fill_chunk(const Desc &D, uint32_t t, uint32_t b0, uint32
     __shared__ uint32_t wred[THREADS / kWave];
     __shared__ unsigned long long obase;
     __shared__ uint32_t onext;
-    uint64_t *const sub = selt(t, D.sub);
+    uint64_t *const sub = S.keys;
     const uint4 *const src = reinterpret_cast<const uint4 *>(sub + (uint64_t)b0 * kSlots);
     constexpr uint32_t kParts = kChunkBuckets * kSlots / 2;  // 16-byte parts of the chunk
     if constexpr (DENSE) {
@@ -182,7 +182,7 @@ __device__ inline void fill_chunk(const Desc &D, uint32_t t, uint32_t b0, uint32
         for (uint32_t u = 0; u < kBatch; ++u) {
             const uint32_t k = key_of(eb[u]);
             if (i0 + u * THREADS >= hi || reserved_key(k)) continue;
-            const uint32_t b = bucket_of_uniform(D, k, t);
+            const uint32_t b = bucket_of_uniform(S, k);
             if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
             else ++stray;
         }
@@ -197,7 +197,7 @@ __device__ inline void fill_chunk(const Desc &D, uint32_t t, uint32_t b0, uint32
                 const uint32_t k = key_of(hb[h][u]);
                 uint32_t at = kReject;  // also past the range (never used)
                 if (lo + threadIdx.x + (h * kBatch + u) * THREADS < hi && !reserved_key(k)) {
-                    const uint32_t b = bucket_of_uniform(D, k, t);
+                    const uint32_t b = bucket_of_uniform(S, k);
                     if (b >= b0 && b - b0 < len) {
                         at = (atomicAdd(&cnt[b - b0], 1u) << 9) | (b - b0);
                     } else {
@@ -277,7 +277,7 @@ __device__ inline void fill_chunk(const Desc &D, uint32_t t, uint32_t b0, uint32
                 ++rejected;
                 continue;
             }
-            const uint32_t b = bucket_of_uniform(D, k, t);
+            const uint32_t b = bucket_of_uniform(S, k);
             bool fits = false;
             if (b >= b0 && b - b0 < len) {
                 const uint32_t j = b - b0;
@@ -354,7 +354,8 @@ __global__ __launch_bounds__(THREADS) void bulk_fill_kernel(Desc D, Chunks Ch, c
             if (Ch.base[q] <= c) t = q;
         const uint32_t b0 = (c - chunk_base(Ch, t)) << kChunkShift;
         const uint32_t len = min(kChunkBuckets, selt(t, D.nb) - b0);
-        fill_chunk<DENSE, THREADS>(D, t, b0, len, ent, lo, hi, ovf, ovf_cap, ovf_n, zeroed, placed, rejected);
+        fill_chunk<DENSE, THREADS>(sub_ref(D, t), b0, len, ent, lo, hi, ovf, ovf_cap, ovf_n, zeroed, placed,
+                                   rejected);
     }
     wave_tally(C, &C->kicks, placed);
     wave_tally(C, &C->rejected, rejected);
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(kCountThreads) void group_bulk_pack_kernel(const De
     // per subtable: buckets, FastDiv multiplier and shifts, the table's first local chunk of it
     __shared__ uint4 sub_tab[kMaxSubtables];
     const GroupItem item = blocks[blockIdx.x];
-    const Desc &D = dir[item.table];
+    const Desc D = dir[item.table];  // by value: loaded once, not per use through memory
     for (uint32_t d = threadIdx.x; d < kGroupBulkChunks; d += kCountThreads) h[d] = 0;
     if (threadIdx.x < kMaxSubtables) {
         const uint32_t t = threadIdx.x;
@@ -479,12 +480,17 @@ __global__ __launch_bounds__(kCountThreads) void group_bulk_pack_kernel(const De
         partial[(uint64_t)blockIdx.x * kGroupBulkChunks + d] = h[d];
 }
 
+__device__ inline const Desc &desc_of(const Desc &D) { return D; }
+__device__ inline const Desc &desc_of(const WideDesc &W) { return W.D; }
+__device__ inline SubRef sub_of(const Desc &D, uint32_t t) { return sub_ref(D, t); }
+__device__ inline SubRef sub_of(const WideDesc &W, uint32_t t) { return sub_ref(W.D, t, selt(t, W.val)); }
+
 // One workgroup per table, one thread per local chunk.
-__global__ __launch_bounds__(kGroupBulkChunks) void group_bulk_scan_kernel(const BulkTable *__restrict__ tabs,
+template <typename Dir>
+__global__ __launch_bounds__(kGroupBulkChunks) void group_bulk_scan_kernel(const Dir *__restrict__ dir,
+                                                                           const BulkTable *__restrict__ tabs,
                                                                            uint32_t *__restrict__ partial,
-                                                                           uint32_t *__restrict__ cstart,
-                                                                           uint32_t *__restrict__ ccount,
-                                                                           uint32_t *__restrict__ ctable)
+                                                                           GroupChunk *__restrict__ chunks)
 {
     __shared__ uint32_t tmp[kScanWaves];
     const BulkTable T = tabs[blockIdx.x];
@@ -500,49 +506,44 @@ __global__ __launch_bounds__(kGroupBulkChunks) void group_bulk_scan_kernel(const
     const uint32_t base = (uint32_t)T.off + block_exclusive_scan(run, tmp, &all);  // the batch has < 2^32 entries
     for (uint32_t b = T.first_block; b < T.first_block + T.blocks; ++b) partial[(uint64_t)b * kGroupBulkChunks + d] += base;
     if (d < T.chunks) {
-        cstart[T.chunk0 + d] = base;
-        ccount[T.chunk0 + d] = run;
-        ctable[T.chunk0 + d] = blockIdx.x;
+        // local chunk d: subtable t, chunk lc of it (T.chunks sums the subtables' chunks)
+        const Dir E = dir[T.table];
+        const Desc &D = desc_of(E);
+        uint32_t lc = d, t = 0;
+        for (; t + 1 < D.d; ++t) {
+            const uint32_t k = sub_chunks(selt(t, D.nb));
+            if (lc < k) break;
+            lc -= k;
+        }
+        GroupChunk R;
+        R.S = sub_of(E, t);
+        R.ovf_off = T.off;
+        R.ovf_cap = T.n;
+        R.b0 = lc << kChunkShift;
+        R.len = min(kChunkBuckets, R.S.nb - R.b0);
+        R.lo = base;
+        R.cnt = run;
+        R.j = blockIdx.x;
+        R.table = T.table;
+        chunks[T.chunk0 + d] = R;
     }
 }
 
 template <bool DENSE, int THREADS>
-__global__ __launch_bounds__(THREADS) void group_bulk_fill_kernel(
-    const Desc *__restrict__ dir, const BulkTable *__restrict__ tabs, const uint32_t *__restrict__ cstart,
-    const uint32_t *__restrict__ ccount, const uint32_t *__restrict__ ctable, const uint64_t *__restrict__ ent,
-    uint64_t *__restrict__ ovf, unsigned long long *__restrict__ ovf_n, unsigned long long *__restrict__ table_counts,
-    Counters *__restrict__ C, bool zeroed)
+__global__ __launch_bounds__(THREADS) void group_bulk_fill_kernel(const GroupChunk *__restrict__ chunks,
+                                                                  const uint64_t *__restrict__ ent,
+                                                                  uint64_t *__restrict__ ovf,
+                                                                  unsigned long long *__restrict__ ovf_n,
+                                                                  unsigned long long *__restrict__ table_counts,
+                                                                  Counters *__restrict__ C, bool zeroed)
 {
-    const uint32_t c = blockIdx.x;
-    const uint32_t lo = cstart[c], cnt = ccount[c];
+    const GroupChunk R = chunks[blockIdx.x];
     unsigned long long placed = 0, rejected = 0;
-    uint32_t table = 0;
-    if (cnt) {  // block-uniform
-        const uint32_t j = ctable[c];
-        const BulkTable T = tabs[j];
-        table = T.table;
-        const Desc &D = dir[T.table];
-        // the subtable and first bucket of local chunk lc
-        uint32_t lc = c - T.chunk0, t = 0;
-        bool found = false;
-        for (uint32_t q = 0; q < D.d && !found; ++q) {
-            const uint32_t k = sub_chunks(selt(q, D.nb));
-            if (lc < k) {
-                t = q;
-                found = true;
-            } else {
-                lc -= k;
-            }
-        }
-        if (found) {
-            const uint32_t b0 = lc << kChunkShift;
-            const uint32_t len = min(kChunkBuckets, selt(t, D.nb) - b0);
-            fill_chunk<DENSE, THREADS>(D, t, b0, len, ent, lo, lo + cnt, ovf + T.off, T.n, ovf_n + j, zeroed, placed,
-                                       rejected);
-        }
-    }
+    if (R.cnt)  // block-uniform
+        fill_chunk<DENSE, THREADS>(R.S, R.b0, R.len, ent, R.lo, R.lo + R.cnt, ovf + R.ovf_off, R.ovf_cap, ovf_n + R.j,
+                                   zeroed, placed, rejected);
     const unsigned long long wp = wave_sum(placed);
-    if (__lane_id() == 0 && wp) atomicAdd(&table_counts[table], wp);
+    if (__lane_id() == 0 && wp) atomicAdd(&table_counts[R.table], wp);
     wave_tally(C, &C->kicks, placed);
     wave_tally(C, &C->rejected, rejected);
 }
@@ -591,12 +592,25 @@ hipError_t launch_group_bulk_pack(const Desc *dir, const GroupItem *blocks, uint
     return hipGetLastError();
 }
 
-hipError_t launch_group_bulk_scan(const BulkTable *tabs, uint32_t n_tabs, uint32_t *partial, uint32_t *cstart,
-                                  uint32_t *ccount, uint32_t *ctable, hipStream_t st)
+template <typename Dir>
+static hipError_t group_bulk_scan(const Dir *dir, const BulkTable *tabs, uint32_t n_tabs, uint32_t *partial,
+                                  GroupChunk *chunks, hipStream_t st)
 {
     if (n_tabs == 0) return hipSuccess;
-    group_bulk_scan_kernel<<<n_tabs, kGroupBulkChunks, 0, st>>>(tabs, partial, cstart, ccount, ctable);
+    group_bulk_scan_kernel<Dir><<<n_tabs, kGroupBulkChunks, 0, st>>>(dir, tabs, partial, chunks);
     return hipGetLastError();
+}
+
+hipError_t launch_group_bulk_scan(const Desc *dir, const BulkTable *tabs, uint32_t n_tabs, uint32_t *partial,
+                                  GroupChunk *chunks, hipStream_t st)
+{
+    return group_bulk_scan(dir, tabs, n_tabs, partial, chunks, st);
+}
+
+hipError_t launch_group_bulk_scan(const WideDesc *dir, const BulkTable *tabs, uint32_t n_tabs, uint32_t *partial,
+                                  GroupChunk *chunks, hipStream_t st)
+{
+    return group_bulk_scan(dir, tabs, n_tabs, partial, chunks, st);
 }
 
 hipError_t launch_group_bulk_scatter(const GroupItem *blocks, uint32_t n_blocks, const uint32_t *keys,
@@ -617,18 +631,16 @@ hipError_t launch_group_bulk_scatter(const GroupItem *blocks, uint32_t n_blocks,
     return hipGetLastError();
 }
 
-hipError_t launch_group_bulk_fill(const Desc *dir, const BulkTable *tabs, uint32_t n_chunks, const uint32_t *cstart,
-                                  const uint32_t *ccount, const uint32_t *ctable, const uint64_t *ent, uint64_t *ovf,
+hipError_t launch_group_bulk_fill(const GroupChunk *chunks, uint32_t n_chunks, const uint64_t *ent, uint64_t *ovf,
                                   unsigned long long *ovf_n, unsigned long long *table_counts, Counters *C, bool dense,
                                   bool zeroed, hipStream_t st)
 {
     if (n_chunks == 0) return hipSuccess;
     if (dense)
-        group_bulk_fill_kernel<true, 1024><<<n_chunks, 1024, 0, st>>>(dir, tabs, cstart, ccount, ctable, ent, ovf, ovf_n,
-                                                                     table_counts, C, zeroed);
+        group_bulk_fill_kernel<true, 1024>
+            <<<n_chunks, 1024, 0, st>>>(chunks, ent, ovf, ovf_n, table_counts, C, zeroed);
     else
-        group_bulk_fill_kernel<false, 256><<<n_chunks, 256, 0, st>>>(dir, tabs, cstart, ccount, ctable, ent, ovf, ovf_n,
-                                                                    table_counts, C, false);
+        group_bulk_fill_kernel<false, 256><<<n_chunks, 256, 0, st>>>(chunks, ent, ovf, ovf_n, table_counts, C, false);
     return hipGetLastError();
 }
 

@@ -220,18 +220,36 @@ __host__ __device__ inline uint32_t bucket_of(const Desc &D, uint32_t k, uint32_
     return fast_mod_sh(hash_t(k, t), selt(t, D.nb), selt(t, D.m), selt(t, D.sh));
 }
 
-// bucket_of for a subtable t that every lane of the wave shares: a scalar branch evaluates the
-// one hash that t uses instead of all four.
-__device__ inline uint32_t bucket_of_uniform(const Desc &D, uint32_t k, uint32_t t)
+// One subtable as code that works inside it alone sees it: its arrays, bucket count and
+// FastDiv constants, and its index (which selects the hash).
+struct SubRef {
+    uint64_t *keys;  // bucket array, 16 slots per bucket (key | value entries, or wide keys)
+    uint64_t *vals;  // value array parallel to it (wide tables), else null
+    uint32_t nb, m, sh, t;
+};
+
+__host__ __device__ inline SubRef sub_ref(const Desc &D, uint32_t t, uint64_t *vals = nullptr)
+{
+    return SubRef{selt(t, D.sub), vals, selt(t, D.nb), selt(t, D.m), selt(t, D.sh), t};
+}
+
+// bucket_of for a subtable that every lane of the wave shares: a scalar branch evaluates the
+// one hash that it uses instead of all four.
+__device__ inline uint32_t bucket_of_uniform(const SubRef &S, uint32_t k)
 {
     uint32_t x;
-    switch (t & 3u) {
+    switch (S.t & 3u) {
     case 0: x = h1(k) % kPrime; break;
     case 1: x = h2(k) % kPrime; break;
     case 2: x = h3(k); break;
     default: x = h4(k) % kPrime; break;
     }
-    return fast_mod_sh(x, selt(t, D.nb), selt(t, D.m), selt(t, D.sh));
+    return fast_mod_sh(x, S.nb, S.m, S.sh);
+}
+
+__device__ inline uint32_t bucket_of_uniform(const Desc &D, uint32_t k, uint32_t t)
+{
+    return bucket_of_uniform(sub_ref(D, t), k);
 }
 
 // Address of slot 0 of key k's bucket in subtable t.

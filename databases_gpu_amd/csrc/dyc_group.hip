@@ -53,7 +53,7 @@ struct dyc_group {
     uint64_t bk_cap = 0;
     uint32_t *bk_partial = nullptr;
     uint64_t bk_partial_cap = 0;
-    uint32_t *bk_cstart = nullptr, *bk_ccount = nullptr, *bk_ctable = nullptr;
+    GroupChunk *bk_chunks = nullptr;  // chunk records of a round
     uint64_t bk_chunk_cap = 0;
     unsigned long long *bk_ovfn = nullptr, *bk_hovfn = nullptr;
     uint32_t bk_tabs_cap = 0;
@@ -291,9 +291,7 @@ int reserve_group_bulk(dyc_group *g, uint64_t n, uint64_t blocks, uint64_t chunk
     }
     if (chunks > g->bk_chunk_cap) {
         g->bk_chunk_cap = 0;
-        regrow((void **)&g->bk_cstart, chunks * sizeof(uint32_t), "chunk starts");
-        regrow((void **)&g->bk_ccount, chunks * sizeof(uint32_t), "chunk counts");
-        regrow((void **)&g->bk_ctable, chunks * sizeof(uint32_t), "chunk tables");
+        regrow((void **)&g->bk_chunks, chunks * sizeof(GroupChunk), "chunk records");
         if (rc) return rc;
         g->bk_chunk_cap = chunks;
     }
@@ -336,22 +334,20 @@ hipError_t enqueue_group_bulk(dyc_group *g, const BulkArrays &A, const uint32_t 
     const bool dense2 = g->cfg.subtables == 2;
     hipError_t e;
     if ((e = launch_group_bulk_pack(g->ddir, blk1, A.nb, keys, nullptr, g->bk_lcid, g->bk_partial, s)) != hipSuccess ||
-        (e = launch_group_bulk_scan(tabs1, A.j, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s)) !=
-            hipSuccess ||
+        (e = launch_group_bulk_scan(g->ddir, tabs1, A.j, g->bk_partial, g->bk_chunks, s)) != hipSuccess ||
         (e = launch_group_bulk_scatter(blk1, A.nb, keys, vals, nullptr, g->bk_lcid, g->bk_partial, total, g->bk_ent,
                                        s)) != hipSuccess ||
         (e = launch_zero(ovf1, 2 * (uint64_t)A.j, s)) != hipSuccess ||
-        (e = launch_group_bulk_fill(g->ddir, tabs1, A.chunks, g->bk_cstart, g->bk_ccount, g->bk_ctable, g->bk_ent,
-                                    g->bk_ovf, ovf1, g->tcounts, g->ctr, true, zeroed, s)) != hipSuccess ||
+        (e = launch_group_bulk_fill(g->bk_chunks, A.chunks, g->bk_ent, g->bk_ovf, ovf1, g->tcounts, g->ctr, true,
+                                    zeroed, s)) != hipSuccess ||
         (e = launch_group_bulk_clip(blk1, A.btab, A.nb, tabs1, A.j, ovf1, blk2, tabs2, s)) != hipSuccess ||
         (e = launch_group_bulk_pack(g->ddir, blk2, A.nb, nullptr, g->bk_ovf, g->bk_lcid, g->bk_partial, s)) !=
             hipSuccess ||
-        (e = launch_group_bulk_scan(tabs2, A.j, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s)) !=
-            hipSuccess ||
+        (e = launch_group_bulk_scan(g->ddir, tabs2, A.j, g->bk_partial, g->bk_chunks, s)) != hipSuccess ||
         (e = launch_group_bulk_scatter(blk2, A.nb, nullptr, nullptr, g->bk_ovf, g->bk_lcid, g->bk_partial, total,
                                        g->bk_ent, s)) != hipSuccess ||
-        (e = launch_group_bulk_fill(g->ddir, tabs2, A.chunks, g->bk_cstart, g->bk_ccount, g->bk_ctable, g->bk_ent,
-                                    g->bk_ovf, ovf2, g->tcounts, g->ctr, dense2, false, s)) != hipSuccess ||
+        (e = launch_group_bulk_fill(g->bk_chunks, A.chunks, g->bk_ent, g->bk_ovf, ovf2, g->tcounts, g->ctr, dense2,
+                                    false, s)) != hipSuccess ||
         (e = launch_group_bulk_clip(blk1, A.btab, A.nb, tabs1, A.j, ovf2, items, nullptr, s)) != hipSuccess)
         return e;
     return launch_group_insert(g->ddir, items, A.nb, nullptr, nullptr, g->policy(), (int)g->launch.insert_lanes,
@@ -410,7 +406,7 @@ int group_bulk_insert(dyc_group *g, const uint32_t *keys, const uint32_t *vals, 
 // The group's shared bulk-fill buffers a plan's graph uses (any of them moving voids the plan).
 std::vector<const void *> bulk_workspace(const dyc_group *g)
 {
-    return {g->bk_ent, g->bk_ovf, g->bk_lcid, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, g->bk_ovfn};
+    return {g->bk_ent, g->bk_ovf, g->bk_lcid, g->bk_partial, g->bk_chunks, g->bk_ovfn};
 }
 
 }  // namespace
@@ -515,7 +511,7 @@ int dyc_group_destroy(dyc_group *g)
     if (g->hitems) (void)hipHostFree(g->hitems);
     if (g->items_free) (void)hipEventDestroy(g->items_free);
     for (void *p : {(void *)g->bk_ent, (void *)g->bk_ovf, (void *)g->bk_lcid, (void *)g->bk_partial,
-                    (void *)g->bk_cstart, (void *)g->bk_ccount, (void *)g->bk_ctable, (void *)g->bk_tabs,
+                    (void *)g->bk_chunks, (void *)g->bk_tabs,
                     (void *)g->bk_ovfn, (void *)g->bk_blk, (void *)g->bk_btab})
         if (p) (void)hipFree(p);
     for (void *p : {(void *)g->bk_htabs, (void *)g->bk_hovfn, (void *)g->bk_hblk, (void *)g->bk_hbtab})

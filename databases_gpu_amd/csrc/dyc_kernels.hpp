@@ -203,6 +203,24 @@ struct BulkTable {
     uint32_t chunk0, chunks;       // its chunks in the call's numbering
 };
 static_assert(sizeof(BulkTable) == 32, "BulkTable is uploaded as a host array");
+// A table of 64-bit keys and values: key lines laid out as a Desc, value lines parallel to them.
+struct WideDesc {
+    Desc D;                        // key arrays (D.sub), geometry and pair scheme
+    uint64_t *val[kMaxSubtables];  // value arrays, slot for slot parallel to the key arrays
+};
+// One chunk of a group fill round with all that its workgroup reads before the entries, so the
+// workgroup starts from a single load: a chain of chunk -> table -> directory loads, each
+// waiting behind the round's store traffic, kept the workgroups of a whole round idle for about
+// half their lifetime. Written by the scan.
+struct alignas(16) GroupChunk {
+    SubRef S;          // the chunk's subtable
+    uint64_t ovf_off;  // the table's overflow places, ovf[ovf_off, ovf_off + ovf_cap)
+    uint32_t ovf_cap;
+    uint32_t b0, len;  // buckets [b0, b0 + len) of the subtable
+    uint32_t lo, cnt;  // its entries, ent[lo, lo + cnt)
+    uint32_t j;        // the table's index in the call's BulkTable list (its overflow counter)
+    uint32_t table;    // the table's index in the group (its live count)
+};
 // Per partition block (blocks[b]: one table's range): every key's local chunk into lcid, and the
 // block's count of each local chunk into partial[b * 256 + lc]. Round 1 (entries null): keys,
 // chunk of the start bucket (reserved keys count as chunk 0, whose fill rejects them). Round 2:
@@ -210,20 +228,22 @@ static_assert(sizeof(BulkTable) == 32, "BulkTable is uploaded as a host array");
 hipError_t launch_group_bulk_pack(const Desc *dir, const GroupItem *blocks, uint32_t n_blocks, const uint32_t *keys,
                                   const uint64_t *entries, uint8_t *lcid, uint32_t *partial, hipStream_t st);
 // Per table: partial[b * 256 + lc] becomes the place of block b's first entry of chunk lc in the
-// partitioned batch, and chunk c = chunk0 + lc of the call gets cstart[c], ccount[c] and
-// ctable[c] (the table's index in tabs).
-hipError_t launch_group_bulk_scan(const BulkTable *tabs, uint32_t n_tabs, uint32_t *partial, uint32_t *cstart,
-                                  uint32_t *ccount, uint32_t *ctable, hipStream_t st);
+// partitioned batch, and chunk c = chunk0 + lc of the call gets its record chunks[c] (subtable
+// from dir[table], entries, overflow region of the table's BulkTable).
+hipError_t launch_group_bulk_scan(const Desc *dir, const BulkTable *tabs, uint32_t n_tabs, uint32_t *partial,
+                                  GroupChunk *chunks, hipStream_t st);
+hipError_t launch_group_bulk_scan(const WideDesc *dir, const BulkTable *tabs, uint32_t n_tabs, uint32_t *partial,
+                                  GroupChunk *chunks, hipStream_t st);
 // The pass itself: keys/vals (or packed entries, when not null) of every block to ent[0, n) by
 // local chunk (no stable order).
 hipError_t launch_group_bulk_scatter(const GroupItem *blocks, uint32_t n_blocks, const uint32_t *keys,
                                      const uint32_t *vals, const uint64_t *entries, const uint8_t *lcid,
                                      const uint32_t *partial, uint64_t n, uint64_t *ent, hipStream_t st);
-// One workgroup per chunk of the call (dense: the chunk's image in LDS; sparse: stores into the
-// free slots of the buckets that received entries). Entries that do not fit go to the table's
-// region of ovf, ovf[off, off + n) of its BulkTable, counted in ovf_n[j] (j its index in tabs;
-// zeroed by the caller). Placements are added to table_counts and to the kick tally, reserved
-// keys to the rejected tally. zeroed (dense only): every slot of every table is zero.
+// One workgroup per chunk record of the call (dense: the chunk's image in LDS; sparse: stores
+// into the free slots of the buckets that received entries). Entries that do not fit go to the
+// table's region of ovf, counted in ovf_n[j] (zeroed by the caller). Placements are added to
+// table_counts and to the kick tally, reserved keys to the rejected tally. zeroed (dense only):
+// every slot of every table is zero.
 // The next round's work from this round's overflow counts, on the device (no read-back): block b
 // of blocks (table tabs[btab[b]]) clipped to its table's overflow, out[b] = its part of
 // [off, off + ovf_n[j]); tabs_out (optional) = tabs with n = ovf_n[j]. Blocks past the overflow
@@ -231,8 +251,7 @@ hipError_t launch_group_bulk_scatter(const GroupItem *blocks, uint32_t n_blocks,
 hipError_t launch_group_bulk_clip(const GroupItem *blocks, const uint32_t *btab, uint32_t n_blocks,
                                   const BulkTable *tabs, uint32_t n_tabs, const unsigned long long *ovf_n,
                                   GroupItem *out, BulkTable *tabs_out, hipStream_t st);
-hipError_t launch_group_bulk_fill(const Desc *dir, const BulkTable *tabs, uint32_t n_chunks, const uint32_t *cstart,
-                                  const uint32_t *ccount, const uint32_t *ctable, const uint64_t *ent, uint64_t *ovf,
+hipError_t launch_group_bulk_fill(const GroupChunk *chunks, uint32_t n_chunks, const uint64_t *ent, uint64_t *ovf,
                                   unsigned long long *ovf_n, unsigned long long *table_counts, Counters *C, bool dense,
                                   bool zeroed, hipStream_t st);
 

@@ -41,11 +41,6 @@ constexpr uint64_t kWideBusy = ~0ull;
 __host__ __device__ inline uint32_t fold64(uint64_t k) { return (uint32_t)k ^ h5((uint32_t)(k >> 32)); }
 __host__ __device__ inline bool wide_reserved(uint64_t k) { return k == 0ull || k == kWideBusy; }
 
-struct WideDesc {
-    Desc D;                        // key arrays (D.sub), geometry and pair scheme
-    uint64_t *val[kMaxSubtables];  // value arrays, slot for slot parallel to the key arrays
-};
-
 // Where an insert parks entries that exhaust the eviction bound (append index Counters::pending).
 // Groups: tags (optional) receives the table index of each parked entry (`tag` for a
 // single-table launch, the item's table for a group launch), and rejected_at (optional) counts
@@ -627,17 +622,16 @@ __global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc 
 }
 
 // The fill of one chunk of a wide table by the whole workgroup: buckets [b0, b0 + len) of
-// subtable t, entries ent[lo, hi) (all targeting subtable t); entries that do not fit go to
+// subtable S (key lines S.keys, value lines S.vals), entries ent[lo, hi) (all targeting S); entries that do not fit go to
 // ovf[0, ovf_cap) at the index *ovf_n hands out. Placements and rejected reserved keys are added
 // to this lane's `placed` and `rejected`. Every thread of the workgroup calls it alike.
 template <bool DENSE, int THREADS>
-__device__ inline void wide_fill_chunk(const WideDesc &W, uint32_t t, uint32_t b0, uint32_t len,
+__device__ inline void wide_fill_chunk(const SubRef &S, uint32_t b0, uint32_t len,
                                        const WideEntry *__restrict__ ent, uint32_t lo, uint32_t hi,
                                        WideEntry *__restrict__ ovf, uint64_t ovf_cap,
                                        unsigned long long *__restrict__ ovf_n, bool zeroed,
                                        unsigned long long &placed, unsigned long long &rejected)
 {
-    const Desc &D = W.D;
     constexpr uint32_t kImg = DENSE ? kChunkBuckets * kSlots : 2;
     __shared__ uint64_t kimg[kImg], vimg[kImg];  // DENSE: the chunk's key and value lines
     __shared__ uint32_t fm[kChunkBuckets];       // free-slot mask of each bucket
@@ -645,8 +639,8 @@ __device__ inline void wide_fill_chunk(const WideDesc &W, uint32_t t, uint32_t b
     __shared__ uint32_t wred[THREADS / kWave];
     __shared__ unsigned long long obase;
     __shared__ uint32_t onext;
-    uint64_t *const ksub = selt(t, D.sub);
-    uint64_t *const vsub = selt(t, W.val);
+    uint64_t *const ksub = S.keys;
+    uint64_t *const vsub = S.vals;
     const uint4 *const ksrc = reinterpret_cast<const uint4 *>(ksub + (uint64_t)b0 * kSlots);
     constexpr uint32_t kParts = kChunkBuckets * kSlots / 2;  // 16-byte parts of a chunk's lines
     if constexpr (DENSE) {
@@ -711,7 +705,7 @@ __device__ inline void wide_fill_chunk(const WideDesc &W, uint32_t t, uint32_t b
         for (uint32_t u = 0; u < kBatch; ++u) {
             uint32_t at = kReject;  // also past the range (never used)
             if (lo + threadIdx.x + u * THREADS < hi && !wide_reserved(hb[u].k)) {
-                const uint32_t b = bucket_of_uniform(D, fold64(hb[u].k), t);
+                const uint32_t b = bucket_of_uniform(S, fold64(hb[u].k));
                 if (b >= b0 && b - b0 < len) {
                     at = (atomicAdd(&cnt[b - b0], 1u) << 9) | (b - b0);
                 } else {
@@ -732,7 +726,7 @@ __device__ inline void wide_fill_chunk(const WideDesc &W, uint32_t t, uint32_t b
 #pragma unroll
             for (uint32_t u = 0; u < kBatch; ++u) {
                 if (i0 + u * THREADS >= hi || wide_reserved(kb[u])) continue;
-                const uint32_t b = bucket_of_uniform(D, fold64(kb[u]), t);
+                const uint32_t b = bucket_of_uniform(S, fold64(kb[u]));
                 if (b >= b0 && b - b0 < len) atomicAdd(&cnt[b - b0], 1u);
                 else ++stray;
             }
@@ -830,7 +824,7 @@ __device__ inline void wide_fill_chunk(const WideDesc &W, uint32_t t, uint32_t b
                 ++rejected;
                 continue;
             }
-            const uint32_t b = bucket_of_uniform(D, fold64(e.k), t);
+            const uint32_t b = bucket_of_uniform(S, fold64(e.k));
             bool fits = false;
             if (b >= b0 && b - b0 < len) {
                 const uint32_t j = b - b0;
@@ -878,7 +872,8 @@ __global__ __launch_bounds__(THREADS) void wide_bulk_fill_kernel(WideDesc W, Chu
             if (Ch.base[q] <= c) t = q;
         const uint32_t b0 = (c - chunk_base(Ch, t)) << kChunkShift;
         const uint32_t len = min(kChunkBuckets, selt(t, W.D.nb) - b0);
-        wide_fill_chunk<DENSE, THREADS>(W, t, b0, len, ent, lo, hi, ovf, ovf_cap, ovf_n, zeroed, placed, rejected);
+        wide_fill_chunk<DENSE, THREADS>(sub_ref(W.D, t, selt(t, W.val)), b0, len, ent, lo, hi, ovf, ovf_cap, ovf_n,
+                                        zeroed, placed, rejected);
     }
     wave_tally(C, &C->kicks, placed);
     wave_tally(C, &C->rejected, rejected);
@@ -913,7 +908,7 @@ __global__ __launch_bounds__(kCountThreads) void wide_group_bulk_pack_kernel(con
     __shared__ uint32_t h[kGroupBulkChunks];
     __shared__ uint4 sub_tab[kMaxSubtables];  // per subtable: buckets, divider, the table's first local chunk of it
     const GroupItem item = blocks[blockIdx.x];
-    const Desc &D = dir[item.table].D;
+    const Desc D = dir[item.table].D;  // by value: loaded once, not per use through memory
     for (uint32_t d = threadIdx.x; d < kGroupBulkChunks; d += kCountThreads) h[d] = 0;
     if (threadIdx.x < kMaxSubtables) {
         const uint32_t t = threadIdx.x;
@@ -953,41 +948,20 @@ __global__ __launch_bounds__(kCountThreads) void wide_group_bulk_pack_kernel(con
 }
 
 template <bool DENSE, int THREADS>
-__global__ __launch_bounds__(THREADS) void wide_group_bulk_fill_kernel(
-    const WideDesc *__restrict__ dir, const BulkTable *__restrict__ tabs, const uint32_t *__restrict__ cstart,
-    const uint32_t *__restrict__ ccount, const uint32_t *__restrict__ ctable, const WideEntry *__restrict__ ent,
-    WideEntry *__restrict__ ovf, unsigned long long *__restrict__ ovf_n, unsigned long long *__restrict__ table_counts,
-    Counters *__restrict__ C, bool zeroed)
+__global__ __launch_bounds__(THREADS) void wide_group_bulk_fill_kernel(const GroupChunk *__restrict__ chunks,
+                                                                       const WideEntry *__restrict__ ent,
+                                                                       WideEntry *__restrict__ ovf,
+                                                                       unsigned long long *__restrict__ ovf_n,
+                                                                       unsigned long long *__restrict__ table_counts,
+                                                                       Counters *__restrict__ C, bool zeroed)
 {
-    const uint32_t c = blockIdx.x;
-    const uint32_t lo = cstart[c], cnt = ccount[c];
+    const GroupChunk R = chunks[blockIdx.x];
     unsigned long long placed = 0, rejected = 0;
-    uint32_t table = 0;
-    if (cnt) {  // block-uniform
-        const uint32_t j = ctable[c];
-        const BulkTable T = tabs[j];
-        table = T.table;
-        const WideDesc &W = dir[T.table];
-        uint32_t lc = c - T.chunk0, t = 0;
-        bool found = false;
-        for (uint32_t q = 0; q < W.D.d && !found; ++q) {
-            const uint32_t k = (selt(q, W.D.nb) + kChunkBuckets - 1) >> kChunkShift;
-            if (lc < k) {
-                t = q;
-                found = true;
-            } else {
-                lc -= k;
-            }
-        }
-        if (found) {
-            const uint32_t b0 = lc << kChunkShift;
-            const uint32_t len = min(kChunkBuckets, selt(t, W.D.nb) - b0);
-            wide_fill_chunk<DENSE, THREADS>(W, t, b0, len, ent, lo, lo + cnt, ovf + T.off, T.n, ovf_n + j, zeroed,
-                                            placed, rejected);
-        }
-    }
+    if (R.cnt)  // block-uniform
+        wide_fill_chunk<DENSE, THREADS>(R.S, R.b0, R.len, ent, R.lo, R.lo + R.cnt, ovf + R.ovf_off, R.ovf_cap,
+                                        ovf_n + R.j, zeroed, placed, rejected);
     const unsigned long long wp = wave_sum(placed);
-    if (__lane_id() == 0 && wp) atomicAdd(&table_counts[table], wp);
+    if (__lane_id() == 0 && wp) atomicAdd(&table_counts[R.table], wp);
     wave_tally(C, &C->kicks, placed);
     wave_tally(C, &C->rejected, rejected);
 }
@@ -2010,7 +1984,7 @@ struct dyc_wide_group {
     uint64_t bk_cap = 0;
     uint32_t *bk_partial = nullptr;
     uint64_t bk_partial_cap = 0;
-    uint32_t *bk_cstart = nullptr, *bk_ccount = nullptr, *bk_ctable = nullptr;
+    GroupChunk *bk_chunks = nullptr;  // chunk records of a round
     uint64_t bk_chunk_cap = 0;
     unsigned long long *bk_ovfn = nullptr, *bk_hovfn = nullptr;
     GroupItem *bk_blk = nullptr, *bk_hblk = nullptr;
@@ -2203,9 +2177,7 @@ int wg_reserve_bulk(dyc_wide_group *g, const dyc::host::BulkLayout &L)
     }
     if (chunks > g->bk_chunk_cap) {
         g->bk_chunk_cap = 0;
-        regrow((void **)&g->bk_cstart, chunks * sizeof(uint32_t), "chunk starts");
-        regrow((void **)&g->bk_ccount, chunks * sizeof(uint32_t), "chunk counts");
-        regrow((void **)&g->bk_ctable, chunks * sizeof(uint32_t), "chunk tables");
+        regrow((void **)&g->bk_chunks, chunks * sizeof(GroupChunk), "chunk records");
         if (rc) return rc;
         g->bk_chunk_cap = chunks;
     }
@@ -2237,15 +2209,13 @@ hipError_t wg_enqueue_bulk(dyc_wide_group *g, uint32_t NB, uint32_t J, uint32_t 
     wide_group_bulk_pack_kernel<false><<<NB, kCountThreads, 0, s>>>(g->ddir, blk1, keys, nullptr, g->bk_lcid,
                                                                     g->bk_partial);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = launch_group_bulk_scan(tabs1, J, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s)) !=
-        hipSuccess)
+    if ((e = launch_group_bulk_scan(g->ddir, tabs1, J, g->bk_partial, g->bk_chunks, s)) != hipSuccess)
         return e;
     part_scatter_kernel<WideEntry, WideKeyValSource, uint8_t, kWideSortThreads, -1, SegLayout>
         <<<NB, kWideSortThreads, 0, s>>>(WideKeyValSource{keys, vals}, g->bk_lcid, lay1, RadixPass{0, 8}, g->bk_ent,
                                          no_cid);
     if ((e = hipGetLastError()) != hipSuccess || (e = launch_zero(ovf1, 2 * (uint64_t)J, s)) != hipSuccess) return e;
-    wide_group_bulk_fill_kernel<true, 1024><<<chunks, 1024, 0, s>>>(g->ddir, tabs1, g->bk_cstart, g->bk_ccount,
-                                                                   g->bk_ctable, g->bk_ent, g->bk_ovf, ovf1,
+    wide_group_bulk_fill_kernel<true, 1024><<<chunks, 1024, 0, s>>>(g->bk_chunks, g->bk_ent, g->bk_ovf, ovf1,
                                                                    g->tcounts, g->ctr, zeroed);
     if ((e = hipGetLastError()) != hipSuccess ||
         (e = launch_group_bulk_clip(blk1, g->bk_btab, NB, tabs1, J, ovf1, blk2, tabs2, s)) != hipSuccess)
@@ -2253,21 +2223,18 @@ hipError_t wg_enqueue_bulk(dyc_wide_group *g, uint32_t NB, uint32_t J, uint32_t 
     wide_group_bulk_pack_kernel<true><<<NB, kCountThreads, 0, s>>>(g->ddir, blk2, nullptr, g->bk_ovf, g->bk_lcid,
                                                                    g->bk_partial);
     if ((e = hipGetLastError()) != hipSuccess ||
-        (e = launch_group_bulk_scan(tabs2, J, g->bk_partial, g->bk_cstart, g->bk_ccount, g->bk_ctable, s)) !=
-            hipSuccess)
+        (e = launch_group_bulk_scan(g->ddir, tabs2, J, g->bk_partial, g->bk_chunks, s)) != hipSuccess)
         return e;
     part_scatter_kernel<WideEntry, EntrySource<WideEntry>, uint8_t, kWideSortThreads, -1, SegLayout>
         <<<NB, kWideSortThreads, 0, s>>>(EntrySource<WideEntry>{g->bk_ovf}, g->bk_lcid, lay2, RadixPass{0, 8},
                                          g->bk_ent, no_cid);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (dense2)
-        wide_group_bulk_fill_kernel<true, 1024><<<chunks, 1024, 0, s>>>(g->ddir, tabs2, g->bk_cstart, g->bk_ccount,
-                                                                       g->bk_ctable, g->bk_ent, g->bk_ovf, ovf2,
-                                                                       g->tcounts, g->ctr, false);
+        wide_group_bulk_fill_kernel<true, 1024><<<chunks, 1024, 0, s>>>(g->bk_chunks, g->bk_ent, g->bk_ovf, ovf2,
+                                                                   g->tcounts, g->ctr, false);
     else
-        wide_group_bulk_fill_kernel<false, 256><<<chunks, 256, 0, s>>>(g->ddir, tabs2, g->bk_cstart, g->bk_ccount,
-                                                                      g->bk_ctable, g->bk_ent, g->bk_ovf, ovf2,
-                                                                      g->tcounts, g->ctr, false);
+        wide_group_bulk_fill_kernel<false, 256><<<chunks, 256, 0, s>>>(g->bk_chunks, g->bk_ent, g->bk_ovf, ovf2,
+                                                                   g->tcounts, g->ctr, false);
     if ((e = hipGetLastError()) != hipSuccess ||
         (e = launch_group_bulk_clip(blk1, g->bk_btab, NB, tabs1, J, ovf2, items, nullptr, s)) != hipSuccess)
         return e;
@@ -2339,7 +2306,7 @@ int dyc_wide_group_destroy(dyc_wide_group *g)
     if (g->hitems) (void)hipHostFree(g->hitems);
     if (g->items_free) (void)hipEventDestroy(g->items_free);
     for (void *p : {(void *)g->bk_ent, (void *)g->bk_ovf, (void *)g->bk_lcid, (void *)g->bk_partial,
-                    (void *)g->bk_cstart, (void *)g->bk_ccount, (void *)g->bk_ctable, (void *)g->bk_tabs,
+                    (void *)g->bk_chunks, (void *)g->bk_tabs,
                     (void *)g->bk_ovfn, (void *)g->bk_blk, (void *)g->bk_btab})
         if (p) (void)hipFree(p);
     for (void *p : {(void *)g->bk_htabs, (void *)g->bk_hovfn, (void *)g->bk_hblk, (void *)g->bk_hbtab})
