@@ -110,7 +110,7 @@ def parse():
                     help="batched: the group insert (3: bulk fill per table chunk, then the eviction insert for "
                          "what overflows; 0: the eviction insert for every key; -1: 3 from 2^22 keys in the batch, "
                          "below that 0, whose single launch costs less than the fill's sequence). With 3 the "
-                         "eviction insert is measured beside it; the graph form always captures the eviction insert")
+                         "eviction insert is measured beside it (its launch shapes tuned for it); the graph form captures the same insert")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--kick-bound", type=int, default=None,
                     help="bulk: eviction bound of the table (with --no-tune it stays fixed; default: the library's 15, "
@@ -904,12 +904,13 @@ def run_batched(args, dev) -> int:
     filled = g.table_stats(0)["last_filled"]
     ok = bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
     evict = None
-    if bulk:  # the same launch shapes with the eviction insert for every key
+    if bulk:  # the eviction insert for every key, with launch shapes tuned for it
         if args.bt_wide:
             g0 = D.DyCuckooWideGroup(hints, insert_order=0)
         else:
             g0 = D.DyCuckooGroup(hints, insert_order=0)
-            g0.set_launch(**g.launch)
+            if tuned is not None:
+                autotune.tune_group(g0, keys, vals, offs, lookups, offs, log=None)
         ins_e, fnd_e, tot_e, _ = measure(False, g0)
         ok &= bool((out_h.bool() == ~absent).all()) and bool((out_v[~absent] == vals[~absent]).all())
         g0.close()
