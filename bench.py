@@ -1052,7 +1052,8 @@ def main() -> int:
     if not args.no_tune:
         from databases_gpu_amd.autotune import tune_table
 
-        tuned = tune_table(t, keys, vals, look_u, log=log, kick_bounds=(15,) if args.reference_kick else (15, 0))
+        tuned = tune_table(t, keys, vals, (look_u, look_z), log=log,
+                           kick_bounds=(15,) if args.reference_kick else (15, 0))
         log(f"autotune installed {tuned['installed']} ({tuned['seconds']} s)")
     cfg = t.config()
 

@@ -170,16 +170,20 @@ TABLE_FIND_SPACE = {
 }
 
 
-def tune_table(table, keys: torch.Tensor, vals: torch.Tensor, lookups: torch.Tensor, reps: int = 2,
+def tune_table(table, keys: torch.Tensor, vals: torch.Tensor, lookups, reps: int = 2,
                use_cache: bool = True, log=None, kick_bounds=(15,)) -> dict:
     """Pick the fastest insert and find shapes of one table for this batch and install them.
 
-    `kick_bounds` lists the eviction bounds to try; the default keeps the reference's bound
-    (15 iterations of its victim rule), so only the launch shape changes. The table is cleared
-    before every insert trial and left holding the batch afterwards."""
+    `lookups` is one tensor of lookups or a sequence of them (say uniform and skewed lookups):
+    the find shape chosen is the one with the least total time over all of them, since one
+    shape serves every find of the table. `kick_bounds` lists the eviction bounds to try; the
+    default keeps the reference's bound (15 iterations of its victim rule), so only the launch
+    shape changes. The table is cleared before every insert trial and left holding the batch
+    afterwards."""
+    sets = [lookups] if isinstance(lookups, torch.Tensor) else list(lookups)
     space = dict(TABLE_INSERT_SPACE, kick_bound=tuple(kick_bounds))
     key = (f"{torch.cuda.get_device_name(table.device)}|table|n{keys.numel()}|cap{table.capacity}"
-           f"|kb{'-'.join(map(str, space['kick_bound']))}")
+           f"|kb{'-'.join(map(str, space['kick_bound']))}|f{'-'.join(str(q.numel()) for q in sets)}")
     cache = _load_cache() if use_cache else {}
     t0 = time.perf_counter()
     if key in cache:
@@ -205,13 +209,14 @@ def tune_table(table, keys: torch.Tensor, vals: torch.Tensor, lookups: torch.Ten
         table.configure(**ins_best[1])
         table.clear()
         table.insert(keys, vals)
-        out_v = torch.empty(lookups.numel(), dtype=torch.int32, device=lookups.device)
-        out_h = torch.empty(lookups.numel(), dtype=torch.uint8, device=lookups.device)
+        m = max(q.numel() for q in sets)
+        out_v = torch.empty(m, dtype=torch.int32, device=sets[0].device)
+        out_h = torch.empty(m, dtype=torch.uint8, device=sets[0].device)
         find_best = (float("inf"), None)
         for combo in itertools.product(*TABLE_FIND_SPACE.values()):
             shape = dict(zip(TABLE_FIND_SPACE.keys(), combo))
             table.configure(**shape)
-            ms = _time(lambda: table.find(lookups, out_v, out_h), reps)
+            ms = sum(_time(lambda q=q: table.find(q, out_v[:q.numel()], out_h[:q.numel()]), reps) for q in sets)
             trials += 1
             if log:
                 log(f"autotune find {shape}: {ms:.3f} ms")
