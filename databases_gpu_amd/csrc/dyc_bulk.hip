@@ -33,9 +33,10 @@ namespace dyc {
 namespace {
 
 // Chunk ids of the batch in input order, and each block's counts of digit P (the partition's
-// first pass). Round 1 (IN_ENTRIES false): keys, target = the start table; reserved keys go to
-// chunk 0 and are rejected by its fill. Round 2 (IN_ENTRIES true): overflow entries, target = the
-// other table of the key's pair.
+// first pass). Round 1 (IN_ENTRIES false): keys, target = fill_target (pos1, the bucket a find
+// reads first, for as many keys as equal subtable shares allow); reserved keys go to chunk 0 and
+// are rejected by its fill. Round 2 (IN_ENTRIES true): overflow entries, target = the other
+// bucket of the pair.
 template <bool IN_ENTRIES, typename CidT>
 __global__ __launch_bounds__(kCountThreads) void bulk_pack_kernel(Desc D, Chunks Ch, const uint32_t *__restrict__ keys,
                                                                   const uint64_t *__restrict__ in, PartGrid G,
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(kCountThreads) void bulk_pack_kernel(Desc D, Chunks
             uint32_t c = 0;
             if (IN_ENTRIES || !reserved_key(k[u])) {
                 const uint32_t p = pair_of(D, k[u]);
-                uint32_t t = (k[u] & 1u) ? pos2(D, p) : pos1(D, p);
+                uint32_t t = fill_target(D, k[u], p);
                 if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
                 const uint4 q = sub_tab[t];
                 c = q.w + (fast_mod_sh(hash_t(k[u], t), q.x, q.y, q.z) >> kChunkShift);
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(kCountThreads) void group_bulk_pack_kernel(const De
             uint32_t lc = 0;
             if (IN_ENTRIES || !reserved_key(k[u])) {
                 const uint32_t p = pair_of(D, k[u]);
-                uint32_t t = (k[u] & 1u) ? pos2(D, p) : pos1(D, p);
+                uint32_t t = fill_target(D, k[u], p);
                 if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
                 const uint4 q = sub_tab[t];
                 lc = q.w + (fast_mod_sh(hash_t(k[u], t), q.x, q.y, q.z) >> kChunkShift);

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
+
 namespace dyc {
 
 constexpr int kDefaultSubtables = 4;           // dy_hash.h:32 TABLE_NUM
@@ -165,6 +167,8 @@ struct Desc {
     uint32_t np;                   // pairs: d (d - 1) / 2, and 2 for d = 2
     uint32_t pm, psh;              // FastDiv of np (multiplier, s1 | s2 << 8)
     uint64_t p1, p2;               // pos1 / pos2 of pair p in bits [4p, 4p + 4)
+    uint32_t fr;                   // bulk fill, first round: bits [2p, 2p + 2) say where pair p's
+                                   // keys go (fill_target); 0 (the start rule) if never set
 };
 
 // Values are read at constant indices first and then selected: a conditional over the array
@@ -327,6 +331,77 @@ inline bool pair_scheme(uint32_t d, uint32_t &np, uint64_t &p1, uint64_t &p2)
     return true;
 }
 
+// The bucket the bulk fill's first round sends key k of pair p to. The eviction insert starts a
+// key by the reference's start rule (odd keys in pos2, even ones in pos1, cuckoo.cu:336-341),
+// which gives every subtable the same share of the keys but puts only half of them in pos1, the
+// bucket a find reads first. Where the fill puts a key is its own choice: per pair, Desc::fr
+// picks 1 = always pos1, 2 = always pos2, 0 = the start rule, so that the subtables' shares stay
+// equal and the most keys go to pos1 (fill_rule).
+__host__ __device__ inline uint32_t fill_target(const Desc &D, uint32_t k, uint32_t p)
+{
+    const uint32_t r = (D.fr >> (2u * p)) & 3u;
+    return (r == 2u || (r == 0u && (k & 1u))) ? pos2(D, p) : pos1(D, p);
+}
+
+// Depth-first search for Desc::fr over the np pairs (3^np choices, pruned): each pair's keys go
+// all to pos1, half and half, or all to pos2, no subtable may exceed an equal share (2 np / d
+// halves of a pair's keys; at the end every share is then equal), and the most halves go to
+// pos1. The shares in pos1 for d = 2..6: 1, 2/3, 2/3, 0.7, 0.7 (the start rule: 1/2).
+struct FillRuleSearch {
+    uint32_t np = 0, share = 0;
+    uint32_t a[16] = {}, b[16] = {};  // pos1, pos2 of each pair
+    uint32_t load[kMaxSubtables] = {};
+    uint32_t cur = 0, best = 0, best_score = 0;
+    bool found = false;
+    void go(uint32_t i, uint32_t score)
+    {
+        if (found && score + 2 * (np - i) <= best_score) return;
+        if (i == np) {
+            found = true;
+            best = cur;
+            best_score = score;
+            return;
+        }
+        static const uint32_t kHalves[3] = {2, 1, 0}, kCode[3] = {1, 0, 2};  // pos1 first
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t x = kHalves[c];
+            load[a[i]] += x;
+            load[b[i]] += 2 - x;
+            if (load[a[i]] <= share && load[b[i]] <= share) {
+                cur = (cur & ~(3u << (2 * i))) | (kCode[c] << (2 * i));
+                go(i + 1, score + x);
+            }
+            load[a[i]] -= x;
+            load[b[i]] -= 2 - x;
+        }
+    }
+};
+
+inline uint32_t fill_rule_search(uint32_t d)
+{
+    FillRuleSearch S;
+    uint64_t p1 = 0, p2 = 0;
+    if (!pair_scheme(d, S.np, p1, p2) || S.np > 15 || (2 * S.np) % d) return 0;
+    S.share = 2 * S.np / d;
+    for (uint32_t p = 0; p < S.np; ++p) {
+        S.a[p] = (uint32_t)(p1 >> (4 * p)) & 15u;
+        S.b[p] = (uint32_t)(p2 >> (4 * p)) & 15u;
+    }
+    S.go(0, 0);
+    return S.found ? S.best : 0u;
+}
+
+// Desc::fr for d subtables (searched once per d).
+inline uint32_t fill_rule(uint32_t d)
+{
+    static const std::array<uint32_t, kMaxSubtables + 1> rules = [] {
+        std::array<uint32_t, kMaxSubtables + 1> r{};
+        for (uint32_t x = (uint32_t)kMinSubtables; x <= (uint32_t)kMaxSubtables; ++x) r[x] = fill_rule_search(x);
+        return r;
+    }();
+    return d <= (uint32_t)kMaxSubtables ? rules[d] : 0u;
+}
+
 // An empty descriptor for d subtables (false if d is unsupported).
 inline bool desc_init(Desc &D, uint32_t d)
 {
@@ -338,6 +413,7 @@ inline bool desc_init(Desc &D, uint32_t d)
     D.np = np;
     D.pm = f.m;
     D.psh = pack_shift(f);
+    D.fr = fill_rule(d);
     return true;
 }
 

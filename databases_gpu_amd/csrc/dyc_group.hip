@@ -329,9 +329,6 @@ hipError_t enqueue_group_bulk(dyc_group *g, const BulkArrays &A, const uint32_t 
     GroupItem *const blk1 = A.blk, *const blk2 = A.blk + A.nb, *const items = A.blk + 2 * A.nb;
     BulkTable *const tabs1 = A.tabs, *const tabs2 = A.tabs + A.j;
     unsigned long long *const ovf1 = g->bk_ovfn, *const ovf2 = g->bk_ovfn + A.j;
-    // round 2 is dense where half the keys overflow round 1: with two subtables every key starts
-    // in subtable 0 (pair k & 1, diff_hash_num_pos_fun.cpp:14-31)
-    const bool dense2 = g->cfg.subtables == 2;
     hipError_t e;
     if ((e = launch_group_bulk_pack(g->ddir, blk1, A.nb, keys, nullptr, g->bk_lcid, g->bk_partial, s)) != hipSuccess ||
         (e = launch_group_bulk_scan(g->ddir, tabs1, A.j, g->bk_partial, g->bk_chunks, s)) != hipSuccess ||
@@ -346,7 +343,7 @@ hipError_t enqueue_group_bulk(dyc_group *g, const BulkArrays &A, const uint32_t 
         (e = launch_group_bulk_scan(g->ddir, tabs2, A.j, g->bk_partial, g->bk_chunks, s)) != hipSuccess ||
         (e = launch_group_bulk_scatter(blk2, A.nb, nullptr, nullptr, g->bk_ovf, g->bk_lcid, g->bk_partial, total,
                                        g->bk_ent, s)) != hipSuccess ||
-        (e = launch_group_bulk_fill(g->bk_chunks, A.chunks, g->bk_ent, g->bk_ovf, ovf2, g->tcounts, g->ctr, dense2,
+        (e = launch_group_bulk_fill(g->bk_chunks, A.chunks, g->bk_ent, g->bk_ovf, ovf2, g->tcounts, g->ctr, false,
                                     false, s)) != hipSuccess ||
         (e = launch_group_bulk_clip(blk1, A.btab, A.nb, tabs1, A.j, ovf2, items, nullptr, s)) != hipSuccess)
         return e;
@@ -364,7 +361,8 @@ int reserve_for(dyc_group *g, const BulkLayout &L, hipStream_t s)
 
 // insert_order 3 for a group: each table's batch is partitioned by its own chunks in one pass
 // (the batch is grouped by table already) and every chunk is filled by one workgroup; what
-// overflows its start bucket gets a second round for the other bucket of its pair, and what
+// overflows its first bucket (fill_target) gets a second round for the other bucket of its pair
+// (sparse: a few percent of the keys), and what
 // overflows that goes through the group's eviction insert. Declines (*n_tabs 0, nothing
 // launched) when bulk_layout does, or when the workspace cannot be allocated. Counters and
 // per-table counts are zeroed here and read by the caller, who also finds the overflow counts of

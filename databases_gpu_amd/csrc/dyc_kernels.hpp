@@ -144,9 +144,9 @@ size_t bulk_cid_bytes(const Chunks &Ch);
 // (8192-entry tiles, or 4096) and three per CU measured slower.
 constexpr uint32_t kBulkSortBlocks = 512;
 constexpr uint32_t kWideSortBlocks = 512;
-// Group n entries by chunk (n < 2^32): round 1 from keys/vals for the start table (keys !=
-// nullptr; reserved keys go to chunk 0), round 2 from the entries in ws.ent[in] for the other
-// table of each key's pair. The sorted entries end in ws.ent[*out] (the round-2 input buffer is
+// Group n entries by chunk (n < 2^32): round 1 from keys/vals for each key's fill_target bucket
+// (keys != nullptr; reserved keys go to chunk 0), round 2 from the entries in ws.ent[in] for the
+// other bucket of the pair. The sorted entries end in ws.ent[*out] (the round-2 input buffer is
 // reused), and ws.start receives the first index of every chunk.
 hipError_t launch_bulk_partition(const Desc &D, const Chunks &Ch, const uint32_t *keys, const uint32_t *vals, int in,
                                  uint64_t n, const BulkSpace &ws, uint32_t max_blocks, hipStream_t st, int *out);
@@ -223,8 +223,8 @@ struct alignas(16) GroupChunk {
 };
 // Per partition block (blocks[b]: one table's range): every key's local chunk into lcid, and the
 // block's count of each local chunk into partial[b * 256 + lc]. Round 1 (entries null): keys,
-// chunk of the start bucket (reserved keys count as chunk 0, whose fill rejects them). Round 2:
-// packed entries (a fill's overflow), chunk of the other bucket of the key's pair.
+// chunk of the fill_target bucket (reserved keys count as chunk 0, whose fill rejects them).
+// Round 2: packed entries (a fill's overflow), chunk of the other bucket of the pair.
 hipError_t launch_group_bulk_pack(const Desc *dir, const GroupItem *blocks, uint32_t n_blocks, const uint32_t *keys,
                                   const uint64_t *entries, uint8_t *lcid, uint32_t *partial, hipStream_t st);
 // Per table: partial[b * 256 + lc] becomes the place of block b's first entry of chunk lc in the

@@ -605,8 +605,9 @@ int bulk_round(dyc_table *t, const Chunks &Ch, const uint32_t *keys, const uint3
     return DYC_OK;
 }
 
-// insert_order 3: place keys by bucket fill in their start bucket, then the overflow in the
-// other bucket of their pair, then the rest by the eviction insert. Counters are zeroed here
+// insert_order 3: place keys by bucket fill in their fill_target bucket (pos1, the bucket a find
+// reads first, wherever equal subtable shares allow), then the overflow in the other bucket of
+// their pair, then the rest by the eviction insert. Counters are zeroed here
 // and read by the caller.
 int bulk_insert(dyc_table *t, const uint32_t *keys, const uint32_t *vals, uint64_t n, bool was_zeroed, hipStream_t s,
                 uint64_t *left)

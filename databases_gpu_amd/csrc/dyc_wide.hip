@@ -568,8 +568,8 @@ constexpr uint64_t kWideInsertBlocks = 16384;
 // eviction insert for the rest. The dense round keeps the chunk's key lines and value lines in
 // LDS (2 x 64 KB), so one 1024-thread workgroup runs per CU.
 // Chunk ids of the batch in input order and each block's counts of the partition's first digit
-// (dyc_part.hpp), over fold(k): round 1 from the keys (start table; reserved keys go to chunk 0,
-// whose fill rejects them), round 2 from overflow entries (the other table of the pair).
+// (dyc_part.hpp), over fold(k): round 1 from the keys (fill_target; reserved keys go to chunk 0,
+// whose fill rejects them), round 2 from overflow entries (the other bucket of the pair).
 template <bool IN_ENTRIES, typename CidT>
 __global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc W, Chunks Ch,
                                                                        const uint64_t *__restrict__ keys,
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc 
             uint32_t c = 0;
             if (IN_ENTRIES || !wide_reserved(k[u])) {
                 const uint32_t f = fold64(k[u]), p = pair_of(D, f);
-                uint32_t t = (k[u] & 1u) ? pos2(D, p) : pos1(D, p);
+                uint32_t t = fill_target(D, (uint32_t)k[u], p);
                 if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
                 const uint4 q = sub_tab[t];
                 c = q.w + (fast_mod_sh(hash_t(f, t), q.x, q.y, q.z) >> kChunkShift);
@@ -933,7 +933,7 @@ __global__ __launch_bounds__(kCountThreads) void wide_group_bulk_pack_kernel(con
             uint32_t lc = 0;
             if (IN_ENTRIES || !wide_reserved(k[u])) {
                 const uint32_t f = fold64(k[u]), p = pair_of(D, f);
-                uint32_t t = (k[u] & 1u) ? pos2(D, p) : pos1(D, p);
+                uint32_t t = fill_target(D, (uint32_t)k[u], p);
                 if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
                 const uint4 q = sub_tab[t];
                 lc = q.w + (fast_mod_sh(hash_t(f, t), q.x, q.y, q.z) >> kChunkShift);
@@ -1594,8 +1594,8 @@ int wbulk_round(dyc_wide *t, const Chunks &Ch, const uint64_t *keys, const uint6
     return DYC_OK;
 }
 
-// insert_order 3 for wide keys: fill the start bucket, then the other bucket, then the eviction
-// insert for the rest (the narrow table's bulk_insert over fold(k)).
+// insert_order 3 for wide keys: fill each key's fill_target bucket, then the other bucket, then
+// the eviction insert for the rest (the narrow table's bulk_insert over fold(k)).
 int wbulk_insert(dyc_wide *t, const uint64_t *keys, const uint64_t *vals, uint64_t n, bool was_zeroed, hipStream_t s,
                  uint64_t *left)
 {
@@ -2202,7 +2202,6 @@ hipError_t wg_enqueue_bulk(dyc_wide_group *g, uint32_t NB, uint32_t J, uint32_t 
     GroupItem *const blk1 = g->bk_blk, *const blk2 = g->bk_blk + NB, *const items = g->bk_blk + 2 * NB;
     BulkTable *const tabs1 = g->bk_tabs, *const tabs2 = g->bk_tabs + J;
     unsigned long long *const ovf1 = g->bk_ovfn, *const ovf2 = g->bk_ovfn + J;
-    const bool dense2 = g->cfg.subtables == 2;  // every key starts in subtable 0: half overflow round 1
     uint8_t *const no_cid = nullptr;
     const SegLayout lay1{blk1, g->bk_partial, total}, lay2{blk2, g->bk_partial, total};
     hipError_t e;
@@ -2229,11 +2228,7 @@ hipError_t wg_enqueue_bulk(dyc_wide_group *g, uint32_t NB, uint32_t J, uint32_t 
         <<<NB, kWideSortThreads, 0, s>>>(EntrySource<WideEntry>{g->bk_ovf}, g->bk_lcid, lay2, RadixPass{0, 8},
                                          g->bk_ent, no_cid);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (dense2)
-        wide_group_bulk_fill_kernel<true, 1024><<<chunks, 1024, 0, s>>>(g->bk_chunks, g->bk_ent, g->bk_ovf, ovf2,
-                                                                   g->tcounts, g->ctr, false);
-    else
-        wide_group_bulk_fill_kernel<false, 256><<<chunks, 256, 0, s>>>(g->bk_chunks, g->bk_ent, g->bk_ovf, ovf2,
+    wide_group_bulk_fill_kernel<false, 256><<<chunks, 256, 0, s>>>(g->bk_chunks, g->bk_ent, g->bk_ovf, ovf2,
                                                                    g->tcounts, g->ctr, false);
     if ((e = hipGetLastError()) != hipSuccess ||
         (e = launch_group_bulk_clip(blk1, g->bk_btab, NB, tabs1, J, ovf2, items, nullptr, s)) != hipSuccess)

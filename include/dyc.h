@@ -73,8 +73,10 @@ typedef struct dyc_config {
     uint32_t insert_order;     /* 0 or 1 = input order (default); 2 = order each batch by the bucket
                                   region of its keys' first bucket before inserting; 3 = bulk
                                   fill for batches of at least two keys per bucket: sort the batch
-                                  by 512-bucket chunk of its start bucket, fill free slots chunk by
-                                  chunk without atomics, the overflow likewise in its other bucket,
+                                  by 512-bucket chunk of a first bucket (pos1, the one a find
+                                  reads first, for as many keys as equal subtable shares allow),
+                                  fill free slots chunk by chunk without atomics, the overflow
+                                  likewise in its other bucket,
                                   then the eviction insert for the rest (workspace 24 B per key;
                                   wide tables 40 B). It reads two overflow counts back, so the
                                   call waits twice more on `stream` before its final wait */

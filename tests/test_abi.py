@@ -180,6 +180,50 @@ def test_nth_set_bit16_is_exact(tmp_path):
     assert "checked 524288 bad 0" in r.stdout
 
 
+def test_fill_rule_keeps_subtable_shares_equal_and_favours_pos1(oracle_lib, tmp_path):
+    """The bulk fill's first-round rule (Desc::fr, dyc_device.hpp, compiled for the host here):
+    per pair, all keys to pos1, all to pos2, or the start rule (odd keys pos2, cuckoo.cu:336-341).
+    Over the oracle's pair tables every subtable gets the same share of the keys, and the share
+    sent to pos1 equals the best an exhaustive search over all 3^np rules finds (d <= 5) or a
+    pruned search's (d = 6); the start rule alone sends half."""
+    import itertools
+
+    import oracle as O
+
+    exe = tmp_path / "fill_rule_check"
+    subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", CSRC, os.path.join(ROOT, "tests", "cpp", "fill_rule_check.cpp"),
+                    "-o", str(exe)], check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, check=True)
+    rules = {int(a): int(b) for a, b in (line.split() for line in r.stdout.split("\n") if line.strip())}
+    import fillrule
+
+    for d in range(2, 7):
+        assert rules[d] == fillrule.rule(d), d  # the tests' restatement picks the same rule
+        pairs = O.pair_table(d)
+        halves = {1: 2, 0: 1, 2: 0}  # code -> halves of the pair's keys sent to pos1
+        load = [0] * d
+        to_pos1 = 0
+        for p, (a, b) in enumerate(pairs):
+            x = halves[(rules[d] >> (2 * p)) & 3]
+            load[a] += x
+            load[b] += 2 - x
+            to_pos1 += x
+        assert len(set(load)) == 1, (d, load)  # equal shares
+        if d <= 5:  # exhaustive
+            best = 0
+            for xs in itertools.product((2, 1, 0), repeat=len(pairs)):
+                ld = [0] * d
+                for (a, b), x in zip(pairs, xs):
+                    ld[a] += x
+                    ld[b] += 2 - x
+                if len(set(ld)) == 1:
+                    best = max(best, sum(xs))
+            assert to_pos1 == best, (d, to_pos1, best)
+        else:
+            assert to_pos1 * 10 == 7 * 2 * len(pairs)  # 0.7 of the keys
+        assert to_pos1 > len(pairs)  # more than the start rule's half
+
+
 def test_device_header_hashes_match_the_oracle(oracle_lib, tmp_path):
     """The kernels' per-key arithmetic (dyc_device.hpp, compiled for the host here) equals the
     oracle's on edge and random keys, for every subtable count. The device enumerates each

@@ -144,14 +144,15 @@ def test_bulk_fill_into_fresh_filled_and_cleared_tables(D):
 
 def test_bulk_fill_chunk_beyond_the_register_hold(D):
     """A dense-round chunk with more entries than its threads hold in registers (8192 for the
-    1024-thread fill) counts them, then reads them again to place them. 9000 keys whose start
-    bucket lies in subtable 0 (one chunk: the subtables have fewer than 512 buckets) plus 2000
-    others: subtable 0 fills, the rest overflows into the other bucket of each pair."""
+    1024-thread fill) counts them, then reads them again to place them. 9000 keys whose first
+    bucket (the fill's round-1 target) lies in subtable 0 (one chunk: the subtables have fewer
+    than 512 buckets) plus 2000 others: subtable 0 fills, the rest overflows into the other
+    bucket of each pair."""
+    import fillrule
+
     banner("bulk fill: one chunk of 9000 entries (beyond the register hold)")
     pool = unique_keys(60_000, 1550)
-    pairs = O.pair_table(4)
-    p = O.hash_batch(pool, 6)
-    start = np.array([pairs[x][1] if k & 1 else pairs[x][0] for x, k in zip(p.tolist(), pool.tolist())])
+    start = fillrule.targets(pool, 4)
     keys = np.concatenate([pool[start == 0][:9000], pool[start != 0][:2000]])
     assert keys.size == 11_000
     rng = np.random.default_rng(1551)
@@ -174,7 +175,7 @@ def test_bulk_fill_chunk_beyond_the_register_hold(D):
 
 
 def test_bulk_fill_overflow_reaches_the_eviction_insert(D):
-    """Many copies of a few keys overflow their start bucket and their other bucket, so all three
+    """Many copies of a few keys overflow their first bucket and their other bucket, so all three
     stages run; every copy is stored or pending, and the placement invariant holds."""
     banner("bulk fill: overflow through both rounds and the eviction insert")
     base = unique_keys(3000, 1400)

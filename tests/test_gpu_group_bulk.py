@@ -1,6 +1,6 @@
 """GPU parity of the group bulk fill (dyc_group_insert with insert_order 3), per table against
 the oracle: each table's batch is partitioned by its own chunks in one pass, every chunk is
-filled by one workgroup, and what overflows its start bucket goes through the group's eviction
+filled by one workgroup, and what overflows its first bucket goes through the group's eviction
 insert. Slot layout after a parallel insert is not fixed, so each table is checked by the
 key->value multiset of its image plus its pending entries, the placement invariant on the
 oracle, counts, and finds; stats last_filled shows which path took the batch."""
