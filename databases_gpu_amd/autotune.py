@@ -82,7 +82,10 @@ def tune_group(group, keys: torch.Tensor, vals: torch.Tensor, offsets, lookups: 
     """
     offsets = np.asarray(offsets, dtype=np.uint64)
     lookup_offsets = np.asarray(lookup_offsets, dtype=np.uint64)
-    key = f"{torch.cuda.get_device_name(group.device)}|group|{_signature(offsets)}"
+    # the insert order changes what the insert launch does (the whole batch, or the few keys the
+    # bulk fill leaves), so shapes tuned under one order are not reused under another
+    order = getattr(getattr(group, "cfg", None), "insert_order", 0)
+    key = f"{torch.cuda.get_device_name(group.device)}|group|{_signature(offsets)}|io{order}"
     cache = _load_cache() if use_cache else {}
     t0 = time.perf_counter()
     if key in cache:
