@@ -47,8 +47,10 @@ __global__ __launch_bounds__(kCountThreads) void bulk_pack_kernel(Desc D, Chunks
     // per subtable: buckets, FastDiv multiplier and shifts, first chunk -- one LDS read per key
     // in place of a selection chain over the descriptor's arrays for each of them
     __shared__ uint4 sub_tab[kMaxSubtables];
+    __shared__ uint32_t tsel[32];  // the round's subtable by key parity and pair (fill_round_sub)
     const uint32_t R = 1u << P.bits;
     for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) h[d] = 0;
+    if (threadIdx.x < 32) tsel[threadIdx.x] = fill_round_sub(D, threadIdx.x, IN_ENTRIES);
     if (threadIdx.x < kMaxSubtables) {
         const uint32_t t = threadIdx.x;
         sub_tab[t] = make_uint4(selt(t, D.nb), selt(t, D.m), selt(t, D.sh), chunk_base(Ch, t));
@@ -73,8 +75,7 @@ __global__ __launch_bounds__(kCountThreads) void bulk_pack_kernel(Desc D, Chunks
             uint32_t c = 0;
             if (IN_ENTRIES || !reserved_key(k[u])) {
                 const uint32_t p = pair_of(D, k[u]);
-                uint32_t t = fill_target(D, k[u], p);
-                if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
+                const uint32_t t = tsel[((k[u] & 1u) << 4) | p];
                 const uint4 q = sub_tab[t];
                 c = q.w + (fast_mod_sh(hash_t(k[u], t), q.x, q.y, q.z) >> kChunkShift);
             }
@@ -441,9 +442,11 @@ __global__ __launch_bounds__(kCountThreads) void group_bulk_pack_kernel(const De
     __shared__ uint32_t h[kGroupBulkChunks];
     // per subtable: buckets, FastDiv multiplier and shifts, the table's first local chunk of it
     __shared__ uint4 sub_tab[kMaxSubtables];
+    __shared__ uint32_t tsel[32];  // the round's subtable by key parity and pair (fill_round_sub)
     const GroupItem item = blocks[blockIdx.x];
     const Desc D = dir[item.table];  // by value: loaded once, not per use through memory
     for (uint32_t d = threadIdx.x; d < kGroupBulkChunks; d += kCountThreads) h[d] = 0;
+    if (threadIdx.x < 32) tsel[threadIdx.x] = fill_round_sub(D, threadIdx.x, IN_ENTRIES);
     if (threadIdx.x < kMaxSubtables) {
         const uint32_t t = threadIdx.x;
         uint32_t first = 0;
@@ -467,8 +470,7 @@ __global__ __launch_bounds__(kCountThreads) void group_bulk_pack_kernel(const De
             uint32_t lc = 0;
             if (IN_ENTRIES || !reserved_key(k[u])) {
                 const uint32_t p = pair_of(D, k[u]);
-                uint32_t t = fill_target(D, k[u], p);
-                if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
+                const uint32_t t = tsel[((k[u] & 1u) << 4) | p];
                 const uint4 q = sub_tab[t];
                 lc = q.w + (fast_mod_sh(hash_t(k[u], t), q.x, q.y, q.z) >> kChunkShift);
             }

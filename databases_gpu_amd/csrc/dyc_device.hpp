@@ -343,6 +343,17 @@ __host__ __device__ inline uint32_t fill_target(const Desc &D, uint32_t k, uint3
     return (r == 2u || (r == 0u && (k & 1u))) ? pos2(D, p) : pos1(D, p);
 }
 
+// A fill round's subtable for keys of pair p = idx & 15 and parity idx >> 4 (the index of a
+// 32-entry table the pack kernels keep in LDS, so a key's subtable costs one LDS read): round 1
+// fill_target, round 2 the other bucket of the pair.
+__host__ __device__ inline uint32_t fill_round_sub(const Desc &D, uint32_t idx, bool round2)
+{
+    const uint32_t p = idx & 15u;
+    if (p >= D.np) return 0;
+    const uint32_t t = fill_target(D, idx >> 4, p);
+    return round2 ? other_pos(D, p, t) : t;
+}
+
 // Depth-first search for Desc::fr over the np pairs (3^np choices, pruned): each pair's keys go
 // all to pos1, half and half, or all to pos2, no subtable may exceed an equal share (2 np / d
 // halves of a pair's keys; at the end every share is then equal), and the most halves go to

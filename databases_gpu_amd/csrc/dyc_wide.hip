@@ -582,8 +582,10 @@ __global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc 
     // per subtable: buckets, FastDiv multiplier and shifts, first chunk -- one LDS read per key
     // in place of a selection chain over the descriptor's arrays for each of them
     __shared__ uint4 sub_tab[kMaxSubtables];
+    __shared__ uint32_t tsel[32];  // the round's subtable by key parity and pair (fill_round_sub)
     const uint32_t R = 1u << P.bits;
     for (uint32_t d = threadIdx.x; d < R; d += kCountThreads) h[d] = 0;
+    if (threadIdx.x < 32) tsel[threadIdx.x] = fill_round_sub(D, threadIdx.x, IN_ENTRIES);
     if (threadIdx.x < kMaxSubtables) {
         const uint32_t t = threadIdx.x;
         sub_tab[t] = make_uint4(selt(t, D.nb), selt(t, D.m), selt(t, D.sh), chunk_base(Ch, t));
@@ -608,8 +610,7 @@ __global__ __launch_bounds__(kCountThreads) void wide_bulk_pack_kernel(WideDesc 
             uint32_t c = 0;
             if (IN_ENTRIES || !wide_reserved(k[u])) {
                 const uint32_t f = fold64(k[u]), p = pair_of(D, f);
-                uint32_t t = fill_target(D, (uint32_t)k[u], p);
-                if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
+                const uint32_t t = tsel[(((uint32_t)k[u] & 1u) << 4) | p];
                 const uint4 q = sub_tab[t];
                 c = q.w + (fast_mod_sh(hash_t(f, t), q.x, q.y, q.z) >> kChunkShift);
             }
@@ -907,9 +908,11 @@ __global__ __launch_bounds__(kCountThreads) void wide_group_bulk_pack_kernel(con
 {
     __shared__ uint32_t h[kGroupBulkChunks];
     __shared__ uint4 sub_tab[kMaxSubtables];  // per subtable: buckets, divider, the table's first local chunk of it
+    __shared__ uint32_t tsel[32];  // the round's subtable by key parity and pair (fill_round_sub)
     const GroupItem item = blocks[blockIdx.x];
     const Desc D = dir[item.table].D;  // by value: loaded once, not per use through memory
     for (uint32_t d = threadIdx.x; d < kGroupBulkChunks; d += kCountThreads) h[d] = 0;
+    if (threadIdx.x < 32) tsel[threadIdx.x] = fill_round_sub(D, threadIdx.x, IN_ENTRIES);
     if (threadIdx.x < kMaxSubtables) {
         const uint32_t t = threadIdx.x;
         uint32_t first = 0;
@@ -933,8 +936,7 @@ __global__ __launch_bounds__(kCountThreads) void wide_group_bulk_pack_kernel(con
             uint32_t lc = 0;
             if (IN_ENTRIES || !wide_reserved(k[u])) {
                 const uint32_t f = fold64(k[u]), p = pair_of(D, f);
-                uint32_t t = fill_target(D, (uint32_t)k[u], p);
-                if constexpr (IN_ENTRIES) t = other_pos(D, p, t);
+                const uint32_t t = tsel[(((uint32_t)k[u] & 1u) << 4) | p];
                 const uint4 q = sub_tab[t];
                 lc = q.w + (fast_mod_sh(hash_t(f, t), q.x, q.y, q.z) >> kChunkShift);
             }
