@@ -77,6 +77,11 @@ def test_bulk_fill_insert_multiset_invariant_and_find(D, d):
     assert 0.95 * n < st["last_filled"] <= n  # the fill rounds placed nearly all of them
     gv, gh = gfind(t, keys)
     assert gh.all() and np.array_equal(gv, vals)
+    # the first round's rule (fill_target) sends all keys to pos1 at d = 2, 2/3 at d = 4 and 0.7
+    # at d = 6 (the insert's start rule: half); overflow and evictions move a few percent
+    _, in_pos1 = t.probe_count(cu(keys))
+    share = {2: 1.0, 4: 2 / 3, 6: 0.7}[d]
+    assert in_pos1 > (share - 0.1) * n, (in_pos1, n)
     t.close()
 
 
