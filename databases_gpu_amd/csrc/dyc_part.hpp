@@ -360,12 +360,13 @@ __global__ __launch_bounds__(THREADS) void part_scatter_kernel(Src src, const Ci
     __shared__ uint32_t tmp[kScanWaves];
     const uint32_t R = 1u << P.bits;
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid / kWave;
+    const uint64_t lo = Lay.begin(blockIdx.x);
+    const uint64_t hi = Lay.end(blockIdx.x, lo);
+    if (lo >= hi) return;  // block-uniform: nothing to place (a clipped block of a fill's later round)
     {
         const uint32_t b = Lay.base(blockIdx.x, tid, R, tmp);
         if (tid < R) gbase[tid] = b;
     }
-    const uint64_t lo = Lay.begin(blockIdx.x);
-    const uint64_t hi = Lay.end(blockIdx.x, lo);
     const uint64_t lanemask = lanemask_lt();
     // wave w holds tile entries w * kSortItems * 64 + u * 64 + lane: stable ranking follows
     // (u, lane) inside the wave, and the waves in order

@@ -910,6 +910,11 @@ __global__ __launch_bounds__(kCountThreads) void wide_group_bulk_pack_kernel(con
     __shared__ uint4 sub_tab[kMaxSubtables];  // per subtable: buckets, divider, the table's first local chunk of it
     __shared__ uint32_t tsel[32];  // the round's subtable by key parity and pair (fill_round_sub)
     const GroupItem item = blocks[blockIdx.x];
+    if (item.len == 0) {  // block-uniform: a clipped block of round 2 counts nothing
+        for (uint32_t d = threadIdx.x; d < kGroupBulkChunks; d += kCountThreads)
+            partial[(uint64_t)blockIdx.x * kGroupBulkChunks + d] = 0;
+        return;
+    }
     const Desc D = dir[item.table].D;  // by value: loaded once, not per use through memory
     for (uint32_t d = threadIdx.x; d < kGroupBulkChunks; d += kCountThreads) h[d] = 0;
     if (threadIdx.x < 32) tsel[threadIdx.x] = fill_round_sub(D, threadIdx.x, IN_ENTRIES);
