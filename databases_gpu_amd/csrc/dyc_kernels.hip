@@ -938,6 +938,7 @@ __global__ __launch_bounds__(256, W) void group_insert_kernel(const Desc *__rest
     for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
          it += waves) {
         const GroupItem item = items[it];
+        if (item.len == 0) continue;  // wave-uniform: a block the bulk fill's rounds emptied
         const Desc D = dir[item.table];
         uint64_t next = item.begin;
         const uint64_t end = item.begin + item.len;

@@ -431,6 +431,7 @@ __global__ __launch_bounds__(256) void wide_group_insert_kernel(const WideDesc *
     for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
          it < n_items; it += waves) {
         const GroupItem item = items[it];
+        if (item.len == 0) continue;  // wave-uniform: a block the bulk fill's rounds emptied
         const WideDesc W = dir[item.table];
         const uint64_t end = item.begin + item.len;
         uint64_t next = item.begin + __lane_id();
