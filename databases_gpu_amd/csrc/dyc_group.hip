@@ -128,10 +128,9 @@ int check_offsets(const dyc_group *g, const uint64_t *offsets)
 
 // Cut segments (begin, len, table) into items of at most `per` indexes and upload them; returns
 // the item count in *n_items.
-int upload_segments(dyc_group *g, const std::vector<GroupItem> &segs, uint64_t per, hipStream_t s, uint32_t *n_items)
+// Room for `need` items in the item buffers (host and device), once the last upload is done.
+int reserve_items(dyc_group *g, uint64_t need)
 {
-    uint64_t need = 0;
-    for (const GroupItem &x : segs) need += (x.len + per - 1) / per;
     if (need > 0xffffffffull) return fail(DYC_ERR_INVALID, "batch too large for 32-bit item indexes");
     HIP_TRY(hipEventSynchronize(g->items_free));  // hitems is free to rewrite
     if (need > g->items_cap) {
@@ -149,6 +148,27 @@ int upload_segments(dyc_group *g, const std::vector<GroupItem> &segs, uint64_t p
         }
         g->items_cap = cap;
     }
+    return DYC_OK;
+}
+
+int upload_item_list(dyc_group *g, const std::vector<GroupItem> &items, hipStream_t s, uint32_t *n_items)
+{
+    const int rc = reserve_items(g, items.size());
+    if (rc) return rc;
+    const uint32_t m = (uint32_t)items.size();
+    std::copy(items.begin(), items.end(), g->hitems);
+    if (m) HIP_TRY(hipMemcpyAsync(g->ditems, g->hitems, m * sizeof(GroupItem), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(g->items_free, s));
+    *n_items = m;
+    return DYC_OK;
+}
+
+int upload_segments(dyc_group *g, const std::vector<GroupItem> &segs, uint64_t per, hipStream_t s, uint32_t *n_items)
+{
+    uint64_t need = 0;
+    for (const GroupItem &x : segs) need += (x.len + per - 1) / per;
+    const int rc = reserve_items(g, need);
+    if (rc) return rc;
     uint32_t m = 0;
     for (const GroupItem &x : segs)
         for (uint64_t b = x.begin; b < x.begin + x.len; b += per)
@@ -171,6 +191,54 @@ int upload_items(dyc_group *g, const uint64_t *offsets, hipStream_t s, uint32_t 
         if (len) segs.push_back(GroupItem{offsets[i], (uint32_t)len, i});
     }
     return upload_segments(g, segs, per, s, n_items);
+}
+
+// Find items laid out by XCD. Workgroups go to the device's XCDs in turn (workgroup b to XCD
+// b % kXcds) and a probe launch's wave w of workgroup b takes item slots b * 4 + w, then every
+// (grid * 4)-th slot after it, so slot i runs on XCD (i / 4) % kXcds (for a grid that is a
+// multiple of kXcds; any other grid only loses the layout's benefit). Each table goes whole to
+// one XCD (largest first, to the one with the fewest items), so the waves of an XCD probe a few
+// tables at a time and its L2 holds their lines, instead of every XCD reading every table the
+// launch has in flight. Slots an XCD has no item for hold empty items.
+constexpr uint32_t kXcds = 8, kProbeWavesPerBlock = 4;
+
+// The layout itself: n tables' batches (offsets, as the calls take them) cut into items of at
+// most per keys. The caller checked that each table's batch fits a 32-bit length.
+std::vector<GroupItem> items_by_xcd(const uint64_t *offsets, uint32_t n, uint64_t per)
+{
+    std::vector<uint32_t> order;
+    for (uint32_t i = 0; i < n; ++i)
+        if (offsets[i + 1] > offsets[i]) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
+    });
+    std::vector<GroupItem> seq[kXcds];
+    for (uint32_t t : order) {
+        uint32_t x = 0;
+        for (uint32_t y = 1; y < kXcds; ++y)
+            if (seq[y].size() < seq[x].size()) x = y;
+        for (uint64_t b = offsets[t]; b < offsets[t + 1]; b += per)
+            seq[x].push_back(GroupItem{b, (uint32_t)std::min<uint64_t>(per, offsets[t + 1] - b), t});
+    }
+    size_t most = 0;
+    for (const auto &q : seq) most = std::max(most, q.size());
+    const uint64_t rows = (most + kProbeWavesPerBlock - 1) / kProbeWavesPerBlock;  // slot groups per XCD
+    std::vector<GroupItem> items((size_t)(rows * kProbeWavesPerBlock * kXcds), GroupItem{0, 0, 0});
+    for (size_t i = 0; i < items.size(); ++i) {
+        const size_t x = (i / kProbeWavesPerBlock) % kXcds;
+        const size_t j = (i / (kProbeWavesPerBlock * kXcds)) * kProbeWavesPerBlock + i % kProbeWavesPerBlock;
+        if (j < seq[x].size()) items[i] = seq[x][j];
+    }
+    return items;
+}
+
+int upload_items_by_xcd(dyc_group *g, const uint64_t *offsets, hipStream_t s, uint32_t *n_items)
+{
+    const uint64_t per = g->launch.item_keys ? g->launch.item_keys : kDefaultItemKeys;
+    for (uint32_t i = 0; i < g->n; ++i)
+        if (offsets[i + 1] - offsets[i] > 0xffffffffull)
+            return fail(DYC_ERR_INVALID, "table %u's batch is too large for 32-bit lengths", i);
+    return upload_item_list(g, items_by_xcd(offsets, g->n, per), s, n_items);
 }
 
 int zero_and_run_prologue(dyc_group *g, hipStream_t s)
@@ -587,7 +655,7 @@ int dyc_group_find(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, 
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
     uint32_t n_items = 0;
-    rc = upload_items(g, offsets, s, &n_items);
+    rc = upload_items_by_xcd(g, offsets, s, &n_items);
     if (rc) return rc;
     Pool sp;
     HIP_TRY(launch_group_find(g->ddir, g->ditems, n_items, keys, vals, hit, (int)g->launch.find_lanes,
@@ -764,13 +832,17 @@ int dyc_group_plan_create(dyc_group *g, const uint32_t *keys, const uint32_t *va
     DeviceGuard dg(g->device);
     rc = reserve_pending(g, std::max<uint64_t>(65536, ins_offsets[g->n] / 16), nullptr);
     if (rc) return rc;
-    // the work items of both launches, cut as the eager calls cut them, uploaded once
+    // the work items of both launches, cut and laid out as the eager calls do, uploaded once
     const uint64_t per = g->launch.item_keys ? g->launch.item_keys : kDefaultItemKeys;
     std::vector<GroupItem> items;
-    for (const uint64_t *o : {ins_offsets, find_offsets})
-        for (uint32_t i = 0; i < g->n; ++i)
-            for (uint64_t b = o[i]; b < o[i + 1]; b += per)
-                items.push_back(GroupItem{b, (uint32_t)std::min<uint64_t>(per, o[i + 1] - b), i});
+    for (uint32_t i = 0; i < g->n; ++i)
+        for (uint64_t b = ins_offsets[i]; b < ins_offsets[i + 1]; b += per)
+            items.push_back(GroupItem{b, (uint32_t)std::min<uint64_t>(per, ins_offsets[i + 1] - b), i});
+    for (uint32_t i = 0; i < g->n; ++i)
+        if (find_offsets[i + 1] - find_offsets[i] > 0xffffffffull)
+            return fail(DYC_ERR_INVALID, "table %u's batch is too large for 32-bit lengths", i);
+    const std::vector<GroupItem> finds = items_by_xcd(find_offsets, g->n, per);
+    items.insert(items.end(), finds.begin(), finds.end());
     if (items.size() > 0xffffffffull) return fail(DYC_ERR_INVALID, "batch too large for 32-bit item indexes");
     dyc_group_plan *p = new dyc_group_plan();
     p->g = g;

@@ -904,6 +904,7 @@ __global__ __launch_bounds__(256, W) void group_probe_kernel(const Desc *__restr
     for (uint32_t it = wave_uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); it < n_items;
          it += waves) {
         const GroupItem item = items[it];
+        if (item.len == 0) continue;  // wave-uniform: an empty slot of a layout by XCD
         const Desc D = dir[item.table];
         const uint64_t end = item.begin + item.len;
         unsigned long long nd = 0;
