@@ -15,6 +15,7 @@
 
 using namespace dyc;
 using dyc::host::BulkLayout;
+using dyc::host::items_by_xcd;
 using dyc::host::DeviceGuard;
 using dyc::host::fail;
 
@@ -191,45 +192,6 @@ int upload_items(dyc_group *g, const uint64_t *offsets, hipStream_t s, uint32_t 
         if (len) segs.push_back(GroupItem{offsets[i], (uint32_t)len, i});
     }
     return upload_segments(g, segs, per, s, n_items);
-}
-
-// Find items laid out by XCD. Workgroups go to the device's XCDs in turn (workgroup b to XCD
-// b % kXcds) and a probe launch's wave w of workgroup b takes item slots b * 4 + w, then every
-// (grid * 4)-th slot after it, so slot i runs on XCD (i / 4) % kXcds (for a grid that is a
-// multiple of kXcds; any other grid only loses the layout's benefit). Each table goes whole to
-// one XCD (largest first, to the one with the fewest items), so the waves of an XCD probe a few
-// tables at a time and its L2 holds their lines, instead of every XCD reading every table the
-// launch has in flight. Slots an XCD has no item for hold empty items.
-constexpr uint32_t kXcds = 8, kProbeWavesPerBlock = 4;
-
-// The layout itself: n tables' batches (offsets, as the calls take them) cut into items of at
-// most per keys. The caller checked that each table's batch fits a 32-bit length.
-std::vector<GroupItem> items_by_xcd(const uint64_t *offsets, uint32_t n, uint64_t per)
-{
-    std::vector<uint32_t> order;
-    for (uint32_t i = 0; i < n; ++i)
-        if (offsets[i + 1] > offsets[i]) order.push_back(i);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
-    });
-    std::vector<GroupItem> seq[kXcds];
-    for (uint32_t t : order) {
-        uint32_t x = 0;
-        for (uint32_t y = 1; y < kXcds; ++y)
-            if (seq[y].size() < seq[x].size()) x = y;
-        for (uint64_t b = offsets[t]; b < offsets[t + 1]; b += per)
-            seq[x].push_back(GroupItem{b, (uint32_t)std::min<uint64_t>(per, offsets[t + 1] - b), t});
-    }
-    size_t most = 0;
-    for (const auto &q : seq) most = std::max(most, q.size());
-    const uint64_t rows = (most + kProbeWavesPerBlock - 1) / kProbeWavesPerBlock;  // slot groups per XCD
-    std::vector<GroupItem> items((size_t)(rows * kProbeWavesPerBlock * kXcds), GroupItem{0, 0, 0});
-    for (size_t i = 0; i < items.size(); ++i) {
-        const size_t x = (i / kProbeWavesPerBlock) % kXcds;
-        const size_t j = (i / (kProbeWavesPerBlock * kXcds)) * kProbeWavesPerBlock + i % kProbeWavesPerBlock;
-        if (j < seq[x].size()) items[i] = seq[x][j];
-    }
-    return items;
 }
 
 int upload_items_by_xcd(dyc_group *g, const uint64_t *offsets, hipStream_t s, uint32_t *n_items)

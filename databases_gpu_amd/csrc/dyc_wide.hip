@@ -401,6 +401,7 @@ __global__ __launch_bounds__(256) void wide_group_probe_kernel(const WideDesc *_
     for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
          it < n_items; it += waves) {
         const GroupItem item = items[it];
+        if (item.len == 0) continue;  // wave-uniform: an empty slot of a layout by XCD
         const WideDesc W = dir[item.table];
         WideStash st = stash;
         st.table = item.table;
@@ -2018,11 +2019,16 @@ int wg_check_offsets(const dyc_wide_group *g, const uint64_t *offsets)
     return DYC_OK;
 }
 
-int wg_upload_items(dyc_wide_group *g, const uint64_t *offsets, hipStream_t s, uint32_t *n_items)
+// by_xcd: the find's layout (dyc::host::items_by_xcd: each table whole on one XCD, empty slots
+// between), else the items in table order.
+int wg_upload_items(dyc_wide_group *g, const uint64_t *offsets, hipStream_t s, uint32_t *n_items, bool by_xcd = false)
 {
     const uint64_t per = kWideGroupItemKeys;
-    uint64_t need = 0;
-    for (uint32_t i = 0; i < g->n; ++i) need += (offsets[i + 1] - offsets[i] + per - 1) / per;
+    std::vector<GroupItem> laid;
+    if (by_xcd) laid = dyc::host::items_by_xcd(offsets, g->n, per);
+    uint64_t need = laid.size();
+    if (!by_xcd)
+        for (uint32_t i = 0; i < g->n; ++i) need += (offsets[i + 1] - offsets[i] + per - 1) / per;
     if (need > 0xffffffffull) return fail(DYC_ERR_INVALID, "batch too large for 32-bit item indexes");
     HIP_TRY(hipEventSynchronize(g->items_free));  // hitems is free to rewrite
     if (need > g->items_cap) {
@@ -2041,9 +2047,14 @@ int wg_upload_items(dyc_wide_group *g, const uint64_t *offsets, hipStream_t s, u
         g->items_cap = cap;
     }
     uint32_t m = 0;
-    for (uint32_t i = 0; i < g->n; ++i)
-        for (uint64_t b = offsets[i]; b < offsets[i + 1]; b += per)
-            g->hitems[m++] = GroupItem{b, (uint32_t)std::min<uint64_t>(per, offsets[i + 1] - b), i};
+    if (by_xcd) {
+        std::copy(laid.begin(), laid.end(), g->hitems);
+        m = (uint32_t)laid.size();
+    } else {
+        for (uint32_t i = 0; i < g->n; ++i)
+            for (uint64_t b = offsets[i]; b < offsets[i + 1]; b += per)
+                g->hitems[m++] = GroupItem{b, (uint32_t)std::min<uint64_t>(per, offsets[i + 1] - b), i};
+    }
     if (m) HIP_TRY(hipMemcpyAsync(g->ditems, g->hitems, m * sizeof(GroupItem), hipMemcpyHostToDevice, s));
     HIP_TRY(hipEventRecord(g->items_free, s));
     *n_items = m;
@@ -2456,7 +2467,7 @@ int dyc_wide_group_find(dyc_wide_group *g, const uint64_t *keys, const uint64_t 
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
     uint32_t n_items = 0;
-    rc = wg_upload_items(g, offsets, s, &n_items);
+    rc = wg_upload_items(g, offsets, s, &n_items, true);
     if (rc) return rc;
     HIP_TRY(launch_wide_group_probe(g->ddir, g->ditems, n_items, keys, vals, hit, false, nullptr, (int)g->cfg.max_blocks,
                                     g->stash(), s));
