@@ -63,6 +63,7 @@ bool bulk_layout(uint32_t n_tables, const uint64_t *offsets, DescOf desc, BulkLa
 // tables at a time and its L2 holds their lines, instead of every XCD reading every table the
 // launch has in flight. Slots an XCD has no item for hold empty items.
 constexpr uint32_t kXcds = 8, kProbeWavesPerBlock = 4;
+constexpr uint64_t kXcdLayoutKeys = 1ull << 22;  // eager finds of fewer keys keep the table order
 
 // The layout itself: n tables' batches (offsets, as the calls take them) cut into items of at
 // most per keys. The caller checked that each table's batch fits a 32-bit length.

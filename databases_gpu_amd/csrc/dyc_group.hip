@@ -16,6 +16,7 @@
 using namespace dyc;
 using dyc::host::BulkLayout;
 using dyc::host::items_by_xcd;
+using dyc::host::kXcdLayoutKeys;
 using dyc::host::DeviceGuard;
 using dyc::host::fail;
 
@@ -617,7 +618,10 @@ int dyc_group_find(dyc_group *g, const uint32_t *keys, const uint64_t *offsets, 
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
     uint32_t n_items = 0;
-    rc = upload_items_by_xcd(g, offsets, s, &n_items);
+    // the layout by XCD pays where the batch spans more than the L2s hold; below that its host
+    // work (a sort of the tables) would be most of a small call
+    rc = offsets[g->n] >= kXcdLayoutKeys ? upload_items_by_xcd(g, offsets, s, &n_items)
+                                          : upload_items(g, offsets, s, &n_items);
     if (rc) return rc;
     Pool sp;
     HIP_TRY(launch_group_find(g->ddir, g->ditems, n_items, keys, vals, hit, (int)g->launch.find_lanes,

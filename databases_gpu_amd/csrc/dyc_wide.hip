@@ -2467,7 +2467,7 @@ int dyc_wide_group_find(dyc_wide_group *g, const uint64_t *keys, const uint64_t 
     DeviceGuard dg(g->device);
     hipStream_t s = (hipStream_t)stream;
     uint32_t n_items = 0;
-    rc = wg_upload_items(g, offsets, s, &n_items, true);
+    rc = wg_upload_items(g, offsets, s, &n_items, offsets[g->n] >= dyc::host::kXcdLayoutKeys);
     if (rc) return rc;
     HIP_TRY(launch_wide_group_probe(g->ddir, g->ditems, n_items, keys, vals, hit, false, nullptr, (int)g->cfg.max_blocks,
                                     g->stash(), s));
